@@ -1,0 +1,1277 @@
+/*
+ * oracle.c -- CPU restatement of SwinMD's nonbonded force path (test
+ * infrastructure; see oracle.h).  Loop order and floating-point expression
+ * order follow the cited reference routines so that results agree with the
+ * compiled reference to rounding.  Build: gcc -O2 (no -ffast-math, no FMA
+ * target), like the reference's own Makefile.
+ */
+#include "oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+static const double PI = 3.14159265358979323846;       /* src/NEMD_defs.h:77 */
+static const double SQRT_PI = 1.7724538509055160273;   /* src/NEMD_defs.h:78 */
+static const double BOLTZ = 0.0083145112;              /* src/NEMD_defs.h:108 */
+
+enum { XX = 0, XY, XZ, YX, YY, YZ, ZX, ZY, ZZ };
+
+/* ---- Vector3 helpers with the reference's operator semantics
+ *      (src/utils/Vector3.h:108-193) ---------------------------------------- */
+typedef struct { double x, y, z; } v3;
+
+static inline v3 v3_ld(const double *p) { v3 r = {p[0], p[1], p[2]}; return r; }
+static inline void v3_st(double *p, v3 a) { p[0] = a.x; p[1] = a.y; p[2] = a.z; }
+static inline v3 v3_sub(v3 a, v3 b) { v3 r = {a.x - b.x, a.y - b.y, a.z - b.z}; return r; }
+static inline v3 v3_add(v3 a, v3 b) { v3 r = {a.x + b.x, a.y + b.y, a.z + b.z}; return r; }
+static inline v3 v3_scale(double f, v3 a) { v3 r = {f * a.x, f * a.y, f * a.z}; return r; }
+static inline v3 v3_div(v3 a, double f) { v3 r = {a.x / f, a.y / f, a.z / f}; return r; }
+static inline double v3_dot(v3 a, v3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+static inline double v3_len2(v3 a) { return (a.x * a.x + a.y * a.y + a.z * a.z); }
+static inline double v3_len(v3 a) { return sqrt(a.x * a.x + a.y * a.y + a.z * a.z); }
+static inline void acc(double *p, v3 a) { p[0] += a.x; p[1] += a.y; p[2] += a.z; }
+static inline void dec(double *p, v3 a) { p[0] -= a.x; p[1] -= a.y; p[2] -= a.z; }
+
+/* Ensemble::apply_pbc (src/Ensemble.h:319-334): half-open [-L/2, L/2). */
+static inline void apply_pbc(v3 *r, const double box[3]) {
+    const double hx = 0.5 * box[0], hy = 0.5 * box[1], hz = 0.5 * box[2];
+    while (r->x >= hx) r->x -= box[0];
+    while (r->x < -hx) r->x += box[0];
+    while (r->y >= hy) r->y -= box[1];
+    while (r->y < -hy) r->y += box[1];
+    while (r->z >= hz) r->z -= box[2];
+    while (r->z < -hz) r->z += box[2];
+}
+
+/* Ensemble::exclusion_check (src/Ensemble.h:294-305). */
+static inline int excluded(const orc_sys *s, int a, int b) {
+    for (int k = s->excl_off[a]; k < s->excl_off[a + 1]; ++k)
+        if (s->excl[k] == b) return 1;
+    return 0;
+}
+
+static inline void vir_add(double *v, v3 f, v3 r) {
+    /* virial[XX] += f.x*r.x; virial[XY] += f.y*r.x; ... (LJForce.cpp:206-214) */
+    v[XX] += f.x * r.x; v[XY] += f.y * r.x; v[XZ] += f.z * r.x;
+    v[YX] += f.x * r.y; v[YY] += f.y * r.y; v[YZ] += f.z * r.y;
+    v[ZX] += f.x * r.z; v[ZY] += f.y * r.z; v[ZZ] += f.z * r.z;
+}
+
+/* ========================================================================== */
+/* (a) cells and pair list                                                    */
+/* ========================================================================== */
+
+int orc_cell_setup(double len, const double box[3], int32_t n[3],
+                   double cell_len[3], double rcell_len[3]) {
+    /* CellManager::set_cellManager (src/utils/CellManager.cpp:43-84) */
+    if ((len <= 0) || (len > box[0] / 3.0) || (len > box[1] / 3.0) ||
+        (len > box[2] / 3.0))
+        return 0;
+    for (int d = 0; d < 3; ++d) {
+        n[d] = (int32_t)(box[d] / len);
+        cell_len[d] = box[d] / n[d];
+        rcell_len[d] = 1.0 / cell_len[d];
+    }
+    return 1;
+}
+
+/* The 13 forward neighbour shifts, in the order of
+ * CellManager::build_cell_neighbor (src/utils/CellManager.h:161-186). */
+static const int STENCIL[13][3] = {
+    {0, 0, -1}, {1, 0, -1}, {1, 0, 0},  {1, 0, 1},  {-1, 1, -1},
+    {0, 1, -1}, {1, 1, -1}, {-1, 1, 0}, {0, 1, 0},  {1, 1, 0},
+    {-1, 1, 1}, {0, 1, 1},  {1, 1, 1}};
+
+static inline int wrap_idx(int v, int n) {
+    /* xPlus / xMinus with wrap (CellManager.h:148-159) */
+    if (v >= n) return 0;
+    if (v < 0) return n - 1;
+    return v;
+}
+
+typedef struct { int32_t owner, j; } pair_t;
+
+static int64_t scatter_rows(int n, const pair_t *tmp, int64_t np, int32_t *off,
+                            int32_t *nb) {
+    /* Rows in atom order; within a row, the visit order is kept. */
+    memset(off, 0, sizeof(int32_t) * (size_t)(n + 1));
+    for (int64_t k = 0; k < np; ++k) off[tmp[k].owner + 1]++;
+    for (int i = 0; i < n; ++i) off[i + 1] += off[i];
+    int32_t *fill = (int32_t *)malloc(sizeof(int32_t) * (size_t)(n > 0 ? n : 1));
+    for (int i = 0; i < n; ++i) fill[i] = off[i];
+    for (int64_t k = 0; k < np; ++k) nb[fill[tmp[k].owner]++] = tmp[k].j;
+    free(fill);
+    return np;
+}
+
+int64_t orc_pairlist(const orc_sys *s, double cutoff, double cut_buff,
+                     int use_cells, int32_t *off, int32_t *nb, int64_t cap,
+                     int *used_cells) {
+    const int n = s->n_atoms;
+    /* Ensemble::set_pairlist (src/Ensemble.cpp:454-473) */
+    double pair_cut = cutoff + cut_buff;
+    pair_cut *= pair_cut;
+    const double cell_len_in = cutoff + cut_buff;
+    int32_t nc[3];
+    double clen[3], rclen[3];
+    int cells = use_cells && orc_cell_setup(cell_len_in, s->box, nc, clen, rclen);
+    if (used_cells) *used_cells = cells;
+
+    int64_t tcap = cap > 0 ? cap : 0;
+    pair_t *tmp = (pair_t *)malloc(sizeof(pair_t) * (size_t)(tcap > 0 ? tcap : 1));
+    int64_t np = 0;
+#define PUSH(o, jj)                                   \
+    do {                                              \
+        if (np < tcap) { tmp[np].owner = (o); tmp[np].j = (jj); } \
+        ++np;                                         \
+    } while (0)
+
+    if (cells) {
+        const int ncell = nc[0] * nc[1] * nc[2];
+        const int nxy = nc[0] * nc[1];
+        const double origin[3] = {-s->box[0] / 2.0, -s->box[1] / 2.0, -s->box[2] / 2.0};
+        /* set_atom/find_cell (CellManager.h:91-121, .cpp:145-170): atoms are
+         * appended to their cell in index order. */
+        int32_t *cell_of = (int32_t *)malloc(sizeof(int32_t) * (size_t)(n > 0 ? n : 1));
+        int32_t *cnt = (int32_t *)calloc((size_t)ncell + 1, sizeof(int32_t));
+        for (int i = 0; i < n; ++i) {
+            v3 d = v3_sub(v3_ld(s->pos + 3 * i), v3_ld(origin));
+            int x = (int)(d.x * rclen[0]);
+            int y = (int)(d.y * rclen[1]);
+            int z = (int)(d.z * rclen[2]);
+            if (x >= nc[0]) x = nc[0] - 1; else if (x < 0) x = 0;
+            if (y >= nc[1]) y = nc[1] - 1; else if (y < 0) y = 0;
+            if (z >= nc[2]) z = nc[2] - 1; else if (z < 0) z = 0;
+            cell_of[i] = z * nxy + y * nc[0] + x;
+            cnt[cell_of[i] + 1]++;
+        }
+        for (int c = 0; c < ncell; ++c) cnt[c + 1] += cnt[c];
+        int32_t *members = (int32_t *)malloc(sizeof(int32_t) * (size_t)(n > 0 ? n : 1));
+        int32_t *fill = (int32_t *)malloc(sizeof(int32_t) * (size_t)ncell);
+        for (int c = 0; c < ncell; ++c) fill[c] = cnt[c];
+        for (int i = 0; i < n; ++i) members[fill[cell_of[i]]++] = i;
+        free(fill);
+
+        /* CellManager::update_pair_list (src/utils/CellManager.cpp:216-314) */
+        for (int c = 0; c < ncell; ++c) {
+            const int cz = c / nxy, cy = (c % nxy) / nc[0], cx = c % nc[0];
+            int nbr[13];
+            for (int k = 0; k < 13; ++k) {
+                int xx = wrap_idx(cx + STENCIL[k][0], nc[0]);
+                int yy = wrap_idx(cy + STENCIL[k][1], nc[1]);
+                int zz = wrap_idx(cz + STENCIL[k][2], nc[2]);
+                nbr[k] = zz * nxy + yy * nc[0] + xx;
+            }
+            for (int a = cnt[c]; a < cnt[c + 1]; ++a) {
+                const int i = members[a];
+                const v3 pi = v3_ld(s->pos + 3 * i);
+                for (int b = a + 1; b < cnt[c + 1]; ++b) {
+                    const int j = members[b];
+                    if (!excluded(s, i, j)) {
+                        v3 d = v3_sub(v3_ld(s->pos + 3 * j), pi);   /* no PBC in-cell */
+                        if (v3_len2(d) < pair_cut) PUSH(i, j);
+                    }
+                }
+                for (int k = 0; k < 13; ++k) {
+                    const int cc = nbr[k];
+                    for (int b = cnt[cc]; b < cnt[cc + 1]; ++b) {
+                        const int j = members[b];
+                        if (!excluded(s, i, j)) {
+                            v3 d = v3_sub(v3_ld(s->pos + 3 * j), pi);
+                            apply_pbc(&d, s->box);
+                            if (v3_len2(d) < pair_cut) PUSH(i, j);
+                        }
+                    }
+                }
+            }
+        }
+        free(cell_of); free(cnt); free(members);
+    } else {
+        /* Ensemble::build_pairlist (src/Ensemble.cpp:492-525) */
+        for (int i = 0; i < n - 1; ++i) {
+            const v3 pi = v3_ld(s->pos + 3 * i);
+            for (int j = i + 1; j < n; ++j) {
+                if (!excluded(s, i, j)) {
+                    v3 d = v3_sub(v3_ld(s->pos + 3 * j), pi);
+                    apply_pbc(&d, s->box);
+                    if (v3_len2(d) < pair_cut) PUSH(i, j);
+                }
+            }
+        }
+    }
+#undef PUSH
+    if (np > cap) { free(tmp); return -np; }
+    scatter_rows(n, tmp, np, off, nb);
+    free(tmp);
+    return np;
+}
+
+/* ========================================================================== */
+/* (b) van der Waals                                                          */
+/* ========================================================================== */
+
+void orc_buff_table(int n_types, const double *type_sigma,
+                    const double *type_eps, double *r_ij, double *eps_ij) {
+    /* BuffvdWForce::set_buff_param (src/forces/BuffvdWForce.cpp:392-433) */
+    for (int i = 0; i < n_types; ++i) {
+        const double epsi = type_eps[i], sigmai = type_sigma[i];
+        for (int j = 0; j < n_types; ++j) {
+            const double epsj = type_eps[j], sigmaj = type_sigma[j];
+            const double sigmai2 = sigmai * sigmai, sigmaj2 = sigmaj * sigmaj;
+            const double epssqrt = sqrt(epsi) + sqrt(epsj);
+            double R, e;
+            if (epsi == 0 || epsj == 0) { R = 0.0000001; e = 0; }
+            else {
+                R = (sigmai2 * sigmai + sigmaj2 * sigmaj) / (sigmai2 + sigmaj2);
+                e = (4.0 * epsi * epsj) / (epssqrt * epssqrt);
+            }
+            r_ij[i * n_types + j] = R;
+            eps_ij[i * n_types + j] = e;
+        }
+    }
+}
+
+void orc_lj_lrc(const orc_sys *s, int n_types, const double *sigma2,
+                const double *eps4, double cutoff, double *e_lrc,
+                double *v_lrc, double *du_lrc, double *d2u_lrc) {
+    /* LJForce::long_range_correct (src/forces/LJForce.cpp:260-370).  The
+     * reference leaves csix uninitialised (:268); it is started at 0 here. */
+    const int n = s->n_atoms;
+    int *num = (int *)calloc((size_t)n_types, sizeof(int));
+    for (int i = 0; i < n; ++i) num[s->type[i]] += 1;
+    const double cutoff3 = cutoff * cutoff * cutoff;
+    const double volrc = s->box[0] * s->box[1] * s->box[2];
+    const double invvol = 1.0 / volrc;
+    double csix = 0.0;
+    int npair = 0, nexcl = 0;
+    for (int i = 0; i < n_types; ++i)
+        for (int j = i; j < n_types; ++j) {
+            const double sg2 = sigma2[i * n_types + j], e4 = eps4[i * n_types + j];
+            const double sigma6 = sg2 * sg2 * sg2;
+            int npair_ij;
+            if (i != j) npair_ij = num[i] * num[j];
+            else npair_ij = (int)(num[i] * (num[i] - 1) * 0.5);
+            csix += npair_ij * e4 * sigma6;
+            npair += npair_ij;
+        }
+    for (int i = 0; i < n; ++i)
+        for (int k = s->excl_off[i]; k < s->excl_off[i + 1]; ++k) {
+            const int j = s->excl[k];
+            if (j < i) continue;
+            const int t = s->type[i] * n_types + s->type[j];
+            const double sigma6 = sigma2[t] * sigma2[t] * sigma2[t];
+            csix -= eps4[t] * sigma6;
+            nexcl += 1;
+        }
+    if (npair - nexcl <= 0) csix = 0;
+    else csix /= npair - nexcl;
+    *e_lrc = -2.0 * PI * n * n * invvol * csix / (3.0 * cutoff3);
+    *du_lrc = 4.0 * PI * n * n * invvol * invvol * csix / (3.0 * cutoff3);
+    *d2u_lrc = -4.0 * PI * n * n * invvol * invvol * invvol * csix / cutoff3;
+    *v_lrc = -2.0 * (*e_lrc);
+    free(num);
+}
+
+/* GROMACS-shift Coulomb used by LJ/Mie/Buckingham when computeCoulomb is on
+ * (LJForce.cpp:188-201). */
+static inline void coulomb_shift(double qij, double r, double r_1, double rij2,
+                                 double rij_2, double cut, double rcut_2,
+                                 double *elec, double *tmpf) {
+    double e = qij * r_1;
+    double f = e * rij_2;
+    double r_rcut = r / cut;
+    double r4_rcut4 = rij2 * rij2 * rcut_2 * rcut_2;
+    *elec += e * (1 - 1.666667 * r_rcut + 1.666667 * r4_rcut4 - r_rcut * r4_rcut4);
+    *tmpf += f * (1 - 5 * r4_rcut4 + 4 * r_rcut * r4_rcut4);
+}
+
+static int vdw_lj_mie(const orc_sys *s, const int32_t *off, const int32_t *nb,
+                      const orc_vdw_params *p, double *force, orc_vdw_out *o) {
+    const int is_mie = (p->kind == ORC_MIE);
+    const double cutOff = p->cutoff, cutOff2 = cutOff * cutOff;
+    double switchDist = cutOff2;
+    if (p->switch_on) switchDist = p->switch_dist * p->switch_dist;
+    double c1 = 0.0, c2 = 0.0, c3 = 0.0;
+    if (p->switch_on && (cutOff2 > switchDist)) {
+        c1 = 1.0 / (cutOff2 - switchDist);
+        c1 = c1 * c1 * c1;
+        c2 = cutOff2 - 3.0 * switchDist;
+        c3 = 4 * c1;
+    }
+    double Cn = 0.0;
+    if (is_mie) {   /* MieForce ctor (src/forces/MieForce.cpp:32-35) */
+        double Cbase = p->n_mie / p->m_mie;
+        double Cexp = p->m_mie / (p->n_mie - p->m_mie);
+        double Cpow = pow(Cbase, Cexp);
+        Cn = (p->n_mie / (p->n_mie - p->m_mie)) * Cpow;
+    }
+    const double rcut_2 = 1.0 / cutOff2;
+    double energy = 0.0, ulus = 0.0, dudv = 0.0, d2udv = 0.0, elecE = 0.0;
+    double vir[9] = {0};
+    const double voll = s->box[0] * s->box[1] * s->box[2];
+    const double vol2 = voll * voll;
+    for (int i = 0; i < s->n_atoms; ++i) {
+        const v3 rim = v3_ld(s->com + 3 * s->mol[i]);
+        const double qi = s->q[i];
+        const v3 pi = v3_ld(s->pos + 3 * i);
+        for (int k = off[i]; k < off[i + 1]; ++k) {
+            const int j = nb[k];
+            if (s->mol[i] == s->mol[j]) continue;
+            v3 rij = v3_sub(v3_ld(s->pos + 3 * j), pi);
+            v3 rijm = v3_sub(v3_ld(s->com + 3 * s->mol[j]), rim);
+            apply_pbc(&rij, s->box);
+            apply_pbc(&rijm, s->box);
+            const double r = v3_len(rij);
+            const double rij2 = r * r;
+            if (!(rij2 <= cutOff2)) continue;
+            const int t = s->type[i] * p->n_types + s->type[j];
+            double tmpE, tmpf, dulj = 0, d2ulj = 0;
+            const double r_1 = 1.0 / r;
+            const double rij_2 = r_1 * r_1;
+            if (!is_mie) {
+                /* LJForce::compute (src/forces/LJForce.cpp:122-176) */
+                const double sigma2 = p->sigma2[t], eps4 = p->eps4[t];
+                const double sr2 = sigma2 * rij_2;
+                const double sr6 = sr2 * sr2 * sr2;
+                const double sr12 = sr6 * sr6;
+                tmpE = eps4 * (sr12 - sr6);
+                tmpf = (sr12 + sr12 - sr6) * 6 * eps4 * rij_2;
+                const double rijRij = p->use_constraint ? v3_dot(rij, rijm) : rij2;
+                if (p->lustig) {
+                    dulj = tmpf * rijRij;
+                    d2ulj = eps4 * (156.0 * sr12 - 42.0 * sr6) * rijRij * rijRij * rij_2 * rij_2 + 2.0 * dulj;
+                }
+                if (p->switch_on) {   /* energy shift only (:152-168) */
+                    const double sr2cut = sigma2 * rcut_2;
+                    const double sr6cut = sr2cut * sr2cut * sr2cut;
+                    const double sr12cut = sr6cut * sr6cut;
+                    const double Eljcut = eps4 * (sr12cut - sr6cut);
+                    tmpE = tmpE - Eljcut;
+                }
+            } else {
+                /* MieForce::compute (src/forces/MieForce.cpp:120-164) */
+                const double sigma = sqrt(p->sigma2[t]);
+                const double eps = 0.250 * p->eps4[t];
+                const double sigrij = sigma * r_1;
+                const double srn = pow(sigrij, p->n_mie);
+                const double srm = pow(sigrij, p->m_mie);
+                tmpE = Cn * eps * (srn - srm);
+                tmpf = Cn * eps * (p->n_mie * srn - p->m_mie * srm) * rij_2;
+                const double rijRij = v3_dot(rij, rijm);
+                if (p->lustig) {
+                    dulj = tmpf * rijRij;
+                    d2ulj = Cn * eps * (p->n_mie * (p->n_mie + 1) * srn - p->m_mie * (p->m_mie + 1) * srm) *
+                                rijRij * rijRij * rij_2 * rij_2 + 2.0 * dulj;
+                }
+                if (p->switch_on && (rij2 > switchDist)) {
+                    const double c = cutOff2 - rij2;
+                    const double c4 = c * (2 * rij2 + c2);
+                    const double sv = c * c4 * c1;
+                    const double dsv = c3 * (c * c - c4);
+                    tmpf = tmpf * sv - tmpE * dsv;
+                    tmpE *= sv;
+                }
+            }
+            energy += tmpE;
+            if (p->lustig) { ulus += tmpE; dudv += dulj; d2udv += d2ulj; }
+            if (p->coulomb)
+                coulomb_shift(qi * s->q[j], r, r_1, rij2, rij_2, cutOff, rcut_2, &elecE, &tmpf);
+            const v3 fij = v3_scale(-tmpf, rij);
+            acc(force + 3 * i, fij);
+            dec(force + 3 * j, fij);
+            vir_add(vir, fij, rij);
+        }
+    }
+    energy += p->e_lrc;
+    vir[XX] += p->v_lrc; vir[YY] += p->v_lrc; vir[ZZ] += p->v_lrc;
+    o->energy = energy;
+    o->lj_energy = energy + elecE;
+    memcpy(o->virial, vir, sizeof vir);
+    o->ulus = o->dudv = o->d2udv = 0.0;
+    if (p->lustig) {
+        dudv = -dudv / (3.0 * voll);
+        d2udv = d2udv / (9.0 * vol2);
+        o->ulus = ulus + p->e_lrc;
+        o->dudv = dudv + p->du_lrc;
+        o->d2udv = d2udv + p->d2u_lrc;
+    }
+    return 0;
+}
+
+static int vdw_buck(const orc_sys *s, const int32_t *off, const int32_t *nb,
+                    const orc_vdw_params *p, double *force, orc_vdw_out *o) {
+    /* BuckinghamForce::compute (src/forces/BuckinghamForce.cpp:79-287) with
+     * VirtualPositions (:372-398).  Every molecule is treated as CO2 with
+     * sites O0, O1, C2 (types 0, 1, 2). */
+    const double cutOff = p->cutoff, cutOff2 = cutOff * cutOff;
+    double switchDist = cutOff2;
+    if (p->switch_on) switchDist = p->switch_dist * p->switch_dist;
+    double c1 = 0.0, c2 = 0.0, c3 = 0.0;
+    if (p->switch_on && (cutOff2 > switchDist)) {
+        c1 = 1.0 / (cutOff2 - switchDist);
+        c1 = c1 * c1 * c1;
+        c2 = cutOff2 - 3.0 * switchDist;
+        c3 = 4 * c1;
+    }
+    const double rcut_2 = 1.0 / cutOff2;
+    const int nm = s->n_mols;
+    double *virt = (double *)malloc(sizeof(double) * 6 * (size_t)(nm > 0 ? nm : 1));
+    const double ll = p->co2_bond * 0.5;
+    const double cc = 0.91 * ll;
+    const double bb = ll - cc;
+    for (int m = 0; m < nm; ++m) {
+        const int a0 = s->mol_start[m];
+        const v3 O0 = v3_ld(s->pos + 3 * a0), O1 = v3_ld(s->pos + 3 * (a0 + 1));
+        const v3 C2 = v3_ld(s->pos + 3 * (a0 + 2));
+        v3_st(virt + 6 * m, v3_add(v3_scale(bb / ll, C2), v3_scale(cc / ll, O0)));
+        v3_st(virt + 6 * m + 3, v3_add(v3_scale(bb / ll, C2), v3_scale(cc / ll, O1)));
+    }
+    double energy = 0.0, ulus = 0.0, dudv = 0.0, d2udv = 0.0, elecE = 0.0;
+    double vir[9] = {0};
+    const double voll = s->box[0] * s->box[1] * s->box[2];
+    const double vol2 = voll * voll;
+    for (int i = 0; i < s->n_atoms; ++i) {
+        const int mi = s->mol[i];
+        const v3 rim = v3_ld(s->com + 3 * mi);
+        const double qi = s->q[i];
+        v3 ri = v3_ld(s->pos + 3 * i);
+        if (s->type[i] == 0) ri = v3_ld(virt + 6 * mi);
+        if (s->type[i] == 1) ri = v3_ld(virt + 6 * mi + 3);
+        for (int k = off[i]; k < off[i + 1]; ++k) {
+            const int j = nb[k];
+            const int mj = s->mol[j];
+            if (mi == mj) continue;
+            v3 rj = v3_ld(s->pos + 3 * j);
+            if (s->type[j] == 0) rj = v3_ld(virt + 6 * mj);
+            if (s->type[j] == 1) rj = v3_ld(virt + 6 * mj + 3);
+            v3 rij = v3_sub(ri, rj);
+            v3 rijm = v3_sub(v3_ld(s->com + 3 * mj), rim);
+            rijm = v3_sub(rijm, rij);
+            apply_pbc(&rij, s->box);
+            rijm = v3_add(rijm, rij);
+            (void)rijm;
+            const double r = v3_len(rij);
+            const double rij2 = r * r;
+            if (!(rij2 <= cutOff2)) continue;
+            const int t = s->type[i] * p->n_types + s->type[j];
+            const double Aa = p->buck_a[t], Bb = p->buck_b[t], Cc = p->buck_c[t];
+            const double r_1 = 1.0 / r;
+            const double rij_2 = r_1 * r_1;
+            const double rij6 = rij_2 * rij_2 * rij_2;
+            double tmpE = Bb * exp(-Cc * r) - Aa * rij6;
+            double tmpf = -Bb * Cc * r_1 * exp(-Cc * r) + 6.0 * Aa * rij6 * rij_2;
+            double dulj = 0, d2ulj = 0;
+            if (p->lustig) {
+                dulj = -Bb * Cc * r * exp(-Cc * r) + 6.0 * Aa * rij6;
+                d2ulj = (2 * r + Cc * rij2) * Bb * Cc * exp(-Cc * r) - 54.0 * Aa * rij6;
+            }
+            if (p->switch_on && (rij2 > switchDist)) {
+                const double c = cutOff2 - rij2;
+                const double c4 = c * (2 * rij2 + c2);
+                const double sv = c * c4 * c1;
+                const double dsv = c3 * (c * c - c4);
+                tmpf = tmpf * sv - tmpE * dsv;
+                tmpE *= sv;
+            }
+            energy += tmpE;
+            if (p->lustig) { ulus += tmpE; dudv += dulj; d2udv += d2ulj; }
+            if (p->coulomb)
+                coulomb_shift(qi * s->q[j], r, r_1, rij2, rij_2, cutOff, rcut_2, &elecE, &tmpf);
+            const v3 fij = v3_scale(-tmpf, rij);
+            const int ci = s->mol_start[mi] + 2, cj = s->mol_start[mj] + 2;
+            if (s->type[i] == 0 || s->type[i] == 1) {
+                acc(force + 3 * i, v3_scale(0.91, fij));
+                acc(force + 3 * ci, v3_scale(0.09, fij));
+            } else if (s->type[i] == 2) {
+                acc(force + 3 * i, fij);
+            }
+            if (s->type[j] == 0 || s->type[j] == 1) {
+                dec(force + 3 * j, v3_scale(0.91, fij));
+                dec(force + 3 * cj, v3_scale(0.09, fij));
+            } else if (s->type[j] == 2) {
+                dec(force + 3 * j, fij);
+            }
+            vir_add(vir, fij, rij);
+        }
+    }
+    free(virt);
+    energy += p->e_lrc;
+    vir[XX] += p->v_lrc; vir[YY] += p->v_lrc; vir[ZZ] += p->v_lrc;
+    o->energy = energy;
+    o->lj_energy = energy + elecE;
+    memcpy(o->virial, vir, sizeof vir);
+    o->ulus = o->dudv = o->d2udv = 0.0;
+    if (p->lustig) {
+        dudv = dudv / (3 * voll);
+        d2udv = d2udv / (9 * vol2);
+        o->ulus = ulus + p->e_lrc;
+        o->dudv = dudv + p->du_lrc;
+        o->d2udv = d2udv + p->d2u_lrc;
+    }
+    return 0;
+}
+
+static int vdw_buff(const orc_sys *s, const int32_t *off, const int32_t *nb,
+                    const orc_vdw_params *p, double *force, orc_vdw_out *o) {
+    /* BuffvdWForce::compute (src/forces/BuffvdWForce.cpp:93-343) */
+    const int nt = p->n_types;
+    double *Rt = (double *)malloc(sizeof(double) * (size_t)(nt * nt));
+    double *Et = (double *)malloc(sizeof(double) * (size_t)(nt * nt));
+    orc_buff_table(nt, p->type_sigma, p->type_eps, Rt, Et);
+    const double cutOff = p->cutoff, cutOff2 = cutOff * cutOff;
+    double switchDist = cutOff2;
+    if (p->switch_on) switchDist = p->switch_dist * p->switch_dist;
+    const double switchd1 = p->switch_dist;
+    double c0 = 0, c1 = 0, c2 = 0, c3 = 0, c4 = 0, c5 = 0;
+    if (p->switch_on && (cutOff2 > switchDist)) {   /* ctor :59-73 */
+        const double offcut = cutOff - switchd1;
+        const double denom = offcut * offcut * offcut * offcut * offcut;
+        c0 = cutOff * cutOff2 * (cutOff2 - 5.0 * cutOff * switchd1 + 10.0 * switchDist) / denom;
+        c1 = -30.0 * cutOff2 * switchDist / denom;
+        c2 = 30.0 * (cutOff2 * switchd1 + cutOff * switchDist) / denom;
+        c3 = -10.0 * (cutOff2 + 4.0 * cutOff * switchd1 + switchDist) / denom;
+        c4 = 15.0 * (cutOff + switchd1) / denom;
+        c5 = -6.0 / denom;
+    }
+    const double factor = p->hfactor, n = p->n_buff, m = p->m_buff;
+    const double delta = p->delta, gamma = p->gamma;
+    /* VirtualPositions (:358-385): H sites pulled towards the molecule's
+     * second site (the oxygen). */
+    double *virt = (double *)malloc(sizeof(double) * 3 * (size_t)(s->n_atoms > 0 ? s->n_atoms : 1));
+    for (int mm = 0; mm < s->n_mols; ++mm) {
+        const int oid = s->mol_start[mm] + 1;
+        if (s->type[oid] != 1) { free(virt); free(Rt); free(Et); return -1; }
+        const v3 po = v3_ld(s->pos + 3 * oid);
+        for (int a = s->mol_start[mm]; a < s->mol_start[mm + 1]; ++a)
+            if (s->type[a] == 0)
+                v3_st(virt + 3 * a, v3_add(v3_scale(factor, v3_ld(s->pos + 3 * a)),
+                                           v3_scale(1.0 - factor, po)));
+    }
+    double energy = 0.0, dudv = 0.0, d2udv = 0.0;
+    double vir[9] = {0};
+    const double voll = s->box[0] * s->box[1] * s->box[2];
+    const double vol2 = voll * voll;
+    double r_1 = 0.0;   /* the reference keeps r_1 from the previous pair when eps==0 */
+    for (int i = 0; i < s->n_atoms; ++i) {
+        const int mi = s->mol[i];
+        const v3 rim = v3_ld(s->com + 3 * mi);
+        v3 ri = (s->type[i] == 0) ? v3_ld(virt + 3 * i) : v3_ld(s->pos + 3 * i);
+        for (int k = off[i]; k < off[i + 1]; ++k) {
+            const int j = nb[k];
+            const int mj = s->mol[j];
+            if (mi == mj) { free(virt); free(Rt); free(Et); return -2; }
+            v3 rj = (s->type[j] == 0) ? v3_ld(virt + 3 * j) : v3_ld(s->pos + 3 * j);
+            v3 rij = v3_sub(ri, rj);
+            v3 rijm = v3_sub(rim, v3_ld(s->com + 3 * mj));
+            apply_pbc(&rij, s->box);
+            apply_pbc(&rijm, s->box);
+            const double r = v3_len(rij);
+            const double rij2 = r * r;
+            if (!(rij2 <= cutOff2)) continue;
+            const int t = s->type[i] * nt + s->type[j];
+            const double sigma = Rt[t], eps = Et[t];
+            double tmpE, tmpf;
+            if (eps == 0) { tmpE = 0; tmpf = 0; }
+            else {
+                const double invsigma = 1.0 / sigma;
+                const double m_1 = m - 1.0;
+                const double rho = r * invsigma;
+                const double rho_m = pow(rho, m);
+                const double rho_m1 = pow(rho, m_1);
+                const double expnm = n - m;
+                r_1 = 1.0 / r;
+                const double par1 = (1.0 + delta) / (rho + delta);
+                const double par2 = (1.0 + gamma) / (rho_m + gamma);
+                const double par1_nm = pow(par1, expnm);
+                tmpE = eps * par1_nm * (par2 - 2.0);
+                tmpf = -eps * expnm * par1_nm * (par2 - 2.0) * invsigma / (rho + delta) -
+                       eps * par1_nm * par2 * m * rho_m1 * invsigma / (rho_m + gamma);
+            }
+            if (p->switch_on && (rij2 > switchDist)) {
+                const double rij3 = rij2 * r, rij4 = rij2 * rij2, rij5 = rij2 * rij3;
+                const double sv = c5 * rij5 + c4 * rij4 + c3 * rij3 + c2 * rij2 + c1 * r + c0;
+                const double dsv = 5.0 * c5 * rij4 + 4.0 * c4 * rij3 + 3.0 * c3 * rij2 + 2.0 * c2 * r + c1;
+                tmpf = tmpf * sv + tmpE * dsv;
+                tmpE *= sv;
+            }
+            const double dulj = tmpf * r;
+            energy += tmpE;
+            dudv += dulj;
+            /* d2ulj is 0 in the reference (:218-221) */
+            v3 fij = v3_scale(-tmpf, rij);
+            fij = v3_scale(r_1, fij);
+            const int oi = s->mol_start[mi] + 1, oj = s->mol_start[mj] + 1;
+            if (s->type[i] == 0) {
+                acc(force + 3 * i, v3_scale(factor, fij));
+                acc(force + 3 * oi, v3_scale(1.0 - factor, fij));
+            } else acc(force + 3 * i, fij);
+            if (s->type[j] == 0) {
+                dec(force + 3 * j, v3_scale(factor, fij));
+                dec(force + 3 * oj, v3_scale(1.0 - factor, fij));
+            } else dec(force + 3 * j, fij);
+            v3 nf = {-fij.x, -fij.y, -fij.z};
+            vir_add(vir, nf, rij);
+            (void)rijm;
+        }
+    }
+    vir[XX] += p->v_lrc; vir[YY] += p->v_lrc; vir[ZZ] += p->v_lrc;
+    dudv = dudv / (3.0 * voll);
+    o->energy = energy;
+    o->lj_energy = energy + p->e_lrc;
+    memcpy(o->virial, vir, sizeof vir);
+    o->ulus = energy + p->e_lrc;
+    o->dudv = dudv + p->du_lrc;
+    o->d2udv = 0.0;
+    if (p->lustig) {
+        d2udv = d2udv / (9.0 * vol2);
+        o->d2udv = d2udv + p->d2u_lrc;
+    }
+    free(virt); free(Rt); free(Et);
+    return 0;
+}
+
+int orc_vdw(const orc_sys *s, const int32_t *off, const int32_t *nb,
+            const orc_vdw_params *p, double *force, orc_vdw_out *out) {
+    memset(out, 0, sizeof *out);
+    switch (p->kind) {
+    case ORC_LJ:
+    case ORC_MIE: return vdw_lj_mie(s, off, nb, p, force, out);
+    case ORC_BUCK: return vdw_buck(s, off, nb, p, force, out);
+    case ORC_BUFF147: return vdw_buff(s, off, nb, p, force, out);
+    default: return -3;
+    }
+}
+
+/* ========================================================================== */
+/* (b/c) Ewald for point charges                                              */
+/* ========================================================================== */
+
+void orc_ewald_self(const orc_sys *s, double alpha, double volume,
+                    double *self_energy, double *charged_energy) {
+    /* EwaldForce ctor (src/forces/EwaldForce.cpp:69-79) */
+    double qSum = 0.0, q2Sum = 0.0;
+    for (int i = 0; i < s->n_atoms; ++i) {
+        qSum += s->q[i];
+        q2Sum += s->q[i] * s->q[i];
+    }
+    *self_energy = -q2Sum * alpha / SQRT_PI;
+    *charged_energy = 0.0;
+    if (fabs(qSum) > 1.0e-5)
+        *charged_energy = -PI * qSum * qSum / (2.0 * volume * (alpha * alpha));
+}
+
+void orc_ewald_real(const orc_sys *s, const int32_t *off, const int32_t *nb,
+                    double alpha, double real_cut, int use_constraint,
+                    double *force, orc_ewald_out *o) {
+    /* EwaldForce::real_term (src/forces/EwaldForce.cpp:296-527), one rank */
+    const double alpha2 = alpha * alpha;
+    const double alpha2PI = 2.0 * alpha / SQRT_PI;
+    const double realCut2 = real_cut * real_cut;
+    double E = 0.0, d1 = 0.0, d2 = 0.0, vir[9] = {0};
+    for (int i = 0; i < s->n_atoms; ++i) {
+        const v3 ri = v3_ld(s->pos + 3 * i);
+        const v3 rim = v3_ld(s->com + 3 * s->mol[i]);
+        const double qi = s->q[i];
+        for (int k = off[i]; k < off[i + 1]; ++k) {
+            const int j = nb[k];
+            v3 rijm = v3_sub(v3_ld(s->com + 3 * s->mol[j]), rim);
+            const double qij = qi * s->q[j];
+            v3 rij = v3_sub(v3_ld(s->pos + 3 * j), ri);
+            apply_pbc(&rij, s->box);
+            apply_pbc(&rijm, s->box);
+            const double r2 = v3_len2(rij);
+            if (r2 < realCut2) {
+                const double rijRij = use_constraint ? v3_dot(rij, rijm) : r2;
+                const double r = sqrt(r2);
+                const double e = erfc(alpha * r) / r;
+                const double f = (e + alpha2PI * exp(-alpha2 * r2)) / r2;
+                E += qij * e;
+                d1 += -qij * f * rijRij;
+                d2 += qij * (2.0 * f + 2.0 * alpha2PI * alpha2 * exp(-alpha2 * r2)) * rijRij * rijRij / r2 +
+                      2.0 * qij * f * rijRij;
+                const v3 fij = v3_scale(-qij * f, rij);
+                acc(force + 3 * i, fij);
+                dec(force + 3 * j, fij);
+                vir_add(vir, fij, rij);
+            }
+        }
+    }
+    o->energy = E; o->deriv = d1; o->deriv2 = d2; o->extra = 0.0;
+    memcpy(o->virial, vir, sizeof vir);
+}
+
+void orc_ewald_intra(const orc_sys *s, double alpha, double volume,
+                     int use_constraint, double *force, orc_ewald_out *o) {
+    /* EwaldForce::intraMolCorrect (src/forces/EwaldForce.cpp:709-813) */
+    const double alpha2 = alpha * alpha;
+    const double alpha2PI = 2.0 * alpha / SQRT_PI;
+    const double volume2 = volume * volume;
+    double E = 0.0, U_i = 0.0, d1 = 0.0, d2 = 0.0;
+    for (int m = 0; m < s->n_mols; ++m) {
+        for (int a = s->mol_start[m]; a < s->mol_start[m + 1]; ++a) {
+            const double qi = s->q[a];
+            const v3 ri = v3_ld(s->pos + 3 * a);
+            for (int b = a + 1; b < s->mol_start[m + 1]; ++b) {
+                if (!excluded(s, a, b)) continue;
+                v3 rij = v3_sub(v3_ld(s->pos + 3 * b), ri);
+                apply_pbc(&rij, s->box);
+                const double r2 = v3_len2(rij);
+                const double r = sqrt(r2);
+                const double qij = qi * s->q[b];
+                const double e = erf(alpha * r) / r;
+                E += qij * e;
+                U_i += alpha2PI * qij * exp(-alpha2 * r2);
+                if (!use_constraint) {
+                    const double dudr = e - alpha2PI * exp(-alpha2 * r2);
+                    d1 += qij * dudr / (3.0 * volume);
+                    d2 += (-2.0 * qij * dudr + qij * (-2.0 * dudr + 2.0 * alpha2PI * alpha2 * exp(-alpha2 * r2) * r2)) /
+                          (9.0 * volume2);
+                    const v3 fij = v3_scale(qij * dudr / r2, rij);
+                    acc(force + 3 * a, fij);
+                    dec(force + 3 * b, fij);
+                }
+            }
+        }
+    }
+    o->energy = -E; o->extra = -U_i; o->deriv = d1; o->deriv2 = d2;
+    memset(o->virial, 0, sizeof o->virial);
+}
+
+void orc_ewald_surface(const orc_sys *s, double volume, int use_constraint,
+                       double *force, orc_ewald_out *o) {
+    /* EwaldForce::surfDipoleCorrect (src/forces/EwaldForce.cpp:816-937) */
+    const double volumer = 1.0 / volume;
+    const double coeff = 2.0 * PI * volumer / 3.0;
+    v3 qr = {0, 0, 0}, qR = {0, 0, 0};
+    for (int i = 0; i < s->n_atoms; ++i) {
+        qr = v3_add(qr, v3_scale(s->q[i], v3_ld(s->pos + 3 * i)));
+        qR = v3_add(qR, v3_scale(s->q[i], v3_ld(s->com + 3 * s->mol[i])));
+    }
+    o->energy = coeff * v3_len2(qr);
+    if (use_constraint) {
+        o->deriv = v3_dot(v3_scale(-coeff * volumer, qr), qr) +
+                   v3_dot(v3_scale(2.0 * coeff * volumer, qr), qR) / 3.0;
+        o->deriv2 = v3_dot(v3_scale(2.0 * coeff * volumer * volumer, qr), qr) -
+                    v3_dot(v3_scale(16.0 * coeff * volumer * volumer, qr), qR) / 9.0 +
+                    v3_dot(v3_scale(2.0 * coeff * volumer * volumer, qR), qR) / 9.0;
+    } else {
+        o->deriv = v3_dot(v3_scale(-coeff * volumer, qr), qr) / 3.0;
+        o->deriv2 = v3_dot(v3_scale(4.0 * coeff * volumer * volumer, qr), qr) / 9.0;
+    }
+    qr = v3_scale(coeff, qr);
+    double vir[9] = {0};
+    for (int i = 0; i < s->n_atoms; ++i) {
+        const v3 temp = v3_scale(-s->q[i], qr);
+        const v3 ri = v3_ld(s->pos + 3 * i);
+        dec(force + 3 * i, v3_scale(2.0 * s->q[i], qr));
+        vir_add(vir, temp, ri);
+    }
+    o->extra = 0.0;
+    memcpy(o->virial, vir, sizeof vir);
+}
+
+int orc_ewald_kvectors(double kcut, const double box[3], double alpha,
+                       double *kvec, double *kmod2, double *expk, int cap) {
+    /* EwaldForce::set_up (src/forces/EwaldForce.cpp:198-257) */
+    const double kCut2 = kcut * kcut;
+    const double alphaR4 = 1.0 / (4.0 * (alpha * alpha));
+    const int nxMax = (int)(kcut * box[0] / (2 * PI));
+    const int nyMax = (int)(kcut * box[1] / (2 * PI));
+    const int nzMax = (int)(kcut * box[2] / (2 * PI));
+    const double vx = 2.0 * PI / box[0], vy = 2.0 * PI / box[1], vz = 2.0 * PI / box[2];
+    int K = 0;
+    for (int nx = 0; nx <= nxMax; nx++) {
+        const double kx = nx * vx;
+        for (int ny = (nx == 0 ? 0 : -nyMax); ny <= nyMax; ny++) {
+            const double ky = ny * vy;
+            for (int nz = ((nx == 0 && ny == 0) ? 1 : -nzMax); nz <= nzMax; nz++) {
+                const double kz = nz * vz;
+                const double k2 = kx * kx + ky * ky + kz * kz;
+                if (k2 < kCut2) {
+                    if (K < cap) {
+                        kvec[3 * K] = kx; kvec[3 * K + 1] = ky; kvec[3 * K + 2] = kz;
+                        kmod2[K] = k2;
+                        expk[K] = exp(-k2 * alphaR4) / k2;
+                    }
+                    ++K;
+                }
+            }
+        }
+    }
+    return K > cap ? -K : K;
+}
+
+void orc_ewald_recip(const orc_sys *s, int K, const double *kvec,
+                     const double *kmod2, const double *expk, double alpha,
+                     double volume, int use_constraint, double *force,
+                     orc_ewald_out *o) {
+    /* EwaldForce::reciprocal_term (src/forces/EwaldForce.cpp:529-698) */
+    const int n = s->n_atoms;
+    const double alphaR4 = 1.0 / (4.0 * (alpha * alpha));
+    const double volumer = 1.0 / volume;
+    const double pi4Volumer = 4.0 * PI * volumer;
+    const double pi8Volumer = 2.0 * pi4Volumer;
+    double *sc = (double *)malloc(sizeof(double) * 2 * (size_t)(n > 0 ? n : 1));
+    double pe = 0.0, ld = 0.0, ld2 = 0.0, lv[9] = {0};
+    for (int j = 0; j < K; ++j) {
+        double sinSum = 0.0, cosSum = 0.0;
+        const v3 kj = v3_ld(kvec + 3 * j);
+        v3 cosrR = {0, 0, 0}, sinrR = {0, 0, 0};
+        const double coeff = 2.0 * (alphaR4 + 1.0 / kmod2[j]);
+        for (int i = 0; i < n; ++i) {
+            const v3 rim = v3_ld(s->com + 3 * s->mol[i]);
+            const double qi = s->q[i];
+            const v3 ri = v3_ld(s->pos + 3 * i);
+            const double a = v3_dot(ri, kj);
+            const double sinAqi = qi * sin(a);
+            const double cosAqi = qi * cos(a);
+            sc[2 * i] = sinAqi;
+            sc[2 * i + 1] = cosAqi;
+            sinSum += sinAqi;
+            cosSum += cosAqi;
+            const v3 d = v3_sub(ri, rim);
+            cosrR = v3_add(cosrR, v3_scale(cosAqi, d));
+            sinrR = v3_add(sinrR, v3_scale(sinAqi, d));
+        }
+        const double te = pi4Volumer * expk[j] * (sinSum * sinSum + cosSum * cosSum);
+        pe += te;
+        const double kk = v3_dot(v3_scale(te, kj), kj);
+        ld += -te + kk * 2.0 * alphaR4;
+        {
+            const double t7 = v3_dot(v3_scale(7.0 * te, kj), kj) * 2.0 * alphaR4;
+            const double dd = kk * 2.0 * alphaR4;
+            const double t2 = v3_dot(v3_scale(dd, kj), kj) * 2.0 * alphaR4;
+            ld2 += 4.0 * te - t7 + t2;
+        }
+        lv[XX] += te - te * coeff * kj.x * kj.x;
+        lv[YY] += te - te * coeff * kj.y * kj.y;
+        lv[ZZ] += te - te * coeff * kj.z * kj.z;
+        lv[XY] -= te * coeff * kj.x * kj.y;
+        lv[XZ] -= te * coeff * kj.x * kj.z;
+        lv[YZ] -= te * coeff * kj.y * kj.z;
+        const double b = pi8Volumer * expk[j];
+        const double bkk = v3_dot(v3_scale(b, kj), kj);
+        double Ff = 0.0, Ffk2 = 0.0, d2UV1 = 0.0, d2UrR = 0.0;
+        for (int i = 0; i < n; ++i) {
+            const double val = b * (sc[2 * i] * cosSum - sc[2 * i + 1] * sinSum);
+            const double d2u1 = bkk * (sc[2 * i + 1] * cosSum + sc[2 * i] * sinSum);
+            const v3 rim = v3_ld(s->com + 3 * s->mol[i]);
+            const v3 ri = v3_ld(s->pos + 3 * i);
+            const v3 d = v3_sub(ri, rim);
+            const v3 w = v3_add(v3_scale(sc[2 * i + 1], cosrR), v3_scale(sc[2 * i], sinrR));
+            const double d2u2 = v3_dot(v3_scale(bkk, w), d);
+            acc(force + 3 * i, v3_scale(val, kj));
+            const double kd = v3_dot(kj, d);
+            Ff += val * kd;
+            Ffk2 += v3_dot(v3_scale(2.0 * val, kj), kj) * 2.0 * alphaR4 * kd;
+            d2UV1 += v3_dot(v3_scale(d2u1, d), d);
+            d2UrR += d2u2;
+        }
+        if (use_constraint) {
+            ld += Ff;
+            ld2 += -6.0 * Ff + Ffk2 - d2UV1 + d2UrR;
+        }
+    }
+    lv[YX] = lv[XY]; lv[ZX] = lv[XZ]; lv[ZY] = lv[YZ];
+    free(sc);
+    o->energy = pe; o->deriv = ld; o->deriv2 = ld2; o->extra = 0.0;
+    memcpy(o->virial, lv, sizeof lv);
+}
+
+/* ========================================================================== */
+/* (e) SHAKE and Nose-Hoover                                                  */
+/* ========================================================================== */
+
+int orc_shake(const orc_sys *s, double *pos, double *vt_minus,
+              double *velocity, double *displacement, const double *force,
+              const double *bond2, const double *mol_mom, double fnh,
+              int nose_hoover, double dt, double tol, int max_count,
+              double virial[9]) {
+    /* LFIntegrator::shake, NVT branch (src/integrators/LFIntegrator.cpp:1678-1828) */
+    const double tol_2 = 2 * tol;
+    for (int k = 0; k < 9; ++k) virial[k] = 0.0;
+    int rc = 0;
+    for (int m = 0; m < s->n_mols; ++m) {
+        const int a0 = s->mol_start[m];
+        const int ns = s->mol_start[m + 1] - a0;
+        if (ns != 3) return -3;
+        const int nBonds = ns;   /* cyclic water */
+        v3 rOld[3], rNew[3], vtPlus[3];
+        const v3 P = v3_ld(mol_mom + 3 * m);
+        for (int i = 0; i < ns; ++i) {
+            const int id = a0 + i;
+            const double mass_r = 1.0 / s->mass[id];
+            rOld[i] = v3_ld(pos + 3 * id);
+            const v3 F = v3_ld(force + 3 * id);
+            if (nose_hoover) {
+                const v3 g = v3_div(v3_scale(fnh, P), (double)ns);
+                vtPlus[i] = v3_add(v3_ld(vt_minus + 3 * id), v3_scale(dt * mass_r, v3_sub(F, g)));
+            } else {
+                vtPlus[i] = v3_add(v3_ld(vt_minus + 3 * id), v3_scale(dt * mass_r, F));
+            }
+            rNew[i] = v3_add(rOld[i], v3_scale(dt, vtPlus[i]));
+        }
+        int count = 0, done = 0;
+        while (!done && count < max_count) {
+            done = 1;
+            for (int i = 0; i < nBonds; ++i) {
+                int next = i + 1;
+                if (next == ns) next = 0;
+                v3 rijNew = v3_sub(rNew[i], rNew[next]);
+                apply_pbc(&rijNew, s->box);
+                const double rijNew2 = v3_len2(rijNew);
+                const double diff = bond2[3 * m + i] - rijNew2;
+                if (fabs(diff) > (bond2[3 * m + i] * tol_2)) {
+                    v3 rijOld = v3_sub(rOld[i], rOld[next]);
+                    apply_pbc(&rijOld, s->box);
+                    const double rijNewOld = v3_dot(rijOld, rijNew);
+                    if (rijNewOld < bond2[3 * m + i] * tol) return -1;
+                    const double mr1 = 1.0 / s->mass[a0 + i], mr2 = 1.0 / s->mass[a0 + next];
+                    const double factor = diff / (2 * rijNewOld * (mr1 + mr2));
+                    virial[XX] += factor * rijNew.x * rijOld.x;
+                    virial[XY] += factor * rijNew.x * rijOld.y;
+                    virial[XZ] += factor * rijNew.x * rijOld.z;
+                    virial[YX] += factor * rijNew.y * rijOld.x;
+                    virial[YY] += factor * rijNew.y * rijOld.y;
+                    virial[YZ] += factor * rijNew.y * rijOld.z;
+                    virial[ZX] += factor * rijNew.z * rijOld.x;
+                    virial[ZY] += factor * rijNew.z * rijOld.y;
+                    virial[ZZ] += factor * rijNew.z * rijOld.z;
+                    const v3 fo = v3_scale(factor, rijOld);
+                    rNew[i] = v3_add(rNew[i], v3_scale(mr1, fo));
+                    rNew[next] = v3_sub(rNew[next], v3_scale(mr2, fo));
+                    done = 0;
+                }
+            }
+            count++;
+        }
+        if (!done) { rc = -2; return rc; }
+        for (int i = 0; i < ns; ++i) {
+            const int id = a0 + i;
+            const v3 p0 = v3_ld(pos + 3 * id);
+            const v3 dr = v3_sub(rNew[i], p0);
+            v3_st(pos + 3 * id, v3_add(p0, dr));
+            v3_st(displacement + 3 * id, v3_add(v3_ld(displacement + 3 * id), dr));
+            const v3 vp = v3_div(dr, dt);
+            const v3 vm = v3_ld(vt_minus + 3 * id);
+            v3_st(velocity + 3 * id, v3_scale(0.5, v3_add(vp, vm)));
+            v3_st(vt_minus + 3 * id, vp);
+        }
+    }
+    for (int k = 0; k < 9; ++k) virial[k] /= (dt * dt);
+    return rc;
+}
+
+void orc_mol_kinetics(const orc_sys *s, const double *pos,
+                      const double *velocity, double *com, double *mol_mom,
+                      double *atom_ke2, double *mol_ke2) {
+    /* LFIntegrator::run (src/integrators/LFIntegrator.cpp:650-686) */
+    double ake = 0.0, mke = 0.0;
+    for (int m = 0; m < s->n_mols; ++m) {
+        v3 P = {0, 0, 0}, C = {0, 0, 0};
+        double M = 0.0;
+        for (int a = s->mol_start[m]; a < s->mol_start[m + 1]; ++a) M += s->mass[a];
+        for (int a = s->mol_start[m]; a < s->mol_start[m + 1]; ++a) {
+            const v3 v = v3_ld(velocity + 3 * a);
+            P = v3_add(P, v3_scale(s->mass[a], v));
+            C = v3_add(C, v3_scale(s->mass[a], v3_ld(pos + 3 * a)));
+            ake += v3_dot(v, v) * s->mass[a];
+        }
+        C = v3_div(C, M);
+        mke += v3_dot(P, P) / M;
+        v3_st(com + 3 * m, C);
+        v3_st(mol_mom + 3 * m, P);
+    }
+    *atom_ke2 = ake;
+    *mol_ke2 = mke;
+}
+
+void orc_nh_friction(double ke2, double ndf, double temperature,
+                     double coupl, double dt, double *fnh, double *fnh_minus) {
+    /* LFIntegrator::run (src/integrators/LFIntegrator.cpp:688-709) */
+    const double qq = ndf * BOLTZ * temperature * coupl * coupl;
+    const double invqq = 1 / qq;
+    const double fnhpl = *fnh_minus + dt * invqq * (ke2 - ndf * BOLTZ * temperature);
+    *fnh = 0.5 * (fnhpl + *fnh_minus);
+    *fnh_minus = fnhpl;
+}
+
+int orc_lf_atoms(int n, double *pos, double *vt_minus, double *velocity,
+                 double *displacement, const double *force,
+                 const double *mass, double fnh, int nose_hoover, double dt) {
+    /* LFIntegrator::run, no constraints, NVT (src/integrators/LFIntegrator.cpp:587-643) */
+    for (int i = 0; i < n; ++i) {
+        const double mass_r = 1.0 / mass[i];
+        const v3 F = v3_ld(force + 3 * i);
+        const v3 vm = v3_ld(vt_minus + 3 * i);
+        v3 vp;
+        if (nose_hoover) {
+            double velcheck = 1.0;
+            int cnt = 0;
+            v3 vel = v3_ld(velocity + 3 * i);
+            while ((velcheck > 1E-10) && (cnt < 11)) {
+                vp = v3_add(vm, v3_scale(dt, v3_sub(v3_scale(mass_r, F), v3_scale(fnh, vel))));
+                const v3 half = v3_scale(0.5, v3_add(vp, vm));
+                const v3 vv = v3_sub(half, vel);
+                velcheck = v3_len(vv);
+                vel = half;
+                cnt++;
+            }
+            if (cnt == 10) return -1;
+        } else {
+            vp = v3_add(vm, v3_scale(dt * mass_r, F));
+        }
+        const v3 dr = v3_scale(dt, vp);
+        v3_st(pos + 3 * i, v3_add(v3_ld(pos + 3 * i), dr));
+        v3_st(displacement + 3 * i, v3_add(v3_ld(displacement + 3 * i), dr));
+        v3_st(velocity + 3 * i, v3_scale(0.5, v3_add(vp, vm)));
+        v3_st(vt_minus + 3 * i, vp);
+    }
+    return 0;
+}
+
+double orc_nh_chain(double ekin, double temperature, double dt, double ndf,
+                    double press_diff, double volume, double wbar,
+                    double *vnh, double *gnh, const double *qnh, double *vlog,
+                    double *glog) {
+    /* NHIntegrator::chain (src/integrators/NHIntegrator.cpp:1814-1912) */
+    double scale = 1.0;
+    const int nc = 5, ns = 3;
+    double w[3];
+    w[0] = 1.0 / (2.0 - pow(2.0, (1.0 / 3.0)));
+    w[1] = 1.0 - 2.0 * w[0];
+    w[2] = w[0];
+    const double kT = BOLTZ * temperature;
+    const double dtnc = dt / (double)nc;
+    const double diff = press_diff;
+    const double Nf = ndf;
+    const double gn1kt = (Nf + 1.0) * kT;
+    const double odnf = 1.0 + 3.0 / Nf;
+    double expterm;
+    for (int i = 0; i < nc; i++) {
+        for (int j = 0; j < ns; j++) {
+            const double dts = w[j] * dtnc;
+            const double dts2 = 0.5 * dts, dts4 = 0.25 * dts, dts8 = 0.125 * dts;
+            gnh[3] = (qnh[2] * vnh[2] * vnh[2] - kT) / qnh[3];
+            vnh[3] = vnh[3] + gnh[3] * dts4;
+            gnh[2] = (qnh[1] * vnh[1] * vnh[1] - kT) / qnh[2];
+            expterm = exp(-vnh[3] * dts8);
+            vnh[2] = expterm * (vnh[2] * expterm + gnh[2] * dts4);
+            gnh[1] = (qnh[0] * vnh[0] * vnh[0] - kT) / qnh[1];
+            expterm = exp(-vnh[2] * dts8);
+            vnh[1] = expterm * (vnh[1] * expterm + gnh[1] * dts4);
+            gnh[0] = (2.0 * ekin + wbar * (*vlog) * (*vlog) - gn1kt) / qnh[0];
+            expterm = exp(-vnh[1] * dts8);
+            vnh[0] = expterm * (vnh[0] * expterm + gnh[0] * dts4);
+            *glog = (2.0 * ekin * odnf + 3.0 * volume * diff) / wbar;
+            expterm = exp(-vnh[0] * dts8);
+            *vlog = expterm * ((*vlog) * expterm + (*glog) * dts4);
+
+            expterm = exp(-(vnh[0] + (*vlog) * odnf) * dts2);
+            scale *= expterm;
+            ekin = ekin * expterm * expterm;
+
+            *glog = (2.0 * ekin * odnf + 3.0 * volume * diff) / wbar;
+            expterm = exp(-vnh[0] * dts8);
+            *vlog = expterm * ((*vlog) * expterm + (*glog) * dts4);
+            gnh[0] = (2.0 * ekin + wbar * (*vlog) * (*vlog) - gn1kt) / qnh[0];
+            expterm = exp(-vnh[1] * dts8);
+            vnh[0] = expterm * (vnh[0] * expterm + gnh[0] * dts4);
+            gnh[1] = (qnh[0] * vnh[0] * vnh[0] - kT) / qnh[1];
+            expterm = exp(-vnh[2] * dts8);
+            vnh[1] = expterm * (vnh[1] * expterm + gnh[1] * dts4);
+            gnh[2] = (qnh[1] * vnh[1] * vnh[1] - kT) / qnh[2];
+            expterm = exp(-vnh[3] * dts8);
+            vnh[2] = expterm * (vnh[2] * expterm + gnh[2] * dts4);
+            gnh[3] = (qnh[2] * vnh[2] * vnh[2] - kT) / qnh[3];
+            vnh[3] = vnh[3] + gnh[3] * dts4;
+        }
+    }
+    return scale;
+}
+
+/* Ensemble::drand / gauss (src/Ensemble.cpp:1831-1864) */
+static double drand_lcg(unsigned int *seed) {
+    const int l = 1029, c = 221591, m = 1048576;
+    *seed = (*seed * l + c) % m;
+    return (double)(*seed) / m;
+}
+
+static double gauss12(unsigned int *seed) {
+    const double A1 = 3.949846138, A3 = 0.252408784, A5 = 0.076542912;
+    const double A7 = 0.008355968, A9 = 0.029899776;
+    double sum = 0.0;
+    for (int i = 0; i < 12; i++) sum += drand_lcg(seed);
+    const double r = (sum - 6.0) / 4.0;
+    const double r2 = r * r;
+    const double rst = (((A9 * r2 + A7) * r2 + A5) * r2 + A3) * r2 + A1;
+    return rst * r;
+}
+
+void orc_init_velocity(const orc_sys *s, int per_atom, int ndf,
+                       double temperature, unsigned int *seed,
+                       double *velocity) {
+    /* Ensemble::init_velocity (src/Ensemble.cpp:1752-1827) */
+    const int n = per_atom ? s->n_atoms : s->n_mols;
+    double *mom = (double *)malloc(sizeof(double) * 3 * (size_t)(n > 0 ? n : 1));
+    double *mmass = NULL;
+    if (!per_atom) {
+        mmass = (double *)calloc((size_t)(s->n_mols > 0 ? s->n_mols : 1), sizeof(double));
+        for (int a = 0; a < s->n_atoms; ++a) mmass[s->mol[a]] += s->mass[a];
+    }
+    double px = 0.0, py = 0.0, pz = 0.0, tavg = 0.0;
+    for (int i = 0; i < n; ++i) {
+        mom[3 * i] = gauss12(seed);
+        mom[3 * i + 1] = gauss12(seed);
+        mom[3 * i + 2] = gauss12(seed);
+        px += mom[3 * i]; py += mom[3 * i + 1]; pz += mom[3 * i + 2];
+    }
+    px = px / n; py = py / n; pz = pz / n;
+    for (int i = 0; i < n; ++i) {
+        mom[3 * i] -= px; mom[3 * i + 1] -= py; mom[3 * i + 2] -= pz;
+        const double l2 = v3_len2(v3_ld(mom + 3 * i));
+        tavg += l2 / (per_atom ? s->mass[i] : mmass[i]);
+    }
+    tavg /= (ndf * BOLTZ);
+    const double scale = sqrt(temperature / tavg);
+    for (int i = 0; i < n; ++i) {
+        const v3 P = v3_scale(scale, v3_ld(mom + 3 * i));
+        if (per_atom) {
+            v3_st(velocity + 3 * i, v3_div(P, s->mass[i]));
+        } else {
+            const v3 v = v3_div(P, mmass[i]);
+            for (int a = s->mol_start[i]; a < s->mol_start[i + 1]; ++a) v3_st(velocity + 3 * a, v);
+        }
+    }
+    free(mom);
+    free(mmass);
+}
+
+/* ========================================================================== */
+/* one leap-frog NVT step (LFIntegrator::run, src/integrators/LFIntegrator.cpp:278-941) */
+/* ========================================================================== */
+
+static void rewrap_molecules(orc_sys *s, double *pos, double *com) {
+    /* apply PBC to the molecular centre and shift its atoms (:329-339, :897-910) */
+    for (int m = 0; m < s->n_mols; ++m) {
+        const v3 c0 = v3_ld(com + 3 * m);
+        v3 c = c0;
+        apply_pbc(&c, s->box);
+        v3_st(com + 3 * m, c);
+        const v3 diff = v3_sub(c, c0);
+        for (int a = s->mol_start[m]; a < s->mol_start[m + 1]; ++a)
+            v3_st(pos + 3 * a, v3_add(v3_ld(pos + 3 * a), diff));
+    }
+}
+
+/* Force block of run()/start(): LJ on the list, Ewald real, intramolecular
+ * and surface terms, then the reciprocal sum (:382-441, :1001-1055).  vir
+ * receives the ensemble virial (each term's virial subtracted, the long
+ * virial added). */
+static void md_forces(orc_sys *s, const orc_md_params *p, orc_md_state *st, int do_long, double vir[9]) {
+    const double volume = s->box[0] * s->box[1] * s->box[2];
+    for (int j = 0; j < 3 * s->n_atoms; ++j) st->force[j] = 0.0;
+    orc_vdw_params vp;
+    memset(&vp, 0, sizeof vp);
+    vp.kind = ORC_LJ; vp.n_types = p->n_types; vp.cutoff = p->cutoff;
+    vp.use_constraint = 1; vp.sigma2 = p->sigma2; vp.eps4 = p->eps4;
+    orc_vdw_out vo;
+    orc_vdw(s, st->off, st->nb, &vp, st->force, &vo);
+    for (int k = 0; k < 9; ++k) vir[k] = -vo.virial[k];
+    orc_ewald_out eo;
+    orc_ewald_real(s, st->off, st->nb, p->alpha, p->cutoff, 1, st->force, &eo);
+    st->e_real = eo.energy;
+    for (int k = 0; k < 9; ++k) vir[k] -= eo.virial[k];
+    orc_ewald_intra(s, p->alpha, volume, 1, st->force, &eo);
+    st->e_intra = eo.energy;
+    st->e_surf = 0.0;
+    if (p->surface) {
+        orc_ewald_surface(s, volume, 1, st->force, &eo);
+        st->e_surf = eo.energy;
+        for (int k = 0; k < 9; ++k) vir[k] -= eo.virial[k];
+    }
+    if (do_long) {
+        orc_ewald_recip(s, st->K, st->kvec, st->kmod2, st->expk, p->alpha, volume, 1, st->force, &eo);
+        st->e_long = eo.energy;
+        for (int k = 0; k < 9; ++k) vir[k] += eo.virial[k];
+    }
+    st->e_lj = vo.lj_energy;
+    double ce;
+    orc_ewald_self(s, p->alpha, volume, &st->e_self, &ce);
+    st->e_self += ce;
+}
+
+int orc_md_start(const orc_sys *s_in, const orc_md_params *p, orc_md_state *st) {
+    /* LFIntegrator::start (src/integrators/LFIntegrator.cpp:991-1174): the
+     * pair list must already be built (Ensemble::set_pairlist). */
+    orc_sys s = *s_in;
+    s.pos = st->pos;
+    s.com = st->com;
+    double vir[9];
+    md_forces(&s, p, st, 1, vir);
+    for (int i = 0; i < s.n_atoms; ++i) {
+        const double mass_r = 1.0 / s.mass[i];
+        for (int d = 0; d < 3; ++d)
+            st->vt_minus[3 * i + d] = st->velocity[3 * i + d] - 0.5 * p->dt * mass_r * st->force[3 * i + d];
+    }
+    for (int k = 0; k < 9; ++k) st->virial[k] = vir[k];
+    return 0;
+}
+
+int orc_md_step(const orc_sys *s_in, const orc_md_params *p, orc_md_state *st) {
+    orc_sys s = *s_in;
+    s.pos = st->pos;
+    s.com = st->com;
+    const int n = s.n_atoms;
+    const double drMax = p->cut_buff * p->cut_buff / 4.0;
+    int need = 0;
+    for (int j = 0; j < n && !need; ++j)
+        if (v3_len2(v3_ld(st->displacement + 3 * j)) > drMax) need = 1;
+    if (need) {
+        for (int j = 0; j < 3 * n; ++j) st->displacement[j] = 0.0;
+        rewrap_molecules(&s, st->pos, st->com);
+        int used;
+        int64_t np = orc_pairlist(&s, p->cutoff, p->cut_buff, p->use_cells, st->off, st->nb, st->cap, &used);
+        if (np < 0) return -10;
+        st->list_updates++;
+    }
+    /* remove centre-of-mass drift of vtMinus (:354-379) */
+    double vx = 0, vy = 0, vz = 0, mt = 0;
+    for (int j = 0; j < n; ++j) {
+        vx += st->vt_minus[3 * j] * s.mass[j];
+        vy += st->vt_minus[3 * j + 1] * s.mass[j];
+        vz += st->vt_minus[3 * j + 2] * s.mass[j];
+        mt += s.mass[j];
+    }
+    vx = vx / mt; vy = vy / mt; vz = vz / mt;
+    if (fabs(vx) > 1e-11 || fabs(vy) > 1e-11 || fabs(vz) > 1e-11)
+        for (int j = 0; j < n; ++j) {
+            st->vt_minus[3 * j] -= vx; st->vt_minus[3 * j + 1] -= vy; st->vt_minus[3 * j + 2] -= vz;
+            st->velocity[3 * j] -= vx; st->velocity[3 * j + 1] -= vy; st->velocity[3 * j + 2] -= vz;
+        }
+    double vir[9];
+    md_forces(&s, p, st, (p->long_steps <= 1 || (st->step % p->long_steps) == 0), vir);
+    /* SHAKE with NH molecular friction (:561-586) */
+    double sv[9];
+    int rc = orc_shake(&s, st->pos, st->vt_minus, st->velocity, st->displacement, st->force, p->bond2,
+                       st->mol_mom, st->fnh, p->nose_hoover, p->dt, p->tol, p->max_count, sv);
+    if (rc != 0) return rc;
+    /* centres, momenta, kinetic energy (:650-686) and friction (:688-722) */
+    orc_mol_kinetics(&s, st->pos, st->velocity, st->com, st->mol_mom, &st->atom_ke2, &st->mol_ke2);
+    if (p->nose_hoover) {
+        if (p->atom_thermo)
+            orc_nh_friction(st->atom_ke2, (double)(6 * s.n_mols - 3), p->temperature, p->coupl, p->dt,
+                            &st->fnh, &st->fnh_minus);
+        else
+            orc_nh_friction(st->mol_ke2, (double)(3 * s.n_mols - 3), p->temperature, p->coupl, p->dt,
+                            &st->fnh, &st->fnh_minus);
+    } else {
+        st->fnh = 0.0;
+    }
+    for (int k = 0; k < 9; ++k) st->virial[k] = vir[k] + sv[k];
+    /* per-step rewrap of molecules by centre (:897-910) */
+    rewrap_molecules(&s, st->pos, st->com);
+    st->step++;
+    return 0;
+}

@@ -1,0 +1,264 @@
+"""ctypes binding of the C oracle (oracle/liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and
+bench.py's cpu_baseline leg, as the checker; the product path never uses it.
+Each wrapper returns numpy arrays; reference citations live in oracle.h.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+
+_i32p = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+_f64p = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+
+
+def build(force: bool = False) -> str:
+    src = os.path.join(HERE, "oracle.c")
+    if force or not os.path.exists(LIB_PATH) or os.path.getmtime(LIB_PATH) < os.path.getmtime(src):
+        subprocess.check_call(["gcc", "-O2", "-std=c99", "-fPIC", "-shared", "-o", LIB_PATH, src, "-lm"])
+    return LIB_PATH
+
+
+class OrcSys(C.Structure):
+    _fields_ = [("n_atoms", C.c_int32), ("n_mols", C.c_int32),
+                ("pos", C.c_void_p), ("type", C.c_void_p), ("mol", C.c_void_p),
+                ("q", C.c_void_p), ("mass", C.c_void_p), ("mol_start", C.c_void_p),
+                ("com", C.c_void_p), ("excl_off", C.c_void_p), ("excl", C.c_void_p),
+                ("box", C.c_double * 3)]
+
+
+class VdwParams(C.Structure):
+    _fields_ = [("kind", C.c_int32), ("n_types", C.c_int32), ("cutoff", C.c_double),
+                ("switch_dist", C.c_double), ("switch_on", C.c_int32), ("use_constraint", C.c_int32),
+                ("lustig", C.c_int32), ("coulomb", C.c_int32), ("sigma2", C.c_void_p), ("eps4", C.c_void_p),
+                ("n_mie", C.c_double), ("m_mie", C.c_double), ("buck_a", C.c_void_p), ("buck_b", C.c_void_p),
+                ("buck_c", C.c_void_p), ("co2_bond", C.c_double), ("type_sigma", C.c_void_p),
+                ("type_eps", C.c_void_p), ("n_buff", C.c_double), ("m_buff", C.c_double),
+                ("delta", C.c_double), ("gamma", C.c_double), ("hfactor", C.c_double),
+                ("e_lrc", C.c_double), ("v_lrc", C.c_double), ("du_lrc", C.c_double), ("d2u_lrc", C.c_double)]
+
+
+class VdwOut(C.Structure):
+    _fields_ = [("energy", C.c_double), ("lj_energy", C.c_double), ("virial", C.c_double * 9),
+                ("ulus", C.c_double), ("dudv", C.c_double), ("d2udv", C.c_double)]
+
+
+class EwaldOut(C.Structure):
+    _fields_ = [("energy", C.c_double), ("deriv", C.c_double), ("deriv2", C.c_double),
+                ("extra", C.c_double), ("virial", C.c_double * 9)]
+
+
+class MdParams(C.Structure):
+    _fields_ = [("cutoff", C.c_double), ("cut_buff", C.c_double), ("alpha", C.c_double),
+                ("kcut", C.c_double), ("dt", C.c_double), ("temperature", C.c_double),
+                ("coupl", C.c_double), ("tol", C.c_double), ("max_count", C.c_int32),
+                ("use_cells", C.c_int32), ("long_steps", C.c_int32), ("surface", C.c_int32),
+                ("nose_hoover", C.c_int32), ("atom_thermo", C.c_int32), ("n_types", C.c_int32),
+                ("sigma2", C.c_void_p), ("eps4", C.c_void_p), ("bond2", C.c_void_p)]
+
+
+class MdState(C.Structure):
+    _fields_ = [("pos", C.c_void_p), ("vt_minus", C.c_void_p), ("velocity", C.c_void_p),
+                ("displacement", C.c_void_p), ("force", C.c_void_p), ("com", C.c_void_p),
+                ("mol_mom", C.c_void_p), ("fnh", C.c_double), ("fnh_minus", C.c_double),
+                ("step", C.c_int64), ("list_updates", C.c_int64),
+                ("off", C.c_void_p), ("nb", C.c_void_p), ("cap", C.c_int64),
+                ("kvec", C.c_void_p), ("kmod2", C.c_void_p), ("expk", C.c_void_p), ("K", C.c_int32),
+                ("e_lj", C.c_double), ("e_real", C.c_double), ("e_long", C.c_double),
+                ("e_self", C.c_double), ("e_intra", C.c_double), ("e_surf", C.c_double),
+                ("mol_ke2", C.c_double), ("atom_ke2", C.c_double), ("virial", C.c_double * 9)]
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        _lib = C.CDLL(build())
+        P = C.POINTER
+        _lib.orc_pairlist.restype = C.c_int64
+        _lib.orc_pairlist.argtypes = [P(OrcSys), C.c_double, C.c_double, C.c_int, C.c_void_p, C.c_void_p,
+                                      C.c_int64, P(C.c_int)]
+        _lib.orc_vdw.argtypes = [P(OrcSys), C.c_void_p, C.c_void_p, P(VdwParams), C.c_void_p, P(VdwOut)]
+        _lib.orc_lj_lrc.argtypes = [P(OrcSys), C.c_int, C.c_void_p, C.c_void_p, C.c_double] + [P(C.c_double)] * 4
+        _lib.orc_ewald_self.argtypes = [P(OrcSys), C.c_double, C.c_double, P(C.c_double), P(C.c_double)]
+        _lib.orc_ewald_real.argtypes = [P(OrcSys), C.c_void_p, C.c_void_p, C.c_double, C.c_double, C.c_int,
+                                        C.c_void_p, P(EwaldOut)]
+        _lib.orc_ewald_intra.argtypes = [P(OrcSys), C.c_double, C.c_double, C.c_int, C.c_void_p, P(EwaldOut)]
+        _lib.orc_ewald_surface.argtypes = [P(OrcSys), C.c_double, C.c_int, C.c_void_p, P(EwaldOut)]
+        _lib.orc_ewald_kvectors.argtypes = [C.c_double, C.c_void_p, C.c_double, C.c_void_p, C.c_void_p,
+                                            C.c_void_p, C.c_int]
+        _lib.orc_ewald_recip.argtypes = [P(OrcSys), C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_double,
+                                         C.c_double, C.c_int, C.c_void_p, P(EwaldOut)]
+        _lib.orc_shake.argtypes = [P(OrcSys)] + [C.c_void_p] * 7 + [C.c_double, C.c_int, C.c_double,
+                                                                   C.c_double, C.c_int, C.c_void_p]
+        _lib.orc_mol_kinetics.argtypes = [P(OrcSys), C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                          P(C.c_double), P(C.c_double)]
+        _lib.orc_nh_friction.argtypes = [C.c_double] * 5 + [P(C.c_double), P(C.c_double)]
+        _lib.orc_lf_atoms.argtypes = [C.c_int] + [C.c_void_p] * 6 + [C.c_double, C.c_int, C.c_double]
+        _lib.orc_nh_chain.restype = C.c_double
+        _lib.orc_nh_chain.argtypes = [C.c_double] * 7 + [C.c_void_p, C.c_void_p, C.c_void_p,
+                                                           P(C.c_double), P(C.c_double)]
+        _lib.orc_init_velocity.argtypes = [P(OrcSys), C.c_int, C.c_int, C.c_double, P(C.c_uint),
+                                           C.c_void_p]
+        _lib.orc_md_step.argtypes = [P(OrcSys), P(MdParams), P(MdState)]
+        _lib.orc_md_start.argtypes = [P(OrcSys), P(MdParams), P(MdState)]
+    return _lib
+
+
+def _p(a):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+class Oracle:
+    """Oracle view of one :class:`swinmd_amd.systems.System`."""
+
+    def __init__(self, system, pos=None, com=None):
+        s = system
+        self.s = s
+        self.pos = np.ascontiguousarray(s.pos if pos is None else pos, dtype=np.float64).reshape(-1, 3)
+        self.type = np.ascontiguousarray(s.type, dtype=np.int32)
+        self.mol = np.ascontiguousarray(s.mol, dtype=np.int32)
+        self.q = np.ascontiguousarray(s.scaled_charge, dtype=np.float64)
+        self.mass = np.ascontiguousarray(s.mass, dtype=np.float64)
+        self.mol_start = np.ascontiguousarray(s.mol_start, dtype=np.int32)
+        self.com = np.ascontiguousarray(s.com(self.pos) if com is None else com, dtype=np.float64).reshape(-1, 3)
+        self.excl_off, self.excl = s.exclusions()
+        if self.excl.size == 0:
+            self.excl = np.zeros(1, dtype=np.int32)
+        self.box = np.array(s.box, dtype=np.float64)
+        self._sys = OrcSys()
+        self._refresh()
+
+    def _refresh(self):
+        o = self._sys
+        o.n_atoms = self.s.n_atoms
+        o.n_mols = self.s.n_mols
+        o.pos, o.type, o.mol, o.q = _p(self.pos), _p(self.type), _p(self.mol), _p(self.q)
+        o.mass, o.mol_start, o.com = _p(self.mass), _p(self.mol_start), _p(self.com)
+        o.excl_off, o.excl = _p(self.excl_off), _p(self.excl)
+        for d in range(3):
+            o.box[d] = self.box[d]
+
+    @property
+    def volume(self):
+        return float(self.box[0] * self.box[1] * self.box[2])
+
+    # ---- (a) -------------------------------------------------------------
+    def pairlist(self, cutoff, cut_buff, use_cells=True):
+        n = self.s.n_atoms
+        off = np.zeros(n + 1, dtype=np.int32)
+        used = C.c_int(0)
+        cap = max(1024, n * 64)
+        while True:
+            nb = np.zeros(cap, dtype=np.int32)
+            np_ = lib().orc_pairlist(C.byref(self._sys), cutoff, cut_buff, int(use_cells), _p(off), _p(nb),
+                                     cap, C.byref(used))
+            if np_ >= 0:
+                return off, nb[:np_].copy(), bool(used.value)
+            cap = -np_
+
+    # ---- (b) -------------------------------------------------------------
+    def vdw(self, off, nb, kind="lj", *, cutoff, switch_on=False, switch_dist=0.0, use_constraint=True,
+            lustig=False, coulomb=False, n_mie=12.0, m_mie=6.0, buck=None, co2_bond=0.0, n_buff=14.0,
+            m_buff=7.0, delta=0.07, gamma=0.12, hfactor=0.91, lrc=(0.0, 0.0, 0.0, 0.0)):
+        kinds = {"lj": 0, "mie": 1, "buck": 2, "buff": 3}
+        p = VdwParams()
+        p.kind = kinds[kind]
+        p.n_types = len(self.s.types)
+        p.cutoff, p.switch_dist = cutoff, switch_dist
+        p.switch_on, p.use_constraint, p.lustig, p.coulomb = int(switch_on), int(use_constraint), int(lustig), int(coulomb)
+        sig2, eps4 = self.s.lj_table()
+        keep = [sig2, eps4]
+        p.sigma2, p.eps4 = _p(sig2), _p(eps4)
+        p.n_mie, p.m_mie = n_mie, m_mie
+        if buck is not None:
+            a, b, c = [np.ascontiguousarray(x, dtype=np.float64) for x in buck]
+            keep += [a, b, c]
+            p.buck_a, p.buck_b, p.buck_c = _p(a), _p(b), _p(c)
+        p.co2_bond = co2_bond
+        ts = np.array([t.sigma for t in self.s.types])
+        te = np.array([t.eps for t in self.s.types])
+        keep += [ts, te]
+        p.type_sigma, p.type_eps = _p(ts), _p(te)
+        p.n_buff, p.m_buff, p.delta, p.gamma, p.hfactor = n_buff, m_buff, delta, gamma, hfactor
+        p.e_lrc, p.v_lrc, p.du_lrc, p.d2u_lrc = lrc
+        f = np.zeros((self.s.n_atoms, 3))
+        out = VdwOut()
+        rc = lib().orc_vdw(C.byref(self._sys), _p(off), _p(nb), C.byref(p), _p(f), C.byref(out))
+        if rc != 0:
+            raise RuntimeError(f"orc_vdw failed: {rc}")
+        return f, {"energy": out.energy, "lj_energy": out.lj_energy, "virial": np.array(out.virial[:]),
+                   "ulus": out.ulus, "dudv": out.dudv, "d2udv": out.d2udv}
+
+    def lj_lrc(self, cutoff):
+        sig2, eps4 = self.s.lj_table()
+        vals = [C.c_double() for _ in range(4)]
+        lib().orc_lj_lrc(C.byref(self._sys), len(self.s.types), _p(sig2), _p(eps4), cutoff, *[C.byref(v) for v in vals])
+        return tuple(v.value for v in vals)   # e_lrc, v_lrc, du_lrc, d2u_lrc
+
+    # ---- Ewald -----------------------------------------------------------
+    @staticmethod
+    def _eo(out):
+        return {"energy": out.energy, "deriv": out.deriv, "deriv2": out.deriv2, "extra": out.extra,
+                "virial": np.array(out.virial[:])}
+
+    def ewald_self(self, alpha):
+        a, b = C.c_double(), C.c_double()
+        lib().orc_ewald_self(C.byref(self._sys), alpha, self.volume, C.byref(a), C.byref(b))
+        return a.value, b.value
+
+    def ewald_real(self, off, nb, alpha, real_cut, use_constraint=True):
+        f = np.zeros((self.s.n_atoms, 3))
+        out = EwaldOut()
+        lib().orc_ewald_real(C.byref(self._sys), _p(off), _p(nb), alpha, real_cut, int(use_constraint), _p(f),
+                             C.byref(out))
+        return f, self._eo(out)
+
+    def ewald_intra(self, alpha, use_constraint=True):
+        f = np.zeros((self.s.n_atoms, 3))
+        out = EwaldOut()
+        lib().orc_ewald_intra(C.byref(self._sys), alpha, self.volume, int(use_constraint), _p(f), C.byref(out))
+        return f, self._eo(out)
+
+    def ewald_surface(self, use_constraint=True):
+        f = np.zeros((self.s.n_atoms, 3))
+        out = EwaldOut()
+        lib().orc_ewald_surface(C.byref(self._sys), self.volume, int(use_constraint), _p(f), C.byref(out))
+        return f, self._eo(out)
+
+    def kvectors(self, kcut, alpha):
+        cap = 1 << 16
+        kv = np.zeros(3 * cap)
+        k2 = np.zeros(cap)
+        ek = np.zeros(cap)
+        K = lib().orc_ewald_kvectors(kcut, _p(self.box), alpha, _p(kv), _p(k2), _p(ek), cap)
+        if K < 0:
+            raise RuntimeError("too many k-vectors")
+        return kv[:3 * K].reshape(K, 3).copy(), k2[:K].copy(), ek[:K].copy()
+
+    def ewald_recip(self, kcut, alpha, use_constraint=True, kv=None):
+        kvec, kmod2, expk = self.kvectors(kcut, alpha) if kv is None else kv
+        f = np.zeros((self.s.n_atoms, 3))
+        out = EwaldOut()
+        kvec = np.ascontiguousarray(kvec)
+        lib().orc_ewald_recip(C.byref(self._sys), len(kmod2), _p(kvec), _p(kmod2), _p(expk), alpha, self.volume,
+                              int(use_constraint), _p(f), C.byref(out))
+        return f, self._eo(out)
+
+    # ---- integrator ------------------------------------------------------
+    def init_velocity(self, temperature, seed, per_atom=True, ndf=None):
+        if ndf is None:
+            ndf = 3 * self.s.n_atoms - 3
+        v = np.zeros((self.s.n_atoms, 3))
+        sd = C.c_uint(seed)
+        lib().orc_init_velocity(C.byref(self._sys), int(per_atom), int(ndf), temperature, C.byref(sd), _p(v))
+        return v

@@ -1,0 +1,302 @@
+// ref_harness.cpp -- drives the compiled SwinMD reference (objects built from
+// the reference's own sources by oracle/Makefile.ref) to produce golden
+// vectors and CPU-baseline timings.  TEST INFRASTRUCTURE ONLY: it is run by
+// tests/golden/make_golden.py and by bench.py's cpu_baseline leg, never by the
+// product path.
+//
+// Usage:  ref_harness <dir> forces <force-list>   (lj,mie,buck,buff,ewald)
+//         ref_harness <dir> pairlist
+//         ref_harness <dir> md <nsteps>
+//         ref_harness <dir> time <nsteps>
+// <dir> holds config.txt, sysData.in and coordinate.in in the reference's
+// formats; outputs (*.bin raw little-endian arrays, meta.txt) are written there.
+//
+// Reference defects are neutralised so that outputs are well defined:
+//  * Heap objects are read before they are written in several places:
+//    Parameters::read_sys_data never stores the atom types it reads
+//    (src/Parameters.cpp:112-134) and then indexes the LJ table with them
+//    (:371-380); LJForce reads eLrc/vLrc/dUlrc/d2Ulrc when the long-range
+//    correction is off (src/forces/LJForce.cpp:232-247).  The harness replaces
+//    the global operator new with a zero-filling one, so these reads see 0.
+//  * The AtomParams section is then parsed here and the LJ table rebuilt with
+//    Parameters::build_lj_param_table().
+#include <mpi.h>
+
+#include <chrono>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <fstream>
+#include <new>
+#include <sstream>
+#include <string>
+#include <vector>
+
+#include <unistd.h>
+
+// LFIntegrator.h pulls in every force header in the order the reference's own
+// translation units see them (several headers define clashing macros).
+#include "integrators/LFIntegrator.h"
+// forces/NccForce.h defines a macro named like AtomParam's polarizability
+// member; the harness reads that member, so the macro is dropped here.
+#undef polar
+
+// Zero-filled storage for every new-expression in the process.
+void* operator new(std::size_t n) {
+    void* p = std::calloc(1, n ? n : 1);
+    if (!p) throw std::bad_alloc();
+    return p;
+}
+void* operator new[](std::size_t n) { return ::operator new(n); }
+void operator delete(void* p) noexcept { std::free(p); }
+void operator delete[](void* p) noexcept { std::free(p); }
+void operator delete(void* p, std::size_t) noexcept { std::free(p); }
+void operator delete[](void* p, std::size_t) noexcept { std::free(p); }
+
+// The reference defines these two globals next to its main() (NEMDMain.cpp),
+// which the harness replaces.
+int mpi_size_g;
+int mpi_rank_g;
+
+namespace {
+
+std::string g_dir;
+
+void write_bin(const std::string& name, const void* data, size_t bytes) {
+    std::string path = g_dir + "/" + name + ".bin";
+    FILE* f = std::fopen(path.c_str(), "wb");
+    if (!f) { std::perror(path.c_str()); std::exit(2); }
+    if (bytes) std::fwrite(data, 1, bytes, f);
+    std::fclose(f);
+}
+
+struct Meta {
+    std::ostringstream os;
+    void put(const std::string& k, double v) {
+        char buf[64];
+        std::snprintf(buf, sizeof buf, "%.17g", v);
+        os << k << " " << buf << "\n";
+    }
+    void save() {
+        std::ofstream f((g_dir + "/meta.txt").c_str());
+        f << os.str();
+    }
+};
+
+// Parse the AtomParams section the way the reader intended and store it.
+void repair_atom_params(Parameters* p, const char* sysdata) {
+    std::ifstream in(sysdata);
+    std::string line;
+    while (std::getline(in, line))
+        if (line.compare(0, 11, "!AtomParams") == 0) break;
+    std::getline(in, line);  // column comment
+    for (unsigned i = 0; i < p->nAtomTypes; ++i) {
+        AtomParam ap;
+        std::memset(&ap, 0, sizeof ap);
+        std::string tname, aname;
+        in >> ap.atomType >> ap.mass >> ap.charge >> ap.eps >> ap.sigma >> ap.dipx >> ap.dipy >> ap.dipz >>
+            ap.quadxx >> ap.quadxy >> ap.quadxz >> ap.quadyy >> ap.quadyz >> ap.quadzz >> ap.polar >> tname >>
+            aname;
+        std::strncpy(ap.typeName, tname.c_str(), sizeof ap.typeName - 1);
+        std::strncpy(ap.atomName, aname.c_str(), sizeof ap.atomName - 1);
+        std::getline(in, line);
+        p->atomParams[ap.atomType] = ap;
+    }
+    p->build_lj_param_table();
+}
+
+template <class T>
+T* make_zeroed(Ensemble* e) {
+    return new T(e);   // storage is zero-filled by the operator new above
+}
+
+std::vector<double> forces_of(Ensemble* e) {
+    std::vector<double> f(3 * e->nAtoms);
+    for (int i = 0; i < e->nAtoms; ++i) {
+        f[3 * i] = e->atoms[i].force.x;
+        f[3 * i + 1] = e->atoms[i].force.y;
+        f[3 * i + 2] = e->atoms[i].force.z;
+    }
+    return f;
+}
+
+void zero_forces(Ensemble* e) {
+    for (int i = 0; i < e->nAtoms; ++i) e->atoms[i].force = 0.0;
+    for (int k = 0; k < 9; ++k) e->virial[k] = 0.0, e->longVirial[k] = 0.0;
+}
+
+void dump_state(Ensemble* e, const std::string& tag) {
+    std::vector<double> p(3 * e->nAtoms), v(3 * e->nAtoms), d(3 * e->nAtoms);
+    for (int i = 0; i < e->nAtoms; ++i) {
+        const Atom& a = e->atoms[i];
+        p[3 * i] = a.position.x; p[3 * i + 1] = a.position.y; p[3 * i + 2] = a.position.z;
+        v[3 * i] = a.velocity.x; v[3 * i + 1] = a.velocity.y; v[3 * i + 2] = a.velocity.z;
+        d[3 * i] = a.displacement.x; d[3 * i + 1] = a.displacement.y; d[3 * i + 2] = a.displacement.z;
+    }
+    write_bin(tag + "pos", p.data(), p.size() * 8);
+    write_bin(tag + "vel", v.data(), v.size() * 8);
+    write_bin(tag + "disp", d.data(), d.size() * 8);
+}
+
+}  // namespace
+
+int main(int argc, char** argv) {
+    if (argc < 3) {
+        std::fprintf(stderr, "usage: %s <dir> forces|pairlist|md|time [...]\n", argv[0]);
+        return 2;
+    }
+    MPI_Init(&argc, &argv);
+    MPI_Comm_rank(MPI_COMM_WORLD, &mpi_rank_g);
+    MPI_Comm_size(MPI_COMM_WORLD, &mpi_size_g);
+    g_dir = argv[1];
+    const std::string mode = argv[2];
+    if (chdir(g_dir.c_str()) != 0) { std::perror("chdir"); return 2; }
+
+    SimConfiguration* cfg = new SimConfiguration("config.txt");
+    cfg->read_config_file();
+    Parameters* params = new Parameters(cfg->get_sysData_file());
+    repair_atom_params(params, cfg->get_sysData_file());
+
+    Ensemble* ens = new Ensemble();
+    ens->set_mol_system(params);
+    ens->set_configuration(cfg);
+    ens->compute_bound_box();
+    ens->set_pairlist();
+    std::ofstream* sysout = new std::ofstream("sysData.out");
+    ens->set_sysdata_file(sysout);
+    ens->set_result_file(sysout);
+
+    Meta meta;
+    meta.put("box", ens->boxLx);
+    meta.put("volume", ens->volume);
+    meta.put("used_cells", ens->useCellPairList ? 1 : 0);
+    {
+        std::vector<double> v(3 * ens->nAtoms);
+        for (int i = 0; i < ens->nAtoms; ++i) {
+            v[3 * i] = ens->atoms[i].velocity.x;
+            v[3 * i + 1] = ens->atoms[i].velocity.y;
+            v[3 * i + 2] = ens->atoms[i].velocity.z;
+        }
+        write_bin("init_vel", v.data(), v.size() * 8);
+        for (int i = 0; i < ens->nAtoms; ++i) {
+            v[3 * i] = ens->atoms[i].position.x;
+            v[3 * i + 1] = ens->atoms[i].position.y;
+            v[3 * i + 2] = ens->atoms[i].position.z;
+        }
+        write_bin("init_pos", v.data(), v.size() * 8);
+        meta.put("alpha", cfg->get_alpha());
+        meta.put("kcut", cfg->get_k_cutoff());
+        meta.put("cutoff", cfg->get_cutoff());
+        meta.put("cut_buff", cfg->get_cutBuff());
+        meta.put("surface", cfg->compute_surf_correct() ? 1 : 0);
+    }
+
+    if (mode == "pairlist") {
+        std::vector<int> off(ens->nAtoms + 1, 0), nb;
+        for (int i = 0; i < ens->nAtoms; ++i) {
+            Atom& a = ens->atoms[i];
+            for (int k = 0; k < a.get_list_size(); ++k) nb.push_back(a.myPairList[k]->atomID);
+            off[i + 1] = (int)nb.size();
+        }
+        write_bin("pl_off", off.data(), off.size() * 4);
+        write_bin("pl_nb", nb.data(), nb.size() * 4);
+    } else if (mode == "forces") {
+        const std::string list = argc > 3 ? argv[3] : "lj,ewald";
+        for (int m = 0; m < ens->nMols; ++m) ens->molecules[m].mass_center();
+        std::vector<double> com(3 * ens->nMols);
+        for (int m = 0; m < ens->nMols; ++m) {
+            com[3 * m] = ens->molecules[m].massCenter.x;
+            com[3 * m + 1] = ens->molecules[m].massCenter.y;
+            com[3 * m + 2] = ens->molecules[m].massCenter.z;
+        }
+        write_bin("com", com.data(), com.size() * 8);
+        std::stringstream ss(list);
+        std::string item;
+        while (std::getline(ss, item, ',')) {
+            zero_forces(ens);
+            std::memset(&ens->myLustig, 0, sizeof ens->myLustig);
+            std::memset(&ens->myEnergy, 0, sizeof ens->myEnergy);
+            double vir[9];
+            if (item == "lj" || item == "mie" || item == "buck" || item == "buff") {
+                Force* f = nullptr;
+                if (item == "lj") f = make_zeroed<LJForce>(ens);
+                if (item == "mie") f = make_zeroed<MieForce>(ens);
+                if (item == "buck") f = make_zeroed<BuckinghamForce>(ens);
+                if (item == "buff") f = make_zeroed<BuffvdWForce>(ens);
+                f->compute();
+                f->get_virial(vir);
+                std::vector<double> F = forces_of(ens);
+                write_bin(item + "_force", F.data(), F.size() * 8);
+                write_bin(item + "_virial", vir, sizeof vir);
+                meta.put(item + "_energy", f->get_energy());
+                meta.put(item + "_ljEnergy", ens->myEnergy.ljEnergy);
+                meta.put(item + "_ulus", ens->myLustig.ljUlus);
+                meta.put(item + "_dudv", ens->myLustig.ljdUlus);
+                meta.put(item + "_d2udv", ens->myLustig.ljd2Ulus);
+            } else if (item == "ewald") {
+                EwaldForce* ew = make_zeroed<EwaldForce>(ens);
+                ew->compute();
+                ew->get_virial(vir);
+                std::vector<double> F = forces_of(ens);
+                write_bin("ewald_short_force", F.data(), F.size() * 8);
+                write_bin("ewald_short_virial", vir, sizeof vir);
+                meta.put("ewald_real", ens->myEnergy.realEnergy);
+                meta.put("ewald_correct", ens->myEnergy.correctEnergy);
+                meta.put("ewald_molCorrect", ens->myEnergy.molCorrectEnergy);
+                meta.put("ewald_surfCorrect", ens->myEnergy.surfCorrectEnergy);
+                meta.put("ewald_intravirCorrect", ens->myLustig.intravirCorrect);
+                meta.put("ewald_realDeriv", ens->myLustig.realDeriv);
+                meta.put("ewald_realSecDeriv", ens->myLustig.realSecDeriv);
+                meta.put("ewald_molDeriv", ens->myLustig.molDeriv);
+                meta.put("ewald_molSecDeriv", ens->myLustig.molSecDeriv);
+                meta.put("ewald_surfDeriv", ens->myLustig.surfDeriv);
+                meta.put("ewald_surfSecDeriv", ens->myLustig.surfSecDeriv);
+                zero_forces(ens);
+                ew->compute_long();
+                ew->get_long_virial(vir);
+                F = forces_of(ens);
+                write_bin("ewald_long_force", F.data(), F.size() * 8);
+                write_bin("ewald_long_virial", vir, sizeof vir);
+                meta.put("ewald_long", ens->myEnergy.longEnergy);
+                meta.put("ewald_longDeriv", ens->myLustig.longDeriv);
+                meta.put("ewald_longSecDeriv", ens->myLustig.longSecDeriv);
+            } else {
+                std::fprintf(stderr, "unknown force '%s'\n", item.c_str());
+                return 2;
+            }
+        }
+    } else if (mode == "md" || mode == "time") {
+        const int nsteps = argc > 3 ? std::atoi(argv[3]) : 1;
+        // The reference never initialises Molecule::massCenter before the
+        // first force call; set it, as the oracle and the HIP path do.
+        for (int m = 0; m < ens->nMols; ++m) ens->molecules[m].mass_center();
+        LFIntegrator* lf = new LFIntegrator(ens, cfg);   // initialise() -> start()
+        if (mode == "md") dump_state(ens, "start_");
+        auto t0 = std::chrono::steady_clock::now();
+        lf->run(nsteps);
+        auto t1 = std::chrono::steady_clock::now();
+        const double secs = std::chrono::duration<double>(t1 - t0).count();
+        meta.put("seconds", secs);
+        meta.put("steps", nsteps);
+        meta.put("e_lj", ens->myEnergy.ljEnergy);
+        meta.put("e_real", ens->myEnergy.realEnergy);
+        meta.put("e_long", ens->myEnergy.longEnergy);
+        meta.put("e_correct", ens->myEnergy.correctEnergy);
+        meta.put("e_molCorrect", ens->myEnergy.molCorrectEnergy);
+        meta.put("e_atomKin", ens->myEnergy.atomKinEnergy);
+        meta.put("e_molKin", ens->myEnergy.molKinEnergy);
+        if (mode == "md") {
+            dump_state(ens, "end_");
+            std::vector<double> F = forces_of(ens);
+            write_bin("end_force", F.data(), F.size() * 8);
+        }
+        std::printf("{\"steps\": %d, \"seconds\": %.6f}\n", nsteps, secs);
+    } else {
+        std::fprintf(stderr, "unknown mode '%s'\n", mode.c_str());
+        return 2;
+    }
+    meta.save();
+    std::fflush(stdout);
+    MPI_Finalize();
+    return 0;
+}
