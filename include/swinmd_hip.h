@@ -1,0 +1,266 @@
+/*
+ * swinmd_hip.h -- C ABI of libswinmd_hip.so, the MI355X (gfx950) implementation
+ * of SwinMD's nonbonded force path.
+ *
+ * The reference reaches its nonbonded terms through the Force plugin interface
+ * (src/Force.h:30-72: compute() accumulates into atoms[i].force and writes
+ * energies, Lustig derivatives and the virial) and through
+ * EwaldForce::compute_long (src/forces/EwaldForce.h:129); the integrator
+ * (src/integrators/LFIntegrator.cpp) owns SHAKE and the Nose-Hoover update.
+ * Each entry point below replaces one of those routines; a reference-side
+ * Force subclass marshals its AoS Atom array into these calls.
+ *
+ * Conventions
+ *  - Every function returns SMD_OK (0) or a negative SMD_E* code; the library
+ *    never exits.  smd_last_error() describes the last failure of a context.
+ *  - Host buffers are caller-owned; device buffers are owned by the context.
+ *  - Arrays of 3-vectors are AoS doubles [3*n] (x0 y0 z0 x1 ...), the layout of
+ *    the reference's forceArray (src/integrators/LFIntegrator.cpp:480-499).
+ *  - Virials are 9 doubles in the order XX XY XZ YX YY YZ ZX ZY ZZ
+ *    (src/NEMD_defs.h:45-53).
+ *  - Force entry points take `double* f_accum`: when non-NULL the term's forces
+ *    are added (+=) into that host array, as compute() adds into
+ *    atoms[i].force; when NULL they are added into the context's device force
+ *    accumulator (see smd_zero_forces / smd_get_forces) and nothing crosses
+ *    PCIe.
+ *  - A context is single-threaded; all work is ordered on one HIP stream that
+ *    the context owns.  Units: nm, ps, amu, kJ/mol, charges pre-scaled by
+ *    sqrt(138.93548501) as the reference's Atom::scaledCharge.
+ */
+#ifndef SWINMD_HIP_H
+#define SWINMD_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SMD_ABI_VERSION 1
+
+enum smd_status {
+    SMD_OK = 0,
+    SMD_EINVAL = -1,      /* bad argument or inconsistent topology */
+    SMD_ENOMEM = -2,      /* device allocation failed */
+    SMD_EDEVICE = -3,     /* HIP runtime error */
+    SMD_ESTATE = -4,      /* call out of order (e.g. no pair list yet) */
+    SMD_ECAPACITY = -5,   /* pair list row capacity exceeded; rebuild with more */
+    SMD_ESHAKE = -6,      /* SHAKE constraint failure (reference: ERRORMSG) */
+    SMD_ENOCONV = -7,     /* SHAKE did not converge in max_count sweeps */
+    SMD_ENOTSUP = -8      /* configuration the library does not implement */
+};
+
+enum smd_prec { SMD_PREC_FP64 = 0, SMD_PREC_MIXED = 1 };
+enum smd_vdw_kind { SMD_VDW_LJ = 0, SMD_VDW_MIE = 1, SMD_VDW_BUCK = 2, SMD_VDW_BUFF147 = 3 };
+enum smd_list_mode { SMD_LIST_CELLS = 1, SMD_LIST_BRUTE = 0 };
+
+typedef struct smd_ctx smd_ctx;
+
+/* Topology: what Parameters/Ensemble::set_mol_system hold
+ * (src/Ensemble.cpp:140-227).  Atoms of one molecule must be contiguous and
+ * in Molecule::myAtoms order; mol_id must be non-decreasing.  The exclusion
+ * table is Ensemble::exclusionTable (src/Ensemble.cpp:323-367) as CSR. */
+typedef struct {
+    int32_t n_atoms, n_mols, n_types;
+    const int32_t* atom_type;      /* [n_atoms] */
+    const int32_t* mol_id;         /* [n_atoms] */
+    const double* scaled_charge;   /* [n_atoms] */
+    const double* mass;            /* [n_atoms] */
+    const int32_t* excl_offsets;   /* [n_atoms+1] */
+    const int32_t* excl_atoms;     /* [excl_offsets[n_atoms]] */
+} smd_topology;
+
+/* ---- lifecycle ---------------------------------------------------------- */
+int smd_abi_version(void);
+const char* smd_strerror(int code);
+/* Creates a context on `device` and uploads the topology (replaces the
+ * allocation half of Ensemble::set_mol_system). */
+int smd_ctx_create(smd_ctx** out, const smd_topology* topo, int device);
+void smd_ctx_destroy(smd_ctx* ctx);
+const char* smd_last_error(const smd_ctx* ctx);
+int smd_sync(smd_ctx* ctx);
+
+/* ---- state (Atom::position / velocity / Molecule::massCenter) ----------- */
+/* Ensemble::compute_bound_box result (src/Ensemble.cpp:370-445). */
+int smd_set_box(smd_ctx* ctx, const double box[3]);
+int smd_set_positions(smd_ctx* ctx, const double* xyz, int64_t n_atoms);
+int smd_get_positions(smd_ctx* ctx, double* xyz, int64_t n_atoms);
+int smd_set_velocities(smd_ctx* ctx, const double* v, int64_t n_atoms);
+int smd_get_velocities(smd_ctx* ctx, double* v, int64_t n_atoms);
+/* Molecule::massCenter, read by the Lustig terms (LJForce.cpp:96,112). */
+int smd_set_mol_centers(smd_ctx* ctx, const double* com, int64_t n_mols);
+int smd_get_mol_centers(smd_ctx* ctx, double* com, int64_t n_mols);
+/* Molecule::mass_center for every molecule (src/Molecule.h:61-67). */
+int smd_compute_mol_centers(smd_ctx* ctx);
+int smd_zero_forces(smd_ctx* ctx);
+int smd_get_forces(smd_ctx* ctx, double* f, int64_t n_atoms);
+
+/* ---- (a) cell grid and half pair list ----------------------------------- */
+/* Ensemble::set_pairlist + update_pairlist (src/Ensemble.cpp:454-490):
+ * CellManager grid and 13-cell stencil when use_cells and the box allows it
+ * (src/utils/CellManager.cpp:43-314), else the brute-force i<j list
+ * (src/Ensemble.cpp:492-525).  Pairs satisfy r^2 < (cutoff+cut_buff)^2 under
+ * the reference's arithmetic, so the pair set is the reference's bit for bit. */
+int smd_build_pairlist(smd_ctx* ctx, double cutoff, double cut_buff, int use_cells);
+/* Sizes of the current list: total pairs, longest row, whether cells ran. */
+int smd_pairlist_info(smd_ctx* ctx, int64_t* n_pairs, int32_t* max_row, int32_t* used_cells);
+/* Per-atom rows (Atom::myPairList, src/Atom.h:61) in atom order and in the
+ * reference's visiting order within a row: offsets [n_atoms+1], nb [cap]. */
+int smd_get_pairlist(smd_ctx* ctx, int32_t* offsets, int32_t* nb, int64_t cap);
+/* Row capacity of the device list (entries per atom); 0 picks automatically. */
+int smd_set_pairlist_capacity(smd_ctx* ctx, int32_t row_capacity);
+
+/* ---- (b) van der Waals ------------------------------------------------- */
+typedef struct {
+    int32_t kind;              /* smd_vdw_kind */
+    double cutoff;             /* SimConfiguration::get_cutoff() */
+    double switch_dist;        /* get_switch_distance() (not squared) */
+    int32_t switch_on;         /* is_switch_on() */
+    int32_t use_constraint;    /* is_constraint_on(): rij.rijm in Lustig terms */
+    int32_t lustig;            /* compute_lustig() */
+    int32_t coulomb;           /* compute_coulomb(): GROMACS-shifted Coulomb */
+    /* LJ/Mie: LJPairParam sigma^2 and 4*eps per type pair [n_types^2]
+     * (Parameters::set_lj_param, src/Parameters.cpp:1360-1372). */
+    const double* sigma2;
+    const double* eps4;
+    double n_mie, m_mie;       /* MieForce exponents */
+    /* Buckingham A, B, C [n_types^2] (BuckinghamParam) and CO2 O-O length. */
+    const double* buck_a;
+    const double* buck_b;
+    const double* buck_c;
+    double co2_bond;
+    /* Buffered 14-7: per-type R and eps (AtomParam.sigma, .eps) and shape. */
+    const double* type_sigma;
+    const double* type_eps;
+    double n_buff, m_buff, delta, gamma, hfactor;
+    /* Long-range correction values, added the way compute() adds them. */
+    double e_lrc, v_lrc, du_lrc, d2u_lrc;
+} smd_vdw_params;
+
+typedef struct {
+    double energy;      /* Force::energy */
+    double lj_energy;   /* value compute() stores in myEnergy.ljEnergy */
+    double virial[9];   /* Force::virial; compute() does ensemble.virial -= virial */
+    double ulus, dudv, d2udv;   /* myLustig.ljUlus / ljdUlus / ljd2Ulus */
+} smd_vdw_out;
+
+/* LJForce::compute (src/forces/LJForce.cpp:58-255), MieForce::compute
+ * (src/forces/MieForce.cpp:62-241), BuckinghamForce::compute
+ * (src/forces/BuckinghamForce.cpp:79-287), BuffvdWForce::compute
+ * (src/forces/BuffvdWForce.cpp:93-343), over the current pair list. */
+int smd_vdw_compute(smd_ctx* ctx, const smd_vdw_params* p, int prec, double* f_accum, smd_vdw_out* out);
+
+/* LJForce::long_range_correct (src/forces/LJForce.cpp:260-370): eLrc, vLrc,
+ * dUlrc, d2Ulrc from the LJ table and the exclusions (host arithmetic). */
+int smd_lj_long_range_correct(smd_ctx* ctx, const double* sigma2, const double* eps4, double cutoff,
+                              double out4[4]);
+
+/* ---- (b)(c) Ewald for point charges ------------------------------------ */
+typedef struct {
+    double alpha;          /* get_alpha() */
+    double real_cut;       /* get_cutoff() (EwaldForce.cpp:37) */
+    double kcut;           /* get_k_cutoff() */
+    int32_t use_constraint;
+    int32_t surface;       /* compute_surf_correct() */
+} smd_ewald_params;
+
+typedef struct {
+    double real_energy, real_deriv, real_deriv2;            /* real_term */
+    double self_energy, charged_energy;                     /* ctor :69-79 */
+    double intra_energy, intra_vir_correct, mol_deriv, mol_deriv2;   /* intraMolCorrect */
+    double surf_energy, surf_deriv, surf_deriv2;            /* surfDipoleCorrect */
+    double virial[9];                                       /* EwaldForce::virial */
+} smd_ewald_out;
+
+typedef struct {
+    double energy, deriv, deriv2;   /* longEnergy, longderiv, long2ndDeriv */
+    double virial[9];               /* longVirial (all nine entries) */
+} smd_ewald_long_out;
+
+/* EwaldForce ctor + set_up (src/forces/EwaldForce.cpp:21-89, :198-257):
+ * k-vector set, exp(-k^2/4a^2)/k^2 and the self energies.  Returns K via
+ * n_kvec when non-NULL. */
+int smd_ewald_setup(smd_ctx* ctx, const smd_ewald_params* p, int32_t* n_kvec);
+/* EwaldForce::compute (:128-176): real_term, intraMolCorrect and, when
+ * enabled, surfDipoleCorrect. */
+int smd_ewald_compute(smd_ctx* ctx, int prec, double* f_accum, smd_ewald_out* out);
+/* EwaldForce::compute_long (:178-196) -> reciprocal_term (:529-698). */
+int smd_ewald_compute_long(smd_ctx* ctx, int prec, double* f_accum, smd_ewald_long_out* out);
+
+/* ---- (e) SHAKE and Nose-Hoover ----------------------------------------- */
+typedef struct {
+    double dt;              /* get_timestep() */
+    double temperature;     /* get_temperature() */
+    double coupl;           /* get_coupl_const() (NH tau) */
+    double tol;             /* get_tolerance() */
+    int32_t max_count;      /* get_max_count() */
+    int32_t nose_hoover;    /* thermotype == "NoseHoover" */
+    int32_t atom_thermo;    /* use_atom_thermo() */
+} smd_integrator_params;
+
+/* Squared constraint lengths per molecule, 3 per water (H1-O, O-H2, H-H), as
+ * LFIntegrator::initialise builds them (src/integrators/LFIntegrator.cpp:143-167). */
+int smd_set_constraints(smd_ctx* ctx, const double* bond2, int64_t n_mols);
+/* Leap-frog velocities: vtMinus (LFIntegrator::vtMinus). */
+int smd_set_vt_minus(smd_ctx* ctx, const double* v, int64_t n_atoms);
+int smd_get_vt_minus(smd_ctx* ctx, double* v, int64_t n_atoms);
+/* Molecule::momenta read by SHAKE's NH friction term. */
+int smd_set_mol_momenta(smd_ctx* ctx, const double* p, int64_t n_mols);
+/* LFIntegrator::shake NVT branch (:1678-1828) with friction fnh, using the
+ * device forces.  Updates positions, displacements, velocities and vtMinus;
+ * writes the constraint virial (already divided by dt^2). */
+int smd_shake(smd_ctx* ctx, const smd_integrator_params* p, double fnh, double virial_out[9]);
+/* Unconstrained leap-frog with the self-consistent NH velocity loop
+ * (:587-643). */
+int smd_lf_atoms(smd_ctx* ctx, const smd_integrator_params* p, double fnh);
+/* Molecular centres, momenta and kinetic energies (:650-686); out2 receives
+ * {2*atomKinEnergy, 2*molKinEnergy} as the reference accumulates them. */
+int smd_mol_kinetics(smd_ctx* ctx, double out2[2]);
+/* MTK Nose-Hoover chain (NHIntegrator::chain,
+ * src/integrators/NHIntegrator.cpp:1814-1912) on the device velocities:
+ * state4 = {vnh[4], gnh[4], qnh[4], vlog, glog, wbar}; returns the scale. */
+int smd_nh_chain(smd_ctx* ctx, double ekin, double temperature, double dt, double ndf, double press_diff,
+                 double volume, double* chain_state, double* scale_out);
+
+/* ---- whole step: LFIntegrator::run for rigid point-charge water --------- */
+typedef struct {
+    double cutoff, cut_buff;
+    smd_ewald_params ewald;
+    smd_integrator_params integ;
+    int32_t use_cells;
+    int32_t long_steps;     /* longElectTsFrequency */
+    int32_t lustig;         /* also evaluate the Lustig volume derivatives */
+    int32_t prec;           /* smd_prec of the pair and k-space math */
+    const double* sigma2;   /* LJ table [n_types^2] */
+    const double* eps4;
+    int32_t use_graph;      /* replay each step as one captured hipGraph */
+} smd_md_params;
+
+typedef struct {
+    int64_t steps, list_updates;
+    double e_lj, e_real, e_long, e_self, e_intra, e_surf;
+    double atom_ke2, mol_ke2, fnh;
+    double virial[9];
+} smd_md_report;
+
+/* LFIntegrator::initialise/start (:79-225, :991-1174): builds the pair list
+ * if none, evaluates forces and sets vtMinus = v - dt/2 F/m. */
+int smd_md_init(smd_ctx* ctx, const smd_md_params* p);
+/* nsteps iterations of LFIntegrator::run's body (:278-941) without output:
+ * Verlet trigger and rebuild, drift removal, LJ + Ewald, SHAKE, kinetics, NH
+ * friction, molecule rewrap.  Work stays on the device. */
+int smd_md_run(smd_ctx* ctx, int64_t nsteps, smd_md_report* out);
+
+/* ---- measurement -------------------------------------------------------- */
+/* When enabled, every launch of the named kernel families is bracketed by HIP
+ * events on the context stream. */
+int smd_profile_enable(smd_ctx* ctx, int on);
+/* Kernel families: "pairlist", "pairs", "ewald_intra", "recip_tables",
+ * "recip_sfac", "recip_force", "shake", "kinetics".  total_ms and launches
+ * are cumulative since enabling. */
+int smd_profile_get(smd_ctx* ctx, const char* family, double* total_ms, int64_t* launches);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -262,3 +262,55 @@ class Oracle:
         sd = C.c_uint(seed)
         lib().orc_init_velocity(C.byref(self._sys), int(per_atom), int(ndf), temperature, C.byref(sd), _p(v))
         return v
+
+
+def md_from_cfg(o: "Oracle", cfg, nsteps, velocity=None):
+    """Runs the C oracle's LFIntegrator (start() then nsteps of run()) for rigid
+    water with the settings of a golden fixture's config; returns the state."""
+    s = o.s
+    n, nm = s.n_atoms, s.n_mols
+    atom_thermo = bool(cfg.get("atom_thermo", True))
+    if velocity is None:
+        if atom_thermo:
+            ndf = 3 * n - len(s.bonds) - len(s.angles) - 3
+        else:
+            ndf = 3 * nm - 3
+        velocity = o.init_velocity(cfg["temperature"], cfg.get("seed", 1), per_atom=atom_thermo, ndf=ndf)
+    p = MdParams()
+    p.cutoff, p.cut_buff, p.alpha, p.kcut = cfg["cutoff"], cfg["cut_buff"], cfg["alpha"], cfg["kcut"]
+    p.dt, p.temperature, p.coupl, p.tol = cfg["dt"], cfg["temperature"], cfg["coupl"], cfg["shake_tol"]
+    p.max_count, p.use_cells, p.long_steps, p.surface = cfg["max_count"], int(cfg.get("use_cells", True)), 1, 0
+    p.nose_hoover, p.atom_thermo = 1, int(atom_thermo)
+    sig2, eps4 = s.lj_table()
+    bond2 = np.ascontiguousarray(s.constraint_lengths2())
+    p.n_types, p.sigma2, p.eps4, p.bond2 = len(s.types), _p(sig2), _p(eps4), _p(bond2)
+    pos = o.pos.copy()
+    vel = np.ascontiguousarray(velocity, dtype=np.float64).copy()
+    bufs = {k: np.zeros((n, 3)) for k in ("vtm", "disp", "force")}
+    com = o.com.copy()
+    mom = np.zeros((nm, 3))
+    off, nb, _ = o.pairlist(cfg["cutoff"], cfg["cut_buff"], bool(cfg.get("use_cells", True)))
+    cap = max(4 * len(nb), 1024)
+    nbuf = np.zeros(cap, dtype=np.int32)
+    nbuf[:len(nb)] = nb
+    kv, k2, ek = o.kvectors(cfg["kcut"], cfg["alpha"])
+    kv = np.ascontiguousarray(kv)
+    st = MdState()
+    st.pos, st.vt_minus, st.velocity = _p(pos), _p(bufs["vtm"]), _p(vel)
+    st.displacement, st.force, st.com, st.mol_mom = _p(bufs["disp"]), _p(bufs["force"]), _p(com), _p(mom)
+    st.off, st.nb, st.cap = _p(off), _p(nbuf), cap
+    st.kvec, st.kmod2, st.expk, st.K = _p(kv), _p(k2), _p(ek), len(k2)
+    keep = (sig2, eps4, bond2, kv, k2, ek)
+    rc = lib().orc_md_start(C.byref(o._sys), C.byref(p), C.byref(st))
+    if rc:
+        raise RuntimeError(f"orc_md_start {rc}")
+    start_vel = vel.copy()
+    for _ in range(nsteps):
+        rc = lib().orc_md_step(C.byref(o._sys), C.byref(p), C.byref(st))
+        if rc:
+            raise RuntimeError(f"orc_md_step {rc}")
+    del keep
+    return {"pos": pos, "vel": vel, "disp": bufs["disp"], "force": bufs["force"], "vtm": bufs["vtm"],
+            "com": com, "start_vel": start_vel, "fnh": st.fnh, "e_lj": st.e_lj, "e_real": st.e_real,
+            "e_long": st.e_long, "e_intra": st.e_intra, "e_self": st.e_self, "atom_ke2": st.atom_ke2,
+            "mol_ke2": st.mol_ke2, "list_updates": st.list_updates, "step": st.step}

@@ -1,0 +1,148 @@
+"""ctypes declarations for libswinmd_hip.so (include/swinmd_hip.h).
+
+The library is built in-tree by ``python -m swinmd_amd.build``.  Loading fails
+loudly when it is missing: there is no CPU fallback in this package.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libswinmd_hip.so")
+
+SMD_OK = 0
+SMD_EINVAL, SMD_ENOMEM, SMD_EDEVICE, SMD_ESTATE = -1, -2, -3, -4
+SMD_ECAPACITY, SMD_ESHAKE, SMD_ENOCONV, SMD_ENOTSUP = -5, -6, -7, -8
+PREC_FP64, PREC_MIXED = 0, 1
+VDW_LJ, VDW_MIE, VDW_BUCK, VDW_BUFF147 = 0, 1, 2, 3
+
+
+class Topology(C.Structure):
+    _fields_ = [("n_atoms", C.c_int32), ("n_mols", C.c_int32), ("n_types", C.c_int32),
+                ("atom_type", C.c_void_p), ("mol_id", C.c_void_p), ("scaled_charge", C.c_void_p),
+                ("mass", C.c_void_p), ("excl_offsets", C.c_void_p), ("excl_atoms", C.c_void_p)]
+
+
+class VdwParams(C.Structure):
+    _fields_ = [("kind", C.c_int32), ("cutoff", C.c_double), ("switch_dist", C.c_double),
+                ("switch_on", C.c_int32), ("use_constraint", C.c_int32), ("lustig", C.c_int32),
+                ("coulomb", C.c_int32), ("sigma2", C.c_void_p), ("eps4", C.c_void_p),
+                ("n_mie", C.c_double), ("m_mie", C.c_double), ("buck_a", C.c_void_p),
+                ("buck_b", C.c_void_p), ("buck_c", C.c_void_p), ("co2_bond", C.c_double),
+                ("type_sigma", C.c_void_p), ("type_eps", C.c_void_p), ("n_buff", C.c_double),
+                ("m_buff", C.c_double), ("delta", C.c_double), ("gamma", C.c_double),
+                ("hfactor", C.c_double), ("e_lrc", C.c_double), ("v_lrc", C.c_double),
+                ("du_lrc", C.c_double), ("d2u_lrc", C.c_double)]
+
+
+class VdwOut(C.Structure):
+    _fields_ = [("energy", C.c_double), ("lj_energy", C.c_double), ("virial", C.c_double * 9),
+                ("ulus", C.c_double), ("dudv", C.c_double), ("d2udv", C.c_double)]
+
+
+class EwaldParams(C.Structure):
+    _fields_ = [("alpha", C.c_double), ("real_cut", C.c_double), ("kcut", C.c_double),
+                ("use_constraint", C.c_int32), ("surface", C.c_int32)]
+
+
+class EwaldOut(C.Structure):
+    _fields_ = [("real_energy", C.c_double), ("real_deriv", C.c_double), ("real_deriv2", C.c_double),
+                ("self_energy", C.c_double), ("charged_energy", C.c_double), ("intra_energy", C.c_double),
+                ("intra_vir_correct", C.c_double), ("mol_deriv", C.c_double), ("mol_deriv2", C.c_double),
+                ("surf_energy", C.c_double), ("surf_deriv", C.c_double), ("surf_deriv2", C.c_double),
+                ("virial", C.c_double * 9)]
+
+
+class EwaldLongOut(C.Structure):
+    _fields_ = [("energy", C.c_double), ("deriv", C.c_double), ("deriv2", C.c_double),
+                ("virial", C.c_double * 9)]
+
+
+class IntegParams(C.Structure):
+    _fields_ = [("dt", C.c_double), ("temperature", C.c_double), ("coupl", C.c_double),
+                ("tol", C.c_double), ("max_count", C.c_int32), ("nose_hoover", C.c_int32),
+                ("atom_thermo", C.c_int32)]
+
+
+class MdParams(C.Structure):
+    _fields_ = [("cutoff", C.c_double), ("cut_buff", C.c_double), ("ewald", EwaldParams),
+                ("integ", IntegParams), ("use_cells", C.c_int32), ("long_steps", C.c_int32),
+                ("lustig", C.c_int32), ("prec", C.c_int32), ("sigma2", C.c_void_p), ("eps4", C.c_void_p),
+                ("use_graph", C.c_int32)]
+
+
+class MdReport(C.Structure):
+    _fields_ = [("steps", C.c_int64), ("list_updates", C.c_int64), ("e_lj", C.c_double),
+                ("e_real", C.c_double), ("e_long", C.c_double), ("e_self", C.c_double),
+                ("e_intra", C.c_double), ("e_surf", C.c_double), ("atom_ke2", C.c_double),
+                ("mol_ke2", C.c_double), ("fnh", C.c_double), ("virial", C.c_double * 9)]
+
+
+# name -> (restype, argtypes); every entry is a symbol declared in include/swinmd_hip.h
+_P = C.c_void_p
+SIGNATURES = {
+    "smd_abi_version": (C.c_int, []),
+    "smd_strerror": (C.c_char_p, [C.c_int]),
+    "smd_ctx_create": (C.c_int, [C.POINTER(C.c_void_p), C.POINTER(Topology), C.c_int]),
+    "smd_ctx_destroy": (None, [_P]),
+    "smd_last_error": (C.c_char_p, [_P]),
+    "smd_sync": (C.c_int, [_P]),
+    "smd_set_box": (C.c_int, [_P, _P]),
+    "smd_set_positions": (C.c_int, [_P, _P, C.c_int64]),
+    "smd_get_positions": (C.c_int, [_P, _P, C.c_int64]),
+    "smd_set_velocities": (C.c_int, [_P, _P, C.c_int64]),
+    "smd_get_velocities": (C.c_int, [_P, _P, C.c_int64]),
+    "smd_set_mol_centers": (C.c_int, [_P, _P, C.c_int64]),
+    "smd_get_mol_centers": (C.c_int, [_P, _P, C.c_int64]),
+    "smd_compute_mol_centers": (C.c_int, [_P]),
+    "smd_zero_forces": (C.c_int, [_P]),
+    "smd_get_forces": (C.c_int, [_P, _P, C.c_int64]),
+    "smd_build_pairlist": (C.c_int, [_P, C.c_double, C.c_double, C.c_int]),
+    "smd_pairlist_info": (C.c_int, [_P, C.POINTER(C.c_int64), C.POINTER(C.c_int32), C.POINTER(C.c_int32)]),
+    "smd_get_pairlist": (C.c_int, [_P, _P, _P, C.c_int64]),
+    "smd_set_pairlist_capacity": (C.c_int, [_P, C.c_int32]),
+    "smd_vdw_compute": (C.c_int, [_P, C.POINTER(VdwParams), C.c_int, _P, C.POINTER(VdwOut)]),
+    "smd_lj_long_range_correct": (C.c_int, [_P, _P, _P, C.c_double, _P]),
+    "smd_ewald_setup": (C.c_int, [_P, C.POINTER(EwaldParams), C.POINTER(C.c_int32)]),
+    "smd_ewald_compute": (C.c_int, [_P, C.c_int, _P, C.POINTER(EwaldOut)]),
+    "smd_ewald_compute_long": (C.c_int, [_P, C.c_int, _P, C.POINTER(EwaldLongOut)]),
+    "smd_set_constraints": (C.c_int, [_P, _P, C.c_int64]),
+    "smd_set_vt_minus": (C.c_int, [_P, _P, C.c_int64]),
+    "smd_get_vt_minus": (C.c_int, [_P, _P, C.c_int64]),
+    "smd_set_mol_momenta": (C.c_int, [_P, _P, C.c_int64]),
+    "smd_shake": (C.c_int, [_P, C.POINTER(IntegParams), C.c_double, _P]),
+    "smd_lf_atoms": (C.c_int, [_P, C.POINTER(IntegParams), C.c_double]),
+    "smd_mol_kinetics": (C.c_int, [_P, _P]),
+    "smd_nh_chain": (C.c_int, [_P, C.c_double, C.c_double, C.c_double, C.c_double, C.c_double, C.c_double,
+                               _P, C.POINTER(C.c_double)]),
+    "smd_md_init": (C.c_int, [_P, C.POINTER(MdParams)]),
+    "smd_md_run": (C.c_int, [_P, C.c_int64, C.POINTER(MdReport)]),
+    "smd_profile_enable": (C.c_int, [_P, C.c_int]),
+    "smd_profile_get": (C.c_int, [_P, C.c_char_p, C.POINTER(C.c_double), C.POINTER(C.c_int64)]),
+}
+
+_lib = None
+
+
+def load(path: str = LIB_PATH):
+    """Loads the HIP library (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise ImportError(f"{path} is missing; build it with `python -m swinmd_amd.build` "
+                          "(the package has no CPU fallback)")
+    lib = C.CDLL(path)
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+class SmdError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"{msg} (code {code})")
+        self.code = code

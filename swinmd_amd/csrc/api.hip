@@ -1,0 +1,282 @@
+// api.hip -- context lifecycle, state transfer, errors and profiling for the
+// C ABI declared in include/swinmd_hip.h.
+#include <cstdarg>
+#include <cstring>
+
+#include "smd_internal.h"
+
+namespace smd {
+
+int fail(Ctx* c, int code, const char* fmt, ...) {
+    if (c) {
+        char buf[512];
+        va_list ap;
+        va_start(ap, fmt);
+        vsnprintf(buf, sizeof buf, fmt, ap);
+        va_end(ap);
+        c->err = buf;
+    }
+    return code;
+}
+
+int hip_fail(Ctx* c, hipError_t e, const char* what) {
+    return fail(c, SMD_EDEVICE, "%s: %s", what, hipGetErrorString(e));
+}
+
+int read_terms(Ctx* c, double* dst) {
+    SMD_HIP(c, hipMemcpyAsync(c->h_pinned, c->d_terms, sizeof(double) * T_COUNT, hipMemcpyDeviceToHost, c->stream));
+    SMD_HIP(c, hipStreamSynchronize(c->stream));
+    std::memcpy(dst, c->h_pinned, sizeof(double) * T_COUNT);
+    return SMD_OK;
+}
+
+int check_status(Ctx* c) {
+    unsigned st = 0;
+    SMD_HIP(c, hipMemcpy(&st, c->d_status, sizeof st, hipMemcpyDeviceToHost));
+    if (st & ST_LIST_OVERFLOW) return fail(c, SMD_ECAPACITY, "pair list row capacity %d exceeded", c->stride);
+    if (st & ST_SHAKE_FAIL) return fail(c, SMD_ESHAKE, "Shake constraint failure");
+    if (st & ST_SHAKE_NOCONV) return fail(c, SMD_ENOCONV, "SHAKE not converged");
+    if (st & ST_NH_ITER) return fail(c, SMD_ENOCONV, "NH iteration error");
+    return SMD_OK;
+}
+
+int zero_terms(Ctx* c) {
+    SMD_HIP(c, hipMemsetAsync(c->d_terms, 0, sizeof(double) * T_COUNT, c->stream));
+    return SMD_OK;
+}
+
+static void resolve_prof(Ctx* c) {
+    if (c->prof.pending.empty()) return;
+    hipStreamSynchronize(c->stream);
+    for (auto& r : c->prof.pending) {
+        float ms = 0.f;
+        hipEventElapsedTime(&ms, r.a, r.b);
+        auto& e = c->prof.acc[r.fam];
+        e.first += ms;
+        e.second += 1;
+        hipEventDestroy(r.a);
+        hipEventDestroy(r.b);
+    }
+    c->prof.pending.clear();
+}
+
+static void release(Ctx* c) {
+    void* ptrs[] = {c->d_type, c->d_mol, c->d_mol_start, c->d_q, c->d_mass, c->d_excl_off, c->d_excl, c->d_pos,
+                    c->d_vel, c->d_vtm, c->d_disp, c->d_force, c->d_com, c->d_mom, c->d_bond2, c->d_terms,
+                    c->d_status, c->d_flag, c->d_scalars, c->d_cell_of, c->d_cell_cnt, c->d_cell_start,
+                    c->d_cell_atoms, c->d_nb, c->d_nb_cnt, c->d_kvec, c->d_kmod2, c->d_expk, c->d_kn, c->d_tab,
+                    c->d_sfac_part, c->d_sfac, c->d_vdw_a, c->d_vdw_b, c->d_vdw_c, c->d_epos, c->d_fa, c->d_fb,
+                    c->d_host};
+    for (void* p : ptrs)
+        if (p) (void)hipFree(p);
+    if (c->h_pinned) (void)hipHostFree(c->h_pinned);
+    if (c->graph_exec) (void)hipGraphExecDestroy(c->graph_exec);
+    if (c->graph) (void)hipGraphDestroy(c->graph);
+    for (auto& r : c->prof.pending) { (void)hipEventDestroy(r.a); (void)hipEventDestroy(r.b); }
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+}
+
+}  // namespace smd
+
+using namespace smd;
+
+extern "C" int smd_abi_version(void) { return SMD_ABI_VERSION; }
+
+extern "C" const char* smd_strerror(int code) {
+    switch (code) {
+    case SMD_OK: return "ok";
+    case SMD_EINVAL: return "invalid argument";
+    case SMD_ENOMEM: return "device allocation failed";
+    case SMD_EDEVICE: return "HIP runtime error";
+    case SMD_ESTATE: return "call out of order";
+    case SMD_ECAPACITY: return "pair list capacity exceeded";
+    case SMD_ESHAKE: return "SHAKE constraint failure";
+    case SMD_ENOCONV: return "iteration did not converge";
+    case SMD_ENOTSUP: return "configuration not supported";
+    default: return "unknown error";
+    }
+}
+
+extern "C" const char* smd_last_error(const smd_ctx* c) { return c ? c->err.c_str() : "null context"; }
+
+extern "C" int smd_ctx_create(smd_ctx** out, const smd_topology* t, int device) {
+    if (!out || !t) return SMD_EINVAL;
+    *out = nullptr;
+    if (t->n_atoms <= 0 || t->n_mols <= 0 || t->n_types <= 0 || !t->atom_type || !t->mol_id || !t->scaled_charge ||
+        !t->mass || !t->excl_offsets)
+        return SMD_EINVAL;
+    const int n = t->n_atoms, nm = t->n_mols;
+    // validate: types in range, molecules contiguous and non-decreasing
+    std::vector<int32_t> ms(nm + 1, 0);
+    for (int i = 0; i < n; ++i) {
+        if (t->atom_type[i] < 0 || t->atom_type[i] >= t->n_types) return SMD_EINVAL;
+        const int m = t->mol_id[i];
+        if (m < 0 || m >= nm) return SMD_EINVAL;
+        if (i > 0 && m < t->mol_id[i - 1]) return SMD_EINVAL;
+        if (i > 0 && m > t->mol_id[i - 1] + 1) return SMD_EINVAL;
+        ms[m + 1]++;
+        if (!(t->mass[i] > 0)) return SMD_EINVAL;
+    }
+    if (t->mol_id[0] != 0 || t->mol_id[n - 1] != nm - 1) return SMD_EINVAL;
+    for (int m = 0; m < nm; ++m) ms[m + 1] += ms[m];
+    if (t->excl_offsets[0] != 0) return SMD_EINVAL;
+    for (int i = 0; i < n; ++i)
+        if (t->excl_offsets[i + 1] < t->excl_offsets[i]) return SMD_EINVAL;
+    const int nex = t->excl_offsets[n];
+    if (nex > 0 && !t->excl_atoms) return SMD_EINVAL;
+    for (int k = 0; k < nex; ++k)
+        if (t->excl_atoms[k] < 0 || t->excl_atoms[k] >= n) return SMD_EINVAL;
+
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || device < 0 || device >= ndev) return SMD_EDEVICE;
+    if (hipSetDevice(device) != hipSuccess) return SMD_EDEVICE;
+
+    smd_ctx* c = new smd_ctx();
+    c->device = device;
+    c->n = n;
+    c->nmol = nm;
+    c->ntypes = t->n_types;
+    c->h_mol_start = ms;
+    c->h_type.assign(t->atom_type, t->atom_type + n);
+    c->h_mass.assign(t->mass, t->mass + n);
+    c->h_q.assign(t->scaled_charge, t->scaled_charge + n);
+    c->h_excl_off.assign(t->excl_offsets, t->excl_offsets + n + 1);
+    c->h_excl.assign(t->excl_atoms ? t->excl_atoms : t->excl_offsets, t->excl_atoms ? t->excl_atoms + nex : t->excl_offsets);
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete c;
+        return SMD_EDEVICE;
+    }
+    bool ok = true;
+    auto A = [&](auto*& p, size_t bytes) {
+        if (!ok) return;
+        if (hipMalloc(&p, bytes ? bytes : 8) != hipSuccess) ok = false;
+        else if (hipMemset(p, 0, bytes ? bytes : 8) != hipSuccess) ok = false;
+    };
+    const int nt2 = t->n_types * t->n_types;
+    A(c->d_type, sizeof(int32_t) * n);
+    A(c->d_mol, sizeof(int32_t) * n);
+    A(c->d_mol_start, sizeof(int32_t) * (nm + 1));
+    A(c->d_q, sizeof(double) * n);
+    A(c->d_mass, sizeof(double) * n);
+    A(c->d_excl_off, sizeof(int32_t) * (n + 1));
+    A(c->d_excl, sizeof(int32_t) * (nex > 0 ? nex : 1));
+    A(c->d_pos, sizeof(double) * 3 * n);
+    A(c->d_vel, sizeof(double) * 3 * n);
+    A(c->d_vtm, sizeof(double) * 3 * n);
+    A(c->d_disp, sizeof(double) * 3 * n);
+    A(c->d_force, sizeof(double) * 3 * n);
+    A(c->d_com, sizeof(double) * 3 * nm);
+    A(c->d_mom, sizeof(double) * 3 * nm);
+    A(c->d_bond2, sizeof(double) * 3 * nm);
+    A(c->d_terms, sizeof(double) * T_COUNT);
+    A(c->d_status, sizeof(unsigned));
+    A(c->d_flag, sizeof(int) * 4);
+    A(c->d_scalars, sizeof(double) * 8);
+    A(c->d_cell_of, sizeof(int32_t) * n);
+    A(c->d_cell_atoms, sizeof(int32_t) * n);
+    A(c->d_nb_cnt, sizeof(int32_t) * n);
+    A(c->d_vdw_a, sizeof(double) * nt2);
+    A(c->d_vdw_b, sizeof(double) * nt2);
+    A(c->d_vdw_c, sizeof(double) * nt2);
+    A(c->d_epos, sizeof(double) * 3 * n);
+    A(c->d_fa, sizeof(double) * n);
+    A(c->d_fb, sizeof(double) * n);
+    A(c->d_host, sizeof(int32_t) * n);
+    if (ok && hipHostMalloc(&c->h_pinned, sizeof(double) * (T_COUNT + 64)) != hipSuccess) ok = false;
+    if (!ok) {
+        release(c);
+        delete c;
+        return SMD_ENOMEM;
+    }
+    hipError_t e = hipSuccess;
+    auto H = [&](void* dst, const void* src, size_t bytes) {
+        if (e == hipSuccess) e = hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice);
+    };
+    H(c->d_type, t->atom_type, sizeof(int32_t) * n);
+    H(c->d_mol, t->mol_id, sizeof(int32_t) * n);
+    H(c->d_mol_start, ms.data(), sizeof(int32_t) * (nm + 1));
+    H(c->d_q, t->scaled_charge, sizeof(double) * n);
+    H(c->d_mass, t->mass, sizeof(double) * n);
+    H(c->d_excl_off, t->excl_offsets, sizeof(int32_t) * (n + 1));
+    if (nex > 0) H(c->d_excl, t->excl_atoms, sizeof(int32_t) * nex);
+    if (e != hipSuccess || hipDeviceSynchronize() != hipSuccess) {
+        release(c);
+        delete c;
+        return SMD_EDEVICE;
+    }
+    *out = c;
+    return SMD_OK;
+}
+
+extern "C" void smd_ctx_destroy(smd_ctx* c) {
+    if (!c) return;
+    hipSetDevice(c->device);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    release(c);
+    delete c;
+}
+
+extern "C" int smd_sync(smd_ctx* c) {
+    if (!c) return SMD_EINVAL;
+    SMD_HIP(c, hipStreamSynchronize(c->stream));
+    return SMD_OK;
+}
+
+extern "C" int smd_set_box(smd_ctx* c, const double box[3]) {
+    if (!c || !box) return SMD_EINVAL;
+    for (int d = 0; d < 3; ++d) {
+        if (!(box[d] > 0)) return fail(c, SMD_EINVAL, "box edge %d not positive", d);
+        c->box.L[d] = box[d];
+        c->box.h[d] = 0.5 * box[d];   // Ensemble::compute_bound_box halfL (src/Ensemble.cpp:431-433)
+    }
+    c->box_set = true;
+    c->list_valid = false;
+    c->ewald_ready = false;
+    c->md_ready = false;
+    return SMD_OK;
+}
+
+static int copy_in(smd_ctx* c, double* dst, const double* src, int64_t n, int64_t want) {
+    if (!c || !src || n != want) return c ? fail(c, SMD_EINVAL, "count %lld != %lld", (long long)n, (long long)want) : SMD_EINVAL;
+    SMD_HIP(c, hipStreamSynchronize(c->stream));
+    SMD_HIP(c, hipMemcpy(dst, src, sizeof(double) * 3 * n, hipMemcpyHostToDevice));
+    return SMD_OK;
+}
+
+static int copy_out(smd_ctx* c, double* dst, const double* src, int64_t n, int64_t want) {
+    if (!c || !dst || n != want) return c ? fail(c, SMD_EINVAL, "count %lld != %lld", (long long)n, (long long)want) : SMD_EINVAL;
+    SMD_HIP(c, hipStreamSynchronize(c->stream));
+    SMD_HIP(c, hipMemcpy(dst, src, sizeof(double) * 3 * n, hipMemcpyDeviceToHost));
+    return SMD_OK;
+}
+
+extern "C" int smd_set_positions(smd_ctx* c, const double* x, int64_t n) { return copy_in(c, c ? c->d_pos : nullptr, x, n, c ? c->n : 0); }
+extern "C" int smd_get_positions(smd_ctx* c, double* x, int64_t n) { return copy_out(c, x, c ? c->d_pos : nullptr, n, c ? c->n : 0); }
+extern "C" int smd_set_velocities(smd_ctx* c, const double* v, int64_t n) { return copy_in(c, c ? c->d_vel : nullptr, v, n, c ? c->n : 0); }
+extern "C" int smd_get_velocities(smd_ctx* c, double* v, int64_t n) { return copy_out(c, v, c ? c->d_vel : nullptr, n, c ? c->n : 0); }
+extern "C" int smd_set_mol_centers(smd_ctx* c, const double* x, int64_t n) { return copy_in(c, c ? c->d_com : nullptr, x, n, c ? c->nmol : 0); }
+extern "C" int smd_get_mol_centers(smd_ctx* c, double* x, int64_t n) { return copy_out(c, x, c ? c->d_com : nullptr, n, c ? c->nmol : 0); }
+extern "C" int smd_get_forces(smd_ctx* c, double* f, int64_t n) { return copy_out(c, f, c ? c->d_force : nullptr, n, c ? c->n : 0); }
+
+extern "C" int smd_zero_forces(smd_ctx* c) {
+    if (!c) return SMD_EINVAL;
+    SMD_HIP(c, hipMemsetAsync(c->d_force, 0, sizeof(double) * 3 * c->n, c->stream));
+    return SMD_OK;
+}
+
+extern "C" int smd_profile_enable(smd_ctx* c, int on) {
+    if (!c) return SMD_EINVAL;
+    resolve_prof(c);
+    c->prof.on = on != 0;
+    if (on) c->prof.acc.clear();
+    return SMD_OK;
+}
+
+extern "C" int smd_profile_get(smd_ctx* c, const char* fam, double* total_ms, int64_t* launches) {
+    if (!c || !fam) return SMD_EINVAL;
+    resolve_prof(c);
+    auto it = c->prof.acc.find(fam);
+    if (total_ms) *total_ms = it == c->prof.acc.end() ? 0.0 : it->second.first;
+    if (launches) *launches = it == c->prof.acc.end() ? 0 : it->second.second;
+    return SMD_OK;
+}

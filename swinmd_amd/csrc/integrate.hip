@@ -1,0 +1,650 @@
+// integrate.hip -- SHAKE, Nose-Hoover and the leap-frog step driver.
+//
+//   LFIntegrator::shake (NVT)       src/integrators/LFIntegrator.cpp:1678-1828
+//   unconstrained LF + NH loop      :587-643
+//   centres, momenta, kinetic E     :650-686
+//   NH friction                     :688-722
+//   drift removal                   :354-379
+//   Verlet trigger + rewrap         :306-341, :897-910
+//   NHIntegrator::chain             src/integrators/NHIntegrator.cpp:1814-1912
+//
+// One thread per molecule for SHAKE and the molecular sums (the constraint
+// sweeps are sequential per molecule in the reference and stay so here);
+// whole-system sums are block-reduced FP64 atomics.  The step is a fixed
+// sequence of kernels whose list rebuild is gated on a device flag, so it can
+// be captured once into a hipGraph and replayed.
+#include "smd_internal.h"
+
+namespace smd {
+
+struct IntegArgs {
+    int n, nmol;
+    Box box;
+    const int32_t* mol_start;
+    const double* mass;
+    double* pos;
+    double* vtm;
+    double* vel;
+    double* disp;
+    double* force;
+    double* com;
+    double* mom;
+    const double* bond2;
+    double* terms;
+    double* scalars;   // 0 fnh, 1 fnh_minus, 2 list updates, 3 steps
+    int* flag;
+    unsigned* status;
+    double dt, tol, temperature, coupl, drmax;
+    int max_count, nose_hoover, atom_thermo;
+    double fnh_host;
+    int fnh_on_device;
+};
+
+__global__ void k_step_begin(double* terms, int* flag) {
+    for (int k = threadIdx.x; k < T_COUNT; k += blockDim.x) terms[k] = 0.0;
+    if (threadIdx.x == 0) flag[0] = 0;
+}
+
+__global__ __launch_bounds__(256) void k_disp_check(IntegArgs a) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    bool need = false;
+    if (i < a.n) {
+        const double x = a.disp[3 * i], y = a.disp[3 * i + 1], z = a.disp[3 * i + 2];
+        need = len2_exact(x, y, z) > a.drmax;
+    }
+    if (__ballot(need) && (threadIdx.x % WAVE) == 0) a.flag[0] = 1;
+}
+
+// Verlet-list rebuild preamble: zero displacements, wrap molecules by centre.
+__global__ __launch_bounds__(256) void k_rebuild_prep(IntegArgs a) {
+    if (!a.flag[0]) return;
+    const int m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m == 0) a.scalars[2] += 1.0;
+    if (m >= a.nmol) return;
+    const int s = a.mol_start[m], e = a.mol_start[m + 1];
+    for (int i = s; i < e; ++i) a.disp[3 * i] = a.disp[3 * i + 1] = a.disp[3 * i + 2] = 0.0;
+    const double c0x = a.com[3 * m], c0y = a.com[3 * m + 1], c0z = a.com[3 * m + 2];
+    double cx = c0x, cy = c0y, cz = c0z;
+    pbc(cx, cy, cz, a.box);
+    const double dx = cx - c0x, dy = cy - c0y, dz = cz - c0z;
+    a.com[3 * m] = cx; a.com[3 * m + 1] = cy; a.com[3 * m + 2] = cz;
+    for (int i = s; i < e; ++i) { a.pos[3 * i] += dx; a.pos[3 * i + 1] += dy; a.pos[3 * i + 2] += dz; }
+}
+
+__global__ __launch_bounds__(256) void k_vom_reduce(IntegArgs a) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    double v[4] = {0, 0, 0, 0};
+    if (i < a.n) {
+        const double m = a.mass[i];
+        v[0] = a.vtm[3 * i] * m; v[1] = a.vtm[3 * i + 1] * m; v[2] = a.vtm[3 * i + 2] * m; v[3] = m;
+    }
+    block_add<4>(v, a.terms + T_VOM);
+}
+
+__global__ __launch_bounds__(256) void k_vom_apply(IntegArgs a) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= a.n) return;
+    a.force[3 * i] = a.force[3 * i + 1] = a.force[3 * i + 2] = 0.0;
+    const double mt = a.terms[T_MTOT];
+    const double vx = a.terms[T_VOM] / mt, vy = a.terms[T_VOM + 1] / mt, vz = a.terms[T_VOM + 2] / mt;
+    if (fabs(vx) > 1e-11 || fabs(vy) > 1e-11 || fabs(vz) > 1e-11) {
+        a.vtm[3 * i] -= vx; a.vtm[3 * i + 1] -= vy; a.vtm[3 * i + 2] -= vz;
+        a.vel[3 * i] -= vx; a.vel[3 * i + 1] -= vy; a.vel[3 * i + 2] -= vz;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_shake(IntegArgs a) {
+    double v[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    const int m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m < a.nmol) {
+        const int a0 = a.mol_start[m];
+        const double fnh = a.fnh_on_device ? a.scalars[0] : a.fnh_host;
+        const double Px = a.mom[3 * m], Py = a.mom[3 * m + 1], Pz = a.mom[3 * m + 2];
+        const double tol_2 = 2 * a.tol;
+        double ro[3][3], rn[3][3], mr[3];
+        for (int s = 0; s < 3; ++s) {
+            const int id = a0 + s;
+            mr[s] = 1.0 / a.mass[id];
+            const double h = a.dt * mr[s];
+            for (int d = 0; d < 3; ++d) {
+                ro[s][d] = a.pos[3 * id + d];
+                const double F = a.force[3 * id + d];
+                double vp;
+                if (a.nose_hoover) {
+                    const double P = d == 0 ? Px : (d == 1 ? Py : Pz);
+                    vp = a.vtm[3 * id + d] + h * (F - (fnh * P) / 3.0);
+                } else {
+                    vp = a.vtm[3 * id + d] + h * F;
+                }
+                rn[s][d] = ro[s][d] + a.dt * vp;
+            }
+        }
+        int count = 0;
+        bool done = false, failed = false;
+        while (!done && count < a.max_count && !failed) {
+            done = true;
+            for (int b = 0; b < 3; ++b) {
+                const int nx = (b + 1) % 3;
+                double nX = rn[b][0] - rn[nx][0], nY = rn[b][1] - rn[nx][1], nZ = rn[b][2] - rn[nx][2];
+                pbc(nX, nY, nZ, a.box);
+                const double bl = a.bond2[3 * m + b];
+                const double diff = bl - len2_exact(nX, nY, nZ);
+                if (fabs(diff) > bl * tol_2) {
+                    double oX = ro[b][0] - ro[nx][0], oY = ro[b][1] - ro[nx][1], oZ = ro[b][2] - ro[nx][2];
+                    pbc(oX, oY, oZ, a.box);
+                    const double rno = oX * nX + oY * nY + oZ * nZ;
+                    if (rno < bl * a.tol) { failed = true; break; }
+                    const double factor = diff / (2 * rno * (mr[b] + mr[nx]));
+                    v[0] += factor * nX * oX; v[1] += factor * nX * oY; v[2] += factor * nX * oZ;
+                    v[3] += factor * nY * oX; v[4] += factor * nY * oY; v[5] += factor * nY * oZ;
+                    v[6] += factor * nZ * oX; v[7] += factor * nZ * oY; v[8] += factor * nZ * oZ;
+                    const double fX = factor * oX, fY = factor * oY, fZ = factor * oZ;
+                    rn[b][0] += mr[b] * fX; rn[b][1] += mr[b] * fY; rn[b][2] += mr[b] * fZ;
+                    rn[nx][0] -= mr[nx] * fX; rn[nx][1] -= mr[nx] * fY; rn[nx][2] -= mr[nx] * fZ;
+                    done = false;
+                }
+            }
+            count++;
+        }
+        if (failed) atomicOr(a.status, ST_SHAKE_FAIL);
+        else if (!done) atomicOr(a.status, ST_SHAKE_NOCONV);
+        for (int s = 0; s < 3; ++s) {
+            const int id = a0 + s;
+            for (int d = 0; d < 3; ++d) {
+                const double p0 = a.pos[3 * id + d];
+                const double dr = rn[s][d] - p0;
+                a.pos[3 * id + d] = p0 + dr;
+                a.disp[3 * id + d] += dr;
+                const double vp = dr / a.dt;
+                a.vel[3 * id + d] = (vp + a.vtm[3 * id + d]) * 0.5;
+                a.vtm[3 * id + d] = vp;
+            }
+        }
+    }
+    block_add<9>(v, a.terms + T_SH_VIR);
+}
+
+__global__ __launch_bounds__(256) void k_lf_atoms(IntegArgs a) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= a.n) return;
+    const double fnh = a.fnh_on_device ? a.scalars[0] : a.fnh_host;
+    const double mr = 1.0 / a.mass[i];
+    double vp[3];
+    if (a.nose_hoover) {
+        double vel[3] = {a.vel[3 * i], a.vel[3 * i + 1], a.vel[3 * i + 2]};
+        double velcheck = 1.0;
+        int cnt = 0;
+        while (velcheck > 1E-10 && cnt < 11) {
+            double vv = 0.0, half[3];
+            for (int d = 0; d < 3; ++d) {
+                vp[d] = a.vtm[3 * i + d] + a.dt * (mr * a.force[3 * i + d] - fnh * vel[d]);
+                half[d] = (vp[d] + a.vtm[3 * i + d]) * 0.5;
+                const double dv = half[d] - vel[d];
+                vv += dv * dv;
+            }
+            velcheck = sqrt(vv);
+            for (int d = 0; d < 3; ++d) vel[d] = half[d];
+            cnt++;
+        }
+        if (cnt == 10) atomicOr(a.status, ST_NH_ITER);
+    } else {
+        for (int d = 0; d < 3; ++d) vp[d] = a.vtm[3 * i + d] + a.dt * mr * a.force[3 * i + d];
+    }
+    for (int d = 0; d < 3; ++d) {
+        const double dr = a.dt * vp[d];
+        a.pos[3 * i + d] += dr;
+        a.disp[3 * i + d] += dr;
+        a.vel[3 * i + d] = (vp[d] + a.vtm[3 * i + d]) * 0.5;
+        a.vtm[3 * i + d] = vp[d];
+    }
+}
+
+__global__ __launch_bounds__(256) void k_kinetics(IntegArgs a, int rewrap) {
+    double v[2] = {0.0, 0.0};
+    const int m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m < a.nmol) {
+        const int s = a.mol_start[m], e = a.mol_start[m + 1];
+        double M = 0.0;
+        for (int i = s; i < e; ++i) M += a.mass[i];
+        double P[3] = {0, 0, 0}, C[3] = {0, 0, 0};
+        for (int i = s; i < e; ++i) {
+            const double mi = a.mass[i];
+            double vv = 0.0;
+            for (int d = 0; d < 3; ++d) {
+                const double vd = a.vel[3 * i + d];
+                P[d] += mi * vd;
+                C[d] += mi * a.pos[3 * i + d];
+                vv += vd * vd;
+            }
+            v[0] += vv * mi;
+        }
+        for (int d = 0; d < 3; ++d) C[d] = C[d] / M;
+        v[1] = (P[0] * P[0] + P[1] * P[1] + P[2] * P[2]) / M;
+        for (int d = 0; d < 3; ++d) a.mom[3 * m + d] = P[d];
+        if (rewrap) {
+            double cx = C[0], cy = C[1], cz = C[2];
+            pbc(cx, cy, cz, a.box);
+            const double dx = cx - C[0], dy = cy - C[1], dz = cz - C[2];
+            for (int i = s; i < e; ++i) { a.pos[3 * i] += dx; a.pos[3 * i + 1] += dy; a.pos[3 * i + 2] += dz; }
+            C[0] = cx; C[1] = cy; C[2] = cz;
+        }
+        for (int d = 0; d < 3; ++d) a.com[3 * m + d] = C[d];
+    }
+    block_add<2>(v, a.terms + T_KE_ATOM);
+}
+
+__global__ void k_nh_friction(IntegArgs a, double ndf) {
+    // LFIntegrator::run (:688-709); stores fnh and fnhmi
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    a.scalars[3] += 1.0;
+    if (!a.nose_hoover) { a.scalars[0] = 0.0; return; }
+    const double ke = a.atom_thermo ? a.terms[T_KE_ATOM] : a.terms[T_KE_MOL];
+    const double qq = ndf * BOLTZ * a.temperature * a.coupl * a.coupl;
+    const double invqq = 1 / qq;
+    const double fm = a.scalars[1];
+    const double fp = fm + a.dt * invqq * (ke - ndf * BOLTZ * a.temperature);
+    a.scalars[0] = 0.5 * (fp + fm);
+    a.scalars[1] = fp;
+}
+
+__global__ __launch_bounds__(256) void k_mol_centers(int nmol, const int32_t* __restrict__ ms,
+                                                     const double* __restrict__ mass, const double* __restrict__ pos,
+                                                     double* __restrict__ com) {
+    // Molecule::mass_center (src/Molecule.h:61-67)
+    const int m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= nmol) return;
+    double M = 0.0;
+    for (int i = ms[m]; i < ms[m + 1]; ++i) M += mass[i];
+    double C[3] = {0, 0, 0};
+    for (int i = ms[m]; i < ms[m + 1]; ++i)
+        for (int d = 0; d < 3; ++d) C[d] += mass[i] * pos[3 * i + d];
+    for (int d = 0; d < 3; ++d) com[3 * m + d] = C[d] / M;
+}
+
+__global__ __launch_bounds__(256) void k_start_vtm(int n, const double* __restrict__ vel,
+                                                   const double* __restrict__ force, const double* __restrict__ mass,
+                                                   double* __restrict__ vtm, double dt) {
+    // LFIntegrator::start (:1172-1173): vtMinus = v - 0.5 dt F/m
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double mr = 1.0 / mass[i];
+    for (int d = 0; d < 3; ++d) vtm[3 * i + d] = vel[3 * i + d] - 0.5 * dt * mr * force[3 * i + d];
+}
+
+// MTK chain for one call (NHIntegrator::chain, :1814-1912), one thread.
+// st: vnh[4] gnh[4] qnh[4] vlog glog wbar ekin temp dt ndf press_diff volume
+__global__ void k_nh_chain(double* st, double* scale_out) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    double* vnh = st;
+    double* gnh = st + 4;
+    const double* qnh = st + 8;
+    double vlog = st[12], glog = st[13];
+    const double wbar = st[14];
+    double ekin = st[15];
+    const double temperature = st[16], dt = st[17], Nf = st[18], diff = st[19], volume = st[20];
+    double scale = 1.0;
+    double w[3];
+    w[0] = 1.0 / (2.0 - pow(2.0, (1.0 / 3.0)));
+    w[1] = 1.0 - 2.0 * w[0];
+    w[2] = w[0];
+    const double kT = BOLTZ * temperature;
+    const double dtnc = dt / 5.0;
+    const double gn1kt = (Nf + 1.0) * kT;
+    const double odnf = 1.0 + 3.0 / Nf;
+    double ex;
+    for (int i = 0; i < 5; i++)
+        for (int j = 0; j < 3; j++) {
+            const double dts = w[j] * dtnc;
+            const double dts2 = 0.5 * dts, dts4 = 0.25 * dts, dts8 = 0.125 * dts;
+            gnh[3] = (qnh[2] * vnh[2] * vnh[2] - kT) / qnh[3];
+            vnh[3] = vnh[3] + gnh[3] * dts4;
+            gnh[2] = (qnh[1] * vnh[1] * vnh[1] - kT) / qnh[2];
+            ex = exp(-vnh[3] * dts8);
+            vnh[2] = ex * (vnh[2] * ex + gnh[2] * dts4);
+            gnh[1] = (qnh[0] * vnh[0] * vnh[0] - kT) / qnh[1];
+            ex = exp(-vnh[2] * dts8);
+            vnh[1] = ex * (vnh[1] * ex + gnh[1] * dts4);
+            gnh[0] = (2.0 * ekin + wbar * vlog * vlog - gn1kt) / qnh[0];
+            ex = exp(-vnh[1] * dts8);
+            vnh[0] = ex * (vnh[0] * ex + gnh[0] * dts4);
+            glog = (2.0 * ekin * odnf + 3.0 * volume * diff) / wbar;
+            ex = exp(-vnh[0] * dts8);
+            vlog = ex * (vlog * ex + glog * dts4);
+            ex = exp(-(vnh[0] + vlog * odnf) * dts2);
+            scale *= ex;
+            ekin = ekin * ex * ex;
+            glog = (2.0 * ekin * odnf + 3.0 * volume * diff) / wbar;
+            ex = exp(-vnh[0] * dts8);
+            vlog = ex * (vlog * ex + glog * dts4);
+            gnh[0] = (2.0 * ekin + wbar * vlog * vlog - gn1kt) / qnh[0];
+            ex = exp(-vnh[1] * dts8);
+            vnh[0] = ex * (vnh[0] * ex + gnh[0] * dts4);
+            gnh[1] = (qnh[0] * vnh[0] * vnh[0] - kT) / qnh[1];
+            ex = exp(-vnh[2] * dts8);
+            vnh[1] = ex * (vnh[1] * ex + gnh[1] * dts4);
+            gnh[2] = (qnh[1] * vnh[1] * vnh[1] - kT) / qnh[2];
+            ex = exp(-vnh[3] * dts8);
+            vnh[2] = ex * (vnh[2] * ex + gnh[2] * dts4);
+            gnh[3] = (qnh[2] * vnh[2] * vnh[2] - kT) / qnh[3];
+            vnh[3] = vnh[3] + gnh[3] * dts4;
+        }
+    st[12] = vlog;
+    st[13] = glog;
+    st[15] = ekin;
+    *scale_out = scale;
+}
+
+__global__ __launch_bounds__(256) void k_scale(int n3, double* __restrict__ v, const double* __restrict__ s) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n3) v[i] *= *s;
+}
+
+// ---------------------------------------------------------------------------
+
+static IntegArgs integ_args(Ctx* c, const smd_integrator_params* p) {
+    IntegArgs a{};
+    a.n = c->n;
+    a.nmol = c->nmol;
+    a.box = c->box;
+    a.mol_start = c->d_mol_start;
+    a.mass = c->d_mass;
+    a.pos = c->d_pos;
+    a.vtm = c->d_vtm;
+    a.vel = c->d_vel;
+    a.disp = c->d_disp;
+    a.force = c->d_force;
+    a.com = c->d_com;
+    a.mom = c->d_mom;
+    a.bond2 = c->d_bond2;
+    a.terms = c->d_terms;
+    a.scalars = c->d_scalars;
+    a.flag = c->d_flag;
+    a.status = c->d_status;
+    if (p) {
+        a.dt = p->dt;
+        a.tol = p->tol;
+        a.temperature = p->temperature;
+        a.coupl = p->coupl;
+        a.max_count = p->max_count;
+        a.nose_hoover = p->nose_hoover;
+        a.atom_thermo = p->atom_thermo;
+    }
+    return a;
+}
+
+int shake_run(Ctx* c, const smd_integrator_params* p, bool fnh_on_device, double fnh) {
+    IntegArgs a = integ_args(c, p);
+    a.fnh_on_device = fnh_on_device ? 1 : 0;
+    a.fnh_host = fnh;
+    ProfScope ps(c, "shake");
+    hipLaunchKernelGGL(k_shake, dim3((c->nmol + 255) / 256), dim3(256), 0, c->stream, a);
+    SMD_HIP(c, hipGetLastError());
+    return SMD_OK;
+}
+
+int kinetics_run(Ctx* c, bool rewrap) {
+    IntegArgs a = integ_args(c, nullptr);
+    ProfScope ps(c, "kinetics");
+    hipLaunchKernelGGL(k_kinetics, dim3((c->nmol + 255) / 256), dim3(256), 0, c->stream, a, rewrap ? 1 : 0);
+    SMD_HIP(c, hipGetLastError());
+    return SMD_OK;
+}
+
+int lf_atoms_run(Ctx* c, const smd_integrator_params* p, double fnh) {
+    IntegArgs a = integ_args(c, p);
+    a.fnh_host = fnh;
+    hipLaunchKernelGGL(k_lf_atoms, dim3((c->n + 255) / 256), dim3(256), 0, c->stream, a);
+    SMD_HIP(c, hipGetLastError());
+    return SMD_OK;
+}
+
+static double md_ndf(Ctx* c) {
+    // Integrator ctor (src/Integrator.cpp:25-29)
+    if (c->md.integ.atom_thermo) return (double)(6 * c->nmol - 3);
+    return (double)(3 * c->nmol - 3);
+}
+
+// One LFIntegrator::run iteration enqueued on the context stream.  Contains
+// no host synchronisation and no memset, so it can be captured.
+int md_step_enqueue(Ctx* c, bool do_long) {
+    const smd_md_params& p = c->md;
+    IntegArgs a = integ_args(c, &p.integ);
+    a.fnh_on_device = 1;
+    a.drmax = p.cut_buff * p.cut_buff / 4.0;   // Integrator.cpp:36
+    const dim3 gA((c->n + 255) / 256), gM((c->nmol + 255) / 256), b256(256);
+    hipLaunchKernelGGL(k_step_begin, dim3(1), b256, 0, c->stream, c->d_terms, c->d_flag);
+    hipLaunchKernelGGL(k_disp_check, gA, b256, 0, c->stream, a);
+    hipLaunchKernelGGL(k_rebuild_prep, gM, b256, 0, c->stream, a);
+    int rc = pairlist_build(c, p.cutoff, p.cut_buff, p.use_cells, true);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_vom_reduce, gA, b256, 0, c->stream, a);
+    hipLaunchKernelGGL(k_vom_apply, gA, b256, 0, c->stream, a);
+    rc = pairs_lj_ewald(c, nullptr, nullptr, p.cutoff, p.ewald.alpha, p.ewald.real_cut, p.ewald.use_constraint,
+                        p.lustig, p.prec);
+    if (rc) return rc;
+    if ((rc = ewald_intra(c, p.lustig))) return rc;
+    if (p.ewald.surface && (rc = ewald_surface(c))) return rc;
+    if (do_long && (rc = ewald_recip(c, p.prec, p.lustig))) return rc;
+    if ((rc = shake_run(c, &p.integ, true, 0.0))) return rc;
+    if ((rc = kinetics_run(c, true))) return rc;
+    hipLaunchKernelGGL(k_nh_friction, dim3(1), dim3(64), 0, c->stream, a, md_ndf(c));
+    SMD_HIP(c, hipGetLastError());
+    return SMD_OK;
+}
+
+}  // namespace smd
+
+using namespace smd;
+
+extern "C" int smd_set_constraints(smd_ctx* c, const double* bond2, int64_t n_mols) {
+    if (!c || !bond2 || n_mols != c->nmol) return c ? fail(c, SMD_EINVAL, "constraint count mismatch") : SMD_EINVAL;
+    for (int m = 0; m < c->nmol; ++m)
+        if (c->h_mol_start[m + 1] - c->h_mol_start[m] != 3)
+            return fail(c, SMD_ENOTSUP, "SHAKE is implemented for 3-site (water) molecules");
+    SMD_HIP(c, hipMemcpy(c->d_bond2, bond2, sizeof(double) * 3 * c->nmol, hipMemcpyHostToDevice));
+    return SMD_OK;
+}
+
+extern "C" int smd_set_vt_minus(smd_ctx* c, const double* v, int64_t n) {
+    if (!c || !v || n != c->n) return SMD_EINVAL;
+    SMD_HIP(c, hipMemcpy(c->d_vtm, v, sizeof(double) * 3 * c->n, hipMemcpyHostToDevice));
+    return SMD_OK;
+}
+
+extern "C" int smd_get_vt_minus(smd_ctx* c, double* v, int64_t n) {
+    if (!c || !v || n != c->n) return SMD_EINVAL;
+    SMD_HIP(c, hipStreamSynchronize(c->stream));
+    SMD_HIP(c, hipMemcpy(v, c->d_vtm, sizeof(double) * 3 * c->n, hipMemcpyDeviceToHost));
+    return SMD_OK;
+}
+
+extern "C" int smd_set_mol_momenta(smd_ctx* c, const double* p, int64_t n) {
+    if (!c || !p || n != c->nmol) return SMD_EINVAL;
+    SMD_HIP(c, hipMemcpy(c->d_mom, p, sizeof(double) * 3 * c->nmol, hipMemcpyHostToDevice));
+    return SMD_OK;
+}
+
+extern "C" int smd_shake(smd_ctx* c, const smd_integrator_params* p, double fnh, double virial_out[9]) {
+    if (!c || !p) return SMD_EINVAL;
+    for (int m = 0; m < c->nmol; ++m)
+        if (c->h_mol_start[m + 1] - c->h_mol_start[m] != 3)
+            return fail(c, SMD_ENOTSUP, "SHAKE is implemented for 3-site (water) molecules");
+    if (!(p->dt > 0) || p->max_count < 1) return fail(c, SMD_EINVAL, "bad SHAKE parameters");
+    SMD_HIP(c, hipMemsetAsync(c->d_terms, 0, sizeof(double) * T_COUNT, c->stream));
+    SMD_HIP(c, hipMemsetAsync(c->d_status, 0, sizeof(unsigned), c->stream));
+    int rc = shake_run(c, p, false, fnh);
+    if (rc) return rc;
+    double t[T_COUNT];
+    if ((rc = read_terms(c, t))) return rc;
+    if ((rc = check_status(c))) return rc;
+    if (virial_out)
+        for (int k = 0; k < 9; ++k) virial_out[k] = t[T_SH_VIR + k] / (p->dt * p->dt);
+    return SMD_OK;
+}
+
+extern "C" int smd_lf_atoms(smd_ctx* c, const smd_integrator_params* p, double fnh) {
+    if (!c || !p) return SMD_EINVAL;
+    SMD_HIP(c, hipMemsetAsync(c->d_status, 0, sizeof(unsigned), c->stream));
+    int rc = lf_atoms_run(c, p, fnh);
+    if (rc) return rc;
+    SMD_HIP(c, hipStreamSynchronize(c->stream));
+    return check_status(c);
+}
+
+extern "C" int smd_mol_kinetics(smd_ctx* c, double out2[2]) {
+    if (!c) return SMD_EINVAL;
+    SMD_HIP(c, hipMemsetAsync(c->d_terms, 0, sizeof(double) * T_COUNT, c->stream));
+    int rc = kinetics_run(c, false);
+    if (rc) return rc;
+    double t[T_COUNT];
+    if ((rc = read_terms(c, t))) return rc;
+    if (out2) { out2[0] = t[T_KE_ATOM]; out2[1] = t[T_KE_MOL]; }
+    return SMD_OK;
+}
+
+extern "C" int smd_compute_mol_centers(smd_ctx* c) {
+    if (!c) return SMD_EINVAL;
+    hipLaunchKernelGGL(k_mol_centers, dim3((c->nmol + 255) / 256), dim3(256), 0, c->stream, c->nmol, c->d_mol_start,
+                       c->d_mass, c->d_pos, c->d_com);
+    SMD_HIP(c, hipGetLastError());
+    SMD_HIP(c, hipStreamSynchronize(c->stream));
+    return SMD_OK;
+}
+
+extern "C" int smd_nh_chain(smd_ctx* c, double ekin, double temperature, double dt, double ndf, double press_diff,
+                            double volume, double* chain_state, double* scale_out) {
+    // chain_state: vnh[4] gnh[4] qnh[4] vlog glog wbar (15 doubles), updated in place
+    if (!c || !chain_state) return SMD_EINVAL;
+    double st[21];
+    for (int k = 0; k < 15; ++k) st[k] = chain_state[k];
+    st[15] = ekin; st[16] = temperature; st[17] = dt; st[18] = ndf; st[19] = press_diff; st[20] = volume;
+    double* d = nullptr;
+    SMD_HIP(c, hipMalloc(&d, sizeof(double) * 22));
+    hipError_t e = hipMemcpy(d, st, sizeof st, hipMemcpyHostToDevice);
+    hipLaunchKernelGGL(k_nh_chain, dim3(1), dim3(64), 0, c->stream, d, d + 21);
+    hipLaunchKernelGGL(k_scale, dim3((3 * c->n + 255) / 256), dim3(256), 0, c->stream, 3 * c->n, c->d_vel, d + 21);
+    if (e == hipSuccess) e = hipGetLastError();
+    double back[22];
+    if (e == hipSuccess) e = hipMemcpyAsync(back, d, sizeof back, hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    hipFree(d);
+    if (e != hipSuccess) return hip_fail(c, e, "NH chain");
+    for (int k = 0; k < 15; ++k) chain_state[k] = back[k];
+    if (scale_out) *scale_out = back[21];
+    return SMD_OK;
+}
+
+extern "C" int smd_md_init(smd_ctx* c, const smd_md_params* p) {
+    if (!c || !p) return SMD_EINVAL;
+    if (!c->box_set) return fail(c, SMD_ESTATE, "box not set");
+    if (!p->sigma2 || !p->eps4) return fail(c, SMD_EINVAL, "LJ table required");
+    if (p->prec != SMD_PREC_FP64 && p->prec != SMD_PREC_MIXED) return fail(c, SMD_EINVAL, "bad precision");
+    for (int m = 0; m < c->nmol; ++m)
+        if (c->h_mol_start[m + 1] - c->h_mol_start[m] != 3)
+            return fail(c, SMD_ENOTSUP, "the MD step is implemented for rigid 3-site water");
+    c->md = *p;
+    const int nt2 = c->ntypes * c->ntypes;
+    c->md_sigma2.assign(p->sigma2, p->sigma2 + nt2);
+    c->md_eps4.assign(p->eps4, p->eps4 + nt2);
+    c->md.sigma2 = c->md_sigma2.data();
+    c->md.eps4 = c->md_eps4.data();
+    smd_vdw_params vp{};
+    vp.kind = SMD_VDW_LJ;
+    vp.sigma2 = c->md.sigma2;
+    vp.eps4 = c->md.eps4;
+    int rc = upload_vdw_tables(c, &vp);
+    if (rc) return rc;
+    if ((rc = ewald_setup(c, &p->ewald))) return rc;
+    if ((rc = pairlist_build(c, p->cutoff, p->cut_buff, p->use_cells, false))) return rc;
+    // Give in-step rebuilds headroom: the row capacity is fixed once a step
+    // is captured.
+    {
+        int64_t np;
+        int32_t mx, uc;
+        if ((rc = smd_pairlist_info(c, &np, &mx, &uc))) return rc;
+        const int32_t want = ((int32_t)(mx * 1.5) + 64 + 63) / 64 * 64;
+        if (want > c->stride) {
+            if ((rc = pairlist_alloc(c, want))) return rc;
+            if ((rc = pairlist_build(c, p->cutoff, p->cut_buff, p->use_cells, false))) return rc;
+        }
+    }
+    if ((rc = smd_compute_mol_centers(c))) return rc;
+    // start(): forces at t, vtMinus = v - dt/2 F/m
+    SMD_HIP(c, hipMemsetAsync(c->d_force, 0, sizeof(double) * 3 * c->n, c->stream));
+    SMD_HIP(c, hipMemsetAsync(c->d_terms, 0, sizeof(double) * T_COUNT, c->stream));
+    if ((rc = pairs_lj_ewald(c, nullptr, nullptr, p->cutoff, p->ewald.alpha, p->ewald.real_cut,
+                             p->ewald.use_constraint, p->lustig, p->prec)))
+        return rc;
+    if ((rc = ewald_intra(c, p->lustig))) return rc;
+    if (p->ewald.surface && (rc = ewald_surface(c))) return rc;
+    if ((rc = ewald_recip(c, p->prec, p->lustig))) return rc;
+    hipLaunchKernelGGL(k_start_vtm, dim3((c->n + 255) / 256), dim3(256), 0, c->stream, c->n, c->d_vel, c->d_force,
+                       c->d_mass, c->d_vtm, p->integ.dt);
+    SMD_HIP(c, hipMemsetAsync(c->d_scalars, 0, sizeof(double) * 8, c->stream));
+    SMD_HIP(c, hipMemsetAsync(c->d_mom, 0, sizeof(double) * 3 * c->nmol, c->stream));
+    SMD_HIP(c, hipMemsetAsync(c->d_disp, 0, sizeof(double) * 3 * c->n, c->stream));
+    SMD_HIP(c, hipMemsetAsync(c->d_status, 0, sizeof(unsigned), c->stream));
+    SMD_HIP(c, hipStreamSynchronize(c->stream));
+    if (c->graph_exec) { (void)hipGraphExecDestroy(c->graph_exec); c->graph_exec = nullptr; }
+    if (c->graph) { (void)hipGraphDestroy(c->graph); c->graph = nullptr; }
+    c->md_step = 0;
+    c->md_ready = true;
+    return SMD_OK;
+}
+
+extern "C" int smd_md_run(smd_ctx* c, int64_t nsteps, smd_md_report* out) {
+    if (!c || nsteps < 0) return SMD_EINVAL;
+    if (!c->md_ready) return fail(c, SMD_ESTATE, "smd_md_init not called");
+    const int ls = c->md.long_steps > 1 ? c->md.long_steps : 1;
+    const bool graph = c->md.use_graph && ls == 1 && !c->prof.on;
+    if (graph && !c->graph_exec) {
+        SMD_HIP(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+        int rc = md_step_enqueue(c, true);
+        hipGraph_t g = nullptr;
+        hipError_t e = hipStreamEndCapture(c->stream, &g);
+        if (rc) { if (g) (void)hipGraphDestroy(g); return rc; }
+        if (e != hipSuccess) return hip_fail(c, e, "hipStreamEndCapture");
+        c->graph = g;
+        SMD_HIP(c, hipGraphInstantiate(&c->graph_exec, g, nullptr, nullptr, 0));
+    }
+    for (int64_t s = 0; s < nsteps; ++s) {
+        if (graph) {
+            SMD_HIP(c, hipGraphLaunch(c->graph_exec, c->stream));
+        } else {
+            // (i % longSteps) == 0 with i counted within this call, as run() does
+            int rc = md_step_enqueue(c, (s % ls) == 0);
+            if (rc) return rc;
+        }
+        c->md_step++;
+    }
+    SMD_HIP(c, hipStreamSynchronize(c->stream));
+    int rc = check_status(c);
+    if (rc) return rc;
+    if (out) {
+        double t[T_COUNT];
+        if ((rc = read_terms(c, t))) return rc;
+        double sc[8];
+        SMD_HIP(c, hipMemcpy(sc, c->d_scalars, sizeof sc, hipMemcpyDeviceToHost));
+        *out = smd_md_report{};
+        out->steps = (int64_t)sc[3];
+        out->list_updates = (int64_t)sc[2];
+        out->e_lj = t[T_LJ_E];
+        out->e_real = t[T_EW_E];
+        out->e_long = t[T_RC_E];
+        out->e_self = c->self_energy + c->charged_energy;
+        out->e_intra = -t[T_IN_E];
+        out->atom_ke2 = t[T_KE_ATOM];
+        out->mol_ke2 = t[T_KE_MOL];
+        out->fnh = sc[0];
+        const double dt2 = c->md.integ.dt * c->md.integ.dt;
+        for (int k = 0; k < 9; ++k)
+            out->virial[k] = -t[T_LJ_VIR + k] - t[T_EW_VIR + k] + t[T_RC_VIR + k] + t[T_SH_VIR + k] / dt2 -
+                             (c->md.ewald.surface ? t[T_SU_VIR + k] : 0.0);
+        if (c->md.ewald.surface) {
+            const double vol = c->box.L[0] * c->box.L[1] * c->box.L[2];
+            const double* qr = t + T_SU_QR;
+            out->e_surf = 2.0 * PI * (1.0 / vol) / 3.0 * (qr[0] * qr[0] + qr[1] * qr[1] + qr[2] * qr[2]);
+        }
+    }
+    return SMD_OK;
+}

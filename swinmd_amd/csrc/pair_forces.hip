@@ -1,0 +1,681 @@
+// pair_forces.hip -- short-range pair terms over the half pair list.
+//
+// One kernel template covers the reference's pair loops:
+//   LJForce::compute          src/forces/LJForce.cpp:58-255
+//   MieForce::compute         src/forces/MieForce.cpp:62-241
+//   BuckinghamForce::compute  src/forces/BuckinghamForce.cpp:79-287
+//   BuffvdWForce::compute     src/forces/BuffvdWForce.cpp:93-343
+//   EwaldForce::real_term     src/forces/EwaldForce.cpp:296-527
+// and the fused LJ + Ewald real-space pass the MD step uses.
+//
+// Work mapping (gfx950, wave64): G consecutive lanes share one owner atom i
+// and stride through its ELL row, so a wave covers 64/G owners.  The owner's
+// force stays in registers and is combined across the G lanes with
+// cross-lane shuffles at the end; partner forces go out as FP64 atomics
+// (global_atomic_add_f64).  Energies, Lustig derivatives and virials are
+// reduced per block and added once per block.  The pair math runs in R
+// (double, or float for the mixed-precision form); coordinate differences and
+// all accumulation stay FP64.
+#include "smd_internal.h"
+
+namespace smd {
+
+enum VdwKind { V_NONE = 0, V_LJ = 1, V_MIE = 2, V_BUCK = 3, V_BUFF = 4 };
+
+struct PairArgs {
+    int n;
+    Box box;
+    const double* pos;
+    const double* epos;     // virtual-site positions (BUCK, BUFF)
+    const double* q;
+    const int32_t* type;
+    const int32_t* mol;
+    const double* com;
+    const int32_t* nb;
+    const int32_t* nb_cnt;
+    int32_t stride;
+    double* force;
+    double* terms;
+    // vdW
+    int ntypes;
+    const double* ta;
+    const double* tb;
+    const double* tc;
+    const double* fa;
+    const double* fb;
+    const int32_t* host;
+    double cutoff, cut2, rcut_2;
+    int switch_on, use_constraint, coulomb;
+    double sw_dist2;
+    double c1, c2, c3;            // CHARMM-type switch (Mie, Buckingham)
+    double s0, s1, s2, s3, s4, s5; // 5th-order taper (buffered 14-7)
+    double n_mie, m_mie, cn_mie;
+    double n_buff, m_buff, delta, gamma;
+    // Ewald
+    double alpha, alpha2, alpha2PI, realcut2;
+};
+
+template <typename R>
+__device__ __forceinline__ R rsqrt_(R x) { return R(1) / sqrt(x); }
+
+template <typename R> __device__ __forceinline__ R erfc_(R x);
+template <> __device__ __forceinline__ double erfc_<double>(double x) { return erfc(x); }
+template <> __device__ __forceinline__ float erfc_<float>(float x) { return erfcf(x); }
+template <typename R> __device__ __forceinline__ R exp_(R x);
+template <> __device__ __forceinline__ double exp_<double>(double x) { return exp(x); }
+template <> __device__ __forceinline__ float exp_<float>(float x) { return expf(x); }
+template <typename R> __device__ __forceinline__ R pow_(R x, R y);
+template <> __device__ __forceinline__ double pow_<double>(double x, double y) { return pow(x, y); }
+template <> __device__ __forceinline__ float pow_<float>(float x, float y) { return powf(x, y); }
+
+// Accumulator layout inside the kernel (per lane, FP64).
+enum {
+    A_VE = 0, A_VELEC, A_VDUDV, A_VD2UDV,   // vdW energy, shifted Coulomb, Lustig
+    A_VVIR,                                  // 9
+    A_EE = A_VVIR + 9, A_ED1, A_ED2,
+    A_EVIR,                                  // 9
+    A_N = A_EVIR + 9
+};
+
+template <int VDW, bool EWALD, bool LUSTIG, typename R, int G>
+__global__ __launch_bounds__(256) void k_pairs(PairArgs a) {
+    const int tid = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = tid / G;
+    const int sub = tid % G;
+    double acc[A_N];
+#pragma unroll
+    for (int k = 0; k < A_N; ++k) acc[k] = 0.0;
+    double fix = 0.0, fiy = 0.0, fiz = 0.0;
+
+    if (i < a.n) {
+        const int cnt = min(a.nb_cnt[i], a.stride);
+        const int32_t* row = a.nb + (int64_t)i * a.stride;
+        const double pix = a.pos[3 * i], piy = a.pos[3 * i + 1], piz = a.pos[3 * i + 2];
+        double eix = pix, eiy = piy, eiz = piz;
+        if constexpr (VDW == V_BUCK || VDW == V_BUFF) {
+            eix = a.epos[3 * i]; eiy = a.epos[3 * i + 1]; eiz = a.epos[3 * i + 2];
+        }
+        const double qi = a.q[i];
+        const int ti = a.type[i];
+        const int mi = a.mol[i];
+        double cix = 0, ciy = 0, ciz = 0;
+        if constexpr (LUSTIG) { cix = a.com[3 * mi]; ciy = a.com[3 * mi + 1]; ciz = a.com[3 * mi + 2]; }
+
+        for (int k = sub; k < cnt; k += G) {
+            const int j = row[k];
+            const int mj = a.mol[j];
+            // rijm = massCenter_j - massCenter_i, minimum image (LJForce.cpp:112,116)
+            R mx = 0, my = 0, mz = 0;
+            if constexpr (LUSTIG && VDW != V_BUCK && VDW != V_BUFF) {
+                double dx = a.com[3 * mj] - cix, dy = a.com[3 * mj + 1] - ciy, dz = a.com[3 * mj + 2] - ciz;
+                pbc(dx, dy, dz, a.box);
+                mx = (R)dx; my = (R)dy; mz = (R)dz;
+            }
+            double gfx = 0.0, gfy = 0.0, gfz = 0.0;   // force on i from this pair (j gets minus)
+
+            if constexpr (VDW == V_LJ || VDW == V_MIE || EWALD) {
+                double dx = a.pos[3 * j] - pix, dy = a.pos[3 * j + 1] - piy, dz = a.pos[3 * j + 2] - piz;
+                pbc(dx, dy, dz, a.box);
+                const R rx = (R)dx, ry = (R)dy, rz = (R)dz;
+                const R r2 = rx * rx + ry * ry + rz * rz;
+
+                if constexpr (VDW == V_LJ || VDW == V_MIE) {
+                    if (mi != mj) {
+                        const R r = sqrt(r2);
+                        const R rij2 = r * r;
+                        if (rij2 <= (R)a.cut2) {
+                            const int t = ti * a.ntypes + a.type[j];
+                            const R r_1 = R(1) / r;
+                            const R rij_2 = r_1 * r_1;
+                            R tmpE, tmpf, dulj = 0, d2ulj = 0;
+                            const R rijRij = (LUSTIG && a.use_constraint) ? (rx * mx + ry * my + rz * mz) : rij2;
+                            if constexpr (VDW == V_LJ) {
+                                const R sigma2 = (R)a.ta[t], eps4 = (R)a.tb[t];
+                                const R sr2 = sigma2 * rij_2;
+                                const R sr6 = sr2 * sr2 * sr2;
+                                const R sr12 = sr6 * sr6;
+                                tmpE = eps4 * (sr12 - sr6);
+                                tmpf = (sr12 + sr12 - sr6) * R(6) * eps4 * rij_2;
+                                if constexpr (LUSTIG) {
+                                    dulj = tmpf * rijRij;
+                                    d2ulj = eps4 * (R(156) * sr12 - R(42) * sr6) * rijRij * rijRij * rij_2 * rij_2 + R(2) * dulj;
+                                }
+                                if (a.switch_on) {   // energy shift only (LJForce.cpp:152-168)
+                                    const R sr2c = sigma2 * (R)a.rcut_2;
+                                    const R sr6c = sr2c * sr2c * sr2c;
+                                    tmpE = tmpE - eps4 * (sr6c * sr6c - sr6c);
+                                }
+                            } else {
+                                const R sigma = sqrt((R)a.ta[t]);
+                                const R eps = R(0.25) * (R)a.tb[t];
+                                const R sg = sigma * r_1;
+                                const R nm = (R)a.n_mie, mm = (R)a.m_mie, cn = (R)a.cn_mie;
+                                const R srn = pow_<R>(sg, nm), srm = pow_<R>(sg, mm);
+                                tmpE = cn * eps * (srn - srm);
+                                tmpf = cn * eps * (nm * srn - mm * srm) * rij_2;
+                                if constexpr (LUSTIG) {
+                                    const R rr = rx * mx + ry * my + rz * mz;   // MieForce.cpp:140 (always rij.rijm)
+                                    dulj = tmpf * rr;
+                                    d2ulj = cn * eps * (nm * (nm + 1) * srn - mm * (mm + 1) * srm) * rr * rr * rij_2 * rij_2 + R(2) * dulj;
+                                }
+                                if (a.switch_on && rij2 > (R)a.sw_dist2) {
+                                    const R cc = (R)a.cut2 - rij2;
+                                    const R c4 = cc * (R(2) * rij2 + (R)a.c2);
+                                    const R sv = cc * c4 * (R)a.c1;
+                                    const R dsv = (R)a.c3 * (cc * cc - c4);
+                                    tmpf = tmpf * sv - tmpE * dsv;
+                                    tmpE *= sv;
+                                }
+                            }
+                            acc[A_VE] += (double)tmpE;
+                            if constexpr (LUSTIG) { acc[A_VDUDV] += (double)dulj; acc[A_VD2UDV] += (double)d2ulj; }
+                            if (a.coulomb) {   // GROMACS shift (LJForce.cpp:188-201)
+                                const R qij = (R)(qi * a.q[j]);
+                                const R e = qij * r_1;
+                                const R f = e * rij_2;
+                                const R r_rc = r / (R)a.cutoff;
+                                const R r4 = rij2 * rij2 * (R)a.rcut_2 * (R)a.rcut_2;
+                                acc[A_VELEC] += (double)(e * (R(1) - R(1.666667) * r_rc + R(1.666667) * r4 - r_rc * r4));
+                                tmpf += f * (R(1) - R(5) * r4 + R(4) * r_rc * r4);
+                            }
+                            const double fx = (double)(-tmpf * rx), fy = (double)(-tmpf * ry), fz = (double)(-tmpf * rz);
+                            gfx += fx; gfy += fy; gfz += fz;
+                            double* v = acc + A_VVIR;
+                            v[0] += fx * dx; v[1] += fy * dx; v[2] += fz * dx;
+                            v[3] += fx * dy; v[4] += fy * dy; v[5] += fz * dy;
+                            v[6] += fx * dz; v[7] += fy * dz; v[8] += fz * dz;
+                        }
+                    }
+                }
+                if constexpr (EWALD) {
+                    if (r2 < (R)a.realcut2) {
+                        const R qij = (R)(qi * a.q[j]);
+                        const R r = sqrt(r2);
+                        const R e = erfc_<R>((R)a.alpha * r) / r;
+                        const R ex = exp_<R>(-(R)a.alpha2 * r2);
+                        const R f = (e + (R)a.alpha2PI * ex) / r2;
+                        acc[A_EE] += (double)(qij * e);
+                        if constexpr (LUSTIG) {
+                            const R rr = a.use_constraint ? (rx * mx + ry * my + rz * mz) : r2;
+                            acc[A_ED1] += (double)(-qij * f * rr);
+                            acc[A_ED2] += (double)(qij * (R(2) * f + R(2) * (R)a.alpha2PI * (R)a.alpha2 * ex) * rr * rr / r2 +
+                                                   R(2) * qij * f * rr);
+                        }
+                        const R s = -qij * f;
+                        const double fx = (double)(s * rx), fy = (double)(s * ry), fz = (double)(s * rz);
+                        gfx += fx; gfy += fy; gfz += fz;
+                        double* v = acc + A_EVIR;
+                        v[0] += fx * dx; v[1] += fy * dx; v[2] += fz * dx;
+                        v[3] += fx * dy; v[4] += fy * dy; v[5] += fz * dy;
+                        v[6] += fx * dz; v[7] += fy * dz; v[8] += fz * dz;
+                    }
+                }
+                if (gfx != 0.0 || gfy != 0.0 || gfz != 0.0) {
+                    fix += gfx; fiy += gfy; fiz += gfz;
+                    atomic_add3(a.force + 3 * j, -gfx, -gfy, -gfz);
+                }
+            } else if constexpr (VDW == V_BUCK || VDW == V_BUFF) {
+                // rij = ri - rj on the virtual sites (BuckinghamForce.cpp:156,
+                // BuffvdWForce.cpp:172)
+                double dx = eix - a.epos[3 * j], dy = eiy - a.epos[3 * j + 1], dz = eiz - a.epos[3 * j + 2];
+                pbc(dx, dy, dz, a.box);
+                const R rx = (R)dx, ry = (R)dy, rz = (R)dz;
+                const R r = sqrt(rx * rx + ry * ry + rz * rz);
+                const R rij2 = r * r;
+                if (rij2 <= (R)a.cut2) {
+                    const int t = ti * a.ntypes + a.type[j];
+                    R tmpE = 0, tmpf = 0, dulj = 0, d2ulj = 0;
+                    bool live = true;
+                    if constexpr (VDW == V_BUCK) {
+                        const R Aa = (R)a.ta[t], Bb = (R)a.tb[t], Cc = (R)a.tc[t];
+                        const R r_1 = R(1) / r;
+                        const R rij_2 = r_1 * r_1;
+                        const R rij6 = rij_2 * rij_2 * rij_2;
+                        const R ex = exp_<R>(-Cc * r);
+                        tmpE = Bb * ex - Aa * rij6;
+                        tmpf = -Bb * Cc * r_1 * ex + R(6) * Aa * rij6 * rij_2;
+                        if constexpr (LUSTIG) {
+                            dulj = -Bb * Cc * r * ex + R(6) * Aa * rij6;
+                            d2ulj = (R(2) * r + Cc * rij2) * Bb * Cc * ex - R(54) * Aa * rij6;
+                        }
+                        if (a.switch_on && rij2 > (R)a.sw_dist2) {
+                            const R cc = (R)a.cut2 - rij2;
+                            const R c4 = cc * (R(2) * rij2 + (R)a.c2);
+                            const R sv = cc * c4 * (R)a.c1;
+                            const R dsv = (R)a.c3 * (cc * cc - c4);
+                            tmpf = tmpf * sv - tmpE * dsv;
+                            tmpE *= sv;
+                        }
+                        if (a.coulomb) {
+                            const R qij = (R)(qi * a.q[j]);
+                            const R e = qij * r_1;
+                            const R f = e * rij_2;
+                            const R r_rc = r / (R)a.cutoff;
+                            const R r4 = rij2 * rij2 * (R)a.rcut_2 * (R)a.rcut_2;
+                            acc[A_VELEC] += (double)(e * (R(1) - R(1.666667) * r_rc + R(1.666667) * r4 - r_rc * r4));
+                            tmpf += f * (R(1) - R(5) * r4 + R(4) * r_rc * r4);
+                        }
+                    } else {
+                        const R sigma = (R)a.ta[t], eps = (R)a.tb[t];
+                        if (eps == R(0)) {
+                            live = false;   // tmpE = tmpf = 0 (BuffvdWForce.cpp:190-192)
+                        } else {
+                            const R invs = R(1) / sigma;
+                            const R m = (R)a.m_buff, nn = (R)a.n_buff, del = (R)a.delta, gam = (R)a.gamma;
+                            const R rho = r * invs;
+                            const R rho_m = pow_<R>(rho, m);
+                            const R rho_m1 = pow_<R>(rho, m - R(1));
+                            const R expnm = nn - m;
+                            const R par1 = (R(1) + del) / (rho + del);
+                            const R par2 = (R(1) + gam) / (rho_m + gam);
+                            const R p1n = pow_<R>(par1, expnm);
+                            tmpE = eps * p1n * (par2 - R(2));
+                            tmpf = -eps * expnm * p1n * (par2 - R(2)) * invs / (rho + del) -
+                                   eps * p1n * par2 * m * rho_m1 * invs / (rho_m + gam);
+                            if (a.switch_on && rij2 > (R)a.sw_dist2) {
+                                const R r3 = rij2 * r, r4 = rij2 * rij2, r5 = rij2 * r3;
+                                const R sv = (R)a.s5 * r5 + (R)a.s4 * r4 + (R)a.s3 * r3 + (R)a.s2 * rij2 + (R)a.s1 * r + (R)a.s0;
+                                const R dsv = R(5) * (R)a.s5 * r4 + R(4) * (R)a.s4 * r3 + R(3) * (R)a.s3 * rij2 +
+                                              R(2) * (R)a.s2 * r + (R)a.s1;
+                                tmpf = tmpf * sv + tmpE * dsv;
+                                tmpE *= sv;
+                            }
+                            dulj = tmpf * r;   // always accumulated (BuffvdWForce.cpp:242-244)
+                        }
+                    }
+                    if (live) {
+                        acc[A_VE] += (double)tmpE;
+                        acc[A_VDUDV] += (double)dulj;
+                        if constexpr (LUSTIG) acc[A_VD2UDV] += (double)d2ulj;
+                        double fx, fy, fz, vx, vy, vz;
+                        if constexpr (VDW == V_BUCK) {
+                            fx = (double)(-tmpf * rx); fy = (double)(-tmpf * ry); fz = (double)(-tmpf * rz);
+                            vx = fx; vy = fy; vz = fz;
+                        } else {
+                            const R r_1 = R(1) / r;
+                            fx = (double)(-tmpf * rx * r_1); fy = (double)(-tmpf * ry * r_1); fz = (double)(-tmpf * rz * r_1);
+                            vx = -fx; vy = -fy; vz = -fz;   // virial += -fij * rij (BuffvdWForce.cpp:276-284)
+                        }
+                        fix += fx; fiy += fy; fiz += fz;
+                        const double faj = a.fa[j], fbj = a.fb[j];
+                        atomic_add3(a.force + 3 * j, -faj * fx, -faj * fy, -faj * fz);
+                        if (fbj != 0.0) atomic_add3(a.force + 3 * a.host[j], -fbj * fx, -fbj * fy, -fbj * fz);
+                        double* v = acc + A_VVIR;
+                        v[0] += vx * dx; v[1] += vy * dx; v[2] += vz * dx;
+                        v[3] += vx * dy; v[4] += vy * dy; v[5] += vz * dy;
+                        v[6] += vx * dz; v[7] += vy * dz; v[8] += vz * dz;
+                    }
+                }
+            }
+        }
+    }
+
+    // owner force: combine the G lanes, then one atomic per component
+#pragma unroll
+    for (int o = G / 2; o > 0; o >>= 1) {
+        fix += __shfl_xor(fix, o, G);
+        fiy += __shfl_xor(fiy, o, G);
+        fiz += __shfl_xor(fiz, o, G);
+    }
+    if (i < a.n && sub == 0 && (fix != 0.0 || fiy != 0.0 || fiz != 0.0)) {
+        if constexpr (VDW == V_BUCK || VDW == V_BUFF) {
+            const double fai = a.fa[i], fbi = a.fb[i];
+            atomic_add3(a.force + 3 * i, fai * fix, fai * fiy, fai * fiz);
+            if (fbi != 0.0) atomic_add3(a.force + 3 * a.host[i], fbi * fix, fbi * fiy, fbi * fiz);
+        } else {
+            atomic_add3(a.force + 3 * i, fix, fiy, fiz);
+        }
+    }
+
+    // energies / derivatives / virials: block sums
+    if constexpr (VDW != V_NONE) {
+        double v[4 + 9];
+#pragma unroll
+        for (int k = 0; k < 13; ++k) v[k] = acc[A_VE + k];
+        block_add<13>(v, a.terms + T_LJ_E);
+    }
+    if constexpr (EWALD) {
+        __syncthreads();
+        double v[3 + 9];
+#pragma unroll
+        for (int k = 0; k < 12; ++k) v[k] = acc[A_EE + k];
+        block_add<12>(v, a.terms + T_EW_E);
+    }
+}
+
+// ---------------------------------------------------------------------------
+
+constexpr int GLANES = 8;
+
+template <int VDW, bool EWALD>
+static hipError_t launch_pairs(Ctx* c, const PairArgs& a, int prec, int lustig) {
+    const int threads = 256;
+    const int64_t total = (int64_t)c->n * GLANES;
+    const dim3 grid((unsigned)((total + threads - 1) / threads));
+    ProfScope ps(c, "pairs");
+    if (prec == SMD_PREC_MIXED) {
+        if (lustig) hipLaunchKernelGGL((k_pairs<VDW, EWALD, true, float, GLANES>), grid, dim3(threads), 0, c->stream, a);
+        else hipLaunchKernelGGL((k_pairs<VDW, EWALD, false, float, GLANES>), grid, dim3(threads), 0, c->stream, a);
+    } else {
+        if (lustig) hipLaunchKernelGGL((k_pairs<VDW, EWALD, true, double, GLANES>), grid, dim3(threads), 0, c->stream, a);
+        else hipLaunchKernelGGL((k_pairs<VDW, EWALD, false, double, GLANES>), grid, dim3(threads), 0, c->stream, a);
+    }
+    return hipGetLastError();
+}
+
+static void base_args(Ctx* c, PairArgs& a) {
+    a = PairArgs{};
+    a.n = c->n;
+    a.box = c->box;
+    a.pos = c->d_pos;
+    a.epos = c->d_epos;
+    a.q = c->d_q;
+    a.type = c->d_type;
+    a.mol = c->d_mol;
+    a.com = c->d_com;
+    a.nb = c->d_nb;
+    a.nb_cnt = c->d_nb_cnt;
+    a.stride = c->stride;
+    a.force = c->d_force;
+    a.terms = c->d_terms;
+    a.ntypes = c->ntypes;
+    a.ta = c->d_vdw_a;
+    a.tb = c->d_vdw_b;
+    a.tc = c->d_vdw_c;
+    a.fa = c->d_fa;
+    a.fb = c->d_fb;
+    a.host = c->d_host;
+}
+
+static void vdw_constants(PairArgs& a, const smd_vdw_params* p) {
+    a.cutoff = p->cutoff;
+    a.cut2 = p->cutoff * p->cutoff;
+    a.rcut_2 = 1.0 / a.cut2;
+    a.switch_on = p->switch_on;
+    a.use_constraint = p->use_constraint;
+    a.coulomb = p->coulomb;
+    double sd2 = a.cut2;
+    if (p->switch_on) sd2 = p->switch_dist * p->switch_dist;
+    a.sw_dist2 = sd2;
+    if (p->switch_on && a.cut2 > sd2) {   // CHARMM switch (MieForce ctor)
+        double c1 = 1.0 / (a.cut2 - sd2);
+        a.c1 = c1 * c1 * c1;
+        a.c2 = a.cut2 - 3.0 * sd2;
+        a.c3 = 4 * a.c1;
+        // 5th-order taper (BuffvdWForce ctor :65-72)
+        const double rc = p->cutoff, rc2 = a.cut2, sw = p->switch_dist;
+        const double off = rc - sw;
+        const double den = off * off * off * off * off;
+        a.s0 = rc * rc2 * (rc2 - 5.0 * rc * sw + 10.0 * sd2) / den;
+        a.s1 = -30.0 * rc2 * sd2 / den;
+        a.s2 = 30.0 * (rc2 * sw + rc * sd2) / den;
+        a.s3 = -10.0 * (rc2 + 4.0 * rc * sw + sd2) / den;
+        a.s4 = 15.0 * (rc + sw) / den;
+        a.s5 = -6.0 / den;
+    }
+    a.n_mie = p->n_mie;
+    a.m_mie = p->m_mie;
+    if (p->kind == SMD_VDW_MIE) {   // MieForce ctor (src/forces/MieForce.cpp:32-35)
+        const double Cbase = p->n_mie / p->m_mie;
+        const double Cexp = p->m_mie / (p->n_mie - p->m_mie);
+        a.cn_mie = (p->n_mie / (p->n_mie - p->m_mie)) * pow(Cbase, Cexp);
+    }
+    a.n_buff = p->n_buff;
+    a.m_buff = p->m_buff;
+    a.delta = p->delta;
+    a.gamma = p->gamma;
+}
+
+// Uploads the per-kind tables and per-atom virtual-site data.
+int upload_vdw_tables(Ctx* c, const smd_vdw_params* p) {
+    const int nt = c->ntypes, nt2 = nt * nt;
+    std::vector<double> A(nt2, 0.0), B(nt2, 0.0), Cc(nt2, 0.0);
+    std::vector<double> fa(c->n, 1.0), fb(c->n, 0.0);
+    std::vector<int32_t> host(c->n, 0);
+    for (int i = 0; i < c->n; ++i) host[i] = i;
+    switch (p->kind) {
+    case SMD_VDW_LJ:
+    case SMD_VDW_MIE:
+        if (!p->sigma2 || !p->eps4) return fail(c, SMD_EINVAL, "LJ/Mie need sigma2 and eps4 tables");
+        for (int k = 0; k < nt2; ++k) { A[k] = p->sigma2[k]; B[k] = p->eps4[k]; }
+        break;
+    case SMD_VDW_BUCK: {
+        if (!p->buck_a || !p->buck_b || !p->buck_c) return fail(c, SMD_EINVAL, "Buckingham needs A, B, C");
+        for (int k = 0; k < nt2; ++k) { A[k] = p->buck_a[k]; B[k] = p->buck_b[k]; Cc[k] = p->buck_c[k]; }
+        // O0, O1 -> 0.91 on the site and 0.09 on the carbon (myAtoms[2]);
+        // carbon keeps its force (BuckinghamForce.cpp:217-238).
+        for (int m = 0; m < c->nmol; ++m) {
+            const int a0 = c->h_mol_start[m];
+            if (c->h_mol_start[m + 1] - a0 != 3) return fail(c, SMD_ENOTSUP, "Buckingham expects 3-site CO2");
+            for (int s = 0; s < 3; ++s) {
+                const int id = a0 + s;
+                const int t = c->h_type[id];
+                if (t == 0 || t == 1) { fa[id] = 0.91; fb[id] = 0.09; host[id] = a0 + 2; }
+                else if (t == 2) { fa[id] = 1.0; fb[id] = 0.0; }
+                else { fa[id] = 0.0; fb[id] = 0.0; }
+            }
+        }
+        break;
+    }
+    case SMD_VDW_BUFF147: {
+        if (!p->type_sigma || !p->type_eps) return fail(c, SMD_EINVAL, "buffered 14-7 needs per-type R and eps");
+        // BuffvdWForce::set_buff_param (src/forces/BuffvdWForce.cpp:392-433)
+        for (int i = 0; i < nt; ++i)
+            for (int j = 0; j < nt; ++j) {
+                const double ei = p->type_eps[i], si = p->type_sigma[i];
+                const double ej = p->type_eps[j], sj = p->type_sigma[j];
+                const double si2 = si * si, sj2 = sj * sj;
+                const double es = sqrt(ei) + sqrt(ej);
+                double R = 0.0000001, E = 0;
+                if (!(ei == 0 || ej == 0)) {
+                    R = (si2 * si + sj2 * sj) / (si2 + sj2);
+                    E = (4.0 * ei * ej) / (es * es);
+                }
+                A[i * nt + j] = R;
+                B[i * nt + j] = E;
+            }
+        for (int m = 0; m < c->nmol; ++m) {
+            const int a0 = c->h_mol_start[m];
+            const int o = a0 + 1;
+            if (c->h_mol_start[m + 1] - a0 < 2 || c->h_type[o] != 1)
+                return fail(c, SMD_EINVAL, "buffered 14-7 expects type 1 (oxygen) as each molecule's second site");
+            for (int id = a0; id < c->h_mol_start[m + 1]; ++id)
+                if (c->h_type[id] == 0) { fa[id] = p->hfactor; fb[id] = 1.0 - p->hfactor; host[id] = o; }
+        }
+        break;
+    }
+    default:
+        return fail(c, SMD_EINVAL, "unknown vdW kind %d", p->kind);
+    }
+    SMD_HIP(c, hipMemcpyAsync(c->d_vdw_a, A.data(), nt2 * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    SMD_HIP(c, hipMemcpyAsync(c->d_vdw_b, B.data(), nt2 * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    SMD_HIP(c, hipMemcpyAsync(c->d_vdw_c, Cc.data(), nt2 * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    SMD_HIP(c, hipMemcpyAsync(c->d_fa, fa.data(), c->n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    SMD_HIP(c, hipMemcpyAsync(c->d_fb, fb.data(), c->n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    SMD_HIP(c, hipMemcpyAsync(c->d_host, host.data(), c->n * sizeof(int32_t), hipMemcpyHostToDevice, c->stream));
+    SMD_HIP(c, hipStreamSynchronize(c->stream));
+    return SMD_OK;
+}
+
+__global__ void k_virtual_sites(int nmol, const int32_t* __restrict__ mol_start, const int32_t* __restrict__ type,
+                                const double* __restrict__ pos, double* __restrict__ epos, int kind, double hfactor,
+                                double co2_bond) {
+    const int m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= nmol) return;
+    const int a0 = mol_start[m], a1 = mol_start[m + 1];
+    for (int id = a0; id < a1; ++id)
+        for (int d = 0; d < 3; ++d) epos[3 * id + d] = pos[3 * id + d];
+    if (kind == SMD_VDW_BUFF147) {
+        // BuffvdWForce::VirtualPositions (src/forces/BuffvdWForce.cpp:358-385)
+        const int o = a0 + 1;
+        for (int id = a0; id < a1; ++id)
+            if (type[id] == 0)
+                for (int d = 0; d < 3; ++d) epos[3 * id + d] = hfactor * pos[3 * id + d] + (1.0 - hfactor) * pos[3 * o + d];
+    } else if (kind == SMD_VDW_BUCK) {
+        // BuckinghamForce::VirtualPositions (src/forces/BuckinghamForce.cpp:372-398)
+        const double ll = co2_bond * 0.5;
+        const double cc = 0.91 * ll;
+        const double bb = ll - cc;
+        const int cid = a0 + 2;
+        for (int s = 0; s < 2; ++s) {
+            const int id = a0 + s;
+            for (int d = 0; d < 3; ++d) epos[3 * id + d] = (bb / ll) * pos[3 * cid + d] + (cc / ll) * pos[3 * id + d];
+        }
+    }
+}
+
+int vdw_run(Ctx* c, const smd_vdw_params* p, int prec, smd_vdw_out* out, bool read_back) {
+    if (!c->list_valid) return fail(c, SMD_ESTATE, "no pair list");
+    int rc = upload_vdw_tables(c, p);
+    if (rc) return rc;
+    PairArgs a;
+    base_args(c, a);
+    vdw_constants(a, p);
+    const int lustig = p->lustig ? 1 : 0;
+    SMD_HIP(c, hipMemsetAsync(c->d_terms, 0, sizeof(double) * T_COUNT, c->stream));
+    hipError_t e = hipSuccess;
+    switch (p->kind) {
+    case SMD_VDW_LJ: e = launch_pairs<V_LJ, false>(c, a, prec, lustig); break;
+    case SMD_VDW_MIE: e = launch_pairs<V_MIE, false>(c, a, prec, lustig); break;
+    case SMD_VDW_BUCK:
+    case SMD_VDW_BUFF147:
+        hipLaunchKernelGGL(k_virtual_sites, dim3((c->nmol + 255) / 256), dim3(256), 0, c->stream, c->nmol,
+                           c->d_mol_start, c->d_type, c->d_pos, c->d_epos, p->kind, p->hfactor, p->co2_bond);
+        e = (p->kind == SMD_VDW_BUCK) ? launch_pairs<V_BUCK, false>(c, a, prec, lustig)
+                                      : launch_pairs<V_BUFF, false>(c, a, prec, lustig);
+        break;
+    default:
+        return fail(c, SMD_EINVAL, "unknown vdW kind");
+    }
+    if (e != hipSuccess) return hip_fail(c, e, "pair kernel launch");
+    if (!read_back) return SMD_OK;
+    double t[T_COUNT];
+    rc = read_terms(c, t);
+    if (rc) return rc;
+    const double vol = c->box.L[0] * c->box.L[1] * c->box.L[2];
+    const double vol2 = vol * vol;
+    double energy = t[T_LJ_E];
+    double vir[9];
+    for (int k = 0; k < 9; ++k) vir[k] = t[T_LJ_VIR + k];
+    const double elec = t[T_LJ_ELEC];
+    double dudv = t[T_LJ_DUDV], d2udv = t[T_LJ_D2UDV];
+    *out = smd_vdw_out{};
+    if (p->kind == SMD_VDW_BUFF147) {
+        // BuffvdWForce::compute tail (:320-336)
+        vir[0] += p->v_lrc; vir[4] += p->v_lrc; vir[8] += p->v_lrc;
+        dudv = dudv / (3.0 * vol);
+        out->energy = energy;
+        out->lj_energy = energy + p->e_lrc;
+        out->ulus = energy + p->e_lrc;
+        out->dudv = dudv + p->du_lrc;
+        out->d2udv = p->lustig ? d2udv / (9.0 * vol2) + p->d2u_lrc : 0.0;
+    } else {
+        // LJForce::compute tail (:232-254); Buckingham has +dudv (:262)
+        const double ulus = energy;
+        energy += p->e_lrc;
+        vir[0] += p->v_lrc; vir[4] += p->v_lrc; vir[8] += p->v_lrc;
+        out->energy = energy;
+        out->lj_energy = energy + elec;
+        if (p->lustig) {
+            dudv = (p->kind == SMD_VDW_BUCK) ? dudv / (3 * vol) : -dudv / (3.0 * vol);
+            d2udv = d2udv / (9.0 * vol2);
+            out->ulus = ulus + p->e_lrc;
+            out->dudv = dudv + p->du_lrc;
+            out->d2udv = d2udv + p->d2u_lrc;
+        }
+    }
+    for (int k = 0; k < 9; ++k) out->virial[k] = vir[k];
+    return SMD_OK;
+}
+
+int ewald_real(Ctx* c, int prec, int lustig) {
+    PairArgs a;
+    base_args(c, a);
+    const double al = c->ew.alpha;
+    a.alpha = al;
+    a.alpha2 = al * al;
+    a.alpha2PI = 2.0 * al / SQRT_PI;
+    a.realcut2 = c->ew.real_cut * c->ew.real_cut;
+    a.use_constraint = c->ew.use_constraint;
+    hipError_t e = launch_pairs<V_NONE, true>(c, a, prec, lustig);
+    if (e != hipSuccess) return hip_fail(c, e, "Ewald real kernel launch");
+    return SMD_OK;
+}
+
+// Fused LJ + Ewald real space for the MD step (one pass over the list).
+int pairs_lj_ewald(Ctx* c, const double* sigma2, const double* eps4, double cutoff, double alpha, double real_cut,
+                   int use_constraint, int lustig, int prec) {
+    (void)sigma2; (void)eps4;   // tables uploaded once by the MD setup
+    PairArgs a;
+    base_args(c, a);
+    a.cutoff = cutoff;
+    a.cut2 = cutoff * cutoff;
+    a.rcut_2 = 1.0 / a.cut2;
+    a.use_constraint = use_constraint;
+    a.alpha = alpha;
+    a.alpha2 = alpha * alpha;
+    a.alpha2PI = 2.0 * alpha / SQRT_PI;
+    a.realcut2 = real_cut * real_cut;
+    hipError_t e = launch_pairs<V_LJ, true>(c, a, prec, lustig);
+    if (e != hipSuccess) return hip_fail(c, e, "LJ+Ewald kernel launch");
+    return SMD_OK;
+}
+
+}  // namespace smd
+
+using namespace smd;
+
+extern "C" int smd_vdw_compute(smd_ctx* c, const smd_vdw_params* p, int prec, double* f_accum, smd_vdw_out* out) {
+    if (!c || !p || !out) return SMD_EINVAL;
+    if (prec != SMD_PREC_FP64 && prec != SMD_PREC_MIXED) return fail(c, SMD_EINVAL, "bad precision %d", prec);
+    if (f_accum) SMD_HIP(c, hipMemsetAsync(c->d_force, 0, sizeof(double) * 3 * c->n, c->stream));
+    int rc = vdw_run(c, p, prec, out, true);
+    if (rc) return rc;
+    if (f_accum) {
+        std::vector<double> f(3 * (size_t)c->n);
+        SMD_HIP(c, hipMemcpyAsync(f.data(), c->d_force, f.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        SMD_HIP(c, hipStreamSynchronize(c->stream));
+        for (size_t k = 0; k < f.size(); ++k) f_accum[k] += f[k];
+    }
+    return SMD_OK;
+}
+
+extern "C" int smd_lj_long_range_correct(smd_ctx* c, const double* sigma2, const double* eps4, double cutoff,
+                                         double out4[4]) {
+    // LJForce::long_range_correct (src/forces/LJForce.cpp:260-370), csix
+    // started at 0 (the reference leaves it uninitialised, :268).
+    if (!c || !sigma2 || !eps4 || !out4) return SMD_EINVAL;
+    if (!c->box_set) return fail(c, SMD_ESTATE, "box not set");
+    const int nt = c->ntypes, n = c->n;
+    std::vector<int> num(nt, 0);
+    for (int i = 0; i < n; ++i) num[c->h_type[i]] += 1;
+    const double cutoff3 = cutoff * cutoff * cutoff;
+    const double volrc = c->box.L[0] * c->box.L[1] * c->box.L[2];
+    const double invvol = 1.0 / volrc;
+    double csix = 0.0;
+    int npair = 0, nexcl = 0;
+    for (int i = 0; i < nt; ++i)
+        for (int j = i; j < nt; ++j) {
+            const double s2 = sigma2[i * nt + j], e4 = eps4[i * nt + j];
+            const double s6 = s2 * s2 * s2;
+            const int np = (i != j) ? num[i] * num[j] : (int)(num[i] * (num[i] - 1) * 0.5);
+            csix += np * e4 * s6;
+            npair += np;
+        }
+    for (int i = 0; i < n; ++i)
+        for (int k = c->h_excl_off[i]; k < c->h_excl_off[i + 1]; ++k) {
+            const int j = c->h_excl[k];
+            if (j < i) continue;
+            const int t = c->h_type[i] * nt + c->h_type[j];
+            const double s6 = sigma2[t] * sigma2[t] * sigma2[t];
+            csix -= eps4[t] * s6;
+            nexcl += 1;
+        }
+    if (npair - nexcl <= 0) csix = 0;
+    else csix /= npair - nexcl;
+    out4[0] = -2.0 * PI * n * n * invvol * csix / (3.0 * cutoff3);
+    out4[2] = 4.0 * PI * n * n * invvol * invvol * csix / (3.0 * cutoff3);
+    out4[3] = -4.0 * PI * n * n * invvol * invvol * invvol * csix / cutoff3;
+    out4[1] = -2.0 * out4[0];
+    return SMD_OK;
+}

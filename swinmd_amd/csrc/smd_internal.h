@@ -1,0 +1,253 @@
+// smd_internal.h -- context layout and device helpers shared by the gfx950
+// kernels of libswinmd_hip.so.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "../../include/swinmd_hip.h"
+
+namespace smd {
+
+constexpr int WAVE = 64;
+constexpr double PI = 3.14159265358979323846;       // src/NEMD_defs.h:77
+constexpr double SQRT_PI = 1.7724538509055160273;   // src/NEMD_defs.h:78
+constexpr double BOLTZ = 0.0083145112;              // src/NEMD_defs.h:108
+
+// Device status bits (set by kernels, read by the host after a sync).
+enum : unsigned {
+    ST_LIST_OVERFLOW = 1u,
+    ST_SHAKE_FAIL = 2u,
+    ST_SHAKE_NOCONV = 4u,
+    ST_NH_ITER = 8u,
+};
+
+// Slots of the device accumulator for scalar results (FP64, zeroed per use).
+// The pair kernel writes its 13 vdW values (E, ELEC, DUDV, D2UDV, VIR[9]) and
+// its 12 Ewald values (E, D1, D2, VIR[9]) contiguously from T_LJ_E / T_EW_E.
+enum TermSlot {
+    T_LJ_E = 0, T_LJ_ELEC, T_LJ_DUDV, T_LJ_D2UDV, T_LJ_VIR,                     // VIR: 9
+    T_EW_E = T_LJ_VIR + 9, T_EW_D1, T_EW_D2, T_EW_VIR,                           // VIR: 9
+    T_IN_E = T_EW_VIR + 9, T_IN_U, T_IN_D1, T_IN_D2,
+    T_SU_QR, T_SU_QRR = T_SU_QR + 3, T_SU_VIR = T_SU_QRR + 3,                     // 3, 3, 9
+    T_RC_E = T_SU_VIR + 9, T_RC_D1, T_RC_D2, T_RC_VIR,                            // VIR: 9
+    T_RC_FF = T_RC_VIR + 9, T_RC_FFK2, T_RC_D2UV1, T_RC_D2URR,
+    T_SH_VIR, T_KE_ATOM = T_SH_VIR + 9, T_KE_MOL, T_VOM, T_MTOT = T_VOM + 3,
+    T_COUNT
+};
+
+// Stream-ordered event pairs for the profiling families.
+struct Prof {
+    bool on = false;
+    struct Rec { std::string fam; hipEvent_t a, b; };
+    std::vector<Rec> pending;
+    std::map<std::string, std::pair<double, int64_t>> acc;
+};
+
+struct Box {
+    double L[3];
+    double h[3];   // L/2, as Ensemble::compute_bound_box stores halfL
+};
+
+struct Ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+    int n = 0, nmol = 0, ntypes = 0;
+    Box box{};
+    bool box_set = false;
+
+    // topology
+    int32_t* d_type = nullptr;
+    int32_t* d_mol = nullptr;
+    int32_t* d_mol_start = nullptr;   // [nmol+1]
+    double* d_q = nullptr;
+    double* d_mass = nullptr;
+    int32_t* d_excl_off = nullptr;
+    int32_t* d_excl = nullptr;
+    std::vector<int32_t> h_excl_off, h_excl, h_type, h_mol_start;
+    std::vector<double> h_mass, h_q;
+
+    // state
+    double* d_pos = nullptr;      // [3n]
+    double* d_vel = nullptr;      // [3n] Atom::velocity
+    double* d_vtm = nullptr;      // [3n] vtMinus
+    double* d_disp = nullptr;     // [3n] Atom::displacement
+    double* d_force = nullptr;    // [3n] device force accumulator
+    double* d_com = nullptr;      // [3nmol]
+    double* d_mom = nullptr;      // [3nmol] Molecule::momenta
+    double* d_bond2 = nullptr;    // [3nmol]
+    double* d_terms = nullptr;    // [T_COUNT]
+    unsigned* d_status = nullptr;
+    int* d_flag = nullptr;        // [4]: 0 rebuild-needed, 1 drift-apply
+    double* d_scalars = nullptr;  // [8]: fnh, fnh_minus, ...
+    double* h_pinned = nullptr;   // pinned staging for small results
+
+    // pair list (ELL rows, reference order within a row)
+    int32_t* d_cell_of = nullptr;
+    int32_t* d_cell_cnt = nullptr;    // [ncell+1]
+    int32_t* d_cell_start = nullptr;  // [ncell+1]
+    int32_t* d_cell_atoms = nullptr;  // [n]
+    int32_t* d_nb = nullptr;          // [n * stride]
+    int32_t* d_nb_cnt = nullptr;      // [n]
+    int32_t stride = 0;               // allocated row capacity
+    int32_t want_stride = 0;          // user-requested capacity (0 = auto)
+    bool list_valid = false;
+    bool list_cells = false;
+    int ncell[3] = {0, 0, 0};
+    double cell_rlen[3] = {0, 0, 0};
+    double list_cutoff = 0, list_buff = 0;
+    int cell_alloc = 0;
+
+    // Ewald setup
+    bool ewald_ready = false;
+    smd_ewald_params ew{};
+    int K = 0;
+    int nmax[3] = {0, 0, 0};
+    double self_energy = 0, charged_energy = 0;
+    double* d_kvec = nullptr;     // [3K]
+    double* d_kmod2 = nullptr;    // [K]
+    double* d_expk = nullptr;     // [K]
+    int32_t* d_kn = nullptr;      // [3K] integer (nx, ny, nz)
+    double* d_tab = nullptr;      // per-atom phase tables
+    double* d_sfac_part = nullptr;  // [nchunk][K][8]
+    double* d_sfac = nullptr;       // [K][8]
+    int sfac_chunks = 0;
+    int tab_stride = 0;             // complex entries per atom
+
+    // vdW tables (device)
+    double* d_vdw_a = nullptr;    // [ntypes^2] per kind meaning
+    double* d_vdw_b = nullptr;
+    double* d_vdw_c = nullptr;
+    double* d_epos = nullptr;     // [3n] effective (virtual-site) positions
+    double* d_fa = nullptr;       // [n] force fraction kept on the atom
+    double* d_fb = nullptr;       // [n] fraction moved to the host atom
+    int32_t* d_host = nullptr;    // [n]
+
+    // MD driver
+    bool md_ready = false;
+    smd_md_params md{};
+    std::vector<double> md_sigma2, md_eps4;
+    int64_t md_step = 0, md_list_updates = 0;
+    hipGraphExec_t graph_exec = nullptr;
+    hipGraph_t graph = nullptr;
+
+    Prof prof;
+};
+
+}  // namespace smd
+
+struct smd_ctx : public smd::Ctx {};
+
+namespace smd {
+
+int fail(Ctx* c, int code, const char* fmt, ...);
+int hip_fail(Ctx* c, hipError_t e, const char* what);
+
+#define SMD_HIP(c, expr)                                        \
+    do {                                                        \
+        hipError_t _e = (expr);                                 \
+        if (_e != hipSuccess) return smd::hip_fail((c), _e, #expr); \
+    } while (0)
+
+// Profiling bracket around a launch.
+struct ProfScope {
+    Ctx* c;
+    const char* fam;
+    hipEvent_t a = nullptr, b = nullptr;
+    ProfScope(Ctx* c_, const char* f) : c(c_), fam(f) {
+        if (c->prof.on) {
+            (void)hipEventCreate(&a);
+            (void)hipEventCreate(&b);
+            (void)hipEventRecord(a, c->stream);
+        }
+    }
+    ~ProfScope() {
+        if (c->prof.on) {
+            (void)hipEventRecord(b, c->stream);
+            c->prof.pending.push_back({fam, a, b});
+        }
+    }
+};
+
+// ---- device helpers -------------------------------------------------------
+
+// Ensemble::apply_pbc (src/Ensemble.h:319-334), half-open [-L/2, L/2).  The
+// loops are bounded so a corrupt coordinate cannot hang a wave; any |x| < 8L
+// gives the reference's result.
+__device__ __forceinline__ void pbc(double& x, double& y, double& z, const Box& b) {
+    for (int it = 0; it < 8 && x >= b.h[0]; ++it) x -= b.L[0];
+    for (int it = 0; it < 8 && x < -b.h[0]; ++it) x += b.L[0];
+    for (int it = 0; it < 8 && y >= b.h[1]; ++it) y -= b.L[1];
+    for (int it = 0; it < 8 && y < -b.h[1]; ++it) y += b.L[1];
+    for (int it = 0; it < 8 && z >= b.h[2]; ++it) z -= b.L[2];
+    for (int it = 0; it < 8 && z < -b.h[2]; ++it) z += b.L[2];
+}
+
+// Vector3::length2 (src/utils/Vector3.h:190-192) with each product and sum
+// rounded separately, as the reference's non-FMA x86 build evaluates it.
+__device__ __forceinline__ double len2_exact(double x, double y, double z) {
+#pragma clang fp contract(off)
+    double xx = x * x;
+    double yy = y * y;
+    double zz = z * z;
+    return (xx + yy) + zz;
+}
+
+template <typename T>
+__device__ __forceinline__ T wave_sum(T v) {
+    for (int o = WAVE / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, WAVE);
+    return v;
+}
+
+// Sum NV values over the block and add them atomically to dst[0..NV).
+template <int NV>
+__device__ __forceinline__ void block_add(double (&v)[NV], double* dst) {
+    __shared__ double red[NV][16];
+    const int lane = threadIdx.x % WAVE, wid = threadIdx.x / WAVE;
+    const int nw = (blockDim.x + WAVE - 1) / WAVE;
+#pragma unroll
+    for (int k = 0; k < NV; ++k) v[k] = wave_sum(v[k]);
+    if (lane == 0)
+#pragma unroll
+        for (int k = 0; k < NV; ++k) red[k][wid] = v[k];
+    __syncthreads();
+    if (threadIdx.x < NV) {
+        double s = 0.0;
+        for (int w = 0; w < nw; ++w) s += red[threadIdx.x][w];
+        if (s != 0.0) unsafeAtomicAdd(dst + threadIdx.x, s);
+    }
+}
+
+__device__ __forceinline__ void atomic_add3(double* f, double x, double y, double z) {
+    unsafeAtomicAdd(f + 0, x);
+    unsafeAtomicAdd(f + 1, y);
+    unsafeAtomicAdd(f + 2, z);
+}
+
+// ---- host-side module entry points (implemented per .hip file) ----------
+int pairlist_build(Ctx* c, double cutoff, double cut_buff, int use_cells, bool on_device_flag);
+int pairlist_alloc(Ctx* c, int32_t stride);
+int vdw_run(Ctx* c, const smd_vdw_params* p, int prec, smd_vdw_out* out, bool read_back);
+int pairs_lj_ewald(Ctx* c, const double* sigma2, const double* eps4, double cutoff, double alpha,
+                   double real_cut, int use_constraint, int lustig, int prec);
+int ewald_setup(Ctx* c, const smd_ewald_params* p);
+int ewald_real(Ctx* c, int prec, int lustig);
+int ewald_intra(Ctx* c, int lustig);
+int ewald_surface(Ctx* c);
+int ewald_recip(Ctx* c, int prec, int lustig);
+int shake_run(Ctx* c, const smd_integrator_params* p, bool fnh_on_device, double fnh);
+int kinetics_run(Ctx* c, bool rewrap);
+int lf_atoms_run(Ctx* c, const smd_integrator_params* p, double fnh);
+int md_step_enqueue(Ctx* c, bool do_long);
+int zero_terms(Ctx* c);
+int read_terms(Ctx* c, double* dst);
+int check_status(Ctx* c);
+int upload_vdw_tables(Ctx* c, const smd_vdw_params* p);
+
+}  // namespace smd

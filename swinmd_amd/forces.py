@@ -1,0 +1,324 @@
+"""Host-side mirror of SwinMD's nonbonded plugin interface over the C ABI.
+
+Class names and method meanings follow the reference:
+
+* :class:`Context` holds what ``Ensemble`` holds for the path (topology, box,
+  positions, molecule centres, the pair list) on one GPU.
+* :class:`LJForce`, :class:`MieForce`, :class:`BuckinghamForce`,
+  :class:`BuffvdWForce` mirror ``Force::compute()`` (src/Force.h:59) of
+  src/forces/{LJ,Mie,Buckingham,BuffvdW}Force.cpp.
+* :class:`EwaldForce` mirrors ``EwaldForce::compute`` / ``compute_long``
+  (src/forces/EwaldForce.cpp:128-196).
+* :class:`LFIntegrator` mirrors the SHAKE + Nose-Hoover leap-frog step of
+  src/integrators/LFIntegrator.cpp.
+
+Errors follow the C ABI: a negative status becomes :class:`SmdError` (the
+reference would exit through ERRORMSG).
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _lib as L
+
+
+def _ptr(a):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+def _f64(a, shape=None):
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    return a if shape is None else a.reshape(shape)
+
+
+class Context:
+    """Device-resident ``Ensemble`` view for one :class:`swinmd_amd.systems.System`."""
+
+    def __init__(self, system, device: int = 0, pos=None, box=None):
+        self.lib = L.load()
+        s = system
+        self.system = s
+        self.n = s.n_atoms
+        self.nmol = s.n_mols
+        self._type = np.ascontiguousarray(s.type, dtype=np.int32)
+        self._mol = np.ascontiguousarray(s.mol, dtype=np.int32)
+        self._q = _f64(s.scaled_charge)
+        self._mass = _f64(s.mass)
+        self._eoff, excl = s.exclusions()
+        self._excl = excl if excl.size else np.zeros(1, dtype=np.int32)
+        t = L.Topology(self.n, self.nmol, len(s.types), _ptr(self._type), _ptr(self._mol), _ptr(self._q),
+                       _ptr(self._mass), _ptr(self._eoff), _ptr(self._excl))
+        h = C.c_void_p()
+        rc = self.lib.smd_ctx_create(C.byref(h), C.byref(t), device)
+        if rc != 0:
+            raise L.SmdError(rc, f"smd_ctx_create: {self.lib.smd_strerror(rc).decode()}")
+        self.h = h
+        self.box = _f64(s.box if box is None else box)
+        self.check(self.lib.smd_set_box(self.h, _ptr(self.box)))
+        self.set_positions(s.pos if pos is None else pos)
+
+    # ---- plumbing -------------------------------------------------------
+    def check(self, rc):
+        if rc != 0:
+            msg = self.lib.smd_last_error(self.h).decode() if self.h else ""
+            raise L.SmdError(rc, msg or self.lib.smd_strerror(rc).decode())
+        return rc
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.smd_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _vec_in(self, fn, a, n):
+        a = _f64(a, (n, 3))
+        self.check(fn(self.h, _ptr(a), n))
+
+    def _vec_out(self, fn, n):
+        out = np.zeros((n, 3))
+        self.check(fn(self.h, _ptr(out), n))
+        return out
+
+    # ---- state ----------------------------------------------------------
+    def set_positions(self, pos):
+        self._vec_in(self.lib.smd_set_positions, pos, self.n)
+
+    def positions(self):
+        return self._vec_out(self.lib.smd_get_positions, self.n)
+
+    def set_velocities(self, v):
+        self._vec_in(self.lib.smd_set_velocities, v, self.n)
+
+    def velocities(self):
+        return self._vec_out(self.lib.smd_get_velocities, self.n)
+
+    def set_mol_centers(self, com):
+        self._vec_in(self.lib.smd_set_mol_centers, com, self.nmol)
+
+    def mol_centers(self):
+        return self._vec_out(self.lib.smd_get_mol_centers, self.nmol)
+
+    def compute_mol_centers(self):
+        self.check(self.lib.smd_compute_mol_centers(self.h))
+
+    def zero_forces(self):
+        self.check(self.lib.smd_zero_forces(self.h))
+
+    def forces(self):
+        return self._vec_out(self.lib.smd_get_forces, self.n)
+
+    # ---- (a) pair list ----------------------------------------------------
+    def set_pairlist(self, cutoff, cut_buff, use_cells=True):
+        """Ensemble::set_pairlist (src/Ensemble.cpp:454-473)."""
+        self.check(self.lib.smd_build_pairlist(self.h, cutoff, cut_buff, int(use_cells)))
+
+    def pairlist_info(self):
+        n, mx, uc = C.c_int64(), C.c_int32(), C.c_int32()
+        self.check(self.lib.smd_pairlist_info(self.h, C.byref(n), C.byref(mx), C.byref(uc)))
+        return n.value, mx.value, bool(uc.value)
+
+    def pairlist(self):
+        """Per-atom rows (Atom::myPairList) as CSR: offsets [n+1], neighbours."""
+        total, _, _ = self.pairlist_info()
+        off = np.zeros(self.n + 1, dtype=np.int32)
+        nb = np.zeros(max(total, 1), dtype=np.int32)
+        self.check(self.lib.smd_get_pairlist(self.h, _ptr(off), _ptr(nb), nb.size))
+        return off, nb[:total]
+
+    def set_pairlist_capacity(self, cap):
+        self.check(self.lib.smd_set_pairlist_capacity(self.h, cap))
+
+    # ---- profiling --------------------------------------------------------
+    def profile(self, on=True):
+        self.check(self.lib.smd_profile_enable(self.h, int(on)))
+
+    def profile_get(self, family):
+        ms, n = C.c_double(), C.c_int64()
+        self.check(self.lib.smd_profile_get(self.h, family.encode(), C.byref(ms), C.byref(n)))
+        return ms.value, n.value
+
+    def sync(self):
+        self.check(self.lib.smd_sync(self.h))
+
+
+class _VdwForce:
+    KIND = L.VDW_LJ
+
+    def __init__(self, ctx: Context, *, cutoff, switch_on=False, switch_dist=0.0, use_constraint=True,
+                 lustig=False, coulomb=False, lrc=(0.0, 0.0, 0.0, 0.0), prec=L.PREC_FP64, **kind_params):
+        self.ctx = ctx
+        self.prec = prec
+        s = ctx.system
+        p = L.VdwParams()
+        p.kind = self.KIND
+        p.cutoff, p.switch_dist = cutoff, switch_dist
+        p.switch_on, p.use_constraint, p.lustig, p.coulomb = int(switch_on), int(use_constraint), int(lustig), int(coulomb)
+        self._keep = []
+        sig2, eps4 = s.lj_table()
+        self._keep += [sig2, eps4]
+        p.sigma2, p.eps4 = _ptr(sig2), _ptr(eps4)
+        p.n_mie, p.m_mie = kind_params.get("n_mie", 12.0), kind_params.get("m_mie", 6.0)
+        if "buck" in kind_params:
+            a, b, c = [_f64(x) for x in kind_params["buck"]]
+            self._keep += [a, b, c]
+            p.buck_a, p.buck_b, p.buck_c = _ptr(a), _ptr(b), _ptr(c)
+        p.co2_bond = kind_params.get("co2_bond", 0.0)
+        ts = _f64([t.sigma for t in s.types])
+        te = _f64([t.eps for t in s.types])
+        self._keep += [ts, te]
+        p.type_sigma, p.type_eps = _ptr(ts), _ptr(te)
+        p.n_buff, p.m_buff = kind_params.get("n_buff", 14.0), kind_params.get("m_buff", 7.0)
+        p.delta, p.gamma = kind_params.get("delta", 0.07), kind_params.get("gamma", 0.12)
+        p.hfactor = kind_params.get("hfactor", 0.91)
+        p.e_lrc, p.v_lrc, p.du_lrc, p.d2u_lrc = lrc
+        self.params = p
+        self.energy = 0.0
+        self.virial = np.zeros(9)
+
+    def compute(self, f_accum=None):
+        """Force::compute(): adds forces into f_accum (host, (n,3)) when given,
+        else into the device accumulator; returns the reference's outputs."""
+        out = L.VdwOut()
+        fp = None
+        if f_accum is not None:
+            assert f_accum.dtype == np.float64 and f_accum.flags.c_contiguous and f_accum.size == 3 * self.ctx.n
+            fp = _ptr(f_accum)
+        self.ctx.check(self.ctx.lib.smd_vdw_compute(self.ctx.h, C.byref(self.params), self.prec, fp, C.byref(out)))
+        self.energy = out.energy
+        self.virial = np.array(out.virial[:])
+        return {"energy": out.energy, "lj_energy": out.lj_energy, "virial": self.virial.copy(),
+                "ulus": out.ulus, "dudv": out.dudv, "d2udv": out.d2udv}
+
+    def get_energy(self):
+        return self.energy
+
+    def get_virial(self):
+        return self.virial.copy()
+
+
+class LJForce(_VdwForce):
+    KIND = L.VDW_LJ
+
+
+class MieForce(_VdwForce):
+    KIND = L.VDW_MIE
+
+
+class BuckinghamForce(_VdwForce):
+    KIND = L.VDW_BUCK
+
+
+class BuffvdWForce(_VdwForce):
+    KIND = L.VDW_BUFF147
+
+
+def lj_long_range_correct(ctx: Context, cutoff):
+    """LJForce::long_range_correct: (eLrc, vLrc, dUlrc, d2Ulrc)."""
+    sig2, eps4 = ctx.system.lj_table()
+    out = np.zeros(4)
+    ctx.check(ctx.lib.smd_lj_long_range_correct(ctx.h, _ptr(sig2), _ptr(eps4), cutoff, _ptr(out)))
+    return tuple(out)
+
+
+class EwaldForce:
+    """EwaldForce (src/forces/EwaldForce.cpp)."""
+
+    def __init__(self, ctx: Context, *, alpha, real_cut, kcut, use_constraint=True, surface=False,
+                 prec=L.PREC_FP64):
+        self.ctx = ctx
+        self.prec = prec
+        self.params = L.EwaldParams(alpha, real_cut, kcut, int(use_constraint), int(surface))
+        k = C.c_int32()
+        ctx.check(ctx.lib.smd_ewald_setup(ctx.h, C.byref(self.params), C.byref(k)))
+        self.n_kvec = k.value
+
+    def _fp(self, f_accum):
+        if f_accum is None:
+            return None
+        assert f_accum.dtype == np.float64 and f_accum.flags.c_contiguous and f_accum.size == 3 * self.ctx.n
+        return _ptr(f_accum)
+
+    def compute(self, f_accum=None):
+        o = L.EwaldOut()
+        self.ctx.check(self.ctx.lib.smd_ewald_compute(self.ctx.h, self.prec, self._fp(f_accum), C.byref(o)))
+        d = {name: getattr(o, name) for name, _ in L.EwaldOut._fields_ if name != "virial"}
+        d["virial"] = np.array(o.virial[:])
+        return d
+
+    def compute_long(self, f_accum=None):
+        o = L.EwaldLongOut()
+        self.ctx.check(self.ctx.lib.smd_ewald_compute_long(self.ctx.h, self.prec, self._fp(f_accum), C.byref(o)))
+        return {"energy": o.energy, "deriv": o.deriv, "deriv2": o.deriv2, "virial": np.array(o.virial[:])}
+
+
+class LFIntegrator:
+    """Leap-frog + SHAKE + Nose-Hoover (src/integrators/LFIntegrator.cpp)."""
+
+    def __init__(self, ctx: Context, *, dt, temperature, coupl=0.1, tol=1e-8, max_count=10000,
+                 nose_hoover=True, atom_thermo=True):
+        self.ctx = ctx
+        self.params = L.IntegParams(dt, temperature, coupl, tol, max_count, int(nose_hoover), int(atom_thermo))
+        if ctx.system.atoms_per_mol == 3 and ctx.system.bond_params:
+            b2 = _f64(ctx.system.constraint_lengths2())
+            ctx.check(ctx.lib.smd_set_constraints(ctx.h, _ptr(b2), ctx.nmol))
+
+    def set_vt_minus(self, v):
+        self.ctx._vec_in(self.ctx.lib.smd_set_vt_minus, v, self.ctx.n)
+
+    def vt_minus(self):
+        return self.ctx._vec_out(self.ctx.lib.smd_get_vt_minus, self.ctx.n)
+
+    def set_mol_momenta(self, p):
+        self.ctx._vec_in(self.ctx.lib.smd_set_mol_momenta, p, self.ctx.nmol)
+
+    def shake(self, fnh=0.0):
+        """LFIntegrator::shake with the device forces; returns the shake virial."""
+        vir = np.zeros(9)
+        self.ctx.check(self.ctx.lib.smd_shake(self.ctx.h, C.byref(self.params), fnh, _ptr(vir)))
+        return vir
+
+    def lf_atoms(self, fnh=0.0):
+        self.ctx.check(self.ctx.lib.smd_lf_atoms(self.ctx.h, C.byref(self.params), fnh))
+
+    def mol_kinetics(self):
+        out = np.zeros(2)
+        self.ctx.check(self.ctx.lib.smd_mol_kinetics(self.ctx.h, _ptr(out)))
+        return out   # (2*atomKinEnergy, 2*molKinEnergy)
+
+
+class WaterMD:
+    """Whole LFIntegrator::run step for rigid point-charge water on the device."""
+
+    def __init__(self, ctx: Context, *, cutoff, cut_buff, alpha, kcut, dt, temperature, coupl=0.1, tol=1e-8,
+                 max_count=10000, use_cells=True, long_steps=1, lustig=True, prec=L.PREC_FP64,
+                 atom_thermo=True, surface=False, use_graph=True):
+        self.ctx = ctx
+        b2 = _f64(ctx.system.constraint_lengths2())
+        ctx.check(ctx.lib.smd_set_constraints(ctx.h, _ptr(b2), ctx.nmol))
+        self._sig2, self._eps4 = ctx.system.lj_table()
+        p = L.MdParams()
+        p.cutoff, p.cut_buff = cutoff, cut_buff
+        p.ewald = L.EwaldParams(alpha, cutoff, kcut, 1, int(surface))
+        p.integ = L.IntegParams(dt, temperature, coupl, tol, max_count, 1, int(atom_thermo))
+        p.use_cells, p.long_steps, p.lustig, p.prec = int(use_cells), long_steps, int(lustig), prec
+        p.sigma2, p.eps4 = _ptr(self._sig2), _ptr(self._eps4)
+        p.use_graph = int(use_graph)
+        self.params = p
+
+    def init(self):
+        """LFIntegrator::start: forces at t and vtMinus."""
+        self.ctx.check(self.ctx.lib.smd_md_init(self.ctx.h, C.byref(self.params)))
+
+    def run(self, nsteps):
+        r = L.MdReport()
+        self.ctx.check(self.ctx.lib.smd_md_run(self.ctx.h, nsteps, C.byref(r)))
+        d = {name: getattr(r, name) for name, _ in L.MdReport._fields_ if name != "virial"}
+        d["virial"] = np.array(r.virial[:])
+        return d
