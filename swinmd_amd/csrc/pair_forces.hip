@@ -55,6 +55,17 @@ struct PairArgs {
     double alpha, alpha2, alpha2PI, realcut2;
 };
 
+// CHARMM-type switch of MieForce/BuckinghamForce (src/forces/MieForce.cpp:145-154)
+// in FP64 from the FP64 displacement: its terms cancel near the cutoff.
+__device__ __forceinline__ void charmm_switch(const PairArgs& a, double dx, double dy, double dz, double& sv,
+                                              double& dsv) {
+    const double r2 = dx * dx + dy * dy + dz * dz;
+    const double cc = a.cut2 - r2;
+    const double c4 = cc * (2 * r2 + a.c2);
+    sv = cc * c4 * a.c1;
+    dsv = a.c3 * (cc * cc - c4);
+}
+
 template <typename R>
 __device__ __forceinline__ R rsqrt_(R x) { return R(1) / sqrt(x); }
 
@@ -159,12 +170,10 @@ __global__ __launch_bounds__(256) void k_pairs(PairArgs a) {
                                     d2ulj = cn * eps * (nm * (nm + 1) * srn - mm * (mm + 1) * srm) * rr * rr * rij_2 * rij_2 + R(2) * dulj;
                                 }
                                 if (a.switch_on && rij2 > (R)a.sw_dist2) {
-                                    const R cc = (R)a.cut2 - rij2;
-                                    const R c4 = cc * (R(2) * rij2 + (R)a.c2);
-                                    const R sv = cc * c4 * (R)a.c1;
-                                    const R dsv = (R)a.c3 * (cc * cc - c4);
-                                    tmpf = tmpf * sv - tmpE * dsv;
-                                    tmpE *= sv;
+                                    double sv, dsv;
+                                    charmm_switch(a, dx, dy, dz, sv, dsv);
+                                    tmpf = (R)((double)tmpf * sv - (double)tmpE * dsv);
+                                    tmpE = (R)((double)tmpE * sv);
                                 }
                             }
                             acc[A_VE] += (double)tmpE;
@@ -239,12 +248,10 @@ __global__ __launch_bounds__(256) void k_pairs(PairArgs a) {
                             d2ulj = (R(2) * r + Cc * rij2) * Bb * Cc * ex - R(54) * Aa * rij6;
                         }
                         if (a.switch_on && rij2 > (R)a.sw_dist2) {
-                            const R cc = (R)a.cut2 - rij2;
-                            const R c4 = cc * (R(2) * rij2 + (R)a.c2);
-                            const R sv = cc * c4 * (R)a.c1;
-                            const R dsv = (R)a.c3 * (cc * cc - c4);
-                            tmpf = tmpf * sv - tmpE * dsv;
-                            tmpE *= sv;
+                            double sv, dsv;
+                            charmm_switch(a, dx, dy, dz, sv, dsv);
+                            tmpf = (R)((double)tmpf * sv - (double)tmpE * dsv);
+                            tmpE = (R)((double)tmpE * sv);
                         }
                         if (a.coulomb) {
                             const R qij = (R)(qi * a.q[j]);
@@ -273,12 +280,14 @@ __global__ __launch_bounds__(256) void k_pairs(PairArgs a) {
                             tmpf = -eps * expnm * p1n * (par2 - R(2)) * invs / (rho + del) -
                                    eps * p1n * par2 * m * rho_m1 * invs / (rho_m + gam);
                             if (a.switch_on && rij2 > (R)a.sw_dist2) {
-                                const R r3 = rij2 * r, r4 = rij2 * rij2, r5 = rij2 * r3;
-                                const R sv = (R)a.s5 * r5 + (R)a.s4 * r4 + (R)a.s3 * r3 + (R)a.s2 * rij2 + (R)a.s1 * r + (R)a.s0;
-                                const R dsv = R(5) * (R)a.s5 * r4 + R(4) * (R)a.s4 * r3 + R(3) * (R)a.s3 * rij2 +
-                                              R(2) * (R)a.s2 * r + (R)a.s1;
-                                tmpf = tmpf * sv + tmpE * dsv;
-                                tmpE *= sv;
+                                // The taper's coefficients scale as 1/(rc - rsw)^5:
+                                // evaluated in FP64 from the FP64 distance.
+                                const double rd = sqrt(dx * dx + dy * dy + dz * dz);
+                                const double r2d = rd * rd, r3 = r2d * rd, r4 = r2d * r2d, r5 = r2d * r3;
+                                const double sv = a.s5 * r5 + a.s4 * r4 + a.s3 * r3 + a.s2 * r2d + a.s1 * rd + a.s0;
+                                const double dsv = 5.0 * a.s5 * r4 + 4.0 * a.s4 * r3 + 3.0 * a.s3 * r2d + 2.0 * a.s2 * rd + a.s1;
+                                tmpf = (R)((double)tmpf * sv + (double)tmpE * dsv);
+                                tmpE = (R)((double)tmpE * sv);
                             }
                             dulj = tmpf * r;   // always accumulated (BuffvdWForce.cpp:242-244)
                         }
