@@ -64,7 +64,7 @@ static void release(Ctx* c) {
     void* ptrs[] = {c->d_type, c->d_mol, c->d_mol_start, c->d_q, c->d_mass, c->d_excl_off, c->d_excl, c->d_pos,
                     c->d_vel, c->d_vtm, c->d_disp, c->d_force, c->d_com, c->d_mom, c->d_bond2, c->d_terms,
                     c->d_status, c->d_flag, c->d_scalars, c->d_cell_of, c->d_cell_cnt, c->d_cell_start,
-                    c->d_cell_atoms, c->d_nb, c->d_nb_cnt, c->d_kvec, c->d_kmod2, c->d_expk, c->d_kn, c->d_tab,
+                    c->d_cell_atoms, c->d_nb, c->d_nb_cnt, c->d_nb_own, c->d_fvirt, c->d_kvec, c->d_kmod2, c->d_expk, c->d_kn, c->d_kgrp, c->d_ksplit, c->d_fpart,
                     c->d_sfac_part, c->d_sfac, c->d_vdw_a, c->d_vdw_b, c->d_vdw_c, c->d_epos, c->d_fa, c->d_fb,
                     c->d_host};
     for (void* p : ptrs)
@@ -175,6 +175,8 @@ extern "C" int smd_ctx_create(smd_ctx** out, const smd_topology* t, int device) 
     A(c->d_cell_of, sizeof(int32_t) * n);
     A(c->d_cell_atoms, sizeof(int32_t) * n);
     A(c->d_nb_cnt, sizeof(int32_t) * n);
+    A(c->d_nb_own, sizeof(int32_t) * n);
+    A(c->d_fvirt, sizeof(double) * 3 * n);
     A(c->d_vdw_a, sizeof(double) * nt2);
     A(c->d_vdw_b, sizeof(double) * nt2);
     A(c->d_vdw_c, sizeof(double) * nt2);

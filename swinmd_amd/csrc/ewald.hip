@@ -18,9 +18,16 @@
 //   pass 2 (k_recip_force): G lanes per atom split the k-vectors; each lane
 //     accumulates its share of the force and the Lustig sums, then the lanes
 //     are combined with shuffles.
+#include <algorithm>
+
 #include "smd_internal.h"
 
 namespace smd {
+
+constexpr int GK = 8;          // k-vectors per group: consecutive nz within one (nx, ny) row
+constexpr int SF_TILE = 64;    // atoms per LDS tile in the structure-factor pass
+constexpr int RF_KSPLIT = 8;   // k-group ranges per atom in the force pass
+constexpr int KD = 14;         // doubles staged per k-vector in the force pass
 
 // ---- host setup -----------------------------------------------------------
 
@@ -66,17 +73,48 @@ int ewald_setup(Ctx* c, const smd_ewald_params* p) {
     if ((rc = realloc(c->d_kmod2, sizeof(double) * K))) return rc;
     if ((rc = realloc(c->d_expk, sizeof(double) * K))) return rc;
     if ((rc = realloc(c->d_kn, sizeof(int32_t) * 3 * K))) return rc;
-    c->tab_stride = (nmax[0] + 1) + (nmax[1] + 1) + (nmax[2] + 1);
-    if ((rc = realloc(c->d_tab, sizeof(double) * 2 * (size_t)c->tab_stride * c->n))) return rc;
-    // Chunking of pass 1: enough blocks to fill the device.
-    const int kt = (K + WAVE - 1) / WAVE;
-    int chunks = (2048 + kt - 1) / kt;
+    // k-groups: runs of at most GK consecutive nz inside one (nx, ny) row
+    std::vector<int32_t> grp;   // (first k, count) pairs
+    for (int k = 0; k < K; ++k) {
+        const bool cont = k > 0 && kn[3 * k] == kn[3 * (k - 1)] && kn[3 * k + 1] == kn[3 * (k - 1) + 1] &&
+                          kn[3 * k + 2] == kn[3 * (k - 1) + 2] + 1 && grp.back() < GK;
+        if (cont) grp.back() += 1;
+        else { grp.push_back(k); grp.push_back(1); }
+    }
+    const int ng = (int)grp.size() / 2;
+    // force-pass ranges: RF_KSPLIT contiguous group ranges of similar k count
+    std::vector<int32_t> split(RF_KSPLIT + 1, ng);
+    split[0] = 0;
+    {
+        int r = 1, acc = 0;
+        for (int g = 0; g < ng && r < RF_KSPLIT; ++g) {
+            acc += grp[2 * g + 1];
+            if (acc >= (int64_t)K * r / RF_KSPLIT) split[r++] = g + 1;
+        }
+        for (; r < RF_KSPLIT; ++r) split[r] = ng;
+    }
+    int kmax_range = 1;
+    for (int r = 0; r < RF_KSPLIT; ++r) {
+        if (split[r + 1] <= split[r]) continue;
+        const int kb = grp[2 * split[r]], ke = grp[2 * (split[r + 1] - 1)] + grp[2 * (split[r + 1] - 1) + 1];
+        kmax_range = std::max(kmax_range, ke - kb);
+    }
+    c->n_kgrp = ng;
+    c->kmax_range = kmax_range;
+    if ((rc = realloc(c->d_kgrp, sizeof(int32_t) * grp.size()))) return rc;
+    if ((rc = realloc(c->d_ksplit, sizeof(int32_t) * (RF_KSPLIT + 1)))) return rc;
+    if ((rc = realloc(c->d_fpart, sizeof(double) * 3 * (size_t)RF_KSPLIT * c->n))) return rc;
+    // structure-factor pass: about 2048 wavefronts over (k-group, atom chunk)
+    const int gt = (ng + 63) / 64;
+    int chunks = (2048 + gt - 1) / gt;
     int per = (c->n + chunks - 1) / chunks;
-    if (per < 32) per = 32;
+    if (per < 16) per = 16;
     chunks = (c->n + per - 1) / per;
     c->sfac_chunks = chunks;
     if ((rc = realloc(c->d_sfac_part, sizeof(double) * 8 * (size_t)K * chunks))) return rc;
     if ((rc = realloc(c->d_sfac, sizeof(double) * 16 * (size_t)K))) return rc;
+    SMD_HIP(c, hipMemcpy(c->d_kgrp, grp.data(), sizeof(int32_t) * grp.size(), hipMemcpyHostToDevice));
+    SMD_HIP(c, hipMemcpy(c->d_ksplit, split.data(), sizeof(int32_t) * (RF_KSPLIT + 1), hipMemcpyHostToDevice));
     SMD_HIP(c, hipMemcpy(c->d_kvec, kv.data(), sizeof(double) * 3 * K, hipMemcpyHostToDevice));
     SMD_HIP(c, hipMemcpy(c->d_kmod2, k2v.data(), sizeof(double) * K, hipMemcpyHostToDevice));
     SMD_HIP(c, hipMemcpy(c->d_expk, ek.data(), sizeof(double) * K, hipMemcpyHostToDevice));
@@ -215,10 +253,10 @@ int ewald_surface(Ctx* c) {
 
 // ---- reciprocal space ---------------------------------------------------------
 
+
 struct RecipArgs {
-    int n, K, chunks, per, tab_stride;
-    int nmax[3];
-    double v[3];       // 2 pi / L
+    int n, K, ng, chunks, per;
+    double vz;                 // 2 pi / Lz
     const double* pos;
     const double* q;
     const int32_t* mol;
@@ -226,14 +264,14 @@ struct RecipArgs {
     const double* kvec;
     const double* kmod2;
     const double* expk;
-    const int32_t* kn;
-    double* tab;
-    double* part;      // [chunk][K][8]
-    double* sfac;      // [K][16]
+    const int2* grp;           // [ng] (first k, count)
+    const int32_t* split;      // [RF_KSPLIT+1] group ranges of the force pass
+    double* part;              // [chunk][K][8]
+    double* sfac;              // [K][16]
+    double* fpart;             // [RF_KSPLIT][n][3]
     double* force;
     double* terms;
     double alphaR4, pi4V, pi8V;
-    int use_constraint;
 };
 
 template <typename R>
@@ -244,58 +282,98 @@ __device__ __forceinline__ cplx<R> cmul(cplx<R> a, cplx<R> b) {
     return {a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re};
 }
 
-// Per-atom tables X[0..nx], Y[0..ny], Z[0..nz] with X[n] = exp(i n 2pi x/L).
-__global__ __launch_bounds__(256) void k_recip_tables(RecipArgs a) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= a.n) return;
-    double* t = a.tab + (size_t)i * 2 * a.tab_stride;
-    int o = 0;
-    for (int d = 0; d < 3; ++d) {
-        const double th = a.v[d] * a.pos[3 * i + d];
-        for (int m = 0; m <= a.nmax[d]; ++m, ++o) {
-            double s, cc;
-            sincos((double)m * th, &s, &cc);
-            t[2 * o] = cc;
-            t[2 * o + 1] = s;
+// exp(i ph) in R; the FP32 form reduces the phase to [-pi, pi] in FP64 first.
+template <typename R>
+__device__ __forceinline__ cplx<R> expi(double ph);
+template <>
+__device__ __forceinline__ cplx<double> expi<double>(double ph) {
+    double s, c;
+    sincos(ph, &s, &c);
+    return {c, s};
+}
+template <>
+__device__ __forceinline__ cplx<float> expi<float>(double ph) {
+    const double red = ph - 6.283185307179586 * rint(ph * 0.15915494309189535);
+    float s, c;
+    sincosf((float)red, &s, &c);
+    return {c, s};
+}
+
+// Pass 1: lane = k-group, atoms of one chunk streamed through LDS.  Sums per
+// k: sum q sin(k.r), sum q cos(k.r) and, for the volume derivatives, their
+// moments with r - R_mol (reciprocal_term :568-594).
+template <typename R, bool LUSTIG>
+__global__ __launch_bounds__(64) void k_recip_sfac(RecipArgs a) {
+    __shared__ double tile[9][SF_TILE];   // x y z q dx dy dz cos(vz z) sin(vz z)
+    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    const int ch = blockIdx.y;
+    const int i0 = ch * a.per, i1 = min(a.n, i0 + a.per);
+    int k0 = 0, cnt = 0;
+    double kx = 0, ky = 0, kz = 0;
+    if (g < a.ng) {
+        const int2 gg = a.grp[g];
+        k0 = gg.x; cnt = gg.y;
+        kx = a.kvec[3 * k0]; ky = a.kvec[3 * k0 + 1]; kz = a.kvec[3 * k0 + 2];
+    }
+    double acc[GK][8];
+#pragma unroll
+    for (int t = 0; t < GK; ++t)
+#pragma unroll
+        for (int u = 0; u < 8; ++u) acc[t][u] = 0.0;
+    for (int t0 = i0; t0 < i1; t0 += SF_TILE) {
+        const int ia = t0 + threadIdx.x;
+        __syncthreads();
+        if (ia < i1) {
+            const int m = a.mol[ia];
+            const double x = a.pos[3 * ia], y = a.pos[3 * ia + 1], z = a.pos[3 * ia + 2];
+            tile[0][threadIdx.x] = x; tile[1][threadIdx.x] = y; tile[2][threadIdx.x] = z;
+            tile[3][threadIdx.x] = a.q[ia];
+            tile[4][threadIdx.x] = x - a.com[3 * m];
+            tile[5][threadIdx.x] = y - a.com[3 * m + 1];
+            tile[6][threadIdx.x] = z - a.com[3 * m + 2];
+            double s, c;
+            sincos(a.vz * z, &s, &c);
+            tile[7][threadIdx.x] = c; tile[8][threadIdx.x] = s;
+        }
+        __syncthreads();
+        const int na = min(SF_TILE, i1 - t0);
+        if (g < a.ng) {
+            for (int u = 0; u < na; ++u) {
+                const double x = tile[0][u], y = tile[1][u], z = tile[2][u];
+                const R q = (R)tile[3][u];
+                cplx<R> e = expi<R>(x * kx + y * ky + z * kz);
+                const cplx<R> z1{(R)tile[7][u], (R)tile[8][u]};
+                const double dx = tile[4][u], dy = tile[5][u], dz = tile[6][u];
+#pragma unroll
+                for (int t = 0; t < GK; ++t) {
+                    if (t < cnt) {
+                        const double sq = (double)(q * e.im), cq = (double)(q * e.re);
+                        acc[t][0] += sq;
+                        acc[t][1] += cq;
+                        if constexpr (LUSTIG) {
+                            acc[t][2] += cq * dx; acc[t][3] += cq * dy; acc[t][4] += cq * dz;
+                            acc[t][5] += sq * dx; acc[t][6] += sq * dy; acc[t][7] += sq * dz;
+                        }
+                    }
+                    e = cmul(e, z1);
+                }
+            }
         }
     }
-}
-
-__device__ __forceinline__ cplx<double> phase(const double* t, int nx, int ny, int nz, int oy, int oz) {
-    cplx<double> X{t[2 * nx], t[2 * nx + 1]};
-    const int ay = ny < 0 ? -ny : ny, az = nz < 0 ? -nz : nz;
-    cplx<double> Y{t[2 * (oy + ay)], t[2 * (oy + ay) + 1]};
-    cplx<double> Z{t[2 * (oz + az)], t[2 * (oz + az) + 1]};
-    if (ny < 0) Y.im = -Y.im;
-    if (nz < 0) Z.im = -Z.im;
-    return cmul(cmul(X, Y), Z);
-}
-
-__global__ __launch_bounds__(64) void k_recip_sfac(RecipArgs a) {
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    const int ch = blockIdx.y;
-    if (k >= a.K) return;
-    const int nx = a.kn[3 * k], ny = a.kn[3 * k + 1], nz = a.kn[3 * k + 2];
-    const int oy = a.nmax[0] + 1, oz = oy + a.nmax[1] + 1;
-    double S = 0, Cs = 0, cx = 0, cy = 0, cz = 0, sx = 0, sy = 0, sz = 0;
-    const int i0 = ch * a.per, i1 = min(a.n, i0 + a.per);
-    for (int i = i0; i < i1; ++i) {
-        const double* t = a.tab + (size_t)i * 2 * a.tab_stride;
-        const cplx<double> e = phase(t, nx, ny, nz, oy, oz);
-        const double qi = a.q[i];
-        const double sq = qi * e.im, cq = qi * e.re;   // sinAqi, cosAqi (:586-587)
-        const int m = a.mol[i];
-        const double dx = a.pos[3 * i] - a.com[3 * m], dy = a.pos[3 * i + 1] - a.com[3 * m + 1],
-                     dz = a.pos[3 * i + 2] - a.com[3 * m + 2];
-        S += sq; Cs += cq;
-        cx += cq * dx; cy += cq * dy; cz += cq * dz;
-        sx += sq * dx; sy += sq * dy; sz += sq * dz;
+    if (g < a.ng) {
+#pragma unroll
+        for (int t = 0; t < GK; ++t)
+            if (t < cnt) {
+                double* p = a.part + ((size_t)ch * a.K + k0 + t) * 8;
+#pragma unroll
+                for (int u = 0; u < 8; ++u) p[u] = acc[t][u];
+            }
     }
-    double* p = a.part + ((size_t)ch * a.K + k) * 8;
-    p[0] = S; p[1] = Cs; p[2] = cx; p[3] = cy; p[4] = cz; p[5] = sx; p[6] = sy; p[7] = sz;
 }
 
-// sfac row layout: 0 kx 1 ky 2 kz 3 b 4 b|k|^2 5 4 aR4 |k|^2 6 C 7 S 8-10 cosrR 11-13 sinrR
+// Pass 1b: chunk partials summed in chunk order; per-k energy, volume
+// derivatives and virial (:596-607).  sfac row: 0-2 k, 3 b, 4 b|k|^2,
+// 5 4 aR4 |k|^2, 6 C, 7 S, 8-10 cosrR, 11-13 sinrR.
 __global__ __launch_bounds__(256) void k_recip_reduce(RecipArgs a) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     double v[3 + 9] = {0};   // energy, longderiv, long2ndDeriv, virial[9]
@@ -309,7 +387,6 @@ __global__ __launch_bounds__(256) void k_recip_reduce(RecipArgs a) {
         const double kx = a.kvec[3 * k], ky = a.kvec[3 * k + 1], kz = a.kvec[3 * k + 2];
         const double k2 = a.kmod2[k];
         const double sinSum = s[0], cosSum = s[1];
-        // reciprocal_term (:596-607)
         const double te = a.pi4V * a.expk[k] * (sinSum * sinSum + cosSum * cosSum);
         const double coeff = 2.0 * (a.alphaR4 + 1.0 / k2);
         const double kk = (te * kx) * kx + (te * ky) * ky + (te * kz) * kz;
@@ -340,58 +417,83 @@ __global__ __launch_bounds__(256) void k_recip_reduce(RecipArgs a) {
     block_add<12>(v, a.terms + T_RC_E);
 }
 
-template <bool LUSTIG, int G>
+// Pass 2: thread per atom over one range of k-groups whose per-k data sit in
+// LDS (every lane reads the same k, so the reads broadcast).  Force on atom i
+// from k: (8 pi/V) expK q_i (sin_i S_c - cos_i S_s) k (:609-658).
+template <typename R, bool LUSTIG>
 __global__ __launch_bounds__(256) void k_recip_force(RecipArgs a) {
-    const int tid = blockIdx.x * blockDim.x + threadIdx.x;
-    const int i = tid / G, sub = tid % G;
-    double fx = 0, fy = 0, fz = 0;
+    extern __shared__ double kd[];
+    const int ks = blockIdx.y;
+    const int gs = a.split[ks], ge = a.split[ks + 1];
+    const int kbeg = gs < ge ? a.grp[gs].x : 0;
+    const int kend = gs < ge ? a.grp[ge - 1].x + a.grp[ge - 1].y : 0;
+    for (int idx = threadIdx.x; idx < (kend - kbeg) * KD; idx += blockDim.x) {
+        const int kk = idx / KD, u = idx % KD;
+        kd[idx] = a.sfac[(size_t)(kbeg + kk) * 16 + u];
+    }
+    __syncthreads();
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    double F[3] = {0, 0, 0};
     double l[4] = {0, 0, 0, 0};   // Ff, Ffk2, d2UV1, d2UrR
     if (i < a.n) {
-        const double* t = a.tab + (size_t)i * 2 * a.tab_stride;
-        const int oy = a.nmax[0] + 1, oz = oy + a.nmax[1] + 1;
-        const double qi = a.q[i];
         const int m = a.mol[i];
-        const double dx = a.pos[3 * i] - a.com[3 * m], dy = a.pos[3 * i + 1] - a.com[3 * m + 1],
-                     dz = a.pos[3 * i + 2] - a.com[3 * m + 2];
-        const double d2 = dx * dx + dy * dy + dz * dz;
-        for (int k = sub; k < a.K; k += G) {
-            const int nx = a.kn[3 * k], ny = a.kn[3 * k + 1], nz = a.kn[3 * k + 2];
-            const cplx<double> e = phase(t, nx, ny, nz, oy, oz);
-            const double s = qi * e.im, c = qi * e.re;
-            const double* r = a.sfac + (size_t)k * 16;
-            const double b = r[3], C = r[6], S = r[7];
-            const double val = b * (s * C - c * S);   // :613
-            fx += val * r[0]; fy += val * r[1]; fz += val * r[2];
-            if constexpr (LUSTIG) {
-                const double kd = r[0] * dx + r[1] * dy + r[2] * dz;
-                l[0] += val * kd;
-                l[1] += val * r[5] * kd;
-                const double bkk = r[4];
-                l[2] += bkk * (c * C + s * S) * d2;
-                l[3] += bkk * ((c * r[8] + s * r[11]) * dx + (c * r[9] + s * r[12]) * dy + (c * r[10] + s * r[13]) * dz);
+        const double x = a.pos[3 * i], y = a.pos[3 * i + 1], z = a.pos[3 * i + 2];
+        const R q = (R)a.q[i];
+        const double dxd = x - a.com[3 * m], dyd = y - a.com[3 * m + 1], dzd = z - a.com[3 * m + 2];
+        const R dx = (R)dxd, dy = (R)dyd, dz = (R)dzd;
+        const R d2 = dx * dx + dy * dy + dz * dz;
+        const cplx<R> z1 = expi<R>(a.vz * z);
+        for (int g = gs; g < ge; ++g) {
+            const int2 gg = a.grp[g];
+            const double* r0 = kd + (gg.x - kbeg) * KD;
+            cplx<R> e = expi<R>(x * r0[0] + y * r0[1] + z * r0[2]);
+#pragma unroll
+            for (int t = 0; t < GK; ++t) {
+                if (t < gg.y) {
+                    const double* r = r0 + t * KD;
+                    const R s = q * e.im, c = q * e.re;
+                    const R C = (R)r[6], S = (R)r[7];
+                    const R val = (R)r[3] * (s * C - c * S);
+                    const R kx = (R)r[0], ky = (R)r[1], kz = (R)r[2];
+                    F[0] += (double)(val * kx); F[1] += (double)(val * ky); F[2] += (double)(val * kz);
+                    if constexpr (LUSTIG) {
+                        const R kdd = kx * dx + ky * dy + kz * dz;
+                        l[0] += (double)(val * kdd);
+                        l[1] += (double)(val * (R)r[5] * kdd);
+                        const R bkk = (R)r[4];
+                        l[2] += (double)(bkk * (c * C + s * S) * d2);
+                        l[3] += (double)(bkk * ((c * (R)r[8] + s * (R)r[11]) * dx + (c * (R)r[9] + s * (R)r[12]) * dy +
+                                                (c * (R)r[10] + s * (R)r[13]) * dz));
+                    }
+                }
+                e = cmul(e, z1);
             }
         }
+        double* fp = a.fpart + ((size_t)ks * a.n + i) * 3;
+        fp[0] = F[0]; fp[1] = F[1]; fp[2] = F[2];
     }
-#pragma unroll
-    for (int o = G / 2; o > 0; o >>= 1) {
-        fx += __shfl_xor(fx, o, G);
-        fy += __shfl_xor(fy, o, G);
-        fz += __shfl_xor(fz, o, G);
-    }
-    if (i < a.n && sub == 0) atomic_add3(a.force + 3 * i, fx, fy, fz);
     if constexpr (LUSTIG) block_add<4>(l, a.terms + T_RC_FF);
 }
 
+// Adds the RF_KSPLIT partial forces in a fixed order.
+__global__ __launch_bounds__(256) void k_recip_fsum(int n, const double* __restrict__ fpart, double* __restrict__ force) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double f[3] = {0, 0, 0};
+    for (int ks = 0; ks < RF_KSPLIT; ++ks)
+        for (int d = 0; d < 3; ++d) f[d] += fpart[((size_t)ks * n + i) * 3 + d];
+    for (int d = 0; d < 3; ++d) force[3 * i + d] += f[d];
+}
+
 int ewald_recip(Ctx* c, int prec, int lustig) {
-    (void)prec;
     if (!c->ewald_ready) return fail(c, SMD_ESTATE, "Ewald not set up");
     RecipArgs a;
     a.n = c->n;
     a.K = c->K;
+    a.ng = c->n_kgrp;
     a.chunks = c->sfac_chunks;
     a.per = (c->n + a.chunks - 1) / a.chunks;
-    a.tab_stride = c->tab_stride;
-    for (int d = 0; d < 3; ++d) { a.nmax[d] = c->nmax[d]; a.v[d] = 2.0 * PI / c->box.L[d]; }
+    a.vz = 2.0 * PI / c->box.L[2];
     a.pos = c->d_pos;
     a.q = c->d_q;
     a.mol = c->d_mol;
@@ -399,32 +501,42 @@ int ewald_recip(Ctx* c, int prec, int lustig) {
     a.kvec = c->d_kvec;
     a.kmod2 = c->d_kmod2;
     a.expk = c->d_expk;
-    a.kn = c->d_kn;
-    a.tab = c->d_tab;
+    a.grp = reinterpret_cast<const int2*>(c->d_kgrp);
+    a.split = c->d_ksplit;
     a.part = c->d_sfac_part;
     a.sfac = c->d_sfac;
+    a.fpart = c->d_fpart;
     a.force = c->d_force;
     a.terms = c->d_terms;
     a.alphaR4 = 1.0 / (4.0 * (c->ew.alpha * c->ew.alpha));
     const double volume = c->box.L[0] * c->box.L[1] * c->box.L[2];
     a.pi4V = 4.0 * PI * (1.0 / volume);
     a.pi8V = 2.0 * a.pi4V;
-    a.use_constraint = c->ew.use_constraint;
-    {
-        ProfScope ps(c, "recip_tables");
-        hipLaunchKernelGGL(k_recip_tables, dim3((c->n + 255) / 256), dim3(256), 0, c->stream, a);
-    }
+    const bool mixed = prec == SMD_PREC_MIXED;
     {
         ProfScope ps(c, "recip_sfac");
-        hipLaunchKernelGGL(k_recip_sfac, dim3((a.K + 63) / 64, a.chunks), dim3(64), 0, c->stream, a);
+        const dim3 g1((a.ng + 63) / 64, a.chunks);
+        if (mixed) {
+            if (lustig) hipLaunchKernelGGL((k_recip_sfac<float, true>), g1, dim3(64), 0, c->stream, a);
+            else hipLaunchKernelGGL((k_recip_sfac<float, false>), g1, dim3(64), 0, c->stream, a);
+        } else {
+            if (lustig) hipLaunchKernelGGL((k_recip_sfac<double, true>), g1, dim3(64), 0, c->stream, a);
+            else hipLaunchKernelGGL((k_recip_sfac<double, false>), g1, dim3(64), 0, c->stream, a);
+        }
         hipLaunchKernelGGL(k_recip_reduce, dim3((a.K + 255) / 256), dim3(256), 0, c->stream, a);
     }
     {
         ProfScope ps(c, "recip_force");
-        constexpr int G = 16;
-        const dim3 grid((unsigned)(((int64_t)c->n * G + 255) / 256));
-        if (lustig) hipLaunchKernelGGL((k_recip_force<true, G>), grid, dim3(256), 0, c->stream, a);
-        else hipLaunchKernelGGL((k_recip_force<false, G>), grid, dim3(256), 0, c->stream, a);
+        const dim3 g2((c->n + 255) / 256, RF_KSPLIT);
+        const size_t lds = sizeof(double) * KD * (size_t)c->kmax_range;
+        if (mixed) {
+            if (lustig) hipLaunchKernelGGL((k_recip_force<float, true>), g2, dim3(256), lds, c->stream, a);
+            else hipLaunchKernelGGL((k_recip_force<float, false>), g2, dim3(256), lds, c->stream, a);
+        } else {
+            if (lustig) hipLaunchKernelGGL((k_recip_force<double, true>), g2, dim3(256), lds, c->stream, a);
+            else hipLaunchKernelGGL((k_recip_force<double, false>), g2, dim3(256), lds, c->stream, a);
+        }
+        hipLaunchKernelGGL(k_recip_fsum, dim3((c->n + 255) / 256), dim3(256), 0, c->stream, c->n, c->d_fpart, c->d_force);
     }
     SMD_HIP(c, hipGetLastError());
     return SMD_OK;

@@ -102,6 +102,7 @@ __global__ __launch_bounds__(256) void k_shake(IntegArgs a) {
         const double Px = a.mom[3 * m], Py = a.mom[3 * m + 1], Pz = a.mom[3 * m + 2];
         const double tol_2 = 2 * a.tol;
         double ro[3][3], rn[3][3], mr[3];
+#pragma unroll
         for (int s = 0; s < 3; ++s) {
             const int id = a0 + s;
             mr[s] = 1.0 / a.mass[id];
@@ -119,37 +120,57 @@ __global__ __launch_bounds__(256) void k_shake(IntegArgs a) {
                 rn[s][d] = ro[s][d] + a.dt * vp;
             }
         }
+        // Squared lengths and old-bond vectors are loop invariants; the three
+        // constraints are unrolled so every array index is a constant and the
+        // per-molecule state stays in registers.
+        double bl[3], od[3][3], msum[3];
+#pragma unroll
+        for (int b = 0; b < 3; ++b) {
+            const int nx = b == 2 ? 0 : b + 1;
+            bl[b] = a.bond2[3 * m + b];
+            od[b][0] = ro[b][0] - ro[nx][0];
+            od[b][1] = ro[b][1] - ro[nx][1];
+            od[b][2] = ro[b][2] - ro[nx][2];
+            pbc(od[b][0], od[b][1], od[b][2], a.box);
+            msum[b] = mr[b] + mr[nx];
+        }
         int count = 0;
         bool done = false, failed = false;
         while (!done && count < a.max_count && !failed) {
             done = true;
+#pragma unroll
             for (int b = 0; b < 3; ++b) {
-                const int nx = (b + 1) % 3;
-                double nX = rn[b][0] - rn[nx][0], nY = rn[b][1] - rn[nx][1], nZ = rn[b][2] - rn[nx][2];
-                pbc(nX, nY, nZ, a.box);
-                const double bl = a.bond2[3 * m + b];
-                const double diff = bl - len2_exact(nX, nY, nZ);
-                if (fabs(diff) > bl * tol_2) {
-                    double oX = ro[b][0] - ro[nx][0], oY = ro[b][1] - ro[nx][1], oZ = ro[b][2] - ro[nx][2];
-                    pbc(oX, oY, oZ, a.box);
-                    const double rno = oX * nX + oY * nY + oZ * nZ;
-                    if (rno < bl * a.tol) { failed = true; break; }
-                    const double factor = diff / (2 * rno * (mr[b] + mr[nx]));
-                    v[0] += factor * nX * oX; v[1] += factor * nX * oY; v[2] += factor * nX * oZ;
-                    v[3] += factor * nY * oX; v[4] += factor * nY * oY; v[5] += factor * nY * oZ;
-                    v[6] += factor * nZ * oX; v[7] += factor * nZ * oY; v[8] += factor * nZ * oZ;
-                    const double fX = factor * oX, fY = factor * oY, fZ = factor * oZ;
-                    rn[b][0] += mr[b] * fX; rn[b][1] += mr[b] * fY; rn[b][2] += mr[b] * fZ;
-                    rn[nx][0] -= mr[nx] * fX; rn[nx][1] -= mr[nx] * fY; rn[nx][2] -= mr[nx] * fZ;
-                    done = false;
+                const int nx = b == 2 ? 0 : b + 1;
+                if (!failed) {
+                    double nX = rn[b][0] - rn[nx][0], nY = rn[b][1] - rn[nx][1], nZ = rn[b][2] - rn[nx][2];
+                    pbc(nX, nY, nZ, a.box);
+                    const double diff = bl[b] - len2_exact(nX, nY, nZ);
+                    if (fabs(diff) > bl[b] * tol_2) {
+                        const double oX = od[b][0], oY = od[b][1], oZ = od[b][2];
+                        const double rno = oX * nX + oY * nY + oZ * nZ;
+                        if (rno < bl[b] * a.tol) {
+                            failed = true;
+                        } else {
+                            const double factor = diff / (2 * rno * msum[b]);
+                            v[0] += factor * nX * oX; v[1] += factor * nX * oY; v[2] += factor * nX * oZ;
+                            v[3] += factor * nY * oX; v[4] += factor * nY * oY; v[5] += factor * nY * oZ;
+                            v[6] += factor * nZ * oX; v[7] += factor * nZ * oY; v[8] += factor * nZ * oZ;
+                            const double fX = factor * oX, fY = factor * oY, fZ = factor * oZ;
+                            rn[b][0] += mr[b] * fX; rn[b][1] += mr[b] * fY; rn[b][2] += mr[b] * fZ;
+                            rn[nx][0] -= mr[nx] * fX; rn[nx][1] -= mr[nx] * fY; rn[nx][2] -= mr[nx] * fZ;
+                            done = false;
+                        }
+                    }
                 }
             }
             count++;
         }
         if (failed) atomicOr(a.status, ST_SHAKE_FAIL);
         else if (!done) atomicOr(a.status, ST_SHAKE_NOCONV);
+#pragma unroll
         for (int s = 0; s < 3; ++s) {
             const int id = a0 + s;
+#pragma unroll
             for (int d = 0; d < 3; ++d) {
                 const double p0 = a.pos[3 * id + d];
                 const double dr = rn[s][d] - p0;

@@ -35,6 +35,7 @@ struct PairArgs {
     const int32_t* nb_cnt;
     int32_t stride;
     double* force;
+    double* fvirt;
     double* terms;
     // vdW
     int ntypes;
@@ -113,7 +114,11 @@ __global__ __launch_bounds__(256) void k_pairs(PairArgs a) {
         if constexpr (LUSTIG) { cix = a.com[3 * mi]; ciy = a.com[3 * mi + 1]; ciz = a.com[3 * mi + 2]; }
 
         for (int k = sub; k < cnt; k += G) {
-            const int j = row[k];
+            const int32_t ent = row[k];
+            const int j = ent & INDEX_MASK;
+            // Each pair sits in both atoms' rows; energies, virials and volume
+            // derivatives are taken from the reference owner's row only.
+            const double w = ent < 0 ? 1.0 : 0.0;
             const int mj = a.mol[j];
             // rijm = massCenter_j - massCenter_i, minimum image (LJForce.cpp:112,116)
             R mx = 0, my = 0, mz = 0;
@@ -176,23 +181,24 @@ __global__ __launch_bounds__(256) void k_pairs(PairArgs a) {
                                     tmpE = (R)((double)tmpE * sv);
                                 }
                             }
-                            acc[A_VE] += (double)tmpE;
-                            if constexpr (LUSTIG) { acc[A_VDUDV] += (double)dulj; acc[A_VD2UDV] += (double)d2ulj; }
+                            acc[A_VE] += w * (double)tmpE;
+                            if constexpr (LUSTIG) { acc[A_VDUDV] += w * (double)dulj; acc[A_VD2UDV] += w * (double)d2ulj; }
                             if (a.coulomb) {   // GROMACS shift (LJForce.cpp:188-201)
                                 const R qij = (R)(qi * a.q[j]);
                                 const R e = qij * r_1;
                                 const R f = e * rij_2;
                                 const R r_rc = r / (R)a.cutoff;
                                 const R r4 = rij2 * rij2 * (R)a.rcut_2 * (R)a.rcut_2;
-                                acc[A_VELEC] += (double)(e * (R(1) - R(1.666667) * r_rc + R(1.666667) * r4 - r_rc * r4));
+                                acc[A_VELEC] += w * (double)(e * (R(1) - R(1.666667) * r_rc + R(1.666667) * r4 - r_rc * r4));
                                 tmpf += f * (R(1) - R(5) * r4 + R(4) * r_rc * r4);
                             }
                             const double fx = (double)(-tmpf * rx), fy = (double)(-tmpf * ry), fz = (double)(-tmpf * rz);
                             gfx += fx; gfy += fy; gfz += fz;
                             double* v = acc + A_VVIR;
-                            v[0] += fx * dx; v[1] += fy * dx; v[2] += fz * dx;
-                            v[3] += fx * dy; v[4] += fy * dy; v[5] += fz * dy;
-                            v[6] += fx * dz; v[7] += fy * dz; v[8] += fz * dz;
+                            const double wx = w * fx, wy = w * fy, wz = w * fz;
+                            v[0] += wx * dx; v[1] += wy * dx; v[2] += wz * dx;
+                            v[3] += wx * dy; v[4] += wy * dy; v[5] += wz * dy;
+                            v[6] += wx * dz; v[7] += wy * dz; v[8] += wz * dz;
                         }
                     }
                 }
@@ -203,26 +209,24 @@ __global__ __launch_bounds__(256) void k_pairs(PairArgs a) {
                         const R e = erfc_<R>((R)a.alpha * r) / r;
                         const R ex = exp_<R>(-(R)a.alpha2 * r2);
                         const R f = (e + (R)a.alpha2PI * ex) / r2;
-                        acc[A_EE] += (double)(qij * e);
+                        acc[A_EE] += w * (double)(qij * e);
                         if constexpr (LUSTIG) {
                             const R rr = a.use_constraint ? (rx * mx + ry * my + rz * mz) : r2;
-                            acc[A_ED1] += (double)(-qij * f * rr);
-                            acc[A_ED2] += (double)(qij * (R(2) * f + R(2) * (R)a.alpha2PI * (R)a.alpha2 * ex) * rr * rr / r2 +
-                                                   R(2) * qij * f * rr);
+                            acc[A_ED1] += w * (double)(-qij * f * rr);
+                            acc[A_ED2] += w * (double)(qij * (R(2) * f + R(2) * (R)a.alpha2PI * (R)a.alpha2 * ex) * rr * rr / r2 +
+                                                       R(2) * qij * f * rr);
                         }
                         const R s = -qij * f;
                         const double fx = (double)(s * rx), fy = (double)(s * ry), fz = (double)(s * rz);
                         gfx += fx; gfy += fy; gfz += fz;
                         double* v = acc + A_EVIR;
-                        v[0] += fx * dx; v[1] += fy * dx; v[2] += fz * dx;
-                        v[3] += fx * dy; v[4] += fy * dy; v[5] += fz * dy;
-                        v[6] += fx * dz; v[7] += fy * dz; v[8] += fz * dz;
+                        const double wx = w * fx, wy = w * fy, wz = w * fz;
+                        v[0] += wx * dx; v[1] += wy * dx; v[2] += wz * dx;
+                        v[3] += wx * dy; v[4] += wy * dy; v[5] += wz * dy;
+                        v[6] += wx * dz; v[7] += wy * dz; v[8] += wz * dz;
                     }
                 }
-                if (gfx != 0.0 || gfy != 0.0 || gfz != 0.0) {
-                    fix += gfx; fiy += gfy; fiz += gfz;
-                    atomic_add3(a.force + 3 * j, -gfx, -gfy, -gfz);
-                }
+                fix += gfx; fiy += gfy; fiz += gfz;
             } else if constexpr (VDW == V_BUCK || VDW == V_BUFF) {
                 // rij = ri - rj on the virtual sites (BuckinghamForce.cpp:156,
                 // BuffvdWForce.cpp:172)
@@ -259,7 +263,7 @@ __global__ __launch_bounds__(256) void k_pairs(PairArgs a) {
                             const R f = e * rij_2;
                             const R r_rc = r / (R)a.cutoff;
                             const R r4 = rij2 * rij2 * (R)a.rcut_2 * (R)a.rcut_2;
-                            acc[A_VELEC] += (double)(e * (R(1) - R(1.666667) * r_rc + R(1.666667) * r4 - r_rc * r4));
+                            acc[A_VELEC] += w * (double)(e * (R(1) - R(1.666667) * r_rc + R(1.666667) * r4 - r_rc * r4));
                             tmpf += f * (R(1) - R(5) * r4 + R(4) * r_rc * r4);
                         }
                     } else {
@@ -293,22 +297,20 @@ __global__ __launch_bounds__(256) void k_pairs(PairArgs a) {
                         }
                     }
                     if (live) {
-                        acc[A_VE] += (double)tmpE;
-                        acc[A_VDUDV] += (double)dulj;
-                        if constexpr (LUSTIG) acc[A_VD2UDV] += (double)d2ulj;
+                        acc[A_VE] += w * (double)tmpE;
+                        acc[A_VDUDV] += w * (double)dulj;
+                        if constexpr (LUSTIG) acc[A_VD2UDV] += w * (double)d2ulj;
                         double fx, fy, fz, vx, vy, vz;
                         if constexpr (VDW == V_BUCK) {
                             fx = (double)(-tmpf * rx); fy = (double)(-tmpf * ry); fz = (double)(-tmpf * rz);
-                            vx = fx; vy = fy; vz = fz;
+                            vx = w * fx; vy = w * fy; vz = w * fz;
                         } else {
                             const R r_1 = R(1) / r;
                             fx = (double)(-tmpf * rx * r_1); fy = (double)(-tmpf * ry * r_1); fz = (double)(-tmpf * rz * r_1);
-                            vx = -fx; vy = -fy; vz = -fz;   // virial += -fij * rij (BuffvdWForce.cpp:276-284)
+                            // virial += -fij * rij (BuffvdWForce.cpp:276-284)
+                            vx = -w * fx; vy = -w * fy; vz = -w * fz;
                         }
                         fix += fx; fiy += fy; fiz += fz;
-                        const double faj = a.fa[j], fbj = a.fb[j];
-                        atomic_add3(a.force + 3 * j, -faj * fx, -faj * fy, -faj * fz);
-                        if (fbj != 0.0) atomic_add3(a.force + 3 * a.host[j], -fbj * fx, -fbj * fy, -fbj * fz);
                         double* v = acc + A_VVIR;
                         v[0] += vx * dx; v[1] += vy * dx; v[2] += vz * dx;
                         v[3] += vx * dy; v[4] += vy * dy; v[5] += vz * dy;
@@ -319,20 +321,20 @@ __global__ __launch_bounds__(256) void k_pairs(PairArgs a) {
         }
     }
 
-    // owner force: combine the G lanes, then one atomic per component
+    // Owner force: combine the G lanes.  Row i holds every partner of i, so
+    // lane 0 of the group is the only writer of atom i in this kernel.
 #pragma unroll
     for (int o = G / 2; o > 0; o >>= 1) {
         fix += __shfl_xor(fix, o, G);
         fiy += __shfl_xor(fiy, o, G);
         fiz += __shfl_xor(fiz, o, G);
     }
-    if (i < a.n && sub == 0 && (fix != 0.0 || fiy != 0.0 || fiz != 0.0)) {
+    if (i < a.n && sub == 0) {
         if constexpr (VDW == V_BUCK || VDW == V_BUFF) {
-            const double fai = a.fa[i], fbi = a.fb[i];
-            atomic_add3(a.force + 3 * i, fai * fix, fai * fiy, fai * fiz);
-            if (fbi != 0.0) atomic_add3(a.force + 3 * a.host[i], fbi * fix, fbi * fiy, fbi * fiz);
+            // force on the virtual site; k_redistribute moves it to atoms
+            a.fvirt[3 * i] = fix; a.fvirt[3 * i + 1] = fiy; a.fvirt[3 * i + 2] = fiz;
         } else {
-            atomic_add3(a.force + 3 * i, fix, fiy, fiz);
+            a.force[3 * i] += fix; a.force[3 * i + 1] += fiy; a.force[3 * i + 2] += fiz;
         }
     }
 
@@ -386,6 +388,7 @@ static void base_args(Ctx* c, PairArgs& a) {
     a.nb_cnt = c->d_nb_cnt;
     a.stride = c->stride;
     a.force = c->d_force;
+    a.fvirt = c->d_fvirt;
     a.terms = c->d_terms;
     a.ntypes = c->ntypes;
     a.ta = c->d_vdw_a;
@@ -533,6 +536,25 @@ __global__ void k_virtual_sites(int nmol, const int32_t* __restrict__ mol_start,
     }
 }
 
+// Moves the force on each virtual site back onto its atom and host atom
+// (BuffvdWForce.cpp:262-274, BuckinghamForce.cpp:217-238); one thread per
+// molecule, so the molecule's atoms have a single writer.
+__global__ __launch_bounds__(256) void k_redistribute(int nmol, const int32_t* __restrict__ mol_start,
+                                                      const double* __restrict__ fa, const double* __restrict__ fb,
+                                                      const int32_t* __restrict__ host,
+                                                      const double* __restrict__ fvirt, double* __restrict__ force) {
+    const int m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= nmol) return;
+    for (int a = mol_start[m]; a < mol_start[m + 1]; ++a) {
+        const double x = fvirt[3 * a], y = fvirt[3 * a + 1], z = fvirt[3 * a + 2];
+        force[3 * a] += fa[a] * x; force[3 * a + 1] += fa[a] * y; force[3 * a + 2] += fa[a] * z;
+        if (fb[a] != 0.0) {
+            const int h = host[a];
+            force[3 * h] += fb[a] * x; force[3 * h + 1] += fb[a] * y; force[3 * h + 2] += fb[a] * z;
+        }
+    }
+}
+
 int vdw_run(Ctx* c, const smd_vdw_params* p, int prec, smd_vdw_out* out, bool read_back) {
     if (!c->list_valid) return fail(c, SMD_ESTATE, "no pair list");
     int rc = upload_vdw_tables(c, p);
@@ -552,6 +574,11 @@ int vdw_run(Ctx* c, const smd_vdw_params* p, int prec, smd_vdw_out* out, bool re
                            c->d_mol_start, c->d_type, c->d_pos, c->d_epos, p->kind, p->hfactor, p->co2_bond);
         e = (p->kind == SMD_VDW_BUCK) ? launch_pairs<V_BUCK, false>(c, a, prec, lustig)
                                       : launch_pairs<V_BUFF, false>(c, a, prec, lustig);
+        if (e == hipSuccess) {
+            hipLaunchKernelGGL(k_redistribute, dim3((c->nmol + 255) / 256), dim3(256), 0, c->stream, c->nmol,
+                               c->d_mol_start, c->d_fa, c->d_fb, c->d_host, c->d_fvirt, c->d_force);
+            e = hipGetLastError();
+        }
         break;
     default:
         return fail(c, SMD_EINVAL, "unknown vdW kind");

@@ -109,6 +109,7 @@ struct ListArgs {
     Box box;
     int32_t* nb;
     int32_t* nb_cnt;
+    int32_t* nb_own;
     int32_t stride;
     unsigned* status;
 };
@@ -130,8 +131,10 @@ __device__ __forceinline__ bool excluded(const Excl& e, int j) {
 }
 
 // Test 64 candidates (one per lane) and append the hits to row i in lane order.
+// Entries the reference's half list holds in row i are tagged with OWNED.
 __device__ __forceinline__ void test_and_append(const ListArgs& a, int i, double px, double py, double pz,
-                                                const Excl& ex, bool valid, int j, bool use_pbc, int& count) {
+                                                const Excl& ex, bool valid, int j, bool owned, bool use_pbc,
+                                                int& count, int& n_owned) {
     const int lane = threadIdx.x % WAVE;
     bool hit = false;
     if (valid && !excluded(ex, j)) {
@@ -144,9 +147,10 @@ __device__ __forceinline__ void test_and_append(const ListArgs& a, int i, double
     const uint64_t m = __ballot(hit);
     if (hit) {
         const int slot = count + __popcll(m & ((1ull << lane) - 1ull));
-        if (slot < a.stride) a.nb[(int64_t)i * a.stride + slot] = j;
+        if (slot < a.stride) a.nb[(int64_t)i * a.stride + slot] = owned ? (int32_t)((uint32_t)j | OWNED) : j;
     }
     count += __popcll(m);
+    n_owned += __popcll(__ballot(hit && owned));
 }
 
 __device__ __forceinline__ Excl load_excl(const ListArgs& a, int i) {
@@ -169,19 +173,25 @@ __global__ __launch_bounds__(256) void k_pairs_cells(ListArgs a, CellGrid g, con
     const double px = a.pos[3 * i], py = a.pos[3 * i + 1], pz = a.pos[3 * i + 2];
     const Excl ex = load_excl(a, i);
     const int c = cell_of[i];
-    int count = 0;
-    // same cell: members after i (no PBC, CellManager.cpp:257-279)
+    int count = 0, n_owned = 0;
+    // Same cell: every other member, no PBC (CellManager.cpp:257-279); the
+    // reference's row holds the members after i.
     {
         const int s = start[c], e = start[c + 1];
         for (int k0 = s; k0 < e; k0 += WAVE) {
             const int k = k0 + lane;
             const int j = (k < e) ? catoms[k] : -1;
-            test_and_append(a, i, px, py, pz, ex, (k < e) && (j > i), j, false, count);
+            test_and_append(a, i, px, py, pz, ex, (k < e) && (j != i), j, j > i, false, count, n_owned);
         }
     }
+    // The 13 forward cells (owned, reference order), then the 13 backward
+    // ones.  With at least 3 cells per edge the 26 neighbours are distinct, so
+    // every pair of the reference's list appears once in each of its rows.
     const int cz = c / g.nxy, cy = (c % g.nxy) / g.n[0], cx = c % g.n[0];
-    for (int s13 = 0; s13 < 13; ++s13) {
-        int xx = cx + c_stencil[s13][0], yy = cy + c_stencil[s13][1], zz = cz + c_stencil[s13][2];
+    for (int s26 = 0; s26 < 26; ++s26) {
+        const int sgn = s26 < 13 ? 1 : -1;
+        const int s13 = s26 < 13 ? s26 : s26 - 13;
+        int xx = cx + sgn * c_stencil[s13][0], yy = cy + sgn * c_stencil[s13][1], zz = cz + sgn * c_stencil[s13][2];
         if (xx >= g.n[0]) xx = 0; else if (xx < 0) xx = g.n[0] - 1;
         if (yy >= g.n[1]) yy = 0; else if (yy < 0) yy = g.n[1] - 1;
         if (zz >= g.n[2]) zz = 0; else if (zz < 0) zz = g.n[2] - 1;
@@ -190,15 +200,18 @@ __global__ __launch_bounds__(256) void k_pairs_cells(ListArgs a, CellGrid g, con
         for (int k0 = s; k0 < e; k0 += WAVE) {
             const int k = k0 + lane;
             const int j = (k < e) ? catoms[k] : -1;
-            test_and_append(a, i, px, py, pz, ex, k < e, j, true, count);
+            test_and_append(a, i, px, py, pz, ex, k < e, j, s26 < 13, true, count, n_owned);
         }
     }
     if (lane == 0) {
         a.nb_cnt[i] = count;
+        a.nb_own[i] = n_owned;
         if (count > a.stride) atomicOr(a.status, ST_LIST_OVERFLOW);
     }
 }
 
+// Ensemble::build_pairlist (src/Ensemble.cpp:492-525): all j != i with the
+// min-image predicate; the reference's row i holds the j > i.
 __global__ __launch_bounds__(256) void k_pairs_brute(ListArgs a, const int* gate) {
     if (gate && !*gate) return;
     const int i = blockIdx.x * (blockDim.x / WAVE) + threadIdx.x / WAVE;
@@ -206,13 +219,14 @@ __global__ __launch_bounds__(256) void k_pairs_brute(ListArgs a, const int* gate
     const int lane = threadIdx.x % WAVE;
     const double px = a.pos[3 * i], py = a.pos[3 * i + 1], pz = a.pos[3 * i + 2];
     const Excl ex = load_excl(a, i);
-    int count = 0;
-    for (int j0 = i + 1; j0 < a.n; j0 += WAVE) {
+    int count = 0, n_owned = 0;
+    for (int j0 = 0; j0 < a.n; j0 += WAVE) {
         const int j = j0 + lane;
-        test_and_append(a, i, px, py, pz, ex, j < a.n, j, true, count);
+        test_and_append(a, i, px, py, pz, ex, j < a.n && j != i, j, j > i, true, count, n_owned);
     }
     if (lane == 0) {
         a.nb_cnt[i] = count;
+        a.nb_own[i] = n_owned;
         if (count > a.stride) atomicOr(a.status, ST_LIST_OVERFLOW);
     }
 }
@@ -255,6 +269,7 @@ static int launch_list(Ctx* c, const int* gate) {
     a.box = c->box;
     a.nb = c->d_nb;
     a.nb_cnt = c->d_nb_cnt;
+    a.nb_own = c->d_nb_own;
     a.stride = c->stride;
     a.status = c->d_status;
     const int waves_per_block = 4;
@@ -314,7 +329,7 @@ int pairlist_build(Ctx* c, double cutoff, double cut_buff, int use_cells, bool o
     if (stride <= 0) {
         const double vol = c->box.L[0] * c->box.L[1] * c->box.L[2];
         const double rho = c->n / vol;
-        const double half = 0.5 * rho * 4.0 / 3.0 * PI * len * len * len;
+        const double half = rho * 4.0 / 3.0 * PI * len * len * len;   // full row
         stride = (int32_t)(2.5 * half) + 64;
         stride = (stride + 63) / 64 * 64;
         if (stride > c->n) stride = ((c->n + 63) / 64) * 64;
@@ -364,34 +379,42 @@ extern "C" int smd_set_pairlist_capacity(smd_ctx* c, int32_t row_capacity) {
 extern "C" int smd_pairlist_info(smd_ctx* c, int64_t* n_pairs, int32_t* max_row, int32_t* used_cells) {
     if (!c) return SMD_EINVAL;
     if (!c->list_valid) return fail(c, SMD_ESTATE, "no pair list");
-    std::vector<int32_t> cnt(c->n);
+    std::vector<int32_t> cnt(c->n), own(c->n);
     SMD_HIP(c, hipStreamSynchronize(c->stream));
     SMD_HIP(c, hipMemcpy(cnt.data(), c->d_nb_cnt, sizeof(int32_t) * c->n, hipMemcpyDeviceToHost));
+    SMD_HIP(c, hipMemcpy(own.data(), c->d_nb_own, sizeof(int32_t) * c->n, hipMemcpyDeviceToHost));
     int64_t tot = 0;
     int32_t mx = 0;
-    for (int32_t v : cnt) { tot += v; mx = v > mx ? v : mx; }
+    for (int i = 0; i < c->n; ++i) { tot += own[i]; mx = cnt[i] > mx ? cnt[i] : mx; }
     if (n_pairs) *n_pairs = tot;
     if (max_row) *max_row = mx;
     if (used_cells) *used_cells = c->list_cells ? 1 : 0;
     return SMD_OK;
 }
 
+// The reference's half list is the OWNED subsequence of each device row.
 extern "C" int smd_get_pairlist(smd_ctx* c, int32_t* offsets, int32_t* nb, int64_t cap) {
     if (!c || !offsets) return SMD_EINVAL;
     if (!c->list_valid) return fail(c, SMD_ESTATE, "no pair list");
-    std::vector<int32_t> cnt(c->n);
+    std::vector<int32_t> cnt(c->n), own(c->n);
     SMD_HIP(c, hipStreamSynchronize(c->stream));
     SMD_HIP(c, hipMemcpy(cnt.data(), c->d_nb_cnt, sizeof(int32_t) * c->n, hipMemcpyDeviceToHost));
+    SMD_HIP(c, hipMemcpy(own.data(), c->d_nb_own, sizeof(int32_t) * c->n, hipMemcpyDeviceToHost));
     offsets[0] = 0;
     for (int i = 0; i < c->n; ++i) {
         if (cnt[i] > c->stride) return fail(c, SMD_ECAPACITY, "row %d overflowed", i);
-        offsets[i + 1] = offsets[i] + cnt[i];
+        offsets[i + 1] = offsets[i] + own[i];
     }
     if (offsets[c->n] > cap) return fail(c, SMD_ECAPACITY, "need %d entries, cap %lld", offsets[c->n], (long long)cap);
     if (!nb) return SMD_OK;
     std::vector<int32_t> rows((size_t)c->n * c->stride);
     SMD_HIP(c, hipMemcpy(rows.data(), c->d_nb, rows.size() * sizeof(int32_t), hipMemcpyDeviceToHost));
-    for (int i = 0; i < c->n; ++i)
-        for (int k = 0; k < cnt[i]; ++k) nb[offsets[i] + k] = rows[(size_t)i * c->stride + k];
+    for (int i = 0; i < c->n; ++i) {
+        int o = offsets[i];
+        for (int k = 0; k < cnt[i]; ++k) {
+            const int32_t e = rows[(size_t)i * c->stride + k];
+            if (e < 0) nb[o++] = e & INDEX_MASK;
+        }
+    }
     return SMD_OK;
 }

@@ -15,6 +15,9 @@
 namespace smd {
 
 constexpr int WAVE = 64;
+// Pair-list entries of the reference's half list carry this bit.
+constexpr uint32_t OWNED = 0x80000000u;
+constexpr int32_t INDEX_MASK = 0x7fffffff;
 constexpr double PI = 3.14159265358979323846;       // src/NEMD_defs.h:77
 constexpr double SQRT_PI = 1.7724538509055160273;   // src/NEMD_defs.h:78
 constexpr double BOLTZ = 0.0083145112;              // src/NEMD_defs.h:108
@@ -94,7 +97,8 @@ struct Ctx {
     int32_t* d_cell_start = nullptr;  // [ncell+1]
     int32_t* d_cell_atoms = nullptr;  // [n]
     int32_t* d_nb = nullptr;          // [n * stride]
-    int32_t* d_nb_cnt = nullptr;      // [n]
+    int32_t* d_nb_cnt = nullptr;      // [n] full row length
+    int32_t* d_nb_own = nullptr;      // [n] entries tagged OWNED (the reference row)
     int32_t stride = 0;               // allocated row capacity
     int32_t want_stride = 0;          // user-requested capacity (0 = auto)
     bool list_valid = false;
@@ -114,11 +118,13 @@ struct Ctx {
     double* d_kmod2 = nullptr;    // [K]
     double* d_expk = nullptr;     // [K]
     int32_t* d_kn = nullptr;      // [3K] integer (nx, ny, nz)
-    double* d_tab = nullptr;      // per-atom phase tables
+    int32_t* d_kgrp = nullptr;    // [2*n_kgrp] (first k, count) of each k-group
+    int32_t* d_ksplit = nullptr;  // group ranges of the reciprocal force pass
+    double* d_fpart = nullptr;    // per-range partial reciprocal forces
+    int n_kgrp = 0, kmax_range = 0;
     double* d_sfac_part = nullptr;  // [nchunk][K][8]
     double* d_sfac = nullptr;       // [K][8]
     int sfac_chunks = 0;
-    int tab_stride = 0;             // complex entries per atom
 
     // vdW tables (device)
     double* d_vdw_a = nullptr;    // [ntypes^2] per kind meaning
@@ -128,6 +134,7 @@ struct Ctx {
     double* d_fa = nullptr;       // [n] force fraction kept on the atom
     double* d_fb = nullptr;       // [n] fraction moved to the host atom
     int32_t* d_host = nullptr;    // [n]
+    double* d_fvirt = nullptr;    // [3n] force on each (virtual) site before redistribution
 
     // MD driver
     bool md_ready = false;
