@@ -1,0 +1,262 @@
+#!/usr/bin/env python3
+"""Throughput of the rigid-water MD step on one MI355X.
+
+Workload: 4,096 rigid 3-site point-charge waters (12,288 atoms) at 1.0 g/cm^3
+in a 4.9669 nm cubic box; LJ on O (sigma 0.3166 nm, eps 0.6502 kJ/mol,
+energy-shifted at 0.9 nm), Ewald with alpha 3.5 nm^-1, real cutoff 0.9 nm and
+the k-vectors |n|^2 <= 81, cell pair list with a 0.1 nm buffer, SHAKE to
+1e-8, Nose-Hoover at 300 K, dt 2 fs: one LFIntegrator::run step per "step"
+(src/integrators/LFIntegrator.cpp:278-941).  The state is resident on the GPU
+for the whole timed region.
+
+Measured: steps/s of the mixed-precision path (FP32 pair and k-space math,
+FP64 accumulation) and of the plain FP64 path from the same equilibrated
+coordinates, the relative RMS difference of their total nonbonded forces on
+those coordinates, per-kernel averages from HIP events on the library's
+stream, and the reference CPU implementation timed on a short sample of the
+same system.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
+Prints one JSON line.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import subprocess
+import sys
+import tempfile
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "steps/s (ns/day) vs plain FP64 GPU path at ≤1e-5 rel RMS force error"
+PEAK_FP32_TFLOPS = 157.3     # MI355X published FP32 vector peak
+PEAK_FP64_TFLOPS = 78.6      # MI355X published FP64 vector peak
+PEAK_HBM_GBS = 8000.0
+
+MD = dict(cutoff=0.9, cut_buff=0.1, alpha=3.5, dt=0.002, temperature=300.0, coupl=0.1, tol=1e-8)
+N2MAX = 81
+FAMILIES = ["pairlist", "pairs", "ewald_intra", "recip_sfac", "recip_reduce", "recip_force", "recip_fsum",
+            "shake", "kinetics"]
+
+# Algorithmic work of the reference method per unit.  Counting rule: add,
+# sub, mul 1; FMA 2; div, sqrt, rsqrt 1; exp 12; erfc 25; sin+cos 20.
+#   pairs: per unique pair inside the cutoff, LJ + Ewald real space with the
+#     force on both atoms, the energies and a symmetric virial.
+FLOP_PER_PAIR = 99
+#   reciprocal space per (atom, k): phase k.r 5, sin+cos 20, structure-factor
+#     sums 4 (the pass that forms S(k)); force 10 (the pass that uses it).
+FLOP_PER_ATOM_K_SFAC = 29
+FLOP_PER_ATOM_K_FORCE = 10
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def build_md(system, prec, pos, vel, use_graph=True):
+    from swinmd_amd import systems as S
+    from swinmd_amd.forces import Context, WaterMD
+    ctx = Context(system)
+    ctx.set_positions(pos)
+    ctx.set_velocities(vel)
+    md = WaterMD(ctx, cutoff=MD["cutoff"], cut_buff=MD["cut_buff"], alpha=MD["alpha"],
+                 kcut=S.kcut_for(N2MAX, system.box[0]), dt=MD["dt"], temperature=MD["temperature"],
+                 coupl=MD["coupl"], tol=MD["tol"], prec=prec, use_graph=use_graph)
+    md.init()
+    return ctx, md
+
+
+def unique_pairs_in_cutoff(ctx, cutoff):
+    """Pairs of the half list closer than the cutoff (minimum image)."""
+    off, nb = ctx.pairlist()
+    pos = ctx.positions()
+    L = np.asarray(ctx.system.box)
+    i = np.repeat(np.arange(len(off) - 1), np.diff(off))
+    d = pos[nb] - pos[i]
+    d -= L * np.round(d / L)
+    r2 = np.einsum("ij,ij->i", d, d)
+    return int(np.count_nonzero(r2 <= cutoff * cutoff)), int(len(nb))
+
+
+def time_steps(ctx, md, steps):
+    ctx.sync()
+    t0 = time.perf_counter()
+    r = md.run(steps)
+    ctx.sync()
+    return time.perf_counter() - t0, r
+
+
+MPIRUN = "/opt/conda/bin/mpirun"
+
+
+def _ref_seconds(exe, d, nsteps, ranks):
+    cmd = [exe, d, "time", str(nsteps)]
+    if ranks > 1:
+        cmd = [MPIRUN, "-np", str(ranks)] + cmd
+    r = subprocess.run(cmd, capture_output=True, text=True, timeout=600)
+    if r.returncode != 0:
+        log(f"cpu baseline ({ranks} ranks) failed: {r.stderr[-400:]}")
+        return None
+    lines = [x for x in r.stdout.splitlines() if x.startswith("{")]
+    return max(json.loads(x)["seconds"] for x in lines)   # slowest rank
+
+
+def cpu_baseline(system, pos, nsteps, ranks):
+    """The reference SwinMD CPU path (its own sources compiled into
+    oracle/_ref) on `nsteps` MD steps of the same system, with `ranks` MPI
+    ranks (its replicated-data decomposition) and with one."""
+    from oracle import refinput
+    from swinmd_amd import systems as S
+    exe = os.path.join(ROOT, "oracle", "_ref", "ref_harness")
+    if not os.path.exists(exe):
+        return None
+    if not os.path.exists(MPIRUN):
+        ranks = 1
+    with tempfile.TemporaryDirectory() as d:
+        refinput.write_inputs(d, system, pos=pos, cutoff=MD["cutoff"], cut_buff=MD["cut_buff"], alpha=MD["alpha"],
+                              kcut=S.kcut_for(N2MAX, system.box[0]), dt=MD["dt"], nsteps=nsteps,
+                              temperature=MD["temperature"], coupl=MD["coupl"], shake_tol=MD["tol"])
+        secs = _ref_seconds(exe, d, nsteps, ranks)
+        secs1 = _ref_seconds(exe, d, max(1, nsteps // 2), 1) if ranks > 1 else secs
+        if secs is None:
+            return None
+    out = {"value": round(nsteps / secs, 4), "unit": "steps/s", "cores": ranks, "kind": "reference",
+           "sample": f"{nsteps} LFIntegrator::run steps of the same 12,288-atom water from the equilibrated "
+                     f"coordinates, reference sources compiled with g++ -O2, {ranks} MPI ranks: {secs:.2f} s"}
+    if ranks > 1 and secs1:
+        out["value_1_core"] = round(max(1, nsteps // 2) / secs1, 4)
+    return out
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2000)
+    ap.add_argument("--warmup", type=int, default=200)
+    ap.add_argument("--equil", type=int, default=3000, help="untimed equilibration steps from the lattice start")
+    ap.add_argument("--profile-steps", type=int, default=500)
+    ap.add_argument("--cpu-steps", type=int, default=10)
+    ap.add_argument("--cpu-ranks", type=int, default=min(16, os.cpu_count() or 1),
+                    help="MPI ranks of the reference CPU run (the GPU's share of host cores)")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-fp64", action="store_true", help="skip the plain FP64 comparison run")
+    args = ap.parse_args()
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print(f"bench.py measures one GPU only: the MD step runs on a single device and --gpus {args.gpus} "
+              f"(WORLD_SIZE={world}) is not supported", file=sys.stderr)
+        sys.exit(2)
+
+    from swinmd_amd import _lib as L
+    from swinmd_amd import ensemble, systems as S
+    L.load()   # raises if libswinmd_hip.so is missing
+
+    system = S.c2_water()
+    n = system.n_atoms
+    vel0 = ensemble.init_velocity(system, MD["temperature"], seed=1, per_atom=True)
+
+    # equilibrate from the lattice start (mixed precision), then start both
+    # precisions from the same coordinates and velocities
+    log(f"equilibrating {args.equil} steps")
+    ctx, md = build_md(system, L.PREC_MIXED, system.pos, vel0)
+    md.run(args.equil)
+    pos_eq, vel_eq = ctx.positions(), ctx.velocities()
+    ctx.close()
+
+    results = {}
+    for prec, name in ((L.PREC_MIXED, "mixed"), (L.PREC_FP64, "fp64")):
+        if name == "fp64" and args.no_fp64:
+            continue
+        ctx, md = build_md(system, prec, pos_eq, vel_eq)
+        f_start = ctx.forces()
+        md.run(args.warmup)
+        r0 = md.run(0)
+        secs, r = time_steps(ctx, md, args.steps)
+        results[name] = dict(secs=secs, force=f_start, list_updates=r["list_updates"] - r0["list_updates"],
+                             report=r)
+        log(f"{name}: {args.steps / secs:.1f} steps/s, list updates {results[name]['list_updates']}")
+        if name == "mixed":
+            # per-kernel averages: HIP events around each launch on the
+            # library's stream, over a further run of the same step
+            ctx.profile(True)
+            md.run(args.profile_steps)
+            fam = {}
+            for f in FAMILIES:
+                ms, cnt = ctx.profile_get(f)
+                if cnt:
+                    fam[f] = dict(us=ms * 1e3 / cnt, launches=cnt, total_ms=ms)
+            ctx.profile(False)
+            results[name]["families"] = fam
+            results[name]["pairs_in_cut"], results[name]["half_list"] = unique_pairs_in_cutoff(ctx, MD["cutoff"])
+        ctx.close()
+
+    mix = results["mixed"]
+    steps_s = args.steps / mix["secs"]
+    ns_day = steps_s * MD["dt"] * 86400.0 / 1000.0
+    out = {
+        "metric": METRIC, "value": round(steps_s, 2), "unit": "steps/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(1e3 * mix["secs"] / args.steps, 5),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32 math, f64 accumulation",
+        "data": f"synthetic: jittered 16^3 lattice water with random orientations (seed 2024), reference "
+                f"Maxwell velocities (seed 1), {args.equil} untimed equilibration steps",
+        "config": {"workload": "rigid 3-site water MD step (LJ + Ewald + SHAKE + Nose-Hoover), 4096 molecules",
+                   "molecules": 4096, "atoms": n, "box_nm": float(system.box[0]), "cutoff_nm": MD["cutoff"],
+                   "buffer_nm": MD["cut_buff"], "alpha_per_nm": MD["alpha"], "n2max": N2MAX, "dt_ps": MD["dt"],
+                   "shake_tol": MD["tol"], "thermostat": "Nose-Hoover 300 K", "parallelism": "single GPU"},
+        "ns_per_day": round(ns_day, 2),
+        "list_updates_per_step": round(mix["list_updates"] / args.steps, 4),
+    }
+    if "fp64" in results:
+        f64 = results["fp64"]
+        out["fp64_steps_per_s"] = round(args.steps / f64["secs"], 2)
+        out["speedup_vs_fp64"] = round(f64["secs"] / mix["secs"], 3)
+        d = mix["force"] - f64["force"]
+        out["force_rel_rms_vs_fp64"] = float(np.sqrt(np.sum(d * d) / np.sum(f64["force"] ** 2)))
+
+    fam = mix["families"]
+    out["kernels_us"] = {k: round(v["us"], 2) for k, v in fam.items()}
+    # roofline of the dominant modelled kernel
+    K = count_kvectors(system.box[0])
+    models = {}
+    if "pairs" in fam:
+        models["pairs"] = FLOP_PER_PAIR * mix["pairs_in_cut"]
+    if K and "recip_sfac" in fam:
+        models["recip_sfac"] = FLOP_PER_ATOM_K_SFAC * n * K
+    if K and "recip_force" in fam:
+        models["recip_force"] = FLOP_PER_ATOM_K_FORCE * n * K
+    if models:
+        top = max(models, key=lambda k: fam[k]["us"])
+        ach = models[top] / (fam[top]["us"] * 1e-6) / 1e12
+        out["roofline"] = {"kernel": top, "bound": "valu", "achieved": round(ach, 3), "peak": PEAK_FP32_TFLOPS,
+                           "unit": "TFLOP/s", "frac": round(ach / PEAK_FP32_TFLOPS, 5), "traffic": None,
+                           "flop_per_launch": models[top], "avg_us": round(fam[top]["us"], 2)}
+    if not args.no_cpu_baseline:
+        log(f"cpu baseline: {args.cpu_steps} reference steps")
+        out["cpu_baseline"] = cpu_baseline(system, pos_eq, args.cpu_steps, args.cpu_ranks)
+    print(json.dumps(out), flush=True)
+
+
+def count_kvectors(L):
+    """Number of k-vectors of the reference's half space with |k| < kCut."""
+    from swinmd_amd import systems as S
+    kcut = S.kcut_for(N2MAX, L)
+    nmax = int(kcut * L / (2 * math.pi))
+    v = 2 * math.pi / L
+    K = 0
+    for nx in range(0, nmax + 1):
+        for ny in range(-nmax if nx else 0, nmax + 1):
+            for nz in range(-nmax if (nx or ny) else 1, nmax + 1):
+                if (nx * nx + ny * ny + nz * nz) * v * v < kcut * kcut:
+                    K += 1
+    return K
+
+
+if __name__ == "__main__":
+    main()

@@ -255,10 +255,17 @@ int smd_md_run(smd_ctx* ctx, int64_t nsteps, smd_md_report* out);
 /* When enabled, every launch of the named kernel families is bracketed by HIP
  * events on the context stream. */
 int smd_profile_enable(smd_ctx* ctx, int on);
-/* Kernel families: "pairlist", "pairs", "ewald_intra", "recip_tables",
- * "recip_sfac", "recip_force", "shake", "kinetics".  total_ms and launches
+/* Kernel families: "pairlist", "pairs", "ewald_intra", "recip_sfac",
+ * "recip_reduce", "recip_force", "shake", "kinetics".  total_ms and launches
  * are cumulative since enabling. */
 int smd_profile_get(smd_ctx* ctx, const char* family, double* total_ms, int64_t* launches);
+
+/* Launch shapes (no counterpart in the reference; results do not depend on
+ * them beyond FP rounding order).  "pair_lanes": lanes sharing one atom's row
+ * in the pair kernels, 4, 8, 16 or 32, or 0 (default: 16 for FP32 pair
+ * math, 32 for FP64).  Drops a captured MD
+ * graph.  SMD_EINVAL for an unknown name or value. */
+int smd_set_tuning(smd_ctx* ctx, const char* name, int value);
 
 #ifdef __cplusplus
 }

@@ -120,6 +120,7 @@ SIGNATURES = {
     "smd_md_run": (C.c_int, [_P, C.c_int64, C.POINTER(MdReport)]),
     "smd_profile_enable": (C.c_int, [_P, C.c_int]),
     "smd_profile_get": (C.c_int, [_P, C.c_char_p, C.POINTER(C.c_double), C.POINTER(C.c_int64)]),
+    "smd_set_tuning": (C.c_int, [_P, C.c_char_p, C.c_int]),
 }
 
 _lib = None

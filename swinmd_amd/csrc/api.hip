@@ -64,8 +64,8 @@ static void release(Ctx* c) {
     void* ptrs[] = {c->d_type, c->d_mol, c->d_mol_start, c->d_q, c->d_mass, c->d_excl_off, c->d_excl, c->d_pos,
                     c->d_vel, c->d_vtm, c->d_disp, c->d_force, c->d_com, c->d_mom, c->d_bond2, c->d_terms,
                     c->d_status, c->d_flag, c->d_scalars, c->d_cell_of, c->d_cell_cnt, c->d_cell_start,
-                    c->d_cell_atoms, c->d_nb, c->d_nb_cnt, c->d_nb_own, c->d_fvirt, c->d_kvec, c->d_kmod2, c->d_expk, c->d_kn, c->d_kgrp, c->d_ksplit, c->d_fpart,
-                    c->d_sfac_part, c->d_sfac, c->d_vdw_a, c->d_vdw_b, c->d_vdw_c, c->d_epos, c->d_fa, c->d_fb,
+                    c->d_cell_atoms, c->d_list_seg, c->d_nb, c->d_nb_cnt, c->d_nb_own, c->d_fvirt, c->d_kvec, c->d_kmod2, c->d_expk, c->d_kn, c->d_kgrp, c->d_ksplit, c->d_kf, c->d_e1,
+                    c->d_sfac_part, c->d_ssum, c->d_vdw_a, c->d_vdw_b, c->d_vdw_c, c->d_epos, c->d_fa, c->d_fb,
                     c->d_host};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
@@ -174,6 +174,7 @@ extern "C" int smd_ctx_create(smd_ctx** out, const smd_topology* t, int device) 
     A(c->d_scalars, sizeof(double) * 8);
     A(c->d_cell_of, sizeof(int32_t) * n);
     A(c->d_cell_atoms, sizeof(int32_t) * n);
+    A(c->d_list_seg, sizeof(int32_t) * 28 * n);
     A(c->d_nb_cnt, sizeof(int32_t) * n);
     A(c->d_nb_own, sizeof(int32_t) * n);
     A(c->d_fvirt, sizeof(double) * 3 * n);
@@ -230,6 +231,7 @@ extern "C" int smd_set_box(smd_ctx* c, const double box[3]) {
         if (!(box[d] > 0)) return fail(c, SMD_EINVAL, "box edge %d not positive", d);
         c->box.L[d] = box[d];
         c->box.h[d] = 0.5 * box[d];   // Ensemble::compute_bound_box halfL (src/Ensemble.cpp:431-433)
+        c->box.inv[d] = 1.0 / box[d];
     }
     c->box_set = true;
     c->list_valid = false;
@@ -271,6 +273,20 @@ extern "C" int smd_profile_enable(smd_ctx* c, int on) {
     resolve_prof(c);
     c->prof.on = on != 0;
     if (on) c->prof.acc.clear();
+    return SMD_OK;
+}
+
+extern "C" int smd_set_tuning(smd_ctx* c, const char* name, int value) {
+    if (!c || !name) return SMD_EINVAL;
+    if (std::string(name) == "pair_lanes") {
+        if (value != 0 && value != 4 && value != 8 && value != 16 && value != 32)
+            return fail(c, SMD_EINVAL, "pair_lanes must be 0 (by precision), 4, 8, 16 or 32 (got %d)", value);
+        c->pair_lanes = value;
+    } else {
+        return fail(c, SMD_EINVAL, "unknown tuning parameter '%s'", name);
+    }
+    if (c->graph_exec) { (void)hipGraphExecDestroy(c->graph_exec); c->graph_exec = nullptr; }
+    if (c->graph) { (void)hipGraphDestroy(c->graph); c->graph = nullptr; }
     return SMD_OK;
 }
 

@@ -5,20 +5,31 @@
 //   surfDipoleCorrect         :816-937   two passes over atoms
 //   reciprocal_term           :529-698   structure factors, then forces
 //
-// Reciprocal space.  exp(i k.r) with k = (nx,ny,nz)*2pi/L factors into
-// X[nx] Y[ny] Z[nz] with X[n] = exp(i n 2pi x/L); each atom's tables (n up
-// to nmax per axis, negative n by conjugation) are built once per step, so
-// every (atom, k) term costs two complex products instead of a sin and a cos.
-//   pass 1 (k_recip_sfac): one lane per k-vector, atoms in chunks along the
-//     grid's second dimension; eight FP64 sums per k per chunk
-//     (sum q sin, sum q cos, and their moments q*(r - R_mol)).
-//   pass 1b (k_recip_reduce): chunk partials summed in chunk order (the
-//     result does not depend on scheduling), per-k energy, virial and volume
-//     derivatives.
-//   pass 2 (k_recip_force): G lanes per atom split the k-vectors; each lane
-//     accumulates its share of the force and the Lustig sums, then the lanes
-//     are combined with shuffles.
+// Reciprocal space.  exp(i k.r) with k = 2 pi (nx/Lx, ny/Ly, nz/Lz) factors
+// into X^nx Y^ny Z^nz with X = exp(i 2 pi x / Lx), ...; the k-vectors of the
+// reference's list come in (nx, ny) rows of consecutive nz, so along a row each
+// term is the previous one times Z.  Rows are cut into groups of at most GK
+// consecutive nz.
+//   pass 1 (k_recip_sfac): a block of SF_WAVES waves owns 64 k-groups (one
+//     per lane) and a chunk of atoms.  Per 64-atom tile the block writes each
+//     atom's powers X^0..X^nmax, Y^0.., Z^0.. to LDS (negative powers are
+//     conjugates); a lane then starts its group at q X^nx Y^ny Z^nz0 and walks
+//     the row with one complex product per k.  Per k it sums q sin, q cos and,
+//     for the volume derivatives, q cos d, q sin d (d = r - R_mol) and
+//     q cos |d|^2, q sin |d|^2.  The waves of a block split the tile's atoms
+//     and are added in LDS, so one partial per (chunk, k) reaches memory.
+//   pass 1b (k_recip_csum): partials summed over chunks in a fixed order.
+//   pass 1c (k_recip_finish): per k, energy, volume derivatives and virial
+//     (:596-607); the reference's per-atom Lustig sums of the force loop
+//     (:609-658) are sums over atoms of products with the per-k sums above,
+//     so they are formed here per k; and (b C, b S) for the force pass.
+//   pass 2 (k_recip_force): a block of RF_WAVES waves owns 64 atoms (one per
+//     lane); wave w takes the w-th range of k-groups, with the groups and
+//     (b C, b S) staged in LDS, walks the rows as pass 1 does, and the waves'
+//     partial forces are added in LDS in wave order.
+// Every sum is taken in a fixed order: results do not depend on scheduling.
 #include <algorithm>
+#include <climits>
 
 #include "smd_internal.h"
 
@@ -26,8 +37,10 @@ namespace smd {
 
 constexpr int GK = 8;          // k-vectors per group: consecutive nz within one (nx, ny) row
 constexpr int SF_TILE = 64;    // atoms per LDS tile in the structure-factor pass
-constexpr int RF_KSPLIT = 8;   // k-group ranges per atom in the force pass
-constexpr int KD = 14;         // doubles staged per k-vector in the force pass
+constexpr int SF_WAVES = 8;    // waves per structure-factor block (same groups, different atoms)
+constexpr int RF_WAVES = 16;   // waves per force block (same atoms, different k ranges)
+constexpr int NSUM = 10;       // per-k sums of the structure-factor pass
+constexpr size_t LDS_MAX = 65536;
 
 // ---- host setup -----------------------------------------------------------
 
@@ -73,48 +86,59 @@ int ewald_setup(Ctx* c, const smd_ewald_params* p) {
     if ((rc = realloc(c->d_kmod2, sizeof(double) * K))) return rc;
     if ((rc = realloc(c->d_expk, sizeof(double) * K))) return rc;
     if ((rc = realloc(c->d_kn, sizeof(int32_t) * 3 * K))) return rc;
-    // k-groups: runs of at most GK consecutive nz inside one (nx, ny) row
-    std::vector<int32_t> grp;   // (first k, count) pairs
-    for (int k = 0; k < K; ++k) {
-        const bool cont = k > 0 && kn[3 * k] == kn[3 * (k - 1)] && kn[3 * k + 1] == kn[3 * (k - 1) + 1] &&
-                          kn[3 * k + 2] == kn[3 * (k - 1) + 2] + 1 && grp.back() < GK;
-        if (cont) grp.back() += 1;
-        else { grp.push_back(k); grp.push_back(1); }
+    // k-groups: runs of at most GK consecutive nz inside one (nx, ny) row,
+    // stored as (first k, count, nx << 16 | (ny & 0xffff), nz of the first k)
+    std::vector<int4> grp;
+    for (int k = 0; k < K;) {
+        int e = k + 1;
+        while (e < K && e - k < GK && kn[3 * e] == kn[3 * k] && kn[3 * e + 1] == kn[3 * k + 1] &&
+               kn[3 * e + 2] == kn[3 * (e - 1) + 2] + 1)
+            ++e;
+        grp.push_back(make_int4(k, e - k, (kn[3 * k] << 16) | (kn[3 * k + 1] & 0xffff), kn[3 * k + 2]));
+        k = e;
     }
-    const int ng = (int)grp.size() / 2;
-    // force-pass ranges: RF_KSPLIT contiguous group ranges of similar k count
-    std::vector<int32_t> split(RF_KSPLIT + 1, ng);
+    const int ng = (int)grp.size();
+    // force-pass ranges: RF_WAVES contiguous group ranges of similar k count
+    std::vector<int32_t> split(RF_WAVES + 1, ng);
     split[0] = 0;
     {
-        int r = 1, acc = 0;
-        for (int g = 0; g < ng && r < RF_KSPLIT; ++g) {
-            acc += grp[2 * g + 1];
-            if (acc >= (int64_t)K * r / RF_KSPLIT) split[r++] = g + 1;
+        int r = 1;
+        int64_t acc = 0;
+        for (int g = 0; g < ng && r < RF_WAVES; ++g) {
+            acc += grp[g].y;
+            if (acc >= (int64_t)K * r / RF_WAVES) split[r++] = g + 1;
         }
-        for (; r < RF_KSPLIT; ++r) split[r] = ng;
+        for (; r < RF_WAVES; ++r) split[r] = ng;
     }
-    int kmax_range = 1;
-    for (int r = 0; r < RF_KSPLIT; ++r) {
-        if (split[r + 1] <= split[r]) continue;
-        const int kb = grp[2 * split[r]], ke = grp[2 * (split[r + 1] - 1)] + grp[2 * (split[r + 1] - 1) + 1];
-        kmax_range = std::max(kmax_range, ke - kb);
+    // LDS of the two passes (FP64 and FP32 forms)
+    const int ne = nmax[0] + nmax[1] + nmax[2] + 3;
+    for (int f = 0; f < 2; ++f) {
+        const size_t rsz = f ? sizeof(float) : sizeof(double);
+        const size_t tables = (size_t)ne * (SF_TILE + 1) * 2 * rsz + 5 * SF_TILE * rsz;
+        const size_t red = (size_t)(SF_WAVES - 1) * WAVE * NSUM * sizeof(double);
+        c->sfac_lds[f] = std::max(tables, red);
+        const size_t kdat = ((size_t)K * 2 * rsz + 15) / 16 * 16 + (size_t)ng * sizeof(int4);
+        c->force_lds[f] = std::max(kdat, (size_t)(RF_WAVES - 1) * WAVE * 3 * sizeof(double));
+        if (c->sfac_lds[f] > LDS_MAX || c->force_lds[f] > LDS_MAX)
+            return fail(c, SMD_ENOTSUP, "reciprocal space too large for the LDS-resident passes (%d k-vectors, nmax %d %d %d)",
+                        K, nmax[0], nmax[1], nmax[2]);
     }
     c->n_kgrp = ng;
-    c->kmax_range = kmax_range;
-    if ((rc = realloc(c->d_kgrp, sizeof(int32_t) * grp.size()))) return rc;
-    if ((rc = realloc(c->d_ksplit, sizeof(int32_t) * (RF_KSPLIT + 1)))) return rc;
-    if ((rc = realloc(c->d_fpart, sizeof(double) * 3 * (size_t)RF_KSPLIT * c->n))) return rc;
-    // structure-factor pass: about 2048 wavefronts over (k-group, atom chunk)
-    const int gt = (ng + 63) / 64;
-    int chunks = (2048 + gt - 1) / gt;
+    if ((rc = realloc(c->d_kgrp, sizeof(int4) * grp.size()))) return rc;
+    if ((rc = realloc(c->d_ksplit, sizeof(int32_t) * (RF_WAVES + 1)))) return rc;
+    if ((rc = realloc(c->d_kf, sizeof(double) * 2 * (size_t)K))) return rc;
+    // structure-factor pass: about one block per CU over (group tile, atom chunk)
+    const int gt = (ng + WAVE - 1) / WAVE;
+    int chunks = std::max(1, 256 / gt);   // at most one block per CU: a single round
     int per = (c->n + chunks - 1) / chunks;
-    if (per < 16) per = 16;
+    if (per < 32) per = 32;
     chunks = (c->n + per - 1) / per;
     c->sfac_chunks = chunks;
-    if ((rc = realloc(c->d_sfac_part, sizeof(double) * 8 * (size_t)K * chunks))) return rc;
-    if ((rc = realloc(c->d_sfac, sizeof(double) * 16 * (size_t)K))) return rc;
-    SMD_HIP(c, hipMemcpy(c->d_kgrp, grp.data(), sizeof(int32_t) * grp.size(), hipMemcpyHostToDevice));
-    SMD_HIP(c, hipMemcpy(c->d_ksplit, split.data(), sizeof(int32_t) * (RF_KSPLIT + 1), hipMemcpyHostToDevice));
+    if ((rc = realloc(c->d_sfac_part, sizeof(double) * NSUM * (size_t)K * chunks))) return rc;
+    if ((rc = realloc(c->d_ssum, sizeof(double) * NSUM * (size_t)K))) return rc;
+    if ((rc = realloc(c->d_e1, sizeof(double) * 6 * (size_t)c->n))) return rc;
+    SMD_HIP(c, hipMemcpy(c->d_kgrp, grp.data(), sizeof(int4) * grp.size(), hipMemcpyHostToDevice));
+    SMD_HIP(c, hipMemcpy(c->d_ksplit, split.data(), sizeof(int32_t) * (RF_WAVES + 1), hipMemcpyHostToDevice));
     SMD_HIP(c, hipMemcpy(c->d_kvec, kv.data(), sizeof(double) * 3 * K, hipMemcpyHostToDevice));
     SMD_HIP(c, hipMemcpy(c->d_kmod2, k2v.data(), sizeof(double) * K, hipMemcpyHostToDevice));
     SMD_HIP(c, hipMemcpy(c->d_expk, ek.data(), sizeof(double) * K, hipMemcpyHostToDevice));
@@ -253,10 +277,10 @@ int ewald_surface(Ctx* c) {
 
 // ---- reciprocal space ---------------------------------------------------------
 
-
 struct RecipArgs {
     int n, K, ng, chunks, per;
-    double vz;                 // 2 pi / Lz
+    int nmax[3];
+    double vk[3];              // 2 pi / L per axis
     const double* pos;
     const double* q;
     const int32_t* mol;
@@ -264,11 +288,12 @@ struct RecipArgs {
     const double* kvec;
     const double* kmod2;
     const double* expk;
-    const int2* grp;           // [ng] (first k, count)
-    const int32_t* split;      // [RF_KSPLIT+1] group ranges of the force pass
-    double* part;              // [chunk][K][8]
-    double* sfac;              // [K][16]
-    double* fpart;             // [RF_KSPLIT][n][3]
+    const int4* grp;           // [ng] (first k, count, nx << 16 | ny, nz0)
+    const int32_t* split;      // [RF_WAVES+1] group ranges of the force pass
+    double* part;              // [chunk][K][NSUM]
+    double* ssum;              // [K][NSUM]
+    void* kf;                  // [K] (b C, b S) as R pairs
+    double2* e1;               // [n][3] exp(i 2 pi r_p / L_p) per atom, written by pass 1
     double* force;
     double* terms;
     double alphaR4, pi4V, pi8V;
@@ -282,77 +307,100 @@ __device__ __forceinline__ cplx<R> cmul(cplx<R> a, cplx<R> b) {
     return {a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re};
 }
 
-// exp(i ph) in R; the FP32 form reduces the phase to [-pi, pi] in FP64 first.
-template <typename R>
-__device__ __forceinline__ cplx<R> expi(double ph);
-template <>
-__device__ __forceinline__ cplx<double> expi<double>(double ph) {
-    double s, c;
-    sincos(ph, &s, &c);
-    return {c, s};
-}
-template <>
-__device__ __forceinline__ cplx<float> expi<float>(double ph) {
-    const double red = ph - 6.283185307179586 * rint(ph * 0.15915494309189535);
-    float s, c;
-    sincosf((float)red, &s, &c);
-    return {c, s};
+// b^n for |n| <= 2^15 by binary powering; negative n through the conjugate
+// (|b| = 1).
+__device__ __forceinline__ cplx<double> cpow(cplx<double> b, int n) {
+    if (n < 0) { b.im = -b.im; n = -n; }
+    cplx<double> r{1.0, 0.0};
+    for (int it = 0; it < 16 && n; ++it) {
+        if (n & 1) r = cmul(r, b);
+        b = cmul(b, b);
+        n >>= 1;
+    }
+    return r;
 }
 
-// Pass 1: lane = k-group, atoms of one chunk streamed through LDS.  Sums per
-// k: sum q sin(k.r), sum q cos(k.r) and, for the volume derivatives, their
-// moments with r - R_mol (reciprocal_term :568-594).
+__device__ __forceinline__ int grp_nx(int v) { return v >> 16; }
+__device__ __forceinline__ int grp_ny(int v) { return (int)(short)(v & 0xffff); }
+
+// Pass 1.  Sums per k (layout of part and ssum): 0 q sin, 1 q cos,
+// 2-4 q cos d, 5-7 q sin d, 8 q cos |d|^2, 9 q sin |d|^2 (reciprocal_term
+// :568-594 and the moments its force loop needs).
 template <typename R, bool LUSTIG>
-__global__ __launch_bounds__(64) void k_recip_sfac(RecipArgs a) {
-    __shared__ double tile[9][SF_TILE];   // x y z q dx dy dz cos(vz z) sin(vz z)
-    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+__global__ __launch_bounds__(SF_WAVES * WAVE) void k_recip_sfac(RecipArgs a) {
+    extern __shared__ double lds[];
+    constexpr int TS = SF_TILE + 1;   // padded table stride: lanes reading different powers spread over banks
+    cplx<R>* tab = reinterpret_cast<cplx<R>*>(lds);
+    const int ey0 = a.nmax[0] + 1, ez0 = ey0 + a.nmax[1] + 1, ne = ez0 + a.nmax[2] + 1;
+    R* misc = reinterpret_cast<R*>(tab + ne * TS);   // [5][SF_TILE]: q, dx, dy, dz, |d|^2
+    const int lane = threadIdx.x % WAVE, w = threadIdx.x / WAVE;
+    const int g = blockIdx.x * WAVE + lane;
     const int ch = blockIdx.y;
     const int i0 = ch * a.per, i1 = min(a.n, i0 + a.per);
-    int k0 = 0, cnt = 0;
-    double kx = 0, ky = 0, kz = 0;
-    if (g < a.ng) {
-        const int2 gg = a.grp[g];
-        k0 = gg.x; cnt = gg.y;
-        kx = a.kvec[3 * k0]; ky = a.kvec[3 * k0 + 1]; kz = a.kvec[3 * k0 + 2];
-    }
-    double acc[GK][8];
+    int4 gg = make_int4(0, 0, 0, 0);
+    if (g < a.ng) gg = a.grp[g];
+    const int gnx = grp_nx(gg.z), gny = grp_ny(gg.z);
+    const int ix = gnx, iy = ey0 + abs(gny), iz = ez0 + abs(gg.w);
+    const R sy = gny < 0 ? R(-1) : R(1), sz = gg.w < 0 ? R(-1) : R(1);
+    double acc[GK][NSUM];
 #pragma unroll
     for (int t = 0; t < GK; ++t)
 #pragma unroll
-        for (int u = 0; u < 8; ++u) acc[t][u] = 0.0;
+        for (int u = 0; u < NSUM; ++u) acc[t][u] = 0.0;
+
     for (int t0 = i0; t0 < i1; t0 += SF_TILE) {
-        const int ia = t0 + threadIdx.x;
+        const int na = min(SF_TILE, i1 - t0);
         __syncthreads();
-        if (ia < i1) {
-            const int m = a.mol[ia];
-            const double x = a.pos[3 * ia], y = a.pos[3 * ia + 1], z = a.pos[3 * ia + 2];
-            tile[0][threadIdx.x] = x; tile[1][threadIdx.x] = y; tile[2][threadIdx.x] = z;
-            tile[3][threadIdx.x] = a.q[ia];
-            tile[4][threadIdx.x] = x - a.com[3 * m];
-            tile[5][threadIdx.x] = y - a.com[3 * m + 1];
-            tile[6][threadIdx.x] = z - a.com[3 * m + 2];
-            double s, c;
-            sincos(a.vz * z, &s, &c);
-            tile[7][threadIdx.x] = c; tile[8][threadIdx.x] = s;
+        for (int idx = threadIdx.x; idx < 4 * SF_TILE; idx += blockDim.x) {
+            const int u = idx % SF_TILE, p = idx / SF_TILE;
+            if (u >= na) continue;
+            const int ia = t0 + u;
+            if (p < 3) {
+                // powers 0..nmax of exp(i 2 pi r_p / L_p), by recurrence in FP64
+                const int base = p == 0 ? 0 : (p == 1 ? ey0 : ez0);
+                double s1, c1;
+                sincos(a.vk[p] * a.pos[3 * ia + p], &s1, &c1);
+                if (blockIdx.x == 0) a.e1[3 * ia + p] = make_double2(c1, s1);
+                double er = 1.0, ei = 0.0;
+                tab[base * TS + u] = {R(1), R(0)};
+                for (int k = 1; k <= a.nmax[p]; ++k) {
+                    const double nr = er * c1 - ei * s1, ni = er * s1 + ei * c1;
+                    er = nr; ei = ni;
+                    tab[(base + k) * TS + u] = {(R)er, (R)ei};
+                }
+            } else {
+                const int m = a.mol[ia];
+                const double dx = a.pos[3 * ia] - a.com[3 * m], dy = a.pos[3 * ia + 1] - a.com[3 * m + 1],
+                             dz = a.pos[3 * ia + 2] - a.com[3 * m + 2];
+                misc[u] = (R)a.q[ia];
+                misc[SF_TILE + u] = (R)dx;
+                misc[2 * SF_TILE + u] = (R)dy;
+                misc[3 * SF_TILE + u] = (R)dz;
+                misc[4 * SF_TILE + u] = (R)(dx * dx + dy * dy + dz * dz);
+            }
         }
         __syncthreads();
-        const int na = min(SF_TILE, i1 - t0);
         if (g < a.ng) {
-            for (int u = 0; u < na; ++u) {
-                const double x = tile[0][u], y = tile[1][u], z = tile[2][u];
-                const R q = (R)tile[3][u];
-                cplx<R> e = expi<R>(x * kx + y * ky + z * kz);
-                const cplx<R> z1{(R)tile[7][u], (R)tile[8][u]};
-                const double dx = tile[4][u], dy = tile[5][u], dz = tile[6][u];
+            for (int u = w; u < na; u += SF_WAVES) {
+                const R q = misc[u];
+                cplx<R> X = tab[ix * TS + u], Y = tab[iy * TS + u], Z = tab[iz * TS + u];
+                Y.im *= sy;
+                Z.im *= sz;
+                cplx<R> e = cmul(cmul(X, Y), Z);
+                e.re *= q;
+                e.im *= q;
+                const cplx<R> z1 = tab[(ez0 + 1) * TS + u];
+                const R dx = misc[SF_TILE + u], dy = misc[2 * SF_TILE + u], dz = misc[3 * SF_TILE + u];
+                const R d2 = misc[4 * SF_TILE + u];
 #pragma unroll
                 for (int t = 0; t < GK; ++t) {
-                    if (t < cnt) {
-                        const double sq = (double)(q * e.im), cq = (double)(q * e.re);
-                        acc[t][0] += sq;
-                        acc[t][1] += cq;
+                    if (t < gg.y) {
+                        acc[t][0] += (double)e.im;
+                        acc[t][1] += (double)e.re;
                         if constexpr (LUSTIG) {
-                            acc[t][2] += cq * dx; acc[t][3] += cq * dy; acc[t][4] += cq * dz;
-                            acc[t][5] += sq * dx; acc[t][6] += sq * dy; acc[t][7] += sq * dz;
+                            acc[t][2] += (double)(e.re * dx); acc[t][3] += (double)(e.re * dy); acc[t][4] += (double)(e.re * dz);
+                            acc[t][5] += (double)(e.im * dx); acc[t][6] += (double)(e.im * dy); acc[t][7] += (double)(e.im * dz);
+                            acc[t][8] += (double)(e.re * d2); acc[t][9] += (double)(e.im * d2);
                         }
                     }
                     e = cmul(e, z1);
@@ -360,30 +408,55 @@ __global__ __launch_bounds__(64) void k_recip_sfac(RecipArgs a) {
             }
         }
     }
-    if (g < a.ng) {
+    // add the waves' sums (wave order) and store one partial per (chunk, k)
+    double* red = lds;   // [(SF_WAVES-1) * WAVE][NSUM]
+    constexpr int NU = LUSTIG ? NSUM : 2;
 #pragma unroll
-        for (int t = 0; t < GK; ++t)
-            if (t < cnt) {
-                double* p = a.part + ((size_t)ch * a.K + k0 + t) * 8;
+    for (int t = 0; t < GK; ++t) {
+        __syncthreads();
+        if (w > 0)
 #pragma unroll
-                for (int u = 0; u < 8; ++u) p[u] = acc[t][u];
+            for (int u = 0; u < NU; ++u) red[((w - 1) * WAVE + lane) * NSUM + u] = acc[t][u];
+        __syncthreads();
+        if (w == 0 && g < a.ng && t < gg.y) {
+            double* p = a.part + ((size_t)ch * a.K + gg.x + t) * NSUM;
+#pragma unroll
+            for (int u = 0; u < NSUM; ++u) {
+                double s = acc[t][u];
+                if (u < NU)
+                    for (int v = 0; v < SF_WAVES - 1; ++v) s += red[(v * WAVE + lane) * NSUM + u];
+                p[u] = s;
             }
+        }
     }
 }
 
-// Pass 1b: chunk partials summed in chunk order; per-k energy, volume
-// derivatives and virial (:596-607).  sfac row: 0-2 k, 3 b, 4 b|k|^2,
-// 5 4 aR4 |k|^2, 6 C, 7 S, 8-10 cosrR, 11-13 sinrR.
-__global__ __launch_bounds__(256) void k_recip_reduce(RecipArgs a) {
+// Pass 1b: chunk partials summed in chunk order, four quarters per value
+// combined in quarter order.
+__global__ __launch_bounds__(256) void k_recip_csum(RecipArgs a) {
+    __shared__ double red[3][WAVE];
+    const int lane = threadIdx.x % WAVE, qt = threadIdx.x / WAVE;
+    const int nv = a.K * NSUM;
+    const int v = blockIdx.x * WAVE + lane;
+    const int c0 = (a.chunks * qt) / 4, c1 = (a.chunks * (qt + 1)) / 4;
+    double s = 0.0;
+    if (v < nv) {
+#pragma unroll 4
+        for (int ch = c0; ch < c1; ++ch) s += a.part[(size_t)ch * nv + v];
+    }
+    if (qt > 0) red[qt - 1][lane] = s;
+    __syncthreads();
+    if (qt == 0 && v < nv) a.ssum[v] = ((s + red[0][lane]) + red[1][lane]) + red[2][lane];
+}
+
+// Pass 1c: per-k energy, volume derivatives and virial (:596-607), the
+// force loop's Lustig sums (:609-658) in per-k form, and (b C, b S).
+template <typename R, bool LUSTIG>
+__global__ __launch_bounds__(256) void k_recip_finish(RecipArgs a) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    double v[3 + 9] = {0};   // energy, longderiv, long2ndDeriv, virial[9]
+    double v[3 + 9 + 4] = {0};   // energy, longDeriv, long2ndDeriv, virial[9], Ff, Ffk2, d2UV1, d2UrR
     if (k < a.K) {
-        double s[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-        for (int ch = 0; ch < a.chunks; ++ch) {
-            const double* p = a.part + ((size_t)ch * a.K + k) * 8;
-#pragma unroll
-            for (int u = 0; u < 8; ++u) s[u] += p[u];
-        }
+        const double* s = a.ssum + (size_t)k * NSUM;
         const double kx = a.kvec[3 * k], ky = a.kvec[3 * k + 1], kz = a.kvec[3 * k + 2];
         const double k2 = a.kmod2[k];
         const double sinSum = s[0], cosSum = s[1];
@@ -406,83 +479,113 @@ __global__ __launch_bounds__(256) void k_recip_reduce(RecipArgs a) {
         v[3 + 6] = v[3 + 2];
         v[3 + 7] = v[3 + 5];
         const double b = a.pi8V * a.expk[k];
-        double* r = a.sfac + (size_t)k * 16;
-        r[0] = kx; r[1] = ky; r[2] = kz; r[3] = b;
-        r[4] = (b * kx) * kx + (b * ky) * ky + (b * kz) * kz;
-        r[5] = 4.0 * a.alphaR4 * k2;
-        r[6] = cosSum; r[7] = sinSum;
-        r[8] = s[2]; r[9] = s[3]; r[10] = s[4];
-        r[11] = s[5]; r[12] = s[6]; r[13] = s[7];
+        if constexpr (LUSTIG) {
+            // sum_i val_i (k.d_i) with val_i = b (q_i sin_i C - q_i cos_i S)
+            const double kSD = kx * s[5] + ky * s[6] + kz * s[7];
+            const double kCD = kx * s[2] + ky * s[3] + kz * s[4];
+            const double ff = b * (cosSum * kSD - sinSum * kCD);
+            v[12] = ff;
+            v[13] = ff * (4.0 * a.alphaR4 * k2);
+            const double bkk = (b * kx) * kx + (b * ky) * ky + (b * kz) * kz;
+            v[14] = bkk * (cosSum * s[8] + sinSum * s[9]);
+            v[15] = bkk * ((s[2] * s[2] + s[3] * s[3] + s[4] * s[4]) + (s[5] * s[5] + s[6] * s[6] + s[7] * s[7]));
+        }
+        R* kf = reinterpret_cast<R*>(a.kf);
+        kf[2 * k] = (R)(b * cosSum);
+        kf[2 * k + 1] = (R)(b * sinSum);
     }
-    block_add<12>(v, a.terms + T_RC_E);
+    block_add<16>(v, a.terms + T_RC_E);
 }
 
-// Pass 2: thread per atom over one range of k-groups whose per-k data sit in
-// LDS (every lane reads the same k, so the reads broadcast).  Force on atom i
-// from k: (8 pi/V) expK q_i (sin_i S_c - cos_i S_s) k (:609-658).
-template <typename R, bool LUSTIG>
-__global__ __launch_bounds__(256) void k_recip_force(RecipArgs a) {
-    extern __shared__ double kd[];
-    const int ks = blockIdx.y;
-    const int gs = a.split[ks], ge = a.split[ks + 1];
-    const int kbeg = gs < ge ? a.grp[gs].x : 0;
-    const int kend = gs < ge ? a.grp[ge - 1].x + a.grp[ge - 1].y : 0;
-    for (int idx = threadIdx.x; idx < (kend - kbeg) * KD; idx += blockDim.x) {
-        const int kk = idx / KD, u = idx % KD;
-        kd[idx] = a.sfac[(size_t)(kbeg + kk) * 16 + u];
+// Bytes of the (b C, b S) table in the force pass's LDS, rounded up so the
+// k-groups that follow it are 16-byte aligned.
+template <typename R>
+__host__ __device__ constexpr size_t kf_bytes(int K) { return ((size_t)K * 2 * sizeof(R) + 15) / 16 * 16; }
+
+// Pass 2: force on atom i, sum_k (b C q sin - b S q cos) k (:609-658).
+template <typename R>
+__global__ __launch_bounds__(RF_WAVES * WAVE) void k_recip_force(RecipArgs a) {
+    extern __shared__ double lds[];
+    R* kf = reinterpret_cast<R*>(lds);                                   // [K][2]
+    int4* grp = reinterpret_cast<int4*>(reinterpret_cast<char*>(lds) + kf_bytes<R>(a.K));   // [ng]
+    {
+        const R* src = reinterpret_cast<const R*>(a.kf);
+        for (int idx = threadIdx.x; idx < 2 * a.K; idx += blockDim.x) kf[idx] = src[idx];
+        for (int idx = threadIdx.x; idx < a.ng; idx += blockDim.x) grp[idx] = a.grp[idx];
     }
     __syncthreads();
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    double F[3] = {0, 0, 0};
-    double l[4] = {0, 0, 0, 0};   // Ff, Ffk2, d2UV1, d2UrR
-    if (i < a.n) {
-        const int m = a.mol[i];
-        const double x = a.pos[3 * i], y = a.pos[3 * i + 1], z = a.pos[3 * i + 2];
-        const R q = (R)a.q[i];
-        const double dxd = x - a.com[3 * m], dyd = y - a.com[3 * m + 1], dzd = z - a.com[3 * m + 2];
-        const R dx = (R)dxd, dy = (R)dyd, dz = (R)dzd;
-        const R d2 = dx * dx + dy * dy + dz * dz;
-        const cplx<R> z1 = expi<R>(a.vz * z);
-        for (int g = gs; g < ge; ++g) {
-            const int2 gg = a.grp[g];
-            const double* r0 = kd + (gg.x - kbeg) * KD;
-            cplx<R> e = expi<R>(x * r0[0] + y * r0[1] + z * r0[2]);
+    const int lane = threadIdx.x % WAVE;
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x / WAVE);
+    const int i = blockIdx.x * WAVE + lane;
+    const int ic = min(i, a.n - 1);   // idle lanes repeat the last atom and store nothing
+    double F[3] = {0.0, 0.0, 0.0};
+    {
+        const double q = a.q[ic];
+        const double vx = a.vk[0], vy = a.vk[1], vz = a.vk[2];
+        const double2 ex = a.e1[3 * ic], ey = a.e1[3 * ic + 1], ez = a.e1[3 * ic + 2];
+        const cplx<double> X1{ex.x, ex.y}, Y1{ey.x, ey.y}, Z1{ez.x, ez.y};
+        const cplx<double> Z1c{Z1.re, -Z1.im};
+        const cplx<R> z1{(R)Z1.re, (R)Z1.im};
+        cplx<double> XY{1.0, 0.0}, Zp{1.0, 0.0};
+        cplx<R> e{R(0), R(0)};
+        int cnx = INT_MIN, cny = 0, zp = 0, next_nz = INT_MIN;
+        const int gs = a.split[w], ge = a.split[w + 1];
+        for (int gi = gs; gi < ge; ++gi) {
+            const int4 gg = grp[gi];
+            const int nx = grp_nx(gg.z), ny = grp_ny(gg.z), nz0 = gg.w;
+            if (!(nx == cnx && ny == cny && nz0 == next_nz)) {
+                // a new row: X^nx Y^ny Z^nz0 q
+                if (nx != cnx) {
+                    XY = cmul(cpow(X1, nx), cpow(Y1, ny));
+                    Zp = cpow(Z1, nz0);
+                    cnx = nx; cny = ny; zp = nz0;
+                } else {
+                    for (; cny < ny; ++cny) XY = cmul(XY, Y1);
+                    for (; zp < nz0; ++zp) Zp = cmul(Zp, Z1);
+                    for (; zp > nz0; --zp) Zp = cmul(Zp, Z1c);
+                }
+                const cplx<double> e0 = cmul(XY, Zp);
+                e = {(R)(q * e0.re), (R)(q * e0.im)};
+            }
+            next_nz = nz0 + GK;   // a following group continues this row only after a full group
+            R sv = R(0), szv = R(0);
+            R nzf = (R)nz0;
+            const R* kd = kf + 2 * gg.x;
 #pragma unroll
             for (int t = 0; t < GK; ++t) {
                 if (t < gg.y) {
-                    const double* r = r0 + t * KD;
-                    const R s = q * e.im, c = q * e.re;
-                    const R C = (R)r[6], S = (R)r[7];
-                    const R val = (R)r[3] * (s * C - c * S);
-                    const R kx = (R)r[0], ky = (R)r[1], kz = (R)r[2];
-                    F[0] += (double)(val * kx); F[1] += (double)(val * ky); F[2] += (double)(val * kz);
-                    if constexpr (LUSTIG) {
-                        const R kdd = kx * dx + ky * dy + kz * dz;
-                        l[0] += (double)(val * kdd);
-                        l[1] += (double)(val * (R)r[5] * kdd);
-                        const R bkk = (R)r[4];
-                        l[2] += (double)(bkk * (c * C + s * S) * d2);
-                        l[3] += (double)(bkk * ((c * (R)r[8] + s * (R)r[11]) * dx + (c * (R)r[9] + s * (R)r[12]) * dy +
-                                                (c * (R)r[10] + s * (R)r[13]) * dz));
-                    }
+                    const R val = e.im * kd[2 * t] - e.re * kd[2 * t + 1];
+                    sv += val;
+                    szv += val * nzf;
                 }
+                nzf += R(1);
                 e = cmul(e, z1);
             }
+            if (gg.y < GK) next_nz = INT_MIN;
+            F[0] += (double)sv * (nx * vx);
+            F[1] += (double)sv * (ny * vy);
+            F[2] += (double)szv * vz;
         }
-        double* fp = a.fpart + ((size_t)ks * a.n + i) * 3;
-        fp[0] = F[0]; fp[1] = F[1]; fp[2] = F[2];
     }
-    if constexpr (LUSTIG) block_add<4>(l, a.terms + T_RC_FF);
-}
-
-// Adds the RF_KSPLIT partial forces in a fixed order.
-__global__ __launch_bounds__(256) void k_recip_fsum(int n, const double* __restrict__ fpart, double* __restrict__ force) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    double f[3] = {0, 0, 0};
-    for (int ks = 0; ks < RF_KSPLIT; ++ks)
-        for (int d = 0; d < 3; ++d) f[d] += fpart[((size_t)ks * n + i) * 3 + d];
-    for (int d = 0; d < 3; ++d) force[3 * i + d] += f[d];
+    // add the waves' forces in wave order
+    __syncthreads();
+    double* red = lds;   // [(RF_WAVES-1) * WAVE][3]
+    if (w > 0) {
+        red[((w - 1) * WAVE + lane) * 3] = F[0];
+        red[((w - 1) * WAVE + lane) * 3 + 1] = F[1];
+        red[((w - 1) * WAVE + lane) * 3 + 2] = F[2];
+    }
+    __syncthreads();
+    if (w == 0 && i < a.n) {
+        for (int v = 0; v < RF_WAVES - 1; ++v) {
+            F[0] += red[(v * WAVE + lane) * 3];
+            F[1] += red[(v * WAVE + lane) * 3 + 1];
+            F[2] += red[(v * WAVE + lane) * 3 + 2];
+        }
+        a.force[3 * i] += F[0];
+        a.force[3 * i + 1] += F[1];
+        a.force[3 * i + 2] += F[2];
+    }
 }
 
 int ewald_recip(Ctx* c, int prec, int lustig) {
@@ -493,7 +596,10 @@ int ewald_recip(Ctx* c, int prec, int lustig) {
     a.ng = c->n_kgrp;
     a.chunks = c->sfac_chunks;
     a.per = (c->n + a.chunks - 1) / a.chunks;
-    a.vz = 2.0 * PI / c->box.L[2];
+    for (int d = 0; d < 3; ++d) {
+        a.nmax[d] = c->nmax[d];
+        a.vk[d] = 2.0 * PI / c->box.L[d];
+    }
     a.pos = c->d_pos;
     a.q = c->d_q;
     a.mol = c->d_mol;
@@ -501,11 +607,12 @@ int ewald_recip(Ctx* c, int prec, int lustig) {
     a.kvec = c->d_kvec;
     a.kmod2 = c->d_kmod2;
     a.expk = c->d_expk;
-    a.grp = reinterpret_cast<const int2*>(c->d_kgrp);
+    a.grp = c->d_kgrp;
     a.split = c->d_ksplit;
     a.part = c->d_sfac_part;
-    a.sfac = c->d_sfac;
-    a.fpart = c->d_fpart;
+    a.ssum = c->d_ssum;
+    a.kf = c->d_kf;
+    a.e1 = reinterpret_cast<double2*>(c->d_e1);
     a.force = c->d_force;
     a.terms = c->d_terms;
     a.alphaR4 = 1.0 / (4.0 * (c->ew.alpha * c->ew.alpha));
@@ -513,30 +620,35 @@ int ewald_recip(Ctx* c, int prec, int lustig) {
     a.pi4V = 4.0 * PI * (1.0 / volume);
     a.pi8V = 2.0 * a.pi4V;
     const bool mixed = prec == SMD_PREC_MIXED;
+    const int f = mixed ? 1 : 0;
     {
         ProfScope ps(c, "recip_sfac");
-        const dim3 g1((a.ng + 63) / 64, a.chunks);
+        const dim3 g1((a.ng + WAVE - 1) / WAVE, a.chunks);
+        const dim3 b1(SF_WAVES * WAVE);
+        // FP64 in both forms: its sums are FP64 FMAs, which cost what FP32
+        // products converted and added in FP64 would.
+        const size_t lds = c->sfac_lds[0];
+        if (lustig) hipLaunchKernelGGL((k_recip_sfac<double, true>), g1, b1, lds, c->stream, a);
+        else hipLaunchKernelGGL((k_recip_sfac<double, false>), g1, b1, lds, c->stream, a);
+    }
+    {
+        ProfScope ps(c, "recip_reduce");
+        hipLaunchKernelGGL(k_recip_csum, dim3((a.K * NSUM + WAVE - 1) / WAVE), dim3(256), 0, c->stream, a);
+        const dim3 gk((a.K + 255) / 256);
         if (mixed) {
-            if (lustig) hipLaunchKernelGGL((k_recip_sfac<float, true>), g1, dim3(64), 0, c->stream, a);
-            else hipLaunchKernelGGL((k_recip_sfac<float, false>), g1, dim3(64), 0, c->stream, a);
+            if (lustig) hipLaunchKernelGGL((k_recip_finish<float, true>), gk, dim3(256), 0, c->stream, a);
+            else hipLaunchKernelGGL((k_recip_finish<float, false>), gk, dim3(256), 0, c->stream, a);
         } else {
-            if (lustig) hipLaunchKernelGGL((k_recip_sfac<double, true>), g1, dim3(64), 0, c->stream, a);
-            else hipLaunchKernelGGL((k_recip_sfac<double, false>), g1, dim3(64), 0, c->stream, a);
+            if (lustig) hipLaunchKernelGGL((k_recip_finish<double, true>), gk, dim3(256), 0, c->stream, a);
+            else hipLaunchKernelGGL((k_recip_finish<double, false>), gk, dim3(256), 0, c->stream, a);
         }
-        hipLaunchKernelGGL(k_recip_reduce, dim3((a.K + 255) / 256), dim3(256), 0, c->stream, a);
     }
     {
         ProfScope ps(c, "recip_force");
-        const dim3 g2((c->n + 255) / 256, RF_KSPLIT);
-        const size_t lds = sizeof(double) * KD * (size_t)c->kmax_range;
-        if (mixed) {
-            if (lustig) hipLaunchKernelGGL((k_recip_force<float, true>), g2, dim3(256), lds, c->stream, a);
-            else hipLaunchKernelGGL((k_recip_force<float, false>), g2, dim3(256), lds, c->stream, a);
-        } else {
-            if (lustig) hipLaunchKernelGGL((k_recip_force<double, true>), g2, dim3(256), lds, c->stream, a);
-            else hipLaunchKernelGGL((k_recip_force<double, false>), g2, dim3(256), lds, c->stream, a);
-        }
-        hipLaunchKernelGGL(k_recip_fsum, dim3((c->n + 255) / 256), dim3(256), 0, c->stream, c->n, c->d_fpart, c->d_force);
+        const dim3 g2((c->n + WAVE - 1) / WAVE);
+        const size_t lds = c->force_lds[f];
+        if (mixed) hipLaunchKernelGGL((k_recip_force<float>), g2, dim3(RF_WAVES * WAVE), lds, c->stream, a);
+        else hipLaunchKernelGGL((k_recip_force<double>), g2, dim3(RF_WAVES * WAVE), lds, c->stream, a);
     }
     SMD_HIP(c, hipGetLastError());
     return SMD_OK;

@@ -33,6 +33,7 @@ struct PairArgs {
     const double* com;
     const int32_t* nb;
     const int32_t* nb_cnt;
+    const int32_t* nb_own;
     int32_t stride;
     double* force;
     double* fvirt;
@@ -80,6 +81,31 @@ template <typename R> __device__ __forceinline__ R pow_(R x, R y);
 template <> __device__ __forceinline__ double pow_<double>(double x, double y) { return pow(x, y); }
 template <> __device__ __forceinline__ float pow_<float>(float x, float y) { return powf(x, y); }
 
+// ---- FP32 real-space Ewald kernel of the mixed-precision form ----------------
+// erfc(z) = exp(-z^2) t P(t), t = 1 / (1 + p z): the form of Abramowitz &
+// Stegun 7.1.26 with a degree-7 P whose coefficients were fitted (weighted
+// least squares toward minimax on the relative error of erfc(z) exp(z^2) / t
+// over z in [0, 5.5]; max relative error 2.3e-8 in exact arithmetic, 3.5e-7
+// evaluated in FP32).  exp(-z^2) is the factor the force needs anyway, so the
+// pair costs one exp2, one rcp and one rsq; taking it from the FP32 r^2 adds
+// the 2 z^2 2^-24 relative error of that rounding.
+constexpr float ERFC_P = 0.415f;
+constexpr float ERFC_C[8] = {0.0771595775739555f, -0.36372916731130467f, 0.5426081378824027f,
+                             -0.2276783459634979f, 0.3230029930806543f, 0.17506060444731647f,
+                             0.23978895752318216f, 0.23378723961927758f};
+constexpr float LOG2E_F = 1.4426950408889634f;
+
+// erfc(alpha r) / r and exp(-alpha^2 r^2) from r^2 and 1/r.
+__device__ __forceinline__ void ewald_erfc_f32(float alpha, float alpha2, float r2, float r, float r_1, float& e,
+                                               float& ex) {
+    ex = __builtin_amdgcn_exp2f(-alpha2 * r2 * LOG2E_F);
+    const float t = __builtin_amdgcn_rcpf(1.0f + ERFC_P * (alpha * r));
+    float p = ERFC_C[0];
+#pragma unroll
+    for (int k = 1; k < 8; ++k) p = p * t + ERFC_C[k];
+    e = t * p * ex * r_1;
+}
+
 // Accumulator layout inside the kernel (per lane, FP64).
 enum {
     A_VE = 0, A_VELEC, A_VDUDV, A_VD2UDV,   // vdW energy, shifted Coulomb, Lustig
@@ -88,6 +114,241 @@ enum {
     A_EVIR,                                  // 9
     A_N = A_EVIR + 9
 };
+
+// Per-owner data the pair term reads.
+struct Owner {
+    double px, py, pz;      // position
+    double ex, ey, ez;      // virtual-site position (BUCK, BUFF)
+    double cx, cy, cz;      // molecule centre (Lustig terms)
+    double qi;
+    int ti, mi;
+};
+
+// One entry of row i.  FULL: a pair of the reference's half list in its
+// owner's row -- energy, volume derivatives, virial and force on i.
+// Otherwise: the same pair seen from its other atom -- force on i only.
+template <int VDW, bool EWALD, bool LUSTIG, typename R, bool FULL>
+__device__ __forceinline__ void pair_term(const PairArgs& a, const Owner& o, int j, double (&acc)[A_N], double& fix,
+                                          double& fiy, double& fiz) {
+    const int mj = a.mol[j];
+    // rijm = massCenter_j - massCenter_i, minimum image (LJForce.cpp:112,116)
+    R mx = 0, my = 0, mz = 0;
+    if constexpr (FULL && LUSTIG && VDW != V_BUCK && VDW != V_BUFF) {
+        double dx = a.com[3 * mj] - o.cx, dy = a.com[3 * mj + 1] - o.cy, dz = a.com[3 * mj + 2] - o.cz;
+        pbc_near(dx, dy, dz, a.box);
+        mx = (R)dx; my = (R)dy; mz = (R)dz;
+    }
+    if constexpr (VDW == V_LJ || VDW == V_MIE || EWALD) {
+        double dx = a.pos[3 * j] - o.px, dy = a.pos[3 * j + 1] - o.py, dz = a.pos[3 * j + 2] - o.pz;
+        pbc_near(dx, dy, dz, a.box);
+        const R rx = (R)dx, ry = (R)dy, rz = (R)dz;
+        const R r2 = rx * rx + ry * ry + rz * rz;
+        R s = R(0);   // force on i = s * rij
+        constexpr bool F32 = sizeof(R) == sizeof(float);
+        R r, r_1;
+        if constexpr (F32) {
+            r_1 = __builtin_amdgcn_rsqf(r2);
+            r = r2 * r_1;
+        } else {
+            r = sqrt(r2);
+            r_1 = R(1) / r;
+        }
+
+        if constexpr (VDW == V_LJ || VDW == V_MIE) {
+            if (o.mi != mj) {
+                const R rij2 = F32 ? r2 : r * r;   // the reference tests |r|*|r| <= cut^2
+                if (rij2 <= (R)a.cut2) {
+                    const int t = o.ti * a.ntypes + a.type[j];
+                    const R rij_2 = r_1 * r_1;
+                    R tmpE, tmpf, dulj = 0, d2ulj = 0;
+                    const R rijRij = (LUSTIG && a.use_constraint) ? (rx * mx + ry * my + rz * mz) : rij2;
+                    if constexpr (VDW == V_LJ) {
+                        const R sigma2 = (R)a.ta[t], eps4 = (R)a.tb[t];
+                        const R sr2 = sigma2 * rij_2;
+                        const R sr6 = sr2 * sr2 * sr2;
+                        const R sr12 = sr6 * sr6;
+                        tmpE = eps4 * (sr12 - sr6);
+                        tmpf = (sr12 + sr12 - sr6) * R(6) * eps4 * rij_2;
+                        if constexpr (FULL && LUSTIG) {
+                            dulj = tmpf * rijRij;
+                            d2ulj = eps4 * (R(156) * sr12 - R(42) * sr6) * rijRij * rijRij * rij_2 * rij_2 + R(2) * dulj;
+                        }
+                        if (FULL && a.switch_on) {   // energy shift only (LJForce.cpp:152-168)
+                            const R sr2c = sigma2 * (R)a.rcut_2;
+                            const R sr6c = sr2c * sr2c * sr2c;
+                            tmpE = tmpE - eps4 * (sr6c * sr6c - sr6c);
+                        }
+                    } else {
+                        const R sigma = sqrt((R)a.ta[t]);
+                        const R eps = R(0.25) * (R)a.tb[t];
+                        const R sg = sigma * r_1;
+                        const R nm = (R)a.n_mie, mm = (R)a.m_mie, cn = (R)a.cn_mie;
+                        const R srn = pow_<R>(sg, nm), srm = pow_<R>(sg, mm);
+                        tmpE = cn * eps * (srn - srm);
+                        tmpf = cn * eps * (nm * srn - mm * srm) * rij_2;
+                        if constexpr (FULL && LUSTIG) {
+                            const R rr = rx * mx + ry * my + rz * mz;   // MieForce.cpp:140 (always rij.rijm)
+                            dulj = tmpf * rr;
+                            d2ulj = cn * eps * (nm * (nm + 1) * srn - mm * (mm + 1) * srm) * rr * rr * rij_2 * rij_2 + R(2) * dulj;
+                        }
+                        if (a.switch_on && rij2 > (R)a.sw_dist2) {
+                            double sv, dsv;
+                            charmm_switch(a, dx, dy, dz, sv, dsv);
+                            tmpf = (R)((double)tmpf * sv - (double)tmpE * dsv);
+                            tmpE = (R)((double)tmpE * sv);
+                        }
+                    }
+                    if constexpr (FULL) {
+                        acc[A_VE] += (double)tmpE;
+                        if constexpr (LUSTIG) { acc[A_VDUDV] += (double)dulj; acc[A_VD2UDV] += (double)d2ulj; }
+                    }
+                    if (a.coulomb) {   // GROMACS shift (LJForce.cpp:188-201)
+                        const R qij = (R)(o.qi * a.q[j]);
+                        const R e = qij * r_1;
+                        const R f = e * rij_2;
+                        const R r_rc = r / (R)a.cutoff;
+                        const R r4 = rij2 * rij2 * (R)a.rcut_2 * (R)a.rcut_2;
+                        if constexpr (FULL)
+                            acc[A_VELEC] += (double)(e * (R(1) - R(1.666667) * r_rc + R(1.666667) * r4 - r_rc * r4));
+                        tmpf += f * (R(1) - R(5) * r4 + R(4) * r_rc * r4);
+                    }
+                    s -= tmpf;
+                    if constexpr (FULL) {
+                        const double fx = (double)(-tmpf * rx), fy = (double)(-tmpf * ry), fz = (double)(-tmpf * rz);
+                        double* v = acc + A_VVIR;
+                        v[0] += fx * dx; v[1] += fy * dx; v[2] += fz * dx;
+                        v[3] += fx * dy; v[4] += fy * dy; v[5] += fz * dy;
+                        v[6] += fx * dz; v[7] += fy * dz; v[8] += fz * dz;
+                    }
+                }
+            }
+        }
+        if constexpr (EWALD) {
+            if (r2 < (R)a.realcut2) {
+                const R qij = (R)(o.qi * a.q[j]);
+                R e, ex, f;
+                if constexpr (F32) {
+                    ewald_erfc_f32((float)a.alpha, (float)a.alpha2, r2, r, r_1, e, ex);
+                    f = (e + (R)a.alpha2PI * ex) * (r_1 * r_1);
+                } else {
+                    e = erfc_<R>((R)a.alpha * r) / r;
+                    ex = exp_<R>(-(R)a.alpha2 * r2);
+                    f = (e + (R)a.alpha2PI * ex) / r2;
+                }
+                const R se = -qij * f;
+                if constexpr (FULL) {
+                    acc[A_EE] += (double)(qij * e);
+                    if constexpr (LUSTIG) {
+                        const R rr = a.use_constraint ? (rx * mx + ry * my + rz * mz) : r2;
+                        acc[A_ED1] += (double)(-qij * f * rr);
+                        acc[A_ED2] += (double)(qij * (R(2) * f + R(2) * (R)a.alpha2PI * (R)a.alpha2 * ex) * rr * rr / r2 +
+                                               R(2) * qij * f * rr);
+                    }
+                    const double fx = (double)(se * rx), fy = (double)(se * ry), fz = (double)(se * rz);
+                    double* v = acc + A_EVIR;
+                    v[0] += fx * dx; v[1] += fy * dx; v[2] += fz * dx;
+                    v[3] += fx * dy; v[4] += fy * dy; v[5] += fz * dy;
+                    v[6] += fx * dz; v[7] += fy * dz; v[8] += fz * dz;
+                }
+                s += se;
+            }
+        }
+        fix += (double)(s * rx); fiy += (double)(s * ry); fiz += (double)(s * rz);
+    } else if constexpr (VDW == V_BUCK || VDW == V_BUFF) {
+        // rij = ri - rj on the virtual sites (BuckinghamForce.cpp:156,
+        // BuffvdWForce.cpp:172)
+        double dx = o.ex - a.epos[3 * j], dy = o.ey - a.epos[3 * j + 1], dz = o.ez - a.epos[3 * j + 2];
+        pbc_near(dx, dy, dz, a.box);
+        const R rx = (R)dx, ry = (R)dy, rz = (R)dz;
+        const R r = sqrt(rx * rx + ry * ry + rz * rz);
+        const R rij2 = r * r;
+        if (rij2 <= (R)a.cut2) {
+            const int t = o.ti * a.ntypes + a.type[j];
+            R tmpE = 0, tmpf = 0, dulj = 0, d2ulj = 0;
+            bool live = true;
+            if constexpr (VDW == V_BUCK) {
+                const R Aa = (R)a.ta[t], Bb = (R)a.tb[t], Cc = (R)a.tc[t];
+                const R r_1 = R(1) / r;
+                const R rij_2 = r_1 * r_1;
+                const R rij6 = rij_2 * rij_2 * rij_2;
+                const R ex = exp_<R>(-Cc * r);
+                tmpE = Bb * ex - Aa * rij6;
+                tmpf = -Bb * Cc * r_1 * ex + R(6) * Aa * rij6 * rij_2;
+                if constexpr (FULL && LUSTIG) {
+                    dulj = -Bb * Cc * r * ex + R(6) * Aa * rij6;
+                    d2ulj = (R(2) * r + Cc * rij2) * Bb * Cc * ex - R(54) * Aa * rij6;
+                }
+                if (a.switch_on && rij2 > (R)a.sw_dist2) {
+                    double sv, dsv;
+                    charmm_switch(a, dx, dy, dz, sv, dsv);
+                    tmpf = (R)((double)tmpf * sv - (double)tmpE * dsv);
+                    tmpE = (R)((double)tmpE * sv);
+                }
+                if (a.coulomb) {
+                    const R qij = (R)(o.qi * a.q[j]);
+                    const R e = qij * r_1;
+                    const R f = e * rij_2;
+                    const R r_rc = r / (R)a.cutoff;
+                    const R r4 = rij2 * rij2 * (R)a.rcut_2 * (R)a.rcut_2;
+                    if constexpr (FULL)
+                        acc[A_VELEC] += (double)(e * (R(1) - R(1.666667) * r_rc + R(1.666667) * r4 - r_rc * r4));
+                    tmpf += f * (R(1) - R(5) * r4 + R(4) * r_rc * r4);
+                }
+            } else {
+                const R sigma = (R)a.ta[t], eps = (R)a.tb[t];
+                if (eps == R(0)) {
+                    live = false;   // tmpE = tmpf = 0 (BuffvdWForce.cpp:190-192)
+                } else {
+                    const R invs = R(1) / sigma;
+                    const R m = (R)a.m_buff, nn = (R)a.n_buff, del = (R)a.delta, gam = (R)a.gamma;
+                    const R rho = r * invs;
+                    const R rho_m = pow_<R>(rho, m);
+                    const R rho_m1 = pow_<R>(rho, m - R(1));
+                    const R expnm = nn - m;
+                    const R par1 = (R(1) + del) / (rho + del);
+                    const R par2 = (R(1) + gam) / (rho_m + gam);
+                    const R p1n = pow_<R>(par1, expnm);
+                    tmpE = eps * p1n * (par2 - R(2));
+                    tmpf = -eps * expnm * p1n * (par2 - R(2)) * invs / (rho + del) -
+                           eps * p1n * par2 * m * rho_m1 * invs / (rho_m + gam);
+                    if (a.switch_on && rij2 > (R)a.sw_dist2) {
+                        // The taper's coefficients scale as 1/(rc - rsw)^5:
+                        // evaluated in FP64 from the FP64 distance.
+                        const double rd = sqrt(dx * dx + dy * dy + dz * dz);
+                        const double r2d = rd * rd, r3 = r2d * rd, r4 = r2d * r2d, r5 = r2d * r3;
+                        const double sv = a.s5 * r5 + a.s4 * r4 + a.s3 * r3 + a.s2 * r2d + a.s1 * rd + a.s0;
+                        const double dsv = 5.0 * a.s5 * r4 + 4.0 * a.s4 * r3 + 3.0 * a.s3 * r2d + 2.0 * a.s2 * rd + a.s1;
+                        tmpf = (R)((double)tmpf * sv + (double)tmpE * dsv);
+                        tmpE = (R)((double)tmpE * sv);
+                    }
+                    dulj = tmpf * r;   // always accumulated (BuffvdWForce.cpp:242-244)
+                }
+            }
+            if (live) {
+                double fx, fy, fz;
+                if constexpr (VDW == V_BUCK) {
+                    fx = (double)(-tmpf * rx); fy = (double)(-tmpf * ry); fz = (double)(-tmpf * rz);
+                } else {
+                    const R r_1 = R(1) / r;
+                    fx = (double)(-tmpf * rx * r_1); fy = (double)(-tmpf * ry * r_1); fz = (double)(-tmpf * rz * r_1);
+                }
+                fix += fx; fiy += fy; fiz += fz;
+                if constexpr (FULL) {
+                    acc[A_VE] += (double)tmpE;
+                    acc[A_VDUDV] += (double)dulj;
+                    if constexpr (LUSTIG) acc[A_VD2UDV] += (double)d2ulj;
+                    // virial += -fij * rij for the buffered 14-7 (BuffvdWForce.cpp:276-284)
+                    const double sg = VDW == V_BUCK ? 1.0 : -1.0;
+                    const double vx = sg * fx, vy = sg * fy, vz = sg * fz;
+                    double* v = acc + A_VVIR;
+                    v[0] += vx * dx; v[1] += vy * dx; v[2] += vz * dx;
+                    v[3] += vx * dy; v[4] += vy * dy; v[5] += vz * dy;
+                    v[6] += vx * dz; v[7] += vy * dz; v[8] += vz * dz;
+                }
+            }
+        }
+    }
+}
 
 template <int VDW, bool EWALD, bool LUSTIG, typename R, int G>
 __global__ __launch_bounds__(256) void k_pairs(PairArgs a) {
@@ -100,225 +361,25 @@ __global__ __launch_bounds__(256) void k_pairs(PairArgs a) {
     double fix = 0.0, fiy = 0.0, fiz = 0.0;
 
     if (i < a.n) {
-        const int cnt = min(a.nb_cnt[i], a.stride);
         const int32_t* row = a.nb + (int64_t)i * a.stride;
-        const double pix = a.pos[3 * i], piy = a.pos[3 * i + 1], piz = a.pos[3 * i + 2];
-        double eix = pix, eiy = piy, eiz = piz;
+        const int own = min(a.nb_own[i], a.stride);
+        const int non = min(a.nb_cnt[i] - a.nb_own[i], a.stride - own);
+        Owner o;
+        o.px = a.pos[3 * i]; o.py = a.pos[3 * i + 1]; o.pz = a.pos[3 * i + 2];
+        o.ex = o.px; o.ey = o.py; o.ez = o.pz;
         if constexpr (VDW == V_BUCK || VDW == V_BUFF) {
-            eix = a.epos[3 * i]; eiy = a.epos[3 * i + 1]; eiz = a.epos[3 * i + 2];
+            o.ex = a.epos[3 * i]; o.ey = a.epos[3 * i + 1]; o.ez = a.epos[3 * i + 2];
         }
-        const double qi = a.q[i];
-        const int ti = a.type[i];
-        const int mi = a.mol[i];
-        double cix = 0, ciy = 0, ciz = 0;
-        if constexpr (LUSTIG) { cix = a.com[3 * mi]; ciy = a.com[3 * mi + 1]; ciz = a.com[3 * mi + 2]; }
-
-        for (int k = sub; k < cnt; k += G) {
-            const int32_t ent = row[k];
-            const int j = ent & INDEX_MASK;
-            // Each pair sits in both atoms' rows; energies, virials and volume
-            // derivatives are taken from the reference owner's row only.
-            const double w = ent < 0 ? 1.0 : 0.0;
-            const int mj = a.mol[j];
-            // rijm = massCenter_j - massCenter_i, minimum image (LJForce.cpp:112,116)
-            R mx = 0, my = 0, mz = 0;
-            if constexpr (LUSTIG && VDW != V_BUCK && VDW != V_BUFF) {
-                double dx = a.com[3 * mj] - cix, dy = a.com[3 * mj + 1] - ciy, dz = a.com[3 * mj + 2] - ciz;
-                pbc(dx, dy, dz, a.box);
-                mx = (R)dx; my = (R)dy; mz = (R)dz;
-            }
-            double gfx = 0.0, gfy = 0.0, gfz = 0.0;   // force on i from this pair (j gets minus)
-
-            if constexpr (VDW == V_LJ || VDW == V_MIE || EWALD) {
-                double dx = a.pos[3 * j] - pix, dy = a.pos[3 * j + 1] - piy, dz = a.pos[3 * j + 2] - piz;
-                pbc(dx, dy, dz, a.box);
-                const R rx = (R)dx, ry = (R)dy, rz = (R)dz;
-                const R r2 = rx * rx + ry * ry + rz * rz;
-
-                if constexpr (VDW == V_LJ || VDW == V_MIE) {
-                    if (mi != mj) {
-                        const R r = sqrt(r2);
-                        const R rij2 = r * r;
-                        if (rij2 <= (R)a.cut2) {
-                            const int t = ti * a.ntypes + a.type[j];
-                            const R r_1 = R(1) / r;
-                            const R rij_2 = r_1 * r_1;
-                            R tmpE, tmpf, dulj = 0, d2ulj = 0;
-                            const R rijRij = (LUSTIG && a.use_constraint) ? (rx * mx + ry * my + rz * mz) : rij2;
-                            if constexpr (VDW == V_LJ) {
-                                const R sigma2 = (R)a.ta[t], eps4 = (R)a.tb[t];
-                                const R sr2 = sigma2 * rij_2;
-                                const R sr6 = sr2 * sr2 * sr2;
-                                const R sr12 = sr6 * sr6;
-                                tmpE = eps4 * (sr12 - sr6);
-                                tmpf = (sr12 + sr12 - sr6) * R(6) * eps4 * rij_2;
-                                if constexpr (LUSTIG) {
-                                    dulj = tmpf * rijRij;
-                                    d2ulj = eps4 * (R(156) * sr12 - R(42) * sr6) * rijRij * rijRij * rij_2 * rij_2 + R(2) * dulj;
-                                }
-                                if (a.switch_on) {   // energy shift only (LJForce.cpp:152-168)
-                                    const R sr2c = sigma2 * (R)a.rcut_2;
-                                    const R sr6c = sr2c * sr2c * sr2c;
-                                    tmpE = tmpE - eps4 * (sr6c * sr6c - sr6c);
-                                }
-                            } else {
-                                const R sigma = sqrt((R)a.ta[t]);
-                                const R eps = R(0.25) * (R)a.tb[t];
-                                const R sg = sigma * r_1;
-                                const R nm = (R)a.n_mie, mm = (R)a.m_mie, cn = (R)a.cn_mie;
-                                const R srn = pow_<R>(sg, nm), srm = pow_<R>(sg, mm);
-                                tmpE = cn * eps * (srn - srm);
-                                tmpf = cn * eps * (nm * srn - mm * srm) * rij_2;
-                                if constexpr (LUSTIG) {
-                                    const R rr = rx * mx + ry * my + rz * mz;   // MieForce.cpp:140 (always rij.rijm)
-                                    dulj = tmpf * rr;
-                                    d2ulj = cn * eps * (nm * (nm + 1) * srn - mm * (mm + 1) * srm) * rr * rr * rij_2 * rij_2 + R(2) * dulj;
-                                }
-                                if (a.switch_on && rij2 > (R)a.sw_dist2) {
-                                    double sv, dsv;
-                                    charmm_switch(a, dx, dy, dz, sv, dsv);
-                                    tmpf = (R)((double)tmpf * sv - (double)tmpE * dsv);
-                                    tmpE = (R)((double)tmpE * sv);
-                                }
-                            }
-                            acc[A_VE] += w * (double)tmpE;
-                            if constexpr (LUSTIG) { acc[A_VDUDV] += w * (double)dulj; acc[A_VD2UDV] += w * (double)d2ulj; }
-                            if (a.coulomb) {   // GROMACS shift (LJForce.cpp:188-201)
-                                const R qij = (R)(qi * a.q[j]);
-                                const R e = qij * r_1;
-                                const R f = e * rij_2;
-                                const R r_rc = r / (R)a.cutoff;
-                                const R r4 = rij2 * rij2 * (R)a.rcut_2 * (R)a.rcut_2;
-                                acc[A_VELEC] += w * (double)(e * (R(1) - R(1.666667) * r_rc + R(1.666667) * r4 - r_rc * r4));
-                                tmpf += f * (R(1) - R(5) * r4 + R(4) * r_rc * r4);
-                            }
-                            const double fx = (double)(-tmpf * rx), fy = (double)(-tmpf * ry), fz = (double)(-tmpf * rz);
-                            gfx += fx; gfy += fy; gfz += fz;
-                            double* v = acc + A_VVIR;
-                            const double wx = w * fx, wy = w * fy, wz = w * fz;
-                            v[0] += wx * dx; v[1] += wy * dx; v[2] += wz * dx;
-                            v[3] += wx * dy; v[4] += wy * dy; v[5] += wz * dy;
-                            v[6] += wx * dz; v[7] += wy * dz; v[8] += wz * dz;
-                        }
-                    }
-                }
-                if constexpr (EWALD) {
-                    if (r2 < (R)a.realcut2) {
-                        const R qij = (R)(qi * a.q[j]);
-                        const R r = sqrt(r2);
-                        const R e = erfc_<R>((R)a.alpha * r) / r;
-                        const R ex = exp_<R>(-(R)a.alpha2 * r2);
-                        const R f = (e + (R)a.alpha2PI * ex) / r2;
-                        acc[A_EE] += w * (double)(qij * e);
-                        if constexpr (LUSTIG) {
-                            const R rr = a.use_constraint ? (rx * mx + ry * my + rz * mz) : r2;
-                            acc[A_ED1] += w * (double)(-qij * f * rr);
-                            acc[A_ED2] += w * (double)(qij * (R(2) * f + R(2) * (R)a.alpha2PI * (R)a.alpha2 * ex) * rr * rr / r2 +
-                                                       R(2) * qij * f * rr);
-                        }
-                        const R s = -qij * f;
-                        const double fx = (double)(s * rx), fy = (double)(s * ry), fz = (double)(s * rz);
-                        gfx += fx; gfy += fy; gfz += fz;
-                        double* v = acc + A_EVIR;
-                        const double wx = w * fx, wy = w * fy, wz = w * fz;
-                        v[0] += wx * dx; v[1] += wy * dx; v[2] += wz * dx;
-                        v[3] += wx * dy; v[4] += wy * dy; v[5] += wz * dy;
-                        v[6] += wx * dz; v[7] += wy * dz; v[8] += wz * dz;
-                    }
-                }
-                fix += gfx; fiy += gfy; fiz += gfz;
-            } else if constexpr (VDW == V_BUCK || VDW == V_BUFF) {
-                // rij = ri - rj on the virtual sites (BuckinghamForce.cpp:156,
-                // BuffvdWForce.cpp:172)
-                double dx = eix - a.epos[3 * j], dy = eiy - a.epos[3 * j + 1], dz = eiz - a.epos[3 * j + 2];
-                pbc(dx, dy, dz, a.box);
-                const R rx = (R)dx, ry = (R)dy, rz = (R)dz;
-                const R r = sqrt(rx * rx + ry * ry + rz * rz);
-                const R rij2 = r * r;
-                if (rij2 <= (R)a.cut2) {
-                    const int t = ti * a.ntypes + a.type[j];
-                    R tmpE = 0, tmpf = 0, dulj = 0, d2ulj = 0;
-                    bool live = true;
-                    if constexpr (VDW == V_BUCK) {
-                        const R Aa = (R)a.ta[t], Bb = (R)a.tb[t], Cc = (R)a.tc[t];
-                        const R r_1 = R(1) / r;
-                        const R rij_2 = r_1 * r_1;
-                        const R rij6 = rij_2 * rij_2 * rij_2;
-                        const R ex = exp_<R>(-Cc * r);
-                        tmpE = Bb * ex - Aa * rij6;
-                        tmpf = -Bb * Cc * r_1 * ex + R(6) * Aa * rij6 * rij_2;
-                        if constexpr (LUSTIG) {
-                            dulj = -Bb * Cc * r * ex + R(6) * Aa * rij6;
-                            d2ulj = (R(2) * r + Cc * rij2) * Bb * Cc * ex - R(54) * Aa * rij6;
-                        }
-                        if (a.switch_on && rij2 > (R)a.sw_dist2) {
-                            double sv, dsv;
-                            charmm_switch(a, dx, dy, dz, sv, dsv);
-                            tmpf = (R)((double)tmpf * sv - (double)tmpE * dsv);
-                            tmpE = (R)((double)tmpE * sv);
-                        }
-                        if (a.coulomb) {
-                            const R qij = (R)(qi * a.q[j]);
-                            const R e = qij * r_1;
-                            const R f = e * rij_2;
-                            const R r_rc = r / (R)a.cutoff;
-                            const R r4 = rij2 * rij2 * (R)a.rcut_2 * (R)a.rcut_2;
-                            acc[A_VELEC] += w * (double)(e * (R(1) - R(1.666667) * r_rc + R(1.666667) * r4 - r_rc * r4));
-                            tmpf += f * (R(1) - R(5) * r4 + R(4) * r_rc * r4);
-                        }
-                    } else {
-                        const R sigma = (R)a.ta[t], eps = (R)a.tb[t];
-                        if (eps == R(0)) {
-                            live = false;   // tmpE = tmpf = 0 (BuffvdWForce.cpp:190-192)
-                        } else {
-                            const R invs = R(1) / sigma;
-                            const R m = (R)a.m_buff, nn = (R)a.n_buff, del = (R)a.delta, gam = (R)a.gamma;
-                            const R rho = r * invs;
-                            const R rho_m = pow_<R>(rho, m);
-                            const R rho_m1 = pow_<R>(rho, m - R(1));
-                            const R expnm = nn - m;
-                            const R par1 = (R(1) + del) / (rho + del);
-                            const R par2 = (R(1) + gam) / (rho_m + gam);
-                            const R p1n = pow_<R>(par1, expnm);
-                            tmpE = eps * p1n * (par2 - R(2));
-                            tmpf = -eps * expnm * p1n * (par2 - R(2)) * invs / (rho + del) -
-                                   eps * p1n * par2 * m * rho_m1 * invs / (rho_m + gam);
-                            if (a.switch_on && rij2 > (R)a.sw_dist2) {
-                                // The taper's coefficients scale as 1/(rc - rsw)^5:
-                                // evaluated in FP64 from the FP64 distance.
-                                const double rd = sqrt(dx * dx + dy * dy + dz * dz);
-                                const double r2d = rd * rd, r3 = r2d * rd, r4 = r2d * r2d, r5 = r2d * r3;
-                                const double sv = a.s5 * r5 + a.s4 * r4 + a.s3 * r3 + a.s2 * r2d + a.s1 * rd + a.s0;
-                                const double dsv = 5.0 * a.s5 * r4 + 4.0 * a.s4 * r3 + 3.0 * a.s3 * r2d + 2.0 * a.s2 * rd + a.s1;
-                                tmpf = (R)((double)tmpf * sv + (double)tmpE * dsv);
-                                tmpE = (R)((double)tmpE * sv);
-                            }
-                            dulj = tmpf * r;   // always accumulated (BuffvdWForce.cpp:242-244)
-                        }
-                    }
-                    if (live) {
-                        acc[A_VE] += w * (double)tmpE;
-                        acc[A_VDUDV] += w * (double)dulj;
-                        if constexpr (LUSTIG) acc[A_VD2UDV] += w * (double)d2ulj;
-                        double fx, fy, fz, vx, vy, vz;
-                        if constexpr (VDW == V_BUCK) {
-                            fx = (double)(-tmpf * rx); fy = (double)(-tmpf * ry); fz = (double)(-tmpf * rz);
-                            vx = w * fx; vy = w * fy; vz = w * fz;
-                        } else {
-                            const R r_1 = R(1) / r;
-                            fx = (double)(-tmpf * rx * r_1); fy = (double)(-tmpf * ry * r_1); fz = (double)(-tmpf * rz * r_1);
-                            // virial += -fij * rij (BuffvdWForce.cpp:276-284)
-                            vx = -w * fx; vy = -w * fy; vz = -w * fz;
-                        }
-                        fix += fx; fiy += fy; fiz += fz;
-                        double* v = acc + A_VVIR;
-                        v[0] += vx * dx; v[1] += vy * dx; v[2] += vz * dx;
-                        v[3] += vx * dy; v[4] += vy * dy; v[5] += vz * dy;
-                        v[6] += vx * dz; v[7] += vy * dz; v[8] += vz * dz;
-                    }
-                }
-            }
-        }
+        o.qi = a.q[i];
+        o.ti = a.type[i];
+        o.mi = a.mol[i];
+        o.cx = o.cy = o.cz = 0.0;
+        if constexpr (LUSTIG) { o.cx = a.com[3 * o.mi]; o.cy = a.com[3 * o.mi + 1]; o.cz = a.com[3 * o.mi + 2]; }
+        // the reference's row i: everything
+        for (int k = sub; k < own; k += G) pair_term<VDW, EWALD, LUSTIG, R, true>(a, o, row[k], acc, fix, fiy, fiz);
+        // partners whose reference row holds the pair: force on i only
+        const int32_t* back = row + a.stride - 1;
+        for (int k = sub; k < non; k += G) pair_term<VDW, EWALD, LUSTIG, R, false>(a, o, back[-k], acc, fix, fiy, fiz);
     }
 
     // Owner force: combine the G lanes.  Row i holds every partner of i, so
@@ -356,20 +417,31 @@ __global__ __launch_bounds__(256) void k_pairs(PairArgs a) {
 
 // ---------------------------------------------------------------------------
 
-constexpr int GLANES = 8;
+template <int VDW, bool EWALD, int G>
+static void launch_pairs_g(Ctx* c, const PairArgs& a, int prec, int lustig) {
+    const int threads = 256;
+    const int64_t total = (int64_t)c->n * G;
+    const dim3 grid((unsigned)((total + threads - 1) / threads));
+    if (prec == SMD_PREC_MIXED) {
+        if (lustig) hipLaunchKernelGGL((k_pairs<VDW, EWALD, true, float, G>), grid, dim3(threads), 0, c->stream, a);
+        else hipLaunchKernelGGL((k_pairs<VDW, EWALD, false, float, G>), grid, dim3(threads), 0, c->stream, a);
+    } else {
+        if (lustig) hipLaunchKernelGGL((k_pairs<VDW, EWALD, true, double, G>), grid, dim3(threads), 0, c->stream, a);
+        else hipLaunchKernelGGL((k_pairs<VDW, EWALD, false, double, G>), grid, dim3(threads), 0, c->stream, a);
+    }
+}
 
 template <int VDW, bool EWALD>
 static hipError_t launch_pairs(Ctx* c, const PairArgs& a, int prec, int lustig) {
-    const int threads = 256;
-    const int64_t total = (int64_t)c->n * GLANES;
-    const dim3 grid((unsigned)((total + threads - 1) / threads));
     ProfScope ps(c, "pairs");
-    if (prec == SMD_PREC_MIXED) {
-        if (lustig) hipLaunchKernelGGL((k_pairs<VDW, EWALD, true, float, GLANES>), grid, dim3(threads), 0, c->stream, a);
-        else hipLaunchKernelGGL((k_pairs<VDW, EWALD, false, float, GLANES>), grid, dim3(threads), 0, c->stream, a);
-    } else {
-        if (lustig) hipLaunchKernelGGL((k_pairs<VDW, EWALD, true, double, GLANES>), grid, dim3(threads), 0, c->stream, a);
-        else hipLaunchKernelGGL((k_pairs<VDW, EWALD, false, double, GLANES>), grid, dim3(threads), 0, c->stream, a);
+    // measured best on the 12,288-atom water list: 16 lanes for FP32 pair
+    // math, 32 for FP64
+    const int lanes = c->pair_lanes ? c->pair_lanes : (prec == SMD_PREC_MIXED ? 16 : 32);
+    switch (lanes) {
+        case 4: launch_pairs_g<VDW, EWALD, 4>(c, a, prec, lustig); break;
+        case 8: launch_pairs_g<VDW, EWALD, 8>(c, a, prec, lustig); break;
+        case 32: launch_pairs_g<VDW, EWALD, 32>(c, a, prec, lustig); break;
+        default: launch_pairs_g<VDW, EWALD, 16>(c, a, prec, lustig); break;
     }
     return hipGetLastError();
 }
@@ -386,6 +458,7 @@ static void base_args(Ctx* c, PairArgs& a) {
     a.com = c->d_com;
     a.nb = c->d_nb;
     a.nb_cnt = c->d_nb_cnt;
+    a.nb_own = c->d_nb_own;
     a.stride = c->stride;
     a.force = c->d_force;
     a.fvirt = c->d_fvirt;

@@ -8,10 +8,15 @@
 // and the same row order (in-cell successors, then stencil cells in order,
 // atoms in index order).  Rows are stored ELL-style: row i at nb[i*stride].
 //
-// Work mapping: one 64-lane wavefront per owner atom.  Lanes test 64
-// candidates at a time; hits are compacted with a ballot so the row keeps the
-// reference's order without atomics.  The candidate scan is HBM/L2 bound
-// (positions are re-read from L2, 24 B per candidate).
+// Rows are written owned-first: the reference's row i fills row i from the
+// front, in its order; the same pairs seen from their other atom fill it from
+// the back, so every atom's row holds all its partners and the pair kernels
+// need no atomics.  Cell grid: one pass per step assigns atoms to cells, scans
+// the counts and compacts each cell's atoms in index order; the pairs are then
+// found per (cell, neighbour cell) block from LDS-staged positions (see
+// k_list_cells).  Without cells, one wavefront per atom tests all atoms.
+#include <algorithm>
+
 #include "smd_internal.h"
 
 namespace smd {
@@ -39,30 +44,42 @@ __device__ __forceinline__ int cell_coord(double v, int n) {
     return x >= n ? n - 1 : x;
 }
 
-__global__ void k_zero_i32(int32_t* p, int n, const int* gate) {
+// Atom -> cell (CellManager::find_cell, src/utils/CellManager.h:97-119) and the
+// per-cell counts, aggregated in LDS when the grid fits there.
+template <bool LDS_HIST>
+__global__ __launch_bounds__(1024) void k_cell_assign(int n, const double* __restrict__ pos, CellGrid g,
+                                                      int32_t* __restrict__ cell_of, int32_t* __restrict__ cnt,
+                                                      const int* gate) {
     if (gate && !*gate) return;
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = 0;
-}
-
-__global__ void k_cell_assign(int n, const double* __restrict__ pos, CellGrid g, int32_t* __restrict__ cell_of,
-                              int32_t* __restrict__ cnt, const int* gate) {
-    if (gate && !*gate) return;
+    extern __shared__ int32_t hist[];
+    if constexpr (LDS_HIST) {
+        for (int k = threadIdx.x; k < g.ncell; k += blockDim.x) hist[k] = 0;
+        __syncthreads();
+    }
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const double dx = pos[3 * i] - g.origin[0];
-    const double dy = pos[3 * i + 1] - g.origin[1];
-    const double dz = pos[3 * i + 2] - g.origin[2];
-    const int x = cell_coord(dx * g.rlen[0], g.n[0]);
-    const int y = cell_coord(dy * g.rlen[1], g.n[1]);
-    const int z = cell_coord(dz * g.rlen[2], g.n[2]);
-    const int c = z * g.nxy + y * g.n[0] + x;
-    cell_of[i] = c;
-    atomicAdd(&cnt[c], 1);
+    if (i < n) {
+        const double dx = pos[3 * i] - g.origin[0];
+        const double dy = pos[3 * i + 1] - g.origin[1];
+        const double dz = pos[3 * i + 2] - g.origin[2];
+        const int x = cell_coord(dx * g.rlen[0], g.n[0]);
+        const int y = cell_coord(dy * g.rlen[1], g.n[1]);
+        const int z = cell_coord(dz * g.rlen[2], g.n[2]);
+        const int c = z * g.nxy + y * g.n[0] + x;
+        cell_of[i] = c;
+        if constexpr (LDS_HIST) atomicAdd(&hist[c], 1);
+        else atomicAdd(&cnt[c], 1);
+    }
+    if constexpr (LDS_HIST) {
+        __syncthreads();
+        for (int k = threadIdx.x; k < g.ncell; k += blockDim.x)
+            if (hist[k]) atomicAdd(&cnt[k], hist[k]);
+    }
 }
 
-// Exclusive scan of the cell counts by one block.
-__global__ void k_cell_scan(int ncell, const int32_t* __restrict__ cnt, int32_t* __restrict__ start,
-                            const int* gate) {
+// Exclusive scan of the cell counts by one block; the counts are cleared for
+// the next build.
+__global__ __launch_bounds__(1024) void k_cell_scan(int ncell, int32_t* __restrict__ cnt, int32_t* __restrict__ start,
+                                                    const int* gate) {
     if (gate && !*gate) return;
     __shared__ int32_t part[1024];
     const int t = threadIdx.x, T = blockDim.x;
@@ -79,24 +96,41 @@ __global__ void k_cell_scan(int ncell, const int32_t* __restrict__ cnt, int32_t*
     }
     __syncthreads();
     int32_t run = part[t];
-    for (int k = lo; k < hi; ++k) { start[k] = run; run += cnt[k]; }
+    for (int k = lo; k < hi; ++k) {
+        start[k] = run;
+        run += cnt[k];
+        cnt[k] = 0;
+    }
 }
 
-// One wavefront per cell compacts its atoms in index order
-// (set_atom appends in atom order, CellManager.cpp:145-170).
-__global__ void k_cell_fill(int n, int ncell, const int32_t* __restrict__ cell_of, const int32_t* __restrict__ start,
-                            int32_t* __restrict__ atoms, const int* gate) {
+// One block per cell compacts its atoms in index order (set_atom appends in
+// atom order, CellManager.cpp:145-170); it stops once the cell is full.
+__global__ __launch_bounds__(256) void k_cell_fill(int n, int ncell, const int32_t* __restrict__ cell_of,
+                                                   const int32_t* __restrict__ start, int32_t* __restrict__ atoms,
+                                                   const int* gate) {
     if (gate && !*gate) return;
+    __shared__ int32_t wsum[4];
     const int c = blockIdx.x;
     if (c >= ncell) return;
-    const int lane = threadIdx.x;
+    const int lane = threadIdx.x % WAVE, w = threadIdx.x / WAVE;
+    const uint64_t below = (1ull << lane) - 1ull;
     int base = start[c];
-    for (int a0 = 0; a0 < n; a0 += WAVE) {
-        const int a = a0 + lane;
+    const int end = start[c + 1];
+    for (int a0 = 0; a0 < n && base < end; a0 += 256) {
+        const int a = a0 + threadIdx.x;
         const bool hit = (a < n) && (cell_of[a] == c);
         const uint64_t m = __ballot(hit);
-        if (hit) atoms[base + __popcll(m & ((1ull << lane) - 1ull))] = a;
-        base += __popcll(m);
+        if (lane == 0) wsum[w] = __popcll(m);
+        __syncthreads();
+        int off = 0, tot = 0;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            off += k < w ? wsum[k] : 0;
+            tot += wsum[k];
+        }
+        if (hit && base + off + __popcll(m & below) < end) atoms[base + off + __popcll(m & below)] = a;
+        base += tot;
+        __syncthreads();
     }
 }
 
@@ -130,8 +164,10 @@ __device__ __forceinline__ bool excluded(const Excl& e, int j) {
     return false;
 }
 
-// Test 64 candidates (one per lane) and append the hits to row i in lane order.
-// Entries the reference's half list holds in row i are tagged with OWNED.
+// Test 64 candidates (one per lane) and append the hits to row i in lane
+// order.  Pairs the reference's half list holds in row i ("owned") fill the
+// row from the front, in the reference's order; the others fill it from the
+// back.
 __device__ __forceinline__ void test_and_append(const ListArgs& a, int i, double px, double py, double pz,
                                                 const Excl& ex, bool valid, int j, bool owned, bool use_pbc,
                                                 int& count, int& n_owned) {
@@ -144,13 +180,17 @@ __device__ __forceinline__ void test_and_append(const ListArgs& a, int i, double
         if (use_pbc) pbc(dx, dy, dz, a.box);
         hit = len2_exact(dx, dy, dz) < a.rlist2;
     }
-    const uint64_t m = __ballot(hit);
+    const uint64_t below = (1ull << lane) - 1ull;
+    const uint64_t mo = __ballot(hit && owned), mn = __ballot(hit && !owned);
+    const int n_non = count - n_owned;
     if (hit) {
-        const int slot = count + __popcll(m & ((1ull << lane) - 1ull));
-        if (slot < a.stride) a.nb[(int64_t)i * a.stride + slot] = owned ? (int32_t)((uint32_t)j | OWNED) : j;
+        // the two segments meet only on overflow, which the count reports
+        const int slot = owned ? n_owned + __popcll(mo & below) : a.stride - 1 - (n_non + __popcll(mn & below));
+        if (slot >= 0 && slot < a.stride && (owned ? slot < a.stride - n_non : slot >= n_owned))
+            a.nb[(int64_t)i * a.stride + slot] = j;
     }
-    count += __popcll(m);
-    n_owned += __popcll(__ballot(hit && owned));
+    count += __popcll(mo) + __popcll(mn);
+    n_owned += __popcll(mo);
 }
 
 __device__ __forceinline__ Excl load_excl(const ListArgs& a, int i) {
@@ -163,50 +203,115 @@ __device__ __forceinline__ Excl load_excl(const ListArgs& a, int i) {
     return e;
 }
 
-__global__ __launch_bounds__(256) void k_pairs_cells(ListArgs a, CellGrid g, const int32_t* __restrict__ cell_of,
-                                                     const int32_t* __restrict__ start,
-                                                     const int32_t* __restrict__ catoms, const int* gate) {
-    if (gate && !*gate) return;
-    const int i = blockIdx.x * (blockDim.x / WAVE) + threadIdx.x / WAVE;
-    if (i >= a.n) return;
-    const int lane = threadIdx.x % WAVE;
-    const double px = a.pos[3 * i], py = a.pos[3 * i + 1], pz = a.pos[3 * i + 2];
-    const Excl ex = load_excl(a, i);
-    const int c = cell_of[i];
-    int count = 0, n_owned = 0;
-    // Same cell: every other member, no PBC (CellManager.cpp:257-279); the
-    // reference's row holds the members after i.
-    {
-        const int s = start[c], e = start[c + 1];
-        for (int k0 = s; k0 < e; k0 += WAVE) {
-            const int k = k0 + lane;
-            const int j = (k < e) ? catoms[k] : -1;
-            test_and_append(a, i, px, py, pz, ex, (k < e) && (j != i), j, j > i, false, count, n_owned);
-        }
-    }
-    // The 13 forward cells (owned, reference order), then the 13 backward
-    // ones.  With at least 3 cells per edge the 26 neighbours are distinct, so
-    // every pair of the reference's list appears once in each of its rows.
+// ---- cell-pair list build ----------------------------------------------------
+//
+// Block (c, s) pairs the atoms of cell c with the atoms of its s-th
+// neighbour: s = 0 the cell itself (no PBC, CellManager.cpp:257-279), s =
+// 1..13 the forward stencil (the reference's row of the atom in c), s =
+// 14..26 the backward stencil (the same pairs seen from the other atom).  The
+// neighbour's positions are staged in LDS and each thread owns one atom of c.
+// A counting pass stores, per atom, the hits of each segment (self j>i, self
+// j<i, forward 1..13, backward 1..13); the writing pass repeats the test and
+// writes each hit at its final slot, so row i's front segment is the
+// reference's row in its order.
+
+constexpr int LB_THREADS = 256;   // atoms of cell c per pass of a block
+constexpr int LB_STAGE = 256;     // neighbour atoms staged in LDS at a time
+constexpr int NSEG = 28;          // per-atom hit counters
+
+__device__ __forceinline__ int neighbour_cell(const CellGrid& g, int c, int s) {
+    if (s == 0) return c;
+    const int sgn = s <= 13 ? 1 : -1;
+    const int k = s <= 13 ? s - 1 : s - 14;
     const int cz = c / g.nxy, cy = (c % g.nxy) / g.n[0], cx = c % g.n[0];
-    for (int s26 = 0; s26 < 26; ++s26) {
-        const int sgn = s26 < 13 ? 1 : -1;
-        const int s13 = s26 < 13 ? s26 : s26 - 13;
-        int xx = cx + sgn * c_stencil[s13][0], yy = cy + sgn * c_stencil[s13][1], zz = cz + sgn * c_stencil[s13][2];
-        if (xx >= g.n[0]) xx = 0; else if (xx < 0) xx = g.n[0] - 1;
-        if (yy >= g.n[1]) yy = 0; else if (yy < 0) yy = g.n[1] - 1;
-        if (zz >= g.n[2]) zz = 0; else if (zz < 0) zz = g.n[2] - 1;
-        const int nc = zz * g.nxy + yy * g.n[0] + xx;
-        const int s = start[nc], e = start[nc + 1];
-        for (int k0 = s; k0 < e; k0 += WAVE) {
-            const int k = k0 + lane;
-            const int j = (k < e) ? catoms[k] : -1;
-            test_and_append(a, i, px, py, pz, ex, k < e, j, s26 < 13, true, count, n_owned);
+    int xx = cx + sgn * c_stencil[k][0], yy = cy + sgn * c_stencil[k][1], zz = cz + sgn * c_stencil[k][2];
+    if (xx >= g.n[0]) xx = 0; else if (xx < 0) xx = g.n[0] - 1;
+    if (yy >= g.n[1]) yy = 0; else if (yy < 0) yy = g.n[1] - 1;
+    if (zz >= g.n[2]) zz = 0; else if (zz < 0) zz = g.n[2] - 1;
+    return zz * g.nxy + yy * g.n[0] + xx;
+}
+
+template <bool WRITE>
+__global__ __launch_bounds__(LB_THREADS) void k_list_cells(ListArgs a, CellGrid g, const int32_t* __restrict__ start,
+                                                           const int32_t* __restrict__ catoms, int32_t* __restrict__ seg,
+                                                           const int* gate) {
+    if (gate && !*gate) return;
+    __shared__ double4 sj[LB_STAGE];   // x, y, z, atom index
+    const int c = blockIdx.x, s = blockIdx.y;
+    const int nc = neighbour_cell(g, c, s);
+    const int ib = start[c], ie = start[c + 1];
+    const int jb = start[nc], je = start[nc + 1];
+    const int n = a.n;
+    for (int i0 = ib; i0 < ie; i0 += LB_THREADS) {
+        const int k = i0 + threadIdx.x;
+        const bool act = k < ie;
+        const int i = act ? catoms[k] : 0;
+        double px = 0, py = 0, pz = 0;
+        Excl ex{};
+        int own_at = 0, non_at = 0;   // counts (first pass) or next slots (second pass)
+        bool fits = true;
+        if (act) {
+            px = a.pos[3 * i]; py = a.pos[3 * i + 1]; pz = a.pos[3 * i + 2];
+            ex = load_excl(a, i);
+            if constexpr (WRITE) {
+                int fwd = 0, bwd = 0, own_before = 0, non_before = 0;
+                for (int t = 1; t <= 13; ++t) {
+                    const int v = seg[(t + 1) * n + i];
+                    if (t < s) own_before += v;
+                    fwd += v;
+                }
+                for (int t = 14; t <= 26; ++t) {
+                    const int v = seg[(t + 1) * n + i];
+                    if (t < s) non_before += v;
+                    bwd += v;
+                }
+                const int so = seg[i], sn = seg[n + i];
+                const int n_own = so + fwd, total = n_own + sn + bwd;
+                fits = total <= a.stride;
+                own_at = s == 0 ? 0 : so + own_before;
+                non_at = s == 0 ? 0 : sn + non_before;
+                if (s == 0) {
+                    a.nb_cnt[i] = total;
+                    a.nb_own[i] = n_own;
+                    if (!fits) atomicOr(a.status, ST_LIST_OVERFLOW);
+                }
+            }
         }
-    }
-    if (lane == 0) {
-        a.nb_cnt[i] = count;
-        a.nb_own[i] = n_owned;
-        if (count > a.stride) atomicOr(a.status, ST_LIST_OVERFLOW);
+        int32_t* row = a.nb + (int64_t)i * a.stride;
+        for (int j0 = jb; j0 < je; j0 += LB_STAGE) {
+            const int nj = min(LB_STAGE, je - j0);
+            __syncthreads();
+            for (int t = threadIdx.x; t < nj; t += blockDim.x) {
+                const int j = catoms[j0 + t];
+                sj[t] = make_double4(a.pos[3 * j], a.pos[3 * j + 1], a.pos[3 * j + 2], (double)j);
+            }
+            __syncthreads();
+            if (act) {
+                for (int t = 0; t < nj; ++t) {
+                    const double4 q = sj[t];
+                    const int j = (int)q.w;
+                    double dx = q.x - px, dy = q.y - py, dz = q.z - pz;
+                    if (s != 0) pbc(dx, dy, dz, a.box);
+                    if (len2_exact(dx, dy, dz) < a.rlist2 && (s != 0 || j != i) && !excluded(ex, j)) {
+                        const bool owned = s == 0 ? (j > i) : (s <= 13);
+                        if constexpr (WRITE) {
+                            if (fits) {
+                                if (owned) row[own_at] = j;
+                                else row[a.stride - 1 - non_at] = j;
+                            }
+                        }
+                        own_at += owned ? 1 : 0;
+                        non_at += owned ? 0 : 1;
+                    }
+                }
+            }
+        }
+        if constexpr (!WRITE) {
+            if (act) {
+                if (s == 0) { seg[i] = own_at; seg[n + i] = non_at; }
+                else seg[(s + 1) * n + i] = s <= 13 ? own_at : non_at;
+            }
+        }
     }
 }
 
@@ -285,14 +390,21 @@ static int launch_list(Ctx* c, const int* gate) {
         g.nxy = g.n[0] * g.n[1];
         g.ncell = g.nxy * g.n[2];
         const int nc = g.ncell;
-        hipLaunchKernelGGL(k_zero_i32, dim3((nc + 256) / 256), dim3(256), 0, c->stream, c->d_cell_cnt, nc + 1, gate);
-        hipLaunchKernelGGL(k_cell_assign, dim3((c->n + 255) / 256), dim3(256), 0, c->stream, c->n, c->d_pos, g,
-                           c->d_cell_of, c->d_cell_cnt, gate);
+        const dim3 ga((c->n + 1023) / 1024);
+        if (nc <= 8192)
+            hipLaunchKernelGGL((k_cell_assign<true>), ga, dim3(1024), sizeof(int32_t) * nc, c->stream, c->n, c->d_pos, g,
+                               c->d_cell_of, c->d_cell_cnt, gate);
+        else
+            hipLaunchKernelGGL((k_cell_assign<false>), ga, dim3(1024), 0, c->stream, c->n, c->d_pos, g, c->d_cell_of,
+                               c->d_cell_cnt, gate);
         hipLaunchKernelGGL(k_cell_scan, dim3(1), dim3(1024), 0, c->stream, nc, c->d_cell_cnt, c->d_cell_start, gate);
-        hipLaunchKernelGGL(k_cell_fill, dim3(nc), dim3(WAVE), 0, c->stream, c->n, nc, c->d_cell_of, c->d_cell_start,
+        hipLaunchKernelGGL(k_cell_fill, dim3(nc), dim3(256), 0, c->stream, c->n, nc, c->d_cell_of, c->d_cell_start,
                            c->d_cell_atoms, gate);
-        hipLaunchKernelGGL(k_pairs_cells, grid, dim3(WAVE * waves_per_block), 0, c->stream, a, g, c->d_cell_of,
-                           c->d_cell_start, c->d_cell_atoms, gate);
+        const dim3 gl(nc, 27);
+        hipLaunchKernelGGL((k_list_cells<false>), gl, dim3(LB_THREADS), 0, c->stream, a, g, c->d_cell_start,
+                           c->d_cell_atoms, c->d_list_seg, gate);
+        hipLaunchKernelGGL((k_list_cells<true>), gl, dim3(LB_THREADS), 0, c->stream, a, g, c->d_cell_start,
+                           c->d_cell_atoms, c->d_list_seg, gate);
     } else {
         hipLaunchKernelGGL(k_pairs_brute, grid, dim3(WAVE * waves_per_block), 0, c->stream, a, gate);
     }
@@ -320,6 +432,8 @@ int pairlist_build(Ctx* c, double cutoff, double cut_buff, int use_cells, bool o
             if (c->d_cell_start) (void)hipFree(c->d_cell_start);
             SMD_HIP(c, hipMalloc(&c->d_cell_cnt, sizeof(int32_t) * (nc + 1)));
             SMD_HIP(c, hipMalloc(&c->d_cell_start, sizeof(int32_t) * (nc + 1)));
+            // k_cell_scan clears the counts after each use; they start at zero
+            SMD_HIP(c, hipMemset(c->d_cell_cnt, 0, sizeof(int32_t) * (nc + 1)));
             c->cell_alloc = nc + 1;
         }
     }
@@ -392,7 +506,7 @@ extern "C" int smd_pairlist_info(smd_ctx* c, int64_t* n_pairs, int32_t* max_row,
     return SMD_OK;
 }
 
-// The reference's half list is the OWNED subsequence of each device row.
+// The reference's half list is the front segment of each device row.
 extern "C" int smd_get_pairlist(smd_ctx* c, int32_t* offsets, int32_t* nb, int64_t cap) {
     if (!c || !offsets) return SMD_EINVAL;
     if (!c->list_valid) return fail(c, SMD_ESTATE, "no pair list");
@@ -410,11 +524,8 @@ extern "C" int smd_get_pairlist(smd_ctx* c, int32_t* offsets, int32_t* nb, int64
     std::vector<int32_t> rows((size_t)c->n * c->stride);
     SMD_HIP(c, hipMemcpy(rows.data(), c->d_nb, rows.size() * sizeof(int32_t), hipMemcpyDeviceToHost));
     for (int i = 0; i < c->n; ++i) {
-        int o = offsets[i];
-        for (int k = 0; k < cnt[i]; ++k) {
-            const int32_t e = rows[(size_t)i * c->stride + k];
-            if (e < 0) nb[o++] = e & INDEX_MASK;
-        }
+        const int32_t* r = rows.data() + (size_t)i * c->stride;
+        std::copy(r, r + own[i], nb + offsets[i]);
     }
     return SMD_OK;
 }

@@ -54,7 +54,8 @@ struct Prof {
 
 struct Box {
     double L[3];
-    double h[3];   // L/2, as Ensemble::compute_bound_box stores halfL
+    double h[3];     // L/2, as Ensemble::compute_bound_box stores halfL
+    double inv[3];   // 1/L
 };
 
 struct Ctx {
@@ -96,6 +97,7 @@ struct Ctx {
     int32_t* d_cell_cnt = nullptr;    // [ncell+1]
     int32_t* d_cell_start = nullptr;  // [ncell+1]
     int32_t* d_cell_atoms = nullptr;  // [n]
+    int32_t* d_list_seg = nullptr;    // [28][n] per-atom hit counts of the cell-pair segments
     int32_t* d_nb = nullptr;          // [n * stride]
     int32_t* d_nb_cnt = nullptr;      // [n] full row length
     int32_t* d_nb_own = nullptr;      // [n] entries tagged OWNED (the reference row)
@@ -118,13 +120,16 @@ struct Ctx {
     double* d_kmod2 = nullptr;    // [K]
     double* d_expk = nullptr;     // [K]
     int32_t* d_kn = nullptr;      // [3K] integer (nx, ny, nz)
-    int32_t* d_kgrp = nullptr;    // [2*n_kgrp] (first k, count) of each k-group
-    int32_t* d_ksplit = nullptr;  // group ranges of the reciprocal force pass
-    double* d_fpart = nullptr;    // per-range partial reciprocal forces
-    int n_kgrp = 0, kmax_range = 0;
-    double* d_sfac_part = nullptr;  // [nchunk][K][8]
-    double* d_sfac = nullptr;       // [K][8]
+    int4* d_kgrp = nullptr;       // [n_kgrp] runs of consecutive nz in one (nx, ny) row
+    int32_t* d_ksplit = nullptr;  // [RF_WAVES+1] group ranges of the reciprocal force pass
+    int n_kgrp = 0;
+    double* d_sfac_part = nullptr;  // [chunks][K][10] per-chunk structure-factor sums
+    double* d_ssum = nullptr;       // [K][10] structure-factor sums
+    double* d_kf = nullptr;         // [K] (b C, b S) in the force pass's precision
+    double* d_e1 = nullptr;         // [n][3] complex exp(i 2 pi r_p / L_p)
     int sfac_chunks = 0;
+    size_t sfac_lds[2] = {0, 0};    // dynamic LDS of the structure-factor pass (FP64, FP32)
+    size_t force_lds[2] = {0, 0};   // dynamic LDS of the force pass (FP64, FP32)
 
     // vdW tables (device)
     double* d_vdw_a = nullptr;    // [ntypes^2] per kind meaning
@@ -145,6 +150,7 @@ struct Ctx {
     hipGraph_t graph = nullptr;
 
     Prof prof;
+    int pair_lanes = 0;           // lanes per atom row in the pair kernels (0: by precision)
 };
 
 }  // namespace smd
@@ -194,6 +200,17 @@ __device__ __forceinline__ void pbc(double& x, double& y, double& z, const Box& 
     for (int it = 0; it < 8 && y < -b.h[1]; ++it) y += b.L[1];
     for (int it = 0; it < 8 && z >= b.h[2]; ++it) z -= b.L[2];
     for (int it = 0; it < 8 && z < -b.h[2]; ++it) z += b.L[2];
+}
+
+// Minimum image for a displacement known to lie within 1.5 L of the image
+// it wraps to: one shift by the nearest multiple of L.  Equals pbc() except
+// for |x| = L/2 exactly (where pbc() keeps -L/2); the pair kernels use it
+// only on pairs of a valid list, whose separations inside the cutoff are far
+// below L/2.
+__device__ __forceinline__ void pbc_near(double& x, double& y, double& z, const Box& b) {
+    x -= b.L[0] * rint(x * b.inv[0]);
+    y -= b.L[1] * rint(y * b.inv[1]);
+    z -= b.L[2] * rint(z * b.inv[2]);
 }
 
 // Vector3::length2 (src/utils/Vector3.h:190-192) with each product and sum

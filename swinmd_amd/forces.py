@@ -147,6 +147,10 @@ class Context:
     def sync(self):
         self.check(self.lib.smd_sync(self.h))
 
+    def set_tuning(self, name, value):
+        """Launch-shape knob (e.g. "pair_lanes"); see smd_set_tuning."""
+        self.check(self.lib.smd_set_tuning(self.h, name.encode(), int(value)))
+
 
 class _VdwForce:
     KIND = L.VDW_LJ

@@ -1,0 +1,257 @@
+"""The C oracle against what the mathematics fixes, independently of the
+reference's own outputs: potential minima, the NaCl Madelung constant through
+the full Ewald sum, a brute-force full-sphere k-space sum, forces as minus the
+energy gradient, and the Nose-Hoover chain against a direct integration of
+its equations of motion.  A dropped term, a wrong sign or factor of two, or a
+transposed index in the oracle fails one of these."""
+import ctypes as C
+import math
+
+import numpy as np
+import pytest
+
+from swinmd_amd.systems import AtomType, System
+
+COULOMB = 138.93548501          # src/NEMD_defs.h:104, kJ/mol nm / e^2
+BOLTZ = 0.0083145112            # src/NEMD_defs.h:108
+
+
+def _atoms(types, pos, type_ids, box, mol=None):
+    pos = np.asarray(pos, dtype=np.float64).reshape(-1, 3)
+    n = len(pos)
+    mol = np.arange(n, dtype=np.int32) if mol is None else np.asarray(mol, dtype=np.int32)
+    apm = int(np.bincount(mol).max())
+    return System(name="t", types=types, mol_type_name="X", atoms_per_mol=apm, pos=pos,
+                  type=np.asarray(type_ids, dtype=np.int32), mol=mol, density=n / box ** 3)
+
+
+def _oracle(s, pos=None):
+    from oracle import oracle as O
+    return O.Oracle(s, pos=pos)
+
+
+# ---- 12-6 LJ -------------------------------------------------------------------
+
+def test_lj_pair_minimum_zero_crossing_and_force_direction(oracle_mod):
+    sigma, eps = 0.3166, 0.6502
+    types = [AtomType(mass=15.9994, charge=0.0, eps=eps, sigma=sigma)]
+    for r, e_want, f_want in ((2 ** (1 / 6) * sigma, -eps, 0.0), (sigma, 0.0, 24 * eps / sigma)):
+        s = _atoms(types, [[0, 0, 0], [r, 0, 0]], [0, 0], box=3.0)
+        o = _oracle(s)
+        off, nb, _ = o.pairlist(1.0, 0.1, use_cells=False)
+        f, out = o.vdw(off, nb, "lj", cutoff=1.0)
+        assert out["lj_energy"] == pytest.approx(e_want, abs=1e-12)
+        # force on the atom at +x: +dU/dr pushes it outward when repulsive
+        assert f[1, 0] == pytest.approx(f_want, abs=1e-9)
+        assert f[0, 0] == pytest.approx(-f_want, abs=1e-9)
+        assert np.abs(f[:, 1:]).max() == 0.0
+
+
+def _lj_cluster(seed=5):
+    rng = np.random.default_rng(seed)
+    types = [AtomType(mass=15.9994, charge=0.0, eps=0.6502, sigma=0.3166),
+             AtomType(mass=12.0, charge=0.0, eps=0.3, sigma=0.35)]
+    # points on a jittered lattice so that no pair sits near the cutoff
+    grid = np.array([[i, j, k] for i in range(2) for j in range(2) for k in range(2)], dtype=float) * 0.42
+    pos = grid + rng.uniform(-0.04, 0.04, grid.shape) + 0.3
+    return _atoms(types, pos, [0, 1, 0, 1, 1, 0, 1, 0], box=2.6)
+
+
+def test_lj_forces_are_minus_energy_gradient(oracle_mod):
+    s = _lj_cluster()
+    o = _oracle(s)
+    off, nb, _ = o.pairlist(1.0, 0.1, use_cells=False)
+    f, _ = o.vdw(off, nb, "lj", cutoff=1.0, switch_on=True)
+    h = 1e-6
+    for a in (0, 3, 6):
+        for d in range(3):
+            e = []
+            for sgn in (1, -1):
+                p = s.pos.copy()
+                p[a, d] += sgn * h
+                op = _oracle(s, pos=p)
+                e.append(op.vdw(off, nb, "lj", cutoff=1.0, switch_on=True)[1]["lj_energy"])
+            fd = -(e[0] - e[1]) / (2 * h)
+            assert f[a, d] == pytest.approx(fd, rel=1e-6, abs=1e-6)
+
+
+# ---- buffered 14-7 ---------------------------------------------------------------
+
+def test_buffered_14_7_energy_and_force_at_rho_one(oracle_mod):
+    """Two waters whose only interaction is O-O.  At r = R (rho = 1) Halgren's
+    form gives E = -eps and dE/drho = 7 eps (1/(1+delta) - 1/(1+gamma))."""
+    R, eps = 0.3405, 0.46
+    types = [AtomType(mass=1.008, charge=0.0, eps=0.0, sigma=0.2),
+             AtomType(mass=15.9994, charge=0.0, eps=eps, sigma=R)]
+    one = np.array([[0.0816, 0.0577, 0.0], [0.0, 0.0, 0.0], [-0.0816, 0.0577, 0.0]])
+    pos = np.vstack([one, one + [R, 0.0, 0.0]])
+    s = _atoms(types, pos, [0, 1, 0, 0, 1, 0], box=3.0, mol=[0, 0, 0, 1, 1, 1])
+    # bonds and the angle exclude the intramolecular pairs, as in water
+    s.bonds = [(0, 1, 0), (1, 2, 0), (3, 4, 0), (4, 5, 0)]
+    s.angles = [(0, 1, 2, 0), (3, 4, 5, 0)]
+    o = _oracle(s)
+    off, nb, _ = o.pairlist(1.0, 0.1, use_cells=False)
+    f, out = o.vdw(off, nb, "buff", cutoff=1.0)
+    assert out["lj_energy"] == pytest.approx(-eps, abs=1e-12)
+    delta, gamma = 0.07, 0.12
+    dEdr = 7.0 * eps / R * (1.0 / (1.0 + delta) - 1.0 / (1.0 + gamma))
+    # molecule 0 sits at -x of molecule 1: its force is +dE/dr along x
+    assert f[0:3, 0].sum() == pytest.approx(dEdr, rel=1e-10)
+    assert f[3:6, 0].sum() == pytest.approx(-dEdr, rel=1e-10)
+    assert np.abs(f[:, 1:].sum(axis=0)).max() < 1e-12
+
+
+# ---- Ewald ---------------------------------------------------------------------
+
+def _rock_salt(cells=3, a0=0.282):
+    a = 2 * a0
+    base = [([0, 0, 0], 0), ([0.5, 0.5, 0], 0), ([0.5, 0, 0.5], 0), ([0, 0.5, 0.5], 0),
+            ([0.5, 0, 0], 1), ([0, 0.5, 0], 1), ([0, 0, 0.5], 1), ([0.5, 0.5, 0.5], 1)]
+    pos, typ = [], []
+    for i in range(cells):
+        for j in range(cells):
+            for k in range(cells):
+                for b, t in base:
+                    pos.append((np.array(b) + [i, j, k]) * a)
+                    typ.append(t)
+    L = cells * a
+    pos = np.array(pos) - L / 2
+    types = [AtomType(mass=22.99, charge=1.0, eps=0.0, sigma=0.0), AtomType(mass=35.45, charge=-1.0, eps=0.0, sigma=0.0)]
+    return _atoms(types, pos, typ, box=L), a0
+
+
+def test_ewald_total_reproduces_nacl_madelung_constant(oracle_mod):
+    s, a0 = _rock_salt()
+    o = _oracle(s)
+    alpha, rc = 5.0, 0.8
+    L = s.box[0]
+    kcut = 2 * alpha * math.sqrt(21.0)             # exp(-k^2/4 alpha^2) < 1e-9 beyond it
+    off, nb, _ = o.pairlist(rc, 0.0, use_cells=False)
+    _, real = o.ewald_real(off, nb, alpha, rc)
+    _, rec = o.ewald_recip(kcut, alpha)
+    self_e, charged = o.ewald_self(alpha)
+    assert charged == 0.0
+    e = real["energy"] + rec["energy"] + self_e
+    madelung = -2.0 * e * a0 / (s.n_atoms * COULOMB)
+    assert madelung == pytest.approx(1.747564594633, abs=2e-6)
+    assert L / 2 > rc
+
+
+def _random_charges(n=24, L=1.6, seed=11):
+    rng = np.random.default_rng(seed)
+    pos = rng.uniform(-L / 2, L / 2, (n, 3))
+    q = np.where(np.arange(n) % 2 == 0, 0.8, -0.8)
+    types = [AtomType(mass=1.0, charge=0.8, eps=0.0, sigma=0.0), AtomType(mass=1.0, charge=-0.8, eps=0.0, sigma=0.0)]
+    return _atoms(types, pos, (q < 0).astype(int), box=L)
+
+
+def _direct_kspace(pos, q, L, alpha, kcut):
+    """(2 pi / V) sum over the full sphere 0 < |k| < kcut of
+    exp(-k^2/4a^2)/k^2 |S(k)|^2."""
+    nmax = int(kcut * L / (2 * math.pi))
+    n = np.array([[i, j, k] for i in range(-nmax, nmax + 1) for j in range(-nmax, nmax + 1)
+                  for k in range(-nmax, nmax + 1) if (i, j, k) != (0, 0, 0)], dtype=float)
+    kv = n * (2 * math.pi / L)
+    k2 = (kv ** 2).sum(1)
+    kv, k2 = kv[k2 < kcut * kcut], k2[k2 < kcut * kcut]
+    S = (q[None, :] * np.exp(1j * kv @ pos.T)).sum(1)
+    V = L ** 3
+    return 2 * math.pi / V * np.sum(np.exp(-k2 / (4 * alpha ** 2)) / k2 * np.abs(S) ** 2)
+
+
+def test_ewald_reciprocal_equals_full_sphere_sum_and_its_gradient(oracle_mod):
+    s = _random_charges()
+    o = _oracle(s)
+    alpha, L = 3.5, s.box[0]
+    kcut = 2 * math.pi * math.sqrt(36.5) / L
+    f, rec = o.ewald_recip(kcut, alpha)
+    e_direct = _direct_kspace(s.pos, s.scaled_charge, L, alpha, kcut)
+    assert rec["energy"] == pytest.approx(e_direct, rel=1e-12)
+    h = 1e-6
+    for a in (0, 7, 13):
+        for d in range(3):
+            e = []
+            for sgn in (1, -1):
+                p = s.pos.copy()
+                p[a, d] += sgn * h
+                e.append(_direct_kspace(p, s.scaled_charge, L, alpha, kcut))
+            assert f[a, d] == pytest.approx(-(e[0] - e[1]) / (2 * h), rel=1e-6, abs=1e-5)
+
+
+def test_ewald_real_forces_are_minus_energy_gradient(oracle_mod):
+    s = _random_charges(n=16, L=2.4, seed=3)
+    o = _oracle(s)
+    off, nb, _ = o.pairlist(1.1, 0.1, use_cells=False)
+    f, out = o.ewald_real(off, nb, 3.5, 1.1)
+    h = 1e-6
+    for a in (1, 8):
+        for d in range(3):
+            e = []
+            for sgn in (1, -1):
+                p = s.pos.copy()
+                p[a, d] += sgn * h
+                e.append(_oracle(s, pos=p).ewald_real(off, nb, 3.5, 1.1)[1]["energy"])
+            assert f[a, d] == pytest.approx(-(e[0] - e[1]) / (2 * h), rel=1e-6, abs=1e-5)
+    # the real-space energy itself: sum over pairs inside rc of qi qj erfc(a r)/r
+    from scipy.special import erfc
+    q = s.scaled_charge
+    e = 0.0
+    for i in range(s.n_atoms):
+        for j in range(i + 1, s.n_atoms):
+            d = s.pos[j] - s.pos[i]
+            d -= s.box[0] * np.round(d / s.box[0])
+            r = np.linalg.norm(d)
+            if r < 1.1:
+                e += q[i] * q[j] * erfc(3.5 * r) / r
+    assert out["energy"] == pytest.approx(e, rel=1e-12)
+
+
+# ---- Nose-Hoover chain -------------------------------------------------------------
+
+def _mtk_rhs(t, y, kT, ndf, Q, W, V, dp):
+    K, v0, v1, v2, v3, vlog, lns = y
+    odnf = 1.0 + 3.0 / ndf
+    return [-2.0 * (v0 + vlog * odnf) * K,
+            (2 * K + W * vlog * vlog - (ndf + 1) * kT) / Q[0] - v0 * v1,
+            (Q[0] * v0 * v0 - kT) / Q[1] - v1 * v2,
+            (Q[1] * v1 * v1 - kT) / Q[2] - v2 * v3,
+            (Q[2] * v2 * v2 - kT) / Q[3],
+            (2 * K * odnf + 3 * V * dp) / W - vlog * v0,
+            -(v0 + vlog * odnf)]
+
+
+def _chain_vs_ode(dt):
+    from scipy.integrate import solve_ivp
+    from oracle import oracle as O
+    T, ndf, V, dp = 300.0, 600.0, 30.0, 0.4
+    kT = BOLTZ * T
+    tau, tau_b = 0.05, 0.2
+    Q = np.array([ndf * kT * tau ** 2, kT * tau ** 2, kT * tau ** 2, kT * tau ** 2])
+    W = (ndf + 3) * kT * tau_b ** 2
+    K0 = 1.15 * 0.5 * ndf * kT
+    vnh = np.array([3.0, -2.0, 1.5, 0.5])
+    gnh = np.zeros(4)
+    vlog, glog = C.c_double(0.8), C.c_double(0.0)
+    y0 = [K0, *vnh, vlog.value, 0.0]
+    p = lambda a: a.ctypes.data_as(C.c_void_p)   # noqa: E731
+    scale = O.lib().orc_nh_chain(K0, T, dt, ndf, dp, V, W, p(vnh), p(gnh), p(Q), C.byref(vlog), C.byref(glog))
+    sol = solve_ivp(_mtk_rhs, (0.0, dt / 2), y0, args=(kT, ndf, Q, W, V, dp), rtol=1e-13, atol=1e-15,
+                    method="DOP853")
+    yT = sol.y[:, -1]
+    got = np.array([math.log(scale), *vnh, vlog.value])
+    want = np.array([yT[6], *yT[1:5], yT[5]])
+    return got, want
+
+
+def test_nh_chain_integrates_the_mtk_equations_over_half_a_step(oracle_mod):
+    """NHIntegrator::chain advances the 4-link chain coupled to the barostat
+    by dt/2 with a 5 x 3 Suzuki-Yoshida factorisation.  Against a
+    high-accuracy integration of the same ODEs it must agree to the
+    factorisation's truncation error (a dropped coupling or a wrong factor
+    shows up at ~1e-3), and that error must fall at least 8x when dt halves."""
+    got, want = _chain_vs_ode(0.002)
+    err = np.abs(got - want) / np.maximum(np.abs(want), 1e-3)
+    assert err.max() < 1e-6
+    got2, want2 = _chain_vs_ode(0.001)
+    err2 = np.abs(got2 - want2) / np.maximum(np.abs(want2), 1e-3)
+    assert err2.max() < err.max() / 8.0
