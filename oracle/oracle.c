@@ -1275,3 +1275,224 @@ int orc_md_step(const orc_sys *s_in, const orc_md_params *p, orc_md_state *st) {
     st->step++;
     return 0;
 }
+
+/* ========================================================================== */
+/* polarizable water: frames, permanent field, induced dipoles                */
+/* ========================================================================== */
+
+int orc_local2global(const orc_sys *s, const double *dip_loc, const double *quad_loc,
+                     double *dip, double *quad) {
+    /* Local2Global::FrameofReference (src/utils/Local2Global.cpp:90-225).
+     * Molecule sites are H1, O, H2 (myAtoms[0..2]). */
+    const v3 oneX = {1.0, 0.0, 0.0}, oneY = {0.0, 1.0, 0.0};
+    for (int m = 0; m < s->n_mols; ++m)
+        if (s->mol_start[m + 1] - s->mol_start[m] != 3) return -1;
+    for (int m = 0; m < s->n_mols; ++m) {
+        const int a0 = s->mol_start[m];
+        const v3 h1 = v3_ld(s->pos + 3 * a0), o = v3_ld(s->pos + 3 * (a0 + 1)), h2 = v3_ld(s->pos + 3 * (a0 + 2));
+        v3 pp1[3], pp2[3];
+        /* H1: axis O - H1, second vector x (y when the axis is close to x) :114-121 */
+        pp1[0] = v3_sub(o, h1);
+        double val = pp1[0].x / v3_len(pp1[0]);
+        pp2[0] = oneX;
+        if (fabs(val) > 0.8660) pp2[0] = oneY;
+        /* O: sum of the unit vectors to the hydrogens, second vector to H2 :123-129 */
+        pp1[1] = v3_sub(h1, o);
+        pp2[1] = v3_sub(h2, o);
+        pp1[1] = v3_div(pp1[1], v3_len(pp1[1]));
+        pp2[1] = v3_div(pp2[1], v3_len(pp2[1]));
+        pp1[1] = v3_add(pp1[1], pp2[1]);
+        /* H2 :131-137 */
+        pp1[2] = v3_sub(o, h2);
+        val = pp1[2].x / v3_len(pp1[2]);
+        pp2[2] = oneX;
+        if (fabs(val) > 0.8660) pp2[2] = oneY;
+        for (int i = 0; i < 3; ++i) {
+            /* orthonormal u, v, w (:147-160) */
+            pp1[i] = v3_div(pp1[i], v3_len(pp1[i]));
+            const v3 w = pp1[i];
+            const v3 ss = v3_sub(pp2[i], v3_scale(v3_dot(pp2[i], w), w));
+            const v3 u = v3_div(ss, v3_len(ss));
+            const v3 v = {w.y * u.z - u.y * w.z, u.x * w.z - w.x * u.z, w.x * u.y - u.x * w.y};
+            const int id = a0 + i;
+            const double dx = dip_loc[3 * id], dy = dip_loc[3 * id + 1], dz = dip_loc[3 * id + 2];
+            dip[3 * id] = u.x * dx + v.x * dy + w.x * dz;
+            dip[3 * id + 1] = u.y * dx + v.y * dy + w.y * dz;
+            dip[3 * id + 2] = u.z * dx + v.z * dy + w.z * dz;
+            const double qxx = quad_loc[6 * id], qxy = quad_loc[6 * id + 1], qxz = quad_loc[6 * id + 2];
+            const double qyy = quad_loc[6 * id + 3], qyz = quad_loc[6 * id + 4], qzz = quad_loc[6 * id + 5];
+            /* R Q R^T with R = [u v w] (:181-205), then the traced form / 3 (:210-215) */
+            const double e[3][3] = {{u.x, v.x, w.x}, {u.y, v.y, w.y}, {u.z, v.z, w.z}};
+            const int ab[6][2] = {{0, 0}, {0, 1}, {0, 2}, {1, 1}, {1, 2}, {2, 2}};
+            for (int k = 0; k < 6; ++k) {
+                const double *ra = e[ab[k][0]], *rb = e[ab[k][1]];
+                const double qk = ra[0] * (rb[0] * qxx + rb[1] * qxy + rb[2] * qxz) +
+                                  ra[1] * (rb[0] * qxy + rb[1] * qyy + rb[2] * qyz) +
+                                  ra[2] * (rb[0] * qxz + rb[1] * qyz + rb[2] * qzz);
+                quad[6 * id + k] = qk / 3;
+            }
+        }
+    }
+    return 0;
+}
+
+int orc_induced_dipoles(const orc_sys *s, const double *dip, const double *quad, const double *polar,
+                        const orc_ind_params *p, double *ei0, double *pi, int *iters, double *final_res) {
+    /* Induction::InducedDipoles (src/forces/Induction.cpp:181-455) with the
+     * dense A[3N][3N], then ConjugateGradient (:2401-2467) or, without
+     * minimisation, Pi = alpha E0 (:446-453).  q is the unscaled charge. */
+    const int n = s->n_atoms, dim = 3 * n;
+    const double polmin = 0.00000001;
+    const double cutoffInd2 = p->cutoff_ind * p->cutoff_ind;
+    const double a = p->a_damp;
+    double *A = (double *)calloc((size_t)dim * dim, sizeof(double));
+    if (!A) return -3;
+    for (int k = 0; k < dim; ++k) { ei0[k] = 0.0; pi[k] = 0.0; }
+    int rc = 0;
+    for (int i = 0; i < n; ++i) {
+        const int i3 = 3 * i;
+        const int molIDi = s->mol[i];
+        const v3 ri = v3_ld(s->pos + 3 * i);
+        const double qi = s->q[i] / 11.78708976;   /* SQRTCOULOMBCONSTANT, src/NEMD_defs.h:103 */
+        const double polarity = polar[i] > polmin ? polar[i] : polmin;
+        const v3 dipi = v3_ld(dip + 3 * i);
+        const double Qxxi = quad[6 * i], Qxyi = quad[6 * i + 1], Qxzi = quad[6 * i + 2];
+        const double Qyyi = quad[6 * i + 3], Qyzi = quad[6 * i + 4], Qzzi = quad[6 * i + 5];
+        const double Qyxi = Qxyi, Qzxi = Qxzi, Qzyi = Qyzi;
+        A[(size_t)i3 * dim + i3] = 1.0 / polarity;
+        A[(size_t)(i3 + 1) * dim + i3 + 1] = 1.0 / polarity;
+        A[(size_t)(i3 + 2) * dim + i3 + 2] = 1.0 / polarity;
+        for (int j = i + 1; j < n; ++j) {
+            const int j3 = 3 * j;
+            const int molIDj = s->mol[j];
+            const v3 rj = v3_ld(s->pos + 3 * j);
+            const double qj = s->q[j] / 11.78708976;
+            const v3 dipj = v3_ld(dip + 3 * j);
+            const double Qxxj = quad[6 * j], Qxyj = quad[6 * j + 1], Qxzj = quad[6 * j + 2];
+            const double Qyyj = quad[6 * j + 3], Qyzj = quad[6 * j + 4], Qzzj = quad[6 * j + 5];
+            const double Qyxj = Qxyj, Qzxj = Qxzj, Qzyj = Qyzj;
+            v3 rij = v3_sub(ri, rj);
+            apply_pbc(&rij, s->box);
+            const double r = v3_len(rij);
+            const double rij2 = r * r;
+            if (!(rij2 <= cutoffInd2)) continue;
+            const double rij_1 = 1.0 / r;
+            const double rij_3 = rij_1 * rij_1 * rij_1;
+            const double rij_5 = rij_3 * rij_1 * rij_1;
+            const double rij_7 = rij_5 * rij_1 * rij_1;
+            const double polarj = polar[j] > polmin ? polar[j] : polmin;
+            const double u3 = rij2 * r / (sqrt(polarity * polarj));
+            const double exp_au3 = exp(-a * u3);
+            const double lamb3 = 1.0 - exp_au3;
+            const double lamb5 = 1.0 - (1.0 + a * u3) * exp_au3;
+            const double lamb7 = 1.0 - (1.0 + a * u3 + 3.0 * a * a * u3 * u3 / 5.0) * exp_au3;
+            double T[3][3];
+            const double rr[3] = {rij.x, rij.y, rij.z};
+            for (int x = 0; x < 3; ++x)
+                for (int y = x; y < 3; ++y) {
+                    double t = p->damp_tij ? lamb5 * 3.0 * rij_5 * rr[x] * rr[y] : 3.0 * rij_5 * rr[x] * rr[y];
+                    if (x == y) t = p->damp_tij ? t - lamb3 * rij_3 : t - rij_3;
+                    T[x][y] = t;
+                    T[y][x] = t;
+                }
+            for (int x = 0; x < 3; ++x)
+                for (int y = 0; y < 3; ++y) {
+                    A[(size_t)(i3 + x) * dim + j3 + y] = -T[x][y];
+                    A[(size_t)(j3 + x) * dim + i3 + y] = -T[x][y];
+                }
+            if (excluded(s, i, j)) continue;
+            if (molIDi == molIDj) { rc = -2; continue; }   /* the reference stops with ERRORMSG */
+            const v3 rji = v3_scale(-1.0, rij);
+            v3 RijXmuj, RjiXmui, QiXrji, QjXrij;
+            RijXmuj.x = rij.x * rij.x * dipj.x + rij.x * rij.y * dipj.y + rij.x * rij.z * dipj.z;
+            RijXmuj.y = rij.y * rij.x * dipj.x + rij.y * rij.y * dipj.y + rij.y * rij.z * dipj.z;
+            RijXmuj.z = rij.z * rij.x * dipj.x + rij.z * rij.y * dipj.y + rij.z * rij.z * dipj.z;
+            RjiXmui.x = rji.x * rji.x * dipi.x + rji.x * rji.y * dipi.y + rji.x * rji.z * dipi.z;
+            RjiXmui.y = rji.y * rji.x * dipi.x + rji.y * rji.y * dipi.y + rji.y * rji.z * dipi.z;
+            RjiXmui.z = rji.z * rji.x * dipi.x + rji.z * rji.y * dipi.y + rji.z * rji.z * dipi.z;
+            QiXrji.x = Qxxi * rji.x + Qxyi * rji.y + Qxzi * rji.z;
+            QiXrji.y = Qyxi * rji.x + Qyyi * rji.y + Qyzi * rji.z;
+            QiXrji.z = Qzxi * rji.x + Qzyi * rji.y + Qzzi * rji.z;
+            QjXrij.x = Qxxj * rij.x + Qxyj * rij.y + Qxzj * rij.z;
+            QjXrij.y = Qyxj * rij.x + Qyyj * rij.y + Qyzj * rij.z;
+            QjXrij.z = Qzxj * rij.x + Qzyj * rij.y + Qzzj * rij.z;
+            const double QiI = Qxxi + Qyyi + Qzzi;
+            const double QjI = Qxxj + Qyyj + Qzzj;
+            const double QjRij = Qxxj * rij.x * rij.x + Qxyj * rij.x * rij.y + Qxzj * rij.x * rij.z +
+                                 Qyxj * rij.y * rij.x + Qyyj * rij.y * rij.y + Qyzj * rij.y * rij.z +
+                                 Qzxj * rij.z * rij.x + Qzyj * rij.z * rij.y + Qzzj * rij.z * rij.z;
+            const double QiRji = Qxxi * rij.x * rij.x + Qxyi * rij.x * rij.y + Qxzi * rij.x * rij.z +
+                                 Qyxi * rij.y * rij.x + Qyyi * rij.y * rij.y + Qyzi * rij.y * rij.z +
+                                 Qzxi * rij.z * rij.x + Qzyi * rij.z * rij.y + Qzzi * rij.z * rij.z;
+            /* bi (:423-425) and bj (:430-432), each term scalar * vector, summed left to right */
+            const double ri3[3] = {rij.x, rij.y, rij.z}, rj3[3] = {rji.x, rji.y, rji.z};
+            const double RM[3] = {RijXmuj.x, RijXmuj.y, RijXmuj.z}, RMi[3] = {RjiXmui.x, RjiXmui.y, RjiXmui.z};
+            const double dj[3] = {dipj.x, dipj.y, dipj.z}, di[3] = {dipi.x, dipi.y, dipi.z};
+            const double QJ[3] = {QjXrij.x, QjXrij.y, QjXrij.z}, QIv[3] = {QiXrji.x, QiXrji.y, QiXrji.z};
+            for (int x = 0; x < 3; ++x) {
+                const double bi = qj * lamb3 * rij_3 * ri3[x] + 3.0 * lamb5 * rij_5 * RM[x] - lamb3 * rij_3 * dj[x] +
+                                  15.0 * lamb7 * rij_7 * QjRij * ri3[x] - 6.0 * lamb5 * rij_5 * QJ[x] -
+                                  3.0 * lamb5 * rij_5 * QjI * ri3[x];
+                ei0[i3 + x] += bi;
+            }
+            for (int x = 0; x < 3; ++x) {
+                const double bj = qi * lamb3 * rij_3 * rj3[x] + 3.0 * lamb5 * rij_5 * RMi[x] - lamb3 * rij_3 * di[x] +
+                                  15.0 * lamb7 * rij_7 * QiRji * rj3[x] - 6.0 * lamb5 * rij_5 * QIv[x] -
+                                  3.0 * lamb5 * rij_5 * QiI * rj3[x];
+                ei0[j3 + x] += bj;
+            }
+        }
+    }
+    int it = 0;
+    double res = 0.0;
+    if (p->do_min) {
+        /* ConjugateGradient (:2401-2467) */
+        const double tolerance = 1.0e-14;
+        double *r = (double *)malloc(sizeof(double) * dim);
+        double *v = (double *)malloc(sizeof(double) * dim);
+        double *z = (double *)malloc(sizeof(double) * dim);
+        double c = 0.0, t = 0.0, d = 0.0, dotV = 0.0;
+        for (int k = 0; k < dim; ++k) pi[k] = 1.0;
+        for (int k = 0; k < dim; ++k) {
+            r[k] = ei0[k];
+            for (int j = 0; j < dim; ++j) r[k] -= A[(size_t)k * dim + j] * pi[j];
+            v[k] = r[k];
+            c += r[k] * r[k];
+            dotV += v[k] * v[k];
+        }
+        res = sqrt(c);
+        for (int k = 0; k < dim; ++k) {
+            if (sqrt(dotV) < tolerance) break;
+            double dotVZ = 0.0;
+            for (int j = 0; j < dim; ++j) {
+                z[j] = 0.0;
+                for (int l = 0; l < dim; ++l) z[j] += A[(size_t)j * dim + l] * v[l];
+                dotVZ += v[j] * z[j];
+            }
+            t = c / dotVZ;
+            d = 0.0;
+            for (int j = 0; j < dim; ++j) {
+                pi[j] += t * v[j];
+                r[j] -= t * z[j];
+                d += r[j] * r[j];
+            }
+            ++it;
+            res = sqrt(d);
+            if (sqrt(d) < tolerance) break;
+            dotV = 0.0;
+            for (int j = 0; j < dim; ++j) {
+                v[j] = r[j] + (d / c) * v[j];
+                dotV += v[j] * v[j];
+            }
+            c = d;
+        }
+        free(r); free(v); free(z);
+    } else {
+        for (int i = 0; i < n; ++i)
+            for (int x = 0; x < 3; ++x) pi[3 * i + x] = polar[i] * ei0[3 * i + x];
+    }
+    free(A);
+    if (iters) *iters = it;
+    if (final_res) *final_res = res;
+    return rc;
+}

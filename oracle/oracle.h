@@ -221,6 +221,35 @@ int orc_md_step(const orc_sys *s_in, const orc_md_params *p, orc_md_state *st);
  * vtMinus = v - dt/2 F/m.  The pair list in st must be built already. */
 int orc_md_start(const orc_sys *s_in, const orc_md_params *p, orc_md_state *st);
 
+
+/* ---- polarizable water ----------------------------------------------------- */
+
+/* Local2Global::FrameofReference (src/utils/Local2Global.cpp:90-225): local
+ * site dipoles dip_loc[3n] and quadrupoles quad_loc[6n] (xx xy xz yy yz zz,
+ * AtomParam dipx.. quadzz) rotated into the lab frame of each water's sites
+ * (H1, O, H2); quad receives the traced form divided by 3, as the reference
+ * stores it.  Returns -1 unless every molecule has 3 sites. */
+int orc_local2global(const orc_sys *s, const double *dip_loc, const double *quad_loc,
+                     double *dip, double *quad);
+
+typedef struct {
+    double cutoff_ind;   /* get_cutoffInd() */
+    double a_damp;       /* get_adamping(): Thole a */
+    int32_t damp_tij;    /* use_damping_induction() */
+    int32_t do_min;      /* do_induction_minimization(): CG, else Pi = alpha E0 */
+} orc_ind_params;
+
+/* Induction::InducedDipoles (src/forces/Induction.cpp:181-455) and
+ * ConjugateGradient (:2401-2467): permanent field ei0[3n] of the lab-frame
+ * multipoles (Thole-damped, non-excluded pairs within cutoff_ind, minimum
+ * image, no Ewald) and the induced dipoles pi[3n] solving A pi = ei0 with
+ * A = diag(1/alpha) - T over all pairs within cutoff_ind.  iters and
+ * final_res report the CG iterations and the last residual norm.  Returns
+ * -2 when a non-excluded pair lies inside one molecule (the reference stops),
+ * -3 on allocation failure. */
+int orc_induced_dipoles(const orc_sys *s, const double *dip, const double *quad, const double *polar,
+                        const orc_ind_params *p, double *ei0, double *pi, int *iters, double *final_res);
+
 #ifdef __cplusplus
 }
 #endif

@@ -76,6 +76,11 @@ class MdState(C.Structure):
                 ("mol_ke2", C.c_double), ("atom_ke2", C.c_double), ("virial", C.c_double * 9)]
 
 
+class IndParams(C.Structure):
+    _fields_ = [("cutoff_ind", C.c_double), ("a_damp", C.c_double), ("damp_tij", C.c_int32),
+                ("do_min", C.c_int32)]
+
+
 _lib = None
 
 
@@ -111,6 +116,9 @@ def lib():
                                            C.c_void_p]
         _lib.orc_md_step.argtypes = [P(OrcSys), P(MdParams), P(MdState)]
         _lib.orc_md_start.argtypes = [P(OrcSys), P(MdParams), P(MdState)]
+        _lib.orc_local2global.argtypes = [P(OrcSys)] + [C.c_void_p] * 4
+        _lib.orc_induced_dipoles.argtypes = [P(OrcSys), C.c_void_p, C.c_void_p, C.c_void_p, P(IndParams),
+                                             C.c_void_p, C.c_void_p, P(C.c_int), P(C.c_double)]
     return _lib
 
 
@@ -253,6 +261,43 @@ class Oracle:
         lib().orc_ewald_recip(C.byref(self._sys), len(kmod2), _p(kvec), _p(kmod2), _p(expk), alpha, self.volume,
                               int(use_constraint), _p(f), C.byref(out))
         return f, self._eo(out)
+
+    # ---- polarizable water -----------------------------------------------
+    def local_multipoles(self):
+        """Per-atom local-frame dipoles [n,3] and quadrupoles [n,6] from the types."""
+        t = self.s.types
+        dip = np.array([t[k].dip for k in self.s.type], dtype=np.float64).reshape(-1, 3)
+        quad = np.array([t[k].quad for k in self.s.type], dtype=np.float64).reshape(-1, 6)
+        return np.ascontiguousarray(dip), np.ascontiguousarray(quad)
+
+    def polarizabilities(self):
+        return np.array([self.s.types[k].polar for k in self.s.type], dtype=np.float64)
+
+    def local2global(self, dip_loc=None, quad_loc=None):
+        if dip_loc is None:
+            dip_loc, quad_loc = self.local_multipoles()
+        dip_loc = np.ascontiguousarray(dip_loc, dtype=np.float64)
+        quad_loc = np.ascontiguousarray(quad_loc, dtype=np.float64)
+        n = self.s.n_atoms
+        dip, quad = np.zeros((n, 3)), np.zeros((n, 6))
+        rc = lib().orc_local2global(C.byref(self._sys), _p(dip_loc), _p(quad_loc), _p(dip), _p(quad))
+        if rc != 0:
+            raise RuntimeError(f"orc_local2global failed: {rc}")
+        return dip, quad
+
+    def induced_dipoles(self, dip, quad, *, cutoff_ind, a_damp, damp_tij=True, do_min=True, polar=None):
+        n = self.s.n_atoms
+        polar = self.polarizabilities() if polar is None else np.ascontiguousarray(polar, dtype=np.float64)
+        dip = np.ascontiguousarray(dip, dtype=np.float64)
+        quad = np.ascontiguousarray(quad, dtype=np.float64)
+        p = IndParams(cutoff_ind, a_damp, int(damp_tij), int(do_min))
+        e0, pi = np.zeros((n, 3)), np.zeros((n, 3))
+        it, res = C.c_int(), C.c_double()
+        rc = lib().orc_induced_dipoles(C.byref(self._sys), _p(dip), _p(quad), _p(polar), C.byref(p), _p(e0), _p(pi),
+                                       C.byref(it), C.byref(res))
+        if rc != 0:
+            raise RuntimeError(f"orc_induced_dipoles failed: {rc}")
+        return e0, pi, it.value, res.value
 
     # ---- integrator ------------------------------------------------------
     def init_velocity(self, temperature, seed, per_atom=True, ndf=None):

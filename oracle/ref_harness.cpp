@@ -5,6 +5,7 @@
 // product path.
 //
 // Usage:  ref_harness <dir> forces <force-list>   (lj,mie,buck,buff,ewald)
+//         ref_harness <dir> induction
 //         ref_harness <dir> pairlist
 //         ref_harness <dir> md <nsteps>
 //         ref_harness <dir> time <nsteps>
@@ -34,9 +35,23 @@
 
 #include <unistd.h>
 
+#include <algorithm>
+#include <cmath>
+#include <iostream>
+#include <list>
+#include <map>
+#include <set>
+
 // LFIntegrator.h pulls in every force header in the order the reference's own
-// translation units see them (several headers define clashing macros).
+// translation units see them (several headers define clashing macros).  The
+// induction mode reads Induction's private state (Pi, Ei0, the Local2Global
+// frames); access specifiers do not change the class layout, so the
+// reference's headers are read here with every member public.
+#define private public
+#define protected public
 #include "integrators/LFIntegrator.h"
+#undef private
+#undef protected
 // forces/NccForce.h defines a macro named like AtomParam's polarizability
 // member; the harness reads that member, so the macro is dropped here.
 #undef polar
@@ -265,6 +280,32 @@ int main(int argc, char** argv) {
                 return 2;
             }
         }
+    } else if (mode == "induction") {
+        // Induction::compute's first steps (src/forces/Induction.cpp:503-507):
+        // frames of reference, then the permanent field and induced dipoles.
+        for (int m = 0; m < ens->nMols; ++m) ens->molecules[m].mass_center();
+        Induction* ind = new Induction(ens);
+        ind->myL2G->FrameofReference();
+        ind->InducedDipoles();
+        const int n = ens->nAtoms;
+        std::vector<double> dip(3 * n), quad(6 * n), e0(ind->Ei0, ind->Ei0 + 3 * n), pi(ind->Pi, ind->Pi + 3 * n);
+        for (int i = 0; i < n; ++i) {
+            dip[3 * i] = ind->myL2G->dip[i].x;
+            dip[3 * i + 1] = ind->myL2G->dip[i].y;
+            dip[3 * i + 2] = ind->myL2G->dip[i].z;
+            quad[6 * i] = ind->myL2G->Qxx[i];
+            quad[6 * i + 1] = ind->myL2G->Qxy[i];
+            quad[6 * i + 2] = ind->myL2G->Qxz[i];
+            quad[6 * i + 3] = ind->myL2G->Qyy[i];
+            quad[6 * i + 4] = ind->myL2G->Qyz[i];
+            quad[6 * i + 5] = ind->myL2G->Qzz[i];
+        }
+        write_bin("ind_dip", dip.data(), dip.size() * 8);
+        write_bin("ind_quad", quad.data(), quad.size() * 8);
+        write_bin("ind_ei0", e0.data(), e0.size() * 8);
+        write_bin("ind_pi", pi.data(), pi.size() * 8);
+        meta.put("cutoff_ind", ens->myConfig->get_cutoffInd());
+        meta.put("a_damp", ens->myConfig->get_adamping());
     } else if (mode == "md" || mode == "time") {
         const int nsteps = argc > 3 ? std::atoi(argv[3]) : 1;
         // The reference never initialises Molecule::massCenter before the

@@ -18,11 +18,11 @@
 //     for the volume derivatives, q cos d, q sin d (d = r - R_mol) and
 //     q cos |d|^2, q sin |d|^2.  The waves of a block split the tile's atoms
 //     and are added in LDS, so one partial per (chunk, k) reaches memory.
-//   pass 1b (k_recip_csum): partials summed over chunks in a fixed order.
-//   pass 1c (k_recip_finish): per k, energy, volume derivatives and virial
-//     (:596-607); the reference's per-atom Lustig sums of the force loop
-//     (:609-658) are sums over atoms of products with the per-k sums above,
-//     so they are formed here per k; and (b C, b S) for the force pass.
+//   pass 1b (k_recip_reduce): partials summed over chunks in a fixed order,
+//     then per k: energy, volume derivatives and virial (:596-607); the
+//     reference's per-atom Lustig sums of the force loop (:609-658) are sums
+//     over atoms of products with the per-k sums above, so they are formed
+//     here per k; and (b C, b S) for the force pass.
 //   pass 2 (k_recip_force): a block of RF_WAVES waves owns 64 atoms (one per
 //     lane); wave w takes the w-th range of k-groups, with the groups and
 //     (b C, b S) staged in LDS, walks the rows as pass 1 does, and the waves'
@@ -135,7 +135,6 @@ int ewald_setup(Ctx* c, const smd_ewald_params* p) {
     chunks = (c->n + per - 1) / per;
     c->sfac_chunks = chunks;
     if ((rc = realloc(c->d_sfac_part, sizeof(double) * NSUM * (size_t)K * chunks))) return rc;
-    if ((rc = realloc(c->d_ssum, sizeof(double) * NSUM * (size_t)K))) return rc;
     if ((rc = realloc(c->d_e1, sizeof(double) * 6 * (size_t)c->n))) return rc;
     SMD_HIP(c, hipMemcpy(c->d_kgrp, grp.data(), sizeof(int4) * grp.size(), hipMemcpyHostToDevice));
     SMD_HIP(c, hipMemcpy(c->d_ksplit, split.data(), sizeof(int32_t) * (RF_WAVES + 1), hipMemcpyHostToDevice));
@@ -291,7 +290,6 @@ struct RecipArgs {
     const int4* grp;           // [ng] (first k, count, nx << 16 | ny, nz0)
     const int32_t* split;      // [RF_WAVES+1] group ranges of the force pass
     double* part;              // [chunk][K][NSUM]
-    double* ssum;              // [K][NSUM]
     void* kf;                  // [K] (b C, b S) as R pairs
     double2* e1;               // [n][3] exp(i 2 pi r_p / L_p) per atom, written by pass 1
     double* force;
@@ -323,7 +321,7 @@ __device__ __forceinline__ cplx<double> cpow(cplx<double> b, int n) {
 __device__ __forceinline__ int grp_nx(int v) { return v >> 16; }
 __device__ __forceinline__ int grp_ny(int v) { return (int)(short)(v & 0xffff); }
 
-// Pass 1.  Sums per k (layout of part and ssum): 0 q sin, 1 q cos,
+// Pass 1.  Sums per k (layout of part): 0 q sin, 1 q cos,
 // 2-4 q cos d, 5-7 q sin d, 8 q cos |d|^2, 9 q sin |d|^2 (reciprocal_term
 // :568-594 and the moments its force loop needs).
 template <typename R, bool LUSTIG>
@@ -431,70 +429,75 @@ __global__ __launch_bounds__(SF_WAVES * WAVE) void k_recip_sfac(RecipArgs a) {
     }
 }
 
-// Pass 1b: chunk partials summed in chunk order, four quarters per value
-// combined in quarter order.
-__global__ __launch_bounds__(256) void k_recip_csum(RecipArgs a) {
+// Pass 1b: per block, RK_PER_BLOCK k-vectors.  Their chunk partials are
+// summed in chunk order (four chunk quarters added in quarter order), then
+// per k: energy, volume derivatives and virial (:596-607), the force loop's
+// Lustig sums (:609-658) in per-k form, and (b C, b S) for the force pass.
+constexpr int RK_PER_BLOCK = WAVE / NSUM;   // 6 k-vectors: 60 of the 64 lanes
+
+template <typename R, bool LUSTIG>
+__global__ __launch_bounds__(256) void k_recip_reduce(RecipArgs a) {
     __shared__ double red[3][WAVE];
+    __shared__ double sums[WAVE];
     const int lane = threadIdx.x % WAVE, qt = threadIdx.x / WAVE;
     const int nv = a.K * NSUM;
-    const int v = blockIdx.x * WAVE + lane;
+    const int k0 = blockIdx.x * RK_PER_BLOCK;
+    const int v = k0 * NSUM + lane;
+    const bool live = lane < RK_PER_BLOCK * NSUM && v < nv;
     const int c0 = (a.chunks * qt) / 4, c1 = (a.chunks * (qt + 1)) / 4;
     double s = 0.0;
-    if (v < nv) {
+    if (live) {
 #pragma unroll 4
         for (int ch = c0; ch < c1; ++ch) s += a.part[(size_t)ch * nv + v];
     }
     if (qt > 0) red[qt - 1][lane] = s;
     __syncthreads();
-    if (qt == 0 && v < nv) a.ssum[v] = ((s + red[0][lane]) + red[1][lane]) + red[2][lane];
-}
+    if (qt == 0) sums[lane] = live ? ((s + red[0][lane]) + red[1][lane]) + red[2][lane] : 0.0;
+    __syncthreads();
 
-// Pass 1c: per-k energy, volume derivatives and virial (:596-607), the
-// force loop's Lustig sums (:609-658) in per-k form, and (b C, b S).
-template <typename R, bool LUSTIG>
-__global__ __launch_bounds__(256) void k_recip_finish(RecipArgs a) {
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    double v[3 + 9 + 4] = {0};   // energy, longDeriv, long2ndDeriv, virial[9], Ff, Ffk2, d2UV1, d2UrR
-    if (k < a.K) {
-        const double* s = a.ssum + (size_t)k * NSUM;
+    double o[3 + 9 + 4] = {0};   // energy, longDeriv, long2ndDeriv, virial[9], Ff, Ffk2, d2UV1, d2UrR
+    const int k = k0 + (int)threadIdx.x;
+    if (threadIdx.x < RK_PER_BLOCK && k < a.K) {
+        const double* sk = sums + threadIdx.x * NSUM;
         const double kx = a.kvec[3 * k], ky = a.kvec[3 * k + 1], kz = a.kvec[3 * k + 2];
         const double k2 = a.kmod2[k];
-        const double sinSum = s[0], cosSum = s[1];
+        const double sinSum = sk[0], cosSum = sk[1];
         const double te = a.pi4V * a.expk[k] * (sinSum * sinSum + cosSum * cosSum);
         const double coeff = 2.0 * (a.alphaR4 + 1.0 / k2);
         const double kk = (te * kx) * kx + (te * ky) * ky + (te * kz) * kz;
-        v[0] = te;
-        v[1] = -te + kk * 2.0 * a.alphaR4;
+        o[0] = te;
+        o[1] = -te + kk * 2.0 * a.alphaR4;
         const double t7 = ((7.0 * te * kx) * kx + (7.0 * te * ky) * ky + (7.0 * te * kz) * kz) * 2.0 * a.alphaR4;
         const double dd = kk * 2.0 * a.alphaR4;
         const double t2 = ((dd * kx) * kx + (dd * ky) * ky + (dd * kz) * kz) * 2.0 * a.alphaR4;
-        v[2] = 4.0 * te - t7 + t2;
-        v[3 + 0] = te - te * coeff * kx * kx;
-        v[3 + 4] = te - te * coeff * ky * ky;
-        v[3 + 8] = te - te * coeff * kz * kz;
-        v[3 + 1] = -te * coeff * kx * ky;
-        v[3 + 2] = -te * coeff * kx * kz;
-        v[3 + 5] = -te * coeff * ky * kz;
-        v[3 + 3] = v[3 + 1];
-        v[3 + 6] = v[3 + 2];
-        v[3 + 7] = v[3 + 5];
+        o[2] = 4.0 * te - t7 + t2;
+        o[3 + 0] = te - te * coeff * kx * kx;
+        o[3 + 4] = te - te * coeff * ky * ky;
+        o[3 + 8] = te - te * coeff * kz * kz;
+        o[3 + 1] = -te * coeff * kx * ky;
+        o[3 + 2] = -te * coeff * kx * kz;
+        o[3 + 5] = -te * coeff * ky * kz;
+        o[3 + 3] = o[3 + 1];
+        o[3 + 6] = o[3 + 2];
+        o[3 + 7] = o[3 + 5];
         const double b = a.pi8V * a.expk[k];
         if constexpr (LUSTIG) {
             // sum_i val_i (k.d_i) with val_i = b (q_i sin_i C - q_i cos_i S)
-            const double kSD = kx * s[5] + ky * s[6] + kz * s[7];
-            const double kCD = kx * s[2] + ky * s[3] + kz * s[4];
+            const double kSD = kx * sk[5] + ky * sk[6] + kz * sk[7];
+            const double kCD = kx * sk[2] + ky * sk[3] + kz * sk[4];
             const double ff = b * (cosSum * kSD - sinSum * kCD);
-            v[12] = ff;
-            v[13] = ff * (4.0 * a.alphaR4 * k2);
+            o[12] = ff;
+            o[13] = ff * (4.0 * a.alphaR4 * k2);
             const double bkk = (b * kx) * kx + (b * ky) * ky + (b * kz) * kz;
-            v[14] = bkk * (cosSum * s[8] + sinSum * s[9]);
-            v[15] = bkk * ((s[2] * s[2] + s[3] * s[3] + s[4] * s[4]) + (s[5] * s[5] + s[6] * s[6] + s[7] * s[7]));
+            o[14] = bkk * (cosSum * sk[8] + sinSum * sk[9]);
+            o[15] = bkk * ((sk[2] * sk[2] + sk[3] * sk[3] + sk[4] * sk[4]) + (sk[5] * sk[5] + sk[6] * sk[6] + sk[7] * sk[7]));
         }
         R* kf = reinterpret_cast<R*>(a.kf);
         kf[2 * k] = (R)(b * cosSum);
         kf[2 * k + 1] = (R)(b * sinSum);
     }
-    block_add<16>(v, a.terms + T_RC_E);
+    __syncthreads();
+    block_add<16>(o, a.terms + T_RC_E);
 }
 
 // Bytes of the (b C, b S) table in the force pass's LDS, rounded up so the
@@ -610,7 +613,6 @@ int ewald_recip(Ctx* c, int prec, int lustig) {
     a.grp = c->d_kgrp;
     a.split = c->d_ksplit;
     a.part = c->d_sfac_part;
-    a.ssum = c->d_ssum;
     a.kf = c->d_kf;
     a.e1 = reinterpret_cast<double2*>(c->d_e1);
     a.force = c->d_force;
@@ -633,14 +635,13 @@ int ewald_recip(Ctx* c, int prec, int lustig) {
     }
     {
         ProfScope ps(c, "recip_reduce");
-        hipLaunchKernelGGL(k_recip_csum, dim3((a.K * NSUM + WAVE - 1) / WAVE), dim3(256), 0, c->stream, a);
-        const dim3 gk((a.K + 255) / 256);
+        const dim3 gk((a.K + RK_PER_BLOCK - 1) / RK_PER_BLOCK);
         if (mixed) {
-            if (lustig) hipLaunchKernelGGL((k_recip_finish<float, true>), gk, dim3(256), 0, c->stream, a);
-            else hipLaunchKernelGGL((k_recip_finish<float, false>), gk, dim3(256), 0, c->stream, a);
+            if (lustig) hipLaunchKernelGGL((k_recip_reduce<float, true>), gk, dim3(256), 0, c->stream, a);
+            else hipLaunchKernelGGL((k_recip_reduce<float, false>), gk, dim3(256), 0, c->stream, a);
         } else {
-            if (lustig) hipLaunchKernelGGL((k_recip_finish<double, true>), gk, dim3(256), 0, c->stream, a);
-            else hipLaunchKernelGGL((k_recip_finish<double, false>), gk, dim3(256), 0, c->stream, a);
+            if (lustig) hipLaunchKernelGGL((k_recip_reduce<double, true>), gk, dim3(256), 0, c->stream, a);
+            else hipLaunchKernelGGL((k_recip_reduce<double, false>), gk, dim3(256), 0, c->stream, a);
         }
     }
     {

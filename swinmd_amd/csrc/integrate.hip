@@ -38,59 +38,30 @@ struct IntegArgs {
     int max_count, nose_hoover, atom_thermo;
     double fnh_host;
     int fnh_on_device;
+    int md_step;      // inside the captured MD step: drift removal and flag reset in SHAKE
+    double ndf;       // thermostat degrees of freedom (MD step)
 };
 
-__global__ void k_step_begin(double* terms, int* flag) {
-    for (int k = threadIdx.x; k < T_COUNT; k += blockDim.x) terms[k] = 0.0;
-    if (threadIdx.x == 0) flag[0] = 0;
-}
-
-__global__ __launch_bounds__(256) void k_disp_check(IntegArgs a) {
+// Head of the MD step: the Verlet trigger (any |displacement|^2 > cutBuff^2/4,
+// LFIntegrator.cpp:306-341) and the momentum sums of the drift removal.
+// Block 0 clears the step's other accumulators.  The drift sums start at zero:
+// the previous step's k_kinetics (or smd_md_init) cleared them, and the rebuild
+// flag was cleared by the previous step's k_shake.
+__global__ __launch_bounds__(256) void k_step_head(IntegArgs a) {
+    if (blockIdx.x == 0)
+        for (int k = threadIdx.x; k < T_COUNT; k += blockDim.x)
+            if (k < T_VOM || k > T_MTOT) a.terms[k] = 0.0;
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     bool need = false;
+    double v[4] = {0, 0, 0, 0};
     if (i < a.n) {
         const double x = a.disp[3 * i], y = a.disp[3 * i + 1], z = a.disp[3 * i + 2];
         need = len2_exact(x, y, z) > a.drmax;
-    }
-    if (__ballot(need) && (threadIdx.x % WAVE) == 0) a.flag[0] = 1;
-}
-
-// Verlet-list rebuild preamble: zero displacements, wrap molecules by centre.
-__global__ __launch_bounds__(256) void k_rebuild_prep(IntegArgs a) {
-    if (!a.flag[0]) return;
-    const int m = blockIdx.x * blockDim.x + threadIdx.x;
-    if (m == 0) a.scalars[2] += 1.0;
-    if (m >= a.nmol) return;
-    const int s = a.mol_start[m], e = a.mol_start[m + 1];
-    for (int i = s; i < e; ++i) a.disp[3 * i] = a.disp[3 * i + 1] = a.disp[3 * i + 2] = 0.0;
-    const double c0x = a.com[3 * m], c0y = a.com[3 * m + 1], c0z = a.com[3 * m + 2];
-    double cx = c0x, cy = c0y, cz = c0z;
-    pbc(cx, cy, cz, a.box);
-    const double dx = cx - c0x, dy = cy - c0y, dz = cz - c0z;
-    a.com[3 * m] = cx; a.com[3 * m + 1] = cy; a.com[3 * m + 2] = cz;
-    for (int i = s; i < e; ++i) { a.pos[3 * i] += dx; a.pos[3 * i + 1] += dy; a.pos[3 * i + 2] += dz; }
-}
-
-__global__ __launch_bounds__(256) void k_vom_reduce(IntegArgs a) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    double v[4] = {0, 0, 0, 0};
-    if (i < a.n) {
         const double m = a.mass[i];
         v[0] = a.vtm[3 * i] * m; v[1] = a.vtm[3 * i + 1] * m; v[2] = a.vtm[3 * i + 2] * m; v[3] = m;
     }
+    if (__ballot(need) && (threadIdx.x % WAVE) == 0) a.flag[0] = 1;
     block_add<4>(v, a.terms + T_VOM);
-}
-
-__global__ __launch_bounds__(256) void k_vom_apply(IntegArgs a) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= a.n) return;
-    a.force[3 * i] = a.force[3 * i + 1] = a.force[3 * i + 2] = 0.0;
-    const double mt = a.terms[T_MTOT];
-    const double vx = a.terms[T_VOM] / mt, vy = a.terms[T_VOM + 1] / mt, vz = a.terms[T_VOM + 2] / mt;
-    if (fabs(vx) > 1e-11 || fabs(vy) > 1e-11 || fabs(vz) > 1e-11) {
-        a.vtm[3 * i] -= vx; a.vtm[3 * i + 1] -= vy; a.vtm[3 * i + 2] -= vz;
-        a.vel[3 * i] -= vx; a.vel[3 * i + 1] -= vy; a.vel[3 * i + 2] -= vz;
-    }
 }
 
 __global__ __launch_bounds__(256) void k_shake(IntegArgs a) {
@@ -101,7 +72,16 @@ __global__ __launch_bounds__(256) void k_shake(IntegArgs a) {
         const double fnh = a.fnh_on_device ? a.scalars[0] : a.fnh_host;
         const double Px = a.mom[3 * m], Py = a.mom[3 * m + 1], Pz = a.mom[3 * m + 2];
         const double tol_2 = 2 * a.tol;
-        double ro[3][3], rn[3][3], mr[3];
+        // MD step: the drift removal of LFIntegrator::run (:354-379) on
+        // vtMinus, from the sums k_step_head formed (velocity is rewritten
+        // below from vtMinus, so only vtMinus needs it).
+        double vcm[3] = {0.0, 0.0, 0.0};
+        if (a.md_step) {
+            const double mt = a.terms[T_MTOT];
+            const double vx = a.terms[T_VOM] / mt, vy = a.terms[T_VOM + 1] / mt, vz = a.terms[T_VOM + 2] / mt;
+            if (fabs(vx) > 1e-11 || fabs(vy) > 1e-11 || fabs(vz) > 1e-11) { vcm[0] = vx; vcm[1] = vy; vcm[2] = vz; }
+        }
+        double ro[3][3], rn[3][3], mr[3], vt[3][3];
 #pragma unroll
         for (int s = 0; s < 3; ++s) {
             const int id = a0 + s;
@@ -109,13 +89,14 @@ __global__ __launch_bounds__(256) void k_shake(IntegArgs a) {
             const double h = a.dt * mr[s];
             for (int d = 0; d < 3; ++d) {
                 ro[s][d] = a.pos[3 * id + d];
+                vt[s][d] = a.vtm[3 * id + d] - vcm[d];
                 const double F = a.force[3 * id + d];
                 double vp;
                 if (a.nose_hoover) {
                     const double P = d == 0 ? Px : (d == 1 ? Py : Pz);
-                    vp = a.vtm[3 * id + d] + h * (F - (fnh * P) / 3.0);
+                    vp = vt[s][d] + h * (F - (fnh * P) / 3.0);
                 } else {
-                    vp = a.vtm[3 * id + d] + h * F;
+                    vp = vt[s][d] + h * F;
                 }
                 rn[s][d] = ro[s][d] + a.dt * vp;
             }
@@ -134,6 +115,19 @@ __global__ __launch_bounds__(256) void k_shake(IntegArgs a) {
             pbc(od[b][0], od[b][1], od[b][2], a.box);
             msum[b] = mr[b] + mr[nx];
         }
+        // Minimum image inside the sweeps only for a molecule that spans a
+        // quarter box; otherwise apply_pbc is the identity on its bonds.
+        bool wrap = false;
+#pragma unroll
+        for (int b = 0; b < 3; ++b) {
+            const int nx = b == 2 ? 0 : b + 1;
+#pragma unroll
+            for (int d = 0; d < 3; ++d)
+                wrap = wrap || fabs(od[b][d]) > 0.25 * a.box.L[d] || fabs(rn[b][d] - rn[nx][d]) > 0.25 * a.box.L[d];
+        }
+        // Gauss-Seidel sweeps (LFIntegrator.cpp:1499-1540).  The update is
+        // predicated instead of branched: an inactive constraint moves its
+        // atoms by exactly 0.0, and its quotient is discarded by a select.
         int count = 0;
         bool done = false, failed = false;
         while (!done && count < a.max_count && !failed) {
@@ -141,30 +135,26 @@ __global__ __launch_bounds__(256) void k_shake(IntegArgs a) {
 #pragma unroll
             for (int b = 0; b < 3; ++b) {
                 const int nx = b == 2 ? 0 : b + 1;
-                if (!failed) {
-                    double nX = rn[b][0] - rn[nx][0], nY = rn[b][1] - rn[nx][1], nZ = rn[b][2] - rn[nx][2];
-                    pbc(nX, nY, nZ, a.box);
-                    const double diff = bl[b] - len2_exact(nX, nY, nZ);
-                    if (fabs(diff) > bl[b] * tol_2) {
-                        const double oX = od[b][0], oY = od[b][1], oZ = od[b][2];
-                        const double rno = oX * nX + oY * nY + oZ * nZ;
-                        if (rno < bl[b] * a.tol) {
-                            failed = true;
-                        } else {
-                            const double factor = diff / (2 * rno * msum[b]);
-                            v[0] += factor * nX * oX; v[1] += factor * nX * oY; v[2] += factor * nX * oZ;
-                            v[3] += factor * nY * oX; v[4] += factor * nY * oY; v[5] += factor * nY * oZ;
-                            v[6] += factor * nZ * oX; v[7] += factor * nZ * oY; v[8] += factor * nZ * oZ;
-                            const double fX = factor * oX, fY = factor * oY, fZ = factor * oZ;
-                            rn[b][0] += mr[b] * fX; rn[b][1] += mr[b] * fY; rn[b][2] += mr[b] * fZ;
-                            rn[nx][0] -= mr[nx] * fX; rn[nx][1] -= mr[nx] * fY; rn[nx][2] -= mr[nx] * fZ;
-                            done = false;
-                        }
-                    }
-                }
+                double nX = rn[b][0] - rn[nx][0], nY = rn[b][1] - rn[nx][1], nZ = rn[b][2] - rn[nx][2];
+                if (wrap) pbc(nX, nY, nZ, a.box);
+                const double diff = bl[b] - len2_exact(nX, nY, nZ);
+                const double oX = od[b][0], oY = od[b][1], oZ = od[b][2];
+                const double rno = oX * nX + oY * nY + oZ * nZ;
+                const bool act = fabs(diff) > bl[b] * tol_2;
+                failed = failed || (act && rno < bl[b] * a.tol);
+                const double q = diff / (2 * rno * msum[b]);
+                const double factor = act ? q : 0.0;
+                v[0] += factor * nX * oX; v[1] += factor * nX * oY; v[2] += factor * nX * oZ;
+                v[3] += factor * nY * oX; v[4] += factor * nY * oY; v[5] += factor * nY * oZ;
+                v[6] += factor * nZ * oX; v[7] += factor * nZ * oY; v[8] += factor * nZ * oZ;
+                const double fX = factor * oX, fY = factor * oY, fZ = factor * oZ;
+                rn[b][0] += mr[b] * fX; rn[b][1] += mr[b] * fY; rn[b][2] += mr[b] * fZ;
+                rn[nx][0] -= mr[nx] * fX; rn[nx][1] -= mr[nx] * fY; rn[nx][2] -= mr[nx] * fZ;
+                done = done && !act;
             }
             count++;
         }
+        if (a.md_step && m == 0) a.flag[0] = 0;   // every consumer of the rebuild flag has run
         if (failed) atomicOr(a.status, ST_SHAKE_FAIL);
         else if (!done) atomicOr(a.status, ST_SHAKE_NOCONV);
 #pragma unroll
@@ -177,7 +167,7 @@ __global__ __launch_bounds__(256) void k_shake(IntegArgs a) {
                 a.pos[3 * id + d] = p0 + dr;
                 a.disp[3 * id + d] += dr;
                 const double vp = dr / a.dt;
-                a.vel[3 * id + d] = (vp + a.vtm[3 * id + d]) * 0.5;
+                a.vel[3 * id + d] = (vp + vt[s][d]) * 0.5;
                 a.vtm[3 * id + d] = vp;
             }
         }
@@ -220,6 +210,22 @@ __global__ __launch_bounds__(256) void k_lf_atoms(IntegArgs a) {
     }
 }
 
+// LF Nose-Hoover friction update (LFIntegrator::run :688-709): fnh, fnhmi.
+__device__ __forceinline__ void nh_friction(const IntegArgs& a, double ke_atom, double ke_mol) {
+    a.scalars[3] += 1.0;
+    if (!a.nose_hoover) { a.scalars[0] = 0.0; return; }
+    const double ke = a.atom_thermo ? ke_atom : ke_mol;
+    const double qq = a.ndf * BOLTZ * a.temperature * a.coupl * a.coupl;
+    const double invqq = 1 / qq;
+    const double fm = a.scalars[1];
+    const double fp = fm + a.dt * invqq * (ke - a.ndf * BOLTZ * a.temperature);
+    a.scalars[0] = 0.5 * (fp + fm);
+    a.scalars[1] = fp;
+}
+
+// Molecular centres, momenta and kinetic sums (:646-686).  In the MD step the
+// last block to finish also applies the thermostat update and clears the
+// drift sums for the next step (its counter lives in flag[2]).
 __global__ __launch_bounds__(256) void k_kinetics(IntegArgs a, int rewrap) {
     double v[2] = {0.0, 0.0};
     const int m = blockIdx.x * blockDim.x + threadIdx.x;
@@ -252,20 +258,20 @@ __global__ __launch_bounds__(256) void k_kinetics(IntegArgs a, int rewrap) {
         for (int d = 0; d < 3; ++d) a.com[3 * m + d] = C[d];
     }
     block_add<2>(v, a.terms + T_KE_ATOM);
-}
-
-__global__ void k_nh_friction(IntegArgs a, double ndf) {
-    // LFIntegrator::run (:688-709); stores fnh and fnhmi
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
-    a.scalars[3] += 1.0;
-    if (!a.nose_hoover) { a.scalars[0] = 0.0; return; }
-    const double ke = a.atom_thermo ? a.terms[T_KE_ATOM] : a.terms[T_KE_MOL];
-    const double qq = ndf * BOLTZ * a.temperature * a.coupl * a.coupl;
-    const double invqq = 1 / qq;
-    const double fm = a.scalars[1];
-    const double fp = fm + a.dt * invqq * (ke - ndf * BOLTZ * a.temperature);
-    a.scalars[0] = 0.5 * (fp + fm);
-    a.scalars[1] = fp;
+    if (a.md_step) {
+        __shared__ bool last;
+        __threadfence();
+        __syncthreads();
+        if (threadIdx.x == 0) last = atomicAdd(&a.flag[2], 1) == (int)gridDim.x - 1;
+        __syncthreads();
+        if (last && threadIdx.x == 0) {
+            __threadfence();
+            volatile const double* t = a.terms;   // sums the other blocks added
+            nh_friction(a, t[T_KE_ATOM], t[T_KE_MOL]);
+            for (int k = T_VOM; k <= T_MTOT; ++k) a.terms[k] = 0.0;
+            a.flag[2] = 0;
+        }
+    }
 }
 
 __global__ __launch_bounds__(256) void k_mol_centers(int nmol, const int32_t* __restrict__ ms,
@@ -393,18 +399,29 @@ static IntegArgs integ_args(Ctx* c, const smd_integrator_params* p) {
     return a;
 }
 
-int shake_run(Ctx* c, const smd_integrator_params* p, bool fnh_on_device, double fnh) {
+static double md_ndf(Ctx* c) {
+    // Integrator ctor (src/Integrator.cpp:25-29)
+    if (c->md.integ.atom_thermo) return (double)(6 * c->nmol - 3);
+    return (double)(3 * c->nmol - 3);
+}
+
+int shake_run(Ctx* c, const smd_integrator_params* p, bool fnh_on_device, double fnh, bool md_step) {
     IntegArgs a = integ_args(c, p);
     a.fnh_on_device = fnh_on_device ? 1 : 0;
     a.fnh_host = fnh;
+    a.md_step = md_step ? 1 : 0;
     ProfScope ps(c, "shake");
     hipLaunchKernelGGL(k_shake, dim3((c->nmol + 255) / 256), dim3(256), 0, c->stream, a);
     SMD_HIP(c, hipGetLastError());
     return SMD_OK;
 }
 
-int kinetics_run(Ctx* c, bool rewrap) {
-    IntegArgs a = integ_args(c, nullptr);
+int kinetics_run(Ctx* c, bool rewrap, const smd_integrator_params* md) {
+    IntegArgs a = integ_args(c, md);
+    if (md) {
+        a.md_step = 1;
+        a.ndf = md_ndf(c);
+    }
     ProfScope ps(c, "kinetics");
     hipLaunchKernelGGL(k_kinetics, dim3((c->nmol + 255) / 256), dim3(256), 0, c->stream, a, rewrap ? 1 : 0);
     SMD_HIP(c, hipGetLastError());
@@ -419,11 +436,6 @@ int lf_atoms_run(Ctx* c, const smd_integrator_params* p, double fnh) {
     return SMD_OK;
 }
 
-static double md_ndf(Ctx* c) {
-    // Integrator ctor (src/Integrator.cpp:25-29)
-    if (c->md.integ.atom_thermo) return (double)(6 * c->nmol - 3);
-    return (double)(3 * c->nmol - 3);
-}
 
 // One LFIntegrator::run iteration enqueued on the context stream.  Contains
 // no host synchronisation and no memset, so it can be captured.
@@ -432,23 +444,21 @@ int md_step_enqueue(Ctx* c, bool do_long) {
     IntegArgs a = integ_args(c, &p.integ);
     a.fnh_on_device = 1;
     a.drmax = p.cut_buff * p.cut_buff / 4.0;   // Integrator.cpp:36
-    const dim3 gA((c->n + 255) / 256), gM((c->nmol + 255) / 256), b256(256);
-    hipLaunchKernelGGL(k_step_begin, dim3(1), b256, 0, c->stream, c->d_terms, c->d_flag);
-    hipLaunchKernelGGL(k_disp_check, gA, b256, 0, c->stream, a);
-    hipLaunchKernelGGL(k_rebuild_prep, gM, b256, 0, c->stream, a);
+    const dim3 gA((c->n + 255) / 256), b256(256);
+    // trigger + drift sums; the list rebuild (with the molecule rewrap) runs
+    // only when the trigger fired, gated on the device flag
+    hipLaunchKernelGGL(k_step_head, gA, b256, 0, c->stream, a);
     int rc = pairlist_build(c, p.cutoff, p.cut_buff, p.use_cells, true);
     if (rc) return rc;
-    hipLaunchKernelGGL(k_vom_reduce, gA, b256, 0, c->stream, a);
-    hipLaunchKernelGGL(k_vom_apply, gA, b256, 0, c->stream, a);
+    // the pair kernel is the first force term of the step: it stores forces
     rc = pairs_lj_ewald(c, nullptr, nullptr, p.cutoff, p.ewald.alpha, p.ewald.real_cut, p.ewald.use_constraint,
-                        p.lustig, p.prec);
+                        p.lustig, p.prec, true);
     if (rc) return rc;
     if ((rc = ewald_intra(c, p.lustig))) return rc;
     if (p.ewald.surface && (rc = ewald_surface(c))) return rc;
     if (do_long && (rc = ewald_recip(c, p.prec, p.lustig))) return rc;
-    if ((rc = shake_run(c, &p.integ, true, 0.0))) return rc;
-    if ((rc = kinetics_run(c, true))) return rc;
-    hipLaunchKernelGGL(k_nh_friction, dim3(1), dim3(64), 0, c->stream, a, md_ndf(c));
+    if ((rc = shake_run(c, &p.integ, true, 0.0, true))) return rc;   // drift removal, SHAKE, clears the flag
+    if ((rc = kinetics_run(c, true, &p.integ))) return rc;         // kinetics, then the NH update
     SMD_HIP(c, hipGetLastError());
     return SMD_OK;
 }
@@ -605,6 +615,7 @@ extern "C" int smd_md_init(smd_ctx* c, const smd_md_params* p) {
     SMD_HIP(c, hipMemsetAsync(c->d_mom, 0, sizeof(double) * 3 * c->nmol, c->stream));
     SMD_HIP(c, hipMemsetAsync(c->d_disp, 0, sizeof(double) * 3 * c->n, c->stream));
     SMD_HIP(c, hipMemsetAsync(c->d_status, 0, sizeof(unsigned), c->stream));
+    SMD_HIP(c, hipMemsetAsync(c->d_flag, 0, sizeof(int) * 4, c->stream));   // rebuild flag, block counter
     SMD_HIP(c, hipStreamSynchronize(c->stream));
     if (c->graph_exec) { (void)hipGraphExecDestroy(c->graph_exec); c->graph_exec = nullptr; }
     if (c->graph) { (void)hipGraphDestroy(c->graph); c->graph = nullptr; }

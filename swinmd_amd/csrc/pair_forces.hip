@@ -34,6 +34,7 @@ struct PairArgs {
     const int32_t* nb;
     const int32_t* nb_cnt;
     const int32_t* nb_own;
+    int store_forces;        // 1: write the atom forces (first term of a step), 0: add
     int32_t stride;
     double* force;
     double* fvirt;
@@ -395,7 +396,11 @@ __global__ __launch_bounds__(256) void k_pairs(PairArgs a) {
             // force on the virtual site; k_redistribute moves it to atoms
             a.fvirt[3 * i] = fix; a.fvirt[3 * i + 1] = fiy; a.fvirt[3 * i + 2] = fiz;
         } else {
-            a.force[3 * i] += fix; a.force[3 * i + 1] += fiy; a.force[3 * i + 2] += fiz;
+            if (a.store_forces) {
+                a.force[3 * i] = fix; a.force[3 * i + 1] = fiy; a.force[3 * i + 2] = fiz;
+            } else {
+                a.force[3 * i] += fix; a.force[3 * i + 1] += fiy; a.force[3 * i + 2] += fiz;
+            }
         }
     }
 
@@ -713,10 +718,11 @@ int ewald_real(Ctx* c, int prec, int lustig) {
 
 // Fused LJ + Ewald real space for the MD step (one pass over the list).
 int pairs_lj_ewald(Ctx* c, const double* sigma2, const double* eps4, double cutoff, double alpha, double real_cut,
-                   int use_constraint, int lustig, int prec) {
+                   int use_constraint, int lustig, int prec, bool store_forces) {
     (void)sigma2; (void)eps4;   // tables uploaded once by the MD setup
     PairArgs a;
     base_args(c, a);
+    a.store_forces = store_forces ? 1 : 0;
     a.cutoff = cutoff;
     a.cut2 = cutoff * cutoff;
     a.rcut_2 = 1.0 / a.cut2;

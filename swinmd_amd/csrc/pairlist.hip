@@ -44,10 +44,22 @@ __device__ __forceinline__ int cell_coord(double v, int n) {
     return x >= n ? n - 1 : x;
 }
 
+// Rebuild preamble of the MD step (LFIntegrator::run :306-341): count the
+// update, zero the displacements and shift each molecule so its centre lies
+// in the box.
+struct RebuildPrep {
+    int on;
+    double* com;
+    double* disp;
+    double* scalars;   // [2] list updates
+};
+
 // Atom -> cell (CellManager::find_cell, src/utils/CellManager.h:97-119) and the
-// per-cell counts, aggregated in LDS when the grid fits there.
+// per-cell counts, aggregated in LDS when the grid fits there.  One thread
+// per molecule, so the MD step's molecule rewrap runs in the same pass.
 template <bool LDS_HIST>
-__global__ __launch_bounds__(1024) void k_cell_assign(int n, const double* __restrict__ pos, CellGrid g,
+__global__ __launch_bounds__(1024) void k_cell_assign(int nmol, const int32_t* __restrict__ mol_start, double* pos,
+                                                      CellGrid g, Box box, RebuildPrep pr,
                                                       int32_t* __restrict__ cell_of, int32_t* __restrict__ cnt,
                                                       const int* gate) {
     if (gate && !*gate) return;
@@ -56,18 +68,31 @@ __global__ __launch_bounds__(1024) void k_cell_assign(int n, const double* __res
         for (int k = threadIdx.x; k < g.ncell; k += blockDim.x) hist[k] = 0;
         __syncthreads();
     }
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) {
-        const double dx = pos[3 * i] - g.origin[0];
-        const double dy = pos[3 * i + 1] - g.origin[1];
-        const double dz = pos[3 * i + 2] - g.origin[2];
-        const int x = cell_coord(dx * g.rlen[0], g.n[0]);
-        const int y = cell_coord(dy * g.rlen[1], g.n[1]);
-        const int z = cell_coord(dz * g.rlen[2], g.n[2]);
-        const int c = z * g.nxy + y * g.n[0] + x;
-        cell_of[i] = c;
-        if constexpr (LDS_HIST) atomicAdd(&hist[c], 1);
-        else atomicAdd(&cnt[c], 1);
+    const int m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m < nmol) {
+        const int s = mol_start[m], e = mol_start[m + 1];
+        if (pr.on) {
+            if (m == 0) pr.scalars[2] += 1.0;
+            for (int i = s; i < e; ++i) pr.disp[3 * i] = pr.disp[3 * i + 1] = pr.disp[3 * i + 2] = 0.0;
+            const double c0x = pr.com[3 * m], c0y = pr.com[3 * m + 1], c0z = pr.com[3 * m + 2];
+            double cx = c0x, cy = c0y, cz = c0z;
+            pbc(cx, cy, cz, box);
+            const double dx = cx - c0x, dy = cy - c0y, dz = cz - c0z;
+            pr.com[3 * m] = cx; pr.com[3 * m + 1] = cy; pr.com[3 * m + 2] = cz;
+            for (int i = s; i < e; ++i) { pos[3 * i] += dx; pos[3 * i + 1] += dy; pos[3 * i + 2] += dz; }
+        }
+        for (int i = s; i < e; ++i) {
+            const double dx = pos[3 * i] - g.origin[0];
+            const double dy = pos[3 * i + 1] - g.origin[1];
+            const double dz = pos[3 * i + 2] - g.origin[2];
+            const int x = cell_coord(dx * g.rlen[0], g.n[0]);
+            const int y = cell_coord(dy * g.rlen[1], g.n[1]);
+            const int z = cell_coord(dz * g.rlen[2], g.n[2]);
+            const int c = z * g.nxy + y * g.n[0] + x;
+            cell_of[i] = c;
+            if constexpr (LDS_HIST) atomicAdd(&hist[c], 1);
+            else atomicAdd(&cnt[c], 1);
+        }
     }
     if constexpr (LDS_HIST) {
         __syncthreads();
@@ -76,46 +101,31 @@ __global__ __launch_bounds__(1024) void k_cell_assign(int n, const double* __res
     }
 }
 
-// Exclusive scan of the cell counts by one block; the counts are cleared for
-// the next build.
-__global__ __launch_bounds__(1024) void k_cell_scan(int ncell, int32_t* __restrict__ cnt, int32_t* __restrict__ start,
-                                                    const int* gate) {
-    if (gate && !*gate) return;
-    __shared__ int32_t part[1024];
-    const int t = threadIdx.x, T = blockDim.x;
-    const int per = (ncell + T - 1) / T;
-    const int lo = t * per, hi = min(ncell, lo + per);
-    int32_t s = 0;
-    for (int k = lo; k < hi; ++k) s += cnt[k];
-    part[t] = s;
-    __syncthreads();
-    if (t == 0) {
-        int32_t run = 0;
-        for (int k = 0; k < T; ++k) { int32_t v = part[k]; part[k] = run; run += v; }
-        start[ncell] = run;
-    }
-    __syncthreads();
-    int32_t run = part[t];
-    for (int k = lo; k < hi; ++k) {
-        start[k] = run;
-        run += cnt[k];
-        cnt[k] = 0;
-    }
-}
-
-// One block per cell compacts its atoms in index order (set_atom appends in
-// atom order, CellManager.cpp:145-170); it stops once the cell is full.
+// One block per cell: its start (the sum of the counts of lower cells) and
+// its atoms compacted in index order (set_atom appends in atom order,
+// CellManager.cpp:145-170); it stops once the cell is full.  The counts are
+// cleared later, by the counting pass of the list build.
 __global__ __launch_bounds__(256) void k_cell_fill(int n, int ncell, const int32_t* __restrict__ cell_of,
-                                                   const int32_t* __restrict__ start, int32_t* __restrict__ atoms,
-                                                   const int* gate) {
+                                                   const int32_t* __restrict__ cnt, int32_t* __restrict__ start,
+                                                   int32_t* __restrict__ atoms, const int* gate) {
     if (gate && !*gate) return;
     __shared__ int32_t wsum[4];
     const int c = blockIdx.x;
     if (c >= ncell) return;
     const int lane = threadIdx.x % WAVE, w = threadIdx.x / WAVE;
+    int32_t part = 0;
+    for (int k = threadIdx.x; k < c; k += blockDim.x) part += cnt[k];
+    part = wave_sum(part);
+    if (lane == 0) wsum[w] = part;
+    __syncthreads();
+    int base = wsum[0] + wsum[1] + wsum[2] + wsum[3];
+    const int end = base + cnt[c];
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        start[c] = base;
+        if (c == ncell - 1) start[ncell] = end;
+    }
     const uint64_t below = (1ull << lane) - 1ull;
-    int base = start[c];
-    const int end = start[c + 1];
     for (int a0 = 0; a0 < n && base < end; a0 += 256) {
         const int a = a0 + threadIdx.x;
         const bool hit = (a < n) && (cell_of[a] == c);
@@ -234,10 +244,14 @@ __device__ __forceinline__ int neighbour_cell(const CellGrid& g, int c, int s) {
 template <bool WRITE>
 __global__ __launch_bounds__(LB_THREADS) void k_list_cells(ListArgs a, CellGrid g, const int32_t* __restrict__ start,
                                                            const int32_t* __restrict__ catoms, int32_t* __restrict__ seg,
-                                                           const int* gate) {
+                                                           int32_t* cell_cnt, const int* gate) {
     if (gate && !*gate) return;
     __shared__ double4 sj[LB_STAGE];   // x, y, z, atom index
     const int c = blockIdx.x, s = blockIdx.y;
+    // the cell counts were last read by k_cell_fill: clear them for the next build
+    if constexpr (!WRITE)
+        if (c == 0 && s == 0)
+            for (int k = threadIdx.x; k < g.ncell; k += blockDim.x) cell_cnt[k] = 0;
     const int nc = neighbour_cell(g, c, s);
     const int ib = start[c], ie = start[c + 1];
     const int jb = start[nc], je = start[nc + 1];
@@ -362,7 +376,29 @@ int pairlist_alloc(Ctx* c, int32_t stride) {
     return SMD_OK;
 }
 
-static int launch_list(Ctx* c, const int* gate) {
+// The molecule rewrap alone, for the brute-force list of the MD step.
+__global__ __launch_bounds__(256) void k_rebuild_prep(int nmol, const int32_t* __restrict__ mol_start, double* pos,
+                                                      Box box, RebuildPrep pr, const int* gate) {
+    if (gate && !*gate) return;
+    const int m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= nmol) return;
+    const int s = mol_start[m], e = mol_start[m + 1];
+    if (m == 0) pr.scalars[2] += 1.0;
+    for (int i = s; i < e; ++i) pr.disp[3 * i] = pr.disp[3 * i + 1] = pr.disp[3 * i + 2] = 0.0;
+    const double c0x = pr.com[3 * m], c0y = pr.com[3 * m + 1], c0z = pr.com[3 * m + 2];
+    double cx = c0x, cy = c0y, cz = c0z;
+    pbc(cx, cy, cz, box);
+    const double dx = cx - c0x, dy = cy - c0y, dz = cz - c0z;
+    pr.com[3 * m] = cx; pr.com[3 * m + 1] = cy; pr.com[3 * m + 2] = cz;
+    for (int i = s; i < e; ++i) { pos[3 * i] += dx; pos[3 * i + 1] += dy; pos[3 * i + 2] += dz; }
+}
+
+static int launch_list(Ctx* c, const int* gate, bool md_prep) {
+    RebuildPrep pr{};
+    pr.on = md_prep ? 1 : 0;
+    pr.com = c->d_com;
+    pr.disp = c->d_disp;
+    pr.scalars = c->d_scalars;
     ListArgs a;
     a.n = c->n;
     a.pos = c->d_pos;
@@ -390,22 +426,24 @@ static int launch_list(Ctx* c, const int* gate) {
         g.nxy = g.n[0] * g.n[1];
         g.ncell = g.nxy * g.n[2];
         const int nc = g.ncell;
-        const dim3 ga((c->n + 1023) / 1024);
+        const dim3 ga((c->nmol + 1023) / 1024);
         if (nc <= 8192)
-            hipLaunchKernelGGL((k_cell_assign<true>), ga, dim3(1024), sizeof(int32_t) * nc, c->stream, c->n, c->d_pos, g,
-                               c->d_cell_of, c->d_cell_cnt, gate);
+            hipLaunchKernelGGL((k_cell_assign<true>), ga, dim3(1024), sizeof(int32_t) * nc, c->stream, c->nmol,
+                               c->d_mol_start, c->d_pos, g, c->box, pr, c->d_cell_of, c->d_cell_cnt, gate);
         else
-            hipLaunchKernelGGL((k_cell_assign<false>), ga, dim3(1024), 0, c->stream, c->n, c->d_pos, g, c->d_cell_of,
-                               c->d_cell_cnt, gate);
-        hipLaunchKernelGGL(k_cell_scan, dim3(1), dim3(1024), 0, c->stream, nc, c->d_cell_cnt, c->d_cell_start, gate);
-        hipLaunchKernelGGL(k_cell_fill, dim3(nc), dim3(256), 0, c->stream, c->n, nc, c->d_cell_of, c->d_cell_start,
-                           c->d_cell_atoms, gate);
+            hipLaunchKernelGGL((k_cell_assign<false>), ga, dim3(1024), 0, c->stream, c->nmol, c->d_mol_start, c->d_pos,
+                               g, c->box, pr, c->d_cell_of, c->d_cell_cnt, gate);
+        hipLaunchKernelGGL(k_cell_fill, dim3(nc), dim3(256), 0, c->stream, c->n, nc, c->d_cell_of, c->d_cell_cnt,
+                           c->d_cell_start, c->d_cell_atoms, gate);
         const dim3 gl(nc, 27);
         hipLaunchKernelGGL((k_list_cells<false>), gl, dim3(LB_THREADS), 0, c->stream, a, g, c->d_cell_start,
-                           c->d_cell_atoms, c->d_list_seg, gate);
+                           c->d_cell_atoms, c->d_list_seg, c->d_cell_cnt, gate);
         hipLaunchKernelGGL((k_list_cells<true>), gl, dim3(LB_THREADS), 0, c->stream, a, g, c->d_cell_start,
-                           c->d_cell_atoms, c->d_list_seg, gate);
+                           c->d_cell_atoms, c->d_list_seg, c->d_cell_cnt, gate);
     } else {
+        if (md_prep)
+            hipLaunchKernelGGL(k_rebuild_prep, dim3((c->nmol + 255) / 256), dim3(256), 0, c->stream, c->nmol,
+                               c->d_mol_start, c->d_pos, c->box, pr, gate);
         hipLaunchKernelGGL(k_pairs_brute, grid, dim3(WAVE * waves_per_block), 0, c->stream, a, gate);
     }
     SMD_HIP(c, hipGetLastError());
@@ -413,10 +451,11 @@ static int launch_list(Ctx* c, const int* gate) {
 }
 
 // Builds the list.  With on_device_flag the kernels are gated on d_flag[0]
-// (used inside the MD step, where the Verlet trigger is evaluated on the
-// device); the grid must then already be configured.
+// and begin with the MD step's rebuild preamble (used inside the MD step,
+// where the Verlet trigger is evaluated on the device); the grid must then
+// already be configured.
 int pairlist_build(Ctx* c, double cutoff, double cut_buff, int use_cells, bool on_device_flag) {
-    if (on_device_flag) return launch_list(c, c->d_flag);
+    if (on_device_flag) return launch_list(c, c->d_flag, true);
     if (!c->box_set) return fail(c, SMD_ESTATE, "box not set");
     c->list_cutoff = cutoff;
     c->list_buff = cut_buff;
@@ -454,7 +493,7 @@ int pairlist_build(Ctx* c, double cutoff, double cut_buff, int use_cells, bool o
         int rc = pairlist_alloc(c, stride);
         if (rc) return rc;
         SMD_HIP(c, hipMemsetAsync(c->d_status, 0, sizeof(unsigned), c->stream));
-        rc = launch_list(c, nullptr);
+        rc = launch_list(c, nullptr, false);
         if (rc) return rc;
         unsigned st = 0;
         SMD_HIP(c, hipMemcpyAsync(&st, c->d_status, sizeof st, hipMemcpyDeviceToHost, c->stream));

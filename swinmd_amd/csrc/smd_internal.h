@@ -124,7 +124,6 @@ struct Ctx {
     int32_t* d_ksplit = nullptr;  // [RF_WAVES+1] group ranges of the reciprocal force pass
     int n_kgrp = 0;
     double* d_sfac_part = nullptr;  // [chunks][K][10] per-chunk structure-factor sums
-    double* d_ssum = nullptr;       // [K][10] structure-factor sums
     double* d_kf = nullptr;         // [K] (b C, b S) in the force pass's precision
     double* d_e1 = nullptr;         // [n][3] complex exp(i 2 pi r_p / L_p)
     int sfac_chunks = 0;
@@ -259,14 +258,14 @@ int pairlist_build(Ctx* c, double cutoff, double cut_buff, int use_cells, bool o
 int pairlist_alloc(Ctx* c, int32_t stride);
 int vdw_run(Ctx* c, const smd_vdw_params* p, int prec, smd_vdw_out* out, bool read_back);
 int pairs_lj_ewald(Ctx* c, const double* sigma2, const double* eps4, double cutoff, double alpha,
-                   double real_cut, int use_constraint, int lustig, int prec);
+                   double real_cut, int use_constraint, int lustig, int prec, bool store_forces = false);
 int ewald_setup(Ctx* c, const smd_ewald_params* p);
 int ewald_real(Ctx* c, int prec, int lustig);
 int ewald_intra(Ctx* c, int lustig);
 int ewald_surface(Ctx* c);
 int ewald_recip(Ctx* c, int prec, int lustig);
-int shake_run(Ctx* c, const smd_integrator_params* p, bool fnh_on_device, double fnh);
-int kinetics_run(Ctx* c, bool rewrap);
+int shake_run(Ctx* c, const smd_integrator_params* p, bool fnh_on_device, double fnh, bool md_step = false);
+int kinetics_run(Ctx* c, bool rewrap, const smd_integrator_params* md = nullptr);
 int lf_atoms_run(Ctx* c, const smd_integrator_params* p, double fnh);
 int md_step_enqueue(Ctx* c, bool do_long);
 int zero_terms(Ctx* c);

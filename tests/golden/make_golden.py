@@ -102,6 +102,18 @@ CASES = [
      "forces", "buck",
      {"cutoff": 0.8, "cut_buff": 0.05, "switch_on": True, "switch_dist": 0.7, "lustig": True,
       "ewald": False, "coulomb": True}),
+    ("amoeba216_induction_cg",
+     {"builder": "water", "args": {"n_mol": 216, "density_mol": 33.4, "seed": 19, "lattice": [6, 6, 6],
+                                   "r_oh": 0.09572, "hoh_deg": 108.5, "types": "amoeba"}},
+     "induction", "",
+     {"cutoff": 0.6, "cut_buff": 0.01, "induction": True, "ind_damp": True, "ind_min": True, "cutoff_ind": 0.6,
+      "a_damp": 0.39, "ewald": False}),
+    ("amoeba216_induction_direct",
+     {"builder": "water", "args": {"n_mol": 216, "density_mol": 33.4, "seed": 23, "lattice": [6, 6, 6],
+                                   "r_oh": 0.09572, "hoh_deg": 108.5, "types": "amoeba"}},
+     "induction", "",
+     {"cutoff": 0.6, "cut_buff": 0.01, "induction": True, "ind_damp": False, "ind_min": False,
+      "cutoff_ind": 0.55, "a_damp": 0.39, "ewald": False}),
     ("water512_md10",
      {"builder": "water", "args": {"n_mol": 512, "density_mol": 33.4, "seed": 5, "lattice": [8, 8, 8]}},
      "md", "10",

@@ -255,3 +255,71 @@ def test_nh_chain_integrates_the_mtk_equations_over_half_a_step(oracle_mod):
     got2, want2 = _chain_vs_ode(0.001)
     err2 = np.abs(got2 - want2) / np.maximum(np.abs(want2), 1e-3)
     assert err2.max() < err.max() / 8.0
+
+
+# ---- induced dipoles -------------------------------------------------------------
+
+def _two_sites(r, q1=0.0, pol2=0.000837):
+    types = [AtomType(mass=1.0, charge=q1, eps=0.0, sigma=0.0, polar=1e-12),
+             AtomType(mass=1.0, charge=0.0, eps=0.0, sigma=0.0, polar=pol2)]
+    return _atoms(types, [[0.0, 0.0, 0.0], [r, 0.0, 0.0]], [0, 1], box=4.0)
+
+
+def test_point_charge_polarises_a_site_by_alpha_q_over_r2(oracle_mod):
+    """Undamped (a large): the field of charge q at distance r is q/r^2 along
+    the separation, and the isolated site's dipole is alpha times it."""
+    r, q, alpha = 0.3, 0.8, 0.000837
+    s = _two_sites(r, q1=q, pol2=alpha)
+    o = _oracle(s)
+    zero3, zero6 = np.zeros((2, 3)), np.zeros((2, 6))
+    e0, pi, _, _ = o.induced_dipoles(zero3, zero6, cutoff_ind=1.0, a_damp=1e9, damp_tij=True, do_min=True)
+    assert e0[1, 0] == pytest.approx(q / r ** 2, rel=1e-12)
+    assert e0[0] == pytest.approx([0.0, 0.0, 0.0], abs=1e-300)
+    # site 0 has alpha = 1e-12: its induced dipole, and its back-reaction, vanish
+    assert pi[1, 0] == pytest.approx(alpha * q / r ** 2, rel=1e-6)
+    assert abs(pi[1, 1]) + abs(pi[1, 2]) < 1e-12 * abs(pi[1, 0])   # CG stops at |r| < 1e-14
+
+
+def test_permanent_dipole_field_on_axis_is_2p_over_r3(oracle_mod):
+    r, p = 0.35, 0.004
+    s = _two_sites(r)
+    o = _oracle(s)
+    dip = np.array([[p, 0.0, 0.0], [0.0, 0.0, 0.0]])
+    e0, _, _, _ = o.induced_dipoles(dip, np.zeros((2, 6)), cutoff_ind=1.0, a_damp=1e9, damp_tij=True,
+                                    do_min=False)
+    assert e0[1, 0] == pytest.approx(2 * p / r ** 3, rel=1e-12)
+    # perpendicular dipole: field -p/r^3
+    dip = np.array([[0.0, p, 0.0], [0.0, 0.0, 0.0]])
+    e0, _, _, _ = o.induced_dipoles(dip, np.zeros((2, 6)), cutoff_ind=1.0, a_damp=1e9, damp_tij=True,
+                                    do_min=False)
+    assert e0[1, 1] == pytest.approx(-p / r ** 3, rel=1e-12)
+
+
+def test_induced_dipoles_solve_the_dense_system(oracle_mod):
+    """The CG result solves (diag(1/alpha) - T) mu = E0 with the Thole-damped
+    dipole tensor, assembled here independently from its definition."""
+    from swinmd_amd import systems as S
+    s = S.water(8, density_mol=33.4, seed=4, types=S.AMOEBA_TYPES, r_oh=0.09572, hoh_deg=108.5)
+    o = _oracle(s)
+    dip, quad = o.local2global()
+    a = 0.39
+    e0, pi, _, _ = o.induced_dipoles(dip, quad, cutoff_ind=0.8, a_damp=a, damp_tij=True, do_min=True)
+    n = s.n_atoms
+    pol = o.polarizabilities()
+    A = np.zeros((3 * n, 3 * n))
+    for i in range(n):
+        A[3 * i:3 * i + 3, 3 * i:3 * i + 3] = np.eye(3) / pol[i]
+        for j in range(i + 1, n):
+            d = s.pos[i] - s.pos[j]
+            d -= s.box[0] * np.round(d / s.box[0])
+            r = np.linalg.norm(d)
+            if r > 0.8:
+                continue
+            u3 = r ** 3 / math.sqrt(pol[i] * pol[j])
+            l3 = 1 - math.exp(-a * u3)
+            l5 = 1 - (1 + a * u3) * math.exp(-a * u3)
+            T = 3 * l5 * np.outer(d, d) / r ** 5 - l3 * np.eye(3) / r ** 3
+            A[3 * i:3 * i + 3, 3 * j:3 * j + 3] = -T
+            A[3 * j:3 * j + 3, 3 * i:3 * i + 3] = -T
+    mu = np.linalg.solve(A, e0.ravel())
+    assert pi.ravel() == pytest.approx(mu, rel=1e-9, abs=1e-15)

@@ -115,3 +115,21 @@ def test_md_trajectory(oracle_mod, name):
     assert rel(r["e_long"], fx.meta["e_long"]) <= TOL
     assert rel(0.5 * r["atom_ke2"], fx.meta["e_atomKin"]) <= TOL
     assert rel(0.5 * r["mol_ke2"], fx.meta["e_molKin"]) <= TOL
+
+
+@pytest.mark.parametrize("name", ["amoeba216_induction_cg", "amoeba216_induction_direct"])
+def test_induction_frames_field_and_dipoles(oracle_mod, name):
+    """Local2Global frames, the permanent field E0 and the induced dipoles
+    (CG solve, or alpha E0 without minimisation) against the reference."""
+    fx = Fixture(name)
+    c = fx.cfg
+    o = oracle_mod.Oracle(fx.system)
+    dip, quad = o.local2global()
+    np.testing.assert_array_equal(dip.ravel(), fx.ref["ind_dip"])
+    np.testing.assert_array_equal(quad.ravel(), fx.ref["ind_quad"])
+    e0, pi, iters, res = o.induced_dipoles(dip, quad, cutoff_ind=c["cutoff_ind"], a_damp=c["a_damp"],
+                                           damp_tij=c["ind_damp"], do_min=c["ind_min"])
+    assert rel_rms(e0, fx.vec("ind_ei0")) <= TOL
+    assert rel_rms(pi, fx.vec("ind_pi")) <= TOL
+    if c["ind_min"]:
+        assert iters > 5 and res < 1e-14
