@@ -222,6 +222,42 @@ int smd_mol_kinetics(smd_ctx* ctx, double out2[2]);
 int smd_nh_chain(smd_ctx* ctx, double ekin, double temperature, double dt, double ndf, double press_diff,
                  double volume, double* chain_state, double* scale_out);
 
+/* ---- (d) induced dipoles of polarizable water ------------------------------ */
+/* Local-frame permanent dipoles [3n], quadrupoles [6n] (xx xy xz yy yz zz)
+ * and polarizabilities [n] of every atom: AtomParam dipx..quadzz, polar as
+ * Ensemble::set_mol_system copies them onto the atoms (src/Ensemble.cpp:190-201).
+ * Molecules must be 3-site waters in the order H1, O, H2. */
+int smd_set_multipoles(smd_ctx* ctx, const double* dip_local, const double* quad_local, const double* polar,
+                       int64_t n_atoms);
+
+typedef struct {
+    double cutoff_ind;     /* get_cutoffInd(); must not exceed the pair-list cutoff */
+    double a_damp;         /* get_adamping(): Thole a */
+    int32_t damp_tij;      /* use_damping_induction(): damp T in A */
+    int32_t do_min;        /* do_induction_minimization(): CG, else Pi = alpha E0 */
+    double tol;            /* CG stops when |r| < tol; 0 -> 1e-14 (ConjugateGradient) */
+    int32_t max_iter;      /* 0 -> 3n, the reference's bound */
+    int32_t guess;         /* 0: Pi = 1 (the reference); 1: the previous solution; 2: alpha E0 */
+    int32_t prec;          /* smd_prec; SMD_PREC_FP64 only */
+} smd_induce_params;
+
+typedef struct {
+    int32_t iterations;    /* CG updates performed */
+    int32_t converged;     /* a residual test ended the loop */
+    double residual;       /* |r| after the last update */
+} smd_induce_report;
+
+/* Induction::compute's first steps (src/forces/Induction.cpp:503-507):
+ * Local2Global::FrameofReference (src/utils/Local2Global.cpp:90-225), the
+ * permanent field and A of Induction::InducedDipoles (:181-455) and the
+ * ConjugateGradient solve (:2401-2467), on the current positions and pair
+ * list.  A is applied without being stored. */
+int smd_induce(smd_ctx* ctx, const smd_induce_params* p, smd_induce_report* report);
+/* Lab-frame dipoles [3n] and quadrupoles [6n] (Local2Global::dip, Qxx..Qzz),
+ * the permanent field Ei0 [3n] and the induced dipoles Pi [3n] of the last
+ * smd_induce; any pointer may be NULL. */
+int smd_get_induction(smd_ctx* ctx, double* dip, double* quad, double* e0, double* mu, int64_t n_atoms);
+
 /* ---- whole step: LFIntegrator::run for rigid point-charge water --------- */
 typedef struct {
     double cutoff, cut_buff;

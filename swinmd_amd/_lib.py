@@ -79,6 +79,16 @@ class MdReport(C.Structure):
                 ("mol_ke2", C.c_double), ("fnh", C.c_double), ("virial", C.c_double * 9)]
 
 
+class InduceParams(C.Structure):
+    _fields_ = [("cutoff_ind", C.c_double), ("a_damp", C.c_double), ("damp_tij", C.c_int32),
+                ("do_min", C.c_int32), ("tol", C.c_double), ("max_iter", C.c_int32), ("guess", C.c_int32),
+                ("prec", C.c_int32)]
+
+
+class InduceReport(C.Structure):
+    _fields_ = [("iterations", C.c_int32), ("converged", C.c_int32), ("residual", C.c_double)]
+
+
 # name -> (restype, argtypes); every entry is a symbol declared in include/swinmd_hip.h
 _P = C.c_void_p
 SIGNATURES = {
@@ -121,6 +131,9 @@ SIGNATURES = {
     "smd_profile_enable": (C.c_int, [_P, C.c_int]),
     "smd_profile_get": (C.c_int, [_P, C.c_char_p, C.POINTER(C.c_double), C.POINTER(C.c_int64)]),
     "smd_set_tuning": (C.c_int, [_P, C.c_char_p, C.c_int]),
+    "smd_set_multipoles": (C.c_int, [_P, _P, _P, _P, C.c_int64]),
+    "smd_induce": (C.c_int, [_P, C.POINTER(InduceParams), C.POINTER(InduceReport)]),
+    "smd_get_induction": (C.c_int, [_P, _P, _P, _P, _P, C.c_int64]),
 }
 
 _lib = None

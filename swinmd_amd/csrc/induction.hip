@@ -1,0 +1,551 @@
+// induction.hip -- induced dipoles of polarizable water on gfx950.
+//
+//   Local2Global::FrameofReference  src/utils/Local2Global.cpp:90-225
+//   Induction::InducedDipoles       src/forces/Induction.cpp:181-455
+//   Induction::ConjugateGradient    :2401-2467
+//
+// The reference assembles A = diag(1/alpha) - T as a dense (3N)^2 matrix every
+// step and runs CG on it.  Here A is never stored: each product A v recomputes
+// the Thole-damped dipole tensor of every pair inside cutoffInd from the
+// positions.
+//  * Non-excluded pairs come from the full neighbour rows of the pair list.
+//    Both of a row's segments are used, so a row holds every partner of its
+//    atom.
+//  * Excluded (intramolecular) pairs come from the exclusion table.  A couples
+//    them too; E0 does not.
+//  * The permanent field E0 uses the same rows.
+// CG follows the reference's recurrence:
+//  * Every reduction is a per-block partial followed by a fixed-order sum, so
+//    results do not depend on scheduling.
+//  * The scalar of iteration k is read from one partial buffer while the next
+//    one is written to the other.
+//  * The host launches iterations in batches and stops when the device flag
+//    says the residual test passed.
+#include <algorithm>
+#include <cmath>
+
+#include "smd_internal.h"
+
+namespace smd {
+
+constexpr int IND_LANES = 16;      // lanes per atom row in the field and T.mu kernels
+constexpr int CG_THREADS = 256;    // threads per block of the vector kernels
+constexpr int CG_BATCH = 8;        // CG iterations launched between host checks
+constexpr double SQRT_COULOMB = 11.78708976;   // SQRTCOULOMBCONSTANT, src/NEMD_defs.h:103
+constexpr double POLMIN = 0.00000001;          // Induction.cpp:195
+
+struct IndArgs {
+    int n, nmol, stride;
+    Box box;
+    const int32_t* mol_start;
+    const double* pos;
+    const double* q;            // scaled charges
+    const int32_t* nb;
+    const int32_t* nb_cnt;
+    const int32_t* nb_own;
+    const int32_t* excl_off;
+    const int32_t* excl;
+    const double* polar;
+    const double* dipl;         // local-frame dipoles [3n]
+    const double* quadl;        // local-frame quadrupoles [6n] xx xy xz yy yz zz
+    double* dip;                // lab frame [3n]
+    double* quad;               // lab frame, traced / 3 [6n]
+    double* e0;                 // [3n]
+    double cut2, a;
+    int damp;
+};
+
+// ---- frames -------------------------------------------------------------------
+
+struct d3 { double x, y, z; };
+__device__ __forceinline__ d3 ld3(const double* p) { return {p[0], p[1], p[2]}; }
+__device__ __forceinline__ d3 sub3(d3 a, d3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+__device__ __forceinline__ d3 add3(d3 a, d3 b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+__device__ __forceinline__ d3 div3(d3 a, double f) { return {a.x / f, a.y / f, a.z / f}; }
+__device__ __forceinline__ d3 scl3(double f, d3 a) { return {f * a.x, f * a.y, f * a.z}; }
+__device__ __forceinline__ double dot3(d3 a, d3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+__device__ __forceinline__ double len3(d3 a) { return sqrt(a.x * a.x + a.y * a.y + a.z * a.z); }
+
+// One thread per water (sites H1, O, H2).
+__global__ __launch_bounds__(256) void k_frames(IndArgs a) {
+    const int m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= a.nmol) return;
+    const int a0 = a.mol_start[m];
+    const d3 h1 = ld3(a.pos + 3 * a0), o = ld3(a.pos + 3 * (a0 + 1)), h2 = ld3(a.pos + 3 * (a0 + 2));
+    const d3 ex{1.0, 0.0, 0.0}, ey{0.0, 1.0, 0.0};
+    d3 p1[3], p2[3];
+    // hydrogens: axis to O, second vector x (y when the axis is within 30 deg of x)
+    p1[0] = sub3(o, h1);
+    p2[0] = fabs(p1[0].x / len3(p1[0])) > 0.8660 ? ey : ex;
+    // oxygen: sum of the unit vectors to the hydrogens, second vector to H2
+    p1[1] = div3(sub3(h1, o), len3(sub3(h1, o)));
+    p2[1] = div3(sub3(h2, o), len3(sub3(h2, o)));
+    p1[1] = add3(p1[1], p2[1]);
+    p1[2] = sub3(o, h2);
+    p2[2] = fabs(p1[2].x / len3(p1[2])) > 0.8660 ? ey : ex;
+#pragma unroll
+    for (int s = 0; s < 3; ++s) {
+        const d3 w = div3(p1[s], len3(p1[s]));
+        const d3 ss = sub3(p2[s], scl3(dot3(p2[s], w), w));
+        const d3 u = div3(ss, len3(ss));
+        const d3 v{w.y * u.z - u.y * w.z, u.x * w.z - w.x * u.z, w.x * u.y - u.x * w.y};
+        const int id = a0 + s;
+        const double dx = a.dipl[3 * id], dy = a.dipl[3 * id + 1], dz = a.dipl[3 * id + 2];
+        a.dip[3 * id] = u.x * dx + v.x * dy + w.x * dz;
+        a.dip[3 * id + 1] = u.y * dx + v.y * dy + w.y * dz;
+        a.dip[3 * id + 2] = u.z * dx + v.z * dy + w.z * dz;
+        const double* ql = a.quadl + 6 * id;
+        const double qxx = ql[0], qxy = ql[1], qxz = ql[2], qyy = ql[3], qyz = ql[4], qzz = ql[5];
+        const double R[3][3] = {{u.x, v.x, w.x}, {u.y, v.y, w.y}, {u.z, v.z, w.z}};
+        const int ab[6][2] = {{0, 0}, {0, 1}, {0, 2}, {1, 1}, {1, 2}, {2, 2}};
+#pragma unroll
+        for (int k = 0; k < 6; ++k) {
+            const double* ra = R[ab[k][0]];
+            const double* rb = R[ab[k][1]];
+            const double qk = ra[0] * (rb[0] * qxx + rb[1] * qxy + rb[2] * qxz) +
+                              ra[1] * (rb[0] * qxy + rb[1] * qyy + rb[2] * qyz) +
+                              ra[2] * (rb[0] * qxz + rb[1] * qyz + rb[2] * qzz);
+            a.quad[6 * id + k] = qk / 3;   // traced form (Local2Global.cpp:210-215)
+        }
+    }
+}
+
+// ---- pair geometry --------------------------------------------------------------
+
+// Thole factors lambda3, lambda5, lambda7 and r^-3, r^-5, r^-7 (:296-313).
+struct Thole { double l3, l5, l7, r3, r5, r7; };
+
+__device__ __forceinline__ bool thole(const IndArgs& a, double rij2, double r, double pi, double pj, Thole& t) {
+    if (!(rij2 <= a.cut2)) return false;
+    const double rij_1 = 1.0 / r;
+    t.r3 = rij_1 * rij_1 * rij_1;
+    t.r5 = t.r3 * rij_1 * rij_1;
+    t.r7 = t.r5 * rij_1 * rij_1;
+    const double u3 = rij2 * r / (sqrt(pi * pj));
+    const double e = exp(-a.a * u3);
+    t.l3 = 1.0 - e;
+    t.l5 = 1.0 - (1.0 + a.a * u3) * e;
+    t.l7 = 1.0 - (1.0 + a.a * u3 + 3.0 * a.a * a.a * u3 * u3 / 5.0) * e;
+    return true;
+}
+
+// Entry k of row i's partners: the front segment, then the back segment.
+__device__ __forceinline__ int row_partner(const IndArgs& a, int i, int own, int k) {
+    const int32_t* row = a.nb + (int64_t)i * a.stride;
+    return k < own ? row[k] : row[a.stride - 1 - (k - own)];
+}
+
+// ---- permanent field E0 --------------------------------------------------------
+
+// Field at i of site j's charge, dipole and quadrupole with r = r_i - r_j: the
+// reference's b_i of a pair (:423-425; b_j is the same with the roles
+// swapped, :430-432).
+__device__ __forceinline__ void field_from(const IndArgs& a, int j, const double r[3], const Thole& t, double f[3]) {
+    const double qj = a.q[j] / SQRT_COULOMB;
+    const double* dj = a.dip + 3 * j;
+    const double* Q = a.quad + 6 * j;
+    const double Qxx = Q[0], Qxy = Q[1], Qxz = Q[2], Qyy = Q[3], Qyz = Q[4], Qzz = Q[5];
+    double RM[3], QR[3];
+#pragma unroll
+    for (int x = 0; x < 3; ++x) RM[x] = r[x] * r[0] * dj[0] + r[x] * r[1] * dj[1] + r[x] * r[2] * dj[2];
+    QR[0] = Qxx * r[0] + Qxy * r[1] + Qxz * r[2];
+    QR[1] = Qxy * r[0] + Qyy * r[1] + Qyz * r[2];
+    QR[2] = Qxz * r[0] + Qyz * r[1] + Qzz * r[2];
+    const double QI = Qxx + Qyy + Qzz;
+    const double QRR = Qxx * r[0] * r[0] + Qxy * r[0] * r[1] + Qxz * r[0] * r[2] + Qxy * r[1] * r[0] + Qyy * r[1] * r[1] +
+                       Qyz * r[1] * r[2] + Qxz * r[2] * r[0] + Qyz * r[2] * r[1] + Qzz * r[2] * r[2];
+#pragma unroll
+    for (int x = 0; x < 3; ++x)
+        f[x] += qj * t.l3 * t.r3 * r[x] + 3.0 * t.l5 * t.r5 * RM[x] - t.l3 * t.r3 * dj[x] +
+                15.0 * t.l7 * t.r7 * QRR * r[x] - 6.0 * t.l5 * t.r5 * QR[x] - 3.0 * t.l5 * t.r5 * QI * r[x];
+}
+
+template <int G>
+__global__ __launch_bounds__(256) void k_field(IndArgs a) {
+    const int tid = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = tid / G, sub = tid % G;
+    double f[3] = {0.0, 0.0, 0.0};
+    if (i < a.n) {
+        const int own = min(a.nb_own[i], a.stride);
+        const int cnt = min(a.nb_cnt[i], a.stride);
+        const double xi = a.pos[3 * i], yi = a.pos[3 * i + 1], zi = a.pos[3 * i + 2];
+        const double pi = a.polar[i] > POLMIN ? a.polar[i] : POLMIN;
+        for (int k = sub; k < cnt; k += G) {
+            const int j = row_partner(a, i, own, k);
+            double dx = xi - a.pos[3 * j], dy = yi - a.pos[3 * j + 1], dz = zi - a.pos[3 * j + 2];
+            pbc_near(dx, dy, dz, a.box);
+            const double r = sqrt(dx * dx + dy * dy + dz * dz);
+            const double pj = a.polar[j] > POLMIN ? a.polar[j] : POLMIN;
+            Thole t;
+            if (!thole(a, r * r, r, pi, pj, t)) continue;
+            const double rr[3] = {dx, dy, dz};
+            field_from(a, j, rr, t, f);
+        }
+    }
+#pragma unroll
+    for (int o = G / 2; o > 0; o >>= 1)
+#pragma unroll
+        for (int x = 0; x < 3; ++x) f[x] += __shfl_xor(f[x], o, G);
+    if (i < a.n && sub == 0)
+        for (int x = 0; x < 3; ++x) a.e0[3 * i + x] = f[x];
+}
+
+// ---- A v without A ------------------------------------------------------------
+
+// (A v)_i = v_i / alpha_i - sum_j T_ij v_j over every pair inside cutoffInd
+// (Induction.cpp:248-256, :309-361).  With a dot output, each block also
+// stores its partial of dot . (A v).
+__device__ __forceinline__ void tmu_pair(const IndArgs& a, double dx, double dy, double dz, double pi, int j,
+                                         const double* v, double acc[3]) {
+    const double r = sqrt(dx * dx + dy * dy + dz * dz);
+    const double pj = a.polar[j] > POLMIN ? a.polar[j] : POLMIN;
+    Thole t;
+    if (!thole(a, r * r, r, pi, pj, t)) return;
+    const double rr[3] = {dx, dy, dz};
+    const double vj[3] = {v[3 * j], v[3 * j + 1], v[3 * j + 2]};
+#pragma unroll
+    for (int x = 0; x < 3; ++x) {
+        double s = 0.0;
+#pragma unroll
+        for (int y = 0; y < 3; ++y) {
+            double T = a.damp ? t.l5 * 3.0 * t.r5 * rr[x] * rr[y] : 3.0 * t.r5 * rr[x] * rr[y];
+            if (x == y) T = a.damp ? T - t.l3 * t.r3 : T - t.r3;
+            s += T * vj[y];
+        }
+        acc[x] += s;
+    }
+}
+
+template <int G>
+__global__ __launch_bounds__(256) void k_apply_a(IndArgs a, const double* __restrict__ v, double* __restrict__ out,
+                                                 const double* __restrict__ dotv, double* part, const int* done) {
+    if (done && *done) return;
+    __shared__ double red[256 / WAVE];
+    const int tid = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = tid / G, sub = tid % G;
+    double acc[3] = {0.0, 0.0, 0.0};
+    double pi = 1.0;
+    if (i < a.n) {
+        const int own = min(a.nb_own[i], a.stride);
+        const int cnt = min(a.nb_cnt[i], a.stride);
+        const double xi = a.pos[3 * i], yi = a.pos[3 * i + 1], zi = a.pos[3 * i + 2];
+        pi = a.polar[i] > POLMIN ? a.polar[i] : POLMIN;
+        for (int k = sub; k < cnt; k += G) {
+            const int j = row_partner(a, i, own, k);
+            double dx = xi - a.pos[3 * j], dy = yi - a.pos[3 * j + 1], dz = zi - a.pos[3 * j + 2];
+            pbc_near(dx, dy, dz, a.box);
+            tmu_pair(a, dx, dy, dz, pi, j, v, acc);
+        }
+        // excluded partners are coupled in A as well (:263-361)
+        for (int k = a.excl_off[i] + sub; k < a.excl_off[i + 1]; k += G) {
+            const int j = a.excl[k];
+            double dx = xi - a.pos[3 * j], dy = yi - a.pos[3 * j + 1], dz = zi - a.pos[3 * j + 2];
+            pbc_near(dx, dy, dz, a.box);
+            tmu_pair(a, dx, dy, dz, pi, j, v, acc);
+        }
+    }
+#pragma unroll
+    for (int o = G / 2; o > 0; o >>= 1)
+#pragma unroll
+        for (int x = 0; x < 3; ++x) acc[x] += __shfl_xor(acc[x], o, G);
+    double d = 0.0;
+    if (i < a.n && sub == 0) {
+#pragma unroll
+        for (int x = 0; x < 3; ++x) {
+            const double z = v[3 * i + x] / pi - acc[x];
+            out[3 * i + x] = z;
+            if (dotv) d += dotv[3 * i + x] * z;
+        }
+    }
+    if (part) {
+        d = wave_sum(d);
+        if (threadIdx.x % WAVE == 0) red[threadIdx.x / WAVE] = d;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            double s = 0.0;
+            for (int w = 0; w < (int)(blockDim.x / WAVE); ++w) s += red[w];
+            part[blockIdx.x] = s;
+        }
+    }
+}
+
+// ---- CG -------------------------------------------------------------------------
+
+// Buffers of block partials, each [max_parts]: VZ (v.Av), DV (v.v), and
+// the ping-pong pair RR[0], RR[1] holding r.r of the current and next
+// iteration.
+struct CgArgs {
+    int dim;          // 3n
+    int nparts_mv;    // blocks of k_apply_a
+    int nparts_vec;   // blocks of the vector kernels
+    double tol;
+    double* x;
+    double* r;
+    double* v;
+    double* z;
+    const double* e0;
+    const double* polar;
+    double* p_vz;
+    double* p_dv;
+    double* p_rr0;
+    double* p_rr1;
+    int* state;       // [0] done, [1] iterations, [2] stop reason
+    double* scal;     // [0] last residual norm
+};
+
+// Sum of n partials in a fixed order: lane-strided sums, then a fixed shuffle
+// tree.  Every lane of the calling wave gets the total.
+__device__ __forceinline__ double sum_parts(const double* p, int n) {
+    const int lane = threadIdx.x % WAVE;
+    double s = 0.0;
+    for (int k = lane; k < n; k += WAVE) s += p[k];
+    return wave_sum(s);
+}
+
+__device__ __forceinline__ void block_partial(double v, double* part) {
+    __shared__ double red[CG_THREADS / WAVE];
+    v = wave_sum(v);
+    if (threadIdx.x % WAVE == 0) red[threadIdx.x / WAVE] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double s = 0.0;
+        for (int w = 0; w < (int)(blockDim.x / WAVE); ++w) s += red[w];
+        part[blockIdx.x] = s;
+    }
+}
+
+// x = guess (1 as the reference, or the caller's), then after z = A x:
+// r = E0 - z, v = r; partials of r.r (c) and v.v (dotV).
+__global__ __launch_bounds__(CG_THREADS) void k_cg_start(CgArgs g) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    double rr = 0.0;
+    if (k < g.dim) {
+        const double r = g.e0[k] - g.z[k];
+        g.r[k] = r;
+        g.v[k] = r;
+        rr = r * r;
+    }
+    block_partial(rr, g.p_rr0);
+    if (blockIdx.x == 0 && threadIdx.x == 0) { g.state[0] = 0; g.state[1] = 0; g.state[2] = 0; }
+    // dotV equals c at the start: the same partials
+    if (threadIdx.x == 0) g.p_dv[blockIdx.x] = g.p_rr0[blockIdx.x];
+}
+
+// Loop top of iteration k: stop when |v| < tol (:2431-2434).
+__global__ void k_cg_check_v(CgArgs g) {
+    if (g.state[0]) return;
+    const double dotV = sum_parts(g.p_dv, g.nparts_vec);
+    if (threadIdx.x == 0 && sqrt(dotV) < g.tol) { g.state[0] = 1; g.state[2] = 1; }
+}
+
+// t = c / v.Av; x += t v; r -= t z; partials of d = r.r (:2446-2454).
+__global__ __launch_bounds__(CG_THREADS) void k_cg_step(CgArgs g, int it) {
+    if (g.state[0]) return;
+    const double* rr_c = (it & 1) ? g.p_rr1 : g.p_rr0;
+    double* rr_d = (it & 1) ? g.p_rr0 : g.p_rr1;
+    const double dotVZ = sum_parts(g.p_vz, g.nparts_mv);
+    const double c = sum_parts(rr_c, g.nparts_vec);
+    const double t = c / dotVZ;
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    double d = 0.0;
+    if (k < g.dim) {
+        g.x[k] += t * g.v[k];
+        const double r = g.r[k] - t * g.z[k];
+        g.r[k] = r;
+        d = r * r;
+    }
+    block_partial(d, rr_d);
+    if (blockIdx.x == 0 && threadIdx.x == 0) g.state[1] = it + 1;
+}
+
+// Stop when |r| < tol (:2456-2457); else v = r + (d/c) v and partials of v.v.
+__global__ __launch_bounds__(CG_THREADS) void k_cg_dir(CgArgs g, int it) {
+    if (g.state[0]) return;
+    const double* rr_c = (it & 1) ? g.p_rr1 : g.p_rr0;
+    const double* rr_d = (it & 1) ? g.p_rr0 : g.p_rr1;
+    const double d = sum_parts(rr_d, g.nparts_vec);
+    if (blockIdx.x == 0 && threadIdx.x == 0) g.scal[0] = sqrt(d);
+    if (sqrt(d) < g.tol) {
+        if (blockIdx.x == 0 && threadIdx.x == 0) { g.state[0] = 1; g.state[2] = 2; }
+        return;
+    }
+    const double c = sum_parts(rr_c, g.nparts_vec);
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    double dv = 0.0;
+    if (k < g.dim) {
+        const double v = g.r[k] + (d / c) * g.v[k];
+        g.v[k] = v;
+        dv = v * v;
+    }
+    block_partial(dv, g.p_dv);
+}
+
+// Without minimisation: Pi = alpha E0 (:446-453).
+__global__ __launch_bounds__(256) void k_direct(int dim, const double* __restrict__ polar, const double* __restrict__ e0,
+                                                double* __restrict__ x) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < dim) x[k] = polar[k / 3] * e0[k];
+}
+
+__global__ __launch_bounds__(256) void k_fill(int dim, double val, double* __restrict__ x) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < dim) x[k] = val;
+}
+
+static IndArgs ind_args(Ctx* c, const smd_induce_params* p) {
+    IndArgs a{};
+    a.n = c->n;
+    a.nmol = c->nmol;
+    a.stride = c->stride;
+    a.box = c->box;
+    a.mol_start = c->d_mol_start;
+    a.pos = c->d_pos;
+    a.q = c->d_q;
+    a.nb = c->d_nb;
+    a.nb_cnt = c->d_nb_cnt;
+    a.nb_own = c->d_nb_own;
+    a.excl_off = c->d_excl_off;
+    a.excl = c->d_excl;
+    a.polar = c->d_polar;
+    a.dipl = c->d_dipl;
+    a.quadl = c->d_quadl;
+    a.dip = c->d_dipg;
+    a.quad = c->d_quadg;
+    a.e0 = c->d_e0;
+    a.cut2 = p->cutoff_ind * p->cutoff_ind;
+    a.a = p->a_damp;
+    a.damp = p->damp_tij;
+    return a;
+}
+
+int induce_run(Ctx* c, const smd_induce_params* p, smd_induce_report* rep) {
+    if (!c->multipoles_set) return fail(c, SMD_ESTATE, "multipoles not set");
+    if (!c->list_valid) return fail(c, SMD_ESTATE, "no pair list");
+    if (!(p->cutoff_ind > 0)) return fail(c, SMD_EINVAL, "cutoff_ind must be positive");
+    if (p->cutoff_ind > c->list_cutoff)
+        return fail(c, SMD_ENOTSUP, "cutoff_ind %g exceeds the pair-list cutoff %g", p->cutoff_ind, c->list_cutoff);
+    if (p->prec != SMD_PREC_FP64) return fail(c, SMD_ENOTSUP, "induced dipoles are FP64 only");
+    for (int m = 0; m < c->nmol; ++m)
+        if (c->h_mol_start[m + 1] - c->h_mol_start[m] != 3)
+            return fail(c, SMD_ENOTSUP, "induced dipoles are implemented for 3-site water");
+    IndArgs a = ind_args(c, p);
+    const int n = c->n, dim = 3 * n;
+    const dim3 b256(256);
+    const dim3 grid_rows((unsigned)(((int64_t)n * IND_LANES + 255) / 256));
+    const dim3 grid_vec((dim + CG_THREADS - 1) / CG_THREADS);
+    {
+        ProfScope ps(c, "ind_field");
+        hipLaunchKernelGGL(k_frames, dim3((c->nmol + 255) / 256), b256, 0, c->stream, a);
+        hipLaunchKernelGGL((k_field<IND_LANES>), grid_rows, b256, 0, c->stream, a);
+    }
+    int iters = 0, reason = 0;
+    double res = 0.0;
+    if (!p->do_min) {
+        hipLaunchKernelGGL(k_direct, grid_vec, b256, 0, c->stream, dim, c->d_polar, c->d_e0, c->d_mu);
+    } else {
+        CgArgs g{};
+        g.dim = dim;
+        g.nparts_mv = (int)grid_rows.x;
+        g.nparts_vec = (int)grid_vec.x;
+        g.tol = p->tol > 0 ? p->tol : 1.0e-14;   // ConjugateGradient's tolerance (:2404)
+        g.x = c->d_mu;
+        g.r = c->d_cg + 0 * (size_t)dim;
+        g.v = c->d_cg + 1 * (size_t)dim;
+        g.z = c->d_cg + 2 * (size_t)dim;
+        g.e0 = c->d_e0;
+        g.polar = c->d_polar;
+        g.p_vz = c->d_cg_part;
+        g.p_dv = c->d_cg_part + c->cg_parts;
+        g.p_rr0 = c->d_cg_part + 2 * c->cg_parts;
+        g.p_rr1 = c->d_cg_part + 3 * c->cg_parts;
+        g.state = c->d_cg_state;
+        g.scal = c->d_cg_scal;
+        const int max_it = p->max_iter > 0 ? p->max_iter : dim;   // the reference's loop bound
+        ProfScope ps(c, "ind_cg");
+        if (p->guess == 0) hipLaunchKernelGGL(k_fill, grid_vec, b256, 0, c->stream, dim, 1.0, g.x);
+        else if (p->guess == 2) hipLaunchKernelGGL(k_direct, grid_vec, b256, 0, c->stream, dim, c->d_polar, c->d_e0, g.x);
+        // guess 1: keep the previous solution in d_mu
+        hipLaunchKernelGGL((k_apply_a<IND_LANES>), grid_rows, b256, 0, c->stream, a, g.x, g.z, nullptr, nullptr, nullptr);
+        hipLaunchKernelGGL(k_cg_start, grid_vec, dim3(CG_THREADS), 0, c->stream, g);
+        for (int it0 = 0; it0 < max_it; it0 += CG_BATCH) {
+            const int it1 = std::min(max_it, it0 + CG_BATCH);
+            for (int it = it0; it < it1; ++it) {
+                hipLaunchKernelGGL(k_cg_check_v, dim3(1), dim3(WAVE), 0, c->stream, g);
+                hipLaunchKernelGGL((k_apply_a<IND_LANES>), grid_rows, b256, 0, c->stream, a, g.v, g.z, g.v, g.p_vz,
+                                   g.state);
+                hipLaunchKernelGGL(k_cg_step, grid_vec, dim3(CG_THREADS), 0, c->stream, g, it);
+                hipLaunchKernelGGL(k_cg_dir, grid_vec, dim3(CG_THREADS), 0, c->stream, g, it);
+            }
+            int st[3];
+            SMD_HIP(c, hipMemcpyAsync(st, c->d_cg_state, sizeof st, hipMemcpyDeviceToHost, c->stream));
+            SMD_HIP(c, hipStreamSynchronize(c->stream));
+            if (st[0]) break;
+        }
+        int st[3];
+        SMD_HIP(c, hipMemcpyAsync(st, c->d_cg_state, sizeof st, hipMemcpyDeviceToHost, c->stream));
+        SMD_HIP(c, hipMemcpyAsync(&res, c->d_cg_scal, sizeof res, hipMemcpyDeviceToHost, c->stream));
+        SMD_HIP(c, hipStreamSynchronize(c->stream));
+        iters = st[1];
+        reason = st[2];
+    }
+    SMD_HIP(c, hipGetLastError());
+    SMD_HIP(c, hipStreamSynchronize(c->stream));
+    if (rep) {
+        rep->iterations = iters;
+        rep->residual = res;
+        rep->converged = p->do_min ? (reason != 0) : 1;
+    }
+    return SMD_OK;
+}
+
+}  // namespace smd
+
+using namespace smd;
+
+extern "C" int smd_set_multipoles(smd_ctx* c, const double* dip_local, const double* quad_local, const double* polar,
+                                  int64_t n_atoms) {
+    if (!c || !dip_local || !quad_local || !polar) return SMD_EINVAL;
+    if (n_atoms != c->n) return fail(c, SMD_EINVAL, "count %lld != %d", (long long)n_atoms, c->n);
+    const int n = c->n;
+    if (!c->d_dipl) {
+        const int parts = std::max((int)(((int64_t)n * IND_LANES + 255) / 256), (3 * n + CG_THREADS - 1) / CG_THREADS);
+        auto A = [&](auto*& p, size_t bytes) -> int {
+            if (hipMalloc(&p, bytes) != hipSuccess) return fail(c, SMD_ENOMEM, "induction allocation of %zu bytes", bytes);
+            return SMD_OK;
+        };
+        int rc;
+        if ((rc = A(c->d_dipl, sizeof(double) * 3 * n)) || (rc = A(c->d_quadl, sizeof(double) * 6 * n)) ||
+            (rc = A(c->d_polar, sizeof(double) * n)) || (rc = A(c->d_dipg, sizeof(double) * 3 * n)) ||
+            (rc = A(c->d_quadg, sizeof(double) * 6 * n)) || (rc = A(c->d_e0, sizeof(double) * 3 * n)) ||
+            (rc = A(c->d_mu, sizeof(double) * 3 * n)) || (rc = A(c->d_cg, sizeof(double) * 9 * n)) ||
+            (rc = A(c->d_cg_part, sizeof(double) * 4 * parts)) || (rc = A(c->d_cg_state, sizeof(int) * 4)) ||
+            (rc = A(c->d_cg_scal, sizeof(double) * 4)))
+            return rc;
+        c->cg_parts = parts;
+        SMD_HIP(c, hipMemset(c->d_mu, 0, sizeof(double) * 3 * n));
+    }
+    SMD_HIP(c, hipStreamSynchronize(c->stream));
+    SMD_HIP(c, hipMemcpy(c->d_dipl, dip_local, sizeof(double) * 3 * n, hipMemcpyHostToDevice));
+    SMD_HIP(c, hipMemcpy(c->d_quadl, quad_local, sizeof(double) * 6 * n, hipMemcpyHostToDevice));
+    SMD_HIP(c, hipMemcpy(c->d_polar, polar, sizeof(double) * n, hipMemcpyHostToDevice));
+    c->multipoles_set = true;
+    return SMD_OK;
+}
+
+extern "C" int smd_induce(smd_ctx* c, const smd_induce_params* p, smd_induce_report* rep) {
+    if (!c || !p) return SMD_EINVAL;
+    return induce_run(c, p, rep);
+}
+
+extern "C" int smd_get_induction(smd_ctx* c, double* dip, double* quad, double* e0, double* mu, int64_t n_atoms) {
+    if (!c) return SMD_EINVAL;
+    if (!c->multipoles_set) return fail(c, SMD_ESTATE, "multipoles not set");
+    if (n_atoms != c->n) return fail(c, SMD_EINVAL, "count %lld != %d", (long long)n_atoms, c->n);
+    const size_t n = (size_t)c->n;
+    SMD_HIP(c, hipStreamSynchronize(c->stream));
+    if (dip) SMD_HIP(c, hipMemcpy(dip, c->d_dipg, sizeof(double) * 3 * n, hipMemcpyDeviceToHost));
+    if (quad) SMD_HIP(c, hipMemcpy(quad, c->d_quadg, sizeof(double) * 6 * n, hipMemcpyDeviceToHost));
+    if (e0) SMD_HIP(c, hipMemcpy(e0, c->d_e0, sizeof(double) * 3 * n, hipMemcpyDeviceToHost));
+    if (mu) SMD_HIP(c, hipMemcpy(mu, c->d_mu, sizeof(double) * 3 * n, hipMemcpyDeviceToHost));
+    return SMD_OK;
+}

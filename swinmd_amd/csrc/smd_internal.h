@@ -140,6 +140,21 @@ struct Ctx {
     int32_t* d_host = nullptr;    // [n]
     double* d_fvirt = nullptr;    // [3n] force on each (virtual) site before redistribution
 
+    // induced dipoles
+    bool multipoles_set = false;
+    double* d_dipl = nullptr;     // [3n] local-frame dipoles
+    double* d_quadl = nullptr;    // [6n] local-frame quadrupoles
+    double* d_polar = nullptr;    // [n]
+    double* d_dipg = nullptr;     // [3n] lab-frame dipoles
+    double* d_quadg = nullptr;    // [6n] lab-frame quadrupoles (traced / 3)
+    double* d_e0 = nullptr;       // [3n] permanent field
+    double* d_mu = nullptr;       // [3n] induced dipoles
+    double* d_cg = nullptr;       // [9n] CG r, v, z
+    double* d_cg_part = nullptr;  // [4][cg_parts] block partials
+    int* d_cg_state = nullptr;    // [4] done, iterations, stop reason
+    double* d_cg_scal = nullptr;  // [4] last residual norm
+    int cg_parts = 0;
+
     // MD driver
     bool md_ready = false;
     smd_md_params md{};

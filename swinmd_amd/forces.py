@@ -326,3 +326,31 @@ class WaterMD:
         d = {name: getattr(r, name) for name, _ in L.MdReport._fields_ if name != "virial"}
         d["virial"] = np.array(r.virial[:])
         return d
+
+
+class Induction:
+    """Induction's induced-dipole solve (src/forces/Induction.cpp:181-455,
+    :2401-2467) on the device: frames, permanent field, A mu = E0 by CG."""
+
+    def __init__(self, ctx: Context, *, cutoff_ind, a_damp=0.39, damp_tij=True, do_min=True, tol=0.0,
+                 max_iter=0, guess=0, prec=L.PREC_FP64):
+        self.ctx = ctx
+        s = ctx.system
+        dip = np.ascontiguousarray([s.types[t].dip for t in s.type], dtype=np.float64)
+        quad = np.ascontiguousarray([s.types[t].quad for t in s.type], dtype=np.float64)
+        pol = np.ascontiguousarray([s.types[t].polar for t in s.type], dtype=np.float64)
+        ctx.check(ctx.lib.smd_set_multipoles(ctx.h, _ptr(dip), _ptr(quad), _ptr(pol), ctx.n))
+        self.params = L.InduceParams(cutoff_ind, a_damp, int(damp_tij), int(do_min), tol, max_iter, guess, prec)
+
+    def induce(self):
+        """Runs the solve; returns (iterations, converged, residual)."""
+        r = L.InduceReport()
+        self.ctx.check(self.ctx.lib.smd_induce(self.ctx.h, C.byref(self.params), C.byref(r)))
+        return r.iterations, bool(r.converged), r.residual
+
+    def results(self):
+        """Lab-frame dipoles [n,3], quadrupoles [n,6], permanent field [n,3], induced dipoles [n,3]."""
+        n = self.ctx.n
+        dip, quad, e0, mu = np.zeros((n, 3)), np.zeros((n, 6)), np.zeros((n, 3)), np.zeros((n, 3))
+        self.ctx.check(self.ctx.lib.smd_get_induction(self.ctx.h, _ptr(dip), _ptr(quad), _ptr(e0), _ptr(mu), n))
+        return dip, quad, e0, mu
