@@ -147,6 +147,8 @@ def main():
                     help="MPI ranks of the reference CPU run (the GPU's share of host cores)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-fp64", action="store_true", help="skip the plain FP64 comparison run")
+    ap.add_argument("--no-graph", action="store_true",
+                    help="launch each step's kernels directly instead of replaying a captured hipGraph")
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -165,7 +167,8 @@ def main():
     # equilibrate from the lattice start (mixed precision), then start both
     # precisions from the same coordinates and velocities
     log(f"equilibrating {args.equil} steps")
-    ctx, md = build_md(system, L.PREC_MIXED, system.pos, vel0)
+    use_graph = not args.no_graph
+    ctx, md = build_md(system, L.PREC_MIXED, system.pos, vel0, use_graph=use_graph)
     md.run(args.equil)
     pos_eq, vel_eq = ctx.positions(), ctx.velocities()
     ctx.close()
@@ -174,7 +177,7 @@ def main():
     for prec, name in ((L.PREC_MIXED, "mixed"), (L.PREC_FP64, "fp64")):
         if name == "fp64" and args.no_fp64:
             continue
-        ctx, md = build_md(system, prec, pos_eq, vel_eq)
+        ctx, md = build_md(system, prec, pos_eq, vel_eq, use_graph=use_graph)
         f_start = ctx.forces()
         md.run(args.warmup)
         r0 = md.run(0)
@@ -209,7 +212,8 @@ def main():
         "config": {"workload": "rigid 3-site water MD step (LJ + Ewald + SHAKE + Nose-Hoover), 4096 molecules",
                    "molecules": 4096, "atoms": n, "box_nm": float(system.box[0]), "cutoff_nm": MD["cutoff"],
                    "buffer_nm": MD["cut_buff"], "alpha_per_nm": MD["alpha"], "n2max": N2MAX, "dt_ps": MD["dt"],
-                   "shake_tol": MD["tol"], "thermostat": "Nose-Hoover 300 K", "parallelism": "single GPU"},
+                   "shake_tol": MD["tol"], "thermostat": "Nose-Hoover 300 K", "parallelism": "single GPU",
+                   "step_launch": "hipGraph replay" if use_graph else "direct launches"},
         "ns_per_day": round(ns_day, 2),
         "list_updates_per_step": round(mix["list_updates"] / args.steps, 4),
     }

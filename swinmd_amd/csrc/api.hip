@@ -64,7 +64,7 @@ static void release(Ctx* c) {
     void* ptrs[] = {c->d_type, c->d_mol, c->d_mol_start, c->d_q, c->d_mass, c->d_excl_off, c->d_excl, c->d_pos,
                     c->d_vel, c->d_vtm, c->d_disp, c->d_force, c->d_com, c->d_mom, c->d_bond2, c->d_terms,
                     c->d_status, c->d_flag, c->d_scalars, c->d_cell_of, c->d_cell_cnt, c->d_cell_start,
-                    c->d_cell_atoms, c->d_list_seg, c->d_nb, c->d_nb_cnt, c->d_nb_own, c->d_fvirt, c->d_kvec, c->d_kmod2, c->d_expk, c->d_kn, c->d_kgrp, c->d_ksplit, c->d_kf, c->d_e1,
+                    c->d_cell_atoms, c->d_list_seg, c->d_list_mask, c->d_nb, c->d_nb_cnt, c->d_nb_own, c->d_fvirt, c->d_kvec, c->d_kmod2, c->d_expk, c->d_kn, c->d_kgrp, c->d_ksplit, c->d_kf, c->d_e1,
                     c->d_sfac_part, c->d_vdw_a, c->d_vdw_b, c->d_vdw_c, c->d_epos, c->d_fa, c->d_fb,
                     c->d_host, c->d_dipl, c->d_quadl, c->d_polar, c->d_dipg, c->d_quadg, c->d_e0, c->d_mu,
                     c->d_cg, c->d_cg_part, c->d_cg_state, c->d_cg_scal};

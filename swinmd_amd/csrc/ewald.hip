@@ -10,6 +10,7 @@
 // reference's list come in (nx, ny) rows of consecutive nz, so along a row each
 // term is the previous one times Z.  Rows are cut into groups of at most GK
 // consecutive nz.
+//   pass 0 (k_recip_phase): exp(i 2 pi r_p / L_p) per atom and axis.
 //   pass 1 (k_recip_sfac): a block of SF_WAVES waves owns 64 k-groups (one
 //     per lane) and a chunk of atoms.  Per 64-atom tile the block writes each
 //     atom's powers X^0..X^nmax, Y^0.., Z^0.. to LDS (negative powers are
@@ -291,7 +292,7 @@ struct RecipArgs {
     const int32_t* split;      // [RF_WAVES+1] group ranges of the force pass
     double* part;              // [chunk][K][NSUM]
     void* kf;                  // [K] (b C, b S) as R pairs
-    double2* e1;               // [n][3] exp(i 2 pi r_p / L_p) per atom, written by pass 1
+    double2* e1;               // [n][3] exp(i 2 pi r_p / L_p) per atom (pass 0)
     double* force;
     double* terms;
     double alphaR4, pi4V, pi8V;
@@ -320,6 +321,18 @@ __device__ __forceinline__ cplx<double> cpow(cplx<double> b, int n) {
 
 __device__ __forceinline__ int grp_nx(int v) { return v >> 16; }
 __device__ __forceinline__ int grp_ny(int v) { return (int)(short)(v & 0xffff); }
+
+// Pass 0: exp(i 2 pi r_p / L_p) per atom and axis, used by both passes.
+__global__ __launch_bounds__(256) void k_recip_phase(int n, const double* __restrict__ pos, double vx, double vy,
+                                                     double vz, double2* __restrict__ e1) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= 3 * n) return;
+    const int p = k % 3;
+    const double v = p == 0 ? vx : (p == 1 ? vy : vz);
+    double s, c;
+    sincos(v * pos[k], &s, &c);
+    e1[k] = make_double2(c, s);
+}
 
 // Pass 1.  Sums per k (layout of part): 0 q sin, 1 q cos,
 // 2-4 q cos d, 5-7 q sin d, 8 q cos |d|^2, 9 q sin |d|^2 (reciprocal_term
@@ -356,9 +369,8 @@ __global__ __launch_bounds__(SF_WAVES * WAVE) void k_recip_sfac(RecipArgs a) {
             if (p < 3) {
                 // powers 0..nmax of exp(i 2 pi r_p / L_p), by recurrence in FP64
                 const int base = p == 0 ? 0 : (p == 1 ? ey0 : ez0);
-                double s1, c1;
-                sincos(a.vk[p] * a.pos[3 * ia + p], &s1, &c1);
-                if (blockIdx.x == 0) a.e1[3 * ia + p] = make_double2(c1, s1);
+                const double2 e = a.e1[3 * ia + p];
+                const double c1 = e.x, s1 = e.y;
                 double er = 1.0, ei = 0.0;
                 tab[base * TS + u] = {R(1), R(0)};
                 for (int k = 1; k <= a.nmax[p]; ++k) {
@@ -625,6 +637,8 @@ int ewald_recip(Ctx* c, int prec, int lustig) {
     const int f = mixed ? 1 : 0;
     {
         ProfScope ps(c, "recip_sfac");
+        hipLaunchKernelGGL(k_recip_phase, dim3((3 * c->n + 255) / 256), dim3(256), 0, c->stream, c->n, c->d_pos,
+                           a.vk[0], a.vk[1], a.vk[2], a.e1);
         const dim3 g1((a.ng + WAVE - 1) / WAVE, a.chunks);
         const dim3 b1(SF_WAVES * WAVE);
         // FP64 in both forms: its sums are FP64 FMAs, which cost what FP32

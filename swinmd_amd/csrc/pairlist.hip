@@ -58,7 +58,7 @@ struct RebuildPrep {
 // per-cell counts, aggregated in LDS when the grid fits there.  One thread
 // per molecule, so the MD step's molecule rewrap runs in the same pass.
 template <bool LDS_HIST>
-__global__ __launch_bounds__(1024) void k_cell_assign(int nmol, const int32_t* __restrict__ mol_start, double* pos,
+__global__ __launch_bounds__(128) void k_cell_assign(int nmol, const int32_t* __restrict__ mol_start, double* pos,
                                                       CellGrid g, Box box, RebuildPrep pr,
                                                       int32_t* __restrict__ cell_of, int32_t* __restrict__ cnt,
                                                       const int* gate) {
@@ -105,11 +105,15 @@ __global__ __launch_bounds__(1024) void k_cell_assign(int nmol, const int32_t* _
 // its atoms compacted in index order (set_atom appends in atom order,
 // CellManager.cpp:145-170); it stops once the cell is full.  The counts are
 // cleared later, by the counting pass of the list build.
-__global__ __launch_bounds__(256) void k_cell_fill(int n, int ncell, const int32_t* __restrict__ cell_of,
-                                                   const int32_t* __restrict__ cnt, int32_t* __restrict__ start,
-                                                   int32_t* __restrict__ atoms, const int* gate) {
+constexpr int FILL_THREADS = 1024;
+constexpr int FILL_WAVES = FILL_THREADS / WAVE;
+
+__global__ __launch_bounds__(FILL_THREADS) void k_cell_fill(int n, int ncell, const int32_t* __restrict__ cell_of,
+                                                            const int32_t* __restrict__ cnt,
+                                                            int32_t* __restrict__ start, int32_t* __restrict__ atoms,
+                                                            const int* gate) {
     if (gate && !*gate) return;
-    __shared__ int32_t wsum[4];
+    __shared__ int32_t wsum[FILL_WAVES];
     const int c = blockIdx.x;
     if (c >= ncell) return;
     const int lane = threadIdx.x % WAVE, w = threadIdx.x / WAVE;
@@ -118,7 +122,9 @@ __global__ __launch_bounds__(256) void k_cell_fill(int n, int ncell, const int32
     part = wave_sum(part);
     if (lane == 0) wsum[w] = part;
     __syncthreads();
-    int base = wsum[0] + wsum[1] + wsum[2] + wsum[3];
+    int base = 0;
+#pragma unroll
+    for (int k = 0; k < FILL_WAVES; ++k) base += wsum[k];
     const int end = base + cnt[c];
     __syncthreads();
     if (threadIdx.x == 0) {
@@ -126,7 +132,7 @@ __global__ __launch_bounds__(256) void k_cell_fill(int n, int ncell, const int32
         if (c == ncell - 1) start[ncell] = end;
     }
     const uint64_t below = (1ull << lane) - 1ull;
-    for (int a0 = 0; a0 < n && base < end; a0 += 256) {
+    for (int a0 = 0; a0 < n && base < end; a0 += FILL_THREADS) {
         const int a = a0 + threadIdx.x;
         const bool hit = (a < n) && (cell_of[a] == c);
         const uint64_t m = __ballot(hit);
@@ -134,7 +140,7 @@ __global__ __launch_bounds__(256) void k_cell_fill(int n, int ncell, const int32
         __syncthreads();
         int off = 0, tot = 0;
 #pragma unroll
-        for (int k = 0; k < 4; ++k) {
+        for (int k = 0; k < FILL_WAVES; ++k) {
             off += k < w ? wsum[k] : 0;
             tot += wsum[k];
         }
@@ -156,6 +162,8 @@ struct ListArgs {
     int32_t* nb_own;
     int32_t stride;
     unsigned* status;
+    uint32_t* mask;    // [27][n][mask_words] hit bits of the cell-pair pass
+    int mask_words;
 };
 
 struct Excl {
@@ -165,13 +173,15 @@ struct Excl {
 };
 
 __device__ __forceinline__ bool excluded(const Excl& e, int j) {
-    // Ensemble::exclusion_check (src/Ensemble.h:294-305)
-    const int k8 = e.cnt < 8 ? e.cnt : 8;
-    for (int k = 0; k < k8; ++k)
-        if (e.ids[k] == j) return true;
-    for (int k = 8; k < e.cnt; ++k)
-        if (e.rest[k] == j) return true;
-    return false;
+    // Ensemble::exclusion_check (src/Ensemble.h:294-305); the first eight
+    // entries are compared without a loop so they stay in registers
+    bool hit = false;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) hit = hit || (e.ids[k] == j);
+    if (!hit && e.cnt > 8)
+        for (int k = 8; k < e.cnt; ++k)
+            if (e.rest[k] == j) return true;
+    return hit;
 }
 
 // Test 64 candidates (one per lane) and append the hits to row i in lane
@@ -229,15 +239,19 @@ constexpr int LB_THREADS = 256;   // atoms of cell c per pass of a block
 constexpr int LB_STAGE = 256;     // neighbour atoms staged in LDS at a time
 constexpr int NSEG = 28;          // per-atom hit counters
 
-__device__ __forceinline__ int neighbour_cell(const CellGrid& g, int c, int s) {
+// Neighbour s of cell c and, per axis, the multiple of L (-1, 0, +1) that
+// brings a partner in that cell next to the atom when the stencil wrapped
+// around the box.
+__device__ __forceinline__ int neighbour_cell(const CellGrid& g, int c, int s, int wrap[3]) {
+    wrap[0] = wrap[1] = wrap[2] = 0;
     if (s == 0) return c;
     const int sgn = s <= 13 ? 1 : -1;
     const int k = s <= 13 ? s - 1 : s - 14;
     const int cz = c / g.nxy, cy = (c % g.nxy) / g.n[0], cx = c % g.n[0];
     int xx = cx + sgn * c_stencil[k][0], yy = cy + sgn * c_stencil[k][1], zz = cz + sgn * c_stencil[k][2];
-    if (xx >= g.n[0]) xx = 0; else if (xx < 0) xx = g.n[0] - 1;
-    if (yy >= g.n[1]) yy = 0; else if (yy < 0) yy = g.n[1] - 1;
-    if (zz >= g.n[2]) zz = 0; else if (zz < 0) zz = g.n[2] - 1;
+    if (xx >= g.n[0]) { xx = 0; wrap[0] = 1; } else if (xx < 0) { xx = g.n[0] - 1; wrap[0] = -1; }
+    if (yy >= g.n[1]) { yy = 0; wrap[1] = 1; } else if (yy < 0) { yy = g.n[1] - 1; wrap[1] = -1; }
+    if (zz >= g.n[2]) { zz = 0; wrap[2] = 1; } else if (zz < 0) { zz = g.n[2] - 1; wrap[2] = -1; }
     return zz * g.nxy + yy * g.n[0] + xx;
 }
 
@@ -252,10 +266,20 @@ __global__ __launch_bounds__(LB_THREADS) void k_list_cells(ListArgs a, CellGrid 
     if constexpr (!WRITE)
         if (c == 0 && s == 0)
             for (int k = threadIdx.x; k < g.ncell; k += blockDim.x) cell_cnt[k] = 0;
-    const int nc = neighbour_cell(g, c, s);
+    int wrap[3];
+    const int nc = neighbour_cell(g, c, s, wrap);
+    // A partner across a wrapped face sits about L away: apply_pbc's loop
+    // then makes exactly one step, d + shift (shift = +L when the stencil
+    // wrapped past the upper face, since d = x_j - x_i is then near -L).
+    // Whenever d + shift lands in [-L/2, L/2) it is apply_pbc's result, the
+    // same single operation; otherwise the loops run.
+    const double shx = wrap[0] * a.box.L[0], shy = wrap[1] * a.box.L[1], shz = wrap[2] * a.box.L[2];
     const int ib = start[c], ie = start[c + 1];
     const int jb = start[nc], je = start[nc + 1];
     const int n = a.n;
+    // the counting pass records its hits as bits when the neighbour cell fits
+    // the mask; the writing pass then expands the bits instead of testing again
+    const bool use_mask = je - jb <= 32 * a.mask_words;
     for (int i0 = ib; i0 < ie; i0 += LB_THREADS) {
         const int k = i0 + threadIdx.x;
         const bool act = k < ie;
@@ -264,6 +288,7 @@ __global__ __launch_bounds__(LB_THREADS) void k_list_cells(ListArgs a, CellGrid 
         Excl ex{};
         int own_at = 0, non_at = 0;   // counts (first pass) or next slots (second pass)
         bool fits = true;
+        uint32_t* mrow = a.mask + ((size_t)s * n + i) * a.mask_words;
         if (act) {
             px = a.pos[3 * i]; py = a.pos[3 * i + 1]; pz = a.pos[3 * i + 2];
             ex = load_excl(a, i);
@@ -292,6 +317,25 @@ __global__ __launch_bounds__(LB_THREADS) void k_list_cells(ListArgs a, CellGrid 
             }
         }
         int32_t* row = a.nb + (int64_t)i * a.stride;
+        if (WRITE && use_mask) {
+            // expand the recorded hits in candidate order
+            if (act && fits) {
+                const int nw = (je - jb + 31) / 32;
+                for (int w = 0; w < nw; ++w) {
+                    uint32_t bits = mrow[w];
+                    while (bits) {
+                        const int b = __ffs(bits) - 1;
+                        bits &= bits - 1u;
+                        const int j = catoms[jb + 32 * w + b];
+                        const bool owned = s == 0 ? (j > i) : (s <= 13);
+                        if (owned) row[own_at++] = j;
+                        else row[a.stride - 1 - non_at++] = j;
+                    }
+                }
+            }
+            continue;
+        }
+        uint32_t word = 0;
         for (int j0 = jb; j0 < je; j0 += LB_STAGE) {
             const int nj = min(LB_STAGE, je - j0);
             __syncthreads();
@@ -301,11 +345,22 @@ __global__ __launch_bounds__(LB_THREADS) void k_list_cells(ListArgs a, CellGrid 
             }
             __syncthreads();
             if (act) {
+#pragma unroll 4
                 for (int t = 0; t < nj; ++t) {
                     const double4 q = sj[t];
                     const int j = (int)q.w;
                     double dx = q.x - px, dy = q.y - py, dz = q.z - pz;
-                    if (s != 0) pbc(dx, dy, dz, a.box);
+                    if (s != 0) {
+                        // |d + shift| < L/2 on every axis: the reference's single
+                        // step (a value exactly at -L/2 takes the exact path)
+                        const double ex = dx + shx, ey = dy + shy, ez = dz + shz;
+                        const bool inside = (fabs(ex) < a.box.h[0]) & (fabs(ey) < a.box.h[1]) & (fabs(ez) < a.box.h[2]);
+                        dx = inside ? ex : dx;
+                        dy = inside ? ey : dy;
+                        dz = inside ? ez : dz;
+                        if (!inside) pbc(dx, dy, dz, a.box);
+                    }
+                    const int tt = j0 - jb + t;   // candidate index within the neighbour cell
                     if (len2_exact(dx, dy, dz) < a.rlist2 && (s != 0 || j != i) && !excluded(ex, j)) {
                         const bool owned = s == 0 ? (j > i) : (s <= 13);
                         if constexpr (WRITE) {
@@ -313,9 +368,15 @@ __global__ __launch_bounds__(LB_THREADS) void k_list_cells(ListArgs a, CellGrid 
                                 if (owned) row[own_at] = j;
                                 else row[a.stride - 1 - non_at] = j;
                             }
+                        } else {
+                            word |= 1u << (tt & 31);
                         }
                         own_at += owned ? 1 : 0;
                         non_at += owned ? 0 : 1;
+                    }
+                    if (!WRITE && use_mask && ((tt & 31) == 31 || tt == je - jb - 1)) {
+                        mrow[tt >> 5] = word;
+                        word = 0;
                     }
                 }
             }
@@ -413,6 +474,8 @@ static int launch_list(Ctx* c, const int* gate, bool md_prep) {
     a.nb_own = c->d_nb_own;
     a.stride = c->stride;
     a.status = c->d_status;
+    a.mask = c->d_list_mask;
+    a.mask_words = c->list_mask_words;
     const int waves_per_block = 4;
     const dim3 grid((c->n + waves_per_block - 1) / waves_per_block);
     ProfScope ps(c, "pairlist");
@@ -426,15 +489,15 @@ static int launch_list(Ctx* c, const int* gate, bool md_prep) {
         g.nxy = g.n[0] * g.n[1];
         g.ncell = g.nxy * g.n[2];
         const int nc = g.ncell;
-        const dim3 ga((c->nmol + 1023) / 1024);
+        const dim3 ga((c->nmol + 127) / 128);
         if (nc <= 8192)
-            hipLaunchKernelGGL((k_cell_assign<true>), ga, dim3(1024), sizeof(int32_t) * nc, c->stream, c->nmol,
+            hipLaunchKernelGGL((k_cell_assign<true>), ga, dim3(128), sizeof(int32_t) * nc, c->stream, c->nmol,
                                c->d_mol_start, c->d_pos, g, c->box, pr, c->d_cell_of, c->d_cell_cnt, gate);
         else
-            hipLaunchKernelGGL((k_cell_assign<false>), ga, dim3(1024), 0, c->stream, c->nmol, c->d_mol_start, c->d_pos,
+            hipLaunchKernelGGL((k_cell_assign<false>), ga, dim3(128), 0, c->stream, c->nmol, c->d_mol_start, c->d_pos,
                                g, c->box, pr, c->d_cell_of, c->d_cell_cnt, gate);
-        hipLaunchKernelGGL(k_cell_fill, dim3(nc), dim3(256), 0, c->stream, c->n, nc, c->d_cell_of, c->d_cell_cnt,
-                           c->d_cell_start, c->d_cell_atoms, gate);
+        hipLaunchKernelGGL(k_cell_fill, dim3(nc), dim3(FILL_THREADS), 0, c->stream, c->n, nc, c->d_cell_of,
+                           c->d_cell_cnt, c->d_cell_start, c->d_cell_atoms, gate);
         const dim3 gl(nc, 27);
         hipLaunchKernelGGL((k_list_cells<false>), gl, dim3(LB_THREADS), 0, c->stream, a, g, c->d_cell_start,
                            c->d_cell_atoms, c->d_list_seg, c->d_cell_cnt, gate);
@@ -471,9 +534,20 @@ int pairlist_build(Ctx* c, double cutoff, double cut_buff, int use_cells, bool o
             if (c->d_cell_start) (void)hipFree(c->d_cell_start);
             SMD_HIP(c, hipMalloc(&c->d_cell_cnt, sizeof(int32_t) * (nc + 1)));
             SMD_HIP(c, hipMalloc(&c->d_cell_start, sizeof(int32_t) * (nc + 1)));
-            // k_cell_scan clears the counts after each use; they start at zero
+            // the counting pass clears the counts after each use; they start at zero
             SMD_HIP(c, hipMemset(c->d_cell_cnt, 0, sizeof(int32_t) * (nc + 1)));
             c->cell_alloc = nc + 1;
+        }
+        // hit bits for neighbour cells of up to twice the mean occupancy
+        const int words = (2 * c->n / nc + 31) / 32 + 1;
+        if (words > c->list_mask_words) {
+            if (c->d_list_mask) (void)hipFree(c->d_list_mask);
+            c->d_list_mask = nullptr;
+            c->list_mask_words = 0;
+            const size_t bytes = sizeof(uint32_t) * 27 * (size_t)c->n * words;
+            if (hipMalloc(&c->d_list_mask, bytes) != hipSuccess)
+                return fail(c, SMD_ENOMEM, "pair-list mask allocation of %zu bytes failed", bytes);
+            c->list_mask_words = words;
         }
     }
     // Row capacity: user value, else estimate from the density and grow on

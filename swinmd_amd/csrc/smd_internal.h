@@ -98,6 +98,8 @@ struct Ctx {
     int32_t* d_cell_start = nullptr;  // [ncell+1]
     int32_t* d_cell_atoms = nullptr;  // [n]
     int32_t* d_list_seg = nullptr;    // [28][n] per-atom hit counts of the cell-pair segments
+    uint32_t* d_list_mask = nullptr;  // [27][n][list_mask_words] hit bits of the cell-pair pass
+    int list_mask_words = 0;
     int32_t* d_nb = nullptr;          // [n * stride]
     int32_t* d_nb_cnt = nullptr;      // [n] full row length
     int32_t* d_nb_own = nullptr;      // [n] entries tagged OWNED (the reference row)
@@ -204,16 +206,26 @@ struct ProfScope {
 
 // ---- device helpers -------------------------------------------------------
 
-// Ensemble::apply_pbc (src/Ensemble.h:319-334), half-open [-L/2, L/2).  The
-// loops are bounded so a corrupt coordinate cannot hang a wave; any |x| < 8L
-// gives the reference's result.
+// Ensemble::apply_pbc (src/Ensemble.h:319-334), half-open [-L/2, L/2): the
+// reference's loops subtract or add L until the value is inside.  The first
+// step is taken with selects (it is the only one for |x| < 1.5 L, which covers
+// displacements between atoms of a wrapped box); further steps, bounded so a
+// corrupt coordinate cannot hang a wave, run only when still outside.  Any
+// |x| < 8 L gives the reference's result, operation for operation.
+__device__ __forceinline__ double pbc1(double x, double L, double h) {
+    x = x >= h ? x - L : x;
+    x = x < -h ? x + L : x;
+    if (x >= h || x < -h) {
+        for (int it = 0; it < 7 && x >= h; ++it) x -= L;
+        for (int it = 0; it < 7 && x < -h; ++it) x += L;
+    }
+    return x;
+}
+
 __device__ __forceinline__ void pbc(double& x, double& y, double& z, const Box& b) {
-    for (int it = 0; it < 8 && x >= b.h[0]; ++it) x -= b.L[0];
-    for (int it = 0; it < 8 && x < -b.h[0]; ++it) x += b.L[0];
-    for (int it = 0; it < 8 && y >= b.h[1]; ++it) y -= b.L[1];
-    for (int it = 0; it < 8 && y < -b.h[1]; ++it) y += b.L[1];
-    for (int it = 0; it < 8 && z >= b.h[2]; ++it) z -= b.L[2];
-    for (int it = 0; it < 8 && z < -b.h[2]; ++it) z += b.L[2];
+    x = pbc1(x, b.L[0], b.h[0]);
+    y = pbc1(y, b.L[1], b.h[1]);
+    z = pbc1(z, b.L[2], b.h[2]);
 }
 
 // Minimum image for a displacement known to lie within 1.5 L of the image
