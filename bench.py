@@ -147,6 +147,8 @@ def main():
                     help="MPI ranks of the reference CPU run (the GPU's share of host cores)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-fp64", action="store_true", help="skip the plain FP64 comparison run")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="first copy MIB MiB device to device (counter calibration for profiling runs)")
     ap.add_argument("--no-graph", action="store_true",
                     help="launch each step's kernels directly instead of replaying a captured hipGraph")
     args = ap.parse_args()
@@ -159,6 +161,8 @@ def main():
     from swinmd_amd import _lib as L
     from swinmd_amd import ensemble, systems as S
     L.load()   # raises if libswinmd_hip.so is missing
+    if args.copy_calibration:
+        copy_calibration(args.copy_calibration)
 
     system = S.c2_water()
     n = system.n_atoms
@@ -196,6 +200,11 @@ def main():
                 if cnt:
                     fam[f] = dict(us=ms * 1e3 / cnt, launches=cnt, total_ms=ms)
             ctx.profile(False)
+            # the modelled kernels alone, launched back to back between two
+            # events (what a profiler's kernel trace reports per launch)
+            for f in ("pairs", "recip_sfac", "recip_force"):
+                if f in fam:
+                    fam[f]["isolated_us"] = ctx.time_family(f, reps=200)
             results[name]["families"] = fam
             results[name]["pairs_in_cut"], results[name]["half_list"] = unique_pairs_in_cutoff(ctx, MD["cutoff"])
         ctx.close()
@@ -235,16 +244,61 @@ def main():
         models["recip_sfac"] = FLOP_PER_ATOM_K_SFAC * n * K
     if K and "recip_force" in fam:
         models["recip_force"] = FLOP_PER_ATOM_K_FORCE * n * K
+    out["kernels_us_isolated"] = {k: round(v["isolated_us"], 2) for k, v in fam.items() if "isolated_us" in v}
     if models:
-        top = max(models, key=lambda k: fam[k]["us"])
-        ach = models[top] / (fam[top]["us"] * 1e-6) / 1e12
+        top = max(models, key=lambda k: fam[k]["isolated_us"])
+        us = fam[top]["isolated_us"]
+        ach = models[top] / (us * 1e-6) / 1e12
+        traffic = measured_traffic().get(top)
         out["roofline"] = {"kernel": top, "bound": "valu", "achieved": round(ach, 3), "peak": PEAK_FP32_TFLOPS,
-                           "unit": "TFLOP/s", "frac": round(ach / PEAK_FP32_TFLOPS, 5), "traffic": None,
-                           "flop_per_launch": models[top], "avg_us": round(fam[top]["us"], 2)}
+                           "unit": "TFLOP/s", "frac": round(ach / PEAK_FP32_TFLOPS, 5), "traffic": traffic,
+                           "traffic_unit": "bytes/launch", "flop_per_launch": models[top], "avg_us": round(us, 2)}
+        if top == "pairs":
+            # the bytes the kernel cannot avoid: the full-row list entries
+            # (both directions of each pair), then per atom its position,
+            # charge, type, molecule and row counts in, its force out
+            alg = 4 * 2 * mix["half_list"] + n * (24 + 8 + 4 + 4 + 8 + 24)
+            out["roofline"]["algorithmic_bytes"] = alg
+            out["roofline"]["hbm_frac_of_peak"] = round(alg / (us * 1e-6) / 1e9 / PEAK_HBM_GBS, 5)
     if not args.no_cpu_baseline:
         log(f"cpu baseline: {args.cpu_steps} reference steps")
         out["cpu_baseline"] = cpu_baseline(system, pos_eq, args.cpu_steps, args.cpu_ranks)
     print(json.dumps(out), flush=True)
+
+
+def measured_traffic():
+    """HBM bytes per launch of the modelled kernels on this workload, from the
+    committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary (lines
+    'TRAFFIC <family> <bytes>'); empty when the summary is absent."""
+    path = os.path.join(ROOT, "profiles", "hbm_traffic_water4096.txt")
+    out = {}
+    if os.path.exists(path):
+        for line in open(path):
+            f = line.split()
+            if len(f) == 3 and f[0] == "TRAFFIC":
+                out[f[1]] = int(f[2])
+    return out
+
+
+def copy_calibration(mib):
+    """One device-to-device copy of `mib` MiB through the HIP runtime: a plain
+    copy kernel of known traffic (mib read, mib written) against which the
+    HBM counters of a profiling run are scaled."""
+    import ctypes as C
+    hip = C.CDLL("libamdhip64.so")
+    nbytes = int(mib) << 20
+    a, b = C.c_void_p(), C.c_void_p()
+    for p in (a, b):
+        if hip.hipMalloc(C.byref(p), C.c_size_t(nbytes)) != 0:
+            raise RuntimeError("hipMalloc failed")
+    hip.hipMemset(a, 1, C.c_size_t(nbytes))
+    hip.hipDeviceSynchronize()
+    if hip.hipMemcpy(b, a, C.c_size_t(nbytes), 3) != 0:   # hipMemcpyDeviceToDevice
+        raise RuntimeError("hipMemcpy failed")
+    hip.hipDeviceSynchronize()
+    hip.hipFree(a)
+    hip.hipFree(b)
+    log(f"copy calibration: {nbytes} bytes device to device")
 
 
 def count_kvectors(L):

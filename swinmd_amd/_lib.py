@@ -130,6 +130,7 @@ SIGNATURES = {
     "smd_md_run": (C.c_int, [_P, C.c_int64, C.POINTER(MdReport)]),
     "smd_profile_enable": (C.c_int, [_P, C.c_int]),
     "smd_profile_get": (C.c_int, [_P, C.c_char_p, C.POINTER(C.c_double), C.POINTER(C.c_int64)]),
+    "smd_md_time_family": (C.c_int, [_P, C.c_char_p, C.c_int, C.POINTER(C.c_double)]),
     "smd_set_tuning": (C.c_int, [_P, C.c_char_p, C.c_int]),
     "smd_set_multipoles": (C.c_int, [_P, _P, _P, _P, C.c_int64]),
     "smd_induce": (C.c_int, [_P, C.POINTER(InduceParams), C.POINTER(InduceReport)]),

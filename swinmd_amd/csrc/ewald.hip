@@ -603,7 +603,7 @@ __global__ __launch_bounds__(RF_WAVES * WAVE) void k_recip_force(RecipArgs a) {
     }
 }
 
-int ewald_recip(Ctx* c, int prec, int lustig) {
+int ewald_recip(Ctx* c, int prec, int lustig, unsigned passes) {
     if (!c->ewald_ready) return fail(c, SMD_ESTATE, "Ewald not set up");
     RecipArgs a;
     a.n = c->n;
@@ -635,7 +635,7 @@ int ewald_recip(Ctx* c, int prec, int lustig) {
     a.pi8V = 2.0 * a.pi4V;
     const bool mixed = prec == SMD_PREC_MIXED;
     const int f = mixed ? 1 : 0;
-    {
+    if (passes & RECIP_SFAC) {
         ProfScope ps(c, "recip_sfac");
         hipLaunchKernelGGL(k_recip_phase, dim3((3 * c->n + 255) / 256), dim3(256), 0, c->stream, c->n, c->d_pos,
                            a.vk[0], a.vk[1], a.vk[2], a.e1);
@@ -647,7 +647,7 @@ int ewald_recip(Ctx* c, int prec, int lustig) {
         if (lustig) hipLaunchKernelGGL((k_recip_sfac<double, true>), g1, b1, lds, c->stream, a);
         else hipLaunchKernelGGL((k_recip_sfac<double, false>), g1, b1, lds, c->stream, a);
     }
-    {
+    if (passes & RECIP_REDUCE) {
         ProfScope ps(c, "recip_reduce");
         const dim3 gk((a.K + RK_PER_BLOCK - 1) / RK_PER_BLOCK);
         if (mixed) {
@@ -658,7 +658,7 @@ int ewald_recip(Ctx* c, int prec, int lustig) {
             else hipLaunchKernelGGL((k_recip_reduce<double, false>), gk, dim3(256), 0, c->stream, a);
         }
     }
-    {
+    if (passes & RECIP_FORCE) {
         ProfScope ps(c, "recip_force");
         const dim3 g2((c->n + WAVE - 1) / WAVE);
         const size_t lds = c->force_lds[f];

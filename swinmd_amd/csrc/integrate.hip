@@ -624,6 +624,53 @@ extern "C" int smd_md_init(smd_ctx* c, const smd_md_params* p) {
     return SMD_OK;
 }
 
+extern "C" int smd_md_time_family(smd_ctx* c, const char* family, int reps, double* avg_us) {
+    if (!c || !family || !avg_us || reps < 1) return SMD_EINVAL;
+    if (!c->md_ready) return fail(c, SMD_ESTATE, "smd_md_init not called");
+    const std::string f(family);
+    const smd_md_params& p = c->md;
+    unsigned pass = 0;
+    if (f == "recip_sfac") pass = RECIP_SFAC;
+    else if (f == "recip_reduce") pass = RECIP_REDUCE;
+    else if (f == "recip_force") pass = RECIP_FORCE;
+    else if (f != "pairs") return fail(c, SMD_EINVAL, "family '%s' cannot be timed alone", family);
+    // The timed kernels overwrite or add to the forces and energy sums; both
+    // are saved here and restored below, so the MD state is unchanged.  Their
+    // other outputs (k-space partials) are recomputed from unchanged inputs.
+    const size_t fb = sizeof(double) * 3 * c->n, tb = sizeof(double) * T_COUNT;
+    double* save = nullptr;
+    SMD_HIP(c, hipMalloc(&save, fb + tb));
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    const bool prof = c->prof.on;
+    c->prof.on = false;
+    int rc = SMD_OK;
+    hipError_t e = hipMemcpyAsync(save, c->d_force, fb, hipMemcpyDeviceToDevice, c->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(save + 3 * c->n, c->d_terms, tb, hipMemcpyDeviceToDevice, c->stream);
+    if (e == hipSuccess) e = hipEventCreate(&e0);
+    if (e == hipSuccess) e = hipEventCreate(&e1);
+    if (e == hipSuccess) e = hipEventRecord(e0, c->stream);
+    for (int r = 0; e == hipSuccess && rc == SMD_OK && r < reps; ++r)
+        rc = pass ? ewald_recip(c, p.prec, p.lustig, pass)
+                  : pairs_lj_ewald(c, nullptr, nullptr, p.cutoff, p.ewald.alpha, p.ewald.real_cut,
+                                   p.ewald.use_constraint, p.lustig, p.prec, true);
+    if (e == hipSuccess && rc == SMD_OK) e = hipEventRecord(e1, c->stream);
+    // restore whatever happened above
+    hipError_t er = hipMemcpyAsync(c->d_force, save, fb, hipMemcpyDeviceToDevice, c->stream);
+    if (er == hipSuccess) er = hipMemcpyAsync(c->d_terms, save + 3 * c->n, tb, hipMemcpyDeviceToDevice, c->stream);
+    if (er == hipSuccess) er = hipStreamSynchronize(c->stream);
+    float ms = 0.0f;
+    if (e == hipSuccess && rc == SMD_OK) e = hipEventElapsedTime(&ms, e0, e1);
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    (void)hipFree(save);
+    c->prof.on = prof;
+    if (rc) return rc;
+    if (e != hipSuccess) return hip_fail(c, e, "timing run");
+    if (er != hipSuccess) return hip_fail(c, er, "restoring forces");
+    *avg_us = 1e3 * (double)ms / reps;
+    return SMD_OK;
+}
+
 extern "C" int smd_md_run(smd_ctx* c, int64_t nsteps, smd_md_report* out) {
     if (!c || nsteps < 0) return SMD_EINVAL;
     if (!c->md_ready) return fail(c, SMD_ESTATE, "smd_md_init not called");

@@ -290,7 +290,9 @@ int ewald_setup(Ctx* c, const smd_ewald_params* p);
 int ewald_real(Ctx* c, int prec, int lustig);
 int ewald_intra(Ctx* c, int lustig);
 int ewald_surface(Ctx* c);
-int ewald_recip(Ctx* c, int prec, int lustig);
+// Reciprocal-space passes: phase + structure factors, per-k reduction, forces.
+enum : unsigned { RECIP_SFAC = 1u, RECIP_REDUCE = 2u, RECIP_FORCE = 4u, RECIP_ALL = 7u };
+int ewald_recip(Ctx* c, int prec, int lustig, unsigned passes = RECIP_ALL);
 int shake_run(Ctx* c, const smd_integrator_params* p, bool fnh_on_device, double fnh, bool md_step = false);
 int kinetics_run(Ctx* c, bool rewrap, const smd_integrator_params* md = nullptr);
 int lf_atoms_run(Ctx* c, const smd_integrator_params* p, double fnh);

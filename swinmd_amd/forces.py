@@ -144,6 +144,13 @@ class Context:
         self.check(self.lib.smd_profile_get(self.h, family.encode(), C.byref(ms), C.byref(n)))
         return ms.value, n.value
 
+    def time_family(self, family, reps=50):
+        """Average microseconds of one launch of an MD-step kernel family,
+        launched back to back (the MD state is left unchanged)."""
+        us = C.c_double()
+        self.check(self.lib.smd_md_time_family(self.h, family.encode(), int(reps), C.byref(us)))
+        return us.value
+
     def sync(self):
         self.check(self.lib.smd_sync(self.h))
 
