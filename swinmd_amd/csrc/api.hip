@@ -67,13 +67,16 @@ static void release(Ctx* c) {
                     c->d_cell_atoms, c->d_list_seg, c->d_list_mask, c->d_nb, c->d_nb_cnt, c->d_nb_own, c->d_fvirt, c->d_kvec, c->d_kmod2, c->d_expk, c->d_kn, c->d_kgrp, c->d_ksplit, c->d_kf, c->d_e1,
                     c->d_sfac_part, c->d_vdw_a, c->d_vdw_b, c->d_vdw_c, c->d_epos, c->d_fa, c->d_fb,
                     c->d_host, c->d_dipl, c->d_quadl, c->d_polar, c->d_dipg, c->d_quadg, c->d_e0, c->d_mu,
-                    c->d_cg, c->d_cg_part, c->d_cg_state, c->d_cg_scal};
+                    c->d_cg, c->d_cg_part, c->d_cg_state, c->d_cg_scal, c->d_frecip};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     if (c->h_pinned) (void)hipHostFree(c->h_pinned);
     if (c->graph_exec) (void)hipGraphExecDestroy(c->graph_exec);
     if (c->graph) (void)hipGraphDestroy(c->graph);
     for (auto& r : c->prof.pending) { (void)hipEventDestroy(r.a); (void)hipEventDestroy(r.b); }
+    if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);
+    if (c->ev_join) (void)hipEventDestroy(c->ev_join);
+    if (c->stream2) (void)hipStreamDestroy(c->stream2);
     if (c->stream) (void)hipStreamDestroy(c->stream);
 }
 
@@ -283,6 +286,9 @@ extern "C" int smd_set_tuning(smd_ctx* c, const char* name, int value) {
         if (value != 0 && value != 4 && value != 8 && value != 16 && value != 32)
             return fail(c, SMD_EINVAL, "pair_lanes must be 0 (by precision), 4, 8, 16 or 32 (got %d)", value);
         c->pair_lanes = value;
+    } else if (std::string(name) == "recip_overlap") {
+        if (value != 0 && value != 1) return fail(c, SMD_EINVAL, "recip_overlap must be 0 or 1 (got %d)", value);
+        c->recip_overlap = value;
     } else {
         return fail(c, SMD_EINVAL, "unknown tuning parameter '%s'", name);
     }

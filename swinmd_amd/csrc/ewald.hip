@@ -294,6 +294,7 @@ struct RecipArgs {
     void* kf;                  // [K] (b C, b S) as R pairs
     double2* e1;               // [n][3] exp(i 2 pi r_p / L_p) per atom (pass 0)
     double* force;
+    int store;                 // 1: the force pass stores (separate array), 0: adds
     double* terms;
     double alphaR4, pi4V, pi8V;
 };
@@ -597,9 +598,15 @@ __global__ __launch_bounds__(RF_WAVES * WAVE) void k_recip_force(RecipArgs a) {
             F[1] += red[(v * WAVE + lane) * 3 + 1];
             F[2] += red[(v * WAVE + lane) * 3 + 2];
         }
-        a.force[3 * i] += F[0];
-        a.force[3 * i + 1] += F[1];
-        a.force[3 * i + 2] += F[2];
+        if (a.store) {
+            a.force[3 * i] = F[0];
+            a.force[3 * i + 1] = F[1];
+            a.force[3 * i + 2] = F[2];
+        } else {
+            a.force[3 * i] += F[0];
+            a.force[3 * i + 1] += F[1];
+            a.force[3 * i + 2] += F[2];
+        }
     }
 }
 
@@ -627,7 +634,8 @@ int ewald_recip(Ctx* c, int prec, int lustig, unsigned passes) {
     a.part = c->d_sfac_part;
     a.kf = c->d_kf;
     a.e1 = reinterpret_cast<double2*>(c->d_e1);
-    a.force = c->d_force;
+    a.store = c->recip_store ? 1 : 0;
+    a.force = c->recip_store ? c->d_frecip : c->d_force;
     a.terms = c->d_terms;
     a.alphaR4 = 1.0 / (4.0 * (c->ew.alpha * c->ew.alpha));
     const double volume = c->box.L[0] * c->box.L[1] * c->box.L[2];

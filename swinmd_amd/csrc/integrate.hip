@@ -27,6 +27,7 @@ struct IntegArgs {
     double* vel;
     double* disp;
     double* force;
+    const double* frecip;   // MD step with reciprocal space on the second stream: its forces, else null
     double* com;
     double* mom;
     const double* bond2;
@@ -90,7 +91,11 @@ __global__ __launch_bounds__(256) void k_shake(IntegArgs a) {
             for (int d = 0; d < 3; ++d) {
                 ro[s][d] = a.pos[3 * id + d];
                 vt[s][d] = a.vtm[3 * id + d] - vcm[d];
-                const double F = a.force[3 * id + d];
+                double F = a.force[3 * id + d];
+                if (a.frecip) {   // the sum the reference forms: short-range terms, then k-space
+                    F += a.frecip[3 * id + d];
+                    a.force[3 * id + d] = F;
+                }
                 double vp;
                 if (a.nose_hoover) {
                     const double P = d == 0 ? Px : (d == 1 ? Py : Pz);
@@ -410,6 +415,7 @@ int shake_run(Ctx* c, const smd_integrator_params* p, bool fnh_on_device, double
     a.fnh_on_device = fnh_on_device ? 1 : 0;
     a.fnh_host = fnh;
     a.md_step = md_step ? 1 : 0;
+    a.frecip = c->shake_frecip ? c->d_frecip : nullptr;
     ProfScope ps(c, "shake");
     hipLaunchKernelGGL(k_shake, dim3((c->nmol + 255) / 256), dim3(256), 0, c->stream, a);
     SMD_HIP(c, hipGetLastError());
@@ -448,16 +454,37 @@ int md_step_enqueue(Ctx* c, bool do_long) {
     // trigger + drift sums; the list rebuild (with the molecule rewrap) runs
     // only when the trigger fired, gated on the device flag
     hipLaunchKernelGGL(k_step_head, gA, b256, 0, c->stream, a);
+    // Reciprocal space needs the positions after the rebuild's rewrap and
+    // nothing of the list: it runs on the second stream beside the list
+    // build and the short-range kernels.
+    const bool fork = do_long && c->recip_overlap;
+    c->list_fork = fork ? c->ev_fork : nullptr;
     int rc = pairlist_build(c, p.cutoff, p.cut_buff, p.use_cells, true);
+    c->list_fork = nullptr;
     if (rc) return rc;
+    if (fork) {
+        SMD_HIP(c, hipStreamWaitEvent(c->stream2, c->ev_fork, 0));
+        {
+            StreamSwap sw(c, c->stream2);
+            c->recip_store = true;
+            rc = ewald_recip(c, p.prec, p.lustig);
+            c->recip_store = false;
+        }
+        if (rc) return rc;
+        SMD_HIP(c, hipEventRecord(c->ev_join, c->stream2));
+    }
     // the pair kernel is the first force term of the step: it stores forces
     rc = pairs_lj_ewald(c, nullptr, nullptr, p.cutoff, p.ewald.alpha, p.ewald.real_cut, p.ewald.use_constraint,
                         p.lustig, p.prec, true);
     if (rc) return rc;
     if ((rc = ewald_intra(c, p.lustig))) return rc;
     if (p.ewald.surface && (rc = ewald_surface(c))) return rc;
-    if (do_long && (rc = ewald_recip(c, p.prec, p.lustig))) return rc;
-    if ((rc = shake_run(c, &p.integ, true, 0.0, true))) return rc;   // drift removal, SHAKE, clears the flag
+    if (fork) SMD_HIP(c, hipStreamWaitEvent(c->stream, c->ev_join, 0));
+    else if (do_long && (rc = ewald_recip(c, p.prec, p.lustig))) return rc;
+    c->shake_frecip = fork;
+    rc = shake_run(c, &p.integ, true, 0.0, true);   // drift removal, SHAKE (adds d_frecip), clears the flag
+    c->shake_frecip = false;
+    if (rc) return rc;
     if ((rc = kinetics_run(c, true, &p.integ))) return rc;         // kinetics, then the NH update
     SMD_HIP(c, hipGetLastError());
     return SMD_OK;
@@ -585,6 +612,12 @@ extern "C" int smd_md_init(smd_ctx* c, const smd_md_params* p) {
     vp.eps4 = c->md.eps4;
     int rc = upload_vdw_tables(c, &vp);
     if (rc) return rc;
+    if (!c->stream2) {
+        SMD_HIP(c, hipStreamCreateWithFlags(&c->stream2, hipStreamNonBlocking));
+        SMD_HIP(c, hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming));
+        SMD_HIP(c, hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming));
+        SMD_HIP(c, hipMalloc(&c->d_frecip, sizeof(double) * 3 * c->n));
+    }
     if ((rc = ewald_setup(c, &p->ewald))) return rc;
     if ((rc = pairlist_build(c, p->cutoff, p->cut_buff, p->use_cells, false))) return rc;
     // Give in-step rebuilds headroom: the row capacity is fixed once a step

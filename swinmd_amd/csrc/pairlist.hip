@@ -496,6 +496,7 @@ static int launch_list(Ctx* c, const int* gate, bool md_prep) {
         else
             hipLaunchKernelGGL((k_cell_assign<false>), ga, dim3(128), 0, c->stream, c->nmol, c->d_mol_start, c->d_pos,
                                g, c->box, pr, c->d_cell_of, c->d_cell_cnt, gate);
+        if (c->list_fork) SMD_HIP(c, hipEventRecord(c->list_fork, c->stream));   // positions are final here
         hipLaunchKernelGGL(k_cell_fill, dim3(nc), dim3(FILL_THREADS), 0, c->stream, c->n, nc, c->d_cell_of,
                            c->d_cell_cnt, c->d_cell_start, c->d_cell_atoms, gate);
         const dim3 gl(nc, 27);
@@ -507,6 +508,7 @@ static int launch_list(Ctx* c, const int* gate, bool md_prep) {
         if (md_prep)
             hipLaunchKernelGGL(k_rebuild_prep, dim3((c->nmol + 255) / 256), dim3(256), 0, c->stream, c->nmol,
                                c->d_mol_start, c->d_pos, c->box, pr, gate);
+        if (c->list_fork) SMD_HIP(c, hipEventRecord(c->list_fork, c->stream));
         hipLaunchKernelGGL(k_pairs_brute, grid, dim3(WAVE * waves_per_block), 0, c->stream, a, gate);
     }
     SMD_HIP(c, hipGetLastError());

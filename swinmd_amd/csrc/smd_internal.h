@@ -164,6 +164,16 @@ struct Ctx {
     int64_t md_step = 0, md_list_updates = 0;
     hipGraphExec_t graph_exec = nullptr;
     hipGraph_t graph = nullptr;
+    // Reciprocal space of the MD step runs on a second stream, forked after
+    // the list rebuild's rewrap and joined before SHAKE; its forces go to
+    // d_frecip, which SHAKE adds after the short-range terms.
+    hipStream_t stream2 = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+    hipEvent_t list_fork = nullptr;   // recorded by the list build after its preamble when set
+    double* d_frecip = nullptr;       // [3n]
+    bool recip_store = false;         // k_recip_force stores into d_frecip instead of adding
+    bool shake_frecip = false;        // k_shake adds d_frecip to the forces it reads
+    int recip_overlap = 1;
 
     Prof prof;
     int pair_lanes = 0;           // lanes per atom row in the pair kernels (0: by precision)
@@ -185,6 +195,14 @@ int hip_fail(Ctx* c, hipError_t e, const char* what);
     } while (0)
 
 // Profiling bracket around a launch.
+// Enqueues on another stream for the scope (every launch helper uses c->stream).
+struct StreamSwap {
+    Ctx* c;
+    hipStream_t keep;
+    StreamSwap(Ctx* c_, hipStream_t s) : c(c_), keep(c_->stream) { c->stream = s; }
+    ~StreamSwap() { c->stream = keep; }
+};
+
 struct ProfScope {
     Ctx* c;
     const char* fam;

@@ -187,6 +187,29 @@ def test_md_graph_replay_equals_eager():
     assert rel_rms(outs[1], outs[0]) <= 1e-11
 
 
+@pytest.mark.parametrize("graph", [False, True])
+def test_md_recip_on_second_stream_equals_one_stream(graph):
+    """Reciprocal space forked onto the second stream (forces kept apart and
+    added in SHAKE) gives the one-stream step's trajectory."""
+    from swinmd_amd.forces import WaterMD
+    fx = Fixture("water512_md10")
+    c = fx.cfg
+    outs = []
+    for overlap in (0, 1):
+        ctx = _ctx(fx.system)
+        ctx.set_tuning("recip_overlap", overlap)
+        ctx.set_velocities(fx.vec("init_vel"))
+        md = WaterMD(ctx, cutoff=c["cutoff"], cut_buff=c["cut_buff"], alpha=c["alpha"], kcut=c["kcut"], dt=c["dt"],
+                     temperature=c["temperature"], coupl=c["coupl"], use_graph=graph)
+        md.init()
+        rep = md.run(10)
+        outs.append((ctx.positions(), ctx.forces(), rep))
+    assert rel_rms(outs[1][0], outs[0][0]) <= 1e-11
+    assert rel_rms(outs[1][1], outs[0][1]) <= 1e-9
+    assert rel(outs[1][2]["e_long"], outs[0][2]["e_long"]) <= 1e-9
+    assert outs[1][2]["list_updates"] == outs[0][2]["list_updates"]
+
+
 def test_shake_step_matches_oracle():
     from oracle import oracle as O
     from swinmd_amd.forces import EwaldForce, LFIntegrator, LJForce
