@@ -1496,3 +1496,588 @@ int orc_induced_dipoles(const orc_sys *s, const double *dip, const double *quad,
     if (final_res) *final_res = res;
     return rc;
 }
+
+/* ---- permanent multipoles under Ewald (src/forces/Multipole.cpp) ---------- */
+
+static const double COULOMBS = 138.93548501;          /* COUlOMBS, src/NEMD_defs.h:104 */
+static const double SQRT_COULOMB = 11.78708976;       /* SQRTCOULOMBCONSTANT, src/NEMD_defs.h:103 */
+
+static inline v3 v3_neg(v3 a) { v3 r = {-a.x, -a.y, -a.z}; return r; }
+/* cross(v1, v2) (src/utils/Vector3.h:206-210) */
+static inline v3 v3_cross(v3 a, v3 b) {
+    v3 r = {a.y * b.z - b.y * a.z, b.x * a.z - a.x * b.z, a.x * b.y - b.x * a.y};
+    return r;
+}
+/* cross(k[3][3], m[3][3], ones): row-wise cross products summed
+ * (src/utils/Vector3.h:220-225) */
+static inline v3 m3_cross(const double k[3][3], const double m[3][3]) {
+    v3 r;
+    r.x = k[1][0] * m[2][0] - k[2][0] * m[1][0] + k[1][1] * m[2][1] - k[2][1] * m[1][1] + k[1][2] * m[2][2] -
+          k[2][2] * m[1][2];
+    r.y = k[2][0] * m[0][0] - k[0][0] * m[2][0] + k[2][1] * m[0][1] - k[0][1] * m[2][1] + k[2][2] * m[0][2] -
+          k[0][2] * m[2][2];
+    r.z = k[0][0] * m[1][0] - k[1][0] * m[0][0] + k[0][1] * m[1][1] - k[1][1] * m[0][1] + k[0][2] * m[1][2] -
+          k[1][2] * m[0][2];
+    return r;
+}
+
+/* A site's lab-frame multipoles: charge q/sqrt(COULOMB), dipole, the full
+ * symmetric quadrupole matrix (Qyx = Qxy, ...) as the reference expands it. */
+typedef struct { double q; v3 d; double Q[3][3]; } mp_site;
+
+static inline mp_site mp_load(const orc_sys *s, const double *dip, const double *quad, int i) {
+    mp_site m;
+    m.q = s->q[i] / SQRT_COULOMB;
+    m.d = v3_ld(dip + 3 * i);
+    const double *p = quad + 6 * i;
+    m.Q[0][0] = p[0]; m.Q[0][1] = p[1]; m.Q[0][2] = p[2];
+    m.Q[1][0] = p[1]; m.Q[1][1] = p[3]; m.Q[1][2] = p[4];
+    m.Q[2][0] = p[2]; m.Q[2][1] = p[4]; m.Q[2][2] = p[5];
+    return m;
+}
+
+/* Q.r, row by row, each row summed x, y, z */
+static inline v3 m3_mul(const double Q[3][3], v3 r) {
+    v3 o = {Q[0][0] * r.x + Q[0][1] * r.y + Q[0][2] * r.z, Q[1][0] * r.x + Q[1][1] * r.y + Q[1][2] * r.z,
+            Q[2][0] * r.x + Q[2][1] * r.y + Q[2][2] * r.z};
+    return o;
+}
+/* sum_ab Q_ab r_a r_b, in the reference's term order */
+static inline double m3_quadform(const double Q[3][3], v3 r) {
+    const double rr[3] = {r.x, r.y, r.z};
+    double s = Q[0][0] * rr[0] * rr[0];
+    for (int a = 0; a < 3; ++a)
+        for (int b = 0; b < 3; ++b)
+            if (a || b) s += Q[a][b] * rr[a] * rr[b];
+    return s;
+}
+/* (Qi Qj) r: (sum_c Qi_ac Qj_cb) r_b summed over b */
+static inline v3 m3_mm_mul(const double A[3][3], const double B[3][3], v3 r) {
+    const double rr[3] = {r.x, r.y, r.z};
+    double o[3];
+    for (int a = 0; a < 3; ++a) {
+        double s = 0.0;
+        for (int b = 0; b < 3; ++b) {
+            const double ab = A[a][0] * B[0][b] + A[a][1] * B[1][b] + A[a][2] * B[2][b];
+            s = b == 0 ? ab * rr[b] : s + ab * rr[b];
+        }
+        o[a] = s;
+    }
+    v3 v = {o[0], o[1], o[2]};
+    return v;
+}
+
+/* The G_l coefficients of one site pair and, optionally, their gradients and
+ * the torque vectors Gt_l on both sites (Multipole::real_term,
+ * src/forces/Multipole.cpp:548-617 and :730-877; the same expressions are
+ * repeated in intraMolCorrect :1449-1590). */
+typedef struct {
+    double G[5];
+    v3 gG[5];          /* gradG1..gradG4 in [1..4] */
+    v3 Gti[5], Gtj[5]; /* torque vectors in [1..4] */
+} mp_terms;
+
+static void mp_pair_terms(const mp_site *I, const mp_site *J, v3 rij, int grads, int torques, mp_terms *t) {
+    const double qi = I->q, qj = J->q;
+    const v3 dipi = I->d, dipj = J->d;
+    const double (*Qi)[3] = I->Q, (*Qj)[3] = J->Q;
+    const double QiI = Qi[0][0] + Qi[1][1] + Qi[2][2];
+    const double QjI = Qj[0][0] + Qj[1][1] + Qj[2][2];
+    const double QiRij = m3_quadform(Qi, rij);
+    const double QjRij = m3_quadform(Qj, rij);
+    double QiQj = Qi[0][0] * Qj[0][0];
+    for (int a = 0; a < 3; ++a)
+        for (int b = 0; b < 3; ++b)
+            if (a || b) QiQj += Qi[a][b] * Qj[a][b];
+    const v3 QiXrij = m3_mul(Qi, rij), QjXrij = m3_mul(Qj, rij);
+    const double QimjXrij = dipj.x * QiXrij.x + dipj.y * QiXrij.y + dipj.z * QiXrij.z;
+    const double QjmiXrij = dipi.x * QjXrij.x + dipi.y * QjXrij.y + dipi.z * QjXrij.z;
+    /* sum over rows a of Qj_a . r weighted by (Qi r)_a: rij.b*Qj[a][b]*(Qi r)_a */
+    double QiXrijXQj = 0.0;
+    {
+        const double rr[3] = {rij.x, rij.y, rij.z};
+        const double qr[3] = {QiXrij.x, QiXrij.y, QiXrij.z};
+        int first = 1;
+        for (int a = 0; a < 3; ++a)
+            for (int b = 0; b < 3; ++b) {
+                const double term = rr[b] * Qj[a][b] * qr[a];
+                QiXrijXQj = first ? term : QiXrijXQj + term;
+                first = 0;
+            }
+    }
+    const double mir = v3_dot(dipi, rij), mjr = v3_dot(dipj, rij);
+    t->G[0] = qi * qj;
+    t->G[1] = v3_dot(v3_scale(qj, dipi), rij) - v3_dot(v3_scale(qi, dipj), rij) - v3_dot(dipi, dipj) + qj * QiI + qi * QjI;
+    t->G[2] = qj * QiRij + qi * QjRij - mir * mjr - mjr * QiI + mir * QjI - 2 * QimjXrij + 2 * QjmiXrij + QiI * QjI +
+              2 * QiQj;
+    t->G[3] = mir * QjRij - mjr * QiRij + QiI * QjRij + QjI * QiRij + 4 * QiXrijXQj;
+    t->G[4] = QiRij * QjRij;
+    if (grads) {
+        const v3 QiXmj = m3_mul(Qi, dipj), QjXmi = m3_mul(Qj, dipi);
+        const v3 QiXQjrij = m3_mm_mul(Qi, Qj, rij), QjXQirij = m3_mm_mul(Qj, Qi, rij);
+        t->gG[1] = v3_sub(v3_scale(qj, dipi), v3_scale(qi, dipj));
+        v3 g = v3_scale(2 * qj, QiXrij);
+        g = v3_sub(g, v3_scale(mjr, dipi));
+        g = v3_sub(g, v3_scale(mir, dipj));
+        g = v3_sub(g, v3_scale(2, QiXmj));
+        g = v3_sub(g, v3_scale(QiI, dipj));
+        g = v3_add(g, v3_scale(2 * qi, QjXrij));
+        g = v3_add(g, v3_scale(2, QjXmi));
+        t->gG[2] = v3_add(g, v3_scale(QjI, dipi));
+        g = v3_scale(-QiRij, dipj);
+        g = v3_sub(g, v3_scale(v3_dot(v3_scale(2, dipj), rij), QiXrij));
+        g = v3_add(g, v3_scale(QjRij, dipi));
+        g = v3_add(g, v3_scale(v3_dot(v3_scale(2, dipi), rij), QjXrij));
+        g = v3_add(g, v3_scale(2 * QiI, QjXrij));
+        g = v3_add(g, v3_scale(2 * QjI, QiXrij));
+        g = v3_add(g, v3_scale(4, QiXQjrij));
+        t->gG[3] = v3_add(g, v3_scale(4, QjXQirij));
+        t->gG[4] = v3_add(v3_scale(2 * QjRij, QiXrij), v3_scale(2 * QiRij, QjXrij));
+    }
+    if (torques) {
+        const v3 rji = v3_neg(rij);
+        const double rv[3] = {rij.x, rij.y, rij.z}, rjv[3] = {rji.x, rji.y, rji.z};
+        const double dj[3] = {dipj.x, dipj.y, dipj.z}, di[3] = {dipi.x, dipi.y, dipi.z};
+        const double qjr[3] = {QjXrij.x, QjXrij.y, QjXrij.z}, qir[3] = {QiXrij.x, QiXrij.y, QiXrij.z};
+        double Rij[3][3], mjrij[3][3], rijmj[3][3], Qjrijrij[3][3], rijQjrij[3][3];
+        double mirji[3][3], rjimi[3][3], Qirjirji[3][3], rjiQirji[3][3];
+        for (int a = 0; a < 3; ++a)
+            for (int b = 0; b < 3; ++b) {
+                Rij[a][b] = rv[a] * rv[b];
+                mjrij[a][b] = dj[a] * rv[b];
+                Qjrijrij[a][b] = qjr[a] * rv[b];
+                mirji[a][b] = di[a] * rjv[b];
+                Qirjirji[a][b] = qir[a] * rv[b];
+            }
+        for (int a = 0; a < 3; ++a)
+            for (int b = 0; b < 3; ++b) {
+                rijmj[a][b] = mjrij[b][a];
+                rijQjrij[a][b] = Qjrijrij[b][a];
+                rjimi[a][b] = mirji[b][a];
+                rjiQirji[a][b] = Qirjirji[b][a];
+            }
+        const v3 rijXmi = v3_cross(rij, dipi), mjXmi = v3_cross(dipj, dipi), QjrijXmi = v3_cross(QjXrij, dipi);
+        const v3 RijXQi = m3_cross(Rij, Qi), mjrijXQi = m3_cross(mjrij, Qi), rijmjXQi = m3_cross(rijmj, Qi);
+        const v3 QjXQi = m3_cross(Qj, Qi), rijQjrijXQi = m3_cross(rijQjrij, Qi), QjrijrijXQi = m3_cross(Qjrijrij, Qi);
+        const v3 rjiXmj = v3_cross(rji, dipj), QirijXmj = v3_cross(QiXrij, dipj), RjiXQj = m3_cross(Rij, Qj);
+        const v3 mirjiXQj = m3_cross(mirji, Qj), rjimiXQj = m3_cross(rjimi, Qj);
+        const v3 rjiQirjiXQj = m3_cross(rjiQirji, Qj), QirjirjiXQj = m3_cross(Qirjirji, Qj);
+        const double mjr2 = v3_dot(dipj, rij), mir2 = v3_dot(dipi, rji);
+        t->Gti[1] = v3_sub(v3_scale(qj, rijXmi), mjXmi);
+        v3 g = v3_scale(-mjr2, rijXmi);
+        g = v3_add(g, v3_scale(2.0, QjrijXmi));
+        g = v3_add(g, v3_scale(QjI, rijXmi));
+        g = v3_add(g, v3_scale(2.0 * qj, RijXQi));
+        g = v3_sub(g, v3_scale(2.0, mjrijXQi));
+        g = v3_sub(g, v3_scale(2.0, rijmjXQi));
+        t->Gti[2] = v3_add(g, v3_scale(4.0, QjXQi));
+        g = v3_scale(QjRij, rijXmi);
+        g = v3_sub(g, v3_scale(2.0 * mjr2, RijXQi));
+        g = v3_add(g, v3_scale(2.0 * QjI, RijXQi));
+        g = v3_add(g, v3_scale(4.0, rijQjrijXQi));
+        t->Gti[3] = v3_add(g, v3_scale(4.0, QjrijrijXQi));
+        t->Gti[4] = v3_scale(2.0 * QjRij, RijXQi);
+        t->Gtj[1] = v3_add(v3_scale(qi, rjiXmj), mjXmi);
+        g = v3_scale(-mir2, rjiXmj);
+        g = v3_sub(g, v3_scale(2.0, QirijXmj));
+        g = v3_add(g, v3_scale(QiI, rjiXmj));
+        g = v3_add(g, v3_scale(2.0 * qi, RjiXQj));
+        g = v3_sub(g, v3_scale(2.0, mirjiXQj));
+        g = v3_sub(g, v3_scale(2.0, rjimiXQj));
+        t->Gtj[2] = v3_sub(g, v3_scale(4.0, QjXQi));
+        g = v3_scale(QiRij, rjiXmj);
+        g = v3_sub(g, v3_scale(2.0 * mir2, RjiXQj));
+        g = v3_add(g, v3_scale(2.0 * QiI, RjiXQj));
+        g = v3_add(g, v3_scale(4.0, rjiQirjiXQj));
+        t->Gtj[4] = v3_scale(2.0 * QiRij, RjiXQj);
+        t->Gtj[3] = v3_add(g, v3_scale(4.0, QirjirjiXQj));
+    }
+}
+
+/* Multipole::torques2forces (src/forces/Multipole.cpp:1946-2136): the torque
+ * on each water site becomes forces on the atoms spanning its frame.  Sites
+ * of a molecule are H1, O, H2.  Returns dU/dV before the 1/3V factor. */
+static double mp_torques2forces(const orc_sys *s, const double *trq, double *force) {
+    const double wrot_f = 0.5;
+    double dUdVol = 0.0;
+    for (int m = 0; m < s->n_mols; ++m) {
+        const int h1 = s->mol_start[m], o = h1 + 1, h2 = h1 + 2;
+        const v3 rh1 = v3_ld(s->pos + 3 * h1), ro = v3_ld(s->pos + 3 * o), rh2 = v3_ld(s->pos + 3 * h2);
+        for (int side = 0; side < 2; ++side) {
+            /* H1 (side 0) and H2 (side 1) share one form (:1988-2022, :2076-2110);
+             * the reference handles H1, then O, then H2. */
+            if (side == 1) {
+                /* O site (:2024-2072) */
+                const v3 tau = v3_ld(trq + 3 * o);
+                v3 u = v3_sub(ro, rh1), v = v3_sub(ro, rh2);
+                v3 w = v3_cross(u, v);
+                const double ul = v3_len(u), vl = v3_len(v), wl = v3_len(w);
+                u = v3_div(u, ul);
+                v = v3_div(v, vl);
+                w = v3_div(w, wl);
+                const v3 vxu = v3_cross(v, u);
+                const double vxuL = v3_len(vxu);
+                const v3 wxu = v3_cross(w, u);
+                const double wxuL = v3_len(wxu);
+                const v3 dUdu = v3_add(v3_div(v3_scale(v3_dot(v3_neg(v), tau), vxu), vxuL * vxuL * ul),
+                                       v3_div(v3_scale(wrot_f * v3_dot(v3_neg(w), tau), wxu), wxuL * wxuL * ul));
+                const v3 wxv = v3_cross(w, v);
+                const double wxvL = v3_len(wxv);
+                const v3 dUdv = v3_add(v3_div(v3_scale(v3_dot(v3_neg(u), tau), v3_neg(vxu)), vxuL * vxuL * vl),
+                                       v3_div(v3_scale(wrot_f * v3_dot(v3_neg(w), tau), wxv), wxvL * wxvL * vl));
+                acc(force + 3 * o, v3_sub(v3_neg(dUdu), dUdv));
+                acc(force + 3 * h1, dUdu);
+                acc(force + 3 * h2, dUdv);
+                dUdVol += v3_dot(dUdu, u) * ul + v3_dot(dUdv, v) * vl;
+            }
+            const int h = side == 0 ? h1 : h2;
+            const v3 rh = side == 0 ? rh1 : rh2;
+            const v3 tau = v3_ld(trq + 3 * h);
+            v3 u = v3_sub(rh, ro);
+            const double ul = v3_len(u);
+            v3 v = {1.0, 0.0, 0.0};
+            const double val = u.x / ul;
+            if (fabs(val) > 0.8666) { v.x = 0.0; v.y = 1.0; }
+            const double vl = 1.0;
+            v3 w = v3_cross(u, v);
+            const double wl = v3_len(w);
+            u = v3_div(u, ul);
+            v = v3_div(v, vl);
+            w = v3_div(w, wl);
+            const v3 vxu = v3_cross(v, u);
+            const double vxuL = v3_len(vxu);
+            const v3 wxu = v3_cross(w, u);
+            const double wxuL = v3_len(wxu);
+            const v3 dUdu = v3_add(v3_div(v3_scale(v3_dot(v3_neg(v), tau), vxu), vxuL * vxuL * ul),
+                                   v3_div(v3_scale(v3_dot(v3_neg(w), tau), wxu), wxuL * wxuL * ul));
+            acc(force + 3 * h, v3_neg(dUdu));
+            acc(force + 3 * o, dUdu);
+            dUdVol += v3_dot(dUdu, u) * ul;
+        }
+    }
+    return dUdVol;
+}
+
+static double mp_torque_norms(int n, const double *trq) {
+    double t = 0.0;
+    for (int i = 0; i < n; ++i) t += v3_len(v3_ld(trq + 3 * i));
+    return t;
+}
+
+int orc_multipole_short(const orc_sys *s, const int32_t *off, const int32_t *nb, const double *dip,
+                        const double *quad, const orc_mp_params *p, double *force, orc_mp_out *out) {
+    /* Multipole::compute (src/forces/Multipole.cpp:163-216): self, real space,
+     * intramolecular correction and the surface term, each followed by its
+     * torques2forces. */
+    const int n = s->n_atoms;
+    for (int m = 0; m < s->n_mols; ++m)
+        if (s->mol_start[m + 1] - s->mol_start[m] != 3) return -1;
+    double *trq = (double *)calloc((size_t)3 * n, sizeof(double));
+    if (!trq) return -3;
+    const double volume = s->box[0] * s->box[1] * s->box[2];
+    const double volumer = 1.0 / volume;
+    const double alpha = p->alpha, alpha2 = alpha * alpha, alpha2PI = 2.0 * alpha / SQRT_PI;
+    const double realCut2 = p->real_cut * p->real_cut;
+    memset(out, 0, sizeof *out);
+
+    /* charged-system energy (ctor :96-104) */
+    double qSum = 0.0;
+    for (int i = 0; i < n; ++i) qSum += s->q[i];
+    if (fabs(qSum) > 1.0e-5) out->charged_energy = -PI * qSum * qSum / (2.0 * volume * alpha2);
+
+    /* self_correct (:338-395) */
+    for (int i = 0; i < n; ++i) {
+        const mp_site I = mp_load(s, dip, quad, i);
+        const double QiI = I.Q[0][0] + I.Q[1][1] + I.Q[2][2];
+        double QiQi = I.Q[0][0] * I.Q[0][0];
+        for (int a = 0; a < 3; ++a)
+            for (int b = 0; b < 3; ++b)
+                if (a || b) QiQi += I.Q[a][b] * I.Q[a][b];
+        const double selfterm = I.q * I.q * alpha2PI +
+                                (v3_dot(I.d, I.d) - 2 * I.q * QiI) * 2 * alpha2PI * alpha2 / 3 +
+                                (QiI * QiI + 2 * QiQi) * 4 * alpha2PI * alpha2 * alpha2 / 5;
+        out->self_energy += -0.5 * selfterm * COULOMBS;
+    }
+
+    /* real_term (:398-949) over the half list */
+    for (int i = 0; i < n; ++i) {
+        const v3 ri = v3_ld(s->pos + 3 * i);
+        const mp_site I = mp_load(s, dip, quad, i);
+        for (int k = off[i]; k < off[i + 1]; ++k) {
+            const int j = nb[k];
+            const v3 rj = v3_ld(s->pos + 3 * j);
+            const mp_site J = mp_load(s, dip, quad, j);
+            v3 rij = v3_sub(ri, rj);
+            apply_pbc(&rij, s->box);
+            const double r2 = v3_len2(rij);
+            if (!(r2 < realCut2)) continue;
+            mp_terms t;
+            mp_pair_terms(&I, &J, rij, 1, p->torques, &t);
+            const double r = sqrt(r2);
+            double B[6];
+            B[0] = erfc(alpha * r) / r;
+            B[1] = (-B[0] - alpha2PI * exp(-alpha2 * r2)) / r2;
+            B[2] = (-3 * B[1] + 2 * alpha2PI * alpha2 * exp(-alpha2 * r2)) / r2;
+            B[3] = (-5 * B[2] - 4 * alpha2PI * alpha2 * alpha2 * exp(-alpha2 * r2)) / r2;
+            B[4] = (-7 * B[3] + 8 * alpha2PI * alpha2 * alpha2 * alpha2 * exp(-alpha2 * r2)) / r2;
+            B[5] = (-9 * B[4] - 16 * alpha2PI * alpha2 * alpha2 * alpha2 * alpha2 * exp(-alpha2 * r2)) / r2;
+            out->real_energy += (t.G[0] * B[0] + t.G[1] * B[1] + t.G[2] * B[2] + t.G[3] * B[3] + t.G[4] * B[4]) * COULOMBS;
+            v3 gsum = v3_scale(B[1], t.gG[1]);
+            gsum = v3_add(gsum, v3_scale(B[2], t.gG[2]));
+            gsum = v3_add(gsum, v3_scale(B[3], t.gG[3]));
+            gsum = v3_add(gsum, v3_scale(B[4], t.gG[4]));
+            const double radial = t.G[0] * B[1] + t.G[1] * B[2] + t.G[2] * B[3] + t.G[3] * B[4] + t.G[4] * B[5];
+            const v3 fij = v3_sub(v3_neg(gsum), v3_scale(radial, rij));
+            out->real_deriv += v3_dot(v3_neg(fij), rij) * COULOMBS;
+            acc(force + 3 * i, v3_scale(COULOMBS, fij));
+            dec(force + 3 * j, v3_scale(COULOMBS, fij));
+            vir_add(out->virial, fij, rij);
+            if (p->torques) {
+                v3 ti = v3_scale(B[1], t.Gti[1]);
+                ti = v3_add(ti, v3_scale(B[2], t.Gti[2]));
+                ti = v3_add(ti, v3_scale(B[3], t.Gti[3]));
+                ti = v3_add(ti, v3_scale(B[4], t.Gti[4]));
+                v3 tj = v3_scale(B[1], t.Gtj[1]);
+                tj = v3_add(tj, v3_scale(B[2], t.Gtj[2]));
+                tj = v3_add(tj, v3_scale(B[3], t.Gtj[3]));
+                tj = v3_add(tj, v3_scale(B[4], t.Gtj[4]));
+                acc(trq + 3 * i, v3_scale(COULOMBS, ti));
+                acc(trq + 3 * j, v3_scale(COULOMBS, tj));
+            }
+        }
+    }
+    if (p->torques) {
+        out->tot_torque += mp_torque_norms(n, trq);
+        out->real_deriv = out->real_deriv + mp_torques2forces(s, trq, force);
+    }
+
+    /* intraMolCorrect (:1290-1750): excluded pairs inside each molecule */
+    memset(trq, 0, sizeof(double) * 3 * n);
+    {
+        double e = 0.0;
+        for (int m = 0; m < s->n_mols; ++m) {
+            const int a0 = s->mol_start[m], ns = s->mol_start[m + 1] - a0;
+            for (int a = 0; a < ns; ++a) {
+                const int idi = a0 + a;
+                const mp_site I = mp_load(s, dip, quad, idi);
+                const v3 ri = v3_ld(s->pos + 3 * idi);
+                for (int b = a + 1; b < ns; ++b) {
+                    const int idj = a0 + b;
+                    if (!excluded(s, idi, idj)) continue;
+                    const v3 rj = v3_ld(s->pos + 3 * idj);
+                    v3 rij = v3_sub(ri, rj);
+                    apply_pbc(&rij, s->box);
+                    const double r2 = v3_len2(rij);
+                    const double r = sqrt(r2);
+                    const mp_site J = mp_load(s, dip, quad, idj);
+                    mp_terms t;
+                    mp_pair_terms(&I, &J, rij, !p->use_constraint, !p->use_constraint && p->torques, &t);
+                    double D[6];
+                    D[0] = erf(alpha * r) / r;
+                    D[1] = (-D[0] + alpha2PI * exp(-alpha2 * r2)) / r2;
+                    D[2] = (-3 * D[1] - 2 * alpha2PI * alpha2 * exp(-alpha2 * r2)) / r2;
+                    D[3] = (-5 * D[2] + 4 * alpha2PI * alpha2 * alpha2 * exp(-alpha2 * r2)) / r2;
+                    D[4] = (-7 * D[3] - 8 * alpha2PI * alpha2 * alpha2 * alpha2 * exp(-alpha2 * r2)) / r2;
+                    D[5] = (-9 * D[4] + 16 * alpha2PI * alpha2 * alpha2 * alpha2 * alpha2 * exp(-alpha2 * r2)) / r2;
+                    e += t.G[0] * D[0] + t.G[1] * D[1] + t.G[2] * D[2] + t.G[3] * D[3] + t.G[4] * D[4];
+                    if (!p->use_constraint) {
+                        v3 fij = v3_scale(D[1], t.gG[1]);
+                        fij = v3_add(fij, v3_scale(D[2], t.gG[2]));
+                        fij = v3_add(fij, v3_scale(D[3], t.gG[3]));
+                        fij = v3_add(fij, v3_scale(D[4], t.gG[4]));
+                        const double radial = t.G[0] * D[1] + t.G[1] * D[2] + t.G[2] * D[3] + t.G[3] * D[4] + t.G[4] * D[5];
+                        fij = v3_add(fij, v3_scale(radial, rij));
+                        out->mol_deriv += -v3_dot(fij, rij) * COULOMBS / (3.0 * volume);
+                        acc(force + 3 * idi, v3_scale(COULOMBS, fij));
+                        dec(force + 3 * idj, v3_scale(COULOMBS, fij));
+                        if (p->torques) {
+                            v3 ti = v3_scale(D[1], t.Gti[1]);
+                            ti = v3_add(ti, v3_scale(D[2], t.Gti[2]));
+                            ti = v3_add(ti, v3_scale(D[3], t.Gti[3]));
+                            ti = v3_add(ti, v3_scale(D[4], t.Gti[4]));
+                            v3 tj = v3_scale(D[1], t.Gtj[1]);
+                            tj = v3_add(tj, v3_scale(D[2], t.Gtj[2]));
+                            tj = v3_add(tj, v3_scale(D[3], t.Gtj[3]));
+                            tj = v3_add(tj, v3_scale(D[4], t.Gtj[4]));
+                            acc(trq + 3 * idi, v3_scale(-COULOMBS, ti));
+                            acc(trq + 3 * idj, v3_scale(-COULOMBS, tj));
+                        }
+                    }
+                }
+            }
+        }
+        out->intra_energy = -e * COULOMBS;
+    }
+    if (p->torques) {
+        out->tot_torque += mp_torque_norms(n, trq);
+        out->mol_deriv += mp_torques2forces(s, trq, force) / (3.0 * volume);
+    }
+
+    /* surfDipoleCorrect (:1753-1945) */
+    if (p->surface) {
+        memset(trq, 0, sizeof(double) * 3 * n);
+        const double coeff = 2.0 * PI * volumer / 3.0;
+        v3 qrSum = {0, 0, 0}, qRsum = {0, 0, 0}, miSum = {0, 0, 0}, qirimi = {0, 0, 0};
+        double selfSum = 0.0;
+        for (int i = 0; i < n; ++i) {
+            const v3 r = v3_ld(s->pos + 3 * i), rim = v3_ld(s->com + 3 * s->mol[i]);
+            const mp_site I = mp_load(s, dip, quad, i);
+            const double QiI = I.Q[0][0] + I.Q[1][1] + I.Q[2][2];
+            qrSum = v3_add(qrSum, v3_scale(I.q, r));
+            qRsum = v3_add(qRsum, v3_scale(I.q, rim));
+            miSum = v3_add(miSum, I.d);
+            qirimi = v3_add(qirimi, v3_add(v3_scale(I.q, r), I.d));
+            selfSum += 2 * I.q * QiI - v3_dot(I.d, I.d);
+        }
+        const double selfterm = v3_dot(qirimi, qirimi) + selfSum;
+        out->surf_energy = coeff * selfterm * COULOMBS;
+        out->surf_deriv = -out->surf_energy * volumer +
+                          v3_dot(v3_scale(2.0 * coeff * volumer, qirimi), qrSum) * COULOMBS / 3.0;
+        if (p->use_constraint)
+            out->surf_deriv2 = v3_dot(v3_scale(2.0 * coeff * volumer * volumer, qrSum), qrSum) -
+                               v3_dot(v3_scale(16.0 * coeff * volumer * volumer, qrSum), qRsum) / 9.0 +
+                               v3_dot(v3_scale(2.0 * coeff * volumer * volumer, qRsum), qRsum) / 9.0;
+        else
+            out->surf_deriv2 = v3_dot(v3_scale(4.0 * coeff * volumer * volumer, qrSum), qrSum) / 9.0;
+        for (int i = 0; i < n; ++i) {
+            const mp_site I = mp_load(s, dip, quad, i);
+            const v3 ri = v3_ld(s->pos + 3 * i);
+            const v3 fi = v3_scale(-2.0 * coeff * I.q, qirimi);
+            const v3 temp = v3_scale(-s->q[i], qrSum);
+            acc(force + 3 * i, v3_scale(COULOMBS, fi));
+            vir_add(out->virial, temp, ri);
+            if (p->torques) acc(trq + 3 * i, v3_scale(2.0 * coeff * COULOMBS, v3_cross(qirimi, I.d)));
+        }
+        if (p->torques) {
+            out->tot_torque += mp_torque_norms(n, trq);
+            out->surf_deriv += mp_torques2forces(s, trq, force) * volumer / 3.0;
+        }
+    }
+    free(trq);
+    return 0;
+}
+
+int orc_multipole_long(const orc_sys *s, int K, const double *kvec, const double *kmod2, const double *expk,
+                       const double *dip, const double *quad, const orc_mp_params *p, double *force,
+                       orc_mp_out *out) {
+    /* Multipole::compute_long / reciprocal_term (src/forces/Multipole.cpp:218-240,
+     * :951-1279). */
+    const int n = s->n_atoms;
+    for (int m = 0; m < s->n_mols; ++m)
+        if (s->mol_start[m + 1] - s->mol_start[m] != 3) return -1;
+    const double volume = s->box[0] * s->box[1] * s->box[2];
+    const double volumer = 1.0 / volume;
+    const double pi4Volumer = 4.0 * PI * volumer, pi8Volumer = 2.0 * pi4Volumer;
+    const double alphaR4 = 1.0 / (4.0 * (p->alpha * p->alpha));
+    double *buf = (double *)calloc((size_t)19 * n, sizeof(double));
+    if (!buf) return -3;
+    double *qQcos = buf, *qQsin = buf + n, *mkcos = buf + 2 * n, *mksin = buf + 3 * n, *trq = buf + 4 * n;
+    double *kXmsina = buf + 7 * n, *kXmcosa = buf + 10 * n, *KXQcosa = buf + 13 * n, *KXQsina = buf + 16 * n;
+    out->long_energy = out->long_deriv = out->long_deriv2 = 0.0;
+    memset(out->long_virial, 0, sizeof out->long_virial);
+    for (int j = 0; j < K; ++j) {
+        const v3 kj = v3_ld(kvec + 3 * j);
+        double Sc = 0.0, Ss = 0.0, Ms = 0.0, Mc = 0.0, Vc = 0.0, Vs = 0.0;
+        const double coeff = 2.0 * (alphaR4 + 1.0 / kmod2[j]);
+        v3 cosrR = {0, 0, 0}, sinrR = {0, 0, 0};
+        double Kj[3][3];
+        const double kk[3] = {kj.x, kj.y, kj.z};
+        for (int a = 0; a < 3; ++a)
+            for (int b = 0; b < 3; ++b) Kj[a][b] = kk[a] * kk[b];
+        for (int i = 0; i < n; ++i) {
+            const v3 rim = v3_ld(s->com + 3 * s->mol[i]);
+            const mp_site I = mp_load(s, dip, quad, i);
+            const v3 ri = v3_ld(s->pos + 3 * i);
+            const double a = v3_dot(ri, kj);
+            const double QiK = m3_quadform(I.Q, kj);
+            const double ca = cos(a), sa = sin(a);
+            const double cosAqiQi = (I.q - QiK) * ca;
+            const double sinAqiQi = (I.q - QiK) * sa;
+            const double mikcos = v3_dot(I.d, kj) * ca;
+            const double miksin = v3_dot(I.d, kj) * sa;
+            const double qi2Qicos = 2.0 * QiK * ca;
+            const double qi2Qisin = 2.0 * QiK * sa;
+            qQcos[i] = cosAqiQi;
+            qQsin[i] = sinAqiQi;
+            mkcos[i] = mikcos;
+            mksin[i] = miksin;
+            Sc += cosAqiQi;
+            Ss += sinAqiQi;
+            Mc += mikcos;
+            Ms += miksin;
+            Vc += qi2Qicos;
+            Vs += qi2Qisin;
+            cosrR = v3_add(cosrR, v3_scale(cosAqiQi, v3_sub(ri, rim)));
+            sinrR = v3_add(sinrR, v3_scale(sinAqiQi, v3_sub(ri, rim)));
+            if (p->torques) {
+                const v3 kjXmi = v3_cross(kj, I.d);
+                const v3 kjkjXQi = m3_cross(Kj, I.Q);
+                v3_st(kXmsina + 3 * i, v3_scale(sa, kjXmi));
+                v3_st(kXmcosa + 3 * i, v3_scale(ca, kjXmi));
+                v3_st(KXQcosa + 3 * i, v3_scale(ca, kjkjXQi));
+                v3_st(KXQsina + 3 * i, v3_scale(sa, kjkjXQi));
+            }
+        }
+        const double UqQ = pi4Volumer * expk[j] * (Sc * Sc + Ss * Ss);
+        const double UmqQ = pi4Volumer * expk[j] * (Ss * Mc - Sc * Ms);
+        const double Umm = pi4Volumer * expk[j] * (Mc * Mc + Ms * Ms);
+        const double tmpenergy = (UqQ + 2 * UmqQ + Umm) * COULOMBS;
+        out->long_energy += tmpenergy;
+        const double dUdVqQ = pi8Volumer * expk[j] * (Vc * Sc + Vs * Ss);
+        const double dUdVqQm = pi8Volumer * expk[j] * (Vs * Mc - Vc * Ms);
+        const double dUdV3rd = (dUdVqQ - 2.0 * UmqQ + dUdVqQm - 2.0 * Umm) * COULOMBS;
+        const double tkk = v3_dot(v3_scale(tmpenergy, kj), kj) * 2.0 * alphaR4;
+        out->long_deriv += -tmpenergy + tkk + dUdV3rd;
+        out->long_deriv2 += 4.0 * tmpenergy - v3_dot(v3_scale(7.0 * tmpenergy, kj), kj) * 2.0 * alphaR4 +
+                            v3_dot(v3_scale(tkk, kj), kj) * 2.0 * alphaR4;
+        double *lv = out->long_virial;
+        lv[XX] += tmpenergy - tmpenergy * coeff * kj.x * kj.x;
+        lv[YY] += tmpenergy - tmpenergy * coeff * kj.y * kj.y;
+        lv[ZZ] += tmpenergy - tmpenergy * coeff * kj.z * kj.z;
+        lv[XY] -= tmpenergy * coeff * kj.x * kj.y;
+        lv[XZ] -= tmpenergy * coeff * kj.x * kj.z;
+        lv[YZ] -= tmpenergy * coeff * kj.y * kj.z;
+        const double b = pi8Volumer * expk[j];
+        double Ff = 0.0, Ffk2 = 0.0, d2UV1 = 0.0, d2UrR = 0.0;
+        for (int i = 0; i < n; ++i) {
+            const double FqQ = b * (qQsin[i] * Sc - qQcos[i] * Ss);
+            const double FmiqQ = b * (mkcos[i] * Sc + mksin[i] * Ss);
+            const double FqQmi = -b * (qQcos[i] * Mc + qQsin[i] * Ms);
+            const double Fmm = b * (mksin[i] * Mc - mkcos[i] * Ms);
+            const double val = COULOMBS * (FqQ + FmiqQ + FqQmi + Fmm);
+            const double bkk = v3_dot(v3_scale(b, kj), kj);
+            const double d2u1 = bkk * (qQcos[i] * Sc + qQsin[i] * Ss);
+            const v3 rim = v3_ld(s->com + 3 * s->mol[i]);
+            const v3 ri = v3_ld(s->pos + 3 * i);
+            const v3 dr = v3_sub(ri, rim);
+            const double d2u2 = v3_dot(v3_scale(bkk, v3_add(v3_scale(qQcos[i], cosrR), v3_scale(qQsin[i], sinrR))), dr);
+            acc(force + 3 * i, v3_scale(val, kj));
+            Ff += val * v3_dot(kj, dr);
+            Ffk2 += v3_dot(v3_scale(2.0 * val, kj), kj) * 2.0 * alphaR4 * v3_dot(kj, dr);
+            d2UV1 += v3_dot(v3_scale(d2u1, dr), dr);
+            d2UrR += d2u2;
+            if (p->torques) {
+                const v3 kmc = v3_ld(kXmcosa + 3 * i), kms = v3_ld(kXmsina + 3 * i);
+                const v3 kqc = v3_ld(KXQcosa + 3 * i), kqs = v3_ld(KXQsina + 3 * i);
+                const v3 taum_qQ = v3_scale(b, v3_sub(v3_scale(Ss, kmc), v3_scale(Sc, kms)));
+                const v3 taum_m = v3_scale(b, v3_add(v3_scale(Mc, kmc), v3_scale(Ms, kms)));
+                const v3 tauQ_qQ = v3_scale(-2.0 * b, v3_add(v3_scale(Sc, kqc), v3_scale(Ss, kqs)));
+                const v3 tauQ_m = v3_scale(2.0 * b, v3_sub(v3_scale(Ms, kqc), v3_scale(Mc, kqs)));
+                acc(trq + 3 * i, v3_scale(COULOMBS, v3_add(v3_add(v3_add(taum_qQ, taum_m), tauQ_qQ), tauQ_m)));
+            }
+        }
+        if (p->use_constraint) {
+            out->long_deriv += Ff;
+            out->long_deriv2 += -6.0 * Ff + Ffk2 - d2UV1 + d2UrR;
+        }
+    }
+    out->long_virial[YX] = out->long_virial[XY];
+    out->long_virial[ZX] = out->long_virial[XZ];
+    out->long_virial[ZY] = out->long_virial[YZ];
+    if (p->torques) {
+        out->tot_torque += mp_torque_norms(n, trq);
+        out->long_deriv += mp_torques2forces(s, trq, force);
+    }
+    free(buf);
+    return 0;
+}

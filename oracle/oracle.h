@@ -250,6 +250,37 @@ typedef struct {
 int orc_induced_dipoles(const orc_sys *s, const double *dip, const double *quad, const double *polar,
                         const orc_ind_params *p, double *ei0, double *pi, int *iters, double *final_res);
 
+/* Permanent multipoles (charge, dipole, quadrupole) under Ewald:
+ * Multipole::compute and compute_long (src/forces/Multipole.cpp). */
+typedef struct {
+    double alpha;             /* get_alpha() */
+    double real_cut;          /* get_cutoffEw() */
+    int32_t use_constraint;   /* is_constraint_on() */
+    int32_t surface;          /* compute_surf_correct() */
+    int32_t torques;          /* compute_torques() */
+} orc_mp_params;
+typedef struct {
+    double self_energy, charged_energy;   /* myEnergy.correctEnergy = their sum */
+    double real_energy, intra_energy, surf_energy, long_energy;
+    double real_deriv, mol_deriv, surf_deriv, surf_deriv2, long_deriv, long_deriv2;
+    double tot_torque;        /* sum of |torque| over sites and parts (myEnergy.TotTorq) */
+    double virial[9];         /* Force::virial after compute(): real space + surface */
+    double long_virial[9];    /* the class's longVirial after compute_long() */
+} orc_mp_out;
+/* compute() (:163-216): self_correct (:338-395), real_term over the half list
+ * (:398-949), intraMolCorrect (:1290-1750), surfDipoleCorrect (:1753-1945),
+ * each part's torques turned into forces by torques2forces (:1946-2136).
+ * dip[3n]/quad[6n] are the lab-frame multipoles orc_local2global returns.
+ * force is accumulated.  Returns -1 unless every molecule has 3 sites, -3 on
+ * allocation failure. */
+int orc_multipole_short(const orc_sys *s, const int32_t *off, const int32_t *nb, const double *dip,
+                        const double *quad, const orc_mp_params *p, double *force, orc_mp_out *out);
+/* compute_long() / reciprocal_term (:218-240, :951-1279) over the k set of
+ * orc_ewald_kvectors (Multipole::set_up builds the same set, :242-302). */
+int orc_multipole_long(const orc_sys *s, int K, const double *kvec, const double *kmod2, const double *expk,
+                       const double *dip, const double *quad, const orc_mp_params *p, double *force,
+                       orc_mp_out *out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -81,6 +81,18 @@ class IndParams(C.Structure):
                 ("do_min", C.c_int32)]
 
 
+class MpParams(C.Structure):
+    _fields_ = [("alpha", C.c_double), ("real_cut", C.c_double), ("use_constraint", C.c_int32),
+                ("surface", C.c_int32), ("torques", C.c_int32)]
+
+
+class MpOut(C.Structure):
+    _fields_ = [(k, C.c_double) for k in ("self_energy", "charged_energy", "real_energy", "intra_energy",
+                                           "surf_energy", "long_energy", "real_deriv", "mol_deriv", "surf_deriv",
+                                           "surf_deriv2", "long_deriv", "long_deriv2", "tot_torque")] + \
+               [("virial", C.c_double * 9), ("long_virial", C.c_double * 9)]
+
+
 _lib = None
 
 
@@ -117,6 +129,10 @@ def lib():
         _lib.orc_md_step.argtypes = [P(OrcSys), P(MdParams), P(MdState)]
         _lib.orc_md_start.argtypes = [P(OrcSys), P(MdParams), P(MdState)]
         _lib.orc_local2global.argtypes = [P(OrcSys)] + [C.c_void_p] * 4
+        _lib.orc_multipole_short.argtypes = [P(OrcSys), C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                             P(MpParams), C.c_void_p, P(MpOut)]
+        _lib.orc_multipole_long.argtypes = [P(OrcSys), C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                            C.c_void_p, P(MpParams), C.c_void_p, P(MpOut)]
         _lib.orc_induced_dipoles.argtypes = [P(OrcSys), C.c_void_p, C.c_void_p, C.c_void_p, P(IndParams),
                                              C.c_void_p, C.c_void_p, P(C.c_int), P(C.c_double)]
     return _lib
@@ -298,6 +314,36 @@ class Oracle:
         if rc != 0:
             raise RuntimeError(f"orc_induced_dipoles failed: {rc}")
         return e0, pi, it.value, res.value
+
+    def multipole(self, dip, quad, *, alpha, real_cut, cutoff, cut_buff, kcut, use_constraint=True,
+                  surface=False, torques=True, use_cells=True):
+        """Multipole::compute() then compute_long(): short-range and k-space
+        forces kept apart, energies, Lustig derivatives and virials."""
+        n = self.s.n_atoms
+        dip = np.ascontiguousarray(dip, dtype=np.float64)
+        quad = np.ascontiguousarray(quad, dtype=np.float64)
+        p = MpParams(alpha, real_cut, int(use_constraint), int(surface), int(torques))
+        off, nb, _ = self.pairlist(cutoff, cut_buff, use_cells)
+        fs, fl = np.zeros((n, 3)), np.zeros((n, 3))
+        out = MpOut()
+        rc = lib().orc_multipole_short(C.byref(self._sys), _p(off), _p(nb), _p(dip), _p(quad), C.byref(p), _p(fs),
+                                       C.byref(out))
+        if rc != 0:
+            raise RuntimeError(f"orc_multipole_short failed: {rc}")
+        kvec, kmod2, expk = self.kvectors(kcut, alpha)
+        kvec = np.ascontiguousarray(kvec)
+        tot_short = out.tot_torque
+        out.tot_torque = 0.0
+        rc = lib().orc_multipole_long(C.byref(self._sys), len(kmod2), _p(kvec), _p(kmod2), _p(expk), _p(dip),
+                                      _p(quad), C.byref(p), _p(fl), C.byref(out))
+        if rc != 0:
+            raise RuntimeError(f"orc_multipole_long failed: {rc}")
+        r = {k: getattr(out, k) for k, _ in MpOut._fields_ if "virial" not in k}
+        r["tot_torque"] = tot_short + out.tot_torque
+        r["virial"] = np.array(out.virial[:])
+        r["long_virial"] = np.array(out.long_virial[:])
+        r["short_force"], r["long_force"] = fs, fl
+        return r
 
     # ---- integrator ------------------------------------------------------
     def init_velocity(self, temperature, seed, per_atom=True, ndf=None):

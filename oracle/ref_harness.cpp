@@ -6,6 +6,7 @@
 //
 // Usage:  ref_harness <dir> forces <force-list>   (lj,mie,buck,buff,ewald)
 //         ref_harness <dir> induction
+//         ref_harness <dir> multipole
 //         ref_harness <dir> pairlist
 //         ref_harness <dir> md <nsteps>
 //         ref_harness <dir> time <nsteps>
@@ -306,6 +307,54 @@ int main(int argc, char** argv) {
         write_bin("ind_pi", pi.data(), pi.size() * 8);
         meta.put("cutoff_ind", ens->myConfig->get_cutoffInd());
         meta.put("a_damp", ens->myConfig->get_adamping());
+    } else if (mode == "multipole") {
+        // Multipole::compute() then compute_long() on the same coordinates
+        // (src/forces/Multipole.cpp:163-240), forces of each written apart.
+        for (int m = 0; m < ens->nMols; ++m) ens->molecules[m].mass_center();
+        Multipole* mp = make_zeroed<Multipole>(ens);
+        zero_forces(ens);
+        std::memset(&ens->myLustig, 0, sizeof ens->myLustig);
+        std::memset(&ens->myEnergy, 0, sizeof ens->myEnergy);
+        double vir[9];
+        mp->compute();
+        mp->get_virial(vir);
+        std::vector<double> F = forces_of(ens);
+        write_bin("mp_short_force", F.data(), F.size() * 8);
+        write_bin("mp_short_virial", vir, sizeof vir);
+        meta.put("mp_real", ens->myEnergy.realEnergy);
+        meta.put("mp_correct", ens->myEnergy.correctEnergy);
+        meta.put("mp_molCorrect", ens->myEnergy.molCorrectEnergy);
+        meta.put("mp_surfCorrect", ens->myEnergy.surfCorrectEnergy);
+        meta.put("mp_realDeriv", ens->myLustig.realDeriv);
+        meta.put("mp_molDeriv", ens->myLustig.molDeriv);
+        meta.put("mp_surfDeriv", ens->myLustig.surfDeriv);
+        meta.put("mp_surfSecDeriv", ens->myLustig.surfSecDeriv);
+        const int n = ens->nAtoms;
+        std::vector<double> dip(3 * n), quad(6 * n);
+        for (int i = 0; i < n; ++i) {
+            dip[3 * i] = mp->myL2G->dip[i].x;
+            dip[3 * i + 1] = mp->myL2G->dip[i].y;
+            dip[3 * i + 2] = mp->myL2G->dip[i].z;
+            quad[6 * i] = mp->myL2G->Qxx[i];
+            quad[6 * i + 1] = mp->myL2G->Qxy[i];
+            quad[6 * i + 2] = mp->myL2G->Qxz[i];
+            quad[6 * i + 3] = mp->myL2G->Qyy[i];
+            quad[6 * i + 4] = mp->myL2G->Qyz[i];
+            quad[6 * i + 5] = mp->myL2G->Qzz[i];
+        }
+        write_bin("mp_dip", dip.data(), dip.size() * 8);
+        write_bin("mp_quad", quad.data(), quad.size() * 8);
+        zero_forces(ens);
+        mp->compute_long();
+        mp->get_long_virial(vir);
+        F = forces_of(ens);
+        write_bin("mp_long_force", F.data(), F.size() * 8);
+        write_bin("mp_long_virial", vir, sizeof vir);
+        meta.put("mp_long", ens->myEnergy.longEnergy);
+        meta.put("mp_longDeriv", ens->myLustig.longDeriv);
+        meta.put("mp_longSecDeriv", ens->myLustig.longSecDeriv);
+        meta.put("mp_totTorque", ens->myEnergy.TotTorq);
+        meta.put("cutoff_ew", ens->myConfig->get_cutoffEw());
     } else if (mode == "md" || mode == "time") {
         const int nsteps = argc > 3 ? std::atoi(argv[3]) : 1;
         // The reference never initialises Molecule::massCenter before the

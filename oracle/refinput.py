@@ -93,7 +93,8 @@ def write_config(path, s, *, cutoff, cut_buff, alpha=3.5, kcut=10.0, dt=0.002, n
                  lcorr=False, switch_on=False, switch_dist=0.0, coulomb=False, ewald=True,
                  surface=False, buff=False, n_buff=14, m_buff=7, delta=0.07, gamma=0.12,
                  hfactor=0.91, mie=False, n_mie=12, m_mie=6, lustig=False, cutoff_ew=None,
-                 induction=False, ind_damp=False, ind_min=False, cutoff_ind=None, a_damp=0.39):
+                 induction=False, ind_damp=False, ind_min=False, cutoff_ind=None, a_damp=0.39,
+                 multipole=False, torques=False, damp_amoeba=False):
     cutoff_ew = cutoff if cutoff_ew is None else cutoff_ew
     cutoff_ind = cutoff if cutoff_ind is None else cutoff_ind
     L = [
@@ -115,7 +116,8 @@ def write_config(path, s, *, cutoff, cut_buff, alpha=3.5, kcut=10.0, dt=0.002, n
         "f\t0.0\t0.0\t// computeharmoUB",
         f"{_b(lcorr)}\t{_b(switch_on)}\t{_g(switch_dist)}\t// LRC; switchOn; switchDistance",
         f"{_b(coulomb)}\t{_b(ewald)}\t{_b(surface)}\tf\t{_g(alpha)}\t{_g(kcut)}\t// coulomb ewald surface wolf alpha kcut",
-        (f"f\t{_b(induction)}\tf\t{_b(ind_damp)}\t{_b(ind_min)}\tf\t{_g(cutoff_ind)}\t{_g(a_damp)}"
+        (f"{_b(multipole)}\t{_b(induction)}\t{_b(torques)}\t{_b(ind_damp)}\t{_b(ind_min)}\t{_b(damp_amoeba)}"
+         f"\t{_g(cutoff_ind)}\t{_g(a_damp)}"
          "\t// multipole; induction; torques; dampInduction; inductionMin; dampAmoeba; cutoffInd; a"),
         f"{_b(buff)}\t{_g(n_buff)}\t{_g(m_buff)}\t{_g(delta)}\t{_g(gamma)}\t{_g(hfactor)}\t// buffered 14-7",
         f"{_b(mie)}\t{_g(n_mie)}\t{_g(m_mie)}\t// Mie",

@@ -114,6 +114,18 @@ CASES = [
      "induction", "",
      {"cutoff": 0.6, "cut_buff": 0.01, "induction": True, "ind_damp": False, "ind_min": False,
       "cutoff_ind": 0.55, "a_damp": 0.39, "ewald": False}),
+    ("amoeba216_multipole",
+     {"builder": "water", "args": {"n_mol": 216, "density_mol": 33.4, "seed": 41, "lattice": [6, 6, 6],
+                                   "r_oh": 0.09572, "hoh_deg": 108.5, "types": "amoeba"}},
+     "multipole", "",
+     {"cutoff": 0.7, "cut_buff": 0.05, "cutoff_ew": 0.65, "alpha": 3.0, "kcut": 14.0, "multipole": True,
+      "torques": True, "surface": True, "constraint": True, "ewald": False}),
+    ("amoeba216_multipole_flexible",
+     {"builder": "water", "args": {"n_mol": 216, "density_mol": 33.4, "seed": 43, "lattice": [6, 6, 6],
+                                   "r_oh": 0.09572, "hoh_deg": 108.5, "types": "amoeba"}},
+     "multipole", "",
+     {"cutoff": 0.6, "cut_buff": 0.1, "cutoff_ew": 0.7, "alpha": 3.4, "kcut": 12.0, "multipole": True,
+      "torques": True, "surface": False, "constraint": False, "ewald": False}),
     ("water512_md10",
      {"builder": "water", "args": {"n_mol": 512, "density_mol": 33.4, "seed": 5, "lattice": [8, 8, 8]}},
      "md", "10",

@@ -323,3 +323,27 @@ def test_induced_dipoles_solve_the_dense_system(oracle_mod):
             A[3 * j:3 * j + 3, 3 * i:3 * i + 3] = -T
     mu = np.linalg.solve(A, e0.ravel())
     assert pi.ravel() == pytest.approx(mu, rel=1e-9, abs=1e-15)
+
+
+def test_multipole_ewald_is_independent_of_alpha(oracle_mod):
+    """With both sums converged, the Ewald split parameter cannot change the
+    physics: the total multipole energy (real + k-space + self +
+    intramolecular correction) and, with the intramolecular forces on, the
+    total forces including the torque-derived ones must agree across alpha.
+    A wrong coefficient in the B_l or D_l recursions, the structure factors or
+    the self term, or a sign error in a torque, moves them apart."""
+    from swinmd_amd import systems as S
+    s = S.water(64, density_mol=33.4, seed=7, lattice=(4, 4, 4), r_oh=0.09572, hoh_deg=108.5, types=S.AMOEBA_TYPES)
+    o = _oracle(s)
+    dip, quad = o.local2global()
+    rc = 0.49 * s.box[0]
+    out = []
+    for alpha in (6.5, 7.5):
+        r = o.multipole(dip, quad, alpha=alpha, real_cut=rc, cutoff=rc, cut_buff=0.0, kcut=2 * alpha * 4.6,
+                        use_constraint=False, surface=False, torques=True, use_cells=False)
+        e = r["real_energy"] + r["long_energy"] + r["self_energy"] + r["intra_energy"]
+        out.append((e, r["short_force"] + r["long_force"], r["real_energy"], r["long_energy"]))
+    (e0, f0, re0, le0), (e1, f1, re1, le1) = out
+    assert abs(le1 - le0) > 100.0 and abs(re1 - re0) > 1.0   # the split really moved
+    assert e1 == pytest.approx(e0, rel=2e-6)
+    assert np.abs(f1 - f0).max() <= 1e-6 * np.abs(f0).max()

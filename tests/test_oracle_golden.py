@@ -133,3 +133,33 @@ def test_induction_frames_field_and_dipoles(oracle_mod, name):
     assert rel_rms(pi, fx.vec("ind_pi")) <= TOL
     if c["ind_min"]:
         assert iters > 5 and res < 1e-14
+
+
+MP_KEYS = (("real_energy", "mp_real"), ("intra_energy", "mp_molCorrect"), ("surf_energy", "mp_surfCorrect"),
+           ("long_energy", "mp_long"), ("real_deriv", "mp_realDeriv"), ("mol_deriv", "mp_molDeriv"),
+           ("surf_deriv", "mp_surfDeriv"), ("surf_deriv2", "mp_surfSecDeriv"), ("long_deriv", "mp_longDeriv"),
+           ("long_deriv2", "mp_longSecDeriv"))
+
+
+@pytest.mark.parametrize("name", ["amoeba216_multipole", "amoeba216_multipole_flexible"])
+def test_multipole_ewald(oracle_mod, name):
+    """Multipole::compute and compute_long (charges, dipoles, quadrupoles with
+    torques turned into forces): forces, virials, energies and Lustig
+    derivatives against the compiled reference.  Rigid water with the surface
+    term, and flexible water, whose intramolecular correction has forces and
+    torques."""
+    fx = Fixture(name)
+    c = fx.cfg
+    o = oracle_mod.Oracle(fx.system)
+    dip, quad = o.local2global()
+    np.testing.assert_array_equal(dip.ravel(), fx.ref["mp_dip"])
+    np.testing.assert_array_equal(quad.ravel(), fx.ref["mp_quad"])
+    r = o.multipole(dip, quad, alpha=c["alpha"], real_cut=c["cutoff_ew"], cutoff=c["cutoff"], cut_buff=c["cut_buff"],
+                    kcut=c["kcut"], use_constraint=c["constraint"], surface=c["surface"], torques=c["torques"])
+    assert rel_rms(r["short_force"], fx.vec("mp_short_force")) <= TOL
+    assert rel_rms(r["long_force"], fx.vec("mp_long_force")) <= TOL
+    assert rel_rms(r["virial"], fx.ref["mp_short_virial"]) <= TOL
+    assert rel_rms(r["long_virial"], fx.ref["mp_long_virial"]) <= TOL
+    for k, mk in MP_KEYS:
+        assert abs(r[k] - fx.meta[mk]) <= TOL * max(abs(fx.meta[mk]), 1e-12), k
+    assert rel(r["self_energy"] + r["charged_energy"], fx.meta["mp_correct"]) <= TOL
