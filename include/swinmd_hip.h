@@ -258,6 +258,50 @@ int smd_induce(smd_ctx* ctx, const smd_induce_params* p, smd_induce_report* repo
  * smd_induce; any pointer may be NULL. */
 int smd_get_induction(smd_ctx* ctx, double* dip, double* quad, double* e0, double* mu, int64_t n_atoms);
 
+/* ---- permanent multipoles under Ewald: class Multipole ------------------ */
+/* The sites' local multipoles come from smd_set_multipoles; each call below
+ * first rotates them into the lab frame (Local2Global::FrameofReference, as
+ * compute() and compute_long() do, src/forces/Multipole.cpp:176,222).  3-site
+ * water (H1, O, H2) only; FP64 only. */
+typedef struct {
+    double alpha;             /* SimConfiguration::get_alpha() */
+    double real_cut;          /* get_cutoffEw(); at most the pair list's cutoff + buffer */
+    int32_t use_constraint;   /* is_constraint_on() */
+    int32_t surface;          /* compute_surf_correct() */
+    int32_t torques;          /* compute_torques() */
+} smd_mp_params;
+
+/* Multipole::compute() results (:163-216). */
+typedef struct {
+    double self_energy, charged_energy;   /* myEnergy.correctEnergy = their sum */
+    double real_energy;                   /* myEnergy.realEnergy */
+    double intra_energy;                  /* myEnergy.molCorrectEnergy */
+    double surf_energy;                   /* myEnergy.surfCorrectEnergy */
+    double real_deriv, mol_deriv;         /* myLustig.realDeriv, molDeriv */
+    double surf_deriv, surf_deriv2;       /* myLustig.surfDeriv, surfSecDeriv */
+    double tot_torque;                    /* sum of |torque| over sites and parts */
+    double virial[9];                     /* Force::virial (caller applies ensemble.virial -= virial) */
+} smd_mp_out;
+
+/* Multipole::compute_long() results (:218-240). */
+typedef struct {
+    double energy;                        /* myEnergy.longEnergy */
+    double deriv, deriv2;                 /* myLustig.longDeriv, longSecDeriv */
+    double tot_torque;
+    double virial[9];                     /* the class's longVirial (all nine entries) */
+} smd_mp_long_out;
+
+/* Multipole ctor and set_up (:21-115, :242-302): parameters and the
+ * half-space k set k^2 < kcut^2; *n_kvec receives its size. */
+int smd_multipole_setup(smd_ctx* ctx, const smd_mp_params* p, double kcut, int32_t* n_kvec);
+/* Multipole::compute (:163-216): self_correct, real_term over the pair list,
+ * intraMolCorrect, surfDipoleCorrect, each with its torques turned into
+ * forces (torques2forces :1946-2136).  Forces are added to f_accum [3n] when
+ * it is not NULL.  prec: SMD_PREC_FP64. */
+int smd_multipole_compute(smd_ctx* ctx, int prec, double* f_accum, smd_mp_out* out);
+/* Multipole::compute_long / reciprocal_term (:218-240, :951-1279). */
+int smd_multipole_compute_long(smd_ctx* ctx, int prec, double* f_accum, smd_mp_long_out* out);
+
 /* ---- whole step: LFIntegrator::run for rigid point-charge water --------- */
 typedef struct {
     double cutoff, cut_buff;

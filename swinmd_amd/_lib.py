@@ -89,6 +89,22 @@ class InduceReport(C.Structure):
     _fields_ = [("iterations", C.c_int32), ("converged", C.c_int32), ("residual", C.c_double)]
 
 
+class MpParams(C.Structure):
+    _fields_ = [("alpha", C.c_double), ("real_cut", C.c_double), ("use_constraint", C.c_int32),
+                ("surface", C.c_int32), ("torques", C.c_int32)]
+
+
+class MpOut(C.Structure):
+    _fields_ = [(k, C.c_double) for k in ("self_energy", "charged_energy", "real_energy", "intra_energy",
+                                           "surf_energy", "real_deriv", "mol_deriv", "surf_deriv", "surf_deriv2",
+                                           "tot_torque")] + [("virial", C.c_double * 9)]
+
+
+class MpLongOut(C.Structure):
+    _fields_ = [("energy", C.c_double), ("deriv", C.c_double), ("deriv2", C.c_double), ("tot_torque", C.c_double),
+                ("virial", C.c_double * 9)]
+
+
 # name -> (restype, argtypes); every entry is a symbol declared in include/swinmd_hip.h
 _P = C.c_void_p
 SIGNATURES = {
@@ -135,6 +151,9 @@ SIGNATURES = {
     "smd_set_multipoles": (C.c_int, [_P, _P, _P, _P, C.c_int64]),
     "smd_induce": (C.c_int, [_P, C.POINTER(InduceParams), C.POINTER(InduceReport)]),
     "smd_get_induction": (C.c_int, [_P, _P, _P, _P, _P, C.c_int64]),
+    "smd_multipole_setup": (C.c_int, [_P, C.POINTER(MpParams), C.c_double, C.POINTER(C.c_int32)]),
+    "smd_multipole_compute": (C.c_int, [_P, C.c_int, _P, C.POINTER(MpOut)]),
+    "smd_multipole_compute_long": (C.c_int, [_P, C.c_int, _P, C.POINTER(MpLongOut)]),
 }
 
 _lib = None

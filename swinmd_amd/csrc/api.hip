@@ -67,7 +67,8 @@ static void release(Ctx* c) {
                     c->d_cell_atoms, c->d_list_seg, c->d_list_mask, c->d_nb, c->d_nb_cnt, c->d_nb_own, c->d_fvirt, c->d_kvec, c->d_kmod2, c->d_expk, c->d_kn, c->d_kgrp, c->d_ksplit, c->d_kf, c->d_e1,
                     c->d_sfac_part, c->d_vdw_a, c->d_vdw_b, c->d_vdw_c, c->d_epos, c->d_fa, c->d_fb,
                     c->d_host, c->d_dipl, c->d_quadl, c->d_polar, c->d_dipg, c->d_quadg, c->d_e0, c->d_mu,
-                    c->d_cg, c->d_cg_part, c->d_cg_state, c->d_cg_scal, c->d_frecip};
+                    c->d_cg, c->d_cg_part, c->d_cg_state, c->d_cg_scal, c->d_frecip, c->d_mp_trq,
+                    c->d_mp_terms, c->d_mp_kvec, c->d_mp_kmod2, c->d_mp_expk, c->d_mp_part, c->d_mp_kf};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     if (c->h_pinned) (void)hipHostFree(c->h_pinned);

@@ -418,6 +418,16 @@ static IndArgs ind_args(Ctx* c, const smd_induce_params* p) {
     return a;
 }
 
+// Lab-frame multipoles of every site from the local ones (k_frames).
+int frames_run(Ctx* c) {
+    smd_induce_params p{};
+    p.cutoff_ind = 1.0;
+    IndArgs a = ind_args(c, &p);
+    hipLaunchKernelGGL(k_frames, dim3((c->nmol + 255) / 256), dim3(256), 0, c->stream, a);
+    SMD_HIP(c, hipGetLastError());
+    return SMD_OK;
+}
+
 int induce_run(Ctx* c, const smd_induce_params* p, smd_induce_report* rep) {
     if (!c->multipoles_set) return fail(c, SMD_ESTATE, "multipoles not set");
     if (!c->list_valid) return fail(c, SMD_ESTATE, "no pair list");

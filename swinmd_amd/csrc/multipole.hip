@@ -1,0 +1,854 @@
+// multipole.hip -- permanent-multipole Ewald (charges, dipoles and traced
+// quadrupoles with torques) on gfx950.
+//
+//   Multipole::compute          src/forces/Multipole.cpp:163-216
+//   Multipole::compute_long     :218-240
+//   self_correct                :338-395
+//   real_term                   :398-949
+//   reciprocal_term             :951-1279
+//   intraMolCorrect             :1290-1750
+//   surfDipoleCorrect           :1753-1945
+//   torques2forces              :1946-2136
+//
+// Sites are water atoms H1, O, H2 with lab-frame multipoles from k_frames.
+//  * Real space walks the full neighbour rows of the pair list (both
+//    segments), so each row gathers the force and the torque on its own atom
+//    with no atomics.  The pair terms are written once, for the site that owns
+//    the row: the torque the reference puts on the partner of a pair is the
+//    same expression with the roles swapped, and the force is its negative.
+//    The owned segment adds the energy, virial and volume derivative.
+//  * Each part's torques (real space, intramolecular, surface, k-space) are
+//    kept apart and turned into forces per molecule, so each part reports its
+//    own dU/dV as the reference does.
+//  * k-space: a structure-factor pass forms 20 sums per k -- the six the
+//    reference forms and the moments from which its per-k force-pass sums
+//    (Ff, d2UV1, d2UrR) follow in closed form -- then a per-k finish, then a
+//    per-atom pass over all k with two coefficients per k:
+//      P = Sc - Ms,  Q = Ss + Mc  give  val = s (A P + Mu Q) + c (Mu P - A Q)
+//    and the torques (c Q - s P) k x mu and -2 (c P + s Q) (kk^T x Qi).
+// All arithmetic is FP64.
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+#include "smd_internal.h"
+
+namespace smd {
+
+constexpr double COULOMBS = 138.93548501;      // COUlOMBS, src/NEMD_defs.h:104
+constexpr double SQRT_COULOMB_MP = 11.78708976;
+constexpr int MP_LANES = 16;                   // lanes per atom row in real space
+constexpr int MP_NSUM = 20;                    // structure-factor sums per k
+constexpr int MP_SF_THREADS = 256;             // k per block of the structure-factor pass
+constexpr int MP_RF_THREADS = 256;             // atoms per block of the force pass
+constexpr int MP_KTILE = 128;                  // k staged per LDS tile in the force pass
+
+// Accumulator slots (FP64, zeroed per call).
+enum MpSlot {
+    M_SELF = 0, M_REAL_E, M_REAL_D1, M_REAL_VIR,                 // VIR: 9
+    M_INTRA_E = M_REAL_VIR + 9, M_MOL_D1,
+    M_SU_QR, M_SU_QRR = M_SU_QR + 3, M_SU_QM = M_SU_QRR + 3, M_SU_SELF = M_SU_QM + 3, M_SU_VIR,   // 3 3 3 1 9
+    M_DUDV = M_SU_VIR + 9,                                        // 4: real, intra, surface, long
+    M_TRQ = M_DUDV + 4,                                           // 4: |torque| sums per part
+    M_LONG_E = M_TRQ + 4, M_LONG_D1, M_LONG_D2, M_LONG_VIR,      // VIR: 6 (xx yy zz xy xz yz)
+    M_COUNT = M_LONG_VIR + 6
+};
+
+struct MpArgs {
+    int n, nmol, stride, K, chunks, per;
+    Box box;
+    const int32_t* mol_start;
+    const int32_t* mol;
+    const double* pos;
+    const double* com;
+    const double* q;            // scaled charges
+    const double* dip;          // lab frame [3n]
+    const double* quad;         // lab frame, traced / 3 [6n]: xx xy xz yy yz zz
+    const int32_t* nb;
+    const int32_t* nb_cnt;
+    const int32_t* nb_own;
+    const int32_t* excl_off;
+    const int32_t* excl;
+    double* force;
+    double* trq;                // [4][3n]: real, intra, surface, k-space
+    double* terms;              // [M_COUNT]
+    const double* kvec;         // [3K]
+    const double* kmod2;        // [K]
+    const double* expk;         // [K]
+    double* part;               // [chunks][K][MP_NSUM]
+    double* kf;                 // [K][2]: b P, b Q
+    double alpha, alpha2, alpha2PI, realcut2, volume, alphaR4;
+    int use_constraint, torques;
+};
+
+struct V3 { double x, y, z; };
+__device__ __forceinline__ V3 v3(double x, double y, double z) { return {x, y, z}; }
+__device__ __forceinline__ V3 operator+(V3 a, V3 b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+__device__ __forceinline__ V3 operator-(V3 a, V3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+__device__ __forceinline__ V3 operator-(V3 a) { return {-a.x, -a.y, -a.z}; }
+__device__ __forceinline__ V3 operator*(double f, V3 a) { return {f * a.x, f * a.y, f * a.z}; }
+__device__ __forceinline__ double dot(V3 a, V3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+__device__ __forceinline__ V3 cross(V3 a, V3 b) {
+    return {a.y * b.z - b.y * a.z, b.x * a.z - a.x * b.z, a.x * b.y - b.x * a.y};
+}
+__device__ __forceinline__ V3 ldv(const double* p) { return {p[0], p[1], p[2]}; }
+
+// Symmetric matrix as xx xy xz yy yz zz.
+struct S6 { double xx, xy, xz, yy, yz, zz; };
+__device__ __forceinline__ S6 lds6(const double* p) { return {p[0], p[1], p[2], p[3], p[4], p[5]}; }
+__device__ __forceinline__ V3 smul(const S6& Q, V3 r) {
+    return {Q.xx * r.x + Q.xy * r.y + Q.xz * r.z, Q.xy * r.x + Q.yy * r.y + Q.yz * r.z,
+            Q.xz * r.x + Q.yz * r.y + Q.zz * r.z};
+}
+__device__ __forceinline__ double trace(const S6& Q) { return Q.xx + Q.yy + Q.zz; }
+
+// cross(A, B, ones) of the reference (src/utils/Vector3.h:220-225) for
+// A = a b^T and B symmetric: sum over columns c of the cross product of
+// column c of A's rows, i.e. cross(a, B b) with B b the matrix-vector product.
+__device__ __forceinline__ V3 mcross_outer(V3 a, V3 b, const S6& B) { return cross(a, smul(B, b)); }
+// cross(A, B, ones) for A, B symmetric: x = sum_c A[1][c] B[2][c] - A[2][c] B[1][c], ...
+__device__ __forceinline__ V3 mcross_sym(const S6& A, const S6& B) {
+    return {A.xy * B.xz - A.xz * B.xy + A.yy * B.yz - A.yz * B.yy + A.yz * B.zz - A.zz * B.yz,
+            A.xz * B.xx - A.xx * B.xz + A.yz * B.xy - A.xy * B.yz + A.zz * B.xz - A.xz * B.zz,
+            A.xx * B.xy - A.xy * B.xx + A.xy * B.yy - A.yy * B.xy + A.xz * B.yz - A.yz * B.xz};
+}
+
+struct Site { double q; V3 d; S6 Q; };
+
+__device__ __forceinline__ Site load_site(const MpArgs& a, int i) {
+    Site s;
+    s.q = a.q[i] / SQRT_COULOMB_MP;
+    s.d = ldv(a.dip + 3 * i);
+    s.Q = lds6(a.quad + 6 * i);
+    return s;
+}
+
+// Multipole pair of "self" site I and partner J at r = r_I - r_J: the G_l
+// coefficients, the force on I in units of the B_l (or D_l) radial functions,
+// and the torque on I (real_term :548-617 and :730-877).  The torque the
+// reference assigns to the partner of a pair (Gt_j) is this expression with
+// the roles of the two sites swapped.
+struct PairOut { double G[5]; V3 gG[5]; V3 Gt[5]; };
+
+template <bool GRAD, bool TORQUE>
+__device__ __forceinline__ void mp_pair(const Site& I, const Site& J, V3 r, PairOut& o) {
+    const double qi = I.q, qj = J.q;
+    const V3 mi = I.d, mj = J.d;
+    const double QiI = trace(I.Q), QjI = trace(J.Q);
+    const V3 Qir = smul(I.Q, r), Qjr = smul(J.Q, r);
+    const double QiR = dot(r, Qir), QjR = dot(r, Qjr);
+    const double QiQj = I.Q.xx * J.Q.xx + I.Q.yy * J.Q.yy + I.Q.zz * J.Q.zz +
+                        2.0 * (I.Q.xy * J.Q.xy + I.Q.xz * J.Q.xz + I.Q.yz * J.Q.yz);
+    const double mir = dot(mi, r), mjr = dot(mj, r);
+    const double QimjXr = dot(mj, Qir), QjmiXr = dot(mi, Qjr);
+    const double QirQj = dot(Qir, Qjr);   // r Qi Qj r
+    o.G[0] = qi * qj;
+    o.G[1] = qj * mir - qi * mjr - dot(mi, mj) + qj * QiI + qi * QjI;
+    o.G[2] = qj * QiR + qi * QjR - mir * mjr - mjr * QiI + mir * QjI - 2.0 * QimjXr + 2.0 * QjmiXr + QiI * QjI +
+             2.0 * QiQj;
+    o.G[3] = mir * QjR - mjr * QiR + QiI * QjR + QjI * QiR + 4.0 * QirQj;
+    o.G[4] = QiR * QjR;
+    if constexpr (GRAD) {
+        const V3 Qimj = smul(I.Q, mj), Qjmi = smul(J.Q, mi);
+        const V3 QiQjr = smul(I.Q, Qjr), QjQir = smul(J.Q, Qir);
+        o.gG[1] = qj * mi - qi * mj;
+        o.gG[2] = (2.0 * qj) * Qir - mjr * mi - mir * mj - 2.0 * Qimj - QiI * mj + (2.0 * qi) * Qjr + 2.0 * Qjmi +
+                  QjI * mi;
+        o.gG[3] = (-QiR) * mj - (2.0 * mjr) * Qir + QjR * mi + (2.0 * mir) * Qjr + (2.0 * QiI) * Qjr +
+                  (2.0 * QjI) * Qir + 4.0 * QiQjr + 4.0 * QjQir;
+        o.gG[4] = (2.0 * QjR) * Qir + (2.0 * QiR) * Qjr;
+    }
+    if constexpr (TORQUE) {
+        const V3 rXmi = cross(r, mi);
+        const V3 mjXmi = cross(mj, mi);
+        const V3 QjrXmi = cross(Qjr, mi);
+        const V3 RXQi = mcross_outer(r, r, I.Q);          // cross(r r^T, Qi)
+        const V3 mjrXQi = mcross_outer(mj, r, I.Q);       // cross(mj r^T, Qi)
+        const V3 rmjXQi = mcross_outer(r, mj, I.Q);       // cross(r mj^T, Qi)
+        const V3 QjXQi = mcross_sym(J.Q, I.Q);
+        const V3 rQjrXQi = mcross_outer(r, Qjr, I.Q);     // cross(r (Qj r)^T, Qi)
+        const V3 QjrrXQi = mcross_outer(Qjr, r, I.Q);     // cross((Qj r) r^T, Qi)
+        o.Gt[1] = qj * rXmi - mjXmi;
+        o.Gt[2] = (-mjr) * rXmi + 2.0 * QjrXmi + QjI * rXmi + (2.0 * qj) * RXQi - 2.0 * mjrXQi - 2.0 * rmjXQi +
+                  4.0 * QjXQi;
+        o.Gt[3] = QjR * rXmi - (2.0 * mjr) * RXQi + (2.0 * QjI) * RXQi + 4.0 * rQjrXQi + 4.0 * QjrrXQi;
+        o.Gt[4] = (2.0 * QjR) * RXQi;
+    }
+}
+
+// erfc-screened radial functions B_0..B_5 (:629-634)
+__device__ __forceinline__ void radial_b(const MpArgs& a, double r2, double (&B)[6]) {
+    const double r = sqrt(r2);
+    const double e = exp(-a.alpha2 * r2);
+    const double inv = 1.0 / r2;
+    B[0] = erfc(a.alpha * r) / r;
+    B[1] = (-B[0] - a.alpha2PI * e) * inv;
+    B[2] = (-3.0 * B[1] + 2.0 * a.alpha2PI * a.alpha2 * e) * inv;
+    B[3] = (-5.0 * B[2] - 4.0 * a.alpha2PI * a.alpha2 * a.alpha2 * e) * inv;
+    B[4] = (-7.0 * B[3] + 8.0 * a.alpha2PI * a.alpha2 * a.alpha2 * a.alpha2 * e) * inv;
+    B[5] = (-9.0 * B[4] - 16.0 * a.alpha2PI * a.alpha2 * a.alpha2 * a.alpha2 * a.alpha2 * e) * inv;
+}
+
+// ---- real space ---------------------------------------------------------------
+
+template <bool FULL>
+__device__ __forceinline__ void real_entry(const MpArgs& a, const Site& I, V3 ri, int j, V3& f, V3& t,
+                                           double (&acc)[11]) {
+    double dx = ri.x - a.pos[3 * j], dy = ri.y - a.pos[3 * j + 1], dz = ri.z - a.pos[3 * j + 2];
+    pbc(dx, dy, dz, a.box);
+    const double r2 = dx * dx + dy * dy + dz * dz;
+    if (!(r2 < a.realcut2)) return;
+    const Site J = load_site(a, j);
+    const V3 r = v3(dx, dy, dz);
+    PairOut o;
+    double B[6];
+    radial_b(a, r2, B);
+    if (a.torques) mp_pair<true, true>(I, J, r, o);
+    else mp_pair<true, false>(I, J, r, o);
+    const double radial = o.G[0] * B[1] + o.G[1] * B[2] + o.G[2] * B[3] + o.G[3] * B[4] + o.G[4] * B[5];
+    const V3 fij = -(B[1] * o.gG[1] + B[2] * o.gG[2] + B[3] * o.gG[3] + B[4] * o.gG[4]) - radial * r;
+    f = f + fij;
+    if (a.torques) t = t + (B[1] * o.Gt[1] + B[2] * o.Gt[2] + B[3] * o.Gt[3] + B[4] * o.Gt[4]);
+    if constexpr (FULL) {
+        acc[0] += (o.G[0] * B[0] + o.G[1] * B[1] + o.G[2] * B[2] + o.G[3] * B[3] + o.G[4] * B[4]);
+        acc[1] += -dot(fij, r);
+        acc[2] += fij.x * r.x; acc[3] += fij.y * r.x; acc[4] += fij.z * r.x;
+        acc[5] += fij.x * r.y; acc[6] += fij.y * r.y; acc[7] += fij.z * r.y;
+        acc[8] += fij.x * r.z; acc[9] += fij.y * r.z; acc[10] += fij.z * r.z;
+    }
+}
+
+template <int G>
+__global__ __launch_bounds__(256) void k_mp_real(MpArgs a) {
+    const int tid = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = tid / G, sub = tid % G;
+    double acc[11];
+#pragma unroll
+    for (int k = 0; k < 11; ++k) acc[k] = 0.0;
+    V3 f = v3(0, 0, 0), t = v3(0, 0, 0);
+    if (i < a.n) {
+        const int32_t* row = a.nb + (int64_t)i * a.stride;
+        const int own = min(a.nb_own[i], a.stride);
+        const int non = min(a.nb_cnt[i] - a.nb_own[i], a.stride - own);
+        const Site I = load_site(a, i);
+        const V3 ri = ldv(a.pos + 3 * i);
+        for (int k = sub; k < own; k += G) real_entry<true>(a, I, ri, row[k], f, t, acc);
+        const int32_t* back = row + a.stride - 1;
+        for (int k = sub; k < non; k += G) real_entry<false>(a, I, ri, back[-k], f, t, acc);
+    }
+#pragma unroll
+    for (int o = G / 2; o > 0; o >>= 1) {
+        f.x += __shfl_xor(f.x, o, G); f.y += __shfl_xor(f.y, o, G); f.z += __shfl_xor(f.z, o, G);
+        t.x += __shfl_xor(t.x, o, G); t.y += __shfl_xor(t.y, o, G); t.z += __shfl_xor(t.z, o, G);
+    }
+    if (i < a.n && sub == 0) {
+        a.force[3 * i] += COULOMBS * f.x;
+        a.force[3 * i + 1] += COULOMBS * f.y;
+        a.force[3 * i + 2] += COULOMBS * f.z;
+        double* tr = a.trq;   // real-space part
+        tr[3 * i] = COULOMBS * t.x;
+        tr[3 * i + 1] = COULOMBS * t.y;
+        tr[3 * i + 2] = COULOMBS * t.z;
+    }
+    acc[0] *= COULOMBS;
+    acc[1] *= COULOMBS;
+    block_add<11>(acc, a.terms + M_REAL_E);
+}
+
+// ---- self, intramolecular, surface --------------------------------------------
+
+// Per site: the self term (:356-392) and the surface sums (:1785-1830).
+__global__ __launch_bounds__(256) void k_mp_site_sums(MpArgs a) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    double v[11] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+    if (i < a.n) {
+        const Site I = load_site(a, i);
+        const double QiI = trace(I.Q);
+        const double QiQi = I.Q.xx * I.Q.xx + I.Q.yy * I.Q.yy + I.Q.zz * I.Q.zz +
+                            2.0 * (I.Q.xy * I.Q.xy + I.Q.xz * I.Q.xz + I.Q.yz * I.Q.yz);
+        const double selfterm = I.q * I.q * a.alpha2PI +
+                                (dot(I.d, I.d) - 2.0 * I.q * QiI) * 2.0 * a.alpha2PI * a.alpha2 / 3.0 +
+                                (QiI * QiI + 2.0 * QiQi) * 4.0 * a.alpha2PI * a.alpha2 * a.alpha2 / 5.0;
+        v[0] = -0.5 * selfterm * COULOMBS;
+        const V3 r = ldv(a.pos + 3 * i), rim = ldv(a.com + 3 * a.mol[i]);
+        v[1] = I.q * r.x; v[2] = I.q * r.y; v[3] = I.q * r.z;              // q r
+        v[4] = I.q * rim.x; v[5] = I.q * rim.y; v[6] = I.q * rim.z;        // q R_mol
+        v[7] = I.q * r.x + I.d.x; v[8] = I.q * r.y + I.d.y; v[9] = I.q * r.z + I.d.z;   // q r + mu
+        v[10] = 2.0 * I.q * QiI - dot(I.d, I.d);
+    }
+    double self[1] = {v[0]};
+    block_add<1>(self, a.terms + M_SELF);
+    __syncthreads();
+    double su[10];
+#pragma unroll
+    for (int k = 0; k < 10; ++k) su[k] = v[k + 1];
+    block_add<10>(su, a.terms + M_SU_QR);
+}
+
+// One thread per molecule: excluded pairs inside it with erf-screened D_l
+// (intraMolCorrect :1350-1600).  Forces and torques only without constraints.
+__global__ __launch_bounds__(256) void k_mp_intra(MpArgs a) {
+    const int m = blockIdx.x * blockDim.x + threadIdx.x;
+    double v[2] = {0.0, 0.0};
+    if (m < a.nmol) {
+        const int a0 = a.mol_start[m], ns = a.mol_start[m + 1] - a0;
+        V3 tq[3] = {v3(0, 0, 0), v3(0, 0, 0), v3(0, 0, 0)};
+        for (int x = 0; x < ns; ++x) {
+            const int idi = a0 + x;
+            const Site I = load_site(a, idi);
+            for (int y = x + 1; y < ns; ++y) {
+                const int idj = a0 + y;
+                bool ex = false;
+                for (int k = a.excl_off[idi]; k < a.excl_off[idi + 1]; ++k) ex = ex || a.excl[k] == idj;
+                if (!ex) continue;
+                double dx = a.pos[3 * idi] - a.pos[3 * idj], dy = a.pos[3 * idi + 1] - a.pos[3 * idj + 1],
+                       dz = a.pos[3 * idi + 2] - a.pos[3 * idj + 2];
+                pbc(dx, dy, dz, a.box);
+                const double r2 = dx * dx + dy * dy + dz * dz, r = sqrt(r2);
+                const V3 rv = v3(dx, dy, dz);
+                const Site J = load_site(a, idj);
+                const double e = exp(-a.alpha2 * r2), inv = 1.0 / r2;
+                double D[6];
+                D[0] = erf(a.alpha * r) / r;
+                D[1] = (-D[0] + a.alpha2PI * e) * inv;
+                D[2] = (-3.0 * D[1] - 2.0 * a.alpha2PI * a.alpha2 * e) * inv;
+                D[3] = (-5.0 * D[2] + 4.0 * a.alpha2PI * a.alpha2 * a.alpha2 * e) * inv;
+                D[4] = (-7.0 * D[3] - 8.0 * a.alpha2PI * a.alpha2 * a.alpha2 * a.alpha2 * e) * inv;
+                D[5] = (-9.0 * D[4] + 16.0 * a.alpha2PI * a.alpha2 * a.alpha2 * a.alpha2 * a.alpha2 * e) * inv;
+                PairOut o;
+                if (a.use_constraint) {
+                    mp_pair<false, false>(I, J, rv, o);
+                } else {
+                    if (a.torques) mp_pair<true, true>(I, J, rv, o);
+                    else mp_pair<true, false>(I, J, rv, o);
+                }
+                v[0] += o.G[0] * D[0] + o.G[1] * D[1] + o.G[2] * D[2] + o.G[3] * D[3] + o.G[4] * D[4];
+                if (!a.use_constraint) {
+                    const double radial = o.G[0] * D[1] + o.G[1] * D[2] + o.G[2] * D[3] + o.G[3] * D[4] + o.G[4] * D[5];
+                    const V3 fij = D[1] * o.gG[1] + D[2] * o.gG[2] + D[3] * o.gG[3] + D[4] * o.gG[4] + radial * rv;
+                    v[1] += -dot(fij, rv) * COULOMBS / (3.0 * a.volume);
+                    a.force[3 * idi] += COULOMBS * fij.x;
+                    a.force[3 * idi + 1] += COULOMBS * fij.y;
+                    a.force[3 * idi + 2] += COULOMBS * fij.z;
+                    a.force[3 * idj] -= COULOMBS * fij.x;
+                    a.force[3 * idj + 1] -= COULOMBS * fij.y;
+                    a.force[3 * idj + 2] -= COULOMBS * fij.z;
+                    if (a.torques) {
+                        tq[x] = tq[x] - COULOMBS * (D[1] * o.Gt[1] + D[2] * o.Gt[2] + D[3] * o.Gt[3] + D[4] * o.Gt[4]);
+                        PairOut oj;
+                        mp_pair<false, true>(J, I, -rv, oj);
+                        tq[y] = tq[y] -
+                                COULOMBS * (D[1] * oj.Gt[1] + D[2] * oj.Gt[2] + D[3] * oj.Gt[3] + D[4] * oj.Gt[4]);
+                    }
+                }
+            }
+        }
+        double* tr = a.trq + 3 * (size_t)a.n;   // intramolecular part
+        for (int x = 0; x < ns; ++x) {
+            tr[3 * (a0 + x)] = tq[x].x;
+            tr[3 * (a0 + x) + 1] = tq[x].y;
+            tr[3 * (a0 + x) + 2] = tq[x].z;
+        }
+    }
+    v[0] *= -COULOMBS;
+    block_add<2>(v, a.terms + M_INTRA_E);
+}
+
+// Surface force, virial and torque per site (:1846-1900), from the sums.
+__global__ __launch_bounds__(256) void k_mp_surface(MpArgs a, double coeff) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    double v[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    if (i < a.n) {
+        const V3 qr = ldv(a.terms + M_SU_QR), qm = ldv(a.terms + M_SU_QM);
+        const Site I = load_site(a, i);
+        const V3 ri = ldv(a.pos + 3 * i);
+        const V3 fi = (-2.0 * coeff * I.q) * qm;
+        const V3 temp = (-a.q[i]) * qr;
+        a.force[3 * i] += COULOMBS * fi.x;
+        a.force[3 * i + 1] += COULOMBS * fi.y;
+        a.force[3 * i + 2] += COULOMBS * fi.z;
+        v[0] = temp.x * ri.x; v[1] = temp.y * ri.x; v[2] = temp.z * ri.x;
+        v[3] = temp.x * ri.y; v[4] = temp.y * ri.y; v[5] = temp.z * ri.y;
+        v[6] = temp.x * ri.z; v[7] = temp.y * ri.z; v[8] = temp.z * ri.z;
+        const V3 tq = (2.0 * coeff * COULOMBS) * cross(qm, I.d);
+        double* tr = a.trq + 6 * (size_t)a.n;   // surface part
+        tr[3 * i] = a.torques ? tq.x : 0.0;
+        tr[3 * i + 1] = a.torques ? tq.y : 0.0;
+        tr[3 * i + 2] = a.torques ? tq.z : 0.0;
+    }
+    block_add<9>(v, a.terms + M_SU_VIR);
+}
+
+// ---- torques to forces ----------------------------------------------------------
+
+// Frame of a hydrogen site: axis from O, second vector x (y when the axis is
+// within ~30 deg of x).  dU/du of the torque (:1988-2022).
+__device__ __forceinline__ V3 h_site(V3 rh, V3 ro, V3 tau, double& dudv) {
+    V3 u = rh - ro;
+    const double ul = sqrt(dot(u, u));
+    V3 v = fabs(u.x / ul) > 0.8666 ? v3(0.0, 1.0, 0.0) : v3(1.0, 0.0, 0.0);
+    V3 w = cross(u, v);
+    const double wl = sqrt(dot(w, w));
+    u = (1.0 / ul) * u;
+    w = (1.0 / wl) * w;
+    const V3 vxu = cross(v, u), wxu = cross(w, u);
+    const double vxuL2 = dot(vxu, vxu), wxuL2 = dot(wxu, wxu);
+    const V3 dUdu = (dot(-v, tau) / (vxuL2 * ul)) * vxu + (dot(-w, tau) / (wxuL2 * ul)) * wxu;
+    dudv += dot(dUdu, u) * ul;
+    return dUdu;
+}
+
+// One thread per water; parts [p0, p1) of the torque array, each part's
+// dU/dV summed into its own slot.
+__global__ __launch_bounds__(256) void k_mp_torques(MpArgs a, int p0, int p1) {
+    const int m = blockIdx.x * blockDim.x + threadIdx.x;
+    double dv[4] = {0, 0, 0, 0}, tn[4] = {0, 0, 0, 0};
+    if (m < a.nmol) {
+        const int h1 = a.mol_start[m], o = h1 + 1, h2 = h1 + 2;
+        const V3 r1 = ldv(a.pos + 3 * h1), ro = ldv(a.pos + 3 * o), r2 = ldv(a.pos + 3 * h2);
+        V3 F1 = v3(0, 0, 0), FO = v3(0, 0, 0), F2 = v3(0, 0, 0);
+        for (int p = p0; p < p1; ++p) {
+            const double* tr = a.trq + 3 * (size_t)a.n * p;
+            const V3 t1 = ldv(tr + 3 * h1), to = ldv(tr + 3 * o), t2 = ldv(tr + 3 * h2);
+            tn[p] = sqrt(dot(t1, t1)) + sqrt(dot(to, to)) + sqrt(dot(t2, t2));
+            // H1
+            V3 d = h_site(r1, ro, t1, dv[p]);
+            F1 = F1 - d;
+            FO = FO + d;
+            // O: bisector-type frame from the two O-H vectors (:2024-2072)
+            {
+                V3 u = ro - r1, v = ro - r2;
+                V3 w = cross(u, v);
+                const double ul = sqrt(dot(u, u)), vl = sqrt(dot(v, v)), wl = sqrt(dot(w, w));
+                u = (1.0 / ul) * u;
+                v = (1.0 / vl) * v;
+                w = (1.0 / wl) * w;
+                const V3 vxu = cross(v, u), wxu = cross(w, u), wxv = cross(w, v);
+                const double vxuL2 = dot(vxu, vxu), wxuL2 = dot(wxu, wxu), wxvL2 = dot(wxv, wxv);
+                const V3 dUdu = (dot(-v, to) / (vxuL2 * ul)) * vxu + (0.5 * dot(-w, to) / (wxuL2 * ul)) * wxu;
+                const V3 dUdv = (dot(-u, to) / (vxuL2 * vl)) * (-vxu) + (0.5 * dot(-w, to) / (wxvL2 * vl)) * wxv;
+                FO = FO - dUdu - dUdv;
+                F1 = F1 + dUdu;
+                F2 = F2 + dUdv;
+                dv[p] += dot(dUdu, u) * ul + dot(dUdv, v) * vl;
+            }
+            // H2
+            d = h_site(r2, ro, t2, dv[p]);
+            F2 = F2 - d;
+            FO = FO + d;
+        }
+        a.force[3 * h1] += F1.x; a.force[3 * h1 + 1] += F1.y; a.force[3 * h1 + 2] += F1.z;
+        a.force[3 * o] += FO.x; a.force[3 * o + 1] += FO.y; a.force[3 * o + 2] += FO.z;
+        a.force[3 * h2] += F2.x; a.force[3 * h2 + 1] += F2.y; a.force[3 * h2 + 2] += F2.z;
+    }
+    block_add<4>(dv, a.terms + M_DUDV);
+    __syncthreads();
+    block_add<4>(tn, a.terms + M_TRQ);
+}
+
+// ---- k-space ---------------------------------------------------------------------
+
+// Pass 1: thread per k over one chunk of sites; 20 partial sums per (chunk, k):
+//   0 Sc = sum (q - kQk) cos      1 Ss = sum (q - kQk) sin
+//   2 Mc = sum (mu.k) cos         3 Ms = sum (mu.k) sin
+//   4 Vc = sum 2 kQk cos          5 Vs = sum 2 kQk sin
+//   6..8 sum (q - kQk) cos d     9..11 sum (q - kQk) sin d      (d = r - R_mol)
+//  12..14 sum (mu.k) cos d      15..17 sum (mu.k) sin d
+//  18 sum (q - kQk) cos |d|^2   19 sum (q - kQk) sin |d|^2
+__global__ __launch_bounds__(MP_SF_THREADS) void k_mp_sfac(MpArgs a) {
+    __shared__ double sh[MP_SF_THREADS][14];   // r(3) q mu(3) Q(6) | d staged separately
+    __shared__ double shd[MP_SF_THREADS][3];
+    const int k = blockIdx.x * MP_SF_THREADS + threadIdx.x;
+    const int chunk = blockIdx.y;
+    const int i0 = chunk * a.per, i1 = min(a.n, i0 + a.per);
+    V3 kv = v3(0, 0, 0);
+    if (k < a.K) kv = ldv(a.kvec + 3 * k);
+    double s[MP_NSUM];
+#pragma unroll
+    for (int x = 0; x < MP_NSUM; ++x) s[x] = 0.0;
+    for (int t0 = i0; t0 < i1; t0 += MP_SF_THREADS) {
+        const int i = t0 + threadIdx.x;
+        __syncthreads();
+        if (i < i1) {
+            double* p = sh[threadIdx.x];
+            p[0] = a.pos[3 * i]; p[1] = a.pos[3 * i + 1]; p[2] = a.pos[3 * i + 2];
+            p[3] = a.q[i] / SQRT_COULOMB_MP;
+            p[4] = a.dip[3 * i]; p[5] = a.dip[3 * i + 1]; p[6] = a.dip[3 * i + 2];
+            for (int c = 0; c < 6; ++c) p[7 + c] = a.quad[6 * i + c];
+            const int m = a.mol[i];
+            shd[threadIdx.x][0] = p[0] - a.com[3 * m];
+            shd[threadIdx.x][1] = p[1] - a.com[3 * m + 1];
+            shd[threadIdx.x][2] = p[2] - a.com[3 * m + 2];
+        }
+        __syncthreads();
+        const int cnt = min(MP_SF_THREADS, i1 - t0);
+        if (k < a.K) {
+            for (int x = 0; x < cnt; ++x) {
+                const double* p = sh[x];
+                const double ph = p[0] * kv.x + p[1] * kv.y + p[2] * kv.z;
+                double sn, cs;
+                sincos(ph, &sn, &cs);
+                const double QK = p[7] * kv.x * kv.x + p[10] * kv.y * kv.y + p[12] * kv.z * kv.z +
+                                  2.0 * (p[8] * kv.x * kv.y + p[9] * kv.x * kv.z + p[11] * kv.y * kv.z);
+                const double A = p[3] - QK;
+                const double Mu = p[4] * kv.x + p[5] * kv.y + p[6] * kv.z;
+                const double ac = A * cs, as = A * sn, mc = Mu * cs, ms = Mu * sn;
+                const double dx = shd[x][0], dy = shd[x][1], dz = shd[x][2];
+                const double d2 = dx * dx + dy * dy + dz * dz;
+                s[0] += ac; s[1] += as; s[2] += mc; s[3] += ms;
+                s[4] += 2.0 * QK * cs; s[5] += 2.0 * QK * sn;
+                s[6] += ac * dx; s[7] += ac * dy; s[8] += ac * dz;
+                s[9] += as * dx; s[10] += as * dy; s[11] += as * dz;
+                s[12] += mc * dx; s[13] += mc * dy; s[14] += mc * dz;
+                s[15] += ms * dx; s[16] += ms * dy; s[17] += ms * dz;
+                s[18] += ac * d2; s[19] += as * d2;
+            }
+        }
+    }
+    if (k < a.K) {
+        double* out = a.part + ((size_t)chunk * a.K + k) * MP_NSUM;
+#pragma unroll
+        for (int x = 0; x < MP_NSUM; ++x) out[x] = s[x];
+    }
+}
+
+// Pass 2: per k, the chunk sums in fixed order, then energy, virial, volume
+// derivatives (:1098-1138 and the force-pass sums :1180-1188 from the
+// moments) and the force-pass coefficients (b P, b Q).
+__global__ __launch_bounds__(256) void k_mp_kfinish(MpArgs a) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    double v[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    if (k < a.K) {
+        double S[MP_NSUM];
+#pragma unroll
+        for (int x = 0; x < MP_NSUM; ++x) S[x] = 0.0;
+        for (int c = 0; c < a.chunks; ++c) {
+            const double* p = a.part + ((size_t)c * a.K + k) * MP_NSUM;
+#pragma unroll
+            for (int x = 0; x < MP_NSUM; ++x) S[x] += p[x];
+        }
+        const V3 kv = ldv(a.kvec + 3 * k);
+        const double pi4V = 4.0 * PI / a.volume, pi8V = 2.0 * pi4V;
+        const double ek = a.expk[k];
+        const double Sc = S[0], Ss = S[1], Mc = S[2], Ms = S[3], Vc = S[4], Vs = S[5];
+        const double UqQ = pi4V * ek * (Sc * Sc + Ss * Ss);
+        const double UmqQ = pi4V * ek * (Ss * Mc - Sc * Ms);
+        const double Umm = pi4V * ek * (Mc * Mc + Ms * Ms);
+        const double t = (UqQ + 2.0 * UmqQ + Umm) * COULOMBS;
+        const double dUdVqQ = pi8V * ek * (Vc * Sc + Vs * Ss);
+        const double dUdVqQm = pi8V * ek * (Vs * Mc - Vc * Ms);
+        const double d3 = (dUdVqQ - 2.0 * UmqQ + dUdVqQm - 2.0 * Umm) * COULOMBS;
+        const double k2 = dot(kv, kv);
+        const double x = k2 * 2.0 * a.alphaR4;
+        double d1 = -t + t * x + d3;
+        double d2 = 4.0 * t - 7.0 * t * x + t * x * x;
+        const double coeff = 2.0 * (a.alphaR4 + 1.0 / a.kmod2[k]);
+        const double b = pi8V * ek;
+        const double P = Sc - Ms, Q = Ss + Mc;
+        if (a.use_constraint) {
+            const V3 cosrR = v3(S[6], S[7], S[8]), sinrR = v3(S[9], S[10], S[11]);
+            const V3 mcR = v3(S[12], S[13], S[14]), msR = v3(S[15], S[16], S[17]);
+            const double Ff = COULOMBS * b * (P * dot(kv, sinrR + mcR) + Q * dot(kv, msR - cosrR));
+            const double Ffk2 = 2.0 * k2 * 2.0 * a.alphaR4 * Ff;
+            const double d2UV1 = b * k2 * (Sc * S[18] + Ss * S[19]);
+            const double d2UrR = b * k2 * (dot(cosrR, cosrR) + dot(sinrR, sinrR));
+            d1 += Ff;
+            d2 += -6.0 * Ff + Ffk2 - d2UV1 + d2UrR;
+        }
+        v[0] = t; v[1] = d1; v[2] = d2;
+        v[3] = t - t * coeff * kv.x * kv.x;
+        v[4] = t - t * coeff * kv.y * kv.y;
+        v[5] = t - t * coeff * kv.z * kv.z;
+        v[6] = -t * coeff * kv.x * kv.y;
+        v[7] = -t * coeff * kv.x * kv.z;
+        v[8] = -t * coeff * kv.y * kv.z;
+        a.kf[2 * k] = b * P;
+        a.kf[2 * k + 1] = b * Q;
+    }
+    block_add<9>(v, a.terms + M_LONG_E);
+}
+
+// Pass 3: thread per site, every k (staged in LDS tiles): force val k and the
+// torque of the dipole and the quadrupole (:1149-1210).
+__global__ __launch_bounds__(MP_RF_THREADS) void k_mp_kforce(MpArgs a) {
+    __shared__ double sk[MP_KTILE][5];
+    const int i = blockIdx.x * MP_RF_THREADS + threadIdx.x;
+    Site I{};
+    V3 ri = v3(0, 0, 0);
+    if (i < a.n) {
+        I = load_site(a, i);
+        ri = ldv(a.pos + 3 * i);
+    }
+    V3 F = v3(0, 0, 0), T = v3(0, 0, 0);
+    for (int k0 = 0; k0 < a.K; k0 += MP_KTILE) {
+        const int cnt = min(MP_KTILE, a.K - k0);
+        __syncthreads();
+        for (int x = threadIdx.x; x < cnt; x += MP_RF_THREADS) {
+            sk[x][0] = a.kvec[3 * (k0 + x)];
+            sk[x][1] = a.kvec[3 * (k0 + x) + 1];
+            sk[x][2] = a.kvec[3 * (k0 + x) + 2];
+            sk[x][3] = a.kf[2 * (k0 + x)];
+            sk[x][4] = a.kf[2 * (k0 + x) + 1];
+        }
+        __syncthreads();
+        if (i < a.n) {
+            for (int x = 0; x < cnt; ++x) {
+                const V3 kv = v3(sk[x][0], sk[x][1], sk[x][2]);
+                const double bP = sk[x][3], bQ = sk[x][4];
+                double sn, cs;
+                sincos(dot(ri, kv), &sn, &cs);
+                const V3 Qk = smul(I.Q, kv);
+                const double A = I.q - dot(kv, Qk);
+                const double Mu = dot(I.d, kv);
+                const double val = sn * (A * bP + Mu * bQ) + cs * (Mu * bP - A * bQ);
+                F = F + val * kv;
+                if (a.torques) {
+                    // (kk^T x Qi) = k x (Qi k); tau = (c bQ - s bP) k x mu - 2 (c bP + s bQ) k x Qk
+                    T = T + (cs * bQ - sn * bP) * cross(kv, I.d) - (2.0 * (cs * bP + sn * bQ)) * cross(kv, Qk);
+                }
+            }
+        }
+    }
+    if (i < a.n) {
+        a.force[3 * i] += COULOMBS * F.x;
+        a.force[3 * i + 1] += COULOMBS * F.y;
+        a.force[3 * i + 2] += COULOMBS * F.z;
+        double* tr = a.trq + 9 * (size_t)a.n;   // k-space part
+        tr[3 * i] = COULOMBS * T.x;
+        tr[3 * i + 1] = COULOMBS * T.y;
+        tr[3 * i + 2] = COULOMBS * T.z;
+    }
+}
+
+// ---- host side ---------------------------------------------------------------------
+
+int frames_run(Ctx* c);
+
+static MpArgs mp_args(Ctx* c) {
+    MpArgs a{};
+    a.n = c->n;
+    a.nmol = c->nmol;
+    a.stride = c->stride;
+    a.K = c->mp_K;
+    a.chunks = c->mp_chunks;
+    a.per = c->mp_K > 0 ? (c->n + c->mp_chunks - 1) / c->mp_chunks : 0;
+    a.box = c->box;
+    a.mol_start = c->d_mol_start;
+    a.mol = c->d_mol;
+    a.pos = c->d_pos;
+    a.com = c->d_com;
+    a.q = c->d_q;
+    a.dip = c->d_dipg;
+    a.quad = c->d_quadg;
+    a.nb = c->d_nb;
+    a.nb_cnt = c->d_nb_cnt;
+    a.nb_own = c->d_nb_own;
+    a.excl_off = c->d_excl_off;
+    a.excl = c->d_excl;
+    a.force = c->d_force;
+    a.trq = c->d_mp_trq;
+    a.terms = c->d_mp_terms;
+    a.kvec = c->d_mp_kvec;
+    a.kmod2 = c->d_mp_kmod2;
+    a.expk = c->d_mp_expk;
+    a.part = c->d_mp_part;
+    a.kf = c->d_mp_kf;
+    const smd_mp_params& p = c->mp;
+    a.alpha = p.alpha;
+    a.alpha2 = p.alpha * p.alpha;
+    a.alpha2PI = 2.0 * p.alpha / SQRT_PI;
+    a.realcut2 = p.real_cut * p.real_cut;
+    a.volume = c->box.L[0] * c->box.L[1] * c->box.L[2];
+    a.alphaR4 = 1.0 / (4.0 * a.alpha2);
+    a.use_constraint = p.use_constraint;
+    a.torques = p.torques;
+    return a;
+}
+
+static int mp_check(Ctx* c) {
+    if (!c->mp_ready) return fail(c, SMD_ESTATE, "smd_multipole_setup not called");
+    if (!c->multipoles_set) return fail(c, SMD_ESTATE, "multipoles not set");
+    return SMD_OK;
+}
+
+static int mp_read_terms(Ctx* c, double* t) {
+    SMD_HIP(c, hipMemcpyAsync(t, c->d_mp_terms, sizeof(double) * M_COUNT, hipMemcpyDeviceToHost, c->stream));
+    SMD_HIP(c, hipStreamSynchronize(c->stream));
+    return SMD_OK;
+}
+
+int multipole_short(Ctx* c, smd_mp_out* out) {
+    int rc = mp_check(c);
+    if (rc) return rc;
+    if (!c->list_valid) return fail(c, SMD_ESTATE, "no pair list");
+    if (c->mp.real_cut > c->list_cutoff + c->list_buff)
+        return fail(c, SMD_ENOTSUP, "real-space cutoff %g exceeds the pair-list radius %g", c->mp.real_cut,
+                    c->list_cutoff + c->list_buff);
+    MpArgs a = mp_args(c);
+    const dim3 b256(256);
+    const dim3 ga((c->n + 255) / 256), gm((c->nmol + 255) / 256);
+    const dim3 gr((unsigned)(((int64_t)c->n * MP_LANES + 255) / 256));
+    ProfScope ps(c, "mp_short");
+    SMD_HIP(c, hipMemsetAsync(c->d_mp_terms, 0, sizeof(double) * M_COUNT, c->stream));
+    if ((rc = frames_run(c))) return rc;
+    hipLaunchKernelGGL(k_mp_site_sums, ga, b256, 0, c->stream, a);
+    hipLaunchKernelGGL((k_mp_real<MP_LANES>), gr, b256, 0, c->stream, a);
+    hipLaunchKernelGGL(k_mp_intra, gm, b256, 0, c->stream, a);
+    const double volume = a.volume, volumer = 1.0 / volume;
+    const double coeff = 2.0 * PI * volumer / 3.0;
+    if (c->mp.surface) hipLaunchKernelGGL(k_mp_surface, ga, b256, 0, c->stream, a, coeff);
+    if (c->mp.torques) hipLaunchKernelGGL(k_mp_torques, gm, b256, 0, c->stream, a, 0, c->mp.surface ? 3 : 2);
+    SMD_HIP(c, hipGetLastError());
+    double t[M_COUNT];
+    if ((rc = mp_read_terms(c, t))) return rc;
+    *out = smd_mp_out{};
+    double qsum = 0.0;
+    for (int i = 0; i < c->n; ++i) qsum += c->h_q[i];
+    if (fabs(qsum) > 1.0e-5) out->charged_energy = -PI * qsum * qsum / (2.0 * volume * a.alpha2);
+    out->self_energy = t[M_SELF];
+    out->real_energy = t[M_REAL_E];
+    out->real_deriv = t[M_REAL_D1] + t[M_DUDV];
+    out->intra_energy = t[M_INTRA_E];
+    out->mol_deriv = t[M_MOL_D1] + t[M_DUDV + 1] / (3.0 * volume);
+    for (int k = 0; k < 9; ++k) out->virial[k] = t[M_REAL_VIR + k];
+    out->tot_torque = t[M_TRQ] + t[M_TRQ + 1] + t[M_TRQ + 2];
+    if (c->mp.surface) {
+        const double* qr = t + M_SU_QR;
+        const double* qR = t + M_SU_QRR;
+        const double* qm = t + M_SU_QM;
+        const double selfterm = qm[0] * qm[0] + qm[1] * qm[1] + qm[2] * qm[2] + t[M_SU_SELF];
+        out->surf_energy = coeff * selfterm * COULOMBS;
+        const double qmqr = qm[0] * qr[0] + qm[1] * qr[1] + qm[2] * qr[2];
+        const double qrqr = qr[0] * qr[0] + qr[1] * qr[1] + qr[2] * qr[2];
+        const double qrqR = qr[0] * qR[0] + qr[1] * qR[1] + qr[2] * qR[2];
+        const double qRqR = qR[0] * qR[0] + qR[1] * qR[1] + qR[2] * qR[2];
+        out->surf_deriv = -out->surf_energy * volumer + 2.0 * coeff * volumer * qmqr * COULOMBS / 3.0;
+        if (c->mp.use_constraint)
+            out->surf_deriv2 = 2.0 * coeff * volumer * volumer * qrqr - 16.0 * coeff * volumer * volumer * qrqR / 9.0 +
+                               2.0 * coeff * volumer * volumer * qRqR / 9.0;
+        else
+            out->surf_deriv2 = 4.0 * coeff * volumer * volumer * qrqr / 9.0;
+        out->surf_deriv += t[M_DUDV + 2] * volumer / 3.0;
+        for (int k = 0; k < 9; ++k) out->virial[k] += t[M_SU_VIR + k];
+    }
+    return SMD_OK;
+}
+
+int multipole_long(Ctx* c, smd_mp_long_out* out) {
+    int rc = mp_check(c);
+    if (rc) return rc;
+    MpArgs a = mp_args(c);
+    const dim3 b256(256);
+    ProfScope ps(c, "mp_long");
+    SMD_HIP(c, hipMemsetAsync(c->d_mp_terms, 0, sizeof(double) * M_COUNT, c->stream));
+    if ((rc = frames_run(c))) return rc;
+    const dim3 g1((a.K + MP_SF_THREADS - 1) / MP_SF_THREADS, a.chunks);
+    hipLaunchKernelGGL(k_mp_sfac, g1, dim3(MP_SF_THREADS), 0, c->stream, a);
+    hipLaunchKernelGGL(k_mp_kfinish, dim3((a.K + 255) / 256), b256, 0, c->stream, a);
+    hipLaunchKernelGGL(k_mp_kforce, dim3((c->n + MP_RF_THREADS - 1) / MP_RF_THREADS), dim3(MP_RF_THREADS), 0, c->stream,
+                       a);
+    if (c->mp.torques) hipLaunchKernelGGL(k_mp_torques, dim3((c->nmol + 255) / 256), b256, 0, c->stream, a, 3, 4);
+    SMD_HIP(c, hipGetLastError());
+    double t[M_COUNT];
+    if ((rc = mp_read_terms(c, t))) return rc;
+    *out = smd_mp_long_out{};
+    out->energy = t[M_LONG_E];
+    out->deriv = t[M_LONG_D1] + t[M_DUDV + 3];
+    out->deriv2 = t[M_LONG_D2];
+    const double* lv = t + M_LONG_VIR;   // xx yy zz xy xz yz
+    out->virial[0] = lv[0]; out->virial[4] = lv[1]; out->virial[8] = lv[2];
+    out->virial[1] = out->virial[3] = lv[3];
+    out->virial[2] = out->virial[6] = lv[4];
+    out->virial[5] = out->virial[7] = lv[5];
+    out->tot_torque = t[M_TRQ + 3];
+    return SMD_OK;
+}
+
+static int mp_setup(Ctx* c, const smd_mp_params* p, double kcut, int32_t* n_kvec) {
+    if (!c->box_set) return fail(c, SMD_ESTATE, "box not set");
+    if (!(p->alpha > 0) || !(p->real_cut > 0) || !(kcut > 0)) return fail(c, SMD_EINVAL, "bad multipole parameters");
+    for (int m = 0; m < c->nmol; ++m)
+        if (c->h_mol_start[m + 1] - c->h_mol_start[m] != 3)
+            return fail(c, SMD_ENOTSUP, "multipole Ewald is implemented for 3-site water (H1, O, H2)");
+    // Multipole::set_up (:242-302): half-space k set, k^2 < kCut^2
+    std::vector<double> kv, k2v, ekv;
+    const double kCut2 = kcut * kcut, alphaR4 = 1.0 / (4.0 * (p->alpha * p->alpha));
+    const int nxMax = (int)(kcut * c->box.L[0] / (2 * PI));
+    const int nyMax = (int)(kcut * c->box.L[1] / (2 * PI));
+    const int nzMax = (int)(kcut * c->box.L[2] / (2 * PI));
+    const double vx = 2.0 * PI / c->box.L[0], vy = 2.0 * PI / c->box.L[1], vz = 2.0 * PI / c->box.L[2];
+    for (int nx = 0; nx <= nxMax; nx++)
+        for (int ny = (nx == 0 ? 0 : -nyMax); ny <= nyMax; ny++)
+            for (int nz = ((nx == 0 && ny == 0) ? 1 : -nzMax); nz <= nzMax; nz++) {
+                const double kx = nx * vx, ky = ny * vy, kz = nz * vz;
+                const double k2 = kx * kx + ky * ky + kz * kz;
+                if (k2 < kCut2) {
+                    kv.push_back(kx); kv.push_back(ky); kv.push_back(kz);
+                    k2v.push_back(k2);
+                    ekv.push_back(exp(-k2 * alphaR4) / k2);
+                }
+            }
+    const int K = (int)k2v.size();
+    if (K == 0) return fail(c, SMD_EINVAL, "no k-vectors below kcut");
+    // chunks of sites so that the structure-factor grid covers the chip
+    const int kblocks = (K + MP_SF_THREADS - 1) / MP_SF_THREADS;
+    int chunks = std::max(1, std::min((512 + kblocks - 1) / kblocks, (c->n + 31) / 32));
+    auto R = [&](double*& ptr, size_t bytes) -> int {
+        if (ptr) (void)hipFree(ptr);
+        ptr = nullptr;
+        if (hipMalloc(&ptr, bytes) != hipSuccess) return fail(c, SMD_ENOMEM, "multipole allocation of %zu bytes", bytes);
+        return SMD_OK;
+    };
+    int rc;
+    if ((rc = R(c->d_mp_kvec, sizeof(double) * 3 * K)) || (rc = R(c->d_mp_kmod2, sizeof(double) * K)) ||
+        (rc = R(c->d_mp_expk, sizeof(double) * K)) || (rc = R(c->d_mp_kf, sizeof(double) * 2 * K)) ||
+        (rc = R(c->d_mp_part, sizeof(double) * MP_NSUM * (size_t)K * chunks)))
+        return rc;
+    if (!c->d_mp_trq && (rc = R(c->d_mp_trq, sizeof(double) * 12 * (size_t)c->n))) return rc;
+    if (!c->d_mp_terms && (rc = R(c->d_mp_terms, sizeof(double) * M_COUNT))) return rc;
+    SMD_HIP(c, hipMemset(c->d_mp_trq, 0, sizeof(double) * 12 * (size_t)c->n));
+    SMD_HIP(c, hipMemcpy(c->d_mp_kvec, kv.data(), sizeof(double) * 3 * K, hipMemcpyHostToDevice));
+    SMD_HIP(c, hipMemcpy(c->d_mp_kmod2, k2v.data(), sizeof(double) * K, hipMemcpyHostToDevice));
+    SMD_HIP(c, hipMemcpy(c->d_mp_expk, ekv.data(), sizeof(double) * K, hipMemcpyHostToDevice));
+    c->mp = *p;
+    c->mp_K = K;
+    c->mp_chunks = chunks;
+    c->mp_ready = true;
+    if (n_kvec) *n_kvec = K;
+    return SMD_OK;
+}
+
+}  // namespace smd
+
+using namespace smd;
+
+extern "C" int smd_multipole_setup(smd_ctx* c, const smd_mp_params* p, double kcut, int32_t* n_kvec) {
+    if (!c || !p) return SMD_EINVAL;
+    return mp_setup(c, p, kcut, n_kvec);
+}
+
+static int mp_forces_out(smd_ctx* c, double* f_accum) {
+    std::vector<double> f(3 * (size_t)c->n);
+    SMD_HIP(c, hipMemcpyAsync(f.data(), c->d_force, f.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    SMD_HIP(c, hipStreamSynchronize(c->stream));
+    for (size_t k = 0; k < f.size(); ++k) f_accum[k] += f[k];
+    return SMD_OK;
+}
+
+extern "C" int smd_multipole_compute(smd_ctx* c, int prec, double* f_accum, smd_mp_out* out) {
+    if (!c || !out) return SMD_EINVAL;
+    if (prec != SMD_PREC_FP64) return fail(c, SMD_ENOTSUP, "multipole Ewald is FP64 only");
+    if (f_accum) SMD_HIP(c, hipMemsetAsync(c->d_force, 0, sizeof(double) * 3 * c->n, c->stream));
+    int rc = multipole_short(c, out);
+    if (rc) return rc;
+    return f_accum ? mp_forces_out(c, f_accum) : SMD_OK;
+}
+
+extern "C" int smd_multipole_compute_long(smd_ctx* c, int prec, double* f_accum, smd_mp_long_out* out) {
+    if (!c || !out) return SMD_EINVAL;
+    if (prec != SMD_PREC_FP64) return fail(c, SMD_ENOTSUP, "multipole Ewald is FP64 only");
+    if (f_accum) SMD_HIP(c, hipMemsetAsync(c->d_force, 0, sizeof(double) * 3 * c->n, c->stream));
+    int rc = multipole_long(c, out);
+    if (rc) return rc;
+    return f_accum ? mp_forces_out(c, f_accum) : SMD_OK;
+}

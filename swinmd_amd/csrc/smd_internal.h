@@ -155,6 +155,18 @@ struct Ctx {
     double* d_cg_part = nullptr;  // [4][cg_parts] block partials
     int* d_cg_state = nullptr;    // [4] done, iterations, stop reason
     double* d_cg_scal = nullptr;  // [4] last residual norm
+
+    // permanent multipoles under Ewald
+    bool mp_ready = false;
+    smd_mp_params mp{};
+    int mp_K = 0, mp_chunks = 1;
+    double* d_mp_trq = nullptr;     // [4][3n] torques of real space, intramolecular, surface, k-space
+    double* d_mp_terms = nullptr;   // accumulator slots
+    double* d_mp_kvec = nullptr;    // [3K]
+    double* d_mp_kmod2 = nullptr;   // [K]
+    double* d_mp_expk = nullptr;    // [K]
+    double* d_mp_part = nullptr;    // [chunks][K][20]
+    double* d_mp_kf = nullptr;      // [K][2]
     int cg_parts = 0;
 
     // MD driver

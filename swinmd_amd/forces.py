@@ -28,6 +28,14 @@ def _ptr(a):
     return a.ctypes.data_as(C.c_void_p)
 
 
+def _fptr(f_accum):
+    """Host force accumulator (n,3) float64 C-contiguous, or NULL."""
+    if f_accum is None:
+        return None
+    assert f_accum.dtype == np.float64 and f_accum.flags.c_contiguous
+    return _ptr(f_accum)
+
+
 def _f64(a, shape=None):
     a = np.ascontiguousarray(a, dtype=np.float64)
     return a if shape is None else a.reshape(shape)
@@ -332,6 +340,47 @@ class WaterMD:
         self.ctx.check(self.ctx.lib.smd_md_run(self.ctx.h, nsteps, C.byref(r)))
         d = {name: getattr(r, name) for name, _ in L.MdReport._fields_ if name != "virial"}
         d["virial"] = np.array(r.virial[:])
+        return d
+
+
+def set_multipoles(ctx: Context):
+    """Uploads each site's local-frame dipole, quadrupole and polarizability
+    (AtomParam dipx..quadzz, polar) from the system's types."""
+    s = ctx.system
+    dip = np.ascontiguousarray([s.types[t].dip for t in s.type], dtype=np.float64)
+    quad = np.ascontiguousarray([s.types[t].quad for t in s.type], dtype=np.float64)
+    pol = np.ascontiguousarray([s.types[t].polar for t in s.type], dtype=np.float64)
+    ctx.check(ctx.lib.smd_set_multipoles(ctx.h, _ptr(dip), _ptr(quad), _ptr(pol), ctx.n))
+
+
+class Multipole:
+    """Multipole (src/forces/Multipole.h): permanent charges, dipoles and
+    quadrupoles under Ewald with torques, on the device."""
+
+    def __init__(self, ctx: Context, *, alpha, real_cut, kcut, use_constraint=True, surface=False, torques=True,
+                 prec=L.PREC_FP64):
+        self.ctx = ctx
+        self.prec = prec
+        set_multipoles(ctx)
+        p = L.MpParams(alpha, real_cut, int(use_constraint), int(surface), int(torques))
+        nk = C.c_int32()
+        ctx.check(ctx.lib.smd_multipole_setup(ctx.h, C.byref(p), kcut, C.byref(nk)))
+        self.num_kvectors = nk.value
+
+    def compute(self, f_accum=None):
+        """Multipole::compute: returns the energies, Lustig terms and virial;
+        forces are added to f_accum [n,3] when given."""
+        o = L.MpOut()
+        self.ctx.check(self.ctx.lib.smd_multipole_compute(self.ctx.h, self.prec, _fptr(f_accum), C.byref(o)))
+        d = {k: getattr(o, k) for k, _ in L.MpOut._fields_ if k != "virial"}
+        d["virial"] = np.array(o.virial[:])
+        return d
+
+    def compute_long(self, f_accum=None):
+        o = L.MpLongOut()
+        self.ctx.check(self.ctx.lib.smd_multipole_compute_long(self.ctx.h, self.prec, _fptr(f_accum), C.byref(o)))
+        d = {k: getattr(o, k) for k, _ in L.MpLongOut._fields_ if k != "virial"}
+        d["virial"] = np.array(o.virial[:])
         return d
 
 
