@@ -2081,3 +2081,456 @@ int orc_multipole_long(const orc_sys *s, int K, const double *kvec, const double
     free(buf);
     return 0;
 }
+
+/* ---- induced dipoles in the Ewald sum (src/forces/Induction.cpp) ---------- */
+
+/* cross(Q, a b^T, ones) with the matrices written out, as real_term builds
+ * them (:1190-1275) */
+static inline v3 m3_cross_qo(const double Q[3][3], v3 a, v3 b) {
+    const double av[3] = {a.x, a.y, a.z}, bv[3] = {b.x, b.y, b.z};
+    double m[3][3];
+    for (int r = 0; r < 3; ++r)
+        for (int c = 0; c < 3; ++c) m[r][c] = av[r] * bv[c];
+    return m3_cross(Q, m);
+}
+
+/* Induction terms of one site pair (real_term :940-1028 and intraMolCorrect
+ * :1880-1935): G_l of site i's permanent multipoles with j's induced dipole
+ * (and the reverse), their gradients, and the torque vectors. */
+typedef struct {
+    double G1i, G2i, G3i, G1j, G2j, G3j;
+    v3 gG1i, gG2i, gG3i, gG1j, gG2j, gG3j;
+    v3 Gt1i, Gt2i, Gt3i, Gt1j, Gt2j, Gt3j;
+} ind_terms;
+
+static void ind_pair_terms(const mp_site *I, const mp_site *J, v3 pi, v3 pj, v3 rij, int grads, int torques,
+                           ind_terms *t) {
+    const double qi = I->q, qj = J->q;
+    const v3 dipi = I->d, dipj = J->d;
+    const double (*Qi)[3] = I->Q, (*Qj)[3] = J->Q;
+    const v3 rji = v3_neg(rij);
+    const double QiI = Qi[0][0] + Qi[1][1] + Qi[2][2];
+    const double QjI = Qj[0][0] + Qj[1][1] + Qj[2][2];
+    const double QiRij = m3_quadform(Qi, rij);
+    const double QjRji = m3_quadform(Qj, rij);
+    const v3 QiXrij = m3_mul(Qi, rij), QjXrji = m3_mul(Qj, rji);
+    const double QipjXrij = pj.x * QiXrij.x + pj.y * QiXrij.y + pj.z * QiXrij.z;
+    const double QjpiXrji = pi.x * QjXrji.x + pi.y * QjXrji.y + pi.z * QjXrji.z;
+    t->G1i = v3_dot(v3_scale(-qi, pj), rij) - v3_dot(dipi, pj);
+    t->G2i = -v3_dot(dipi, rij) * v3_dot(pj, rij) - 2.0 * QipjXrij - v3_dot(pj, rij) * QiI;
+    t->G3i = v3_dot(v3_neg(pj), rij) * QiRij;
+    t->G1j = v3_dot(v3_scale(-qj, pi), rji) - v3_dot(dipj, pi);
+    t->G2j = -v3_dot(dipj, rji) * v3_dot(pi, rji) - 2.0 * QjpiXrji - v3_dot(pi, rji) * QjI;
+    t->G3j = v3_dot(v3_neg(pi), rji) * QjRji;
+    if (grads) {
+        const v3 QiXpj = m3_mul(Qi, pj), QjXpi = m3_mul(Qj, pi);
+        t->gG1i = v3_scale(-qi, pj);
+        v3 g = v3_scale(-v3_dot(pj, rij), dipi);
+        g = v3_sub(g, v3_scale(v3_dot(dipi, rij), pj));
+        g = v3_sub(g, v3_scale(2, QiXpj));
+        t->gG2i = v3_sub(g, v3_scale(QiI, pj));
+        t->gG3i = v3_sub(v3_scale(-QiRij, pj), v3_scale(v3_dot(v3_scale(2, pj), rij), QiXrij));
+        t->gG1j = v3_scale(-qj, pi);
+        g = v3_scale(-v3_dot(pi, rji), dipj);
+        g = v3_sub(g, v3_scale(v3_dot(dipj, rji), pi));
+        g = v3_sub(g, v3_scale(2, QjXpi));
+        t->gG2j = v3_sub(g, v3_scale(QjI, pi));
+        t->gG3j = v3_sub(v3_scale(-QjRji, pi), v3_scale(v3_dot(v3_scale(2, pi), rji), QjXrji));
+    }
+    if (torques) {
+        double Rij[3][3];
+        const double rv[3] = {rij.x, rij.y, rij.z};
+        for (int a = 0; a < 3; ++a)
+            for (int b = 0; b < 3; ++b) Rij[a][b] = rv[a] * rv[b];
+        const v3 QiXpjrij = m3_cross_qo(Qi, pj, rij), QiXrijpj = m3_cross_qo(Qi, rij, pj);
+        const v3 QiXRij = m3_cross(Qi, Rij);
+        const v3 QjXpirji = m3_cross_qo(Qj, pi, rji), QjXrjipi = m3_cross_qo(Qj, rji, pi);
+        const v3 QjXRji = m3_cross(Qj, Rij);
+        t->Gt1i = v3_cross(dipi, pj);
+        t->Gt2i = v3_add(v3_add(v3_scale(v3_dot(pj, rij), v3_cross(dipi, rij)), v3_scale(2.0, QiXpjrij)),
+                         v3_scale(2.0, QiXrijpj));
+        t->Gt3i = v3_scale(2.0 * v3_dot(pj, rij), QiXRij);
+        t->Gt1j = v3_cross(dipj, pi);
+        t->Gt2j = v3_add(v3_add(v3_scale(v3_dot(pi, rji), v3_cross(dipj, rji)), v3_scale(2.0, QjXpirji)),
+                         v3_scale(2.0, QjXrjipi));
+        t->Gt3j = v3_scale(2.0 * v3_dot(pi, rji), QjXRji);
+    }
+}
+
+int orc_induction_short(const orc_sys *s, const double *dip, const double *quad, const double *polar,
+                        const double *pind, const orc_indf_params *p, double *force, orc_indf_out *out) {
+    /* Induction::compute after InducedDipoles (src/forces/Induction.cpp:486-565):
+     * self_correct (:688-760), real_term (:762-1393), intraMolCorrect
+     * (:1760-2186), surfDipoleCorrect (:2188-2399), torques2forces after each
+     * (:2557-2744, the same map as Multipole's). */
+    const int n = s->n_atoms;
+    for (int m = 0; m < s->n_mols; ++m)
+        if (s->mol_start[m + 1] - s->mol_start[m] != 3) return -1;
+    double *trq = (double *)calloc((size_t)3 * n, sizeof(double));
+    if (!trq) return -3;
+    const double volume = s->box[0] * s->box[1] * s->box[2], volumer = 1.0 / volume;
+    const double alpha = p->alpha, alpha2 = alpha * alpha, alpha2PI = 2.0 * alpha / SQRT_PI;
+    const double realCut2 = p->real_cut * p->real_cut, a = p->a_damp;
+    memset(out, 0, sizeof *out);
+
+    /* self_correct */
+    for (int i = 0; i < n; ++i) {
+        const mp_site I = mp_load(s, dip, quad, i);
+        const v3 indPi = v3_ld(pind + 3 * i);
+        const double selfterm = v3_dot(I.d, indPi) * alpha2PI * alpha2 / 3;
+        out->self_energy += -selfterm * COULOMBS;
+        if (p->torques) {
+            const double term = COULOMBS * (2.0 / 3.0) * alpha2PI * alpha2;
+            v3_st(trq + 3 * i, v3_scale(term, v3_cross(I.d, indPi)));
+        }
+    }
+    if (p->torques) {
+        out->tot_torque += mp_torque_norms(n, trq);
+        out->self_deriv += mp_torques2forces(s, trq, force);
+    }
+
+    /* real_term: every pair j > i, minimum image */
+    memset(trq, 0, sizeof(double) * 3 * n);
+    for (int i = 0; i < n; ++i) {
+        const mp_site I = mp_load(s, dip, quad, i);
+        const v3 ri = v3_ld(s->pos + 3 * i), indPi = v3_ld(pind + 3 * i);
+        for (int j = i + 1; j < n; ++j) {
+            const v3 rj = v3_ld(s->pos + 3 * j);
+            v3 rij = v3_sub(ri, rj);
+            apply_pbc(&rij, s->box);
+            const double r2 = v3_len2(rij);
+            if (!(r2 < realCut2)) continue;
+            const mp_site J = mp_load(s, dip, quad, j);
+            const v3 indPj = v3_ld(pind + 3 * j);
+            const v3 rji = v3_neg(rij);
+            ind_terms t;
+            ind_pair_terms(&I, &J, indPi, indPj, rij, 1, p->torques, &t);
+            const double r = sqrt(r2);
+            const double u3 = r * r * r / (sqrt(polar[i] * polar[j]));
+            double B[6];
+            B[0] = erfc(alpha * r) / r;
+            B[1] = (-B[0] - alpha2PI * exp(-alpha2 * r2)) / r2;
+            B[2] = (-3 * B[1] + 2 * alpha2PI * alpha2 * exp(-alpha2 * r2)) / r2;
+            B[3] = (-5 * B[2] - 4 * alpha2PI * alpha2 * alpha2 * exp(-alpha2 * r2)) / r2;
+            B[4] = (-7 * B[3] + 8 * alpha2PI * alpha2 * alpha2 * alpha2 * exp(-alpha2 * r2)) / r2;
+            B[5] = (-9 * B[4] - 16 * alpha2PI * alpha2 * alpha2 * alpha2 * alpha2 * exp(-alpha2 * r2)) / r2;
+            const double expau3 = exp(-a * u3), au3 = a * u3, a2u6 = au3 * au3, a3u9 = a2u6 * au3;
+            const double lambda3 = 1.0 - expau3;
+            const double lambda5 = 1.0 - (1.0 + au3) * expau3;
+            const double lambda7 = 1.0 - (1.0 + au3 + 3.0 * a2u6 / 5.0) * expau3;
+            const double gradlam3 = 3.0 * au3 * expau3 / r2;
+            const double gradlam5 = 3.0 * a2u6 * expau3 / r2;
+            const double gradlam7 = (-3.0 * a2u6 + 9.0 * a3u9) * expau3 / (5.0 * r2);
+            const double scale = excluded(s, i, j) ? 0.0 : 1.0;
+            double B1d, B2d, B3d, B4d = 0, gradB1 = 0, gradB2 = 0, gradB3 = 0, B2du = 0, gradB1u = 0, gradB2u = 0;
+            if (p->damp_amoeba) {
+                const double invr = 1.0 / r, invr2 = invr * invr, invr3 = invr2 * invr;
+                const double invr5 = 3.0 * invr3 * invr2, invr7 = 5.0 * invr5 * invr2, invr9 = 7.0 * invr7 * invr2;
+                B1d = B[1] + (1.0 - scale * lambda3) * invr3;
+                B2d = B[2] - (1.0 - scale * lambda5) * invr5;
+                B3d = B[3] + (1.0 - scale * lambda7) * invr7;
+                gradB1 = B[2] - scale * gradlam3 * invr3 - (1.0 - scale * lambda3) * invr5;
+                gradB2 = B[3] + scale * gradlam5 * invr5 + (1.0 - scale * lambda5) * invr7;
+                gradB3 = B[4] - scale * gradlam7 * invr7 - (1.0 - scale * lambda7) * invr9;
+                B2du = B[2] - (1.0 - lambda5) * invr5;
+                gradB1u = B[2] - gradlam3 * invr3 - (1.0 - lambda3) * invr5;
+                gradB2u = B[3] + gradlam5 * invr5 + (1.0 - lambda5) * invr7;
+            } else {
+                B1d = B[1] * lambda3;
+                B2d = B[2] * lambda3 + B[1] * gradlam3;
+                B3d = B[3] * lambda3 + B[2] * 2.0 * gradlam3 + B[1] * (gradlam3 - 3.0 * gradlam5) / r2;
+                B4d = B[4] * lambda3 + B[3] * 3.0 * gradlam3 + B[2] * (3.0 * gradlam3 - 9.0 * gradlam5) / r2 +
+                      B[1] * (-gradlam3 - 6.0 * gradlam5 + 15.0 * gradlam7) / (r2 * r2);
+            }
+            const double Gsum = (t.G1i + t.G1j) * B1d + (t.G2i + t.G2j) * B2d + (t.G3i + t.G3j) * B3d;
+            out->real_energy += p->damp_amoeba ? 0.5 * Gsum * COULOMBS : 0.5 * Gsum * scale * COULOMBS;
+            const double G1_ip = v3_dot(v3_neg(indPi), indPj);
+            const double G2_ip = -v3_dot(indPi, rij) * v3_dot(indPj, rij);
+            const v3 gradG2_ip = v3_scale(-v3_dot(indPj, rij), indPi);
+            const v3 gradG2_jp = v3_scale(-v3_dot(indPi, rji), indPj);
+            v3 fi, fj;
+            if (p->damp_amoeba) {
+                v3 gi = v3_add(v3_add(v3_scale(B1d, t.gG1i), v3_scale(B2d, t.gG2i)), v3_scale(B3d, t.gG3i));
+                v3 gj = v3_add(v3_add(v3_scale(B1d, t.gG1j), v3_scale(B2d, t.gG2j)), v3_scale(B3d, t.gG3j));
+                double si = t.G1i * gradB1 + t.G2i * gradB2 + t.G3i * gradB3;
+                const double sj = t.G1j * gradB1 + t.G2j * gradB2 + t.G3j * gradB3;
+                if (p->do_min) {
+                    gi = v3_add(gi, v3_scale(B2du, gradG2_ip));
+                    gj = v3_add(gj, v3_scale(B2du, gradG2_jp));
+                    si = si + G1_ip * gradB1u + G2_ip * gradB2u;
+                }
+                fi = v3_sub(v3_neg(gi), v3_scale(si, rij));
+                fj = v3_sub(v3_neg(gj), v3_scale(sj, rji));
+            } else if (p->do_min) {
+                v3 gi = v3_scale(scale, v3_scale(B1d, t.gG1i));
+                gi = v3_add(gi, v3_scale(scale, v3_scale(B2d, t.gG2i)));
+                gi = v3_add(gi, v3_scale(scale, v3_scale(B3d, t.gG3i)));
+                gi = v3_add(gi, v3_scale(B2d, gradG2_ip));
+                v3 gj = v3_scale(scale, v3_scale(B1d, t.gG1j));
+                gj = v3_add(gj, v3_scale(scale, v3_scale(B2d, t.gG2j)));
+                gj = v3_add(gj, v3_scale(scale, v3_scale(B3d, t.gG3j)));
+                gj = v3_add(gj, v3_scale(B2d, gradG2_jp));
+                const double si = t.G1i * B2d * scale + t.G2i * B3d * scale + t.G3i * B4d * scale + G1_ip * B2d + G2_ip * B3d;
+                const double sj = t.G1j * B2d * scale + t.G2j * B3d * scale + t.G3j * B4d * scale;
+                fi = v3_sub(v3_neg(gi), v3_scale(si, rij));
+                fj = v3_sub(v3_neg(gj), v3_scale(sj, rji));
+            } else {
+                const v3 gi = v3_add(v3_add(v3_scale(B1d, t.gG1i), v3_scale(B2d, t.gG2i)), v3_scale(B3d, t.gG3i));
+                const v3 gj = v3_add(v3_add(v3_scale(B1d, t.gG1j), v3_scale(B2d, t.gG2j)), v3_scale(B3d, t.gG3j));
+                const double si = t.G1i * B2d + t.G2i * B3d + t.G3i * B4d;
+                const double sj = t.G1j * B2d + t.G2j * B3d + t.G3j * B4d;
+                fi = v3_sub(v3_scale(scale, v3_neg(gi)), v3_scale(scale, v3_scale(si, rij)));
+                fj = v3_sub(v3_scale(scale, v3_neg(gj)), v3_scale(scale, v3_scale(sj, rji)));
+            }
+            const v3 fij = v3_sub(fi, fj);
+            out->real_deriv += v3_dot(v3_neg(fij), rij) * COULOMBS;
+            acc(force + 3 * i, v3_scale(COULOMBS, fij));
+            dec(force + 3 * j, v3_scale(COULOMBS, fij));
+            vir_add(out->virial, fij, rij);
+            if (p->torques) {
+                const v3 ti = v3_add(v3_add(v3_scale(B1d, t.Gt1i), v3_scale(B2d, t.Gt2i)), v3_scale(B3d, t.Gt3i));
+                const v3 tj = v3_add(v3_add(v3_scale(B1d, t.Gt1j), v3_scale(B2d, t.Gt2j)), v3_scale(B3d, t.Gt3j));
+                if (p->damp_amoeba) {
+                    acc(trq + 3 * i, v3_scale(COULOMBS, ti));
+                    acc(trq + 3 * j, v3_scale(COULOMBS, tj));
+                } else {
+                    acc(trq + 3 * i, v3_scale(scale, v3_scale(COULOMBS, ti)));
+                    acc(trq + 3 * j, v3_scale(scale, v3_scale(COULOMBS, tj)));
+                }
+            }
+        }
+    }
+    if (p->torques) {
+        out->tot_torque += mp_torque_norms(n, trq);
+        out->real_deriv += mp_torques2forces(s, trq, force);
+    }
+
+    /* intraMolCorrect */
+    memset(trq, 0, sizeof(double) * 3 * n);
+    {
+        double e = 0.0;
+        for (int m = 0; m < s->n_mols; ++m) {
+            const int a0 = s->mol_start[m], ns = s->mol_start[m + 1] - a0;
+            for (int x = 0; x < ns; ++x) {
+                const int idi = a0 + x;
+                const mp_site I = mp_load(s, dip, quad, idi);
+                const v3 indPi = v3_ld(pind + 3 * idi);
+                const v3 ri = v3_ld(s->pos + 3 * idi);
+                for (int y = x + 1; y < ns; ++y) {
+                    const int idj = a0 + y;
+                    if (!excluded(s, idi, idj)) continue;
+                    const v3 rj = v3_ld(s->pos + 3 * idj);
+                    v3 rij = v3_sub(ri, rj);
+                    apply_pbc(&rij, s->box);
+                    const double r2 = v3_len2(rij), r = sqrt(r2);
+                    const mp_site J = mp_load(s, dip, quad, idj);
+                    const v3 indPj = v3_ld(pind + 3 * idj);
+                    const v3 rji = v3_neg(rij);
+                    ind_terms t;
+                    ind_pair_terms(&I, &J, indPi, indPj, rij, !p->use_constraint, !p->use_constraint && p->torques, &t);
+                    double D[6];
+                    D[0] = erf(alpha * r) / r;
+                    D[1] = (-D[0] + alpha2PI * exp(-alpha2 * r2)) / r2;
+                    D[2] = (-3 * D[1] - 2 * alpha2PI * alpha2 * exp(-alpha2 * r2)) / r2;
+                    D[3] = (-5 * D[2] + 4 * alpha2PI * alpha2 * alpha2 * exp(-alpha2 * r2)) / r2;
+                    D[4] = (-7 * D[3] - 8 * alpha2PI * alpha2 * alpha2 * alpha2 * exp(-alpha2 * r2)) / r2;
+                    D[5] = (-9 * D[4] + 16 * alpha2PI * alpha2 * alpha2 * alpha2 * alpha2 * exp(-alpha2 * r2)) / r2;
+                    e += (t.G1i + t.G1j) * D[1] + (t.G2i + t.G2j) * D[2] + (t.G3i + t.G3j) * D[3];
+                    if (!p->use_constraint) {
+                        v3 fi = v3_add(v3_add(v3_scale(D[1], t.gG1i), v3_scale(D[2], t.gG2i)), v3_scale(D[3], t.gG3i));
+                        fi = v3_add(fi, v3_scale(t.G1i * D[2] + t.G2i * D[3] + t.G3i * D[4], rij));
+                        v3 fj = v3_add(v3_add(v3_scale(D[1], t.gG1j), v3_scale(D[2], t.gG2j)), v3_scale(D[3], t.gG3j));
+                        fj = v3_add(fj, v3_scale(t.G1j * D[2] + t.G2j * D[3] + t.G3j * D[4], rji));
+                        const v3 fij = v3_sub(fi, fj);
+                        out->mol_deriv += v3_dot(v3_neg(fij), rij) * COULOMBS / (3.0 * volume);
+                        acc(force + 3 * idi, v3_scale(COULOMBS, fij));
+                        dec(force + 3 * idj, v3_scale(COULOMBS, fij));
+                        if (p->torques) {
+                            const v3 ti = v3_add(v3_add(v3_scale(D[1], t.Gt1i), v3_scale(D[2], t.Gt2i)), v3_scale(D[3], t.Gt3i));
+                            const v3 tj = v3_add(v3_add(v3_scale(D[1], t.Gt1j), v3_scale(D[2], t.Gt2j)), v3_scale(D[3], t.Gt3j));
+                            acc(trq + 3 * idi, v3_scale(-COULOMBS, ti));
+                            acc(trq + 3 * idj, v3_scale(-COULOMBS, tj));
+                        }
+                    }
+                }
+            }
+        }
+        out->intra_energy = -0.5 * e * COULOMBS;
+    }
+    if (p->torques) {
+        out->tot_torque += mp_torque_norms(n, trq);
+        out->mol_deriv += mp_torques2forces(s, trq, force) / (3.0 * volume);
+    }
+
+    /* surfDipoleCorrect */
+    if (p->surface) {
+        memset(trq, 0, sizeof(double) * 3 * n);
+        const double coeff = 2.0 * PI * volumer / 3.0;
+        v3 qrSum = {0, 0, 0}, qRsum = {0, 0, 0}, miSum = {0, 0, 0}, piSum = {0, 0, 0};
+        double selfSum = 0.0, selfpi = 0.0;
+        for (int i = 0; i < n; ++i) {
+            const v3 r = v3_ld(s->pos + 3 * i), rim = v3_ld(s->com + 3 * s->mol[i]);
+            const mp_site I = mp_load(s, dip, quad, i);
+            const v3 indPi = v3_ld(pind + 3 * i);
+            qrSum = v3_add(qrSum, v3_scale(I.q, r));
+            qRsum = v3_add(qRsum, v3_scale(I.q, rim));
+            miSum = v3_add(miSum, I.d);
+            piSum = v3_add(piSum, indPi);
+            selfSum += v3_dot(I.d, indPi);
+            selfpi += v3_dot(indPi, indPi);
+        }
+        const double selfterm = v3_dot(qrSum, piSum) + v3_dot(miSum, piSum) - selfSum;
+        out->surf_energy = coeff * selfterm * COULOMBS;
+        double dUdV1;
+        if (p->do_min)
+            dUdV1 = -coeff * volumer * (2.0 * v3_dot(qrSum, piSum) + 2.0 * v3_dot(miSum, piSum) + v3_dot(piSum, piSum) -
+                                        2.0 * selfSum - selfpi);
+        else
+            dUdV1 = -coeff * volumer * (2.0 * v3_dot(qrSum, piSum) + 2.0 * v3_dot(miSum, piSum) - 2.0 * selfSum);
+        const double dUdV2 = v3_dot(v3_scale(2.0 * coeff * volumer, qrSum), piSum) / 3.0;
+        out->surf_deriv = COULOMBS * (dUdV1 + dUdV2);
+        if (p->use_constraint)
+            out->surf_deriv2 = v3_dot(v3_scale(2.0 * coeff * volumer * volumer, qrSum), qrSum) -
+                               v3_dot(v3_scale(16.0 * coeff * volumer * volumer, qrSum), qRsum) / 9.0 +
+                               v3_dot(v3_scale(2.0 * coeff * volumer * volumer, qRsum), qRsum) / 9.0;
+        else
+            out->surf_deriv2 = v3_dot(v3_scale(4.0 * coeff * volumer * volumer, qrSum), qrSum) / 9.0;
+        for (int i = 0; i < n; ++i) {
+            const mp_site I = mp_load(s, dip, quad, i);
+            const v3 indPi = v3_ld(pind + 3 * i);
+            const v3 ri = v3_ld(s->pos + 3 * i);
+            const v3 fi = v3_scale(-2.0 * coeff * I.q, piSum);
+            const v3 temp = v3_scale(-s->q[i], qrSum);
+            acc(force + 3 * i, v3_scale(COULOMBS, fi));
+            vir_add(out->virial, temp, ri);
+            if (p->torques)
+                acc(trq + 3 * i, v3_scale(COULOMBS * 2.0 * coeff, v3_sub(v3_cross(piSum, I.d), v3_cross(indPi, I.d))));
+        }
+        if (p->torques) {
+            out->tot_torque += mp_torque_norms(n, trq);
+            out->surf_deriv += mp_torques2forces(s, trq, force) * volumer / 3.0;
+        }
+    }
+    free(trq);
+    return 0;
+}
+
+int orc_induction_long(const orc_sys *s, int K, const double *kvec, const double *kmod2, const double *expk,
+                       const double *dip, const double *quad, const double *pind, const orc_indf_params *p,
+                       double *force, orc_indf_out *out) {
+    /* Induction::compute_long / reciprocal_term (src/forces/Induction.cpp:567-590,
+     * :1395-1758) */
+    const int n = s->n_atoms;
+    for (int m = 0; m < s->n_mols; ++m)
+        if (s->mol_start[m + 1] - s->mol_start[m] != 3) return -1;
+    const double volume = s->box[0] * s->box[1] * s->box[2], volumer = 1.0 / volume;
+    const double pi4Volumer = 4.0 * PI * volumer, pi8Volumer = 2.0 * pi4Volumer;
+    const double alphaR4 = 1.0 / (4.0 * (p->alpha * p->alpha));
+    double *buf = (double *)calloc((size_t)21 * n, sizeof(double));
+    if (!buf) return -3;
+    double *qQcos = buf, *qQsin = buf + n, *mkcos = buf + 2 * n, *mksin = buf + 3 * n, *pkcos = buf + 4 * n,
+           *pksin = buf + 5 * n, *trq = buf + 6 * n;
+    double *kXmsina = buf + 9 * n, *kXmcosa = buf + 12 * n, *KXQcosa = buf + 15 * n, *KXQsina = buf + 18 * n;
+    out->long_energy = out->long_deriv = out->long_deriv2 = 0.0;
+    memset(out->long_virial, 0, sizeof out->long_virial);
+    for (int j = 0; j < K; ++j) {
+        const v3 kj = v3_ld(kvec + 3 * j);
+        double Sc = 0, Ss = 0, Ms = 0, Mc = 0, Ps = 0, Pc = 0, Q2s = 0, Q2c = 0;
+        const double coeff = 2.0 * (alphaR4 + 1.0 / kmod2[j]);
+        v3 cosrR = {0, 0, 0}, sinrR = {0, 0, 0};
+        double Kj[3][3];
+        const double kk[3] = {kj.x, kj.y, kj.z};
+        for (int a = 0; a < 3; ++a)
+            for (int b = 0; b < 3; ++b) Kj[a][b] = kk[a] * kk[b];
+        for (int i = 0; i < n; ++i) {
+            const v3 rim = v3_ld(s->com + 3 * s->mol[i]);
+            const mp_site I = mp_load(s, dip, quad, i);
+            const v3 indPi = v3_ld(pind + 3 * i);
+            const v3 ri = v3_ld(s->pos + 3 * i);
+            const double rk = v3_dot(ri, kj);
+            const double QiK = m3_quadform(I.Q, kj);
+            const double cr = cos(rk), sr = sin(rk);
+            const double cosAqiQi = (I.q - QiK) * cr, sinAqiQi = (I.q - QiK) * sr;
+            const double mikcos = v3_dot(I.d, kj) * cr, miksin = v3_dot(I.d, kj) * sr;
+            const double pikcos = v3_dot(indPi, kj) * cr, piksin = v3_dot(indPi, kj) * sr;
+            const double Q2sin = 2.0 * QiK * sr, Q2cos = 2.0 * QiK * cr;
+            qQcos[i] = cosAqiQi; qQsin[i] = sinAqiQi;
+            mkcos[i] = mikcos; mksin[i] = miksin;
+            pkcos[i] = pikcos; pksin[i] = piksin;
+            Sc += cosAqiQi; Ss += sinAqiQi;
+            Mc += mikcos; Ms += miksin;
+            Pc += pikcos; Ps += piksin;
+            Q2s += Q2sin; Q2c += Q2cos;
+            cosrR = v3_add(cosrR, v3_scale(cosAqiQi, v3_sub(ri, rim)));
+            sinrR = v3_add(sinrR, v3_scale(sinAqiQi, v3_sub(ri, rim)));
+            if (p->torques) {
+                const v3 kXmi = v3_cross(kj, I.d);
+                const v3 KXQi = m3_cross(Kj, I.Q);
+                v3_st(kXmsina + 3 * i, v3_scale(sr, kXmi));
+                v3_st(kXmcosa + 3 * i, v3_scale(cr, kXmi));
+                v3_st(KXQcosa + 3 * i, v3_scale(cr, KXQi));
+                v3_st(KXQsina + 3 * i, v3_scale(sr, KXQi));
+            }
+        }
+        const double UmqQ = pi4Volumer * expk[j] * (Ss * Pc - Sc * Ps);
+        const double Ump = pi4Volumer * expk[j] * (Pc * Mc + Ps * Ms);
+        const double tmpenergy = (UmqQ + Ump) * COULOMBS;
+        out->long_energy += tmpenergy;
+        const double dUdVpQ = pi4Volumer * expk[j] * (Q2s * Pc - Q2c * Ps);
+        const double Upp = p->do_min ? pi4Volumer * expk[j] * (Pc * Pc + Ps * Ps) : 0.0;
+        const double Sk = 2.0 * UmqQ + 2.0 * Ump + Upp;
+        const double dUdV1 = -Sk;
+        const double dUdV2 = v3_dot(v3_scale(Sk, kj), kj) * 2.0 * alphaR4;
+        const double dUdV3 = -2.0 * UmqQ + 2.0 * dUdVpQ - 4.0 * Ump - 2.0 * Upp;
+        out->long_deriv += (dUdV1 + dUdV2 + dUdV3) * COULOMBS;
+        const double tkk = v3_dot(v3_scale(tmpenergy, kj), kj) * 2.0 * alphaR4;
+        out->long_deriv2 += 4.0 * tmpenergy - v3_dot(v3_scale(7.0 * tmpenergy, kj), kj) * 2.0 * alphaR4 +
+                            v3_dot(v3_scale(tkk, kj), kj) * 2.0 * alphaR4;
+        double *lv = out->long_virial;
+        lv[XX] += tmpenergy - tmpenergy * coeff * kj.x * kj.x;
+        lv[YY] += tmpenergy - tmpenergy * coeff * kj.y * kj.y;
+        lv[ZZ] += tmpenergy - tmpenergy * coeff * kj.z * kj.z;
+        lv[XY] -= tmpenergy * coeff * kj.x * kj.y;
+        lv[XZ] -= tmpenergy * coeff * kj.x * kj.z;
+        lv[YZ] -= tmpenergy * coeff * kj.y * kj.z;
+        const double b = pi8Volumer * expk[j];
+        double Ff = 0.0, Ffk2 = 0.0, d2UV1 = 0.0, d2UrR = 0.0;
+        for (int i = 0; i < n; ++i) {
+            const double FpiqQ = b * (pkcos[i] * Sc + pksin[i] * Ss);
+            const double FqQpj = -b * (qQcos[i] * Pc + qQsin[i] * Ps);
+            const double Fupj = b * (mksin[i] * Pc - mkcos[i] * Ps);
+            const double Fpmj = p->do_min ? b * (pksin[i] * (Mc + Pc) - pkcos[i] * (Ms + Ps))
+                                          : b * (pksin[i] * Mc - pkcos[i] * Ms);
+            const double val = COULOMBS * (FpiqQ + FqQpj + Fpmj + Fupj);
+            const double bkk = v3_dot(v3_scale(b, kj), kj);
+            const double d2u1 = bkk * (qQcos[i] * Sc + qQsin[i] * Ss);
+            const v3 rim = v3_ld(s->com + 3 * s->mol[i]);
+            const v3 ri = v3_ld(s->pos + 3 * i);
+            const v3 dr = v3_sub(ri, rim);
+            const double d2u2 = v3_dot(v3_scale(bkk, v3_add(v3_scale(qQcos[i], cosrR), v3_scale(qQsin[i], sinrR))), dr);
+            acc(force + 3 * i, v3_scale(val, kj));
+            Ff += val * v3_dot(kj, dr);
+            Ffk2 += v3_dot(v3_scale(2.0 * val, kj), kj) * 2.0 * alphaR4 * v3_dot(kj, dr);
+            d2UV1 += v3_dot(v3_scale(d2u1, dr), dr);
+            d2UrR += d2u2;
+            if (p->torques) {
+                const v3 taum = v3_scale(b, v3_add(v3_scale(Pc, v3_ld(kXmcosa + 3 * i)), v3_scale(Ps, v3_ld(kXmsina + 3 * i))));
+                const v3 tauQ = v3_scale(2.0 * b, v3_sub(v3_scale(Ps, v3_ld(KXQcosa + 3 * i)), v3_scale(Pc, v3_ld(KXQsina + 3 * i))));
+                acc(trq + 3 * i, v3_scale(COULOMBS, v3_add(taum, tauQ)));
+            }
+        }
+        if (p->use_constraint) {
+            out->long_deriv += Ff;
+            out->long_deriv2 += -6.0 * Ff + Ffk2 - d2UV1 + d2UrR;
+        }
+    }
+    out->long_virial[YX] = out->long_virial[XY];
+    out->long_virial[ZX] = out->long_virial[XZ];
+    out->long_virial[ZY] = out->long_virial[YZ];
+    if (p->torques) {
+        out->tot_torque += mp_torque_norms(n, trq);
+        out->long_deriv += mp_torques2forces(s, trq, force);
+    }
+    free(buf);
+    return 0;
+}

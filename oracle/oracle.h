@@ -281,6 +281,38 @@ int orc_multipole_long(const orc_sys *s, int K, const double *kvec, const double
                        const double *dip, const double *quad, const orc_mp_params *p, double *force,
                        orc_mp_out *out);
 
+/* Energies, forces and torques of the induced dipoles Pi, treated as fixed,
+ * against the permanent multipoles under Ewald: Induction::compute after
+ * InducedDipoles, and compute_long (src/forces/Induction.cpp). */
+typedef struct {
+    double alpha;             /* get_alpha() */
+    double real_cut;          /* get_cutoffEw() */
+    double a_damp;            /* Thole a (get_adamping()) */
+    int32_t use_constraint;   /* is_constraint_on() */
+    int32_t surface;          /* compute_surf_correct() */
+    int32_t torques;          /* compute_torques() */
+    int32_t damp_amoeba;      /* use_damping_indAmoeba(): Tinker-style damping of the real space */
+    int32_t do_min;           /* do_induction_minimization(): adds the induced-induced force terms */
+} orc_indf_params;
+typedef struct {
+    double self_energy, real_energy, intra_energy, surf_energy, long_energy;   /* myEnergy.*PolEnergy */
+    double self_deriv, real_deriv, mol_deriv, surf_deriv, surf_deriv2, long_deriv, long_deriv2;
+    double tot_torque;
+    double virial[9];         /* Force::virial after compute() */
+    double long_virial[9];    /* the class's longVirial after compute_long() */
+} orc_indf_out;
+/* compute() minus the dipole solve (:486-565): self_correct (:688-760),
+ * real_term over every pair j > i within cutoffEw, minimum image (:762-1393),
+ * intraMolCorrect (:1760-2186), surfDipoleCorrect (:2188-2399), each with
+ * torques2forces (:2557-2744).  pind [3n] are the induced dipoles, polar [n]
+ * the site polarizabilities.  force is accumulated. */
+int orc_induction_short(const orc_sys *s, const double *dip, const double *quad, const double *polar,
+                        const double *pind, const orc_indf_params *p, double *force, orc_indf_out *out);
+/* compute_long / reciprocal_term (:567-590, :1395-1758). */
+int orc_induction_long(const orc_sys *s, int K, const double *kvec, const double *kmod2, const double *expk,
+                       const double *dip, const double *quad, const double *pind, const orc_indf_params *p,
+                       double *force, orc_indf_out *out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -93,6 +93,19 @@ class MpOut(C.Structure):
                [("virial", C.c_double * 9), ("long_virial", C.c_double * 9)]
 
 
+class IndfParams(C.Structure):
+    _fields_ = [("alpha", C.c_double), ("real_cut", C.c_double), ("a_damp", C.c_double),
+                ("use_constraint", C.c_int32), ("surface", C.c_int32), ("torques", C.c_int32),
+                ("damp_amoeba", C.c_int32), ("do_min", C.c_int32)]
+
+
+class IndfOut(C.Structure):
+    _fields_ = [(k, C.c_double) for k in ("self_energy", "real_energy", "intra_energy", "surf_energy",
+                                           "long_energy", "self_deriv", "real_deriv", "mol_deriv", "surf_deriv",
+                                           "surf_deriv2", "long_deriv", "long_deriv2", "tot_torque")] + \
+               [("virial", C.c_double * 9), ("long_virial", C.c_double * 9)]
+
+
 _lib = None
 
 
@@ -133,6 +146,9 @@ def lib():
                                              P(MpParams), C.c_void_p, P(MpOut)]
         _lib.orc_multipole_long.argtypes = [P(OrcSys), C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
                                             C.c_void_p, P(MpParams), C.c_void_p, P(MpOut)]
+        _lib.orc_induction_short.argtypes = [P(OrcSys)] + [C.c_void_p] * 4 + [P(IndfParams), C.c_void_p, P(IndfOut)]
+        _lib.orc_induction_long.argtypes = [P(OrcSys), C.c_int] + [C.c_void_p] * 6 + [P(IndfParams), C.c_void_p,
+                                                                                    P(IndfOut)]
         _lib.orc_induced_dipoles.argtypes = [P(OrcSys), C.c_void_p, C.c_void_p, C.c_void_p, P(IndParams),
                                              C.c_void_p, C.c_void_p, P(C.c_int), P(C.c_double)]
     return _lib
@@ -339,6 +355,38 @@ class Oracle:
         if rc != 0:
             raise RuntimeError(f"orc_multipole_long failed: {rc}")
         r = {k: getattr(out, k) for k, _ in MpOut._fields_ if "virial" not in k}
+        r["tot_torque"] = tot_short + out.tot_torque
+        r["virial"] = np.array(out.virial[:])
+        r["long_virial"] = np.array(out.long_virial[:])
+        r["short_force"], r["long_force"] = fs, fl
+        return r
+
+    def induction_terms(self, dip, quad, pind, *, alpha, real_cut, kcut, a_damp, use_constraint=True,
+                        surface=False, torques=True, damp_amoeba=False, do_min=True, polar=None):
+        """Induction::compute's terms after the dipole solve, then compute_long,
+        for given induced dipoles pind [n,3]."""
+        n = self.s.n_atoms
+        polar = self.polarizabilities() if polar is None else np.ascontiguousarray(polar, dtype=np.float64)
+        dip = np.ascontiguousarray(dip, dtype=np.float64)
+        quad = np.ascontiguousarray(quad, dtype=np.float64)
+        pind = np.ascontiguousarray(pind, dtype=np.float64)
+        p = IndfParams(alpha, real_cut, a_damp, int(use_constraint), int(surface), int(torques), int(damp_amoeba),
+                       int(do_min))
+        fs, fl = np.zeros((n, 3)), np.zeros((n, 3))
+        out = IndfOut()
+        rc = lib().orc_induction_short(C.byref(self._sys), _p(dip), _p(quad), _p(polar), _p(pind), C.byref(p),
+                                       _p(fs), C.byref(out))
+        if rc != 0:
+            raise RuntimeError(f"orc_induction_short failed: {rc}")
+        kvec, kmod2, expk = self.kvectors(kcut, alpha)
+        kvec = np.ascontiguousarray(kvec)
+        tot_short = out.tot_torque
+        out.tot_torque = 0.0
+        rc = lib().orc_induction_long(C.byref(self._sys), len(kmod2), _p(kvec), _p(kmod2), _p(expk), _p(dip),
+                                      _p(quad), _p(pind), C.byref(p), _p(fl), C.byref(out))
+        if rc != 0:
+            raise RuntimeError(f"orc_induction_long failed: {rc}")
+        r = {k: getattr(out, k) for k, _ in IndfOut._fields_ if "virial" not in k}
         r["tot_torque"] = tot_short + out.tot_torque
         r["virial"] = np.array(out.virial[:])
         r["long_virial"] = np.array(out.long_virial[:])

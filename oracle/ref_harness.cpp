@@ -7,6 +7,7 @@
 // Usage:  ref_harness <dir> forces <force-list>   (lj,mie,buck,buff,ewald)
 //         ref_harness <dir> induction
 //         ref_harness <dir> multipole
+//         ref_harness <dir> induction_forces
 //         ref_harness <dir> pairlist
 //         ref_harness <dir> md <nsteps>
 //         ref_harness <dir> time <nsteps>
@@ -355,6 +356,44 @@ int main(int argc, char** argv) {
         meta.put("mp_longSecDeriv", ens->myLustig.longSecDeriv);
         meta.put("mp_totTorque", ens->myEnergy.TotTorq);
         meta.put("cutoff_ew", ens->myConfig->get_cutoffEw());
+    } else if (mode == "induction_forces") {
+        // Induction::compute() (dipole solve, then the energy, force and torque
+        // terms) and compute_long() (src/forces/Induction.cpp:486-590).
+        for (int m = 0; m < ens->nMols; ++m) ens->molecules[m].mass_center();
+        Induction* ind = make_zeroed<Induction>(ens);
+        zero_forces(ens);
+        std::memset(&ens->myLustig, 0, sizeof ens->myLustig);
+        std::memset(&ens->myEnergy, 0, sizeof ens->myEnergy);
+        double vir[9];
+        ind->compute();
+        ind->get_virial(vir);
+        std::vector<double> F = forces_of(ens);
+        write_bin("indf_short_force", F.data(), F.size() * 8);
+        write_bin("indf_short_virial", vir, sizeof vir);
+        const int n = ens->nAtoms;
+        std::vector<double> pi(ind->Pi, ind->Pi + 3 * n);
+        write_bin("indf_pi", pi.data(), pi.size() * 8);
+        meta.put("indf_real", ens->myEnergy.realPolEnergy);
+        meta.put("indf_self", ens->myEnergy.correctPolEnergy);
+        meta.put("indf_molCorrect", ens->myEnergy.molCorrectPolEnergy);
+        meta.put("indf_surf", ens->myEnergy.surfacePolEnergy);
+        meta.put("indf_realDeriv", ens->myLustig.realPolDeriv);
+        meta.put("indf_molDeriv", ens->myLustig.molPolDeriv);
+        meta.put("indf_surfDeriv", ens->myLustig.surfPolDeriv);
+        meta.put("indf_surfSecDeriv", ens->myLustig.surfSecPolDeriv);
+        meta.put("indf_selfDeriv", ens->myLustig.selfPolDeriv);
+        zero_forces(ens);
+        ind->compute_long();
+        ind->get_long_virial(vir);
+        F = forces_of(ens);
+        write_bin("indf_long_force", F.data(), F.size() * 8);
+        write_bin("indf_long_virial", vir, sizeof vir);
+        meta.put("indf_long", ens->myEnergy.longPolEnergy);
+        meta.put("indf_longDeriv", ens->myLustig.longPolDeriv);
+        meta.put("indf_longSecDeriv", ens->myLustig.longSecPolDeriv);
+        meta.put("indf_totTorque", ind->totTrqI);
+        meta.put("cutoff_ew", ens->myConfig->get_cutoffEw());
+        meta.put("cutoff_ind", ens->myConfig->get_cutoffInd());
     } else if (mode == "md" || mode == "time") {
         const int nsteps = argc > 3 ? std::atoi(argv[3]) : 1;
         // The reference never initialises Molecule::massCenter before the

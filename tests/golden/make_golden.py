@@ -126,6 +126,27 @@ CASES = [
      "multipole", "",
      {"cutoff": 0.6, "cut_buff": 0.1, "cutoff_ew": 0.7, "alpha": 3.4, "kcut": 12.0, "multipole": True,
       "torques": True, "surface": False, "constraint": False, "ewald": False}),
+    ("amoeba216_indforces_cg",
+     {"builder": "water", "args": {"n_mol": 216, "density_mol": 33.4, "seed": 47, "lattice": [6, 6, 6],
+                                   "r_oh": 0.09572, "hoh_deg": 108.5, "types": "amoeba"}},
+     "induction_forces", "",
+     {"cutoff": 0.7, "cut_buff": 0.05, "cutoff_ew": 0.7, "alpha": 3.0, "kcut": 14.0, "induction": True,
+      "torques": True, "surface": True, "constraint": True, "ewald": False, "ind_damp": True, "ind_min": True,
+      "cutoff_ind": 0.7, "a_damp": 0.39, "damp_amoeba": False}),
+    ("amoeba216_indforces_tinker_flexible",
+     {"builder": "water", "args": {"n_mol": 216, "density_mol": 33.4, "seed": 53, "lattice": [6, 6, 6],
+                                   "r_oh": 0.09572, "hoh_deg": 108.5, "types": "amoeba"}},
+     "induction_forces", "",
+     {"cutoff": 0.65, "cut_buff": 0.05, "cutoff_ew": 0.68, "alpha": 3.3, "kcut": 13.0, "induction": True,
+      "torques": True, "surface": False, "constraint": False, "ewald": False, "ind_damp": True, "ind_min": True,
+      "cutoff_ind": 0.65, "a_damp": 0.39, "damp_amoeba": True}),
+    ("amoeba216_indforces_tinker_direct",
+     {"builder": "water", "args": {"n_mol": 216, "density_mol": 33.4, "seed": 59, "lattice": [6, 6, 6],
+                                   "r_oh": 0.09572, "hoh_deg": 108.5, "types": "amoeba"}},
+     "induction_forces", "",
+     {"cutoff": 0.7, "cut_buff": 0.05, "cutoff_ew": 0.7, "alpha": 3.0, "kcut": 14.0, "induction": True,
+      "torques": True, "surface": True, "constraint": True, "ewald": False, "ind_damp": False, "ind_min": False,
+      "cutoff_ind": 0.6, "a_damp": 0.39, "damp_amoeba": True}),
     ("water512_md10",
      {"builder": "water", "args": {"n_mol": 512, "density_mol": 33.4, "seed": 5, "lattice": [8, 8, 8]}},
      "md", "10",

@@ -347,3 +347,29 @@ def test_multipole_ewald_is_independent_of_alpha(oracle_mod):
     assert abs(le1 - le0) > 100.0 and abs(re1 - re0) > 1.0   # the split really moved
     assert e1 == pytest.approx(e0, rel=2e-6)
     assert np.abs(f1 - f0).max() <= 1e-6 * np.abs(f0).max()
+
+
+@pytest.mark.parametrize("do_min", [False, True])
+def test_induction_terms_are_independent_of_alpha(oracle_mod, do_min):
+    """Fixed induced dipoles against the permanent multipoles under Ewald, with
+    Thole damping switched off (a huge a makes lambda = 1): the total energy and
+    forces must not depend on alpha.  The product-rule damping form only; the
+    Tinker form subtracts the bare interaction of excluded pairs in real space
+    on top of the intramolecular correction, so it is not split-invariant by
+    construction and is pinned by the reference's own outputs instead."""
+    from swinmd_amd import systems as S
+    s = S.water(64, density_mol=33.4, seed=7, lattice=(4, 4, 4), r_oh=0.09572, hoh_deg=108.5, types=S.AMOEBA_TYPES)
+    o = _oracle(s)
+    dip, quad = o.local2global()
+    rc = 0.49 * s.box[0]
+    pind = np.random.default_rng(3).normal(scale=0.01, size=(s.n_atoms, 3))
+    out = []
+    for alpha in (6.5, 7.5):
+        r = o.induction_terms(dip, quad, pind, alpha=alpha, real_cut=rc, kcut=2 * alpha * 4.6, a_damp=1e9,
+                              use_constraint=False, surface=False, torques=True, damp_amoeba=False, do_min=do_min)
+        e = r["real_energy"] + r["long_energy"] + r["self_energy"] + r["intra_energy"]
+        out.append((e, r["short_force"] + r["long_force"], r["long_energy"]))
+    (e0, f0, l0), (e1, f1, l1) = out
+    assert abs(l1 - l0) > 10.0                      # the split really moved
+    assert e1 == pytest.approx(e0, rel=1e-6)
+    assert np.abs(f1 - f0).max() <= 5e-6 * np.abs(f0).max()

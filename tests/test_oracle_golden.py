@@ -163,3 +163,34 @@ def test_multipole_ewald(oracle_mod, name):
     for k, mk in MP_KEYS:
         assert abs(r[k] - fx.meta[mk]) <= TOL * max(abs(fx.meta[mk]), 1e-12), k
     assert rel(r["self_energy"] + r["charged_energy"], fx.meta["mp_correct"]) <= TOL
+
+
+INDF_KEYS = (("real_energy", "indf_real"), ("self_energy", "indf_self"), ("intra_energy", "indf_molCorrect"),
+             ("surf_energy", "indf_surf"), ("long_energy", "indf_long"), ("real_deriv", "indf_realDeriv"),
+             ("mol_deriv", "indf_molDeriv"), ("surf_deriv", "indf_surfDeriv"), ("surf_deriv2", "indf_surfSecDeriv"),
+             ("self_deriv", "indf_selfDeriv"), ("long_deriv", "indf_longDeriv"), ("long_deriv2", "indf_longSecDeriv"))
+
+
+@pytest.mark.parametrize("name", ["amoeba216_indforces_cg", "amoeba216_indforces_tinker_flexible",
+                                  "amoeba216_indforces_tinker_direct"])
+def test_induction_energy_force_terms(oracle_mod, name):
+    """Induction::compute after the dipole solve, and compute_long: the induced
+    dipoles' energies, forces, torques, virials and Lustig derivatives against
+    the compiled reference.  Covers both real-space damping forms (product
+    rule; Tinker), CG and direct dipoles, rigid and flexible water."""
+    fx = Fixture(name)
+    c = fx.cfg
+    o = oracle_mod.Oracle(fx.system)
+    dip, quad = o.local2global()
+    _, pi, _, _ = o.induced_dipoles(dip, quad, cutoff_ind=c["cutoff_ind"], a_damp=c["a_damp"], damp_tij=c["ind_damp"],
+                                    do_min=c["ind_min"])
+    assert rel_rms(pi, fx.vec("indf_pi")) <= TOL
+    r = o.induction_terms(dip, quad, pi, alpha=c["alpha"], real_cut=c["cutoff_ew"], kcut=c["kcut"],
+                          a_damp=c["a_damp"], use_constraint=c["constraint"], surface=c["surface"],
+                          torques=c["torques"], damp_amoeba=c["damp_amoeba"], do_min=c["ind_min"])
+    assert rel_rms(r["short_force"], fx.vec("indf_short_force")) <= TOL
+    assert rel_rms(r["long_force"], fx.vec("indf_long_force")) <= TOL
+    assert rel_rms(r["virial"], fx.ref["indf_short_virial"]) <= TOL
+    assert rel_rms(r["long_virial"], fx.ref["indf_long_virial"]) <= TOL
+    for k, mk in INDF_KEYS:
+        assert abs(r[k] - fx.meta[mk]) <= TOL * max(abs(fx.meta[mk]), 1e-12), k
