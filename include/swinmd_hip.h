@@ -291,6 +291,31 @@ typedef struct {
     double virial[9];                     /* the class's longVirial (all nine entries) */
 } smd_mp_long_out;
 
+/* Induction's energy, force and torque terms for the induced dipoles of the
+ * last smd_induce (src/forces/Induction.cpp). */
+typedef struct {
+    double alpha;             /* get_alpha() */
+    double real_cut;          /* get_cutoffEw(); at most the pair list's cutoff + buffer */
+    double a_damp;            /* Thole a (get_adamping()) */
+    int32_t use_constraint;   /* is_constraint_on() */
+    int32_t surface;          /* compute_surf_correct() */
+    int32_t torques;          /* compute_torques() */
+    int32_t damp_amoeba;      /* use_damping_indAmoeba(): Tinker-style real-space damping */
+    int32_t do_min;           /* do_induction_minimization(): induced-induced force terms */
+} smd_indf_params;
+
+/* Induction::compute() results after the dipole solve (:486-565). */
+typedef struct {
+    double self_energy;                   /* myEnergy.correctPolEnergy */
+    double real_energy;                   /* myEnergy.realPolEnergy */
+    double intra_energy;                  /* myEnergy.molCorrectPolEnergy */
+    double surf_energy;                   /* myEnergy.surfacePolEnergy */
+    double self_deriv, real_deriv, mol_deriv;   /* myLustig.selfPolDeriv, realPolDeriv, molPolDeriv */
+    double surf_deriv, surf_deriv2;       /* myLustig.surfPolDeriv, surfSecPolDeriv */
+    double tot_torque;
+    double virial[9];                     /* Force::virial */
+} smd_indf_out;
+
 /* Multipole ctor and set_up (:21-115, :242-302): parameters and the
  * half-space k set k^2 < kcut^2; *n_kvec receives its size. */
 int smd_multipole_setup(smd_ctx* ctx, const smd_mp_params* p, double kcut, int32_t* n_kvec);
@@ -301,6 +326,16 @@ int smd_multipole_setup(smd_ctx* ctx, const smd_mp_params* p, double kcut, int32
 int smd_multipole_compute(smd_ctx* ctx, int prec, double* f_accum, smd_mp_out* out);
 /* Multipole::compute_long / reciprocal_term (:218-240, :951-1279). */
 int smd_multipole_compute_long(smd_ctx* ctx, int prec, double* f_accum, smd_mp_long_out* out);
+/* Induction ctor and set_up (src/forces/Induction.cpp:19-158, :592-652). */
+int smd_induction_setup(smd_ctx* ctx, const smd_indf_params* p, double kcut, int32_t* n_kvec);
+/* Induction::compute after InducedDipoles (:486-565): self_correct
+ * (:688-760), real_term over every pair within cutoffEw including the
+ * excluded ones (:762-1393), intraMolCorrect (:1760-2186), surfDipoleCorrect
+ * (:2188-2399), each with torques2forces (:2557-2744).  Uses the induced
+ * dipoles of the last smd_induce. */
+int smd_induction_compute(smd_ctx* ctx, int prec, double* f_accum, smd_indf_out* out);
+/* Induction::compute_long / reciprocal_term (:567-590, :1395-1758). */
+int smd_induction_compute_long(smd_ctx* ctx, int prec, double* f_accum, smd_mp_long_out* out);
 
 /* ---- whole step: LFIntegrator::run for rigid point-charge water --------- */
 typedef struct {

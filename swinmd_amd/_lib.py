@@ -105,6 +105,18 @@ class MpLongOut(C.Structure):
                 ("virial", C.c_double * 9)]
 
 
+class IndfParams(C.Structure):
+    _fields_ = [("alpha", C.c_double), ("real_cut", C.c_double), ("a_damp", C.c_double),
+                ("use_constraint", C.c_int32), ("surface", C.c_int32), ("torques", C.c_int32),
+                ("damp_amoeba", C.c_int32), ("do_min", C.c_int32)]
+
+
+class IndfOut(C.Structure):
+    _fields_ = [(k, C.c_double) for k in ("self_energy", "real_energy", "intra_energy", "surf_energy",
+                                           "self_deriv", "real_deriv", "mol_deriv", "surf_deriv", "surf_deriv2",
+                                           "tot_torque")] + [("virial", C.c_double * 9)]
+
+
 # name -> (restype, argtypes); every entry is a symbol declared in include/swinmd_hip.h
 _P = C.c_void_p
 SIGNATURES = {
@@ -154,6 +166,9 @@ SIGNATURES = {
     "smd_multipole_setup": (C.c_int, [_P, C.POINTER(MpParams), C.c_double, C.POINTER(C.c_int32)]),
     "smd_multipole_compute": (C.c_int, [_P, C.c_int, _P, C.POINTER(MpOut)]),
     "smd_multipole_compute_long": (C.c_int, [_P, C.c_int, _P, C.POINTER(MpLongOut)]),
+    "smd_induction_setup": (C.c_int, [_P, C.POINTER(IndfParams), C.c_double, C.POINTER(C.c_int32)]),
+    "smd_induction_compute": (C.c_int, [_P, C.c_int, _P, C.POINTER(IndfOut)]),
+    "smd_induction_compute_long": (C.c_int, [_P, C.c_int, _P, C.POINTER(MpLongOut)]),
 }
 
 _lib = None

@@ -47,10 +47,13 @@ constexpr int MP_KTILE = 128;                  // k staged per LDS tile in the f
 enum MpSlot {
     M_SELF = 0, M_REAL_E, M_REAL_D1, M_REAL_VIR,                 // VIR: 9
     M_INTRA_E = M_REAL_VIR + 9, M_MOL_D1,
-    M_SU_QR, M_SU_QRR = M_SU_QR + 3, M_SU_QM = M_SU_QRR + 3, M_SU_SELF = M_SU_QM + 3, M_SU_VIR,   // 3 3 3 1 9
-    M_DUDV = M_SU_VIR + 9,                                        // 4: real, intra, surface, long
-    M_TRQ = M_DUDV + 4,                                           // 4: |torque| sums per part
-    M_LONG_E = M_TRQ + 4, M_LONG_D1, M_LONG_D2, M_LONG_VIR,      // VIR: 6 (xx yy zz xy xz yz)
+    // surface sums: q r, q R_mol, q r + mu (multipoles) or mu (induction),
+    // the self sum, then the induced dipoles' sum and their squares
+    M_SU_QR, M_SU_QRR = M_SU_QR + 3, M_SU_QM = M_SU_QRR + 3, M_SU_SELF = M_SU_QM + 3, M_SU_PI,
+    M_SU_PP = M_SU_PI + 3, M_SU_VIR,                              // VIR: 9
+    M_DUDV = M_SU_VIR + 9,                                        // 5 parts: real, intra, surface, long, self
+    M_TRQ = M_DUDV + 5,                                           // 5: |torque| sums per part
+    M_LONG_E = M_TRQ + 5, M_LONG_D1, M_LONG_D2, M_LONG_VIR,      // VIR: 6 (xx yy zz xy xz yz)
     M_COUNT = M_LONG_VIR + 6
 };
 
@@ -70,15 +73,20 @@ struct MpArgs {
     const int32_t* excl_off;
     const int32_t* excl;
     double* force;
-    double* trq;                // [4][3n]: real, intra, surface, k-space
+    double* trq;                // [5][3n]: real, intra, surface, k-space, self
     double* terms;              // [M_COUNT]
     const double* kvec;         // [3K]
     const double* kmod2;        // [K]
     const double* expk;         // [K]
-    double* part;               // [chunks][K][MP_NSUM]
-    double* kf;                 // [K][2]: b P, b Q
+    double* part;               // [chunks][K][sums]
+    double* kf;                 // [K][4]: force-pass coefficients
     double alpha, alpha2, alpha2PI, realcut2, volume, alphaR4;
     int use_constraint, torques;
+    // induction terms
+    const double* pind;         // induced dipoles [3n]
+    const double* polar;        // polarizabilities [n]
+    double a_damp;
+    int damp_amoeba, do_min;
 };
 
 struct V3 { double x, y, z; };
@@ -398,16 +406,17 @@ __device__ __forceinline__ V3 h_site(V3 rh, V3 ro, V3 tau, double& dudv) {
     return dUdu;
 }
 
-// One thread per water; parts [p0, p1) of the torque array, each part's
-// dU/dV summed into its own slot.
-__global__ __launch_bounds__(256) void k_mp_torques(MpArgs a, int p0, int p1) {
+// One thread per water; the torque-array parts set in `parts` (bit p for
+// part p), each part's dU/dV summed into its own slot.
+__global__ __launch_bounds__(256) void k_mp_torques(MpArgs a, unsigned parts) {
     const int m = blockIdx.x * blockDim.x + threadIdx.x;
-    double dv[4] = {0, 0, 0, 0}, tn[4] = {0, 0, 0, 0};
+    double dv[5] = {0, 0, 0, 0, 0}, tn[5] = {0, 0, 0, 0, 0};
     if (m < a.nmol) {
         const int h1 = a.mol_start[m], o = h1 + 1, h2 = h1 + 2;
         const V3 r1 = ldv(a.pos + 3 * h1), ro = ldv(a.pos + 3 * o), r2 = ldv(a.pos + 3 * h2);
         V3 F1 = v3(0, 0, 0), FO = v3(0, 0, 0), F2 = v3(0, 0, 0);
-        for (int p = p0; p < p1; ++p) {
+        for (int p = 0; p < 5; ++p) {
+            if (!(parts >> p & 1u)) continue;
             const double* tr = a.trq + 3 * (size_t)a.n * p;
             const V3 t1 = ldv(tr + 3 * h1), to = ldv(tr + 3 * o), t2 = ldv(tr + 3 * h2);
             tn[p] = sqrt(dot(t1, t1)) + sqrt(dot(to, to)) + sqrt(dot(t2, t2));
@@ -441,9 +450,9 @@ __global__ __launch_bounds__(256) void k_mp_torques(MpArgs a, int p0, int p1) {
         a.force[3 * o] += FO.x; a.force[3 * o + 1] += FO.y; a.force[3 * o + 2] += FO.z;
         a.force[3 * h2] += F2.x; a.force[3 * h2 + 1] += F2.y; a.force[3 * h2 + 2] += F2.z;
     }
-    block_add<4>(dv, a.terms + M_DUDV);
+    block_add<5>(dv, a.terms + M_DUDV);
     __syncthreads();
-    block_add<4>(tn, a.terms + M_TRQ);
+    block_add<5>(tn, a.terms + M_TRQ);
 }
 
 // ---- k-space ---------------------------------------------------------------------
@@ -562,8 +571,8 @@ __global__ __launch_bounds__(256) void k_mp_kfinish(MpArgs a) {
         v[6] = -t * coeff * kv.x * kv.y;
         v[7] = -t * coeff * kv.x * kv.z;
         v[8] = -t * coeff * kv.y * kv.z;
-        a.kf[2 * k] = b * P;
-        a.kf[2 * k + 1] = b * Q;
+        a.kf[4 * k] = b * P;
+        a.kf[4 * k + 1] = b * Q;
     }
     block_add<9>(v, a.terms + M_LONG_E);
 }
@@ -587,8 +596,8 @@ __global__ __launch_bounds__(MP_RF_THREADS) void k_mp_kforce(MpArgs a) {
             sk[x][0] = a.kvec[3 * (k0 + x)];
             sk[x][1] = a.kvec[3 * (k0 + x) + 1];
             sk[x][2] = a.kvec[3 * (k0 + x) + 2];
-            sk[x][3] = a.kf[2 * (k0 + x)];
-            sk[x][4] = a.kf[2 * (k0 + x) + 1];
+            sk[x][3] = a.kf[4 * (k0 + x)];
+            sk[x][4] = a.kf[4 * (k0 + x) + 1];
         }
         __syncthreads();
         if (i < a.n) {
@@ -624,14 +633,15 @@ __global__ __launch_bounds__(MP_RF_THREADS) void k_mp_kforce(MpArgs a) {
 
 int frames_run(Ctx* c);
 
-static MpArgs mp_args(Ctx* c) {
+static MpArgs mp_args(Ctx* c, const Ctx::KSet& ks, double alpha, double real_cut, int use_constraint,
+                      int torques) {
     MpArgs a{};
     a.n = c->n;
     a.nmol = c->nmol;
     a.stride = c->stride;
-    a.K = c->mp_K;
-    a.chunks = c->mp_chunks;
-    a.per = c->mp_K > 0 ? (c->n + c->mp_chunks - 1) / c->mp_chunks : 0;
+    a.K = ks.K;
+    a.chunks = ks.chunks;
+    a.per = ks.K > 0 ? (c->n + ks.chunks - 1) / ks.chunks : 0;
     a.box = c->box;
     a.mol_start = c->d_mol_start;
     a.mol = c->d_mol;
@@ -648,21 +658,26 @@ static MpArgs mp_args(Ctx* c) {
     a.force = c->d_force;
     a.trq = c->d_mp_trq;
     a.terms = c->d_mp_terms;
-    a.kvec = c->d_mp_kvec;
-    a.kmod2 = c->d_mp_kmod2;
-    a.expk = c->d_mp_expk;
-    a.part = c->d_mp_part;
-    a.kf = c->d_mp_kf;
-    const smd_mp_params& p = c->mp;
-    a.alpha = p.alpha;
-    a.alpha2 = p.alpha * p.alpha;
-    a.alpha2PI = 2.0 * p.alpha / SQRT_PI;
-    a.realcut2 = p.real_cut * p.real_cut;
+    a.kvec = ks.kvec;
+    a.kmod2 = ks.kmod2;
+    a.expk = ks.expk;
+    a.part = ks.part;
+    a.kf = ks.kf;
+    a.alpha = alpha;
+    a.alpha2 = alpha * alpha;
+    a.alpha2PI = 2.0 * alpha / SQRT_PI;
+    a.realcut2 = real_cut * real_cut;
     a.volume = c->box.L[0] * c->box.L[1] * c->box.L[2];
     a.alphaR4 = 1.0 / (4.0 * a.alpha2);
-    a.use_constraint = p.use_constraint;
-    a.torques = p.torques;
+    a.use_constraint = use_constraint;
+    a.torques = torques;
+    a.pind = c->d_mu;
+    a.polar = c->d_polar;
     return a;
+}
+
+static MpArgs mp_args(Ctx* c) {
+    return mp_args(c, c->mp_ks, c->mp.alpha, c->mp.real_cut, c->mp.use_constraint, c->mp.torques);
 }
 
 static int mp_check(Ctx* c) {
@@ -697,7 +712,7 @@ int multipole_short(Ctx* c, smd_mp_out* out) {
     const double volume = a.volume, volumer = 1.0 / volume;
     const double coeff = 2.0 * PI * volumer / 3.0;
     if (c->mp.surface) hipLaunchKernelGGL(k_mp_surface, ga, b256, 0, c->stream, a, coeff);
-    if (c->mp.torques) hipLaunchKernelGGL(k_mp_torques, gm, b256, 0, c->stream, a, 0, c->mp.surface ? 3 : 2);
+    if (c->mp.torques) hipLaunchKernelGGL(k_mp_torques, gm, b256, 0, c->stream, a, c->mp.surface ? 7u : 3u);
     SMD_HIP(c, hipGetLastError());
     double t[M_COUNT];
     if ((rc = mp_read_terms(c, t))) return rc;
@@ -747,7 +762,7 @@ int multipole_long(Ctx* c, smd_mp_long_out* out) {
     hipLaunchKernelGGL(k_mp_kfinish, dim3((a.K + 255) / 256), b256, 0, c->stream, a);
     hipLaunchKernelGGL(k_mp_kforce, dim3((c->n + MP_RF_THREADS - 1) / MP_RF_THREADS), dim3(MP_RF_THREADS), 0, c->stream,
                        a);
-    if (c->mp.torques) hipLaunchKernelGGL(k_mp_torques, dim3((c->nmol + 255) / 256), b256, 0, c->stream, a, 3, 4);
+    if (c->mp.torques) hipLaunchKernelGGL(k_mp_torques, dim3((c->nmol + 255) / 256), b256, 0, c->stream, a, 8u);
     SMD_HIP(c, hipGetLastError());
     double t[M_COUNT];
     if ((rc = mp_read_terms(c, t))) return rc;
@@ -764,15 +779,16 @@ int multipole_long(Ctx* c, smd_mp_long_out* out) {
     return SMD_OK;
 }
 
-static int mp_setup(Ctx* c, const smd_mp_params* p, double kcut, int32_t* n_kvec) {
+// Half-space k set, k^2 < kCut^2, and expK (Multipole::set_up :242-302;
+// Induction::set_up builds the same set, src/forces/Induction.cpp:592-652),
+// with structure-factor scratch for `nsum` sums per k.
+static int kset_build(Ctx* c, Ctx::KSet& ks, double alpha, double kcut, int nsum) {
     if (!c->box_set) return fail(c, SMD_ESTATE, "box not set");
-    if (!(p->alpha > 0) || !(p->real_cut > 0) || !(kcut > 0)) return fail(c, SMD_EINVAL, "bad multipole parameters");
     for (int m = 0; m < c->nmol; ++m)
         if (c->h_mol_start[m + 1] - c->h_mol_start[m] != 3)
-            return fail(c, SMD_ENOTSUP, "multipole Ewald is implemented for 3-site water (H1, O, H2)");
-    // Multipole::set_up (:242-302): half-space k set, k^2 < kCut^2
+            return fail(c, SMD_ENOTSUP, "multipole and induction Ewald are implemented for 3-site water (H1, O, H2)");
     std::vector<double> kv, k2v, ekv;
-    const double kCut2 = kcut * kcut, alphaR4 = 1.0 / (4.0 * (p->alpha * p->alpha));
+    const double kCut2 = kcut * kcut, alphaR4 = 1.0 / (4.0 * (alpha * alpha));
     const int nxMax = (int)(kcut * c->box.L[0] / (2 * PI));
     const int nyMax = (int)(kcut * c->box.L[1] / (2 * PI));
     const int nzMax = (int)(kcut * c->box.L[2] / (2 * PI));
@@ -796,25 +812,629 @@ static int mp_setup(Ctx* c, const smd_mp_params* p, double kcut, int32_t* n_kvec
     auto R = [&](double*& ptr, size_t bytes) -> int {
         if (ptr) (void)hipFree(ptr);
         ptr = nullptr;
-        if (hipMalloc(&ptr, bytes) != hipSuccess) return fail(c, SMD_ENOMEM, "multipole allocation of %zu bytes", bytes);
+        if (hipMalloc(&ptr, bytes) != hipSuccess) return fail(c, SMD_ENOMEM, "Ewald k-space allocation of %zu bytes", bytes);
         return SMD_OK;
     };
     int rc;
-    if ((rc = R(c->d_mp_kvec, sizeof(double) * 3 * K)) || (rc = R(c->d_mp_kmod2, sizeof(double) * K)) ||
-        (rc = R(c->d_mp_expk, sizeof(double) * K)) || (rc = R(c->d_mp_kf, sizeof(double) * 2 * K)) ||
-        (rc = R(c->d_mp_part, sizeof(double) * MP_NSUM * (size_t)K * chunks)))
+    if ((rc = R(ks.kvec, sizeof(double) * 3 * K)) || (rc = R(ks.kmod2, sizeof(double) * K)) ||
+        (rc = R(ks.expk, sizeof(double) * K)) || (rc = R(ks.kf, sizeof(double) * 4 * K)) ||
+        (rc = R(ks.part, sizeof(double) * nsum * (size_t)K * chunks)))
         return rc;
-    if (!c->d_mp_trq && (rc = R(c->d_mp_trq, sizeof(double) * 12 * (size_t)c->n))) return rc;
+    if (!c->d_mp_trq && (rc = R(c->d_mp_trq, sizeof(double) * 15 * (size_t)c->n))) return rc;
     if (!c->d_mp_terms && (rc = R(c->d_mp_terms, sizeof(double) * M_COUNT))) return rc;
-    SMD_HIP(c, hipMemset(c->d_mp_trq, 0, sizeof(double) * 12 * (size_t)c->n));
-    SMD_HIP(c, hipMemcpy(c->d_mp_kvec, kv.data(), sizeof(double) * 3 * K, hipMemcpyHostToDevice));
-    SMD_HIP(c, hipMemcpy(c->d_mp_kmod2, k2v.data(), sizeof(double) * K, hipMemcpyHostToDevice));
-    SMD_HIP(c, hipMemcpy(c->d_mp_expk, ekv.data(), sizeof(double) * K, hipMemcpyHostToDevice));
+    SMD_HIP(c, hipMemset(c->d_mp_trq, 0, sizeof(double) * 15 * (size_t)c->n));
+    SMD_HIP(c, hipMemcpy(ks.kvec, kv.data(), sizeof(double) * 3 * K, hipMemcpyHostToDevice));
+    SMD_HIP(c, hipMemcpy(ks.kmod2, k2v.data(), sizeof(double) * K, hipMemcpyHostToDevice));
+    SMD_HIP(c, hipMemcpy(ks.expk, ekv.data(), sizeof(double) * K, hipMemcpyHostToDevice));
+    ks.K = K;
+    ks.chunks = chunks;
+    return SMD_OK;
+}
+
+static int mp_setup(Ctx* c, const smd_mp_params* p, double kcut, int32_t* n_kvec) {
+    if (!(p->alpha > 0) || !(p->real_cut > 0) || !(kcut > 0)) return fail(c, SMD_EINVAL, "bad multipole parameters");
+    int rc = kset_build(c, c->mp_ks, p->alpha, kcut, MP_NSUM);
+    if (rc) return rc;
     c->mp = *p;
-    c->mp_K = K;
-    c->mp_chunks = chunks;
     c->mp_ready = true;
-    if (n_kvec) *n_kvec = K;
+    if (n_kvec) *n_kvec = c->mp_ks.K;
+    return SMD_OK;
+}
+
+
+// =====================================================================================
+// Induced dipoles against the permanent multipoles (Induction::compute after
+// the dipole solve, and compute_long; src/forces/Induction.cpp).
+// =====================================================================================
+
+constexpr int IN_NSUM = 28;   // structure-factor sums per k of the induction pass
+
+// Site I's permanent multipoles with partner J's induced dipole pj at
+// r = r_I - r_J (the "i" side of real_term :940-1028 and the torque vectors
+// :1276-1281).  The reference's "j" side is this with the roles swapped.
+struct IndSide { double G1, G2, G3; V3 g1, g2, g3; V3 Gt1, Gt2, Gt3; };
+
+template <bool GRAD, bool TORQUE>
+__device__ __forceinline__ void ind_side(const Site& I, V3 pj, V3 r, IndSide& t) {
+    const double QiI = trace(I.Q);
+    const V3 Qir = smul(I.Q, r);
+    const double QiR = dot(r, Qir);
+    const double pjr = dot(pj, r), mir = dot(I.d, r);
+    t.G1 = -I.q * pjr - dot(I.d, pj);
+    t.G2 = -mir * pjr - 2.0 * dot(pj, Qir) - pjr * QiI;
+    t.G3 = -pjr * QiR;
+    if constexpr (GRAD) {
+        t.g1 = (-I.q) * pj;
+        t.g2 = (-pjr) * I.d - mir * pj - 2.0 * smul(I.Q, pj) - QiI * pj;
+        t.g3 = (-QiR) * pj - (2.0 * pjr) * Qir;
+    }
+    if constexpr (TORQUE) {
+        // cross(Q, a b^T, ones) = cross(Q b, a)
+        t.Gt1 = cross(I.d, pj);
+        t.Gt2 = pjr * cross(I.d, r) + 2.0 * cross(Qir, pj) + 2.0 * cross(smul(I.Q, pj), r);
+        t.Gt3 = (2.0 * pjr) * cross(Qir, r);
+    }
+}
+
+// Damped radial functions of one pair (:1010-1066).
+struct IndRadial { double B1d, B2d, B3d, B4d, gB1, gB2, gB3, B2du, gB1u, gB2u; };
+
+__device__ __forceinline__ void ind_radial(const MpArgs& a, double r2, double pi_, double pj_, double scale,
+                                           IndRadial& o) {
+    double B[6];
+    radial_b(a, r2, B);
+    const double r = sqrt(r2);
+    const double u3 = r * r * r / sqrt(pi_ * pj_);
+    const double expau3 = exp(-a.a_damp * u3), au3 = a.a_damp * u3, a2u6 = au3 * au3, a3u9 = a2u6 * au3;
+    const double l3 = 1.0 - expau3;
+    const double l5 = 1.0 - (1.0 + au3) * expau3;
+    const double l7 = 1.0 - (1.0 + au3 + 3.0 * a2u6 / 5.0) * expau3;
+    const double gl3 = 3.0 * au3 * expau3 / r2;
+    const double gl5 = 3.0 * a2u6 * expau3 / r2;
+    const double gl7 = (-3.0 * a2u6 + 9.0 * a3u9) * expau3 / (5.0 * r2);
+    if (a.damp_amoeba) {
+        const double invr = 1.0 / r, invr2 = invr * invr, invr3 = invr2 * invr;
+        const double invr5 = 3.0 * invr3 * invr2, invr7 = 5.0 * invr5 * invr2, invr9 = 7.0 * invr7 * invr2;
+        o.B1d = B[1] + (1.0 - scale * l3) * invr3;
+        o.B2d = B[2] - (1.0 - scale * l5) * invr5;
+        o.B3d = B[3] + (1.0 - scale * l7) * invr7;
+        o.B4d = 0.0;
+        o.gB1 = B[2] - scale * gl3 * invr3 - (1.0 - scale * l3) * invr5;
+        o.gB2 = B[3] + scale * gl5 * invr5 + (1.0 - scale * l5) * invr7;
+        o.gB3 = B[4] - scale * gl7 * invr7 - (1.0 - scale * l7) * invr9;
+        o.B2du = B[2] - (1.0 - l5) * invr5;
+        o.gB1u = B[2] - gl3 * invr3 - (1.0 - l3) * invr5;
+        o.gB2u = B[3] + gl5 * invr5 + (1.0 - l5) * invr7;
+    } else {
+        o.B1d = B[1] * l3;
+        o.B2d = B[2] * l3 + B[1] * gl3;
+        o.B3d = B[3] * l3 + B[2] * 2.0 * gl3 + B[1] * (gl3 - 3.0 * gl5) / r2;
+        o.B4d = B[4] * l3 + B[3] * 3.0 * gl3 + B[2] * (3.0 * gl3 - 9.0 * gl5) / r2 +
+                B[1] * (-gl3 - 6.0 * gl5 + 15.0 * gl7) / (r2 * r2);
+        o.gB1 = o.gB2 = o.gB3 = o.B2du = o.gB1u = o.gB2u = 0.0;
+    }
+}
+
+// One pair seen from row atom i (partner j, r = r_i - r_j): force on i, torque
+// on i, and the pair's energy / virial / dU/dV contributions at half weight
+// (every pair is visited from both rows).
+__device__ __forceinline__ void ind_entry(const MpArgs& a, const Site& I, V3 pi, double poli, V3 ri, int j,
+                                          double scale, V3& f, V3& t, double (&acc)[11]) {
+    double dx = ri.x - a.pos[3 * j], dy = ri.y - a.pos[3 * j + 1], dz = ri.z - a.pos[3 * j + 2];
+    pbc(dx, dy, dz, a.box);
+    const double r2 = dx * dx + dy * dy + dz * dz;
+    if (!(r2 < a.realcut2)) return;
+    const Site J = load_site(a, j);
+    const V3 pj = ldv(a.pind + 3 * j);
+    const V3 r = v3(dx, dy, dz), rji = -r;
+    IndSide si, sj;
+    if (a.torques) ind_side<true, true>(I, pj, r, si);
+    else ind_side<true, false>(I, pj, r, si);
+    ind_side<true, false>(J, pi, rji, sj);
+    IndRadial R;
+    ind_radial(a, r2, poli, a.polar[j], scale, R);
+    const double Gsum = (si.G1 + sj.G1) * R.B1d + (si.G2 + sj.G2) * R.B2d + (si.G3 + sj.G3) * R.B3d;
+    const double uen = a.damp_amoeba ? 0.5 * Gsum * COULOMBS : 0.5 * Gsum * scale * COULOMBS;
+    const double G1ip = -dot(pi, pj);
+    const double G2ip = -dot(pi, r) * dot(pj, r);
+    const V3 g2ip = (-dot(pj, r)) * pi;
+    const V3 g2jp = (-dot(pi, rji)) * pj;
+    V3 fi, fj;
+    if (a.damp_amoeba) {
+        V3 gi = R.B1d * si.g1 + R.B2d * si.g2 + R.B3d * si.g3;
+        V3 gj = R.B1d * sj.g1 + R.B2d * sj.g2 + R.B3d * sj.g3;
+        double ri_ = si.G1 * R.gB1 + si.G2 * R.gB2 + si.G3 * R.gB3;
+        const double rj_ = sj.G1 * R.gB1 + sj.G2 * R.gB2 + sj.G3 * R.gB3;
+        if (a.do_min) {
+            gi = gi + R.B2du * g2ip;
+            gj = gj + R.B2du * g2jp;
+            ri_ += G1ip * R.gB1u + G2ip * R.gB2u;
+        }
+        fi = -gi - ri_ * r;
+        fj = -gj - rj_ * rji;
+    } else {
+        V3 gi = scale * (R.B1d * si.g1 + R.B2d * si.g2 + R.B3d * si.g3);
+        V3 gj = scale * (R.B1d * sj.g1 + R.B2d * sj.g2 + R.B3d * sj.g3);
+        double ri_ = scale * (si.G1 * R.B2d + si.G2 * R.B3d + si.G3 * R.B4d);
+        const double rj_ = scale * (sj.G1 * R.B2d + sj.G2 * R.B3d + sj.G3 * R.B4d);
+        if (a.do_min) {
+            gi = gi + R.B2d * g2ip;
+            gj = gj + R.B2d * g2jp;
+            ri_ += G1ip * R.B2d + G2ip * R.B3d;
+        }
+        fi = -gi - ri_ * r;
+        fj = -gj - rj_ * rji;
+    }
+    const V3 fij = fi - fj;
+    f = f + fij;
+    if (a.torques) {
+        const V3 ti = R.B1d * si.Gt1 + R.B2d * si.Gt2 + R.B3d * si.Gt3;
+        t = t + (a.damp_amoeba ? 1.0 : scale) * ti;
+    }
+    acc[0] += 0.5 * uen;
+    acc[1] += -0.5 * dot(fij, r);
+    acc[2] += 0.5 * fij.x * r.x; acc[3] += 0.5 * fij.y * r.x; acc[4] += 0.5 * fij.z * r.x;
+    acc[5] += 0.5 * fij.x * r.y; acc[6] += 0.5 * fij.y * r.y; acc[7] += 0.5 * fij.z * r.y;
+    acc[8] += 0.5 * fij.x * r.z; acc[9] += 0.5 * fij.y * r.z; acc[10] += 0.5 * fij.z * r.z;
+}
+
+// Real space over every pair within cutoffEw: a row's pair-list partners
+// (both segments, not excluded) and its excluded partners from the exclusion
+// table (scale 0).
+template <int G>
+__global__ __launch_bounds__(256) void k_ind_real(MpArgs a) {
+    const int tid = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = tid / G, sub = tid % G;
+    double acc[11];
+#pragma unroll
+    for (int k = 0; k < 11; ++k) acc[k] = 0.0;
+    V3 f = v3(0, 0, 0), t = v3(0, 0, 0);
+    if (i < a.n) {
+        const int32_t* row = a.nb + (int64_t)i * a.stride;
+        const int own = min(a.nb_own[i], a.stride);
+        const int non = min(a.nb_cnt[i] - a.nb_own[i], a.stride - own);
+        const Site I = load_site(a, i);
+        const V3 pi = ldv(a.pind + 3 * i);
+        const double poli = a.polar[i];
+        const V3 ri = ldv(a.pos + 3 * i);
+        for (int k = sub; k < own; k += G) ind_entry(a, I, pi, poli, ri, row[k], 1.0, f, t, acc);
+        const int32_t* back = row + a.stride - 1;
+        for (int k = sub; k < non; k += G) ind_entry(a, I, pi, poli, ri, back[-k], 1.0, f, t, acc);
+        const int e0 = a.excl_off[i], e1 = a.excl_off[i + 1];
+        for (int k = e0 + sub; k < e1; k += G) ind_entry(a, I, pi, poli, ri, a.excl[k], 0.0, f, t, acc);
+    }
+#pragma unroll
+    for (int o = G / 2; o > 0; o >>= 1) {
+        f.x += __shfl_xor(f.x, o, G); f.y += __shfl_xor(f.y, o, G); f.z += __shfl_xor(f.z, o, G);
+        t.x += __shfl_xor(t.x, o, G); t.y += __shfl_xor(t.y, o, G); t.z += __shfl_xor(t.z, o, G);
+    }
+    if (i < a.n && sub == 0) {
+        a.force[3 * i] += COULOMBS * f.x;
+        a.force[3 * i + 1] += COULOMBS * f.y;
+        a.force[3 * i + 2] += COULOMBS * f.z;
+        a.trq[3 * i] = COULOMBS * t.x;
+        a.trq[3 * i + 1] = COULOMBS * t.y;
+        a.trq[3 * i + 2] = COULOMBS * t.z;
+    }
+    acc[1] *= COULOMBS;
+    block_add<11>(acc, a.terms + M_REAL_E);
+}
+
+// Per site: self energy and torque (:688-760) and the surface sums
+// (:2222-2250).
+__global__ __launch_bounds__(256) void k_ind_site_sums(MpArgs a) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    double v[15];
+#pragma unroll
+    for (int k = 0; k < 15; ++k) v[k] = 0.0;
+    if (i < a.n) {
+        const Site I = load_site(a, i);
+        const V3 p = ldv(a.pind + 3 * i);
+        v[0] = -(dot(I.d, p) * a.alpha2PI * a.alpha2 / 3.0) * COULOMBS;
+        const V3 tq = (COULOMBS * (2.0 / 3.0) * a.alpha2PI * a.alpha2) * cross(I.d, p);
+        double* tr = a.trq + 12 * (size_t)a.n;   // self part
+        tr[3 * i] = a.torques ? tq.x : 0.0;
+        tr[3 * i + 1] = a.torques ? tq.y : 0.0;
+        tr[3 * i + 2] = a.torques ? tq.z : 0.0;
+        const V3 r = ldv(a.pos + 3 * i), rim = ldv(a.com + 3 * a.mol[i]);
+        v[1] = I.q * r.x; v[2] = I.q * r.y; v[3] = I.q * r.z;
+        v[4] = I.q * rim.x; v[5] = I.q * rim.y; v[6] = I.q * rim.z;
+        v[7] = I.d.x; v[8] = I.d.y; v[9] = I.d.z;
+        v[10] = dot(I.d, p);
+        v[11] = p.x; v[12] = p.y; v[13] = p.z;
+        v[14] = dot(p, p);
+    }
+    double self[1] = {v[0]};
+    block_add<1>(self, a.terms + M_SELF);
+    __syncthreads();
+    double su[14];
+#pragma unroll
+    for (int k = 0; k < 14; ++k) su[k] = v[k + 1];
+    block_add<14>(su, a.terms + M_SU_QR);
+}
+
+// One thread per molecule: excluded pairs inside it (intraMolCorrect
+// :1850-1985).  Forces and torques only without constraints.
+__global__ __launch_bounds__(256) void k_ind_intra(MpArgs a) {
+    const int m = blockIdx.x * blockDim.x + threadIdx.x;
+    double v[2] = {0.0, 0.0};
+    if (m < a.nmol) {
+        const int a0 = a.mol_start[m], ns = a.mol_start[m + 1] - a0;
+        V3 tq[3] = {v3(0, 0, 0), v3(0, 0, 0), v3(0, 0, 0)};
+        for (int x = 0; x < ns; ++x) {
+            const int idi = a0 + x;
+            const Site I = load_site(a, idi);
+            const V3 pi = ldv(a.pind + 3 * idi);
+            for (int y = x + 1; y < ns; ++y) {
+                const int idj = a0 + y;
+                bool ex = false;
+                for (int k = a.excl_off[idi]; k < a.excl_off[idi + 1]; ++k) ex = ex || a.excl[k] == idj;
+                if (!ex) continue;
+                double dx = a.pos[3 * idi] - a.pos[3 * idj], dy = a.pos[3 * idi + 1] - a.pos[3 * idj + 1],
+                       dz = a.pos[3 * idi + 2] - a.pos[3 * idj + 2];
+                pbc(dx, dy, dz, a.box);
+                const double r2 = dx * dx + dy * dy + dz * dz, r = sqrt(r2);
+                const V3 rv = v3(dx, dy, dz);
+                const Site J = load_site(a, idj);
+                const V3 pj = ldv(a.pind + 3 * idj);
+                const double e = exp(-a.alpha2 * r2), inv = 1.0 / r2;
+                double D[5];
+                D[0] = erf(a.alpha * r) / r;
+                D[1] = (-D[0] + a.alpha2PI * e) * inv;
+                D[2] = (-3.0 * D[1] - 2.0 * a.alpha2PI * a.alpha2 * e) * inv;
+                D[3] = (-5.0 * D[2] + 4.0 * a.alpha2PI * a.alpha2 * a.alpha2 * e) * inv;
+                D[4] = (-7.0 * D[3] - 8.0 * a.alpha2PI * a.alpha2 * a.alpha2 * a.alpha2 * e) * inv;
+                IndSide si, sj;
+                if (!a.use_constraint && a.torques) {
+                    ind_side<true, true>(I, pj, rv, si);
+                    ind_side<true, true>(J, pi, -rv, sj);
+                } else {
+                    ind_side<true, false>(I, pj, rv, si);
+                    ind_side<true, false>(J, pi, -rv, sj);
+                }
+                v[0] += (si.G1 + sj.G1) * D[1] + (si.G2 + sj.G2) * D[2] + (si.G3 + sj.G3) * D[3];
+                if (!a.use_constraint) {
+                    const V3 fi = D[1] * si.g1 + D[2] * si.g2 + D[3] * si.g3 + (si.G1 * D[2] + si.G2 * D[3] + si.G3 * D[4]) * rv;
+                    const V3 fj =
+                        D[1] * sj.g1 + D[2] * sj.g2 + D[3] * sj.g3 + (sj.G1 * D[2] + sj.G2 * D[3] + sj.G3 * D[4]) * (-rv);
+                    const V3 fij = fi - fj;
+                    v[1] += -dot(fij, rv) * COULOMBS / (3.0 * a.volume);
+                    a.force[3 * idi] += COULOMBS * fij.x;
+                    a.force[3 * idi + 1] += COULOMBS * fij.y;
+                    a.force[3 * idi + 2] += COULOMBS * fij.z;
+                    a.force[3 * idj] -= COULOMBS * fij.x;
+                    a.force[3 * idj + 1] -= COULOMBS * fij.y;
+                    a.force[3 * idj + 2] -= COULOMBS * fij.z;
+                    if (a.torques) {
+                        tq[x] = tq[x] - COULOMBS * (D[1] * si.Gt1 + D[2] * si.Gt2 + D[3] * si.Gt3);
+                        tq[y] = tq[y] - COULOMBS * (D[1] * sj.Gt1 + D[2] * sj.Gt2 + D[3] * sj.Gt3);
+                    }
+                }
+            }
+        }
+        double* tr = a.trq + 3 * (size_t)a.n;   // intramolecular part
+        for (int x = 0; x < ns; ++x) {
+            tr[3 * (a0 + x)] = tq[x].x;
+            tr[3 * (a0 + x) + 1] = tq[x].y;
+            tr[3 * (a0 + x) + 2] = tq[x].z;
+        }
+    }
+    v[0] *= -0.5 * COULOMBS;
+    block_add<2>(v, a.terms + M_INTRA_E);
+}
+
+// Surface force, virial and torque per site (:2290-2345).
+__global__ __launch_bounds__(256) void k_ind_surface(MpArgs a, double coeff) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    double v[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    if (i < a.n) {
+        const V3 qr = ldv(a.terms + M_SU_QR), ps = ldv(a.terms + M_SU_PI);
+        const Site I = load_site(a, i);
+        const V3 p = ldv(a.pind + 3 * i);
+        const V3 ri = ldv(a.pos + 3 * i);
+        const V3 fi = (-2.0 * coeff * I.q) * ps;
+        const V3 temp = (-a.q[i]) * qr;
+        a.force[3 * i] += COULOMBS * fi.x;
+        a.force[3 * i + 1] += COULOMBS * fi.y;
+        a.force[3 * i + 2] += COULOMBS * fi.z;
+        v[0] = temp.x * ri.x; v[1] = temp.y * ri.x; v[2] = temp.z * ri.x;
+        v[3] = temp.x * ri.y; v[4] = temp.y * ri.y; v[5] = temp.z * ri.y;
+        v[6] = temp.x * ri.z; v[7] = temp.y * ri.z; v[8] = temp.z * ri.z;
+        const V3 tq = (COULOMBS * 2.0 * coeff) * (cross(ps, I.d) - cross(p, I.d));
+        double* tr = a.trq + 6 * (size_t)a.n;   // surface part
+        tr[3 * i] = a.torques ? tq.x : 0.0;
+        tr[3 * i + 1] = a.torques ? tq.y : 0.0;
+        tr[3 * i + 2] = a.torques ? tq.z : 0.0;
+    }
+    block_add<9>(v, a.terms + M_SU_VIR);
+}
+
+// k-space pass 1: thread per k over a chunk of sites; 28 sums per (chunk, k):
+//   0 Sc  1 Ss   (q - kQk) cos/sin        2 Mc  3 Ms   (mu.k) cos/sin
+//   4 Pc  5 Ps   (p.k) cos/sin            6 Q2c 7 Q2s  2 kQk cos/sin
+//   8-10 (q - kQk) cos d   11-13 (q - kQk) sin d   14-16 (mu.k) cos d
+//  17-19 (mu.k) sin d     20-22 (p.k) cos d       23-25 (p.k) sin d
+//  26 (q - kQk) cos |d|^2  27 (q - kQk) sin |d|^2
+__global__ __launch_bounds__(MP_SF_THREADS) void k_ind_sfac(MpArgs a) {
+    __shared__ double sh[MP_SF_THREADS][19];   // r q mu Q(6) p d
+    const int k = blockIdx.x * MP_SF_THREADS + threadIdx.x;
+    const int chunk = blockIdx.y;
+    const int i0 = chunk * a.per, i1 = min(a.n, i0 + a.per);
+    V3 kv = v3(0, 0, 0);
+    if (k < a.K) kv = ldv(a.kvec + 3 * k);
+    double s[IN_NSUM];
+#pragma unroll
+    for (int x = 0; x < IN_NSUM; ++x) s[x] = 0.0;
+    for (int t0 = i0; t0 < i1; t0 += MP_SF_THREADS) {
+        const int i = t0 + threadIdx.x;
+        __syncthreads();
+        if (i < i1) {
+            double* p = sh[threadIdx.x];
+            p[0] = a.pos[3 * i]; p[1] = a.pos[3 * i + 1]; p[2] = a.pos[3 * i + 2];
+            p[3] = a.q[i] / SQRT_COULOMB_MP;
+            p[4] = a.dip[3 * i]; p[5] = a.dip[3 * i + 1]; p[6] = a.dip[3 * i + 2];
+            for (int c = 0; c < 6; ++c) p[7 + c] = a.quad[6 * i + c];
+            p[13] = a.pind[3 * i]; p[14] = a.pind[3 * i + 1]; p[15] = a.pind[3 * i + 2];
+            const int m = a.mol[i];
+            p[16] = p[0] - a.com[3 * m];
+            p[17] = p[1] - a.com[3 * m + 1];
+            p[18] = p[2] - a.com[3 * m + 2];
+        }
+        __syncthreads();
+        const int cnt = min(MP_SF_THREADS, i1 - t0);
+        if (k < a.K) {
+            for (int x = 0; x < cnt; ++x) {
+                const double* p = sh[x];
+                double sn, cs;
+                sincos(p[0] * kv.x + p[1] * kv.y + p[2] * kv.z, &sn, &cs);
+                const double QK = p[7] * kv.x * kv.x + p[10] * kv.y * kv.y + p[12] * kv.z * kv.z +
+                                  2.0 * (p[8] * kv.x * kv.y + p[9] * kv.x * kv.z + p[11] * kv.y * kv.z);
+                const double A = p[3] - QK;
+                const double Mu = p[4] * kv.x + p[5] * kv.y + p[6] * kv.z;
+                const double Pk = p[13] * kv.x + p[14] * kv.y + p[15] * kv.z;
+                const double ac = A * cs, as = A * sn, mc = Mu * cs, ms = Mu * sn, pc = Pk * cs, ps = Pk * sn;
+                const double dx = p[16], dy = p[17], dz = p[18];
+                s[0] += ac; s[1] += as; s[2] += mc; s[3] += ms; s[4] += pc; s[5] += ps;
+                s[6] += 2.0 * QK * cs; s[7] += 2.0 * QK * sn;
+                s[8] += ac * dx; s[9] += ac * dy; s[10] += ac * dz;
+                s[11] += as * dx; s[12] += as * dy; s[13] += as * dz;
+                s[14] += mc * dx; s[15] += mc * dy; s[16] += mc * dz;
+                s[17] += ms * dx; s[18] += ms * dy; s[19] += ms * dz;
+                s[20] += pc * dx; s[21] += pc * dy; s[22] += pc * dz;
+                s[23] += ps * dx; s[24] += ps * dy; s[25] += ps * dz;
+                const double d2 = dx * dx + dy * dy + dz * dz;
+                s[26] += ac * d2; s[27] += as * d2;
+            }
+        }
+    }
+    if (k < a.K) {
+        double* out = a.part + ((size_t)chunk * a.K + k) * IN_NSUM;
+#pragma unroll
+        for (int x = 0; x < IN_NSUM; ++x) out[x] = s[x];
+    }
+}
+
+// k-space pass 2 (:1564-1600, the force-pass sums :1640-1650 from moments).
+// Force-pass coefficients per k: X = b (Sc - Ms'), Y = b (Ss + Mc'), b Pc,
+// b Ps with Mc' = Mc (+ Pc), Ms' = Ms (+ Ps) when dipoles are minimised.
+__global__ __launch_bounds__(256) void k_ind_kfinish(MpArgs a) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    double v[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    if (k < a.K) {
+        double S[IN_NSUM];
+#pragma unroll
+        for (int x = 0; x < IN_NSUM; ++x) S[x] = 0.0;
+        for (int c = 0; c < a.chunks; ++c) {
+            const double* p = a.part + ((size_t)c * a.K + k) * IN_NSUM;
+#pragma unroll
+            for (int x = 0; x < IN_NSUM; ++x) S[x] += p[x];
+        }
+        const V3 kv = ldv(a.kvec + 3 * k);
+        const double pi4V = 4.0 * PI / a.volume, pi8V = 2.0 * pi4V;
+        const double ek = a.expk[k];
+        const double Sc = S[0], Ss = S[1], Mc = S[2], Ms = S[3], Pc = S[4], Ps = S[5], Q2c = S[6], Q2s = S[7];
+        const double UmqQ = pi4V * ek * (Ss * Pc - Sc * Ps);
+        const double Ump = pi4V * ek * (Pc * Mc + Ps * Ms);
+        const double t = (UmqQ + Ump) * COULOMBS;
+        const double dUdVpQ = pi4V * ek * (Q2s * Pc - Q2c * Ps);
+        const double Upp = a.do_min ? pi4V * ek * (Pc * Pc + Ps * Ps) : 0.0;
+        const double Sk = 2.0 * UmqQ + 2.0 * Ump + Upp;
+        const double k2 = dot(kv, kv);
+        const double x = k2 * 2.0 * a.alphaR4;
+        double d1 = (-Sk + Sk * x + (-2.0 * UmqQ + 2.0 * dUdVpQ - 4.0 * Ump - 2.0 * Upp)) * COULOMBS;
+        double d2 = 4.0 * t - 7.0 * t * x + t * x * x;
+        const double coeff = 2.0 * (a.alphaR4 + 1.0 / a.kmod2[k]);
+        const double b = pi8V * ek;
+        const double Mcp = a.do_min ? Mc + Pc : Mc, Msp = a.do_min ? Ms + Ps : Ms;
+        const double X = b * (Sc - Msp), Y = b * (Ss + Mcp), bPc = b * Pc, bPs = b * Ps;
+        if (a.use_constraint) {
+            const V3 cosrR = v3(S[8], S[9], S[10]), sinrR = v3(S[11], S[12], S[13]);
+            const V3 mcR = v3(S[14], S[15], S[16]), msR = v3(S[17], S[18], S[19]);
+            const V3 pcR = v3(S[20], S[21], S[22]), psR = v3(S[23], S[24], S[25]);
+            const double Ff = COULOMBS * dot(kv, X * pcR + Y * psR - bPc * cosrR - bPs * mcR - bPs * sinrR + bPc * msR);
+            const double Ffk2 = 2.0 * k2 * 2.0 * a.alphaR4 * Ff;
+            const double d2UV1 = b * k2 * (Sc * S[26] + Ss * S[27]);
+            const double d2UrR = b * k2 * (dot(cosrR, cosrR) + dot(sinrR, sinrR));
+            d1 += Ff;
+            d2 += -6.0 * Ff + Ffk2 - d2UV1 + d2UrR;
+        }
+        v[0] = t; v[1] = d1; v[2] = d2;
+        v[3] = t - t * coeff * kv.x * kv.x;
+        v[4] = t - t * coeff * kv.y * kv.y;
+        v[5] = t - t * coeff * kv.z * kv.z;
+        v[6] = -t * coeff * kv.x * kv.y;
+        v[7] = -t * coeff * kv.x * kv.z;
+        v[8] = -t * coeff * kv.y * kv.z;
+        a.kf[4 * k] = X;
+        a.kf[4 * k + 1] = Y;
+        a.kf[4 * k + 2] = bPc;
+        a.kf[4 * k + 3] = bPs;
+    }
+    block_add<9>(v, a.terms + M_LONG_E);
+}
+
+// k-space pass 3: thread per site over all k (:1603-1690).
+__global__ __launch_bounds__(MP_RF_THREADS) void k_ind_kforce(MpArgs a) {
+    __shared__ double sk[MP_KTILE][7];
+    const int i = blockIdx.x * MP_RF_THREADS + threadIdx.x;
+    Site I{};
+    V3 ri = v3(0, 0, 0), p = v3(0, 0, 0);
+    if (i < a.n) {
+        I = load_site(a, i);
+        ri = ldv(a.pos + 3 * i);
+        p = ldv(a.pind + 3 * i);
+    }
+    V3 F = v3(0, 0, 0), T = v3(0, 0, 0);
+    for (int k0 = 0; k0 < a.K; k0 += MP_KTILE) {
+        const int cnt = min(MP_KTILE, a.K - k0);
+        __syncthreads();
+        for (int x = threadIdx.x; x < cnt; x += MP_RF_THREADS) {
+            sk[x][0] = a.kvec[3 * (k0 + x)];
+            sk[x][1] = a.kvec[3 * (k0 + x) + 1];
+            sk[x][2] = a.kvec[3 * (k0 + x) + 2];
+            sk[x][3] = a.kf[4 * (k0 + x)];
+            sk[x][4] = a.kf[4 * (k0 + x) + 1];
+            sk[x][5] = a.kf[4 * (k0 + x) + 2];
+            sk[x][6] = a.kf[4 * (k0 + x) + 3];
+        }
+        __syncthreads();
+        if (i < a.n) {
+            for (int x = 0; x < cnt; ++x) {
+                const V3 kv = v3(sk[x][0], sk[x][1], sk[x][2]);
+                const double X = sk[x][3], Y = sk[x][4], bPc = sk[x][5], bPs = sk[x][6];
+                double sn, cs;
+                sincos(dot(ri, kv), &sn, &cs);
+                const V3 Qk = smul(I.Q, kv);
+                const double A = I.q - dot(kv, Qk), Mu = dot(I.d, kv), Pk = dot(p, kv);
+                const double val = cs * (Pk * X - A * bPc - Mu * bPs) + sn * (Pk * Y - A * bPs + Mu * bPc);
+                F = F + val * kv;
+                if (a.torques)
+                    T = T + (cs * bPc + sn * bPs) * cross(kv, I.d) + (2.0 * (cs * bPs - sn * bPc)) * cross(kv, Qk);
+            }
+        }
+    }
+    if (i < a.n) {
+        a.force[3 * i] += COULOMBS * F.x;
+        a.force[3 * i + 1] += COULOMBS * F.y;
+        a.force[3 * i + 2] += COULOMBS * F.z;
+        double* tr = a.trq + 9 * (size_t)a.n;   // k-space part
+        tr[3 * i] = COULOMBS * T.x;
+        tr[3 * i + 1] = COULOMBS * T.y;
+        tr[3 * i + 2] = COULOMBS * T.z;
+    }
+}
+
+static MpArgs ind_args(Ctx* c) {
+    const smd_indf_params& p = c->indf;
+    MpArgs a = mp_args(c, c->ind_ks, p.alpha, p.real_cut, p.use_constraint, p.torques);
+    a.a_damp = p.a_damp;
+    a.damp_amoeba = p.damp_amoeba;
+    a.do_min = p.do_min;
+    return a;
+}
+
+static int ind_check(Ctx* c) {
+    if (!c->indf_ready) return fail(c, SMD_ESTATE, "smd_induction_setup not called");
+    if (!c->multipoles_set) return fail(c, SMD_ESTATE, "multipoles not set");
+    return SMD_OK;
+}
+
+int induction_short(Ctx* c, smd_indf_out* out) {
+    int rc = ind_check(c);
+    if (rc) return rc;
+    if (!c->list_valid) return fail(c, SMD_ESTATE, "no pair list");
+    if (c->indf.real_cut > c->list_cutoff + c->list_buff)
+        return fail(c, SMD_ENOTSUP, "real-space cutoff %g exceeds the pair-list radius %g", c->indf.real_cut,
+                    c->list_cutoff + c->list_buff);
+    MpArgs a = ind_args(c);
+    const dim3 b256(256);
+    const dim3 ga((c->n + 255) / 256), gm((c->nmol + 255) / 256);
+    const dim3 gr((unsigned)(((int64_t)c->n * MP_LANES + 255) / 256));
+    ProfScope ps(c, "ind_short");
+    SMD_HIP(c, hipMemsetAsync(c->d_mp_terms, 0, sizeof(double) * M_COUNT, c->stream));
+    if ((rc = frames_run(c))) return rc;
+    hipLaunchKernelGGL(k_ind_site_sums, ga, b256, 0, c->stream, a);
+    hipLaunchKernelGGL((k_ind_real<MP_LANES>), gr, b256, 0, c->stream, a);
+    hipLaunchKernelGGL(k_ind_intra, gm, b256, 0, c->stream, a);
+    const double volume = a.volume, volumer = 1.0 / volume;
+    const double coeff = 2.0 * PI * volumer / 3.0;
+    if (c->indf.surface) hipLaunchKernelGGL(k_ind_surface, ga, b256, 0, c->stream, a, coeff);
+    // torques of the self term, real space, intramolecular, surface
+    if (c->indf.torques)
+        hipLaunchKernelGGL(k_mp_torques, gm, b256, 0, c->stream, a, 16u | 3u | (c->indf.surface ? 4u : 0u));
+    SMD_HIP(c, hipGetLastError());
+    double t[M_COUNT];
+    if ((rc = mp_read_terms(c, t))) return rc;
+    *out = smd_indf_out{};
+    out->self_energy = t[M_SELF];
+    out->self_deriv = t[M_DUDV + 4];
+    out->real_energy = t[M_REAL_E];
+    out->real_deriv = t[M_REAL_D1] + t[M_DUDV];
+    out->intra_energy = t[M_INTRA_E];
+    out->mol_deriv = t[M_MOL_D1] + t[M_DUDV + 1] / (3.0 * volume);
+    for (int k = 0; k < 9; ++k) out->virial[k] = t[M_REAL_VIR + k];
+    out->tot_torque = t[M_TRQ] + t[M_TRQ + 1] + t[M_TRQ + 2] + t[M_TRQ + 4];
+    if (c->indf.surface) {
+        const double* qr = t + M_SU_QR;
+        const double* qR = t + M_SU_QRR;
+        const double* mi = t + M_SU_QM;
+        const double* ps = t + M_SU_PI;
+        auto d3 = [](const double* x, const double* y) { return x[0] * y[0] + x[1] * y[1] + x[2] * y[2]; };
+        const double selfSum = t[M_SU_SELF], selfpi = t[M_SU_PP];
+        out->surf_energy = coeff * (d3(qr, ps) + d3(mi, ps) - selfSum) * COULOMBS;
+        const double dUdV1 = c->indf.do_min
+                                 ? -coeff * volumer * (2.0 * d3(qr, ps) + 2.0 * d3(mi, ps) + d3(ps, ps) - 2.0 * selfSum - selfpi)
+                                 : -coeff * volumer * (2.0 * d3(qr, ps) + 2.0 * d3(mi, ps) - 2.0 * selfSum);
+        const double dUdV2 = 2.0 * coeff * volumer * d3(qr, ps) / 3.0;
+        out->surf_deriv = COULOMBS * (dUdV1 + dUdV2) + t[M_DUDV + 2] * volumer / 3.0;
+        if (c->indf.use_constraint)
+            out->surf_deriv2 = 2.0 * coeff * volumer * volumer * d3(qr, qr) - 16.0 * coeff * volumer * volumer * d3(qr, qR) / 9.0 +
+                               2.0 * coeff * volumer * volumer * d3(qR, qR) / 9.0;
+        else
+            out->surf_deriv2 = 4.0 * coeff * volumer * volumer * d3(qr, qr) / 9.0;
+        for (int k = 0; k < 9; ++k) out->virial[k] += t[M_SU_VIR + k];
+    }
+    return SMD_OK;
+}
+
+int induction_long(Ctx* c, smd_mp_long_out* out) {
+    int rc = ind_check(c);
+    if (rc) return rc;
+    MpArgs a = ind_args(c);
+    const dim3 b256(256);
+    ProfScope ps(c, "ind_long");
+    SMD_HIP(c, hipMemsetAsync(c->d_mp_terms, 0, sizeof(double) * M_COUNT, c->stream));
+    if ((rc = frames_run(c))) return rc;
+    const dim3 g1((a.K + MP_SF_THREADS - 1) / MP_SF_THREADS, a.chunks);
+    hipLaunchKernelGGL(k_ind_sfac, g1, dim3(MP_SF_THREADS), 0, c->stream, a);
+    hipLaunchKernelGGL(k_ind_kfinish, dim3((a.K + 255) / 256), b256, 0, c->stream, a);
+    hipLaunchKernelGGL(k_ind_kforce, dim3((c->n + MP_RF_THREADS - 1) / MP_RF_THREADS), dim3(MP_RF_THREADS), 0,
+                       c->stream, a);
+    if (c->indf.torques) hipLaunchKernelGGL(k_mp_torques, dim3((c->nmol + 255) / 256), b256, 0, c->stream, a, 8u);
+    SMD_HIP(c, hipGetLastError());
+    double t[M_COUNT];
+    if ((rc = mp_read_terms(c, t))) return rc;
+    *out = smd_mp_long_out{};
+    out->energy = t[M_LONG_E];
+    out->deriv = t[M_LONG_D1] + t[M_DUDV + 3];
+    out->deriv2 = t[M_LONG_D2];
+    const double* lv = t + M_LONG_VIR;
+    out->virial[0] = lv[0]; out->virial[4] = lv[1]; out->virial[8] = lv[2];
+    out->virial[1] = out->virial[3] = lv[3];
+    out->virial[2] = out->virial[6] = lv[4];
+    out->virial[5] = out->virial[7] = lv[5];
+    out->tot_torque = t[M_TRQ + 3];
+    return SMD_OK;
+}
+
+static int ind_setup(Ctx* c, const smd_indf_params* p, double kcut, int32_t* n_kvec) {
+    if (!(p->alpha > 0) || !(p->real_cut > 0) || !(kcut > 0) || !(p->a_damp >= 0))
+        return fail(c, SMD_EINVAL, "bad induction parameters");
+    int rc = kset_build(c, c->ind_ks, p->alpha, kcut, IN_NSUM);
+    if (rc) return rc;
+    c->indf = *p;
+    c->indf_ready = true;
+    if (n_kvec) *n_kvec = c->ind_ks.K;
     return SMD_OK;
 }
 
@@ -849,6 +1469,29 @@ extern "C" int smd_multipole_compute_long(smd_ctx* c, int prec, double* f_accum,
     if (prec != SMD_PREC_FP64) return fail(c, SMD_ENOTSUP, "multipole Ewald is FP64 only");
     if (f_accum) SMD_HIP(c, hipMemsetAsync(c->d_force, 0, sizeof(double) * 3 * c->n, c->stream));
     int rc = multipole_long(c, out);
+    if (rc) return rc;
+    return f_accum ? mp_forces_out(c, f_accum) : SMD_OK;
+}
+
+extern "C" int smd_induction_setup(smd_ctx* c, const smd_indf_params* p, double kcut, int32_t* n_kvec) {
+    if (!c || !p) return SMD_EINVAL;
+    return ind_setup(c, p, kcut, n_kvec);
+}
+
+extern "C" int smd_induction_compute(smd_ctx* c, int prec, double* f_accum, smd_indf_out* out) {
+    if (!c || !out) return SMD_EINVAL;
+    if (prec != SMD_PREC_FP64) return fail(c, SMD_ENOTSUP, "induction terms are FP64 only");
+    if (f_accum) SMD_HIP(c, hipMemsetAsync(c->d_force, 0, sizeof(double) * 3 * c->n, c->stream));
+    int rc = induction_short(c, out);
+    if (rc) return rc;
+    return f_accum ? mp_forces_out(c, f_accum) : SMD_OK;
+}
+
+extern "C" int smd_induction_compute_long(smd_ctx* c, int prec, double* f_accum, smd_mp_long_out* out) {
+    if (!c || !out) return SMD_EINVAL;
+    if (prec != SMD_PREC_FP64) return fail(c, SMD_ENOTSUP, "induction terms are FP64 only");
+    if (f_accum) SMD_HIP(c, hipMemsetAsync(c->d_force, 0, sizeof(double) * 3 * c->n, c->stream));
+    int rc = induction_long(c, out);
     if (rc) return rc;
     return f_accum ? mp_forces_out(c, f_accum) : SMD_OK;
 }

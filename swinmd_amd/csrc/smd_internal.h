@@ -156,17 +156,21 @@ struct Ctx {
     int* d_cg_state = nullptr;    // [4] done, iterations, stop reason
     double* d_cg_scal = nullptr;  // [4] last residual norm
 
-    // permanent multipoles under Ewald
-    bool mp_ready = false;
+    // permanent multipoles and induced dipoles under Ewald
+    struct KSet {                   // half-space k set of one Ewald class
+        int K = 0, chunks = 1;
+        double* kvec = nullptr;     // [3K]
+        double* kmod2 = nullptr;    // [K]
+        double* expk = nullptr;     // [K]
+        double* part = nullptr;     // [chunks][K][sums]
+        double* kf = nullptr;       // [K][4] per-k force-pass coefficients
+    };
+    bool mp_ready = false, indf_ready = false;
     smd_mp_params mp{};
-    int mp_K = 0, mp_chunks = 1;
-    double* d_mp_trq = nullptr;     // [4][3n] torques of real space, intramolecular, surface, k-space
+    smd_indf_params indf{};
+    KSet mp_ks, ind_ks;
+    double* d_mp_trq = nullptr;     // [5][3n] torques: real space, intramolecular, surface, k-space, self
     double* d_mp_terms = nullptr;   // accumulator slots
-    double* d_mp_kvec = nullptr;    // [3K]
-    double* d_mp_kmod2 = nullptr;   // [K]
-    double* d_mp_expk = nullptr;    // [K]
-    double* d_mp_part = nullptr;    // [chunks][K][20]
-    double* d_mp_kf = nullptr;      // [K][2]
     int cg_parts = 0;
 
     // MD driver

@@ -404,6 +404,32 @@ class Induction:
         self.ctx.check(self.ctx.lib.smd_induce(self.ctx.h, C.byref(self.params), C.byref(r)))
         return r.iterations, bool(r.converged), r.residual
 
+    def setup_terms(self, *, alpha, real_cut, kcut, use_constraint=True, surface=False, torques=True,
+                    damp_amoeba=False):
+        """Induction ctor and set_up for the energy, force and torque terms of
+        the induced dipoles (src/forces/Induction.cpp:19-158, :592-652)."""
+        p = L.IndfParams(alpha, real_cut, self.params.a_damp, int(use_constraint), int(surface), int(torques),
+                         int(damp_amoeba), int(self.params.do_min))
+        nk = C.c_int32()
+        self.ctx.check(self.ctx.lib.smd_induction_setup(self.ctx.h, C.byref(p), kcut, C.byref(nk)))
+        self.num_kvectors = nk.value
+
+    def compute(self, f_accum=None):
+        """Induction::compute after the dipole solve: energies, Lustig terms,
+        virial; forces added to f_accum [n,3] when given."""
+        o = L.IndfOut()
+        self.ctx.check(self.ctx.lib.smd_induction_compute(self.ctx.h, L.PREC_FP64, _fptr(f_accum), C.byref(o)))
+        d = {k: getattr(o, k) for k, _ in L.IndfOut._fields_ if k != "virial"}
+        d["virial"] = np.array(o.virial[:])
+        return d
+
+    def compute_long(self, f_accum=None):
+        o = L.MpLongOut()
+        self.ctx.check(self.ctx.lib.smd_induction_compute_long(self.ctx.h, L.PREC_FP64, _fptr(f_accum), C.byref(o)))
+        d = {k: getattr(o, k) for k, _ in L.MpLongOut._fields_ if k != "virial"}
+        d["virial"] = np.array(o.virial[:])
+        return d
+
     def results(self):
         """Lab-frame dipoles [n,3], quadrupoles [n,6], permanent field [n,3], induced dipoles [n,3]."""
         n = self.ctx.n

@@ -81,3 +81,45 @@ def test_induction_512_waters_against_oracle(oracle_mod):
     assert rel_rms(e0, oe0) <= TOL
     assert rel_rms(mu, omu) <= TOL
     assert abs(iters - oit) <= 2
+
+
+INDF_SHORT = (("real_energy", "indf_real"), ("self_energy", "indf_self"), ("intra_energy", "indf_molCorrect"),
+              ("surf_energy", "indf_surf"), ("real_deriv", "indf_realDeriv"), ("mol_deriv", "indf_molDeriv"),
+              ("surf_deriv", "indf_surfDeriv"), ("surf_deriv2", "indf_surfSecDeriv"), ("self_deriv", "indf_selfDeriv"))
+INDF_LONG = (("energy", "indf_long"), ("deriv", "indf_longDeriv"), ("deriv2", "indf_longSecDeriv"))
+
+
+@pytest.mark.parametrize("name", ["amoeba216_indforces_cg", "amoeba216_indforces_tinker_flexible",
+                                  "amoeba216_indforces_tinker_direct"])
+def test_induction_terms_match_reference(name):
+    """Induced dipoles solved on the device, then Induction's energy, force and
+    torque terms (real space over every pair within cutoffEw, excluded pairs
+    included; self, intramolecular, surface; k-space) against the reference.
+    Bars as above: 1e-10 relative RMS on forces and virials, 1e-10 relative on
+    the energies and volume derivatives, measured on the largest term's scale
+    for terms that nearly cancel."""
+    from swinmd_amd.forces import Induction
+    fx = Fixture(name)
+    c = fx.cfg
+    print(f"case {name}", flush=True)
+    ctx = _ctx(fx.system)
+    ctx.set_pairlist(c["cutoff"], c["cut_buff"], True)
+    ctx.compute_mol_centers()
+    ind = Induction(ctx, cutoff_ind=c["cutoff_ind"], a_damp=c["a_damp"], damp_tij=c["ind_damp"], do_min=c["ind_min"])
+    ind.induce()
+    assert rel_rms(ind.results()[3], fx.vec("indf_pi")) <= TOL
+    ind.setup_terms(alpha=c["alpha"], real_cut=c["cutoff_ew"], kcut=c["kcut"], use_constraint=c["constraint"],
+                    surface=c["surface"], torques=c["torques"], damp_amoeba=c["damp_amoeba"])
+    n = fx.system.n_atoms
+    fs, fl = np.zeros((n, 3)), np.zeros((n, 3))
+    out = ind.compute(fs)
+    lo = ind.compute_long(fl)
+    assert rel_rms(fs, fx.vec("indf_short_force")) <= TOL
+    assert rel_rms(fl, fx.vec("indf_long_force")) <= TOL
+    assert rel_rms(out["virial"], fx.ref["indf_short_virial"]) <= TOL
+    assert rel_rms(lo["virial"], fx.ref["indf_long_virial"]) <= TOL
+    scale = max(abs(fx.meta[k]) for _, k in INDF_SHORT) * 1e-6
+    for k, mk in INDF_SHORT:
+        assert abs(out[k] - fx.meta[mk]) <= TOL * max(abs(fx.meta[mk]), scale), (k, out[k], fx.meta[mk])
+    for k, mk in INDF_LONG:
+        assert abs(lo[k] - fx.meta[mk]) <= TOL * abs(fx.meta[mk]), (k, lo[k], fx.meta[mk])
