@@ -54,6 +54,9 @@ FLOP_PER_PAIR = 99
 #     sums 4 (the pass that forms S(k)); force 10 (the pass that uses it).
 FLOP_PER_ATOM_K_SFAC = 29
 FLOP_PER_ATOM_K_FORCE = 10
+#   induced dipoles: per pair within cutoffInd per CG iteration, the
+#     Thole-damped dipole tensor recomputed and applied to both sites' dipoles.
+FLOP_PER_PAIR_TMU = 76
 
 
 def log(msg):
@@ -135,8 +138,138 @@ def cpu_baseline(system, pos, nsteps, ranks):
     return out
 
 
+POLAR = dict(cutoff=0.9, cut_buff=0.1, alpha=3.5, cutoff_ind=0.9, a_damp=0.39)
+POLAR_FAMILIES = ["pairs", "mp_short", "mp_long", "ind_field", "ind_cg", "ind_short", "ind_long"]
+
+
+def polar_cpu_baseline(system, reps=1):
+    """The reference's polarizable force evaluation (buffered 14-7, Multipole
+    and Induction compute + compute_long with its dense-A CG), compiled from
+    its sources, on one core: its multi-rank CG (Induction::ParallelCG) reads
+    an uninitialised accumulator (src/forces/Induction.cpp:2473), so only the
+    serial ConjugateGradient path is timed.  Returns (baseline, forces)."""
+    from oracle import refinput
+    from swinmd_amd import systems as S
+    exe = os.path.join(ROOT, "oracle", "_ref", "ref_harness")
+    if not os.path.exists(exe):
+        return None, None
+    with tempfile.TemporaryDirectory() as d:
+        refinput.write_inputs(d, system, cutoff=POLAR["cutoff"], cut_buff=POLAR["cut_buff"],
+                              cutoff_ew=POLAR["cutoff"], alpha=POLAR["alpha"], kcut=S.kcut_for(N2MAX, system.box[0]),
+                              ewald=False, buff=True, multipole=True, induction=True, torques=True, ind_damp=True,
+                              ind_min=True, cutoff_ind=POLAR["cutoff_ind"], a_damp=POLAR["a_damp"], constraint=True)
+        r = subprocess.run([exe, d, "time_polar", str(reps)], capture_output=True, text=True, timeout=1200)
+        if r.returncode != 0:
+            log(f"polarizable cpu baseline failed: {r.stderr[-400:]}")
+            return None, None
+        secs = [json.loads(x)["seconds"] for x in r.stdout.splitlines() if x.startswith("{")][0]
+        forces = np.fromfile(os.path.join(d, "polar_force.bin"), dtype=np.float64).reshape(-1, 3)
+    base = {"value": round(reps / secs, 5), "unit": "force evaluations/s", "cores": 1, "kind": "reference",
+            "sample": f"{reps} polarizable force evaluation(s) of the same {system.n_atoms}-site water, reference "
+                      f"sources compiled with g++ -O2, 1 MPI rank: {secs:.2f} s"}
+    return base, forces
+
+
+def bench_polar(args):
+    """Polarizable water (charges, dipoles, quadrupoles, Thole-damped induced
+    dipoles, buffered 14-7), FP64: one evaluation = the reference's force
+    sequence of LFIntegrator::run for this model."""
+    from swinmd_amd import _lib as L
+    from swinmd_amd import systems as S
+    from swinmd_amd.forces import BuffvdWForce, Context, Induction, Multipole
+    L.load()
+    nmol = 1024 if args.workload == "polar1024" else 8192
+    system = S.polar_water(nmol)
+    n = system.n_atoms
+    kcut = S.kcut_for(N2MAX, system.box[0])
+    ctx = Context(system)
+    ctx.set_pairlist(POLAR["cutoff"], POLAR["cut_buff"], True)
+    ctx.compute_mol_centers()
+    vdw = BuffvdWForce(ctx, cutoff=POLAR["cutoff"], use_constraint=True)
+    mp = Multipole(ctx, alpha=POLAR["alpha"], real_cut=POLAR["cutoff"], kcut=kcut, use_constraint=True,
+                   surface=False, torques=True)
+    ind = Induction(ctx, cutoff_ind=POLAR["cutoff_ind"], a_damp=POLAR["a_damp"], damp_tij=True, do_min=True)
+    ind.setup_terms(alpha=POLAR["alpha"], real_cut=POLAR["cutoff"], kcut=kcut, use_constraint=True, surface=False,
+                    torques=True)
+    iters = []
+
+    def evaluate():
+        ctx.zero_forces()
+        vdw.compute()
+        mp.compute()
+        it, conv, _ = ind.induce()
+        if not conv:
+            raise RuntimeError("CG did not converge")
+        iters.append(it)
+        ind.compute()
+        mp.compute_long()
+        ind.compute_long()
+
+    for _ in range(max(1, args.warmup // 50)):
+        evaluate()
+    reps = max(1, args.steps // 40)
+    ctx.sync()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        evaluate()
+    ctx.sync()
+    secs = time.perf_counter() - t0
+    forces = ctx.forces()
+    ctx.profile(True)
+    for _ in range(3):
+        evaluate()
+    fam = {}
+    for f in POLAR_FAMILIES:
+        ms, cnt = ctx.profile_get(f)
+        if cnt:
+            fam[f] = round(ms * 1e3 / 3, 1)
+    ctx.profile(False)
+    rate = reps / secs
+    # roofline of the dipole solve, the dominant part: per CG iteration the
+    # Thole-damped T.mu over every pair within cutoffInd, 76 flop per pair when
+    # T is recomputed (the intramolecular pairs count too: A couples them)
+    ind_pairs, _ = unique_pairs_in_cutoff(ctx, POLAR["cutoff_ind"])
+    ind_pairs += 3 * nmol
+    it_avg = sum(iters[-3:]) / 3
+    cg_us = fam.get("ind_cg", 0.0) / max(it_avg, 1)
+    out_roof = None
+    if cg_us > 0:
+        ach = FLOP_PER_PAIR_TMU * ind_pairs / (cg_us * 1e-6) / 1e12
+        out_roof = {"kernel": "CG iteration (T.mu with Thole recomputed + vector updates)", "bound": "valu",
+                    "achieved": round(ach, 4), "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
+                    "frac": round(ach / PEAK_FP64_TFLOPS, 5), "traffic": None, "pairs": ind_pairs,
+                    "flop_per_pair": FLOP_PER_PAIR_TMU, "us_per_iteration": round(cg_us, 2),
+                    "note": "time per iteration from event brackets around the whole CG (launch gaps and host "
+                            "checks included), so achieved is a lower bound"}
+    out = {
+        "metric": "polarizable force evaluations/s (FP64, reference CG tolerance)", "value": round(rate, 3),
+        "unit": "evaluations/s", "n_gpus": 1, "steps": reps, "warmup": max(1, args.warmup // 50),
+        "ms_per_step": round(1e3 * secs / reps, 4), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": f"synthetic: seeded lattice water with random orientations",
+        "config": {"workload": f"polarizable water force evaluation, {nmol} molecules", "molecules": nmol,
+                   "atoms": n, "box_nm": float(system.box[0]), "cutoff_nm": POLAR["cutoff"],
+                   "alpha_per_nm": POLAR["alpha"], "n2max": N2MAX, "thole_a": POLAR["a_damp"],
+                   "cg_tolerance": "|r|_2 < 1e-14 (reference)", "parallelism": "single GPU"},
+        "cg_iterations_per_evaluation": round(sum(iters[-reps:]) / reps, 2),
+        "families_us_per_evaluation": fam,
+    }
+    if out_roof:
+        out["roofline"] = out_roof
+    if not args.no_cpu_baseline and nmol == 1024:
+        log("polarizable cpu baseline (reference, 1 core)")
+        base, fref = polar_cpu_baseline(system)
+        if base:
+            out["cpu_baseline"] = base
+            d = forces - fref
+            out["force_rel_rms_vs_reference"] = float(np.sqrt(np.sum(d * d) / np.sum(fref * fref)))
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--workload", default="water4096", choices=["water4096", "polar1024", "polar8192"],
+                    help="water4096: the rigid-water MD step (default); polar1024/polar8192: the polarizable "
+                         "water force evaluation in FP64")
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=2000)
     ap.add_argument("--warmup", type=int, default=200)
@@ -157,6 +290,9 @@ def main():
         print(f"bench.py measures one GPU only: the MD step runs on a single device and --gpus {args.gpus} "
               f"(WORLD_SIZE={world}) is not supported", file=sys.stderr)
         sys.exit(2)
+    if args.workload != "water4096":
+        bench_polar(args)
+        return
 
     from swinmd_amd import _lib as L
     from swinmd_amd import ensemble, systems as S

@@ -80,6 +80,8 @@ struct MpArgs {
     const double* expk;         // [K]
     double* part;               // [chunks][K][sums]
     double* kf;                 // [K][4]: force-pass coefficients
+    double* fpart;              // [ksplit][n][6]: force and torque of each k-range
+    int ksplit, kper;           // k-ranges of the force pass and their length
     double alpha, alpha2, alpha2PI, realcut2, volume, alphaR4;
     int use_constraint, torques;
     // induction terms
@@ -524,18 +526,24 @@ __global__ __launch_bounds__(MP_SF_THREADS) void k_mp_sfac(MpArgs a) {
 // Pass 2: per k, the chunk sums in fixed order, then energy, virial, volume
 // derivatives (:1098-1138 and the force-pass sums :1180-1188 from the
 // moments) and the force-pass coefficients (b P, b Q).
-__global__ __launch_bounds__(256) void k_mp_kfinish(MpArgs a) {
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    double v[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-    if (k < a.K) {
-        double S[MP_NSUM];
+// Chunk sums of one k by one wave: lane l adds chunks l, l + 64, ... in
+// order, then a butterfly over the wave (every lane ends with the same total).
+template <int NS>
+__device__ __forceinline__ void wave_chunk_sum(const MpArgs& a, int k, double (&S)[NS]) {
+    const int lane = threadIdx.x % WAVE;
 #pragma unroll
-        for (int x = 0; x < MP_NSUM; ++x) S[x] = 0.0;
-        for (int c = 0; c < a.chunks; ++c) {
-            const double* p = a.part + ((size_t)c * a.K + k) * MP_NSUM;
+    for (int x = 0; x < NS; ++x) S[x] = 0.0;
+    for (int c = lane; c < a.chunks; c += WAVE) {
+        const double* p = a.part + ((size_t)c * a.K + k) * NS;
 #pragma unroll
-            for (int x = 0; x < MP_NSUM; ++x) S[x] += p[x];
-        }
+        for (int x = 0; x < NS; ++x) S[x] += p[x];
+    }
+#pragma unroll
+    for (int x = 0; x < NS; ++x) S[x] = wave_sum(S[x]);
+}
+
+__device__ __forceinline__ void mp_finish(const MpArgs& a, int k, const double (&S)[MP_NSUM], double (&v)[9]) {
+    {
         const V3 kv = ldv(a.kvec + 3 * k);
         const double pi4V = 4.0 * PI / a.volume, pi8V = 2.0 * pi4V;
         const double ek = a.expk[k];
@@ -574,14 +582,27 @@ __global__ __launch_bounds__(256) void k_mp_kfinish(MpArgs a) {
         a.kf[4 * k] = b * P;
         a.kf[4 * k + 1] = b * Q;
     }
+}
+
+// One wave per k (four per block).
+__global__ __launch_bounds__(256) void k_mp_kfinish(MpArgs a) {
+    const int k = blockIdx.x * (blockDim.x / WAVE) + threadIdx.x / WAVE;
+    double v[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    if (k < a.K) {
+        double S[MP_NSUM];
+        wave_chunk_sum<MP_NSUM>(a, k, S);
+        if (threadIdx.x % WAVE == 0) mp_finish(a, k, S, v);
+    }
     block_add<9>(v, a.terms + M_LONG_E);
 }
 
-// Pass 3: thread per site, every k (staged in LDS tiles): force val k and the
-// torque of the dipole and the quadrupole (:1149-1210).
-__global__ __launch_bounds__(MP_RF_THREADS) void k_mp_kforce(MpArgs a) {
+// Pass 3: force val k and the torque of the dipole and the quadrupole
+// (:1149-1210).  Block (64 sites, one k-range); each lane sums its site over
+// the range and writes a partial; k_kforce_sum adds the ranges in order.
+__global__ __launch_bounds__(WAVE) void k_mp_kforce(MpArgs a) {
     __shared__ double sk[MP_KTILE][5];
-    const int i = blockIdx.x * MP_RF_THREADS + threadIdx.x;
+    const int i = blockIdx.x * WAVE + threadIdx.x;
+    const int kb = blockIdx.y * a.kper, ke = min(a.K, kb + a.kper);
     Site I{};
     V3 ri = v3(0, 0, 0);
     if (i < a.n) {
@@ -589,10 +610,10 @@ __global__ __launch_bounds__(MP_RF_THREADS) void k_mp_kforce(MpArgs a) {
         ri = ldv(a.pos + 3 * i);
     }
     V3 F = v3(0, 0, 0), T = v3(0, 0, 0);
-    for (int k0 = 0; k0 < a.K; k0 += MP_KTILE) {
-        const int cnt = min(MP_KTILE, a.K - k0);
+    for (int k0 = kb; k0 < ke; k0 += MP_KTILE) {
+        const int cnt = min(MP_KTILE, ke - k0);
         __syncthreads();
-        for (int x = threadIdx.x; x < cnt; x += MP_RF_THREADS) {
+        for (int x = threadIdx.x; x < cnt; x += WAVE) {
             sk[x][0] = a.kvec[3 * (k0 + x)];
             sk[x][1] = a.kvec[3 * (k0 + x) + 1];
             sk[x][2] = a.kvec[3 * (k0 + x) + 2];
@@ -619,14 +640,30 @@ __global__ __launch_bounds__(MP_RF_THREADS) void k_mp_kforce(MpArgs a) {
         }
     }
     if (i < a.n) {
-        a.force[3 * i] += COULOMBS * F.x;
-        a.force[3 * i + 1] += COULOMBS * F.y;
-        a.force[3 * i + 2] += COULOMBS * F.z;
-        double* tr = a.trq + 9 * (size_t)a.n;   // k-space part
-        tr[3 * i] = COULOMBS * T.x;
-        tr[3 * i + 1] = COULOMBS * T.y;
-        tr[3 * i + 2] = COULOMBS * T.z;
+        double* p = a.fpart + ((size_t)blockIdx.y * a.n + i) * 6;
+        p[0] = F.x; p[1] = F.y; p[2] = F.z;
+        p[3] = T.x; p[4] = T.y; p[5] = T.z;
     }
+}
+
+// The k-range partials of each site in order: force added, k-space torque
+// stored (torque-array part 3).
+__global__ __launch_bounds__(256) void k_kforce_sum(MpArgs a) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= a.n) return;
+    double F[6] = {0, 0, 0, 0, 0, 0};
+    for (int s = 0; s < a.ksplit; ++s) {
+        const double* p = a.fpart + ((size_t)s * a.n + i) * 6;
+#pragma unroll
+        for (int x = 0; x < 6; ++x) F[x] += p[x];
+    }
+    a.force[3 * i] += COULOMBS * F[0];
+    a.force[3 * i + 1] += COULOMBS * F[1];
+    a.force[3 * i + 2] += COULOMBS * F[2];
+    double* tr = a.trq + 9 * (size_t)a.n;
+    tr[3 * i] = COULOMBS * F[3];
+    tr[3 * i + 1] = COULOMBS * F[4];
+    tr[3 * i + 2] = COULOMBS * F[5];
 }
 
 // ---- host side ---------------------------------------------------------------------
@@ -663,6 +700,9 @@ static MpArgs mp_args(Ctx* c, const Ctx::KSet& ks, double alpha, double real_cut
     a.expk = ks.expk;
     a.part = ks.part;
     a.kf = ks.kf;
+    a.fpart = ks.fpart;
+    a.ksplit = ks.ksplit;
+    a.kper = ks.ksplit > 0 ? (ks.K + ks.ksplit - 1) / ks.ksplit : 0;
     a.alpha = alpha;
     a.alpha2 = alpha * alpha;
     a.alpha2PI = 2.0 * alpha / SQRT_PI;
@@ -759,9 +799,9 @@ int multipole_long(Ctx* c, smd_mp_long_out* out) {
     if ((rc = frames_run(c))) return rc;
     const dim3 g1((a.K + MP_SF_THREADS - 1) / MP_SF_THREADS, a.chunks);
     hipLaunchKernelGGL(k_mp_sfac, g1, dim3(MP_SF_THREADS), 0, c->stream, a);
-    hipLaunchKernelGGL(k_mp_kfinish, dim3((a.K + 255) / 256), b256, 0, c->stream, a);
-    hipLaunchKernelGGL(k_mp_kforce, dim3((c->n + MP_RF_THREADS - 1) / MP_RF_THREADS), dim3(MP_RF_THREADS), 0, c->stream,
-                       a);
+    hipLaunchKernelGGL(k_mp_kfinish, dim3((a.K + 3) / 4), b256, 0, c->stream, a);
+    hipLaunchKernelGGL(k_mp_kforce, dim3((c->n + WAVE - 1) / WAVE, a.ksplit), dim3(WAVE), 0, c->stream, a);
+    hipLaunchKernelGGL(k_kforce_sum, dim3((c->n + 255) / 256), b256, 0, c->stream, a);
     if (c->mp.torques) hipLaunchKernelGGL(k_mp_torques, dim3((c->nmol + 255) / 256), b256, 0, c->stream, a, 8u);
     SMD_HIP(c, hipGetLastError());
     double t[M_COUNT];
@@ -816,9 +856,13 @@ static int kset_build(Ctx* c, Ctx::KSet& ks, double alpha, double kcut, int nsum
         return SMD_OK;
     };
     int rc;
+    // k-ranges of the force pass: enough (site tile, k-range) blocks to fill the chip
+    const int tiles = (c->n + WAVE - 1) / WAVE;
+    const int ksplit = std::max(1, std::min(std::min(64, K), (2048 + tiles - 1) / tiles));
     if ((rc = R(ks.kvec, sizeof(double) * 3 * K)) || (rc = R(ks.kmod2, sizeof(double) * K)) ||
         (rc = R(ks.expk, sizeof(double) * K)) || (rc = R(ks.kf, sizeof(double) * 4 * K)) ||
-        (rc = R(ks.part, sizeof(double) * nsum * (size_t)K * chunks)))
+        (rc = R(ks.part, sizeof(double) * nsum * (size_t)K * chunks)) ||
+        (rc = R(ks.fpart, sizeof(double) * 6 * (size_t)ksplit * c->n)))
         return rc;
     if (!c->d_mp_trq && (rc = R(c->d_mp_trq, sizeof(double) * 15 * (size_t)c->n))) return rc;
     if (!c->d_mp_terms && (rc = R(c->d_mp_terms, sizeof(double) * M_COUNT))) return rc;
@@ -828,6 +872,7 @@ static int kset_build(Ctx* c, Ctx::KSet& ks, double alpha, double kcut, int nsum
     SMD_HIP(c, hipMemcpy(ks.expk, ekv.data(), sizeof(double) * K, hipMemcpyHostToDevice));
     ks.K = K;
     ks.chunks = chunks;
+    ks.ksplit = ksplit;
     return SMD_OK;
 }
 
@@ -1217,18 +1262,8 @@ __global__ __launch_bounds__(MP_SF_THREADS) void k_ind_sfac(MpArgs a) {
 // k-space pass 2 (:1564-1600, the force-pass sums :1640-1650 from moments).
 // Force-pass coefficients per k: X = b (Sc - Ms'), Y = b (Ss + Mc'), b Pc,
 // b Ps with Mc' = Mc (+ Pc), Ms' = Ms (+ Ps) when dipoles are minimised.
-__global__ __launch_bounds__(256) void k_ind_kfinish(MpArgs a) {
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    double v[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-    if (k < a.K) {
-        double S[IN_NSUM];
-#pragma unroll
-        for (int x = 0; x < IN_NSUM; ++x) S[x] = 0.0;
-        for (int c = 0; c < a.chunks; ++c) {
-            const double* p = a.part + ((size_t)c * a.K + k) * IN_NSUM;
-#pragma unroll
-            for (int x = 0; x < IN_NSUM; ++x) S[x] += p[x];
-        }
+__device__ __forceinline__ void ind_finish(const MpArgs& a, int k, const double (&S)[IN_NSUM], double (&v)[9]) {
+    {
         const V3 kv = ldv(a.kvec + 3 * k);
         const double pi4V = 4.0 * PI / a.volume, pi8V = 2.0 * pi4V;
         const double ek = a.expk[k];
@@ -1270,13 +1305,24 @@ __global__ __launch_bounds__(256) void k_ind_kfinish(MpArgs a) {
         a.kf[4 * k + 2] = bPc;
         a.kf[4 * k + 3] = bPs;
     }
+}
+
+__global__ __launch_bounds__(256) void k_ind_kfinish(MpArgs a) {
+    const int k = blockIdx.x * (blockDim.x / WAVE) + threadIdx.x / WAVE;
+    double v[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    if (k < a.K) {
+        double S[IN_NSUM];
+        wave_chunk_sum<IN_NSUM>(a, k, S);
+        if (threadIdx.x % WAVE == 0) ind_finish(a, k, S, v);
+    }
     block_add<9>(v, a.terms + M_LONG_E);
 }
 
-// k-space pass 3: thread per site over all k (:1603-1690).
-__global__ __launch_bounds__(MP_RF_THREADS) void k_ind_kforce(MpArgs a) {
+// k-space pass 3 (:1603-1690), split over k-ranges like k_mp_kforce.
+__global__ __launch_bounds__(WAVE) void k_ind_kforce(MpArgs a) {
     __shared__ double sk[MP_KTILE][7];
-    const int i = blockIdx.x * MP_RF_THREADS + threadIdx.x;
+    const int i = blockIdx.x * WAVE + threadIdx.x;
+    const int kb = blockIdx.y * a.kper, ke = min(a.K, kb + a.kper);
     Site I{};
     V3 ri = v3(0, 0, 0), p = v3(0, 0, 0);
     if (i < a.n) {
@@ -1285,10 +1331,10 @@ __global__ __launch_bounds__(MP_RF_THREADS) void k_ind_kforce(MpArgs a) {
         p = ldv(a.pind + 3 * i);
     }
     V3 F = v3(0, 0, 0), T = v3(0, 0, 0);
-    for (int k0 = 0; k0 < a.K; k0 += MP_KTILE) {
-        const int cnt = min(MP_KTILE, a.K - k0);
+    for (int k0 = kb; k0 < ke; k0 += MP_KTILE) {
+        const int cnt = min(MP_KTILE, ke - k0);
         __syncthreads();
-        for (int x = threadIdx.x; x < cnt; x += MP_RF_THREADS) {
+        for (int x = threadIdx.x; x < cnt; x += WAVE) {
             sk[x][0] = a.kvec[3 * (k0 + x)];
             sk[x][1] = a.kvec[3 * (k0 + x) + 1];
             sk[x][2] = a.kvec[3 * (k0 + x) + 2];
@@ -1314,13 +1360,9 @@ __global__ __launch_bounds__(MP_RF_THREADS) void k_ind_kforce(MpArgs a) {
         }
     }
     if (i < a.n) {
-        a.force[3 * i] += COULOMBS * F.x;
-        a.force[3 * i + 1] += COULOMBS * F.y;
-        a.force[3 * i + 2] += COULOMBS * F.z;
-        double* tr = a.trq + 9 * (size_t)a.n;   // k-space part
-        tr[3 * i] = COULOMBS * T.x;
-        tr[3 * i + 1] = COULOMBS * T.y;
-        tr[3 * i + 2] = COULOMBS * T.z;
+        double* q = a.fpart + ((size_t)blockIdx.y * a.n + i) * 6;
+        q[0] = F.x; q[1] = F.y; q[2] = F.z;
+        q[3] = T.x; q[4] = T.y; q[5] = T.z;
     }
 }
 
@@ -1407,9 +1449,9 @@ int induction_long(Ctx* c, smd_mp_long_out* out) {
     if ((rc = frames_run(c))) return rc;
     const dim3 g1((a.K + MP_SF_THREADS - 1) / MP_SF_THREADS, a.chunks);
     hipLaunchKernelGGL(k_ind_sfac, g1, dim3(MP_SF_THREADS), 0, c->stream, a);
-    hipLaunchKernelGGL(k_ind_kfinish, dim3((a.K + 255) / 256), b256, 0, c->stream, a);
-    hipLaunchKernelGGL(k_ind_kforce, dim3((c->n + MP_RF_THREADS - 1) / MP_RF_THREADS), dim3(MP_RF_THREADS), 0,
-                       c->stream, a);
+    hipLaunchKernelGGL(k_ind_kfinish, dim3((a.K + 3) / 4), b256, 0, c->stream, a);
+    hipLaunchKernelGGL(k_ind_kforce, dim3((c->n + WAVE - 1) / WAVE, a.ksplit), dim3(WAVE), 0, c->stream, a);
+    hipLaunchKernelGGL(k_kforce_sum, dim3((c->n + 255) / 256), b256, 0, c->stream, a);
     if (c->indf.torques) hipLaunchKernelGGL(k_mp_torques, dim3((c->nmol + 255) / 256), b256, 0, c->stream, a, 8u);
     SMD_HIP(c, hipGetLastError());
     double t[M_COUNT];

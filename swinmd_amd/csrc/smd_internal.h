@@ -164,6 +164,8 @@ struct Ctx {
         double* expk = nullptr;     // [K]
         double* part = nullptr;     // [chunks][K][sums]
         double* kf = nullptr;       // [K][4] per-k force-pass coefficients
+        int ksplit = 1;             // k-ranges of the force pass
+        double* fpart = nullptr;    // [ksplit][n][6] force and torque per k-range
     };
     bool mp_ready = false, indf_ready = false;
     smd_mp_params mp{};

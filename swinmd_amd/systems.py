@@ -329,6 +329,23 @@ def c2_water(seed: int = 2024) -> System:
     return water(4096, box=4.9669, seed=seed, lattice=(16, 16, 16), jitter=0.02, name="c2_water")
 
 
+def polar_water(n_mol: int = 1024, seed: int = 2025) -> System:
+    """Polarizable water at 1.0 g/cm^3 with the 3-site water's geometry
+    (O-H 0.1 nm, H-O-H 109.47 deg): site charges, dipoles, quadrupoles and
+    polarizabilities of AMOEBA_TYPES, buffered 14-7 on O only (H eps = 0).
+    1,024 molecules: a seeded subset of an 11^3 lattice in L = 3.1289 nm;
+    8,192 molecules: a 16 x 16 x 32 lattice in L = 6.2579 nm."""
+    h, o = AMOEBA_TYPES
+    types = [AtomType(mass=h.mass, charge=h.charge, eps=0.0, sigma=h.sigma, dip=h.dip, quad=h.quad, polar=h.polar,
+                      type_name=h.type_name, atom_name=h.atom_name), o]
+    if n_mol == 1024:
+        return water(1024, box=3.1289, seed=seed, lattice=(11, 11, 11), subset=True, types=types,
+                     name="polar_water1024")
+    if n_mol == 8192:
+        return water(8192, box=6.2579, seed=seed, lattice=(16, 16, 32), types=types, name="polar_water8192")
+    raise ValueError("polar_water: 1024 or 8192 molecules")
+
+
 def kcut_for(n2max: float, L: float) -> float:
     """k cutoff selecting |n|^2 <= n2max in a cubic box: kCut = 2*pi*sqrt(n2max+0.5)/L."""
     return 2 * PI * math.sqrt(n2max + 0.5) / L
