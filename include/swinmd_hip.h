@@ -382,7 +382,10 @@ int smd_profile_get(smd_ctx* ctx, const char* family, double* total_ms, int64_t*
 int smd_md_time_family(smd_ctx* ctx, const char* family, int reps, double* avg_us);
 
 /* Launch shapes (no counterpart in the reference; results do not depend on
- * them beyond FP rounding order).  "pair_lanes": lanes sharing one atom's row
+ * them beyond FP rounding order).  "pair_packed" (0/1, default 1): the MD
+ * step's mixed LJ + Ewald pairs read packed FP32 hi/lo site records.
+ * "recip_overlap" (0/1, default 1): the MD step's k-space on a second stream.
+ * "pair_lanes": lanes sharing one atom's row
  * in the pair kernels, 4, 8, 16 or 32, or 0 (default: 16 for FP32 pair
  * math, 32 for FP64).  Drops a captured MD
  * graph.  SMD_EINVAL for an unknown name or value. */

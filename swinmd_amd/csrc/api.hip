@@ -70,7 +70,7 @@ static void release(Ctx* c) {
                     c->d_cg, c->d_cg_part, c->d_cg_state, c->d_cg_scal, c->d_frecip, c->d_mp_trq,
                     c->d_mp_terms, c->mp_ks.kvec, c->mp_ks.kmod2, c->mp_ks.expk, c->mp_ks.part, c->mp_ks.kf,
                     c->mp_ks.fpart, c->ind_ks.kvec, c->ind_ks.kmod2, c->ind_ks.expk, c->ind_ks.part,
-                    c->ind_ks.kf, c->ind_ks.fpart};
+                    c->ind_ks.kf, c->ind_ks.fpart, c->d_pack};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     if (c->h_pinned) (void)hipHostFree(c->h_pinned);
@@ -289,6 +289,9 @@ extern "C" int smd_set_tuning(smd_ctx* c, const char* name, int value) {
         if (value != 0 && value != 4 && value != 8 && value != 16 && value != 32)
             return fail(c, SMD_EINVAL, "pair_lanes must be 0 (by precision), 4, 8, 16 or 32 (got %d)", value);
         c->pair_lanes = value;
+    } else if (std::string(name) == "pair_packed") {
+        if (value != 0 && value != 1) return fail(c, SMD_EINVAL, "pair_packed must be 0 or 1 (got %d)", value);
+        c->pair_packed = value;
     } else if (std::string(name) == "recip_overlap") {
         if (value != 0 && value != 1) return fail(c, SMD_EINVAL, "recip_overlap must be 0 or 1 (got %d)", value);
         c->recip_overlap = value;

@@ -30,7 +30,7 @@ namespace smd {
 
 constexpr int IND_LANES = 16;      // lanes per atom row in the field and T.mu kernels
 constexpr int CG_THREADS = 256;    // threads per block of the vector kernels
-constexpr int CG_BATCH = 8;        // CG iterations launched between host checks
+constexpr int CG_BATCH = 16;       // CG iterations launched between host checks
 constexpr double SQRT_COULOMB = 11.78708976;   // SQRTCOULOMBCONSTANT, src/NEMD_defs.h:103
 constexpr double POLMIN = 0.00000001;          // Induction.cpp:195
 
@@ -216,10 +216,28 @@ __device__ __forceinline__ void tmu_pair(const IndArgs& a, double dx, double dy,
     }
 }
 
+// Loop-top test of a CG iteration (:2431-2434), evaluated by every block
+// from the same partials in the same order: stop when |v| < tol.
+struct CgTop {
+    int* state;          // [0] done, [2] stop reason; null: no test
+    const double* p_dv;  // partials of v.v
+    int nparts;
+    double tol;
+};
+
 template <int G>
 __global__ __launch_bounds__(256) void k_apply_a(IndArgs a, const double* __restrict__ v, double* __restrict__ out,
-                                                 const double* __restrict__ dotv, double* part, const int* done) {
-    if (done && *done) return;
+                                                 const double* __restrict__ dotv, double* part, CgTop top) {
+    if (top.state) {
+        if (top.state[0]) return;
+        double dv = 0.0;
+        for (int k = threadIdx.x % WAVE; k < top.nparts; k += WAVE) dv += top.p_dv[k];
+        dv = wave_sum(dv);   // identical in every wave of every block
+        if (sqrt(dv) < top.tol) {
+            if (blockIdx.x == 0 && threadIdx.x == 0) { top.state[0] = 1; top.state[2] = 1; }
+            return;
+        }
+    }
     __shared__ double red[256 / WAVE];
     const int tid = blockIdx.x * blockDim.x + threadIdx.x;
     const int i = tid / G, sub = tid % G;
@@ -329,13 +347,6 @@ __global__ __launch_bounds__(CG_THREADS) void k_cg_start(CgArgs g) {
     if (blockIdx.x == 0 && threadIdx.x == 0) { g.state[0] = 0; g.state[1] = 0; g.state[2] = 0; }
     // dotV equals c at the start: the same partials
     if (threadIdx.x == 0) g.p_dv[blockIdx.x] = g.p_rr0[blockIdx.x];
-}
-
-// Loop top of iteration k: stop when |v| < tol (:2431-2434).
-__global__ void k_cg_check_v(CgArgs g) {
-    if (g.state[0]) return;
-    const double dotV = sum_parts(g.p_dv, g.nparts_vec);
-    if (threadIdx.x == 0 && sqrt(dotV) < g.tol) { g.state[0] = 1; g.state[2] = 1; }
 }
 
 // t = c / v.Av; x += t v; r -= t z; partials of d = r.r (:2446-2454).
@@ -475,14 +486,14 @@ int induce_run(Ctx* c, const smd_induce_params* p, smd_induce_report* rep) {
         if (p->guess == 0) hipLaunchKernelGGL(k_fill, grid_vec, b256, 0, c->stream, dim, 1.0, g.x);
         else if (p->guess == 2) hipLaunchKernelGGL(k_direct, grid_vec, b256, 0, c->stream, dim, c->d_polar, c->d_e0, g.x);
         // guess 1: keep the previous solution in d_mu
-        hipLaunchKernelGGL((k_apply_a<IND_LANES>), grid_rows, b256, 0, c->stream, a, g.x, g.z, nullptr, nullptr, nullptr);
+        hipLaunchKernelGGL((k_apply_a<IND_LANES>), grid_rows, b256, 0, c->stream, a, g.x, g.z, nullptr, nullptr,
+                           CgTop{nullptr, nullptr, 0, 0.0});
         hipLaunchKernelGGL(k_cg_start, grid_vec, dim3(CG_THREADS), 0, c->stream, g);
         for (int it0 = 0; it0 < max_it; it0 += CG_BATCH) {
             const int it1 = std::min(max_it, it0 + CG_BATCH);
             for (int it = it0; it < it1; ++it) {
-                hipLaunchKernelGGL(k_cg_check_v, dim3(1), dim3(WAVE), 0, c->stream, g);
                 hipLaunchKernelGGL((k_apply_a<IND_LANES>), grid_rows, b256, 0, c->stream, a, g.v, g.z, g.v, g.p_vz,
-                                   g.state);
+                                   CgTop{g.state, g.p_dv, g.nparts_vec, g.tol});
                 hipLaunchKernelGGL(k_cg_step, grid_vec, dim3(CG_THREADS), 0, c->stream, g, it);
                 hipLaunchKernelGGL(k_cg_dir, grid_vec, dim3(CG_THREADS), 0, c->stream, g, it);
             }

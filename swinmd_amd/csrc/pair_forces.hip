@@ -716,6 +716,247 @@ int ewald_real(Ctx* c, int prec, int lustig) {
     return SMD_OK;
 }
 
+// ---------------------------------------------------------------------------
+// Mixed-precision LJ + Ewald pairs of the MD step from packed site records.
+// Every row reads one 32-byte record per partner instead of separate
+// position, charge, type and molecule gathers:
+//   rec[2j]   = (x_hi, y_hi, z_hi, q)            x_hi = float(x)
+//   rec[2j+1] = (x_lo, y_lo, z_lo, type<<24|mol)  x_lo = float(x - x_hi)
+// and, for the Lustig terms of owned pairs, off[j] = float(r_j - R_mol(j)).
+// The displacement is formed as (x_hi_j - x_hi_i - n L_hi) + (x_lo_j - x_lo_i
+// - n L_lo): the first difference is exact (the operands are within a factor
+// of two once the image is taken), so the FP32 displacement carries only its
+// own rounding, as the FP64 difference converted to FP32 does.  The molecule
+// vector of the Lustig terms is r_ij + off_i - off_j.
+
+__global__ __launch_bounds__(256) void k_pack_sites(int n, const double* __restrict__ pos, const double* __restrict__ q,
+                                                    const int32_t* __restrict__ type, const int32_t* __restrict__ mol,
+                                                    const double* __restrict__ com, float4* __restrict__ rec,
+                                                    float4* __restrict__ off) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double x = pos[3 * i], y = pos[3 * i + 1], z = pos[3 * i + 2];
+    const float xh = (float)x, yh = (float)y, zh = (float)z;
+    const int m = mol[i];
+    rec[2 * i] = make_float4(xh, yh, zh, (float)q[i]);
+    rec[2 * i + 1] = make_float4((float)(x - (double)xh), (float)(y - (double)yh), (float)(z - (double)zh),
+                                 __int_as_float((type[i] << 24) | (m & 0xffffff)));
+    off[i] = make_float4((float)(x - com[3 * m]), (float)(y - com[3 * m + 1]), (float)(z - com[3 * m + 2]), 0.0f);
+}
+
+struct PackedArgs {
+    int n, stride, ntypes, store_forces, use_constraint, switch_on;
+    const int32_t* nb;
+    const int32_t* nb_cnt;
+    const int32_t* nb_own;
+    const float4* rec;
+    const float4* off;
+    const double* ta;   // sigma^2 [ntypes^2]
+    const double* tb;   // 4 eps
+    double* force;
+    double* terms;
+    float Lh[3], Ll[3], invL[3];
+    float cut2, rcut_2, alpha, alpha2, alpha2PI, realcut2;
+};
+
+struct PackedOwner {
+    float xh, yh, zh, xl, yl, zl, qi, ox, oy, oz;
+    int ti, mi;
+};
+
+// Per-lane energy, Lustig and virial sums in FP32 (a lane adds ~15 terms);
+// the block and grid sums are FP64.
+template <bool LUSTIG, bool FULL>
+__device__ __forceinline__ void packed_term(const PackedArgs& a, const PackedOwner& o, int j, float (&acc)[A_N],
+                                            double& fix, double& fiy, double& fiz) {
+    const float4 r0 = a.rec[2 * j], r1 = a.rec[2 * j + 1];
+    float dx, dy, dz;
+    {
+        const float hx = r0.x - o.xh, hy = r0.y - o.yh, hz = r0.z - o.zh;
+        const float nx = rintf(hx * a.invL[0]), ny = rintf(hy * a.invL[1]), nz = rintf(hz * a.invL[2]);
+        dx = (hx - nx * a.Lh[0]) + ((r1.x - o.xl) - nx * a.Ll[0]);
+        dy = (hy - ny * a.Lh[1]) + ((r1.y - o.yl) - ny * a.Ll[1]);
+        dz = (hz - nz * a.Lh[2]) + ((r1.z - o.zl) - nz * a.Ll[2]);
+    }
+    const int tm = __float_as_int(r1.w);
+    const int mj = tm & 0xffffff, tj = (int)((unsigned)tm >> 24);
+    const float r2 = dx * dx + dy * dy + dz * dz;
+    const float r_1 = __builtin_amdgcn_rsqf(r2);
+    const float r = r2 * r_1;
+    float mx = 0.f, my = 0.f, mz = 0.f;
+    if constexpr (FULL && LUSTIG) {
+        const float4 oj = a.off[j];
+        mx = dx + (o.ox - oj.x);
+        my = dy + (o.oy - oj.y);
+        mz = dz + (o.oz - oj.z);
+    }
+    float s = 0.f;
+    if (o.mi != mj && r2 <= a.cut2) {
+        const int t = o.ti * a.ntypes + tj;
+        const float sigma2 = (float)a.ta[t], eps4 = (float)a.tb[t];
+        const float rij_2 = r_1 * r_1;
+        const float sr2 = sigma2 * rij_2;
+        const float sr6 = sr2 * sr2 * sr2;
+        const float sr12 = sr6 * sr6;
+        float tmpE = eps4 * (sr12 - sr6);
+        const float tmpf = (sr12 + sr12 - sr6) * 6.f * eps4 * rij_2;
+        if constexpr (FULL) {
+            if constexpr (LUSTIG) {
+                const float rijRij = a.use_constraint ? (dx * mx + dy * my + dz * mz) : r2;
+                const float dulj = tmpf * rijRij;
+                acc[A_VDUDV] += dulj;
+                acc[A_VD2UDV] += eps4 * (156.f * sr12 - 42.f * sr6) * rijRij * rijRij * rij_2 * rij_2 + 2.f * dulj;
+            }
+            if (a.switch_on) {   // energy shift only (LJForce.cpp:152-168)
+                const float sr2c = sigma2 * a.rcut_2;
+                const float sr6c = sr2c * sr2c * sr2c;
+                tmpE = tmpE - eps4 * (sr6c * sr6c - sr6c);
+            }
+            acc[A_VE] += tmpE;
+            const float fx = -tmpf * dx, fy = -tmpf * dy, fz = -tmpf * dz;
+            float* v = acc + A_VVIR;
+            v[0] += fx * dx; v[1] += fy * dx; v[2] += fz * dx;
+            v[3] += fx * dy; v[4] += fy * dy; v[5] += fz * dy;
+            v[6] += fx * dz; v[7] += fy * dz; v[8] += fz * dz;
+        }
+        s -= tmpf;
+    }
+    if (r2 < a.realcut2) {
+        const float qij = o.qi * r0.w;
+        float e, ex;
+        ewald_erfc_f32(a.alpha, a.alpha2, r2, r, r_1, e, ex);
+        const float f = (e + a.alpha2PI * ex) * (r_1 * r_1);
+        const float se = -qij * f;
+        if constexpr (FULL) {
+            acc[A_EE] += qij * e;
+            if constexpr (LUSTIG) {
+                const float rr = a.use_constraint ? (dx * mx + dy * my + dz * mz) : r2;
+                acc[A_ED1] += -qij * f * rr;
+                acc[A_ED2] += qij * (2.f * f + 2.f * a.alpha2PI * a.alpha2 * ex) * rr * rr / r2 + 2.f * qij * f * rr;
+            }
+            const float fx = se * dx, fy = se * dy, fz = se * dz;
+            float* v = acc + A_EVIR;
+            v[0] += fx * dx; v[1] += fy * dx; v[2] += fz * dx;
+            v[3] += fx * dy; v[4] += fy * dy; v[5] += fz * dy;
+            v[6] += fx * dz; v[7] += fy * dz; v[8] += fz * dz;
+        }
+        s += se;
+    }
+    fix += (double)(s * dx);
+    fiy += (double)(s * dy);
+    fiz += (double)(s * dz);
+}
+
+template <bool LUSTIG, int G>
+__global__ __launch_bounds__(256) void k_pairs_packed(PackedArgs a) {
+    const int tid = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = tid / G;
+    const int sub = tid % G;
+    float acc[A_N];
+#pragma unroll
+    for (int k = 0; k < A_N; ++k) acc[k] = 0.f;
+    double fix = 0.0, fiy = 0.0, fiz = 0.0;
+    if (i < a.n) {
+        const int32_t* row = a.nb + (int64_t)i * a.stride;
+        const int own = min(a.nb_own[i], a.stride);
+        const int non = min(a.nb_cnt[i] - a.nb_own[i], a.stride - own);
+        const float4 r0 = a.rec[2 * i], r1 = a.rec[2 * i + 1];
+        PackedOwner o;
+        o.xh = r0.x; o.yh = r0.y; o.zh = r0.z; o.qi = r0.w;
+        o.xl = r1.x; o.yl = r1.y; o.zl = r1.z;
+        const int tm = __float_as_int(r1.w);
+        o.mi = tm & 0xffffff;
+        o.ti = (int)((unsigned)tm >> 24);
+        o.ox = o.oy = o.oz = 0.f;
+        if constexpr (LUSTIG) {
+            const float4 oi = a.off[i];
+            o.ox = oi.x; o.oy = oi.y; o.oz = oi.z;
+        }
+        for (int k = sub; k < own; k += G) packed_term<LUSTIG, true>(a, o, row[k], acc, fix, fiy, fiz);
+        const int32_t* back = row + a.stride - 1;
+        for (int k = sub; k < non; k += G) packed_term<LUSTIG, false>(a, o, back[-k], acc, fix, fiy, fiz);
+    }
+#pragma unroll
+    for (int o = G / 2; o > 0; o >>= 1) {
+        fix += __shfl_xor(fix, o, G);
+        fiy += __shfl_xor(fiy, o, G);
+        fiz += __shfl_xor(fiz, o, G);
+    }
+    if (i < a.n && sub == 0) {
+        if (a.store_forces) {
+            a.force[3 * i] = fix; a.force[3 * i + 1] = fiy; a.force[3 * i + 2] = fiz;
+        } else {
+            a.force[3 * i] += fix; a.force[3 * i + 1] += fiy; a.force[3 * i + 2] += fiz;
+        }
+    }
+    {
+        double v[4 + 9];
+#pragma unroll
+        for (int k = 0; k < 13; ++k) v[k] = acc[A_VE + k];
+        block_add<13>(v, a.terms + T_LJ_E);
+    }
+    __syncthreads();
+    {
+        double v[3 + 9];
+#pragma unroll
+        for (int k = 0; k < 12; ++k) v[k] = acc[A_EE + k];
+        block_add<12>(v, a.terms + T_EW_E);
+    }
+}
+
+static int pairs_packed(Ctx* c, const PairArgs& pa, int lustig) {
+    const int n = c->n;
+    if (!c->d_pack) {
+        SMD_HIP(c, hipMalloc(&c->d_pack, sizeof(float4) * 3 * (size_t)n));   // rec [2n], off [n]
+    }
+    float4* rec = reinterpret_cast<float4*>(c->d_pack);
+    float4* off = rec + 2 * (size_t)n;
+    PackedArgs a{};
+    a.n = n;
+    a.stride = pa.stride;
+    a.ntypes = pa.ntypes;
+    a.store_forces = pa.store_forces;
+    a.use_constraint = pa.use_constraint;
+    a.switch_on = pa.switch_on;
+    a.nb = pa.nb;
+    a.nb_cnt = pa.nb_cnt;
+    a.nb_own = pa.nb_own;
+    a.rec = rec;
+    a.off = off;
+    a.ta = pa.ta;
+    a.tb = pa.tb;
+    a.force = pa.force;
+    a.terms = pa.terms;
+    for (int d = 0; d < 3; ++d) {
+        const double L = c->box.L[d];
+        a.Lh[d] = (float)L;
+        a.Ll[d] = (float)(L - (double)a.Lh[d]);
+        a.invL[d] = (float)(1.0 / L);
+    }
+    a.cut2 = (float)pa.cut2;
+    a.rcut_2 = (float)pa.rcut_2;
+    a.alpha = (float)pa.alpha;
+    a.alpha2 = (float)pa.alpha2;
+    a.alpha2PI = (float)pa.alpha2PI;
+    a.realcut2 = (float)pa.realcut2;
+    ProfScope ps(c, "pairs");
+    hipLaunchKernelGGL(k_pack_sites, dim3((n + 255) / 256), dim3(256), 0, c->stream, n, c->d_pos, c->d_q, c->d_type,
+                       c->d_mol, c->d_com, rec, off);
+    const int lanes = c->pair_lanes == 32 || c->pair_lanes == 64 ? c->pair_lanes : 16;
+    const dim3 grid((unsigned)(((int64_t)n * lanes + 255) / 256));
+#define SMD_PACKED(G)                                                                              \
+    do {                                                                                           \
+        if (lustig) hipLaunchKernelGGL((k_pairs_packed<true, G>), grid, dim3(256), 0, c->stream, a);  \
+        else hipLaunchKernelGGL((k_pairs_packed<false, G>), grid, dim3(256), 0, c->stream, a);        \
+    } while (0)
+    if (lanes == 64) SMD_PACKED(64);
+    else if (lanes == 32) SMD_PACKED(32);
+    else SMD_PACKED(16);
+#undef SMD_PACKED
+    SMD_HIP(c, hipGetLastError());
+    return SMD_OK;
+}
+
 // Fused LJ + Ewald real space for the MD step (one pass over the list).
 int pairs_lj_ewald(Ctx* c, const double* sigma2, const double* eps4, double cutoff, double alpha, double real_cut,
                    int use_constraint, int lustig, int prec, bool store_forces) {
@@ -731,6 +972,8 @@ int pairs_lj_ewald(Ctx* c, const double* sigma2, const double* eps4, double cuto
     a.alpha2 = alpha * alpha;
     a.alpha2PI = 2.0 * alpha / SQRT_PI;
     a.realcut2 = real_cut * real_cut;
+    if (prec == SMD_PREC_MIXED && c->pair_packed && c->n < (1 << 24) && c->ntypes <= 127 && !a.coulomb)
+        return pairs_packed(c, a, lustig);
     hipError_t e = launch_pairs<V_LJ, true>(c, a, prec, lustig);
     if (e != hipSuccess) return hip_fail(c, e, "LJ+Ewald kernel launch");
     return SMD_OK;

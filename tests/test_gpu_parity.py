@@ -313,3 +313,30 @@ def test_c2_water_forces_against_oracle():
     _close(eo["real_energy"], r["energy"])
     _close(lo["energy"], rc["energy"])
     _close(lo["deriv2"], rc["deriv2"])
+
+
+def test_md_packed_pair_records_match_unpacked():
+    """The MD step's mixed LJ + Ewald kernel reading packed FP32 hi/lo site
+    records gives the forces and energies of the kernel reading FP64
+    positions, to FP32 rounding, and both stay within the mixed form's bar of
+    the FP64 forces."""
+    from swinmd_amd import systems as S
+    from swinmd_amd import _lib as L
+    from swinmd_amd.forces import WaterMD
+    s = S.water(1000, density_mol=33.4, seed=11, lattice=(10, 10, 10))
+    kcut = S.kcut_for(27, s.box[0])
+    out = {}
+    for name, prec, packed in (("fp64", L.PREC_FP64, 1), ("packed", L.PREC_MIXED, 1), ("plain", L.PREC_MIXED, 0)):
+        ctx = _ctx(s)
+        ctx.set_tuning("pair_packed", packed)
+        md = WaterMD(ctx, cutoff=0.9, cut_buff=0.1, alpha=3.5, kcut=kcut, dt=0.002, temperature=300.0, prec=prec,
+                     use_graph=False)
+        md.init()
+        f0 = ctx.forces()            # start(): the first force evaluation
+        rep = md.run(3)
+        out[name] = (f0, rep)
+    f64, fp, fu = out["fp64"][0], out["packed"][0], out["plain"][0]
+    assert rel_rms(fp, fu) <= 2e-6
+    assert rel_rms(fp, f64) <= 1e-5 and rel_rms(fu, f64) <= 1e-5
+    for k in ("e_lj", "e_real"):
+        assert rel(out["packed"][1][k], out["plain"][1][k]) <= 1e-5
