@@ -767,9 +767,8 @@ struct PackedOwner {
 // Per-lane energy, Lustig and virial sums in FP32 (a lane adds ~15 terms);
 // the block and grid sums are FP64.
 template <bool LUSTIG, bool FULL>
-__device__ __forceinline__ void packed_term(const PackedArgs& a, const PackedOwner& o, int j, float (&acc)[A_N],
-                                            double& fix, double& fiy, double& fiz) {
-    const float4 r0 = a.rec[2 * j], r1 = a.rec[2 * j + 1];
+__device__ __forceinline__ void packed_term(const PackedArgs& a, const PackedOwner& o, int j, float4 r0, float4 r1,
+                                            float (&acc)[A_N], double& fix, double& fiy, double& fiz) {
     float dx, dy, dz;
     {
         const float hx = r0.x - o.xh, hy = r0.y - o.yh, hz = r0.z - o.zh;
@@ -872,9 +871,29 @@ __global__ __launch_bounds__(256) void k_pairs_packed(PackedArgs a) {
             const float4 oi = a.off[i];
             o.ox = oi.x; o.oy = oi.y; o.oz = oi.z;
         }
-        for (int k = sub; k < own; k += G) packed_term<LUSTIG, true>(a, o, row[k], acc, fix, fiy, fiz);
-        const int32_t* back = row + a.stride - 1;
-        for (int k = sub; k < non; k += G) packed_term<LUSTIG, false>(a, o, back[-k], acc, fix, fiy, fiz);
+        // Each segment is walked with the next entry's index and record
+        // loaded before the current entry is evaluated.
+        {
+            int j = sub < own ? row[sub] : 0;
+            float4 r0 = a.rec[2 * j], r1 = a.rec[2 * j + 1];
+            for (int k = sub; k < own; k += G) {
+                const int jn = k + G < own ? row[k + G] : j;
+                const float4 n0 = a.rec[2 * jn], n1 = a.rec[2 * jn + 1];
+                packed_term<LUSTIG, true>(a, o, j, r0, r1, acc, fix, fiy, fiz);
+                j = jn; r0 = n0; r1 = n1;
+            }
+        }
+        {
+            const int32_t* back = row + a.stride - 1;
+            int j = sub < non ? back[-sub] : 0;
+            float4 r0 = a.rec[2 * j], r1 = a.rec[2 * j + 1];
+            for (int k = sub; k < non; k += G) {
+                const int jn = k + G < non ? back[-(k + G)] : j;
+                const float4 n0 = a.rec[2 * jn], n1 = a.rec[2 * jn + 1];
+                packed_term<LUSTIG, false>(a, o, j, r0, r1, acc, fix, fiy, fiz);
+                j = jn; r0 = n0; r1 = n1;
+            }
+        }
     }
 #pragma unroll
     for (int o = G / 2; o > 0; o >>= 1) {
