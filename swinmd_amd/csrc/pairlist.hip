@@ -237,6 +237,7 @@ __device__ __forceinline__ Excl load_excl(const ListArgs& a, int i) {
 
 constexpr int LB_THREADS = 256;   // atoms of cell c per pass of a block
 constexpr int LB_STAGE = 256;     // neighbour atoms staged in LDS at a time
+constexpr int LB_MASK_MAX = 1024; // neighbour-cell atom indices staged for the mask expansion
 constexpr int NSEG = 28;          // per-atom hit counters
 
 // Neighbour s of cell c and, per axis, the multiple of L (-1, 0, +1) that
@@ -280,6 +281,13 @@ __global__ __launch_bounds__(LB_THREADS) void k_list_cells(ListArgs a, CellGrid 
     // the counting pass records its hits as bits when the neighbour cell fits
     // the mask; the writing pass then expands the bits instead of testing again
     const bool use_mask = je - jb <= 32 * a.mask_words;
+    // the writing pass reads the neighbour cell's atom indices from LDS
+    __shared__ int32_t sidx[LB_MASK_MAX];
+    const bool lds_idx = WRITE && use_mask && je - jb <= LB_MASK_MAX;
+    if (lds_idx) {
+        for (int t = threadIdx.x; t < je - jb; t += blockDim.x) sidx[t] = catoms[jb + t];
+        __syncthreads();
+    }
     for (int i0 = ib; i0 < ie; i0 += LB_THREADS) {
         const int k = i0 + threadIdx.x;
         const bool act = k < ie;
@@ -293,27 +301,11 @@ __global__ __launch_bounds__(LB_THREADS) void k_list_cells(ListArgs a, CellGrid 
             px = a.pos[3 * i]; py = a.pos[3 * i + 1]; pz = a.pos[3 * i + 2];
             ex = load_excl(a, i);
             if constexpr (WRITE) {
-                int fwd = 0, bwd = 0, own_before = 0, non_before = 0;
-                for (int t = 1; t <= 13; ++t) {
-                    const int v = seg[(t + 1) * n + i];
-                    if (t < s) own_before += v;
-                    fwd += v;
-                }
-                for (int t = 14; t <= 26; ++t) {
-                    const int v = seg[(t + 1) * n + i];
-                    if (t < s) non_before += v;
-                    bwd += v;
-                }
-                const int so = seg[i], sn = seg[n + i];
-                const int n_own = so + fwd, total = n_own + sn + bwd;
-                fits = total <= a.stride;
-                own_at = s == 0 ? 0 : so + own_before;
-                non_at = s == 0 ? 0 : sn + non_before;
-                if (s == 0) {
-                    a.nb_cnt[i] = total;
-                    a.nb_own[i] = n_own;
-                    if (!fits) atomicOr(a.status, ST_LIST_OVERFLOW);
-                }
+                // k_seg_offsets left each segment's first slot in seg
+                fits = a.nb_cnt[i] <= a.stride;
+                const int at = s == 0 ? 0 : seg[s * n + i];
+                own_at = at;
+                non_at = at;
             }
         }
         int32_t* row = a.nb + (int64_t)i * a.stride;
@@ -326,7 +318,7 @@ __global__ __launch_bounds__(LB_THREADS) void k_list_cells(ListArgs a, CellGrid 
                     while (bits) {
                         const int b = __ffs(bits) - 1;
                         bits &= bits - 1u;
-                        const int j = catoms[jb + 32 * w + b];
+                        const int j = lds_idx ? sidx[32 * w + b] : catoms[jb + 32 * w + b];
                         const bool owned = s == 0 ? (j > i) : (s <= 13);
                         if (owned) row[own_at++] = j;
                         else row[a.stride - 1 - non_at++] = j;
@@ -388,6 +380,31 @@ __global__ __launch_bounds__(LB_THREADS) void k_list_cells(ListArgs a, CellGrid 
             }
         }
     }
+}
+
+// Between the two passes: per atom, the first slot of each stencil cell's
+// segment -- owned hits (same cell j > i, forward cells 1..13) from the front
+// of the row in stencil order, the others (same cell j < i, backward cells
+// 14..26) from the back -- written over the counts; the row totals.
+__global__ __launch_bounds__(256) void k_seg_offsets(ListArgs a, int32_t* __restrict__ seg, const int* gate) {
+    if (gate && !*gate) return;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int n = a.n;
+    if (i >= n) return;
+    int cnt[NSEG];
+#pragma unroll
+    for (int t = 0; t < NSEG; ++t) cnt[t] = seg[t * n + i];
+    // cnt[0], cnt[1]: same cell, owned and not; cnt[s + 1]: stencil cell s
+    int own_at = cnt[0], non_at = cnt[1];
+    seg[i] = 0;
+#pragma unroll
+    for (int s = 1; s <= 13; ++s) { seg[s * n + i] = own_at; own_at += cnt[s + 1]; }
+#pragma unroll
+    for (int s = 14; s <= 26; ++s) { seg[s * n + i] = non_at; non_at += cnt[s + 1]; }
+    const int total = own_at + non_at;
+    a.nb_cnt[i] = total;
+    a.nb_own[i] = own_at;
+    if (total > a.stride) atomicOr(a.status, ST_LIST_OVERFLOW);
 }
 
 // Ensemble::build_pairlist (src/Ensemble.cpp:492-525): all j != i with the
@@ -502,6 +519,7 @@ static int launch_list(Ctx* c, const int* gate, bool md_prep) {
         const dim3 gl(nc, 27);
         hipLaunchKernelGGL((k_list_cells<false>), gl, dim3(LB_THREADS), 0, c->stream, a, g, c->d_cell_start,
                            c->d_cell_atoms, c->d_list_seg, c->d_cell_cnt, gate);
+        hipLaunchKernelGGL(k_seg_offsets, dim3((c->n + 255) / 256), dim3(256), 0, c->stream, a, c->d_list_seg, gate);
         hipLaunchKernelGGL((k_list_cells<true>), gl, dim3(LB_THREADS), 0, c->stream, a, g, c->d_cell_start,
                            c->d_cell_atoms, c->d_list_seg, c->d_cell_cnt, gate);
     } else {
