@@ -11,6 +11,7 @@
 //         ref_harness <dir> pairlist
 //         ref_harness <dir> md <nsteps>
 //         ref_harness <dir> time <nsteps>
+//         ref_harness <dir> time_polar <reps>
 // <dir> holds config.txt, sysData.in and coordinate.in in the reference's
 // formats; outputs (*.bin raw little-endian arrays, meta.txt) are written there.
 //
@@ -394,6 +395,36 @@ int main(int argc, char** argv) {
         meta.put("indf_totTorque", ind->totTrqI);
         meta.put("cutoff_ew", ens->myConfig->get_cutoffEw());
         meta.put("cutoff_ind", ens->myConfig->get_cutoffInd());
+    } else if (mode == "time_polar") {
+        // The polarizable water's force evaluation, as LFIntegrator::run calls
+        // it (src/integrators/LFIntegrator.cpp:382-470): buffered 14-7, then
+        // Multipole and Induction compute() and compute_long(), reps times.
+        const int reps = argc > 3 ? std::atoi(argv[3]) : 1;
+        for (int m = 0; m < ens->nMols; ++m) ens->molecules[m].mass_center();
+        BuffvdWForce* vdw = make_zeroed<BuffvdWForce>(ens);
+        Multipole* mp = make_zeroed<Multipole>(ens);
+        Induction* ind = make_zeroed<Induction>(ens);
+        auto t0 = std::chrono::steady_clock::now();
+        for (int r = 0; r < reps; ++r) {
+            zero_forces(ens);
+            vdw->compute();
+            mp->compute();
+            ind->compute();
+            mp->compute_long();
+            ind->compute_long();
+        }
+        auto t1 = std::chrono::steady_clock::now();
+        meta.put("seconds", std::chrono::duration<double>(t1 - t0).count());
+        meta.put("reps", reps);
+        meta.put("e_buff", vdw->get_energy());
+        meta.put("e_mp_real", ens->myEnergy.realEnergy);
+        meta.put("e_pol_real", ens->myEnergy.realPolEnergy);
+        meta.put("e_mp_long", ens->myEnergy.longEnergy);
+        meta.put("e_pol_long", ens->myEnergy.longPolEnergy);
+        std::vector<double> F = forces_of(ens);
+        write_bin("polar_force", F.data(), F.size() * 8);
+        std::printf("{\"reps\": %d, \"seconds\": %.6f}\n", reps,
+                    std::chrono::duration<double>(t1 - t0).count());
     } else if (mode == "md" || mode == "time") {
         const int nsteps = argc > 3 ? std::atoi(argv[3]) : 1;
         // The reference never initialises Molecule::massCenter before the
