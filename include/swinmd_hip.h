@@ -377,8 +377,10 @@ int smd_profile_get(smd_ctx* ctx, const char* family, double* total_ms, int64_t*
 /* Average duration of one launch of an MD-step kernel family ("pairs",
  * "recip_sfac", "recip_reduce", "recip_force"), launched reps times back to
  * back between two events with the current step's inputs, so launch gaps
- * overlap the previous launch as in a profiler's kernel trace.  Forces and
- * energy sums are restored afterwards.  No counterpart in the reference. */
+ * overlap the previous launch as in a profiler's kernel trace.  "pairs" times
+ * the pair kernel alone (the mixed path's site packing runs once, untimed).
+ * Forces and energy sums are restored afterwards.  No counterpart in the
+ * reference. */
 int smd_md_time_family(smd_ctx* ctx, const char* family, int reps, double* avg_us);
 
 /* Launch shapes (no counterpart in the reference; results do not depend on

