@@ -43,6 +43,10 @@ constexpr int RF_WAVES = 16;   // waves per force block (same atoms, different k
 constexpr int NSUM = 10;       // per-k sums of the structure-factor pass
 constexpr size_t LDS_MAX = 65536;
 
+// d_e1 holds the phases [n][3] (complex) and then, 32-byte aligned, the
+// per-atom (q, d) of the structure-factor pass.
+static inline size_t e1_am_offset(int n) { return (6 * (size_t)n + 3) & ~(size_t)3; }
+
 // ---- host setup -----------------------------------------------------------
 
 int ewald_setup(Ctx* c, const smd_ewald_params* p) {
@@ -136,7 +140,7 @@ int ewald_setup(Ctx* c, const smd_ewald_params* p) {
     chunks = (c->n + per - 1) / per;
     c->sfac_chunks = chunks;
     if ((rc = realloc(c->d_sfac_part, sizeof(double) * NSUM * (size_t)K * chunks))) return rc;
-    if ((rc = realloc(c->d_e1, sizeof(double) * 6 * (size_t)c->n))) return rc;
+    if ((rc = realloc(c->d_e1, sizeof(double) * (e1_am_offset(c->n) + 4 * (size_t)c->n)))) return rc;
     SMD_HIP(c, hipMemcpy(c->d_kgrp, grp.data(), sizeof(int4) * grp.size(), hipMemcpyHostToDevice));
     SMD_HIP(c, hipMemcpy(c->d_ksplit, split.data(), sizeof(int32_t) * (RF_WAVES + 1), hipMemcpyHostToDevice));
     SMD_HIP(c, hipMemcpy(c->d_kvec, kv.data(), sizeof(double) * 3 * K, hipMemcpyHostToDevice));
@@ -293,6 +297,7 @@ struct RecipArgs {
     double* part;              // [chunk][K][NSUM]
     void* kf;                  // [K] (b C, b S) as R pairs
     double2* e1;               // [n][3] exp(i 2 pi r_p / L_p) per atom (pass 0)
+    double4* am;               // [n] (q, d) per atom, d = r - R_mol (pass 0)
     double* force;
     int store;                 // 1: the force pass stores (separate array), 0: adds
     double* terms;
@@ -323,10 +328,18 @@ __device__ __forceinline__ cplx<double> cpow(cplx<double> b, int n) {
 __device__ __forceinline__ int grp_nx(int v) { return v >> 16; }
 __device__ __forceinline__ int grp_ny(int v) { return (int)(short)(v & 0xffff); }
 
-// Pass 0: exp(i 2 pi r_p / L_p) per atom and axis, used by both passes.
+// Pass 0: exp(i 2 pi r_p / L_p) per atom and axis, used by both passes, and
+// each atom's charge and offset from its molecule's centre, read by pass 1.
 __global__ __launch_bounds__(256) void k_recip_phase(int n, const double* __restrict__ pos, double vx, double vy,
-                                                     double vz, double2* __restrict__ e1) {
+                                                     double vz, double2* __restrict__ e1, const double* __restrict__ q,
+                                                     const int32_t* __restrict__ mol, const double* __restrict__ com,
+                                                     double4* __restrict__ am) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n) {
+        const int m = mol[k];
+        am[k] = make_double4(q[k], pos[3 * k] - com[3 * m], pos[3 * k + 1] - com[3 * m + 1],
+                             pos[3 * k + 2] - com[3 * m + 2]);
+    }
     if (k >= 3 * n) return;
     const int p = k % 3;
     const double v = p == 0 ? vx : (p == 1 ? vy : vz);
@@ -360,37 +373,48 @@ __global__ __launch_bounds__(SF_WAVES * WAVE) void k_recip_sfac(RecipArgs a) {
 #pragma unroll
         for (int u = 0; u < NSUM; ++u) acc[t][u] = 0.0;
 
+    // Tile fill: thread fu + SF_TILE fp writes atom fu's powers along axis fp
+    // (fp < 3) or its moments (fp == 3).  Its inputs for the next tile are
+    // loaded before the current tile is evaluated.
+    static_assert(SF_WAVES * WAVE >= 4 * SF_TILE, "one fill task per thread");
+    const int fu = threadIdx.x % SF_TILE, fp = threadIdx.x / SF_TILE;
+    const bool filler = fp < 4;
+    double2 pe = make_double2(1.0, 0.0);
+    double4 pm = make_double4(0.0, 0.0, 0.0, 0.0);
+    auto fetch = [&](int t0) {
+        const int ia = t0 + fu;
+        if (filler && ia < i1) {
+            if (fp < 3) pe = a.e1[3 * ia + fp];
+            else pm = a.am[ia];
+        }
+    };
+    fetch(i0);
     for (int t0 = i0; t0 < i1; t0 += SF_TILE) {
         const int na = min(SF_TILE, i1 - t0);
         __syncthreads();
-        for (int idx = threadIdx.x; idx < 4 * SF_TILE; idx += blockDim.x) {
-            const int u = idx % SF_TILE, p = idx / SF_TILE;
-            if (u >= na) continue;
-            const int ia = t0 + u;
-            if (p < 3) {
+        if (filler && fu < na) {
+            const int u = fu;
+            if (fp < 3) {
                 // powers 0..nmax of exp(i 2 pi r_p / L_p), by recurrence in FP64
-                const int base = p == 0 ? 0 : (p == 1 ? ey0 : ez0);
-                const double2 e = a.e1[3 * ia + p];
-                const double c1 = e.x, s1 = e.y;
+                const int base = fp == 0 ? 0 : (fp == 1 ? ey0 : ez0);
+                const double c1 = pe.x, s1 = pe.y;
                 double er = 1.0, ei = 0.0;
                 tab[base * TS + u] = {R(1), R(0)};
-                for (int k = 1; k <= a.nmax[p]; ++k) {
+                for (int k = 1; k <= a.nmax[fp]; ++k) {
                     const double nr = er * c1 - ei * s1, ni = er * s1 + ei * c1;
                     er = nr; ei = ni;
                     tab[(base + k) * TS + u] = {(R)er, (R)ei};
                 }
             } else {
-                const int m = a.mol[ia];
-                const double dx = a.pos[3 * ia] - a.com[3 * m], dy = a.pos[3 * ia + 1] - a.com[3 * m + 1],
-                             dz = a.pos[3 * ia + 2] - a.com[3 * m + 2];
-                misc[u] = (R)a.q[ia];
-                misc[SF_TILE + u] = (R)dx;
-                misc[2 * SF_TILE + u] = (R)dy;
-                misc[3 * SF_TILE + u] = (R)dz;
-                misc[4 * SF_TILE + u] = (R)(dx * dx + dy * dy + dz * dz);
+                misc[u] = (R)pm.x;
+                misc[SF_TILE + u] = (R)pm.y;
+                misc[2 * SF_TILE + u] = (R)pm.z;
+                misc[3 * SF_TILE + u] = (R)pm.w;
+                misc[4 * SF_TILE + u] = (R)(pm.y * pm.y + pm.z * pm.z + pm.w * pm.w);
             }
         }
         __syncthreads();
+        fetch(t0 + SF_TILE);
         if (g < a.ng) {
             for (int u = w; u < na; u += SF_WAVES) {
                 const R q = misc[u];
@@ -403,16 +427,16 @@ __global__ __launch_bounds__(SF_WAVES * WAVE) void k_recip_sfac(RecipArgs a) {
                 const cplx<R> z1 = tab[(ez0 + 1) * TS + u];
                 const R dx = misc[SF_TILE + u], dy = misc[2 * SF_TILE + u], dz = misc[3 * SF_TILE + u];
                 const R d2 = misc[4 * SF_TILE + u];
+                // all GK sums are formed (uniform control flow); those past
+                // the group's count are never stored
 #pragma unroll
                 for (int t = 0; t < GK; ++t) {
-                    if (t < gg.y) {
-                        acc[t][0] += (double)e.im;
-                        acc[t][1] += (double)e.re;
-                        if constexpr (LUSTIG) {
-                            acc[t][2] += (double)(e.re * dx); acc[t][3] += (double)(e.re * dy); acc[t][4] += (double)(e.re * dz);
-                            acc[t][5] += (double)(e.im * dx); acc[t][6] += (double)(e.im * dy); acc[t][7] += (double)(e.im * dz);
-                            acc[t][8] += (double)(e.re * d2); acc[t][9] += (double)(e.im * d2);
-                        }
+                    acc[t][0] += (double)e.im;
+                    acc[t][1] += (double)e.re;
+                    if constexpr (LUSTIG) {
+                        acc[t][2] += (double)(e.re * dx); acc[t][3] += (double)(e.re * dy); acc[t][4] += (double)(e.re * dz);
+                        acc[t][5] += (double)(e.im * dx); acc[t][6] += (double)(e.im * dy); acc[t][7] += (double)(e.im * dz);
+                        acc[t][8] += (double)(e.re * d2); acc[t][9] += (double)(e.im * d2);
                     }
                     e = cmul(e, z1);
                 }
@@ -634,6 +658,7 @@ int ewald_recip(Ctx* c, int prec, int lustig, unsigned passes) {
     a.part = c->d_sfac_part;
     a.kf = c->d_kf;
     a.e1 = reinterpret_cast<double2*>(c->d_e1);
+    a.am = reinterpret_cast<double4*>(c->d_e1 + e1_am_offset(c->n));
     a.store = c->recip_store ? 1 : 0;
     a.force = c->recip_store ? c->d_frecip : c->d_force;
     a.terms = c->d_terms;
@@ -646,7 +671,7 @@ int ewald_recip(Ctx* c, int prec, int lustig, unsigned passes) {
     if (passes & RECIP_SFAC) {
         ProfScope ps(c, "recip_sfac");
         hipLaunchKernelGGL(k_recip_phase, dim3((3 * c->n + 255) / 256), dim3(256), 0, c->stream, c->n, c->d_pos,
-                           a.vk[0], a.vk[1], a.vk[2], a.e1);
+                           a.vk[0], a.vk[1], a.vk[2], a.e1, c->d_q, c->d_mol, c->d_com, a.am);
         const dim3 g1((a.ng + WAVE - 1) / WAVE, a.chunks);
         const dim3 b1(SF_WAVES * WAVE);
         // FP64 in both forms: its sums are FP64 FMAs, which cost what FP32

@@ -16,6 +16,7 @@
 // found per (cell, neighbour cell) block from LDS-staged positions (see
 // k_list_cells).  Without cells, one wavefront per atom tests all atoms.
 #include <algorithm>
+#include <climits>
 
 #include "smd_internal.h"
 
@@ -162,12 +163,13 @@ struct ListArgs {
     int32_t* nb_own;
     int32_t stride;
     unsigned* status;
-    uint32_t* mask;    // [27][n][mask_words] hit bits of the cell-pair pass
+    uint32_t* mask;    // [27][mask_words][n] hit bits of the cell-pair pass, by cell-order slot
     int mask_words;
 };
 
 struct Excl {
     int cnt;
+    int lo, hi;        // smallest and largest excluded index
     int ids[8];
     const int32_t* rest;
 };
@@ -175,6 +177,7 @@ struct Excl {
 __device__ __forceinline__ bool excluded(const Excl& e, int j) {
     // Ensemble::exclusion_check (src/Ensemble.h:294-305); the first eight
     // entries are compared without a loop so they stay in registers
+    if (j < e.lo || j > e.hi) return false;
     bool hit = false;
 #pragma unroll
     for (int k = 0; k < 8; ++k) hit = hit || (e.ids[k] == j);
@@ -220,6 +223,12 @@ __device__ __forceinline__ Excl load_excl(const ListArgs& a, int i) {
     e.rest = a.excl + o;
 #pragma unroll
     for (int k = 0; k < 8; ++k) e.ids[k] = (k < e.cnt) ? e.rest[k] : -1;
+    e.lo = INT_MAX;
+    e.hi = -1;
+    for (int k = 0; k < e.cnt; ++k) {
+        e.lo = min(e.lo, (int)e.rest[k]);
+        e.hi = max(e.hi, (int)e.rest[k]);
+    }
     return e;
 }
 
@@ -254,6 +263,48 @@ __device__ __forceinline__ int neighbour_cell(const CellGrid& g, int c, int s, i
     if (yy >= g.n[1]) { yy = 0; wrap[1] = 1; } else if (yy < 0) { yy = g.n[1] - 1; wrap[1] = -1; }
     if (zz >= g.n[2]) { zz = 0; wrap[2] = 1; } else if (zz < 0) { zz = g.n[2] - 1; wrap[2] = -1; }
     return zz * g.nxy + yy * g.n[0] + xx;
+}
+
+// Tests up to 32 staged candidates against atom i and returns their hit
+// bits.  SELF: the cell itself (no PBC; j > i owned).  Otherwise the pair
+// vector is shifted by the stencil's wrap: |d + shift| < L/2 on every axis
+// is the reference's single PBC step (a value exactly at -L/2 takes the
+// exact path), and every hit has the block's ownership.  The first pass only
+// counts; the second writes each hit at its slot.
+template <bool WRITE, bool SELF>
+__device__ __forceinline__ uint32_t test_word(const ListArgs& a, const double4* __restrict__ sj, int nb, int i,
+                                              double px, double py, double pz, double shx, double shy, double shz,
+                                              const Excl& ex, bool fits, int32_t* row, int& own_at, int& non_at,
+                                              bool block_owned = true) {
+    uint32_t word = 0;
+#pragma unroll 4
+    for (int b = 0; b < nb; ++b) {
+        const double4 q = sj[b];
+        const int j = (int)q.w;
+        double dx = q.x - px, dy = q.y - py, dz = q.z - pz;
+        if constexpr (!SELF) {
+            const double ex_ = dx + shx, ey_ = dy + shy, ez_ = dz + shz;
+            const bool inside = (fabs(ex_) < a.box.h[0]) & (fabs(ey_) < a.box.h[1]) & (fabs(ez_) < a.box.h[2]);
+            dx = inside ? ex_ : dx;
+            dy = inside ? ey_ : dy;
+            dz = inside ? ez_ : dz;
+            if (!inside) pbc(dx, dy, dz, a.box);
+        }
+        if (len2_exact(dx, dy, dz) < a.rlist2 && (!SELF || j != i) && !excluded(ex, j)) {
+            const bool owned = SELF ? (j > i) : block_owned;
+            if constexpr (WRITE) {
+                if (fits) {
+                    if (owned) row[own_at] = j;
+                    else row[a.stride - 1 - non_at] = j;
+                }
+            } else {
+                word |= 1u << b;
+            }
+            own_at += owned ? 1 : 0;
+            non_at += owned ? 0 : 1;
+        }
+    }
+    return word;
 }
 
 template <bool WRITE>
@@ -296,7 +347,9 @@ __global__ __launch_bounds__(LB_THREADS) void k_list_cells(ListArgs a, CellGrid 
         Excl ex{};
         int own_at = 0, non_at = 0;   // counts (first pass) or next slots (second pass)
         bool fits = true;
-        uint32_t* mrow = a.mask + ((size_t)s * n + i) * a.mask_words;
+        // word w of this atom's hits: mrow[w * n] (slot-major, so a wave's
+        // stores of one word are contiguous)
+        uint32_t* mrow = a.mask + (size_t)s * a.mask_words * n + k;
         if (act) {
             px = a.pos[3 * i]; py = a.pos[3 * i + 1]; pz = a.pos[3 * i + 2];
             ex = load_excl(a, i);
@@ -314,7 +367,7 @@ __global__ __launch_bounds__(LB_THREADS) void k_list_cells(ListArgs a, CellGrid 
             if (act && fits) {
                 const int nw = (je - jb + 31) / 32;
                 for (int w = 0; w < nw; ++w) {
-                    uint32_t bits = mrow[w];
+                    uint32_t bits = mrow[(size_t)w * n];
                     while (bits) {
                         const int b = __ffs(bits) - 1;
                         bits &= bits - 1u;
@@ -327,7 +380,6 @@ __global__ __launch_bounds__(LB_THREADS) void k_list_cells(ListArgs a, CellGrid 
             }
             continue;
         }
-        uint32_t word = 0;
         for (int j0 = jb; j0 < je; j0 += LB_STAGE) {
             const int nj = min(LB_STAGE, je - j0);
             __syncthreads();
@@ -337,39 +389,16 @@ __global__ __launch_bounds__(LB_THREADS) void k_list_cells(ListArgs a, CellGrid 
             }
             __syncthreads();
             if (act) {
-#pragma unroll 4
-                for (int t = 0; t < nj; ++t) {
-                    const double4 q = sj[t];
-                    const int j = (int)q.w;
-                    double dx = q.x - px, dy = q.y - py, dz = q.z - pz;
-                    if (s != 0) {
-                        // |d + shift| < L/2 on every axis: the reference's single
-                        // step (a value exactly at -L/2 takes the exact path)
-                        const double ex = dx + shx, ey = dy + shy, ez = dz + shz;
-                        const bool inside = (fabs(ex) < a.box.h[0]) & (fabs(ey) < a.box.h[1]) & (fabs(ez) < a.box.h[2]);
-                        dx = inside ? ex : dx;
-                        dy = inside ? ey : dy;
-                        dz = inside ? ez : dz;
-                        if (!inside) pbc(dx, dy, dz, a.box);
-                    }
-                    const int tt = j0 - jb + t;   // candidate index within the neighbour cell
-                    if (len2_exact(dx, dy, dz) < a.rlist2 && (s != 0 || j != i) && !excluded(ex, j)) {
-                        const bool owned = s == 0 ? (j > i) : (s <= 13);
-                        if constexpr (WRITE) {
-                            if (fits) {
-                                if (owned) row[own_at] = j;
-                                else row[a.stride - 1 - non_at] = j;
-                            }
-                        } else {
-                            word |= 1u << (tt & 31);
-                        }
-                        own_at += owned ? 1 : 0;
-                        non_at += owned ? 0 : 1;
-                    }
-                    if (!WRITE && use_mask && ((tt & 31) == 31 || tt == je - jb - 1)) {
-                        mrow[tt >> 5] = word;
-                        word = 0;
-                    }
+                // candidates in words of 32: word w0/32 of the hit mask is
+                // stored once its 32 candidates are tested
+                for (int w0 = 0; w0 < nj; w0 += 32) {
+                    const int nb = min(32, nj - w0);
+                    const uint32_t word = s == 0
+                        ? test_word<WRITE, true>(a, sj + w0, nb, i, px, py, pz, 0.0, 0.0, 0.0, ex, fits, row,
+                                                 own_at, non_at)
+                        : test_word<WRITE, false>(a, sj + w0, nb, i, px, py, pz, shx, shy, shz, ex, fits, row,
+                                                  own_at, non_at, s <= 13);
+                    if (!WRITE && use_mask) mrow[(size_t)((j0 - jb + w0) >> 5) * n] = word;
                 }
             }
         }

@@ -127,7 +127,7 @@ struct Ctx {
     int n_kgrp = 0;
     double* d_sfac_part = nullptr;  // [chunks][K][10] per-chunk structure-factor sums
     double* d_kf = nullptr;         // [K] (b C, b S) in the force pass's precision
-    double* d_e1 = nullptr;         // [n][3] complex exp(i 2 pi r_p / L_p)
+    double* d_e1 = nullptr;         // [n][3] complex exp(i 2 pi r_p / L_p), then [n] (q, r - R_mol)
     int sfac_chunks = 0;
     size_t sfac_lds[2] = {0, 0};    // dynamic LDS of the structure-factor pass (FP64, FP32)
     size_t force_lds[2] = {0, 0};   // dynamic LDS of the force pass (FP64, FP32)
