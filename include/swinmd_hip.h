@@ -30,6 +30,7 @@
 #ifndef SWINMD_HIP_H
 #define SWINMD_HIP_H
 
+#include <stddef.h>
 #include <stdint.h>
 
 #ifdef __cplusplus
@@ -47,7 +48,8 @@ enum smd_status {
     SMD_ECAPACITY = -5,   /* pair list row capacity exceeded; rebuild with more */
     SMD_ESHAKE = -6,      /* SHAKE constraint failure (reference: ERRORMSG) */
     SMD_ENOCONV = -7,     /* SHAKE did not converge in max_count sweeps */
-    SMD_ENOTSUP = -8      /* configuration the library does not implement */
+    SMD_ENOTSUP = -8,     /* configuration the library does not implement */
+    SMD_EIO = -9          /* input file missing or malformed (readers, state files) */
 };
 
 enum smd_prec { SMD_PREC_FP64 = 0, SMD_PREC_MIXED = 1 };
@@ -392,6 +394,119 @@ int smd_md_time_family(smd_ctx* ctx, const char* family, int reps, double* avg_u
  * math, 32 for FP64).  Drops a captured MD
  * graph.  SMD_EINVAL for an unknown name or value. */
 int smd_set_tuning(smd_ctx* ctx, const char* name, int value);
+
+/* ---- input formats ------------------------------------------------------
+ * Host-only readers of the reference's input files.  They need no context
+ * and no GPU.  On failure they return SMD_EIO (SMD_EINVAL for bad arguments)
+ * and, when err is not NULL, write a message naming the file, line and field
+ * into err[0..errlen).  The reference reads the same files with unchecked
+ * stream extraction and exits on a missing file (ERRMSG). */
+
+/* SimConfiguration after SimConfiguration::init() (src/SimConfiguration.cpp:
+ * 29-84) and read_config_file() (:695-932): the positional config.txt, one
+ * record per line, the rest of each line ignored (SKIP_REST), booleans as
+ * "t", set_ensemble_status (:951-978) and set_run_type (:933-949) applied.
+ * Member names follow the reference's fields; flags are 0/1.  kCutoff is read
+ * twice by the reference (cutoff line, then Ewald line); the second value is
+ * the one kept. */
+typedef struct {
+    char sys_data_file[80], coordinate_file[80], velocity_file[80];
+    char result_file[80], restart_file[80], trajectory_file[80], dump_file[80];
+    char mol_trajectory_file[80], mol_velocity_file[80], thermo_type[80];
+    double time_step;
+    int32_t n_steps, start_steps, short_ts_freq, long_ts_freq;
+    int32_t avg_ts_freq, print_ts_freq, backup_ts_freq;
+    int32_t start_sampling, sampling_ts_freq, start_ttcf, ttcf_ts_freq, rdf_nbins;
+    int32_t integrator_type;      /* 0 Gear, 1 velocity Verlet, 2 leap-frog */
+    int32_t ensemble_status;      /* 0 NVT, 1 NPT, 2 NVE, 3 NPE */
+    int32_t run_type;             /* 0 equilibrium, 1 compression, 2 shear, 3 elongation */
+    int32_t constraint_mode;      /* 0 none, 1 Gaussian, 2 SHAKE */
+    int32_t max_count, bending_type;
+    uint32_t seed;
+    double temperature, pressure, volume, energy, density;
+    double coupl_thermo, coupl_baros, rate;
+    double cutoff, cutoff_ew, cutoff_ind, k_cutoff, alpha, cut_buff, cell_side_len;
+    double lx_box, ly_box, lz_box;
+    double cfb, dfb, tfb, tolerance;
+    double kl_ub, l0_ub, switch_dist, a_damp;
+    double n_mie, m_mie, n_buff, m_buff, delta_buff, gamma_buff, h_factor;
+    int32_t write_trajectory, is_new_start, read_velocity, use_atom_thermo, use_cell_list;
+    int32_t constraint_on, harmonic_bend, amoeba_bend, harmonic_ub, long_correct, switch_on;
+    int32_t coulomb, ewald, surf_correct, wolf, multipole, induction, torques;
+    int32_t damp_induction, induction_min, damp_amoeba, buff_vdw, mie;
+    int32_t ext_forces, fixed_atom, ttcf, rdf, msd, vacf, lustig;
+    int32_t constant_volume, constant_pressure, constant_temperature, constant_energy;
+    int32_t do_equilibrium, do_compression, do_shear, do_elongation;
+} smd_config;
+
+int smd_config_read(const char* path, smd_config* out, char* err, size_t errlen);
+
+/* Parameters::read_sys_data (src/Parameters.cpp:76-382) with the section
+ * headings checked as check_heading does (:1309-1324).  Atom parameters are
+ * stored at their type index as the reader intends (the reference reads them
+ * into a copy, :112, and keeps none).  Angle rest values are converted from
+ * degrees to radians as the reference does (:186). */
+typedef struct {
+    int32_t n_mols, n_mol_types, n_atoms, n_atom_types;
+    int32_t n_bonds, n_bond_types, n_angles, n_angle_types;
+    int32_t n_dihedrals, n_dihedral_types, n_impropers, n_improper_types;
+} smd_sysdata_sizes;
+
+typedef struct {
+    int32_t type;
+    double mass, charge, eps, sigma;
+    double dip[3];                /* x y z */
+    double quad[6];               /* xx xy xz yy yz zz */
+    double polar;
+    char type_name[16], atom_name[16];
+} smd_atom_param;
+
+typedef struct { int32_t type, t1, t2; double r0, k; } smd_bond_param;
+typedef struct {
+    int32_t type, t1, t2, t3;
+    double theta0, k;             /* theta0 in radians, as the reference stores it */
+    double theta0_deg;            /* the file's value */
+} smd_angle_param;
+typedef struct { int32_t type, t1, t2, t3, t4; double k; int32_t d, n; } smd_torsion_param;
+typedef struct { int32_t type, n_atoms; char name[16]; } smd_mol_type;
+
+/* Arrays owned by the sysdata handle (valid until smd_sysdata_free).  Rows
+ * are stored at the ID the file gives them, as the reference's arrays are. */
+typedef struct {
+    smd_sysdata_sizes sizes;
+    const smd_atom_param* atom_params;         /* [n_atom_types] */
+    const smd_bond_param* bond_params;         /* [n_bond_types] */
+    const smd_angle_param* angle_params;       /* [n_angle_types] */
+    const smd_torsion_param* dihedral_params;  /* [n_dihedral_types] */
+    const smd_torsion_param* improper_params;  /* [n_improper_types] */
+    const smd_mol_type* mol_types;             /* [n_mol_types] */
+    const int32_t* atom_type;                  /* [n_atoms] */
+    const int32_t* atom_mol;                   /* [n_atoms] */
+    const int32_t* mol_type;                   /* [n_mols] */
+    const int32_t* bonds;                      /* [n_bonds][3]: atom1 atom2 type */
+    const int32_t* angles;                     /* [n_angles][4]: atom1 atom2 atom3 type */
+    const int32_t* dihedrals;                  /* [n_dihedrals][5] */
+    const int32_t* impropers;                  /* [n_impropers][5] */
+} smd_sysdata_view;
+
+typedef struct smd_sysdata smd_sysdata;
+int smd_sysdata_read(const char* path, smd_sysdata** out, char* err, size_t errlen);
+int smd_sysdata_view_get(const smd_sysdata* sd, smd_sysdata_view* out);
+void smd_sysdata_free(smd_sysdata* sd);
+
+/* A coordinate or velocity block as Ensemble::read_coordinate / read_velocity
+ * / read_realPos read it (src/Ensemble.cpp:630-758): a count line that must
+ * equal n_atoms, then one "x y z" line per atom.  skip_lines lines are
+ * skipped first (the restart branch skips one line before each block,
+ * :303-317). */
+int smd_read_xyz(const char* path, int64_t n_atoms, int32_t skip_lines, double* out, char* err, size_t errlen);
+
+/* The restart file as the Ensemble constructor reads it when isNewStart is
+ * off (src/Ensemble.cpp:299-317): a skipped line, the coordinate block, a
+ * skipped line, the unwrapped ("real") positions, a skipped line, the
+ * velocities.  Each output is [n_atoms][3]. */
+int smd_read_state(const char* path, int64_t n_atoms, double* pos, double* real_pos, double* vel, char* err,
+                   size_t errlen);
 
 #ifdef __cplusplus
 }

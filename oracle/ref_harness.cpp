@@ -9,6 +9,7 @@
 //         ref_harness <dir> multipole
 //         ref_harness <dir> induction_forces
 //         ref_harness <dir> pairlist
+//         ref_harness <dir> inputs
 //         ref_harness <dir> md <nsteps>
 //         ref_harness <dir> time <nsteps>
 //         ref_harness <dir> time_polar <reps>
@@ -209,7 +210,102 @@ int main(int argc, char** argv) {
         meta.put("surface", cfg->compute_surf_correct() ? 1 : 0);
     }
 
-    if (mode == "pairlist") {
+    if (mode == "inputs") {
+        // What the reference's readers made of the files: SimConfiguration
+        // (getters, src/SimConfiguration.h:209-317), Parameters (sizes,
+        // tables and topology, src/Parameters.h:100-190) and the Ensemble's
+        // per-atom data and exclusion table (src/Ensemble.cpp:140-367).
+        SimConfiguration* c = cfg;
+        const double num[] = {
+            c->get_timestep(), (double)c->get_n_steps(), (double)c->get_start_steps(), (double)c->get_short_ts_freq(),
+            (double)c->get_long_ts_freq(), (double)c->get_avg_ts_freq(), (double)c->get_print_ts_freq(),
+            (double)c->get_backup_ts_freq(), (double)c->get_start_sampling(), (double)c->get_sampling_ts_freq(),
+            (double)c->get_start_TTCF(), (double)c->get_ttcf_ts_freq(), (double)c->get_integrator_type(),
+            (double)c->get_ensemble_status(), (double)c->get_rdf_nbins(), (double)c->get_seed(), c->get_cutoff(),
+            c->get_cutoffEw(), c->get_cutoffInd(), c->get_adamping(), c->get_cutBuff(), c->get_k_cutoff(),
+            c->get_alpha(), c->get_cell_side_len(), c->get_temperature(), c->get_coupl_const(), c->get_pressure(),
+            c->get_baros_const(), c->get_tolerance(), c->get_max_count(), c->get_switch_distance(), c->get_kl_ub(),
+            c->get_l0_ub(), c->get_nMie_param(), c->get_mMie_param(), c->get_nBuff_param(), c->get_mBuff_param(),
+            c->get_deltaBuff_param(), c->get_gammaBuff_param(), c->get_hfactor_reduction(),
+            (double)c->is_read_velocity(), (double)c->is_new_start(), (double)c->is_constraint_on(),
+            (double)c->is_switch_on(), (double)c->write_trajectory(), (double)c->use_long_correct(),
+            (double)c->use_cell_list(), (double)c->use_TTCF(), (double)c->use_harmonic_potential(),
+            (double)c->use_amoeba_bending(), (double)c->use_atom_thermo(), (double)c->compute_coulomb(),
+            (double)c->compute_mie(), (double)c->compute_buffamoeba(), (double)c->compute_harmonic_ub(),
+            (double)c->compute_ewald_force(), (double)c->compute_surf_correct(), (double)c->compute_wolf(),
+            (double)c->compute_multipol(), (double)c->compute_induction(), (double)c->compute_torques(),
+            (double)c->use_damping_induction(), (double)c->do_induction_minimization(),
+            (double)c->use_damping_indAmoeba(), (double)c->has_ext_forces(), (double)c->has_fixed_atom(),
+            (double)c->compute_msd(), (double)c->compute_rdf(), (double)c->compute_lustig(), (double)c->compute_vacf()};
+        write_bin("cfg_num", num, sizeof num);
+        {
+            std::ofstream f("names.txt");
+            f << "sysData " << c->get_sysData_file() << "\ncoordinate " << c->get_coordinate_file()
+              << "\nvelocity " << c->get_velocity_file() << "\nresult " << c->get_result_file() << "\nrestart "
+              << c->get_restart_file() << "\ndump " << c->get_dump_file() << "\nthermo " << c->get_thermo_type()
+              << "\nmoltraj " << c->get_mol_traject_file() << "\nmolvel " << c->get_mol_velocity_file() << "\n";
+        }
+        Parameters* P = params;
+        const int sizes[] = {(int)P->nMolecules, (int)P->nMolTypes, (int)P->nAtoms, (int)P->nAtomTypes,
+                             (int)P->nBonds, (int)P->nBondTypes, (int)P->nAngles, (int)P->nAngleTypes,
+                             (int)P->nDihedrals, (int)P->nDihedralTypes, (int)P->nImpropers, (int)P->nImproperTypes};
+        write_bin("i_sizes", sizes, sizeof sizes);
+        std::vector<double> ap;
+        for (unsigned t = 0; t < P->nAtomTypes; ++t) {
+            const AtomParam& a = P->atomParams[t];
+            for (double v : {(double)a.atomType, a.mass, a.charge, a.eps, a.sigma, a.dipx, a.dipy, a.dipz, a.quadxx,
+                             a.quadxy, a.quadxz, a.quadyy, a.quadyz, a.quadzz, a.polar})
+                ap.push_back(v);
+        }
+        write_bin("atom_params", ap.data(), ap.size() * 8);
+        std::vector<double> bp, gp;
+        for (unsigned t = 0; t < P->nBondTypes; ++t) {
+            const BondParam& b = P->bondParams[t];
+            for (double v : {(double)b.bondType, (double)b.atomType1, (double)b.atomType2, b.r0, b.k}) bp.push_back(v);
+        }
+        for (unsigned t = 0; t < P->nAngleTypes; ++t) {
+            const AngleParam& a = P->angleParams[t];
+            for (double v : {(double)a.angleType, (double)a.atomType1, (double)a.atomType2, (double)a.atomType3,
+                             a.theta0, a.k})
+                gp.push_back(v);
+        }
+        write_bin("bond_params", bp.data(), bp.size() * 8);
+        write_bin("angle_params", gp.data(), gp.size() * 8);
+        std::vector<int> topo, mols, bonds, angles, mtypes;
+        for (unsigned t = 0; t < P->nMolTypes; ++t) {
+            mtypes.push_back(P->molTypes[t].molType);
+            mtypes.push_back(P->molTypes[t].numOfAtoms);
+        }
+        for (unsigned i = 0; i < P->nAtoms; ++i) {
+            topo.push_back(P->atomArray[i].atomType);
+            topo.push_back((int)P->atomArray[i].molID);
+        }
+        for (unsigned m = 0; m < P->nMolecules; ++m) mols.push_back(P->molArray[m].molType);
+        for (unsigned b = 0; b < P->nBonds; ++b)
+            for (int v : {P->bondArray[b].atom1, P->bondArray[b].atom2, P->bondArray[b].bondType}) bonds.push_back(v);
+        for (unsigned a = 0; a < P->nAngles; ++a)
+            for (int v : {P->angleArray[a].atom1, P->angleArray[a].atom2, P->angleArray[a].atom3,
+                          P->angleArray[a].angleType})
+                angles.push_back(v);
+        write_bin("i_mol_types", mtypes.data(), mtypes.size() * 4);
+        write_bin("i_atoms", topo.data(), topo.size() * 4);
+        write_bin("i_mols", mols.data(), mols.size() * 4);
+        write_bin("i_bonds", bonds.data(), bonds.size() * 4);
+        write_bin("i_angles", angles.data(), angles.size() * 4);
+        std::vector<double> mq(2 * ens->nAtoms);
+        for (int i = 0; i < ens->nAtoms; ++i) {
+            mq[2 * i] = ens->atoms[i].mass;
+            mq[2 * i + 1] = ens->atoms[i].scaledCharge;
+        }
+        write_bin("mass_scaled_charge", mq.data(), mq.size() * 8);
+        std::vector<int> eoff(ens->nAtoms + 1, 0), excl;
+        for (int i = 0; i < ens->nAtoms; ++i) {
+            for (int k = 0; ens->exclusionTable[i][k] != -1; ++k) excl.push_back(ens->exclusionTable[i][k]);
+            eoff[i + 1] = (int)excl.size();
+        }
+        write_bin("i_excl_off", eoff.data(), eoff.size() * 4);
+        write_bin("i_excl", excl.data(), excl.size() * 4);
+    } else if (mode == "pairlist") {
         std::vector<int> off(ens->nAtoms + 1, 0), nb;
         for (int i = 0; i < ens->nAtoms; ++i) {
             Atom& a = ens->atoms[i];

@@ -14,6 +14,7 @@ LIB_PATH = os.path.join(HERE, "libswinmd_hip.so")
 SMD_OK = 0
 SMD_EINVAL, SMD_ENOMEM, SMD_EDEVICE, SMD_ESTATE = -1, -2, -3, -4
 SMD_ECAPACITY, SMD_ESHAKE, SMD_ENOCONV, SMD_ENOTSUP = -5, -6, -7, -8
+SMD_EIO = -9
 PREC_FP64, PREC_MIXED = 0, 1
 VDW_LJ, VDW_MIE, VDW_BUCK, VDW_BUFF147 = 0, 1, 2, 3
 
@@ -117,6 +118,70 @@ class IndfOut(C.Structure):
                                            "tot_torque")] + [("virial", C.c_double * 9)]
 
 
+_NAME80 = C.c_char * 80
+_CONFIG_NAMES = ("sys_data_file", "coordinate_file", "velocity_file", "result_file", "restart_file",
+                 "trajectory_file", "dump_file", "mol_trajectory_file", "mol_velocity_file", "thermo_type")
+_CONFIG_I32 = ("n_steps", "start_steps", "short_ts_freq", "long_ts_freq", "avg_ts_freq", "print_ts_freq",
+               "backup_ts_freq", "start_sampling", "sampling_ts_freq", "start_ttcf", "ttcf_ts_freq", "rdf_nbins",
+               "integrator_type", "ensemble_status", "run_type", "constraint_mode", "max_count", "bending_type")
+_CONFIG_F64 = ("temperature", "pressure", "volume", "energy", "density", "coupl_thermo", "coupl_baros", "rate",
+               "cutoff", "cutoff_ew", "cutoff_ind", "k_cutoff", "alpha", "cut_buff", "cell_side_len",
+               "lx_box", "ly_box", "lz_box", "cfb", "dfb", "tfb", "tolerance", "kl_ub", "l0_ub", "switch_dist",
+               "a_damp", "n_mie", "m_mie", "n_buff", "m_buff", "delta_buff", "gamma_buff", "h_factor")
+_CONFIG_FLAGS = ("write_trajectory", "is_new_start", "read_velocity", "use_atom_thermo", "use_cell_list",
+                 "constraint_on", "harmonic_bend", "amoeba_bend", "harmonic_ub", "long_correct", "switch_on",
+                 "coulomb", "ewald", "surf_correct", "wolf", "multipole", "induction", "torques",
+                 "damp_induction", "induction_min", "damp_amoeba", "buff_vdw", "mie", "ext_forces", "fixed_atom",
+                 "ttcf", "rdf", "msd", "vacf", "lustig", "constant_volume", "constant_pressure",
+                 "constant_temperature", "constant_energy", "do_equilibrium", "do_compression", "do_shear",
+                 "do_elongation")
+
+
+class Config(C.Structure):
+    _fields_ = ([(k, _NAME80) for k in _CONFIG_NAMES] + [("time_step", C.c_double)]
+                + [(k, C.c_int32) for k in _CONFIG_I32] + [("seed", C.c_uint32)]
+                + [(k, C.c_double) for k in _CONFIG_F64] + [(k, C.c_int32) for k in _CONFIG_FLAGS])
+
+
+class SysdataSizes(C.Structure):
+    _fields_ = [(k, C.c_int32) for k in ("n_mols", "n_mol_types", "n_atoms", "n_atom_types", "n_bonds",
+                                         "n_bond_types", "n_angles", "n_angle_types", "n_dihedrals",
+                                         "n_dihedral_types", "n_impropers", "n_improper_types")]
+
+
+class AtomParam(C.Structure):
+    _fields_ = [("type", C.c_int32), ("mass", C.c_double), ("charge", C.c_double), ("eps", C.c_double),
+                ("sigma", C.c_double), ("dip", C.c_double * 3), ("quad", C.c_double * 6), ("polar", C.c_double),
+                ("type_name", C.c_char * 16), ("atom_name", C.c_char * 16)]
+
+
+class BondParam(C.Structure):
+    _fields_ = [("type", C.c_int32), ("t1", C.c_int32), ("t2", C.c_int32), ("r0", C.c_double), ("k", C.c_double)]
+
+
+class AngleParam(C.Structure):
+    _fields_ = [("type", C.c_int32), ("t1", C.c_int32), ("t2", C.c_int32), ("t3", C.c_int32),
+                ("theta0", C.c_double), ("k", C.c_double), ("theta0_deg", C.c_double)]
+
+
+class TorsionParam(C.Structure):
+    _fields_ = [("type", C.c_int32), ("t1", C.c_int32), ("t2", C.c_int32), ("t3", C.c_int32), ("t4", C.c_int32),
+                ("k", C.c_double), ("d", C.c_int32), ("n", C.c_int32)]
+
+
+class MolType(C.Structure):
+    _fields_ = [("type", C.c_int32), ("n_atoms", C.c_int32), ("name", C.c_char * 16)]
+
+
+class SysdataView(C.Structure):
+    _fields_ = [("sizes", SysdataSizes), ("atom_params", C.POINTER(AtomParam)),
+                ("bond_params", C.POINTER(BondParam)), ("angle_params", C.POINTER(AngleParam)),
+                ("dihedral_params", C.POINTER(TorsionParam)), ("improper_params", C.POINTER(TorsionParam)),
+                ("mol_types", C.POINTER(MolType))] + [
+        (k, C.POINTER(C.c_int32)) for k in ("atom_type", "atom_mol", "mol_type", "bonds", "angles", "dihedrals",
+                                            "impropers")]
+
+
 # name -> (restype, argtypes); every entry is a symbol declared in include/swinmd_hip.h
 _P = C.c_void_p
 SIGNATURES = {
@@ -169,6 +234,12 @@ SIGNATURES = {
     "smd_induction_setup": (C.c_int, [_P, C.POINTER(IndfParams), C.c_double, C.POINTER(C.c_int32)]),
     "smd_induction_compute": (C.c_int, [_P, C.c_int, _P, C.POINTER(IndfOut)]),
     "smd_induction_compute_long": (C.c_int, [_P, C.c_int, _P, C.POINTER(MpLongOut)]),
+    "smd_config_read": (C.c_int, [C.c_char_p, C.POINTER(Config), C.c_char_p, C.c_size_t]),
+    "smd_sysdata_read": (C.c_int, [C.c_char_p, C.POINTER(C.c_void_p), C.c_char_p, C.c_size_t]),
+    "smd_sysdata_view_get": (C.c_int, [_P, C.POINTER(SysdataView)]),
+    "smd_sysdata_free": (None, [_P]),
+    "smd_read_xyz": (C.c_int, [C.c_char_p, C.c_int64, C.c_int32, _P, C.c_char_p, C.c_size_t]),
+    "smd_read_state": (C.c_int, [C.c_char_p, C.c_int64, _P, _P, _P, C.c_char_p, C.c_size_t]),
 }
 
 _lib = None

@@ -1,6 +1,6 @@
 """Builds libswinmd_hip.so for gfx950 in-tree (hipcc, no JIT cache).
 
-    python -m swinmd_amd.build          # compile every csrc/*.hip and link
+    python -m swinmd_amd.build          # compile every csrc/*.hip (hipcc) and *.cpp (host) and link
 """
 from __future__ import annotations
 
@@ -24,16 +24,24 @@ FLAGS = [
 ]
 
 
+# host-only sources (input readers): the host compiler, no device pass
+CXX = os.environ.get("CXX", "g++")
+CXX_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-Wall"]
+
+
 def sources():
-    return sorted(os.path.join(CSRC, f) for f in os.listdir(CSRC) if f.endswith(".hip"))
+    return sorted(os.path.join(CSRC, f) for f in os.listdir(CSRC) if f.endswith((".hip", ".cpp")))
 
 
 def _compile(src):
     obj = os.path.join(OBJ, os.path.basename(src) + ".o")
-    cmd = [HIPCC, *FLAGS, "-c", src, "-o", obj]
+    if src.endswith(".cpp"):
+        cmd = [CXX, *CXX_FLAGS, "-c", src, "-o", obj]
+    else:
+        cmd = [HIPCC, *FLAGS, "-c", src, "-o", obj]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
-        raise RuntimeError(f"hipcc failed for {src}:\n{r.stdout}\n{r.stderr}")
+        raise RuntimeError(f"{cmd[0]} failed for {src}:\n{r.stdout}\n{r.stderr}")
     return obj
 
 

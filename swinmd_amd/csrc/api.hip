@@ -100,6 +100,7 @@ extern "C" const char* smd_strerror(int code) {
     case SMD_ESHAKE: return "SHAKE constraint failure";
     case SMD_ENOCONV: return "iteration did not converge";
     case SMD_ENOTSUP: return "configuration not supported";
+    case SMD_EIO: return "input file missing or malformed";
     default: return "unknown error";
     }
 }

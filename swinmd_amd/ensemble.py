@@ -39,10 +39,13 @@ def gauss_stream(seed: int, count: int) -> np.ndarray:
     return rst * r
 
 
-def degrees_of_freedom(system, per_atom: bool = True) -> int:
-    """ndf of Ensemble::init_velocity for a constrained system (:1760-1771)."""
+def degrees_of_freedom(system, per_atom: bool = True, constraint_on: bool = True) -> int:
+    """ndf of Ensemble::init_velocity (:1760-1771): per atom, less one per
+    bond and angle when the bonds are constrained; per molecule otherwise."""
     if per_atom:
-        return 3 * system.n_atoms - len(system.bonds) - len(system.angles) - 3
+        if constraint_on:
+            return 3 * system.n_atoms - len(system.bonds) - len(system.angles) - 3
+        return 3 * system.n_atoms - 3
     return 3 * system.n_mols - 3
 
 

@@ -14,6 +14,7 @@ from __future__ import annotations
 
 import json
 import os
+import shutil
 import subprocess
 import sys
 import tempfile
@@ -43,13 +44,58 @@ def run(spec, mode, extra, cfg):
         for line in open(os.path.join(d, "meta.txt")):
             k, v = line.split()
             meta[k] = float(v)
-        arrays = {}
-        for f in os.listdir(d):
-            if f.endswith(".bin"):
-                name = f[:-4]
-                dt = np.int32 if name.startswith("pl_") else np.float64
-                arrays[name] = np.fromfile(os.path.join(d, f), dtype=dt)
+        arrays = read_outputs(d)
     return s, meta, arrays
+
+
+def read_outputs(d):
+    """The harness's *.bin arrays (int32 for names starting pl_ or i_, else
+    float64) and, when present, names.txt as an array of "key value" rows."""
+    arrays = {}
+    for f in os.listdir(d):
+        if f.endswith(".bin"):
+            name = f[:-4]
+            dt = np.int32 if name.startswith(("pl_", "i_")) else np.float64
+            arrays[name] = np.fromfile(os.path.join(d, f), dtype=dt)
+    names = os.path.join(d, "names.txt")
+    if os.path.exists(names):
+        arrays["names"] = np.array([ln.rstrip("\n") for ln in open(names)])
+    return arrays
+
+
+# The reference's own example inputs (its test/ directory), kept as data under
+# tests/golden/ref_test_inputs/.
+REF_TEST_INPUTS = os.path.join(OUT, "ref_test_inputs")
+
+
+def run_ref_test_inputs(mode, extra=None):
+    with tempfile.TemporaryDirectory() as d:
+        for f in os.listdir(REF_TEST_INPUTS):
+            shutil.copy(os.path.join(REF_TEST_INPUTS, f), d)
+        cmd = [HARNESS, d, mode] + ([extra] if extra else [])
+        r = subprocess.run(cmd, capture_output=True, text=True, timeout=1200)
+        if r.returncode != 0:
+            raise RuntimeError(f"harness failed ({r.returncode}):\n{r.stdout[-2000:]}\n{r.stderr[-2000:]}")
+        meta = {}
+        for line in open(os.path.join(d, "meta.txt")):
+            k, v = line.split()
+            meta[k] = float(v)
+        return meta, read_outputs(d)
+
+
+def save_ref_test(name, mode, extra, meta, arrays):
+    path = os.path.join(OUT, name + ".npz")
+    payload = {"mode": mode, "extra": extra or "", "meta": json.dumps(meta)}
+    for k, v in arrays.items():
+        payload["ref_" + k] = v
+    np.savez_compressed(path, **payload)
+    print(f"{path}: {os.path.getsize(path) / 1024:.0f} KiB, {len(arrays)} arrays")
+
+
+# (fixture name, harness mode, extra argument) on the reference's own inputs
+REF_TEST_CASES = [
+    ("ref_test_inputs_parsed", "inputs", None),
+]
 
 
 def save(name, spec, cfg, mode, extra, s, meta, arrays):
@@ -169,6 +215,11 @@ def main(only=None):
             continue
         s, meta, arrays = run(spec, mode, extra, cfg)
         save(name, spec, cfg, mode, extra, s, meta, arrays)
+    for name, mode, extra in REF_TEST_CASES:
+        if only and name not in only:
+            continue
+        meta, arrays = run_ref_test_inputs(mode, extra)
+        save_ref_test(name, mode, extra, meta, arrays)
 
 
 if __name__ == "__main__":
