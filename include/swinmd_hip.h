@@ -395,6 +395,43 @@ int smd_md_time_family(smd_ctx* ctx, const char* family, int reps, double* avg_u
  * graph.  SMD_EINVAL for an unknown name or value. */
 int smd_set_tuning(smd_ctx* ctx, const char* name, int value);
 
+/* ---- bonded terms of flexible molecules ---------------------------------
+ * BondForce (src/forces/BondForce.cpp:46-122), AngleForce
+ * (src/forces/AngleForce.cpp:35-126) and UreyBradleyForce
+ * (src/forces/UreyBradleyForce.cpp:41-101), the terms Integrator::initialise
+ * creates for unconstrained bonds and for computeharmoUB
+ * (src/Integrator.cpp:162-192).  potential is the config's bendingType: 1
+ * harmonic, 2 AMOEBA anharmonic (bonds and angles alike).  Tables are indexed
+ * by type: BondParam r0 and k, AngleParam theta0 (radians) and k.  Every bond
+ * and angle must join atoms of one molecule (SMD_ENOTSUP otherwise); the
+ * Urey-Bradley term acts between the first and third atom of every molecule
+ * (Molecule::myAtoms[0] and [2]). */
+typedef struct {
+    int32_t potential;
+    int32_t n_bond_types, n_angle_types;
+    const double* bond_r0;        /* [n_bond_types] */
+    const double* bond_k;
+    const double* angle_theta0;   /* [n_angle_types] */
+    const double* angle_k;
+    int64_t n_bonds, n_angles;
+    const int32_t* bonds;         /* [n_bonds][3]: atom1 atom2 type (Parameters::bondArray) */
+    const int32_t* angles;        /* [n_angles][4]: atom1 atom2 atom3 type (vertex atom2) */
+    double kl_ub, l0_ub;          /* config klUB, l0UB */
+} smd_bonded_params;
+
+/* energy; the Lustig first and second volume derivatives the term stores
+ * (BondDeriv/BondSecDeriv, ubDeriv/ubSecDeriv; 0 for angles); the term's own
+ * virial, which the reference subtracts from Ensemble::virial. */
+typedef struct {
+    double energy, deriv, deriv2;
+    double virial[9];
+} smd_bonded_out;
+
+int smd_bonded_setup(smd_ctx* ctx, const smd_bonded_params* p);
+int smd_bond_compute(smd_ctx* ctx, double* f_accum, smd_bonded_out* out);
+int smd_angle_compute(smd_ctx* ctx, double* f_accum, smd_bonded_out* out);
+int smd_ub_compute(smd_ctx* ctx, double* f_accum, smd_bonded_out* out);
+
 /* ---- input formats ------------------------------------------------------
  * Host-only readers of the reference's input files.  They need no context
  * and no GPU.  On failure they return SMD_EIO (SMD_EINVAL for bad arguments)

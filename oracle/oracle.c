@@ -2534,3 +2534,126 @@ int orc_induction_long(const orc_sys *s, int K, const double *kvec, const double
     free(buf);
     return 0;
 }
+
+/* ========================================================================== */
+/* bonded terms of flexible molecules                                         */
+/* ========================================================================== */
+
+/* v[3p+q] += f_p r_q, the index order of the bonded terms' virials */
+static inline void vir_fr(double *v, v3 f, v3 r, double sign) {
+    v[XX] += sign * f.x * r.x; v[XY] += sign * f.x * r.y; v[XZ] += sign * f.x * r.z;
+    v[YX] += sign * f.y * r.x; v[YY] += sign * f.y * r.y; v[YZ] += sign * f.y * r.z;
+    v[ZX] += sign * f.z * r.x; v[ZY] += sign * f.z * r.y; v[ZZ] += sign * f.z * r.z;
+}
+
+int orc_bond(const orc_sys *s, int64_t nb, const int32_t *bonds, const double *r0v, const double *kv,
+             int potential, double *force, double out[12]) {
+    /* src/forces/BondForce.cpp:46-122 */
+    if (potential != 1 && potential != 2) return -1;
+    const double c1 = potential == 2 ? 25.5 : 0.0, c2 = potential == 2 ? 379.3125 : 0.0;
+    double energy = 0.0, dubond = 0.0, d2ubond = 0.0, vir[9] = {0};
+    for (int64_t i = 0; i < nb; ++i) {
+        const int a1 = bonds[3 * i], a2 = bonds[3 * i + 1], t = bonds[3 * i + 2];
+        const double k = kv[t], r0 = r0v[t];
+        v3 r12 = v3_sub(v3_ld(s->pos + 3 * a1), v3_ld(s->pos + 3 * a2));
+        apply_pbc(&r12, s->box);
+        const double r = v3_len(r12);
+        const double diff = r - r0;
+        double U, val;
+        if (potential == 1) {
+            U = 0.5 * k * diff * diff;
+            val = k * diff;
+        } else {
+            const double diff2 = diff * diff;
+            const double anh = 1.0 - c1 * diff + c2 * diff2;
+            U = k * diff2 * anh;
+            val = 2.0 * k * diff * anh + k * diff2 * (-c1 + c2 * 2.0 * diff);
+        }
+        const v3 f = v3_div(v3_scale(-val, r12), r);
+        energy += U;
+        dubond += val * r;
+        d2ubond += -2.0 * r * val + r * r * k;
+        vir_fr(vir, f, r12, 1.0);
+        acc(force + 3 * a1, f);
+        dec(force + 3 * a2, f);
+    }
+    out[0] = energy; out[1] = dubond; out[2] = d2ubond;
+    for (int k = 0; k < 9; ++k) out[3 + k] = vir[k];
+    return 0;
+}
+
+int orc_angle(const orc_sys *s, int64_t na, const int32_t *angles, const double *theta0v, const double *kv,
+              int potential, double *force, double out[12]) {
+    /* src/forces/AngleForce.cpp:35-126 */
+    if (potential != 1 && potential != 2) return -1;
+    double energy = 0.0, vir[9] = {0};
+    for (int64_t i = 0; i < na; ++i) {
+        const int a1 = angles[4 * i], a2 = angles[4 * i + 1], a3 = angles[4 * i + 2], t = angles[4 * i + 3];
+        const double theta0 = theta0v[t], k = kv[t];
+        v3 r12 = v3_sub(v3_ld(s->pos + 3 * a2), v3_ld(s->pos + 3 * a1));
+        apply_pbc(&r12, s->box);
+        const double d12r = 1.0 / v3_len(r12);
+        v3 r32 = v3_sub(v3_ld(s->pos + 3 * a2), v3_ld(s->pos + 3 * a3));
+        apply_pbc(&r32, s->box);
+        const double d32r = 1.0 / v3_len(r32);
+        double cosT = v3_dot(r12, r32) * d12r * d32r;
+        cosT = cosT > -0.9999999 ? cosT : -0.9999999;   /* MAX(cosTheta, -0.9999999) */
+        const double sinT = sqrt(1.0 - cosT * cosT);
+        const double invSin = 1.0 / sinT;
+        double diff = acos(cosT) - theta0;
+        const v3 u12 = v3_scale(d12r, r12), u32 = v3_scale(d32r, r32);
+        double U, v;
+        if (potential == 1) {
+            U = 0.5 * k * diff * diff;
+            v = k * diff;
+        } else {
+            const double deg = 180 / PI;
+            const double c1 = 0.014, c2 = 5.6E-5, c3 = 7.0E-7, c4 = 2.2E-8;
+            diff *= deg;
+            const double diff2 = diff * diff, diff3 = diff * diff2, diff4 = diff2 * diff2;
+            const double anh = 1.0 - c1 * diff + c2 * diff2 - c3 * diff3 + c4 * diff4;
+            U = k * diff2 * anh / (deg * deg);
+            v = 2.0 * k * diff * anh / deg + k * diff2 * (-c1 + c2 * 2.0 * diff - c3 * 3.0 * diff2 + c4 * 4.0 * diff3) / deg;
+        }
+        energy += U;
+        const v3 f1 = v3_scale(v * d12r * invSin, v3_sub(v3_scale(cosT, u12), u32));
+        const v3 f3 = v3_scale(v * d32r * invSin, v3_sub(v3_scale(cosT, u32), u12));
+        const v3 f2 = v3_scale(-1.0, v3_add(f1, f3));
+        /* virial[XX] += force1.x * r12.x + force3.x * r32.x; ... */
+        const double a1v[3] = {f1.x, f1.y, f1.z}, a3v[3] = {f3.x, f3.y, f3.z};
+        const double r1v[3] = {r12.x, r12.y, r12.z}, r3v[3] = {r32.x, r32.y, r32.z};
+        for (int p = 0; p < 3; ++p)
+            for (int q = 0; q < 3; ++q) vir[3 * p + q] += a1v[p] * r1v[q] + a3v[p] * r3v[q];
+        acc(force + 3 * a1, f1);
+        acc(force + 3 * a2, f2);
+        acc(force + 3 * a3, f3);
+    }
+    out[0] = energy; out[1] = 0.0; out[2] = 0.0;
+    for (int k = 0; k < 9; ++k) out[3 + k] = vir[k];
+    return 0;
+}
+
+int orc_ub(const orc_sys *s, double kl, double l0, double *force, double out[12]) {
+    /* src/forces/UreyBradleyForce.cpp:41-101 */
+    double energy = 0.0, dubond = 0.0, d2ubond = 0.0, vir[9] = {0};
+    for (int m = 0; m < s->n_mols; ++m) {
+        if (s->mol_start[m + 1] - s->mol_start[m] < 3) return -1;
+        const int id1 = s->mol_start[m], id2 = s->mol_start[m] + 2;
+        v3 r12 = v3_sub(v3_ld(s->pos + 3 * id1), v3_ld(s->pos + 3 * id2));
+        apply_pbc(&r12, s->box);
+        const double r = v3_len(r12);
+        const double diff = r - l0;
+        const double U = kl * diff * diff;
+        const double val = 2.0 * kl * diff;
+        const v3 f = v3_div(v3_scale(-val, r12), r);
+        energy += U;
+        dubond += val * r;
+        d2ubond += -2.0 * r * val + r * r * kl;
+        vir_fr(vir, f, r12, -1.0);
+        acc(force + 3 * id1, f);
+        dec(force + 3 * id2, f);
+    }
+    out[0] = energy; out[1] = dubond; out[2] = d2ubond;
+    for (int k = 0; k < 9; ++k) out[3 + k] = vir[k];
+    return 0;
+}

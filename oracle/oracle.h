@@ -313,6 +313,21 @@ int orc_induction_long(const orc_sys *s, int K, const double *kvec, const double
                        const double *dip, const double *quad, const double *pind, const orc_indf_params *p,
                        double *force, orc_indf_out *out);
 
+/* ---- bonded terms of flexible molecules ---------------------------------
+ * out[12] = energy, Lustig first and second derivative (0 for angles), the
+ * term's virial (virial[XY] = f.x r.y here, as these three routines index
+ * it).  potential: 1 harmonic, 2 AMOEBA anharmonic.  force is accumulated. */
+/* BondForce::compute (src/forces/BondForce.cpp:46-122); bonds [nb][3]. */
+int orc_bond(const orc_sys *s, int64_t nb, const int32_t *bonds, const double *r0, const double *k,
+             int potential, double *force, double out[12]);
+/* AngleForce::compute (src/forces/AngleForce.cpp:35-126); angles [na][4],
+ * theta0 in radians. */
+int orc_angle(const orc_sys *s, int64_t na, const int32_t *angles, const double *theta0, const double *k,
+              int potential, double *force, double out[12]);
+/* UreyBradleyForce::compute (src/forces/UreyBradleyForce.cpp:41-101):
+ * between atoms 0 and 2 of every molecule. */
+int orc_ub(const orc_sys *s, double kl, double l0, double *force, double out[12]);
+
 #ifdef __cplusplus
 }
 #endif

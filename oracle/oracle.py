@@ -14,6 +14,7 @@ import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "liboracle.so")
+PI = 3.14159265358979323846   # src/NEMD_defs.h:77
 
 _i32p = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
 _f64p = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
@@ -151,6 +152,11 @@ def lib():
                                                                                     P(IndfOut)]
         _lib.orc_induced_dipoles.argtypes = [P(OrcSys), C.c_void_p, C.c_void_p, C.c_void_p, P(IndParams),
                                              C.c_void_p, C.c_void_p, P(C.c_int), P(C.c_double)]
+        _lib.orc_bond.argtypes = [P(OrcSys), C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p,
+                                  C.c_void_p]
+        _lib.orc_angle.argtypes = [P(OrcSys), C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p,
+                                   C.c_void_p]
+        _lib.orc_ub.argtypes = [P(OrcSys), C.c_double, C.c_double, C.c_void_p, C.c_void_p]
     return _lib
 
 
@@ -392,6 +398,36 @@ class Oracle:
         r["long_virial"] = np.array(out.long_virial[:])
         r["short_force"], r["long_force"] = fs, fl
         return r
+
+    # ---- bonded terms (BondForce, AngleForce, UreyBradleyForce) ------------
+    @staticmethod
+    def _bonded_out(rc, name, out, force):
+        if rc != 0:
+            raise RuntimeError(f"{name} failed: {rc}")
+        return {"energy": out[0], "deriv": out[1], "deriv2": out[2], "virial": out[3:].copy(), "force": force}
+
+    def bond(self, potential=1):
+        s = self.s
+        rows = np.ascontiguousarray(np.array(s.bonds, dtype=np.int32).reshape(-1, 3))
+        r0 = np.ascontiguousarray([b[2] for b in s.bond_params], dtype=np.float64)
+        k = np.ascontiguousarray([b[3] for b in s.bond_params], dtype=np.float64)
+        f, out = np.zeros((s.n_atoms, 3)), np.zeros(12)
+        rc = lib().orc_bond(C.byref(self._sys), len(rows), _p(rows), _p(r0), _p(k), potential, _p(f), _p(out))
+        return self._bonded_out(rc, "orc_bond", out, f)
+
+    def angle(self, potential=1):
+        s = self.s
+        rows = np.ascontiguousarray(np.array(s.angles, dtype=np.int32).reshape(-1, 4))
+        th = np.ascontiguousarray([a[3] * 2 * PI / 360.0 for a in s.angle_params], dtype=np.float64)
+        k = np.ascontiguousarray([a[4] for a in s.angle_params], dtype=np.float64)
+        f, out = np.zeros((s.n_atoms, 3)), np.zeros(12)
+        rc = lib().orc_angle(C.byref(self._sys), len(rows), _p(rows), _p(th), _p(k), potential, _p(f), _p(out))
+        return self._bonded_out(rc, "orc_angle", out, f)
+
+    def urey_bradley(self, kl, l0):
+        f, out = np.zeros((self.s.n_atoms, 3)), np.zeros(12)
+        rc = lib().orc_ub(C.byref(self._sys), kl, l0, _p(f), _p(out))
+        return self._bonded_out(rc, "orc_ub", out, f)
 
     # ---- integrator ------------------------------------------------------
     def init_velocity(self, temperature, seed, per_atom=True, ndf=None):

@@ -347,6 +347,29 @@ int main(int argc, char** argv) {
                 meta.put(item + "_ulus", ens->myLustig.ljUlus);
                 meta.put(item + "_dudv", ens->myLustig.ljdUlus);
                 meta.put(item + "_d2udv", ens->myLustig.ljd2Ulus);
+            } else if (item == "bond" || item == "angle" || item == "ub") {
+                // BondForce / AngleForce / UreyBradleyForce::compute
+                Force* f = nullptr;
+                if (item == "bond") f = make_zeroed<BondForce>(ens);
+                if (item == "angle") f = make_zeroed<AngleForce>(ens);
+                if (item == "ub") f = make_zeroed<UreyBradleyForce>(ens);
+                f->compute();
+                f->get_virial(vir);
+                std::vector<double> F = forces_of(ens);
+                write_bin(item + "_force", F.data(), F.size() * 8);
+                write_bin(item + "_virial", vir, sizeof vir);
+                meta.put(item + "_energy", f->get_energy());
+                if (item == "bond") {
+                    meta.put("bond_deriv", ens->myLustig.BondDeriv);
+                    meta.put("bond_deriv2", ens->myLustig.BondSecDeriv);
+                    meta.put("bond_ensEnergy", ens->myEnergy.bondEnergy);
+                } else if (item == "angle") {
+                    meta.put("angle_ensEnergy", ens->myEnergy.angleEnergy);
+                } else {
+                    meta.put("ub_deriv", ens->myLustig.ubDeriv);
+                    meta.put("ub_deriv2", ens->myLustig.ubSecDeriv);
+                    meta.put("ub_ensEnergy", ens->myEnergy.ubEnergy);
+                }
             } else if (item == "ewald") {
                 EwaldForce* ew = make_zeroed<EwaldForce>(ens);
                 ew->compute();

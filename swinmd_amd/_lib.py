@@ -182,6 +182,17 @@ class SysdataView(C.Structure):
                                             "impropers")]
 
 
+class BondedParams(C.Structure):
+    _fields_ = [("potential", C.c_int32), ("n_bond_types", C.c_int32), ("n_angle_types", C.c_int32),
+                ("bond_r0", C.c_void_p), ("bond_k", C.c_void_p), ("angle_theta0", C.c_void_p),
+                ("angle_k", C.c_void_p), ("n_bonds", C.c_int64), ("n_angles", C.c_int64), ("bonds", C.c_void_p),
+                ("angles", C.c_void_p), ("kl_ub", C.c_double), ("l0_ub", C.c_double)]
+
+
+class BondedOut(C.Structure):
+    _fields_ = [("energy", C.c_double), ("deriv", C.c_double), ("deriv2", C.c_double), ("virial", C.c_double * 9)]
+
+
 # name -> (restype, argtypes); every entry is a symbol declared in include/swinmd_hip.h
 _P = C.c_void_p
 SIGNATURES = {
@@ -234,6 +245,10 @@ SIGNATURES = {
     "smd_induction_setup": (C.c_int, [_P, C.POINTER(IndfParams), C.c_double, C.POINTER(C.c_int32)]),
     "smd_induction_compute": (C.c_int, [_P, C.c_int, _P, C.POINTER(IndfOut)]),
     "smd_induction_compute_long": (C.c_int, [_P, C.c_int, _P, C.POINTER(MpLongOut)]),
+    "smd_bonded_setup": (C.c_int, [_P, C.POINTER(BondedParams)]),
+    "smd_bond_compute": (C.c_int, [_P, _P, C.POINTER(BondedOut)]),
+    "smd_angle_compute": (C.c_int, [_P, _P, C.POINTER(BondedOut)]),
+    "smd_ub_compute": (C.c_int, [_P, _P, C.POINTER(BondedOut)]),
     "smd_config_read": (C.c_int, [C.c_char_p, C.POINTER(Config), C.c_char_p, C.c_size_t]),
     "smd_sysdata_read": (C.c_int, [C.c_char_p, C.POINTER(C.c_void_p), C.c_char_p, C.c_size_t]),
     "smd_sysdata_view_get": (C.c_int, [_P, C.POINTER(SysdataView)]),

@@ -196,7 +196,17 @@ struct Ctx {
     Prof prof;
     int pair_lanes = 0;           // lanes per atom row in the pair kernels (0: by precision)
     int pair_packed = 1;
-    bool pack_reuse = false;     // packed site records are current: skip the repack (kernel timing)          // mixed LJ + Ewald of the MD step from packed site records
+    bool pack_reuse = false;
+    // bonded terms of flexible molecules (bonded.hip)
+    bool bonded_ready = false, bd_ub_ok = false;
+    int bd_potential = 1, bd_n_bond_types = 0;
+    double bd_kl = 0.0, bd_l0 = 0.0;
+    void* d_bd_bonds = nullptr;     // int4 [n_bonds] grouped by molecule
+    void* d_bd_angles = nullptr;    // int4 [n_angles]
+    int32_t* d_bd_bond_off = nullptr;
+    int32_t* d_bd_angle_off = nullptr;
+    double* d_bd_par = nullptr;
+    double* d_bd_out = nullptr;     // packed site records are current: skip the repack (kernel timing)          // mixed LJ + Ewald of the MD step from packed site records
     void* d_pack = nullptr;       // float4 [3n]: packed records, then offsets from the molecule centres
 };
 

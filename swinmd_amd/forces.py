@@ -9,6 +9,9 @@ Class names and method meanings follow the reference:
   src/forces/{LJ,Mie,Buckingham,BuffvdW}Force.cpp.
 * :class:`EwaldForce` mirrors ``EwaldForce::compute`` / ``compute_long``
   (src/forces/EwaldForce.cpp:128-196).
+* :class:`BondForce`, :class:`AngleForce`, :class:`UreyBradleyForce` mirror
+  the bonded terms of flexible molecules (src/forces/{Bond,Angle,
+  UreyBradley}Force.cpp).
 * :class:`LFIntegrator` mirrors the SHAKE + Nose-Hoover leap-frog step of
   src/integrators/LFIntegrator.cpp.
 
@@ -22,6 +25,7 @@ import ctypes as C
 import numpy as np
 
 from . import _lib as L
+from .systems import PI
 
 
 def _ptr(a):
@@ -236,6 +240,66 @@ class BuckinghamForce(_VdwForce):
 
 class BuffvdWForce(_VdwForce):
     KIND = L.VDW_BUFF147
+
+
+class _BondedTerm:
+    """Common set-up of BondForce, AngleForce and UreyBradleyForce: the
+    topology's bonds and angles with their type tables (Parameters
+    bondParams/angleParams), the bending potential (config bendingType: 1
+    harmonic, 2 AMOEBA) and the Urey-Bradley kl, l0 (config klUB, l0UB)."""
+
+    FN = None
+
+    def __init__(self, ctx: Context, *, bending_type=1, kl_ub=0.0, l0_ub=0.0):
+        self.ctx = ctx
+        key = (int(bending_type), float(kl_ub), float(l0_ub))
+        if getattr(ctx, "_bonded_key", None) != key:
+            s = ctx.system
+            bonds = np.ascontiguousarray(np.array(s.bonds, dtype=np.int32).reshape(-1, 3))
+            angles = np.ascontiguousarray(np.array(s.angles, dtype=np.int32).reshape(-1, 4))
+            r0 = _f64([b[2] for b in s.bond_params])
+            kb = _f64([b[3] for b in s.bond_params])
+            th = _f64([a[3] * 2 * PI / 360.0 for a in s.angle_params])   # AngleParam theta0 (Parameters.cpp:186)
+            ka = _f64([a[4] for a in s.angle_params])
+            keep = [bonds, angles, r0, kb, th, ka]
+            p = L.BondedParams(int(bending_type), len(r0), len(th), _ptr(r0), _ptr(kb), _ptr(th), _ptr(ka),
+                               len(bonds), len(angles), _ptr(bonds), _ptr(angles), float(kl_ub), float(l0_ub))
+            ctx.check(ctx.lib.smd_bonded_setup(ctx.h, C.byref(p)))
+            ctx._bonded_key = key
+            ctx._bonded_keep = keep
+        self.energy = 0.0
+        self.virial = np.zeros(9)
+
+    def compute(self, f_accum=None):
+        """Force::compute(): adds forces into f_accum (host, (n,3)) when given,
+        else into the device accumulator; returns energy, the Lustig
+        derivatives and the term's virial."""
+        out = L.BondedOut()
+        self.ctx.check(getattr(self.ctx.lib, self.FN)(self.ctx.h, _fptr(f_accum), C.byref(out)))
+        self.energy = out.energy
+        self.virial = np.array(out.virial[:])
+        return {"energy": out.energy, "deriv": out.deriv, "deriv2": out.deriv2, "virial": self.virial.copy()}
+
+    def get_energy(self):
+        return self.energy
+
+    def get_virial(self):
+        return self.virial.copy()
+
+
+class BondForce(_BondedTerm):
+    """BondForce (src/forces/BondForce.cpp:46-122)."""
+    FN = "smd_bond_compute"
+
+
+class AngleForce(_BondedTerm):
+    """AngleForce (src/forces/AngleForce.cpp:35-126)."""
+    FN = "smd_angle_compute"
+
+
+class UreyBradleyForce(_BondedTerm):
+    """UreyBradleyForce (src/forces/UreyBradleyForce.cpp:41-101)."""
+    FN = "smd_ub_compute"
 
 
 def lj_long_range_correct(ctx: Context, cutoff):

@@ -68,10 +68,18 @@ def read_outputs(d):
 REF_TEST_INPUTS = os.path.join(OUT, "ref_test_inputs")
 
 
-def run_ref_test_inputs(mode, extra=None):
+def run_ref_test_inputs(mode, extra=None, edits=None):
+    """Runs the harness on a copy of the reference's example inputs; edits
+    maps a config.txt line index to its replacement."""
     with tempfile.TemporaryDirectory() as d:
         for f in os.listdir(REF_TEST_INPUTS):
             shutil.copy(os.path.join(REF_TEST_INPUTS, f), d)
+        if edits:
+            cfg = os.path.join(d, "config.txt")
+            lines = open(cfg).read().split("\n")
+            for k, v in edits.items():
+                lines[k] = v
+            open(cfg, "w").write("\n".join(lines))
         cmd = [HARNESS, d, mode] + ([extra] if extra else [])
         r = subprocess.run(cmd, capture_output=True, text=True, timeout=1200)
         if r.returncode != 0:
@@ -83,18 +91,23 @@ def run_ref_test_inputs(mode, extra=None):
         return meta, read_outputs(d)
 
 
-def save_ref_test(name, mode, extra, meta, arrays):
+def save_ref_test(name, mode, extra, edits, meta, arrays):
     path = os.path.join(OUT, name + ".npz")
-    payload = {"mode": mode, "extra": extra or "", "meta": json.dumps(meta)}
+    payload = {"mode": mode, "extra": extra or "", "meta": json.dumps(meta),
+               "edits": json.dumps({str(k): v for k, v in (edits or {}).items()})}
     for k, v in arrays.items():
         payload["ref_" + k] = v
     np.savez_compressed(path, **payload)
     print(f"{path}: {os.path.getsize(path) / 1024:.0f} KiB, {len(arrays)} arrays")
 
 
-# (fixture name, harness mode, extra argument) on the reference's own inputs
+# (fixture name, harness mode, extra argument, config.txt line edits) on the
+# reference's own inputs
+HARMONIC_BENDING = {14: "0\t0.00001 1\t0.01\t// constrainMode; tolerance; bendType 1 (harmonic)"}
 REF_TEST_CASES = [
-    ("ref_test_inputs_parsed", "inputs", None),
+    ("ref_test_inputs_parsed", "inputs", None, None),
+    ("ref_test_bonded_amoeba", "forces", "bond,angle,ub", None),
+    ("ref_test_bonded_harmonic", "forces", "bond,angle,ub", HARMONIC_BENDING),
 ]
 
 
@@ -215,11 +228,11 @@ def main(only=None):
             continue
         s, meta, arrays = run(spec, mode, extra, cfg)
         save(name, spec, cfg, mode, extra, s, meta, arrays)
-    for name, mode, extra in REF_TEST_CASES:
+    for name, mode, extra, edits in REF_TEST_CASES:
         if only and name not in only:
             continue
-        meta, arrays = run_ref_test_inputs(mode, extra)
-        save_ref_test(name, mode, extra, meta, arrays)
+        meta, arrays = run_ref_test_inputs(mode, extra, edits)
+        save_ref_test(name, mode, extra, edits, meta, arrays)
 
 
 if __name__ == "__main__":

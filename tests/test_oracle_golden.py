@@ -5,7 +5,9 @@ and is in practice exact."""
 import numpy as np
 import pytest
 
-from tests.helpers import Fixture, pair_set, rel, rel_rms
+import os
+
+from tests.helpers import GOLDEN, Fixture, pair_set, rel, rel_rms
 
 TOL = 1e-12
 
@@ -194,3 +196,25 @@ def test_induction_energy_force_terms(oracle_mod, name):
     assert rel_rms(r["long_virial"], fx.ref["indf_long_virial"]) <= TOL
     for k, mk in INDF_KEYS:
         assert abs(r[k] - fx.meta[mk]) <= TOL * max(abs(fx.meta[mk]), 1e-12), k
+
+
+@pytest.mark.parametrize("bending", ["amoeba", "harmonic"])
+def test_bonded_terms_match_reference(bending):
+    """BondForce, AngleForce and UreyBradleyForce on the reference's own
+    example inputs (flexible polarizable water, 500 molecules), bit for bit."""
+    import json
+
+    from oracle.oracle import Oracle
+    from swinmd_amd import inputs
+    cfg, _, s, _ = inputs.load_inputs(os.path.join(GOLDEN, "ref_test_inputs"))
+    g = np.load(os.path.join(GOLDEN, f"ref_test_bonded_{bending}.npz"))
+    meta = json.loads(str(g["meta"]))
+    pot = 2 if bending == "amoeba" else 1
+    o = Oracle(s)
+    got = {"bond": o.bond(pot), "angle": o.angle(pot), "ub": o.urey_bradley(cfg.get_kl_ub(), cfg.get_l0_ub())}
+    for term, r in got.items():
+        np.testing.assert_array_equal(r["force"].ravel(), g[f"ref_{term}_force"])
+        np.testing.assert_array_equal(r["virial"], g[f"ref_{term}_virial"])
+        assert r["energy"] == meta[f"{term}_energy"]
+        assert r["deriv"] == meta.get(f"{term}_deriv", 0.0)
+        assert r["deriv2"] == meta.get(f"{term}_deriv2", 0.0)
