@@ -13,6 +13,7 @@
 //         ref_harness <dir> md <nsteps>
 //         ref_harness <dir> time <nsteps>
 //         ref_harness <dir> time_polar <reps>
+//         ref_harness <dir> lf_forces
 // <dir> holds config.txt, sysData.in and coordinate.in in the reference's
 // formats; outputs (*.bin raw little-endian arrays, meta.txt) are written there.
 //
@@ -544,6 +545,61 @@ int main(int argc, char** argv) {
         write_bin("polar_force", F.data(), F.size() * 8);
         std::printf("{\"reps\": %d, \"seconds\": %.6f}\n", reps,
                     std::chrono::duration<double>(t1 - t0).count());
+    } else if (mode == "lf_forces") {
+        // One force evaluation as LFIntegrator::run performs it
+        // (src/integrators/LFIntegrator.cpp:382-557), with the terms
+        // Integrator::initialise creates for this config
+        // (src/Integrator.cpp:111-197): vdW, then Ewald or multipoles (+
+        // induction), long-range virial added, then bond, angle and
+        // Urey-Bradley.  Wolf and constrained-bond configs are not handled.
+        for (int m = 0; m < ens->nMols; ++m) ens->molecules[m].mass_center();
+        zero_forces(ens);
+        Force* vdw = cfg->compute_buffamoeba() ? (Force*)make_zeroed<BuffvdWForce>(ens)
+                                               : (Force*)make_zeroed<LJForce>(ens);
+        vdw->compute();
+        if (cfg->compute_ewald_force()) {
+            EwaldForce* ew = make_zeroed<EwaldForce>(ens);
+            ew->compute();
+            ew->compute_long();
+        } else if (cfg->compute_multipol()) {
+            Multipole* mp = make_zeroed<Multipole>(ens);
+            mp->compute();
+            mp->compute_long();
+            if (cfg->compute_induction()) {
+                Induction* ind = make_zeroed<Induction>(ens);
+                ind->compute();
+                ind->compute_long();
+            }
+        }
+        for (int j = 0; j < 9; ++j) ens->virial[j] += ens->longVirial[j];
+        const bool flexible = !cfg->is_constraint_on();
+        if (flexible && ens->nBonds > 0) make_zeroed<BondForce>(ens)->compute();
+        if (flexible && ens->nAngles > 0) make_zeroed<AngleForce>(ens)->compute();
+        if (cfg->compute_harmonic_ub()) make_zeroed<UreyBradleyForce>(ens)->compute();
+        Energy& E = ens->myEnergy;
+        const double induc = E.realPolEnergy + E.longPolEnergy + E.correctPolEnergy + E.molCorrectPolEnergy +
+                             E.surfacePolEnergy;
+        const double tot = E.ljEnergy + E.realEnergy + E.longEnergy + E.molCorrectEnergy + E.surfCorrectEnergy +
+                           E.correctEnergy + E.angleEnergy + E.bondEnergy + E.ubEnergy + induc + E.TBEnergy;
+        meta.put("ljEnergy", E.ljEnergy);
+        meta.put("realEnergy", E.realEnergy);
+        meta.put("longEnergy", E.longEnergy);
+        meta.put("correctEnergy", E.correctEnergy);
+        meta.put("molCorrectEnergy", E.molCorrectEnergy);
+        meta.put("surfCorrectEnergy", E.surfCorrectEnergy);
+        meta.put("realPolEnergy", E.realPolEnergy);
+        meta.put("longPolEnergy", E.longPolEnergy);
+        meta.put("correctPolEnergy", E.correctPolEnergy);
+        meta.put("molCorrectPolEnergy", E.molCorrectPolEnergy);
+        meta.put("surfacePolEnergy", E.surfacePolEnergy);
+        meta.put("bondEnergy", E.bondEnergy);
+        meta.put("angleEnergy", E.angleEnergy);
+        meta.put("ubEnergy", E.ubEnergy);
+        meta.put("inducEnergy", induc);
+        meta.put("totEnergy", tot);
+        std::vector<double> F = forces_of(ens);
+        write_bin("lf_force", F.data(), F.size() * 8);
+        write_bin("lf_virial", ens->virial, 9 * sizeof(double));
     } else if (mode == "md" || mode == "time") {
         const int nsteps = argc > 3 ? std::atoi(argv[3]) : 1;
         // The reference never initialises Molecule::massCenter before the

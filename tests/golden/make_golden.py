@@ -108,6 +108,7 @@ REF_TEST_CASES = [
     ("ref_test_inputs_parsed", "inputs", None, None),
     ("ref_test_bonded_amoeba", "forces", "bond,angle,ub", None),
     ("ref_test_bonded_harmonic", "forces", "bond,angle,ub", HARMONIC_BENDING),
+    ("ref_test_lf_forces", "lf_forces", None, None),
 ]
 
 
