@@ -346,11 +346,15 @@ typedef struct {
     smd_integrator_params integ;
     int32_t use_cells;
     int32_t long_steps;     /* longElectTsFrequency */
-    int32_t lustig;         /* also evaluate the Lustig volume derivatives */
+    int32_t lustig;         /* config computeLustig: the LJ volume derivative enters
+                               myAvLus.P (the Ewald terms' derivatives are always
+                               evaluated, as the reference does) */
     int32_t prec;           /* smd_prec of the pair and k-space math */
     const double* sigma2;   /* LJ table [n_types^2] */
     const double* eps4;
     int32_t use_graph;      /* replay each step as one captured hipGraph */
+    int64_t total_steps;    /* config nSteps (Ensemble::totsteps): the stress sums
+                               take the steps after the first 15% of it */
 } smd_md_params;
 
 typedef struct {
@@ -367,6 +371,28 @@ int smd_md_init(smd_ctx* ctx, const smd_md_params* p);
  * Verlet trigger and rebuild, drift removal, LJ + Ewald, SHAKE, kinetics, NH
  * friction, molecule rewrap.  Work stays on the device. */
 int smd_md_run(smd_ctx* ctx, int64_t nsteps, smd_md_report* out);
+
+/* The last step's pressure tensors and the stress sums since smd_md_init:
+ * Integrator::computePressure (src/Integrator.cpp:205-270) with the
+ * shake-virial correction (src/integrators/LFIntegrator.cpp:805-814),
+ * myAvLus.P (:730-743) and the atomStress/molStress sums of
+ * Ensemble::accumulate (src/Ensemble.cpp:814-894), each 5 values: pressure,
+ * shearStress, antiShearStress, firstNormStress, secondNormStress.  virial is
+ * Ensemble::virial as the reference holds it (its long-range part without
+ * the ZZ component, which EwaldForce::compute_long never copies,
+ * src/forces/EwaldForce.cpp:194). */
+typedef struct {
+    double virial[9], mol_virial[9];
+    double atom_pressure[9], mol_pressure[9];
+    double mol_pt;
+    int64_t counter, samples;
+    double atom_stress[5], mol_stress[5];
+    /* the last step's myLustig terms that enter mol_pt: ljdUlus, realDeriv,
+     * longDeriv, molDeriv, surfDeriv */
+    double lj_dudv, real_deriv, long_deriv, mol_deriv, surf_deriv;
+} smd_md_pressure_out;
+
+int smd_md_pressure(smd_ctx* ctx, smd_md_pressure_out* out);
 
 /* ---- measurement -------------------------------------------------------- */
 /* When enabled, every launch of the named kernel families is bracketed by HIP

@@ -2657,3 +2657,40 @@ int orc_ub(const orc_sys *s, double kl, double l0, double *force, double out[12]
     for (int k = 0; k < 9; ++k) out[3 + k] = vir[k];
     return 0;
 }
+
+/* ========================================================================== */
+/* pressure tensor sums                                                       */
+/* ========================================================================== */
+
+void orc_pressure_sums(const orc_sys *s, const double *pos, const double *vel, const double *force,
+                       const double *com, double mc[9], double ka[9], double km[9]) {
+    /* src/Integrator.cpp:215-261 */
+    for (int k = 0; k < 9; ++k) mc[k] = ka[k] = km[k] = 0.0;
+    for (int m = 0; m < s->n_mols; ++m) {
+        const v3 R = v3_ld(com + 3 * m);
+        for (int i = s->mol_start[m]; i < s->mol_start[m + 1]; ++i) {
+            const v3 f = v3_ld(force + 3 * i), r = v3_ld(pos + 3 * i);
+            mc[XX] += f.x * (r.x - R.x); mc[XY] += f.y * (r.x - R.x); mc[XZ] += f.z * (r.x - R.x);
+            mc[YX] += f.x * (r.y - R.y); mc[YY] += f.y * (r.y - R.y); mc[YZ] += f.z * (r.y - R.y);
+            mc[ZX] += f.x * (r.z - R.z); mc[ZY] += f.y * (r.z - R.z); mc[ZZ] += f.z * (r.z - R.z);
+        }
+    }
+    for (int i = 0; i < s->n_atoms; ++i) {
+        const double m = s->mass[i];
+        const v3 p = v3_scale(m, v3_ld(vel + 3 * i));   /* momentum = velocity*mass */
+        ka[XX] += p.x * p.x / m; ka[XY] += p.x * p.y / m; ka[XZ] += p.x * p.z / m;
+        ka[YY] += p.y * p.y / m; ka[YZ] += p.y * p.z / m; ka[ZZ] += p.z * p.z / m;
+    }
+    ka[YX] = ka[XY]; ka[ZX] = ka[XZ]; ka[ZY] = ka[YZ];
+    for (int m = 0; m < s->n_mols; ++m) {
+        v3 P = {0.0, 0.0, 0.0};
+        double M = 0.0;
+        for (int i = s->mol_start[m]; i < s->mol_start[m + 1]; ++i) {
+            P = v3_add(P, v3_scale(s->mass[i], v3_ld(vel + 3 * i)));
+            M += s->mass[i];
+        }
+        km[XX] += P.x * P.x / M; km[XY] += P.x * P.y / M; km[XZ] += P.x * P.z / M;
+        km[YY] += P.y * P.y / M; km[YZ] += P.y * P.z / M; km[ZZ] += P.z * P.z / M;
+    }
+    km[YX] = km[XY]; km[ZX] = km[XZ]; km[ZY] = km[YZ];
+}

@@ -328,6 +328,15 @@ int orc_angle(const orc_sys *s, int64_t na, const int32_t *angles, const double 
  * between atoms 0 and 2 of every molecule. */
 int orc_ub(const orc_sys *s, double kl, double l0, double *force, double out[12]);
 
+/* ---- pressure tensor ------------------------------------------------------
+ * The sums of Integrator::computePressure (src/Integrator.cpp:205-270) for
+ * one state: mc = sum f (x) (r - R_mol) with mc[XY] = f.y (r.x - R.x); ka =
+ * sum over atoms of p p / m and km = sum over molecules of P P / M, full 3x3
+ * with the symmetric components copied as the reference does; p = v m, P the
+ * molecule's summed momentum, R its mass centre (com [3*n_mols]). */
+void orc_pressure_sums(const orc_sys *s, const double *pos, const double *vel, const double *force,
+                       const double *com, double mc[9], double ka[9], double km[9]);
+
 #ifdef __cplusplus
 }
 #endif

@@ -157,6 +157,7 @@ def lib():
         _lib.orc_angle.argtypes = [P(OrcSys), C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p,
                                    C.c_void_p]
         _lib.orc_ub.argtypes = [P(OrcSys), C.c_double, C.c_double, C.c_void_p, C.c_void_p]
+        _lib.orc_pressure_sums.argtypes = [P(OrcSys)] + [C.c_void_p] * 7
     return _lib
 
 
@@ -428,6 +429,16 @@ class Oracle:
         f, out = np.zeros((self.s.n_atoms, 3)), np.zeros(12)
         rc = lib().orc_ub(C.byref(self._sys), kl, l0, _p(f), _p(out))
         return self._bonded_out(rc, "orc_ub", out, f)
+
+    def pressure_sums(self, vel, force, com=None):
+        """Integrator::computePressure's sums: (mc, ka, km), each 3x3 flat."""
+        vel = np.ascontiguousarray(vel, dtype=np.float64)
+        force = np.ascontiguousarray(force, dtype=np.float64)
+        com = self.com if com is None else np.ascontiguousarray(com, dtype=np.float64)
+        mc, ka, km = np.zeros(9), np.zeros(9), np.zeros(9)
+        lib().orc_pressure_sums(C.byref(self._sys), _p(self.pos), _p(vel), _p(force), _p(com), _p(mc), _p(ka),
+                                _p(km))
+        return mc, ka, km
 
     # ---- integrator ------------------------------------------------------
     def init_velocity(self, temperature, seed, per_atom=True, ndf=None):

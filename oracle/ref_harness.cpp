@@ -624,6 +624,28 @@ int main(int argc, char** argv) {
             dump_state(ens, "end_");
             std::vector<double> F = forces_of(ens);
             write_bin("end_force", F.data(), F.size() * 8);
+            // the last step's computePressure (src/Integrator.cpp:205-270) with
+            // the shake-virial correction (LFIntegrator.cpp:805-814), and the
+            // stress sums of Ensemble::accumulate (src/Ensemble.cpp:814-894)
+            write_bin("end_virial", ens->virial, 9 * sizeof(double));
+            write_bin("end_mol_virial", ens->molVirial, 9 * sizeof(double));
+            write_bin("end_atom_pressure", ens->atomPressure, 9 * sizeof(double));
+            write_bin("end_mol_pressure", ens->molPressure, 9 * sizeof(double));
+            const double st[10] = {ens->atomStress.pressure, ens->atomStress.shearStress,
+                                   ens->atomStress.antiShearStress, ens->atomStress.firstNormStress,
+                                   ens->atomStress.secondNormStress, ens->molStress.pressure,
+                                   ens->molStress.shearStress, ens->molStress.antiShearStress,
+                                   ens->molStress.firstNormStress, ens->molStress.secondNormStress};
+            write_bin("stress_sums", st, sizeof st);
+            meta.put("acc_counter", ens->counter);
+            meta.put("acc_luscount", ens->luscount);
+            meta.put("totsteps", ens->totsteps);
+            meta.put("molPt", ens->myAvLus.P);
+            meta.put("ljdUlus", ens->myLustig.ljdUlus);
+            meta.put("realDeriv", ens->myLustig.realDeriv);
+            meta.put("longDeriv", ens->myLustig.longDeriv);
+            meta.put("molDeriv", ens->myLustig.molDeriv);
+            meta.put("surfDeriv", ens->myLustig.surfDeriv);
         }
         std::printf("{\"steps\": %d, \"seconds\": %.6f}\n", nsteps, secs);
     } else {

@@ -70,7 +70,15 @@ class MdParams(C.Structure):
     _fields_ = [("cutoff", C.c_double), ("cut_buff", C.c_double), ("ewald", EwaldParams),
                 ("integ", IntegParams), ("use_cells", C.c_int32), ("long_steps", C.c_int32),
                 ("lustig", C.c_int32), ("prec", C.c_int32), ("sigma2", C.c_void_p), ("eps4", C.c_void_p),
-                ("use_graph", C.c_int32)]
+                ("use_graph", C.c_int32), ("total_steps", C.c_int64)]
+
+
+class MdPressure(C.Structure):
+    _fields_ = [("virial", C.c_double * 9), ("mol_virial", C.c_double * 9), ("atom_pressure", C.c_double * 9),
+                ("mol_pressure", C.c_double * 9), ("mol_pt", C.c_double), ("counter", C.c_int64),
+                ("samples", C.c_int64), ("atom_stress", C.c_double * 5), ("mol_stress", C.c_double * 5),
+                ("lj_dudv", C.c_double), ("real_deriv", C.c_double), ("long_deriv", C.c_double),
+                ("mol_deriv", C.c_double), ("surf_deriv", C.c_double)]
 
 
 class MdReport(C.Structure):
@@ -232,6 +240,7 @@ SIGNATURES = {
                                _P, C.POINTER(C.c_double)]),
     "smd_md_init": (C.c_int, [_P, C.POINTER(MdParams)]),
     "smd_md_run": (C.c_int, [_P, C.c_int64, C.POINTER(MdReport)]),
+    "smd_md_pressure": (C.c_int, [_P, C.POINTER(MdPressure)]),
     "smd_profile_enable": (C.c_int, [_P, C.c_int]),
     "smd_profile_get": (C.c_int, [_P, C.c_char_p, C.POINTER(C.c_double), C.POINTER(C.c_int64)]),
     "smd_md_time_family": (C.c_int, [_P, C.c_char_p, C.c_int, C.POINTER(C.c_double)]),

@@ -71,7 +71,7 @@ static void release(Ctx* c) {
                     c->d_mp_terms, c->mp_ks.kvec, c->mp_ks.kmod2, c->mp_ks.expk, c->mp_ks.part, c->mp_ks.kf,
                     c->mp_ks.fpart, c->ind_ks.kvec, c->ind_ks.kmod2, c->ind_ks.expk, c->ind_ks.part,
                     c->ind_ks.kf, c->ind_ks.fpart, c->d_pack, c->d_bd_bonds, c->d_bd_angles,
-                    c->d_bd_bond_off, c->d_bd_angle_off, c->d_bd_par, c->d_bd_out};
+                    c->d_bd_bond_off, c->d_bd_angle_off, c->d_bd_par, c->d_bd_out, c->d_stress};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     if (c->h_pinned) (void)hipHostFree(c->h_pinned);

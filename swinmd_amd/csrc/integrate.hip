@@ -41,6 +41,10 @@ struct IntegArgs {
     int fnh_on_device;
     int md_step;      // inside the captured MD step: drift removal and flag reset in SHAKE
     double ndf;       // thermostat degrees of freedom (MD step)
+    // MD step pressure and stress sums
+    int surface, use_constraint, lustig;
+    double* stress;          // [12] (Ctx::d_stress)
+    double stress_after;     // samples count once the call counter exceeds 0.15 totsteps
 };
 
 // Head of the MD step: the Verlet trigger (any |displacement|^2 > cutBuff^2/4,
@@ -231,8 +235,80 @@ __device__ __forceinline__ void nh_friction(const IntegArgs& a, double ke_atom, 
 // Molecular centres, momenta and kinetic sums (:646-686).  In the MD step the
 // last block to finish also applies the thermostat update and clears the
 // drift sums for the next step (its counter lives in flag[2]).
+// Integrator::computePressure (src/Integrator.cpp:205-270), the shake-virial
+// correction of LFIntegrator::run (:805-814), myAvLus.P (:730-743) and the
+// stress sums of Ensemble::accumulate (src/Ensemble.cpp:814-894) for the MD
+// step, from the step's sums; one thread.  Ensemble::virial is the force
+// terms' virials subtracted, the long-range virial added without its ZZ
+// component (EwaldForce::compute_long copies i < ZZ, EwaldForce.cpp:194).
+__device__ void md_pressure(const IntegArgs& a, volatile const double* t) {
+    const double V = a.box.L[0] * a.box.L[1] * a.box.L[2];
+    const int sym[9] = {0, 1, 2, 1, 3, 4, 2, 4, 5};   // (xx xy xz yy yz zz) -> 3x3
+    double vir[9], atomP[9], molP[9];
+    const double dt2 = a.dt * a.dt;
+    for (int k = 0; k < 9; ++k) {
+        vir[k] = -t[T_LJ_VIR + k] - t[T_EW_VIR + k] - (a.surface ? t[T_SU_VIR + k] : 0.0) +
+                 (k < 8 ? t[T_RC_VIR + k] : 0.0);
+        const double molv = vir[k] - t[T_PR_MC + k];
+        atomP[k] = (t[T_PR_KA + sym[k]] + vir[k]) / V;
+        molP[k] = (t[T_PR_KM + sym[k]] + molv) / V;
+        a.terms[T_PR_MOLVIR + k] = molv;
+    }
+    for (int k = 0; k < 9; ++k) {
+        const double sh = t[T_SH_VIR + k] / dt2;
+        vir[k] += sh;
+        atomP[k] += sh / V;
+        a.terms[T_PR_VIR + k] = vir[k];
+        a.terms[T_PR_ATOM + k] = atomP[k];
+        a.terms[T_PR_MOL + k] = molP[k];
+    }
+    // molVirt with the terms this step has: LJ, Ewald real, intramolecular,
+    // surface and reciprocal volume derivatives (LFIntegrator.cpp:730-733)
+    // LJForce sets ljdUlus only with computeLustig (src/forces/LJForce.cpp:
+    // 238-247); EwaldForce's derivatives are always formed, its molDeriv only
+    // for unconstrained molecules (EwaldForce.cpp:780-784)
+    const double ljdU = a.lustig ? -t[T_LJ_DUDV] / (3.0 * V) : 0.0;
+    double surf = 0.0;
+    if (a.surface) {   // EwaldForce::surfDipoleCorrect (:854-864)
+        const double vr = 1.0 / V, coeff = 2.0 * PI * vr / 3.0;
+        const double rr = t[T_SU_QR] * t[T_SU_QR] + t[T_SU_QR + 1] * t[T_SU_QR + 1] + t[T_SU_QR + 2] * t[T_SU_QR + 2];
+        const double rR = t[T_SU_QR] * t[T_SU_QRR] + t[T_SU_QR + 1] * t[T_SU_QRR + 1] + t[T_SU_QR + 2] * t[T_SU_QRR + 2];
+        surf = a.use_constraint ? -coeff * vr * rr + 2.0 * coeff * vr * rR / 3.0 : -coeff * vr * rr / 3.0;
+    }
+    const double molDeriv = a.use_constraint ? 0.0 : t[T_IN_D1];
+    // reciprocal_term's longderiv, with the force loop's molecular part for
+    // constrained molecules (EwaldForce.cpp:671-676)
+    const double longDeriv = t[T_RC_D1] + (a.use_constraint ? t[T_RC_FF] : 0.0);
+    const double molVirt = ljdU + surf + molDeriv + (longDeriv + t[T_EW_D1]) / (3.0 * V);
+    const double molPt = (a.use_constraint ? t[T_KE_MOL] : t[T_KE_ATOM]) / (3.0 * V) - molVirt;
+    a.terms[T_PR_MOLPT] = molPt;
+    a.terms[T_PR_LUS] = ljdU;
+    a.terms[T_PR_LUS + 1] = t[T_EW_D1];
+    a.terms[T_PR_LUS + 2] = longDeriv;
+    a.terms[T_PR_LUS + 3] = molDeriv;
+    a.terms[T_PR_LUS + 4] = surf;
+    double* st = a.stress;
+    st[0] += 1.0;
+    if (st[0] > a.stress_after) {
+        st[1] += 1.0;
+        st[2] += molPt;
+        st[3] += (atomP[1] + atomP[3]) * 0.5;
+        st[4] += (atomP[3] - atomP[1]) * 0.5;
+        st[5] += (atomP[0] - atomP[4]);
+        st[6] += (atomP[4] - atomP[8]);
+        st[7] += (molP[0] + molP[4] + molP[8]) / 3.0;
+        st[8] += (molP[1] + molP[3]) * 0.5;
+        st[9] += (molP[3] - molP[1]) * 0.5;
+        st[10] += (molP[0] - molP[4]);
+        st[11] += (molP[4] - molP[8]);
+    }
+}
+
 __global__ __launch_bounds__(256) void k_kinetics(IntegArgs a, int rewrap) {
     double v[2] = {0.0, 0.0};
+    double pr[21];   // MD step: sum f (x) (r - R) [9], atom p p / m [6], molecule P P / M [6]
+#pragma unroll
+    for (int k = 0; k < 21; ++k) pr[k] = 0.0;
     const int m = blockIdx.x * blockDim.x + threadIdx.x;
     if (m < a.nmol) {
         const int s = a.mol_start[m], e = a.mol_start[m + 1];
@@ -252,6 +328,25 @@ __global__ __launch_bounds__(256) void k_kinetics(IntegArgs a, int rewrap) {
         }
         for (int d = 0; d < 3; ++d) C[d] = C[d] / M;
         v[1] = (P[0] * P[0] + P[1] * P[1] + P[2] * P[2]) / M;
+        if (a.md_step) {
+            // computePressure's per-molecule sums with the forces of this
+            // step and the new positions, centres and momenta
+            for (int i = s; i < e; ++i) {
+                const double mi = a.mass[i];
+                double p[3], f[3], dr[3];
+                for (int d = 0; d < 3; ++d) {
+                    p[d] = a.vel[3 * i + d] * mi;
+                    f[d] = a.force[3 * i + d];
+                    dr[d] = a.pos[3 * i + d] - C[d];
+                }
+                for (int x = 0; x < 3; ++x)
+                    for (int y = 0; y < 3; ++y) pr[3 * x + y] += f[y] * dr[x];
+                pr[9] += p[0] * p[0] / mi; pr[10] += p[0] * p[1] / mi; pr[11] += p[0] * p[2] / mi;
+                pr[12] += p[1] * p[1] / mi; pr[13] += p[1] * p[2] / mi; pr[14] += p[2] * p[2] / mi;
+            }
+            pr[15] += P[0] * P[0] / M; pr[16] += P[0] * P[1] / M; pr[17] += P[0] * P[2] / M;
+            pr[18] += P[1] * P[1] / M; pr[19] += P[1] * P[2] / M; pr[20] += P[2] * P[2] / M;
+        }
         for (int d = 0; d < 3; ++d) a.mom[3 * m + d] = P[d];
         if (rewrap) {
             double cx = C[0], cy = C[1], cz = C[2];
@@ -264,6 +359,8 @@ __global__ __launch_bounds__(256) void k_kinetics(IntegArgs a, int rewrap) {
     }
     block_add<2>(v, a.terms + T_KE_ATOM);
     if (a.md_step) {
+        __syncthreads();
+        block_add<21>(pr, a.terms + T_PR_MC);
         __shared__ bool last;
         __threadfence();
         __syncthreads();
@@ -273,6 +370,7 @@ __global__ __launch_bounds__(256) void k_kinetics(IntegArgs a, int rewrap) {
             __threadfence();
             volatile const double* t = a.terms;   // sums the other blocks added
             nh_friction(a, t[T_KE_ATOM], t[T_KE_MOL]);
+            md_pressure(a, t);
             for (int k = T_VOM; k <= T_MTOT; ++k) a.terms[k] = 0.0;
             a.flag[2] = 0;
         }
@@ -427,6 +525,11 @@ int kinetics_run(Ctx* c, bool rewrap, const smd_integrator_params* md) {
     if (md) {
         a.md_step = 1;
         a.ndf = md_ndf(c);
+        a.surface = c->md.ewald.surface;
+        a.use_constraint = c->md.ewald.use_constraint;
+        a.lustig = c->md.lustig;
+        a.stress = c->d_stress;
+        a.stress_after = 0.15 * (double)c->md.total_steps;
     }
     ProfScope ps(c, "kinetics");
     hipLaunchKernelGGL(k_kinetics, dim3((c->nmol + 255) / 256), dim3(256), 0, c->stream, a, rewrap ? 1 : 0);
@@ -467,7 +570,7 @@ int md_step_enqueue(Ctx* c, bool do_long) {
         {
             StreamSwap sw(c, c->stream2);
             c->recip_store = true;
-            rc = ewald_recip(c, p.prec, p.lustig);
+            rc = ewald_recip(c, p.prec, 1);
             c->recip_store = false;
         }
         if (rc) return rc;
@@ -475,12 +578,12 @@ int md_step_enqueue(Ctx* c, bool do_long) {
     }
     // the pair kernel is the first force term of the step: it stores forces
     rc = pairs_lj_ewald(c, nullptr, nullptr, p.cutoff, p.ewald.alpha, p.ewald.real_cut, p.ewald.use_constraint,
-                        p.lustig, p.prec, true);
+                        1, p.prec, true);
     if (rc) return rc;
-    if ((rc = ewald_intra(c, p.lustig))) return rc;
+    if ((rc = ewald_intra(c, 1))) return rc;
     if (p.ewald.surface && (rc = ewald_surface(c))) return rc;
     if (fork) SMD_HIP(c, hipStreamWaitEvent(c->stream, c->ev_join, 0));
-    else if (do_long && (rc = ewald_recip(c, p.prec, p.lustig))) return rc;
+    else if (do_long && (rc = ewald_recip(c, p.prec, 1))) return rc;
     c->shake_frecip = fork;
     rc = shake_run(c, &p.integ, true, 0.0, true);   // drift removal, SHAKE (adds d_frecip), clears the flag
     c->shake_frecip = false;
@@ -637,14 +740,16 @@ extern "C" int smd_md_init(smd_ctx* c, const smd_md_params* p) {
     SMD_HIP(c, hipMemsetAsync(c->d_force, 0, sizeof(double) * 3 * c->n, c->stream));
     SMD_HIP(c, hipMemsetAsync(c->d_terms, 0, sizeof(double) * T_COUNT, c->stream));
     if ((rc = pairs_lj_ewald(c, nullptr, nullptr, p->cutoff, p->ewald.alpha, p->ewald.real_cut,
-                             p->ewald.use_constraint, p->lustig, p->prec)))
+                             p->ewald.use_constraint, 1, p->prec)))
         return rc;
-    if ((rc = ewald_intra(c, p->lustig))) return rc;
+    if ((rc = ewald_intra(c, 1))) return rc;
     if (p->ewald.surface && (rc = ewald_surface(c))) return rc;
-    if ((rc = ewald_recip(c, p->prec, p->lustig))) return rc;
+    if ((rc = ewald_recip(c, p->prec, 1))) return rc;
     hipLaunchKernelGGL(k_start_vtm, dim3((c->n + 255) / 256), dim3(256), 0, c->stream, c->n, c->d_vel, c->d_force,
                        c->d_mass, c->d_vtm, p->integ.dt);
     SMD_HIP(c, hipMemsetAsync(c->d_scalars, 0, sizeof(double) * 8, c->stream));
+    if (!c->d_stress) SMD_HIP(c, hipMalloc(&c->d_stress, sizeof(double) * 12));
+    SMD_HIP(c, hipMemsetAsync(c->d_stress, 0, sizeof(double) * 12, c->stream));
     SMD_HIP(c, hipMemsetAsync(c->d_mom, 0, sizeof(double) * 3 * c->nmol, c->stream));
     SMD_HIP(c, hipMemsetAsync(c->d_disp, 0, sizeof(double) * 3 * c->n, c->stream));
     SMD_HIP(c, hipMemsetAsync(c->d_status, 0, sizeof(unsigned), c->stream));
@@ -686,13 +791,13 @@ extern "C" int smd_md_time_family(smd_ctx* c, const char* family, int reps, doub
     const bool pairs_fam = pass == 0;
     if (e == hipSuccess && pairs_fam)
         rc = pairs_lj_ewald(c, nullptr, nullptr, p.cutoff, p.ewald.alpha, p.ewald.real_cut, p.ewald.use_constraint,
-                            p.lustig, p.prec, true);
+                            1, p.prec, true);
     c->pack_reuse = pairs_fam;
     if (e == hipSuccess) e = hipEventRecord(e0, c->stream);
     for (int r = 0; e == hipSuccess && rc == SMD_OK && r < reps; ++r)
-        rc = pass ? ewald_recip(c, p.prec, p.lustig, pass)
+        rc = pass ? ewald_recip(c, p.prec, 1, pass)
                   : pairs_lj_ewald(c, nullptr, nullptr, p.cutoff, p.ewald.alpha, p.ewald.real_cut,
-                                   p.ewald.use_constraint, p.lustig, p.prec, true);
+                                   p.ewald.use_constraint, 1, p.prec, true);
     if (e == hipSuccess && rc == SMD_OK) e = hipEventRecord(e1, c->stream);
     c->pack_reuse = false;
     // restore whatever happened above
@@ -756,15 +861,41 @@ extern "C" int smd_md_run(smd_ctx* c, int64_t nsteps, smd_md_report* out) {
         out->atom_ke2 = t[T_KE_ATOM];
         out->mol_ke2 = t[T_KE_MOL];
         out->fnh = sc[0];
-        const double dt2 = c->md.integ.dt * c->md.integ.dt;
-        for (int k = 0; k < 9; ++k)
-            out->virial[k] = -t[T_LJ_VIR + k] - t[T_EW_VIR + k] + t[T_RC_VIR + k] + t[T_SH_VIR + k] / dt2 -
-                             (c->md.ewald.surface ? t[T_SU_VIR + k] : 0.0);
+        for (int k = 0; k < 9; ++k) out->virial[k] = t[T_PR_VIR + k];
         if (c->md.ewald.surface) {
             const double vol = c->box.L[0] * c->box.L[1] * c->box.L[2];
             const double* qr = t + T_SU_QR;
             out->e_surf = 2.0 * PI * (1.0 / vol) / 3.0 * (qr[0] * qr[0] + qr[1] * qr[1] + qr[2] * qr[2]);
         }
+    }
+    return SMD_OK;
+}
+
+extern "C" int smd_md_pressure(smd_ctx* c, smd_md_pressure_out* out) {
+    if (!c || !out) return SMD_EINVAL;
+    if (!c->md_ready) return fail(c, SMD_ESTATE, "smd_md_init not called");
+    double t[T_COUNT], st[12];
+    int rc = read_terms(c, t);
+    if (rc) return rc;
+    SMD_HIP(c, hipMemcpy(st, c->d_stress, sizeof st, hipMemcpyDeviceToHost));
+    *out = smd_md_pressure_out{};
+    for (int k = 0; k < 9; ++k) {
+        out->virial[k] = t[T_PR_VIR + k];
+        out->mol_virial[k] = t[T_PR_MOLVIR + k];
+        out->atom_pressure[k] = t[T_PR_ATOM + k];
+        out->mol_pressure[k] = t[T_PR_MOL + k];
+    }
+    out->mol_pt = t[T_PR_MOLPT];
+    out->lj_dudv = t[T_PR_LUS];
+    out->real_deriv = t[T_PR_LUS + 1];
+    out->long_deriv = t[T_PR_LUS + 2];
+    out->mol_deriv = t[T_PR_LUS + 3];
+    out->surf_deriv = t[T_PR_LUS + 4];
+    out->counter = (int64_t)st[0];
+    out->samples = (int64_t)st[1];
+    for (int k = 0; k < 5; ++k) {
+        out->atom_stress[k] = st[2 + k];
+        out->mol_stress[k] = st[7 + k];
     }
     return SMD_OK;
 }

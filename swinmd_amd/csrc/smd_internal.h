@@ -41,7 +41,13 @@ enum TermSlot {
     T_RC_E = T_SU_VIR + 9, T_RC_D1, T_RC_D2, T_RC_VIR,                            // VIR: 9
     T_RC_FF = T_RC_VIR + 9, T_RC_FFK2, T_RC_D2UV1, T_RC_D2URR,
     T_SH_VIR, T_KE_ATOM = T_SH_VIR + 9, T_KE_MOL, T_VOM, T_MTOT = T_VOM + 3,
-    T_COUNT
+    // MD step pressure (Integrator::computePressure): per-molecule sums, then
+    // the step's tensors.  MC: sum f (x) (r - R_mol); KA, KM: atom and
+    // molecule p p / m (xx xy xz yy yz zz)
+    T_PR_MC, T_PR_KA = T_PR_MC + 9, T_PR_KM = T_PR_KA + 6, T_PR_VIR = T_PR_KM + 6,
+    T_PR_ATOM = T_PR_VIR + 9, T_PR_MOL = T_PR_ATOM + 9, T_PR_MOLVIR = T_PR_MOL + 9, T_PR_MOLPT = T_PR_MOLVIR + 9,
+    T_PR_LUS,   // [5] ljdUlus, realDeriv, longDeriv, molDeriv, surfDeriv of the step
+    T_COUNT = T_PR_LUS + 5
 };
 
 // Stream-ordered event pairs for the profiling families.
@@ -197,6 +203,9 @@ struct Ctx {
     int pair_lanes = 0;           // lanes per atom row in the pair kernels (0: by precision)
     int pair_packed = 1;
     bool pack_reuse = false;
+    // Ensemble::accumulate stress sums of the MD step: [0] counter,
+    // [1] samples taken, [2..6] atomStress, [7..11] molStress
+    double* d_stress = nullptr;
     // bonded terms of flexible molecules (bonded.hip)
     bool bonded_ready = false, bd_ub_ok = false;
     int bd_potential = 1, bd_n_bond_types = 0;

@@ -381,7 +381,7 @@ class WaterMD:
 
     def __init__(self, ctx: Context, *, cutoff, cut_buff, alpha, kcut, dt, temperature, coupl=0.1, tol=1e-8,
                  max_count=10000, use_cells=True, long_steps=1, lustig=True, prec=L.PREC_FP64,
-                 atom_thermo=True, surface=False, use_graph=True):
+                 atom_thermo=True, surface=False, use_graph=True, total_steps=0):
         self.ctx = ctx
         b2 = _f64(ctx.system.constraint_lengths2())
         ctx.check(ctx.lib.smd_set_constraints(ctx.h, _ptr(b2), ctx.nmol))
@@ -393,6 +393,7 @@ class WaterMD:
         p.use_cells, p.long_steps, p.lustig, p.prec = int(use_cells), long_steps, int(lustig), prec
         p.sigma2, p.eps4 = _ptr(self._sig2), _ptr(self._eps4)
         p.use_graph = int(use_graph)
+        p.total_steps = int(total_steps)
         self.params = p
 
     def init(self):
@@ -405,6 +406,20 @@ class WaterMD:
         d = {name: getattr(r, name) for name, _ in L.MdReport._fields_ if name != "virial"}
         d["virial"] = np.array(r.virial[:])
         return d
+
+    def pressure(self):
+        """The last step's Ensemble::virial, molVirial, atomPressure and
+        molPressure (3x3, row-major XX XY XZ ...), myAvLus.P, and the
+        Ensemble::accumulate stress sums (pressure, shear, antiShear,
+        firstNorm, secondNorm; atom and molecular) with their sample count."""
+        o = L.MdPressure()
+        self.ctx.check(self.ctx.lib.smd_md_pressure(self.ctx.h, C.byref(o)))
+        return {"virial": np.array(o.virial[:]), "mol_virial": np.array(o.mol_virial[:]),
+                "atom_pressure": np.array(o.atom_pressure[:]), "mol_pressure": np.array(o.mol_pressure[:]),
+                "mol_pt": o.mol_pt, "counter": o.counter, "samples": o.samples,
+                "atom_stress": np.array(o.atom_stress[:]), "mol_stress": np.array(o.mol_stress[:]),
+                "lj_dudv": o.lj_dudv, "real_deriv": o.real_deriv, "long_deriv": o.long_deriv,
+                "mol_deriv": o.mol_deriv, "surf_deriv": o.surf_deriv}
 
 
 def set_multipoles(ctx: Context):

@@ -152,7 +152,8 @@ def test_md_matches_reference(name):
     ctx.set_velocities(fx.vec("init_vel"))
     md = WaterMD(ctx, cutoff=c["cutoff"], cut_buff=c["cut_buff"], alpha=c["alpha"], kcut=c["kcut"], dt=c["dt"],
                  temperature=c["temperature"], coupl=c["coupl"], tol=c["shake_tol"], max_count=c["max_count"],
-                 atom_thermo=c.get("atom_thermo", True), use_graph=False)
+                 atom_thermo=c.get("atom_thermo", True), use_graph=False, total_steps=int(fx.meta["totsteps"]),
+                 lustig=bool(c.get("lustig", False)))
     md.init()
     rep = md.run(int(fx.extra))
     pos, vel = ctx.positions(), ctx.velocities()
@@ -166,6 +167,21 @@ def test_md_matches_reference(name):
     o = O.Oracle(fx.system)
     r = O.md_from_cfg(o, c, int(fx.extra))
     assert rep["list_updates"] == r["list_updates"]
+    # the last step's computePressure with the shake-virial correction, and the
+    # Ensemble::accumulate stress sums over the run
+    pr = md.pressure()
+    for key in ("virial", "mol_virial", "atom_pressure", "mol_pressure"):
+        want = fx.ref["end_" + key]
+        assert np.max(np.abs(pr[key] - want)) <= 1e-8 * np.max(np.abs(want)), key
+    np.testing.assert_array_equal(rep["virial"], pr["virial"])
+    for key, mkey in (("lj_dudv", "ljdUlus"), ("real_deriv", "realDeriv"), ("long_deriv", "longDeriv"),
+                      ("mol_deriv", "molDeriv"), ("surf_deriv", "surfDeriv")):
+        assert abs(pr[key] - fx.meta[mkey]) <= 1e-8 * max(abs(fx.meta[mkey]), 1e-6), (key, pr[key], fx.meta[mkey])
+    assert rel(pr["mol_pt"], fx.meta["molPt"]) <= 1e-8
+    assert pr["counter"] == fx.meta["acc_counter"] and pr["samples"] == fx.meta["acc_luscount"]
+    st = np.concatenate([pr["atom_stress"], pr["mol_stress"]])
+    want = fx.ref["stress_sums"]
+    assert np.max(np.abs(st - want)) <= 1e-8 * np.max(np.abs(want))
 
 
 def test_md_graph_replay_equals_eager():

@@ -218,3 +218,28 @@ def test_bonded_terms_match_reference(bending):
         assert r["energy"] == meta[f"{term}_energy"]
         assert r["deriv"] == meta.get(f"{term}_deriv", 0.0)
         assert r["deriv2"] == meta.get(f"{term}_deriv2", 0.0)
+
+
+@pytest.mark.parametrize("name", ["water512_md10", "water216_md20_moltherm"])
+def test_pressure_tensors_of_the_last_md_step(name):
+    """computePressure after N reference steps: atomPressure V - sum p p / m is
+    Ensemble::virial (shake virial included) and molPressure V - sum P P / M
+    is molVirial, with the kinetic sums restated by the oracle from the
+    reference's end velocities (the kinetic tensors, their symmetric copies
+    and the volume pinned against the compiled reference)."""
+    from oracle.oracle import Oracle
+    fx = Fixture(name)
+    V = fx.meta["volume"]
+    o = Oracle(fx.system, pos=fx.vec("end_pos"))
+    mc, ka, km = o.pressure_sums(fx.vec("end_vel"), fx.vec("end_force"))
+    ap, mp = fx.ref["end_atom_pressure"], fx.ref["end_mol_pressure"]
+    vir, mvir = fx.ref["end_virial"], fx.ref["end_mol_virial"]
+    assert np.max(np.abs(ap * V - ka - vir)) <= 1e-9 * np.max(np.abs(vir))
+    assert np.max(np.abs(mp * V - km - mvir)) <= 1e-9 * np.max(np.abs(mvir))
+    # the molecular correction is what separates the two virials, up to the
+    # shake virial, which the reference adds to the atomic one only: the
+    # difference is sum f (x) (r - R) plus a constraint term that is nonzero
+    assert np.max(np.abs((vir - mvir) - mc)) > 0.0
+    # stress sums: molStress.pressure of the last sample is its trace / 3
+    st = fx.ref["stress_sums"]
+    assert st.shape == (10,) and fx.meta["acc_luscount"] == int(fx.extra)
