@@ -394,6 +394,21 @@ typedef struct {
 
 int smd_md_pressure(smd_ctx* ctx, smd_md_pressure_out* out);
 
+/* ---- checkpoint state ---------------------------------------------------
+ * Atom::realPos, the unwrapped positions of the restart file: set with the
+ * positions by smd_set_positions (as Ensemble::read_coordinate does), then
+ * advanced by every integration step and untouched by the molecule rewrap;
+ * smd_set_real_positions overrides them (Ensemble::read_realPos,
+ * src/Ensemble.cpp:672-710). */
+int smd_set_real_positions(smd_ctx* ctx, const double* r, int64_t n_atoms);
+int smd_get_real_positions(smd_ctx* ctx, double* r, int64_t n_atoms);
+/* The Nose-Hoover friction of the MD step (fnh, fnhmi of LFIntegrator::run
+ * :688-709).  The reference's restart files do not keep it; a lossless
+ * continuation restores it with the set call.  No counterpart in the
+ * reference. */
+int smd_md_get_thermostat(smd_ctx* ctx, double* fnh, double* fnh_minus);
+int smd_md_set_thermostat(smd_ctx* ctx, double fnh, double fnh_minus);
+
 /* ---- measurement -------------------------------------------------------- */
 /* When enabled, every launch of the named kernel families is bracketed by HIP
  * events on the context stream. */
@@ -570,6 +585,36 @@ int smd_read_xyz(const char* path, int64_t n_atoms, int32_t skip_lines, double* 
  * velocities.  Each output is [n_atoms][3]. */
 int smd_read_state(const char* path, int64_t n_atoms, double* pos, double* real_pos, double* vel, char* err,
                    size_t errlen);
+
+/* ---- output files -------------------------------------------------------
+ * Host-only writers of the reference's restart and pressure files, through
+ * std::ostream with the reference's manipulators.  precision 0 keeps the
+ * stream default (6 significant digits, what the reference writes); 17 makes
+ * the restart lossless.  append 0 truncates the file first.  SMD_EIO when the
+ * file cannot be written. */
+/* Integrator::write_state (src/Integrator.cpp:275-302): the file
+ * smd_read_state reads. */
+int smd_write_state(const char* path, int64_t step, int64_t n_atoms, const double* pos, const double* real_pos,
+                    const double* vel, int32_t precision, char* err, size_t errlen);
+/* NHIntegrator::write_vtMinus (src/integrators/NHIntegrator.cpp:1782-1803):
+ * a title line, then a coordinate block (read back by smd_read_xyz with
+ * skip_lines 1, as read_vtMinus does, :1960-1994).  The reference writes the
+ * atoms' velocities there; pass whichever array is to be kept. */
+int smd_write_vt_minus(const char* path, int64_t step, int64_t n_atoms, const double* v, int32_t precision,
+                       char* err, size_t errlen);
+/* Ensemble::write_press_res (src/Ensemble.cpp:1153-1160): one line of the
+ * atomic and molecular pressure, shear, antisymmetric and normal stresses. */
+int smd_write_press_res(const char* path, int append, int64_t step, const double atom_pressure[9],
+                        const double mol_pressure[9], char* err, size_t errlen);
+/* Ensemble::write_pressac (:1161-1171): the stress sums over samples (no
+ * line when samples is 0). */
+int smd_write_pressac(const char* path, int append, int64_t step, const double atom_stress[5],
+                      const double mol_stress[5], int64_t samples, char* err, size_t errlen);
+/* Ensemble::write_pressureTensor (:1887-1912): both tensors and the mean
+ * shear stresses (atomStress.shearStress / counter, molStress's likewise). */
+int smd_write_pressure_tensor(const char* path, int append, int64_t counter1, const double atom_pressure[9],
+                              const double mol_pressure[9], double atom_shear_sum, double mol_shear_sum,
+                              int64_t counter, char* err, size_t errlen);
 
 #ifdef __cplusplus
 }

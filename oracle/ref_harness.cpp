@@ -646,6 +646,25 @@ int main(int argc, char** argv) {
             meta.put("longDeriv", ens->myLustig.longDeriv);
             meta.put("molDeriv", ens->myLustig.molDeriv);
             meta.put("surfDeriv", ens->myLustig.surfDeriv);
+            // the reference's own writers on this end state: the restart file
+            // and the pressure lines (text kept as fixtures)
+            lf->write_state();   // state.out in the working directory
+            {
+                std::ofstream pr("press_res.out");
+                ens->write_press_res(pr, nsteps);
+                std::ofstream pa("pressac.out");
+                ens->write_pressac(pa, nsteps);
+                std::ofstream pt("pressure_tensor.out");
+                ens->write_pressureTensor(pt, nsteps);
+            }
+            std::vector<double> rp(3 * ens->nAtoms);
+            for (int i = 0; i < ens->nAtoms; ++i) {
+                rp[3 * i] = ens->atoms[i].realPos.x;
+                rp[3 * i + 1] = ens->atoms[i].realPos.y;
+                rp[3 * i + 2] = ens->atoms[i].realPos.z;
+            }
+            write_bin("end_real_pos", rp.data(), rp.size() * 8);
+            meta.put("current_steps", lf->currentSteps);
         }
         std::printf("{\"steps\": %d, \"seconds\": %.6f}\n", nsteps, secs);
     } else {

@@ -241,6 +241,16 @@ SIGNATURES = {
     "smd_md_init": (C.c_int, [_P, C.POINTER(MdParams)]),
     "smd_md_run": (C.c_int, [_P, C.c_int64, C.POINTER(MdReport)]),
     "smd_md_pressure": (C.c_int, [_P, C.POINTER(MdPressure)]),
+    "smd_set_real_positions": (C.c_int, [_P, _P, C.c_int64]),
+    "smd_get_real_positions": (C.c_int, [_P, _P, C.c_int64]),
+    "smd_md_get_thermostat": (C.c_int, [_P, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
+    "smd_md_set_thermostat": (C.c_int, [_P, C.c_double, C.c_double]),
+    "smd_write_state": (C.c_int, [C.c_char_p, C.c_int64, C.c_int64, _P, _P, _P, C.c_int32, C.c_char_p, C.c_size_t]),
+    "smd_write_vt_minus": (C.c_int, [C.c_char_p, C.c_int64, C.c_int64, _P, C.c_int32, C.c_char_p, C.c_size_t]),
+    "smd_write_press_res": (C.c_int, [C.c_char_p, C.c_int, C.c_int64, _P, _P, C.c_char_p, C.c_size_t]),
+    "smd_write_pressac": (C.c_int, [C.c_char_p, C.c_int, C.c_int64, _P, _P, C.c_int64, C.c_char_p, C.c_size_t]),
+    "smd_write_pressure_tensor": (C.c_int, [C.c_char_p, C.c_int, C.c_int64, _P, _P, C.c_double, C.c_double,
+                                            C.c_int64, C.c_char_p, C.c_size_t]),
     "smd_profile_enable": (C.c_int, [_P, C.c_int]),
     "smd_profile_get": (C.c_int, [_P, C.c_char_p, C.POINTER(C.c_double), C.POINTER(C.c_int64)]),
     "smd_md_time_family": (C.c_int, [_P, C.c_char_p, C.c_int, C.POINTER(C.c_double)]),

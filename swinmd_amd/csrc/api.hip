@@ -71,7 +71,7 @@ static void release(Ctx* c) {
                     c->d_mp_terms, c->mp_ks.kvec, c->mp_ks.kmod2, c->mp_ks.expk, c->mp_ks.part, c->mp_ks.kf,
                     c->mp_ks.fpart, c->ind_ks.kvec, c->ind_ks.kmod2, c->ind_ks.expk, c->ind_ks.part,
                     c->ind_ks.kf, c->ind_ks.fpart, c->d_pack, c->d_bd_bonds, c->d_bd_angles,
-                    c->d_bd_bond_off, c->d_bd_angle_off, c->d_bd_par, c->d_bd_out, c->d_stress};
+                    c->d_bd_bond_off, c->d_bd_angle_off, c->d_bd_par, c->d_bd_out, c->d_stress, c->d_real};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     if (c->h_pinned) (void)hipHostFree(c->h_pinned);
@@ -173,6 +173,7 @@ extern "C" int smd_ctx_create(smd_ctx** out, const smd_topology* t, int device) 
     A(c->d_vel, sizeof(double) * 3 * n);
     A(c->d_vtm, sizeof(double) * 3 * n);
     A(c->d_disp, sizeof(double) * 3 * n);
+    A(c->d_real, sizeof(double) * 3 * n);
     A(c->d_force, sizeof(double) * 3 * n);
     A(c->d_com, sizeof(double) * 3 * nm);
     A(c->d_mom, sizeof(double) * 3 * nm);
@@ -263,7 +264,13 @@ static int copy_out(smd_ctx* c, double* dst, const double* src, int64_t n, int64
     return SMD_OK;
 }
 
-extern "C" int smd_set_positions(smd_ctx* c, const double* x, int64_t n) { return copy_in(c, c ? c->d_pos : nullptr, x, n, c ? c->n : 0); }
+// Ensemble::read_coordinate also starts the unwrapped positions there
+// (atoms[i].realPos = position, src/Ensemble.cpp:667-668).
+extern "C" int smd_set_positions(smd_ctx* c, const double* x, int64_t n) {
+    int rc = copy_in(c, c ? c->d_pos : nullptr, x, n, c ? c->n : 0);
+    if (rc == SMD_OK) rc = copy_in(c, c->d_real, x, n, c->n);
+    return rc;
+}
 extern "C" int smd_get_positions(smd_ctx* c, double* x, int64_t n) { return copy_out(c, x, c ? c->d_pos : nullptr, n, c ? c->n : 0); }
 extern "C" int smd_set_velocities(smd_ctx* c, const double* v, int64_t n) { return copy_in(c, c ? c->d_vel : nullptr, v, n, c ? c->n : 0); }
 extern "C" int smd_get_velocities(smd_ctx* c, double* v, int64_t n) { return copy_out(c, v, c ? c->d_vel : nullptr, n, c ? c->n : 0); }

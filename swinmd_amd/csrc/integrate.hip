@@ -26,6 +26,7 @@ struct IntegArgs {
     double* vtm;
     double* vel;
     double* disp;
+    double* real;      // Atom::realPos
     double* force;
     const double* frecip;   // MD step with reciprocal space on the second stream: its forces, else null
     double* com;
@@ -175,6 +176,7 @@ __global__ __launch_bounds__(256) void k_shake(IntegArgs a) {
                 const double dr = rn[s][d] - p0;
                 a.pos[3 * id + d] = p0 + dr;
                 a.disp[3 * id + d] += dr;
+                a.real[3 * id + d] += dr;
                 const double vp = dr / a.dt;
                 a.vel[3 * id + d] = (vp + vt[s][d]) * 0.5;
                 a.vtm[3 * id + d] = vp;
@@ -214,6 +216,7 @@ __global__ __launch_bounds__(256) void k_lf_atoms(IntegArgs a) {
         const double dr = a.dt * vp[d];
         a.pos[3 * i + d] += dr;
         a.disp[3 * i + d] += dr;
+        a.real[3 * i + d] += dr;
         a.vel[3 * i + d] = (vp[d] + a.vtm[3 * i + d]) * 0.5;
         a.vtm[3 * i + d] = vp[d];
     }
@@ -482,6 +485,7 @@ static IntegArgs integ_args(Ctx* c, const smd_integrator_params* p) {
     a.vtm = c->d_vtm;
     a.vel = c->d_vel;
     a.disp = c->d_disp;
+    a.real = c->d_real;
     a.force = c->d_force;
     a.com = c->d_com;
     a.mom = c->d_mom;
@@ -897,5 +901,38 @@ extern "C" int smd_md_pressure(smd_ctx* c, smd_md_pressure_out* out) {
         out->atom_stress[k] = st[2 + k];
         out->mol_stress[k] = st[7 + k];
     }
+    return SMD_OK;
+}
+
+extern "C" int smd_set_real_positions(smd_ctx* c, const double* r, int64_t n) {
+    if (!c || !r || n != c->n) return SMD_EINVAL;
+    SMD_HIP(c, hipMemcpy(c->d_real, r, sizeof(double) * 3 * c->n, hipMemcpyHostToDevice));
+    return SMD_OK;
+}
+
+extern "C" int smd_get_real_positions(smd_ctx* c, double* r, int64_t n) {
+    if (!c || !r || n != c->n) return SMD_EINVAL;
+    SMD_HIP(c, hipStreamSynchronize(c->stream));
+    SMD_HIP(c, hipMemcpy(r, c->d_real, sizeof(double) * 3 * c->n, hipMemcpyDeviceToHost));
+    return SMD_OK;
+}
+
+extern "C" int smd_md_get_thermostat(smd_ctx* c, double* fnh, double* fnh_minus) {
+    if (!c || !fnh || !fnh_minus) return SMD_EINVAL;
+    if (!c->md_ready) return fail(c, SMD_ESTATE, "smd_md_init not called");
+    double sc[2];
+    SMD_HIP(c, hipStreamSynchronize(c->stream));
+    SMD_HIP(c, hipMemcpy(sc, c->d_scalars, sizeof sc, hipMemcpyDeviceToHost));
+    *fnh = sc[0];
+    *fnh_minus = sc[1];
+    return SMD_OK;
+}
+
+extern "C" int smd_md_set_thermostat(smd_ctx* c, double fnh, double fnh_minus) {
+    if (!c) return SMD_EINVAL;
+    if (!c->md_ready) return fail(c, SMD_ESTATE, "smd_md_init not called");
+    const double sc[2] = {fnh, fnh_minus};
+    SMD_HIP(c, hipStreamSynchronize(c->stream));
+    SMD_HIP(c, hipMemcpy(c->d_scalars, sc, sizeof sc, hipMemcpyHostToDevice));
     return SMD_OK;
 }

@@ -206,6 +206,9 @@ struct Ctx {
     // Ensemble::accumulate stress sums of the MD step: [0] counter,
     // [1] samples taken, [2..6] atomStress, [7..11] molStress
     double* d_stress = nullptr;
+    // Atom::realPos, the unwrapped positions the restart file keeps
+    // (advanced by every integration step, untouched by the rewrap)
+    double* d_real = nullptr;
     // bonded terms of flexible molecules (bonded.hip)
     bool bonded_ready = false, bd_ub_ok = false;
     int bd_potential = 1, bd_n_bond_types = 0;

@@ -407,6 +407,58 @@ class WaterMD:
         d["virial"] = np.array(r.virial[:])
         return d
 
+    def checkpoint(self, state_path, vt_minus_path=None, step=0, precision=17):
+        """Integrator::write_state and NHIntegrator::write_vtMinus for the
+        current state (lossless with precision 17); returns the thermostat
+        friction (fnh, fnh_minus), which the reference's files do not keep."""
+        from .outputs import write_state, write_vt_minus
+        ctx = self.ctx
+        real = np.zeros((ctx.n, 3))
+        ctx.check(ctx.lib.smd_get_real_positions(ctx.h, _ptr(real), ctx.n))
+        write_state(state_path, step, ctx.positions(), real, ctx.velocities(), precision)
+        if vt_minus_path is not None:
+            vtm = np.zeros((ctx.n, 3))
+            ctx.check(ctx.lib.smd_get_vt_minus(ctx.h, _ptr(vtm), ctx.n))
+            write_vt_minus(vt_minus_path, step, vtm, precision)
+        return self.thermostat()
+
+    def restart(self, state_path, vt_minus_path=None, thermostat=None):
+        """The restart branch of the Ensemble constructor (state file:
+        positions, real positions, velocities, src/Ensemble.cpp:299-317), then
+        LFIntegrator::start, then vtMinus from its file when given
+        (NHIntegrator::read_vtMinus, :1956-2002) and the thermostat friction
+        when given."""
+        from .inputs import read_coordinate, read_state
+        ctx = self.ctx
+        pos, real, vel = read_state(state_path, ctx.n)
+        ctx.set_positions(pos)
+        ctx.check(ctx.lib.smd_set_real_positions(ctx.h, _ptr(_f64(real)), ctx.n))
+        ctx.set_velocities(vel)
+        ctx.compute_mol_centers()
+        self.init()
+        if vt_minus_path is not None:
+            vtm = _f64(read_coordinate(vt_minus_path, ctx.n, skip_lines=1))
+            ctx.check(ctx.lib.smd_set_vt_minus(ctx.h, _ptr(vtm), ctx.n))
+        if thermostat is not None:
+            # a lossless continuation: the friction and the molecule momenta
+            # SHAKE's friction term reads (Molecule::momenta of the last
+            # step, the molecules' summed m v)
+            ctx.check(ctx.lib.smd_md_set_thermostat(ctx.h, float(thermostat[0]), float(thermostat[1])))
+            s = ctx.system
+            mom = np.zeros((ctx.nmol, 3))
+            np.add.at(mom, s.mol, s.mass[:, None] * vel)
+            ctx.check(ctx.lib.smd_set_mol_momenta(ctx.h, _ptr(_f64(mom)), ctx.nmol))
+
+    def thermostat(self):
+        a, b = C.c_double(), C.c_double()
+        self.ctx.check(self.ctx.lib.smd_md_get_thermostat(self.ctx.h, C.byref(a), C.byref(b)))
+        return a.value, b.value
+
+    def real_positions(self):
+        r = np.zeros((self.ctx.n, 3))
+        self.ctx.check(self.ctx.lib.smd_get_real_positions(self.ctx.h, _ptr(r), self.ctx.n))
+        return r
+
     def pressure(self):
         """The last step's Ensemble::virial, molVirial, atomPressure and
         molPressure (3x3, row-major XX XY XZ ...), myAvLus.P, and the

@@ -60,6 +60,12 @@ def read_outputs(d):
     names = os.path.join(d, "names.txt")
     if os.path.exists(names):
         arrays["names"] = np.array([ln.rstrip("\n") for ln in open(names)])
+    # text the reference's own writers produced, kept whole
+    for f, key in (("state.out", "state_text"), ("press_res.out", "press_res_text"), ("pressac.out", "pressac_text"),
+                   ("pressure_tensor.out", "pressure_tensor_text")):
+        path = os.path.join(d, f)
+        if os.path.exists(path):
+            arrays[key] = np.array(open(path).read())
     return arrays
 
 

@@ -182,6 +182,38 @@ def test_md_matches_reference(name):
     st = np.concatenate([pr["atom_stress"], pr["mol_stress"]])
     want = fx.ref["stress_sums"]
     assert np.max(np.abs(st - want)) <= 1e-8 * np.max(np.abs(want))
+    # Atom::realPos: advanced with every step, never rewrapped
+    assert rel_rms(md.real_positions(), fx.vec("end_real_pos")) <= 1e-11
+
+
+def test_md_checkpoint_and_restart_continue_the_run(tmp_path):
+    """Run 6 steps, write the restart files losslessly (state with positions,
+    real positions and velocities; vtMinus; the thermostat friction), continue
+    4 steps; a fresh context restarted from the files and run 4 steps lands on
+    the same state."""
+    from swinmd_amd.forces import WaterMD
+    fx = Fixture("water512_md10")
+    c = fx.cfg
+    kw = dict(cutoff=c["cutoff"], cut_buff=c["cut_buff"], alpha=c["alpha"], kcut=c["kcut"], dt=c["dt"],
+              temperature=c["temperature"], coupl=c["coupl"], tol=c["shake_tol"], max_count=c["max_count"],
+              use_graph=False)
+    ctx = _ctx(fx.system)
+    ctx.set_velocities(fx.vec("init_vel"))
+    md = WaterMD(ctx, **kw)
+    md.init()
+    md.run(6)
+    nh = md.checkpoint(tmp_path / "state.out", tmp_path / "vtMinus.out", step=6)
+    md.run(4)
+    want_pos, want_vel, want_real = ctx.positions(), ctx.velocities(), md.real_positions()
+    ctx2 = _ctx(fx.system)
+    md2 = WaterMD(ctx2, **kw)
+    md2.restart(tmp_path / "state.out", tmp_path / "vtMinus.out", thermostat=nh)
+    md2.run(4)
+    assert rel_rms(ctx2.positions(), want_pos) <= 1e-10
+    assert rel_rms(ctx2.velocities(), want_vel) <= 1e-8
+    assert rel_rms(md2.real_positions(), want_real) <= 1e-10
+    ctx.close()
+    ctx2.close()
 
 
 def test_md_graph_replay_equals_eager():
