@@ -244,7 +244,7 @@ __device__ __forceinline__ void nh_friction(const IntegArgs& a, double ke_atom, 
 // step, from the step's sums; one thread.  Ensemble::virial is the force
 // terms' virials subtracted, the long-range virial added without its ZZ
 // component (EwaldForce::compute_long copies i < ZZ, EwaldForce.cpp:194).
-__device__ void md_pressure(const IntegArgs& a, volatile const double* t) {
+__device__ void md_pressure(const IntegArgs& a, const double* t) {
     const double V = a.box.L[0] * a.box.L[1] * a.box.L[2];
     const int sym[9] = {0, 1, 2, 1, 3, 4, 2, 4, 5};   // (xx xy xz yy yz zz) -> 3x3
     double vir[9], atomP[9], molP[9];
@@ -307,7 +307,11 @@ __device__ void md_pressure(const IntegArgs& a, volatile const double* t) {
     }
 }
 
-__global__ __launch_bounds__(256) void k_kinetics(IntegArgs a, int rewrap) {
+// One wave per block: the kinetic and pressure sums are wave reductions
+// spread over as many CUs as there are waves of molecules.
+constexpr int KIN_THREADS = 64;
+
+__global__ __launch_bounds__(KIN_THREADS) void k_kinetics(IntegArgs a, int rewrap) {
     double v[2] = {0.0, 0.0};
     double pr[21];   // MD step: sum f (x) (r - R) [9], atom p p / m [6], molecule P P / M [6]
 #pragma unroll
@@ -344,11 +348,14 @@ __global__ __launch_bounds__(256) void k_kinetics(IntegArgs a, int rewrap) {
                 }
                 for (int x = 0; x < 3; ++x)
                     for (int y = 0; y < 3; ++y) pr[3 * x + y] += f[y] * dr[x];
-                pr[9] += p[0] * p[0] / mi; pr[10] += p[0] * p[1] / mi; pr[11] += p[0] * p[2] / mi;
-                pr[12] += p[1] * p[1] / mi; pr[13] += p[1] * p[2] / mi; pr[14] += p[2] * p[2] / mi;
+                // p p / m as (p / m) p, one reciprocal per atom
+                const double rm = 1.0 / mi;
+                pr[9] += p[0] * rm * p[0]; pr[10] += p[0] * rm * p[1]; pr[11] += p[0] * rm * p[2];
+                pr[12] += p[1] * rm * p[1]; pr[13] += p[1] * rm * p[2]; pr[14] += p[2] * rm * p[2];
             }
-            pr[15] += P[0] * P[0] / M; pr[16] += P[0] * P[1] / M; pr[17] += P[0] * P[2] / M;
-            pr[18] += P[1] * P[1] / M; pr[19] += P[1] * P[2] / M; pr[20] += P[2] * P[2] / M;
+            const double rM = 1.0 / M;
+            pr[15] += P[0] * rM * P[0]; pr[16] += P[0] * rM * P[1]; pr[17] += P[0] * rM * P[2];
+            pr[18] += P[1] * rM * P[1]; pr[19] += P[1] * rM * P[2]; pr[20] += P[2] * rM * P[2];
         }
         for (int d = 0; d < 3; ++d) a.mom[3 * m + d] = P[d];
         if (rewrap) {
@@ -369,13 +376,21 @@ __global__ __launch_bounds__(256) void k_kinetics(IntegArgs a, int rewrap) {
         __syncthreads();
         if (threadIdx.x == 0) last = atomicAdd(&a.flag[2], 1) == (int)gridDim.x - 1;
         __syncthreads();
-        if (last && threadIdx.x == 0) {
+        if (last) {   // block-uniform
+            // the sums every block added, read once by the whole block (the
+            // epilogue's thread then works from LDS, not from a chain of
+            // dependent global loads)
+            __shared__ double tl[T_COUNT];
             __threadfence();
-            volatile const double* t = a.terms;   // sums the other blocks added
-            nh_friction(a, t[T_KE_ATOM], t[T_KE_MOL]);
-            md_pressure(a, t);
-            for (int k = T_VOM; k <= T_MTOT; ++k) a.terms[k] = 0.0;
-            a.flag[2] = 0;
+            volatile const double* tv = a.terms;
+            for (int k = threadIdx.x; k < T_COUNT; k += blockDim.x) tl[k] = tv[k];
+            __syncthreads();
+            if (threadIdx.x == 0) {
+                nh_friction(a, tl[T_KE_ATOM], tl[T_KE_MOL]);
+                md_pressure(a, tl);
+                for (int k = T_VOM; k <= T_MTOT; ++k) a.terms[k] = 0.0;
+                a.flag[2] = 0;
+            }
         }
     }
 }
@@ -536,7 +551,8 @@ int kinetics_run(Ctx* c, bool rewrap, const smd_integrator_params* md) {
         a.stress_after = 0.15 * (double)c->md.total_steps;
     }
     ProfScope ps(c, "kinetics");
-    hipLaunchKernelGGL(k_kinetics, dim3((c->nmol + 255) / 256), dim3(256), 0, c->stream, a, rewrap ? 1 : 0);
+    hipLaunchKernelGGL(k_kinetics, dim3((c->nmol + KIN_THREADS - 1) / KIN_THREADS), dim3(KIN_THREADS), 0, c->stream,
+                       a, rewrap ? 1 : 0);
     SMD_HIP(c, hipGetLastError());
     return SMD_OK;
 }
