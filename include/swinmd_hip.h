@@ -239,7 +239,8 @@ typedef struct {
     int32_t do_min;        /* do_induction_minimization(): CG, else Pi = alpha E0 */
     double tol;            /* CG stops when |r| < tol; 0 -> 1e-14 (ConjugateGradient) */
     int32_t max_iter;      /* 0 -> 3n, the reference's bound */
-    int32_t guess;         /* 0: Pi = 1 (the reference); 1: the previous solution; 2: alpha E0 */
+    int32_t guess;         /* 0: Pi = 1 (the reference); 1: the previous solution; 2: alpha E0;
+                              3: the learned guess (smd_mlp_*) */
     int32_t prec;          /* smd_prec; SMD_PREC_FP64 only */
 } smd_induce_params;
 
@@ -259,6 +260,73 @@ int smd_induce(smd_ctx* ctx, const smd_induce_params* p, smd_induce_report* repo
  * the permanent field Ei0 [3n] and the induced dipoles Pi [3n] of the last
  * smd_induce; any pointer may be NULL. */
 int smd_get_induction(smd_ctx* ctx, double* dip, double* quad, double* e0, double* mu, int64_t n_atoms);
+
+/* ---- learned initial guess of the induced dipoles --------------------------- */
+/* No counterpart in the reference, whose ConjugateGradient starts from Pi = 1
+ * (src/forces/Induction.cpp:2404-2410).  smd_induce with guess 3 starts CG
+ * from Pi_i = alpha_i E0_i + F_i^T (sd * y_i + mean), y_i the output of a
+ * per-site network on a 32-value descriptor of site i's environment in its
+ * molecular frame F_i, then refines to the same tolerance: the solve decides
+ * the result, the network only the number of iterations.
+ *   descriptor  alpha E0, alpha T alpha E0, alpha sum_j q_j r/r^3 of the
+ *               intermolecular partners in four shells to 0.6 nm split by the
+ *               sign of q_j, alpha, positive-site flag (needs a pair list of
+ *               cutoff >= 0.6 nm; 3-site waters H1, O, H2)
+ *   network     32 -> 128 -> 128 -> 128 -> 3, SiLU, BF16 weights and
+ *               activations on the matrix cores, FP32 accumulation
+ *   training    on the GPU: mean squared error of the standardised residual
+ *               F_i (Pi_i - alpha_i E0_i) of converged solves, Adam with a
+ *               cosine learning-rate decay; deterministic for a given seed
+ * Parameters in the order W1[128][32] b1 W2[128][128] b2 W3[128][128] b3
+ * W4[3][128] b4[3]; weights [out][in].  Normalisation: feature mean[32],
+ * feature sd[32], output mean[3], output sd[3]. */
+#define SMD_MLP_FEATURES 32
+#define SMD_MLP_HIDDEN 128
+#define SMD_MLP_OUTPUTS 3
+#define SMD_MLP_NPARAMS 37635
+#define SMD_MLP_NNORM 70
+
+/* Allocates the network on the context (once), draws uniform weights
+ * (+-sqrt(6/fan_in), the last layer x0.1) from the seed, zero biases, an
+ * identity normalisation that smd_induce does not accept until
+ * smd_mlp_train or smd_mlp_set_norm has run. */
+int smd_mlp_init(smd_ctx* ctx, uint64_t seed);
+int smd_mlp_set_params(smd_ctx* ctx, const float* params, int64_t count);
+int smd_mlp_get_params(smd_ctx* ctx, float* params, int64_t count);
+int smd_mlp_set_norm(smd_ctx* ctx, const float* norm, int64_t count);
+int smd_mlp_get_norm(smd_ctx* ctx, float* norm, int64_t count);
+/* The network on host rows: x [rows][32] standardised descriptors (rounded to
+ * BF16), y [rows][3] standardised outputs. */
+int smd_mlp_forward(smd_ctx* ctx, const float* x, int64_t rows, float* y);
+/* Mean squared error over rows x 3 of the network's outputs against target
+ * [rows][3], and its gradient [SMD_MLP_NPARAMS] in the parameter order
+ * (the training kernels' forward and backward pass); either output may be NULL. */
+int smd_mlp_loss_grad(smd_ctx* ctx, const float* x, const float* target, int64_t rows, double* loss, float* grad);
+/* Raw descriptors [n][32] of the state of the last converged smd_induce. */
+int smd_mlp_descriptors(smd_ctx* ctx, float* desc, int64_t n_atoms);
+/* Appends the descriptors and targets of the last converged smd_induce to the
+ * training set on the device; total_rows (may be NULL) gets its size. */
+int smd_mlp_add_frame(smd_ctx* ctx, int64_t* total_rows);
+int smd_mlp_clear_frames(smd_ctx* ctx);
+
+typedef struct {
+    int32_t epochs;
+    int32_t batch;         /* rows per Adam step, >= 16 */
+    float lr;              /* initial learning rate, decayed by a cosine to lr_final */
+    float lr_final;
+    uint64_t seed;         /* order of the rows in each epoch */
+    int32_t renormalize;   /* recompute the normalisation from the training set (always on the first run) */
+} smd_mlp_train_params;
+
+typedef struct {
+    int64_t samples;       /* rows of the training set */
+    int32_t steps;         /* Adam steps */
+    double loss_first;     /* mean squared error over the first epoch, standardised units */
+    double loss_last;      /* over the last epoch */
+    double ms;             /* device time of the training */
+} smd_mlp_train_report;
+
+int smd_mlp_train(smd_ctx* ctx, const smd_mlp_train_params* p, smd_mlp_train_report* report);
 
 /* ---- permanent multipoles under Ewald: class Multipole ------------------ */
 /* The sites' local multipoles come from smd_set_multipoles; each call below

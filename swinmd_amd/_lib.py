@@ -98,6 +98,19 @@ class InduceReport(C.Structure):
     _fields_ = [("iterations", C.c_int32), ("converged", C.c_int32), ("residual", C.c_double)]
 
 
+MLP_FEATURES, MLP_HIDDEN, MLP_OUTPUTS, MLP_NPARAMS, MLP_NNORM = 32, 128, 3, 37635, 70
+
+
+class MlpTrainParams(C.Structure):
+    _fields_ = [("epochs", C.c_int32), ("batch", C.c_int32), ("lr", C.c_float), ("lr_final", C.c_float),
+                ("seed", C.c_uint64), ("renormalize", C.c_int32)]
+
+
+class MlpTrainReport(C.Structure):
+    _fields_ = [("samples", C.c_int64), ("steps", C.c_int32), ("loss_first", C.c_double),
+                ("loss_last", C.c_double), ("ms", C.c_double)]
+
+
 class MpParams(C.Structure):
     _fields_ = [("alpha", C.c_double), ("real_cut", C.c_double), ("use_constraint", C.c_int32),
                 ("surface", C.c_int32), ("torques", C.c_int32)]
@@ -258,6 +271,17 @@ SIGNATURES = {
     "smd_set_multipoles": (C.c_int, [_P, _P, _P, _P, C.c_int64]),
     "smd_induce": (C.c_int, [_P, C.POINTER(InduceParams), C.POINTER(InduceReport)]),
     "smd_get_induction": (C.c_int, [_P, _P, _P, _P, _P, C.c_int64]),
+    "smd_mlp_init": (C.c_int, [_P, C.c_uint64]),
+    "smd_mlp_set_params": (C.c_int, [_P, _P, C.c_int64]),
+    "smd_mlp_get_params": (C.c_int, [_P, _P, C.c_int64]),
+    "smd_mlp_set_norm": (C.c_int, [_P, _P, C.c_int64]),
+    "smd_mlp_get_norm": (C.c_int, [_P, _P, C.c_int64]),
+    "smd_mlp_forward": (C.c_int, [_P, _P, C.c_int64, _P]),
+    "smd_mlp_loss_grad": (C.c_int, [_P, _P, _P, C.c_int64, C.POINTER(C.c_double), _P]),
+    "smd_mlp_descriptors": (C.c_int, [_P, _P, C.c_int64]),
+    "smd_mlp_add_frame": (C.c_int, [_P, C.POINTER(C.c_int64)]),
+    "smd_mlp_clear_frames": (C.c_int, [_P]),
+    "smd_mlp_train": (C.c_int, [_P, C.POINTER(MlpTrainParams), C.POINTER(MlpTrainReport)]),
     "smd_multipole_setup": (C.c_int, [_P, C.POINTER(MpParams), C.c_double, C.POINTER(C.c_int32)]),
     "smd_multipole_compute": (C.c_int, [_P, C.c_int, _P, C.POINTER(MpOut)]),
     "smd_multipole_compute_long": (C.c_int, [_P, C.c_int, _P, C.POINTER(MpLongOut)]),

@@ -74,6 +74,7 @@ static void release(Ctx* c) {
                     c->d_bd_bond_off, c->d_bd_angle_off, c->d_bd_par, c->d_bd_out, c->d_stress, c->d_real};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
+    mlp_release(c);
     if (c->h_pinned) (void)hipHostFree(c->h_pinned);
     if (c->graph_exec) (void)hipGraphExecDestroy(c->graph_exec);
     if (c->graph) (void)hipGraphDestroy(c->graph);

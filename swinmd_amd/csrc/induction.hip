@@ -439,13 +439,27 @@ int frames_run(Ctx* c) {
     return SMD_OK;
 }
 
+// x = alpha E0 and z = A x for the permanent field of the last solve.
+int induce_direct_apply(Ctx* c, const smd_induce_params* p, double* x, double* z) {
+    IndArgs a = ind_args(c, p);
+    const int dim = 3 * c->n;
+    const dim3 grid_rows((unsigned)(((int64_t)c->n * IND_LANES + 255) / 256));
+    hipLaunchKernelGGL(k_direct, dim3((dim + 255) / 256), dim3(256), 0, c->stream, dim, c->d_polar, c->d_e0, x);
+    hipLaunchKernelGGL((k_apply_a<IND_LANES>), grid_rows, dim3(256), 0, c->stream, a, x, z, nullptr, nullptr,
+                       CgTop{nullptr, nullptr, 0, 0.0});
+    SMD_HIP(c, hipGetLastError());
+    return SMD_OK;
+}
+
 int induce_run(Ctx* c, const smd_induce_params* p, smd_induce_report* rep) {
+    c->ind_last_converged = false;
     if (!c->multipoles_set) return fail(c, SMD_ESTATE, "multipoles not set");
     if (!c->list_valid) return fail(c, SMD_ESTATE, "no pair list");
     if (!(p->cutoff_ind > 0)) return fail(c, SMD_EINVAL, "cutoff_ind must be positive");
     if (p->cutoff_ind > c->list_cutoff)
         return fail(c, SMD_ENOTSUP, "cutoff_ind %g exceeds the pair-list cutoff %g", p->cutoff_ind, c->list_cutoff);
     if (p->prec != SMD_PREC_FP64) return fail(c, SMD_ENOTSUP, "induced dipoles are FP64 only");
+    if (p->guess < 0 || p->guess > 3) return fail(c, SMD_EINVAL, "guess %d is not 0..3", p->guess);
     for (int m = 0; m < c->nmol; ++m)
         if (c->h_mol_start[m + 1] - c->h_mol_start[m] != 3)
             return fail(c, SMD_ENOTSUP, "induced dipoles are implemented for 3-site water");
@@ -485,6 +499,14 @@ int induce_run(Ctx* c, const smd_induce_params* p, smd_induce_report* rep) {
         ProfScope ps(c, "ind_cg");
         if (p->guess == 0) hipLaunchKernelGGL(k_fill, grid_vec, b256, 0, c->stream, dim, 1.0, g.x);
         else if (p->guess == 2) hipLaunchKernelGGL(k_direct, grid_vec, b256, 0, c->stream, dim, c->d_polar, c->d_e0, g.x);
+        else if (p->guess == 3) {
+            // learned: x = alpha E0, z = A x feed the descriptors, then x = the network's guess
+            hipLaunchKernelGGL(k_direct, grid_vec, b256, 0, c->stream, dim, c->d_polar, c->d_e0, g.x);
+            hipLaunchKernelGGL((k_apply_a<IND_LANES>), grid_rows, b256, 0, c->stream, a, g.x, g.z, nullptr, nullptr,
+                               CgTop{nullptr, nullptr, 0, 0.0});
+            const int rc = mlp_guess(c, g.x, g.z);
+            if (rc) return rc;
+        }
         // guess 1: keep the previous solution in d_mu
         hipLaunchKernelGGL((k_apply_a<IND_LANES>), grid_rows, b256, 0, c->stream, a, g.x, g.z, nullptr, nullptr,
                            CgTop{nullptr, nullptr, 0, 0.0});
@@ -511,6 +533,8 @@ int induce_run(Ctx* c, const smd_induce_params* p, smd_induce_report* rep) {
     }
     SMD_HIP(c, hipGetLastError());
     SMD_HIP(c, hipStreamSynchronize(c->stream));
+    c->ind_last = *p;
+    c->ind_last_converged = p->do_min && reason != 0;
     if (rep) {
         rep->iterations = iters;
         rep->residual = res;

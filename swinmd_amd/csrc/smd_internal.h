@@ -218,8 +218,13 @@ struct Ctx {
     int32_t* d_bd_bond_off = nullptr;
     int32_t* d_bd_angle_off = nullptr;
     double* d_bd_par = nullptr;
-    double* d_bd_out = nullptr;     // packed site records are current: skip the repack (kernel timing)          // mixed LJ + Ewald of the MD step from packed site records
+    double* d_bd_out = nullptr;
+    // mixed LJ + Ewald of the MD step from packed site records
     void* d_pack = nullptr;       // float4 [3n]: packed records, then offsets from the molecule centres
+    // learned initial guess of the induced dipoles (learned_guess.hip)
+    struct Mlp* mlp = nullptr;
+    smd_induce_params ind_last{};   // parameters of the last smd_induce
+    bool ind_last_converged = false;
 };
 
 }  // namespace smd
@@ -229,6 +234,10 @@ struct smd_ctx : public smd::Ctx {};
 namespace smd {
 
 int fail(Ctx* c, int code, const char* fmt, ...);
+// learned guess (learned_guess.hip) and the induction helpers it uses (induction.hip)
+void mlp_release(Ctx* c);
+int mlp_guess(Ctx* c, double* x, const double* z);
+int induce_direct_apply(Ctx* c, const smd_induce_params* p, double* x, double* z);
 int hip_fail(Ctx* c, hipError_t e, const char* what);
 
 #define SMD_HIP(c, expr)                                        \

@@ -567,3 +567,77 @@ class Induction:
         dip, quad, e0, mu = np.zeros((n, 3)), np.zeros((n, 6)), np.zeros((n, 3)), np.zeros((n, 3))
         self.ctx.check(self.ctx.lib.smd_get_induction(self.ctx.h, _ptr(dip), _ptr(quad), _ptr(e0), _ptr(mu), n))
         return dip, quad, e0, mu
+
+
+class LearnedGuess:
+    """The learned initial guess of the induced dipoles (smd_mlp_*): a per-site
+    network, trained on the GPU on the dipoles of converged solves, that
+    Induction(guess=3) uses as CG's starting point.  The reference's CG starts
+    from Pi = 1 (src/forces/Induction.cpp:2404-2410); the solve's tolerance is
+    unchanged, so the guess changes the iteration count, not the result."""
+
+    def __init__(self, ctx: Context, seed=1):
+        self.ctx = ctx
+        ctx.check(ctx.lib.smd_mlp_init(ctx.h, int(seed)))
+
+    def params(self):
+        p = np.zeros(L.MLP_NPARAMS, dtype=np.float32)
+        self.ctx.check(self.ctx.lib.smd_mlp_get_params(self.ctx.h, _ptr(p), p.size))
+        return p
+
+    def set_params(self, p):
+        p = np.ascontiguousarray(p, dtype=np.float32).ravel()
+        self.ctx.check(self.ctx.lib.smd_mlp_set_params(self.ctx.h, _ptr(p), p.size))
+
+    def norm(self):
+        v = np.zeros(L.MLP_NNORM, dtype=np.float32)
+        self.ctx.check(self.ctx.lib.smd_mlp_get_norm(self.ctx.h, _ptr(v), v.size))
+        return v
+
+    def set_norm(self, v):
+        v = np.ascontiguousarray(v, dtype=np.float32).ravel()
+        self.ctx.check(self.ctx.lib.smd_mlp_set_norm(self.ctx.h, _ptr(v), v.size))
+
+    def forward(self, x):
+        """Standardised outputs [rows, 3] of standardised descriptors x [rows, 32]."""
+        x = np.ascontiguousarray(x, dtype=np.float32).reshape(-1, L.MLP_FEATURES)
+        y = np.zeros((x.shape[0], L.MLP_OUTPUTS), dtype=np.float32)
+        self.ctx.check(self.ctx.lib.smd_mlp_forward(self.ctx.h, _ptr(x), x.shape[0], _ptr(y)))
+        return y
+
+    def loss_grad(self, x, target):
+        x = np.ascontiguousarray(x, dtype=np.float32).reshape(-1, L.MLP_FEATURES)
+        t = np.ascontiguousarray(target, dtype=np.float32).reshape(-1, L.MLP_OUTPUTS)
+        g = np.zeros(L.MLP_NPARAMS, dtype=np.float32)
+        loss = C.c_double()
+        self.ctx.check(self.ctx.lib.smd_mlp_loss_grad(self.ctx.h, _ptr(x), _ptr(t), x.shape[0], C.byref(loss),
+                                                      _ptr(g)))
+        return loss.value, g
+
+    def descriptors(self):
+        d = np.zeros((self.ctx.n, L.MLP_FEATURES), dtype=np.float32)
+        self.ctx.check(self.ctx.lib.smd_mlp_descriptors(self.ctx.h, _ptr(d), self.ctx.n))
+        return d
+
+    def add_frame(self):
+        """Appends the last converged solve to the training set; returns its size."""
+        k = C.c_int64()
+        self.ctx.check(self.ctx.lib.smd_mlp_add_frame(self.ctx.h, C.byref(k)))
+        return k.value
+
+    def clear_frames(self):
+        self.ctx.check(self.ctx.lib.smd_mlp_clear_frames(self.ctx.h))
+
+    def train(self, epochs=60, batch=1024, lr=2e-3, lr_final=2e-5, seed=7, renormalize=True):
+        p = L.MlpTrainParams(epochs, batch, lr, lr_final, seed, int(renormalize))
+        r = L.MlpTrainReport()
+        self.ctx.check(self.ctx.lib.smd_mlp_train(self.ctx.h, C.byref(p), C.byref(r)))
+        return {k: getattr(r, k) for k, _ in L.MlpTrainReport._fields_}
+
+    def state(self):
+        """Parameters and normalisation, to carry a trained network to another context."""
+        return {"params": self.params(), "norm": self.norm()}
+
+    def load_state(self, st):
+        self.set_params(st["params"])
+        self.set_norm(st["norm"])
