@@ -71,12 +71,14 @@ static void release(Ctx* c) {
                     c->d_mp_terms, c->mp_ks.kvec, c->mp_ks.kmod2, c->mp_ks.expk, c->mp_ks.part, c->mp_ks.kf,
                     c->mp_ks.fpart, c->ind_ks.kvec, c->ind_ks.kmod2, c->ind_ks.expk, c->ind_ks.part,
                     c->ind_ks.kf, c->ind_ks.fpart, c->d_pack, c->d_bd_bonds, c->d_bd_angles,
-                    c->d_bd_bond_off, c->d_bd_angle_off, c->d_bd_par, c->d_bd_out, c->d_stress, c->d_real};
+                    c->d_bd_bond_off, c->d_bd_angle_off, c->d_bd_par, c->d_bd_out, c->d_stress, c->d_real,
+                    c->d_it_cnt, c->d_it_j, c->d_it_a, c->d_it_b};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     mlp_release(c);
     if (c->h_pinned) (void)hipHostFree(c->h_pinned);
     if (c->graph_exec) (void)hipGraphExecDestroy(c->graph_exec);
+    if (c->cg_exec) (void)hipGraphExecDestroy(c->cg_exec);
     if (c->graph) (void)hipGraphDestroy(c->graph);
     for (auto& r : c->prof.pending) { (void)hipEventDestroy(r.a); (void)hipEventDestroy(r.b); }
     if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);
@@ -270,6 +272,7 @@ static int copy_out(smd_ctx* c, double* dst, const double* src, int64_t n, int64
 extern "C" int smd_set_positions(smd_ctx* c, const double* x, int64_t n) {
     int rc = copy_in(c, c ? c->d_pos : nullptr, x, n, c ? c->n : 0);
     if (rc == SMD_OK) rc = copy_in(c, c->d_real, x, n, c->n);
+    if (rc == SMD_OK) c->ind_last_converged = false;   // the last solve no longer describes this state
     return rc;
 }
 extern "C" int smd_get_positions(smd_ctx* c, double* x, int64_t n) { return copy_out(c, x, c ? c->d_pos : nullptr, n, c ? c->n : 0); }

@@ -23,6 +23,8 @@
 //    says the residual test passed.
 #include <algorithm>
 #include <cmath>
+#include <cstring>
+#include <string>
 
 #include "smd_internal.h"
 
@@ -53,6 +55,13 @@ struct IndArgs {
     double* e0;                 // [3n]
     double cut2, a;
     int damp;
+    // coupling table of the solve: per site the partners inside cutoffInd
+    // (list partners, then excluded ones) with 3 lambda5 / r^5 and lambda3 / r^3
+    int tstride;
+    int32_t* tcnt;              // [n]
+    int32_t* tj;                // [n][tstride]
+    double* ta;                 // [n][tstride]
+    double* tb;                 // [n][tstride]
 };
 
 // ---- frames -------------------------------------------------------------------
@@ -216,29 +225,10 @@ __device__ __forceinline__ void tmu_pair(const IndArgs& a, double dx, double dy,
     }
 }
 
-// Loop-top test of a CG iteration (:2431-2434), evaluated by every block
-// from the same partials in the same order: stop when |v| < tol.
-struct CgTop {
-    int* state;          // [0] done, [2] stop reason; null: no test
-    const double* p_dv;  // partials of v.v
-    int nparts;
-    double tol;
-};
-
+// A v recomputing every pair's Thole factors (outside CG, where no coupling
+// table has been built).
 template <int G>
-__global__ __launch_bounds__(256) void k_apply_a(IndArgs a, const double* __restrict__ v, double* __restrict__ out,
-                                                 const double* __restrict__ dotv, double* part, CgTop top) {
-    if (top.state) {
-        if (top.state[0]) return;
-        double dv = 0.0;
-        for (int k = threadIdx.x % WAVE; k < top.nparts; k += WAVE) dv += top.p_dv[k];
-        dv = wave_sum(dv);   // identical in every wave of every block
-        if (sqrt(dv) < top.tol) {
-            if (blockIdx.x == 0 && threadIdx.x == 0) { top.state[0] = 1; top.state[2] = 1; }
-            return;
-        }
-    }
-    __shared__ double red[256 / WAVE];
+__global__ __launch_bounds__(256) void k_apply_a(IndArgs a, const double* __restrict__ v, double* __restrict__ out) {
     const int tid = blockIdx.x * blockDim.x + threadIdx.x;
     const int i = tid / G, sub = tid % G;
     double acc[3] = {0.0, 0.0, 0.0};
@@ -266,6 +256,172 @@ __global__ __launch_bounds__(256) void k_apply_a(IndArgs a, const double* __rest
     for (int o = G / 2; o > 0; o >>= 1)
 #pragma unroll
         for (int x = 0; x < 3; ++x) acc[x] += __shfl_xor(acc[x], o, G);
+    if (i < a.n && sub == 0)
+#pragma unroll
+        for (int x = 0; x < 3; ++x) out[3 * i + x] = v[3 * i + x] / pi - acc[x];
+}
+
+// Sum of n block partials in a fixed order (lane-strided sums, then a
+// shuffle tree), computed by the block's first wave and shared through LDS:
+// every block gets the same value, whatever the scheduling.
+__device__ __forceinline__ double block_total(const double* p, int n) {
+    __shared__ double tot;
+    if (threadIdx.x < WAVE) {
+        // eight loads in flight per lane; the additions keep the order k, k + WAVE, ...
+        double s = 0.0;
+        int k = threadIdx.x;
+        for (; k + 7 * WAVE < n; k += 8 * WAVE) {
+            double q[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) q[u] = p[k + u * WAVE];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) s += q[u];
+        }
+        for (; k < n; k += WAVE) s += p[k];
+        s = wave_sum(s);
+        if (threadIdx.x == 0) tot = s;
+    }
+    __syncthreads();
+    const double v = tot;
+    __syncthreads();   // tot may be rewritten by the next call
+    return v;
+}
+
+__device__ __forceinline__ void block_partial(double v, double* part) {
+    __shared__ double red[256 / WAVE];
+    v = wave_sum(v);
+    if (threadIdx.x % WAVE == 0) red[threadIdx.x / WAVE] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double s = 0.0;
+        for (int w = 0; w < (int)(blockDim.x / WAVE); ++w) s += red[w];
+        part[blockIdx.x] = s;
+    }
+}
+
+// Loop-top test of a CG iteration (:2431-2434), the same in every block:
+// stop when |v| < tol.
+struct CgTop {
+    int* state;          // [0] done, [2] stop reason; null: plain A v
+    const double* p_dv;  // partials of v.v
+    int nparts;
+    double tol;
+};
+
+// Coupling table of a solve.  A is fixed while CG runs, so the Thole factors
+// of every pair are computed once: each group of G lanes compacts its row's
+// partners inside cutoffInd, in list order, with a ballot.
+template <int G>
+__global__ __launch_bounds__(256) void k_ind_table(IndArgs a) {
+    static_assert(WAVE % G == 0, "a lane group must not straddle waves");
+    const int tid = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = tid / G, sub = tid % G;
+    const int gbase = (threadIdx.x % WAVE) - sub;
+    const uint64_t gmask = G == 64 ? ~0ull : ((1ull << G) - 1ull);
+    int count = 0;
+    if (i < a.n) {
+        const int own = min(a.nb_own[i], a.stride);
+        const int cnt = min(a.nb_cnt[i], a.stride);
+        const int e0 = a.excl_off[i], e1 = a.excl_off[i + 1];
+        const double xi = a.pos[3 * i], yi = a.pos[3 * i + 1], zi = a.pos[3 * i + 2];
+        const double pi = a.polar[i] > POLMIN ? a.polar[i] : POLMIN;
+        const int total = cnt + (e1 - e0);
+        for (int kb = 0; kb < total; kb += G) {
+            const int k = kb + sub;
+            bool ok = false;
+            int j = 0;
+            double ca = 0.0, cb = 0.0;
+            if (k < total) {
+                j = k < cnt ? row_partner(a, i, own, k) : a.excl[e0 + (k - cnt)];
+                double dx = xi - a.pos[3 * j], dy = yi - a.pos[3 * j + 1], dz = zi - a.pos[3 * j + 2];
+                pbc_near(dx, dy, dz, a.box);
+                const double r = sqrt(dx * dx + dy * dy + dz * dz);
+                const double pj = a.polar[j] > POLMIN ? a.polar[j] : POLMIN;
+                Thole t;
+                if (thole(a, r * r, r, pi, pj, t)) {
+                    ok = true;
+                    ca = a.damp ? t.l5 * 3.0 * t.r5 : 3.0 * t.r5;
+                    cb = a.damp ? t.l3 * t.r3 : t.r3;
+                }
+            }
+            const uint64_t m = (__ballot(ok) >> gbase) & gmask;
+            if (ok) {
+                const int at = count + __popcll(m & ((1ull << sub) - 1ull));
+                const int64_t e = (int64_t)i * a.tstride + at;
+                a.tj[e] = j;
+                a.ta[e] = ca;
+                a.tb[e] = cb;
+            }
+            count += __popcll(m);
+        }
+        if (sub == 0) a.tcnt[i] = count;
+    }
+}
+
+// A v from the coupling table: (A v)_i = v_i / alpha_i - sum_j (a_ij (r.v_j) r - b_ij v_j).
+template <int G>
+__global__ __launch_bounds__(256) void k_apply_table(IndArgs a, const double* __restrict__ v,
+                                                     double* __restrict__ out, const double* __restrict__ dotv,
+                                                     double* part, CgTop top) {
+    if (top.state) {
+        if (top.state[0]) return;
+        if (sqrt(block_total(top.p_dv, top.nparts)) < top.tol) {
+            if (blockIdx.x == 0 && threadIdx.x == 0) { top.state[0] = 1; top.state[2] = 1; }
+            return;
+        }
+    }
+    const int tid = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = tid / G, sub = tid % G;
+    double ax = 0.0, ay = 0.0, az = 0.0;
+    double pi = 1.0;
+    if (i < a.n) {
+        const int cnt = a.tcnt[i];
+        const double xi = a.pos[3 * i], yi = a.pos[3 * i + 1], zi = a.pos[3 * i + 2];
+        pi = a.polar[i] > POLMIN ? a.polar[i] : POLMIN;
+        const int32_t* tj = a.tj + (int64_t)i * a.tstride;
+        const double* ta = a.ta + (int64_t)i * a.tstride;
+        const double* tb = a.tb + (int64_t)i * a.tstride;
+        auto term = [&](int j, double ca, double cb, double px, double py, double pz, double vx, double vy,
+                        double vz) {
+            double dx = xi - px, dy = yi - py, dz = zi - pz;
+            pbc_near(dx, dy, dz, a.box);
+            const double rv = ca * (dx * vx + dy * vy + dz * vz);
+            ax += rv * dx - cb * vx;
+            ay += rv * dy - cb * vy;
+            az += rv * dz - cb * vz;
+        };
+        // four partners per lane in flight: their table entries, then the
+        // positions and v of all four, then the arithmetic in partner order
+        int k = sub;
+        for (; k + 3 * G < cnt; k += 4 * G) {
+            int jj[4];
+            double ca[4], cb[4], px[4], py[4], pz[4], vx[4], vy[4], vz[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                jj[u] = tj[k + u * G];
+                ca[u] = ta[k + u * G];
+                cb[u] = tb[k + u * G];
+            }
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const int j = jj[u];
+                px[u] = a.pos[3 * j]; py[u] = a.pos[3 * j + 1]; pz[u] = a.pos[3 * j + 2];
+                vx[u] = v[3 * j]; vy[u] = v[3 * j + 1]; vz[u] = v[3 * j + 2];
+            }
+#pragma unroll
+            for (int u = 0; u < 4; ++u) term(jj[u], ca[u], cb[u], px[u], py[u], pz[u], vx[u], vy[u], vz[u]);
+        }
+        for (; k < cnt; k += G) {
+            const int j = tj[k];
+            term(j, ta[k], tb[k], a.pos[3 * j], a.pos[3 * j + 1], a.pos[3 * j + 2], v[3 * j], v[3 * j + 1],
+                 v[3 * j + 2]);
+        }
+    }
+    double acc[3] = {ax, ay, az};
+#pragma unroll
+    for (int o = G / 2; o > 0; o >>= 1)
+#pragma unroll
+        for (int x = 0; x < 3; ++x) acc[x] += __shfl_xor(acc[x], o, G);
     double d = 0.0;
     if (i < a.n && sub == 0) {
 #pragma unroll
@@ -275,26 +431,19 @@ __global__ __launch_bounds__(256) void k_apply_a(IndArgs a, const double* __rest
             if (dotv) d += dotv[3 * i + x] * z;
         }
     }
-    if (part) {
-        d = wave_sum(d);
-        if (threadIdx.x % WAVE == 0) red[threadIdx.x / WAVE] = d;
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            double s = 0.0;
-            for (int w = 0; w < (int)(blockDim.x / WAVE); ++w) s += red[w];
-            part[blockIdx.x] = s;
-        }
-    }
+    if (part) block_partial(d, part);   // v.Av of the CG step
 }
 
 // ---- CG -------------------------------------------------------------------------
 
 // Buffers of block partials, each [max_parts]: VZ (v.Av), DV (v.v), and
 // the ping-pong pair RR[0], RR[1] holding r.r of the current and next
-// iteration.
+// iteration (iteration it reads RR[it & 1]).  The iteration count is kept
+// on the device, so a batch of iterations is the same sequence of launches
+// whatever iteration it starts at (an even batch keeps the ping-pong phase).
 struct CgArgs {
     int dim;          // 3n
-    int nparts_mv;    // blocks of k_apply_a
+    int nparts_mv;    // blocks of the A v kernel
     int nparts_vec;   // blocks of the vector kernels
     double tol;
     double* x;
@@ -310,27 +459,6 @@ struct CgArgs {
     int* state;       // [0] done, [1] iterations, [2] stop reason
     double* scal;     // [0] last residual norm
 };
-
-// Sum of n partials in a fixed order: lane-strided sums, then a fixed shuffle
-// tree.  Every lane of the calling wave gets the total.
-__device__ __forceinline__ double sum_parts(const double* p, int n) {
-    const int lane = threadIdx.x % WAVE;
-    double s = 0.0;
-    for (int k = lane; k < n; k += WAVE) s += p[k];
-    return wave_sum(s);
-}
-
-__device__ __forceinline__ void block_partial(double v, double* part) {
-    __shared__ double red[CG_THREADS / WAVE];
-    v = wave_sum(v);
-    if (threadIdx.x % WAVE == 0) red[threadIdx.x / WAVE] = v;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        double s = 0.0;
-        for (int w = 0; w < (int)(blockDim.x / WAVE); ++w) s += red[w];
-        part[blockIdx.x] = s;
-    }
-}
 
 // x = guess (1 as the reference, or the caller's), then after z = A x:
 // r = E0 - z, v = r; partials of r.r (c) and v.v (dotV).
@@ -350,12 +478,13 @@ __global__ __launch_bounds__(CG_THREADS) void k_cg_start(CgArgs g) {
 }
 
 // t = c / v.Av; x += t v; r -= t z; partials of d = r.r (:2446-2454).
-__global__ __launch_bounds__(CG_THREADS) void k_cg_step(CgArgs g, int it) {
+template <int PHASE>
+__global__ __launch_bounds__(CG_THREADS) void k_cg_step(CgArgs g) {
     if (g.state[0]) return;
-    const double* rr_c = (it & 1) ? g.p_rr1 : g.p_rr0;
-    double* rr_d = (it & 1) ? g.p_rr0 : g.p_rr1;
-    const double dotVZ = sum_parts(g.p_vz, g.nparts_mv);
-    const double c = sum_parts(rr_c, g.nparts_vec);
+    const double* rr_c = PHASE ? g.p_rr1 : g.p_rr0;
+    double* rr_d = PHASE ? g.p_rr0 : g.p_rr1;
+    const double dotVZ = block_total(g.p_vz, g.nparts_mv);
+    const double c = block_total(rr_c, g.nparts_vec);
     const double t = c / dotVZ;
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     double d = 0.0;
@@ -366,21 +495,22 @@ __global__ __launch_bounds__(CG_THREADS) void k_cg_step(CgArgs g, int it) {
         d = r * r;
     }
     block_partial(d, rr_d);
-    if (blockIdx.x == 0 && threadIdx.x == 0) g.state[1] = it + 1;
+    if (blockIdx.x == 0 && threadIdx.x == 0) g.state[1] += 1;
 }
 
 // Stop when |r| < tol (:2456-2457); else v = r + (d/c) v and partials of v.v.
-__global__ __launch_bounds__(CG_THREADS) void k_cg_dir(CgArgs g, int it) {
+template <int PHASE>
+__global__ __launch_bounds__(CG_THREADS) void k_cg_dir(CgArgs g) {
     if (g.state[0]) return;
-    const double* rr_c = (it & 1) ? g.p_rr1 : g.p_rr0;
-    const double* rr_d = (it & 1) ? g.p_rr0 : g.p_rr1;
-    const double d = sum_parts(rr_d, g.nparts_vec);
+    const double* rr_c = PHASE ? g.p_rr1 : g.p_rr0;
+    const double* rr_d = PHASE ? g.p_rr0 : g.p_rr1;
+    const double d = block_total(rr_d, g.nparts_vec);
     if (blockIdx.x == 0 && threadIdx.x == 0) g.scal[0] = sqrt(d);
     if (sqrt(d) < g.tol) {
         if (blockIdx.x == 0 && threadIdx.x == 0) { g.state[0] = 1; g.state[2] = 2; }
         return;
     }
-    const double c = sum_parts(rr_c, g.nparts_vec);
+    const double c = block_total(rr_c, g.nparts_vec);
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     double dv = 0.0;
     if (k < g.dim) {
@@ -426,7 +556,35 @@ static IndArgs ind_args(Ctx* c, const smd_induce_params* p) {
     a.cut2 = p->cutoff_ind * p->cutoff_ind;
     a.a = p->a_damp;
     a.damp = p->damp_tij;
+    a.tstride = c->it_stride;
+    a.tcnt = c->d_it_cnt;
+    a.tj = c->d_it_j;
+    a.ta = c->d_it_a;
+    a.tb = c->d_it_b;
     return a;
+}
+
+// Room for every list partner and every excluded partner of each site.
+static int table_alloc(Ctx* c) {
+    int max_excl = 0;
+    for (int i = 0; i < c->n; ++i) max_excl = std::max(max_excl, c->h_excl_off[i + 1] - c->h_excl_off[i]);
+    const int stride = c->stride + max_excl;
+    const int64_t need = (int64_t)stride * c->n;
+    if (need > c->it_cap) {
+        SMD_HIP(c, hipStreamSynchronize(c->stream));
+        void* ps[] = {c->d_it_cnt, c->d_it_j, c->d_it_a, c->d_it_b};
+        for (void* q : ps)
+            if (q) (void)hipFree(q);
+        c->d_it_cnt = nullptr; c->d_it_j = nullptr; c->d_it_a = nullptr; c->d_it_b = nullptr;
+        c->it_cap = 0;
+        SMD_HIP(c, hipMalloc(&c->d_it_cnt, sizeof(int32_t) * c->n));
+        SMD_HIP(c, hipMalloc(&c->d_it_j, sizeof(int32_t) * need));
+        SMD_HIP(c, hipMalloc(&c->d_it_a, sizeof(double) * need));
+        SMD_HIP(c, hipMalloc(&c->d_it_b, sizeof(double) * need));
+        c->it_cap = need;
+    }
+    c->it_stride = stride;
+    return SMD_OK;
 }
 
 // Lab-frame multipoles of every site from the local ones (k_frames).
@@ -445,8 +603,7 @@ int induce_direct_apply(Ctx* c, const smd_induce_params* p, double* x, double* z
     const int dim = 3 * c->n;
     const dim3 grid_rows((unsigned)(((int64_t)c->n * IND_LANES + 255) / 256));
     hipLaunchKernelGGL(k_direct, dim3((dim + 255) / 256), dim3(256), 0, c->stream, dim, c->d_polar, c->d_e0, x);
-    hipLaunchKernelGGL((k_apply_a<IND_LANES>), grid_rows, dim3(256), 0, c->stream, a, x, z, nullptr, nullptr,
-                       CgTop{nullptr, nullptr, 0, 0.0});
+    hipLaunchKernelGGL((k_apply_a<IND_LANES>), grid_rows, dim3(256), 0, c->stream, a, x, z);
     SMD_HIP(c, hipGetLastError());
     return SMD_OK;
 }
@@ -463,6 +620,10 @@ int induce_run(Ctx* c, const smd_induce_params* p, smd_induce_report* rep) {
     for (int m = 0; m < c->nmol; ++m)
         if (c->h_mol_start[m + 1] - c->h_mol_start[m] != 3)
             return fail(c, SMD_ENOTSUP, "induced dipoles are implemented for 3-site water");
+    if (p->do_min) {
+        const int rc = table_alloc(c);
+        if (rc) return rc;
+    }
     IndArgs a = ind_args(c, p);
     const int n = c->n, dim = 3 * n;
     const dim3 b256(256);
@@ -472,6 +633,7 @@ int induce_run(Ctx* c, const smd_induce_params* p, smd_induce_report* rep) {
         ProfScope ps(c, "ind_field");
         hipLaunchKernelGGL(k_frames, dim3((c->nmol + 255) / 256), b256, 0, c->stream, a);
         hipLaunchKernelGGL((k_field<IND_LANES>), grid_rows, b256, 0, c->stream, a);
+        if (p->do_min) hipLaunchKernelGGL((k_ind_table<IND_LANES>), grid_rows, b256, 0, c->stream, a);
     }
     int iters = 0, reason = 0;
     double res = 0.0;
@@ -480,8 +642,6 @@ int induce_run(Ctx* c, const smd_induce_params* p, smd_induce_report* rep) {
     } else {
         CgArgs g{};
         g.dim = dim;
-        g.nparts_mv = (int)grid_rows.x;
-        g.nparts_vec = (int)grid_vec.x;
         g.tol = p->tol > 0 ? p->tol : 1.0e-14;   // ConjugateGradient's tolerance (:2404)
         g.x = c->d_mu;
         g.r = c->d_cg + 0 * (size_t)dim;
@@ -489,6 +649,8 @@ int induce_run(Ctx* c, const smd_induce_params* p, smd_induce_report* rep) {
         g.z = c->d_cg + 2 * (size_t)dim;
         g.e0 = c->d_e0;
         g.polar = c->d_polar;
+        g.nparts_mv = (int)grid_rows.x;
+        g.nparts_vec = (int)grid_vec.x;
         g.p_vz = c->d_cg_part;
         g.p_dv = c->d_cg_part + c->cg_parts;
         g.p_rr0 = c->d_cg_part + 2 * c->cg_parts;
@@ -502,22 +664,71 @@ int induce_run(Ctx* c, const smd_induce_params* p, smd_induce_report* rep) {
         else if (p->guess == 3) {
             // learned: x = alpha E0, z = A x feed the descriptors, then x = the network's guess
             hipLaunchKernelGGL(k_direct, grid_vec, b256, 0, c->stream, dim, c->d_polar, c->d_e0, g.x);
-            hipLaunchKernelGGL((k_apply_a<IND_LANES>), grid_rows, b256, 0, c->stream, a, g.x, g.z, nullptr, nullptr,
+            hipLaunchKernelGGL((k_apply_table<IND_LANES>), grid_rows, b256, 0, c->stream, a, g.x, g.z, nullptr, nullptr,
                                CgTop{nullptr, nullptr, 0, 0.0});
             const int rc = mlp_guess(c, g.x, g.z);
             if (rc) return rc;
         }
         // guess 1: keep the previous solution in d_mu
-        hipLaunchKernelGGL((k_apply_a<IND_LANES>), grid_rows, b256, 0, c->stream, a, g.x, g.z, nullptr, nullptr,
+        hipLaunchKernelGGL((k_apply_table<IND_LANES>), grid_rows, b256, 0, c->stream, a, g.x, g.z, nullptr, nullptr,
                            CgTop{nullptr, nullptr, 0, 0.0});
         hipLaunchKernelGGL(k_cg_start, grid_vec, dim3(CG_THREADS), 0, c->stream, g);
+        // count iterations from an even one: A v, the step, the direction
+        auto iterations = [&](int count) {
+            for (int k = 0; k < count; ++k) {
+                hipLaunchKernelGGL((k_apply_table<IND_LANES>), grid_rows, b256, 0, c->stream, a, g.v, g.z, g.v, g.p_vz,
+                                   CgTop{g.state, g.p_dv, g.nparts_vec, g.tol});
+                if (k & 1) {
+                    hipLaunchKernelGGL(k_cg_step<1>, grid_vec, dim3(CG_THREADS), 0, c->stream, g);
+                    hipLaunchKernelGGL(k_cg_dir<1>, grid_vec, dim3(CG_THREADS), 0, c->stream, g);
+                } else {
+                    hipLaunchKernelGGL(k_cg_step<0>, grid_vec, dim3(CG_THREADS), 0, c->stream, g);
+                    hipLaunchKernelGGL(k_cg_dir<0>, grid_vec, dim3(CG_THREADS), 0, c->stream, g);
+                }
+            }
+        };
+        // A full batch is replayed from a captured graph; it is captured again
+        // when any kernel argument changes (sizes, buffers, box, parameters).
+        std::string key;
+        auto put = [&key](const void* p, size_t bytes) { key.append(static_cast<const char*>(p), bytes); };
+        put(&a.n, sizeof a.n);
+        put(&a.box, sizeof a.box);
+        put(&a.pos, sizeof a.pos);
+        put(&a.polar, sizeof a.polar);
+        put(&a.tstride, sizeof a.tstride);
+        put(&a.tcnt, sizeof a.tcnt);
+        put(&a.tj, sizeof a.tj);
+        put(&a.ta, sizeof a.ta);
+        put(&a.tb, sizeof a.tb);
+        put(&g.dim, sizeof g.dim);
+        put(&g.nparts_mv, sizeof g.nparts_mv);
+        put(&g.nparts_vec, sizeof g.nparts_vec);
+        put(&g.tol, sizeof g.tol);
+        const void* gp[] = {g.x, g.r, g.v, g.z, g.e0, g.polar, g.p_vz, g.p_dv, g.p_rr0, g.p_rr1, g.state, g.scal};
+        put(gp, sizeof gp);
+        if (c->cg_exec && c->cg_key != key) {
+            (void)hipGraphExecDestroy(c->cg_exec);
+            c->cg_exec = nullptr;
+        }
         for (int it0 = 0; it0 < max_it; it0 += CG_BATCH) {
             const int it1 = std::min(max_it, it0 + CG_BATCH);
-            for (int it = it0; it < it1; ++it) {
-                hipLaunchKernelGGL((k_apply_a<IND_LANES>), grid_rows, b256, 0, c->stream, a, g.v, g.z, g.v, g.p_vz,
-                                   CgTop{g.state, g.p_dv, g.nparts_vec, g.tol});
-                hipLaunchKernelGGL(k_cg_step, grid_vec, dim3(CG_THREADS), 0, c->stream, g, it);
-                hipLaunchKernelGGL(k_cg_dir, grid_vec, dim3(CG_THREADS), 0, c->stream, g, it);
+            if (it1 - it0 == CG_BATCH) {
+                if (!c->cg_exec) {
+                    hipGraph_t graph = nullptr;
+                    SMD_HIP(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+                    iterations(CG_BATCH);
+                    SMD_HIP(c, hipStreamEndCapture(c->stream, &graph));
+                    const hipError_t e = hipGraphInstantiate(&c->cg_exec, graph, nullptr, nullptr, 0);
+                    (void)hipGraphDestroy(graph);
+                    if (e != hipSuccess) {
+                        c->cg_exec = nullptr;
+                        return hip_fail(c, e, "hipGraphInstantiate (CG batch)");
+                    }
+                    c->cg_key = key;
+                }
+                SMD_HIP(c, hipGraphLaunch(c->cg_exec, c->stream));
+            } else {
+                iterations(it1 - it0);
             }
             int st[3];
             SMD_HIP(c, hipMemcpyAsync(st, c->d_cg_state, sizeof st, hipMemcpyDeviceToHost, c->stream));

@@ -161,6 +161,14 @@ struct Ctx {
     double* d_cg_part = nullptr;  // [4][cg_parts] block partials
     int* d_cg_state = nullptr;    // [4] done, iterations, stop reason
     double* d_cg_scal = nullptr;  // [4] last residual norm
+    hipGraphExec_t cg_exec = nullptr;   // a captured batch of CG iterations
+    std::string cg_key;                 // the kernel arguments it was captured with
+    int it_stride = 0;            // coupling table of the solve (induction.hip)
+    int64_t it_cap = 0;
+    int32_t* d_it_cnt = nullptr;
+    int32_t* d_it_j = nullptr;
+    double* d_it_a = nullptr;
+    double* d_it_b = nullptr;
 
     // permanent multipoles and induced dipoles under Ewald
     struct KSet {                   // half-space k set of one Ewald class
