@@ -54,9 +54,13 @@ FLOP_PER_PAIR = 99
 #     sums 4 (the pass that forms S(k)); force 10 (the pass that uses it).
 FLOP_PER_ATOM_K_SFAC = 29
 FLOP_PER_ATOM_K_FORCE = 10
-#   induced dipoles: per pair within cutoffInd per CG iteration, the
-#     Thole-damped dipole tensor recomputed and applied to both sites' dipoles.
-FLOP_PER_PAIR_TMU = 76
+#   induced dipoles: per CG iteration, A v streams the solve's coupling table,
+#     4 B (partner) + 16 B (two Thole factors) per directed pair inside
+#     cutoffInd, and the CG vectors move 11 x 24 B per site (A v reads v and
+#     writes z; the step reads x, v, r, z and writes x, r; the direction reads
+#     r, v and writes v).
+BYTES_PER_DIRECTED_PAIR_TMU = 20
+BYTES_PER_SITE_CG = 11 * 24
 
 
 def log(msg):
@@ -170,10 +174,36 @@ def polar_cpu_baseline(system, reps=1):
     return base, forces
 
 
+def train_learned_guess(ctx, ind, nmol, seeds, epochs=60, batch=1024):
+    """Trains the learned guess on the GPU from converged solves of other
+    seeds' frames of the same workload; returns the network's report."""
+    from swinmd_amd import systems as S
+    from swinmd_amd.forces import LearnedGuess
+    net = LearnedGuess(ctx, seed=1)
+    guess = ind.params.guess
+    t0 = time.perf_counter()
+    for sd in seeds:
+        ctx.set_positions(S.polar_water(nmol, seed=sd).pos)
+        ctx.set_pairlist(POLAR["cutoff"], POLAR["cut_buff"], True)
+        ind.params.guess = 2
+        _, conv, _ = ind.induce()
+        if not conv:
+            raise RuntimeError("CG did not converge on a training frame")
+        net.add_frame()
+    frames_s = time.perf_counter() - t0
+    rep = net.train(epochs=epochs, batch=batch, lr=2e-3, lr_final=2e-5, seed=9)
+    ind.params.guess = guess
+    rep["frames_seconds"] = round(frames_s, 3)
+    rep["seeds"] = list(seeds)
+    return net, rep
+
+
 def bench_polar(args):
     """Polarizable water (charges, dipoles, quadrupoles, Thole-damped induced
     dipoles, buffered 14-7), FP64: one evaluation = the reference's force
-    sequence of LFIntegrator::run for this model."""
+    sequence of LFIntegrator::run for this model.  Timed twice on the same
+    frame: CG from the reference's start (Pi = 1) and CG from the learned
+    guess, both to the reference's tolerance."""
     from swinmd_amd import _lib as L
     from swinmd_amd import systems as S
     from swinmd_amd.forces import BuffvdWForce, Context, Induction, Multipole
@@ -191,6 +221,13 @@ def bench_polar(args):
     ind = Induction(ctx, cutoff_ind=POLAR["cutoff_ind"], a_damp=POLAR["a_damp"], damp_tij=True, do_min=True)
     ind.setup_terms(alpha=POLAR["alpha"], real_cut=POLAR["cutoff"], kcut=kcut, use_constraint=True, surface=False,
                     torques=True)
+    if args.no_graph:
+        ctx.set_tuning("cg_graph", 0)
+    log("training the learned guess on other seeds' frames")
+    net, train_rep = train_learned_guess(ctx, ind, nmol, seeds=(1, 2, 3) if nmol == 8192 else (1, 2, 3, 4, 5, 6))
+    ctx.set_positions(system.pos)
+    ctx.set_pairlist(POLAR["cutoff"], POLAR["cut_buff"], True)
+    ctx.compute_mol_centers()
     iters = []
 
     def evaluate():
@@ -205,53 +242,74 @@ def bench_polar(args):
         mp.compute_long()
         ind.compute_long()
 
-    for _ in range(max(1, args.warmup // 50)):
-        evaluate()
-    reps = max(1, args.steps // 40)
-    ctx.sync()
-    t0 = time.perf_counter()
-    for _ in range(reps):
-        evaluate()
-    ctx.sync()
-    secs = time.perf_counter() - t0
-    forces = ctx.forces()
-    ctx.profile(True)
-    for _ in range(3):
-        evaluate()
-    fam = {}
-    for f in POLAR_FAMILIES:
-        ms, cnt = ctx.profile_get(f)
-        if cnt:
-            fam[f] = round(ms * 1e3 / 3, 1)
-    ctx.profile(False)
-    rate = reps / secs
-    # roofline of the dipole solve, the dominant part: per CG iteration the
-    # Thole-damped T.mu over every pair within cutoffInd, 76 flop per pair when
-    # T is recomputed (the intramolecular pairs count too: A couples them)
+    runs = {}
+    for name, guess in (("reference_start", 0), ("learned_guess", 3)):
+        ind.params.guess = guess
+        iters.clear()
+        for _ in range(max(1, args.warmup // 50)):
+            evaluate()
+        reps = max(1, args.steps // 40)
+        ctx.sync()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            evaluate()
+        ctx.sync()
+        secs = time.perf_counter() - t0
+        forces = ctx.forces()
+        mu = ind.results()[3]
+        ctx.profile(True)
+        for _ in range(3):
+            evaluate()
+        fam = {}
+        for f in POLAR_FAMILIES:
+            ms, cnt = ctx.profile_get(f)
+            if cnt:
+                fam[f] = round(ms * 1e3 / 3, 1)
+        ctx.profile(False)
+        runs[name] = dict(rate=reps / secs, secs=secs, reps=reps, forces=forces, mu=mu, fam=fam,
+                          iters=sum(iters[-reps - 3:-3]) / reps)
+        log(f"{name}: {reps / secs:.2f} evaluations/s, {runs[name]['iters']:.1f} CG iterations")
+    ref, lrn = runs["reference_start"], runs["learned_guess"]
+    d = lrn["forces"] - ref["forces"]
+    force_dev = float(np.sqrt(np.sum(d * d) / np.sum(ref["forces"] ** 2)))
+    dm = lrn["mu"] - ref["mu"]
+    mu_dev = float(np.sqrt(np.sum(dm * dm) / np.sum(ref["mu"] ** 2)))
+    # roofline of the dipole solve: per CG iteration the coupling table
+    # streamed once plus the CG vectors; time per iteration from the CG
+    # family's event brackets (launch gaps and host checks included)
     ind_pairs, _ = unique_pairs_in_cutoff(ctx, POLAR["cutoff_ind"])
-    ind_pairs += 3 * nmol
-    it_avg = sum(iters[-3:]) / 3
-    cg_us = fam.get("ind_cg", 0.0) / max(it_avg, 1)
+    ind_pairs += 3 * nmol          # intramolecular pairs: A couples them too
+    it_avg = ref["iters"]
+    cg_us = ref["fam"].get("ind_cg", 0.0) / max(it_avg, 1)
     out_roof = None
     if cg_us > 0:
-        ach = FLOP_PER_PAIR_TMU * ind_pairs / (cg_us * 1e-6) / 1e12
-        out_roof = {"kernel": "CG iteration (T.mu with Thole recomputed + vector updates)", "bound": "valu",
-                    "achieved": round(ach, 4), "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
-                    "frac": round(ach / PEAK_FP64_TFLOPS, 5), "traffic": None, "pairs": ind_pairs,
-                    "flop_per_pair": FLOP_PER_PAIR_TMU, "us_per_iteration": round(cg_us, 2),
-                    "note": "time per iteration from event brackets around the whole CG (launch gaps and host "
-                            "checks included), so achieved is a lower bound"}
+        traffic = BYTES_PER_DIRECTED_PAIR_TMU * 2 * ind_pairs + BYTES_PER_SITE_CG * n
+        ach = traffic / (cg_us * 1e-6) / 1e9
+        out_roof = {"kernel": "CG iteration (A v from the coupling table + vector updates)", "bound": "hbm",
+                    "achieved": round(ach, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                    "frac": round(ach / PEAK_HBM_GBS, 4), "traffic": None, "bytes_per_iteration": traffic,
+                    "pairs": ind_pairs, "us_per_iteration": round(cg_us, 2),
+                    "note": "time per iteration from event brackets around the whole CG (launch gaps, the "
+                            "first A x and host checks included), so achieved is a lower bound"}
     out = {
-        "metric": "polarizable force evaluations/s (FP64, reference CG tolerance)", "value": round(rate, 3),
-        "unit": "evaluations/s", "n_gpus": 1, "steps": reps, "warmup": max(1, args.warmup // 50),
-        "ms_per_step": round(1e3 * secs / reps, 4), "higher_is_better": True, "scaling": "weak",
-        "vs_baseline": None, "dtype": "f64", "data": f"synthetic: seeded lattice water with random orientations",
+        "metric": "polarizable force evaluations/s (FP64, reference CG tolerance)", "value": round(lrn["rate"], 3),
+        "unit": "evaluations/s", "n_gpus": 1, "steps": lrn["reps"], "warmup": max(1, args.warmup // 50),
+        "ms_per_step": round(1e3 * lrn["secs"] / lrn["reps"], 4), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic: seeded lattice water with random orientations",
         "config": {"workload": f"polarizable water force evaluation, {nmol} molecules", "molecules": nmol,
                    "atoms": n, "box_nm": float(system.box[0]), "cutoff_nm": POLAR["cutoff"],
                    "alpha_per_nm": POLAR["alpha"], "n2max": N2MAX, "thole_a": POLAR["a_damp"],
-                   "cg_tolerance": "|r|_2 < 1e-14 (reference)", "parallelism": "single GPU"},
-        "cg_iterations_per_evaluation": round(sum(iters[-reps:]) / reps, 2),
-        "families_us_per_evaluation": fam,
+                   "cg_tolerance": "|r|_2 < 1e-14 (reference)", "cg_start": "learned guess",
+                   "parallelism": "single GPU"},
+        "cg_iterations_per_evaluation": round(lrn["iters"], 2),
+        "families_us_per_evaluation": lrn["fam"],
+        "reference_start": {"value": round(ref["rate"], 3), "ms_per_step": round(1e3 * ref["secs"] / ref["reps"], 4),
+                            "cg_iterations_per_evaluation": round(ref["iters"], 2),
+                            "families_us_per_evaluation": ref["fam"]},
+        "learned_vs_reference_start": round(lrn["rate"] / ref["rate"], 4),
+        "learned_force_rel_rms_vs_reference_start": force_dev,
+        "learned_dipole_rel_rms_vs_reference_start": mu_dev,
+        "learned_guess_training": train_rep,
     }
     if out_roof:
         out["roofline"] = out_roof
@@ -260,7 +318,7 @@ def bench_polar(args):
         base, fref = polar_cpu_baseline(system)
         if base:
             out["cpu_baseline"] = base
-            d = forces - fref
+            d = lrn["forces"] - fref
             out["force_rel_rms_vs_reference"] = float(np.sqrt(np.sum(d * d) / np.sum(fref * fref)))
     print(json.dumps(out), flush=True)
 

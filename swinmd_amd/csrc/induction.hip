@@ -712,7 +712,7 @@ int induce_run(Ctx* c, const smd_induce_params* p, smd_induce_report* rep) {
         }
         for (int it0 = 0; it0 < max_it; it0 += CG_BATCH) {
             const int it1 = std::min(max_it, it0 + CG_BATCH);
-            if (it1 - it0 == CG_BATCH) {
+            if (it1 - it0 == CG_BATCH && c->cg_graph) {
                 if (!c->cg_exec) {
                     hipGraph_t graph = nullptr;
                     SMD_HIP(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
