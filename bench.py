@@ -43,7 +43,7 @@ PEAK_HBM_GBS = 8000.0
 MD = dict(cutoff=0.9, cut_buff=0.1, alpha=3.5, dt=0.002, temperature=300.0, coupl=0.1, tol=1e-8)
 N2MAX = 81
 FAMILIES = ["pairlist", "pairs", "ewald_intra", "recip_sfac", "recip_reduce", "recip_force", "recip_fsum",
-            "shake", "kinetics"]
+            "shake", "kinetics", "shake_kinetics"]
 
 # Algorithmic work of the reference method per unit.  Counting rule: add,
 # sub, mul 1; FMA 2; div, sqrt, rsqrt 1; exp 12; erfc 25; sin+cos 20.

@@ -498,6 +498,8 @@ int smd_md_time_family(smd_ctx* ctx, const char* family, int reps, double* avg_u
  * them beyond FP rounding order).  "pair_packed" (0/1, default 1): the MD
  * step's mixed LJ + Ewald pairs read packed FP32 hi/lo site records.
  * "recip_overlap" (0/1, default 1): the MD step's k-space on a second stream.
+ * "md_fused_shake" (0/1, default 1): the MD step runs SHAKE and the kinetic
+ * and pressure sums as one kernel (0: two kernels).
  * "cg_graph" (0/1, default 1): smd_induce replays each batch of 16 CG
  * iterations from a captured hipGraph (0: direct launches, for profilers).
  * "pair_lanes": lanes sharing one atom's row

@@ -305,6 +305,9 @@ extern "C" int smd_set_tuning(smd_ctx* c, const char* name, int value) {
     } else if (std::string(name) == "pair_packed") {
         if (value != 0 && value != 1) return fail(c, SMD_EINVAL, "pair_packed must be 0 or 1 (got %d)", value);
         c->pair_packed = value;
+    } else if (std::string(name) == "md_fused_shake") {
+        if (value != 0 && value != 1) return fail(c, SMD_EINVAL, "md_fused_shake must be 0 or 1 (got %d)", value);
+        c->md_fused_shake = value;
     } else if (std::string(name) == "cg_graph") {
         if (value != 0 && value != 1) return fail(c, SMD_EINVAL, "cg_graph must be 0 or 1 (got %d)", value);
         c->cg_graph = value;

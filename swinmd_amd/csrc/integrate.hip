@@ -70,10 +70,13 @@ __global__ __launch_bounds__(256) void k_step_head(IntegArgs a) {
     block_add<4>(v, a.terms + T_VOM);
 }
 
-__global__ __launch_bounds__(256) void k_shake(IntegArgs a) {
-    double v[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-    const int m = blockIdx.x * blockDim.x + threadIdx.x;
-    if (m < a.nmol) {
+// SHAKE of one water (a.mol_start[m], three sites).  Stores the new
+// positions, displacements, unwrapped positions, velocities and vtMinus, adds
+// its constraint virial to v, and leaves the new positions and velocities,
+// the forces it used and the masses in pn, vel, F and ms.
+__device__ __forceinline__ void shake_mol(const IntegArgs& a, int m, double (&v)[9], double (&pn)[3][3],
+                                          double (&vel)[3][3], double (&F)[3][3], double (&ms)[3]) {
+    {
         const int a0 = a.mol_start[m];
         const double fnh = a.fnh_on_device ? a.scalars[0] : a.fnh_host;
         const double Px = a.mom[3 * m], Py = a.mom[3 * m + 1], Pz = a.mom[3 * m + 2];
@@ -91,22 +94,24 @@ __global__ __launch_bounds__(256) void k_shake(IntegArgs a) {
 #pragma unroll
         for (int s = 0; s < 3; ++s) {
             const int id = a0 + s;
-            mr[s] = 1.0 / a.mass[id];
+            ms[s] = a.mass[id];
+            mr[s] = 1.0 / ms[s];
             const double h = a.dt * mr[s];
             for (int d = 0; d < 3; ++d) {
                 ro[s][d] = a.pos[3 * id + d];
                 vt[s][d] = a.vtm[3 * id + d] - vcm[d];
-                double F = a.force[3 * id + d];
+                double Fd = a.force[3 * id + d];
                 if (a.frecip) {   // the sum the reference forms: short-range terms, then k-space
-                    F += a.frecip[3 * id + d];
-                    a.force[3 * id + d] = F;
+                    Fd += a.frecip[3 * id + d];
+                    a.force[3 * id + d] = Fd;
                 }
+                F[s][d] = Fd;
                 double vp;
                 if (a.nose_hoover) {
                     const double P = d == 0 ? Px : (d == 1 ? Py : Pz);
-                    vp = vt[s][d] + h * (F - (fnh * P) / 3.0);
+                    vp = vt[s][d] + h * (Fd - (fnh * P) / 3.0);
                 } else {
-                    vp = vt[s][d] + h * F;
+                    vp = vt[s][d] + h * Fd;
                 }
                 rn[s][d] = ro[s][d] + a.dt * vp;
             }
@@ -174,14 +179,25 @@ __global__ __launch_bounds__(256) void k_shake(IntegArgs a) {
             for (int d = 0; d < 3; ++d) {
                 const double p0 = a.pos[3 * id + d];
                 const double dr = rn[s][d] - p0;
-                a.pos[3 * id + d] = p0 + dr;
+                pn[s][d] = p0 + dr;
+                a.pos[3 * id + d] = pn[s][d];
                 a.disp[3 * id + d] += dr;
                 a.real[3 * id + d] += dr;
                 const double vp = dr / a.dt;
-                a.vel[3 * id + d] = (vp + vt[s][d]) * 0.5;
+                vel[s][d] = (vp + vt[s][d]) * 0.5;
+                a.vel[3 * id + d] = vel[s][d];
                 a.vtm[3 * id + d] = vp;
             }
         }
+    }
+}
+
+__global__ __launch_bounds__(256) void k_shake(IntegArgs a) {
+    double v[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    const int m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m < a.nmol) {
+        double pn[3][3], vel[3][3], F[3][3], ms[3];
+        shake_mol(a, m, v, pn, vel, F, ms);
     }
     block_add<9>(v, a.terms + T_SH_VIR);
 }
@@ -311,6 +327,32 @@ __device__ void md_pressure(const IntegArgs& a, const double* t) {
 // spread over as many CUs as there are waves of molecules.
 constexpr int KIN_THREADS = 64;
 
+// The MD step's last block (atomic ticket): Nose-Hoover update, pressure
+// tensors and stress sums from the sums every block added.
+__device__ __forceinline__ void step_epilogue(const IntegArgs& a) {
+    __shared__ bool last;
+    __threadfence();
+    __syncthreads();
+    if (threadIdx.x == 0) last = atomicAdd(&a.flag[2], 1) == (int)gridDim.x - 1;
+    __syncthreads();
+    if (last) {   // block-uniform
+        // the sums every block added, read once by the whole block (the
+        // epilogue's thread then works from LDS, not from a chain of
+        // dependent global loads)
+        __shared__ double tl[T_COUNT];
+        __threadfence();
+        volatile const double* tv = a.terms;
+        for (int k = threadIdx.x; k < T_COUNT; k += blockDim.x) tl[k] = tv[k];
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            nh_friction(a, tl[T_KE_ATOM], tl[T_KE_MOL]);
+            md_pressure(a, tl);
+            for (int k = T_VOM; k <= T_MTOT; ++k) a.terms[k] = 0.0;
+            a.flag[2] = 0;
+        }
+    }
+}
+
 __global__ __launch_bounds__(KIN_THREADS) void k_kinetics(IntegArgs a, int rewrap) {
     double v[2] = {0.0, 0.0};
     double pr[21];   // MD step: sum f (x) (r - R) [9], atom p p / m [6], molecule P P / M [6]
@@ -371,28 +413,71 @@ __global__ __launch_bounds__(KIN_THREADS) void k_kinetics(IntegArgs a, int rewra
     if (a.md_step) {
         __syncthreads();
         block_add<21>(pr, a.terms + T_PR_MC);
-        __shared__ bool last;
-        __threadfence();
-        __syncthreads();
-        if (threadIdx.x == 0) last = atomicAdd(&a.flag[2], 1) == (int)gridDim.x - 1;
-        __syncthreads();
-        if (last) {   // block-uniform
-            // the sums every block added, read once by the whole block (the
-            // epilogue's thread then works from LDS, not from a chain of
-            // dependent global loads)
-            __shared__ double tl[T_COUNT];
-            __threadfence();
-            volatile const double* tv = a.terms;
-            for (int k = threadIdx.x; k < T_COUNT; k += blockDim.x) tl[k] = tv[k];
-            __syncthreads();
-            if (threadIdx.x == 0) {
-                nh_friction(a, tl[T_KE_ATOM], tl[T_KE_MOL]);
-                md_pressure(a, tl);
-                for (int k = T_VOM; k <= T_MTOT; ++k) a.terms[k] = 0.0;
-                a.flag[2] = 0;
-            }
-        }
+        step_epilogue(a);
     }
+}
+
+// MD step, rigid water: SHAKE of each molecule and, from the positions and
+// velocities still in registers, its kinetic and pressure sums (the
+// arithmetic of k_kinetics), then the step's epilogue in the last block.
+__global__ __launch_bounds__(KIN_THREADS) void k_shake_kinetics(IntegArgs a, int rewrap) {
+    double sv[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    double v[2] = {0.0, 0.0};
+    double pr[21];
+#pragma unroll
+    for (int k = 0; k < 21; ++k) pr[k] = 0.0;
+    const int m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m < a.nmol) {
+        double pn[3][3], vel[3][3], F[3][3], ms[3];
+        shake_mol(a, m, sv, pn, vel, F, ms);
+        const int s0 = a.mol_start[m];
+        const double M = (0.0 + ms[0] + ms[1]) + ms[2];
+        double P[3] = {0, 0, 0}, C[3] = {0, 0, 0};
+#pragma unroll
+        for (int s = 0; s < 3; ++s) {
+            double vv = 0.0;
+#pragma unroll
+            for (int d = 0; d < 3; ++d) {
+                P[d] += ms[s] * vel[s][d];
+                C[d] += ms[s] * pn[s][d];
+                vv += vel[s][d] * vel[s][d];
+            }
+            v[0] += vv * ms[s];
+        }
+        for (int d = 0; d < 3; ++d) C[d] = C[d] / M;
+        v[1] = (P[0] * P[0] + P[1] * P[1] + P[2] * P[2]) / M;
+#pragma unroll
+        for (int s = 0; s < 3; ++s) {
+            double p[3], dr[3];
+            for (int d = 0; d < 3; ++d) {
+                p[d] = vel[s][d] * ms[s];
+                dr[d] = pn[s][d] - C[d];
+            }
+            for (int x = 0; x < 3; ++x)
+                for (int y = 0; y < 3; ++y) pr[3 * x + y] += F[s][y] * dr[x];
+            const double rm = 1.0 / ms[s];
+            pr[9] += p[0] * rm * p[0]; pr[10] += p[0] * rm * p[1]; pr[11] += p[0] * rm * p[2];
+            pr[12] += p[1] * rm * p[1]; pr[13] += p[1] * rm * p[2]; pr[14] += p[2] * rm * p[2];
+        }
+        const double rM = 1.0 / M;
+        pr[15] += P[0] * rM * P[0]; pr[16] += P[0] * rM * P[1]; pr[17] += P[0] * rM * P[2];
+        pr[18] += P[1] * rM * P[1]; pr[19] += P[1] * rM * P[2]; pr[20] += P[2] * rM * P[2];
+        for (int d = 0; d < 3; ++d) a.mom[3 * m + d] = P[d];
+        if (rewrap) {
+            double cx = C[0], cy = C[1], cz = C[2];
+            pbc(cx, cy, cz, a.box);
+            const double dx = cx - C[0], dy = cy - C[1], dz = cz - C[2];
+            for (int i = s0; i < s0 + 3; ++i) { a.pos[3 * i] += dx; a.pos[3 * i + 1] += dy; a.pos[3 * i + 2] += dz; }
+            C[0] = cx; C[1] = cy; C[2] = cz;
+        }
+        for (int d = 0; d < 3; ++d) a.com[3 * m + d] = C[d];
+    }
+    block_add<9>(sv, a.terms + T_SH_VIR);
+    __syncthreads();
+    block_add<2>(v, a.terms + T_KE_ATOM);
+    __syncthreads();
+    block_add<21>(pr, a.terms + T_PR_MC);
+    step_epilogue(a);
 }
 
 __global__ __launch_bounds__(256) void k_mol_centers(int nmol, const int32_t* __restrict__ ms,
@@ -557,6 +642,25 @@ int kinetics_run(Ctx* c, bool rewrap, const smd_integrator_params* md) {
     return SMD_OK;
 }
 
+// The MD step's SHAKE and kinetics in one kernel (rigid 3-site water).
+static int shake_kinetics_run(Ctx* c, const smd_integrator_params* md) {
+    IntegArgs a = integ_args(c, md);
+    a.fnh_on_device = 1;
+    a.md_step = 1;
+    a.frecip = c->shake_frecip ? c->d_frecip : nullptr;
+    a.ndf = md_ndf(c);
+    a.surface = c->md.ewald.surface;
+    a.use_constraint = c->md.ewald.use_constraint;
+    a.lustig = c->md.lustig;
+    a.stress = c->d_stress;
+    a.stress_after = 0.15 * (double)c->md.total_steps;
+    ProfScope ps(c, "shake_kinetics");
+    hipLaunchKernelGGL(k_shake_kinetics, dim3((c->nmol + KIN_THREADS - 1) / KIN_THREADS), dim3(KIN_THREADS), 0,
+                       c->stream, a, 1);
+    SMD_HIP(c, hipGetLastError());
+    return SMD_OK;
+}
+
 int lf_atoms_run(Ctx* c, const smd_integrator_params* p, double fnh) {
     IntegArgs a = integ_args(c, p);
     a.fnh_host = fnh;
@@ -604,11 +708,17 @@ int md_step_enqueue(Ctx* c, bool do_long) {
     if (p.ewald.surface && (rc = ewald_surface(c))) return rc;
     if (fork) SMD_HIP(c, hipStreamWaitEvent(c->stream, c->ev_join, 0));
     else if (do_long && (rc = ewald_recip(c, p.prec, 1))) return rc;
+    // drift removal, SHAKE (adds d_frecip), clears the rebuild flag; then the
+    // kinetic and pressure sums, the NH update and the per-step rewrap
     c->shake_frecip = fork;
-    rc = shake_run(c, &p.integ, true, 0.0, true);   // drift removal, SHAKE (adds d_frecip), clears the flag
+    if (c->md_fused_shake) {
+        rc = shake_kinetics_run(c, &p.integ);
+    } else {
+        rc = shake_run(c, &p.integ, true, 0.0, true);
+        if (!rc) rc = kinetics_run(c, true, &p.integ);
+    }
     c->shake_frecip = false;
     if (rc) return rc;
-    if ((rc = kinetics_run(c, true, &p.integ))) return rc;         // kinetics, then the NH update
     SMD_HIP(c, hipGetLastError());
     return SMD_OK;
 }

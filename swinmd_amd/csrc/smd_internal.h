@@ -207,6 +207,7 @@ struct Ctx {
     bool recip_store = false;         // k_recip_force stores into d_frecip instead of adding
     bool shake_frecip = false;        // k_shake adds d_frecip to the forces it reads
     int recip_overlap = 1;
+    int md_fused_shake = 1;           // the MD step's SHAKE and kinetics in one kernel
 
     Prof prof;
     int pair_lanes = 0;           // lanes per atom row in the pair kernels (0: by precision)

@@ -236,6 +236,31 @@ def test_md_graph_replay_equals_eager():
 
 
 @pytest.mark.parametrize("graph", [False, True])
+def test_md_fused_shake_kinetics_equals_two_kernels(graph):
+    """SHAKE and the kinetic/pressure sums as one kernel give the two-kernel
+    step's trajectory, pressure tensors and thermostat."""
+    from swinmd_amd.forces import WaterMD
+    fx = Fixture("water512_md10")
+    c = fx.cfg
+    outs = []
+    for fused in (0, 1):
+        ctx = _ctx(fx.system)
+        ctx.set_tuning("md_fused_shake", fused)
+        ctx.set_velocities(fx.vec("init_vel"))
+        md = WaterMD(ctx, cutoff=c["cutoff"], cut_buff=c["cut_buff"], alpha=c["alpha"], kcut=c["kcut"], dt=c["dt"],
+                     temperature=c["temperature"], coupl=c["coupl"], use_graph=graph)
+        md.init()
+        rep = md.run(10)
+        outs.append((ctx.positions(), ctx.velocities(), md.pressure(), md.thermostat(), rep))
+    assert rel_rms(outs[1][0], outs[0][0]) <= 1e-11
+    assert rel_rms(outs[1][1], outs[0][1]) <= 1e-9
+    for k in ("atom_pressure", "mol_pressure", "virial"):
+        assert rel_rms(outs[1][2][k], outs[0][2][k]) <= 1e-9, k
+    assert rel(outs[1][3][0], outs[0][3][0]) <= 1e-9
+    assert outs[1][4]["list_updates"] == outs[0][4]["list_updates"]
+
+
+@pytest.mark.parametrize("graph", [False, True])
 def test_md_recip_on_second_stream_equals_one_stream(graph):
     """Reciprocal space forked onto the second stream (forces kept apart and
     added in SHAKE) gives the one-stream step's trajectory."""
