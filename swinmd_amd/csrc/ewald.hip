@@ -367,11 +367,14 @@ __global__ __launch_bounds__(SF_WAVES * WAVE) void k_recip_sfac(RecipArgs a) {
     const int gnx = grp_nx(gg.z), gny = grp_ny(gg.z);
     const int ix = gnx, iy = ey0 + abs(gny), iz = ez0 + abs(gg.w);
     const R sy = gny < 0 ? R(-1) : R(1), sz = gg.w < 0 ? R(-1) : R(1);
-    double acc[GK][NSUM];
+    // Sums of this lane's k-group over the wave's atoms of the chunk (a few
+    // dozen), in the pass's precision; they enter FP64 at the cross-wave sum,
+    // and chunks are added in FP64 by k_recip_reduce.
+    R acc[GK][NSUM];
 #pragma unroll
     for (int t = 0; t < GK; ++t)
 #pragma unroll
-        for (int u = 0; u < NSUM; ++u) acc[t][u] = 0.0;
+        for (int u = 0; u < NSUM; ++u) acc[t][u] = R(0);
 
     // Tile fill: thread fu + SF_TILE fp writes atom fu's powers along axis fp
     // (fp < 3) or its moments (fp == 3).  Its inputs for the next tile are
@@ -431,12 +434,12 @@ __global__ __launch_bounds__(SF_WAVES * WAVE) void k_recip_sfac(RecipArgs a) {
                 // the group's count are never stored
 #pragma unroll
                 for (int t = 0; t < GK; ++t) {
-                    acc[t][0] += (double)e.im;
-                    acc[t][1] += (double)e.re;
+                    acc[t][0] += e.im;
+                    acc[t][1] += e.re;
                     if constexpr (LUSTIG) {
-                        acc[t][2] += (double)(e.re * dx); acc[t][3] += (double)(e.re * dy); acc[t][4] += (double)(e.re * dz);
-                        acc[t][5] += (double)(e.im * dx); acc[t][6] += (double)(e.im * dy); acc[t][7] += (double)(e.im * dz);
-                        acc[t][8] += (double)(e.re * d2); acc[t][9] += (double)(e.im * d2);
+                        acc[t][2] += e.re * dx; acc[t][3] += e.re * dy; acc[t][4] += e.re * dz;
+                        acc[t][5] += e.im * dx; acc[t][6] += e.im * dy; acc[t][7] += e.im * dz;
+                        acc[t][8] += e.re * d2; acc[t][9] += e.im * d2;
                     }
                     e = cmul(e, z1);
                 }
@@ -451,13 +454,13 @@ __global__ __launch_bounds__(SF_WAVES * WAVE) void k_recip_sfac(RecipArgs a) {
         __syncthreads();
         if (w > 0)
 #pragma unroll
-            for (int u = 0; u < NU; ++u) red[((w - 1) * WAVE + lane) * NSUM + u] = acc[t][u];
+            for (int u = 0; u < NU; ++u) red[((w - 1) * WAVE + lane) * NSUM + u] = (double)acc[t][u];
         __syncthreads();
         if (w == 0 && g < a.ng && t < gg.y) {
             double* p = a.part + ((size_t)ch * a.K + gg.x + t) * NSUM;
 #pragma unroll
             for (int u = 0; u < NSUM; ++u) {
-                double s = acc[t][u];
+                double s = (double)acc[t][u];
                 if (u < NU)
                     for (int v = 0; v < SF_WAVES - 1; ++v) s += red[(v * WAVE + lane) * NSUM + u];
                 p[u] = s;
@@ -674,11 +677,16 @@ int ewald_recip(Ctx* c, int prec, int lustig, unsigned passes) {
                            a.vk[0], a.vk[1], a.vk[2], a.e1, c->d_q, c->d_mol, c->d_com, a.am);
         const dim3 g1((a.ng + WAVE - 1) / WAVE, a.chunks);
         const dim3 b1(SF_WAVES * WAVE);
-        // FP64 in both forms: its sums are FP64 FMAs, which cost what FP32
-        // products converted and added in FP64 would.
-        const size_t lds = c->sfac_lds[0];
-        if (lustig) hipLaunchKernelGGL((k_recip_sfac<double, true>), g1, b1, lds, c->stream, a);
-        else hipLaunchKernelGGL((k_recip_sfac<double, false>), g1, b1, lds, c->stream, a);
+        // mixed: FP32 phases, products and per-wave sums over the chunk's
+        // atoms; FP64 from the cross-wave sum on
+        const size_t lds = c->sfac_lds[f];
+        if (mixed) {
+            if (lustig) hipLaunchKernelGGL((k_recip_sfac<float, true>), g1, b1, lds, c->stream, a);
+            else hipLaunchKernelGGL((k_recip_sfac<float, false>), g1, b1, lds, c->stream, a);
+        } else {
+            if (lustig) hipLaunchKernelGGL((k_recip_sfac<double, true>), g1, b1, lds, c->stream, a);
+            else hipLaunchKernelGGL((k_recip_sfac<double, false>), g1, b1, lds, c->stream, a);
+        }
     }
     if (passes & RECIP_REDUCE) {
         ProfScope ps(c, "recip_reduce");
