@@ -13,6 +13,8 @@
 // whole-system sums are block-reduced FP64 atomics.  The step is a fixed
 // sequence of kernels whose list rebuild is gated on a device flag, so it can
 // be captured once into a hipGraph and replayed.
+#include <type_traits>
+
 #include "smd_internal.h"
 
 namespace smd {
@@ -143,32 +145,46 @@ __device__ __forceinline__ void shake_mol(const IntegArgs& a, int m, double (&v)
         // Gauss-Seidel sweeps (LFIntegrator.cpp:1499-1540).  The update is
         // predicated instead of branched: an inactive constraint moves its
         // atoms by exactly 0.0, and its quotient is discarded by a select.
+        // The sweep loop is compiled twice, with and without the minimum
+        // image, and a wave takes the plain one unless one of its molecules
+        // needs wrapping.  The constraint virial sum_s f_s n_s (x) o is kept
+        // as sum_s f_s n_s per constraint and multiplied by o once.
         int count = 0;
         bool done = false, failed = false;
-        while (!done && count < a.max_count && !failed) {
-            done = true;
+        double wv[3][3] = {{0.0, 0.0, 0.0}, {0.0, 0.0, 0.0}, {0.0, 0.0, 0.0}};
+        auto sweeps = [&](auto wrap_tag) {
+            constexpr bool WRAP = decltype(wrap_tag)::value;
+            while (!done && count < a.max_count && !failed) {
+                done = true;
 #pragma unroll
-            for (int b = 0; b < 3; ++b) {
-                const int nx = b == 2 ? 0 : b + 1;
-                double nX = rn[b][0] - rn[nx][0], nY = rn[b][1] - rn[nx][1], nZ = rn[b][2] - rn[nx][2];
-                if (wrap) pbc(nX, nY, nZ, a.box);
-                const double diff = bl[b] - len2_exact(nX, nY, nZ);
-                const double oX = od[b][0], oY = od[b][1], oZ = od[b][2];
-                const double rno = oX * nX + oY * nY + oZ * nZ;
-                const bool act = fabs(diff) > bl[b] * tol_2;
-                failed = failed || (act && rno < bl[b] * a.tol);
-                const double q = diff / (2 * rno * msum[b]);
-                const double factor = act ? q : 0.0;
-                v[0] += factor * nX * oX; v[1] += factor * nX * oY; v[2] += factor * nX * oZ;
-                v[3] += factor * nY * oX; v[4] += factor * nY * oY; v[5] += factor * nY * oZ;
-                v[6] += factor * nZ * oX; v[7] += factor * nZ * oY; v[8] += factor * nZ * oZ;
-                const double fX = factor * oX, fY = factor * oY, fZ = factor * oZ;
-                rn[b][0] += mr[b] * fX; rn[b][1] += mr[b] * fY; rn[b][2] += mr[b] * fZ;
-                rn[nx][0] -= mr[nx] * fX; rn[nx][1] -= mr[nx] * fY; rn[nx][2] -= mr[nx] * fZ;
-                done = done && !act;
+                for (int b = 0; b < 3; ++b) {
+                    const int nx = b == 2 ? 0 : b + 1;
+                    double nX = rn[b][0] - rn[nx][0], nY = rn[b][1] - rn[nx][1], nZ = rn[b][2] - rn[nx][2];
+                    if (WRAP && wrap) pbc(nX, nY, nZ, a.box);
+                    const double diff = bl[b] - len2_exact(nX, nY, nZ);
+                    const double oX = od[b][0], oY = od[b][1], oZ = od[b][2];
+                    const double rno = oX * nX + oY * nY + oZ * nZ;
+                    const bool act = fabs(diff) > bl[b] * tol_2;
+                    failed = failed || (act && rno < bl[b] * a.tol);
+                    const double q = diff / (2 * rno * msum[b]);
+                    const double factor = act ? q : 0.0;
+                    wv[b][0] += factor * nX; wv[b][1] += factor * nY; wv[b][2] += factor * nZ;
+                    const double fX = factor * oX, fY = factor * oY, fZ = factor * oZ;
+                    rn[b][0] += mr[b] * fX; rn[b][1] += mr[b] * fY; rn[b][2] += mr[b] * fZ;
+                    rn[nx][0] -= mr[nx] * fX; rn[nx][1] -= mr[nx] * fY; rn[nx][2] -= mr[nx] * fZ;
+                    done = done && !act;
+                }
+                count++;
             }
-            count++;
-        }
+        };
+        if (__any(wrap)) sweeps(std::true_type{});
+        else sweeps(std::false_type{});
+#pragma unroll
+        for (int b = 0; b < 3; ++b)
+#pragma unroll
+            for (int x = 0; x < 3; ++x)
+#pragma unroll
+                for (int y = 0; y < 3; ++y) v[3 * x + y] += wv[b][x] * od[b][y];
         if (a.md_step && m == 0) a.flag[0] = 0;   // every consumer of the rebuild flag has run
         if (failed) atomicOr(a.status, ST_SHAKE_FAIL);
         else if (!done) atomicOr(a.status, ST_SHAKE_NOCONV);

@@ -94,7 +94,7 @@ struct Ctx {
     double* d_bond2 = nullptr;    // [3nmol]
     double* d_terms = nullptr;    // [T_COUNT]
     unsigned* d_status = nullptr;
-    int* d_flag = nullptr;        // [4]: 0 rebuild-needed, 1 drift-apply
+    int* d_flag = nullptr;        // [4]: 0 rebuild-needed, 1 drift-apply, 2 kinetics ticket
     double* d_scalars = nullptr;  // [8]: fnh, fnh_minus, ...
     double* h_pinned = nullptr;   // pinned staging for small results
 
