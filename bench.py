@@ -338,6 +338,8 @@ def main():
                     help="MPI ranks of the reference CPU run (the GPU's share of host cores)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-fp64", action="store_true", help="skip the plain FP64 comparison run")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="keep the default launch forms instead of timing every combination first")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="first copy MIB MiB device to device (counter calibration for profiling runs)")
     ap.add_argument("--no-graph", action="store_true",
@@ -372,11 +374,19 @@ def main():
     ctx.close()
 
     results = {}
+    tuning = None
     for prec, name in ((L.PREC_MIXED, "mixed"), (L.PREC_FP64, "fp64")):
         if name == "fp64" and args.no_fp64:
             continue
         ctx, md = build_md(system, prec, pos_eq, vel_eq, use_graph=use_graph)
         f_start = ctx.forces()
+        if name == "mixed" and not args.no_autotune:
+            # launch forms picked on the live system (untimed, before the warm-up)
+            from swinmd_amd.autotune import tune_md_step
+            best, table = tune_md_step(md)
+            tuning = {"chosen": best, "candidates": [
+                {"config": t["config"], "median_steps_per_s": round(t["median_steps_per_s"], 1)} for t in table]}
+            log(f"autotune: {best}")
         md.run(args.warmup)
         r0 = md.run(0)
         secs, r = time_steps(ctx, md, args.steps)
@@ -429,6 +439,8 @@ def main():
 
     fam = mix["families"]
     out["kernels_us"] = {k: round(v["us"], 2) for k, v in fam.items()}
+    if tuning:
+        out["autotune"] = tuning
     # roofline of the dominant modelled kernel
     K = count_kvectors(system.box[0])
     models = {}

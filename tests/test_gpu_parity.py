@@ -413,3 +413,39 @@ def test_md_packed_pair_records_match_unpacked():
     assert rel_rms(fp, f64) <= 1e-5 and rel_rms(fu, f64) <= 1e-5
     for k in ("e_lj", "e_real"):
         assert rel(out["packed"][1][k], out["plain"][1][k]) <= 1e-5
+
+
+def test_md_autotuner_picks_a_candidate_and_keeps_the_physics():
+    """The launch-shape tuner times every combination of its knobs on the
+    live run and leaves the fastest set; the step then gives the same
+    trajectory as the default forms from the same state."""
+    from swinmd_amd.autotune import DEFAULT_KNOBS, tune_md_step
+    from swinmd_amd.forces import WaterMD
+    fx = Fixture("water512_md10")
+    c = fx.cfg
+    kw = dict(cutoff=c["cutoff"], cut_buff=c["cut_buff"], alpha=c["alpha"], kcut=c["kcut"], dt=c["dt"],
+              temperature=c["temperature"], coupl=c["coupl"], prec=1)
+    ctx = _ctx(fx.system)
+    ctx.set_velocities(fx.vec("init_vel"))
+    md = WaterMD(ctx, **kw)
+    md.init()
+    best, table = tune_md_step(md, steps=10, warmup=4, rounds=1)
+    print("autotune", best, [(t["config"], round(t["median_steps_per_s"])) for t in table], flush=True)
+    assert len(table) == 8 and all(t["median_steps_per_s"] > 0 for t in table)
+    assert best in [t["config"] for t in table] and set(best) == set(DEFAULT_KNOBS)
+    # from the tuned run's state: the chosen forms and the default forms agree
+    pos, vel = ctx.positions(), ctx.velocities()
+    outs = []
+    for cfg in (best, {k: v[0] for k, v in DEFAULT_KNOBS.items()}):
+        ctx2 = _ctx(fx.system)
+        for k, v in cfg.items():
+            ctx2.set_tuning(k, v)
+        ctx2.set_positions(pos)
+        ctx2.set_velocities(vel)
+        md2 = WaterMD(ctx2, **kw)
+        md2.init()
+        md2.run(10)
+        outs.append(ctx2.positions())
+        ctx2.close()
+    assert rel_rms(outs[0], outs[1]) <= 1e-7   # mixed forms: FP32 rounding differs between pair-kernel forms
+    ctx.close()
