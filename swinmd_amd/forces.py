@@ -641,3 +641,11 @@ class LearnedGuess:
     def load_state(self, st):
         self.set_params(st["params"])
         self.set_norm(st["norm"])
+
+    def save(self, path):
+        """Writes the trained network (parameters and normalisation) as .npz."""
+        np.savez(path, params=self.params(), norm=self.norm())
+
+    def load(self, path):
+        with np.load(path) as st:          # arrays only; nothing is unpickled
+            self.load_state({"params": st["params"], "norm": st["norm"]})

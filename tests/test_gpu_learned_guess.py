@@ -102,7 +102,6 @@ def test_descriptors_match_oracle():
 
 
 def _frames(net, ctx, ind, seeds):
-    from swinmd_amd import systems as S
     for sd in seeds:
         s = _system(sd)
         ctx.set_positions(s.pos)
@@ -114,11 +113,13 @@ def _frames(net, ctx, ind, seeds):
     return rows
 
 
-def test_training_reduces_iterations_and_keeps_the_solution():
+def test_training_reduces_iterations_and_keeps_the_solution(tmp_path):
     """Ablation: on a frame the network never saw, CG from the learned guess
     and CG from alpha E0 (the best guess without the network) stop at the same
     tolerance on the same dipoles; the learned start needs fewer iterations.
-    Training twice from the same seed gives the same network bit for bit."""
+    A saved network loaded into another context starts CG at the same point,
+    and training twice from the same seed gives the same network bit for
+    bit."""
     from swinmd_amd.forces import LearnedGuess
     ctx, ind = _ctx(_system(31))
     net = LearnedGuess(ctx, seed=1)
@@ -142,6 +143,15 @@ def test_training_reduces_iterations_and_keeps_the_solution():
     print("iterations (reference start, alpha E0, learned):", out[0][0], out[2][0], out[3][0], flush=True)
     assert rel_rms(out[3][1], out[0][1]) <= 1e-10 and rel_rms(out[2][1], out[0][1]) <= 1e-10
     assert out[3][0] < out[2][0] < out[0][0]
+    # a saved network loaded into another context gives the same start
+    net.save(tmp_path / "net.npz")
+    ctx3, ind3 = _ctx(_system(31))
+    LearnedGuess(ctx3, seed=5).load(tmp_path / "net.npz")
+    ind3.params.guess = 3
+    it3, conv3, _ = ind3.induce()
+    assert conv3 and it3 == out[3][0]
+    assert rel_rms(ind3.results()[3], out[3][1]) <= 1e-13
+    ctx3.close()
     # determinism
     net2 = LearnedGuess(ctx, seed=1)
     net2.train(epochs=40, batch=512, lr=2e-3, lr_final=2e-5, seed=9)
