@@ -260,6 +260,11 @@ int smd_induce(smd_ctx* ctx, const smd_induce_params* p, smd_induce_report* repo
  * the permanent field Ei0 [3n] and the induced dipoles Pi [3n] of the last
  * smd_induce; any pointer may be NULL. */
 int smd_get_induction(smd_ctx* ctx, double* dip, double* quad, double* e0, double* mu, int64_t n_atoms);
+/* Average microseconds of one launch of the CG's A v (the coupling-table
+ * product of the last converged smd_induce, on its dipoles), reps launches
+ * back to back; the solution is left unchanged.  No counterpart in the
+ * reference. */
+int smd_induce_time_apply(smd_ctx* ctx, int reps, double* avg_us);
 
 /* ---- learned initial guess of the induced dipoles --------------------------- */
 /* No counterpart in the reference, whose ConjugateGradient starts from Pi = 1

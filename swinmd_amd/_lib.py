@@ -271,6 +271,7 @@ SIGNATURES = {
     "smd_set_multipoles": (C.c_int, [_P, _P, _P, _P, C.c_int64]),
     "smd_induce": (C.c_int, [_P, C.POINTER(InduceParams), C.POINTER(InduceReport)]),
     "smd_get_induction": (C.c_int, [_P, _P, _P, _P, _P, C.c_int64]),
+    "smd_induce_time_apply": (C.c_int, [_P, C.c_int, C.POINTER(C.c_double)]),
     "smd_mlp_init": (C.c_int, [_P, C.c_uint64]),
     "smd_mlp_set_params": (C.c_int, [_P, _P, C.c_int64]),
     "smd_mlp_get_params": (C.c_int, [_P, _P, C.c_int64]),

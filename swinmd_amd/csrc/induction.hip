@@ -758,6 +758,32 @@ int induce_run(Ctx* c, const smd_induce_params* p, smd_induce_report* rep) {
 
 using namespace smd;
 
+// reps launches of the CG's A v (k_apply_table on the last solve's coupling
+// table and dipoles) between two events on the context stream.
+extern "C" int smd_induce_time_apply(smd_ctx* c, int reps, double* avg_us) {
+    if (!c || reps <= 0 || !avg_us) return SMD_EINVAL;
+    if (!c->ind_last_converged) return fail(c, SMD_ESTATE, "no converged smd_induce on this state");
+    IndArgs a = ind_args(c, &c->ind_last);
+    const dim3 grid_rows((unsigned)(((int64_t)c->n * IND_LANES + 255) / 256));
+    double* z = c->d_cg + 2 * (size_t)(3 * c->n);   // CG scratch; the solution in d_mu is untouched
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    SMD_HIP(c, hipEventCreate(&e0));
+    SMD_HIP(c, hipEventCreate(&e1));
+    SMD_HIP(c, hipEventRecord(e0, c->stream));
+    for (int k = 0; k < reps; ++k)
+        hipLaunchKernelGGL((k_apply_table<IND_LANES>), grid_rows, dim3(256), 0, c->stream, a, c->d_mu, z, nullptr,
+                           nullptr, CgTop{nullptr, nullptr, 0, 0.0});
+    SMD_HIP(c, hipEventRecord(e1, c->stream));
+    SMD_HIP(c, hipEventSynchronize(e1));
+    float ms = 0.f;
+    SMD_HIP(c, hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    SMD_HIP(c, hipGetLastError());
+    *avg_us = 1e3 * (double)ms / reps;
+    return SMD_OK;
+}
+
 extern "C" int smd_set_multipoles(smd_ctx* c, const double* dip_local, const double* quad_local, const double* polar,
                                   int64_t n_atoms) {
     if (!c || !dip_local || !quad_local || !polar) return SMD_EINVAL;

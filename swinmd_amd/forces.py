@@ -561,6 +561,12 @@ class Induction:
         d["virial"] = np.array(o.virial[:])
         return d
 
+    def time_apply(self, reps=50):
+        """Average microseconds of one launch of the CG's A v (after a converged induce())."""
+        us = C.c_double()
+        self.ctx.check(self.ctx.lib.smd_induce_time_apply(self.ctx.h, int(reps), C.byref(us)))
+        return us.value
+
     def results(self):
         """Lab-frame dipoles [n,3], quadrupoles [n,6], permanent field [n,3], induced dipoles [n,3]."""
         n = self.ctx.n

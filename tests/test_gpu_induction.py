@@ -123,3 +123,17 @@ def test_induction_terms_match_reference(name):
         assert abs(out[k] - fx.meta[mk]) <= TOL * max(abs(fx.meta[mk]), scale), (k, out[k], fx.meta[mk])
     for k, mk in INDF_LONG:
         assert abs(lo[k] - fx.meta[mk]) <= TOL * abs(fx.meta[mk]), (k, lo[k], fx.meta[mk])
+
+
+def test_time_apply_leaves_the_solution_unchanged():
+    from swinmd_amd import _lib as L
+    fx = Fixture("amoeba216_induction_cg")
+    ctx, ind = _induction(fx)
+    with pytest.raises(L.SmdError) as e:
+        ind.time_apply(3)                  # no solve yet
+    assert e.value.code == L.SMD_ESTATE
+    ind.induce()
+    mu = ind.results()[3]
+    assert ind.time_apply(5) > 0.0
+    assert np.array_equal(ind.results()[3], mu)
+    ctx.close()
