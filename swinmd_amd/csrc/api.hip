@@ -72,7 +72,7 @@ static void release(Ctx* c) {
                     c->mp_ks.fpart, c->ind_ks.kvec, c->ind_ks.kmod2, c->ind_ks.expk, c->ind_ks.part,
                     c->ind_ks.kf, c->ind_ks.fpart, c->d_pack, c->d_bd_bonds, c->d_bd_angles,
                     c->d_bd_bond_off, c->d_bd_angle_off, c->d_bd_par, c->d_bd_out, c->d_stress, c->d_real,
-                    c->d_it_cnt, c->d_it_j, c->d_it_a, c->d_it_b};
+                    c->d_it_cnt, c->d_it_j, c->d_it_a};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     mlp_release(c);

@@ -60,8 +60,7 @@ struct IndArgs {
     int tstride;
     int32_t* tcnt;              // [n]
     int32_t* tj;                // [n][tstride]
-    double* ta;                 // [n][tstride]
-    double* tb;                 // [n][tstride]
+    double2* tab;               // [n][tstride] (a, b): one 16-byte load per entry
 };
 
 // ---- frames -------------------------------------------------------------------
@@ -349,8 +348,7 @@ __global__ __launch_bounds__(256) void k_ind_table(IndArgs a) {
                 const int at = count + __popcll(m & ((1ull << sub) - 1ull));
                 const int64_t e = (int64_t)i * a.tstride + at;
                 a.tj[e] = j;
-                a.ta[e] = ca;
-                a.tb[e] = cb;
+                a.tab[e] = make_double2(ca, cb);
             }
             count += __popcll(m);
         }
@@ -379,8 +377,7 @@ __global__ __launch_bounds__(256) void k_apply_table(IndArgs a, const double* __
         const double xi = a.pos[3 * i], yi = a.pos[3 * i + 1], zi = a.pos[3 * i + 2];
         pi = a.polar[i] > POLMIN ? a.polar[i] : POLMIN;
         const int32_t* tj = a.tj + (int64_t)i * a.tstride;
-        const double* ta = a.ta + (int64_t)i * a.tstride;
-        const double* tb = a.tb + (int64_t)i * a.tstride;
+        const double2* tab = a.tab + (int64_t)i * a.tstride;
         auto term = [&](int j, double ca, double cb, double px, double py, double pz, double vx, double vy,
                         double vz) {
             double dx = xi - px, dy = yi - py, dz = zi - pz;
@@ -399,8 +396,9 @@ __global__ __launch_bounds__(256) void k_apply_table(IndArgs a, const double* __
 #pragma unroll
             for (int u = 0; u < 4; ++u) {
                 jj[u] = tj[k + u * G];
-                ca[u] = ta[k + u * G];
-                cb[u] = tb[k + u * G];
+                const double2 ab = tab[k + u * G];
+                ca[u] = ab.x;
+                cb[u] = ab.y;
             }
 #pragma unroll
             for (int u = 0; u < 4; ++u) {
@@ -413,7 +411,8 @@ __global__ __launch_bounds__(256) void k_apply_table(IndArgs a, const double* __
         }
         for (; k < cnt; k += G) {
             const int j = tj[k];
-            term(j, ta[k], tb[k], a.pos[3 * j], a.pos[3 * j + 1], a.pos[3 * j + 2], v[3 * j], v[3 * j + 1],
+            const double2 ab = tab[k];
+            term(j, ab.x, ab.y, a.pos[3 * j], a.pos[3 * j + 1], a.pos[3 * j + 2], v[3 * j], v[3 * j + 1],
                  v[3 * j + 2]);
         }
     }
@@ -559,8 +558,7 @@ static IndArgs ind_args(Ctx* c, const smd_induce_params* p) {
     a.tstride = c->it_stride;
     a.tcnt = c->d_it_cnt;
     a.tj = c->d_it_j;
-    a.ta = c->d_it_a;
-    a.tb = c->d_it_b;
+    a.tab = reinterpret_cast<double2*>(c->d_it_a);
     return a;
 }
 
@@ -572,15 +570,14 @@ static int table_alloc(Ctx* c) {
     const int64_t need = (int64_t)stride * c->n;
     if (need > c->it_cap) {
         SMD_HIP(c, hipStreamSynchronize(c->stream));
-        void* ps[] = {c->d_it_cnt, c->d_it_j, c->d_it_a, c->d_it_b};
+        void* ps[] = {c->d_it_cnt, c->d_it_j, c->d_it_a};
         for (void* q : ps)
             if (q) (void)hipFree(q);
-        c->d_it_cnt = nullptr; c->d_it_j = nullptr; c->d_it_a = nullptr; c->d_it_b = nullptr;
+        c->d_it_cnt = nullptr; c->d_it_j = nullptr; c->d_it_a = nullptr;
         c->it_cap = 0;
         SMD_HIP(c, hipMalloc(&c->d_it_cnt, sizeof(int32_t) * c->n));
         SMD_HIP(c, hipMalloc(&c->d_it_j, sizeof(int32_t) * need));
-        SMD_HIP(c, hipMalloc(&c->d_it_a, sizeof(double) * need));
-        SMD_HIP(c, hipMalloc(&c->d_it_b, sizeof(double) * need));
+        SMD_HIP(c, hipMalloc(&c->d_it_a, sizeof(double2) * need));
         c->it_cap = need;
     }
     c->it_stride = stride;
@@ -698,8 +695,7 @@ int induce_run(Ctx* c, const smd_induce_params* p, smd_induce_report* rep) {
         put(&a.tstride, sizeof a.tstride);
         put(&a.tcnt, sizeof a.tcnt);
         put(&a.tj, sizeof a.tj);
-        put(&a.ta, sizeof a.ta);
-        put(&a.tb, sizeof a.tb);
+        put(&a.tab, sizeof a.tab);
         put(&g.dim, sizeof g.dim);
         put(&g.nparts_mv, sizeof g.nparts_mv);
         put(&g.nparts_vec, sizeof g.nparts_vec);

@@ -168,8 +168,7 @@ struct Ctx {
     int64_t it_cap = 0;
     int32_t* d_it_cnt = nullptr;
     int32_t* d_it_j = nullptr;
-    double* d_it_a = nullptr;
-    double* d_it_b = nullptr;
+    double* d_it_a = nullptr;     // (a, b) pairs [n][it_stride]
 
     // permanent multipoles and induced dipoles under Ewald
     struct KSet {                   // half-space k set of one Ewald class
