@@ -505,6 +505,7 @@ int smd_md_time_family(smd_ctx* ctx, const char* family, int reps, double* avg_u
  * "recip_overlap" (0/1, default 1): the MD step's k-space on a second stream.
  * "md_fused_shake" (0/1, default 1): the MD step runs SHAKE and the kinetic
  * and pressure sums as one kernel (0: two kernels).
+ * "cg_lanes" (8, 16 or 32, default 16): lanes sharing one row of the CG's A v.
  * "cg_graph" (0/1, default 1): smd_induce replays each batch of 16 CG
  * iterations from a captured hipGraph (0: direct launches, for profilers).
  * "pair_lanes": lanes sharing one atom's row

@@ -308,6 +308,10 @@ extern "C" int smd_set_tuning(smd_ctx* c, const char* name, int value) {
     } else if (std::string(name) == "md_fused_shake") {
         if (value != 0 && value != 1) return fail(c, SMD_EINVAL, "md_fused_shake must be 0 or 1 (got %d)", value);
         c->md_fused_shake = value;
+    } else if (std::string(name) == "cg_lanes") {
+        if (value != 8 && value != 16 && value != 32) return fail(c, SMD_EINVAL, "cg_lanes must be 8, 16 or 32 (got %d)", value);
+        c->cg_lanes = value;
+        if (c->cg_exec) { (void)hipGraphExecDestroy(c->cg_exec); c->cg_exec = nullptr; }
     } else if (std::string(name) == "cg_graph") {
         if (value != 0 && value != 1) return fail(c, SMD_EINVAL, "cg_graph must be 0 or 1 (got %d)", value);
         c->cg_graph = value;

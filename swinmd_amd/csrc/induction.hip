@@ -433,6 +433,15 @@ __global__ __launch_bounds__(256) void k_apply_table(IndArgs a, const double* __
     if (part) block_partial(d, part);   // v.Av of the CG step
 }
 
+// k_apply_table with `lanes` lanes per row (8, 16 or 32).
+static void launch_apply(Ctx* c, int lanes, const IndArgs& a, const double* v, double* out, const double* dotv,
+                         double* part, CgTop top) {
+    const dim3 grid((unsigned)(((int64_t)c->n * lanes + 255) / 256)), b256(256);
+    if (lanes == 8) hipLaunchKernelGGL((k_apply_table<8>), grid, b256, 0, c->stream, a, v, out, dotv, part, top);
+    else if (lanes == 32) hipLaunchKernelGGL((k_apply_table<32>), grid, b256, 0, c->stream, a, v, out, dotv, part, top);
+    else hipLaunchKernelGGL((k_apply_table<16>), grid, b256, 0, c->stream, a, v, out, dotv, part, top);
+}
+
 // ---- CG -------------------------------------------------------------------------
 
 // Buffers of block partials, each [max_parts]: VZ (v.Av), DV (v.v), and
@@ -646,7 +655,8 @@ int induce_run(Ctx* c, const smd_induce_params* p, smd_induce_report* rep) {
         g.z = c->d_cg + 2 * (size_t)dim;
         g.e0 = c->d_e0;
         g.polar = c->d_polar;
-        g.nparts_mv = (int)grid_rows.x;
+        const int lanes = c->cg_lanes;
+        g.nparts_mv = (int)(((int64_t)n * lanes + 255) / 256);   // blocks of the A v kernel
         g.nparts_vec = (int)grid_vec.x;
         g.p_vz = c->d_cg_part;
         g.p_dv = c->d_cg_part + c->cg_parts;
@@ -661,20 +671,17 @@ int induce_run(Ctx* c, const smd_induce_params* p, smd_induce_report* rep) {
         else if (p->guess == 3) {
             // learned: x = alpha E0, z = A x feed the descriptors, then x = the network's guess
             hipLaunchKernelGGL(k_direct, grid_vec, b256, 0, c->stream, dim, c->d_polar, c->d_e0, g.x);
-            hipLaunchKernelGGL((k_apply_table<IND_LANES>), grid_rows, b256, 0, c->stream, a, g.x, g.z, nullptr, nullptr,
-                               CgTop{nullptr, nullptr, 0, 0.0});
+            launch_apply(c, lanes, a, g.x, g.z, nullptr, nullptr, CgTop{nullptr, nullptr, 0, 0.0});
             const int rc = mlp_guess(c, g.x, g.z);
             if (rc) return rc;
         }
         // guess 1: keep the previous solution in d_mu
-        hipLaunchKernelGGL((k_apply_table<IND_LANES>), grid_rows, b256, 0, c->stream, a, g.x, g.z, nullptr, nullptr,
-                           CgTop{nullptr, nullptr, 0, 0.0});
+        launch_apply(c, lanes, a, g.x, g.z, nullptr, nullptr, CgTop{nullptr, nullptr, 0, 0.0});
         hipLaunchKernelGGL(k_cg_start, grid_vec, dim3(CG_THREADS), 0, c->stream, g);
         // count iterations from an even one: A v, the step, the direction
         auto iterations = [&](int count) {
             for (int k = 0; k < count; ++k) {
-                hipLaunchKernelGGL((k_apply_table<IND_LANES>), grid_rows, b256, 0, c->stream, a, g.v, g.z, g.v, g.p_vz,
-                                   CgTop{g.state, g.p_dv, g.nparts_vec, g.tol});
+                launch_apply(c, lanes, a, g.v, g.z, g.v, g.p_vz, CgTop{g.state, g.p_dv, g.nparts_vec, g.tol});
                 if (k & 1) {
                     hipLaunchKernelGGL(k_cg_step<1>, grid_vec, dim3(CG_THREADS), 0, c->stream, g);
                     hipLaunchKernelGGL(k_cg_dir<1>, grid_vec, dim3(CG_THREADS), 0, c->stream, g);
@@ -688,6 +695,7 @@ int induce_run(Ctx* c, const smd_induce_params* p, smd_induce_report* rep) {
         // when any kernel argument changes (sizes, buffers, box, parameters).
         std::string key;
         auto put = [&key](const void* p, size_t bytes) { key.append(static_cast<const char*>(p), bytes); };
+        put(&lanes, sizeof lanes);
         put(&a.n, sizeof a.n);
         put(&a.box, sizeof a.box);
         put(&a.pos, sizeof a.pos);
@@ -760,15 +768,12 @@ extern "C" int smd_induce_time_apply(smd_ctx* c, int reps, double* avg_us) {
     if (!c || reps <= 0 || !avg_us) return SMD_EINVAL;
     if (!c->ind_last_converged) return fail(c, SMD_ESTATE, "no converged smd_induce on this state");
     IndArgs a = ind_args(c, &c->ind_last);
-    const dim3 grid_rows((unsigned)(((int64_t)c->n * IND_LANES + 255) / 256));
     double* z = c->d_cg + 2 * (size_t)(3 * c->n);   // CG scratch; the solution in d_mu is untouched
     hipEvent_t e0 = nullptr, e1 = nullptr;
     SMD_HIP(c, hipEventCreate(&e0));
     SMD_HIP(c, hipEventCreate(&e1));
     SMD_HIP(c, hipEventRecord(e0, c->stream));
-    for (int k = 0; k < reps; ++k)
-        hipLaunchKernelGGL((k_apply_table<IND_LANES>), grid_rows, dim3(256), 0, c->stream, a, c->d_mu, z, nullptr,
-                           nullptr, CgTop{nullptr, nullptr, 0, 0.0});
+    for (int k = 0; k < reps; ++k) launch_apply(c, c->cg_lanes, a, c->d_mu, z, nullptr, nullptr, CgTop{nullptr, nullptr, 0, 0.0});
     SMD_HIP(c, hipEventRecord(e1, c->stream));
     SMD_HIP(c, hipEventSynchronize(e1));
     float ms = 0.f;
@@ -786,7 +791,7 @@ extern "C" int smd_set_multipoles(smd_ctx* c, const double* dip_local, const dou
     if (n_atoms != c->n) return fail(c, SMD_EINVAL, "count %lld != %d", (long long)n_atoms, c->n);
     const int n = c->n;
     if (!c->d_dipl) {
-        const int parts = std::max((int)(((int64_t)n * IND_LANES + 255) / 256), (3 * n + CG_THREADS - 1) / CG_THREADS);
+        const int parts = std::max((int)(((int64_t)n * 32 + 255) / 256), (3 * n + CG_THREADS - 1) / CG_THREADS);   // up to 32 lanes per row
         auto A = [&](auto*& p, size_t bytes) -> int {
             if (hipMalloc(&p, bytes) != hipSuccess) return fail(c, SMD_ENOMEM, "induction allocation of %zu bytes", bytes);
             return SMD_OK;

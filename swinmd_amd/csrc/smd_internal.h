@@ -162,6 +162,7 @@ struct Ctx {
     int* d_cg_state = nullptr;    // [4] done, iterations, stop reason
     double* d_cg_scal = nullptr;  // [4] last residual norm
     int cg_graph = 1;                   // replay CG batches from a captured graph
+    int cg_lanes = 16;                  // lanes per row of the CG's A v
     hipGraphExec_t cg_exec = nullptr;   // a captured batch of CG iterations
     std::string cg_key;                 // the kernel arguments it was captured with
     int it_stride = 0;            // coupling table of the solve (induction.hip)
