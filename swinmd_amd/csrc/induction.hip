@@ -630,6 +630,7 @@ int induce_run(Ctx* c, const smd_induce_params* p, smd_induce_report* rep) {
         const int rc = table_alloc(c);
         if (rc) return rc;
     }
+    c->ind_last = *p;   // the solve in progress (the learned guess reads its parameters)
     IndArgs a = ind_args(c, p);
     const int n = c->n, dim = 3 * n;
     const dim3 b256(256);
@@ -748,7 +749,6 @@ int induce_run(Ctx* c, const smd_induce_params* p, smd_induce_report* rep) {
     }
     SMD_HIP(c, hipGetLastError());
     SMD_HIP(c, hipStreamSynchronize(c->stream));
-    c->ind_last = *p;
     c->ind_last_converged = p->do_min && reason != 0;
     if (rep) {
         rep->iterations = iters;

@@ -361,9 +361,9 @@ struct DescArgs {
     const double* q;
     const double* polar;
     const double* e0;
-    const int32_t* nb;
-    const int32_t* nb_cnt;
-    const int32_t* nb_own;
+    int tstride;               // the solve's coupling table: partners inside cutoffInd in list order
+    const int32_t* tcnt;
+    const int32_t* tj;
     double* frame;             // [n][9] rows x, y, z of each site's frame
     const double* x;           // alpha E0 [3n]
     const double* z;           // A (alpha E0) [3n]
@@ -430,13 +430,14 @@ __global__ __launch_bounds__(256) void k_descriptors(DescArgs a) {
 #pragma unroll
     for (int b = 0; b < 8; ++b) acc[b][0] = acc[b][1] = acc[b][2] = 0.0;
     if (i < a.n) {
-        const int own = min(a.nb_own[i], a.stride);
-        const int cnt = min(a.nb_cnt[i], a.stride);
+        // the coupling table's row (cutoffInd >= 0.6 nm holds every partner
+        // the shells need; its excluded partners are skipped as same-molecule)
+        const int cnt = a.tcnt[i];
         const double xi = a.pos[3 * i], yi = a.pos[3 * i + 1], zi = a.pos[3 * i + 2];
-        const int32_t* row = a.nb + (int64_t)i * a.stride;
+        const int32_t* row = a.tj + (int64_t)i * a.tstride;
         const int mi = a.mol[i];
         for (int k = sub; k < cnt; k += G) {
-            const int j = k < own ? row[k] : row[a.stride - 1 - (k - own)];
+            const int j = row[k];
             if (a.mol[j] == mi) continue;
             double dx = xi - a.pos[3 * j], dy = yi - a.pos[3 * j + 1], dz = zi - a.pos[3 * j + 2];
             pbc_near(dx, dy, dz, a.box);
@@ -716,9 +717,9 @@ static DescArgs desc_args(Ctx* c) {
     d.q = c->d_q;
     d.polar = c->d_polar;
     d.e0 = c->d_e0;
-    d.nb = c->d_nb;
-    d.nb_cnt = c->d_nb_cnt;
-    d.nb_own = c->d_nb_own;
+    d.tstride = c->it_stride;
+    d.tcnt = c->d_it_cnt;
+    d.tj = c->d_it_j;
     d.frame = c->mlp->frame;
     d.norm = c->mlp->norm;
     return d;
@@ -731,6 +732,8 @@ static int check_sites(Ctx* c) {
     if (!c->list_valid) return fail(c, SMD_ESTATE, "no pair list");
     if (c->list_cutoff < 0.6)
         return fail(c, SMD_ENOTSUP, "the learned guess's descriptors need a pair-list cutoff of at least 0.6 nm");
+    if (!c->d_it_cnt || !c->ind_last.do_min || c->ind_last.cutoff_ind < 0.6)
+        return fail(c, SMD_ENOTSUP, "the learned guess needs a CG solve with cutoffInd of at least 0.6 nm");
     return SMD_OK;
 }
 
