@@ -276,7 +276,7 @@ int smd_induce_time_apply(smd_ctx* ctx, int reps, double* avg_us);
  *   descriptor  alpha E0, alpha T alpha E0, alpha sum_j q_j r/r^3 of the
  *               intermolecular partners in four shells to 0.6 nm split by the
  *               sign of q_j, alpha, positive-site flag (needs a pair list of
- *               cutoff >= 0.6 nm; 3-site waters H1, O, H2)
+ *               cutoff and a CG cutoffInd >= 0.6 nm; 3-site waters H1, O, H2)
  *   network     32 -> 128 -> 128 -> 128 -> 3, SiLU, BF16 weights and
  *               activations on the matrix cores, FP32 accumulation
  *   training    on the GPU: mean squared error of the standardised residual
