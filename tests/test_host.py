@@ -67,3 +67,19 @@ def test_fp32_erfc_kernel_constants_meet_their_stated_accuracy():
     approx = (t * P * ex).astype(np.float64)
     rel = np.abs(approx - erfc(zz)) / erfc(zz)
     assert np.all(rel < 4e-7 + 2.0 * zz * zz * 2.0 ** -24 + 2e-7)
+
+
+def test_bench_reads_the_committed_traffic_summaries():
+    """bench.py's roofline `traffic` comes from the committed rocprofv3
+    summaries; each must carry its machine-readable TRAFFIC lines."""
+    import importlib.util
+    import os
+    from tests.helpers import ROOT
+    spec = importlib.util.spec_from_file_location("bench_mod", os.path.join(ROOT, "bench.py"))
+    bench = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(bench)
+    md = bench.measured_traffic("water4096")
+    assert md.get("pairs", 0) > 20_000_000 and md.get("recip_sfac", 0) > 5_000_000
+    polar = bench.measured_traffic("polar_water8192")
+    assert polar.get("apply_table", 0) > 150_000_000
+    assert bench.measured_traffic("no_such_workload") == {}
