@@ -412,7 +412,10 @@ int smd_induction_compute(smd_ctx* ctx, int prec, double* f_accum, smd_indf_out*
 /* Induction::compute_long / reciprocal_term (:567-590, :1395-1758). */
 int smd_induction_compute_long(smd_ctx* ctx, int prec, double* f_accum, smd_mp_long_out* out);
 
-/* ---- whole step: LFIntegrator::run for rigid point-charge water --------- */
+/* ---- whole step: LFIntegrator::run ---------------------------------------
+ * Rigid point-charge water (LJ + Ewald, SHAKE) or unconstrained LJ atoms
+ * (the per-atom update with the self-consistent Nose-Hoover loop,
+ * src/integrators/LFIntegrator.cpp:587-643), NVT. */
 typedef struct {
     double cutoff, cut_buff;
     smd_ewald_params ewald;
@@ -428,6 +431,10 @@ typedef struct {
     int32_t use_graph;      /* replay each step as one captured hipGraph */
     int64_t total_steps;    /* config nSteps (Ensemble::totsteps): the stress sums
                                take the steps after the first 15% of it */
+    int32_t constraint_mode;   /* config constraintMode: 2 SHAKE of rigid 3-site
+                                  water, 0 none (atoms updated one by one) */
+    int32_t ewald_on;       /* computeEwald: point-charge Ewald terms (else LJ only) */
+    int32_t switch_on;      /* switchOn: LJ energy shifted by E(rc) (LJForce.cpp:152-168) */
 } smd_md_params;
 
 typedef struct {
@@ -441,8 +448,9 @@ typedef struct {
  * if none, evaluates forces and sets vtMinus = v - dt/2 F/m. */
 int smd_md_init(smd_ctx* ctx, const smd_md_params* p);
 /* nsteps iterations of LFIntegrator::run's body (:278-941) without output:
- * Verlet trigger and rebuild, drift removal, LJ + Ewald, SHAKE, kinetics, NH
- * friction, molecule rewrap.  Work stays on the device. */
+ * Verlet trigger and rebuild, drift removal, LJ (+ Ewald), SHAKE or the
+ * unconstrained update, kinetics, NH friction, molecule rewrap.  Work stays
+ * on the device. */
 int smd_md_run(smd_ctx* ctx, int64_t nsteps, smd_md_report* out);
 
 /* The last step's pressure tensors and the stress sums since smd_md_init:

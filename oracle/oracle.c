@@ -1175,10 +1175,14 @@ static void md_forces(orc_sys *s, const orc_md_params *p, orc_md_state *st, int 
     orc_vdw_params vp;
     memset(&vp, 0, sizeof vp);
     vp.kind = ORC_LJ; vp.n_types = p->n_types; vp.cutoff = p->cutoff;
-    vp.use_constraint = 1; vp.sigma2 = p->sigma2; vp.eps4 = p->eps4;
+    vp.use_constraint = p->constraint; vp.sigma2 = p->sigma2; vp.eps4 = p->eps4;
+    vp.switch_on = p->switch_on; vp.switch_dist = p->cutoff;
     orc_vdw_out vo;
     orc_vdw(s, st->off, st->nb, &vp, st->force, &vo);
     for (int k = 0; k < 9; ++k) vir[k] = -vo.virial[k];
+    st->e_lj = vo.lj_energy;
+    st->e_real = st->e_intra = st->e_surf = st->e_self = 0.0;
+    if (!p->ewald) return;
     orc_ewald_out eo;
     orc_ewald_real(s, st->off, st->nb, p->alpha, p->cutoff, 1, st->force, &eo);
     st->e_real = eo.energy;
@@ -1196,7 +1200,6 @@ static void md_forces(orc_sys *s, const orc_md_params *p, orc_md_state *st, int 
         st->e_long = eo.energy;
         for (int k = 0; k < 9; ++k) vir[k] += eo.virial[k];
     }
-    st->e_lj = vo.lj_energy;
     double ce;
     orc_ewald_self(s, p->alpha, volume, &st->e_self, &ce);
     st->e_self += ce;
@@ -1252,17 +1255,25 @@ int orc_md_step(const orc_sys *s_in, const orc_md_params *p, orc_md_state *st) {
         }
     double vir[9];
     md_forces(&s, p, st, (p->long_steps <= 1 || (st->step % p->long_steps) == 0), vir);
-    /* SHAKE with NH molecular friction (:561-586) */
-    double sv[9];
-    int rc = orc_shake(&s, st->pos, st->vt_minus, st->velocity, st->displacement, st->force, p->bond2,
-                       st->mol_mom, st->fnh, p->nose_hoover, p->dt, p->tol, p->max_count, sv);
-    if (rc != 0) return rc;
-    /* centres, momenta, kinetic energy (:650-686) and friction (:688-722) */
+    double sv[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    if (p->constraint) {
+        /* SHAKE with NH molecular friction (:561-586) */
+        int rc = orc_shake(&s, st->pos, st->vt_minus, st->velocity, st->displacement, st->force, p->bond2,
+                           st->mol_mom, st->fnh, p->nose_hoover, p->dt, p->tol, p->max_count, sv);
+        if (rc != 0) return rc;
+    } else {
+        /* per-atom update with the self-consistent NH loop (:587-643) */
+        int rc = orc_lf_atoms(n, st->pos, st->vt_minus, st->velocity, st->displacement, st->force, s.mass,
+                              st->fnh, p->nose_hoover, p->dt);
+        if (rc != 0) return rc;
+    }
+    /* centres, momenta, kinetic energy (:650-686) and friction (:688-722);
+     * atomNDF is 6 Nmol - 3 with constraints, else 3 N - 3 (src/Integrator.cpp:24-29) */
     orc_mol_kinetics(&s, st->pos, st->velocity, st->com, st->mol_mom, &st->atom_ke2, &st->mol_ke2);
     if (p->nose_hoover) {
         if (p->atom_thermo)
-            orc_nh_friction(st->atom_ke2, (double)(6 * s.n_mols - 3), p->temperature, p->coupl, p->dt,
-                            &st->fnh, &st->fnh_minus);
+            orc_nh_friction(st->atom_ke2, p->constraint ? (double)(6 * s.n_mols - 3) : (double)(3 * n - 3),
+                            p->temperature, p->coupl, p->dt, &st->fnh, &st->fnh_minus);
         else
             orc_nh_friction(st->mol_ke2, (double)(3 * s.n_mols - 3), p->temperature, p->coupl, p->dt,
                             &st->fnh, &st->fnh_minus);

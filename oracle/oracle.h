@@ -196,6 +196,11 @@ typedef struct {
     int32_t n_types;
     const double *sigma2, *eps4;  /* LJ table */
     const double *bond2;          /* [3*n_mols] squared constraint lengths */
+    /* 1: SHAKE of rigid 3-site water; 0: the unconstrained per-atom update
+     * with the self-consistent Nose-Hoover loop (LFIntegrator.cpp:587-643) */
+    int32_t constraint;
+    int32_t ewald;                /* point-charge Ewald terms on */
+    int32_t switch_on;            /* LJ energy shifted by E(rc) (LJForce.cpp:152-168) */
 } orc_md_params;
 
 typedef struct {
@@ -212,8 +217,8 @@ typedef struct {
     double virial[9];
 } orc_md_state;
 
-/* LFIntegrator::run body (:278-941) for LJ + point-charge Ewald + SHAKE +
- * NH(NVT), minus output and pressure averaging.  Returns 0 or a negative
+/* LFIntegrator::run body (:278-941) for LJ (+ point-charge Ewald) with SHAKE
+ * or the unconstrained update, NH(NVT), minus output and pressure averaging.  Returns 0 or a negative
  * error (pair list capacity, SHAKE). */
 int orc_md_step(const orc_sys *s_in, const orc_md_params *p, orc_md_state *st);
 

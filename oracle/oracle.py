@@ -62,7 +62,8 @@ class MdParams(C.Structure):
                 ("coupl", C.c_double), ("tol", C.c_double), ("max_count", C.c_int32),
                 ("use_cells", C.c_int32), ("long_steps", C.c_int32), ("surface", C.c_int32),
                 ("nose_hoover", C.c_int32), ("atom_thermo", C.c_int32), ("n_types", C.c_int32),
-                ("sigma2", C.c_void_p), ("eps4", C.c_void_p), ("bond2", C.c_void_p)]
+                ("sigma2", C.c_void_p), ("eps4", C.c_void_p), ("bond2", C.c_void_p),
+                ("constraint", C.c_int32), ("ewald", C.c_int32), ("switch_on", C.c_int32)]
 
 
 class MdState(C.Structure):
@@ -451,11 +452,15 @@ class Oracle:
 
 
 def md_from_cfg(o: "Oracle", cfg, nsteps, velocity=None):
-    """Runs the C oracle's LFIntegrator (start() then nsteps of run()) for rigid
-    water with the settings of a golden fixture's config; returns the state."""
+    """Runs the C oracle's LFIntegrator (start() then nsteps of run()) with the
+    settings of a golden fixture's config: rigid water with SHAKE and Ewald, or
+    (constraint and ewald off) LJ atoms with the unconstrained update; returns
+    the state."""
     s = o.s
     n, nm = s.n_atoms, s.n_mols
     atom_thermo = bool(cfg.get("atom_thermo", True))
+    constraint = bool(cfg.get("constraint", True))
+    ewald = bool(cfg.get("ewald", True))
     if velocity is None:
         if atom_thermo:
             ndf = 3 * n - len(s.bonds) - len(s.angles) - 3
@@ -463,12 +468,15 @@ def md_from_cfg(o: "Oracle", cfg, nsteps, velocity=None):
             ndf = 3 * nm - 3
         velocity = o.init_velocity(cfg["temperature"], cfg.get("seed", 1), per_atom=atom_thermo, ndf=ndf)
     p = MdParams()
-    p.cutoff, p.cut_buff, p.alpha, p.kcut = cfg["cutoff"], cfg["cut_buff"], cfg["alpha"], cfg["kcut"]
-    p.dt, p.temperature, p.coupl, p.tol = cfg["dt"], cfg["temperature"], cfg["coupl"], cfg["shake_tol"]
-    p.max_count, p.use_cells, p.long_steps, p.surface = cfg["max_count"], int(cfg.get("use_cells", True)), 1, 0
+    p.cutoff, p.cut_buff = cfg["cutoff"], cfg["cut_buff"]
+    p.alpha, p.kcut = cfg.get("alpha", 3.5), cfg.get("kcut", 10.0)
+    p.dt, p.temperature, p.coupl, p.tol = cfg["dt"], cfg["temperature"], cfg["coupl"], cfg.get("shake_tol", 1e-8)
+    p.max_count, p.use_cells, p.long_steps = cfg.get("max_count", 10000), int(cfg.get("use_cells", True)), 1
+    p.surface = 0
     p.nose_hoover, p.atom_thermo = 1, int(atom_thermo)
+    p.constraint, p.ewald, p.switch_on = int(constraint), int(ewald), int(cfg.get("switch_on", False))
     sig2, eps4 = s.lj_table()
-    bond2 = np.ascontiguousarray(s.constraint_lengths2())
+    bond2 = np.ascontiguousarray(s.constraint_lengths2() if constraint else np.zeros(3 * nm))
     p.n_types, p.sigma2, p.eps4, p.bond2 = len(s.types), _p(sig2), _p(eps4), _p(bond2)
     pos = o.pos.copy()
     vel = np.ascontiguousarray(velocity, dtype=np.float64).copy()
@@ -479,7 +487,7 @@ def md_from_cfg(o: "Oracle", cfg, nsteps, velocity=None):
     cap = max(4 * len(nb), 1024)
     nbuf = np.zeros(cap, dtype=np.int32)
     nbuf[:len(nb)] = nb
-    kv, k2, ek = o.kvectors(cfg["kcut"], cfg["alpha"])
+    kv, k2, ek = o.kvectors(p.kcut, p.alpha) if ewald else (np.zeros(3), np.zeros(1), np.zeros(1))
     kv = np.ascontiguousarray(kv)
     st = MdState()
     st.pos, st.vt_minus, st.velocity = _p(pos), _p(bufs["vtm"]), _p(vel)

@@ -70,7 +70,8 @@ class MdParams(C.Structure):
     _fields_ = [("cutoff", C.c_double), ("cut_buff", C.c_double), ("ewald", EwaldParams),
                 ("integ", IntegParams), ("use_cells", C.c_int32), ("long_steps", C.c_int32),
                 ("lustig", C.c_int32), ("prec", C.c_int32), ("sigma2", C.c_void_p), ("eps4", C.c_void_p),
-                ("use_graph", C.c_int32), ("total_steps", C.c_int64)]
+                ("use_graph", C.c_int32), ("total_steps", C.c_int64), ("constraint_mode", C.c_int32),
+                ("ewald_on", C.c_int32), ("switch_on", C.c_int32)]
 
 
 class MdPressure(C.Structure):

@@ -220,9 +220,23 @@ __global__ __launch_bounds__(256) void k_shake(IntegArgs a) {
 
 __global__ __launch_bounds__(256) void k_lf_atoms(IntegArgs a) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (a.md_step && i == 0) a.flag[0] = 0;   // every consumer of the rebuild flag has run
     if (i >= a.n) return;
     const double fnh = a.fnh_on_device ? a.scalars[0] : a.fnh_host;
     const double mr = 1.0 / a.mass[i];
+    if (a.md_step) {
+        // the drift removal of LFIntegrator::run (:354-379) on vtMinus and
+        // the velocity, from the sums k_step_head formed
+        const double mt = a.terms[T_MTOT];
+        const double vx = a.terms[T_VOM] / mt, vy = a.terms[T_VOM + 1] / mt, vz = a.terms[T_VOM + 2] / mt;
+        if (fabs(vx) > 1e-11 || fabs(vy) > 1e-11 || fabs(vz) > 1e-11) {
+            const double vc[3] = {vx, vy, vz};
+            for (int d = 0; d < 3; ++d) {
+                a.vtm[3 * i + d] -= vc[d];
+                a.vel[3 * i + d] -= vc[d];
+            }
+        }
+    }
     double vp[3];
     if (a.nose_hoover) {
         double vel[3] = {a.vel[3 * i], a.vel[3 * i + 1], a.vel[3 * i + 2]};
@@ -623,8 +637,9 @@ static IntegArgs integ_args(Ctx* c, const smd_integrator_params* p) {
 }
 
 static double md_ndf(Ctx* c) {
-    // Integrator ctor (src/Integrator.cpp:25-29)
-    if (c->md.integ.atom_thermo) return (double)(6 * c->nmol - 3);
+    // Integrator ctor (src/Integrator.cpp:25-29): atomNDF is 6 Nmol - 3 with
+    // constraints, else 3 N - 3
+    if (c->md.integ.atom_thermo) return c->md.constraint_mode ? (double)(6 * c->nmol - 3) : (double)(3 * c->n - 3);
     return (double)(3 * c->nmol - 3);
 }
 
@@ -645,8 +660,8 @@ int kinetics_run(Ctx* c, bool rewrap, const smd_integrator_params* md) {
     if (md) {
         a.md_step = 1;
         a.ndf = md_ndf(c);
-        a.surface = c->md.ewald.surface;
-        a.use_constraint = c->md.ewald.use_constraint;
+        a.surface = c->md.ewald_on ? c->md.ewald.surface : 0;
+        a.use_constraint = c->md.ewald_on ? c->md.ewald.use_constraint : (c->md.constraint_mode ? 1 : 0);
         a.lustig = c->md.lustig;
         a.stress = c->d_stress;
         a.stress_after = 0.15 * (double)c->md.total_steps;
@@ -665,8 +680,8 @@ static int shake_kinetics_run(Ctx* c, const smd_integrator_params* md) {
     a.md_step = 1;
     a.frecip = c->shake_frecip ? c->d_frecip : nullptr;
     a.ndf = md_ndf(c);
-    a.surface = c->md.ewald.surface;
-    a.use_constraint = c->md.ewald.use_constraint;
+    a.surface = c->md.ewald_on ? c->md.ewald.surface : 0;
+    a.use_constraint = c->md.ewald_on ? c->md.ewald.use_constraint : 1;
     a.lustig = c->md.lustig;
     a.stress = c->d_stress;
     a.stress_after = 0.15 * (double)c->md.total_steps;
@@ -677,9 +692,12 @@ static int shake_kinetics_run(Ctx* c, const smd_integrator_params* md) {
     return SMD_OK;
 }
 
-int lf_atoms_run(Ctx* c, const smd_integrator_params* p, double fnh) {
+int lf_atoms_run(Ctx* c, const smd_integrator_params* p, double fnh, bool md_step) {
     IntegArgs a = integ_args(c, p);
     a.fnh_host = fnh;
+    a.fnh_on_device = md_step ? 1 : 0;
+    a.md_step = md_step ? 1 : 0;
+    ProfScope ps(c, "shake");
     hipLaunchKernelGGL(k_lf_atoms, dim3((c->n + 255) / 256), dim3(256), 0, c->stream, a);
     SMD_HIP(c, hipGetLastError());
     return SMD_OK;
@@ -700,6 +718,16 @@ int md_step_enqueue(Ctx* c, bool do_long) {
     // Reciprocal space needs the positions after the rebuild's rewrap and
     // nothing of the list: it runs on the second stream beside the list
     // build and the short-range kernels.
+    if (!p.ewald_on) {
+        // LJ atoms: list, LJ, the unconstrained update (drift removal, NH
+        // velocity loop, clears the rebuild flag), then the kinetic and
+        // pressure sums, the NH update and the per-step rewrap
+        int rc = pairlist_build(c, p.cutoff, p.cut_buff, p.use_cells, true);
+        if (!rc) rc = pairs_lj(c, p.cutoff, p.switch_on, p.constraint_mode ? 1 : 0, p.lustig, p.prec, true);
+        if (!rc) rc = p.constraint_mode ? shake_run(c, &p.integ, true, 0.0, true) : lf_atoms_run(c, &p.integ, 0.0, true);
+        if (!rc) rc = kinetics_run(c, true, &p.integ);
+        return rc;
+    }
     const bool fork = do_long && c->recip_overlap;
     c->list_fork = fork ? c->ev_fork : nullptr;
     int rc = pairlist_build(c, p.cutoff, p.cut_buff, p.use_cells, true);
@@ -792,7 +820,7 @@ extern "C" int smd_shake(smd_ctx* c, const smd_integrator_params* p, double fnh,
 extern "C" int smd_lf_atoms(smd_ctx* c, const smd_integrator_params* p, double fnh) {
     if (!c || !p) return SMD_EINVAL;
     SMD_HIP(c, hipMemsetAsync(c->d_status, 0, sizeof(unsigned), c->stream));
-    int rc = lf_atoms_run(c, p, fnh);
+    int rc = lf_atoms_run(c, p, fnh, false);
     if (rc) return rc;
     SMD_HIP(c, hipStreamSynchronize(c->stream));
     return check_status(c);
@@ -846,9 +874,14 @@ extern "C" int smd_md_init(smd_ctx* c, const smd_md_params* p) {
     if (!c->box_set) return fail(c, SMD_ESTATE, "box not set");
     if (!p->sigma2 || !p->eps4) return fail(c, SMD_EINVAL, "LJ table required");
     if (p->prec != SMD_PREC_FP64 && p->prec != SMD_PREC_MIXED) return fail(c, SMD_EINVAL, "bad precision");
-    for (int m = 0; m < c->nmol; ++m)
-        if (c->h_mol_start[m + 1] - c->h_mol_start[m] != 3)
-            return fail(c, SMD_ENOTSUP, "the MD step is implemented for rigid 3-site water");
+    if (p->constraint_mode != 0 && p->constraint_mode != 2)
+        return fail(c, SMD_ENOTSUP, "constraint mode %d (0 none, 2 SHAKE)", p->constraint_mode);
+    if (p->constraint_mode == 2)
+        for (int m = 0; m < c->nmol; ++m)
+            if (c->h_mol_start[m + 1] - c->h_mol_start[m] != 3)
+                return fail(c, SMD_ENOTSUP, "SHAKE in the MD step is implemented for rigid 3-site water");
+    if (p->constraint_mode == 0 && p->ewald_on)
+        return fail(c, SMD_ENOTSUP, "the MD step with Ewald is implemented for rigid water (constraint mode 2)");
     c->md = *p;
     const int nt2 = c->ntypes * c->ntypes;
     c->md_sigma2.assign(p->sigma2, p->sigma2 + nt2);
@@ -867,7 +900,7 @@ extern "C" int smd_md_init(smd_ctx* c, const smd_md_params* p) {
         SMD_HIP(c, hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming));
         SMD_HIP(c, hipMalloc(&c->d_frecip, sizeof(double) * 3 * c->n));
     }
-    if ((rc = ewald_setup(c, &p->ewald))) return rc;
+    if (p->ewald_on && (rc = ewald_setup(c, &p->ewald))) return rc;
     if ((rc = pairlist_build(c, p->cutoff, p->cut_buff, p->use_cells, false))) return rc;
     // Give in-step rebuilds headroom: the row capacity is fixed once a step
     // is captured.
@@ -885,12 +918,16 @@ extern "C" int smd_md_init(smd_ctx* c, const smd_md_params* p) {
     // start(): forces at t, vtMinus = v - dt/2 F/m
     SMD_HIP(c, hipMemsetAsync(c->d_force, 0, sizeof(double) * 3 * c->n, c->stream));
     SMD_HIP(c, hipMemsetAsync(c->d_terms, 0, sizeof(double) * T_COUNT, c->stream));
-    if ((rc = pairs_lj_ewald(c, nullptr, nullptr, p->cutoff, p->ewald.alpha, p->ewald.real_cut,
-                             p->ewald.use_constraint, 1, p->prec)))
+    if (p->ewald_on) {
+        if ((rc = pairs_lj_ewald(c, nullptr, nullptr, p->cutoff, p->ewald.alpha, p->ewald.real_cut,
+                                 p->ewald.use_constraint, 1, p->prec)))
+            return rc;
+        if ((rc = ewald_intra(c, 1))) return rc;
+        if (p->ewald.surface && (rc = ewald_surface(c))) return rc;
+        if ((rc = ewald_recip(c, p->prec, 1))) return rc;
+    } else if ((rc = pairs_lj(c, p->cutoff, p->switch_on, p->constraint_mode ? 1 : 0, 1, p->prec, false))) {
         return rc;
-    if ((rc = ewald_intra(c, 1))) return rc;
-    if (p->ewald.surface && (rc = ewald_surface(c))) return rc;
-    if ((rc = ewald_recip(c, p->prec, 1))) return rc;
+    }
     hipLaunchKernelGGL(k_start_vtm, dim3((c->n + 255) / 256), dim3(256), 0, c->stream, c->n, c->d_vel, c->d_force,
                        c->d_mass, c->d_vtm, p->integ.dt);
     SMD_HIP(c, hipMemsetAsync(c->d_scalars, 0, sizeof(double) * 8, c->stream));

@@ -1022,6 +1022,8 @@ int pairs_lj_ewald(Ctx* c, const double* sigma2, const double* eps4, double cuto
     a.cut2 = cutoff * cutoff;
     a.rcut_2 = 1.0 / a.cut2;
     a.use_constraint = use_constraint;
+    a.switch_on = c->md.switch_on;   // the MD step's LJ energy shift
+    a.sw_dist2 = a.cut2;
     a.alpha = alpha;
     a.alpha2 = alpha * alpha;
     a.alpha2PI = 2.0 * alpha / SQRT_PI;
@@ -1030,6 +1032,24 @@ int pairs_lj_ewald(Ctx* c, const double* sigma2, const double* eps4, double cuto
         return pairs_packed(c, a, lustig);
     hipError_t e = launch_pairs<V_LJ, true>(c, a, prec, lustig);
     if (e != hipSuccess) return hip_fail(c, e, "LJ+Ewald kernel launch");
+    return SMD_OK;
+}
+
+// LJ alone over the list (no Ewald): the MD step of unconstrained LJ atoms.
+// Tables uploaded by the MD setup; the energy is shifted by E(rc) when
+// switch_on (LJForce.cpp:152-168).
+int pairs_lj(Ctx* c, double cutoff, int switch_on, int use_constraint, int lustig, int prec, bool store_forces) {
+    PairArgs a;
+    base_args(c, a);
+    a.store_forces = store_forces ? 1 : 0;
+    a.cutoff = cutoff;
+    a.cut2 = cutoff * cutoff;
+    a.rcut_2 = 1.0 / a.cut2;
+    a.switch_on = switch_on;
+    a.sw_dist2 = a.cut2;
+    a.use_constraint = use_constraint;
+    hipError_t e = launch_pairs<V_LJ, false>(c, a, prec, lustig);
+    if (e != hipSuccess) return hip_fail(c, e, "LJ kernel launch");
     return SMD_OK;
 }
 

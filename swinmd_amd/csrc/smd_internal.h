@@ -366,6 +366,7 @@ int pairlist_alloc(Ctx* c, int32_t stride);
 int vdw_run(Ctx* c, const smd_vdw_params* p, int prec, smd_vdw_out* out, bool read_back);
 int pairs_lj_ewald(Ctx* c, const double* sigma2, const double* eps4, double cutoff, double alpha,
                    double real_cut, int use_constraint, int lustig, int prec, bool store_forces = false);
+int pairs_lj(Ctx* c, double cutoff, int switch_on, int use_constraint, int lustig, int prec, bool store_forces);
 int ewald_setup(Ctx* c, const smd_ewald_params* p);
 int ewald_real(Ctx* c, int prec, int lustig);
 int ewald_intra(Ctx* c, int lustig);
@@ -375,7 +376,7 @@ enum : unsigned { RECIP_SFAC = 1u, RECIP_REDUCE = 2u, RECIP_FORCE = 4u, RECIP_AL
 int ewald_recip(Ctx* c, int prec, int lustig, unsigned passes = RECIP_ALL);
 int shake_run(Ctx* c, const smd_integrator_params* p, bool fnh_on_device, double fnh, bool md_step = false);
 int kinetics_run(Ctx* c, bool rewrap, const smd_integrator_params* md = nullptr);
-int lf_atoms_run(Ctx* c, const smd_integrator_params* p, double fnh);
+int lf_atoms_run(Ctx* c, const smd_integrator_params* p, double fnh, bool md_step);
 int md_step_enqueue(Ctx* c, bool do_long);
 int zero_terms(Ctx* c);
 int read_terms(Ctx* c, double* dst);

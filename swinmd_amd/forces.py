@@ -377,18 +377,24 @@ class LFIntegrator:
 
 
 class WaterMD:
-    """Whole LFIntegrator::run step for rigid point-charge water on the device."""
+    """Whole LFIntegrator::run step on the device: rigid point-charge water
+    (LJ + Ewald, SHAKE), or with constraint=False and ewald=False unconstrained
+    LJ atoms (the per-atom update with the self-consistent Nose-Hoover loop,
+    src/integrators/LFIntegrator.cpp:587-643)."""
 
-    def __init__(self, ctx: Context, *, cutoff, cut_buff, alpha, kcut, dt, temperature, coupl=0.1, tol=1e-8,
-                 max_count=10000, use_cells=True, long_steps=1, lustig=True, prec=L.PREC_FP64,
-                 atom_thermo=True, surface=False, use_graph=True, total_steps=0):
+    def __init__(self, ctx: Context, *, cutoff, cut_buff, dt, temperature, alpha=3.5, kcut=10.0, coupl=0.1,
+                 tol=1e-8, max_count=10000, use_cells=True, long_steps=1, lustig=True, prec=L.PREC_FP64,
+                 atom_thermo=True, surface=False, use_graph=True, total_steps=0, constraint=True, ewald=True,
+                 switch_on=False):
         self.ctx = ctx
-        b2 = _f64(ctx.system.constraint_lengths2())
-        ctx.check(ctx.lib.smd_set_constraints(ctx.h, _ptr(b2), ctx.nmol))
+        if constraint:
+            b2 = _f64(ctx.system.constraint_lengths2())
+            ctx.check(ctx.lib.smd_set_constraints(ctx.h, _ptr(b2), ctx.nmol))
         self._sig2, self._eps4 = ctx.system.lj_table()
         p = L.MdParams()
         p.cutoff, p.cut_buff = cutoff, cut_buff
-        p.ewald = L.EwaldParams(alpha, cutoff, kcut, 1, int(surface))
+        p.constraint_mode, p.ewald_on, p.switch_on = (2 if constraint else 0), int(ewald), int(switch_on)
+        p.ewald = L.EwaldParams(alpha, cutoff, kcut, int(constraint), int(surface))
         p.integ = L.IntegParams(dt, temperature, coupl, tol, max_count, 1, int(atom_thermo))
         p.use_cells, p.long_steps, p.lustig, p.prec = int(use_cells), long_steps, int(lustig), prec
         p.sigma2, p.eps4 = _ptr(self._sig2), _ptr(self._eps4)

@@ -224,6 +224,18 @@ CASES = [
      {"cutoff": 0.6, "cut_buff": 0.05, "alpha": 3.5, "kcut": S.kcut_for(27, 1.9), "dt": 0.002,
       "temperature": 280.0, "coupl": 0.05, "shake_tol": 1e-8, "max_count": 10000, "seed": 29,
       "atom_thermo": False}),
+    # 864 LJ atoms on the fcc lattice at rho* = 0.8442 (sigma = eps = m = 1),
+    # energy-shifted cutoff 2.5 sigma, unconstrained leap-frog with the
+    # self-consistent Nose-Hoover velocity loop (LFIntegrator.cpp:587-643):
+    # T* = 0.722 is 0.722 / BOLTZMAN K, Q = 10 is tau = sqrt(Q / (Ndf kB T))
+    # with Ndf = 3N - 3; dt* = 0.005.  A 0.1 sigma buffer makes the Verlet
+    # trigger fire within the run.
+    ("lj864_md20_nh",
+     {"builder": "lj_fcc", "args": {"cells": 6, "rho": 0.8442, "jitter": 0.05, "seed": 7}},
+     "md", "20",
+     {"cutoff": 2.5, "cut_buff": 0.1, "constraint": False, "ewald": False, "switch_on": True,
+      "switch_dist": 2.5, "lustig": True, "dt": 0.005, "temperature": 0.722 / 0.0083145112,
+      "coupl": (10.0 / (2589 * 0.722)) ** 0.5, "seed": 17, "atom_thermo": True}),
 ]
 
 

@@ -441,3 +441,69 @@ def test_bond_and_ub_volume_derivatives(oracle_mod):
         if name == "bond":
             d2 = (ep - 2 * r0["energy"] + em) / (e * e)
             assert d2 == pytest.approx(r0["deriv2"] + 2 * r0["deriv"], rel=1e-4)
+
+
+# ---- LJ long-range correction -----------------------------------------------------
+
+def test_lj_long_range_correction_is_the_dispersion_tail(oracle_mod):
+    """orc_lj_lrc (LJForce::long_range_correct, src/forces/LJForce.cpp:260-370)
+    against the tail integrals of the dispersion term -C6/r^6 beyond rc.  The
+    reference's own output cannot pin it (its csix starts uninitialised,
+    :268), so the integrals are evaluated here by quadrature:
+      E(V, rc)  = N^2/(2V) int_rc^inf 4 pi r^2 (-C6/r^6) dr
+      eLrc      = E(V0, rc0)
+      dUlrc     = dE/dV and d2Ulrc = d2E/dV2 at fixed scaled coordinates,
+                  i.e. with rc scaling with the box, rc = rc0 (V/V0)^(1/3)
+      vLrc      = each diagonal component of the pair virial sum
+                  sum f_i (x) (r_j - r_i) (the reference's convention,
+                  LJForce.cpp:211-220), a pair at r adding -F(r) r / 3 with
+                  F = -du/dr, over the same tail
+    with C6 the mean 4 eps sigma^6 over the non-excluded pairs, counted by
+    brute force.  Two LJ types with both epsilons nonzero make the mixed
+    pairs and the intramolecular exclusions matter."""
+    from dataclasses import replace
+
+    from scipy.integrate import quad
+
+    from swinmd_amd import systems as S
+
+    types = [replace(S.SPCE_TYPES[0], eps=0.0565, sigma=0.2655), S.SPCE_TYPES[1]]
+    s = S.water(64, density_mol=33.4, seed=3, types=types)
+    o = _oracle(s)
+    rc0 = 0.7
+    e_lrc, v_lrc, du_lrc, d2u_lrc = o.lj_lrc(rc0)
+    n = s.n_atoms
+    sig2, eps4 = s.lj_table()
+    nt = len(types)
+    excl_off, excl = s.exclusions()
+    excluded = {(i, int(j)) for i in range(n) for j in excl[excl_off[i]:excl_off[i + 1]]}
+    c6 = []
+    for i in range(n):
+        for j in range(i + 1, n):
+            if (i, j) in excluded or (j, i) in excluded:
+                continue
+            t = s.type[i] * nt + s.type[j]
+            c6.append(eps4[t] * sig2[t] ** 3)
+    c6 = float(np.mean(c6))
+    V0 = s.volume
+
+    def tail(V, integrand):
+        rc = rc0 * (V / V0) ** (1.0 / 3.0)
+        val, _ = quad(integrand, rc, np.inf, epsabs=0.0, epsrel=1e-13, limit=200)
+        return n * n / (2.0 * V) * val
+
+    def energy(V):
+        return tail(V, lambda r: 4.0 * math.pi * r * r * (-c6 / r ** 6))
+
+    assert e_lrc == pytest.approx(energy(V0), rel=1e-11)
+    h = 1e-3 * V0
+    d1 = (energy(V0 - 2 * h) - 8 * energy(V0 - h) + 8 * energy(V0 + h) - energy(V0 + 2 * h)) / (12 * h)
+    d2 = (-energy(V0 - 2 * h) + 16 * energy(V0 - h) - 30 * energy(V0) + 16 * energy(V0 + h)
+          - energy(V0 + 2 * h)) / (12 * h * h)
+    assert du_lrc == pytest.approx(d1, rel=1e-8)
+    assert d2u_lrc == pytest.approx(d2, rel=1e-6)
+    # F(r) = -du/dr = -6 C6 / r^7; a pair adds -F r / 3 to each diagonal component
+    vir = tail(V0, lambda r: 4.0 * math.pi * r * r * (6.0 * c6 / r ** 6) / 3.0)
+    assert v_lrc == pytest.approx(vir, rel=1e-11)
+    # a plausible slip: the derivative at fixed rc differs by a factor of two
+    assert abs(du_lrc - (-energy(V0) / V0)) > 0.1 * abs(du_lrc)

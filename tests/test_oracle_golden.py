@@ -102,9 +102,11 @@ def test_ewald_terms(oracle_mod, name):
     np.testing.assert_allclose(lo["virial"], fx.ref["ewald_long_virial"], rtol=TOL, atol=1e-12)
 
 
-@pytest.mark.parametrize("name", ["water512_md10", "water216_md20_moltherm"])
+@pytest.mark.parametrize("name", ["water512_md10", "water216_md20_moltherm", "lj864_md20_nh"])
 def test_md_trajectory(oracle_mod, name):
-    """start() + N steps of LFIntegrator::run with SHAKE and Nose-Hoover."""
+    """start() + N steps of LFIntegrator::run with SHAKE and Nose-Hoover, or
+    (lj864_md20_nh) the unconstrained per-atom update with the self-consistent
+    Nose-Hoover loop, orc_lf_atoms (LFIntegrator.cpp:587-643), on 864 LJ atoms."""
     fx = Fixture(name)
     o = oracle_mod.Oracle(fx.system)
     r = oracle_mod.md_from_cfg(o, fx.cfg, int(fx.extra))
