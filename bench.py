@@ -201,141 +201,207 @@ def train_learned_guess(ctx, ind, nmol, seeds, epochs=60, batch=1024):
     return net, rep
 
 
-def bench_polar(args):
-    """Polarizable water (charges, dipoles, quadrupoles, Thole-damped induced
-    dipoles, buffered 14-7), FP64: one evaluation = the reference's force
-    sequence of LFIntegrator::run for this model.  Timed twice on the same
-    frame: CG from the reference's start (Pi = 1) and CG from the learned
-    guess, both to the reference's tolerance."""
-    from swinmd_amd import _lib as L
+# The polarizable-water step (the 8,192-molecule headline workload and its
+# 1,024-molecule form): buffered 14-7 on O, multipole Ewald with torques,
+# Thole-damped induced dipoles solved to 1e-6 D RMS, SHAKE, Nose-Hoover.
+POLAR_EQUIL = 400
+# algorithmic work per unique pair inside the real-space cutoff (the
+# reference's real_term of each class; counting rule as above)
+FLOP_PER_PAIR_MP_REAL = 1050    # G_l, their gradients, B_0..B_5, force (650) and torque (400)
+FLOP_PER_PAIR_IND_REAL = 500    # both sides' G_l and gradients, Thole-damped B_l, force and torque
+
+
+def polar_cpu_baseline_md(system, nsteps=1):
+    """The reference's LFIntegrator::run on the polarizable model, compiled
+    from its sources, on one core (its multi-rank CG, Induction::ParallelCG,
+    reads an uninitialised accumulator, src/forces/Induction.cpp:2473): the
+    start() evaluation plus `nsteps` steps; returns steps/s of the run."""
+    from oracle import refinput
     from swinmd_amd import systems as S
-    from swinmd_amd.forces import BuffvdWForce, Context, Induction, Multipole
-    L.load()
+    exe = os.path.join(ROOT, "oracle", "_ref", "ref_harness")
+    if not os.path.exists(exe):
+        return None
+    with tempfile.TemporaryDirectory() as d:
+        refinput.write_inputs(d, system, cutoff=POLAR["cutoff"], cut_buff=POLAR["cut_buff"],
+                              cutoff_ew=POLAR["cutoff"], alpha=POLAR["alpha"], kcut=S.kcut_for(N2MAX, system.box[0]),
+                              ewald=False, buff=True, multipole=True, induction=True, torques=True, ind_damp=True,
+                              ind_min=True, cutoff_ind=POLAR["cutoff_ind"], a_damp=POLAR["a_damp"], constraint=True,
+                              dt=MD["dt"], nsteps=nsteps, temperature=MD["temperature"], coupl=MD["coupl"],
+                              shake_tol=MD["tol"])
+        secs = _ref_seconds(exe, d, nsteps, 1)
+    if not secs:
+        return None
+    return {"value": round(nsteps / secs, 5), "unit": "steps/s", "cores": 1, "kind": "reference",
+            "sample": f"{nsteps} LFIntegrator::run step(s) of the polarizable model at 1,024 molecules (3,072 "
+                      f"sites, the 1,024-molecule config), reference sources compiled with g++ -O2, 1 MPI rank: "
+                      f"{secs:.2f} s",
+            "headline_size": "reference method not run at 8,192 molecules: its dense A is (3N)^2 FP64 = 43.5 GB "
+                             "rebuilt every step and applied by a dense CG, minutes per step on one core, beyond "
+                             "this bench's bounded CPU sample; the 1,024-molecule point is timed instead"}
+
+
+def bench_polar_md(args):
+    """Polarizable water MD steps: the mixed-precision path with the learned
+    initial guess against the plain FP64 path (the reference's start, Pi = 1),
+    both solving to the same 1e-6 D RMS tolerance with the preconditioned CG,
+    from the same equilibrated state."""
+    from swinmd_amd import _lib as L
+    from swinmd_amd import ensemble
+    from swinmd_amd import systems as S
+    from swinmd_amd.forces import Context, Induction, LearnedGuess, PolarMD
     nmol = 1024 if args.workload == "polar1024" else 8192
     system = S.polar_water(nmol)
     n = system.n_atoms
     kcut = S.kcut_for(N2MAX, system.box[0])
-    ctx = Context(system)
-    ctx.set_pairlist(POLAR["cutoff"], POLAR["cut_buff"], True)
-    ctx.compute_mol_centers()
-    vdw = BuffvdWForce(ctx, cutoff=POLAR["cutoff"], use_constraint=True)
-    mp = Multipole(ctx, alpha=POLAR["alpha"], real_cut=POLAR["cutoff"], kcut=kcut, use_constraint=True,
-                   surface=False, torques=True)
-    ind = Induction(ctx, cutoff_ind=POLAR["cutoff_ind"], a_damp=POLAR["a_damp"], damp_tij=True, do_min=True)
-    ind.setup_terms(alpha=POLAR["alpha"], real_cut=POLAR["cutoff"], kcut=kcut, use_constraint=True, surface=False,
-                    torques=True)
-    if args.no_graph:
-        ctx.set_tuning("cg_graph", 0)
+    tol = 1e-6 * L.DEBYE_E_NM
+    use_graph = not args.no_graph
+    vel0 = ensemble.init_velocity(system, MD["temperature"], seed=1, per_atom=True)
+
+    def make(pos, vel, prec, guess, net_state=None, cg_batch=4):
+        ctx = Context(system)
+        ctx.set_positions(pos)
+        ctx.set_velocities(vel)
+        ctx.set_pairlist(POLAR["cutoff"], POLAR["cut_buff"], True)
+        ctx.set_tuning("cg_batch", cg_batch)
+        ind = Induction(ctx, cutoff_ind=POLAR["cutoff_ind"], a_damp=POLAR["a_damp"], damp_tij=True, do_min=True,
+                        tol=tol, precond=True, guess=guess, prec=prec)
+        if net_state is not None:
+            net = LearnedGuess(ctx, seed=1)
+            net.set_params(net_state[0])
+            net.set_norm(net_state[1])
+        md = PolarMD(ctx, ind, cutoff=POLAR["cutoff"], cut_buff=POLAR["cut_buff"], alpha=POLAR["alpha"], kcut=kcut,
+                     dt=MD["dt"], temperature=MD["temperature"], coupl=MD["coupl"], tol=MD["tol"], prec=prec,
+                     use_graph=use_graph)
+        md.init()
+        return ctx, ind, md
+
+    # the learned guess: trained once on the GPU on converged FP64 solves of
+    # other seeds' frames, then loaded into each context that uses it
     log("training the learned guess on other seeds' frames")
-    net, train_rep = train_learned_guess(ctx, ind, nmol, seeds=(1, 2, 3) if nmol == 8192 else (1, 2, 3, 4, 5, 6))
-    ctx.set_positions(system.pos)
-    ctx.set_pairlist(POLAR["cutoff"], POLAR["cut_buff"], True)
-    ctx.compute_mol_centers()
-    iters = []
+    tctx = Context(system)
+    tctx.set_pairlist(POLAR["cutoff"], POLAR["cut_buff"], True)
+    tind = Induction(tctx, cutoff_ind=POLAR["cutoff_ind"], a_damp=POLAR["a_damp"], damp_tij=True, do_min=True,
+                     tol=tol, precond=True, guess=2)
+    net, train_rep = train_learned_guess(tctx, tind, nmol, seeds=(1, 2, 3) if nmol == 8192 else (1, 2, 3, 4, 5, 6))
+    net_state = (net.params(), net.norm())
+    tctx.close()
 
-    def evaluate():
-        ctx.zero_forces()
-        vdw.compute()
-        mp.compute()
-        it, conv, _ = ind.induce()
-        if not conv:
-            raise RuntimeError("CG did not converge")
-        iters.append(it)
-        ind.compute()
-        mp.compute_long()
-        ind.compute_long()
+    # equilibrate from the lattice start on the mixed path, then start every
+    # path from the same coordinates and velocities
+    log(f"equilibrating {args.polar_equil} steps")
+    ctx, ind, md = make(system.pos, vel0, L.PREC_MIXED, 3, net_state)
+    md.run(args.polar_equil)
+    pos_eq, vel_eq = ctx.positions(), ctx.velocities()
+    ctx.close()
 
-    runs = {}
-    for name, guess in (("reference_start", 0), ("learned_guess", 3)):
-        ind.params.guess = guess
-        iters.clear()
-        for _ in range(max(1, args.warmup // 50)):
-            evaluate()
-        reps = max(1, args.steps // 40)
+    paths = [("mixed_learned", L.PREC_MIXED, 3), ("fp64_reference_start", L.PREC_FP64, 0),
+             ("fp64_learned", L.PREC_FP64, 3), ("mixed_reference_start", L.PREC_MIXED, 0)]
+    if args.no_fp64:
+        paths = paths[:1]
+    res = {}
+    for name, prec, guess in paths:
+        ctx, ind, md = make(pos_eq, vel_eq, prec, guess, net_state if guess == 3 else None)
+        f0 = ctx.forces()
+        e0 = md.run(0)
+        md.run(args.warmup)
+        r0 = md.run(0)
         ctx.sync()
         t0 = time.perf_counter()
-        for _ in range(reps):
-            evaluate()
+        r = md.run(args.steps)
         ctx.sync()
         secs = time.perf_counter() - t0
-        forces = ctx.forces()
-        mu = ind.results()[3]
-        ctx.profile(True)
-        for _ in range(3):
-            evaluate()
-        fam = {}
-        for f in POLAR_FAMILIES:
-            ms, cnt = ctx.profile_get(f)
-            if cnt:
-                fam[f] = round(ms * 1e3 / 3, 1)
-        ctx.profile(False)
-        runs[name] = dict(rate=reps / secs, secs=secs, reps=reps, forces=forces, mu=mu, fam=fam,
-                          iters=sum(iters[-reps - 3:-3]) / reps)
-        log(f"{name}: {reps / secs:.2f} evaluations/s, {runs[name]['iters']:.1f} CG iterations")
-    ref, lrn = runs["reference_start"], runs["learned_guess"]
-    d = lrn["forces"] - ref["forces"]
-    force_dev = float(np.sqrt(np.sum(d * d) / np.sum(ref["forces"] ** 2)))
-    dm = lrn["mu"] - ref["mu"]
-    mu_dev = float(np.sqrt(np.sum(dm * dm) / np.sum(ref["mu"] ** 2)))
-    # roofline of the dipole solve: per CG iteration the coupling table
-    # streamed once plus the CG vectors; time per iteration from the CG
-    # family's event brackets (launch gaps and host checks included)
-    ind_pairs, _ = unique_pairs_in_cutoff(ctx, POLAR["cutoff_ind"])
-    ind_pairs += 3 * nmol          # intramolecular pairs: A couples them too
-    it_avg = ref["iters"]
-    cg_us = ref["fam"].get("ind_cg", 0.0) / max(it_avg, 1)
-    # the dominant kernel, the CG's A v, timed alone with HIP events on the
-    # library's stream (the last solve's table and dipoles, back to back)
-    apply_us = ind.time_apply(100)
-    alg = BYTES_PER_DIRECTED_PAIR_TMU * 2 * ind_pairs + BYTES_PER_SITE_APPLY * n
-    ach = alg / (apply_us * 1e-6) / 1e9
-    traffic = measured_traffic(f"polar_water{nmol}").get("apply_table")
-    out_roof = {"kernel": "k_apply_table (the CG's A v)", "bound": "hbm", "achieved": round(ach, 1),
-                "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": round(ach / PEAK_HBM_GBS, 4), "traffic": traffic,
-                "traffic_unit": "bytes/launch", "algorithmic_bytes": alg, "avg_us": round(apply_us, 2),
-                "pairs": ind_pairs}
-    if cg_us > 0:
-        it_bytes = BYTES_PER_DIRECTED_PAIR_TMU * 2 * ind_pairs + BYTES_PER_SITE_CG * n
-        out_roof["cg_iteration"] = {"us": round(cg_us, 2), "algorithmic_bytes": it_bytes,
-                                    "achieved_GBs": round(it_bytes / (cg_us * 1e-6) / 1e9, 1),
-                                    "note": "whole iteration from the CG family's event brackets: launch gaps, "
-                                            "the first A x and host checks included"}
+        res[name] = dict(secs=secs, force=f0, energies=e0, iters=r["cg_iterations"] / args.steps,
+                         list_updates=r["list_updates"] - r0["list_updates"])
+        log(f"{name}: {args.steps / secs:.2f} steps/s, {res[name]['iters']:.2f} solve iterations per step")
+        if name == "mixed_learned":
+            # per-family averages (HIP events around each family's launches,
+            # direct launches), then the dominant kernels alone
+            ctx.profile(True)
+            md.run(args.polar_profile_steps)
+            fam = {}
+            for f in ("pairlist", "pairs", "mp_short", "mp_long", "ind_field", "ind_cg", "ind_short", "ind_long",
+                      "shake_kinetics"):
+                ms, cnt = ctx.profile_get(f)
+                if cnt:
+                    fam[f] = round(ms * 1e3 / args.polar_profile_steps, 2)
+            ctx.profile(False)
+            res[name]["families_us_per_step"] = fam
+            iso = {"mp_real": md.time_kernel("mp_real", 50), "ind_real": md.time_kernel("ind_real", 50),
+                   "cg_apply": ind.time_apply(100)}
+            res[name]["isolated_us"] = iso
+            res[name]["pairs_in_cut"], _ = unique_pairs_in_cutoff(ctx, POLAR["cutoff"])
+            ind_pairs, _ = unique_pairs_in_cutoff(ctx, POLAR["cutoff_ind"])
+            res[name]["ind_pairs"] = ind_pairs + 3 * nmol   # A couples the intramolecular pairs too
+        ctx.close()
+
+    head = res["mixed_learned"]
+    steps_s = args.steps / head["secs"]
     out = {
-        "metric": "polarizable force evaluations/s (FP64, reference CG tolerance)", "value": round(lrn["rate"], 3),
-        "unit": "evaluations/s", "n_gpus": 1, "steps": lrn["reps"], "warmup": max(1, args.warmup // 50),
-        "ms_per_step": round(1e3 * lrn["secs"] / lrn["reps"], 4), "higher_is_better": True, "scaling": "weak",
-        "vs_baseline": None, "dtype": "f64", "data": "synthetic: seeded lattice water with random orientations",
-        "config": {"workload": f"polarizable water force evaluation, {nmol} molecules", "molecules": nmol,
-                   "atoms": n, "box_nm": float(system.box[0]), "cutoff_nm": POLAR["cutoff"],
+        "metric": METRIC, "value": round(steps_s, 3), "unit": "steps/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(1e3 * head["secs"] / args.steps, 4), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32 pair and A v math, f64 accumulation and solve",
+        "data": f"synthetic: seeded lattice of {nmol} polarizable waters with random orientations (seed 2025), "
+                f"reference Maxwell velocities (seed 1), {args.polar_equil} untimed equilibration steps",
+        "config": {"workload": f"polarizable water MD step (buffered 14-7 + multipole Ewald + Thole induced dipoles "
+                               f"+ SHAKE + Nose-Hoover), {nmol} molecules", "molecules": nmol, "atoms": n,
+                   "box_nm": float(system.box[0]), "cutoff_nm": POLAR["cutoff"], "buffer_nm": POLAR["cut_buff"],
                    "alpha_per_nm": POLAR["alpha"], "n2max": N2MAX, "thole_a": POLAR["a_damp"],
-                   "cg_tolerance": "|r|_2 < 1e-14 (reference)", "cg_start": "learned guess",
-                   "parallelism": "single GPU"},
-        "cg_iterations_per_evaluation": round(lrn["iters"], 2),
-        "families_us_per_evaluation": lrn["fam"],
-        "reference_start": {"value": round(ref["rate"], 3), "ms_per_step": round(1e3 * ref["secs"] / ref["reps"], 4),
-                            "cg_iterations_per_evaluation": round(ref["iters"], 2),
-                            "families_us_per_evaluation": ref["fam"]},
-        "learned_vs_reference_start": round(lrn["rate"] / ref["rate"], 4),
-        "learned_force_rel_rms_vs_reference_start": force_dev,
-        "learned_dipole_rel_rms_vs_reference_start": mu_dev,
+                   "scf": "CG preconditioned by diag(alpha) to an RMS dipole residual of 1e-6 D on every path",
+                   "path": "mixed precision + learned initial guess", "dt_ps": MD["dt"],
+                   "thermostat": "Nose-Hoover 300 K", "parallelism": "single GPU",
+                   "step_launch": "hipGraph replay of the force parts, captured batches of solve iterations"
+                   if use_graph else "direct launches"},
+        "ns_per_day": round(steps_s * MD["dt"] * 86400.0 / 1000.0, 3),
+        "scf_iterations_per_step": {k: round(v["iters"], 2) for k, v in res.items()},
+        "list_updates_per_step": round(head["list_updates"] / args.steps, 4),
         "learned_guess_training": train_rep,
     }
-    out["roofline"] = out_roof
-    if not args.no_cpu_baseline and nmol == 1024:
-        log("polarizable cpu baseline (reference, 1 core)")
-        base, fref = polar_cpu_baseline(system)
-        if base:
-            out["cpu_baseline"] = base
-            d = lrn["forces"] - fref
-            out["force_rel_rms_vs_reference"] = float(np.sqrt(np.sum(d * d) / np.sum(fref * fref)))
+    if "fp64_reference_start" in res:
+        ref = res["fp64_reference_start"]
+        out["fp64_steps_per_s"] = round(args.steps / ref["secs"], 3)
+        out["speedup_vs_fp64"] = round(ref["secs"] / head["secs"], 3)
+        out["steps_per_s_by_path"] = {k: round(args.steps / v["secs"], 3) for k, v in res.items()}
+        d = head["force"] - ref["force"]
+        out["force_rel_rms_vs_fp64"] = float(np.sqrt(np.sum(d * d) / np.sum(ref["force"] ** 2)))
+        eh, er = head["energies"], ref["energies"]
+        keys = ("e_vdw", "e_mp_real", "e_mp_self", "e_mp_intra", "e_mp_long", "e_ind_real", "e_ind_self",
+                "e_ind_intra", "e_ind_long")
+        out["energy_rel_err_vs_fp64"] = {k: float(abs(eh[k] - er[k]) / max(abs(er[k]), 1e-30)) for k in keys}
+        tot_h, tot_r = sum(eh[k] for k in keys), sum(er[k] for k in keys)
+        out["potential_energy_rel_err_vs_fp64"] = float(abs(tot_h - tot_r) / abs(tot_r))
+        out["virial_rel_err_vs_fp64"] = None   # the polarizable step does not assemble the pressure tensor
+    out["families_us_per_step"] = head["families_us_per_step"]
+    iso = head["isolated_us"]
+    out["kernels_us_isolated"] = {k: round(v, 2) for k, v in iso.items()}
+    models = {"mp_real": FLOP_PER_PAIR_MP_REAL * head["pairs_in_cut"],
+              "ind_real": FLOP_PER_PAIR_IND_REAL * (head["pairs_in_cut"] + 3 * nmol)}
+    top = max(models, key=lambda k: iso[k])
+    ach = models[top] / (iso[top] * 1e-6) / 1e12
+    out["roofline"] = {"kernel": top, "bound": "valu", "achieved": round(ach, 3), "peak": PEAK_FP32_TFLOPS,
+                       "unit": "TFLOP/s", "frac": round(ach / PEAK_FP32_TFLOPS, 5),
+                       "traffic": measured_traffic(f"polar_water{nmol}").get(top), "traffic_unit": "bytes/launch",
+                       "flop_per_launch": models[top], "avg_us": round(iso[top], 2),
+                       "pairs": head["pairs_in_cut"]}
+    alg = 12 * 2 * head["ind_pairs"] + BYTES_PER_SITE_APPLY * n
+    out["cg_apply"] = {"us": round(iso["cg_apply"], 2), "algorithmic_bytes": alg,
+                       "achieved_GBs": round(alg / (iso["cg_apply"] * 1e-6) / 1e9, 1),
+                       "hbm_frac": round(alg / (iso["cg_apply"] * 1e-6) / 1e9 / PEAK_HBM_GBS, 4),
+                       "note": "the mixed solve's A v: 12 B per directed pair of the FP32 coupling table"}
+    if not args.no_cpu_baseline:
+        log("polarizable cpu baseline (reference, 1 core, 1,024 molecules)")
+        out["cpu_baseline"] = polar_cpu_baseline_md(S.polar_water(1024) if nmol != 1024 else system)
     print(json.dumps(out), flush=True)
 
 
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
-    ap.add_argument("--workload", default="water4096", choices=["water4096", "polar1024", "polar8192"],
-                    help="water4096: the rigid-water MD step (default); polar1024/polar8192: the polarizable "
-                         "water force evaluation in FP64")
+    ap.add_argument("--workload", default="polar8192", choices=["polar8192", "polar1024", "water4096"],
+                    help="polar8192: the polarizable-water MD step on 8,192 molecules, mixed precision + "
+                         "learned guess against plain FP64 (default); polar1024: the same on 1,024 molecules; "
+                         "water4096: the rigid point-charge water MD step")
+    ap.add_argument("--polar-equil", type=int, default=POLAR_EQUIL,
+                    help="untimed equilibration steps of the polarizable workloads")
+    ap.add_argument("--polar-profile-steps", type=int, default=10)
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=2000)
     ap.add_argument("--warmup", type=int, default=200)
@@ -363,7 +429,7 @@ def main():
     if args.copy_calibration:
         copy_calibration(args.copy_calibration)
     if args.workload != "water4096":
-        bench_polar(args)
+        bench_polar_md(args)
         return
 
     from swinmd_amd import ensemble, systems as S

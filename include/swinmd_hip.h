@@ -241,13 +241,23 @@ typedef struct {
     int32_t max_iter;      /* 0 -> 3n, the reference's bound */
     int32_t guess;         /* 0: Pi = 1 (the reference); 1: the previous solution; 2: alpha E0;
                               3: the learned guess (smd_mlp_*) */
-    int32_t prec;          /* smd_prec; SMD_PREC_FP64 only */
+    int32_t prec;          /* smd_prec: SMD_PREC_FP64, or SMD_PREC_MIXED (FP32 coupling table and
+                              A v, FP64 vectors, scalars and residual; preconditioned solve only) */
+    int32_t precond;       /* 0: the reference's CG; 1: CG preconditioned by M^-1 = diag(alpha)
+                              (no counterpart in the reference, whose solve is unpreconditioned) */
+    int32_t tol_rms;       /* 0: stop when |r|_2 < tol (the reference, :2431-2457); 1: stop when
+                              the RMS over polarizable sites of |alpha_i r_i| -- the dipole step
+                              of the preconditioner, in e nm -- is < tol (1e-6 D = 2.0819e-8
+                              e nm); 1 requires precond 1 and 0 requires precond 0 */
 } smd_induce_params;
+
+/* 1 Debye in e nm: the unit of the tol_rms tolerance's configs. */
+#define SMD_DEBYE_E_NM 0.020819434
 
 typedef struct {
     int32_t iterations;    /* CG updates performed */
     int32_t converged;     /* a residual test ended the loop */
-    double residual;       /* |r| after the last update */
+    double residual;       /* |r| (tol_rms 0) or the RMS dipole residual (tol_rms 1) after the last update */
 } smd_induce_report;
 
 /* Induction::compute's first steps (src/forces/Induction.cpp:503-507):
@@ -475,6 +485,58 @@ typedef struct {
 
 int smd_md_pressure(smd_ctx* ctx, smd_md_pressure_out* out);
 
+/* ---- whole step: LFIntegrator::run for rigid polarizable water ----------
+ * The force block of src/integrators/LFIntegrator.cpp:385-470 for a
+ * polarizable config -- BuffvdWForce (or LJForce) compute, Multipole compute
+ * and compute_long, Induction compute (the induced-dipole solve) and
+ * compute_long -- then SHAKE, kinetics and the Nose-Hoover update
+ * (:561-722) and the per-step molecule rewrap (:897-910), on the device.
+ * Each step is three parts: the forces up to the solve's first iteration and
+ * the tail after it are captured hipGraphs; the solve runs in captured
+ * batches of cg_batch iterations until the device's stop flag is set (the
+ * host reads it between batches).  Rigid 3-site water (SHAKE) only.  The
+ * context must hold the multipoles (smd_set_multipoles) and constraints
+ * (smd_set_constraints); the multipole and induction k sets are built here. */
+typedef struct {
+    double cutoff, cut_buff;    /* vdW cutoff; the list is built at cutoff + cut_buff */
+    smd_vdw_params vdw;         /* SMD_VDW_BUFF147 or SMD_VDW_LJ (tables as smd_vdw_compute takes) */
+    smd_mp_params mp;
+    double mp_kcut;
+    smd_indf_params indf;
+    double ind_kcut;
+    smd_induce_params induce;   /* the solve: start, preconditioner, tolerance, precision */
+    smd_integrator_params integ;
+    int32_t use_cells;
+    int32_t prec;               /* smd_prec of the vdW, multipole and induction real-space pair
+                                   math (the solve's is induce.prec) */
+    int32_t use_graph;          /* replay the two force parts as captured hipGraphs */
+    int64_t total_steps;        /* config nSteps (the stress sums' 15% rule) */
+} smd_polar_md_params;
+
+typedef struct {
+    int64_t steps, list_updates;
+    int64_t cg_iterations;      /* solve iterations summed over the steps of this call */
+    int32_t last_iterations;    /* the last step's */
+    double last_residual;       /* the last step's final residual (the solve's own measure) */
+    double e_vdw;               /* myEnergy.ljEnergy of BuffvdWForce / LJForce */
+    double e_mp_real, e_mp_self, e_mp_intra, e_mp_long;      /* Multipole's realEnergy, correctEnergy
+                                                                (self part), molCorrectEnergy, longEnergy */
+    double e_ind_real, e_ind_self, e_ind_intra, e_ind_long;  /* Induction's realPolEnergy, correctPolEnergy,
+                                                                molCorrectPolEnergy, longPolEnergy */
+    double atom_ke2, mol_ke2, fnh;
+} smd_polar_md_report;
+
+/* LFIntegrator::start for the polarizable model: list, k sets, one force
+ * evaluation (with a complete solve) and vtMinus = v - dt/2 F/m. */
+int smd_polar_md_init(smd_ctx* ctx, const smd_polar_md_params* p);
+int smd_polar_md_run(smd_ctx* ctx, int64_t nsteps, smd_polar_md_report* out);
+/* Average microseconds of one launch of the polarizable step's real-space
+ * kernel "mp_real" (Multipole::real_term) or "ind_real" (Induction::
+ * real_term), in the step's precision, reps launches back to back on the
+ * current state; the step's forces and sums are left unchanged.  No
+ * counterpart in the reference. */
+int smd_polar_time_kernel(smd_ctx* ctx, const char* name, int reps, double* avg_us);
+
 /* ---- checkpoint state ---------------------------------------------------
  * Atom::realPos, the unwrapped positions of the restart file: set with the
  * positions by smd_set_positions (as Ensemble::read_coordinate does), then
@@ -514,8 +576,10 @@ int smd_md_time_family(smd_ctx* ctx, const char* family, int reps, double* avg_u
  * "md_fused_shake" (0/1, default 1): the MD step runs SHAKE and the kinetic
  * and pressure sums as one kernel (0: two kernels).
  * "cg_lanes" (8, 16 or 32, default 16): lanes sharing one row of the CG's A v.
- * "cg_graph" (0/1, default 1): smd_induce replays each batch of 16 CG
+ * "cg_graph" (0/1, default 1): smd_induce replays each batch of CG
  * iterations from a captured hipGraph (0: direct launches, for profilers).
+ * "cg_batch" (1..64, default 16): CG iterations per batch, i.e. between the
+ * host's checks of the stop flag.
  * "pair_lanes": lanes sharing one atom's row
  * in the pair kernels, 4, 8, 16 or 32, or 0 (default: 16 for FP32 pair
  * math, 32 for FP64).  Drops a captured MD

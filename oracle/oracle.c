@@ -33,13 +33,17 @@ static inline double v3_len(v3 a) { return sqrt(a.x * a.x + a.y * a.y + a.z * a.
 static inline void acc(double *p, v3 a) { p[0] += a.x; p[1] += a.y; p[2] += a.z; }
 static inline void dec(double *p, v3 a) { p[0] -= a.x; p[1] -= a.y; p[2] -= a.z; }
 
-/* Ensemble::apply_pbc (src/Ensemble.h:319-334): half-open [-L/2, L/2). */
-static inline void apply_pbc(v3 *r, const double box[3]) {
+/* Ensemble::apply_pbc (src/Ensemble.h:319-334): half-open [-L/2, L/2).  With
+ * a Lees-Edwards tilt (the periodic image at +L_y sits at +tilt in x; the
+ * sheared replicas, which the reference does not have) the y image is taken
+ * first and moves x by the tilt; with tilt 0 every value is the reference's. */
+static inline void apply_pbc(v3 *r, const orc_sys *s) {
+    const double *box = s->box;
     const double hx = 0.5 * box[0], hy = 0.5 * box[1], hz = 0.5 * box[2];
+    while (r->y >= hy) { r->y -= box[1]; r->x -= s->tilt; }
+    while (r->y < -hy) { r->y += box[1]; r->x += s->tilt; }
     while (r->x >= hx) r->x -= box[0];
     while (r->x < -hx) r->x += box[0];
-    while (r->y >= hy) r->y -= box[1];
-    while (r->y < -hy) r->y += box[1];
     while (r->z >= hz) r->z -= box[2];
     while (r->z < -hz) r->z += box[2];
 }
@@ -179,7 +183,7 @@ int64_t orc_pairlist(const orc_sys *s, double cutoff, double cut_buff,
                         const int j = members[b];
                         if (!excluded(s, i, j)) {
                             v3 d = v3_sub(v3_ld(s->pos + 3 * j), pi);
-                            apply_pbc(&d, s->box);
+                            apply_pbc(&d, s);
                             if (v3_len2(d) < pair_cut) PUSH(i, j);
                         }
                     }
@@ -194,7 +198,7 @@ int64_t orc_pairlist(const orc_sys *s, double cutoff, double cut_buff,
             for (int j = i + 1; j < n; ++j) {
                 if (!excluded(s, i, j)) {
                     v3 d = v3_sub(v3_ld(s->pos + 3 * j), pi);
-                    apply_pbc(&d, s->box);
+                    apply_pbc(&d, s);
                     if (v3_len2(d) < pair_cut) PUSH(i, j);
                 }
             }
@@ -320,8 +324,8 @@ static int vdw_lj_mie(const orc_sys *s, const int32_t *off, const int32_t *nb,
             if (s->mol[i] == s->mol[j]) continue;
             v3 rij = v3_sub(v3_ld(s->pos + 3 * j), pi);
             v3 rijm = v3_sub(v3_ld(s->com + 3 * s->mol[j]), rim);
-            apply_pbc(&rij, s->box);
-            apply_pbc(&rijm, s->box);
+            apply_pbc(&rij, s);
+            apply_pbc(&rijm, s);
             const double r = v3_len(rij);
             const double rij2 = r * r;
             if (!(rij2 <= cutOff2)) continue;
@@ -448,7 +452,7 @@ static int vdw_buck(const orc_sys *s, const int32_t *off, const int32_t *nb,
             v3 rij = v3_sub(ri, rj);
             v3 rijm = v3_sub(v3_ld(s->com + 3 * mj), rim);
             rijm = v3_sub(rijm, rij);
-            apply_pbc(&rij, s->box);
+            apply_pbc(&rij, s);
             rijm = v3_add(rijm, rij);
             (void)rijm;
             const double r = v3_len(rij);
@@ -564,8 +568,8 @@ static int vdw_buff(const orc_sys *s, const int32_t *off, const int32_t *nb,
             v3 rj = (s->type[j] == 0) ? v3_ld(virt + 3 * j) : v3_ld(s->pos + 3 * j);
             v3 rij = v3_sub(ri, rj);
             v3 rijm = v3_sub(rim, v3_ld(s->com + 3 * mj));
-            apply_pbc(&rij, s->box);
-            apply_pbc(&rijm, s->box);
+            apply_pbc(&rij, s);
+            apply_pbc(&rijm, s);
             const double r = v3_len(rij);
             const double rij2 = r * r;
             if (!(rij2 <= cutOff2)) continue;
@@ -678,8 +682,8 @@ void orc_ewald_real(const orc_sys *s, const int32_t *off, const int32_t *nb,
             v3 rijm = v3_sub(v3_ld(s->com + 3 * s->mol[j]), rim);
             const double qij = qi * s->q[j];
             v3 rij = v3_sub(v3_ld(s->pos + 3 * j), ri);
-            apply_pbc(&rij, s->box);
-            apply_pbc(&rijm, s->box);
+            apply_pbc(&rij, s);
+            apply_pbc(&rijm, s);
             const double r2 = v3_len2(rij);
             if (r2 < realCut2) {
                 const double rijRij = use_constraint ? v3_dot(rij, rijm) : r2;
@@ -715,7 +719,7 @@ void orc_ewald_intra(const orc_sys *s, double alpha, double volume,
             for (int b = a + 1; b < s->mol_start[m + 1]; ++b) {
                 if (!excluded(s, a, b)) continue;
                 v3 rij = v3_sub(v3_ld(s->pos + 3 * b), ri);
-                apply_pbc(&rij, s->box);
+                apply_pbc(&rij, s);
                 const double r2 = v3_len2(rij);
                 const double r = sqrt(r2);
                 const double qij = qi * s->q[b];
@@ -919,12 +923,12 @@ int orc_shake(const orc_sys *s, double *pos, double *vt_minus,
                 int next = i + 1;
                 if (next == ns) next = 0;
                 v3 rijNew = v3_sub(rNew[i], rNew[next]);
-                apply_pbc(&rijNew, s->box);
+                apply_pbc(&rijNew, s);
                 const double rijNew2 = v3_len2(rijNew);
                 const double diff = bond2[3 * m + i] - rijNew2;
                 if (fabs(diff) > (bond2[3 * m + i] * tol_2)) {
                     v3 rijOld = v3_sub(rOld[i], rOld[next]);
-                    apply_pbc(&rijOld, s->box);
+                    apply_pbc(&rijOld, s);
                     const double rijNewOld = v3_dot(rijOld, rijNew);
                     if (rijNewOld < bond2[3 * m + i] * tol) return -1;
                     const double mr1 = 1.0 / s->mass[a0 + i], mr2 = 1.0 / s->mass[a0 + next];
@@ -1157,7 +1161,7 @@ static void rewrap_molecules(orc_sys *s, double *pos, double *com) {
     for (int m = 0; m < s->n_mols; ++m) {
         const v3 c0 = v3_ld(com + 3 * m);
         v3 c = c0;
-        apply_pbc(&c, s->box);
+        apply_pbc(&c, s);
         v3_st(com + 3 * m, c);
         const v3 diff = v3_sub(c, c0);
         for (int a = s->mol_start[m]; a < s->mol_start[m + 1]; ++a)
@@ -1383,7 +1387,7 @@ int orc_induced_dipoles(const orc_sys *s, const double *dip, const double *quad,
             const double Qyyj = quad[6 * j + 3], Qyzj = quad[6 * j + 4], Qzzj = quad[6 * j + 5];
             const double Qyxj = Qxyj, Qzxj = Qxzj, Qzyj = Qyzj;
             v3 rij = v3_sub(ri, rj);
-            apply_pbc(&rij, s->box);
+            apply_pbc(&rij, s);
             const double r = v3_len(rij);
             const double rij2 = r * r;
             if (!(rij2 <= cutoffInd2)) continue;
@@ -1819,7 +1823,7 @@ int orc_multipole_short(const orc_sys *s, const int32_t *off, const int32_t *nb,
             const v3 rj = v3_ld(s->pos + 3 * j);
             const mp_site J = mp_load(s, dip, quad, j);
             v3 rij = v3_sub(ri, rj);
-            apply_pbc(&rij, s->box);
+            apply_pbc(&rij, s);
             const double r2 = v3_len2(rij);
             if (!(r2 < realCut2)) continue;
             mp_terms t;
@@ -1877,7 +1881,7 @@ int orc_multipole_short(const orc_sys *s, const int32_t *off, const int32_t *nb,
                     if (!excluded(s, idi, idj)) continue;
                     const v3 rj = v3_ld(s->pos + 3 * idj);
                     v3 rij = v3_sub(ri, rj);
-                    apply_pbc(&rij, s->box);
+                    apply_pbc(&rij, s);
                     const double r2 = v3_len2(rij);
                     const double r = sqrt(r2);
                     const mp_site J = mp_load(s, dip, quad, idj);
@@ -2208,7 +2212,7 @@ int orc_induction_short(const orc_sys *s, const double *dip, const double *quad,
         for (int j = i + 1; j < n; ++j) {
             const v3 rj = v3_ld(s->pos + 3 * j);
             v3 rij = v3_sub(ri, rj);
-            apply_pbc(&rij, s->box);
+            apply_pbc(&rij, s);
             const double r2 = v3_len2(rij);
             if (!(r2 < realCut2)) continue;
             const mp_site J = mp_load(s, dip, quad, j);
@@ -2332,7 +2336,7 @@ int orc_induction_short(const orc_sys *s, const double *dip, const double *quad,
                     if (!excluded(s, idi, idj)) continue;
                     const v3 rj = v3_ld(s->pos + 3 * idj);
                     v3 rij = v3_sub(ri, rj);
-                    apply_pbc(&rij, s->box);
+                    apply_pbc(&rij, s);
                     const double r2 = v3_len2(rij), r = sqrt(r2);
                     const mp_site J = mp_load(s, dip, quad, idj);
                     const v3 indPj = v3_ld(pind + 3 * idj);
@@ -2567,7 +2571,7 @@ int orc_bond(const orc_sys *s, int64_t nb, const int32_t *bonds, const double *r
         const int a1 = bonds[3 * i], a2 = bonds[3 * i + 1], t = bonds[3 * i + 2];
         const double k = kv[t], r0 = r0v[t];
         v3 r12 = v3_sub(v3_ld(s->pos + 3 * a1), v3_ld(s->pos + 3 * a2));
-        apply_pbc(&r12, s->box);
+        apply_pbc(&r12, s);
         const double r = v3_len(r12);
         const double diff = r - r0;
         double U, val;
@@ -2602,10 +2606,10 @@ int orc_angle(const orc_sys *s, int64_t na, const int32_t *angles, const double 
         const int a1 = angles[4 * i], a2 = angles[4 * i + 1], a3 = angles[4 * i + 2], t = angles[4 * i + 3];
         const double theta0 = theta0v[t], k = kv[t];
         v3 r12 = v3_sub(v3_ld(s->pos + 3 * a2), v3_ld(s->pos + 3 * a1));
-        apply_pbc(&r12, s->box);
+        apply_pbc(&r12, s);
         const double d12r = 1.0 / v3_len(r12);
         v3 r32 = v3_sub(v3_ld(s->pos + 3 * a2), v3_ld(s->pos + 3 * a3));
-        apply_pbc(&r32, s->box);
+        apply_pbc(&r32, s);
         const double d32r = 1.0 / v3_len(r32);
         double cosT = v3_dot(r12, r32) * d12r * d32r;
         cosT = cosT > -0.9999999 ? cosT : -0.9999999;   /* MAX(cosTheta, -0.9999999) */
@@ -2651,7 +2655,7 @@ int orc_ub(const orc_sys *s, double kl, double l0, double *force, double out[12]
         if (s->mol_start[m + 1] - s->mol_start[m] < 3) return -1;
         const int id1 = s->mol_start[m], id2 = s->mol_start[m] + 2;
         v3 r12 = v3_sub(v3_ld(s->pos + 3 * id1), v3_ld(s->pos + 3 * id2));
-        apply_pbc(&r12, s->box);
+        apply_pbc(&r12, s);
         const double r = v3_len(r12);
         const double diff = r - l0;
         const double U = kl * diff * diff;

@@ -40,6 +40,7 @@ typedef struct {
     const int32_t *excl_off;  /* [n_atoms+1] exclusion table (CSR) */
     const int32_t *excl;      /* ascending per row, as std::set orders it */
     double box[3];
+    double tilt;              /* Lees-Edwards xy tilt of the sheared replicas (0: the reference's box) */
 } orc_sys;
 
 /* Cell grid of CellManager::set_cellManager (src/utils/CellManager.cpp:43-84).

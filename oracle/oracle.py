@@ -32,7 +32,7 @@ class OrcSys(C.Structure):
                 ("pos", C.c_void_p), ("type", C.c_void_p), ("mol", C.c_void_p),
                 ("q", C.c_void_p), ("mass", C.c_void_p), ("mol_start", C.c_void_p),
                 ("com", C.c_void_p), ("excl_off", C.c_void_p), ("excl", C.c_void_p),
-                ("box", C.c_double * 3)]
+                ("box", C.c_double * 3), ("tilt", C.c_double)]
 
 
 class VdwParams(C.Structure):

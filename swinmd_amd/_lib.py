@@ -92,7 +92,9 @@ class MdReport(C.Structure):
 class InduceParams(C.Structure):
     _fields_ = [("cutoff_ind", C.c_double), ("a_damp", C.c_double), ("damp_tij", C.c_int32),
                 ("do_min", C.c_int32), ("tol", C.c_double), ("max_iter", C.c_int32), ("guess", C.c_int32),
-                ("prec", C.c_int32)]
+                ("prec", C.c_int32), ("precond", C.c_int32), ("tol_rms", C.c_int32)]
+
+DEBYE_E_NM = 0.020819434   # 1 Debye in e nm
 
 
 class InduceReport(C.Structure):
@@ -138,6 +140,21 @@ class IndfOut(C.Structure):
     _fields_ = [(k, C.c_double) for k in ("self_energy", "real_energy", "intra_energy", "surf_energy",
                                            "self_deriv", "real_deriv", "mol_deriv", "surf_deriv", "surf_deriv2",
                                            "tot_torque")] + [("virial", C.c_double * 9)]
+
+
+class PolarMdParams(C.Structure):
+    _fields_ = [("cutoff", C.c_double), ("cut_buff", C.c_double), ("vdw", VdwParams), ("mp", MpParams),
+                ("mp_kcut", C.c_double), ("indf", IndfParams), ("ind_kcut", C.c_double),
+                ("induce", InduceParams), ("integ", IntegParams), ("use_cells", C.c_int32), ("prec", C.c_int32),
+                ("use_graph", C.c_int32), ("total_steps", C.c_int64)]
+
+
+class PolarMdReport(C.Structure):
+    _fields_ = [("steps", C.c_int64), ("list_updates", C.c_int64), ("cg_iterations", C.c_int64),
+                ("last_iterations", C.c_int32), ("last_residual", C.c_double)] + \
+               [(k, C.c_double) for k in ("e_vdw", "e_mp_real", "e_mp_self", "e_mp_intra", "e_mp_long",
+                                           "e_ind_real", "e_ind_self", "e_ind_intra", "e_ind_long", "atom_ke2",
+                                           "mol_ke2", "fnh")]
 
 
 _NAME80 = C.c_char * 80
@@ -253,6 +270,9 @@ SIGNATURES = {
     "smd_nh_chain": (C.c_int, [_P, C.c_double, C.c_double, C.c_double, C.c_double, C.c_double, C.c_double,
                                _P, C.POINTER(C.c_double)]),
     "smd_md_init": (C.c_int, [_P, C.POINTER(MdParams)]),
+    "smd_polar_md_init": (C.c_int, [_P, C.POINTER(PolarMdParams)]),
+    "smd_polar_md_run": (C.c_int, [_P, C.c_int64, C.POINTER(PolarMdReport)]),
+    "smd_polar_time_kernel": (C.c_int, [_P, C.c_char_p, C.c_int, C.POINTER(C.c_double)]),
     "smd_md_run": (C.c_int, [_P, C.c_int64, C.POINTER(MdReport)]),
     "smd_md_pressure": (C.c_int, [_P, C.POINTER(MdPressure)]),
     "smd_set_real_positions": (C.c_int, [_P, _P, C.c_int64]),

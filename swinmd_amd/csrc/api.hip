@@ -72,13 +72,15 @@ static void release(Ctx* c) {
                     c->mp_ks.fpart, c->ind_ks.kvec, c->ind_ks.kmod2, c->ind_ks.expk, c->ind_ks.part,
                     c->ind_ks.kf, c->ind_ks.fpart, c->d_pack, c->d_bd_bonds, c->d_bd_angles,
                     c->d_bd_bond_off, c->d_bd_angle_off, c->d_bd_par, c->d_bd_out, c->d_stress, c->d_real,
-                    c->d_it_cnt, c->d_it_j, c->d_it_a};
+                    c->d_it_cnt, c->d_it_j, c->d_it_a, c->d_part, c->d_cg32, c->d_polar_terms, c->d_site32, c->d_pind32};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     mlp_release(c);
     if (c->h_pinned) (void)hipHostFree(c->h_pinned);
     if (c->graph_exec) (void)hipGraphExecDestroy(c->graph_exec);
     if (c->cg_exec) (void)hipGraphExecDestroy(c->cg_exec);
+    if (c->pmd_a) (void)hipGraphExecDestroy(c->pmd_a);
+    if (c->pmd_c) (void)hipGraphExecDestroy(c->pmd_c);
     if (c->graph) (void)hipGraphDestroy(c->graph);
     for (auto& r : c->prof.pending) { (void)hipEventDestroy(r.a); (void)hipEventDestroy(r.b); }
     if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);
@@ -183,7 +185,18 @@ extern "C" int smd_ctx_create(smd_ctx** out, const smd_topology* t, int device) 
     A(c->d_bond2, sizeof(double) * 3 * nm);
     A(c->d_terms, sizeof(double) * T_COUNT);
     A(c->d_status, sizeof(unsigned));
-    A(c->d_flag, sizeof(int) * 4);
+    A(c->d_flag, sizeof(int) * 8);
+    c->nb_atoms = (n + 255) / 256;
+    c->nb_mols = (nm + 255) / 256;
+    c->nb_mols64 = (nm + 63) / 64;
+    A(c->d_part, sizeof(double) * ((size_t)(4 + 16) * c->nb_atoms + (size_t)KIN_PARTS * c->nb_mols64 +
+                                   (size_t)9 * c->nb_mols));
+    if (ok) {
+        c->part_vom = c->d_part;
+        c->part_su = c->part_vom + (size_t)4 * c->nb_atoms;
+        c->part_kin = c->part_su + (size_t)16 * c->nb_atoms;
+        c->part_sh = c->part_kin + (size_t)KIN_PARTS * c->nb_mols64;
+    }
     A(c->d_scalars, sizeof(double) * 8);
     A(c->d_cell_of, sizeof(int32_t) * n);
     A(c->d_cell_atoms, sizeof(int32_t) * n);
@@ -311,6 +324,10 @@ extern "C" int smd_set_tuning(smd_ctx* c, const char* name, int value) {
     } else if (std::string(name) == "cg_lanes") {
         if (value != 8 && value != 16 && value != 32) return fail(c, SMD_EINVAL, "cg_lanes must be 8, 16 or 32 (got %d)", value);
         c->cg_lanes = value;
+        if (c->cg_exec) { (void)hipGraphExecDestroy(c->cg_exec); c->cg_exec = nullptr; }
+    } else if (std::string(name) == "cg_batch") {
+        if (value < 1 || value > 64) return fail(c, SMD_EINVAL, "cg_batch must be 1..64 (got %d)", value);
+        c->cg_batch = value;
         if (c->cg_exec) { (void)hipGraphExecDestroy(c->cg_exec); c->cg_exec = nullptr; }
     } else if (std::string(name) == "cg_graph") {
         if (value != 0 && value != 1) return fail(c, SMD_EINVAL, "cg_graph must be 0 or 1 (got %d)", value);

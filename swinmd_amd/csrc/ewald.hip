@@ -238,9 +238,11 @@ int ewald_intra(Ctx* c, int lustig) {
 
 // ---- surface dipole ---------------------------------------------------------
 
+// The dipole sums feed every atom's force: block partials summed by the
+// last block in a fixed order, so the forces do not depend on scheduling.
 __global__ __launch_bounds__(256) void k_surf_sums(int n, const double* __restrict__ pos, const double* __restrict__ q,
                                                    const int32_t* __restrict__ mol, const double* __restrict__ com,
-                                                   double* terms) {
+                                                   double* terms, double* part, int* ticket) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     double v[6] = {0, 0, 0, 0, 0, 0};
     if (i < n) {
@@ -248,7 +250,8 @@ __global__ __launch_bounds__(256) void k_surf_sums(int n, const double* __restri
         const int m = mol[i];
         for (int d = 0; d < 3; ++d) { v[d] = qi * pos[3 * i + d]; v[3 + d] = qi * com[3 * m + d]; }
     }
-    block_add<6>(v, terms + T_SU_QR);
+    block_store<6>(v, part, gridDim.x);
+    if (last_block(ticket)) fixed_totals(part, gridDim.x, gridDim.x, 6, terms + T_SU_QR);
 }
 
 __global__ __launch_bounds__(256) void k_surf_force(int n, const double* __restrict__ pos, const double* __restrict__ q,
@@ -273,7 +276,8 @@ int ewald_surface(Ctx* c) {
     const double volume = c->box.L[0] * c->box.L[1] * c->box.L[2];
     const double coeff = 2.0 * PI * (1.0 / volume) / 3.0;
     const dim3 g((c->n + 255) / 256);
-    hipLaunchKernelGGL(k_surf_sums, g, dim3(256), 0, c->stream, c->n, c->d_pos, c->d_q, c->d_mol, c->d_com, c->d_terms);
+    hipLaunchKernelGGL(k_surf_sums, g, dim3(256), 0, c->stream, c->n, c->d_pos, c->d_q, c->d_mol, c->d_com, c->d_terms,
+                       c->part_su, c->d_flag + 4);
     hipLaunchKernelGGL(k_surf_force, g, dim3(256), 0, c->stream, c->n, c->d_pos, c->d_q, c->d_force, c->d_terms, coeff);
     SMD_HIP(c, hipGetLastError());
     return SMD_OK;
@@ -637,7 +641,17 @@ __global__ __launch_bounds__(RF_WAVES * WAVE) void k_recip_force(RecipArgs a) {
     }
 }
 
+// Scratch of the reciprocal-space passes: per-k coefficients, (chunk, k)
+// partials, per-atom phases and moments.
+struct RecipScratch { double* kf; double* part; double* e1; };
+
+static int ewald_recip_with(Ctx* c, int prec, int lustig, unsigned passes, const RecipScratch& sc);
+
 int ewald_recip(Ctx* c, int prec, int lustig, unsigned passes) {
+    return ewald_recip_with(c, prec, lustig, passes, RecipScratch{c->d_kf, c->d_sfac_part, c->d_e1});
+}
+
+static int ewald_recip_with(Ctx* c, int prec, int lustig, unsigned passes, const RecipScratch& sc) {
     if (!c->ewald_ready) return fail(c, SMD_ESTATE, "Ewald not set up");
     RecipArgs a;
     a.n = c->n;
@@ -658,10 +672,10 @@ int ewald_recip(Ctx* c, int prec, int lustig, unsigned passes) {
     a.expk = c->d_expk;
     a.grp = c->d_kgrp;
     a.split = c->d_ksplit;
-    a.part = c->d_sfac_part;
-    a.kf = c->d_kf;
-    a.e1 = reinterpret_cast<double2*>(c->d_e1);
-    a.am = reinterpret_cast<double4*>(c->d_e1 + e1_am_offset(c->n));
+    a.part = sc.part;
+    a.kf = sc.kf;
+    a.e1 = reinterpret_cast<double2*>(sc.e1);
+    a.am = reinterpret_cast<double4*>(sc.e1 + e1_am_offset(c->n));
     a.store = c->recip_store ? 1 : 0;
     a.force = c->recip_store ? c->d_frecip : c->d_force;
     a.terms = c->d_terms;
@@ -778,6 +792,8 @@ extern "C" int smd_ewald_compute(smd_ctx* c, int prec, double* f_accum, smd_ewal
     return SMD_OK;
 }
 
+static int recip_long_out(smd_ctx* c, smd_ewald_long_out* out);
+
 extern "C" int smd_ewald_compute_long(smd_ctx* c, int prec, double* f_accum, smd_ewald_long_out* out) {
     // EwaldForce::compute_long (src/forces/EwaldForce.cpp:178-196)
     if (!c || !out) return SMD_EINVAL;
@@ -786,6 +802,14 @@ extern "C" int smd_ewald_compute_long(smd_ctx* c, int prec, double* f_accum, smd
     SMD_HIP(c, hipMemsetAsync(c->d_terms, 0, sizeof(double) * T_COUNT, c->stream));
     int rc = ewald_recip(c, prec, 1);
     if (rc) return rc;
+    if ((rc = recip_long_out(c, out))) return rc;
+    if (f_accum) return fetch_forces_into(c, f_accum);
+    return SMD_OK;
+}
+
+// compute_long's outputs from the term accumulators (reads them back).
+static int recip_long_out(smd_ctx* c, smd_ewald_long_out* out) {
+    int rc;
     double t[T_COUNT];
     if ((rc = read_terms(c, t))) return rc;
     out->energy = t[T_RC_E];
@@ -796,6 +820,5 @@ extern "C" int smd_ewald_compute_long(smd_ctx* c, int prec, double* f_accum, smd
         out->deriv2 += -6.0 * t[T_RC_FF] + t[T_RC_FFK2] - t[T_RC_D2UV1] + t[T_RC_D2URR];
     }
     for (int k = 0; k < 9; ++k) out->virial[k] = t[T_RC_VIR + k];
-    if (f_accum) return fetch_forces_into(c, f_accum);
     return SMD_OK;
 }

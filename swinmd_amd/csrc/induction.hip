@@ -32,7 +32,6 @@ namespace smd {
 
 constexpr int IND_LANES = 16;      // lanes per atom row in the field and T.mu kernels
 constexpr int CG_THREADS = 256;    // threads per block of the vector kernels
-constexpr int CG_BATCH = 16;       // CG iterations launched between host checks
 constexpr double SQRT_COULOMB = 11.78708976;   // SQRTCOULOMBCONSTANT, src/NEMD_defs.h:103
 constexpr double POLMIN = 0.00000001;          // Induction.cpp:195
 
@@ -61,6 +60,7 @@ struct IndArgs {
     int32_t* tcnt;              // [n]
     int32_t* tj;                // [n][tstride]
     double2* tab;               // [n][tstride] (a, b): one 16-byte load per entry
+    float2* tab32;              // mixed solve: the same factors in FP32, [n][tstride] (else null)
 };
 
 // ---- frames -------------------------------------------------------------------
@@ -348,7 +348,8 @@ __global__ __launch_bounds__(256) void k_ind_table(IndArgs a) {
                 const int at = count + __popcll(m & ((1ull << sub) - 1ull));
                 const int64_t e = (int64_t)i * a.tstride + at;
                 a.tj[e] = j;
-                a.tab[e] = make_double2(ca, cb);
+                if (a.tab32) a.tab32[e] = make_float2((float)ca, (float)cb);
+                else a.tab[e] = make_double2(ca, cb);
             }
             count += __popcll(m);
         }
@@ -363,7 +364,7 @@ __global__ __launch_bounds__(256) void k_apply_table(IndArgs a, const double* __
                                                      double* part, CgTop top) {
     if (top.state) {
         if (top.state[0]) return;
-        if (sqrt(block_total(top.p_dv, top.nparts)) < top.tol) {
+        if (top.p_dv && sqrt(block_total(top.p_dv, top.nparts)) < top.tol) {
             if (blockIdx.x == 0 && threadIdx.x == 0) { top.state[0] = 1; top.state[2] = 1; }
             return;
         }
@@ -433,6 +434,77 @@ __global__ __launch_bounds__(256) void k_apply_table(IndArgs a, const double* __
     if (part) block_partial(d, part);   // v.Av of the CG step
 }
 
+// A v of the mixed solve: the FP32 coupling table and an FP32 copy of v
+// (float4 per site), the displacement from the FP64 positions rounded to
+// FP32, FP32 products and FP64 accumulation; v_i / alpha_i and the output in
+// FP64.
+template <int G>
+__global__ __launch_bounds__(256) void k_apply_table32(IndArgs a, const float4* __restrict__ v32,
+                                                       const double* __restrict__ v, double* __restrict__ out,
+                                                       double* part, CgTop top) {
+    if (top.state && top.state[0]) return;
+    const int tid = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = tid / G, sub = tid % G;
+    double ax = 0.0, ay = 0.0, az = 0.0;
+    double pi = 1.0;
+    if (i < a.n) {
+        const int cnt = a.tcnt[i];
+        const double xi = a.pos[3 * i], yi = a.pos[3 * i + 1], zi = a.pos[3 * i + 2];
+        pi = a.polar[i] > POLMIN ? a.polar[i] : POLMIN;
+        const int32_t* tj = a.tj + (int64_t)i * a.tstride;
+        const float2* tab = a.tab32 + (int64_t)i * a.tstride;
+        auto term = [&](float ca, float cb, double px, double py, double pz, float4 vj) {
+            double dx = xi - px, dy = yi - py, dz = zi - pz;
+            pbc_near(dx, dy, dz, a.box);
+            const float fx = (float)dx, fy = (float)dy, fz = (float)dz;
+            const float rv = ca * (fx * vj.x + fy * vj.y + fz * vj.z);
+            ax += (double)(rv * fx - cb * vj.x);
+            ay += (double)(rv * fy - cb * vj.y);
+            az += (double)(rv * fz - cb * vj.z);
+        };
+        int k = sub;
+        for (; k + 3 * G < cnt; k += 4 * G) {
+            int jj[4];
+            float2 ab[4];
+            double px[4], py[4], pz[4];
+            float4 vj[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                jj[u] = tj[k + u * G];
+                ab[u] = tab[k + u * G];
+            }
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const int j = jj[u];
+                px[u] = a.pos[3 * j]; py[u] = a.pos[3 * j + 1]; pz[u] = a.pos[3 * j + 2];
+                vj[u] = v32[j];
+            }
+#pragma unroll
+            for (int u = 0; u < 4; ++u) term(ab[u].x, ab[u].y, px[u], py[u], pz[u], vj[u]);
+        }
+        for (; k < cnt; k += G) {
+            const int j = tj[k];
+            const float2 ab = tab[k];
+            term(ab.x, ab.y, a.pos[3 * j], a.pos[3 * j + 1], a.pos[3 * j + 2], v32[j]);
+        }
+    }
+    double acc[3] = {ax, ay, az};
+#pragma unroll
+    for (int o = G / 2; o > 0; o >>= 1)
+#pragma unroll
+        for (int x = 0; x < 3; ++x) acc[x] += __shfl_xor(acc[x], o, G);
+    double d = 0.0;
+    if (i < a.n && sub == 0) {
+#pragma unroll
+        for (int x = 0; x < 3; ++x) {
+            const double z = v[3 * i + x] / pi - acc[x];
+            out[3 * i + x] = z;
+            d += v[3 * i + x] * z;
+        }
+    }
+    if (part) block_partial(d, part);   // p.Ap of the CG step
+}
+
 // k_apply_table with `lanes` lanes per row (8, 16 or 32).
 static void launch_apply(Ctx* c, int lanes, const IndArgs& a, const double* v, double* out, const double* dotv,
                          double* part, CgTop top) {
@@ -440,6 +512,14 @@ static void launch_apply(Ctx* c, int lanes, const IndArgs& a, const double* v, d
     if (lanes == 8) hipLaunchKernelGGL((k_apply_table<8>), grid, b256, 0, c->stream, a, v, out, dotv, part, top);
     else if (lanes == 32) hipLaunchKernelGGL((k_apply_table<32>), grid, b256, 0, c->stream, a, v, out, dotv, part, top);
     else hipLaunchKernelGGL((k_apply_table<16>), grid, b256, 0, c->stream, a, v, out, dotv, part, top);
+}
+
+static void launch_apply32(Ctx* c, int lanes, const IndArgs& a, const float4* v32, const double* v, double* out,
+                           double* part, CgTop top) {
+    const dim3 grid((unsigned)(((int64_t)c->n * lanes + 255) / 256)), b256(256);
+    if (lanes == 8) hipLaunchKernelGGL((k_apply_table32<8>), grid, b256, 0, c->stream, a, v32, v, out, part, top);
+    else if (lanes == 32) hipLaunchKernelGGL((k_apply_table32<32>), grid, b256, 0, c->stream, a, v32, v, out, part, top);
+    else hipLaunchKernelGGL((k_apply_table32<16>), grid, b256, 0, c->stream, a, v32, v, out, part, top);
 }
 
 // ---- CG -------------------------------------------------------------------------
@@ -456,16 +536,18 @@ struct CgArgs {
     double tol;
     double* x;
     double* r;
-    double* v;
-    double* z;
+    double* v;        // CG direction v (PCG: p)
+    double* z;        // A v
     const double* e0;
     const double* polar;
     double* p_vz;
-    double* p_dv;
+    double* p_dv;     // CG: v.v; PCG: |alpha r|^2
     double* p_rr0;
     double* p_rr1;
     int* state;       // [0] done, [1] iterations, [2] stop reason
     double* scal;     // [0] last residual norm
+    float4* v32;      // mixed solve: FP32 copy of the direction (else null)
+    double inv_npol;  // 1 / (polarizable sites), for the RMS dipole residual
 };
 
 // x = guess (1 as the reference, or the caller's), then after z = A x:
@@ -529,6 +611,95 @@ __global__ __launch_bounds__(CG_THREADS) void k_cg_dir(CgArgs g) {
     block_partial(dv, g.p_dv);
 }
 
+// ---- preconditioned CG ------------------------------------------------------------
+// Jacobi preconditioner M^-1 = diag(alpha): the preconditioned residual
+// alpha_i r_i is the dipole step a Jacobi iteration would take, and its RMS
+// over the polarizable sites (e nm) is the stopping test.  Same recurrence
+// structure as the reference's CG: fixed-order partial sums, iteration count
+// on the device, the scalars of iteration k read from one partial buffer
+// while the next is written to the other.
+
+__device__ __forceinline__ double site_alpha(const double* polar, int k) {
+    const double p = polar[k / 3];
+    return p > POLMIN ? p : POLMIN;
+}
+
+__device__ __forceinline__ void store_dir32(const CgArgs& g, int k, double v) {
+    // the mixed solve's FP32 copy of the direction, one float4 per site
+    if (g.v32) reinterpret_cast<float*>(g.v32)[4 * (k / 3) + k % 3] = (float)v;
+}
+
+// After z = A x: r = E0 - z, p = alpha r; partials of r.alpha r and |alpha r|^2.
+__global__ __launch_bounds__(CG_THREADS) void k_pcg_start(CgArgs g) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    double rz = 0.0, zz = 0.0;
+    if (k < g.dim) {
+        const double r = g.e0[k] - g.z[k];
+        const double zr = site_alpha(g.polar, k) * r;
+        g.r[k] = r;
+        g.v[k] = zr;
+        store_dir32(g, k, zr);
+        rz = r * zr;
+        zz = zr * zr;
+    }
+    block_partial(rz, g.p_rr0);
+    __syncthreads();
+    block_partial(zz, g.p_dv);
+    if (blockIdx.x == 0 && threadIdx.x == 0) { g.state[0] = 0; g.state[1] = 0; g.state[2] = 0; }
+}
+
+// t = rz / p.Ap; x += t p; r -= t Ap; partials of the new r.alpha r and |alpha r|^2.
+template <int PHASE>
+__global__ __launch_bounds__(CG_THREADS) void k_pcg_step(CgArgs g) {
+    if (g.state[0]) return;
+    const double* rz_c = PHASE ? g.p_rr1 : g.p_rr0;
+    double* rz_d = PHASE ? g.p_rr0 : g.p_rr1;
+    const double pAp = block_total(g.p_vz, g.nparts_mv);
+    const double rz = block_total(rz_c, g.nparts_vec);
+    const double t = rz / pAp;
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    double rzn = 0.0, zz = 0.0;
+    if (k < g.dim) {
+        g.x[k] += t * g.v[k];
+        const double r = g.r[k] - t * g.z[k];
+        g.r[k] = r;
+        const double zr = site_alpha(g.polar, k) * r;
+        rzn = r * zr;
+        zz = zr * zr;
+    }
+    block_partial(rzn, rz_d);
+    __syncthreads();
+    block_partial(zz, g.p_dv);
+    if (blockIdx.x == 0 && threadIdx.x == 0) g.state[1] += 1;
+}
+
+// Stop when the RMS dipole residual is below tol; else p = alpha r + beta p.
+template <int PHASE>
+__global__ __launch_bounds__(CG_THREADS) void k_pcg_dir(CgArgs g) {
+    if (g.state[0]) return;
+    const double* rz_c = PHASE ? g.p_rr1 : g.p_rr0;
+    const double* rz_d = PHASE ? g.p_rr0 : g.p_rr1;
+    const double res = sqrt(block_total(g.p_dv, g.nparts_vec) * g.inv_npol);
+    if (blockIdx.x == 0 && threadIdx.x == 0) g.scal[0] = res;
+    if (res < g.tol) {
+        if (blockIdx.x == 0 && threadIdx.x == 0) { g.state[0] = 1; g.state[2] = 2; }
+        return;
+    }
+    const double beta = block_total(rz_d, g.nparts_vec) / block_total(rz_c, g.nparts_vec);
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < g.dim) {
+        const double v = site_alpha(g.polar, k) * g.r[k] + beta * g.v[k];
+        g.v[k] = v;
+        store_dir32(g, k, v);
+    }
+}
+
+// FP32 copy of x (float4 per site) for the mixed solve's first A x.
+__global__ __launch_bounds__(256) void k_to_f32(int dim, const double* __restrict__ x, float* __restrict__ x32) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < dim) x32[4 * (k / 3) + k % 3] = (float)x[k];
+}
+
 // Without minimisation: Pi = alpha E0 (:446-453).
 __global__ __launch_bounds__(256) void k_direct(int dim, const double* __restrict__ polar, const double* __restrict__ e0,
                                                 double* __restrict__ x) {
@@ -568,6 +739,7 @@ static IndArgs ind_args(Ctx* c, const smd_induce_params* p) {
     a.tcnt = c->d_it_cnt;
     a.tj = c->d_it_j;
     a.tab = reinterpret_cast<double2*>(c->d_it_a);
+    a.tab32 = p->prec == SMD_PREC_MIXED ? reinterpret_cast<float2*>(c->d_it_a) : nullptr;
     return a;
 }
 
@@ -614,22 +786,65 @@ int induce_direct_apply(Ctx* c, const smd_induce_params* p, double* x, double* z
     return SMD_OK;
 }
 
-int induce_run(Ctx* c, const smd_induce_params* p, smd_induce_report* rep) {
-    c->ind_last_converged = false;
+static int induce_check(Ctx* c, const smd_induce_params* p) {
     if (!c->multipoles_set) return fail(c, SMD_ESTATE, "multipoles not set");
     if (!c->list_valid) return fail(c, SMD_ESTATE, "no pair list");
     if (!(p->cutoff_ind > 0)) return fail(c, SMD_EINVAL, "cutoff_ind must be positive");
     if (p->cutoff_ind > c->list_cutoff)
         return fail(c, SMD_ENOTSUP, "cutoff_ind %g exceeds the pair-list cutoff %g", p->cutoff_ind, c->list_cutoff);
-    if (p->prec != SMD_PREC_FP64) return fail(c, SMD_ENOTSUP, "induced dipoles are FP64 only");
+    if (p->prec != SMD_PREC_FP64 && p->prec != SMD_PREC_MIXED) return fail(c, SMD_EINVAL, "bad precision %d", p->prec);
     if (p->guess < 0 || p->guess > 3) return fail(c, SMD_EINVAL, "guess %d is not 0..3", p->guess);
+    if (p->precond != 0 && p->precond != 1) return fail(c, SMD_EINVAL, "precond %d is not 0 or 1", p->precond);
+    if (p->tol_rms != p->precond)
+        return fail(c, SMD_ENOTSUP, "the RMS dipole tolerance goes with the preconditioned solve, |r| with the reference's");
+    if (p->prec == SMD_PREC_MIXED && (!p->precond || !p->do_min))
+        return fail(c, SMD_ENOTSUP, "the mixed-precision solve is the preconditioned CG");
     for (int m = 0; m < c->nmol; ++m)
         if (c->h_mol_start[m + 1] - c->h_mol_start[m] != 3)
             return fail(c, SMD_ENOTSUP, "induced dipoles are implemented for 3-site water");
-    if (p->do_min) {
-        const int rc = table_alloc(c);
-        if (rc) return rc;
-    }
+    return SMD_OK;
+}
+
+static CgArgs cg_args(Ctx* c, const smd_induce_params* p) {
+    const int n = c->n, dim = 3 * n;
+    CgArgs g{};
+    g.dim = dim;
+    g.tol = p->tol > 0 ? p->tol : 1.0e-14;   // ConjugateGradient's tolerance (:2404)
+    g.x = c->d_mu;
+    g.r = c->d_cg + 0 * (size_t)dim;
+    g.v = c->d_cg + 1 * (size_t)dim;
+    g.z = c->d_cg + 2 * (size_t)dim;
+    g.e0 = c->d_e0;
+    g.polar = c->d_polar;
+    g.nparts_mv = (int)(((int64_t)n * c->cg_lanes + 255) / 256);   // blocks of the A v kernel
+    g.nparts_vec = (dim + CG_THREADS - 1) / CG_THREADS;
+    g.p_vz = c->d_cg_part;
+    g.p_dv = c->d_cg_part + c->cg_parts;
+    g.p_rr0 = c->d_cg_part + 2 * c->cg_parts;
+    g.p_rr1 = c->d_cg_part + 3 * c->cg_parts;
+    g.state = c->d_cg_state;
+    g.scal = c->d_cg_scal;
+    g.v32 = p->prec == SMD_PREC_MIXED ? c->d_cg32 : nullptr;
+    g.inv_npol = 1.0 / (double)(c->npol > 0 ? c->npol : 1);
+    return g;
+}
+
+// A v of the solve in its precision; v32 is v's FP32 copy (mixed).
+static void solve_apply(Ctx* c, const smd_induce_params* p, const IndArgs& a, const double* v, const float4* v32,
+                        double* out, double* part, CgTop top) {
+    if (p->prec == SMD_PREC_MIXED) launch_apply32(c, c->cg_lanes, a, v32, v, out, part, top);
+    else launch_apply(c, c->cg_lanes, a, v, out, part ? v : nullptr, part, top);
+}
+
+// Enqueues everything of a solve up to the first CG iteration: frames (when
+// `frames`), the permanent field, the coupling table, the start, A x and the
+// residual.  Without minimisation it also sets Pi = alpha E0, and the solve
+// is complete.  Launches only (capturable once the table is sized).
+int induce_begin(Ctx* c, const smd_induce_params* p, bool frames) {
+    int rc = induce_check(c, p);
+    if (rc) return rc;
+    c->ind_last_converged = false;
+    if (p->do_min && (rc = table_alloc(c))) return rc;
     c->ind_last = *p;   // the solve in progress (the learned guess reads its parameters)
     IndArgs a = ind_args(c, p);
     const int n = c->n, dim = 3 * n;
@@ -638,114 +853,144 @@ int induce_run(Ctx* c, const smd_induce_params* p, smd_induce_report* rep) {
     const dim3 grid_vec((dim + CG_THREADS - 1) / CG_THREADS);
     {
         ProfScope ps(c, "ind_field");
-        hipLaunchKernelGGL(k_frames, dim3((c->nmol + 255) / 256), b256, 0, c->stream, a);
+        if (frames) hipLaunchKernelGGL(k_frames, dim3((c->nmol + 255) / 256), b256, 0, c->stream, a);
         hipLaunchKernelGGL((k_field<IND_LANES>), grid_rows, b256, 0, c->stream, a);
         if (p->do_min) hipLaunchKernelGGL((k_ind_table<IND_LANES>), grid_rows, b256, 0, c->stream, a);
     }
-    int iters = 0, reason = 0;
-    double res = 0.0;
     if (!p->do_min) {
         hipLaunchKernelGGL(k_direct, grid_vec, b256, 0, c->stream, dim, c->d_polar, c->d_e0, c->d_mu);
-    } else {
-        CgArgs g{};
-        g.dim = dim;
-        g.tol = p->tol > 0 ? p->tol : 1.0e-14;   // ConjugateGradient's tolerance (:2404)
-        g.x = c->d_mu;
-        g.r = c->d_cg + 0 * (size_t)dim;
-        g.v = c->d_cg + 1 * (size_t)dim;
-        g.z = c->d_cg + 2 * (size_t)dim;
-        g.e0 = c->d_e0;
-        g.polar = c->d_polar;
-        const int lanes = c->cg_lanes;
-        g.nparts_mv = (int)(((int64_t)n * lanes + 255) / 256);   // blocks of the A v kernel
-        g.nparts_vec = (int)grid_vec.x;
-        g.p_vz = c->d_cg_part;
-        g.p_dv = c->d_cg_part + c->cg_parts;
-        g.p_rr0 = c->d_cg_part + 2 * c->cg_parts;
-        g.p_rr1 = c->d_cg_part + 3 * c->cg_parts;
-        g.state = c->d_cg_state;
-        g.scal = c->d_cg_scal;
-        const int max_it = p->max_iter > 0 ? p->max_iter : dim;   // the reference's loop bound
-        ProfScope ps(c, "ind_cg");
-        if (p->guess == 0) hipLaunchKernelGGL(k_fill, grid_vec, b256, 0, c->stream, dim, 1.0, g.x);
-        else if (p->guess == 2) hipLaunchKernelGGL(k_direct, grid_vec, b256, 0, c->stream, dim, c->d_polar, c->d_e0, g.x);
-        else if (p->guess == 3) {
-            // learned: x = alpha E0, z = A x feed the descriptors, then x = the network's guess
-            hipLaunchKernelGGL(k_direct, grid_vec, b256, 0, c->stream, dim, c->d_polar, c->d_e0, g.x);
-            launch_apply(c, lanes, a, g.x, g.z, nullptr, nullptr, CgTop{nullptr, nullptr, 0, 0.0});
-            const int rc = mlp_guess(c, g.x, g.z);
-            if (rc) return rc;
-        }
-        // guess 1: keep the previous solution in d_mu
-        launch_apply(c, lanes, a, g.x, g.z, nullptr, nullptr, CgTop{nullptr, nullptr, 0, 0.0});
-        hipLaunchKernelGGL(k_cg_start, grid_vec, dim3(CG_THREADS), 0, c->stream, g);
-        // count iterations from an even one: A v, the step, the direction
-        auto iterations = [&](int count) {
-            for (int k = 0; k < count; ++k) {
-                launch_apply(c, lanes, a, g.v, g.z, g.v, g.p_vz, CgTop{g.state, g.p_dv, g.nparts_vec, g.tol});
-                if (k & 1) {
-                    hipLaunchKernelGGL(k_cg_step<1>, grid_vec, dim3(CG_THREADS), 0, c->stream, g);
-                    hipLaunchKernelGGL(k_cg_dir<1>, grid_vec, dim3(CG_THREADS), 0, c->stream, g);
-                } else {
-                    hipLaunchKernelGGL(k_cg_step<0>, grid_vec, dim3(CG_THREADS), 0, c->stream, g);
-                    hipLaunchKernelGGL(k_cg_dir<0>, grid_vec, dim3(CG_THREADS), 0, c->stream, g);
-                }
-            }
-        };
-        // A full batch is replayed from a captured graph; it is captured again
-        // when any kernel argument changes (sizes, buffers, box, parameters).
-        std::string key;
-        auto put = [&key](const void* p, size_t bytes) { key.append(static_cast<const char*>(p), bytes); };
-        put(&lanes, sizeof lanes);
-        put(&a.n, sizeof a.n);
-        put(&a.box, sizeof a.box);
-        put(&a.pos, sizeof a.pos);
-        put(&a.polar, sizeof a.polar);
-        put(&a.tstride, sizeof a.tstride);
-        put(&a.tcnt, sizeof a.tcnt);
-        put(&a.tj, sizeof a.tj);
-        put(&a.tab, sizeof a.tab);
-        put(&g.dim, sizeof g.dim);
-        put(&g.nparts_mv, sizeof g.nparts_mv);
-        put(&g.nparts_vec, sizeof g.nparts_vec);
-        put(&g.tol, sizeof g.tol);
-        const void* gp[] = {g.x, g.r, g.v, g.z, g.e0, g.polar, g.p_vz, g.p_dv, g.p_rr0, g.p_rr1, g.state, g.scal};
-        put(gp, sizeof gp);
-        if (c->cg_exec && c->cg_key != key) {
-            (void)hipGraphExecDestroy(c->cg_exec);
-            c->cg_exec = nullptr;
-        }
-        for (int it0 = 0; it0 < max_it; it0 += CG_BATCH) {
-            const int it1 = std::min(max_it, it0 + CG_BATCH);
-            if (it1 - it0 == CG_BATCH && c->cg_graph) {
-                if (!c->cg_exec) {
-                    hipGraph_t graph = nullptr;
-                    SMD_HIP(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-                    iterations(CG_BATCH);
-                    SMD_HIP(c, hipStreamEndCapture(c->stream, &graph));
-                    const hipError_t e = hipGraphInstantiate(&c->cg_exec, graph, nullptr, nullptr, 0);
-                    (void)hipGraphDestroy(graph);
-                    if (e != hipSuccess) {
-                        c->cg_exec = nullptr;
-                        return hip_fail(c, e, "hipGraphInstantiate (CG batch)");
-                    }
-                    c->cg_key = key;
-                }
-                SMD_HIP(c, hipGraphLaunch(c->cg_exec, c->stream));
+        SMD_HIP(c, hipGetLastError());
+        return SMD_OK;
+    }
+    CgArgs g = cg_args(c, p);
+    const bool mixed = p->prec == SMD_PREC_MIXED;
+    const CgTop plain{nullptr, nullptr, 0, 0.0};
+    ProfScope ps(c, "ind_cg");
+    if (p->guess == 0) hipLaunchKernelGGL(k_fill, grid_vec, b256, 0, c->stream, dim, 1.0, g.x);
+    else if (p->guess == 2) hipLaunchKernelGGL(k_direct, grid_vec, b256, 0, c->stream, dim, c->d_polar, c->d_e0, g.x);
+    else if (p->guess == 3) {
+        // learned: x = alpha E0, z = A x feed the descriptors, then x = the network's guess
+        hipLaunchKernelGGL(k_direct, grid_vec, b256, 0, c->stream, dim, c->d_polar, c->d_e0, g.x);
+        if (mixed) hipLaunchKernelGGL(k_to_f32, grid_vec, b256, 0, c->stream, dim, g.x, reinterpret_cast<float*>(g.v32));
+        solve_apply(c, p, a, g.x, g.v32, g.z, nullptr, plain);
+        if ((rc = mlp_guess(c, g.x, g.z))) return rc;
+    }
+    // guess 1: keep the previous solution in d_mu
+    if (mixed) hipLaunchKernelGGL(k_to_f32, grid_vec, b256, 0, c->stream, dim, g.x, reinterpret_cast<float*>(g.v32));
+    solve_apply(c, p, a, g.x, g.v32, g.z, nullptr, plain);
+    if (p->precond) hipLaunchKernelGGL(k_pcg_start, grid_vec, dim3(CG_THREADS), 0, c->stream, g);
+    else hipLaunchKernelGGL(k_cg_start, grid_vec, dim3(CG_THREADS), 0, c->stream, g);
+    SMD_HIP(c, hipGetLastError());
+    return SMD_OK;
+}
+
+// count CG iterations (from an even one): A v, the step, the direction.
+static void cg_iterations(Ctx* c, const smd_induce_params* p, const IndArgs& a, const CgArgs& g, int count) {
+    const dim3 grid_vec(g.nparts_vec), bt(CG_THREADS);
+    for (int k = 0; k < count; ++k) {
+        if (p->precond) {
+            solve_apply(c, p, a, g.v, g.v32, g.z, g.p_vz, CgTop{g.state, nullptr, g.nparts_vec, g.tol});
+            if (k & 1) {
+                hipLaunchKernelGGL(k_pcg_step<1>, grid_vec, bt, 0, c->stream, g);
+                hipLaunchKernelGGL(k_pcg_dir<1>, grid_vec, bt, 0, c->stream, g);
             } else {
-                iterations(it1 - it0);
+                hipLaunchKernelGGL(k_pcg_step<0>, grid_vec, bt, 0, c->stream, g);
+                hipLaunchKernelGGL(k_pcg_dir<0>, grid_vec, bt, 0, c->stream, g);
             }
-            int st[3];
-            SMD_HIP(c, hipMemcpyAsync(st, c->d_cg_state, sizeof st, hipMemcpyDeviceToHost, c->stream));
-            SMD_HIP(c, hipStreamSynchronize(c->stream));
-            if (st[0]) break;
+        } else {
+            launch_apply(c, c->cg_lanes, a, g.v, g.z, g.v, g.p_vz, CgTop{g.state, g.p_dv, g.nparts_vec, g.tol});
+            if (k & 1) {
+                hipLaunchKernelGGL(k_cg_step<1>, grid_vec, bt, 0, c->stream, g);
+                hipLaunchKernelGGL(k_cg_dir<1>, grid_vec, bt, 0, c->stream, g);
+            } else {
+                hipLaunchKernelGGL(k_cg_step<0>, grid_vec, bt, 0, c->stream, g);
+                hipLaunchKernelGGL(k_cg_dir<0>, grid_vec, bt, 0, c->stream, g);
+            }
         }
-        int st[3];
-        SMD_HIP(c, hipMemcpyAsync(st, c->d_cg_state, sizeof st, hipMemcpyDeviceToHost, c->stream));
-        SMD_HIP(c, hipMemcpyAsync(&res, c->d_cg_scal, sizeof res, hipMemcpyDeviceToHost, c->stream));
-        SMD_HIP(c, hipStreamSynchronize(c->stream));
-        iters = st[1];
-        reason = st[2];
+    }
+}
+
+// Enqueues `batches` batches of cg_batch iterations (iterations after the
+// stop test passed return at once).  A batch is replayed from a captured
+// graph, captured again when any kernel argument changes.
+int induce_batches(Ctx* c, const smd_induce_params* p, int batches) {
+    IndArgs a = ind_args(c, p);
+    CgArgs g = cg_args(c, p);
+    if (!c->cg_graph) {
+        for (int b = 0; b < batches; ++b) cg_iterations(c, p, a, g, c->cg_batch);
+        SMD_HIP(c, hipGetLastError());
+        return SMD_OK;
+    }
+    std::string key;
+    auto put = [&key](const void* q, size_t bytes) { key.append(static_cast<const char*>(q), bytes); };
+    put(&c->cg_lanes, sizeof c->cg_lanes);
+    put(&c->cg_batch, sizeof c->cg_batch);
+    put(&p->precond, sizeof p->precond);
+    put(&p->prec, sizeof p->prec);
+    put(&a.n, sizeof a.n);
+    put(&a.box, sizeof a.box);
+    put(&a.pos, sizeof a.pos);
+    put(&a.polar, sizeof a.polar);
+    put(&a.tstride, sizeof a.tstride);
+    put(&a.tcnt, sizeof a.tcnt);
+    put(&a.tj, sizeof a.tj);
+    put(&a.tab, sizeof a.tab);
+    put(&a.tab32, sizeof a.tab32);
+    put(&g, sizeof g);
+    if (c->cg_exec && c->cg_key != key) {
+        (void)hipGraphExecDestroy(c->cg_exec);
+        c->cg_exec = nullptr;
+    }
+    if (!c->cg_exec) {
+        hipGraph_t graph = nullptr;
+        SMD_HIP(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+        cg_iterations(c, p, a, g, c->cg_batch);
+        SMD_HIP(c, hipStreamEndCapture(c->stream, &graph));
+        const hipError_t e = hipGraphInstantiate(&c->cg_exec, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        if (e != hipSuccess) {
+            c->cg_exec = nullptr;
+            return hip_fail(c, e, "hipGraphInstantiate (CG batch)");
+        }
+        c->cg_key = key;
+    }
+    for (int b = 0; b < batches; ++b) SMD_HIP(c, hipGraphLaunch(c->cg_exec, c->stream));
+    return SMD_OK;
+}
+
+// Waits for the stream and reads the solve's state: done, iterations, stop
+// reason, last residual.
+int induce_poll(Ctx* c, int* done, int* iters, int* reason, double* res) {
+    int st[3];
+    double r = 0.0;
+    SMD_HIP(c, hipMemcpyAsync(st, c->d_cg_state, sizeof st, hipMemcpyDeviceToHost, c->stream));
+    SMD_HIP(c, hipMemcpyAsync(&r, c->d_cg_scal, sizeof r, hipMemcpyDeviceToHost, c->stream));
+    SMD_HIP(c, hipStreamSynchronize(c->stream));
+    if (done) *done = st[0];
+    if (iters) *iters = st[1];
+    if (reason) *reason = st[2];
+    if (res) *res = r;
+    return SMD_OK;
+}
+
+int induce_run(Ctx* c, const smd_induce_params* p, smd_induce_report* rep) {
+    int rc = induce_begin(c, p, true);
+    if (rc) return rc;
+    int iters = 0, reason = 0;
+    double res = 0.0;
+    if (p->do_min) {
+        const int max_it = p->max_iter > 0 ? p->max_iter : 3 * c->n;   // the reference's loop bound
+        int done = 0;
+        const int batch = c->cg_batch;
+        for (int it0 = 0; it0 < max_it && !done; it0 += batch) {
+            const int it1 = std::min(max_it, it0 + batch);
+            if (it1 - it0 == batch) {
+                if ((rc = induce_batches(c, p, 1))) return rc;
+            } else {
+                cg_iterations(c, p, ind_args(c, p), cg_args(c, p), it1 - it0);
+            }
+            if ((rc = induce_poll(c, &done, &iters, &reason, &res))) return rc;
+        }
     }
     SMD_HIP(c, hipGetLastError());
     SMD_HIP(c, hipStreamSynchronize(c->stream));
@@ -769,11 +1014,19 @@ extern "C" int smd_induce_time_apply(smd_ctx* c, int reps, double* avg_us) {
     if (!c->ind_last_converged) return fail(c, SMD_ESTATE, "no converged smd_induce on this state");
     IndArgs a = ind_args(c, &c->ind_last);
     double* z = c->d_cg + 2 * (size_t)(3 * c->n);   // CG scratch; the solution in d_mu is untouched
+    const bool mixed = c->ind_last.prec == SMD_PREC_MIXED;
+    const int dim = 3 * c->n;
+    if (mixed)
+        hipLaunchKernelGGL(k_to_f32, dim3((dim + 255) / 256), dim3(256), 0, c->stream, dim, c->d_mu,
+                           reinterpret_cast<float*>(c->d_cg32));
     hipEvent_t e0 = nullptr, e1 = nullptr;
     SMD_HIP(c, hipEventCreate(&e0));
     SMD_HIP(c, hipEventCreate(&e1));
     SMD_HIP(c, hipEventRecord(e0, c->stream));
-    for (int k = 0; k < reps; ++k) launch_apply(c, c->cg_lanes, a, c->d_mu, z, nullptr, nullptr, CgTop{nullptr, nullptr, 0, 0.0});
+    for (int k = 0; k < reps; ++k) {
+        if (mixed) launch_apply32(c, c->cg_lanes, a, c->d_cg32, c->d_mu, z, nullptr, CgTop{nullptr, nullptr, 0, 0.0});
+        else launch_apply(c, c->cg_lanes, a, c->d_mu, z, nullptr, nullptr, CgTop{nullptr, nullptr, 0, 0.0});
+    }
     SMD_HIP(c, hipEventRecord(e1, c->stream));
     SMD_HIP(c, hipEventSynchronize(e1));
     float ms = 0.f;
@@ -801,6 +1054,8 @@ extern "C" int smd_set_multipoles(smd_ctx* c, const double* dip_local, const dou
             (rc = A(c->d_polar, sizeof(double) * n)) || (rc = A(c->d_dipg, sizeof(double) * 3 * n)) ||
             (rc = A(c->d_quadg, sizeof(double) * 6 * n)) || (rc = A(c->d_e0, sizeof(double) * 3 * n)) ||
             (rc = A(c->d_mu, sizeof(double) * 3 * n)) || (rc = A(c->d_cg, sizeof(double) * 9 * n)) ||
+            (rc = A(c->d_cg32, sizeof(float4) * n)) || (rc = A(c->d_site32, sizeof(float4) * 3 * n)) ||
+            (rc = A(c->d_pind32, sizeof(float4) * n)) ||
             (rc = A(c->d_cg_part, sizeof(double) * 4 * parts)) || (rc = A(c->d_cg_state, sizeof(int) * 4)) ||
             (rc = A(c->d_cg_scal, sizeof(double) * 4)))
             return rc;
@@ -811,6 +1066,8 @@ extern "C" int smd_set_multipoles(smd_ctx* c, const double* dip_local, const dou
     SMD_HIP(c, hipMemcpy(c->d_dipl, dip_local, sizeof(double) * 3 * n, hipMemcpyHostToDevice));
     SMD_HIP(c, hipMemcpy(c->d_quadl, quad_local, sizeof(double) * 6 * n, hipMemcpyHostToDevice));
     SMD_HIP(c, hipMemcpy(c->d_polar, polar, sizeof(double) * n, hipMemcpyHostToDevice));
+    c->npol = 0;
+    for (int i = 0; i < n; ++i) c->npol += polar[i] > POLMIN ? 1 : 0;
     c->multipoles_set = true;
     return SMD_OK;
 }

@@ -48,50 +48,69 @@ struct IntegArgs {
     int surface, use_constraint, lustig;
     double* stress;          // [12] (Ctx::d_stress)
     double stress_after;     // samples count once the call counter exceeds 0.15 totsteps
+    // fixed-order block partials (Ctx::part_*)
+    double* part_vom;        // [4][nb_vom] drift momentum and mass of k_step_head
+    double* part_kin;        // [KIN_PARTS][gridDim] kinetics kernels
+    double* part_sh;         // [9][gridDim] k_shake
+    int nb_vom;
+    int zero_force;          // k_step_head also clears the forces (the polarizable step)
 };
 
+// The drift velocity of LFIntegrator::run (:354-379) from k_step_head's
+// block partials: every block forms the same fixed-order totals, and the
+// velocity is zero unless a component exceeds 1e-11.
+__device__ __forceinline__ void drift_velocity(const IntegArgs& a, double (&vc)[3]) {
+    __shared__ double tot[4];
+    if (threadIdx.x < WAVE)
+        for (int k = 0; k < 4; ++k) {
+            const double t = wave_fixed_sum(a.part_vom + (size_t)k * a.nb_vom, a.nb_vom);
+            if (threadIdx.x == 0) tot[k] = t;
+        }
+    __syncthreads();
+    const double mt = tot[3];
+    const double vx = tot[0] / mt, vy = tot[1] / mt, vz = tot[2] / mt;
+    const bool on = fabs(vx) > 1e-11 || fabs(vy) > 1e-11 || fabs(vz) > 1e-11;
+    vc[0] = on ? vx : 0.0;
+    vc[1] = on ? vy : 0.0;
+    vc[2] = on ? vz : 0.0;
+}
+
 // Head of the MD step: the Verlet trigger (any |displacement|^2 > cutBuff^2/4,
-// LFIntegrator.cpp:306-341) and the momentum sums of the drift removal.
-// Block 0 clears the step's other accumulators.  The drift sums start at zero:
-// the previous step's k_kinetics (or smd_md_init) cleared them, and the rebuild
-// flag was cleared by the previous step's k_shake.
+// LFIntegrator.cpp:306-341) and the block partials of the drift removal's
+// momentum sums.  Block 0 clears the step's accumulators.  The rebuild flag
+// was cleared by the previous step's integration kernel.
 __global__ __launch_bounds__(256) void k_step_head(IntegArgs a) {
     if (blockIdx.x == 0)
-        for (int k = threadIdx.x; k < T_COUNT; k += blockDim.x)
-            if (k < T_VOM || k > T_MTOT) a.terms[k] = 0.0;
+        for (int k = threadIdx.x; k < T_COUNT; k += blockDim.x) a.terms[k] = 0.0;
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     bool need = false;
     double v[4] = {0, 0, 0, 0};
     if (i < a.n) {
+        if (a.zero_force) { a.force[3 * i] = 0.0; a.force[3 * i + 1] = 0.0; a.force[3 * i + 2] = 0.0; }
         const double x = a.disp[3 * i], y = a.disp[3 * i + 1], z = a.disp[3 * i + 2];
         need = len2_exact(x, y, z) > a.drmax;
         const double m = a.mass[i];
         v[0] = a.vtm[3 * i] * m; v[1] = a.vtm[3 * i + 1] * m; v[2] = a.vtm[3 * i + 2] * m; v[3] = m;
     }
     if (__ballot(need) && (threadIdx.x % WAVE) == 0) a.flag[0] = 1;
-    block_add<4>(v, a.terms + T_VOM);
+    block_store<4>(v, a.part_vom, a.nb_vom);
 }
 
 // SHAKE of one water (a.mol_start[m], three sites).  Stores the new
 // positions, displacements, unwrapped positions, velocities and vtMinus, adds
 // its constraint virial to v, and leaves the new positions and velocities,
 // the forces it used and the masses in pn, vel, F and ms.
-__device__ __forceinline__ void shake_mol(const IntegArgs& a, int m, double (&v)[9], double (&pn)[3][3],
-                                          double (&vel)[3][3], double (&F)[3][3], double (&ms)[3]) {
+// vcm: the MD step's drift velocity (LFIntegrator::run :354-379), removed
+// from vtMinus (the velocity is rewritten below from vtMinus, so only vtMinus
+// needs it); zero outside the MD step.
+__device__ __forceinline__ void shake_mol(const IntegArgs& a, int m, const double (&vcm)[3], double (&v)[9],
+                                          double (&pn)[3][3], double (&vel)[3][3], double (&F)[3][3],
+                                          double (&ms)[3]) {
     {
         const int a0 = a.mol_start[m];
         const double fnh = a.fnh_on_device ? a.scalars[0] : a.fnh_host;
         const double Px = a.mom[3 * m], Py = a.mom[3 * m + 1], Pz = a.mom[3 * m + 2];
         const double tol_2 = 2 * a.tol;
-        // MD step: the drift removal of LFIntegrator::run (:354-379) on
-        // vtMinus, from the sums k_step_head formed (velocity is rewritten
-        // below from vtMinus, so only vtMinus needs it).
-        double vcm[3] = {0.0, 0.0, 0.0};
-        if (a.md_step) {
-            const double mt = a.terms[T_MTOT];
-            const double vx = a.terms[T_VOM] / mt, vy = a.terms[T_VOM + 1] / mt, vz = a.terms[T_VOM + 2] / mt;
-            if (fabs(vx) > 1e-11 || fabs(vy) > 1e-11 || fabs(vz) > 1e-11) { vcm[0] = vx; vcm[1] = vy; vcm[2] = vz; }
-        }
         double ro[3][3], rn[3][3], mr[3], vt[3][3];
 #pragma unroll
         for (int s = 0; s < 3; ++s) {
@@ -210,32 +229,34 @@ __device__ __forceinline__ void shake_mol(const IntegArgs& a, int m, double (&v)
 
 __global__ __launch_bounds__(256) void k_shake(IntegArgs a) {
     double v[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    double vcm[3] = {0.0, 0.0, 0.0};
+    if (a.md_step) drift_velocity(a, vcm);
     const int m = blockIdx.x * blockDim.x + threadIdx.x;
     if (m < a.nmol) {
         double pn[3][3], vel[3][3], F[3][3], ms[3];
-        shake_mol(a, m, v, pn, vel, F, ms);
+        shake_mol(a, m, vcm, v, pn, vel, F, ms);
     }
-    block_add<9>(v, a.terms + T_SH_VIR);
+    // the constraint virial: block partials, summed in a fixed order by the
+    // last block
+    block_store<9>(v, a.part_sh, gridDim.x);
+    if (last_block(a.flag + 3)) fixed_totals(a.part_sh, gridDim.x, gridDim.x, 9, a.terms + T_SH_VIR);
 }
 
 __global__ __launch_bounds__(256) void k_lf_atoms(IntegArgs a) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (a.md_step && i == 0) a.flag[0] = 0;   // every consumer of the rebuild flag has run
+    double vc[3] = {0.0, 0.0, 0.0};
+    if (a.md_step) {
+        drift_velocity(a, vc);
+        if (i == 0) a.flag[0] = 0;   // every consumer of the rebuild flag has run
+    }
     if (i >= a.n) return;
     const double fnh = a.fnh_on_device ? a.scalars[0] : a.fnh_host;
     const double mr = 1.0 / a.mass[i];
-    if (a.md_step) {
-        // the drift removal of LFIntegrator::run (:354-379) on vtMinus and
-        // the velocity, from the sums k_step_head formed
-        const double mt = a.terms[T_MTOT];
-        const double vx = a.terms[T_VOM] / mt, vy = a.terms[T_VOM + 1] / mt, vz = a.terms[T_VOM + 2] / mt;
-        if (fabs(vx) > 1e-11 || fabs(vy) > 1e-11 || fabs(vz) > 1e-11) {
-            const double vc[3] = {vx, vy, vz};
-            for (int d = 0; d < 3; ++d) {
-                a.vtm[3 * i + d] -= vc[d];
-                a.vel[3 * i + d] -= vc[d];
-            }
-        }
+    // the drift removal of LFIntegrator::run (:354-379) on vtMinus and the
+    // velocity (zero outside the MD step)
+    for (int d = 0; d < 3; ++d) {
+        a.vtm[3 * i + d] -= vc[d];
+        a.vel[3 * i + d] -= vc[d];
     }
     double vp[3];
     if (a.nose_hoover) {
@@ -357,29 +378,30 @@ __device__ void md_pressure(const IntegArgs& a, const double* t) {
 // spread over as many CUs as there are waves of molecules.
 constexpr int KIN_THREADS = 64;
 
-// The MD step's last block (atomic ticket): Nose-Hoover update, pressure
-// tensors and stress sums from the sums every block added.
-__device__ __forceinline__ void step_epilogue(const IntegArgs& a) {
-    __shared__ bool last;
-    __threadfence();
+// Rows of Ctx::part_kin: the kinetic sums (2), the pressure sums (21), the
+// SHAKE virial (9).
+constexpr int KP_KE = 0, KP_PR = 2, KP_SH = 23;
+
+// End of the kinetics kernels.  Every block stores its partials; the last
+// block (atomic ticket) sums them in a fixed order into the accumulators
+// and, in the MD step, forms the Nose-Hoover update, the pressure tensors
+// and the stress sums.
+__device__ __forceinline__ void kin_epilogue(const IntegArgs& a, bool shake_rows) {
+    if (!last_block(a.flag + 2)) return;
+    const int nb = gridDim.x;
+    fixed_totals(a.part_kin + (size_t)KP_KE * nb, nb, nb, 2, a.terms + T_KE_ATOM);
+    if (a.md_step) fixed_totals(a.part_kin + (size_t)KP_PR * nb, nb, nb, 21, a.terms + T_PR_MC);
+    if (shake_rows) fixed_totals(a.part_kin + (size_t)KP_SH * nb, nb, nb, 9, a.terms + T_SH_VIR);
+    if (!a.md_step) return;
+    // the step's sums, read once by the whole block (the epilogue's thread
+    // then works from LDS, not from a chain of dependent global loads)
+    __shared__ double tl[T_COUNT];
+    volatile const double* tv = a.terms;
+    for (int k = threadIdx.x; k < T_COUNT; k += blockDim.x) tl[k] = tv[k];
     __syncthreads();
-    if (threadIdx.x == 0) last = atomicAdd(&a.flag[2], 1) == (int)gridDim.x - 1;
-    __syncthreads();
-    if (last) {   // block-uniform
-        // the sums every block added, read once by the whole block (the
-        // epilogue's thread then works from LDS, not from a chain of
-        // dependent global loads)
-        __shared__ double tl[T_COUNT];
-        __threadfence();
-        volatile const double* tv = a.terms;
-        for (int k = threadIdx.x; k < T_COUNT; k += blockDim.x) tl[k] = tv[k];
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            nh_friction(a, tl[T_KE_ATOM], tl[T_KE_MOL]);
-            md_pressure(a, tl);
-            for (int k = T_VOM; k <= T_MTOT; ++k) a.terms[k] = 0.0;
-            a.flag[2] = 0;
-        }
+    if (threadIdx.x == 0) {
+        nh_friction(a, tl[T_KE_ATOM], tl[T_KE_MOL]);
+        md_pressure(a, tl);
     }
 }
 
@@ -439,12 +461,10 @@ __global__ __launch_bounds__(KIN_THREADS) void k_kinetics(IntegArgs a, int rewra
         }
         for (int d = 0; d < 3; ++d) a.com[3 * m + d] = C[d];
     }
-    block_add<2>(v, a.terms + T_KE_ATOM);
-    if (a.md_step) {
-        __syncthreads();
-        block_add<21>(pr, a.terms + T_PR_MC);
-        step_epilogue(a);
-    }
+    const int nb = gridDim.x;
+    block_store<2>(v, a.part_kin + (size_t)KP_KE * nb, nb);
+    if (a.md_step) block_store<21>(pr, a.part_kin + (size_t)KP_PR * nb, nb);
+    kin_epilogue(a, false);
 }
 
 // MD step, rigid water: SHAKE of each molecule and, from the positions and
@@ -456,10 +476,12 @@ __global__ __launch_bounds__(KIN_THREADS) void k_shake_kinetics(IntegArgs a, int
     double pr[21];
 #pragma unroll
     for (int k = 0; k < 21; ++k) pr[k] = 0.0;
+    double vcm[3];
+    drift_velocity(a, vcm);
     const int m = blockIdx.x * blockDim.x + threadIdx.x;
     if (m < a.nmol) {
         double pn[3][3], vel[3][3], F[3][3], ms[3];
-        shake_mol(a, m, sv, pn, vel, F, ms);
+        shake_mol(a, m, vcm, sv, pn, vel, F, ms);
         const int s0 = a.mol_start[m];
         const double M = (0.0 + ms[0] + ms[1]) + ms[2];
         double P[3] = {0, 0, 0}, C[3] = {0, 0, 0};
@@ -502,12 +524,11 @@ __global__ __launch_bounds__(KIN_THREADS) void k_shake_kinetics(IntegArgs a, int
         }
         for (int d = 0; d < 3; ++d) a.com[3 * m + d] = C[d];
     }
-    block_add<9>(sv, a.terms + T_SH_VIR);
-    __syncthreads();
-    block_add<2>(v, a.terms + T_KE_ATOM);
-    __syncthreads();
-    block_add<21>(pr, a.terms + T_PR_MC);
-    step_epilogue(a);
+    const int nb = gridDim.x;
+    block_store<9>(sv, a.part_kin + (size_t)KP_SH * nb, nb);
+    block_store<2>(v, a.part_kin + (size_t)KP_KE * nb, nb);
+    block_store<21>(pr, a.part_kin + (size_t)KP_PR * nb, nb);
+    kin_epilogue(a, true);
 }
 
 __global__ __launch_bounds__(256) void k_mol_centers(int nmol, const int32_t* __restrict__ ms,
@@ -624,6 +645,10 @@ static IntegArgs integ_args(Ctx* c, const smd_integrator_params* p) {
     a.scalars = c->d_scalars;
     a.flag = c->d_flag;
     a.status = c->d_status;
+    a.part_vom = c->part_vom;
+    a.part_kin = c->part_kin;
+    a.part_sh = c->part_sh;
+    a.nb_vom = c->nb_atoms;
     if (p) {
         a.dt = p->dt;
         a.tol = p->tol;
@@ -703,6 +728,31 @@ int lf_atoms_run(Ctx* c, const smd_integrator_params* p, double fnh, bool md_ste
     return SMD_OK;
 }
 
+
+// Pieces of the polarizable MD step (polar_md.hip): the head (trigger,
+// drift partials, cleared terms and forces) and the SHAKE + kinetics +
+// Nose-Hoover tail, with c->md describing the step.
+int md_head_enqueue(Ctx* c) {
+    IntegArgs a = integ_args(c, &c->md.integ);
+    a.fnh_on_device = 1;
+    a.drmax = c->md.cut_buff * c->md.cut_buff / 4.0;   // Integrator.cpp:36
+    a.zero_force = 1;
+    hipLaunchKernelGGL(k_step_head, dim3((c->n + 255) / 256), dim3(256), 0, c->stream, a);
+    SMD_HIP(c, hipGetLastError());
+    return SMD_OK;
+}
+
+int md_tail_enqueue(Ctx* c) {
+    c->shake_frecip = false;
+    return shake_kinetics_run(c, &c->md.integ);
+}
+
+int md_start_vtm(Ctx* c) {
+    hipLaunchKernelGGL(k_start_vtm, dim3((c->n + 255) / 256), dim3(256), 0, c->stream, c->n, c->d_vel, c->d_force,
+                       c->d_mass, c->d_vtm, c->md.integ.dt);
+    SMD_HIP(c, hipGetLastError());
+    return SMD_OK;
+}
 
 // One LFIntegrator::run iteration enqueued on the context stream.  Contains
 // no host synchronisation and no memset, so it can be captured.
@@ -936,7 +986,7 @@ extern "C" int smd_md_init(smd_ctx* c, const smd_md_params* p) {
     SMD_HIP(c, hipMemsetAsync(c->d_mom, 0, sizeof(double) * 3 * c->nmol, c->stream));
     SMD_HIP(c, hipMemsetAsync(c->d_disp, 0, sizeof(double) * 3 * c->n, c->stream));
     SMD_HIP(c, hipMemsetAsync(c->d_status, 0, sizeof(unsigned), c->stream));
-    SMD_HIP(c, hipMemsetAsync(c->d_flag, 0, sizeof(int) * 4, c->stream));   // rebuild flag, block counter
+    SMD_HIP(c, hipMemsetAsync(c->d_flag, 0, sizeof(int) * 8, c->stream));   // rebuild flag, block tickets
     SMD_HIP(c, hipStreamSynchronize(c->stream));
     if (c->graph_exec) { (void)hipGraphExecDestroy(c->graph_exec); c->graph_exec = nullptr; }
     if (c->graph) { (void)hipGraphDestroy(c->graph); c->graph = nullptr; }

@@ -29,6 +29,7 @@
 // All arithmetic is FP64.
 #include <algorithm>
 #include <cmath>
+#include <string>
 #include <vector>
 
 #include "smd_internal.h"
@@ -89,41 +90,57 @@ struct MpArgs {
     const double* polar;        // polarizabilities [n]
     double a_damp;
     int damp_amoeba, do_min;
+    // fixed-order partials of the per-site sums (Ctx::part_su) and their ticket
+    double* spart;
+    int* sticket;
+    // mixed form: FP32 site records [3n] and induced dipoles [n]
+    const float4* site32;
+    const float4* pind32;
 };
 
-struct V3 { double x, y, z; };
+// Vector, symmetric-matrix and site types in the pair math's precision R
+// (double for the plain form, float for the mixed form's pair math).
+template <typename R> struct V3T { R x, y, z; };
+using V3 = V3T<double>;
+template <typename R> __device__ __forceinline__ V3T<R> vec3(R x, R y, R z) { return {x, y, z}; }
 __device__ __forceinline__ V3 v3(double x, double y, double z) { return {x, y, z}; }
-__device__ __forceinline__ V3 operator+(V3 a, V3 b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
-__device__ __forceinline__ V3 operator-(V3 a, V3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
-__device__ __forceinline__ V3 operator-(V3 a) { return {-a.x, -a.y, -a.z}; }
-__device__ __forceinline__ V3 operator*(double f, V3 a) { return {f * a.x, f * a.y, f * a.z}; }
-__device__ __forceinline__ double dot(V3 a, V3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
-__device__ __forceinline__ V3 cross(V3 a, V3 b) {
+template <typename R> __device__ __forceinline__ V3T<R> operator+(V3T<R> a, V3T<R> b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+template <typename R> __device__ __forceinline__ V3T<R> operator-(V3T<R> a, V3T<R> b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+template <typename R> __device__ __forceinline__ V3T<R> operator-(V3T<R> a) { return {-a.x, -a.y, -a.z}; }
+template <typename R> __device__ __forceinline__ V3T<R> operator*(R f, V3T<R> a) { return {f * a.x, f * a.y, f * a.z}; }
+template <typename R> __device__ __forceinline__ R dot(V3T<R> a, V3T<R> b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+template <typename R> __device__ __forceinline__ V3T<R> cross(V3T<R> a, V3T<R> b) {
     return {a.y * b.z - b.y * a.z, b.x * a.z - a.x * b.z, a.x * b.y - b.x * a.y};
 }
 __device__ __forceinline__ V3 ldv(const double* p) { return {p[0], p[1], p[2]}; }
+// FP64 accumulation of an R-precision vector
+template <typename R> __device__ __forceinline__ V3 to64(V3T<R> a) { return {(double)a.x, (double)a.y, (double)a.z}; }
 
 // Symmetric matrix as xx xy xz yy yz zz.
-struct S6 { double xx, xy, xz, yy, yz, zz; };
+template <typename R> struct S6T { R xx, xy, xz, yy, yz, zz; };
+using S6 = S6T<double>;
 __device__ __forceinline__ S6 lds6(const double* p) { return {p[0], p[1], p[2], p[3], p[4], p[5]}; }
-__device__ __forceinline__ V3 smul(const S6& Q, V3 r) {
+template <typename R> __device__ __forceinline__ V3T<R> smul(const S6T<R>& Q, V3T<R> r) {
     return {Q.xx * r.x + Q.xy * r.y + Q.xz * r.z, Q.xy * r.x + Q.yy * r.y + Q.yz * r.z,
             Q.xz * r.x + Q.yz * r.y + Q.zz * r.z};
 }
-__device__ __forceinline__ double trace(const S6& Q) { return Q.xx + Q.yy + Q.zz; }
+template <typename R> __device__ __forceinline__ R trace(const S6T<R>& Q) { return Q.xx + Q.yy + Q.zz; }
 
 // cross(A, B, ones) of the reference (src/utils/Vector3.h:220-225) for
 // A = a b^T and B symmetric: sum over columns c of the cross product of
 // column c of A's rows, i.e. cross(a, B b) with B b the matrix-vector product.
-__device__ __forceinline__ V3 mcross_outer(V3 a, V3 b, const S6& B) { return cross(a, smul(B, b)); }
+template <typename R>
+__device__ __forceinline__ V3T<R> mcross_outer(V3T<R> a, V3T<R> b, const S6T<R>& B) { return cross(a, smul(B, b)); }
 // cross(A, B, ones) for A, B symmetric: x = sum_c A[1][c] B[2][c] - A[2][c] B[1][c], ...
-__device__ __forceinline__ V3 mcross_sym(const S6& A, const S6& B) {
+template <typename R>
+__device__ __forceinline__ V3T<R> mcross_sym(const S6T<R>& A, const S6T<R>& B) {
     return {A.xy * B.xz - A.xz * B.xy + A.yy * B.yz - A.yz * B.yy + A.yz * B.zz - A.zz * B.yz,
             A.xz * B.xx - A.xx * B.xz + A.yz * B.xy - A.xy * B.yz + A.zz * B.xz - A.xz * B.zz,
             A.xx * B.xy - A.xy * B.xx + A.xy * B.yy - A.yy * B.xy + A.xz * B.yz - A.yz * B.xz};
 }
 
-struct Site { double q; V3 d; S6 Q; };
+template <typename R> struct SiteT { R q; V3T<R> d; S6T<R> Q; };
+using Site = SiteT<double>;
 
 __device__ __forceinline__ Site load_site(const MpArgs& a, int i) {
     Site s;
@@ -133,102 +150,155 @@ __device__ __forceinline__ Site load_site(const MpArgs& a, int i) {
     return s;
 }
 
+template <typename R> __device__ __forceinline__ SiteT<R> site_as(const Site& s) {
+    return {(R)s.q, {(R)s.d.x, (R)s.d.y, (R)s.d.z}, {(R)s.Q.xx, (R)s.Q.xy, (R)s.Q.xz, (R)s.Q.yy, (R)s.Q.yz, (R)s.Q.zz}};
+}
+template <typename R> __device__ __forceinline__ SiteT<R> load_site_as(const MpArgs& a, int i) {
+    return site_as<R>(load_site(a, i));
+}
+
 // Multipole pair of "self" site I and partner J at r = r_I - r_J: the G_l
 // coefficients, the force on I in units of the B_l (or D_l) radial functions,
 // and the torque on I (real_term :548-617 and :730-877).  The torque the
 // reference assigns to the partner of a pair (Gt_j) is this expression with
 // the roles of the two sites swapped.
-struct PairOut { double G[5]; V3 gG[5]; V3 Gt[5]; };
+template <typename R> struct PairOutT { R G[5]; V3T<R> gG[5]; V3T<R> Gt[5]; };
+using PairOut = PairOutT<double>;
 
-template <bool GRAD, bool TORQUE>
-__device__ __forceinline__ void mp_pair(const Site& I, const Site& J, V3 r, PairOut& o) {
-    const double qi = I.q, qj = J.q;
-    const V3 mi = I.d, mj = J.d;
-    const double QiI = trace(I.Q), QjI = trace(J.Q);
-    const V3 Qir = smul(I.Q, r), Qjr = smul(J.Q, r);
-    const double QiR = dot(r, Qir), QjR = dot(r, Qjr);
-    const double QiQj = I.Q.xx * J.Q.xx + I.Q.yy * J.Q.yy + I.Q.zz * J.Q.zz +
-                        2.0 * (I.Q.xy * J.Q.xy + I.Q.xz * J.Q.xz + I.Q.yz * J.Q.yz);
-    const double mir = dot(mi, r), mjr = dot(mj, r);
-    const double QimjXr = dot(mj, Qir), QjmiXr = dot(mi, Qjr);
-    const double QirQj = dot(Qir, Qjr);   // r Qi Qj r
+template <bool GRAD, bool TORQUE, typename R>
+__device__ __forceinline__ void mp_pair(const SiteT<R>& I, const SiteT<R>& J, V3T<R> r, PairOutT<R>& o) {
+    const R qi = I.q, qj = J.q;
+    const V3T<R> mi = I.d, mj = J.d;
+    const R QiI = trace(I.Q), QjI = trace(J.Q);
+    const V3T<R> Qir = smul(I.Q, r), Qjr = smul(J.Q, r);
+    const R QiR = dot(r, Qir), QjR = dot(r, Qjr);
+    const R QiQj = I.Q.xx * J.Q.xx + I.Q.yy * J.Q.yy + I.Q.zz * J.Q.zz +
+                   R(2) * (I.Q.xy * J.Q.xy + I.Q.xz * J.Q.xz + I.Q.yz * J.Q.yz);
+    const R mir = dot(mi, r), mjr = dot(mj, r);
+    const R QimjXr = dot(mj, Qir), QjmiXr = dot(mi, Qjr);
+    const R QirQj = dot(Qir, Qjr);   // r Qi Qj r
     o.G[0] = qi * qj;
     o.G[1] = qj * mir - qi * mjr - dot(mi, mj) + qj * QiI + qi * QjI;
-    o.G[2] = qj * QiR + qi * QjR - mir * mjr - mjr * QiI + mir * QjI - 2.0 * QimjXr + 2.0 * QjmiXr + QiI * QjI +
-             2.0 * QiQj;
-    o.G[3] = mir * QjR - mjr * QiR + QiI * QjR + QjI * QiR + 4.0 * QirQj;
+    o.G[2] = qj * QiR + qi * QjR - mir * mjr - mjr * QiI + mir * QjI - R(2) * QimjXr + R(2) * QjmiXr + QiI * QjI +
+             R(2) * QiQj;
+    o.G[3] = mir * QjR - mjr * QiR + QiI * QjR + QjI * QiR + R(4) * QirQj;
     o.G[4] = QiR * QjR;
     if constexpr (GRAD) {
-        const V3 Qimj = smul(I.Q, mj), Qjmi = smul(J.Q, mi);
-        const V3 QiQjr = smul(I.Q, Qjr), QjQir = smul(J.Q, Qir);
+        const V3T<R> Qimj = smul(I.Q, mj), Qjmi = smul(J.Q, mi);
+        const V3T<R> QiQjr = smul(I.Q, Qjr), QjQir = smul(J.Q, Qir);
         o.gG[1] = qj * mi - qi * mj;
-        o.gG[2] = (2.0 * qj) * Qir - mjr * mi - mir * mj - 2.0 * Qimj - QiI * mj + (2.0 * qi) * Qjr + 2.0 * Qjmi +
+        o.gG[2] = (R(2) * qj) * Qir - mjr * mi - mir * mj - R(2) * Qimj - QiI * mj + (R(2) * qi) * Qjr + R(2) * Qjmi +
                   QjI * mi;
-        o.gG[3] = (-QiR) * mj - (2.0 * mjr) * Qir + QjR * mi + (2.0 * mir) * Qjr + (2.0 * QiI) * Qjr +
-                  (2.0 * QjI) * Qir + 4.0 * QiQjr + 4.0 * QjQir;
-        o.gG[4] = (2.0 * QjR) * Qir + (2.0 * QiR) * Qjr;
+        o.gG[3] = (-QiR) * mj - (R(2) * mjr) * Qir + QjR * mi + (R(2) * mir) * Qjr + (R(2) * QiI) * Qjr +
+                  (R(2) * QjI) * Qir + R(4) * QiQjr + R(4) * QjQir;
+        o.gG[4] = (R(2) * QjR) * Qir + (R(2) * QiR) * Qjr;
     }
     if constexpr (TORQUE) {
-        const V3 rXmi = cross(r, mi);
-        const V3 mjXmi = cross(mj, mi);
-        const V3 QjrXmi = cross(Qjr, mi);
-        const V3 RXQi = mcross_outer(r, r, I.Q);          // cross(r r^T, Qi)
-        const V3 mjrXQi = mcross_outer(mj, r, I.Q);       // cross(mj r^T, Qi)
-        const V3 rmjXQi = mcross_outer(r, mj, I.Q);       // cross(r mj^T, Qi)
-        const V3 QjXQi = mcross_sym(J.Q, I.Q);
-        const V3 rQjrXQi = mcross_outer(r, Qjr, I.Q);     // cross(r (Qj r)^T, Qi)
-        const V3 QjrrXQi = mcross_outer(Qjr, r, I.Q);     // cross((Qj r) r^T, Qi)
+        const V3T<R> rXmi = cross(r, mi);
+        const V3T<R> mjXmi = cross(mj, mi);
+        const V3T<R> QjrXmi = cross(Qjr, mi);
+        const V3T<R> RXQi = mcross_outer(r, r, I.Q);          // cross(r r^T, Qi)
+        const V3T<R> mjrXQi = mcross_outer(mj, r, I.Q);       // cross(mj r^T, Qi)
+        const V3T<R> rmjXQi = mcross_outer(r, mj, I.Q);       // cross(r mj^T, Qi)
+        const V3T<R> QjXQi = mcross_sym(J.Q, I.Q);
+        const V3T<R> rQjrXQi = mcross_outer(r, Qjr, I.Q);     // cross(r (Qj r)^T, Qi)
+        const V3T<R> QjrrXQi = mcross_outer(Qjr, r, I.Q);     // cross((Qj r) r^T, Qi)
         o.Gt[1] = qj * rXmi - mjXmi;
-        o.Gt[2] = (-mjr) * rXmi + 2.0 * QjrXmi + QjI * rXmi + (2.0 * qj) * RXQi - 2.0 * mjrXQi - 2.0 * rmjXQi +
-                  4.0 * QjXQi;
-        o.Gt[3] = QjR * rXmi - (2.0 * mjr) * RXQi + (2.0 * QjI) * RXQi + 4.0 * rQjrXQi + 4.0 * QjrrXQi;
-        o.Gt[4] = (2.0 * QjR) * RXQi;
+        o.Gt[2] = (-mjr) * rXmi + R(2) * QjrXmi + QjI * rXmi + (R(2) * qj) * RXQi - R(2) * mjrXQi - R(2) * rmjXQi +
+                  R(4) * QjXQi;
+        o.Gt[3] = QjR * rXmi - (R(2) * mjr) * RXQi + (R(2) * QjI) * RXQi + R(4) * rQjrXQi + R(4) * QjrrXQi;
+        o.Gt[4] = (R(2) * QjR) * RXQi;
     }
 }
 
-// erfc-screened radial functions B_0..B_5 (:629-634)
-__device__ __forceinline__ void radial_b(const MpArgs& a, double r2, double (&B)[6]) {
-    const double r = sqrt(r2);
-    const double e = exp(-a.alpha2 * r2);
-    const double inv = 1.0 / r2;
-    B[0] = erfc(a.alpha * r) / r;
-    B[1] = (-B[0] - a.alpha2PI * e) * inv;
-    B[2] = (-3.0 * B[1] + 2.0 * a.alpha2PI * a.alpha2 * e) * inv;
-    B[3] = (-5.0 * B[2] - 4.0 * a.alpha2PI * a.alpha2 * a.alpha2 * e) * inv;
-    B[4] = (-7.0 * B[3] + 8.0 * a.alpha2PI * a.alpha2 * a.alpha2 * a.alpha2 * e) * inv;
-    B[5] = (-9.0 * B[4] - 16.0 * a.alpha2PI * a.alpha2 * a.alpha2 * a.alpha2 * a.alpha2 * e) * inv;
+// erfc-screened radial functions B_0..B_5 (:629-634).  FP32: exp2 and the
+// fitted erfc that shares exp(-alpha^2 r^2), a reciprocal of r^2.
+template <typename R>
+__device__ __forceinline__ void radial_b(const MpArgs& a, R r2, R (&B)[6]) {
+    if constexpr (sizeof(R) == sizeof(double)) {
+        const double r = sqrt(r2);
+        const double e = exp(-a.alpha2 * r2);
+        const double inv = 1.0 / r2;
+        B[0] = erfc(a.alpha * r) / r;
+        B[1] = (-B[0] - a.alpha2PI * e) * inv;
+        B[2] = (-3.0 * B[1] + 2.0 * a.alpha2PI * a.alpha2 * e) * inv;
+        B[3] = (-5.0 * B[2] - 4.0 * a.alpha2PI * a.alpha2 * a.alpha2 * e) * inv;
+        B[4] = (-7.0 * B[3] + 8.0 * a.alpha2PI * a.alpha2 * a.alpha2 * a.alpha2 * e) * inv;
+        B[5] = (-9.0 * B[4] - 16.0 * a.alpha2PI * a.alpha2 * a.alpha2 * a.alpha2 * a.alpha2 * e) * inv;
+    } else {
+        const float r_1 = __builtin_amdgcn_rsqf(r2), r = r2 * r_1;
+        const float al = (float)a.alpha, al2 = (float)a.alpha2;
+        float e, b0;
+        ewald_erfc_f32(al, al2, r2, r, r_1, b0, e);
+        const float inv = r_1 * r_1;
+        const float c1 = (float)a.alpha2PI, c2 = (float)(2.0 * a.alpha2PI * a.alpha2),
+                    c3 = (float)(4.0 * a.alpha2PI * a.alpha2 * a.alpha2),
+                    c4 = (float)(8.0 * a.alpha2PI * a.alpha2 * a.alpha2 * a.alpha2),
+                    c5 = (float)(16.0 * a.alpha2PI * a.alpha2 * a.alpha2 * a.alpha2 * a.alpha2);
+        B[0] = b0;
+        B[1] = (-B[0] - c1 * e) * inv;
+        B[2] = (-3.0f * B[1] + c2 * e) * inv;
+        B[3] = (-5.0f * B[2] - c3 * e) * inv;
+        B[4] = (-7.0f * B[3] + c4 * e) * inv;
+        B[5] = (-9.0f * B[4] - c5 * e) * inv;
+    }
+}
+
+// sin and cos of a k-space phase k.r: FP64, or (mixed form) the phase
+// formed and reduced to [-pi, pi] in FP64 and the hardware FP32 sin/cos.
+template <typename R> __device__ __forceinline__ void phase_sincos(double ph, R& sn, R& cs) {
+    if constexpr (sizeof(R) == sizeof(double)) {
+        sincos(ph, &sn, &cs);
+    } else {
+        constexpr double TWO_PI = 6.283185307179586476925, INV_TWO_PI = 0.159154943091895335769;
+        const float t = (float)(ph - TWO_PI * rint(ph * INV_TWO_PI));
+        sn = __sinf(t);
+        cs = __cosf(t);
+    }
 }
 
 // ---- real space ---------------------------------------------------------------
 
-template <bool FULL>
-__device__ __forceinline__ void real_entry(const MpArgs& a, const Site& I, V3 ri, int j, V3& f, V3& t,
-                                           double (&acc)[11]) {
-    double dx = ri.x - a.pos[3 * j], dy = ri.y - a.pos[3 * j + 1], dz = ri.z - a.pos[3 * j + 2];
-    pbc(dx, dy, dz, a.box);
-    const double r2 = dx * dx + dy * dy + dz * dz;
-    if (!(r2 < a.realcut2)) return;
-    const Site J = load_site(a, j);
-    const V3 r = v3(dx, dy, dz);
-    PairOut o;
-    double B[6];
-    radial_b(a, r2, B);
-    if (a.torques) mp_pair<true, true>(I, J, r, o);
-    else mp_pair<true, false>(I, J, r, o);
-    const double radial = o.G[0] * B[1] + o.G[1] * B[2] + o.G[2] * B[3] + o.G[3] * B[4] + o.G[4] * B[5];
-    const V3 fij = -(B[1] * o.gG[1] + B[2] * o.gG[2] + B[3] * o.gG[3] + B[4] * o.gG[4]) - radial * r;
-    f = f + fij;
-    if (a.torques) t = t + (B[1] * o.Gt[1] + B[2] * o.Gt[2] + B[3] * o.Gt[3] + B[4] * o.Gt[4]);
-    if constexpr (FULL) {
-        acc[0] += (o.G[0] * B[0] + o.G[1] * B[1] + o.G[2] * B[2] + o.G[3] * B[3] + o.G[4] * B[4]);
-        acc[1] += -dot(fij, r);
-        acc[2] += fij.x * r.x; acc[3] += fij.y * r.x; acc[4] += fij.z * r.x;
-        acc[5] += fij.x * r.y; acc[6] += fij.y * r.y; acc[7] += fij.z * r.y;
-        acc[8] += fij.x * r.z; acc[9] += fij.y * r.z; acc[10] += fij.z * r.z;
+// Site j in the pair math's precision: FP64 from the lab-frame arrays, FP32
+// from the packed records (q, mu | Qxx Qxy Qxz Qyy | Qyz Qzz alpha -).
+template <typename R> __device__ __forceinline__ SiteT<R> site_r(const MpArgs& a, int j) {
+    if constexpr (sizeof(R) == sizeof(double)) {
+        return load_site(a, j);
+    } else {
+        const float4 s0 = a.site32[3 * j], s1 = a.site32[3 * j + 1], s2 = a.site32[3 * j + 2];
+        return {s0.x, {s0.y, s0.z, s0.w}, {s1.x, s1.y, s1.z, s1.w, s2.x, s2.y}};
     }
 }
 
-template <int G>
+template <bool FULL, typename R>
+__device__ __forceinline__ void real_entry(const MpArgs& a, const SiteT<R>& I, V3 ri, int j, V3& f, V3& t,
+                                           double (&acc)[11]) {
+    double dx = ri.x - a.pos[3 * j], dy = ri.y - a.pos[3 * j + 1], dz = ri.z - a.pos[3 * j + 2];
+    pbc(dx, dy, dz, a.box);
+    const double r2d = dx * dx + dy * dy + dz * dz;
+    if (!(r2d < a.realcut2)) return;
+    const SiteT<R> J = site_r<R>(a, j);
+    const V3T<R> r = vec3<R>((R)dx, (R)dy, (R)dz);
+    PairOutT<R> o;
+    R B[6];
+    radial_b<R>(a, (R)r2d, B);
+    if (a.torques) mp_pair<true, true>(I, J, r, o);
+    else mp_pair<true, false>(I, J, r, o);
+    const R radial = o.G[0] * B[1] + o.G[1] * B[2] + o.G[2] * B[3] + o.G[3] * B[4] + o.G[4] * B[5];
+    const V3T<R> fij = -(B[1] * o.gG[1] + B[2] * o.gG[2] + B[3] * o.gG[3] + B[4] * o.gG[4]) - radial * r;
+    f = f + to64(fij);
+    if (a.torques) t = t + to64(B[1] * o.Gt[1] + B[2] * o.Gt[2] + B[3] * o.Gt[3] + B[4] * o.Gt[4]);
+    if constexpr (FULL) {
+        acc[0] += (double)(o.G[0] * B[0] + o.G[1] * B[1] + o.G[2] * B[2] + o.G[3] * B[3] + o.G[4] * B[4]);
+        const V3 fd = to64(fij), rd = to64(r);
+        acc[1] += -dot(fd, rd);
+        acc[2] += fd.x * rd.x; acc[3] += fd.y * rd.x; acc[4] += fd.z * rd.x;
+        acc[5] += fd.x * rd.y; acc[6] += fd.y * rd.y; acc[7] += fd.z * rd.y;
+        acc[8] += fd.x * rd.z; acc[9] += fd.y * rd.z; acc[10] += fd.z * rd.z;
+    }
+}
+
+template <int G, typename R>
 __global__ __launch_bounds__(256) void k_mp_real(MpArgs a) {
     const int tid = blockIdx.x * blockDim.x + threadIdx.x;
     const int i = tid / G, sub = tid % G;
@@ -240,11 +310,11 @@ __global__ __launch_bounds__(256) void k_mp_real(MpArgs a) {
         const int32_t* row = a.nb + (int64_t)i * a.stride;
         const int own = min(a.nb_own[i], a.stride);
         const int non = min(a.nb_cnt[i] - a.nb_own[i], a.stride - own);
-        const Site I = load_site(a, i);
+        const SiteT<R> I = site_r<R>(a, i);
         const V3 ri = ldv(a.pos + 3 * i);
-        for (int k = sub; k < own; k += G) real_entry<true>(a, I, ri, row[k], f, t, acc);
+        for (int k = sub; k < own; k += G) real_entry<true, R>(a, I, ri, row[k], f, t, acc);
         const int32_t* back = row + a.stride - 1;
-        for (int k = sub; k < non; k += G) real_entry<false>(a, I, ri, back[-k], f, t, acc);
+        for (int k = sub; k < non; k += G) real_entry<false, R>(a, I, ri, back[-k], f, t, acc);
     }
 #pragma unroll
     for (int o = G / 2; o > 0; o >>= 1) {
@@ -263,6 +333,42 @@ __global__ __launch_bounds__(256) void k_mp_real(MpArgs a) {
     acc[0] *= COULOMBS;
     acc[1] *= COULOMBS;
     block_add<11>(acc, a.terms + M_REAL_E);
+}
+
+template <int G, typename R> __global__ __launch_bounds__(256) void k_ind_real(MpArgs a);
+__global__ __launch_bounds__(256) void k_pack_sites32(MpArgs a, float4* __restrict__ site32,
+                                                      float4* __restrict__ pind32, int with_pind);
+
+// Real-space kernels in the context's precision for these classes (FP64, or
+// FP32 pair math from packed records with FP64 accumulation); the mixed form
+// first packs the records (and, for induction, the induced dipoles).
+static void launch_real(Ctx* c, const MpArgs& a, bool induction) {
+    const dim3 gr((unsigned)(((int64_t)c->n * MP_LANES + 255) / 256)), b256(256);
+    if (c->mp_prec == SMD_PREC_MIXED) {
+        hipLaunchKernelGGL(k_pack_sites32, dim3((c->n + 255) / 256), b256, 0, c->stream, a, c->d_site32, c->d_pind32,
+                           induction ? 1 : 0);
+        if (induction) hipLaunchKernelGGL((k_ind_real<MP_LANES, float>), gr, b256, 0, c->stream, a);
+        else hipLaunchKernelGGL((k_mp_real<MP_LANES, float>), gr, b256, 0, c->stream, a);
+    } else {
+        if (induction) hipLaunchKernelGGL((k_ind_real<MP_LANES, double>), gr, b256, 0, c->stream, a);
+        else hipLaunchKernelGGL((k_mp_real<MP_LANES, double>), gr, b256, 0, c->stream, a);
+    }
+}
+
+// FP32 records of each site's lab-frame multipoles and polarizability for the
+// mixed form's real-space partner gathers (3 x 16 bytes per site), and of the
+// induced dipoles (16 bytes per site).
+__global__ __launch_bounds__(256) void k_pack_sites32(MpArgs a, float4* __restrict__ site32,
+                                                      float4* __restrict__ pind32, int with_pind) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= a.n) return;
+    const double* d = a.dip + 3 * i;
+    const double* Q = a.quad + 6 * i;
+    site32[3 * i] = make_float4((float)(a.q[i] / SQRT_COULOMB_MP), (float)d[0], (float)d[1], (float)d[2]);
+    site32[3 * i + 1] = make_float4((float)Q[0], (float)Q[1], (float)Q[2], (float)Q[3]);
+    site32[3 * i + 2] = make_float4((float)Q[4], (float)Q[5], (float)a.polar[i], 0.0f);
+    if (with_pind)
+        pind32[i] = make_float4((float)a.pind[3 * i], (float)a.pind[3 * i + 1], (float)a.pind[3 * i + 2], 0.0f);
 }
 
 // ---- self, intramolecular, surface --------------------------------------------
@@ -286,13 +392,13 @@ __global__ __launch_bounds__(256) void k_mp_site_sums(MpArgs a) {
         v[7] = I.q * r.x + I.d.x; v[8] = I.q * r.y + I.d.y; v[9] = I.q * r.z + I.d.z;   // q r + mu
         v[10] = 2.0 * I.q * QiI - dot(I.d, I.d);
     }
-    double self[1] = {v[0]};
-    block_add<1>(self, a.terms + M_SELF);
-    __syncthreads();
-    double su[10];
-#pragma unroll
-    for (int k = 0; k < 10; ++k) su[k] = v[k + 1];
-    block_add<10>(su, a.terms + M_SU_QR);
+    // the surface sums feed the forces: block partials summed by the last
+    // block in a fixed order
+    block_store<11>(v, a.spart, gridDim.x);
+    if (last_block(a.sticket)) {
+        fixed_totals(a.spart, gridDim.x, gridDim.x, 1, a.terms + M_SELF);
+        fixed_totals(a.spart + gridDim.x, gridDim.x, gridDim.x, 10, a.terms + M_SU_QR);
+    }
 }
 
 // One thread per molecule: excluded pairs inside it with erf-screened D_l
@@ -466,60 +572,76 @@ __global__ __launch_bounds__(256) void k_mp_torques(MpArgs a, unsigned parts) {
 //   6..8 sum (q - kQk) cos d     9..11 sum (q - kQk) sin d      (d = r - R_mol)
 //  12..14 sum (mu.k) cos d      15..17 sum (mu.k) sin d
 //  18 sum (q - kQk) cos |d|^2   19 sum (q - kQk) sin |d|^2
+template <typename R>
 __global__ __launch_bounds__(MP_SF_THREADS) void k_mp_sfac(MpArgs a) {
-    __shared__ double sh[MP_SF_THREADS][14];   // r(3) q mu(3) Q(6) | d staged separately
-    __shared__ double shd[MP_SF_THREADS][3];
+    // per atom of the tile: r in FP64 (the phase); q, mu, Q, d = r - R_mol in R
+    __shared__ double shr[MP_SF_THREADS][3];
+    __shared__ R shm[MP_SF_THREADS][13];
     const int k = blockIdx.x * MP_SF_THREADS + threadIdx.x;
     const int chunk = blockIdx.y;
     const int i0 = chunk * a.per, i1 = min(a.n, i0 + a.per);
     V3 kv = v3(0, 0, 0);
     if (k < a.K) kv = ldv(a.kvec + 3 * k);
-    double s[MP_NSUM];
+    const R kx = (R)kv.x, ky = (R)kv.y, kz = (R)kv.z;
+    double S[MP_NSUM];
 #pragma unroll
-    for (int x = 0; x < MP_NSUM; ++x) s[x] = 0.0;
+    for (int x = 0; x < MP_NSUM; ++x) S[x] = 0.0;
     for (int t0 = i0; t0 < i1; t0 += MP_SF_THREADS) {
         const int i = t0 + threadIdx.x;
         __syncthreads();
         if (i < i1) {
-            double* p = sh[threadIdx.x];
-            p[0] = a.pos[3 * i]; p[1] = a.pos[3 * i + 1]; p[2] = a.pos[3 * i + 2];
-            p[3] = a.q[i] / SQRT_COULOMB_MP;
-            p[4] = a.dip[3 * i]; p[5] = a.dip[3 * i + 1]; p[6] = a.dip[3 * i + 2];
-            for (int c = 0; c < 6; ++c) p[7 + c] = a.quad[6 * i + c];
+            shr[threadIdx.x][0] = a.pos[3 * i]; shr[threadIdx.x][1] = a.pos[3 * i + 1]; shr[threadIdx.x][2] = a.pos[3 * i + 2];
+            R* p = shm[threadIdx.x];
+            p[0] = (R)(a.q[i] / SQRT_COULOMB_MP);
+            p[1] = (R)a.dip[3 * i]; p[2] = (R)a.dip[3 * i + 1]; p[3] = (R)a.dip[3 * i + 2];
+            for (int c = 0; c < 6; ++c) p[4 + c] = (R)a.quad[6 * i + c];
             const int m = a.mol[i];
-            shd[threadIdx.x][0] = p[0] - a.com[3 * m];
-            shd[threadIdx.x][1] = p[1] - a.com[3 * m + 1];
-            shd[threadIdx.x][2] = p[2] - a.com[3 * m + 2];
+            p[10] = (R)(a.pos[3 * i] - a.com[3 * m]);
+            p[11] = (R)(a.pos[3 * i + 1] - a.com[3 * m + 1]);
+            p[12] = (R)(a.pos[3 * i + 2] - a.com[3 * m + 2]);
         }
         __syncthreads();
         const int cnt = min(MP_SF_THREADS, i1 - t0);
         if (k < a.K) {
+            // FP64: the sums accumulate directly; FP32: the tile's sums, added
+            // to the FP64 totals after the tile
+            auto tile = [&](auto& s) {
             for (int x = 0; x < cnt; ++x) {
-                const double* p = sh[x];
-                const double ph = p[0] * kv.x + p[1] * kv.y + p[2] * kv.z;
-                double sn, cs;
-                sincos(ph, &sn, &cs);
-                const double QK = p[7] * kv.x * kv.x + p[10] * kv.y * kv.y + p[12] * kv.z * kv.z +
-                                  2.0 * (p[8] * kv.x * kv.y + p[9] * kv.x * kv.z + p[11] * kv.y * kv.z);
-                const double A = p[3] - QK;
-                const double Mu = p[4] * kv.x + p[5] * kv.y + p[6] * kv.z;
-                const double ac = A * cs, as = A * sn, mc = Mu * cs, ms = Mu * sn;
-                const double dx = shd[x][0], dy = shd[x][1], dz = shd[x][2];
-                const double d2 = dx * dx + dy * dy + dz * dz;
+                const R* p = shm[x];
+                R sn, cs;
+                phase_sincos<R>(shr[x][0] * kv.x + shr[x][1] * kv.y + shr[x][2] * kv.z, sn, cs);
+                const R QK = p[4] * kx * kx + p[7] * ky * ky + p[9] * kz * kz +
+                             R(2) * (p[5] * kx * ky + p[6] * kx * kz + p[8] * ky * kz);
+                const R A = p[0] - QK;
+                const R Mu = p[1] * kx + p[2] * ky + p[3] * kz;
+                const R ac = A * cs, as = A * sn, mc = Mu * cs, ms = Mu * sn;
+                const R dx = p[10], dy = p[11], dz = p[12];
+                const R d2 = dx * dx + dy * dy + dz * dz;
                 s[0] += ac; s[1] += as; s[2] += mc; s[3] += ms;
-                s[4] += 2.0 * QK * cs; s[5] += 2.0 * QK * sn;
+                s[4] += R(2) * QK * cs; s[5] += R(2) * QK * sn;
                 s[6] += ac * dx; s[7] += ac * dy; s[8] += ac * dz;
                 s[9] += as * dx; s[10] += as * dy; s[11] += as * dz;
                 s[12] += mc * dx; s[13] += mc * dy; s[14] += mc * dz;
                 s[15] += ms * dx; s[16] += ms * dy; s[17] += ms * dz;
                 s[18] += ac * d2; s[19] += as * d2;
             }
+            };
+            if constexpr (sizeof(R) == sizeof(double)) {
+                tile(S);
+            } else {
+                R s[MP_NSUM];
+#pragma unroll
+                for (int x = 0; x < MP_NSUM; ++x) s[x] = R(0);
+                tile(s);
+#pragma unroll
+                for (int x = 0; x < MP_NSUM; ++x) S[x] += (double)s[x];
+            }
         }
     }
     if (k < a.K) {
         double* out = a.part + ((size_t)chunk * a.K + k) * MP_NSUM;
 #pragma unroll
-        for (int x = 0; x < MP_NSUM; ++x) out[x] = s[x];
+        for (int x = 0; x < MP_NSUM; ++x) out[x] = S[x];
     }
 }
 
@@ -599,16 +721,18 @@ __global__ __launch_bounds__(256) void k_mp_kfinish(MpArgs a) {
 // Pass 3: force val k and the torque of the dipole and the quadrupole
 // (:1149-1210).  Block (64 sites, one k-range); each lane sums its site over
 // the range and writes a partial; k_kforce_sum adds the ranges in order.
+template <typename R>
 __global__ __launch_bounds__(WAVE) void k_mp_kforce(MpArgs a) {
     __shared__ double sk[MP_KTILE][5];
     const int i = blockIdx.x * WAVE + threadIdx.x;
     const int kb = blockIdx.y * a.kper, ke = min(a.K, kb + a.kper);
-    Site I{};
+    Site I64{};
     V3 ri = v3(0, 0, 0);
     if (i < a.n) {
-        I = load_site(a, i);
+        I64 = load_site(a, i);
         ri = ldv(a.pos + 3 * i);
     }
+    const SiteT<R> I = site_as<R>(I64);
     V3 F = v3(0, 0, 0), T = v3(0, 0, 0);
     for (int k0 = kb; k0 < ke; k0 += MP_KTILE) {
         const int cnt = min(MP_KTILE, ke - k0);
@@ -622,21 +746,25 @@ __global__ __launch_bounds__(WAVE) void k_mp_kforce(MpArgs a) {
         }
         __syncthreads();
         if (i < a.n) {
+            // the tile's force and torque in R, added to FP64 after the tile
+            V3T<R> f = vec3<R>(R(0), R(0), R(0)), t = f;
             for (int x = 0; x < cnt; ++x) {
-                const V3 kv = v3(sk[x][0], sk[x][1], sk[x][2]);
-                const double bP = sk[x][3], bQ = sk[x][4];
-                double sn, cs;
-                sincos(dot(ri, kv), &sn, &cs);
-                const V3 Qk = smul(I.Q, kv);
-                const double A = I.q - dot(kv, Qk);
-                const double Mu = dot(I.d, kv);
-                const double val = sn * (A * bP + Mu * bQ) + cs * (Mu * bP - A * bQ);
-                F = F + val * kv;
+                R sn, cs;
+                phase_sincos<R>(ri.x * sk[x][0] + ri.y * sk[x][1] + ri.z * sk[x][2], sn, cs);
+                const V3T<R> kv = vec3<R>((R)sk[x][0], (R)sk[x][1], (R)sk[x][2]);
+                const R bP = (R)sk[x][3], bQ = (R)sk[x][4];
+                const V3T<R> Qk = smul(I.Q, kv);
+                const R A = I.q - dot(kv, Qk);
+                const R Mu = dot(I.d, kv);
+                const R val = sn * (A * bP + Mu * bQ) + cs * (Mu * bP - A * bQ);
+                f = f + val * kv;
                 if (a.torques) {
                     // (kk^T x Qi) = k x (Qi k); tau = (c bQ - s bP) k x mu - 2 (c bP + s bQ) k x Qk
-                    T = T + (cs * bQ - sn * bP) * cross(kv, I.d) - (2.0 * (cs * bP + sn * bQ)) * cross(kv, Qk);
+                    t = t + (cs * bQ - sn * bP) * cross(kv, I.d) - (R(2) * (cs * bP + sn * bQ)) * cross(kv, Qk);
                 }
             }
+            F = F + to64(f);
+            T = T + to64(t);
         }
     }
     if (i < a.n) {
@@ -669,6 +797,12 @@ __global__ __launch_bounds__(256) void k_kforce_sum(MpArgs a) {
 // ---- host side ---------------------------------------------------------------------
 
 int frames_run(Ctx* c);
+
+static_assert(M_COUNT <= POLAR_TERMS, "polar accumulator too small");
+
+__global__ __launch_bounds__(256) void k_zero_terms(double* t) {
+    for (int k = threadIdx.x; k < M_COUNT; k += blockDim.x) t[k] = 0.0;
+}
 
 static MpArgs mp_args(Ctx* c, const Ctx::KSet& ks, double alpha, double real_cut, int use_constraint,
                       int torques) {
@@ -713,6 +847,10 @@ static MpArgs mp_args(Ctx* c, const Ctx::KSet& ks, double alpha, double real_cut
     a.torques = torques;
     a.pind = c->d_mu;
     a.polar = c->d_polar;
+    a.spart = c->part_su;
+    a.sticket = c->d_flag + 5;
+    a.site32 = c->d_site32;
+    a.pind32 = c->d_pind32;
     return a;
 }
 
@@ -747,7 +885,7 @@ int multipole_short(Ctx* c, smd_mp_out* out) {
     SMD_HIP(c, hipMemsetAsync(c->d_mp_terms, 0, sizeof(double) * M_COUNT, c->stream));
     if ((rc = frames_run(c))) return rc;
     hipLaunchKernelGGL(k_mp_site_sums, ga, b256, 0, c->stream, a);
-    hipLaunchKernelGGL((k_mp_real<MP_LANES>), gr, b256, 0, c->stream, a);
+    launch_real(c, a, false);
     hipLaunchKernelGGL(k_mp_intra, gm, b256, 0, c->stream, a);
     const double volume = a.volume, volumer = 1.0 / volume;
     const double coeff = 2.0 * PI * volumer / 3.0;
@@ -798,9 +936,11 @@ int multipole_long(Ctx* c, smd_mp_long_out* out) {
     SMD_HIP(c, hipMemsetAsync(c->d_mp_terms, 0, sizeof(double) * M_COUNT, c->stream));
     if ((rc = frames_run(c))) return rc;
     const dim3 g1((a.K + MP_SF_THREADS - 1) / MP_SF_THREADS, a.chunks);
-    hipLaunchKernelGGL(k_mp_sfac, g1, dim3(MP_SF_THREADS), 0, c->stream, a);
+    if (c->mp_prec == SMD_PREC_MIXED) hipLaunchKernelGGL((k_mp_sfac<float>), g1, dim3(MP_SF_THREADS), 0, c->stream, a);
+        else hipLaunchKernelGGL((k_mp_sfac<double>), g1, dim3(MP_SF_THREADS), 0, c->stream, a);
     hipLaunchKernelGGL(k_mp_kfinish, dim3((a.K + 3) / 4), b256, 0, c->stream, a);
-    hipLaunchKernelGGL(k_mp_kforce, dim3((c->n + WAVE - 1) / WAVE, a.ksplit), dim3(WAVE), 0, c->stream, a);
+    if (c->mp_prec == SMD_PREC_MIXED) hipLaunchKernelGGL((k_mp_kforce<float>), dim3((c->n + WAVE - 1) / WAVE, a.ksplit), dim3(WAVE), 0, c->stream, a);
+        else hipLaunchKernelGGL((k_mp_kforce<double>), dim3((c->n + WAVE - 1) / WAVE, a.ksplit), dim3(WAVE), 0, c->stream, a);
     hipLaunchKernelGGL(k_kforce_sum, dim3((c->n + 255) / 256), b256, 0, c->stream, a);
     if (c->mp.torques) hipLaunchKernelGGL(k_mp_torques, dim3((c->nmol + 255) / 256), b256, 0, c->stream, a, 8u);
     SMD_HIP(c, hipGetLastError());
@@ -897,136 +1037,182 @@ constexpr int IN_NSUM = 28;   // structure-factor sums per k of the induction pa
 // Site I's permanent multipoles with partner J's induced dipole pj at
 // r = r_I - r_J (the "i" side of real_term :940-1028 and the torque vectors
 // :1276-1281).  The reference's "j" side is this with the roles swapped.
-struct IndSide { double G1, G2, G3; V3 g1, g2, g3; V3 Gt1, Gt2, Gt3; };
+template <typename R> struct IndSideT { R G1, G2, G3; V3T<R> g1, g2, g3; V3T<R> Gt1, Gt2, Gt3; };
+using IndSide = IndSideT<double>;
 
-template <bool GRAD, bool TORQUE>
-__device__ __forceinline__ void ind_side(const Site& I, V3 pj, V3 r, IndSide& t) {
-    const double QiI = trace(I.Q);
-    const V3 Qir = smul(I.Q, r);
-    const double QiR = dot(r, Qir);
-    const double pjr = dot(pj, r), mir = dot(I.d, r);
+template <bool GRAD, bool TORQUE, typename R>
+__device__ __forceinline__ void ind_side(const SiteT<R>& I, V3T<R> pj, V3T<R> r, IndSideT<R>& t) {
+    const R QiI = trace(I.Q);
+    const V3T<R> Qir = smul(I.Q, r);
+    const R QiR = dot(r, Qir);
+    const R pjr = dot(pj, r), mir = dot(I.d, r);
     t.G1 = -I.q * pjr - dot(I.d, pj);
-    t.G2 = -mir * pjr - 2.0 * dot(pj, Qir) - pjr * QiI;
+    t.G2 = -mir * pjr - R(2) * dot(pj, Qir) - pjr * QiI;
     t.G3 = -pjr * QiR;
     if constexpr (GRAD) {
         t.g1 = (-I.q) * pj;
-        t.g2 = (-pjr) * I.d - mir * pj - 2.0 * smul(I.Q, pj) - QiI * pj;
-        t.g3 = (-QiR) * pj - (2.0 * pjr) * Qir;
+        t.g2 = (-pjr) * I.d - mir * pj - R(2) * smul(I.Q, pj) - QiI * pj;
+        t.g3 = (-QiR) * pj - (R(2) * pjr) * Qir;
     }
     if constexpr (TORQUE) {
         // cross(Q, a b^T, ones) = cross(Q b, a)
         t.Gt1 = cross(I.d, pj);
-        t.Gt2 = pjr * cross(I.d, r) + 2.0 * cross(Qir, pj) + 2.0 * cross(smul(I.Q, pj), r);
-        t.Gt3 = (2.0 * pjr) * cross(Qir, r);
+        t.Gt2 = pjr * cross(I.d, r) + R(2) * cross(Qir, pj) + R(2) * cross(smul(I.Q, pj), r);
+        t.Gt3 = (R(2) * pjr) * cross(Qir, r);
     }
 }
 
 // Damped radial functions of one pair (:1010-1066).
-struct IndRadial { double B1d, B2d, B3d, B4d, gB1, gB2, gB3, B2du, gB1u, gB2u; };
+template <typename R> struct IndRadialT { R B1d, B2d, B3d, B4d, gB1, gB2, gB3, B2du, gB1u, gB2u; };
+using IndRadial = IndRadialT<double>;
 
-__device__ __forceinline__ void ind_radial(const MpArgs& a, double r2, double pi_, double pj_, double scale,
-                                           IndRadial& o) {
-    double B[6];
-    radial_b(a, r2, B);
-    const double r = sqrt(r2);
-    const double u3 = r * r * r / sqrt(pi_ * pj_);
-    const double expau3 = exp(-a.a_damp * u3), au3 = a.a_damp * u3, a2u6 = au3 * au3, a3u9 = a2u6 * au3;
-    const double l3 = 1.0 - expau3;
-    const double l5 = 1.0 - (1.0 + au3) * expau3;
-    const double l7 = 1.0 - (1.0 + au3 + 3.0 * a2u6 / 5.0) * expau3;
-    const double gl3 = 3.0 * au3 * expau3 / r2;
-    const double gl5 = 3.0 * a2u6 * expau3 / r2;
-    const double gl7 = (-3.0 * a2u6 + 9.0 * a3u9) * expau3 / (5.0 * r2);
+template <typename R>
+__device__ __forceinline__ void ind_radial(const MpArgs& a, R r2, R pi_, R pj_, R scale, IndRadialT<R>& o) {
+    R B[6];
+    radial_b<R>(a, r2, B);
+    R r, u3, expau3, inv_r2;
+    const R ad = (R)a.a_damp;
+    if constexpr (sizeof(R) == sizeof(double)) {
+        r = sqrt(r2);
+        u3 = r * r * r / sqrt(pi_ * pj_);
+        expau3 = exp(-a.a_damp * u3);
+        inv_r2 = R(1);   // unused: the FP64 form divides as the reference does
+    } else {
+        const float r_1 = __builtin_amdgcn_rsqf(r2);
+        r = r2 * r_1;
+        u3 = r * r * r * __builtin_amdgcn_rsqf(pi_ * pj_);
+        expau3 = __builtin_amdgcn_exp2f(-ad * u3 * LOG2E_F);
+        inv_r2 = r_1 * r_1;
+    }
+    const R au3 = ad * u3, a2u6 = au3 * au3, a3u9 = a2u6 * au3;
+    const R l3 = R(1) - expau3;
+    const R l5 = R(1) - (R(1) + au3) * expau3;
+    const R l7 = R(1) - (R(1) + au3 + R(3) * a2u6 / R(5)) * expau3;
+    R gl3, gl5, gl7;
+    if constexpr (sizeof(R) == sizeof(double)) {
+        gl3 = 3.0 * au3 * expau3 / r2;
+        gl5 = 3.0 * a2u6 * expau3 / r2;
+        gl7 = (-3.0 * a2u6 + 9.0 * a3u9) * expau3 / (5.0 * r2);
+    } else {
+        gl3 = 3.0f * au3 * expau3 * inv_r2;
+        gl5 = 3.0f * a2u6 * expau3 * inv_r2;
+        gl7 = (-3.0f * a2u6 + 9.0f * a3u9) * expau3 * (0.2f * inv_r2);
+    }
     if (a.damp_amoeba) {
-        const double invr = 1.0 / r, invr2 = invr * invr, invr3 = invr2 * invr;
-        const double invr5 = 3.0 * invr3 * invr2, invr7 = 5.0 * invr5 * invr2, invr9 = 7.0 * invr7 * invr2;
-        o.B1d = B[1] + (1.0 - scale * l3) * invr3;
-        o.B2d = B[2] - (1.0 - scale * l5) * invr5;
-        o.B3d = B[3] + (1.0 - scale * l7) * invr7;
-        o.B4d = 0.0;
-        o.gB1 = B[2] - scale * gl3 * invr3 - (1.0 - scale * l3) * invr5;
-        o.gB2 = B[3] + scale * gl5 * invr5 + (1.0 - scale * l5) * invr7;
-        o.gB3 = B[4] - scale * gl7 * invr7 - (1.0 - scale * l7) * invr9;
-        o.B2du = B[2] - (1.0 - l5) * invr5;
-        o.gB1u = B[2] - gl3 * invr3 - (1.0 - l3) * invr5;
-        o.gB2u = B[3] + gl5 * invr5 + (1.0 - l5) * invr7;
+        R invr;
+        if constexpr (sizeof(R) == sizeof(double)) invr = 1.0 / r;
+        else invr = __builtin_amdgcn_rsqf(r2);
+        const R invr2 = invr * invr, invr3 = invr2 * invr;
+        const R invr5 = R(3) * invr3 * invr2, invr7 = R(5) * invr5 * invr2, invr9 = R(7) * invr7 * invr2;
+        o.B1d = B[1] + (R(1) - scale * l3) * invr3;
+        o.B2d = B[2] - (R(1) - scale * l5) * invr5;
+        o.B3d = B[3] + (R(1) - scale * l7) * invr7;
+        o.B4d = R(0);
+        o.gB1 = B[2] - scale * gl3 * invr3 - (R(1) - scale * l3) * invr5;
+        o.gB2 = B[3] + scale * gl5 * invr5 + (R(1) - scale * l5) * invr7;
+        o.gB3 = B[4] - scale * gl7 * invr7 - (R(1) - scale * l7) * invr9;
+        o.B2du = B[2] - (R(1) - l5) * invr5;
+        o.gB1u = B[2] - gl3 * invr3 - (R(1) - l3) * invr5;
+        o.gB2u = B[3] + gl5 * invr5 + (R(1) - l5) * invr7;
     } else {
         o.B1d = B[1] * l3;
         o.B2d = B[2] * l3 + B[1] * gl3;
-        o.B3d = B[3] * l3 + B[2] * 2.0 * gl3 + B[1] * (gl3 - 3.0 * gl5) / r2;
-        o.B4d = B[4] * l3 + B[3] * 3.0 * gl3 + B[2] * (3.0 * gl3 - 9.0 * gl5) / r2 +
-                B[1] * (-gl3 - 6.0 * gl5 + 15.0 * gl7) / (r2 * r2);
-        o.gB1 = o.gB2 = o.gB3 = o.B2du = o.gB1u = o.gB2u = 0.0;
+        if constexpr (sizeof(R) == sizeof(double)) {
+            o.B3d = B[3] * l3 + B[2] * 2.0 * gl3 + B[1] * (gl3 - 3.0 * gl5) / r2;
+            o.B4d = B[4] * l3 + B[3] * 3.0 * gl3 + B[2] * (3.0 * gl3 - 9.0 * gl5) / r2 +
+                    B[1] * (-gl3 - 6.0 * gl5 + 15.0 * gl7) / (r2 * r2);
+        } else {
+            o.B3d = B[3] * l3 + B[2] * 2.0f * gl3 + B[1] * (gl3 - 3.0f * gl5) * inv_r2;
+            o.B4d = B[4] * l3 + B[3] * 3.0f * gl3 + B[2] * (3.0f * gl3 - 9.0f * gl5) * inv_r2 +
+                    B[1] * (-gl3 - 6.0f * gl5 + 15.0f * gl7) * (inv_r2 * inv_r2);
+        }
+        o.gB1 = o.gB2 = o.gB3 = o.B2du = o.gB1u = o.gB2u = R(0);
     }
+}
+
+// Site j's induced dipole and polarizability in the pair math's precision.
+template <typename R> __device__ __forceinline__ V3T<R> pind_r(const MpArgs& a, int j) {
+    if constexpr (sizeof(R) == sizeof(double)) {
+        return ldv(a.pind + 3 * j);
+    } else {
+        const float4 p = a.pind32[j];
+        return {p.x, p.y, p.z};
+    }
+}
+template <typename R> __device__ __forceinline__ R polar_r(const MpArgs& a, int j) {
+    if constexpr (sizeof(R) == sizeof(double)) return a.polar[j];
+    else return a.site32[3 * j + 2].z;
 }
 
 // One pair seen from row atom i (partner j, r = r_i - r_j): force on i, torque
 // on i, and the pair's energy / virial / dU/dV contributions at half weight
 // (every pair is visited from both rows).
-__device__ __forceinline__ void ind_entry(const MpArgs& a, const Site& I, V3 pi, double poli, V3 ri, int j,
-                                          double scale, V3& f, V3& t, double (&acc)[11]) {
+template <typename R>
+__device__ __forceinline__ void ind_entry(const MpArgs& a, const SiteT<R>& I, V3T<R> pi, R poli, V3 ri, int j,
+                                          R scale, V3& f, V3& t, double (&acc)[11]) {
     double dx = ri.x - a.pos[3 * j], dy = ri.y - a.pos[3 * j + 1], dz = ri.z - a.pos[3 * j + 2];
     pbc(dx, dy, dz, a.box);
-    const double r2 = dx * dx + dy * dy + dz * dz;
-    if (!(r2 < a.realcut2)) return;
-    const Site J = load_site(a, j);
-    const V3 pj = ldv(a.pind + 3 * j);
-    const V3 r = v3(dx, dy, dz), rji = -r;
-    IndSide si, sj;
+    const double r2d = dx * dx + dy * dy + dz * dz;
+    if (!(r2d < a.realcut2)) return;
+    const SiteT<R> J = site_r<R>(a, j);
+    const V3T<R> pj = pind_r<R>(a, j);
+    const V3T<R> r = vec3<R>((R)dx, (R)dy, (R)dz), rji = -r;
+    IndSideT<R> si, sj;
     if (a.torques) ind_side<true, true>(I, pj, r, si);
     else ind_side<true, false>(I, pj, r, si);
     ind_side<true, false>(J, pi, rji, sj);
-    IndRadial R;
-    ind_radial(a, r2, poli, a.polar[j], scale, R);
-    const double Gsum = (si.G1 + sj.G1) * R.B1d + (si.G2 + sj.G2) * R.B2d + (si.G3 + sj.G3) * R.B3d;
-    const double uen = a.damp_amoeba ? 0.5 * Gsum * COULOMBS : 0.5 * Gsum * scale * COULOMBS;
-    const double G1ip = -dot(pi, pj);
-    const double G2ip = -dot(pi, r) * dot(pj, r);
-    const V3 g2ip = (-dot(pj, r)) * pi;
-    const V3 g2jp = (-dot(pi, rji)) * pj;
-    V3 fi, fj;
+    IndRadialT<R> Rd;
+    ind_radial<R>(a, (R)r2d, poli, polar_r<R>(a, j), scale, Rd);
+    const R Gsum = (si.G1 + sj.G1) * Rd.B1d + (si.G2 + sj.G2) * Rd.B2d + (si.G3 + sj.G3) * Rd.B3d;
+    const double uen = a.damp_amoeba ? 0.5 * (double)Gsum * COULOMBS : 0.5 * (double)Gsum * (double)scale * COULOMBS;
+    const R G1ip = -dot(pi, pj);
+    const R G2ip = -dot(pi, r) * dot(pj, r);
+    const V3T<R> g2ip = (-dot(pj, r)) * pi;
+    const V3T<R> g2jp = (-dot(pi, rji)) * pj;
+    V3T<R> fi, fj;
     if (a.damp_amoeba) {
-        V3 gi = R.B1d * si.g1 + R.B2d * si.g2 + R.B3d * si.g3;
-        V3 gj = R.B1d * sj.g1 + R.B2d * sj.g2 + R.B3d * sj.g3;
-        double ri_ = si.G1 * R.gB1 + si.G2 * R.gB2 + si.G3 * R.gB3;
-        const double rj_ = sj.G1 * R.gB1 + sj.G2 * R.gB2 + sj.G3 * R.gB3;
+        V3T<R> gi = Rd.B1d * si.g1 + Rd.B2d * si.g2 + Rd.B3d * si.g3;
+        V3T<R> gj = Rd.B1d * sj.g1 + Rd.B2d * sj.g2 + Rd.B3d * sj.g3;
+        R ri_ = si.G1 * Rd.gB1 + si.G2 * Rd.gB2 + si.G3 * Rd.gB3;
+        const R rj_ = sj.G1 * Rd.gB1 + sj.G2 * Rd.gB2 + sj.G3 * Rd.gB3;
         if (a.do_min) {
-            gi = gi + R.B2du * g2ip;
-            gj = gj + R.B2du * g2jp;
-            ri_ += G1ip * R.gB1u + G2ip * R.gB2u;
+            gi = gi + Rd.B2du * g2ip;
+            gj = gj + Rd.B2du * g2jp;
+            ri_ += G1ip * Rd.gB1u + G2ip * Rd.gB2u;
         }
         fi = -gi - ri_ * r;
         fj = -gj - rj_ * rji;
     } else {
-        V3 gi = scale * (R.B1d * si.g1 + R.B2d * si.g2 + R.B3d * si.g3);
-        V3 gj = scale * (R.B1d * sj.g1 + R.B2d * sj.g2 + R.B3d * sj.g3);
-        double ri_ = scale * (si.G1 * R.B2d + si.G2 * R.B3d + si.G3 * R.B4d);
-        const double rj_ = scale * (sj.G1 * R.B2d + sj.G2 * R.B3d + sj.G3 * R.B4d);
+        V3T<R> gi = scale * (Rd.B1d * si.g1 + Rd.B2d * si.g2 + Rd.B3d * si.g3);
+        V3T<R> gj = scale * (Rd.B1d * sj.g1 + Rd.B2d * sj.g2 + Rd.B3d * sj.g3);
+        R ri_ = scale * (si.G1 * Rd.B2d + si.G2 * Rd.B3d + si.G3 * Rd.B4d);
+        const R rj_ = scale * (sj.G1 * Rd.B2d + sj.G2 * Rd.B3d + sj.G3 * Rd.B4d);
         if (a.do_min) {
-            gi = gi + R.B2d * g2ip;
-            gj = gj + R.B2d * g2jp;
-            ri_ += G1ip * R.B2d + G2ip * R.B3d;
+            gi = gi + Rd.B2d * g2ip;
+            gj = gj + Rd.B2d * g2jp;
+            ri_ += G1ip * Rd.B2d + G2ip * Rd.B3d;
         }
         fi = -gi - ri_ * r;
         fj = -gj - rj_ * rji;
     }
-    const V3 fij = fi - fj;
+    const V3 fij = to64(fi - fj);
     f = f + fij;
     if (a.torques) {
-        const V3 ti = R.B1d * si.Gt1 + R.B2d * si.Gt2 + R.B3d * si.Gt3;
-        t = t + (a.damp_amoeba ? 1.0 : scale) * ti;
+        const V3T<R> ti = Rd.B1d * si.Gt1 + Rd.B2d * si.Gt2 + Rd.B3d * si.Gt3;
+        t = t + to64((a.damp_amoeba ? R(1) : scale) * ti);
     }
+    const V3 rd = to64(r);
     acc[0] += 0.5 * uen;
-    acc[1] += -0.5 * dot(fij, r);
-    acc[2] += 0.5 * fij.x * r.x; acc[3] += 0.5 * fij.y * r.x; acc[4] += 0.5 * fij.z * r.x;
-    acc[5] += 0.5 * fij.x * r.y; acc[6] += 0.5 * fij.y * r.y; acc[7] += 0.5 * fij.z * r.y;
-    acc[8] += 0.5 * fij.x * r.z; acc[9] += 0.5 * fij.y * r.z; acc[10] += 0.5 * fij.z * r.z;
+    acc[1] += -0.5 * dot(fij, rd);
+    acc[2] += 0.5 * fij.x * rd.x; acc[3] += 0.5 * fij.y * rd.x; acc[4] += 0.5 * fij.z * rd.x;
+    acc[5] += 0.5 * fij.x * rd.y; acc[6] += 0.5 * fij.y * rd.y; acc[7] += 0.5 * fij.z * rd.y;
+    acc[8] += 0.5 * fij.x * rd.z; acc[9] += 0.5 * fij.y * rd.z; acc[10] += 0.5 * fij.z * rd.z;
 }
 
 // Real space over every pair within cutoffEw: a row's pair-list partners
 // (both segments, not excluded) and its excluded partners from the exclusion
 // table (scale 0).
-template <int G>
+template <int G, typename R>
 __global__ __launch_bounds__(256) void k_ind_real(MpArgs a) {
     const int tid = blockIdx.x * blockDim.x + threadIdx.x;
     const int i = tid / G, sub = tid % G;
@@ -1038,15 +1224,15 @@ __global__ __launch_bounds__(256) void k_ind_real(MpArgs a) {
         const int32_t* row = a.nb + (int64_t)i * a.stride;
         const int own = min(a.nb_own[i], a.stride);
         const int non = min(a.nb_cnt[i] - a.nb_own[i], a.stride - own);
-        const Site I = load_site(a, i);
-        const V3 pi = ldv(a.pind + 3 * i);
-        const double poli = a.polar[i];
+        const SiteT<R> I = site_r<R>(a, i);
+        const V3T<R> pi = pind_r<R>(a, i);
+        const R poli = polar_r<R>(a, i);
         const V3 ri = ldv(a.pos + 3 * i);
-        for (int k = sub; k < own; k += G) ind_entry(a, I, pi, poli, ri, row[k], 1.0, f, t, acc);
+        for (int k = sub; k < own; k += G) ind_entry<R>(a, I, pi, poli, ri, row[k], R(1), f, t, acc);
         const int32_t* back = row + a.stride - 1;
-        for (int k = sub; k < non; k += G) ind_entry(a, I, pi, poli, ri, back[-k], 1.0, f, t, acc);
+        for (int k = sub; k < non; k += G) ind_entry<R>(a, I, pi, poli, ri, back[-k], R(1), f, t, acc);
         const int e0 = a.excl_off[i], e1 = a.excl_off[i + 1];
-        for (int k = e0 + sub; k < e1; k += G) ind_entry(a, I, pi, poli, ri, a.excl[k], 0.0, f, t, acc);
+        for (int k = e0 + sub; k < e1; k += G) ind_entry<R>(a, I, pi, poli, ri, a.excl[k], R(0), f, t, acc);
     }
 #pragma unroll
     for (int o = G / 2; o > 0; o >>= 1) {
@@ -1089,13 +1275,11 @@ __global__ __launch_bounds__(256) void k_ind_site_sums(MpArgs a) {
         v[11] = p.x; v[12] = p.y; v[13] = p.z;
         v[14] = dot(p, p);
     }
-    double self[1] = {v[0]};
-    block_add<1>(self, a.terms + M_SELF);
-    __syncthreads();
-    double su[14];
-#pragma unroll
-    for (int k = 0; k < 14; ++k) su[k] = v[k + 1];
-    block_add<14>(su, a.terms + M_SU_QR);
+    block_store<15>(v, a.spart, gridDim.x);
+    if (last_block(a.sticket)) {
+        fixed_totals(a.spart, gridDim.x, gridDim.x, 1, a.terms + M_SELF);
+        fixed_totals(a.spart + gridDim.x, gridDim.x, gridDim.x, 14, a.terms + M_SU_QR);
+    }
 }
 
 // One thread per molecule: excluded pairs inside it (intraMolCorrect
@@ -1200,62 +1384,80 @@ __global__ __launch_bounds__(256) void k_ind_surface(MpArgs a, double coeff) {
 //   8-10 (q - kQk) cos d   11-13 (q - kQk) sin d   14-16 (mu.k) cos d
 //  17-19 (mu.k) sin d     20-22 (p.k) cos d       23-25 (p.k) sin d
 //  26 (q - kQk) cos |d|^2  27 (q - kQk) sin |d|^2
+template <typename R>
 __global__ __launch_bounds__(MP_SF_THREADS) void k_ind_sfac(MpArgs a) {
-    __shared__ double sh[MP_SF_THREADS][19];   // r q mu Q(6) p d
+    // per atom of the tile: r in FP64 (the phase); q, mu, Q, p, d in R
+    __shared__ double shr[MP_SF_THREADS][3];
+    __shared__ R shm[MP_SF_THREADS][16];
     const int k = blockIdx.x * MP_SF_THREADS + threadIdx.x;
     const int chunk = blockIdx.y;
     const int i0 = chunk * a.per, i1 = min(a.n, i0 + a.per);
     V3 kv = v3(0, 0, 0);
     if (k < a.K) kv = ldv(a.kvec + 3 * k);
-    double s[IN_NSUM];
+    const R kx = (R)kv.x, ky = (R)kv.y, kz = (R)kv.z;
+    double S[IN_NSUM];
 #pragma unroll
-    for (int x = 0; x < IN_NSUM; ++x) s[x] = 0.0;
+    for (int x = 0; x < IN_NSUM; ++x) S[x] = 0.0;
     for (int t0 = i0; t0 < i1; t0 += MP_SF_THREADS) {
         const int i = t0 + threadIdx.x;
         __syncthreads();
         if (i < i1) {
-            double* p = sh[threadIdx.x];
-            p[0] = a.pos[3 * i]; p[1] = a.pos[3 * i + 1]; p[2] = a.pos[3 * i + 2];
-            p[3] = a.q[i] / SQRT_COULOMB_MP;
-            p[4] = a.dip[3 * i]; p[5] = a.dip[3 * i + 1]; p[6] = a.dip[3 * i + 2];
-            for (int c = 0; c < 6; ++c) p[7 + c] = a.quad[6 * i + c];
-            p[13] = a.pind[3 * i]; p[14] = a.pind[3 * i + 1]; p[15] = a.pind[3 * i + 2];
+            shr[threadIdx.x][0] = a.pos[3 * i]; shr[threadIdx.x][1] = a.pos[3 * i + 1]; shr[threadIdx.x][2] = a.pos[3 * i + 2];
+            R* p = shm[threadIdx.x];
+            p[0] = (R)(a.q[i] / SQRT_COULOMB_MP);
+            p[1] = (R)a.dip[3 * i]; p[2] = (R)a.dip[3 * i + 1]; p[3] = (R)a.dip[3 * i + 2];
+            for (int c = 0; c < 6; ++c) p[4 + c] = (R)a.quad[6 * i + c];
+            p[10] = (R)a.pind[3 * i]; p[11] = (R)a.pind[3 * i + 1]; p[12] = (R)a.pind[3 * i + 2];
             const int m = a.mol[i];
-            p[16] = p[0] - a.com[3 * m];
-            p[17] = p[1] - a.com[3 * m + 1];
-            p[18] = p[2] - a.com[3 * m + 2];
+            p[13] = (R)(a.pos[3 * i] - a.com[3 * m]);
+            p[14] = (R)(a.pos[3 * i + 1] - a.com[3 * m + 1]);
+            p[15] = (R)(a.pos[3 * i + 2] - a.com[3 * m + 2]);
         }
         __syncthreads();
         const int cnt = min(MP_SF_THREADS, i1 - t0);
         if (k < a.K) {
+            // FP64: the sums accumulate directly; FP32: the tile's sums, added
+            // to the FP64 totals after the tile
+            auto tile = [&](auto& s) {
             for (int x = 0; x < cnt; ++x) {
-                const double* p = sh[x];
-                double sn, cs;
-                sincos(p[0] * kv.x + p[1] * kv.y + p[2] * kv.z, &sn, &cs);
-                const double QK = p[7] * kv.x * kv.x + p[10] * kv.y * kv.y + p[12] * kv.z * kv.z +
-                                  2.0 * (p[8] * kv.x * kv.y + p[9] * kv.x * kv.z + p[11] * kv.y * kv.z);
-                const double A = p[3] - QK;
-                const double Mu = p[4] * kv.x + p[5] * kv.y + p[6] * kv.z;
-                const double Pk = p[13] * kv.x + p[14] * kv.y + p[15] * kv.z;
-                const double ac = A * cs, as = A * sn, mc = Mu * cs, ms = Mu * sn, pc = Pk * cs, ps = Pk * sn;
-                const double dx = p[16], dy = p[17], dz = p[18];
+                const R* p = shm[x];
+                R sn, cs;
+                phase_sincos<R>(shr[x][0] * kv.x + shr[x][1] * kv.y + shr[x][2] * kv.z, sn, cs);
+                const R QK = p[4] * kx * kx + p[7] * ky * ky + p[9] * kz * kz +
+                             R(2) * (p[5] * kx * ky + p[6] * kx * kz + p[8] * ky * kz);
+                const R A = p[0] - QK;
+                const R Mu = p[1] * kx + p[2] * ky + p[3] * kz;
+                const R Pk = p[10] * kx + p[11] * ky + p[12] * kz;
+                const R ac = A * cs, as = A * sn, mc = Mu * cs, ms = Mu * sn, pc = Pk * cs, ps = Pk * sn;
+                const R dx = p[13], dy = p[14], dz = p[15];
                 s[0] += ac; s[1] += as; s[2] += mc; s[3] += ms; s[4] += pc; s[5] += ps;
-                s[6] += 2.0 * QK * cs; s[7] += 2.0 * QK * sn;
+                s[6] += R(2) * QK * cs; s[7] += R(2) * QK * sn;
                 s[8] += ac * dx; s[9] += ac * dy; s[10] += ac * dz;
                 s[11] += as * dx; s[12] += as * dy; s[13] += as * dz;
                 s[14] += mc * dx; s[15] += mc * dy; s[16] += mc * dz;
                 s[17] += ms * dx; s[18] += ms * dy; s[19] += ms * dz;
                 s[20] += pc * dx; s[21] += pc * dy; s[22] += pc * dz;
                 s[23] += ps * dx; s[24] += ps * dy; s[25] += ps * dz;
-                const double d2 = dx * dx + dy * dy + dz * dz;
+                const R d2 = dx * dx + dy * dy + dz * dz;
                 s[26] += ac * d2; s[27] += as * d2;
+            }
+            };
+            if constexpr (sizeof(R) == sizeof(double)) {
+                tile(S);
+            } else {
+                R s[IN_NSUM];
+#pragma unroll
+                for (int x = 0; x < IN_NSUM; ++x) s[x] = R(0);
+                tile(s);
+#pragma unroll
+                for (int x = 0; x < IN_NSUM; ++x) S[x] += (double)s[x];
             }
         }
     }
     if (k < a.K) {
         double* out = a.part + ((size_t)chunk * a.K + k) * IN_NSUM;
 #pragma unroll
-        for (int x = 0; x < IN_NSUM; ++x) out[x] = s[x];
+        for (int x = 0; x < IN_NSUM; ++x) out[x] = S[x];
     }
 }
 
@@ -1319,17 +1521,20 @@ __global__ __launch_bounds__(256) void k_ind_kfinish(MpArgs a) {
 }
 
 // k-space pass 3 (:1603-1690), split over k-ranges like k_mp_kforce.
+template <typename R>
 __global__ __launch_bounds__(WAVE) void k_ind_kforce(MpArgs a) {
     __shared__ double sk[MP_KTILE][7];
     const int i = blockIdx.x * WAVE + threadIdx.x;
     const int kb = blockIdx.y * a.kper, ke = min(a.K, kb + a.kper);
-    Site I{};
-    V3 ri = v3(0, 0, 0), p = v3(0, 0, 0);
+    Site I64{};
+    V3 ri = v3(0, 0, 0), p64 = v3(0, 0, 0);
     if (i < a.n) {
-        I = load_site(a, i);
+        I64 = load_site(a, i);
         ri = ldv(a.pos + 3 * i);
-        p = ldv(a.pind + 3 * i);
+        p64 = ldv(a.pind + 3 * i);
     }
+    const SiteT<R> I = site_as<R>(I64);
+    const V3T<R> p = vec3<R>((R)p64.x, (R)p64.y, (R)p64.z);
     V3 F = v3(0, 0, 0), T = v3(0, 0, 0);
     for (int k0 = kb; k0 < ke; k0 += MP_KTILE) {
         const int cnt = min(MP_KTILE, ke - k0);
@@ -1345,18 +1550,21 @@ __global__ __launch_bounds__(WAVE) void k_ind_kforce(MpArgs a) {
         }
         __syncthreads();
         if (i < a.n) {
+            V3T<R> f = vec3<R>(R(0), R(0), R(0)), t = f;
             for (int x = 0; x < cnt; ++x) {
-                const V3 kv = v3(sk[x][0], sk[x][1], sk[x][2]);
-                const double X = sk[x][3], Y = sk[x][4], bPc = sk[x][5], bPs = sk[x][6];
-                double sn, cs;
-                sincos(dot(ri, kv), &sn, &cs);
-                const V3 Qk = smul(I.Q, kv);
-                const double A = I.q - dot(kv, Qk), Mu = dot(I.d, kv), Pk = dot(p, kv);
-                const double val = cs * (Pk * X - A * bPc - Mu * bPs) + sn * (Pk * Y - A * bPs + Mu * bPc);
-                F = F + val * kv;
+                R sn, cs;
+                phase_sincos<R>(ri.x * sk[x][0] + ri.y * sk[x][1] + ri.z * sk[x][2], sn, cs);
+                const V3T<R> kv = vec3<R>((R)sk[x][0], (R)sk[x][1], (R)sk[x][2]);
+                const R X = (R)sk[x][3], Y = (R)sk[x][4], bPc = (R)sk[x][5], bPs = (R)sk[x][6];
+                const V3T<R> Qk = smul(I.Q, kv);
+                const R A = I.q - dot(kv, Qk), Mu = dot(I.d, kv), Pk = dot(p, kv);
+                const R val = cs * (Pk * X - A * bPc - Mu * bPs) + sn * (Pk * Y - A * bPs + Mu * bPc);
+                f = f + val * kv;
                 if (a.torques)
-                    T = T + (cs * bPc + sn * bPs) * cross(kv, I.d) + (2.0 * (cs * bPs - sn * bPc)) * cross(kv, Qk);
+                    t = t + (cs * bPc + sn * bPs) * cross(kv, I.d) + (R(2) * (cs * bPs - sn * bPc)) * cross(kv, Qk);
             }
+            F = F + to64(f);
+            T = T + to64(t);
         }
     }
     if (i < a.n) {
@@ -1396,7 +1604,7 @@ int induction_short(Ctx* c, smd_indf_out* out) {
     SMD_HIP(c, hipMemsetAsync(c->d_mp_terms, 0, sizeof(double) * M_COUNT, c->stream));
     if ((rc = frames_run(c))) return rc;
     hipLaunchKernelGGL(k_ind_site_sums, ga, b256, 0, c->stream, a);
-    hipLaunchKernelGGL((k_ind_real<MP_LANES>), gr, b256, 0, c->stream, a);
+    launch_real(c, a, true);
     hipLaunchKernelGGL(k_ind_intra, gm, b256, 0, c->stream, a);
     const double volume = a.volume, volumer = 1.0 / volume;
     const double coeff = 2.0 * PI * volumer / 3.0;
@@ -1448,9 +1656,11 @@ int induction_long(Ctx* c, smd_mp_long_out* out) {
     SMD_HIP(c, hipMemsetAsync(c->d_mp_terms, 0, sizeof(double) * M_COUNT, c->stream));
     if ((rc = frames_run(c))) return rc;
     const dim3 g1((a.K + MP_SF_THREADS - 1) / MP_SF_THREADS, a.chunks);
-    hipLaunchKernelGGL(k_ind_sfac, g1, dim3(MP_SF_THREADS), 0, c->stream, a);
+    if (c->mp_prec == SMD_PREC_MIXED) hipLaunchKernelGGL((k_ind_sfac<float>), g1, dim3(MP_SF_THREADS), 0, c->stream, a);
+        else hipLaunchKernelGGL((k_ind_sfac<double>), g1, dim3(MP_SF_THREADS), 0, c->stream, a);
     hipLaunchKernelGGL(k_ind_kfinish, dim3((a.K + 3) / 4), b256, 0, c->stream, a);
-    hipLaunchKernelGGL(k_ind_kforce, dim3((c->n + WAVE - 1) / WAVE, a.ksplit), dim3(WAVE), 0, c->stream, a);
+    if (c->mp_prec == SMD_PREC_MIXED) hipLaunchKernelGGL((k_ind_kforce<float>), dim3((c->n + WAVE - 1) / WAVE, a.ksplit), dim3(WAVE), 0, c->stream, a);
+        else hipLaunchKernelGGL((k_ind_kforce<double>), dim3((c->n + WAVE - 1) / WAVE, a.ksplit), dim3(WAVE), 0, c->stream, a);
     hipLaunchKernelGGL(k_kforce_sum, dim3((c->n + 255) / 256), b256, 0, c->stream, a);
     if (c->indf.torques) hipLaunchKernelGGL(k_mp_torques, dim3((c->nmol + 255) / 256), b256, 0, c->stream, a, 8u);
     SMD_HIP(c, hipGetLastError());
@@ -1466,6 +1676,95 @@ int induction_long(Ctx* c, smd_mp_long_out* out) {
     out->virial[2] = out->virial[6] = lv[4];
     out->virial[5] = out->virial[7] = lv[5];
     out->tot_torque = t[M_TRQ + 3];
+    return SMD_OK;
+}
+
+// The four classes' kernels without read-back for the polarizable MD step:
+// terms go to the given accumulator (cleared here by a fill kernel, so the
+// sequence can be captured), forces and torque-derived forces into d_force.
+// The lab-frame multipoles must be current (frames_run).
+int multipole_enqueue(Ctx* c, double* terms) {
+    MpArgs a = mp_args(c);
+    a.terms = terms;
+    const dim3 b256(256);
+    const dim3 ga((c->n + 255) / 256), gm((c->nmol + 255) / 256);
+    const dim3 gr((unsigned)(((int64_t)c->n * MP_LANES + 255) / 256));
+    hipLaunchKernelGGL(k_zero_terms, dim3(1), b256, 0, c->stream, terms);
+    {
+        ProfScope ps(c, "mp_short");
+        hipLaunchKernelGGL(k_mp_site_sums, ga, b256, 0, c->stream, a);
+        launch_real(c, a, false);
+        hipLaunchKernelGGL(k_mp_intra, gm, b256, 0, c->stream, a);
+        if (c->mp.surface) hipLaunchKernelGGL(k_mp_surface, ga, b256, 0, c->stream, a, 2.0 * PI * (1.0 / a.volume) / 3.0);
+        if (c->mp.torques) hipLaunchKernelGGL(k_mp_torques, gm, b256, 0, c->stream, a, c->mp.surface ? 7u : 3u);
+    }
+    {
+        ProfScope ps(c, "mp_long");
+        const dim3 g1((a.K + MP_SF_THREADS - 1) / MP_SF_THREADS, a.chunks);
+        if (c->mp_prec == SMD_PREC_MIXED) hipLaunchKernelGGL((k_mp_sfac<float>), g1, dim3(MP_SF_THREADS), 0, c->stream, a);
+        else hipLaunchKernelGGL((k_mp_sfac<double>), g1, dim3(MP_SF_THREADS), 0, c->stream, a);
+        hipLaunchKernelGGL(k_mp_kfinish, dim3((a.K + 3) / 4), b256, 0, c->stream, a);
+        if (c->mp_prec == SMD_PREC_MIXED) hipLaunchKernelGGL((k_mp_kforce<float>), dim3((c->n + WAVE - 1) / WAVE, a.ksplit), dim3(WAVE), 0, c->stream, a);
+        else hipLaunchKernelGGL((k_mp_kforce<double>), dim3((c->n + WAVE - 1) / WAVE, a.ksplit), dim3(WAVE), 0, c->stream, a);
+        hipLaunchKernelGGL(k_kforce_sum, ga, b256, 0, c->stream, a);
+        if (c->mp.torques) hipLaunchKernelGGL(k_mp_torques, gm, b256, 0, c->stream, a, 8u);
+    }
+    SMD_HIP(c, hipGetLastError());
+    return SMD_OK;
+}
+
+int induction_enqueue(Ctx* c, double* terms) {
+    MpArgs a = ind_args(c);
+    a.terms = terms;
+    const dim3 b256(256);
+    const dim3 ga((c->n + 255) / 256), gm((c->nmol + 255) / 256);
+    const dim3 gr((unsigned)(((int64_t)c->n * MP_LANES + 255) / 256));
+    hipLaunchKernelGGL(k_zero_terms, dim3(1), b256, 0, c->stream, terms);
+    {
+        ProfScope ps(c, "ind_short");
+        hipLaunchKernelGGL(k_ind_site_sums, ga, b256, 0, c->stream, a);
+        launch_real(c, a, true);
+        hipLaunchKernelGGL(k_ind_intra, gm, b256, 0, c->stream, a);
+        if (c->indf.surface) hipLaunchKernelGGL(k_ind_surface, ga, b256, 0, c->stream, a, 2.0 * PI * (1.0 / a.volume) / 3.0);
+        if (c->indf.torques)
+            hipLaunchKernelGGL(k_mp_torques, gm, b256, 0, c->stream, a, 16u | 3u | (c->indf.surface ? 4u : 0u));
+    }
+    {
+        ProfScope ps(c, "ind_long");
+        const dim3 g1((a.K + MP_SF_THREADS - 1) / MP_SF_THREADS, a.chunks);
+        if (c->mp_prec == SMD_PREC_MIXED) hipLaunchKernelGGL((k_ind_sfac<float>), g1, dim3(MP_SF_THREADS), 0, c->stream, a);
+        else hipLaunchKernelGGL((k_ind_sfac<double>), g1, dim3(MP_SF_THREADS), 0, c->stream, a);
+        hipLaunchKernelGGL(k_ind_kfinish, dim3((a.K + 3) / 4), b256, 0, c->stream, a);
+        if (c->mp_prec == SMD_PREC_MIXED) hipLaunchKernelGGL((k_ind_kforce<float>), dim3((c->n + WAVE - 1) / WAVE, a.ksplit), dim3(WAVE), 0, c->stream, a);
+        else hipLaunchKernelGGL((k_ind_kforce<double>), dim3((c->n + WAVE - 1) / WAVE, a.ksplit), dim3(WAVE), 0, c->stream, a);
+        hipLaunchKernelGGL(k_kforce_sum, ga, b256, 0, c->stream, a);
+        if (c->indf.torques) hipLaunchKernelGGL(k_mp_torques, gm, b256, 0, c->stream, a, 8u);
+    }
+    SMD_HIP(c, hipGetLastError());
+    return SMD_OK;
+}
+
+// Checks the polarizable step's classes once, outside any capture.
+int polar_classes_check(Ctx* c) {
+    int rc = mp_check(c);
+    if (rc) return rc;
+    if ((rc = ind_check(c))) return rc;
+    const double rl = c->list_cutoff + c->list_buff;
+    if (c->mp.real_cut > rl || c->indf.real_cut > rl)
+        return fail(c, SMD_ENOTSUP, "real-space cutoff exceeds the pair-list radius %g", rl);
+    return SMD_OK;
+}
+
+// Energies of the multipole and induction accumulators of the MD step.
+int polar_terms_energies(Ctx* c, const double* mp_t, const double* ind_t, double* e) {
+    e[0] = mp_t[M_REAL_E];
+    e[1] = mp_t[M_SELF];
+    e[2] = mp_t[M_INTRA_E];
+    e[3] = mp_t[M_LONG_E];
+    e[4] = ind_t[M_REAL_E];
+    e[5] = ind_t[M_SELF];
+    e[6] = ind_t[M_INTRA_E];
+    e[7] = ind_t[M_LONG_E];
     return SMD_OK;
 }
 
@@ -1499,7 +1798,8 @@ static int mp_forces_out(smd_ctx* c, double* f_accum) {
 
 extern "C" int smd_multipole_compute(smd_ctx* c, int prec, double* f_accum, smd_mp_out* out) {
     if (!c || !out) return SMD_EINVAL;
-    if (prec != SMD_PREC_FP64) return fail(c, SMD_ENOTSUP, "multipole Ewald is FP64 only");
+    if (prec != SMD_PREC_FP64 && prec != SMD_PREC_MIXED) return fail(c, SMD_EINVAL, "bad precision %d", prec);
+    c->mp_prec = prec;
     if (f_accum) SMD_HIP(c, hipMemsetAsync(c->d_force, 0, sizeof(double) * 3 * c->n, c->stream));
     int rc = multipole_short(c, out);
     if (rc) return rc;
@@ -1508,7 +1808,8 @@ extern "C" int smd_multipole_compute(smd_ctx* c, int prec, double* f_accum, smd_
 
 extern "C" int smd_multipole_compute_long(smd_ctx* c, int prec, double* f_accum, smd_mp_long_out* out) {
     if (!c || !out) return SMD_EINVAL;
-    if (prec != SMD_PREC_FP64) return fail(c, SMD_ENOTSUP, "multipole Ewald is FP64 only");
+    if (prec != SMD_PREC_FP64 && prec != SMD_PREC_MIXED) return fail(c, SMD_EINVAL, "bad precision %d", prec);
+    c->mp_prec = prec;
     if (f_accum) SMD_HIP(c, hipMemsetAsync(c->d_force, 0, sizeof(double) * 3 * c->n, c->stream));
     int rc = multipole_long(c, out);
     if (rc) return rc;
@@ -1522,7 +1823,8 @@ extern "C" int smd_induction_setup(smd_ctx* c, const smd_indf_params* p, double 
 
 extern "C" int smd_induction_compute(smd_ctx* c, int prec, double* f_accum, smd_indf_out* out) {
     if (!c || !out) return SMD_EINVAL;
-    if (prec != SMD_PREC_FP64) return fail(c, SMD_ENOTSUP, "induction terms are FP64 only");
+    if (prec != SMD_PREC_FP64 && prec != SMD_PREC_MIXED) return fail(c, SMD_EINVAL, "bad precision %d", prec);
+    c->mp_prec = prec;
     if (f_accum) SMD_HIP(c, hipMemsetAsync(c->d_force, 0, sizeof(double) * 3 * c->n, c->stream));
     int rc = induction_short(c, out);
     if (rc) return rc;
@@ -1531,9 +1833,52 @@ extern "C" int smd_induction_compute(smd_ctx* c, int prec, double* f_accum, smd_
 
 extern "C" int smd_induction_compute_long(smd_ctx* c, int prec, double* f_accum, smd_mp_long_out* out) {
     if (!c || !out) return SMD_EINVAL;
-    if (prec != SMD_PREC_FP64) return fail(c, SMD_ENOTSUP, "induction terms are FP64 only");
+    if (prec != SMD_PREC_FP64 && prec != SMD_PREC_MIXED) return fail(c, SMD_EINVAL, "bad precision %d", prec);
+    c->mp_prec = prec;
     if (f_accum) SMD_HIP(c, hipMemsetAsync(c->d_force, 0, sizeof(double) * 3 * c->n, c->stream));
     int rc = induction_long(c, out);
     if (rc) return rc;
     return f_accum ? mp_forces_out(c, f_accum) : SMD_OK;
+}
+
+// reps launches of one real-space kernel of the polarizable step on the
+// current state, between two events on the context stream.  The kernels
+// write their forces and sums to scratch (the vdW virtual-site force buffer
+// and a spare accumulator), so the step's state is unchanged; the torque
+// scratch is rewritten by every step anyway.
+extern "C" int smd_polar_time_kernel(smd_ctx* c, const char* name, int reps, double* avg_us) {
+    if (!c || !name || reps < 1 || !avg_us) return SMD_EINVAL;
+    if (!c->pmd_ready) return fail(c, SMD_ESTATE, "smd_polar_md_init not called");
+    const std::string f(name);
+    const bool ind = f == "ind_real";
+    if (!ind && f != "mp_real") return fail(c, SMD_EINVAL, "kernel '%s' is not mp_real or ind_real", name);
+    MpArgs a = ind ? ind_args(c) : mp_args(c);
+    a.force = c->d_fvirt;
+    a.terms = c->d_polar_terms + 2 * POLAR_TERMS;
+    const dim3 gr((unsigned)(((int64_t)c->n * MP_LANES + 255) / 256)), b256(256);
+    const bool mixed = c->mp_prec == SMD_PREC_MIXED;
+    if (mixed)
+        hipLaunchKernelGGL(k_pack_sites32, dim3((c->n + 255) / 256), b256, 0, c->stream, a, c->d_site32, c->d_pind32, 1);
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    SMD_HIP(c, hipEventCreate(&e0));
+    SMD_HIP(c, hipEventCreate(&e1));
+    SMD_HIP(c, hipEventRecord(e0, c->stream));
+    for (int r = 0; r < reps; ++r) {
+        if (ind) {
+            if (mixed) hipLaunchKernelGGL((k_ind_real<MP_LANES, float>), gr, b256, 0, c->stream, a);
+            else hipLaunchKernelGGL((k_ind_real<MP_LANES, double>), gr, b256, 0, c->stream, a);
+        } else {
+            if (mixed) hipLaunchKernelGGL((k_mp_real<MP_LANES, float>), gr, b256, 0, c->stream, a);
+            else hipLaunchKernelGGL((k_mp_real<MP_LANES, double>), gr, b256, 0, c->stream, a);
+        }
+    }
+    SMD_HIP(c, hipEventRecord(e1, c->stream));
+    SMD_HIP(c, hipEventSynchronize(e1));
+    float ms = 0.f;
+    SMD_HIP(c, hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    SMD_HIP(c, hipGetLastError());
+    *avg_us = 1e3 * (double)ms / reps;
+    return SMD_OK;
 }

@@ -82,31 +82,6 @@ template <typename R> __device__ __forceinline__ R pow_(R x, R y);
 template <> __device__ __forceinline__ double pow_<double>(double x, double y) { return pow(x, y); }
 template <> __device__ __forceinline__ float pow_<float>(float x, float y) { return powf(x, y); }
 
-// ---- FP32 real-space Ewald kernel of the mixed-precision form ----------------
-// erfc(z) = exp(-z^2) t P(t), t = 1 / (1 + p z): the form of Abramowitz &
-// Stegun 7.1.26 with a degree-7 P whose coefficients were fitted (weighted
-// least squares toward minimax on the relative error of erfc(z) exp(z^2) / t
-// over z in [0, 5.5]; max relative error 2.3e-8 in exact arithmetic, 3.5e-7
-// evaluated in FP32).  exp(-z^2) is the factor the force needs anyway, so the
-// pair costs one exp2, one rcp and one rsq; taking it from the FP32 r^2 adds
-// the 2 z^2 2^-24 relative error of that rounding.
-constexpr float ERFC_P = 0.415f;
-constexpr float ERFC_C[8] = {0.0771595775739555f, -0.36372916731130467f, 0.5426081378824027f,
-                             -0.2276783459634979f, 0.3230029930806543f, 0.17506060444731647f,
-                             0.23978895752318216f, 0.23378723961927758f};
-constexpr float LOG2E_F = 1.4426950408889634f;
-
-// erfc(alpha r) / r and exp(-alpha^2 r^2) from r^2 and 1/r.
-__device__ __forceinline__ void ewald_erfc_f32(float alpha, float alpha2, float r2, float r, float r_1, float& e,
-                                               float& ex) {
-    ex = __builtin_amdgcn_exp2f(-alpha2 * r2 * LOG2E_F);
-    const float t = __builtin_amdgcn_rcpf(1.0f + ERFC_P * (alpha * r));
-    float p = ERFC_C[0];
-#pragma unroll
-    for (int k = 1; k < 8; ++k) p = p * t + ERFC_C[k];
-    e = t * p * ex * r_1;
-}
-
 // Accumulator layout inside the kernel (per lane, FP64).
 enum {
     A_VE = 0, A_VELEC, A_VDUDV, A_VD2UDV,   // vdW energy, shifted Coulomb, Lustig
@@ -1032,6 +1007,34 @@ int pairs_lj_ewald(Ctx* c, const double* sigma2, const double* eps4, double cuto
         return pairs_packed(c, a, lustig);
     hipError_t e = launch_pairs<V_LJ, true>(c, a, prec, lustig);
     if (e != hipSuccess) return hip_fail(c, e, "LJ+Ewald kernel launch");
+    return SMD_OK;
+}
+
+// vdW of the polarizable MD step: the kernels of vdw_run without the table
+// upload, the accumulator clear and the read-back (capturable).  Forces are
+// added to d_force; energies, Lustig sums and the virial to the step's terms.
+int vdw_enqueue(Ctx* c, const smd_vdw_params* p, int prec) {
+    PairArgs a;
+    base_args(c, a);
+    vdw_constants(a, p);
+    const int lustig = p->lustig ? 1 : 0;
+    hipError_t e = hipSuccess;
+    switch (p->kind) {
+    case SMD_VDW_LJ: e = launch_pairs<V_LJ, false>(c, a, prec, lustig); break;
+    case SMD_VDW_BUFF147:
+        hipLaunchKernelGGL(k_virtual_sites, dim3((c->nmol + 255) / 256), dim3(256), 0, c->stream, c->nmol,
+                           c->d_mol_start, c->d_type, c->d_pos, c->d_epos, p->kind, p->hfactor, p->co2_bond);
+        e = launch_pairs<V_BUFF, false>(c, a, prec, lustig);
+        if (e == hipSuccess) {
+            hipLaunchKernelGGL(k_redistribute, dim3((c->nmol + 255) / 256), dim3(256), 0, c->stream, c->nmol,
+                               c->d_mol_start, c->d_fa, c->d_fb, c->d_host, c->d_fvirt, c->d_force);
+            e = hipGetLastError();
+        }
+        break;
+    default:
+        return fail(c, SMD_ENOTSUP, "the polarizable MD step takes LJ or buffered 14-7 vdW");
+    }
+    if (e != hipSuccess) return hip_fail(c, e, "vdW kernel launch");
     return SMD_OK;
 }
 

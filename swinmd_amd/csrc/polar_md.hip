@@ -1,0 +1,242 @@
+// polar_md.hip -- the leap-frog step of rigid polarizable water on the device.
+//
+//   LFIntegrator::run force block, polarizable config
+//                                   src/integrators/LFIntegrator.cpp:385-470
+//   BuffvdWForce::compute           src/forces/BuffvdWForce.cpp:93-343
+//   Multipole::compute/compute_long src/forces/Multipole.cpp:163-240
+//   Induction::compute/compute_long src/forces/Induction.cpp:486-590
+//   SHAKE, kinetics, NH friction    LFIntegrator.cpp:561-722
+//   per-step rewrap                 :897-910
+//
+// A step is three parts on the context stream:
+//   A  the Verlet trigger and drift partials (cleared forces and terms), the
+//      gated list rebuild, vdW, the lab frames, Multipole's short- and
+//      long-range kernels, then the solve's field, coupling table, start,
+//      A x and residual;
+//   B  the solve's iterations in captured batches; the host reads the
+//      device's stop flag between batches (the first read comes after as
+//      many batches as the previous step needed);
+//   C  Induction's short- and long-range kernels with the converged dipoles,
+//      then SHAKE with the kinetic and pressure sums and the NH update.
+// A and C hold no host synchronisation and no memset: each is captured once
+// as a hipGraph and replayed.
+#include <algorithm>
+#include <cstring>
+#include <vector>
+
+#include "smd_internal.h"
+
+namespace smd {
+
+// The polarizable step's term accumulators: Multipole's, then Induction's.
+static double* mp_terms(Ctx* c) { return c->d_polar_terms; }
+static double* ind_terms(Ctx* c) { return c->d_polar_terms + POLAR_TERMS; }
+
+// Part A: everything up to the solve's first iteration.
+static int part_a(Ctx* c, bool on_device_rebuild) {
+    const smd_polar_md_params& p = c->pmd;
+    int rc = md_head_enqueue(c);
+    if (!rc) rc = pairlist_build(c, p.cutoff, p.cut_buff, p.use_cells, on_device_rebuild);
+    if (!rc) rc = vdw_enqueue(c, &c->pmd_vdw, p.prec);
+    if (!rc) rc = frames_run(c);
+    if (!rc) rc = multipole_enqueue(c, mp_terms(c));
+    if (!rc) rc = induce_begin(c, &p.induce, false);
+    return rc;
+}
+
+// Part C: induction terms and the integration tail.
+static int part_c(Ctx* c) {
+    int rc = induction_enqueue(c, ind_terms(c));
+    if (!rc) rc = md_tail_enqueue(c);
+    return rc;
+}
+
+static int capture(Ctx* c, hipGraphExec_t* exec, int (*fn)(Ctx*)) {
+    hipGraph_t g = nullptr;
+    SMD_HIP(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    const int rc = fn(c);
+    const hipError_t e = hipStreamEndCapture(c->stream, &g);
+    if (rc) {
+        if (g) (void)hipGraphDestroy(g);
+        return rc;
+    }
+    if (e != hipSuccess) return hip_fail(c, e, "hipStreamEndCapture (polarizable step)");
+    const hipError_t ei = hipGraphInstantiate(exec, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (ei != hipSuccess) {
+        *exec = nullptr;
+        return hip_fail(c, ei, "hipGraphInstantiate (polarizable step)");
+    }
+    return SMD_OK;
+}
+
+static int part_a_dev(Ctx* c) { return part_a(c, true); }
+
+// Runs the solve's iterations until the device says it stopped; the first
+// check comes after `first` batches.  Returns the iteration count.
+static int solve(Ctx* c, int first, int* iters, double* res) {
+    const smd_induce_params& ip = c->pmd.induce;
+    if (!ip.do_min) {
+        *iters = 0;
+        *res = 0.0;
+        return SMD_OK;
+    }
+    const int max_it = ip.max_iter > 0 ? ip.max_iter : 3 * c->n;
+    int launched = 0, done = 0, reason = 0, rc;
+    int batch = std::max(1, first);
+    while (true) {
+        if ((rc = induce_batches(c, &ip, batch))) return rc;
+        launched += batch * c->cg_batch;
+        if ((rc = induce_poll(c, &done, iters, &reason, res))) return rc;
+        if (done) {
+            c->ind_last_converged = reason != 0;
+            break;
+        }
+        if (launched >= max_it) return fail(c, SMD_ENOCONV, "induced-dipole solve did not converge in %d iterations", *iters);
+        batch = 1;
+    }
+    return SMD_OK;
+}
+
+static void free_graphs(Ctx* c) {
+    if (c->pmd_a) { (void)hipGraphExecDestroy(c->pmd_a); c->pmd_a = nullptr; }
+    if (c->pmd_c) { (void)hipGraphExecDestroy(c->pmd_c); c->pmd_c = nullptr; }
+}
+
+}  // namespace smd
+
+using namespace smd;
+
+extern "C" int smd_polar_md_init(smd_ctx* c, const smd_polar_md_params* p) {
+    if (!c || !p) return SMD_EINVAL;
+    if (!c->box_set) return fail(c, SMD_ESTATE, "box not set");
+    if (!c->multipoles_set) return fail(c, SMD_ESTATE, "multipoles not set (smd_set_multipoles)");
+    if (p->prec != SMD_PREC_FP64 && p->prec != SMD_PREC_MIXED) return fail(c, SMD_EINVAL, "bad precision %d", p->prec);
+    if (p->vdw.kind != SMD_VDW_BUFF147 && p->vdw.kind != SMD_VDW_LJ)
+        return fail(c, SMD_ENOTSUP, "the polarizable step takes LJ or buffered 14-7 vdW");
+    for (int m = 0; m < c->nmol; ++m)
+        if (c->h_mol_start[m + 1] - c->h_mol_start[m] != 3)
+            return fail(c, SMD_ENOTSUP, "the polarizable step is implemented for rigid 3-site water");
+    free_graphs(c);
+    c->md_ready = false;
+    c->pmd_ready = false;
+    c->pmd = *p;
+    // the step's own copies of the vdW tables
+    const int nt = c->ntypes, nt2 = nt * nt;
+    auto keep = [](std::vector<double>& v, const double* src, int count) -> const double* {
+        if (!src) return nullptr;
+        v.assign(src, src + count);
+        return v.data();
+    };
+    c->pmd_vdw = p->vdw;
+    c->pmd_vdw.sigma2 = keep(c->pmd_tab[0], p->vdw.sigma2, nt2);
+    c->pmd_vdw.eps4 = keep(c->pmd_tab[1], p->vdw.eps4, nt2);
+    c->pmd_vdw.type_sigma = keep(c->pmd_tab[2], p->vdw.type_sigma, nt);
+    c->pmd_vdw.type_eps = keep(c->pmd_tab[3], p->vdw.type_eps, nt);
+    c->pmd_vdw.buck_a = c->pmd_vdw.buck_b = c->pmd_vdw.buck_c = nullptr;
+    // the MD head and tail read the step description from c->md
+    c->md = smd_md_params{};
+    c->md.cutoff = p->cutoff;
+    c->md.cut_buff = p->cut_buff;
+    c->md.integ = p->integ;
+    c->md.use_cells = p->use_cells;
+    c->md.lustig = p->vdw.lustig;
+    c->md.prec = p->prec;
+    c->md.total_steps = p->total_steps;
+    c->md.constraint_mode = 2;
+    c->md.ewald_on = 0;
+    c->mp_prec = p->prec;
+    int rc = upload_vdw_tables(c, &c->pmd_vdw);
+    if (rc) return rc;
+    if ((rc = smd_multipole_setup(c, &p->mp, p->mp_kcut, nullptr))) return rc;
+    if ((rc = smd_induction_setup(c, &p->indf, p->ind_kcut, nullptr))) return rc;
+    if (!c->d_polar_terms) SMD_HIP(c, hipMalloc(&c->d_polar_terms, sizeof(double) * 2 * POLAR_TERMS));
+    if (!c->d_stress) SMD_HIP(c, hipMalloc(&c->d_stress, sizeof(double) * 12));
+    if ((rc = pairlist_build(c, p->cutoff, p->cut_buff, p->use_cells, false))) return rc;
+    {
+        // headroom for in-step rebuilds: the row capacity is fixed once captured
+        int64_t np;
+        int32_t mx, uc;
+        if ((rc = smd_pairlist_info(c, &np, &mx, &uc))) return rc;
+        const int32_t want = ((int32_t)(mx * 1.5) + 64 + 63) / 64 * 64;
+        if (want > c->stride) {
+            if ((rc = pairlist_alloc(c, want))) return rc;
+            if ((rc = pairlist_build(c, p->cutoff, p->cut_buff, p->use_cells, false))) return rc;
+        }
+    }
+    if ((rc = polar_classes_check(c))) return rc;
+    if ((rc = smd_compute_mol_centers(c))) return rc;
+    // state of the step
+    SMD_HIP(c, hipMemsetAsync(c->d_scalars, 0, sizeof(double) * 8, c->stream));
+    SMD_HIP(c, hipMemsetAsync(c->d_stress, 0, sizeof(double) * 12, c->stream));
+    SMD_HIP(c, hipMemsetAsync(c->d_mom, 0, sizeof(double) * 3 * c->nmol, c->stream));
+    SMD_HIP(c, hipMemsetAsync(c->d_disp, 0, sizeof(double) * 3 * c->n, c->stream));
+    SMD_HIP(c, hipMemsetAsync(c->d_status, 0, sizeof(unsigned), c->stream));
+    SMD_HIP(c, hipMemsetAsync(c->d_flag, 0, sizeof(int) * 8, c->stream));
+    // start(): the forces at t (a full evaluation with the list just built),
+    // then vtMinus = v - dt/2 F/m
+    if ((rc = part_a(c, false))) return rc;
+    int iters = 0;
+    double res = 0.0;
+    if ((rc = solve(c, 1, &iters, &res))) return rc;
+    if ((rc = induction_enqueue(c, ind_terms(c)))) return rc;
+    if ((rc = md_start_vtm(c))) return rc;
+    SMD_HIP(c, hipStreamSynchronize(c->stream));
+    SMD_HIP(c, hipMemsetAsync(c->d_disp, 0, sizeof(double) * 3 * c->n, c->stream));
+    SMD_HIP(c, hipStreamSynchronize(c->stream));
+    if ((rc = check_status(c))) return rc;
+    c->pmd_last_iters = iters;
+    c->md_step = 0;
+    c->pmd_ready = true;
+    return SMD_OK;
+}
+
+extern "C" int smd_polar_md_run(smd_ctx* c, int64_t nsteps, smd_polar_md_report* out) {
+    if (!c || nsteps < 0) return SMD_EINVAL;
+    if (!c->pmd_ready) return fail(c, SMD_ESTATE, "smd_polar_md_init not called");
+    c->mp_prec = c->pmd.prec;
+    const bool graph = c->pmd.use_graph && !c->prof.on;
+    int rc;
+    if (graph && !c->pmd_a) {
+        if ((rc = capture(c, &c->pmd_a, part_a_dev))) return rc;
+        if ((rc = capture(c, &c->pmd_c, part_c))) return rc;
+    }
+    int64_t total_iters = 0;
+    int iters = 0;
+    double res = 0.0;
+    for (int64_t s = 0; s < nsteps; ++s) {
+        if (graph) SMD_HIP(c, hipGraphLaunch(c->pmd_a, c->stream));
+        else if ((rc = part_a(c, true))) return rc;
+        // the first check after the batches the previous step needed
+        const int first = (c->pmd_last_iters + 1 + c->cg_batch - 1) / c->cg_batch;
+        if ((rc = solve(c, first, &iters, &res))) return rc;
+        c->pmd_last_iters = iters;
+        total_iters += iters;
+        if (graph) SMD_HIP(c, hipGraphLaunch(c->pmd_c, c->stream));
+        else if ((rc = part_c(c))) return rc;
+        c->md_step++;
+    }
+    SMD_HIP(c, hipStreamSynchronize(c->stream));
+    if ((rc = check_status(c))) return rc;
+    if (out) {
+        double t[T_COUNT], pt[2 * POLAR_TERMS], sc[8];
+        if ((rc = read_terms(c, t))) return rc;
+        SMD_HIP(c, hipMemcpy(pt, c->d_polar_terms, sizeof pt, hipMemcpyDeviceToHost));
+        SMD_HIP(c, hipMemcpy(sc, c->d_scalars, sizeof sc, hipMemcpyDeviceToHost));
+        double e[8];
+        polar_terms_energies(c, pt, pt + POLAR_TERMS, e);
+        *out = smd_polar_md_report{};
+        out->steps = (int64_t)sc[3];
+        out->list_updates = (int64_t)sc[2];
+        out->cg_iterations = total_iters;
+        out->last_iterations = iters;
+        out->last_residual = res;
+        out->e_vdw = t[T_LJ_E] + c->pmd_vdw.e_lrc;
+        out->e_mp_real = e[0]; out->e_mp_self = e[1]; out->e_mp_intra = e[2]; out->e_mp_long = e[3];
+        out->e_ind_real = e[4]; out->e_ind_self = e[5]; out->e_ind_intra = e[6]; out->e_ind_long = e[7];
+        out->atom_ke2 = t[T_KE_ATOM];
+        out->mol_ke2 = t[T_KE_MOL];
+        out->fnh = sc[0];
+    }
+    return SMD_OK;
+}

@@ -15,6 +15,7 @@
 namespace smd {
 
 constexpr int WAVE = 64;
+constexpr int KIN_PARTS = 32;   // SHAKE virial 9, kinetic sums 2, pressure sums 21
 // Pair-list entries of the reference's half list carry this bit.
 constexpr uint32_t OWNED = 0x80000000u;
 constexpr int32_t INDEX_MASK = 0x7fffffff;
@@ -94,7 +95,20 @@ struct Ctx {
     double* d_bond2 = nullptr;    // [3nmol]
     double* d_terms = nullptr;    // [T_COUNT]
     unsigned* d_status = nullptr;
-    int* d_flag = nullptr;        // [4]: 0 rebuild-needed, 1 drift-apply, 2 kinetics ticket
+    int* d_flag = nullptr;        // [8]: 0 rebuild-needed, 1 drift-apply, 2 kinetics ticket,
+                                  // 3 SHAKE ticket, 4 surface-sum ticket, 5 multipole/induction site-sum ticket
+    // Fixed-order block partials of the sums that feed the trajectory (drift
+    // momentum, SHAKE virial, kinetic and pressure sums, surface dipoles): each
+    // block stores its sums, and a fixed-order total replaces FP64 atomics, so
+    // the trajectory does not depend on block scheduling.
+    double* d_part = nullptr;
+    double* part_vom = nullptr;   // [4][nb_atoms]: k_step_head
+    double* part_kin = nullptr;   // [KIN_PARTS][nb_mols64]: kinetics kernels
+    double* part_sh = nullptr;    // [9][nb_mols]: k_shake
+    double* part_su = nullptr;    // [16][nb_atoms]: surface sums
+    int nb_atoms = 0;             // blocks of 256 over the atoms
+    int nb_mols = 0;              // blocks of 256 over the molecules
+    int nb_mols64 = 0;            // blocks of 64 over the molecules
     double* d_scalars = nullptr;  // [8]: fnh, fnh_minus, ...
     double* h_pinned = nullptr;   // pinned staging for small results
 
@@ -150,6 +164,11 @@ struct Ctx {
 
     // induced dipoles
     bool multipoles_set = false;
+    int npol = 0;                  // sites with a polarizability above POLMIN
+    float4* d_cg32 = nullptr;      // mixed solve: FP32 copy of the CG direction [n]
+    int mp_prec = 0;               // smd_prec of the multipole / induction real-space pair math
+    float4* d_site32 = nullptr;    // mixed real space: FP32 site records [3n]
+    float4* d_pind32 = nullptr;    // mixed real space: FP32 induced dipoles [n]
     double* d_dipl = nullptr;     // [3n] local-frame dipoles
     double* d_quadl = nullptr;    // [6n] local-frame quadrupoles
     double* d_polar = nullptr;    // [n]
@@ -163,6 +182,7 @@ struct Ctx {
     double* d_cg_scal = nullptr;  // [4] last residual norm
     int cg_graph = 1;                   // replay CG batches from a captured graph
     int cg_lanes = 16;                  // lanes per row of the CG's A v
+    int cg_batch = 16;                  // CG iterations per captured batch
     hipGraphExec_t cg_exec = nullptr;   // a captured batch of CG iterations
     std::string cg_key;                 // the kernel arguments it was captured with
     int it_stride = 0;            // coupling table of the solve (induction.hip)
@@ -233,6 +253,15 @@ struct Ctx {
     void* d_pack = nullptr;       // float4 [3n]: packed records, then offsets from the molecule centres
     // learned initial guess of the induced dipoles (learned_guess.hip)
     struct Mlp* mlp = nullptr;
+    // polarizable MD step (polar_md.hip)
+    smd_polar_md_params pmd{};
+    smd_vdw_params pmd_vdw{};           // pmd.vdw with the step's own table copies
+    std::vector<double> pmd_tab[4];     // sigma2, eps4, type sigma, type eps
+    double* d_polar_terms = nullptr;    // [2][POLAR_TERMS]: Multipole's, Induction's accumulators
+    hipGraphExec_t pmd_a = nullptr;     // captured forces up to the solve
+    hipGraphExec_t pmd_c = nullptr;     // captured induction terms and integration tail
+    bool pmd_ready = false;
+    int pmd_last_iters = 0;             // the previous step's solve iterations
     smd_induce_params ind_last{};   // parameters of the last smd_induce
     bool ind_last_converged = false;
 };
@@ -247,6 +276,15 @@ int fail(Ctx* c, int code, const char* fmt, ...);
 // learned guess (learned_guess.hip) and the induction helpers it uses (induction.hip)
 void mlp_release(Ctx* c);
 int mlp_guess(Ctx* c, double* x, const double* z);
+int induce_begin(Ctx* c, const smd_induce_params* p, bool frames);
+int induce_batches(Ctx* c, const smd_induce_params* p, int batches);
+int induce_poll(Ctx* c, int* done, int* iters, int* reason, double* res);
+int frames_run(Ctx* c);
+constexpr int POLAR_TERMS = 128;   // doubles of one multipole/induction accumulator (>= M_COUNT)
+int multipole_enqueue(Ctx* c, double* terms);
+int induction_enqueue(Ctx* c, double* terms);
+int polar_classes_check(Ctx* c);
+int polar_terms_energies(Ctx* c, const double* mp_t, const double* ind_t, double* e);
 int induce_direct_apply(Ctx* c, const smd_induce_params* p, double* x, double* z);
 int hip_fail(Ctx* c, hipError_t e, const char* what);
 
@@ -354,6 +392,88 @@ __device__ __forceinline__ void block_add(double (&v)[NV], double* dst) {
     }
 }
 
+// ---- FP32 erfc of the mixed-precision real-space kernels ----------------------
+// erfc(z) = exp(-z^2) t P(t), t = 1 / (1 + p z): the form of Abramowitz &
+// Stegun 7.1.26 with a degree-7 P whose coefficients were fitted (weighted
+// least squares toward minimax on the relative error of erfc(z) exp(z^2) / t
+// over z in [0, 5.5]; max relative error 2.3e-8 in exact arithmetic, 3.5e-7
+// evaluated in FP32).  exp(-z^2) is the factor the force needs anyway, so the
+// pair costs one exp2, one rcp and one rsq; taking it from the FP32 r^2 adds
+// the 2 z^2 2^-24 relative error of that rounding.
+constexpr float ERFC_P = 0.415f;
+constexpr float ERFC_C[8] = {0.0771595775739555f, -0.36372916731130467f, 0.5426081378824027f,
+                             -0.2276783459634979f, 0.3230029930806543f, 0.17506060444731647f,
+                             0.23978895752318216f, 0.23378723961927758f};
+constexpr float LOG2E_F = 1.4426950408889634f;
+
+// erfc(alpha r) / r and exp(-alpha^2 r^2) from r^2 and 1/r.
+__device__ __forceinline__ void ewald_erfc_f32(float alpha, float alpha2, float r2, float r, float r_1, float& e,
+                                               float& ex) {
+    ex = __builtin_amdgcn_exp2f(-alpha2 * r2 * LOG2E_F);
+    const float t = __builtin_amdgcn_rcpf(1.0f + ERFC_P * (alpha * r));
+    float p = ERFC_C[0];
+#pragma unroll
+    for (int k = 1; k < 8; ++k) p = p * t + ERFC_C[k];
+    e = t * p * ex * r_1;
+}
+
+// Stores the block's NV sums at part[k * stride + blockIdx.x] (no atomics).
+// A fixed-order total of the blocks' partials then gives the same bits
+// whatever order the blocks ran in.
+template <int NV>
+__device__ __forceinline__ void block_store(double (&v)[NV], double* part, int stride) {
+    __shared__ double red[NV][16];
+    const int lane = threadIdx.x % WAVE, wid = threadIdx.x / WAVE;
+    const int nw = (blockDim.x + WAVE - 1) / WAVE;
+#pragma unroll
+    for (int k = 0; k < NV; ++k) v[k] = wave_sum(v[k]);
+    if (lane == 0)
+#pragma unroll
+        for (int k = 0; k < NV; ++k) red[k][wid] = v[k];
+    __syncthreads();
+    if (threadIdx.x < NV) {
+        double s = 0.0;
+        for (int w = 0; w < nw; ++w) s += red[threadIdx.x][w];
+        part[(size_t)threadIdx.x * stride + blockIdx.x] = s;
+    }
+    __syncthreads();   // red may be reused by the next call
+}
+
+// Fixed-order sum of p[0..n) by one wave: lane l adds p[l], p[l + 64], ...
+// in order, then a shuffle tree; every lane gets the same bits.
+__device__ __forceinline__ double wave_fixed_sum(const volatile double* p, int n) {
+    double s = 0.0;
+    for (int k = threadIdx.x % WAVE; k < n; k += WAVE) s += p[k];
+    return wave_sum(s);
+}
+
+// True in the last block of the grid to get here (a ticket on *counter,
+// which that block resets).  Every block calls it after storing its
+// partials; the result is block-uniform.
+__device__ __forceinline__ bool last_block(int* counter) {
+    __shared__ bool last;
+    __threadfence();
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        last = atomicAdd(counter, 1) == (int)(gridDim.x * gridDim.y * gridDim.z) - 1;
+        if (last) *counter = 0;
+    }
+    __syncthreads();
+    if (last) __threadfence();
+    return last;
+}
+
+// In the last block: dst[k] += fixed-order total of part[k * stride + 0..nb)
+// for k < nv; wave w takes rows w, w + waves, ...
+__device__ __forceinline__ void fixed_totals(const double* part, int stride, int nb, int nv, double* dst) {
+    const int nw = blockDim.x / WAVE, w = threadIdx.x / WAVE;
+    for (int k = w; k < nv; k += nw) {
+        const double s = wave_fixed_sum((const volatile double*)(part + (size_t)k * stride), nb);
+        if (threadIdx.x % WAVE == 0) dst[k] += s;
+    }
+    __syncthreads();
+}
+
 __device__ __forceinline__ void atomic_add3(double* f, double x, double y, double z) {
     unsafeAtomicAdd(f + 0, x);
     unsafeAtomicAdd(f + 1, y);
@@ -367,6 +487,7 @@ int vdw_run(Ctx* c, const smd_vdw_params* p, int prec, smd_vdw_out* out, bool re
 int pairs_lj_ewald(Ctx* c, const double* sigma2, const double* eps4, double cutoff, double alpha,
                    double real_cut, int use_constraint, int lustig, int prec, bool store_forces = false);
 int pairs_lj(Ctx* c, double cutoff, int switch_on, int use_constraint, int lustig, int prec, bool store_forces);
+int vdw_enqueue(Ctx* c, const smd_vdw_params* p, int prec);
 int ewald_setup(Ctx* c, const smd_ewald_params* p);
 int ewald_real(Ctx* c, int prec, int lustig);
 int ewald_intra(Ctx* c, int lustig);
@@ -378,6 +499,9 @@ int shake_run(Ctx* c, const smd_integrator_params* p, bool fnh_on_device, double
 int kinetics_run(Ctx* c, bool rewrap, const smd_integrator_params* md = nullptr);
 int lf_atoms_run(Ctx* c, const smd_integrator_params* p, double fnh, bool md_step);
 int md_step_enqueue(Ctx* c, bool do_long);
+int md_head_enqueue(Ctx* c);
+int md_tail_enqueue(Ctx* c);
+int md_start_vtm(Ctx* c);
 int zero_terms(Ctx* c);
 int read_terms(Ctx* c, double* dst);
 int check_status(Ctx* c);

@@ -526,14 +526,16 @@ class Induction:
     :2401-2467) on the device: frames, permanent field, A mu = E0 by CG."""
 
     def __init__(self, ctx: Context, *, cutoff_ind, a_damp=0.39, damp_tij=True, do_min=True, tol=0.0,
-                 max_iter=0, guess=0, prec=L.PREC_FP64):
+                 max_iter=0, guess=0, prec=L.PREC_FP64, precond=False):
+        """precond=False: the reference's CG, stopping at |r|_2 < tol (0: its
+        1e-14).  precond=True: CG preconditioned by diag(alpha), stopping when
+        the RMS dipole residual |alpha_i r_i| over the sites is below tol
+        (e nm; the configs' 1e-6 D is 1e-6 * L.DEBYE_E_NM); prec MIXED takes
+        this solve with an FP32 coupling table."""
         self.ctx = ctx
-        s = ctx.system
-        dip = np.ascontiguousarray([s.types[t].dip for t in s.type], dtype=np.float64)
-        quad = np.ascontiguousarray([s.types[t].quad for t in s.type], dtype=np.float64)
-        pol = np.ascontiguousarray([s.types[t].polar for t in s.type], dtype=np.float64)
-        ctx.check(ctx.lib.smd_set_multipoles(ctx.h, _ptr(dip), _ptr(quad), _ptr(pol), ctx.n))
-        self.params = L.InduceParams(cutoff_ind, a_damp, int(damp_tij), int(do_min), tol, max_iter, guess, prec)
+        set_multipoles(ctx)
+        self.params = L.InduceParams(cutoff_ind, a_damp, int(damp_tij), int(do_min), tol, max_iter, guess, prec,
+                                     int(precond), int(precond))
 
     def induce(self):
         """Runs the solve; returns (iterations, converged, residual)."""
@@ -541,10 +543,14 @@ class Induction:
         self.ctx.check(self.ctx.lib.smd_induce(self.ctx.h, C.byref(self.params), C.byref(r)))
         return r.iterations, bool(r.converged), r.residual
 
+    terms_prec = L.PREC_FP64
+
     def setup_terms(self, *, alpha, real_cut, kcut, use_constraint=True, surface=False, torques=True,
-                    damp_amoeba=False):
+                    damp_amoeba=False, prec=L.PREC_FP64):
         """Induction ctor and set_up for the energy, force and torque terms of
-        the induced dipoles (src/forces/Induction.cpp:19-158, :592-652)."""
+        the induced dipoles (src/forces/Induction.cpp:19-158, :592-652); prec
+        MIXED evaluates their real-space pair math in FP32."""
+        self.terms_prec = prec
         p = L.IndfParams(alpha, real_cut, self.params.a_damp, int(use_constraint), int(surface), int(torques),
                          int(damp_amoeba), int(self.params.do_min))
         nk = C.c_int32()
@@ -555,14 +561,15 @@ class Induction:
         """Induction::compute after the dipole solve: energies, Lustig terms,
         virial; forces added to f_accum [n,3] when given."""
         o = L.IndfOut()
-        self.ctx.check(self.ctx.lib.smd_induction_compute(self.ctx.h, L.PREC_FP64, _fptr(f_accum), C.byref(o)))
+        self.ctx.check(self.ctx.lib.smd_induction_compute(self.ctx.h, self.terms_prec, _fptr(f_accum), C.byref(o)))
         d = {k: getattr(o, k) for k, _ in L.IndfOut._fields_ if k != "virial"}
         d["virial"] = np.array(o.virial[:])
         return d
 
     def compute_long(self, f_accum=None):
         o = L.MpLongOut()
-        self.ctx.check(self.ctx.lib.smd_induction_compute_long(self.ctx.h, L.PREC_FP64, _fptr(f_accum), C.byref(o)))
+        self.ctx.check(self.ctx.lib.smd_induction_compute_long(self.ctx.h, self.terms_prec, _fptr(f_accum),
+                                                               C.byref(o)))
         d = {k: getattr(o, k) for k, _ in L.MpLongOut._fields_ if k != "virial"}
         d["virial"] = np.array(o.virial[:])
         return d
@@ -579,6 +586,53 @@ class Induction:
         dip, quad, e0, mu = np.zeros((n, 3)), np.zeros((n, 6)), np.zeros((n, 3)), np.zeros((n, 3))
         self.ctx.check(self.ctx.lib.smd_get_induction(self.ctx.h, _ptr(dip), _ptr(quad), _ptr(e0), _ptr(mu), n))
         return dip, quad, e0, mu
+
+
+class PolarMD:
+    """Whole LFIntegrator::run step for rigid polarizable water on the device
+    (smd_polar_md_*): BuffvdWForce (or LJForce), Multipole, Induction with its
+    dipole solve, SHAKE and Nose-Hoover (src/integrators/LFIntegrator.cpp:
+    385-470, 561-722).  `induce` is an Induction whose solve parameters the
+    step uses; constraints are the water's."""
+
+    def __init__(self, ctx: Context, induce: "Induction", *, cutoff, cut_buff, alpha, kcut, dt, temperature,
+                 coupl=0.1, tol=1e-8, max_count=10000, real_cut=None, vdw="buff", switch_on=False,
+                 switch_dist=0.0, lustig=True, torques=True, surface=False, damp_amoeba=False, prec=L.PREC_FP64,
+                 use_graph=True, use_cells=True, total_steps=0, atom_thermo=True):
+        self.ctx = ctx
+        real_cut = cutoff if real_cut is None else real_cut
+        b2 = _f64(ctx.system.constraint_lengths2())
+        ctx.check(ctx.lib.smd_set_constraints(ctx.h, _ptr(b2), ctx.nmol))
+        cls = BuffvdWForce if vdw == "buff" else LJForce
+        self._vdw = cls(ctx, cutoff=cutoff, switch_on=switch_on, switch_dist=switch_dist, use_constraint=True,
+                        lustig=lustig, prec=prec)
+        p = L.PolarMdParams()
+        p.cutoff, p.cut_buff = cutoff, cut_buff
+        p.vdw = self._vdw.params
+        p.mp = L.MpParams(alpha, real_cut, 1, int(surface), int(torques))
+        p.mp_kcut = kcut
+        p.indf = L.IndfParams(alpha, real_cut, induce.params.a_damp, 1, int(surface), int(torques),
+                              int(damp_amoeba), int(induce.params.do_min))
+        p.ind_kcut = kcut
+        p.induce = induce.params
+        p.integ = L.IntegParams(dt, temperature, coupl, tol, max_count, 1, int(atom_thermo))
+        p.use_cells, p.prec, p.use_graph, p.total_steps = int(use_cells), prec, int(use_graph), int(total_steps)
+        self.params = p
+
+    def init(self):
+        """LFIntegrator::start: forces at t (with a full solve) and vtMinus."""
+        self.ctx.check(self.ctx.lib.smd_polar_md_init(self.ctx.h, C.byref(self.params)))
+
+    def run(self, nsteps):
+        r = L.PolarMdReport()
+        self.ctx.check(self.ctx.lib.smd_polar_md_run(self.ctx.h, int(nsteps), C.byref(r)))
+        return {k: getattr(r, k) for k, _ in L.PolarMdReport._fields_}
+
+    def time_kernel(self, name, reps=50):
+        """Average microseconds of one launch of "mp_real" or "ind_real"."""
+        us = C.c_double()
+        self.ctx.check(self.ctx.lib.smd_polar_time_kernel(self.ctx.h, name.encode(), int(reps), C.byref(us)))
+        return us.value
 
 
 class LearnedGuess:

@@ -324,6 +324,14 @@ def co2_buckingham_table():
     return A, B, C
 
 
+# The polarizable configs' sites: AMOEBA_TYPES with the buffered 14-7 on O
+# only (H eps = 0).
+POLAR_TYPES = [AtomType(mass=AMOEBA_TYPES[0].mass, charge=AMOEBA_TYPES[0].charge, eps=0.0,
+                        sigma=AMOEBA_TYPES[0].sigma, dip=AMOEBA_TYPES[0].dip, quad=AMOEBA_TYPES[0].quad,
+                        polar=AMOEBA_TYPES[0].polar, type_name="H", atom_name="Hydrogen"),
+               AMOEBA_TYPES[1]]
+
+
 def c2_water(seed: int = 2024) -> System:
     """4,096 rigid SPC/E-charge waters, 16^3 lattice, L = 4.9669 nm."""
     return water(4096, box=4.9669, seed=seed, lattice=(16, 16, 16), jitter=0.02, name="c2_water")
@@ -335,9 +343,7 @@ def polar_water(n_mol: int = 1024, seed: int = 2025) -> System:
     polarizabilities of AMOEBA_TYPES, buffered 14-7 on O only (H eps = 0).
     1,024 molecules: a seeded subset of an 11^3 lattice in L = 3.1289 nm;
     8,192 molecules: a 16 x 16 x 32 lattice in L = 6.2579 nm."""
-    h, o = AMOEBA_TYPES
-    types = [AtomType(mass=h.mass, charge=h.charge, eps=0.0, sigma=h.sigma, dip=h.dip, quad=h.quad, polar=h.polar,
-                      type_name=h.type_name, atom_name=h.atom_name), o]
+    types = list(POLAR_TYPES)
     if n_mol == 1024:
         return water(1024, box=3.1289, seed=seed, lattice=(11, 11, 11), subset=True, types=types,
                      name="polar_water1024")

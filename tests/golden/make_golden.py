@@ -224,6 +224,19 @@ CASES = [
      {"cutoff": 0.6, "cut_buff": 0.05, "alpha": 3.5, "kcut": S.kcut_for(27, 1.9), "dt": 0.002,
       "temperature": 280.0, "coupl": 0.05, "shake_tol": 1e-8, "max_count": 10000, "seed": 29,
       "atom_thermo": False}),
+    # rigid polarizable water (the AMOEBA-like site parameters, H eps = 0 as the
+    # polarizable configs set it): buffered 14-7, multipole Ewald with torques,
+    # induced dipoles by the reference's CG, then SHAKE and Nose-Hoover, 10
+    # leap-frog steps (LFIntegrator.cpp:442-473, 561-586)
+    ("amoeba216_md10_polar",
+     {"builder": "water", "args": {"n_mol": 216, "density_mol": 33.4, "seed": 67, "lattice": [6, 6, 6],
+                                   "types": "polar"}},
+     "md", "10",
+     {"cutoff": 0.7, "cut_buff": 0.05, "cutoff_ew": 0.7, "alpha": 3.0, "kcut": 14.0, "buff": True,
+      "switch_on": False, "multipole": True, "induction": True, "torques": True, "surface": False,
+      "constraint": True, "ewald": False, "ind_damp": True, "ind_min": True, "cutoff_ind": 0.7, "a_damp": 0.39,
+      "damp_amoeba": False, "dt": 0.002, "temperature": 300.0, "coupl": 0.1, "shake_tol": 1e-8,
+      "max_count": 10000, "seed": 71, "atom_thermo": True, "lustig": True}),
     # 864 LJ atoms on the fcc lattice at rho* = 0.8442 (sigma = eps = m = 1),
     # energy-shifted cutoff 2.5 sigma, unconstrained leap-frog with the
     # self-consistent Nose-Hoover velocity loop (LFIntegrator.cpp:587-643):

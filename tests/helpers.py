@@ -21,7 +21,7 @@ def build_system(spec):
     args = dict(spec["args"])
     if kind == "water":
         if "types" in args:
-            args["types"] = {"spce": S.SPCE_TYPES, "amoeba": S.AMOEBA_TYPES}[args["types"]]
+            args["types"] = {"spce": S.SPCE_TYPES, "amoeba": S.AMOEBA_TYPES, "polar": S.POLAR_TYPES}[args["types"]]
         if "lattice" in args:
             args["lattice"] = tuple(args["lattice"])
         return S.water(**args)

@@ -449,3 +449,27 @@ def test_md_autotuner_picks_a_candidate_and_keeps_the_physics():
         ctx2.close()
     assert rel_rms(outs[0], outs[1]) <= 1e-7   # mixed forms: FP32 rounding differs between pair-kernel forms
     ctx.close()
+
+
+@pytest.mark.parametrize("prec", [0, 1])
+def test_md_trajectory_is_bitwise_reproducible(prec):
+    """Two runs of the same 30 steps from the same state on fresh contexts
+    give identical bits: the sums that feed the trajectory (drift momentum,
+    kinetic energy, SHAKE virial, surface dipole) are fixed-order block
+    partials, the forces are written once per atom by one lane group."""
+    from swinmd_amd.forces import WaterMD
+    fx = Fixture("water512_md10")
+    c = fx.cfg
+    out = []
+    for _ in range(2):
+        ctx = _ctx(fx.system)
+        ctx.set_velocities(fx.vec("init_vel"))
+        md = WaterMD(ctx, cutoff=c["cutoff"], cut_buff=c["cut_buff"], alpha=c["alpha"], kcut=c["kcut"], dt=c["dt"],
+                     temperature=c["temperature"], coupl=c["coupl"], tol=c["shake_tol"], max_count=c["max_count"],
+                     prec=prec, surface=True)
+        md.init()
+        rep = md.run(30)
+        out.append((ctx.positions(), ctx.velocities(), rep["fnh"], rep["atom_ke2"]))
+    assert np.array_equal(out[0][0], out[1][0])
+    assert np.array_equal(out[0][1], out[1][1])
+    assert out[0][2] == out[1][2] and out[0][3] == out[1][3]

@@ -1,0 +1,120 @@
+"""The polarizable-water leap-frog step on the device (smd_polar_md_*) and the
+preconditioned induced-dipole solve (smd_induce with precond, FP64 and mixed)
+through the C ABI.
+
+Bars: the reference-method step (the reference's CG to |r| < 1e-14) matches
+the compiled reference's 10-step trajectory (positions 1e-10, velocities
+1e-8, energies 1e-8 relative); the preconditioned solves stop at the RMS
+dipole tolerance and their dipoles agree with the reference's to within a
+small multiple of it."""
+import numpy as np
+import pytest
+
+from tests.helpers import Fixture, rel, rel_rms
+
+pytestmark = pytest.mark.gpu
+
+
+def _ctx(system):
+    from swinmd_amd.forces import Context
+    return Context(system)
+
+
+def _polar_md(fx, ctx, use_graph, **induce_kw):
+    from swinmd_amd.forces import Induction, PolarMD
+    c = fx.cfg
+    ind = Induction(ctx, cutoff_ind=c["cutoff_ind"], a_damp=c["a_damp"], damp_tij=c["ind_damp"],
+                    do_min=c["ind_min"], **induce_kw)
+    md = PolarMD(ctx, ind, cutoff=c["cutoff"], cut_buff=c["cut_buff"], alpha=c["alpha"], kcut=c["kcut"],
+                 dt=c["dt"], temperature=c["temperature"], coupl=c["coupl"], tol=c["shake_tol"],
+                 max_count=c["max_count"], real_cut=c["cutoff_ew"], vdw="buff", switch_on=c["switch_on"],
+                 lustig=c["lustig"], torques=c["torques"], surface=c["surface"], damp_amoeba=c["damp_amoeba"],
+                 use_graph=use_graph, total_steps=int(fx.meta["totsteps"]), atom_thermo=c["atom_thermo"])
+    return ind, md
+
+
+@pytest.mark.parametrize("use_graph", [False, True])
+def test_polar_md_matches_reference(use_graph):
+    """start() + 10 steps: buffered 14-7, Multipole, Induction (the
+    reference's CG from Pi = 1 to |r| < 1e-14), SHAKE, Nose-Hoover."""
+    fx = Fixture("amoeba216_md10_polar")
+    ctx = _ctx(fx.system)
+    ctx.set_velocities(fx.vec("init_vel"))
+    _, md = _polar_md(fx, ctx, use_graph)
+    md.init()
+    rep = md.run(int(fx.extra))
+    assert rep["steps"] == int(fx.extra)
+    assert rel_rms(ctx.positions(), fx.vec("end_pos")) <= 1e-10
+    assert rel_rms(ctx.velocities(), fx.vec("end_vel")) <= 1e-8
+    assert rel_rms(ctx.forces(), fx.vec("end_force")) <= 1e-8
+    assert rel(rep["e_vdw"], fx.meta["e_lj"]) <= 1e-8
+    assert rel(rep["e_mp_real"], fx.meta["e_real"]) <= 1e-8
+    assert rel(rep["e_mp_long"], fx.meta["e_long"]) <= 1e-8
+    assert rel(rep["e_mp_self"], fx.meta["e_correct"]) <= 1e-8
+    assert rel(rep["e_mp_intra"], fx.meta["e_molCorrect"]) <= 1e-8
+    assert rel(0.5 * rep["atom_ke2"], fx.meta["e_atomKin"]) <= 1e-8
+    assert rel(0.5 * rep["mol_ke2"], fx.meta["e_molKin"]) <= 1e-8
+    assert rep["cg_iterations"] > 0 and rep["last_residual"] < 1e-14
+    assert rel_rms(md_real(ctx), fx.vec("end_real_pos")) <= 1e-10
+
+
+def md_real(ctx):
+    r = np.zeros((ctx.n, 3))
+    import ctypes as C
+    ctx.check(ctx.lib.smd_get_real_positions(ctx.h, r.ctypes.data_as(C.c_void_p), ctx.n))
+    return r
+
+
+@pytest.mark.parametrize("prec", [0, 1])
+def test_preconditioned_solve_reaches_the_reference_dipoles(prec):
+    """CG preconditioned by diag(alpha), stopping at an RMS dipole residual of
+    1e-6 D, FP64 or with the FP32 coupling table: the dipoles are within a
+    few times the tolerance of the reference's (|r| < 1e-14) and the solve
+    needs fewer iterations; a tight tolerance reaches them to 1e-9."""
+    from swinmd_amd import _lib as L
+    from swinmd_amd.forces import Induction
+    fx = Fixture("amoeba216_induction_cg")
+    c = fx.cfg
+    ctx = _ctx(fx.system)
+    ctx.set_pairlist(c["cutoff"], c["cut_buff"], True)
+    ind = Induction(ctx, cutoff_ind=c["cutoff_ind"], a_damp=c["a_damp"], damp_tij=c["ind_damp"], do_min=True)
+    it_ref, conv, _ = ind.induce()
+    mu_ref = ind.results()[3]
+    assert conv and rel_rms(mu_ref, fx.vec("ind_pi")) <= 1e-10
+    tol = 1e-6 * L.DEBYE_E_NM
+    ind2 = Induction(ctx, cutoff_ind=c["cutoff_ind"], a_damp=c["a_damp"], damp_tij=c["ind_damp"], do_min=True,
+                     tol=tol, precond=True, prec=prec)
+    it, conv, res = ind2.induce()
+    mu = ind2.results()[3]
+    assert conv and res < tol
+    err = np.sqrt(np.mean(np.sum((mu - mu_ref) ** 2, axis=1)))
+    assert err <= 20 * tol, (err, tol)
+    assert it < it_ref
+    if prec == 0:
+        ind2.params.tol = 1e-9 * tol
+        it_t, conv, _ = ind2.induce()
+        assert conv and rel_rms(ind2.results()[3], mu_ref) <= 1e-9
+    # every start reaches the same dipoles
+    for guess in (1, 2):
+        ind2.params.guess = guess
+        ind2.params.tol = tol
+        it_g, conv, _ = ind2.induce()
+        mu_g = ind2.results()[3]
+        assert conv and np.sqrt(np.mean(np.sum((mu_g - mu_ref) ** 2, axis=1))) <= 20 * tol
+
+
+def test_polar_md_preconditioned_steps_follow_the_reference_method():
+    """The step with the preconditioned FP64 solve at 1e-6 D, FP64, stays on
+    the reference-method trajectory over 10 steps to the accuracy the
+    tolerance allows, and needs fewer solve iterations per step."""
+    from swinmd_amd import _lib as L
+    fx = Fixture("amoeba216_md10_polar")
+    ctx = _ctx(fx.system)
+    ctx.set_velocities(fx.vec("init_vel"))
+    _, md = _polar_md(fx, ctx, True, tol=1e-6 * L.DEBYE_E_NM, precond=True, guess=2)
+    md.init()
+    rep = md.run(int(fx.extra))
+    assert rel_rms(ctx.positions(), fx.vec("end_pos")) <= 1e-7
+    assert rel_rms(ctx.velocities(), fx.vec("end_vel")) <= 1e-4
+    assert rel(rep["e_mp_real"], fx.meta["e_real"]) <= 1e-5
+    assert rep["cg_iterations"] / rep["steps"] < 40

@@ -40,7 +40,7 @@ def test_uniform_stream_is_the_reference_lcg():
 
 
 def _erfc_constants():
-    src = open(os.path.join(ROOT, "swinmd_amd", "csrc", "pair_forces.hip")).read()
+    src = open(os.path.join(ROOT, "swinmd_amd", "csrc", "smd_internal.h")).read()
     p = float(re.search(r"ERFC_P\s*=\s*([0-9.eE+-]+)f", src).group(1))
     body = re.search(r"ERFC_C\[8\]\s*=\s*\{([^}]*)\}", src).group(1)
     c = [float(x.strip().rstrip("f")) for x in body.split(",")]
