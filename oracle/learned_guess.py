@@ -236,6 +236,36 @@ def tmu(pos, polar, v, box, cutoff_ind, a_damp, damp=True):
     return out
 
 
+def tmu_rows(pos, polar, v, box, cutoff_ind, a_damp, rows, damp=True):
+    """(T v)_i of tmu for the sites `rows` only (the same sum, restricted to a
+    sample of rows so that a large system can be checked)."""
+    pos = np.asarray(pos, dtype=np.float64).reshape(-1, 3)
+    v = np.asarray(v, dtype=np.float64).reshape(-1, 3)
+    p = np.maximum(np.asarray(polar, dtype=np.float64), POLMIN)
+    box = np.asarray(box, dtype=np.float64)
+    rows = np.asarray(rows)
+    out = np.zeros((len(rows), 3))
+    for b0 in range(0, len(rows), 256):
+        idx = rows[b0:b0 + 256]
+        d = _min_image(pos[idx, None, :] - pos[None, :, :], box)
+        r2 = np.sum(d * d, axis=2)
+        use = (r2 <= cutoff_ind * cutoff_ind)
+        use[np.arange(len(idx)), idx] = False
+        r = np.sqrt(np.where(use, r2, 1.0))
+        r3, r5 = 1.0 / r ** 3, 1.0 / r ** 5
+        if damp:
+            u3 = r2 * r / np.sqrt(p[idx, None] * p[None, :])
+            e = np.exp(-a_damp * u3)
+            l3 = 1.0 - e
+            l5 = 1.0 - (1.0 + a_damp * u3) * e
+        else:
+            l3 = l5 = np.ones_like(r)
+        dv = np.sum(d * v[None, :, :], axis=2)
+        contrib = (3.0 * l5 * r5 * dv)[:, :, None] * d - (l3 * r3)[:, :, None] * v[None, :, :]
+        out[b0:b0 + len(idx)] = np.sum(np.where(use[:, :, None], contrib, 0.0), axis=1)
+    return out
+
+
 def descriptors(pos, mol_start, mol, q_scaled, polar, e0, box, cutoff_ind, a_damp):
     """[n, 32] raw descriptors (the order of the module docstring)."""
     F = site_frames(pos, mol_start)

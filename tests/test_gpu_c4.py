@@ -1,0 +1,154 @@
+"""The 8,192-molecule polarizable water -- the headline workload -- on the GPU
+at its full size (24,576 sites, 1,535 k-vectors at |n|^2 <= 81):
+
+ * buffered 14-7 and Multipole short and long range against the C oracle
+   (FP64, 1e-10 relative RMS);
+ * the device solve checked by an independent residual: on a sample of
+   sites, |Pi_i / alpha_i - (T Pi)_i - E0_i| with T Pi from the numpy
+   restatement of the Thole tensor (oracle/learned_guess.py, pinned to the
+   dense A of the C oracle), for the reference's CG (|r| < 1e-14) and for
+   the preconditioned solves at 1e-6 D RMS in FP64 and mixed precision;
+ * the learned guess trained on other frames reaching the reference start's
+   dipoles to 1e-10 (the same tolerance, fewer iterations);
+ * the mixed-precision evaluation (FP32 pair and k-space math, FP32-table
+   solve) within 1e-5 relative RMS force of the FP64 one."""
+import numpy as np
+import pytest
+
+from tests.helpers import rel, rel_rms
+
+pytestmark = pytest.mark.gpu
+
+CUT, BUFF, ALPHA, CUT_IND, A_DAMP = 0.9, 0.1, 3.5, 0.9, 0.39
+
+
+@pytest.fixture(scope="module")
+def c4():
+    from swinmd_amd import systems as S
+    return S.polar_water(8192)
+
+
+def _ctx(system):
+    from swinmd_amd.forces import Context
+    ctx = Context(system)
+    ctx.set_pairlist(CUT, BUFF, True)
+    ctx.compute_mol_centers()
+    return ctx
+
+
+def test_c4_vdw_and_multipole_against_oracle(c4, oracle_mod):
+    from swinmd_amd import systems as S
+    from swinmd_amd.forces import BuffvdWForce, Multipole
+    print("case c4 vdw + multipole", flush=True)
+    ctx = _ctx(c4)
+    n = c4.n_atoms
+    kcut = S.kcut_for(81, c4.box[0])
+    fv = np.zeros((n, 3))
+    vo = BuffvdWForce(ctx, cutoff=CUT, use_constraint=True).compute(fv)
+    mp = Multipole(ctx, alpha=ALPHA, real_cut=CUT, kcut=kcut, use_constraint=True, surface=False, torques=True)
+    assert mp.num_kvectors == 1535
+    fs, fl = np.zeros((n, 3)), np.zeros((n, 3))
+    out = mp.compute(fs)
+    lo = mp.compute_long(fl)
+    o = oracle_mod.Oracle(c4)
+    off, nb, _ = o.pairlist(CUT, BUFF, True)
+    goff, gnb = ctx.pairlist()
+    assert np.array_equal(goff, off) and np.array_equal(gnb, nb)
+    ofv, ovo = o.vdw(off, nb, "buff", cutoff=CUT, use_constraint=True)
+    assert rel_rms(fv, ofv) <= 1e-10
+    assert rel(vo["energy"], ovo["energy"]) <= 1e-10
+    dip, quad = o.local2global()
+    r = o.multipole(dip, quad, alpha=ALPHA, real_cut=CUT, cutoff=CUT, cut_buff=BUFF, kcut=kcut, use_constraint=True,
+                    surface=False, torques=True)
+    assert rel_rms(fs, r["short_force"]) <= 1e-10
+    assert rel_rms(fl, r["long_force"]) <= 1e-10
+    assert rel(out["real_energy"], r["real_energy"]) <= 1e-10
+    assert rel(lo["energy"], r["long_energy"]) <= 1e-10
+    ctx.close()
+
+
+def _residual_rms(ctx, ind, rows):
+    """RMS over `rows` of |alpha_i (E0_i - Pi_i / alpha_i + (T Pi)_i)| (e nm)."""
+    from oracle import learned_guess as G
+    s = ctx.system
+    _, _, e0, mu = ind.results()
+    polar = np.array([s.types[t].polar for t in s.type])
+    tpi = G.tmu_rows(ctx.positions(), polar, mu, np.asarray(s.box), CUT_IND, A_DAMP, rows)
+    a = np.maximum(polar[rows], 1e-8)[:, None]
+    r = e0[rows] - mu[rows] / a + tpi
+    return float(np.sqrt(np.mean(np.sum((a * r) ** 2, axis=1)))), float(np.sqrt(np.mean(np.sum(mu[rows] ** 2, axis=1))))
+
+
+def test_c4_solves_by_independent_residual_and_learned_start(c4):
+    from swinmd_amd import _lib as L
+    from swinmd_amd import systems as S
+    from swinmd_amd.forces import Induction, LearnedGuess
+    print("case c4 solves", flush=True)
+    rows = np.random.default_rng(8).choice(c4.n_atoms, 768, replace=False)
+    ctx = _ctx(c4)
+    ind = Induction(ctx, cutoff_ind=CUT_IND, a_damp=A_DAMP)
+    it_ref, conv, res = ind.induce()
+    assert conv and res < 1e-14
+    mu_ref = ind.results()[3]
+    rr, mu_rms = _residual_rms(ctx, ind, rows)
+    assert rr <= 1e-12 * mu_rms, (rr, mu_rms)
+    tol = 1e-6 * L.DEBYE_E_NM
+    for prec in (L.PREC_FP64, L.PREC_MIXED):
+        print(f"case c4 preconditioned solve prec {prec}", flush=True)
+        pind = Induction(ctx, cutoff_ind=CUT_IND, a_damp=A_DAMP, tol=tol, precond=True, guess=2, prec=prec)
+        it, conv, res = pind.induce()
+        assert conv and res < tol and it < it_ref
+        rr, _ = _residual_rms(ctx, pind, rows)
+        assert rr <= 3 * tol, (prec, rr, tol)
+    # the learned guess, trained on other seeds' frames
+    print("case c4 learned guess", flush=True)
+    net = LearnedGuess(ctx, seed=1)
+    ind.params.guess = 2
+    for sd in (1, 2):
+        ctx.set_positions(S.polar_water(8192, seed=sd).pos)
+        ctx.set_pairlist(CUT, BUFF, True)
+        assert ind.induce()[1]
+        net.add_frame()
+    net.train(epochs=30, batch=1024, lr=2e-3, lr_final=2e-5, seed=9)
+    ctx.set_positions(c4.pos)
+    ctx.set_pairlist(CUT, BUFF, True)
+    ind.params.guess = 3
+    it_l, conv, _ = ind.induce()
+    assert conv and rel_rms(ind.results()[3], mu_ref) <= 1e-10
+    assert it_l < it_ref
+    ctx.close()
+
+
+def test_c4_mixed_forces_within_1e5_of_fp64(c4):
+    """One whole polarizable evaluation (vdW, Multipole, the solve at 1e-6 D,
+    Induction) in each precision on the same coordinates."""
+    from swinmd_amd import _lib as L
+    from swinmd_amd import systems as S
+    from swinmd_amd.forces import BuffvdWForce, Induction, Multipole
+    kcut = S.kcut_for(81, c4.box[0])
+    tol = 1e-6 * L.DEBYE_E_NM
+    out = {}
+    for prec in (L.PREC_FP64, L.PREC_MIXED):
+        print(f"case c4 evaluation prec {prec}", flush=True)
+        ctx = _ctx(c4)
+        ctx.zero_forces()
+        BuffvdWForce(ctx, cutoff=CUT, use_constraint=True, prec=prec).compute()
+        mp = Multipole(ctx, alpha=ALPHA, real_cut=CUT, kcut=kcut, use_constraint=True, surface=False, torques=True,
+                       prec=prec)
+        e = {"mp_real": mp.compute()["real_energy"], "mp_long": mp.compute_long()["energy"]}
+        ind = Induction(ctx, cutoff_ind=CUT_IND, a_damp=A_DAMP, tol=tol, precond=True, guess=2, prec=prec)
+        assert ind.induce()[1]
+        ind.setup_terms(alpha=ALPHA, real_cut=CUT, kcut=kcut, use_constraint=True, surface=False, torques=True,
+                        prec=prec)
+        e["ind_real"] = ind.compute()["real_energy"]
+        e["ind_long"] = ind.compute_long()["energy"]
+        out[prec] = (ctx.forces(), e, ind.results()[3])
+        ctx.close()
+    f64, e64, mu64 = out[L.PREC_FP64]
+    f32, e32, mu32 = out[L.PREC_MIXED]
+    err = rel_rms(f32, f64)
+    print(f"mixed force error {err:.3e}", flush=True)
+    assert err <= 1e-5
+    assert rel_rms(mu32, mu64) <= 1e-5
+    for k in e64:
+        assert rel(e32[k], e64[k]) <= 1e-5, (k, e32[k], e64[k])

@@ -118,3 +118,30 @@ def test_polar_md_preconditioned_steps_follow_the_reference_method():
     assert rel_rms(ctx.velocities(), fx.vec("end_vel")) <= 1e-4
     assert rel(rep["e_mp_real"], fx.meta["e_real"]) <= 1e-5
     assert rep["cg_iterations"] / rep["steps"] < 40
+
+
+def test_polar_md_mixed_step_stays_on_the_fp64_trajectory():
+    """The mixed form of the step (FP32 pair and k-space math, FP32-table
+    solve) against the FP64 form with the same preconditioned solve at
+    1e-6 D, 10 steps from the same state."""
+    from swinmd_amd import _lib as L
+    fx = Fixture("amoeba216_md10_polar")
+    out = {}
+    for prec in (L.PREC_FP64, L.PREC_MIXED):
+        print(f"case polar md prec {prec}", flush=True)
+        ctx = _ctx(fx.system)
+        ctx.set_velocities(fx.vec("init_vel"))
+        _, md = _polar_md(fx, ctx, True, tol=1e-6 * L.DEBYE_E_NM, precond=True, guess=2, prec=prec)
+        md.params.prec = prec
+        md.init()
+        f0 = ctx.forces()
+        rep = md.run(10)
+        out[prec] = (f0, ctx.positions(), ctx.velocities(), rep)
+        ctx.close()
+    f64, p64, v64, r64 = out[L.PREC_FP64]
+    f32, p32, v32, r32 = out[L.PREC_MIXED]
+    assert rel_rms(f32, f64) <= 1e-5
+    assert rel_rms(p32, p64) <= 1e-6
+    assert rel_rms(v32, v64) <= 1e-2
+    assert rel(r32["e_mp_real"], r64["e_mp_real"]) <= 1e-5
+    assert rel(r32["e_ind_real"], r64["e_ind_real"]) <= 1e-4

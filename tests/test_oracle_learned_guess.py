@@ -178,3 +178,17 @@ def test_guess_inverts_targets():
     mean, sd = np.array([0.1, -0.2, 0.3]), np.array([2.0, 0.5, 1.5])
     back = G.guess(F, polar, e0, (t - mean) / sd, mean, sd)
     assert np.allclose(back, mu, atol=1e-15)
+
+
+def test_tmu_rows_equals_tmu_on_those_rows():
+    """The sampled T v used to check large solves is tmu's sum on its rows."""
+    import numpy as np
+    from oracle import learned_guess as G
+    rng = np.random.default_rng(5)
+    pos = rng.uniform(-0.8, 0.8, size=(300, 3))
+    polar = rng.uniform(0.0004, 0.0009, size=300)
+    v = rng.normal(size=(300, 3))
+    box = np.array([1.6, 1.6, 1.6])
+    rows = rng.choice(300, 40, replace=False)
+    full = G.tmu(pos, polar, v, box, 0.6, 0.39)
+    np.testing.assert_allclose(G.tmu_rows(pos, polar, v, box, 0.6, 0.39, rows), full[rows], rtol=1e-12, atol=1e-9)
