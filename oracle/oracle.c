@@ -887,12 +887,22 @@ void orc_ewald_recip(const orc_sys *s, int K, const double *kvec,
 /* (e) SHAKE and Nose-Hoover                                                  */
 /* ========================================================================== */
 
-int orc_shake(const orc_sys *s, double *pos, double *vt_minus,
-              double *velocity, double *displacement, const double *force,
-              const double *bond2, const double *mol_mom, double fnh,
-              int nose_hoover, double dt, double tol, int max_count,
-              double virial[9]) {
-    /* LFIntegrator::shake, NVT branch (src/integrators/LFIntegrator.cpp:1678-1828) */
+/* LFIntegrator::shake, NVT branch (src/integrators/LFIntegrator.cpp:1678-1828),
+ * with an optional molecular SLLOD shear along x with gradient in y (gamma,
+ * 1/ps; no counterpart in the reference).  With gamma = 0 every expression
+ * reduces to the reference's.  For gamma != 0: the peculiar unconstrained
+ * velocity takes -dt gamma P_y / M in x (P the molecule's peculiar momentum
+ * of the previous step, M its mass), the move adds the streaming velocity
+ * gamma Y of the molecule's centre (the same for all its atoms, so the bond
+ * vectors SHAKE corrects are unchanged), the new peculiar velocity is
+ * dr/dt - gamma Y, and the displacement used by the list trigger is the
+ * non-affine part: dr - dt gamma (y - d_y) in x, y - d_y being the atom's
+ * y when the list was built. */
+static int shake_core(const orc_sys *s, double *pos, double *vt_minus,
+                      double *velocity, double *displacement, const double *force,
+                      const double *bond2, const double *mol_mom, const double *com,
+                      double fnh, int nose_hoover, double dt, double tol, int max_count,
+                      double gamma, double virial[9]) {
     const double tol_2 = 2 * tol;
     for (int k = 0; k < 9; ++k) virial[k] = 0.0;
     int rc = 0;
@@ -903,6 +913,9 @@ int orc_shake(const orc_sys *s, double *pos, double *vt_minus,
         const int nBonds = ns;   /* cyclic water */
         v3 rOld[3], rNew[3], vtPlus[3];
         const v3 P = v3_ld(mol_mom + 3 * m);
+        double M = 0.0;
+        for (int i = 0; i < ns; ++i) M += s->mass[a0 + i];
+        const double Y = com ? com[3 * m + 1] : 0.0;
         for (int i = 0; i < ns; ++i) {
             const int id = a0 + i;
             const double mass_r = 1.0 / s->mass[id];
@@ -914,7 +927,10 @@ int orc_shake(const orc_sys *s, double *pos, double *vt_minus,
             } else {
                 vtPlus[i] = v3_add(v3_ld(vt_minus + 3 * id), v3_scale(dt * mass_r, F));
             }
-            rNew[i] = v3_add(rOld[i], v3_scale(dt, vtPlus[i]));
+            vtPlus[i].x -= dt * gamma * P.y / M;
+            v3 adv = vtPlus[i];
+            adv.x += gamma * Y;
+            rNew[i] = v3_add(rOld[i], v3_scale(dt, adv));
         }
         int count = 0, done = 0;
         while (!done && count < max_count) {
@@ -956,8 +972,11 @@ int orc_shake(const orc_sys *s, double *pos, double *vt_minus,
             const v3 p0 = v3_ld(pos + 3 * id);
             const v3 dr = v3_sub(rNew[i], p0);
             v3_st(pos + 3 * id, v3_add(p0, dr));
-            v3_st(displacement + 3 * id, v3_add(v3_ld(displacement + 3 * id), dr));
-            const v3 vp = v3_div(dr, dt);
+            v3 dn = dr;
+            dn.x -= dt * gamma * (p0.y - displacement[3 * id + 1]);
+            v3_st(displacement + 3 * id, v3_add(v3_ld(displacement + 3 * id), dn));
+            v3 vp = v3_div(dr, dt);
+            vp.x -= gamma * Y;
             const v3 vm = v3_ld(vt_minus + 3 * id);
             v3_st(velocity + 3 * id, v3_scale(0.5, v3_add(vp, vm)));
             v3_st(vt_minus + 3 * id, vp);
@@ -965,6 +984,15 @@ int orc_shake(const orc_sys *s, double *pos, double *vt_minus,
     }
     for (int k = 0; k < 9; ++k) virial[k] /= (dt * dt);
     return rc;
+}
+
+int orc_shake(const orc_sys *s, double *pos, double *vt_minus,
+              double *velocity, double *displacement, const double *force,
+              const double *bond2, const double *mol_mom, double fnh,
+              int nose_hoover, double dt, double tol, int max_count,
+              double virial[9]) {
+    return shake_core(s, pos, vt_minus, velocity, displacement, force, bond2, mol_mom, NULL, fnh,
+                      nose_hoover, dt, tol, max_count, 0.0, virial);
 }
 
 void orc_mol_kinetics(const orc_sys *s, const double *pos,
@@ -1288,6 +1316,157 @@ int orc_md_step(const orc_sys *s_in, const orc_md_params *p, orc_md_state *st) {
     /* per-step rewrap of molecules by centre (:897-910) */
     rewrap_molecules(&s, st->pos, st->com);
     st->step++;
+    return 0;
+}
+
+/* ========================================================================== */
+/* Sheared replicas: molecular SLLOD with Lees-Edwards boundaries.  The       */
+/* reference has no shear flow; these functions are this project's own       */
+/* definition, built from the reference's step above so that a zero shear    */
+/* rate gives the reference's step.                                           */
+/* ========================================================================== */
+
+int orc_ewald_kvectors_le(double kcut, const double box[3], double tilt, double alpha,
+                          double *kvec, double *kmod2, double *expk, int cap) {
+    /* Reciprocal lattice of the sheared cell a = (Lx,0,0), b = (tilt,Ly,0),
+     * c = (0,0,Lz): k = 2 pi (h/Lx, (k - h tilt/Lx)/Ly, l/Lz), every k with
+     * |k| < kcut, one of each +-k pair (h > 0, or h = 0 and k > 0, or h = k = 0
+     * and l > 0), in h, k, l order.  The set is that of the lattice, so it does
+     * not change when tilt moves by a multiple of Lx; at tilt = 0 it is
+     * EwaldForce::set_up's set in its order (src/forces/EwaldForce.cpp:198-257). */
+    const double kCut2 = kcut * kcut;
+    const double alphaR4 = 1.0 / (4.0 * (alpha * alpha));
+    const int nxMax = (int)(kcut * box[0] / (2 * PI));
+    const double nyR = kcut * box[1] / (2 * PI);
+    const int nzMax = (int)(kcut * box[2] / (2 * PI));
+    const double vx = 2.0 * PI / box[0], vy = 2.0 * PI / box[1], vz = 2.0 * PI / box[2];
+    int K = 0;
+    for (int nx = 0; nx <= nxMax; nx++) {
+        const double kx = nx * vx;
+        const double shift = nx * tilt / box[0];
+        const int lo = nx == 0 ? 0 : (int)floor(shift - nyR) - 1;
+        const int hi = (int)ceil(shift + nyR) + 1;
+        for (int ny = lo; ny <= hi; ny++) {
+            const double ky = (ny - shift) * vy;
+            for (int nz = ((nx == 0 && ny == 0) ? 1 : -nzMax); nz <= nzMax; nz++) {
+                const double kz = nz * vz;
+                const double k2 = kx * kx + ky * ky + kz * kz;
+                if (k2 < kCut2) {
+                    if (K < cap) {
+                        kvec[3 * K] = kx; kvec[3 * K + 1] = ky; kvec[3 * K + 2] = kz;
+                        kmod2[K] = k2;
+                        expk[K] = exp(-k2 * alphaR4) / k2;
+                    }
+                    ++K;
+                }
+            }
+        }
+    }
+    return K > cap ? -K : K;
+}
+
+int orc_sllod_start(const orc_sys *s_in, const orc_md_params *p, orc_md_state *st, double tilt, int kcap) {
+    /* orc_md_start at the current tilt: brute-force pair list (built here),
+     * sheared k set, first forces, vtMinus = v - dt/2 F/m. */
+    orc_sys s = *s_in;
+    s.pos = st->pos;
+    s.com = st->com;
+    s.tilt = tilt;
+    if (p->ewald) {
+        const int K = orc_ewald_kvectors_le(p->kcut, s.box, tilt, p->alpha, st->kvec, st->kmod2, st->expk, kcap);
+        if (K < 0) return -11;
+        st->K = K;
+    }
+    int used;
+    if (orc_pairlist(&s, p->cutoff, p->cut_buff, 0, st->off, st->nb, st->cap, &used) < 0) return -10;
+    for (int j = 0; j < 3 * s.n_atoms; ++j) st->displacement[j] = 0.0;
+    double vir[9];
+    md_forces(&s, p, st, 1, vir);
+    for (int i = 0; i < s.n_atoms; ++i) {
+        const double mass_r = 1.0 / s.mass[i];
+        for (int d = 0; d < 3; ++d)
+            st->vt_minus[3 * i + d] = st->velocity[3 * i + d] - 0.5 * p->dt * mass_r * st->force[3 * i + d];
+    }
+    for (int k = 0; k < 9; ++k) st->virial[k] = vir[k];
+    return 0;
+}
+
+int orc_sllod_step(const orc_sys *s_in, const orc_md_params *p, orc_md_state *st, double gamma,
+                   double *tilt, int64_t *since_build, int kcap) {
+    /* One step of orc_md_step (rigid water, SHAKE, molecular NH) under shear
+     * rate gamma (1/ps): streaming velocity gamma*y along x, Lees-Edwards
+     * images (the image across +y sits at x + tilt), tilt advanced by
+     * gamma*Ly*dt after the move and kept in [-Lx/2, Lx/2) (a move by Lx maps
+     * the lattice onto itself).  Velocities are peculiar.  The pair list is
+     * brute force; it is rebuilt when any atom's non-affine displacement d
+     * since the build has 2|d| + |gamma| rlist t > cut_buff (t the time since
+     * the build): the pair separations change by at most that much, since the
+     * affine part of a pair's motion is gamma t times its y separation. */
+    orc_sys s = *s_in;
+    s.pos = st->pos;
+    s.com = st->com;
+    s.tilt = *tilt;
+    const int n = s.n_atoms;
+    if (!p->constraint) return -12;
+    const double rlist = p->cutoff + p->cut_buff;
+    const double slack = p->cut_buff - fabs(gamma) * rlist * p->dt * (double)(*since_build);
+    const double drMax = slack * slack / 4.0;
+    int need = slack <= 0.0;
+    for (int j = 0; j < n && !need; ++j)
+        if (v3_len2(v3_ld(st->displacement + 3 * j)) > drMax) need = 1;
+    if (need) {
+        for (int j = 0; j < 3 * n; ++j) st->displacement[j] = 0.0;
+        rewrap_molecules(&s, st->pos, st->com);
+        int used;
+        int64_t np = orc_pairlist(&s, p->cutoff, p->cut_buff, 0, st->off, st->nb, st->cap, &used);
+        if (np < 0) return -10;
+        st->list_updates++;
+        *since_build = 0;
+    }
+    double vx = 0, vy = 0, vz = 0, mt = 0;
+    for (int j = 0; j < n; ++j) {
+        vx += st->vt_minus[3 * j] * s.mass[j];
+        vy += st->vt_minus[3 * j + 1] * s.mass[j];
+        vz += st->vt_minus[3 * j + 2] * s.mass[j];
+        mt += s.mass[j];
+    }
+    vx = vx / mt; vy = vy / mt; vz = vz / mt;
+    if (fabs(vx) > 1e-11 || fabs(vy) > 1e-11 || fabs(vz) > 1e-11)
+        for (int j = 0; j < n; ++j) {
+            st->vt_minus[3 * j] -= vx; st->vt_minus[3 * j + 1] -= vy; st->vt_minus[3 * j + 2] -= vz;
+            st->velocity[3 * j] -= vx; st->velocity[3 * j + 1] -= vy; st->velocity[3 * j + 2] -= vz;
+        }
+    if (p->ewald) {
+        const int K = orc_ewald_kvectors_le(p->kcut, s.box, s.tilt, p->alpha, st->kvec, st->kmod2, st->expk, kcap);
+        if (K < 0) return -11;
+        st->K = K;
+    }
+    double vir[9];
+    md_forces(&s, p, st, 1, vir);
+    double sv[9];
+    int rc = shake_core(&s, st->pos, st->vt_minus, st->velocity, st->displacement, st->force, p->bond2,
+                        st->mol_mom, st->com, st->fnh, p->nose_hoover, p->dt, p->tol, p->max_count, gamma, sv);
+    if (rc != 0) return rc;
+    orc_mol_kinetics(&s, st->pos, st->velocity, st->com, st->mol_mom, &st->atom_ke2, &st->mol_ke2);
+    if (p->nose_hoover) {
+        if (p->atom_thermo)
+            orc_nh_friction(st->atom_ke2, (double)(6 * s.n_mols - 3), p->temperature, p->coupl, p->dt,
+                            &st->fnh, &st->fnh_minus);
+        else
+            orc_nh_friction(st->mol_ke2, (double)(3 * s.n_mols - 3), p->temperature, p->coupl, p->dt,
+                            &st->fnh, &st->fnh_minus);
+    } else {
+        st->fnh = 0.0;
+    }
+    for (int k = 0; k < 9; ++k) st->virial[k] = vir[k] + sv[k];
+    double t = *tilt + gamma * s.box[1] * p->dt;
+    if (t >= 0.5 * s.box[0]) t -= s.box[0];
+    else if (t < -0.5 * s.box[0]) t += s.box[0];
+    *tilt = t;
+    s.tilt = t;
+    rewrap_molecules(&s, st->pos, st->com);
+    st->step++;
+    (*since_build)++;
     return 0;
 }
 

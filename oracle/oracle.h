@@ -227,6 +227,23 @@ int orc_md_step(const orc_sys *s_in, const orc_md_params *p, orc_md_state *st);
  * vtMinus = v - dt/2 F/m.  The pair list in st must be built already. */
 int orc_md_start(const orc_sys *s_in, const orc_md_params *p, orc_md_state *st);
 
+/* ---- sheared replicas (molecular SLLOD, Lees-Edwards) ---------------------
+ * No counterpart in the reference: the project's definition, reducing to
+ * orc_md_step (brute-force list) at zero shear rate.  orc_sys.tilt is the
+ * x offset of the image across +y. */
+
+/* Reciprocal vectors of the sheared cell with |k| < kcut, one per +-k pair.
+ * Returns K or -(needed) when cap is too small. */
+int orc_ewald_kvectors_le(double kcut, const double box[3], double tilt, double alpha,
+                          double *kvec, double *kmod2, double *expk, int cap);
+/* Builds the brute-force list and the k set (capacity kcap) at tilt, then
+ * does orc_md_start. */
+int orc_sllod_start(const orc_sys *s_in, const orc_md_params *p, orc_md_state *st, double tilt, int kcap);
+/* One step at shear rate gamma (1/ps).  *tilt and *since_build (steps since
+ * the list was built) are advanced.  Rigid water only (p->constraint = 1). */
+int orc_sllod_step(const orc_sys *s_in, const orc_md_params *p, orc_md_state *st, double gamma,
+                   double *tilt, int64_t *since_build, int kcap);
+
 
 /* ---- polarizable water ----------------------------------------------------- */
 

@@ -143,6 +143,11 @@ def lib():
                                            C.c_void_p]
         _lib.orc_md_step.argtypes = [P(OrcSys), P(MdParams), P(MdState)]
         _lib.orc_md_start.argtypes = [P(OrcSys), P(MdParams), P(MdState)]
+        _lib.orc_ewald_kvectors_le.argtypes = [C.c_double, C.c_void_p, C.c_double, C.c_double, C.c_void_p,
+                                               C.c_void_p, C.c_void_p, C.c_int]
+        _lib.orc_sllod_start.argtypes = [P(OrcSys), P(MdParams), P(MdState), C.c_double, C.c_int]
+        _lib.orc_sllod_step.argtypes = [P(OrcSys), P(MdParams), P(MdState), C.c_double, P(C.c_double),
+                                        P(C.c_int64), C.c_int]
         _lib.orc_local2global.argtypes = [P(OrcSys)] + [C.c_void_p] * 4
         _lib.orc_multipole_short.argtypes = [P(OrcSys), C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
                                              P(MpParams), C.c_void_p, P(MpOut)]
@@ -183,6 +188,7 @@ class Oracle:
         if self.excl.size == 0:
             self.excl = np.zeros(1, dtype=np.int32)
         self.box = np.array(s.box, dtype=np.float64)
+        self.tilt = 0.0
         self._sys = OrcSys()
         self._refresh()
 
@@ -195,6 +201,12 @@ class Oracle:
         o.excl_off, o.excl = _p(self.excl_off), _p(self.excl)
         for d in range(3):
             o.box[d] = self.box[d]
+        o.tilt = self.tilt
+
+    def set_tilt(self, tilt):
+        """Lees-Edwards offset of the image across +y (nm)."""
+        self.tilt = float(tilt)
+        self._sys.tilt = self.tilt
 
     @property
     def volume(self):
@@ -289,6 +301,16 @@ class Oracle:
         k2 = np.zeros(cap)
         ek = np.zeros(cap)
         K = lib().orc_ewald_kvectors(kcut, _p(self.box), alpha, _p(kv), _p(k2), _p(ek), cap)
+        if K < 0:
+            raise RuntimeError("too many k-vectors")
+        return kv[:3 * K].reshape(K, 3).copy(), k2[:K].copy(), ek[:K].copy()
+
+    def kvectors_le(self, kcut, alpha, tilt):
+        cap = 1 << 16
+        kv = np.zeros(3 * cap)
+        k2 = np.zeros(cap)
+        ek = np.zeros(cap)
+        K = lib().orc_ewald_kvectors_le(kcut, _p(self.box), tilt, alpha, _p(kv), _p(k2), _p(ek), cap)
         if K < 0:
             raise RuntimeError("too many k-vectors")
         return kv[:3 * K].reshape(K, 3).copy(), k2[:K].copy(), ek[:K].copy()
@@ -508,3 +530,68 @@ def md_from_cfg(o: "Oracle", cfg, nsteps, velocity=None):
             "com": com, "start_vel": start_vel, "fnh": st.fnh, "e_lj": st.e_lj, "e_real": st.e_real,
             "e_long": st.e_long, "e_intra": st.e_intra, "e_self": st.e_self, "atom_ke2": st.atom_ke2,
             "mol_ke2": st.mol_ke2, "list_updates": st.list_updates, "step": st.step}
+
+
+KCAP_LE = 1 << 14
+
+
+def sllod_run(o: "Oracle", cfg, gamma, nsteps, velocity=None, tilt=0.0, on_step=None):
+    """Runs the oracle's sheared-replica integrator (orc_sllod_start, then
+    nsteps of orc_sllod_step at shear rate gamma, 1/ps) for rigid water with
+    Ewald and molecular NH (cfg["nose_hoover"] False switches the thermostat
+    off).  on_step(k, state_dict) is called after every step with views of the
+    live arrays.  Returns the final state and per-step records."""
+    s = o.s
+    n, nm = s.n_atoms, s.n_mols
+    if velocity is None:
+        velocity = o.init_velocity(cfg["temperature"], cfg.get("seed", 1), per_atom=False, ndf=3 * nm - 3)
+    p = MdParams()
+    p.cutoff, p.cut_buff = cfg["cutoff"], cfg["cut_buff"]
+    p.alpha, p.kcut = cfg["alpha"], cfg["kcut"]
+    p.dt, p.temperature, p.coupl, p.tol = cfg["dt"], cfg["temperature"], cfg["coupl"], cfg.get("shake_tol", 1e-8)
+    p.max_count, p.use_cells, p.long_steps, p.surface = cfg.get("max_count", 10000), 0, 1, 0
+    p.nose_hoover, p.atom_thermo = int(cfg.get("nose_hoover", True)), 0
+    p.constraint, p.ewald, p.switch_on = 1, 1, int(cfg.get("switch_on", False))
+    sig2, eps4 = s.lj_table()
+    bond2 = np.ascontiguousarray(s.constraint_lengths2())
+    p.n_types, p.sigma2, p.eps4, p.bond2 = len(s.types), _p(sig2), _p(eps4), _p(bond2)
+    pos = o.pos.copy()
+    vel = np.ascontiguousarray(velocity, dtype=np.float64).copy()
+    bufs = {k: np.zeros((n, 3)) for k in ("vtm", "disp", "force")}
+    com = o.com.copy()
+    mom = np.zeros((nm, 3))
+    off = np.zeros(n + 1, dtype=np.int32)
+    cap = max(n * 512, 1024)
+    nbuf = np.zeros(cap, dtype=np.int32)
+    kv, k2, ek = np.zeros(3 * KCAP_LE), np.zeros(KCAP_LE), np.zeros(KCAP_LE)
+    st = MdState()
+    st.pos, st.vt_minus, st.velocity = _p(pos), _p(bufs["vtm"]), _p(vel)
+    st.displacement, st.force, st.com, st.mol_mom = _p(bufs["disp"]), _p(bufs["force"]), _p(com), _p(mom)
+    st.off, st.nb, st.cap = _p(off), _p(nbuf), cap
+    st.kvec, st.kmod2, st.expk, st.K = _p(kv), _p(k2), _p(ek), 0
+    keep = (sig2, eps4, bond2)
+    sys_ = OrcSys()
+    C.pointer(sys_)[0] = o._sys
+    rc = lib().orc_sllod_start(C.byref(sys_), C.byref(p), C.byref(st), float(tilt), KCAP_LE)
+    if rc:
+        raise RuntimeError(f"orc_sllod_start {rc}")
+    t = C.c_double(tilt)
+    since = C.c_int64(0)
+    start_vel = vel.copy()
+    rec = []
+    for k in range(nsteps):
+        rc = lib().orc_sllod_step(C.byref(sys_), C.byref(p), C.byref(st), float(gamma), C.byref(t),
+                                  C.byref(since), KCAP_LE)
+        if rc:
+            raise RuntimeError(f"orc_sllod_step {rc} at step {k}")
+        u = st.e_lj + st.e_real + st.e_long + st.e_intra + st.e_self
+        rec.append((t.value, u, st.atom_ke2, st.mol_ke2, st.fnh, st.list_updates, st.K))
+        if on_step is not None:
+            on_step(k, {"pos": pos, "com": com, "vel": vel, "mom": mom, "tilt": t.value, "virial": list(st.virial)})
+    del keep
+    return {"pos": pos, "vel": vel, "vtm": bufs["vtm"], "force": bufs["force"], "com": com, "mom": mom,
+            "disp": bufs["disp"], "start_vel": start_vel, "tilt": t.value, "fnh": st.fnh,
+            "fnh_minus": st.fnh_minus, "list_updates": st.list_updates, "since_build": since.value,
+            "e_lj": st.e_lj, "e_real": st.e_real, "e_long": st.e_long, "e_intra": st.e_intra,
+            "e_self": st.e_self, "atom_ke2": st.atom_ke2, "mol_ke2": st.mol_ke2, "K": st.K,
+            "virial": np.array(list(st.virial)), "records": np.array(rec)}

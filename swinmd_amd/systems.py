@@ -342,13 +342,16 @@ def polar_water(n_mol: int = 1024, seed: int = 2025) -> System:
     (O-H 0.1 nm, H-O-H 109.47 deg): site charges, dipoles, quadrupoles and
     polarizabilities of AMOEBA_TYPES, buffered 14-7 on O only (H eps = 0).
     1,024 molecules: a seeded subset of an 11^3 lattice in L = 3.1289 nm;
-    8,192 molecules: a 16 x 16 x 32 lattice in L = 6.2579 nm."""
+    8,192 molecules: a seeded subset of a 21^3 lattice in L = 6.2579 nm
+    (0.298 nm spacing; a 16 x 16 x 32 lattice in the cubic box would put
+    neighbours 0.196 nm apart along z, a start no MD step survives)."""
     types = list(POLAR_TYPES)
     if n_mol == 1024:
         return water(1024, box=3.1289, seed=seed, lattice=(11, 11, 11), subset=True, types=types,
                      name="polar_water1024")
     if n_mol == 8192:
-        return water(8192, box=6.2579, seed=seed, lattice=(16, 16, 32), types=types, name="polar_water8192")
+        return water(8192, box=6.2579, seed=seed, lattice=(21, 21, 21), subset=True, types=types,
+                     name="polar_water8192")
     raise ValueError("polar_water: 1024 or 8192 molecules")
 
 
