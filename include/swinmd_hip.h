@@ -188,6 +188,18 @@ int smd_ewald_setup(smd_ctx* ctx, const smd_ewald_params* p, int32_t* n_kvec);
 int smd_ewald_compute(smd_ctx* ctx, int prec, double* f_accum, smd_ewald_out* out);
 /* EwaldForce::compute_long (:178-196) -> reciprocal_term (:529-698). */
 int smd_ewald_compute_long(smd_ctx* ctx, int prec, double* f_accum, smd_ewald_long_out* out);
+/* smd_ewald_compute_long without any device allocation.  No counterpart in
+ * the reference: for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * The reciprocal-space scratch (per-atom phases and moments, per-k
+ * coefficients, structure-factor partials) lives in one static device array
+ * of SMD_STATIC_SCRATCH_BYTES per GPU, held until the process ends; a system
+ * that needs more returns SMD_ECAPACITY.  Forces are added to the context's
+ * device force accumulator (smd_get_forces).  Every call on a GPU shares that
+ * GPU's array: calls that overlap on one GPU (other streams, threads or
+ * contexts) race. */
+#define SMD_STATIC_SCRATCH_BYTES ((size_t)16 << 20)
+int smd_ewald_compute_long_static(smd_ctx* ctx, int prec, smd_ewald_long_out* out);
 
 /* ---- (e) SHAKE and Nose-Hoover ----------------------------------------- */
 typedef struct {

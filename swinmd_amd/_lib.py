@@ -260,6 +260,7 @@ SIGNATURES = {
     "smd_ewald_setup": (C.c_int, [_P, C.POINTER(EwaldParams), C.POINTER(C.c_int32)]),
     "smd_ewald_compute": (C.c_int, [_P, C.c_int, _P, C.POINTER(EwaldOut)]),
     "smd_ewald_compute_long": (C.c_int, [_P, C.c_int, _P, C.POINTER(EwaldLongOut)]),
+    "smd_ewald_compute_long_static": (C.c_int, [_P, C.c_int, C.POINTER(EwaldLongOut)]),
     "smd_set_constraints": (C.c_int, [_P, _P, C.c_int64]),
     "smd_set_vt_minus": (C.c_int, [_P, _P, C.c_int64]),
     "smd_get_vt_minus": (C.c_int, [_P, _P, C.c_int64]),

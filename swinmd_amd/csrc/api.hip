@@ -233,6 +233,12 @@ extern "C" int smd_ctx_create(smd_ctx** out, const smd_topology* t, int device) 
         delete c;
         return SMD_EDEVICE;
     }
+    // this device's address of the allocation-free entry point's scratch
+    if (static_scratch_init(c) != SMD_OK) {
+        release(c);
+        delete c;
+        return SMD_EDEVICE;
+    }
     *out = c;
     return SMD_OK;
 }

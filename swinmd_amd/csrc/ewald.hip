@@ -32,6 +32,8 @@
 #include <algorithm>
 #include <climits>
 
+#include <mutex>
+
 #include "smd_internal.h"
 
 namespace smd {
@@ -651,6 +653,32 @@ int ewald_recip(Ctx* c, int prec, int lustig, unsigned passes) {
     return ewald_recip_with(c, prec, lustig, passes, RecipScratch{c->d_kf, c->d_sfac_part, c->d_e1});
 }
 
+// ---- the allocation-free form's scratch -------------------------------------
+// One static device array per GPU holds the reciprocal-space scratch of
+// smd_ewald_compute_long_static.  In host code `scratch` is only a host-side
+// stand-in: its device address, one per GPU, comes from hipGetSymbolAddress
+// with that GPU current (static_scratch_init, called by smd_ctx_create) and is
+// the only form ever passed to a kernel or a copy.
+constexpr size_t STATIC_SCRATCH_DOUBLES = SMD_STATIC_SCRATCH_BYTES / sizeof(double);
+static __device__ __align__(256) double scratch[STATIC_SCRATCH_DOUBLES];
+
+constexpr int MAX_DEVICES = 64;
+static std::mutex g_scratch_mu;
+static double* g_scratch_ptr[MAX_DEVICES] = {};
+
+int static_scratch_init(Ctx* c) {
+    if (c->device < 0 || c->device >= MAX_DEVICES) return fail(c, SMD_EINVAL, "device %d out of range", c->device);
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[c->device]) {
+        void* ptr = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !ptr) return hip_fail(c, e, "hipGetSymbolAddress(scratch)");
+        g_scratch_ptr[c->device] = static_cast<double*>(ptr);
+    }
+    c->static_scratch = g_scratch_ptr[c->device];
+    return SMD_OK;
+}
+
 static int ewald_recip_with(Ctx* c, int prec, int lustig, unsigned passes, const RecipScratch& sc) {
     if (!c->ewald_ready) return fail(c, SMD_ESTATE, "Ewald not set up");
     RecipArgs a;
@@ -805,6 +833,38 @@ extern "C" int smd_ewald_compute_long(smd_ctx* c, int prec, double* f_accum, smd
     if ((rc = recip_long_out(c, out))) return rc;
     if (f_accum) return fetch_forces_into(c, f_accum);
     return SMD_OK;
+}
+
+// Clears the term accumulators with a kernel rather than a memset node.
+__global__ __launch_bounds__(256) void k_clear_terms(double* t, int n) {
+    for (int i = threadIdx.x; i < n; i += blockDim.x) t[i] = 0.0;
+}
+
+// The same passes with their scratch in the static array: no device
+// allocation.  Every atom's phases and moments, the per-k coefficients and
+// the (chunk, k) partials are written before they are read in each call, so
+// nothing in the array needs clearing; the term accumulators (context state)
+// are cleared by a fill kernel on the context stream.
+extern "C" int smd_ewald_compute_long_static(smd_ctx* c, int prec, smd_ewald_long_out* out) {
+    if (!c || !out) return SMD_EINVAL;
+    if (!c->ewald_ready) return fail(c, SMD_ESTATE, "Ewald not set up");
+    if (prec != SMD_PREC_FP64 && prec != SMD_PREC_MIXED) return fail(c, SMD_EINVAL, "bad precision %d", prec);
+    if (!c->static_scratch) return fail(c, SMD_ESTATE, "static scratch not resolved for device %d", c->device);
+    auto up = [](size_t v) { return (v + 31) / 32 * 32; };   // 256-byte segments
+    const size_t n_e1 = up(e1_am_offset(c->n) + 4 * (size_t)c->n);
+    const size_t n_kf = up(2 * (size_t)c->K);
+    const size_t n_part = up((size_t)NSUM * c->K * c->sfac_chunks);
+    const size_t need = n_e1 + n_kf + n_part;
+    if (need > STATIC_SCRATCH_DOUBLES)
+        return fail(c, SMD_ECAPACITY,
+                    "static scratch holds %zu bytes; %d atoms and %d k-vectors need %zu (use smd_ewald_compute_long)",
+                    (size_t)SMD_STATIC_SCRATCH_BYTES, c->n, c->K, need * sizeof(double));
+    const RecipScratch sc{c->static_scratch + n_e1, c->static_scratch + n_e1 + n_kf, c->static_scratch};
+    hipLaunchKernelGGL(k_clear_terms, dim3(1), dim3(256), 0, c->stream, c->d_terms, (int)T_COUNT);
+    SMD_HIP(c, hipGetLastError());
+    int rc = ewald_recip_with(c, prec, 1, RECIP_ALL, sc);
+    if (rc) return rc;
+    return recip_long_out(c, out);
 }
 
 // compute_long's outputs from the term accumulators (reads them back).

@@ -102,6 +102,7 @@ struct Ctx {
     // block stores its sums, and a fixed-order total replaces FP64 atomics, so
     // the trajectory does not depend on block scheduling.
     double* d_part = nullptr;
+    double* static_scratch = nullptr;   // this device's address of ewald.hip's static scratch array
     double* part_vom = nullptr;   // [4][nb_atoms]: k_step_head
     double* part_kin = nullptr;   // [KIN_PARTS][nb_mols64]: kinetics kernels
     double* part_sh = nullptr;    // [9][nb_mols]: k_shake
@@ -495,6 +496,7 @@ int ewald_surface(Ctx* c);
 // Reciprocal-space passes: phase + structure factors, per-k reduction, forces.
 enum : unsigned { RECIP_SFAC = 1u, RECIP_REDUCE = 2u, RECIP_FORCE = 4u, RECIP_ALL = 7u };
 int ewald_recip(Ctx* c, int prec, int lustig, unsigned passes = RECIP_ALL);
+int static_scratch_init(Ctx* c);
 int shake_run(Ctx* c, const smd_integrator_params* p, bool fnh_on_device, double fnh, bool md_step = false);
 int kinetics_run(Ctx* c, bool rewrap, const smd_integrator_params* md = nullptr);
 int lf_atoms_run(Ctx* c, const smd_integrator_params* p, double fnh, bool md_step);
