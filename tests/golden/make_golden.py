@@ -237,6 +237,19 @@ CASES = [
       "constraint": True, "ewald": False, "ind_damp": True, "ind_min": True, "cutoff_ind": 0.7, "a_damp": 0.39,
       "damp_amoeba": False, "dt": 0.002, "temperature": 300.0, "coupl": 0.1, "shake_tol": 1e-8,
       "max_count": 10000, "seed": 71, "atom_thermo": True, "lustig": True}),
+    # the 1,024-molecule polarizable water of the headline workload's small
+    # form (a seeded subset of an 11^3 lattice in L = 3.1289 nm, polarizable
+    # site parameters): 3 steps from the lattice start with its parameters
+    # (cutoff 0.9 nm, Ewald alpha 3.5 /nm, |n|^2 <= 81, Thole a = 0.39)
+    ("polar1024_md3",
+     {"builder": "water", "args": {"n_mol": 1024, "box": 3.1289, "seed": 2025, "lattice": [11, 11, 11],
+                                   "subset": True, "types": "polar"}},
+     "md", "3",
+     {"cutoff": 0.9, "cut_buff": 0.1, "cutoff_ew": 0.9, "alpha": 3.5, "kcut": 18.128713661985863, "buff": True,
+      "switch_on": False, "multipole": True, "induction": True, "torques": True, "surface": False,
+      "constraint": True, "ewald": False, "ind_damp": True, "ind_min": True, "cutoff_ind": 0.9, "a_damp": 0.39,
+      "damp_amoeba": False, "dt": 0.002, "temperature": 300.0, "coupl": 0.1, "shake_tol": 1e-8,
+      "max_count": 10000, "seed": 73, "atom_thermo": True, "lustig": True}),
     # 864 LJ atoms on the fcc lattice at rho* = 0.8442 (sigma = eps = m = 1),
     # energy-shifted cutoff 2.5 sigma, unconstrained leap-frog with the
     # self-consistent Nose-Hoover velocity loop (LFIntegrator.cpp:587-643):

@@ -58,6 +58,27 @@ def test_polar_md_matches_reference(use_graph):
     assert rel_rms(md_real(ctx), fx.vec("end_real_pos")) <= 1e-10
 
 
+def test_polar_md_1024_from_the_lattice_start_matches_reference():
+    """start() + 3 steps of the 1,024-molecule polarizable water from its
+    lattice start (cutoff 0.9 nm, |n|^2 <= 81, cell list), the reference's
+    CG, against the compiled reference."""
+    fx = Fixture("polar1024_md3")
+    print("case polar1024_md3", flush=True)
+    ctx = _ctx(fx.system)
+    ctx.set_velocities(fx.vec("init_vel"))
+    _, md = _polar_md(fx, ctx, True)
+    md.init()
+    rep = md.run(int(fx.extra))
+    assert rep["steps"] == int(fx.extra)
+    assert rel_rms(ctx.positions(), fx.vec("end_pos")) <= 1e-10
+    assert rel_rms(ctx.velocities(), fx.vec("end_vel")) <= 1e-8
+    assert rel_rms(ctx.forces(), fx.vec("end_force")) <= 1e-8
+    assert rel(rep["e_vdw"], fx.meta["e_lj"]) <= 1e-8
+    assert rel(rep["e_mp_real"], fx.meta["e_real"]) <= 1e-8
+    assert rel(rep["e_mp_long"], fx.meta["e_long"]) <= 1e-8
+    assert rel(0.5 * rep["atom_ke2"], fx.meta["e_atomKin"]) <= 1e-8
+
+
 def md_real(ctx):
     r = np.zeros((ctx.n, 3))
     import ctypes as C
