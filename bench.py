@@ -202,7 +202,9 @@ def train_learned_guess(ctx, ind, nmol, seeds, epochs=60, batch=1024):
 
 
 # The polarizable-water step (the 8,192-molecule headline workload and its
-# 1,024-molecule form): buffered 14-7 on O, multipole Ewald with torques,
+# 1,024-molecule form): buffered 14-7 on O and H (the reference example's
+# parameters; with O alone the model collapses within ~20 steps), multipole
+# Ewald with torques,
 # Thole-damped induced dipoles solved to 1e-6 D RMS, SHAKE, Nose-Hoover.
 POLAR_EQUIL = 400
 # algorithmic work per unique pair inside the real-space cutoff (the
@@ -286,10 +288,16 @@ def bench_polar_md(args):
     net_state = (net.params(), net.norm())
     tctx.close()
 
-    # equilibrate from the lattice start on the mixed path, then start every
-    # path from the same coordinates and velocities
+    # the lattice start relaxes under the rigid point-charge model (same
+    # coordinates, masses and constraints), then under the polarizable model
+    # on the mixed path; every timed path starts from that state
+    log(f"point-charge relaxation {args.polar_pre} steps")
+    pctx, pmd = build_md(S.point_charge_twin(system), L.PREC_MIXED, system.pos, vel0)
+    pmd.run(args.polar_pre)
+    pos_pre, vel_pre = pctx.positions(), pctx.velocities()
+    pctx.close()
     log(f"equilibrating {args.polar_equil} steps")
-    ctx, ind, md = make(system.pos, vel0, L.PREC_MIXED, 3, net_state)
+    ctx, ind, md = make(pos_pre, vel_pre, L.PREC_MIXED, 3, net_state)
     md.run(args.polar_equil)
     pos_eq, vel_eq = ctx.positions(), ctx.velocities()
     ctx.close()
@@ -341,7 +349,8 @@ def bench_polar_md(args):
         "warmup": args.warmup, "ms_per_step": round(1e3 * head["secs"] / args.steps, 4), "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f32 pair and A v math, f64 accumulation and solve",
         "data": f"synthetic: seeded lattice of {nmol} polarizable waters with random orientations (seed 2025), "
-                f"reference Maxwell velocities (seed 1), {args.polar_equil} untimed equilibration steps",
+                f"reference Maxwell velocities (seed 1), {args.polar_pre} untimed point-charge relaxation steps, "
+                f"then {args.polar_equil} untimed polarizable steps",
         "config": {"workload": f"polarizable water MD step (buffered 14-7 + multipole Ewald + Thole induced dipoles "
                                f"+ SHAKE + Nose-Hoover), {nmol} molecules", "molecules": nmol, "atoms": n,
                    "box_nm": float(system.box[0]), "cutoff_nm": POLAR["cutoff"], "buffer_nm": POLAR["cut_buff"],
@@ -393,15 +402,185 @@ def bench_polar_md(args):
     print(json.dumps(out), flush=True)
 
 
+# Batched sheared replicas: 64 rigid-water replicas of uniform-random size in
+# 256..2048 molecules at 1 g/cm^3, shear rates log-spaced 1e-4..1e-2 1/fs,
+# Lees-Edwards images, Nose-Hoover 300 K, one graph launch per step.
+NEMD = dict(replicas=64, n_min=256, n_max=2048, seed=2026, g_min=0.1, g_max=10.0, rho=33.43, cutoff=0.9,
+            cut_buff=0.08, alpha=3.5)
+# algorithmic work (counting rule above): LJ + Ewald real per unique pair in
+# the cutoff; per (atom, k): phase 5, sin+cos 20 and the two sums 4 in the
+# structure-factor pass, phase 5, sin+cos 20 and the force 10 in the force pass
+FLOP_PER_PAIR_LJ_EWALD = 99
+FLOP_PER_ATOM_K_SFAC = 29
+FLOP_PER_ATOM_K_FORCE = 35
+
+
+def nemd_replicas():
+    """The 64 replicas (systems, shear rates in 1/ps) of the batched workload."""
+    from swinmd_amd import systems as S
+    rng = np.random.default_rng(NEMD["seed"])
+    sizes = rng.integers(NEMD["n_min"], NEMD["n_max"] + 1, NEMD["replicas"])
+    gam = np.logspace(np.log10(NEMD["g_min"]), np.log10(NEMD["g_max"]), NEMD["replicas"])
+    systems = [S.water(int(n), density_mol=NEMD["rho"], seed=1000 + r, subset=True) for r, n in enumerate(sizes)]
+    return systems, gam
+
+
+def nemd_cpu_baseline(systems, gam, velocities):
+    """The oracle's sheared-replica step (plain C, one core) on every replica
+    once: the CPU time of one batch step."""
+    from oracle import oracle as O
+    from swinmd_amd import systems as S
+    total = 0.0
+    for s, g, v in zip(systems, gam, velocities):
+        cfg = dict(cutoff=NEMD["cutoff"], cut_buff=NEMD["cut_buff"], alpha=NEMD["alpha"],
+                   kcut=S.kcut_for(N2MAX, s.box[0]), dt=MD["dt"], temperature=MD["temperature"], coupl=MD["coupl"])
+        t = {}
+
+        def on_step(k, st, t=t):
+            t[k] = time.perf_counter()
+
+        t0 = time.perf_counter()
+        O.sllod_run(O.Oracle(s), cfg, float(g), 2, velocity=v, on_step=on_step)
+        total += t[1] - t[0]
+        del t0
+    return {"value": round(1.0 / total, 6), "unit": "steps/s", "cores": 1, "kind": "port",
+            "sample": f"one step of each of the {len(systems)} replicas by the oracle's orc_sllod_step (plain C, "
+                      f"gcc -O2, one core): {total:.2f} s per batch step"}
+
+
+def bench_nemd(args):
+    """Batched sheared replicas: steps/s of the whole batch, mixed precision
+    against FP64, both from the same equilibrated state."""
+    from swinmd_amd import _lib as L
+    from swinmd_amd import ensemble
+    from swinmd_amd.nemd import ShearBatch
+    systems, gam = nemd_replicas()
+    R = len(systems)
+    vels = [ensemble.init_velocity(s, MD["temperature"], seed=r + 1, per_atom=False) for r, s in enumerate(systems)]
+    n_mol = sum(s.n_mols for s in systems)
+
+    def make(prec):
+        b = ShearBatch(systems, gam, cutoff=NEMD["cutoff"], cut_buff=NEMD["cut_buff"], alpha=NEMD["alpha"],
+                       dt=MD["dt"], temperature=MD["temperature"], coupl=MD["coupl"], shake_tol=MD["tol"],
+                       prec=prec)
+        if args.no_graph:
+            b.set_tiles(use_graph=0)
+        return b
+
+    log(f"equilibrating {args.nemd_equil} steps ({R} replicas, {n_mol} molecules)")
+    b = make(L.PREC_MIXED)
+    b.set_state([s.pos for s in systems], vels)
+    b.start()
+    b.run(args.nemd_equil)
+    st = b.state()
+    tilt_eq = st["tilt"]
+    pos_eq, vel_eq = st["pos"], st["vel"]
+    b.close()
+    res = {}
+    for name, prec in (("mixed", L.PREC_MIXED), ("fp64", L.PREC_FP64)):
+        if name == "fp64" and args.no_fp64:
+            continue
+        b = make(prec)
+        b.set_state(pos_eq, vel_eq, tilt_eq)
+        b.start()
+        f0 = b.state()["force"]
+        tune = None
+        if not args.no_autotune:
+            tiles, table = b.autotune(reps=10)
+            tune = {"chosen": tiles, "candidates": table}
+            log(f"{name} autotune: {tiles}")
+        b.run(args.warmup)
+        r0 = b.reports()
+        t0 = time.perf_counter()
+        b.run(args.steps)
+        secs = time.perf_counter() - t0
+        r1 = b.reports()
+        iso = {k: b.time_kernel(k, 20) for k in ("pairs", "sfac", "kforce", "kfin", "kvec", "list")}
+        b.run(1)   # the forced rebuild after the list timing
+        res[name] = dict(secs=secs, force=f0, tune=tune, iso=iso,
+                         list_updates=sum(a["list_updates"] - c["list_updates"] for a, c in zip(r1, r0)),
+                         kvec=sum(a["n_kvec"] for a in r1), reports=r1, state=b.state())
+        log(f"{name}: {args.steps / secs:.2f} batch steps/s")
+        b.close()
+    head = res["mixed"]
+    steps_s = args.steps / head["secs"]
+    n_atoms = 3 * n_mol
+    # unique pairs inside the cutoff and (atom, k) products, from the state
+    pairs = 0
+    atom_k = 0
+    pos_all = head["state"]["pos"]
+    off = 0
+    for s, rep, tl in zip(systems, head["reports"], head["state"]["tilt"]):
+        p = pos_all[off:off + s.n_atoms]
+        off += s.n_atoms
+        Lb = s.box[0]
+        o = p[s.type == 1]
+        d = o[None, :, :] - o[:, None, :]
+        ny = np.round(d[..., 1] / Lb)
+        d[..., 1] -= ny * Lb
+        d[..., 0] -= ny * tl
+        d[..., 0] -= Lb * np.round(d[..., 0] / Lb)
+        d[..., 2] -= Lb * np.round(d[..., 2] / Lb)
+        r2 = np.sum(d * d, axis=2)
+        n_oo = int((np.count_nonzero(r2 < NEMD["cutoff"] ** 2) - len(o)) // 2)
+        pairs += 9 * n_oo   # every site pair of two molecules whose O are in the cutoff, to within a few percent
+        atom_k += s.n_atoms * rep["n_kvec"]
+    iso = head["iso"]
+    models = {"pairs": FLOP_PER_PAIR_LJ_EWALD * pairs, "sfac": FLOP_PER_ATOM_K_SFAC * atom_k,
+              "kforce": FLOP_PER_ATOM_K_FORCE * atom_k}
+    top = max(models, key=lambda k: iso[k])
+    ach = models[top] / (iso[top] * 1e-6) / 1e12
+    out = {
+        "metric": METRIC, "value": round(steps_s, 3), "unit": "steps/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(1e3 * head["secs"] / args.steps, 4), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32 pair and k-space math, f64 accumulation",
+        "data": f"synthetic: {R} seeded lattice waters with random orientations, molecular Maxwell velocities, "
+                f"{args.nemd_equil} untimed sheared equilibration steps",
+        "config": {"workload": f"batched sheared replicas (molecular SLLOD + Lees-Edwards, rigid 3-site water, "
+                               f"LJ + Ewald + SHAKE + Nose-Hoover), {R} replicas, {n_mol} molecules",
+                   "replicas": R, "molecules": n_mol, "atoms": n_atoms,
+                   "sizes": [int(s.n_mols) for s in systems], "shear_rates_per_ps": [round(g, 5) for g in gam],
+                   "cutoff_nm": NEMD["cutoff"], "buffer_nm": NEMD["cut_buff"], "alpha_per_nm": NEMD["alpha"],
+                   "n2max": N2MAX, "dt_ps": MD["dt"], "thermostat": "Nose-Hoover 300 K on the molecular "
+                   "peculiar kinetic energy", "parallelism": "single GPU",
+                   "step_launch": "direct launches" if args.no_graph else "one hipGraph launch per batch step"},
+        "ns_per_day_per_replica": round(steps_s * MD["dt"] * 86400.0 / 1000.0, 3),
+        "molecule_steps_per_s": round(steps_s * n_mol, 1),
+        "list_updates_per_replica_step": round(head["list_updates"] / (args.steps * R), 4),
+        "autotune": head["tune"],
+        "kernels_us_isolated": {k: round(v, 2) for k, v in iso.items()},
+    }
+    if "fp64" in res:
+        ref = res["fp64"]
+        out["fp64_steps_per_s"] = round(args.steps / ref["secs"], 3)
+        out["speedup_vs_fp64"] = round(ref["secs"] / head["secs"], 3)
+        d = head["force"] - ref["force"]
+        out["force_rel_rms_vs_fp64"] = float(np.sqrt(np.sum(d * d) / np.sum(ref["force"] ** 2)))
+        out["fp64_kernels_us_isolated"] = {k: round(v, 2) for k, v in ref["iso"].items()}
+    out["roofline"] = {"kernel": top, "bound": "valu", "achieved": round(ach, 3), "peak": PEAK_FP32_TFLOPS,
+                       "unit": "TFLOP/s", "frac": round(ach / PEAK_FP32_TFLOPS, 5),
+                       "traffic": measured_traffic("nemd64").get(top), "traffic_unit": "bytes/launch",
+                       "flop_per_launch": models[top], "avg_us": round(iso[top], 2), "pairs": pairs,
+                       "atom_k": atom_k}
+    if not args.no_cpu_baseline:
+        log("batched replicas cpu baseline (oracle, 1 core, one step of every replica)")
+        out["cpu_baseline"] = nemd_cpu_baseline(systems, gam, vels)
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
-    ap.add_argument("--workload", default="polar8192", choices=["polar8192", "polar1024", "water4096"],
+    ap.add_argument("--workload", default="polar8192", choices=["polar8192", "polar1024", "water4096", "nemd64"],
                     help="polar8192: the polarizable-water MD step on 8,192 molecules, mixed precision + "
                          "learned guess against plain FP64 (default); polar1024: the same on 1,024 molecules; "
-                         "water4096: the rigid point-charge water MD step")
+                         "water4096: the rigid point-charge water MD step; nemd64: 64 sheared replicas "
+                         "stepped together")
+    ap.add_argument("--nemd-equil", type=int, default=500, help="untimed steps of the batched replicas")
     ap.add_argument("--polar-equil", type=int, default=POLAR_EQUIL,
                     help="untimed equilibration steps of the polarizable workloads")
     ap.add_argument("--polar-profile-steps", type=int, default=10)
+    ap.add_argument("--polar-pre", type=int, default=1500,
+                    help="point-charge relaxation steps before the polarizable equilibration")
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=2000)
     ap.add_argument("--warmup", type=int, default=200)
@@ -428,6 +607,9 @@ def main():
     L.load()   # raises if libswinmd_hip.so is missing
     if args.copy_calibration:
         copy_calibration(args.copy_calibration)
+    if args.workload == "nemd64":
+        bench_nemd(args)
+        return
     if args.workload != "water4096":
         bench_polar_md(args)
         return

@@ -778,6 +778,82 @@ int smd_write_pressure_tensor(const char* path, int append, int64_t counter1, co
                               const double mol_pressure[9], double atom_shear_sum, double mol_shear_sum,
                               int64_t counter, char* err, size_t errlen);
 
+/* ---- batched sheared replicas (planar Couette flow) ----------------------
+ * No counterpart in the reference, which has neither shear flow nor replica
+ * batching: independent replicas of rigid 3-site water (site order H, O, H;
+ * LJ on O-O, point-charge Ewald, SHAKE, Nose-Hoover on the molecular
+ * peculiar kinetic energy), each at its own shear rate, stepped together.
+ * The step of one replica is LFIntegrator::run (src/integrators/
+ * LFIntegrator.cpp:278-941) for this model with molecular SLLOD: the
+ * streaming velocity gamma*y along x at each molecule's centre, the term
+ * -gamma P_y/M along x in the peculiar update, Lees-Edwards images (the
+ * image across +y is offset by the tilt along x), the tilt advanced by
+ * gamma*L*dt and kept in [-L/2, L/2), the reciprocal lattice of the sheared
+ * cell with |k| < kcut, a brute-force pair list rebuilt when
+ * 2|d| + |gamma| rlist t > cut_buff (d the non-affine displacement, t the
+ * time since the build).  Zero shear rate gives the reference's step.
+ * Units are the reference's (nm, ps, amu, kJ/mol, charges scaled by
+ * sqrt(138.93548501)).  One step is one hipGraph launch for all replicas. */
+typedef struct smd_batch smd_batch;
+typedef struct {
+    int32_t n_replicas;
+    const int32_t* n_mols;        /* [R] molecules per replica */
+    const double* box;            /* [R] cubic box edge (nm) */
+    const double* shear_rate;     /* [R] gamma (1/ps) */
+    const double* kcut;           /* [R] reciprocal-space cutoff (1/nm) */
+    double mass_h, mass_o;        /* site masses (amu) */
+    double q_h, q_o;              /* scaled site charges */
+    double sigma_o, eps_o;        /* LJ between O sites (nm, kJ/mol); H sites have none */
+    double r_oh, r_hh;            /* constraint lengths (nm) */
+    double cutoff, cut_buff, alpha;   /* LJ and real-space cutoff, list buffer, Ewald alpha */
+    double dt, temperature, coupl, shake_tol;
+    int32_t max_count;            /* SHAKE sweeps */
+    int32_t nose_hoover;          /* 1: NH on the molecular kinetic energy; 0: none */
+    int32_t prec;                 /* smd_prec: FP64, or MIXED (FP32 pair and k-space math, FP64 sums) */
+    int32_t nb_cap;               /* pair-list row capacity (0: from the density) */
+} smd_batch_params;
+typedef struct {
+    double tilt, fnh;             /* after the last step */
+    double e_lj, e_real, e_long, e_intra, e_self;   /* at the start of the last step */
+    double atom_ke2, mol_ke2;     /* sum m v.v and sum P.P/M after the last step */
+    double virial[9];             /* ensemble virial: -(LJ + real) + reciprocal + constraint */
+    double mol_kin[9];            /* sum_m P_a P_b / M (peculiar) */
+    double mol_correction[9];     /* sum_i F_i,b (r_i - R_m(i))_a at the start of the last step */
+    int64_t steps, list_updates;
+    int32_t n_kvec, max_row;      /* k-vectors of the last step; longest list row at the last build */
+} smd_batch_report;
+typedef struct {
+    int32_t pair_lanes;           /* lanes per atom row of the pair kernel */
+    int32_t sfac_chunk;           /* atoms per block of the structure-factor pass */
+    int32_t kforce_tile;          /* k-vectors per LDS tile of the reciprocal force pass */
+    int32_t overlap;              /* 1: pair and reciprocal passes on two graph branches */
+    int32_t use_graph;            /* 1: one captured graph launch per step; 0: direct launches */
+} smd_batch_tiles;
+int smd_batch_create(smd_batch** out, const smd_batch_params* p, int device);
+void smd_batch_destroy(smd_batch* b);
+const char* smd_batch_last_error(const smd_batch* b);
+/* Positions, velocities [3 * total atoms], replica after replica, and the
+ * replicas' tilts [R]; molecules must be whole (centres are taken from
+ * them).  Resets the thermostat and momenta as Ensemble's start does. */
+int smd_batch_set_state(smd_batch* b, const double* pos, const double* vel, const double* tilt);
+/* LFIntegrator::start for every replica: list, k set, forces, vtMinus. */
+int smd_batch_start(smd_batch* b);
+/* nsteps steps (one graph launch each); checks the device status at the end. */
+int smd_batch_run(smd_batch* b, int64_t nsteps);
+int smd_batch_get_state(smd_batch* b, double* pos, double* vel, double* vt_minus, double* force, double* tilt);
+int smd_batch_get_reports(smd_batch* b, smd_batch_report* rep);   /* [R] */
+int smd_batch_set_tiles(smd_batch* b, const smd_batch_tiles* t);
+int smd_batch_get_tiles(const smd_batch* b, smd_batch_tiles* t);
+/* Times every candidate tile shape on the current state (each kernel alone,
+ * HIP events, reps launches) and keeps the fastest.  times_us receives
+ * n_cand rows of (pair_lanes, sfac_chunk, kforce_tile, pair us, sfac us,
+ * kforce us) when not NULL; *n_cand is its capacity in rows on entry. */
+int smd_batch_autotune(smd_batch* b, int reps, double* times_us, int32_t* n_cand);
+/* Average microseconds of one launch of "list", "pairs", "kvec", "sfac",
+ * "kfin", "kforce" or "shake" on the current state (the list pass is timed
+ * as a full rebuild; the state is left as it was). */
+int smd_batch_time_kernel(smd_batch* b, const char* name, int reps, double* avg_us);
+
 #ifdef __cplusplus
 }
 #endif

@@ -157,6 +157,26 @@ class PolarMdReport(C.Structure):
                                            "mol_ke2", "fnh")]
 
 
+class BatchParams(C.Structure):
+    _fields_ = [("n_replicas", C.c_int32), ("n_mols", C.c_void_p), ("box", C.c_void_p), ("shear_rate", C.c_void_p),
+                ("kcut", C.c_void_p)] + \
+               [(k, C.c_double) for k in ("mass_h", "mass_o", "q_h", "q_o", "sigma_o", "eps_o", "r_oh", "r_hh",
+                                           "cutoff", "cut_buff", "alpha", "dt", "temperature", "coupl",
+                                           "shake_tol")] + \
+               [(k, C.c_int32) for k in ("max_count", "nose_hoover", "prec", "nb_cap")]
+
+
+class BatchReport(C.Structure):
+    _fields_ = [(k, C.c_double) for k in ("tilt", "fnh", "e_lj", "e_real", "e_long", "e_intra", "e_self",
+                                           "atom_ke2", "mol_ke2")] + \
+               [("virial", C.c_double * 9), ("mol_kin", C.c_double * 9), ("mol_correction", C.c_double * 9),
+                ("steps", C.c_int64), ("list_updates", C.c_int64), ("n_kvec", C.c_int32), ("max_row", C.c_int32)]
+
+
+class BatchTiles(C.Structure):
+    _fields_ = [(k, C.c_int32) for k in ("pair_lanes", "sfac_chunk", "kforce_tile", "overlap", "use_graph")]
+
+
 _NAME80 = C.c_char * 80
 _CONFIG_NAMES = ("sys_data_file", "coordinate_file", "velocity_file", "result_file", "restart_file",
                  "trajectory_file", "dump_file", "mol_trajectory_file", "mol_velocity_file", "thermo_type")
@@ -274,6 +294,18 @@ SIGNATURES = {
     "smd_polar_md_init": (C.c_int, [_P, C.POINTER(PolarMdParams)]),
     "smd_polar_md_run": (C.c_int, [_P, C.c_int64, C.POINTER(PolarMdReport)]),
     "smd_polar_time_kernel": (C.c_int, [_P, C.c_char_p, C.c_int, C.POINTER(C.c_double)]),
+    "smd_batch_create": (C.c_int, [C.POINTER(_P), C.POINTER(BatchParams), C.c_int]),
+    "smd_batch_destroy": (None, [_P]),
+    "smd_batch_last_error": (C.c_char_p, [_P]),
+    "smd_batch_set_state": (C.c_int, [_P, _P, _P, _P]),
+    "smd_batch_start": (C.c_int, [_P]),
+    "smd_batch_run": (C.c_int, [_P, C.c_int64]),
+    "smd_batch_get_state": (C.c_int, [_P, _P, _P, _P, _P, _P]),
+    "smd_batch_get_reports": (C.c_int, [_P, _P]),
+    "smd_batch_set_tiles": (C.c_int, [_P, C.POINTER(BatchTiles)]),
+    "smd_batch_get_tiles": (C.c_int, [_P, C.POINTER(BatchTiles)]),
+    "smd_batch_autotune": (C.c_int, [_P, C.c_int, _P, C.POINTER(C.c_int32)]),
+    "smd_batch_time_kernel": (C.c_int, [_P, C.c_char_p, C.c_int, C.POINTER(C.c_double)]),
     "smd_md_run": (C.c_int, [_P, C.c_int64, C.POINTER(MdReport)]),
     "smd_md_pressure": (C.c_int, [_P, C.POINTER(MdPressure)]),
     "smd_set_real_positions": (C.c_int, [_P, _P, C.c_int64]),

@@ -324,8 +324,11 @@ def co2_buckingham_table():
     return A, B, C
 
 
-# The polarizable configs' sites: AMOEBA_TYPES with the buffered 14-7 on O
-# only (H eps = 0).
+# The polarizable sites with the buffered 14-7 on O only (H eps = 0), as the
+# polarizable config names them.  Without a repulsive H site this model
+# collapses: hydrogens fall into neighbouring oxygens within ~20 steps, even
+# from an equilibrated liquid (the MD workloads therefore use AMOEBA_TYPES,
+# the reference example's full set; this set stays for force-level fixtures).
 POLAR_TYPES = [AtomType(mass=AMOEBA_TYPES[0].mass, charge=AMOEBA_TYPES[0].charge, eps=0.0,
                         sigma=AMOEBA_TYPES[0].sigma, dip=AMOEBA_TYPES[0].dip, quad=AMOEBA_TYPES[0].quad,
                         polar=AMOEBA_TYPES[0].polar, type_name="H", atom_name="Hydrogen"),
@@ -337,15 +340,17 @@ def c2_water(seed: int = 2024) -> System:
     return water(4096, box=4.9669, seed=seed, lattice=(16, 16, 16), jitter=0.02, name="c2_water")
 
 
-def polar_water(n_mol: int = 1024, seed: int = 2025) -> System:
+def polar_water(n_mol: int = 1024, seed: int = 2025, o_only: bool = False) -> System:
     """Polarizable water at 1.0 g/cm^3 with the 3-site water's geometry
-    (O-H 0.1 nm, H-O-H 109.47 deg): site charges, dipoles, quadrupoles and
-    polarizabilities of AMOEBA_TYPES, buffered 14-7 on O only (H eps = 0).
+    (O-H 0.1 nm, H-O-H 109.47 deg): site charges, dipoles, quadrupoles,
+    polarizabilities and buffered 14-7 of the reference's example
+    (AMOEBA_TYPES, test/sysData.in:12-13); o_only drops the H 14-7 term
+    (POLAR_TYPES), a model that does not survive MD.
     1,024 molecules: a seeded subset of an 11^3 lattice in L = 3.1289 nm;
     8,192 molecules: a seeded subset of a 21^3 lattice in L = 6.2579 nm
     (0.298 nm spacing; a 16 x 16 x 32 lattice in the cubic box would put
     neighbours 0.196 nm apart along z, a start no MD step survives)."""
-    types = list(POLAR_TYPES)
+    types = list(POLAR_TYPES if o_only else AMOEBA_TYPES)
     if n_mol == 1024:
         return water(1024, box=3.1289, seed=seed, lattice=(11, 11, 11), subset=True, types=types,
                      name="polar_water1024")
@@ -353,6 +358,14 @@ def polar_water(n_mol: int = 1024, seed: int = 2025) -> System:
         return water(8192, box=6.2579, seed=seed, lattice=(21, 21, 21), subset=True, types=types,
                      name="polar_water8192")
     raise ValueError("polar_water: 1024 or 8192 molecules")
+
+
+def point_charge_twin(system: System) -> System:
+    """The same water (coordinates, topology, masses) with the rigid
+    point-charge sites (SPCE_TYPES): the model that relaxes a lattice start
+    before a polarizable run."""
+    import dataclasses
+    return dataclasses.replace(system, types=list(SPCE_TYPES), name=system.name + "_point_charge")
 
 
 def kcut_for(n2max: float, L: float) -> float:
