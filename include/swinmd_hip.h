@@ -283,8 +283,9 @@ int smd_induce(smd_ctx* ctx, const smd_induce_params* p, smd_induce_report* repo
  * smd_induce; any pointer may be NULL. */
 int smd_get_induction(smd_ctx* ctx, double* dip, double* quad, double* e0, double* mu, int64_t n_atoms);
 /* Average microseconds of one launch of the CG's A v (the coupling-table
- * product of the last converged smd_induce, on its dipoles), reps launches
- * back to back; the solution is left unchanged.  No counterpart in the
+ * product of the last minimised smd_induce or polarizable step, on the
+ * current dipoles), reps launches back to back; the solution is left
+ * unchanged.  SMD_ESTATE before any minimised solve.  No counterpart in the
  * reference. */
 int smd_induce_time_apply(smd_ctx* ctx, int reps, double* avg_us);
 
@@ -593,8 +594,9 @@ int smd_md_time_family(smd_ctx* ctx, const char* family, int reps, double* avg_u
  * "cg_batch" (1..64, default 16): CG iterations per batch, i.e. between the
  * host's checks of the stop flag.
  * "pair_lanes": lanes sharing one atom's row
- * in the pair kernels, 4, 8, 16 or 32, or 0 (default: 16 for FP32 pair
- * math, 32 for FP64).  Drops a captured MD
+ * in the pair kernels (both the packed mixed kernel and the general one),
+ * 4, 8, 16, 32 or 64, or 0 (default: 16 for FP32 pair math, 32 for FP64).
+ * Drops a captured MD
  * graph.  SMD_EINVAL for an unknown name or value. */
 int smd_set_tuning(smd_ctx* ctx, const char* name, int value);
 

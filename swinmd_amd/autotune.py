@@ -8,7 +8,9 @@ forms against each other:
   recip_overlap   k-space on a second stream beside the list and pair
                   kernels (1) or on the step's stream (0);
   md_fused_shake  SHAKE and the kinetic and pressure sums as one kernel (1)
-                  or two (0).
+                  or two (0);
+  pair_lanes      lanes per atom row of the pair kernel (0: 16 for the
+                  mixed form, 32 for FP64).
 The tuner times the step under every combination on the live system, for
 its size, cutoff and k-set on this GPU, and keeps the fastest.  The
 trajectory runs on while it measures, so a few hundred steps of the run are
@@ -18,7 +20,7 @@ decomposition is fixed by MPI.
 import itertools
 import time
 
-DEFAULT_KNOBS = {"pair_packed": (1, 0), "recip_overlap": (1, 0), "md_fused_shake": (1, 0)}
+DEFAULT_KNOBS = {"pair_packed": (1, 0), "recip_overlap": (1, 0), "md_fused_shake": (1, 0), "pair_lanes": (0, 8, 32)}
 
 
 def tune_md_step(md, knobs=None, steps=200, warmup=40, rounds=2):

@@ -1,5 +1,6 @@
 // api.hip -- context lifecycle, state transfer, errors and profiling for the
 // C ABI declared in include/swinmd_hip.h.
+#include <algorithm>
 #include <cstdarg>
 #include <cstring>
 
@@ -40,6 +41,79 @@ int check_status(Ctx* c) {
     return SMD_OK;
 }
 
+// One wave per value: lane l adds partials l, l + 64, ... in order (the
+// loads unrolled, so several are in flight), then a shuffle tree.
+__global__ __launch_bounds__(1024) void k_red_finish(const double* __restrict__ part, int cap, int nb, int nv,
+                                                     double* dst) {
+    const int w = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
+    for (int k = w; k < nv; k += blockDim.x / WAVE) {
+        const double* p = part + (size_t)k * cap;
+        double s = 0.0;
+#pragma unroll 8
+        for (int i = lane; i < nb; i += WAVE) s += p[i];
+        s = wave_sum(s);
+        if (lane == 0) dst[k] += s;
+    }
+}
+
+int red_finish(Ctx* c, int site, unsigned nblocks, int nv, double* dst) {
+    if ((int)nblocks > c->red.cap)
+        return fail(c, SMD_ECAPACITY, "reduction over %u blocks exceeds the arena's %d", nblocks, c->red.cap);
+    if (nblocks == 0) return SMD_OK;
+    if (c->red_defer) {
+        for (const RedPending& q : c->red_pending)
+            if (q.site == site) return fail(c, SMD_ESTATE, "reduction site %d queued twice in one step", site);
+        c->red_pending.push_back(RedPending{site, nblocks, nv, dst});
+        return SMD_OK;
+    }
+    hipLaunchKernelGGL(k_red_finish, dim3(1), dim3(1024), 0, c->stream, c->red.base + (size_t)site * RED_NV * c->red.cap,
+                       c->red.cap, (int)nblocks, nv, dst);
+    SMD_HIP(c, hipGetLastError());
+    return SMD_OK;
+}
+
+constexpr int FLUSH_MAX = 24;
+struct FlushList {
+    const double* part[FLUSH_MAX];
+    double* dst[FLUSH_MAX];
+    int nb[FLUSH_MAX];
+    int nv[FLUSH_MAX];
+    int cap;
+};
+
+// Block b adds queued finish b: one wave per value, as k_red_finish.
+__global__ __launch_bounds__(1024) void k_red_flush(FlushList f) {
+    const int b = blockIdx.x;
+    const int w = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
+    for (int k = w; k < f.nv[b]; k += blockDim.x / WAVE) {
+        const double* p = f.part[b] + (size_t)k * f.cap;
+        double s = 0.0;
+#pragma unroll 8
+        for (int i = lane; i < f.nb[b]; i += WAVE) s += p[i];
+        s = wave_sum(s);
+        if (lane == 0) f.dst[b][k] += s;
+    }
+}
+
+int red_flush(Ctx* c) {
+    const size_t n = c->red_pending.size();
+    if (n == 0) return SMD_OK;
+    if (n > (size_t)FLUSH_MAX) return fail(c, SMD_ESTATE, "%zu queued reductions exceed %d", n, FLUSH_MAX);
+    FlushList f{};
+    f.cap = c->red.cap;
+    for (size_t q = 0; q < n; ++q) {
+        const RedPending& r = c->red_pending[q];
+        f.part[q] = c->red.base + (size_t)r.site * RED_NV * c->red.cap;
+        f.dst[q] = r.dst;
+        f.nb[q] = (int)r.nb;
+        f.nv[q] = r.nv;
+    }
+    c->red_pending.clear();
+    hipLaunchKernelGGL(k_red_flush, dim3((unsigned)n), dim3(1024), 0, c->stream, f);
+    SMD_HIP(c, hipGetLastError());
+    return SMD_OK;
+}
+
 int zero_terms(Ctx* c) {
     SMD_HIP(c, hipMemsetAsync(c->d_terms, 0, sizeof(double) * T_COUNT, c->stream));
     return SMD_OK;
@@ -72,7 +146,8 @@ static void release(Ctx* c) {
                     c->mp_ks.fpart, c->ind_ks.kvec, c->ind_ks.kmod2, c->ind_ks.expk, c->ind_ks.part,
                     c->ind_ks.kf, c->ind_ks.fpart, c->d_pack, c->d_bd_bonds, c->d_bd_angles,
                     c->d_bd_bond_off, c->d_bd_angle_off, c->d_bd_par, c->d_bd_out, c->d_stress, c->d_real,
-                    c->d_it_cnt, c->d_it_j, c->d_it_a, c->d_part, c->d_cg32, c->d_polar_terms, c->d_site32, c->d_pind32};
+                    c->d_it_cnt, c->d_it_j, c->d_it_a, c->d_part, c->d_cg32, c->d_polar_terms, c->d_site32, c->d_pind32,
+                    c->red.base};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     mlp_release(c);
@@ -198,6 +273,10 @@ extern "C" int smd_ctx_create(smd_ctx** out, const smd_topology* t, int device) 
         c->part_sh = c->part_kin + (size_t)KIN_PARTS * c->nb_mols64;
     }
     A(c->d_scalars, sizeof(double) * 8);
+    // reduction arena of block_part: up to n/4 + 64 blocks per site (row
+    // kernels with 64 lanes per atom), and at least 8,192 (per-k grids)
+    c->red.cap = std::max((n + 3) / 4 + 64, 8192);
+    A(c->red.base, sizeof(double) * red_doubles(c->red.cap));
     A(c->d_cell_of, sizeof(int32_t) * n);
     A(c->d_cell_atoms, sizeof(int32_t) * n);
     A(c->d_list_seg, sizeof(int32_t) * 28 * n);
@@ -269,6 +348,7 @@ extern "C" int smd_set_box(smd_ctx* c, const double box[3]) {
     c->list_valid = false;
     c->ewald_ready = false;
     c->md_ready = false;
+    c->ind_last_converged = false;   // the last solve's coupling table is of the old box
     return SMD_OK;
 }
 
@@ -318,8 +398,8 @@ extern "C" int smd_profile_enable(smd_ctx* c, int on) {
 extern "C" int smd_set_tuning(smd_ctx* c, const char* name, int value) {
     if (!c || !name) return SMD_EINVAL;
     if (std::string(name) == "pair_lanes") {
-        if (value != 0 && value != 4 && value != 8 && value != 16 && value != 32)
-            return fail(c, SMD_EINVAL, "pair_lanes must be 0 (by precision), 4, 8, 16 or 32 (got %d)", value);
+        if (value != 0 && value != 4 && value != 8 && value != 16 && value != 32 && value != 64)
+            return fail(c, SMD_EINVAL, "pair_lanes must be 0 (by precision), 4, 8, 16, 32 or 64 (got %d)", value);
         c->pair_lanes = value;
     } else if (std::string(name) == "pair_packed") {
         if (value != 0 && value != 1) return fail(c, SMD_EINVAL, "pair_packed must be 0 or 1 (got %d)", value);

@@ -31,6 +31,7 @@ struct BondedArgs {
     const double* pos;
     double* force;
     double* out;              // [BD_N]
+    Red red;
     Box box;
     double kl, l0;
 };
@@ -82,7 +83,7 @@ __global__ __launch_bounds__(256) void k_bonds(BondedArgs a) {
             }
         }
     }
-    block_add<BD_N>(acc, a.out);
+    block_part<BD_N>(acc, a.red, RS_BOND);
 }
 
 // AngleForce::compute: harmonic 0.5 k dtheta^2 or the AMOEBA sextic in
@@ -142,7 +143,7 @@ __global__ __launch_bounds__(256) void k_angles(BondedArgs a) {
             }
         }
     }
-    block_add<BD_N>(acc, a.out);
+    block_part<BD_N>(acc, a.red, RS_ANGLE);
 }
 
 // UreyBradleyForce::compute: kl (r - l0)^2 between the first and third atom
@@ -177,7 +178,7 @@ __global__ __launch_bounds__(256) void k_urey_bradley(BondedArgs a) {
             a.force[3 * i2 + p] -= f[p];
         }
     }
-    block_add<BD_N>(acc, a.out);
+    block_part<BD_N>(acc, a.red, RS_UB);
 }
 
 static int bd_alloc(Ctx* c, void** ptr, size_t bytes) {
@@ -238,6 +239,7 @@ static int bonded_run(Ctx* c, int term, double* f_accum, smd_bonded_out* out) {
     if (term == 2 && !c->bd_ub_ok) return fail(c, SMD_ENOTSUP, "Urey-Bradley term needs at least 3 atoms per molecule");
     BondedArgs a{};
     bonded_args(c, a, term);
+    a.red = c->red;
     if (f_accum) SMD_HIP(c, hipMemsetAsync(c->d_force, 0, sizeof(double) * 3 * c->n, c->stream));
     SMD_HIP(c, hipMemsetAsync(a.out, 0, sizeof(double) * BD_N, c->stream));
     const dim3 grid((c->nmol + 255) / 256);
@@ -245,6 +247,10 @@ static int bonded_run(Ctx* c, int term, double* f_accum, smd_bonded_out* out) {
     else if (term == 1) hipLaunchKernelGGL(k_angles, grid, dim3(256), 0, c->stream, a);
     else hipLaunchKernelGGL(k_urey_bradley, grid, dim3(256), 0, c->stream, a);
     SMD_HIP(c, hipGetLastError());
+    {
+        const int rc = red_finish(c, term == 0 ? RS_BOND : term == 1 ? RS_ANGLE : RS_UB, grid.x, BD_N, a.out);
+        if (rc) return rc;
+    }
     double h[BD_N];
     SMD_HIP(c, hipMemcpyAsync(h, a.out, sizeof h, hipMemcpyDeviceToHost, c->stream));
     std::vector<double> f;

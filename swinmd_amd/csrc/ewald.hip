@@ -173,6 +173,7 @@ struct IntraArgs {
     const int32_t* excl;
     double* force;
     double* terms;
+    Red red;
     double alpha, alpha2, alpha2PI, volume;
     int use_constraint;
 };
@@ -212,7 +213,7 @@ __global__ __launch_bounds__(256) void k_ewald_intra(IntraArgs a) {
             }
         }
     }
-    block_add<4>(v, a.terms + T_IN_E);
+    block_part<4>(v, a.red, RS_EW_INTRA);
 }
 
 int ewald_intra(Ctx* c, int lustig) {
@@ -227,6 +228,7 @@ int ewald_intra(Ctx* c, int lustig) {
     a.excl = c->d_excl;
     a.force = c->d_force;
     a.terms = c->d_terms;
+    a.red = c->red;
     a.alpha = c->ew.alpha;
     a.alpha2 = a.alpha * a.alpha;
     a.alpha2PI = 2.0 * a.alpha / SQRT_PI;
@@ -235,7 +237,7 @@ int ewald_intra(Ctx* c, int lustig) {
     ProfScope ps(c, "ewald_intra");
     hipLaunchKernelGGL(k_ewald_intra, dim3((c->nmol + 255) / 256), dim3(256), 0, c->stream, a);
     SMD_HIP(c, hipGetLastError());
-    return SMD_OK;
+    return red_finish(c, RS_EW_INTRA, (c->nmol + 255) / 256, 4, c->d_terms + T_IN_E);
 }
 
 // ---- surface dipole ---------------------------------------------------------
@@ -257,7 +259,8 @@ __global__ __launch_bounds__(256) void k_surf_sums(int n, const double* __restri
 }
 
 __global__ __launch_bounds__(256) void k_surf_force(int n, const double* __restrict__ pos, const double* __restrict__ q,
-                                                    double* __restrict__ force, double* terms, double coeff) {
+                                                    double* __restrict__ force, double* terms, double coeff,
+                                                    Red red) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     const double qx = coeff * terms[T_SU_QR], qy = coeff * terms[T_SU_QR + 1], qz = coeff * terms[T_SU_QR + 2];
     double v[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
@@ -271,7 +274,7 @@ __global__ __launch_bounds__(256) void k_surf_force(int n, const double* __restr
         v[3] = tx * ry; v[4] = ty * ry; v[5] = tz * ry;
         v[6] = tx * rz; v[7] = ty * rz; v[8] = tz * rz;
     }
-    block_add<9>(v, terms + T_SU_VIR);
+    block_part<9>(v, red, RS_EW_SURF);
 }
 
 int ewald_surface(Ctx* c) {
@@ -280,9 +283,10 @@ int ewald_surface(Ctx* c) {
     const dim3 g((c->n + 255) / 256);
     hipLaunchKernelGGL(k_surf_sums, g, dim3(256), 0, c->stream, c->n, c->d_pos, c->d_q, c->d_mol, c->d_com, c->d_terms,
                        c->part_su, c->d_flag + 4);
-    hipLaunchKernelGGL(k_surf_force, g, dim3(256), 0, c->stream, c->n, c->d_pos, c->d_q, c->d_force, c->d_terms, coeff);
+    hipLaunchKernelGGL(k_surf_force, g, dim3(256), 0, c->stream, c->n, c->d_pos, c->d_q, c->d_force, c->d_terms, coeff,
+                       c->red);
     SMD_HIP(c, hipGetLastError());
-    return SMD_OK;
+    return red_finish(c, RS_EW_SURF, g.x, 9, c->d_terms + T_SU_VIR);
 }
 
 // ---- reciprocal space ---------------------------------------------------------
@@ -307,6 +311,7 @@ struct RecipArgs {
     double* force;
     int store;                 // 1: the force pass stores (separate array), 0: adds
     double* terms;
+    Red red;
     double alphaR4, pi4V, pi8V;
 };
 
@@ -543,7 +548,7 @@ __global__ __launch_bounds__(256) void k_recip_reduce(RecipArgs a) {
         kf[2 * k + 1] = (R)(b * sinSum);
     }
     __syncthreads();
-    block_add<16>(o, a.terms + T_RC_E);
+    block_part<16>(o, a.red, RS_RECIP);
 }
 
 // Bytes of the (b C, b S) table in the force pass's LDS, rounded up so the
@@ -707,6 +712,7 @@ static int ewald_recip_with(Ctx* c, int prec, int lustig, unsigned passes, const
     a.store = c->recip_store ? 1 : 0;
     a.force = c->recip_store ? c->d_frecip : c->d_force;
     a.terms = c->d_terms;
+    a.red = c->red;
     a.alphaR4 = 1.0 / (4.0 * (c->ew.alpha * c->ew.alpha));
     const double volume = c->box.L[0] * c->box.L[1] * c->box.L[2];
     a.pi4V = 4.0 * PI * (1.0 / volume);
@@ -740,6 +746,9 @@ static int ewald_recip_with(Ctx* c, int prec, int lustig, unsigned passes, const
             if (lustig) hipLaunchKernelGGL((k_recip_reduce<double, true>), gk, dim3(256), 0, c->stream, a);
             else hipLaunchKernelGGL((k_recip_reduce<double, false>), gk, dim3(256), 0, c->stream, a);
         }
+        SMD_HIP(c, hipGetLastError());
+        const int rc = red_finish(c, RS_RECIP, gk.x, 16, a.terms + T_RC_E);
+        if (rc) return rc;
     }
     if (passes & RECIP_FORCE) {
         ProfScope ps(c, "recip_force");

@@ -845,6 +845,7 @@ int induce_begin(Ctx* c, const smd_induce_params* p, bool frames) {
     if (rc) return rc;
     c->ind_last_converged = false;
     if (p->do_min && (rc = table_alloc(c))) return rc;
+    if (p->do_min) c->ind_table_built = true;
     c->ind_last = *p;   // the solve in progress (the learned guess reads its parameters)
     IndArgs a = ind_args(c, p);
     const int n = c->n, dim = 3 * n;
@@ -1011,7 +1012,9 @@ using namespace smd;
 // table and dipoles) between two events on the context stream.
 extern "C" int smd_induce_time_apply(smd_ctx* c, int reps, double* avg_us) {
     if (!c || reps <= 0 || !avg_us) return SMD_EINVAL;
-    if (!c->ind_last_converged) return fail(c, SMD_ESTATE, "no converged smd_induce on this state");
+    // timing needs a coupling table of this context, not one of the current
+    // positions: the last solve's is used as it is
+    if (!c->ind_table_built) return fail(c, SMD_ESTATE, "no minimised smd_induce has built a coupling table");
     IndArgs a = ind_args(c, &c->ind_last);
     double* z = c->d_cg + 2 * (size_t)(3 * c->n);   // CG scratch; the solution in d_mu is untouched
     const bool mixed = c->ind_last.prec == SMD_PREC_MIXED;

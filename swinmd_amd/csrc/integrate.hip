@@ -758,6 +758,9 @@ int md_start_vtm(Ctx* c) {
 // no host synchronisation and no memset, so it can be captured.
 int md_step_enqueue(Ctx* c, bool do_long) {
     const smd_md_params& p = c->md;
+    // the force terms' reductions are queued and added by one kernel before
+    // the tail reads them
+    RedDefer defer(c);
     IntegArgs a = integ_args(c, &p.integ);
     a.fnh_on_device = 1;
     a.drmax = p.cut_buff * p.cut_buff / 4.0;   // Integrator.cpp:36
@@ -774,6 +777,7 @@ int md_step_enqueue(Ctx* c, bool do_long) {
         // pressure sums, the NH update and the per-step rewrap
         int rc = pairlist_build(c, p.cutoff, p.cut_buff, p.use_cells, true);
         if (!rc) rc = pairs_lj(c, p.cutoff, p.switch_on, p.constraint_mode ? 1 : 0, p.lustig, p.prec, true);
+        if (!rc) rc = red_flush(c);
         if (!rc) rc = p.constraint_mode ? shake_run(c, &p.integ, true, 0.0, true) : lf_atoms_run(c, &p.integ, 0.0, true);
         if (!rc) rc = kinetics_run(c, true, &p.integ);
         return rc;
@@ -802,6 +806,7 @@ int md_step_enqueue(Ctx* c, bool do_long) {
     if (p.ewald.surface && (rc = ewald_surface(c))) return rc;
     if (fork) SMD_HIP(c, hipStreamWaitEvent(c->stream, c->ev_join, 0));
     else if (do_long && (rc = ewald_recip(c, p.prec, 1))) return rc;
+    if ((rc = red_flush(c))) return rc;
     // drift removal, SHAKE (adds d_frecip), clears the rebuild flag; then the
     // kinetic and pressure sums, the NH update and the per-step rewrap
     c->shake_frecip = fork;
@@ -857,6 +862,7 @@ extern "C" int smd_shake(smd_ctx* c, const smd_integrator_params* p, double fnh,
     if (!(p->dt > 0) || p->max_count < 1) return fail(c, SMD_EINVAL, "bad SHAKE parameters");
     SMD_HIP(c, hipMemsetAsync(c->d_terms, 0, sizeof(double) * T_COUNT, c->stream));
     SMD_HIP(c, hipMemsetAsync(c->d_status, 0, sizeof(unsigned), c->stream));
+    c->ind_last_converged = false;   // positions move
     int rc = shake_run(c, p, false, fnh);
     if (rc) return rc;
     double t[T_COUNT];
@@ -870,6 +876,7 @@ extern "C" int smd_shake(smd_ctx* c, const smd_integrator_params* p, double fnh,
 extern "C" int smd_lf_atoms(smd_ctx* c, const smd_integrator_params* p, double fnh) {
     if (!c || !p) return SMD_EINVAL;
     SMD_HIP(c, hipMemsetAsync(c->d_status, 0, sizeof(unsigned), c->stream));
+    c->ind_last_converged = false;   // positions move
     int rc = lf_atoms_run(c, p, fnh, false);
     if (rc) return rc;
     SMD_HIP(c, hipStreamSynchronize(c->stream));
@@ -1053,6 +1060,7 @@ extern "C" int smd_md_time_family(smd_ctx* c, const char* family, int reps, doub
 extern "C" int smd_md_run(smd_ctx* c, int64_t nsteps, smd_md_report* out) {
     if (!c || nsteps < 0) return SMD_EINVAL;
     if (!c->md_ready) return fail(c, SMD_ESTATE, "smd_md_init not called");
+    c->ind_last_converged = false;   // positions move
     const int ls = c->md.long_steps > 1 ? c->md.long_steps : 1;
     const bool graph = c->md.use_graph && ls == 1 && !c->prof.on;
     if (graph && !c->graph_exec) {

@@ -76,6 +76,7 @@ struct MpArgs {
     double* force;
     double* trq;                // [5][3n]: real, intra, surface, k-space, self
     double* terms;              // [M_COUNT]
+    Red red;
     const double* kvec;         // [3K]
     const double* kmod2;        // [K]
     const double* expk;         // [K]
@@ -332,7 +333,7 @@ __global__ __launch_bounds__(256) void k_mp_real(MpArgs a) {
     }
     acc[0] *= COULOMBS;
     acc[1] *= COULOMBS;
-    block_add<11>(acc, a.terms + M_REAL_E);
+    block_part<11>(acc, a.red, RS_MP_REAL);
 }
 
 template <int G, typename R> __global__ __launch_bounds__(256) void k_ind_real(MpArgs a);
@@ -342,7 +343,7 @@ __global__ __launch_bounds__(256) void k_pack_sites32(MpArgs a, float4* __restri
 // Real-space kernels in the context's precision for these classes (FP64, or
 // FP32 pair math from packed records with FP64 accumulation); the mixed form
 // first packs the records (and, for induction, the induced dipoles).
-static void launch_real(Ctx* c, const MpArgs& a, bool induction) {
+static int launch_real(Ctx* c, const MpArgs& a, bool induction) {
     const dim3 gr((unsigned)(((int64_t)c->n * MP_LANES + 255) / 256)), b256(256);
     if (c->mp_prec == SMD_PREC_MIXED) {
         hipLaunchKernelGGL(k_pack_sites32, dim3((c->n + 255) / 256), b256, 0, c->stream, a, c->d_site32, c->d_pind32,
@@ -353,6 +354,18 @@ static void launch_real(Ctx* c, const MpArgs& a, bool induction) {
         if (induction) hipLaunchKernelGGL((k_ind_real<MP_LANES, double>), gr, b256, 0, c->stream, a);
         else hipLaunchKernelGGL((k_mp_real<MP_LANES, double>), gr, b256, 0, c->stream, a);
     }
+    return red_finish(c, induction ? RS_IND_REAL : RS_MP_REAL, gr.x, 11, a.terms + M_REAL_E);
+}
+
+__global__ __launch_bounds__(256) void k_mp_torques(MpArgs a, unsigned parts, int site);
+
+// Torques to forces, then the sums of its dU/dV and torque terms (sites
+// `site` and `site + 1`: each launch of a step has its own pair).
+static int launch_torques(Ctx* c, const MpArgs& a, dim3 g, unsigned parts, int site) {
+    hipLaunchKernelGGL(k_mp_torques, g, dim3(256), 0, c->stream, a, parts, site);
+    int rc = red_finish(c, site, g.x, 5, a.terms + M_DUDV);
+    if (!rc) rc = red_finish(c, site + 1, g.x, 5, a.terms + M_TRQ);
+    return rc;
 }
 
 // FP32 records of each site's lab-frame multipoles and polarizability for the
@@ -467,7 +480,7 @@ __global__ __launch_bounds__(256) void k_mp_intra(MpArgs a) {
         }
     }
     v[0] *= -COULOMBS;
-    block_add<2>(v, a.terms + M_INTRA_E);
+    block_part<2>(v, a.red, RS_MP_INTRA);
 }
 
 // Surface force, virial and torque per site (:1846-1900), from the sums.
@@ -492,7 +505,7 @@ __global__ __launch_bounds__(256) void k_mp_surface(MpArgs a, double coeff) {
         tr[3 * i + 1] = a.torques ? tq.y : 0.0;
         tr[3 * i + 2] = a.torques ? tq.z : 0.0;
     }
-    block_add<9>(v, a.terms + M_SU_VIR);
+    block_part<9>(v, a.red, RS_MP_SURF);
 }
 
 // ---- torques to forces ----------------------------------------------------------
@@ -516,7 +529,7 @@ __device__ __forceinline__ V3 h_site(V3 rh, V3 ro, V3 tau, double& dudv) {
 
 // One thread per water; the torque-array parts set in `parts` (bit p for
 // part p), each part's dU/dV summed into its own slot.
-__global__ __launch_bounds__(256) void k_mp_torques(MpArgs a, unsigned parts) {
+__global__ __launch_bounds__(256) void k_mp_torques(MpArgs a, unsigned parts, int site) {
     const int m = blockIdx.x * blockDim.x + threadIdx.x;
     double dv[5] = {0, 0, 0, 0, 0}, tn[5] = {0, 0, 0, 0, 0};
     if (m < a.nmol) {
@@ -558,9 +571,9 @@ __global__ __launch_bounds__(256) void k_mp_torques(MpArgs a, unsigned parts) {
         a.force[3 * o] += FO.x; a.force[3 * o + 1] += FO.y; a.force[3 * o + 2] += FO.z;
         a.force[3 * h2] += F2.x; a.force[3 * h2 + 1] += F2.y; a.force[3 * h2 + 2] += F2.z;
     }
-    block_add<5>(dv, a.terms + M_DUDV);
+    block_part<5>(dv, a.red, site);
     __syncthreads();
-    block_add<5>(tn, a.terms + M_TRQ);
+    block_part<5>(tn, a.red, site + 1);
 }
 
 // ---- k-space ---------------------------------------------------------------------
@@ -715,7 +728,7 @@ __global__ __launch_bounds__(256) void k_mp_kfinish(MpArgs a) {
         wave_chunk_sum<MP_NSUM>(a, k, S);
         if (threadIdx.x % WAVE == 0) mp_finish(a, k, S, v);
     }
-    block_add<9>(v, a.terms + M_LONG_E);
+    block_part<9>(v, a.red, RS_MP_LONG);
 }
 
 // Pass 3: force val k and the torque of the dipole and the quadrupole
@@ -829,6 +842,7 @@ static MpArgs mp_args(Ctx* c, const Ctx::KSet& ks, double alpha, double real_cut
     a.force = c->d_force;
     a.trq = c->d_mp_trq;
     a.terms = c->d_mp_terms;
+    a.red = c->red;
     a.kvec = ks.kvec;
     a.kmod2 = ks.kmod2;
     a.expk = ks.expk;
@@ -885,12 +899,16 @@ int multipole_short(Ctx* c, smd_mp_out* out) {
     SMD_HIP(c, hipMemsetAsync(c->d_mp_terms, 0, sizeof(double) * M_COUNT, c->stream));
     if ((rc = frames_run(c))) return rc;
     hipLaunchKernelGGL(k_mp_site_sums, ga, b256, 0, c->stream, a);
-    launch_real(c, a, false);
+    if ((rc = launch_real(c, a, false))) return rc;
     hipLaunchKernelGGL(k_mp_intra, gm, b256, 0, c->stream, a);
+    if ((rc = red_finish(c, RS_MP_INTRA, gm.x, 2, a.terms + M_INTRA_E))) return rc;
     const double volume = a.volume, volumer = 1.0 / volume;
     const double coeff = 2.0 * PI * volumer / 3.0;
-    if (c->mp.surface) hipLaunchKernelGGL(k_mp_surface, ga, b256, 0, c->stream, a, coeff);
-    if (c->mp.torques) hipLaunchKernelGGL(k_mp_torques, gm, b256, 0, c->stream, a, c->mp.surface ? 7u : 3u);
+    if (c->mp.surface) {
+        hipLaunchKernelGGL(k_mp_surface, ga, b256, 0, c->stream, a, coeff);
+        if ((rc = red_finish(c, RS_MP_SURF, ga.x, 9, a.terms + M_SU_VIR))) return rc;
+    }
+    if (c->mp.torques && (rc = launch_torques(c, a, gm, c->mp.surface ? 7u : 3u, RS_MP_DUDV))) return rc;
     SMD_HIP(c, hipGetLastError());
     double t[M_COUNT];
     if ((rc = mp_read_terms(c, t))) return rc;
@@ -939,10 +957,11 @@ int multipole_long(Ctx* c, smd_mp_long_out* out) {
     if (c->mp_prec == SMD_PREC_MIXED) hipLaunchKernelGGL((k_mp_sfac<float>), g1, dim3(MP_SF_THREADS), 0, c->stream, a);
         else hipLaunchKernelGGL((k_mp_sfac<double>), g1, dim3(MP_SF_THREADS), 0, c->stream, a);
     hipLaunchKernelGGL(k_mp_kfinish, dim3((a.K + 3) / 4), b256, 0, c->stream, a);
+    if ((rc = red_finish(c, RS_MP_LONG, (a.K + 3) / 4, 9, a.terms + M_LONG_E))) return rc;
     if (c->mp_prec == SMD_PREC_MIXED) hipLaunchKernelGGL((k_mp_kforce<float>), dim3((c->n + WAVE - 1) / WAVE, a.ksplit), dim3(WAVE), 0, c->stream, a);
         else hipLaunchKernelGGL((k_mp_kforce<double>), dim3((c->n + WAVE - 1) / WAVE, a.ksplit), dim3(WAVE), 0, c->stream, a);
     hipLaunchKernelGGL(k_kforce_sum, dim3((c->n + 255) / 256), b256, 0, c->stream, a);
-    if (c->mp.torques) hipLaunchKernelGGL(k_mp_torques, dim3((c->nmol + 255) / 256), b256, 0, c->stream, a, 8u);
+    if (c->mp.torques && (rc = launch_torques(c, a, dim3((c->nmol + 255) / 256), 8u, RS_MP_DUDV_L))) return rc;
     SMD_HIP(c, hipGetLastError());
     double t[M_COUNT];
     if ((rc = mp_read_terms(c, t))) return rc;
@@ -1248,7 +1267,7 @@ __global__ __launch_bounds__(256) void k_ind_real(MpArgs a) {
         a.trq[3 * i + 2] = COULOMBS * t.z;
     }
     acc[1] *= COULOMBS;
-    block_add<11>(acc, a.terms + M_REAL_E);
+    block_part<11>(acc, a.red, RS_IND_REAL);
 }
 
 // Per site: self energy and torque (:688-760) and the surface sums
@@ -1349,7 +1368,7 @@ __global__ __launch_bounds__(256) void k_ind_intra(MpArgs a) {
         }
     }
     v[0] *= -0.5 * COULOMBS;
-    block_add<2>(v, a.terms + M_INTRA_E);
+    block_part<2>(v, a.red, RS_IND_INTRA);
 }
 
 // Surface force, virial and torque per site (:2290-2345).
@@ -1375,7 +1394,7 @@ __global__ __launch_bounds__(256) void k_ind_surface(MpArgs a, double coeff) {
         tr[3 * i + 1] = a.torques ? tq.y : 0.0;
         tr[3 * i + 2] = a.torques ? tq.z : 0.0;
     }
-    block_add<9>(v, a.terms + M_SU_VIR);
+    block_part<9>(v, a.red, RS_IND_SURF);
 }
 
 // k-space pass 1: thread per k over a chunk of sites; 28 sums per (chunk, k):
@@ -1517,7 +1536,7 @@ __global__ __launch_bounds__(256) void k_ind_kfinish(MpArgs a) {
         wave_chunk_sum<IN_NSUM>(a, k, S);
         if (threadIdx.x % WAVE == 0) ind_finish(a, k, S, v);
     }
-    block_add<9>(v, a.terms + M_LONG_E);
+    block_part<9>(v, a.red, RS_IND_LONG);
 }
 
 // k-space pass 3 (:1603-1690), split over k-ranges like k_mp_kforce.
@@ -1604,14 +1623,17 @@ int induction_short(Ctx* c, smd_indf_out* out) {
     SMD_HIP(c, hipMemsetAsync(c->d_mp_terms, 0, sizeof(double) * M_COUNT, c->stream));
     if ((rc = frames_run(c))) return rc;
     hipLaunchKernelGGL(k_ind_site_sums, ga, b256, 0, c->stream, a);
-    launch_real(c, a, true);
+    if ((rc = launch_real(c, a, true))) return rc;
     hipLaunchKernelGGL(k_ind_intra, gm, b256, 0, c->stream, a);
+    if ((rc = red_finish(c, RS_IND_INTRA, gm.x, 2, a.terms + M_INTRA_E))) return rc;
     const double volume = a.volume, volumer = 1.0 / volume;
     const double coeff = 2.0 * PI * volumer / 3.0;
-    if (c->indf.surface) hipLaunchKernelGGL(k_ind_surface, ga, b256, 0, c->stream, a, coeff);
+    if (c->indf.surface) {
+        hipLaunchKernelGGL(k_ind_surface, ga, b256, 0, c->stream, a, coeff);
+        if ((rc = red_finish(c, RS_IND_SURF, ga.x, 9, a.terms + M_SU_VIR))) return rc;
+    }
     // torques of the self term, real space, intramolecular, surface
-    if (c->indf.torques)
-        hipLaunchKernelGGL(k_mp_torques, gm, b256, 0, c->stream, a, 16u | 3u | (c->indf.surface ? 4u : 0u));
+    if (c->indf.torques && (rc = launch_torques(c, a, gm, 16u | 3u | (c->indf.surface ? 4u : 0u), RS_IND_DUDV))) return rc;
     SMD_HIP(c, hipGetLastError());
     double t[M_COUNT];
     if ((rc = mp_read_terms(c, t))) return rc;
@@ -1659,10 +1681,11 @@ int induction_long(Ctx* c, smd_mp_long_out* out) {
     if (c->mp_prec == SMD_PREC_MIXED) hipLaunchKernelGGL((k_ind_sfac<float>), g1, dim3(MP_SF_THREADS), 0, c->stream, a);
         else hipLaunchKernelGGL((k_ind_sfac<double>), g1, dim3(MP_SF_THREADS), 0, c->stream, a);
     hipLaunchKernelGGL(k_ind_kfinish, dim3((a.K + 3) / 4), b256, 0, c->stream, a);
+    if ((rc = red_finish(c, RS_IND_LONG, (a.K + 3) / 4, 9, a.terms + M_LONG_E))) return rc;
     if (c->mp_prec == SMD_PREC_MIXED) hipLaunchKernelGGL((k_ind_kforce<float>), dim3((c->n + WAVE - 1) / WAVE, a.ksplit), dim3(WAVE), 0, c->stream, a);
         else hipLaunchKernelGGL((k_ind_kforce<double>), dim3((c->n + WAVE - 1) / WAVE, a.ksplit), dim3(WAVE), 0, c->stream, a);
     hipLaunchKernelGGL(k_kforce_sum, dim3((c->n + 255) / 256), b256, 0, c->stream, a);
-    if (c->indf.torques) hipLaunchKernelGGL(k_mp_torques, dim3((c->nmol + 255) / 256), b256, 0, c->stream, a, 8u);
+    if (c->indf.torques && (rc = launch_torques(c, a, dim3((c->nmol + 255) / 256), 8u, RS_IND_DUDV_L))) return rc;
     SMD_HIP(c, hipGetLastError());
     double t[M_COUNT];
     if ((rc = mp_read_terms(c, t))) return rc;
@@ -1684,6 +1707,7 @@ int induction_long(Ctx* c, smd_mp_long_out* out) {
 // sequence can be captured), forces and torque-derived forces into d_force.
 // The lab-frame multipoles must be current (frames_run).
 int multipole_enqueue(Ctx* c, double* terms) {
+    int rc;
     MpArgs a = mp_args(c);
     a.terms = terms;
     const dim3 b256(256);
@@ -1693,10 +1717,14 @@ int multipole_enqueue(Ctx* c, double* terms) {
     {
         ProfScope ps(c, "mp_short");
         hipLaunchKernelGGL(k_mp_site_sums, ga, b256, 0, c->stream, a);
-        launch_real(c, a, false);
+        if ((rc = launch_real(c, a, false))) return rc;
         hipLaunchKernelGGL(k_mp_intra, gm, b256, 0, c->stream, a);
-        if (c->mp.surface) hipLaunchKernelGGL(k_mp_surface, ga, b256, 0, c->stream, a, 2.0 * PI * (1.0 / a.volume) / 3.0);
-        if (c->mp.torques) hipLaunchKernelGGL(k_mp_torques, gm, b256, 0, c->stream, a, c->mp.surface ? 7u : 3u);
+        if ((rc = red_finish(c, RS_MP_INTRA, gm.x, 2, a.terms + M_INTRA_E))) return rc;
+        if (c->mp.surface) {
+            hipLaunchKernelGGL(k_mp_surface, ga, b256, 0, c->stream, a, 2.0 * PI * (1.0 / a.volume) / 3.0);
+            if ((rc = red_finish(c, RS_MP_SURF, ga.x, 9, a.terms + M_SU_VIR))) return rc;
+        }
+        if (c->mp.torques && (rc = launch_torques(c, a, gm, c->mp.surface ? 7u : 3u, RS_MP_DUDV))) return rc;
     }
     {
         ProfScope ps(c, "mp_long");
@@ -1704,16 +1732,18 @@ int multipole_enqueue(Ctx* c, double* terms) {
         if (c->mp_prec == SMD_PREC_MIXED) hipLaunchKernelGGL((k_mp_sfac<float>), g1, dim3(MP_SF_THREADS), 0, c->stream, a);
         else hipLaunchKernelGGL((k_mp_sfac<double>), g1, dim3(MP_SF_THREADS), 0, c->stream, a);
         hipLaunchKernelGGL(k_mp_kfinish, dim3((a.K + 3) / 4), b256, 0, c->stream, a);
+        if ((rc = red_finish(c, RS_MP_LONG, (a.K + 3) / 4, 9, a.terms + M_LONG_E))) return rc;
         if (c->mp_prec == SMD_PREC_MIXED) hipLaunchKernelGGL((k_mp_kforce<float>), dim3((c->n + WAVE - 1) / WAVE, a.ksplit), dim3(WAVE), 0, c->stream, a);
         else hipLaunchKernelGGL((k_mp_kforce<double>), dim3((c->n + WAVE - 1) / WAVE, a.ksplit), dim3(WAVE), 0, c->stream, a);
         hipLaunchKernelGGL(k_kforce_sum, ga, b256, 0, c->stream, a);
-        if (c->mp.torques) hipLaunchKernelGGL(k_mp_torques, gm, b256, 0, c->stream, a, 8u);
+        if (c->mp.torques && (rc = launch_torques(c, a, gm, 8u, RS_MP_DUDV_L))) return rc;
     }
     SMD_HIP(c, hipGetLastError());
     return SMD_OK;
 }
 
 int induction_enqueue(Ctx* c, double* terms) {
+    int rc;
     MpArgs a = ind_args(c);
     a.terms = terms;
     const dim3 b256(256);
@@ -1723,11 +1753,14 @@ int induction_enqueue(Ctx* c, double* terms) {
     {
         ProfScope ps(c, "ind_short");
         hipLaunchKernelGGL(k_ind_site_sums, ga, b256, 0, c->stream, a);
-        launch_real(c, a, true);
+        if ((rc = launch_real(c, a, true))) return rc;
         hipLaunchKernelGGL(k_ind_intra, gm, b256, 0, c->stream, a);
-        if (c->indf.surface) hipLaunchKernelGGL(k_ind_surface, ga, b256, 0, c->stream, a, 2.0 * PI * (1.0 / a.volume) / 3.0);
-        if (c->indf.torques)
-            hipLaunchKernelGGL(k_mp_torques, gm, b256, 0, c->stream, a, 16u | 3u | (c->indf.surface ? 4u : 0u));
+        if ((rc = red_finish(c, RS_IND_INTRA, gm.x, 2, a.terms + M_INTRA_E))) return rc;
+        if (c->indf.surface) {
+            hipLaunchKernelGGL(k_ind_surface, ga, b256, 0, c->stream, a, 2.0 * PI * (1.0 / a.volume) / 3.0);
+            if ((rc = red_finish(c, RS_IND_SURF, ga.x, 9, a.terms + M_SU_VIR))) return rc;
+        }
+        if (c->indf.torques && (rc = launch_torques(c, a, gm, 16u | 3u | (c->indf.surface ? 4u : 0u), RS_IND_DUDV))) return rc;
     }
     {
         ProfScope ps(c, "ind_long");
@@ -1735,10 +1768,11 @@ int induction_enqueue(Ctx* c, double* terms) {
         if (c->mp_prec == SMD_PREC_MIXED) hipLaunchKernelGGL((k_ind_sfac<float>), g1, dim3(MP_SF_THREADS), 0, c->stream, a);
         else hipLaunchKernelGGL((k_ind_sfac<double>), g1, dim3(MP_SF_THREADS), 0, c->stream, a);
         hipLaunchKernelGGL(k_ind_kfinish, dim3((a.K + 3) / 4), b256, 0, c->stream, a);
+        if ((rc = red_finish(c, RS_IND_LONG, (a.K + 3) / 4, 9, a.terms + M_LONG_E))) return rc;
         if (c->mp_prec == SMD_PREC_MIXED) hipLaunchKernelGGL((k_ind_kforce<float>), dim3((c->n + WAVE - 1) / WAVE, a.ksplit), dim3(WAVE), 0, c->stream, a);
         else hipLaunchKernelGGL((k_ind_kforce<double>), dim3((c->n + WAVE - 1) / WAVE, a.ksplit), dim3(WAVE), 0, c->stream, a);
         hipLaunchKernelGGL(k_kforce_sum, ga, b256, 0, c->stream, a);
-        if (c->indf.torques) hipLaunchKernelGGL(k_mp_torques, gm, b256, 0, c->stream, a, 8u);
+        if (c->indf.torques && (rc = launch_torques(c, a, gm, 8u, RS_IND_DUDV_L))) return rc;
     }
     SMD_HIP(c, hipGetLastError());
     return SMD_OK;

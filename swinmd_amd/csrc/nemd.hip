@@ -86,6 +86,7 @@ struct Params {
     double sigma2_oo, eps4_oo;
     double b2[3];                 // squared constraint lengths H1-O, O-H2, H2-H1
     double mass[3], q[3];         // per site of a molecule
+    double pre2;                  // (rlist + 2 r_OH)^2: the list's O-O prefilter
     int32_t max_count, nose_hoover, nb_cap, n;
 };
 
@@ -95,7 +96,6 @@ struct Args {
     RepState* st;
     smd_batch_report* out;
     unsigned* err;
-    const int32_t* atom_rep;      // [n]
     double* pos;                  // [3n]
     double* vtm;
     double* vel;
@@ -117,6 +117,7 @@ struct Args {
     const int2* fin_blk;          // (replica, k tile)
     const int2* kf_blk;           // (replica, first atom)
     const int2* sh_blk;           // (replica, first molecule)
+    const int2* list_blk;         // (replica, first molecule)
     int32_t n_pair_blk, n_fin_blk, n_sh_blk, sf_chunk;
     int32_t force_rebuild, zero_disp, count_update, kvec_only;
 };
@@ -236,42 +237,86 @@ __global__ __launch_bounds__(256) void k_begin(Args a) {
     }
 }
 
-// ---- pair list: brute force over the replica, one wave per row ------------
+// ---- pair list: brute force over the replica, one wave per molecule -------
 
-__global__ __launch_bounds__(256) void k_list(Args a) {
-    const int row = (int)((blockIdx.x * blockDim.x + threadIdx.x) / WAVE);
-    const int lane = threadIdx.x % WAVE;
-    if (row >= a.p.n) return;
-    const int r = a.atom_rep[row];
-    if (!a.st[r].need) return;
+// A block's waves take consecutive molecules of one replica (rows 3m..3m+2);
+// the replica's molecules pass through LDS in tiles of 64, lane l testing
+// partner molecule l of the tile.  The O-O minimum image first: no site pair
+// of two molecules can be within rlist unless their O are within
+// rlist + 2 r_OH (pre2), since every site is within r_OH of its O.  Then the
+// nine site pairs, each with its own minimum image (as the brute-force list
+// does), and an ordered compaction per row: entries are in ascending partner
+// index, the order of an atom-by-atom scan.
+constexpr int LIST_WAVES = 4;
+constexpr int LIST_TILE = WAVE;
+
+__global__ __launch_bounds__(LIST_WAVES * WAVE) void k_list(Args a) {
+    __shared__ double tile[LIST_TILE][9];
+    const int2 bm = a.list_blk[blockIdx.x];
+    const int r = bm.x;
+    if (!a.st[r].need) return;   // block-uniform
     const Rep rp = a.rep[r];
     const double tilt = a.st[r].tilt;
-    if (a.zero_disp && lane < 3) a.disp[3 * row + lane] = 0.0;
-    const double xi = a.pos[3 * row], yi = a.pos[3 * row + 1], zi = a.pos[3 * row + 2];
+    const int lane = threadIdx.x % WAVE, w = threadIdx.x / WAVE;
+    const int mi = bm.y + w;
+    const bool live = mi < rp.m1;   // wave-uniform
     const double rl = a.p.cutoff + a.p.cut_buff;
     const double rl2 = rl * rl;
-    const int mi = row / 3;
-    int32_t* out = a.nb + (size_t)row * a.p.nb_cap;
-    int cnt = 0;
-    for (int base = rp.a0; base < rp.a1; base += WAVE) {
-        const int j = base + lane;
-        bool take = false;
-        if (j < rp.a1 && j / 3 != mi) {
-            double dx = a.pos[3 * j] - xi, dy = a.pos[3 * j + 1] - yi, dz = a.pos[3 * j + 2] - zi;
-            le_image(dx, dy, dz, rp.L, rp.invL, tilt);
-            take = dx * dx + dy * dy + dz * dz < rl2;
-        }
-        const unsigned long long m = __ballot(take);
-        if (take) {
-            const int slot = cnt + __popcll(m & ((1ull << lane) - 1ull));
-            if (slot < a.p.nb_cap) out[slot] = j;
-        }
-        cnt += __popcll(m);
+    double p[3][3] = {{0.0, 0.0, 0.0}, {0.0, 0.0, 0.0}, {0.0, 0.0, 0.0}};
+    if (live) {
+#pragma unroll
+        for (int s = 0; s < 3; ++s)
+#pragma unroll
+            for (int d = 0; d < 3; ++d) p[s][d] = a.pos[9 * mi + 3 * s + d];
+        if (a.zero_disp && lane < 9) a.disp[9 * mi + lane] = 0.0;
     }
-    if (lane == 0) {
-        a.nb_cnt[row] = min(cnt, a.p.nb_cap);
-        atomicMax(&a.st[r].max_row, cnt);
-        if (cnt > a.p.nb_cap) atomicOr(a.err, E_LIST);
+    int cnt[3] = {0, 0, 0};
+    const unsigned long long lt = (1ull << lane) - 1ull;
+    for (int m0 = rp.m0; m0 < rp.m1; m0 += LIST_TILE) {
+        __syncthreads();
+        const int nt = min(LIST_TILE, rp.m1 - m0);
+        for (int t = threadIdx.x; t < 9 * nt; t += blockDim.x) tile[t / 9][t % 9] = a.pos[9 * m0 + t];
+        __syncthreads();
+        if (!live) continue;
+        const int mj = m0 + lane;
+        unsigned bits[3] = {0u, 0u, 0u};   // bit s of bits[q]: site s of mj is within rlist of row site q
+        if (lane < nt && mj != mi) {
+            double dx = tile[lane][3] - p[1][0], dy = tile[lane][4] - p[1][1], dz = tile[lane][5] - p[1][2];
+            le_image(dx, dy, dz, rp.L, rp.invL, tilt);
+            if (dx * dx + dy * dy + dz * dz < a.p.pre2) {
+#pragma unroll
+                for (int q = 0; q < 3; ++q)
+#pragma unroll
+                    for (int s = 0; s < 3; ++s) {
+                        double ex = tile[lane][3 * s] - p[q][0], ey = tile[lane][3 * s + 1] - p[q][1];
+                        double ez = tile[lane][3 * s + 2] - p[q][2];
+                        le_image(ex, ey, ez, rp.L, rp.invL, tilt);
+                        if (ex * ex + ey * ey + ez * ez < rl2) bits[q] |= 1u << s;
+                    }
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < 3; ++q) {
+            const unsigned long long b0 = __ballot(bits[q] & 1u), b1 = __ballot(bits[q] & 2u),
+                                     b2 = __ballot(bits[q] & 4u);
+            int slot = cnt[q] + __popcll(b0 & lt) + __popcll(b1 & lt) + __popcll(b2 & lt);
+            int32_t* out = a.nb + (size_t)(3 * mi + q) * a.p.nb_cap;
+#pragma unroll
+            for (int s = 0; s < 3; ++s)
+                if (bits[q] & (1u << s)) {
+                    if (slot < a.p.nb_cap) out[slot] = 3 * mj + s;
+                    ++slot;
+                }
+            cnt[q] += __popcll(b0) + __popcll(b1) + __popcll(b2);
+        }
+    }
+    if (live && lane == 0) {
+#pragma unroll
+        for (int q = 0; q < 3; ++q) {
+            a.nb_cnt[3 * mi + q] = min(cnt[q], a.p.nb_cap);
+            atomicMax(&a.st[r].max_row, cnt[q]);
+            if (cnt[q] > a.p.nb_cap) atomicOr(a.err, E_LIST);
+        }
     }
 }
 
@@ -858,7 +903,6 @@ struct smd_batch {
     smd::nemd::RepState* d_st = nullptr;
     smd_batch_report* d_out = nullptr;
     unsigned* d_err = nullptr;
-    int32_t* d_atom_rep = nullptr;
     double *d_pos = nullptr, *d_vtm = nullptr, *d_vel = nullptr, *d_disp = nullptr;
     double *d_fpair = nullptr, *d_frec = nullptr, *d_com = nullptr, *d_mom = nullptr;
     int32_t *d_nb = nullptr, *d_nb_cnt = nullptr;
@@ -867,9 +911,10 @@ struct smd_batch {
     double *d_sf_part = nullptr, *d_pair_part = nullptr, *d_kf_part = nullptr, *d_sh_part = nullptr;
     size_t sf_part_cap = 0, pair_part_cap = 0;
     int2 *d_pair_blk = nullptr, *d_fin_blk = nullptr, *d_kf_blk = nullptr, *d_sh_blk = nullptr;
+    int2* d_list_blk = nullptr;
     int4* d_sf_blk = nullptr;
     size_t pair_blk_cap = 0, sf_blk_cap = 0;
-    int n_pair_blk = 0, n_sf_blk = 0, n_fin_blk = 0, n_kf_blk = 0, n_sh_blk = 0;
+    int n_pair_blk = 0, n_sf_blk = 0, n_fin_blk = 0, n_kf_blk = 0, n_sh_blk = 0, n_list_blk = 0;
     int ktot = 0;
     hipGraphExec_t graph = nullptr;
     bool have_state = false, started = false;
@@ -918,7 +963,7 @@ static bool lanes_ok(int g) { return g == 1 || g == 2 || g == 4 || g == 8 || g =
 static int build_tables(smd_batch* b) {
     const int G = b->tiles.pair_lanes, CH = b->tiles.sfac_chunk;
     const int rpb = 256 / G;
-    std::vector<int2> pair_blk, fin_blk, kf_blk, sh_blk;
+    std::vector<int2> pair_blk, fin_blk, kf_blk, sh_blk, list_blk;
     std::vector<int4> sf_blk;
     int64_t pbase = 0;
     for (int r = 0; r < b->R; ++r) {
@@ -940,6 +985,7 @@ static int build_tables(smd_batch* b) {
         rp.sb0 = (int)sh_blk.size();
         for (int m = rp.m0; m < rp.m1; m += KIN_BLOCK) sh_blk.push_back(make_int2(r, m));
         rp.sb1 = (int)sh_blk.size();
+        for (int m = rp.m0; m < rp.m1; m += LIST_WAVES) list_blk.push_back(make_int2(r, m));
     }
     int rc;
     if ((size_t)pbase > b->sf_part_cap) {
@@ -961,17 +1007,20 @@ static int build_tables(smd_batch* b) {
         if ((rc = dalloc(b, b->d_kf_blk, kf_blk.size()))) return rc;
         if ((rc = dalloc(b, b->d_sh_blk, sh_blk.size()))) return rc;
         if ((rc = dalloc(b, b->d_sh_part, NSH * sh_blk.size()))) return rc;
+        if ((rc = dalloc(b, b->d_list_blk, list_blk.size()))) return rc;
     }
     b->n_pair_blk = (int)pair_blk.size();
     b->n_sf_blk = (int)sf_blk.size();
     b->n_fin_blk = (int)fin_blk.size();
     b->n_kf_blk = (int)kf_blk.size();
     b->n_sh_blk = (int)sh_blk.size();
+    b->n_list_blk = (int)list_blk.size();
     BHIP(b, hipMemcpy(b->d_pair_blk, pair_blk.data(), sizeof(int2) * pair_blk.size(), hipMemcpyHostToDevice));
     BHIP(b, hipMemcpy(b->d_sf_blk, sf_blk.data(), sizeof(int4) * sf_blk.size(), hipMemcpyHostToDevice));
     BHIP(b, hipMemcpy(b->d_fin_blk, fin_blk.data(), sizeof(int2) * fin_blk.size(), hipMemcpyHostToDevice));
     BHIP(b, hipMemcpy(b->d_kf_blk, kf_blk.data(), sizeof(int2) * kf_blk.size(), hipMemcpyHostToDevice));
     BHIP(b, hipMemcpy(b->d_sh_blk, sh_blk.data(), sizeof(int2) * sh_blk.size(), hipMemcpyHostToDevice));
+    BHIP(b, hipMemcpy(b->d_list_blk, list_blk.data(), sizeof(int2) * list_blk.size(), hipMemcpyHostToDevice));
     BHIP(b, hipMemcpy(b->d_rep, b->rep.data(), sizeof(Rep) * b->R, hipMemcpyHostToDevice));
     drop_graph(b);
     return SMD_OK;
@@ -984,7 +1033,6 @@ static Args make_args(smd_batch* b) {
     a.st = b->d_st;
     a.out = b->d_out;
     a.err = b->d_err;
-    a.atom_rep = b->d_atom_rep;
     a.pos = b->d_pos; a.vtm = b->d_vtm; a.vel = b->d_vel; a.disp = b->d_disp;
     a.fpair = b->d_fpair; a.frec = b->d_frec; a.com = b->d_com; a.mom = b->d_mom;
     a.nb = b->d_nb; a.nb_cnt = b->d_nb_cnt;
@@ -992,6 +1040,7 @@ static Args make_args(smd_batch* b) {
     a.sf_part = b->d_sf_part; a.pair_part = b->d_pair_part; a.kf_part = b->d_kf_part; a.sh_part = b->d_sh_part;
     a.pair_blk = b->d_pair_blk; a.sf_blk = b->d_sf_blk; a.fin_blk = b->d_fin_blk; a.kf_blk = b->d_kf_blk;
     a.sh_blk = b->d_sh_blk;
+    a.list_blk = b->d_list_blk;
     a.n_pair_blk = b->n_pair_blk; a.n_fin_blk = b->n_fin_blk; a.n_sh_blk = b->n_sh_blk;
     a.sf_chunk = b->tiles.sfac_chunk;
     return a;
@@ -1016,7 +1065,7 @@ static void launch_pass(smd_batch* b, hipStream_t s, const Args& a, unsigned pas
     const bool mixed = b->prec == SMD_PREC_MIXED;
     switch (pass) {
     case P_BEGIN: hipLaunchKernelGGL(k_begin, dim3(b->R), dim3(256), 0, s, a); break;
-    case P_LIST: hipLaunchKernelGGL(k_list, dim3((b->n + 3) / 4), dim3(256), 0, s, a); break;
+    case P_LIST: hipLaunchKernelGGL(k_list, dim3(b->n_list_blk), dim3(LIST_WAVES * WAVE), 0, s, a); break;
     case P_PAIRS:
         if (mixed) launch_pairs<float>(s, a, b->tiles.pair_lanes, b->n_pair_blk);
         else launch_pairs<double>(s, a, b->tiles.pair_lanes, b->n_pair_blk);
@@ -1160,6 +1209,10 @@ extern "C" int smd_batch_create(smd_batch** out, const smd_batch_params* p, int 
     q.b2[0] = p->r_oh * p->r_oh; q.b2[1] = p->r_oh * p->r_oh; q.b2[2] = p->r_hh * p->r_hh;
     q.mass[0] = p->mass_h; q.mass[1] = p->mass_o; q.mass[2] = p->mass_h;
     q.q[0] = p->q_h; q.q[1] = p->q_o; q.q[2] = p->q_h;
+    {
+        const double pre = rl + 2.0 * p->r_oh * (1.0 + 1e-6) + 1e-9;
+        q.pre2 = pre * pre;
+    }
     q.max_count = p->max_count;
     q.nose_hoover = p->nose_hoover;
     q.n = a;
@@ -1175,16 +1228,12 @@ extern "C" int smd_batch_create(smd_batch** out, const smd_batch_params* p, int 
     int rc;
     const size_t n3 = 3 * (size_t)a;
     if ((rc = dalloc(b, b->d_rep, b->R)) || (rc = dalloc(b, b->d_st, b->R)) || (rc = dalloc(b, b->d_out, b->R)) ||
-        (rc = dalloc(b, b->d_err, 1)) || (rc = dalloc(b, b->d_atom_rep, a)) || (rc = dalloc(b, b->d_pos, n3)) ||
+        (rc = dalloc(b, b->d_err, 1)) || (rc = dalloc(b, b->d_pos, n3)) ||
         (rc = dalloc(b, b->d_vtm, n3)) || (rc = dalloc(b, b->d_vel, n3)) || (rc = dalloc(b, b->d_disp, n3)) ||
         (rc = dalloc(b, b->d_fpair, n3)) || (rc = dalloc(b, b->d_frec, n3)) || (rc = dalloc(b, b->d_com, 3 * (size_t)m)) ||
         (rc = dalloc(b, b->d_mom, 3 * (size_t)m)) || (rc = dalloc(b, b->d_nb, (size_t)a * q.nb_cap)) ||
         (rc = dalloc(b, b->d_nb_cnt, a)) || (rc = dalloc(b, b->d_kvec, kb)) || (rc = dalloc(b, b->d_kf, kb)))
         return rc;
-    std::vector<int32_t> ar(a);
-    for (int r = 0; r < b->R; ++r)
-        for (int i = b->rep[r].a0; i < b->rep[r].a1; ++i) ar[i] = r;
-    BHIP(b, hipMemcpy(b->d_atom_rep, ar.data(), sizeof(int32_t) * a, hipMemcpyHostToDevice));
     BHIP(b, hipMemset(b->d_err, 0, sizeof(unsigned)));
     BHIP(b, hipMemset(b->d_out, 0, sizeof(smd_batch_report) * b->R));
     BHIP(b, hipMemset(b->d_nb_cnt, 0, sizeof(int32_t) * a));
@@ -1196,10 +1245,10 @@ extern "C" void smd_batch_destroy(smd_batch* b) {
     (void)hipSetDevice(b->device);
     if (b->stream) (void)hipStreamSynchronize(b->stream);
     drop_graph(b);
-    void* ptrs[] = {b->d_rep, b->d_st, b->d_out, b->d_err, b->d_atom_rep, b->d_pos, b->d_vtm, b->d_vel, b->d_disp,
+    void* ptrs[] = {b->d_rep, b->d_st, b->d_out, b->d_err, b->d_pos, b->d_vtm, b->d_vel, b->d_disp,
                     b->d_fpair, b->d_frec, b->d_com, b->d_mom, b->d_nb, b->d_nb_cnt, b->d_kvec, b->d_kf,
                     b->d_sf_part, b->d_pair_part, b->d_kf_part, b->d_sh_part, b->d_pair_blk, b->d_fin_blk,
-                    b->d_kf_blk, b->d_sh_blk, b->d_sf_blk};
+                    b->d_kf_blk, b->d_sh_blk, b->d_sf_blk, b->d_list_blk};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     if (b->ev_fork) (void)hipEventDestroy(b->ev_fork);

@@ -39,6 +39,7 @@ struct PairArgs {
     double* force;
     double* fvirt;
     double* terms;
+    Red red;
     // vdW
     int ntypes;
     const double* ta;
@@ -384,14 +385,14 @@ __global__ __launch_bounds__(256) void k_pairs(PairArgs a) {
         double v[4 + 9];
 #pragma unroll
         for (int k = 0; k < 13; ++k) v[k] = acc[A_VE + k];
-        block_add<13>(v, a.terms + T_LJ_E);
+        block_part<13>(v, a.red, RS_PAIR_LJ);
     }
     if constexpr (EWALD) {
         __syncthreads();
         double v[3 + 9];
 #pragma unroll
         for (int k = 0; k < 12; ++k) v[k] = acc[A_EE + k];
-        block_add<12>(v, a.terms + T_EW_E);
+        block_part<12>(v, a.red, RS_PAIR_EW);
     }
 }
 
@@ -421,9 +422,17 @@ static hipError_t launch_pairs(Ctx* c, const PairArgs& a, int prec, int lustig) 
         case 4: launch_pairs_g<VDW, EWALD, 4>(c, a, prec, lustig); break;
         case 8: launch_pairs_g<VDW, EWALD, 8>(c, a, prec, lustig); break;
         case 32: launch_pairs_g<VDW, EWALD, 32>(c, a, prec, lustig); break;
+        case 64: launch_pairs_g<VDW, EWALD, 64>(c, a, prec, lustig); break;
         default: launch_pairs_g<VDW, EWALD, 16>(c, a, prec, lustig); break;
     }
-    return hipGetLastError();
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    // the energies' and virials' block partials, added in block order
+    const int g = lanes == 4 || lanes == 8 || lanes == 32 || lanes == 64 ? lanes : 16;
+    const unsigned nb = (unsigned)(((int64_t)c->n * g + 255) / 256);
+    if (VDW != V_NONE && red_finish(c, RS_PAIR_LJ, nb, 13, a.terms + T_LJ_E)) return hipErrorInvalidValue;
+    if (EWALD && red_finish(c, RS_PAIR_EW, nb, 12, a.terms + T_EW_E)) return hipErrorInvalidValue;
+    return hipSuccess;
 }
 
 static void base_args(Ctx* c, PairArgs& a) {
@@ -443,6 +452,7 @@ static void base_args(Ctx* c, PairArgs& a) {
     a.force = c->d_force;
     a.fvirt = c->d_fvirt;
     a.terms = c->d_terms;
+    a.red = c->red;
     a.ntypes = c->ntypes;
     a.ta = c->d_vdw_a;
     a.tb = c->d_vdw_b;
@@ -730,6 +740,7 @@ struct PackedArgs {
     const double* tb;   // 4 eps
     double* force;
     double* terms;
+    Red red;
     float Lh[3], Ll[3], invL[3];
     float cut2, rcut_2, alpha, alpha2, alpha2PI, realcut2;
 };
@@ -745,16 +756,18 @@ struct PackedOwner {
 // Per-lane sums of the packed kernel.  A pair's virial s d d^T is symmetric,
 // so each part keeps its 6 upper-triangle components (xx xy xz yy yz zz).
 enum {
-    PK_VE = 0, PK_VDUDV, PK_VD2UDV, PK_VVIR,   // vdW energy, Lustig, virial (6)
-    PK_EE = PK_VVIR + 6, PK_ED1, PK_ED2, PK_EVIR,
-    PK_N = PK_EVIR + 6
+    PK_VE = 0, PK_VDUDV, PK_VD2UDV,            // vdW energy, Lustig
+    PK_EE, PK_ED1, PK_ED2,                     // Ewald real energy, Lustig
+    PK_VIR,                                    // virial of both terms (6)
+    PK_N = PK_VIR + 6
 };
 
-// Per-lane energy, Lustig and virial sums in FP32 (a lane adds ~15 terms);
-// the wave sums are FP32 (6 pairwise levels) and the block and grid sums FP64.
+// Pair arithmetic in FP32; each pair's energy, Lustig and virial terms are
+// added to per-lane FP64 sums, which the wave, block and grid sums keep in
+// FP64 (the grid's in a fixed block order).
 template <bool LUSTIG, bool FULL>
 __device__ __forceinline__ void packed_term(const PackedArgs& a, const float2* __restrict__ ljt, const PackedOwner& o,
-                                            int j, float4 r0, float4 r1, float (&acc)[PK_N], double& fix,
+                                            int j, float4 r0, float4 r1, double (&acc)[PK_N], double& fix,
                                             double& fiy, double& fiz) {
     float dx, dy, dz;
     {
@@ -795,17 +808,15 @@ __device__ __forceinline__ void packed_term(const PackedArgs& a, const float2* _
             if constexpr (LUSTIG) {
                 const float rijRij = a.use_constraint ? (dx * mx + dy * my + dz * mz) : r2;
                 const float dulj = tmpf * rijRij;
-                acc[PK_VDUDV] += dulj;
-                acc[PK_VD2UDV] += eps4 * (156.f * sr12 - 42.f * sr6) * rijRij * rijRij * rij_2 * rij_2 + 2.f * dulj;
+                acc[PK_VDUDV] += (double)dulj;
+                acc[PK_VD2UDV] += (double)(eps4 * (156.f * sr12 - 42.f * sr6) * rijRij * rijRij * rij_2 * rij_2 + 2.f * dulj);
             }
             if (a.switch_on) {   // energy shift only (LJForce.cpp:152-168)
                 const float sr2c = sigma2 * a.rcut_2;
                 const float sr6c = sr2c * sr2c * sr2c;
                 tmpE = tmpE - eps4 * (sr6c * sr6c - sr6c);
             }
-            acc[PK_VE] += tmpE;
-#pragma unroll
-            for (int k = 0; k < 6; ++k) acc[PK_VVIR + k] -= tmpf * dd[k];
+            acc[PK_VE] += (double)tmpE;
         }
         s -= tmpf;
     }
@@ -816,16 +827,19 @@ __device__ __forceinline__ void packed_term(const PackedArgs& a, const float2* _
         const float f = (e + a.alpha2PI * ex) * rij_2;
         const float se = -qij * f;
         if constexpr (FULL) {
-            acc[PK_EE] += qij * e;
+            acc[PK_EE] += (double)(qij * e);
             if constexpr (LUSTIG) {
                 const float rr = a.use_constraint ? (dx * mx + dy * my + dz * mz) : r2;
-                acc[PK_ED1] += -qij * f * rr;
-                acc[PK_ED2] += qij * (2.f * f + 2.f * a.alpha2PI * a.alpha2 * ex) * rr * rr * rij_2 + 2.f * qij * f * rr;
+                acc[PK_ED1] += (double)(-qij * f * rr);
+                acc[PK_ED2] += (double)(qij * (2.f * f + 2.f * a.alpha2PI * a.alpha2 * ex) * rr * rr * rij_2 +
+                                        2.f * qij * f * rr);
             }
-#pragma unroll
-            for (int k = 0; k < 6; ++k) acc[PK_EVIR + k] += se * dd[k];
         }
         s += se;
+    }
+    if constexpr (FULL) {
+#pragma unroll
+        for (int k = 0; k < 6; ++k) acc[PK_VIR + k] += (double)(s * dd[k]);
     }
     fix += (double)(s * dx);
     fiy += (double)(s * dy);
@@ -838,7 +852,7 @@ __device__ __forceinline__ void packed_term(const PackedArgs& a, const float2* _
 template <bool LUSTIG, bool FULL, int G, typename At>
 __device__ __forceinline__ void packed_segment(const PackedArgs& a, const float2* __restrict__ ljt,
                                                const PackedOwner& o, int first, int count, At at,
-                                               float (&acc)[PK_N], double& fix, double& fiy, double& fiz) {
+                                               double (&acc)[PK_N], double& fix, double& fiy, double& fiz) {
     if (first >= count) return;
     int j = at(first);
     int jn = first + G < count ? at(first + G) : j;
@@ -861,9 +875,9 @@ __global__ __launch_bounds__(256) void k_pairs_packed(PackedArgs a) {
     const int tid = blockIdx.x * blockDim.x + threadIdx.x;
     const int i = tid / G;
     const int sub = tid % G;
-    float acc[PK_N];
+    double acc[PK_N];
 #pragma unroll
-    for (int k = 0; k < PK_N; ++k) acc[k] = 0.f;
+    for (int k = 0; k < PK_N; ++k) acc[k] = 0.0;
     double fix = 0.0, fiy = 0.0, fiz = 0.0;
     if (i < a.n) {
         const int32_t* row = a.nb + (int64_t)i * a.stride;
@@ -908,27 +922,29 @@ __global__ __launch_bounds__(256) void k_pairs_packed(PackedArgs a) {
         for (int k = 0; k < PK_N; ++k) red[k][wid] = (double)acc[k];
     }
     __syncthreads();
-    // Thread t < 25 adds term T_LJ_E + t: the LJ block (E, ELEC, DUDV, D2UDV,
+    // Thread t < 25 forms term T_LJ_E + t: the LJ block (E, ELEC, DUDV, D2UDV,
     // VIR[9]) then the Ewald block (E, D1, D2, VIR[9]); VIR[3a+b] is the
-    // symmetric component (a, b).
+    // symmetric component (a, b).  The fused kernel keeps one virial for both
+    // terms (the step uses only their sum): it goes to the LJ block, the
+    // Ewald block's virial stays zero.
     constexpr int NT = (T_EW_VIR + 9) - T_LJ_E;
     static_assert(NT == 25, "LJ and Ewald term blocks are contiguous");
     const int t = threadIdx.x;
     if (t < NT) {
         int src;
         if (t < 4) src = t == 0 ? PK_VE : t == 1 ? -1 : t == 2 ? PK_VDUDV : PK_VD2UDV;
-        else if (t < 13 || t >= 16) {
-            const int base = t < 13 ? PK_VVIR : PK_EVIR;
-            const int v = t < 13 ? t - 4 : t - 16;
+        else if (t < 13) {
+            const int v = t - 4;
             const int ra = v / 3, cb = v % 3;
             const int lo = min(ra, cb), hi = max(ra, cb);
-            src = base + (lo == 0 ? hi : lo == 1 ? 2 + hi : 5);
-        } else src = PK_EE + (t - 13);
-        if (src >= 0) {
-            double s = 0.0;
+            src = PK_VIR + (lo == 0 ? hi : lo == 1 ? 2 + hi : 5);
+        } else if (t < 16) src = PK_EE + (t - 13);
+        else src = -1;
+        double s = 0.0;
+        if (src >= 0)
             for (int w = 0; w < (int)(blockDim.x / WAVE); ++w) s += red[src][w];
-            if (s != 0.0) unsafeAtomicAdd(a.terms + T_LJ_E + t, s);
-        }
+        // the block's sums; red_finish adds the grid's in block order
+        if ((int)gridDim.x <= a.red.cap) red_part(a.red, RS_PACKED)[(size_t)t * a.red.cap + blockIdx.x] = s;
     }
 }
 
@@ -955,6 +971,7 @@ static int pairs_packed(Ctx* c, const PairArgs& pa, int lustig) {
     a.tb = pa.tb;
     a.force = pa.force;
     a.terms = pa.terms;
+    a.red = pa.red;
     for (int d = 0; d < 3; ++d) {
         const double L = c->box.L[d];
         a.Lh[d] = (float)L;
@@ -971,7 +988,8 @@ static int pairs_packed(Ctx* c, const PairArgs& pa, int lustig) {
     if (!c->pack_reuse)
         hipLaunchKernelGGL(k_pack_sites, dim3((n + 255) / 256), dim3(256), 0, c->stream, n, c->d_pos, c->d_q,
                            c->d_type, c->d_mol, c->d_com, rec, off);
-    const int lanes = c->pair_lanes == 32 || c->pair_lanes == 64 ? c->pair_lanes : 16;
+    const int pl = c->pair_lanes;
+    const int lanes = pl == 4 || pl == 8 || pl == 32 || pl == 64 ? pl : 16;
     const dim3 grid((unsigned)(((int64_t)n * lanes + 255) / 256));
 #define SMD_PACKED(G)                                                                              \
     do {                                                                                           \
@@ -980,10 +998,12 @@ static int pairs_packed(Ctx* c, const PairArgs& pa, int lustig) {
     } while (0)
     if (lanes == 64) SMD_PACKED(64);
     else if (lanes == 32) SMD_PACKED(32);
+    else if (lanes == 8) SMD_PACKED(8);
+    else if (lanes == 4) SMD_PACKED(4);
     else SMD_PACKED(16);
 #undef SMD_PACKED
     SMD_HIP(c, hipGetLastError());
-    return SMD_OK;
+    return red_finish(c, RS_PACKED, grid.x, (T_EW_VIR + 9) - T_LJ_E, a.terms + T_LJ_E);
 }
 
 // Fused LJ + Ewald real space for the MD step (one pass over the list).

@@ -35,18 +35,25 @@ static double* ind_terms(Ctx* c) { return c->d_polar_terms + POLAR_TERMS; }
 // Part A: everything up to the solve's first iteration.
 static int part_a(Ctx* c, bool on_device_rebuild) {
     const smd_polar_md_params& p = c->pmd;
+    RedDefer defer(c);   // one kernel adds the part's reductions at its end
     int rc = md_head_enqueue(c);
     if (!rc) rc = pairlist_build(c, p.cutoff, p.cut_buff, p.use_cells, on_device_rebuild);
     if (!rc) rc = vdw_enqueue(c, &c->pmd_vdw, p.prec);
     if (!rc) rc = frames_run(c);
     if (!rc) rc = multipole_enqueue(c, mp_terms(c));
     if (!rc) rc = induce_begin(c, &p.induce, false);
+    if (!rc) rc = red_flush(c);
     return rc;
 }
 
 // Part C: induction terms and the integration tail.
 static int part_c(Ctx* c) {
-    int rc = induction_enqueue(c, ind_terms(c));
+    int rc;
+    {
+        RedDefer defer(c);
+        rc = induction_enqueue(c, ind_terms(c));
+        if (!rc) rc = red_flush(c);
+    }
     if (!rc) rc = md_tail_enqueue(c);
     return rc;
 }
@@ -215,6 +222,7 @@ extern "C" int smd_polar_md_run(smd_ctx* c, int64_t nsteps, smd_polar_md_report*
         if (graph) SMD_HIP(c, hipGraphLaunch(c->pmd_c, c->stream));
         else if ((rc = part_c(c))) return rc;
         c->md_step++;
+        c->ind_last_converged = false;   // the solve was of the positions before the move
     }
     SMD_HIP(c, hipStreamSynchronize(c->stream));
     if ((rc = check_status(c))) return rc;

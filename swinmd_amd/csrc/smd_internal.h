@@ -23,6 +23,29 @@ constexpr double PI = 3.14159265358979323846;       // src/NEMD_defs.h:77
 constexpr double SQRT_PI = 1.7724538509055160273;   // src/NEMD_defs.h:78
 constexpr double BOLTZ = 0.0083145112;              // src/NEMD_defs.h:108
 
+// Reduction sites of block_part (see there).
+enum RedSite : int {
+    RS_BOND = 0, RS_ANGLE, RS_UB, RS_EW_INTRA, RS_EW_SURF, RS_RECIP, RS_MP_REAL, RS_MP_INTRA, RS_MP_SURF,
+    RS_MP_LONG, RS_IND_REAL, RS_IND_INTRA, RS_IND_SURF, RS_IND_LONG, RS_PAIR_LJ, RS_PAIR_EW, RS_PACKED,
+    // the torque kernel's dU/dV and torque sums, one pair per launch of a step
+    RS_MP_DUDV, RS_MP_TRQ, RS_MP_DUDV_L, RS_MP_TRQ_L, RS_IND_DUDV, RS_IND_TRQ, RS_IND_DUDV_L, RS_IND_TRQ_L,
+    RS_COUNT
+};
+constexpr int RED_NV = 25;
+struct Red {
+    double* base = nullptr;
+    int cap = 0;                  // blocks per site
+};
+inline size_t red_doubles(int cap) { return (size_t)RS_COUNT * RED_NV * cap; }
+// A finish that waits for red_flush (inside a step, where one kernel then
+// adds every site's partials).
+struct RedPending {
+    int site;
+    unsigned nb;
+    int nv;
+    double* dst;
+};
+
 // Device status bits (set by kernels, read by the host after a sync).
 enum : unsigned {
     ST_LIST_OVERFLOW = 1u,
@@ -102,6 +125,9 @@ struct Ctx {
     // block stores its sums, and a fixed-order total replaces FP64 atomics, so
     // the trajectory does not depend on block scheduling.
     double* d_part = nullptr;
+    Red red;                      // reduction arena (block_part, red_finish)
+    bool red_defer = false;       // red_finish queues instead of launching (red_flush launches)
+    std::vector<RedPending> red_pending;
     double* static_scratch = nullptr;   // this device's address of ewald.hip's static scratch array
     double* part_vom = nullptr;   // [4][nb_atoms]: k_step_head
     double* part_kin = nullptr;   // [KIN_PARTS][nb_mols64]: kinetics kernels
@@ -265,6 +291,7 @@ struct Ctx {
     int pmd_last_iters = 0;             // the previous step's solve iterations
     smd_induce_params ind_last{};   // parameters of the last smd_induce
     bool ind_last_converged = false;
+    bool ind_table_built = false;   // a minimised solve has filled the coupling table
 };
 
 }  // namespace smd
@@ -374,25 +401,6 @@ __device__ __forceinline__ T wave_sum(T v) {
     return v;
 }
 
-// Sum NV values over the block and add them atomically to dst[0..NV).
-template <int NV>
-__device__ __forceinline__ void block_add(double (&v)[NV], double* dst) {
-    __shared__ double red[NV][16];
-    const int lane = threadIdx.x % WAVE, wid = threadIdx.x / WAVE;
-    const int nw = (blockDim.x + WAVE - 1) / WAVE;
-#pragma unroll
-    for (int k = 0; k < NV; ++k) v[k] = wave_sum(v[k]);
-    if (lane == 0)
-#pragma unroll
-        for (int k = 0; k < NV; ++k) red[k][wid] = v[k];
-    __syncthreads();
-    if (threadIdx.x < NV) {
-        double s = 0.0;
-        for (int w = 0; w < nw; ++w) s += red[threadIdx.x][w];
-        if (s != 0.0) unsafeAtomicAdd(dst + threadIdx.x, s);
-    }
-}
-
 // ---- FP32 erfc of the mixed-precision real-space kernels ----------------------
 // erfc(z) = exp(-z^2) t P(t), t = 1 / (1 + p z): the form of Abramowitz &
 // Stegun 7.1.26 with a degree-7 P whose coefficients were fitted (weighted
@@ -475,6 +483,24 @@ __device__ __forceinline__ void fixed_totals(const double* part, int stride, int
     __syncthreads();
 }
 
+// ---- deterministic block sums ---------------------------------------------
+// Each kernel that reduces energies or virials over its grid owns a site of
+// the context's reduction arena (at most RED_NV values for at most cap
+// blocks).  block_part stores the block's sums there with plain stores; the
+// host then enqueues red_finish on the same stream, one small kernel that
+// adds the partials to the destination in block order.  The totals do not
+// depend on block scheduling, and no block waits on another: a ticket and
+// the device-wide release fence it needs cost more than the extra launch on
+// a multi-XCD part.  Sites that can run at the same time are distinct.
+__device__ __forceinline__ double* red_part(const Red& r, int site) { return r.base + (size_t)site * RED_NV * r.cap; }
+
+template <int NV>
+__device__ __forceinline__ void block_part(double (&v)[NV], const Red& r, int site) {
+    static_assert(NV <= RED_NV, "block_part: too many values for a reduction site");
+    if ((int)gridDim.x > r.cap) return;   // grid-uniform; red_finish reports it
+    block_store<NV>(v, red_part(r, site), r.cap);
+}
+
 __device__ __forceinline__ void atomic_add3(double* f, double x, double y, double z) {
     unsafeAtomicAdd(f + 0, x);
     unsafeAtomicAdd(f + 1, y);
@@ -497,6 +523,18 @@ int ewald_surface(Ctx* c);
 enum : unsigned { RECIP_SFAC = 1u, RECIP_REDUCE = 2u, RECIP_FORCE = 4u, RECIP_ALL = 7u };
 int ewald_recip(Ctx* c, int prec, int lustig, unsigned passes = RECIP_ALL);
 int static_scratch_init(Ctx* c);
+// Adds the nv block partials of a site's last launch (nblocks blocks) to dst,
+// in block order, on the context's stream.
+int red_finish(Ctx* c, int site, unsigned nblocks, int nv, double* dst);
+// Launches one kernel that adds every queued finish (in queue order, each in
+// block order) and clears the queue; no-op when the queue is empty.
+int red_flush(Ctx* c);
+// Queues finishes for the scope (a step's kernels), dropping any left over.
+struct RedDefer {
+    Ctx* c;
+    explicit RedDefer(Ctx* ctx) : c(ctx) { c->red_defer = true; c->red_pending.clear(); }
+    ~RedDefer() { c->red_defer = false; c->red_pending.clear(); }
+};
 int shake_run(Ctx* c, const smd_integrator_params* p, bool fnh_on_device, double fnh, bool md_step = false);
 int kinetics_run(Ctx* c, bool rewrap, const smd_integrator_params* md = nullptr);
 int lf_atoms_run(Ctx* c, const smd_integrator_params* p, double fnh, bool md_step);

@@ -20,7 +20,8 @@ class _Ctx:
 
 
 class _Md:
-    """Steps cost less with each knob on; pair_packed matters most."""
+    """Steps cost less with each knob on (pair_packed matters most) and
+    with the default lane count."""
 
     def __init__(self):
         self.ctx = _Ctx()
@@ -32,6 +33,7 @@ class _Md:
         k = self.ctx.knobs
         self.seen.append(dict(k))
         per = 1e-6 + sum(c for name, c in self.cost.items() if not k.get(name, 1))
+        per += {0: 0.0, 8: 0.5e-6, 32: 0.8e-6}[k.get("pair_lanes", 0)]
         time.sleep(n * per)
 
 
@@ -41,11 +43,11 @@ def test_every_combination_is_timed_and_the_fastest_kept():
     combos = [dict(zip(DEFAULT_KNOBS, v)) for v in itertools.product(*DEFAULT_KNOBS.values())]
     assert [t["config"] for t in table] == combos
     assert all(len(t["rates"]) == 2 for t in table)
-    assert best == {"pair_packed": 1, "recip_overlap": 1, "md_fused_shake": 1}
+    assert best == {"pair_packed": 1, "recip_overlap": 1, "md_fused_shake": 1, "pair_lanes": 0}
     assert md.ctx.knobs == best
     # the slowest stand-in combination (all off) has the lowest median
     slow = min(table, key=lambda t: t["median_steps_per_s"])
-    assert slow["config"] == {"pair_packed": 0, "recip_overlap": 0, "md_fused_shake": 0}
+    assert slow["config"] == {"pair_packed": 0, "recip_overlap": 0, "md_fused_shake": 0, "pair_lanes": 32}
 
 
 def test_a_custom_knob_set():

@@ -186,4 +186,26 @@ def test_learned_guess_errors():
     with pytest.raises(_lib.SmdError) as e:
         ind.induce()
     assert e.value.code == _lib.SMD_EINVAL
+    # a frame is only the converged solve of the current state: a new box
+    # or moved positions make the solve's coupling table stale
+    import ctypes as C
+
+    def set_box(box):
+        b = np.ascontiguousarray(box, dtype=np.float64)
+        ctx.check(ctx.lib.smd_set_box(ctx.h, b.ctypes.data_as(C.c_void_p)))
+
+    ind.params.guess = 2
+    assert ind.induce()[1]
+    net.add_frame()
+    set_box(ctx.system.box * 1.001)
+    with pytest.raises(_lib.SmdError) as e:
+        net.add_frame()
+    assert e.value.code == _lib.SMD_ESTATE
+    set_box(ctx.system.box)
+    ctx.set_pairlist(CUT, 0.05, True)
+    assert ind.induce()[1]
+    ctx.set_positions(ctx.positions() + 1e-4)
+    with pytest.raises(_lib.SmdError) as e:
+        net.add_frame()
+    assert e.value.code == _lib.SMD_ESTATE
     ctx.close()

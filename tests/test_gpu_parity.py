@@ -415,6 +415,36 @@ def test_md_packed_pair_records_match_unpacked():
         assert rel(out["packed"][1][k], out["plain"][1][k]) <= 1e-5
 
 
+@pytest.mark.parametrize("packed", [1, 0])
+def test_every_pair_lane_count_runs_and_agrees(packed):
+    """Each lane count smd_set_tuning accepts is the one the pair kernel
+    runs (packed mixed and general kernels alike): the start forces and
+    energies agree across 4..64 lanes to FP64 summation order."""
+    from swinmd_amd import systems as S
+    from swinmd_amd import _lib as L
+    from swinmd_amd.forces import WaterMD
+    s = S.water(1000, density_mol=33.4, seed=11, lattice=(10, 10, 10))
+    kcut = S.kcut_for(27, s.box[0])
+    out = {}
+    for lanes in (4, 8, 16, 32, 64):
+        print(f"case pair lanes {lanes} packed {packed}", flush=True)
+        ctx = _ctx(s)
+        ctx.set_tuning("pair_packed", packed)
+        ctx.set_tuning("pair_lanes", lanes)
+        md = WaterMD(ctx, cutoff=0.9, cut_buff=0.1, alpha=3.5, kcut=kcut, dt=0.002, temperature=300.0,
+                     prec=L.PREC_MIXED, use_graph=False)
+        md.init()
+        out[lanes] = (ctx.forces(), md.run(0))
+        ctx.close()
+    f16, r16 = out[16]
+    for lanes, (f, r) in out.items():
+        assert rel_rms(f, f16) <= 1e-12, lanes
+        for k in ("e_lj", "e_real"):
+            assert rel(r[k], r16[k]) <= 1e-12, (lanes, k)
+    with pytest.raises(L.SmdError):
+        _ctx(s).set_tuning("pair_lanes", 12)
+
+
 def test_md_autotuner_picks_a_candidate_and_keeps_the_physics():
     """The launch-shape tuner times every combination of its knobs on the
     live run and leaves the fastest set; the step then gives the same
@@ -431,7 +461,8 @@ def test_md_autotuner_picks_a_candidate_and_keeps_the_physics():
     md.init()
     best, table = tune_md_step(md, steps=10, warmup=4, rounds=1)
     print("autotune", best, [(t["config"], round(t["median_steps_per_s"])) for t in table], flush=True)
-    assert len(table) == 8 and all(t["median_steps_per_s"] > 0 for t in table)
+    combos = int(np.prod([len(v) for v in DEFAULT_KNOBS.values()]))
+    assert len(table) == combos and all(t["median_steps_per_s"] > 0 for t in table)
     assert best in [t["config"] for t in table] and set(best) == set(DEFAULT_KNOBS)
     # from the tuned run's state: the chosen forms and the default forms agree
     pos, vel = ctx.positions(), ctx.velocities()
