@@ -61,9 +61,12 @@ FLOP_PER_ATOM_K_FORCE = 10
 #     r, v and writes v).
 BYTES_PER_DIRECTED_PAIR_TMU = 20
 BYTES_PER_SITE_CG = 11 * 24
-#   A v alone: the table entries, and per site v read, z written, the site's
-#     position and polarizability and its row count (24 + 24 + 24 + 8 + 4).
-BYTES_PER_SITE_APPLY = 84
+#   A v alone (mixed solve): 4 B per table entry (the partner index; the
+#     Thole factors are recomputed), and per site the FP32 direction and the
+#     two hi/lo position records (16 + 32), v read and z written in FP64
+#     (24 + 24), the polarizability and the row count (8 + 4).
+BYTES_PER_ENTRY_APPLY = 4
+BYTES_PER_SITE_APPLY = 108
 
 
 def log(msg):
@@ -302,8 +305,12 @@ def bench_polar_md(args):
     pos_eq, vel_eq = ctx.positions(), ctx.velocities()
     ctx.close()
 
+    # guess 0: the reference's Pi = 1; 1: the previous step's dipoles; 2: alpha
+    # E0; 3: the learned guess; 4: the previous dipoles moved by alpha dE0
     paths = [("mixed_learned", L.PREC_MIXED, 3), ("fp64_reference_start", L.PREC_FP64, 0),
-             ("fp64_learned", L.PREC_FP64, 3), ("mixed_reference_start", L.PREC_MIXED, 0)]
+             ("fp64_learned", L.PREC_FP64, 3), ("mixed_reference_start", L.PREC_MIXED, 0),
+             ("mixed_alpha_e0_start", L.PREC_MIXED, 2), ("mixed_previous_start", L.PREC_MIXED, 1),
+             ("mixed_extrapolated_start", L.PREC_MIXED, 4)]
     if args.no_fp64:
         paths = paths[:1]
     res = {}
@@ -391,11 +398,12 @@ def bench_polar_md(args):
                        "traffic": measured_traffic(f"polar_water{nmol}").get(top), "traffic_unit": "bytes/launch",
                        "flop_per_launch": models[top], "avg_us": round(iso[top], 2),
                        "pairs": head["pairs_in_cut"]}
-    alg = 12 * 2 * head["ind_pairs"] + BYTES_PER_SITE_APPLY * n
+    alg = BYTES_PER_ENTRY_APPLY * 2 * head["ind_pairs"] + BYTES_PER_SITE_APPLY * n
     out["cg_apply"] = {"us": round(iso["cg_apply"], 2), "algorithmic_bytes": alg,
                        "achieved_GBs": round(alg / (iso["cg_apply"] * 1e-6) / 1e9, 1),
                        "hbm_frac": round(alg / (iso["cg_apply"] * 1e-6) / 1e9 / PEAK_HBM_GBS, 4),
-                       "note": "the mixed solve's A v: 12 B per directed pair of the FP32 coupling table"}
+                       "note": "the mixed solve's A v: a 4 B partner index per directed pair, Thole factors "
+                               "recomputed in FP32 from hi/lo site records"}
     if not args.no_cpu_baseline:
         log("polarizable cpu baseline (reference, 1 core, 1,024 molecules)")
         out["cpu_baseline"] = polar_cpu_baseline_md(S.polar_water(1024) if nmol != 1024 else system)

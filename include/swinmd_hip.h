@@ -252,7 +252,8 @@ typedef struct {
     double tol;            /* CG stops when |r| < tol; 0 -> 1e-14 (ConjugateGradient) */
     int32_t max_iter;      /* 0 -> 3n, the reference's bound */
     int32_t guess;         /* 0: Pi = 1 (the reference); 1: the previous solution; 2: alpha E0;
-                              3: the learned guess (smd_mlp_*) */
+                              3: the learned guess (smd_mlp_*); 4: the previous solution plus
+                              alpha (E0 - E0 of the previous guess-4 solve) */
     int32_t prec;          /* smd_prec: SMD_PREC_FP64, or SMD_PREC_MIXED (FP32 coupling table and
                               A v, FP64 vectors, scalars and residual; preconditioned solve only) */
     int32_t precond;       /* 0: the reference's CG; 1: CG preconditioned by M^-1 = diag(alpha)

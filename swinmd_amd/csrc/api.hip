@@ -41,19 +41,18 @@ int check_status(Ctx* c) {
     return SMD_OK;
 }
 
-// One wave per value: lane l adds partials l, l + 64, ... in order (the
-// loads unrolled, so several are in flight), then a shuffle tree.
-__global__ __launch_bounds__(1024) void k_red_finish(const double* __restrict__ part, int cap, int nb, int nv,
-                                                     double* dst) {
-    const int w = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
-    for (int k = w; k < nv; k += blockDim.x / WAVE) {
-        const double* p = part + (size_t)k * cap;
-        double s = 0.0;
-#pragma unroll 8
-        for (int i = lane; i < nb; i += WAVE) s += p[i];
-        s = wave_sum(s);
-        if (lane == 0) dst[k] += s;
-    }
+// One wave per value (rows_total's order).
+__global__ __launch_bounds__(FINISH_THREADS) void k_red_finish(const double* __restrict__ part, int cap, int nb,
+                                                               int nv, double* dst, int split, double* dst2) {
+    rows_total(part, cap, nb, nv, dst, split, dst2);
+}
+
+int rows_finish(Ctx* c, hipStream_t s, const double* part, int stride, int nb, int nrows, double* dst, int split,
+                double* dst2) {
+    if (nb <= 0 || nrows <= 0) return SMD_OK;
+    hipLaunchKernelGGL(k_red_finish, dim3(1), dim3(FINISH_THREADS), 0, s, part, stride, nb, nrows, dst, split, dst2);
+    SMD_HIP(c, hipGetLastError());
+    return SMD_OK;
 }
 
 int red_finish(Ctx* c, int site, unsigned nblocks, int nv, double* dst) {
@@ -66,8 +65,9 @@ int red_finish(Ctx* c, int site, unsigned nblocks, int nv, double* dst) {
         c->red_pending.push_back(RedPending{site, nblocks, nv, dst});
         return SMD_OK;
     }
-    hipLaunchKernelGGL(k_red_finish, dim3(1), dim3(1024), 0, c->stream, c->red.base + (size_t)site * RED_NV * c->red.cap,
-                       c->red.cap, (int)nblocks, nv, dst);
+    hipLaunchKernelGGL(k_red_finish, dim3(1), dim3(FINISH_THREADS), 0, c->stream,
+                       c->red.base + (size_t)site * RED_NV * c->red.cap, c->red.cap, (int)nblocks, nv, dst, 1 << 30,
+                       (double*)nullptr);
     SMD_HIP(c, hipGetLastError());
     return SMD_OK;
 }
@@ -146,7 +146,7 @@ static void release(Ctx* c) {
                     c->mp_ks.fpart, c->ind_ks.kvec, c->ind_ks.kmod2, c->ind_ks.expk, c->ind_ks.part,
                     c->ind_ks.kf, c->ind_ks.fpart, c->d_pack, c->d_bd_bonds, c->d_bd_angles,
                     c->d_bd_bond_off, c->d_bd_angle_off, c->d_bd_par, c->d_bd_out, c->d_stress, c->d_real,
-                    c->d_it_cnt, c->d_it_j, c->d_it_a, c->d_part, c->d_cg32, c->d_polar_terms, c->d_site32, c->d_pind32,
+                    c->d_it_cnt, c->d_it_j, c->d_it_a, c->d_part, c->d_cg32, c->d_ind_rec32, c->d_e0_prev, c->d_polar_terms, c->d_site32, c->d_pind32,
                     c->red.base};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
@@ -427,6 +427,8 @@ extern "C" int smd_set_tuning(smd_ctx* c, const char* name, int value) {
     }
     if (c->graph_exec) { (void)hipGraphExecDestroy(c->graph_exec); c->graph_exec = nullptr; }
     if (c->graph) { (void)hipGraphDestroy(c->graph); c->graph = nullptr; }
+    if (c->pmd_a) { (void)hipGraphExecDestroy(c->pmd_a); c->pmd_a = nullptr; }
+    if (c->pmd_c) { (void)hipGraphExecDestroy(c->pmd_c); c->pmd_c = nullptr; }
     return SMD_OK;
 }
 

@@ -242,11 +242,11 @@ int ewald_intra(Ctx* c, int lustig) {
 
 // ---- surface dipole ---------------------------------------------------------
 
-// The dipole sums feed every atom's force: block partials summed by the
-// last block in a fixed order, so the forces do not depend on scheduling.
+// The dipole sums feed every atom's force: block partials added in block
+// order by a finisher launch, so the forces do not depend on scheduling.
 __global__ __launch_bounds__(256) void k_surf_sums(int n, const double* __restrict__ pos, const double* __restrict__ q,
                                                    const int32_t* __restrict__ mol, const double* __restrict__ com,
-                                                   double* terms, double* part, int* ticket) {
+                                                   double* part) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     double v[6] = {0, 0, 0, 0, 0, 0};
     if (i < n) {
@@ -255,7 +255,6 @@ __global__ __launch_bounds__(256) void k_surf_sums(int n, const double* __restri
         for (int d = 0; d < 3; ++d) { v[d] = qi * pos[3 * i + d]; v[3 + d] = qi * com[3 * m + d]; }
     }
     block_store<6>(v, part, gridDim.x);
-    if (last_block(ticket)) fixed_totals(part, gridDim.x, gridDim.x, 6, terms + T_SU_QR);
 }
 
 __global__ __launch_bounds__(256) void k_surf_force(int n, const double* __restrict__ pos, const double* __restrict__ q,
@@ -281,8 +280,10 @@ int ewald_surface(Ctx* c) {
     const double volume = c->box.L[0] * c->box.L[1] * c->box.L[2];
     const double coeff = 2.0 * PI * (1.0 / volume) / 3.0;
     const dim3 g((c->n + 255) / 256);
-    hipLaunchKernelGGL(k_surf_sums, g, dim3(256), 0, c->stream, c->n, c->d_pos, c->d_q, c->d_mol, c->d_com, c->d_terms,
-                       c->part_su, c->d_flag + 4);
+    hipLaunchKernelGGL(k_surf_sums, g, dim3(256), 0, c->stream, c->n, c->d_pos, c->d_q, c->d_mol, c->d_com,
+                       c->part_su);
+    int rc = rows_finish(c, c->stream, c->part_su, (int)g.x, (int)g.x, 6, c->d_terms + T_SU_QR);
+    if (rc) return rc;
     hipLaunchKernelGGL(k_surf_force, g, dim3(256), 0, c->stream, c->n, c->d_pos, c->d_q, c->d_force, c->d_terms, coeff,
                        c->red);
     SMD_HIP(c, hipGetLastError());

@@ -59,8 +59,12 @@ struct IndArgs {
     int tstride;
     int32_t* tcnt;              // [n]
     int32_t* tj;                // [n][tstride]
-    double2* tab;               // [n][tstride] (a, b): one 16-byte load per entry
-    float2* tab32;              // mixed solve: the same factors in FP32, [n][tstride] (else null)
+    double2* tab;               // [n][tstride] (a, b): one 16-byte load per entry (FP64 solve)
+    // mixed solve: the factors are recomputed in FP32 from hi/lo site records
+    // rec32[2i] = (x_hi, y_hi, z_hi, 1/sqrt(alpha)), rec32[2i+1] = (x_lo, y_lo, z_lo, 0)
+    int mixed;
+    float4* rec32;
+    float Lh[3], Ll[3], invL[3];
 };
 
 // ---- frames -------------------------------------------------------------------
@@ -263,27 +267,51 @@ __global__ __launch_bounds__(256) void k_apply_a(IndArgs a, const double* __rest
 // Sum of n block partials in a fixed order (lane-strided sums, then a
 // shuffle tree), computed by the block's first wave and shared through LDS:
 // every block gets the same value, whatever the scheduling.
+__device__ __forceinline__ double wave_total(const double* p, int n) {
+    // eight loads in flight per lane; the additions keep the order k, k + WAVE, ...
+    double s = 0.0;
+    int k = threadIdx.x % WAVE;
+    for (; k + 7 * WAVE < n; k += 8 * WAVE) {
+        double q[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) q[u] = p[k + u * WAVE];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) s += q[u];
+    }
+    for (; k < n; k += WAVE) s += p[k];
+    return wave_sum(s);
+}
+
 __device__ __forceinline__ double block_total(const double* p, int n) {
     __shared__ double tot;
     if (threadIdx.x < WAVE) {
-        // eight loads in flight per lane; the additions keep the order k, k + WAVE, ...
-        double s = 0.0;
-        int k = threadIdx.x;
-        for (; k + 7 * WAVE < n; k += 8 * WAVE) {
-            double q[8];
-#pragma unroll
-            for (int u = 0; u < 8; ++u) q[u] = p[k + u * WAVE];
-#pragma unroll
-            for (int u = 0; u < 8; ++u) s += q[u];
-        }
-        for (; k < n; k += WAVE) s += p[k];
-        s = wave_sum(s);
+        const double s = wave_total(p, n);
         if (threadIdx.x == 0) tot = s;
     }
     __syncthreads();
     const double v = tot;
     __syncthreads();   // tot may be rewritten by the next call
     return v;
+}
+
+// Up to four totals at once, wave w summing buffer w in block_total's order
+// (the same values, with one load latency chain instead of NB).
+template <int NB>
+__device__ __forceinline__ void block_totals(const double* p0, int n0, const double* p1, int n1, const double* p2,
+                                             int n2, double (&out)[NB]) {
+    static_assert(NB >= 2 && NB <= 3, "two or three totals");
+    __shared__ double tot[NB];
+    const int w = threadIdx.x / WAVE;
+    if (w < NB) {
+        const double* p = w == 0 ? p0 : (w == 1 ? p1 : p2);
+        const int n = w == 0 ? n0 : (w == 1 ? n1 : n2);
+        const double s = wave_total(p, n);
+        if (threadIdx.x % WAVE == 0) tot[w] = s;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int b = 0; b < NB; ++b) out[b] = tot[b];
+    __syncthreads();
 }
 
 __device__ __forceinline__ void block_partial(double v, double* part) {
@@ -348,12 +376,97 @@ __global__ __launch_bounds__(256) void k_ind_table(IndArgs a) {
                 const int at = count + __popcll(m & ((1ull << sub) - 1ull));
                 const int64_t e = (int64_t)i * a.tstride + at;
                 a.tj[e] = j;
-                if (a.tab32) a.tab32[e] = make_float2((float)ca, (float)cb);
-                else a.tab[e] = make_double2(ca, cb);
+                a.tab[e] = make_double2(ca, cb);
             }
             count += __popcll(m);
         }
         if (sub == 0) a.tcnt[i] = count;
+    }
+}
+
+// The mixed solve's permanent field and coupling table in one pass over the
+// rows, with FP32 pair math.  Which pairs count is decided in FP64 (r^2 <=
+// cutoffInd^2, as in the FP64 kernels), and the field sums in FP64.  The
+// table holds the partners only: the mixed A v recomputes their Thole
+// factors.  The
+// field of a pair is field_from's expression gathered by powers of r:
+// f = (q c3 + c5 (3 r.d - 3 tr Q) + 15 c7 r.Q.r) r - c3 d - 6 c5 Q r with
+// cn = lambda_n / r^n.
+template <int G>
+__global__ __launch_bounds__(256) void k_field_table32(IndArgs a) {
+    static_assert(WAVE % G == 0, "a lane group must not straddle waves");
+    const int tid = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = tid / G, sub = tid % G;
+    const int gbase = (threadIdx.x % WAVE) - sub;
+    const uint64_t gmask = G == 64 ? ~0ull : ((1ull << G) - 1ull);
+    double f[3] = {0.0, 0.0, 0.0};
+    int count = 0;
+    if (i < a.n) {
+        const int own = min(a.nb_own[i], a.stride);
+        const int cnt = min(a.nb_cnt[i], a.stride);
+        const int e0 = a.excl_off[i], e1 = a.excl_off[i + 1];
+        const double xi = a.pos[3 * i], yi = a.pos[3 * i + 1], zi = a.pos[3 * i + 2];
+        const float pi = (float)(a.polar[i] > POLMIN ? a.polar[i] : POLMIN);
+        const float damp = (float)a.a;
+        const float inv_sqc = (float)(1.0 / SQRT_COULOMB);
+        const int total = cnt + (e1 - e0);
+        for (int kb = 0; kb < total; kb += G) {
+            const int k = kb + sub;
+            bool ok = false;
+            int j = 0;
+            if (k < total) {
+                j = k < cnt ? row_partner(a, i, own, k) : a.excl[e0 + (k - cnt)];
+                double dx = xi - a.pos[3 * j], dy = yi - a.pos[3 * j + 1], dz = zi - a.pos[3 * j + 2];
+                pbc_near(dx, dy, dz, a.box);
+                const double rd64 = sqrt(dx * dx + dy * dy + dz * dz);
+                if (rd64 * rd64 <= a.cut2) {   // the FP64 kernels' test, bit for bit
+                    ok = true;
+                    if (k < cnt) {
+                        // the permanent field (E0 leaves the excluded partners out)
+                        const float rx = (float)dx, ry = (float)dy, rz = (float)dz, r2 = (float)(rd64 * rd64);
+                        const float r_1 = __builtin_amdgcn_rsqf(r2);
+                        const float r = r2 * r_1;
+                        const float pj = (float)(a.polar[j] > POLMIN ? a.polar[j] : POLMIN);
+                        const float x = damp * (r2 * r) * __builtin_amdgcn_rsqf(pi * pj);
+                        const float e = __expf(-x);
+                        const float r3 = r_1 * r_1 * r_1, r5 = r3 * r_1 * r_1;
+                        const float l3 = 1.0f - e, l5 = 1.0f - (1.0f + x) * e;
+                        const float l7 = 1.0f - (1.0f + x + 0.6f * x * x) * e;
+                        const float c7 = l7 * r5 * r_1 * r_1;
+                        const float c3f = l3 * r3, c5f = l5 * r5;
+                        const float qj = (float)a.q[j] * inv_sqc;
+                        const double* dj = a.dip + 3 * j;
+                        const double* Q = a.quad + 6 * j;
+                        const float djx = (float)dj[0], djy = (float)dj[1], djz = (float)dj[2];
+                        const float Qxx = (float)Q[0], Qxy = (float)Q[1], Qxz = (float)Q[2];
+                        const float Qyy = (float)Q[3], Qyz = (float)Q[4], Qzz = (float)Q[5];
+                        const float QRx = Qxx * rx + Qxy * ry + Qxz * rz;
+                        const float QRy = Qxy * rx + Qyy * ry + Qyz * rz;
+                        const float QRz = Qxz * rx + Qyz * ry + Qzz * rz;
+                        const float QRR = rx * QRx + ry * QRy + rz * QRz;
+                        const float rd = rx * djx + ry * djy + rz * djz;
+                        const float s = qj * c3f + c5f * (3.0f * rd - 3.0f * (Qxx + Qyy + Qzz)) + 15.0f * c7 * QRR;
+                        f[0] += (double)(s * rx - c3f * djx - 6.0f * c5f * QRx);
+                        f[1] += (double)(s * ry - c3f * djy - 6.0f * c5f * QRy);
+                        f[2] += (double)(s * rz - c3f * djz - 6.0f * c5f * QRz);
+                    }
+                }
+            }
+            const uint64_t m = (__ballot(ok) >> gbase) & gmask;
+            if (ok) {
+                const int at = count + __popcll(m & ((1ull << sub) - 1ull));
+                a.tj[(int64_t)i * a.tstride + at] = j;
+            }
+            count += __popcll(m);
+        }
+    }
+#pragma unroll
+    for (int o = G / 2; o > 0; o >>= 1)
+#pragma unroll
+        for (int x = 0; x < 3; ++x) f[x] += __shfl_xor(f[x], o, G);
+    if (i < a.n && sub == 0) {
+        a.tcnt[i] = count;
+        for (int x = 0; x < 3; ++x) a.e0[3 * i + x] = f[x];
     }
 }
 
@@ -434,14 +547,29 @@ __global__ __launch_bounds__(256) void k_apply_table(IndArgs a, const double* __
     if (part) block_partial(d, part);   // v.Av of the CG step
 }
 
-// A v of the mixed solve: the FP32 coupling table and an FP32 copy of v
-// (float4 per site), the displacement from the FP64 positions rounded to
-// FP32, FP32 products and FP64 accumulation; v_i / alpha_i and the output in
-// FP64.
+// Hi/lo FP32 site records of the mixed solve: x_hi = float(x), x_lo =
+// float(x - x_hi), and 1/sqrt(alpha) (alpha clamped at POLMIN) in the first
+// record's w.
+__global__ __launch_bounds__(256) void k_pack_ind32(IndArgs a) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= a.n) return;
+    const double x = a.pos[3 * i], y = a.pos[3 * i + 1], z = a.pos[3 * i + 2];
+    const float xh = (float)x, yh = (float)y, zh = (float)z;
+    const double p = a.polar[i] > POLMIN ? a.polar[i] : POLMIN;
+    a.rec32[2 * i] = make_float4(xh, yh, zh, (float)(1.0 / sqrt(p)));
+    a.rec32[2 * i + 1] = make_float4((float)(x - (double)xh), (float)(y - (double)yh), (float)(z - (double)zh), 0.0f);
+}
+
+// A v of the mixed solve over the coupling table's partners: each pair's
+// Thole factors recomputed in FP32 from the hi/lo records (the displacement
+// is (x_hi_j - x_hi_i - n L_hi) + (x_lo_j - x_lo_i - n L_lo), whose first
+// difference is exact once the image is taken), an FP32 copy of v (float4
+// per site), FP64 accumulation; v_i / alpha_i and the output in FP64.  The
+// table stores 4 bytes per entry instead of 12.
 template <int G>
-__global__ __launch_bounds__(256) void k_apply_table32(IndArgs a, const float4* __restrict__ v32,
-                                                       const double* __restrict__ v, double* __restrict__ out,
-                                                       double* part, CgTop top) {
+__global__ __launch_bounds__(256) void k_apply_pos32(IndArgs a, const float4* __restrict__ v32,
+                                                     const double* __restrict__ v, double* __restrict__ out,
+                                                     double* part, CgTop top) {
     if (top.state && top.state[0]) return;
     const int tid = blockIdx.x * blockDim.x + threadIdx.x;
     const int i = tid / G, sub = tid % G;
@@ -449,43 +577,49 @@ __global__ __launch_bounds__(256) void k_apply_table32(IndArgs a, const float4* 
     double pi = 1.0;
     if (i < a.n) {
         const int cnt = a.tcnt[i];
-        const double xi = a.pos[3 * i], yi = a.pos[3 * i + 1], zi = a.pos[3 * i + 2];
         pi = a.polar[i] > POLMIN ? a.polar[i] : POLMIN;
+        const float4 hi = a.rec32[2 * i], lo = a.rec32[2 * i + 1];
+        const float damp = (float)a.a;
         const int32_t* tj = a.tj + (int64_t)i * a.tstride;
-        const float2* tab = a.tab32 + (int64_t)i * a.tstride;
-        auto term = [&](float ca, float cb, double px, double py, double pz, float4 vj) {
-            double dx = xi - px, dy = yi - py, dz = zi - pz;
-            pbc_near(dx, dy, dz, a.box);
-            const float fx = (float)dx, fy = (float)dy, fz = (float)dz;
-            const float rv = ca * (fx * vj.x + fy * vj.y + fz * vj.z);
-            ax += (double)(rv * fx - cb * vj.x);
-            ay += (double)(rv * fy - cb * vj.y);
-            az += (double)(rv * fz - cb * vj.z);
+        auto term = [&](float4 h, float4 l, float4 vj) {
+            const float hx = h.x - hi.x, hy = h.y - hi.y, hz = h.z - hi.z;
+            const float nx = rintf(hx * a.invL[0]), ny = rintf(hy * a.invL[1]), nz = rintf(hz * a.invL[2]);
+            const float dx = (hx - nx * a.Lh[0]) + ((l.x - lo.x) - nx * a.Ll[0]);
+            const float dy = (hy - ny * a.Lh[1]) + ((l.y - lo.y) - ny * a.Ll[1]);
+            const float dz = (hz - nz * a.Lh[2]) + ((l.z - lo.z) - nz * a.Ll[2]);
+            const float r2 = dx * dx + dy * dy + dz * dz;
+            const float r_1 = __builtin_amdgcn_rsqf(r2);
+            const float r3 = r_1 * r_1 * r_1, r5 = r3 * r_1 * r_1;
+            float ca = 3.0f * r5, cb = r3;
+            if (a.damp) {
+                const float x = damp * (r2 * r2 * r_1) * (hi.w * h.w);
+                const float e = __expf(-x);
+                cb = (1.0f - e) * r3;
+                ca = 3.0f * (1.0f - (1.0f + x) * e) * r5;
+            }
+            const float rv = ca * (dx * vj.x + dy * vj.y + dz * vj.z);
+            ax += (double)(rv * dx - cb * vj.x);
+            ay += (double)(rv * dy - cb * vj.y);
+            az += (double)(rv * dz - cb * vj.z);
         };
         int k = sub;
         for (; k + 3 * G < cnt; k += 4 * G) {
             int jj[4];
-            float2 ab[4];
-            double px[4], py[4], pz[4];
-            float4 vj[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) jj[u] = tj[k + u * G];
+            float4 h[4], l[4], vj[4];
 #pragma unroll
             for (int u = 0; u < 4; ++u) {
-                jj[u] = tj[k + u * G];
-                ab[u] = tab[k + u * G];
+                h[u] = a.rec32[2 * jj[u]];
+                l[u] = a.rec32[2 * jj[u] + 1];
+                vj[u] = v32[jj[u]];
             }
 #pragma unroll
-            for (int u = 0; u < 4; ++u) {
-                const int j = jj[u];
-                px[u] = a.pos[3 * j]; py[u] = a.pos[3 * j + 1]; pz[u] = a.pos[3 * j + 2];
-                vj[u] = v32[j];
-            }
-#pragma unroll
-            for (int u = 0; u < 4; ++u) term(ab[u].x, ab[u].y, px[u], py[u], pz[u], vj[u]);
+            for (int u = 0; u < 4; ++u) term(h[u], l[u], vj[u]);
         }
         for (; k < cnt; k += G) {
             const int j = tj[k];
-            const float2 ab = tab[k];
-            term(ab.x, ab.y, a.pos[3 * j], a.pos[3 * j + 1], a.pos[3 * j + 2], v32[j]);
+            term(a.rec32[2 * j], a.rec32[2 * j + 1], v32[j]);
         }
     }
     double acc[3] = {ax, ay, az};
@@ -517,9 +651,9 @@ static void launch_apply(Ctx* c, int lanes, const IndArgs& a, const double* v, d
 static void launch_apply32(Ctx* c, int lanes, const IndArgs& a, const float4* v32, const double* v, double* out,
                            double* part, CgTop top) {
     const dim3 grid((unsigned)(((int64_t)c->n * lanes + 255) / 256)), b256(256);
-    if (lanes == 8) hipLaunchKernelGGL((k_apply_table32<8>), grid, b256, 0, c->stream, a, v32, v, out, part, top);
-    else if (lanes == 32) hipLaunchKernelGGL((k_apply_table32<32>), grid, b256, 0, c->stream, a, v32, v, out, part, top);
-    else hipLaunchKernelGGL((k_apply_table32<16>), grid, b256, 0, c->stream, a, v32, v, out, part, top);
+    if (lanes == 8) hipLaunchKernelGGL((k_apply_pos32<8>), grid, b256, 0, c->stream, a, v32, v, out, part, top);
+    else if (lanes == 32) hipLaunchKernelGGL((k_apply_pos32<32>), grid, b256, 0, c->stream, a, v32, v, out, part, top);
+    else hipLaunchKernelGGL((k_apply_pos32<16>), grid, b256, 0, c->stream, a, v32, v, out, part, top);
 }
 
 // ---- CG -------------------------------------------------------------------------
@@ -573,9 +707,10 @@ __global__ __launch_bounds__(CG_THREADS) void k_cg_step(CgArgs g) {
     if (g.state[0]) return;
     const double* rr_c = PHASE ? g.p_rr1 : g.p_rr0;
     double* rr_d = PHASE ? g.p_rr0 : g.p_rr1;
-    const double dotVZ = block_total(g.p_vz, g.nparts_mv);
-    const double c = block_total(rr_c, g.nparts_vec);
-    const double t = c / dotVZ;
+    double tot[2];   // v.Av, c
+    block_totals<2>(g.p_vz, g.nparts_mv, rr_c, g.nparts_vec, nullptr, 0, tot);
+    const double c = tot[1];
+    const double t = c / tot[0];
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     double d = 0.0;
     if (k < g.dim) {
@@ -594,13 +729,14 @@ __global__ __launch_bounds__(CG_THREADS) void k_cg_dir(CgArgs g) {
     if (g.state[0]) return;
     const double* rr_c = PHASE ? g.p_rr1 : g.p_rr0;
     const double* rr_d = PHASE ? g.p_rr0 : g.p_rr1;
-    const double d = block_total(rr_d, g.nparts_vec);
+    double tot[2];   // d, c
+    block_totals<2>(rr_d, g.nparts_vec, rr_c, g.nparts_vec, nullptr, 0, tot);
+    const double d = tot[0], c = tot[1];
     if (blockIdx.x == 0 && threadIdx.x == 0) g.scal[0] = sqrt(d);
     if (sqrt(d) < g.tol) {
         if (blockIdx.x == 0 && threadIdx.x == 0) { g.state[0] = 1; g.state[2] = 2; }
         return;
     }
-    const double c = block_total(rr_c, g.nparts_vec);
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     double dv = 0.0;
     if (k < g.dim) {
@@ -654,9 +790,9 @@ __global__ __launch_bounds__(CG_THREADS) void k_pcg_step(CgArgs g) {
     if (g.state[0]) return;
     const double* rz_c = PHASE ? g.p_rr1 : g.p_rr0;
     double* rz_d = PHASE ? g.p_rr0 : g.p_rr1;
-    const double pAp = block_total(g.p_vz, g.nparts_mv);
-    const double rz = block_total(rz_c, g.nparts_vec);
-    const double t = rz / pAp;
+    double tot[2];   // p.Ap, r.alpha r
+    block_totals<2>(g.p_vz, g.nparts_mv, rz_c, g.nparts_vec, nullptr, 0, tot);
+    const double t = tot[1] / tot[0];
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     double rzn = 0.0, zz = 0.0;
     if (k < g.dim) {
@@ -679,13 +815,15 @@ __global__ __launch_bounds__(CG_THREADS) void k_pcg_dir(CgArgs g) {
     if (g.state[0]) return;
     const double* rz_c = PHASE ? g.p_rr1 : g.p_rr0;
     const double* rz_d = PHASE ? g.p_rr0 : g.p_rr1;
-    const double res = sqrt(block_total(g.p_dv, g.nparts_vec) * g.inv_npol);
+    double tot[3];   // |alpha r|^2, new r.alpha r, old r.alpha r
+    block_totals<3>(g.p_dv, g.nparts_vec, rz_d, g.nparts_vec, rz_c, g.nparts_vec, tot);
+    const double res = sqrt(tot[0] * g.inv_npol);
     if (blockIdx.x == 0 && threadIdx.x == 0) g.scal[0] = res;
     if (res < g.tol) {
         if (blockIdx.x == 0 && threadIdx.x == 0) { g.state[0] = 1; g.state[2] = 2; }
         return;
     }
-    const double beta = block_total(rz_d, g.nparts_vec) / block_total(rz_c, g.nparts_vec);
+    const double beta = tot[1] / tot[2];
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k < g.dim) {
         const double v = site_alpha(g.polar, k) * g.r[k] + beta * g.v[k];
@@ -705,6 +843,17 @@ __global__ __launch_bounds__(256) void k_direct(int dim, const double* __restric
                                                 double* __restrict__ x) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k < dim) x[k] = polar[k / 3] * e0[k];
+}
+
+// x += alpha (E0 - E0_prev); E0_prev = E0 (the solve's guess 4).
+__global__ __launch_bounds__(256) void k_extrapolate(int dim, const double* __restrict__ polar,
+                                                     const double* __restrict__ e0, double* __restrict__ e0_prev,
+                                                     double* __restrict__ x) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= dim) return;
+    const double e = e0[k];
+    x[k] += polar[k / 3] * (e - e0_prev[k]);
+    e0_prev[k] = e;
 }
 
 __global__ __launch_bounds__(256) void k_fill(int dim, double val, double* __restrict__ x) {
@@ -739,7 +888,14 @@ static IndArgs ind_args(Ctx* c, const smd_induce_params* p) {
     a.tcnt = c->d_it_cnt;
     a.tj = c->d_it_j;
     a.tab = reinterpret_cast<double2*>(c->d_it_a);
-    a.tab32 = p->prec == SMD_PREC_MIXED ? reinterpret_cast<float2*>(c->d_it_a) : nullptr;
+    a.mixed = p->prec == SMD_PREC_MIXED ? 1 : 0;
+    a.rec32 = c->d_ind_rec32;
+    for (int d = 0; d < 3; ++d) {
+        const double L = c->box.L[d];
+        a.Lh[d] = (float)L;
+        a.Ll[d] = (float)(L - (double)a.Lh[d]);
+        a.invL[d] = (float)(1.0 / L);
+    }
     return a;
 }
 
@@ -793,7 +949,7 @@ static int induce_check(Ctx* c, const smd_induce_params* p) {
     if (p->cutoff_ind > c->list_cutoff)
         return fail(c, SMD_ENOTSUP, "cutoff_ind %g exceeds the pair-list cutoff %g", p->cutoff_ind, c->list_cutoff);
     if (p->prec != SMD_PREC_FP64 && p->prec != SMD_PREC_MIXED) return fail(c, SMD_EINVAL, "bad precision %d", p->prec);
-    if (p->guess < 0 || p->guess > 3) return fail(c, SMD_EINVAL, "guess %d is not 0..3", p->guess);
+    if (p->guess < 0 || p->guess > 4) return fail(c, SMD_EINVAL, "guess %d is not 0..4", p->guess);
     if (p->precond != 0 && p->precond != 1) return fail(c, SMD_EINVAL, "precond %d is not 0 or 1", p->precond);
     if (p->tol_rms != p->precond)
         return fail(c, SMD_ENOTSUP, "the RMS dipole tolerance goes with the preconditioned solve, |r| with the reference's");
@@ -855,8 +1011,13 @@ int induce_begin(Ctx* c, const smd_induce_params* p, bool frames) {
     {
         ProfScope ps(c, "ind_field");
         if (frames) hipLaunchKernelGGL(k_frames, dim3((c->nmol + 255) / 256), b256, 0, c->stream, a);
-        hipLaunchKernelGGL((k_field<IND_LANES>), grid_rows, b256, 0, c->stream, a);
-        if (p->do_min) hipLaunchKernelGGL((k_ind_table<IND_LANES>), grid_rows, b256, 0, c->stream, a);
+        if (p->do_min && a.mixed) {
+            hipLaunchKernelGGL(k_pack_ind32, dim3((n + 255) / 256), b256, 0, c->stream, a);
+            hipLaunchKernelGGL((k_field_table32<IND_LANES>), grid_rows, b256, 0, c->stream, a);
+        } else {
+            hipLaunchKernelGGL((k_field<IND_LANES>), grid_rows, b256, 0, c->stream, a);
+            if (p->do_min) hipLaunchKernelGGL((k_ind_table<IND_LANES>), grid_rows, b256, 0, c->stream, a);
+        }
     }
     if (!p->do_min) {
         hipLaunchKernelGGL(k_direct, grid_vec, b256, 0, c->stream, dim, c->d_polar, c->d_e0, c->d_mu);
@@ -875,6 +1036,11 @@ int induce_begin(Ctx* c, const smd_induce_params* p, bool frames) {
         if (mixed) hipLaunchKernelGGL(k_to_f32, grid_vec, b256, 0, c->stream, dim, g.x, reinterpret_cast<float*>(g.v32));
         solve_apply(c, p, a, g.x, g.v32, g.z, nullptr, plain);
         if ((rc = mlp_guess(c, g.x, g.z))) return rc;
+    }
+    else if (p->guess == 4) {
+        // the previous solution moved by the change of the direct term:
+        // x = mu_prev + alpha (E0 - E0_prev), and E0 kept for the next solve
+        hipLaunchKernelGGL(k_extrapolate, grid_vec, b256, 0, c->stream, dim, c->d_polar, c->d_e0, c->d_e0_prev, g.x);
     }
     // guess 1: keep the previous solution in d_mu
     if (mixed) hipLaunchKernelGGL(k_to_f32, grid_vec, b256, 0, c->stream, dim, g.x, reinterpret_cast<float*>(g.v32));
@@ -936,7 +1102,11 @@ int induce_batches(Ctx* c, const smd_induce_params* p, int batches) {
     put(&a.tcnt, sizeof a.tcnt);
     put(&a.tj, sizeof a.tj);
     put(&a.tab, sizeof a.tab);
-    put(&a.tab32, sizeof a.tab32);
+    put(&a.mixed, sizeof a.mixed);
+    put(&a.rec32, sizeof a.rec32);
+    put(&a.Lh, sizeof a.Lh);
+    put(&a.Ll, sizeof a.Ll);
+    put(&a.invL, sizeof a.invL);
     put(&g, sizeof g);
     if (c->cg_exec && c->cg_key != key) {
         (void)hipGraphExecDestroy(c->cg_exec);
@@ -1056,14 +1226,17 @@ extern "C" int smd_set_multipoles(smd_ctx* c, const double* dip_local, const dou
         if ((rc = A(c->d_dipl, sizeof(double) * 3 * n)) || (rc = A(c->d_quadl, sizeof(double) * 6 * n)) ||
             (rc = A(c->d_polar, sizeof(double) * n)) || (rc = A(c->d_dipg, sizeof(double) * 3 * n)) ||
             (rc = A(c->d_quadg, sizeof(double) * 6 * n)) || (rc = A(c->d_e0, sizeof(double) * 3 * n)) ||
-            (rc = A(c->d_mu, sizeof(double) * 3 * n)) || (rc = A(c->d_cg, sizeof(double) * 9 * n)) ||
-            (rc = A(c->d_cg32, sizeof(float4) * n)) || (rc = A(c->d_site32, sizeof(float4) * 3 * n)) ||
+            (rc = A(c->d_mu, sizeof(double) * 3 * n)) || (rc = A(c->d_e0_prev, sizeof(double) * 3 * n)) ||
+            (rc = A(c->d_cg, sizeof(double) * 9 * n)) ||
+            (rc = A(c->d_cg32, sizeof(float4) * n)) || (rc = A(c->d_ind_rec32, sizeof(float4) * 2 * n)) ||
+            (rc = A(c->d_site32, sizeof(float4) * 3 * n)) ||
             (rc = A(c->d_pind32, sizeof(float4) * n)) ||
             (rc = A(c->d_cg_part, sizeof(double) * 4 * parts)) || (rc = A(c->d_cg_state, sizeof(int) * 4)) ||
             (rc = A(c->d_cg_scal, sizeof(double) * 4)))
             return rc;
         c->cg_parts = parts;
         SMD_HIP(c, hipMemset(c->d_mu, 0, sizeof(double) * 3 * n));
+        SMD_HIP(c, hipMemset(c->d_e0_prev, 0, sizeof(double) * 3 * n));
     }
     SMD_HIP(c, hipStreamSynchronize(c->stream));
     SMD_HIP(c, hipMemcpy(c->d_dipl, dip_local, sizeof(double) * 3 * n, hipMemcpyHostToDevice));

@@ -61,11 +61,19 @@ struct IntegArgs {
 // velocity is zero unless a component exceeds 1e-11.
 __device__ __forceinline__ void drift_velocity(const IntegArgs& a, double (&vc)[3]) {
     __shared__ double tot[4];
-    if (threadIdx.x < WAVE)
-        for (int k = 0; k < 4; ++k) {
-            const double t = wave_fixed_sum(a.part_vom + (size_t)k * a.nb_vom, a.nb_vom);
-            if (threadIdx.x == 0) tot[k] = t;
-        }
+    if (threadIdx.x < WAVE) {
+        // the four sums of wave_fixed_sum's order, their loads issued together
+        // (the partials come from an earlier kernel)
+        double s[4] = {0.0, 0.0, 0.0, 0.0};
+        for (int k = threadIdx.x; k < a.nb_vom; k += WAVE)
+#pragma unroll
+            for (int b = 0; b < 4; ++b) s[b] += a.part_vom[(size_t)b * a.nb_vom + k];
+#pragma unroll
+        for (int b = 0; b < 4; ++b) s[b] = wave_sum(s[b]);
+        if (threadIdx.x == 0)
+#pragma unroll
+            for (int b = 0; b < 4; ++b) tot[b] = s[b];
+    }
     __syncthreads();
     const double mt = tot[3];
     const double vx = tot[0] / mt, vy = tot[1] / mt, vz = tot[2] / mt;
@@ -236,10 +244,9 @@ __global__ __launch_bounds__(256) void k_shake(IntegArgs a) {
         double pn[3][3], vel[3][3], F[3][3], ms[3];
         shake_mol(a, m, vcm, v, pn, vel, F, ms);
     }
-    // the constraint virial: block partials, summed in a fixed order by the
-    // last block
+    // the constraint virial: block partials, added in block order by
+    // k_rows_finish
     block_store<9>(v, a.part_sh, gridDim.x);
-    if (last_block(a.flag + 3)) fixed_totals(a.part_sh, gridDim.x, gridDim.x, 9, a.terms + T_SH_VIR);
 }
 
 __global__ __launch_bounds__(256) void k_lf_atoms(IntegArgs a) {
@@ -302,9 +309,6 @@ __device__ __forceinline__ void nh_friction(const IntegArgs& a, double ke_atom, 
     a.scalars[1] = fp;
 }
 
-// Molecular centres, momenta and kinetic sums (:646-686).  In the MD step the
-// last block to finish also applies the thermostat update and clears the
-// drift sums for the next step (its counter lives in flag[2]).
 // Integrator::computePressure (src/Integrator.cpp:205-270), the shake-virial
 // correction of LFIntegrator::run (:805-814), myAvLus.P (:730-743) and the
 // stress sums of Ensemble::accumulate (src/Ensemble.cpp:814-894) for the MD
@@ -382,22 +386,20 @@ constexpr int KIN_THREADS = 64;
 // SHAKE virial (9).
 constexpr int KP_KE = 0, KP_PR = 2, KP_SH = 23;
 
-// End of the kinetics kernels.  Every block stores its partials; the last
-// block (atomic ticket) sums them in a fixed order into the accumulators
-// and, in the MD step, forms the Nose-Hoover update, the pressure tensors
-// and the stress sums.
-__device__ __forceinline__ void kin_epilogue(const IntegArgs& a, bool shake_rows) {
-    if (!last_block(a.flag + 2)) return;
-    const int nb = gridDim.x;
-    fixed_totals(a.part_kin + (size_t)KP_KE * nb, nb, nb, 2, a.terms + T_KE_ATOM);
-    if (a.md_step) fixed_totals(a.part_kin + (size_t)KP_PR * nb, nb, nb, 21, a.terms + T_PR_MC);
-    if (shake_rows) fixed_totals(a.part_kin + (size_t)KP_SH * nb, nb, nb, 9, a.terms + T_SH_VIR);
+// The kinetics kernels' totals (kinetic sums, in the MD step the pressure
+// sums and, after SHAKE, its virial) and, in the MD step, the Nose-Hoover
+// update, the pressure tensors and the stress sums.
+// One block of FINISH_THREADS after the kinetics kernel, on its stream.
+__global__ __launch_bounds__(FINISH_THREADS) void k_kin_finish(IntegArgs a, int nb, int shake_rows) {
+    rows_total(a.part_kin + (size_t)KP_KE * nb, nb, nb, 2, a.terms + T_KE_ATOM);
+    if (a.md_step) rows_total(a.part_kin + (size_t)KP_PR * nb, nb, nb, 21, a.terms + T_PR_MC);
+    if (shake_rows) rows_total(a.part_kin + (size_t)KP_SH * nb, nb, nb, 9, a.terms + T_SH_VIR);
     if (!a.md_step) return;
     // the step's sums, read once by the whole block (the epilogue's thread
     // then works from LDS, not from a chain of dependent global loads)
     __shared__ double tl[T_COUNT];
-    volatile const double* tv = a.terms;
-    for (int k = threadIdx.x; k < T_COUNT; k += blockDim.x) tl[k] = tv[k];
+    __syncthreads();
+    for (int k = threadIdx.x; k < T_COUNT; k += blockDim.x) tl[k] = a.terms[k];
     __syncthreads();
     if (threadIdx.x == 0) {
         nh_friction(a, tl[T_KE_ATOM], tl[T_KE_MOL]);
@@ -405,6 +407,8 @@ __device__ __forceinline__ void kin_epilogue(const IntegArgs& a, bool shake_rows
     }
 }
 
+// Molecular centres, momenta and kinetic sums (:646-686) as block partials;
+// k_kin_finish adds them and, in the MD step, applies the thermostat update.
 __global__ __launch_bounds__(KIN_THREADS) void k_kinetics(IntegArgs a, int rewrap) {
     double v[2] = {0.0, 0.0};
     double pr[21];   // MD step: sum f (x) (r - R) [9], atom p p / m [6], molecule P P / M [6]
@@ -464,7 +468,6 @@ __global__ __launch_bounds__(KIN_THREADS) void k_kinetics(IntegArgs a, int rewra
     const int nb = gridDim.x;
     block_store<2>(v, a.part_kin + (size_t)KP_KE * nb, nb);
     if (a.md_step) block_store<21>(pr, a.part_kin + (size_t)KP_PR * nb, nb);
-    kin_epilogue(a, false);
 }
 
 // MD step, rigid water: SHAKE of each molecule and, from the positions and
@@ -528,7 +531,6 @@ __global__ __launch_bounds__(KIN_THREADS) void k_shake_kinetics(IntegArgs a, int
     block_store<9>(sv, a.part_kin + (size_t)KP_SH * nb, nb);
     block_store<2>(v, a.part_kin + (size_t)KP_KE * nb, nb);
     block_store<21>(pr, a.part_kin + (size_t)KP_PR * nb, nb);
-    kin_epilogue(a, true);
 }
 
 __global__ __launch_bounds__(256) void k_mol_centers(int nmol, const int32_t* __restrict__ ms,
@@ -675,9 +677,10 @@ int shake_run(Ctx* c, const smd_integrator_params* p, bool fnh_on_device, double
     a.md_step = md_step ? 1 : 0;
     a.frecip = c->shake_frecip ? c->d_frecip : nullptr;
     ProfScope ps(c, "shake");
-    hipLaunchKernelGGL(k_shake, dim3((c->nmol + 255) / 256), dim3(256), 0, c->stream, a);
+    const int nb = (c->nmol + 255) / 256;
+    hipLaunchKernelGGL(k_shake, dim3(nb), dim3(256), 0, c->stream, a);
     SMD_HIP(c, hipGetLastError());
-    return SMD_OK;
+    return rows_finish(c, c->stream, a.part_sh, nb, nb, 9, a.terms + T_SH_VIR);
 }
 
 int kinetics_run(Ctx* c, bool rewrap, const smd_integrator_params* md) {
@@ -692,8 +695,9 @@ int kinetics_run(Ctx* c, bool rewrap, const smd_integrator_params* md) {
         a.stress_after = 0.15 * (double)c->md.total_steps;
     }
     ProfScope ps(c, "kinetics");
-    hipLaunchKernelGGL(k_kinetics, dim3((c->nmol + KIN_THREADS - 1) / KIN_THREADS), dim3(KIN_THREADS), 0, c->stream,
-                       a, rewrap ? 1 : 0);
+    const int nb = (c->nmol + KIN_THREADS - 1) / KIN_THREADS;
+    hipLaunchKernelGGL(k_kinetics, dim3(nb), dim3(KIN_THREADS), 0, c->stream, a, rewrap ? 1 : 0);
+    hipLaunchKernelGGL(k_kin_finish, dim3(1), dim3(FINISH_THREADS), 0, c->stream, a, nb, 0);
     SMD_HIP(c, hipGetLastError());
     return SMD_OK;
 }
@@ -711,8 +715,9 @@ static int shake_kinetics_run(Ctx* c, const smd_integrator_params* md) {
     a.stress = c->d_stress;
     a.stress_after = 0.15 * (double)c->md.total_steps;
     ProfScope ps(c, "shake_kinetics");
-    hipLaunchKernelGGL(k_shake_kinetics, dim3((c->nmol + KIN_THREADS - 1) / KIN_THREADS), dim3(KIN_THREADS), 0,
-                       c->stream, a, 1);
+    const int nb = (c->nmol + KIN_THREADS - 1) / KIN_THREADS;
+    hipLaunchKernelGGL(k_shake_kinetics, dim3(nb), dim3(KIN_THREADS), 0, c->stream, a, 1);
+    hipLaunchKernelGGL(k_kin_finish, dim3(1), dim3(FINISH_THREADS), 0, c->stream, a, nb, 1);
     SMD_HIP(c, hipGetLastError());
     return SMD_OK;
 }
@@ -955,8 +960,8 @@ extern "C" int smd_md_init(smd_ctx* c, const smd_md_params* p) {
         SMD_HIP(c, hipStreamCreateWithFlags(&c->stream2, hipStreamNonBlocking));
         SMD_HIP(c, hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming));
         SMD_HIP(c, hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming));
-        SMD_HIP(c, hipMalloc(&c->d_frecip, sizeof(double) * 3 * c->n));
     }
+    if (!c->d_frecip) SMD_HIP(c, hipMalloc(&c->d_frecip, sizeof(double) * 3 * c->n));
     if (p->ewald_on && (rc = ewald_setup(c, &p->ewald))) return rc;
     if ((rc = pairlist_build(c, p->cutoff, p->cut_buff, p->use_cells, false))) return rc;
     // Give in-step rebuilds headroom: the row capacity is fixed once a step

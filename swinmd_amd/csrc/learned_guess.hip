@@ -271,6 +271,108 @@ __global__ __launch_bounds__(256) void k_mlp_rows(MlpArgs a) {
     back_layer<MH>(a, 0, L1, nullptr, a.wt + OFF_W2, z1, row0, live, r, g);
 }
 
+// ---- inference: weights staged in LDS, 16x16x32 matrix products ----------
+
+typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
+
+__device__ __forceinline__ v4f mfma32(v8bf a, v8bf b, v4f c) {
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+}
+
+// v_mfma_f32_16x16x32_bf16 fragments of lane l (r = l % 16, g = l / 16):
+//   A[r][8g .. 8g+7],  B[8g .. 8g+7][r],  D[4g + t][r] for t = 0..3.
+// Each block stages the four BF16 weight matrices in LDS once (rows padded
+// by 8 so that the 16 rows a fragment read touches fall in different banks)
+// and carries INF_WAVES x 16 rows through the network, 16 per wave.
+constexpr int INF_WAVES = 8;
+constexpr int W1_STRIDE = MF + 8;                       // BF16 per LDS row of W1
+constexpr int INF_W1 = 0, INF_W2 = MH * W1_STRIDE, INF_W3 = INF_W2 + MH * LDS_STRIDE,
+              INF_W4 = INF_W3 + MH * LDS_STRIDE, INF_WSIZE = INF_W4 + MOP * LDS_STRIDE;
+
+__device__ __forceinline__ v8bf ld8(const uint16_t* p) { return *reinterpret_cast<const v8bf*>(p); }
+
+// rows x cols BF16 (row-major, cols a multiple of 8) from global to LDS with
+// the given LDS row stride, 16 bytes per thread and step
+__device__ __forceinline__ void stage(uint16_t* dst, int stride, const uint16_t* src, int rows, int cols) {
+    const int per_row = cols / 8;
+    for (int t = threadIdx.x; t < rows * per_row; t += blockDim.x) {
+        const int row = t / per_row, c8 = t % per_row;
+        *reinterpret_cast<uint4*>(dst + row * stride + 8 * c8) =
+            *reinterpret_cast<const uint4*>(src + (size_t)row * cols + 8 * c8);
+    }
+}
+
+// acc += A (16 x 128, LDS rows) . W^T over 8 column tiles, W [128][128] in LDS
+__device__ __forceinline__ void dense128(v4f (&acc)[8], const uint16_t* lds, const uint16_t* W, int r, int g) {
+#pragma unroll
+    for (int ks = 0; ks < MH / 32; ++ks) {
+        const v8bf av = ld8(lds + r * LDS_STRIDE + ks * 32 + 8 * g);
+#pragma unroll
+        for (int c = 0; c < 8; ++c) acc[c] = mfma32(av, ld8(W + (c * 16 + r) * LDS_STRIDE + ks * 32 + 8 * g), acc[c]);
+    }
+}
+
+__device__ __forceinline__ void silu_lds(const v4f (&acc)[8], uint16_t* lds, int r, int g) {
+#pragma unroll
+    for (int c = 0; c < 8; ++c)
+#pragma unroll
+        for (int t = 0; t < 4; ++t) lds[(4 * g + t) * LDS_STRIDE + c * 16 + r] = f2bf(silu_f(acc[c][t]));
+}
+
+__global__ __launch_bounds__(INF_WAVES * WAVE) void k_mlp_infer(MlpArgs a) {
+    __shared__ __align__(16) uint16_t wl[INF_WSIZE];
+    __shared__ __align__(16) uint16_t act[INF_WAVES][2][16 * LDS_STRIDE];
+    stage(wl + INF_W1, W1_STRIDE, a.wb + OFF_W1, MH, MF);
+    stage(wl + INF_W2, LDS_STRIDE, a.wb + OFF_W2, MH, MH);
+    stage(wl + INF_W3, LDS_STRIDE, a.wb + OFF_W3, MH, MH);
+    stage(wl + INF_W4, LDS_STRIDE, a.wb + OFF_W4, MOP, MH);
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int r = lane & 15, g = lane >> 4;
+    const int row0 = (blockIdx.x * INF_WAVES + wv) * 16;
+    uint16_t* L0 = act[wv][0];
+    uint16_t* L1 = act[wv][1];
+    uint4 xin = make_uint4(0u, 0u, 0u, 0u);   // rows past the end compute on zeros
+    {
+        const int row = row0 + r;
+        if (row < a.rows) {
+            const int sidx = a.idx ? a.idx[row] : row;
+            xin = *reinterpret_cast<const uint4*>(a.x + (size_t)sidx * MF + 8 * g);
+        }
+    }
+    const v8bf ax = __builtin_bit_cast(v8bf, xin);
+    __syncthreads();   // the weights are in LDS
+    // every wave runs every layer, so the barriers between them stay uniform
+    v4f z[8];
+    bias_init(z, a.w + OFF_B1, r);
+#pragma unroll
+    for (int c = 0; c < 8; ++c) z[c] = mfma32(ax, ld8(wl + INF_W1 + (c * 16 + r) * W1_STRIDE + 8 * g), z[c]);
+    silu_lds(z, L0, r, g);
+    __syncthreads();
+    bias_init(z, a.w + OFF_B2, r);
+    dense128(z, L0, wl + INF_W2, r, g);
+    silu_lds(z, L1, r, g);
+    __syncthreads();
+    bias_init(z, a.w + OFF_B3, r);
+    dense128(z, L1, wl + INF_W3, r, g);
+    __syncthreads();   // every lane has read L0 (layer 2's input)
+    silu_lds(z, L0, r, g);
+    __syncthreads();
+    v4f o;
+    {
+        const float b = a.w[OFF_B4 + r];
+        o = v4f{b, b, b, b};
+#pragma unroll
+        for (int ks = 0; ks < MH / 32; ++ks)
+            o = mfma32(ld8(L0 + r * LDS_STRIDE + ks * 32 + 8 * g), ld8(wl + INF_W4 + r * LDS_STRIDE + ks * 32 + 8 * g), o);
+    }
+    if (r < MO)
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+            const int row = row0 + 4 * g + t;
+            if (row < a.rows) a.y[(size_t)row * MO + r] = o[t];
+        }
+}
+
 // dW[m][k] = sum_n dz[m][n] in[k][n] over one batch slice, one wave per tile;
 // tiles with k0 = 0 also sum dz for the bias.  Writes every entry of its
 // slice's partial vector.
@@ -760,7 +862,8 @@ int mlp_guess(Ctx* c, double* x, const double* z) {
     if ((rc = ensure_rows(c, round16(c->n)))) return rc;
     if ((rc = descriptors_run(c, x, z, nullptr, m->x))) return rc;
     MlpArgs a = rows_args(m, c->n, nullptr, m->x, nullptr);
-    hipLaunchKernelGGL(k_mlp_rows<false>, dim3(row_blocks(a.rows16)), dim3(256), 0, c->stream, a);
+    hipLaunchKernelGGL(k_mlp_infer, dim3((unsigned)((a.rows16 + 16 * INF_WAVES - 1) / (16 * INF_WAVES))),
+                       dim3(INF_WAVES * WAVE), 0, c->stream, a);
     hipLaunchKernelGGL(k_guess, dim3((c->n + 255) / 256), dim3(256), 0, c->stream, c->n, m->frame, c->d_polar,
                        c->d_e0, m->y, m->norm, x);
     SMD_HIP(c, hipGetLastError());
@@ -908,7 +1011,8 @@ int smd_mlp_forward(smd_ctx* c, const float* x, int64_t rows, float* y) {
     if ((rc = upload_rows(c, x, rows))) return rc;
     Mlp* m = c->mlp;
     MlpArgs a = rows_args(m, rows, nullptr, m->x, nullptr);
-    hipLaunchKernelGGL(k_mlp_rows<false>, dim3(row_blocks(a.rows16)), dim3(256), 0, c->stream, a);
+    hipLaunchKernelGGL(k_mlp_infer, dim3((unsigned)((a.rows16 + 16 * INF_WAVES - 1) / (16 * INF_WAVES))),
+                       dim3(INF_WAVES * WAVE), 0, c->stream, a);
     SMD_HIP(c, hipGetLastError());
     SMD_HIP(c, hipMemcpyAsync(y, m->y, sizeof(float) * MO * rows, hipMemcpyDeviceToHost, c->stream));
     SMD_HIP(c, hipStreamSynchronize(c->stream));

@@ -93,7 +93,6 @@ struct MpArgs {
     int damp_amoeba, do_min;
     // fixed-order partials of the per-site sums (Ctx::part_su) and their ticket
     double* spart;
-    int* sticket;
     // mixed form: FP32 site records [3n] and induced dipoles [n]
     const float4* site32;
     const float4* pind32;
@@ -368,6 +367,13 @@ static int launch_torques(Ctx* c, const MpArgs& a, dim3 g, unsigned parts, int s
     return rc;
 }
 
+// The site-sums kernels' totals: row 0 the self energy, the rest the surface
+// sums (10 for the permanent multipoles, 14 with the induced dipoles).
+static int site_sums_finish(Ctx* c, const MpArgs& a, unsigned nb, bool induced) {
+    return rows_finish(c, c->stream, a.spart, (int)nb, (int)nb, induced ? 15 : 11, a.terms + M_SELF, 1,
+                       a.terms + M_SU_QR);
+}
+
 // FP32 records of each site's lab-frame multipoles and polarizability for the
 // mixed form's real-space partner gathers (3 x 16 bytes per site), and of the
 // induced dipoles (16 bytes per site).
@@ -405,13 +411,9 @@ __global__ __launch_bounds__(256) void k_mp_site_sums(MpArgs a) {
         v[7] = I.q * r.x + I.d.x; v[8] = I.q * r.y + I.d.y; v[9] = I.q * r.z + I.d.z;   // q r + mu
         v[10] = 2.0 * I.q * QiI - dot(I.d, I.d);
     }
-    // the surface sums feed the forces: block partials summed by the last
-    // block in a fixed order
+    // the surface sums feed the forces: block partials added in block order
+    // by site_sums_finish before the next kernel
     block_store<11>(v, a.spart, gridDim.x);
-    if (last_block(a.sticket)) {
-        fixed_totals(a.spart, gridDim.x, gridDim.x, 1, a.terms + M_SELF);
-        fixed_totals(a.spart + gridDim.x, gridDim.x, gridDim.x, 10, a.terms + M_SU_QR);
-    }
 }
 
 // One thread per molecule: excluded pairs inside it with erf-screened D_l
@@ -862,7 +864,6 @@ static MpArgs mp_args(Ctx* c, const Ctx::KSet& ks, double alpha, double real_cut
     a.pind = c->d_mu;
     a.polar = c->d_polar;
     a.spart = c->part_su;
-    a.sticket = c->d_flag + 5;
     a.site32 = c->d_site32;
     a.pind32 = c->d_pind32;
     return a;
@@ -899,6 +900,7 @@ int multipole_short(Ctx* c, smd_mp_out* out) {
     SMD_HIP(c, hipMemsetAsync(c->d_mp_terms, 0, sizeof(double) * M_COUNT, c->stream));
     if ((rc = frames_run(c))) return rc;
     hipLaunchKernelGGL(k_mp_site_sums, ga, b256, 0, c->stream, a);
+    if ((rc = site_sums_finish(c, a, ga.x, false))) return rc;
     if ((rc = launch_real(c, a, false))) return rc;
     hipLaunchKernelGGL(k_mp_intra, gm, b256, 0, c->stream, a);
     if ((rc = red_finish(c, RS_MP_INTRA, gm.x, 2, a.terms + M_INTRA_E))) return rc;
@@ -1295,10 +1297,6 @@ __global__ __launch_bounds__(256) void k_ind_site_sums(MpArgs a) {
         v[14] = dot(p, p);
     }
     block_store<15>(v, a.spart, gridDim.x);
-    if (last_block(a.sticket)) {
-        fixed_totals(a.spart, gridDim.x, gridDim.x, 1, a.terms + M_SELF);
-        fixed_totals(a.spart + gridDim.x, gridDim.x, gridDim.x, 14, a.terms + M_SU_QR);
-    }
 }
 
 // One thread per molecule: excluded pairs inside it (intraMolCorrect
@@ -1623,6 +1621,7 @@ int induction_short(Ctx* c, smd_indf_out* out) {
     SMD_HIP(c, hipMemsetAsync(c->d_mp_terms, 0, sizeof(double) * M_COUNT, c->stream));
     if ((rc = frames_run(c))) return rc;
     hipLaunchKernelGGL(k_ind_site_sums, ga, b256, 0, c->stream, a);
+    if ((rc = site_sums_finish(c, a, ga.x, true))) return rc;
     if ((rc = launch_real(c, a, true))) return rc;
     hipLaunchKernelGGL(k_ind_intra, gm, b256, 0, c->stream, a);
     if ((rc = red_finish(c, RS_IND_INTRA, gm.x, 2, a.terms + M_INTRA_E))) return rc;
@@ -1706,6 +1705,58 @@ int induction_long(Ctx* c, smd_mp_long_out* out) {
 // terms go to the given accumulator (cleared here by a fill kernel, so the
 // sequence can be captured), forces and torque-derived forces into d_force.
 // The lab-frame multipoles must be current (frames_run).
+// The k-space passes of the step up to the per-site partial forces
+// (structure factors, per-k sums and coefficients, the force pass over
+// k-ranges) for the permanent multipoles (mp) or the induced dipoles.  They
+// write only their own partial buffers and, through the reduction arena,
+// the long-range terms.
+static int mp_long_sums(Ctx* c, const MpArgs& a, bool mp) {
+    const char* fam = mp ? "mp_long" : "ind_long";
+    ProfScope ps(c, fam);
+    const dim3 g1((a.K + MP_SF_THREADS - 1) / MP_SF_THREADS, a.chunks), b256(256);
+    const dim3 gk((c->n + WAVE - 1) / WAVE, a.ksplit);
+    const bool mixed = c->mp_prec == SMD_PREC_MIXED;
+    if (mp) {
+        if (mixed) hipLaunchKernelGGL((k_mp_sfac<float>), g1, dim3(MP_SF_THREADS), 0, c->stream, a);
+        else hipLaunchKernelGGL((k_mp_sfac<double>), g1, dim3(MP_SF_THREADS), 0, c->stream, a);
+        hipLaunchKernelGGL(k_mp_kfinish, dim3((a.K + 3) / 4), b256, 0, c->stream, a);
+        const int rc = red_finish(c, RS_MP_LONG, (a.K + 3) / 4, 9, a.terms + M_LONG_E);
+        if (rc) return rc;
+        if (mixed) hipLaunchKernelGGL((k_mp_kforce<float>), gk, dim3(WAVE), 0, c->stream, a);
+        else hipLaunchKernelGGL((k_mp_kforce<double>), gk, dim3(WAVE), 0, c->stream, a);
+    } else {
+        if (mixed) hipLaunchKernelGGL((k_ind_sfac<float>), g1, dim3(MP_SF_THREADS), 0, c->stream, a);
+        else hipLaunchKernelGGL((k_ind_sfac<double>), g1, dim3(MP_SF_THREADS), 0, c->stream, a);
+        hipLaunchKernelGGL(k_ind_kfinish, dim3((a.K + 3) / 4), b256, 0, c->stream, a);
+        const int rc = red_finish(c, RS_IND_LONG, (a.K + 3) / 4, 9, a.terms + M_LONG_E);
+        if (rc) return rc;
+        if (mixed) hipLaunchKernelGGL((k_ind_kforce<float>), gk, dim3(WAVE), 0, c->stream, a);
+        else hipLaunchKernelGGL((k_ind_kforce<double>), gk, dim3(WAVE), 0, c->stream, a);
+    }
+    SMD_HIP(c, hipGetLastError());
+    return SMD_OK;
+}
+
+// Fork of the step's k-space passes onto the second stream (when the
+// context has one and reciprocal overlap is on): the second stream waits for
+// everything enqueued so far and becomes the context's stream until
+// polar_fork_end; polar_join makes the step's stream wait for it.
+static bool polar_fork_begin(Ctx* c) {
+    if (!c->recip_overlap || !c->stream2 || c->red_defer == false) return false;
+    if (hipEventRecord(c->ev_fork, c->stream) != hipSuccess) return false;
+    if (hipStreamWaitEvent(c->stream2, c->ev_fork, 0) != hipSuccess) return false;
+    c->polar_main = c->stream;
+    c->stream = c->stream2;
+    return true;
+}
+
+static void polar_fork_end(Ctx* c) {
+    (void)hipEventRecord(c->ev_join, c->stream2);
+    c->stream = c->polar_main;
+}
+
+static void polar_join(Ctx* c) { (void)hipStreamWaitEvent(c->stream, c->ev_join, 0); }
+
 int multipole_enqueue(Ctx* c, double* terms) {
     int rc;
     MpArgs a = mp_args(c);
@@ -1714,9 +1765,16 @@ int multipole_enqueue(Ctx* c, double* terms) {
     const dim3 ga((c->n + 255) / 256), gm((c->nmol + 255) / 256);
     const dim3 gr((unsigned)(((int64_t)c->n * MP_LANES + 255) / 256));
     hipLaunchKernelGGL(k_zero_terms, dim3(1), b256, 0, c->stream, terms);
+    // the k-space sums and per-site partial forces on the second stream,
+    // beside the real-space kernels (they touch none of the same buffers)
+    const bool fork = polar_fork_begin(c);
+    int rc_long = fork ? mp_long_sums(c, a, true) : SMD_OK;
+    if (fork) polar_fork_end(c);
+    if (rc_long) return rc_long;
     {
         ProfScope ps(c, "mp_short");
         hipLaunchKernelGGL(k_mp_site_sums, ga, b256, 0, c->stream, a);
+        if ((rc = site_sums_finish(c, a, ga.x, false))) return rc;
         if ((rc = launch_real(c, a, false))) return rc;
         hipLaunchKernelGGL(k_mp_intra, gm, b256, 0, c->stream, a);
         if ((rc = red_finish(c, RS_MP_INTRA, gm.x, 2, a.terms + M_INTRA_E))) return rc;
@@ -1726,15 +1784,10 @@ int multipole_enqueue(Ctx* c, double* terms) {
         }
         if (c->mp.torques && (rc = launch_torques(c, a, gm, c->mp.surface ? 7u : 3u, RS_MP_DUDV))) return rc;
     }
+    if (fork) polar_join(c);
+    else if ((rc = mp_long_sums(c, a, true))) return rc;
     {
         ProfScope ps(c, "mp_long");
-        const dim3 g1((a.K + MP_SF_THREADS - 1) / MP_SF_THREADS, a.chunks);
-        if (c->mp_prec == SMD_PREC_MIXED) hipLaunchKernelGGL((k_mp_sfac<float>), g1, dim3(MP_SF_THREADS), 0, c->stream, a);
-        else hipLaunchKernelGGL((k_mp_sfac<double>), g1, dim3(MP_SF_THREADS), 0, c->stream, a);
-        hipLaunchKernelGGL(k_mp_kfinish, dim3((a.K + 3) / 4), b256, 0, c->stream, a);
-        if ((rc = red_finish(c, RS_MP_LONG, (a.K + 3) / 4, 9, a.terms + M_LONG_E))) return rc;
-        if (c->mp_prec == SMD_PREC_MIXED) hipLaunchKernelGGL((k_mp_kforce<float>), dim3((c->n + WAVE - 1) / WAVE, a.ksplit), dim3(WAVE), 0, c->stream, a);
-        else hipLaunchKernelGGL((k_mp_kforce<double>), dim3((c->n + WAVE - 1) / WAVE, a.ksplit), dim3(WAVE), 0, c->stream, a);
         hipLaunchKernelGGL(k_kforce_sum, ga, b256, 0, c->stream, a);
         if (c->mp.torques && (rc = launch_torques(c, a, gm, 8u, RS_MP_DUDV_L))) return rc;
     }
@@ -1750,9 +1803,14 @@ int induction_enqueue(Ctx* c, double* terms) {
     const dim3 ga((c->n + 255) / 256), gm((c->nmol + 255) / 256);
     const dim3 gr((unsigned)(((int64_t)c->n * MP_LANES + 255) / 256));
     hipLaunchKernelGGL(k_zero_terms, dim3(1), b256, 0, c->stream, terms);
+    const bool fork = polar_fork_begin(c);
+    int rc_long = fork ? mp_long_sums(c, a, false) : SMD_OK;
+    if (fork) polar_fork_end(c);
+    if (rc_long) return rc_long;
     {
         ProfScope ps(c, "ind_short");
         hipLaunchKernelGGL(k_ind_site_sums, ga, b256, 0, c->stream, a);
+        if ((rc = site_sums_finish(c, a, ga.x, true))) return rc;
         if ((rc = launch_real(c, a, true))) return rc;
         hipLaunchKernelGGL(k_ind_intra, gm, b256, 0, c->stream, a);
         if ((rc = red_finish(c, RS_IND_INTRA, gm.x, 2, a.terms + M_INTRA_E))) return rc;
@@ -1762,15 +1820,10 @@ int induction_enqueue(Ctx* c, double* terms) {
         }
         if (c->indf.torques && (rc = launch_torques(c, a, gm, 16u | 3u | (c->indf.surface ? 4u : 0u), RS_IND_DUDV))) return rc;
     }
+    if (fork) polar_join(c);
+    else if ((rc = mp_long_sums(c, a, false))) return rc;
     {
         ProfScope ps(c, "ind_long");
-        const dim3 g1((a.K + MP_SF_THREADS - 1) / MP_SF_THREADS, a.chunks);
-        if (c->mp_prec == SMD_PREC_MIXED) hipLaunchKernelGGL((k_ind_sfac<float>), g1, dim3(MP_SF_THREADS), 0, c->stream, a);
-        else hipLaunchKernelGGL((k_ind_sfac<double>), g1, dim3(MP_SF_THREADS), 0, c->stream, a);
-        hipLaunchKernelGGL(k_ind_kfinish, dim3((a.K + 3) / 4), b256, 0, c->stream, a);
-        if ((rc = red_finish(c, RS_IND_LONG, (a.K + 3) / 4, 9, a.terms + M_LONG_E))) return rc;
-        if (c->mp_prec == SMD_PREC_MIXED) hipLaunchKernelGGL((k_ind_kforce<float>), dim3((c->n + WAVE - 1) / WAVE, a.ksplit), dim3(WAVE), 0, c->stream, a);
-        else hipLaunchKernelGGL((k_ind_kforce<double>), dim3((c->n + WAVE - 1) / WAVE, a.ksplit), dim3(WAVE), 0, c->stream, a);
         hipLaunchKernelGGL(k_kforce_sum, ga, b256, 0, c->stream, a);
         if (c->indf.torques && (rc = launch_torques(c, a, gm, 8u, RS_IND_DUDV_L))) return rc;
     }

@@ -76,7 +76,7 @@ struct RepState {
     double tilt, fnh, fnh_minus;
     double drift[3];
     int32_t K, need, since, apply_drift;
-    int32_t ticket, pending, max_row, pad;   // pending: rebuild at the next step
+    int32_t pad0, pending, max_row, pad;   // pending: rebuild at the next step
     unsigned long long maxd2;     // bits of the largest |d|^2 of the last step (non-negative doubles order as integers)
     int64_t steps, list_updates;
 };
@@ -87,6 +87,7 @@ struct Params {
     double b2[3];                 // squared constraint lengths H1-O, O-H2, H2-H1
     double mass[3], q[3];         // per site of a molecule
     double pre2;                  // (rlist + 2 r_OH)^2: the list's O-O prefilter
+    double two_roh;               // 2 r_OH with a rounding margin
     int32_t max_count, nose_hoover, nb_cap, n;
 };
 
@@ -281,20 +282,38 @@ __global__ __launch_bounds__(LIST_WAVES * WAVE) void k_list(Args a) {
         const int mj = m0 + lane;
         unsigned bits[3] = {0u, 0u, 0u};   // bit s of bits[q]: site s of mj is within rlist of row site q
         if (lane < nt && mj != mi) {
-            double dx = tile[lane][3] - p[1][0], dy = tile[lane][4] - p[1][1], dz = tile[lane][5] - p[1][2];
+            const double ox = tile[lane][3] - p[1][0], oy = tile[lane][4] - p[1][1], oz = tile[lane][5] - p[1][2];
+            double dx = ox, dy = oy, dz = oz;
             le_image(dx, dy, dz, rp.L, rp.invL, tilt);
-            if (dx * dx + dy * dy + dz * dz < a.p.pre2) {
+            const double d2 = dx * dx + dy * dy + dz * dz;
+            if (d2 < a.p.pre2) {
+                // Within L/2 - 2 r_OH of each other, every site pair of the two
+                // molecules is closer than L/2, so its minimum image is the O-O
+                // one: shift the partner by that image's lattice vector once
+                // instead of searching per site pair.
+                const double hc = 0.5 * rp.L - a.p.two_roh;
+                const bool same = hc > 0.0 && d2 < hc * hc;
+                const double sx = same ? dx - ox : 0.0, sy = same ? dy - oy : 0.0, sz = same ? dz - oz : 0.0;
+                double t[3][3];
+#pragma unroll
+                for (int u = 0; u < 3; ++u) {
+                    t[u][0] = tile[lane][3 * u] + sx;
+                    t[u][1] = tile[lane][3 * u + 1] + sy;
+                    t[u][2] = tile[lane][3 * u + 2] + sz;
+                }
 #pragma unroll
                 for (int q = 0; q < 3; ++q)
 #pragma unroll
-                    for (int s = 0; s < 3; ++s) {
-                        double ex = tile[lane][3 * s] - p[q][0], ey = tile[lane][3 * s + 1] - p[q][1];
-                        double ez = tile[lane][3 * s + 2] - p[q][2];
-                        le_image(ex, ey, ez, rp.L, rp.invL, tilt);
-                        if (ex * ex + ey * ey + ez * ez < rl2) bits[q] |= 1u << s;
+                    for (int u = 0; u < 3; ++u) {
+                        double ex = t[u][0] - p[q][0], ey = t[u][1] - p[q][1], ez = t[u][2] - p[q][2];
+                        if (!same) le_image(ex, ey, ez, rp.L, rp.invL, tilt);
+                        if (ex * ex + ey * ey + ez * ez < rl2) bits[q] |= 1u << u;
                     }
             }
         }
+        // the rows' ordered compaction, skipped by a wave with no site pair
+        // inside rlist in this tile
+        if (__ballot((bits[0] | bits[1] | bits[2]) != 0u) == 0ull) continue;
 #pragma unroll
         for (int q = 0; q < 3; ++q) {
             const unsigned long long b0 = __ballot(bits[q] & 1u), b1 = __ballot(bits[q] & 2u),
@@ -555,8 +574,6 @@ __global__ __launch_bounds__(256) void k_kforce(Args a, int tile) {
 
 // ---- SHAKE with SLLOD, kinetics, rewrap, and the replica's epilogue -------
 
-__device__ __forceinline__ void replica_finish(const Args& a, int r, int step_kind);
-
 // One thread per molecule.  vtMinus loses the drift the previous epilogue
 // found (LFIntegrator.cpp:354-379), the forces are the pair and reciprocal
 // sums, the update is LFIntegrator::shake's with the SLLOD terms, then the
@@ -741,42 +758,43 @@ __global__ __launch_bounds__(KIN_BLOCK) void k_shake(Args a) {
         if (threadIdx.x % WAVE == 0 && d2 > 0.0) atomicMax(&st.maxd2, (unsigned long long)__double_as_longlong(d2));
     }
     block_store<NSH>(v, a.sh_part, a.n_sh_blk);
-    // the replica's last block to finish runs its epilogue
-    __shared__ bool last;
-    __threadfence();
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        last = atomicAdd(&st.ticket, 1) == rp.sb1 - rp.sb0 - 1;
-        if (last) st.ticket = 0;
-    }
-    __syncthreads();
-    if (last) {
-        __threadfence();
-        replica_finish(a, r, 0);
-    }
 }
 
-// Fixed-order total of part[k * stride + b0 .. b1) for k < nv into dst (one
-// wave per row, every lane of the block takes part).
-__device__ __forceinline__ void range_totals(const double* part, int stride, int b0, int b1, int nv, double* dst) {
-    const int nw = blockDim.x / WAVE, w = threadIdx.x / WAVE;
-    for (int k = w; k < nv; k += nw) {
-        const double s = wave_fixed_sum((const volatile double*)(part + (size_t)k * stride + b0), b1 - b0);
-        if (threadIdx.x % WAVE == 0) dst[k] = s;
-    }
-    __syncthreads();
-}
-
-// The replica's epilogue: its sums in a fixed order, the report, the
-// friction (:688-709), the drift for the next step, the tilt and the list
-// age.
-__device__ __forceinline__ void replica_finish(const Args& a, int r, int step_kind) {
+// The replica's epilogue, one block of FINISH_THREADS per replica after
+// k_shake (step_kind 0) or k_start (1): its sums in a fixed order (one wave
+// per row, lane l adding blocks l, l + 64, ... of the replica in order, then
+// a shuffle tree), the report, the friction (:688-709), the drift for the
+// next step, the tilt and the list age.
+__global__ __launch_bounds__(FINISH_THREADS) void k_finish(Args a, int step_kind) {
+    const int r = blockIdx.x;
     __shared__ double tp[NPAIR], tk[NKF], ts[NSH];
     const Rep rp = a.rep[r];
     RepState& st = a.st[r];
-    range_totals(a.pair_part, a.n_pair_blk, rp.pb0, rp.pb1, NPAIR, tp);
-    range_totals(a.kf_part, a.n_fin_blk, rp.fb0, rp.fb1, NKF, tk);
-    range_totals(a.sh_part, a.n_sh_blk, rp.sb0, rp.sb1, NSH, ts);
+    const int w = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
+    for (int k = w; k < NPAIR + NKF + NSH; k += blockDim.x / WAVE) {
+        const double* p;
+        int nb;
+        double* dst;
+        if (k < NPAIR) {
+            p = a.pair_part + (size_t)k * a.n_pair_blk + rp.pb0;
+            nb = rp.pb1 - rp.pb0;
+            dst = tp + k;
+        } else if (k < NPAIR + NKF) {
+            p = a.kf_part + (size_t)(k - NPAIR) * a.n_fin_blk + rp.fb0;
+            nb = rp.fb1 - rp.fb0;
+            dst = tk + (k - NPAIR);
+        } else {
+            p = a.sh_part + (size_t)(k - NPAIR - NKF) * a.n_sh_blk + rp.sb0;
+            nb = rp.sb1 - rp.sb0;
+            dst = ts + (k - NPAIR - NKF);
+        }
+        double sum = 0.0;
+#pragma unroll 8
+        for (int b = lane; b < nb; b += WAVE) sum += p[b];
+        sum = wave_sum(sum);
+        if (lane == 0) *dst = sum;
+    }
+    __syncthreads();
     if (threadIdx.x != 0) return;
     smd_batch_report& o = a.out[r];
     const double dt = a.p.dt;
@@ -865,18 +883,6 @@ __global__ __launch_bounds__(KIN_BLOCK) void k_start(Args a) {
         }
     }
     block_store<NSH>(v, a.sh_part, a.n_sh_blk);
-    __shared__ bool last;
-    __threadfence();
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        last = atomicAdd(&st.ticket, 1) == rp.sb1 - rp.sb0 - 1;
-        if (last) st.ticket = 0;
-    }
-    __syncthreads();
-    if (last) {
-        __threadfence();
-        replica_finish(a, r, 1);
-    }
 }
 
 }  // namespace nemd
@@ -1112,6 +1118,7 @@ static int enqueue_step(smd_batch* b) {
         BHIP(b, hipStreamWaitEvent(b->stream, b->ev_join, 0));
     }
     hipLaunchKernelGGL(k_shake, dim3(b->n_sh_blk), dim3(KIN_BLOCK), 0, b->stream, a);
+    hipLaunchKernelGGL(k_finish, dim3(b->R), dim3(FINISH_THREADS), 0, b->stream, a, 0);
     BHIP(b, hipGetLastError());
     return SMD_OK;
 }
@@ -1210,7 +1217,8 @@ extern "C" int smd_batch_create(smd_batch** out, const smd_batch_params* p, int 
     q.mass[0] = p->mass_h; q.mass[1] = p->mass_o; q.mass[2] = p->mass_h;
     q.q[0] = p->q_h; q.q[1] = p->q_o; q.q[2] = p->q_h;
     {
-        const double pre = rl + 2.0 * p->r_oh * (1.0 + 1e-6) + 1e-9;
+        q.two_roh = 2.0 * p->r_oh * (1.0 + 1e-6) + 1e-9;
+        const double pre = rl + q.two_roh;
         q.pre2 = pre * pre;
     }
     q.max_count = p->max_count;
@@ -1302,6 +1310,7 @@ extern "C" int smd_batch_start(smd_batch* b) {
     a.zero_disp = 1;
     for (unsigned pass : {P_BEGIN, P_LIST, P_PAIRS, P_SFAC, P_KFIN, P_KFORCE}) launch_pass(b, b->stream, a, pass);
     hipLaunchKernelGGL(k_start, dim3(b->n_sh_blk), dim3(KIN_BLOCK), 0, b->stream, a);
+    hipLaunchKernelGGL(k_finish, dim3(b->R), dim3(FINISH_THREADS), 0, b->stream, a, 1);
     BHIP(b, hipGetLastError());
     int rc = check_err(b);
     if (rc) return rc;

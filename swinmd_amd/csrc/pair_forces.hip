@@ -55,6 +55,7 @@ struct PairArgs {
     double s0, s1, s2, s3, s4, s5; // 5th-order taper (buffered 14-7)
     double n_mie, m_mie, cn_mie;
     double n_buff, m_buff, delta, gamma;
+    int buff_147;                  // n_buff = 14 and m_buff = 7: integer powers
     // Ewald
     double alpha, alpha2, alpha2PI, realcut2;
 };
@@ -279,12 +280,22 @@ __device__ __forceinline__ void pair_term(const PairArgs& a, const Owner& o, int
                     const R invs = R(1) / sigma;
                     const R m = (R)a.m_buff, nn = (R)a.n_buff, del = (R)a.delta, gam = (R)a.gamma;
                     const R rho = r * invs;
-                    const R rho_m = pow_<R>(rho, m);
-                    const R rho_m1 = pow_<R>(rho, m - R(1));
                     const R expnm = nn - m;
                     const R par1 = (R(1) + del) / (rho + del);
+                    R rho_m, rho_m1, p1n;
+                    if (a.buff_147) {
+                        // the 14-7 exponents by products: rho^6, rho^7, par1^7
+                        const R rho2 = rho * rho, rho4 = rho2 * rho2;
+                        rho_m1 = rho2 * rho4;
+                        rho_m = rho_m1 * rho;
+                        const R q2 = par1 * par1, q4 = q2 * q2;
+                        p1n = q4 * q2 * par1;
+                    } else {
+                        rho_m = pow_<R>(rho, m);
+                        rho_m1 = pow_<R>(rho, m - R(1));
+                        p1n = pow_<R>(par1, expnm);
+                    }
                     const R par2 = (R(1) + gam) / (rho_m + gam);
-                    const R p1n = pow_<R>(par1, expnm);
                     tmpE = eps * p1n * (par2 - R(2));
                     tmpf = -eps * expnm * p1n * (par2 - R(2)) * invs / (rho + del) -
                            eps * p1n * par2 * m * rho_m1 * invs / (rho_m + gam);
@@ -497,6 +508,7 @@ static void vdw_constants(PairArgs& a, const smd_vdw_params* p) {
     }
     a.n_buff = p->n_buff;
     a.m_buff = p->m_buff;
+    a.buff_147 = p->n_buff == 14.0 && p->m_buff == 7.0;
     a.delta = p->delta;
     a.gamma = p->gamma;
 }

@@ -18,6 +18,8 @@
 //      many batches as the previous step needed);
 //   C  Induction's short- and long-range kernels with the converged dipoles,
 //      then SHAKE with the kinetic and pressure sums and the NH update.
+// In A and C the k-space sums and per-site partial forces run on a second
+// stream beside the real-space kernels (the recip_overlap tuning switch).
 // A and C hold no host synchronisation and no memset: each is captured once
 // as a hipGraph and replayed.
 #include <algorithm>
@@ -158,6 +160,12 @@ extern "C" int smd_polar_md_init(smd_ctx* c, const smd_polar_md_params* p) {
     if ((rc = smd_multipole_setup(c, &p->mp, p->mp_kcut, nullptr))) return rc;
     if ((rc = smd_induction_setup(c, &p->indf, p->ind_kcut, nullptr))) return rc;
     if (!c->d_polar_terms) SMD_HIP(c, hipMalloc(&c->d_polar_terms, sizeof(double) * 2 * POLAR_TERMS));
+    if (!c->stream2) {
+        // the second stream of the step's k-space passes (default priority)
+        SMD_HIP(c, hipStreamCreateWithFlags(&c->stream2, hipStreamNonBlocking));
+        SMD_HIP(c, hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming));
+        SMD_HIP(c, hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming));
+    }
     if (!c->d_stress) SMD_HIP(c, hipMalloc(&c->d_stress, sizeof(double) * 12));
     if ((rc = pairlist_build(c, p->cutoff, p->cut_buff, p->use_cells, false))) return rc;
     {

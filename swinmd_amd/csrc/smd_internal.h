@@ -193,6 +193,8 @@ struct Ctx {
     bool multipoles_set = false;
     int npol = 0;                  // sites with a polarizability above POLMIN
     float4* d_cg32 = nullptr;      // mixed solve: FP32 copy of the CG direction [n]
+    float4* d_ind_rec32 = nullptr; // mixed solve: hi/lo FP32 site records [2n]
+    double* d_e0_prev = nullptr;   // E0 of the previous solve with guess 4 [3n]
     int mp_prec = 0;               // smd_prec of the multipole / induction real-space pair math
     float4* d_site32 = nullptr;    // mixed real space: FP32 site records [3n]
     float4* d_pind32 = nullptr;    // mixed real space: FP32 induced dipoles [n]
@@ -248,6 +250,7 @@ struct Ctx {
     // the list rebuild's rewrap and joined before SHAKE; its forces go to
     // d_frecip, which SHAKE adds after the short-range terms.
     hipStream_t stream2 = nullptr;
+    hipStream_t polar_main = nullptr;   // the step's stream while a polarizable k-space fork is open
     hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     hipEvent_t list_fork = nullptr;   // recorded by the list build after its preamble when set
     double* d_frecip = nullptr;       // [3n]
@@ -456,33 +459,6 @@ __device__ __forceinline__ double wave_fixed_sum(const volatile double* p, int n
     return wave_sum(s);
 }
 
-// True in the last block of the grid to get here (a ticket on *counter,
-// which that block resets).  Every block calls it after storing its
-// partials; the result is block-uniform.
-__device__ __forceinline__ bool last_block(int* counter) {
-    __shared__ bool last;
-    __threadfence();
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        last = atomicAdd(counter, 1) == (int)(gridDim.x * gridDim.y * gridDim.z) - 1;
-        if (last) *counter = 0;
-    }
-    __syncthreads();
-    if (last) __threadfence();
-    return last;
-}
-
-// In the last block: dst[k] += fixed-order total of part[k * stride + 0..nb)
-// for k < nv; wave w takes rows w, w + waves, ...
-__device__ __forceinline__ void fixed_totals(const double* part, int stride, int nb, int nv, double* dst) {
-    const int nw = blockDim.x / WAVE, w = threadIdx.x / WAVE;
-    for (int k = w; k < nv; k += nw) {
-        const double s = wave_fixed_sum((const volatile double*)(part + (size_t)k * stride), nb);
-        if (threadIdx.x % WAVE == 0) dst[k] += s;
-    }
-    __syncthreads();
-}
-
 // ---- deterministic block sums ---------------------------------------------
 // Each kernel that reduces energies or virials over its grid owns a site of
 // the context's reduction arena (at most RED_NV values for at most cap
@@ -493,6 +469,26 @@ __device__ __forceinline__ void fixed_totals(const double* part, int stride, int
 // the device-wide release fence it needs cost more than the extra launch on
 // a multi-XCD part.  Sites that can run at the same time are distinct.
 __device__ __forceinline__ double* red_part(const Red& r, int site) { return r.base + (size_t)site * RED_NV * r.cap; }
+
+// Fixed-order totals of block partials, one wave per row: dst[k] += the sum
+// over b < nb of part[k * stride + b], lane l adding b = l, l + 64, ... in
+// order (the loads unrolled, so several are in flight), then a shuffle tree.
+// Rows from `split` on go to dst2[k - split] instead.
+__device__ __forceinline__ void rows_total(const double* __restrict__ part, int stride, int nb, int nrows,
+                                           double* dst, int split = 1 << 30, double* dst2 = nullptr) {
+    const int w = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
+    for (int k = w; k < nrows; k += blockDim.x / WAVE) {
+        const double* p = part + (size_t)k * stride;
+        double s = 0.0;
+#pragma unroll 8
+        for (int i = lane; i < nb; i += WAVE) s += p[i];
+        s = wave_sum(s);
+        if (lane == 0) {
+            if (k < split) dst[k] += s;
+            else dst2[k - split] += s;
+        }
+    }
+}
 
 template <int NV>
 __device__ __forceinline__ void block_part(double (&v)[NV], const Red& r, int site) {
@@ -526,6 +522,14 @@ int static_scratch_init(Ctx* c);
 // Adds the nv block partials of a site's last launch (nblocks blocks) to dst,
 // in block order, on the context's stream.
 int red_finish(Ctx* c, int site, unsigned nblocks, int nv, double* dst);
+// Adds rows [0, nrows) of block partials part[k * stride + b], b < nb, to
+// dst in block order: one block of FINISH_THREADS on stream s, after the
+// kernel that stored them (in place of a last-block ticket, whose
+// device-wide release fence costs more than a launch on a multi-XCD part).
+constexpr int FINISH_THREADS = 1024;
+// Rows from `split` on go to dst2[k - split].
+int rows_finish(Ctx* c, hipStream_t s, const double* part, int stride, int nb, int nrows, double* dst,
+                int split = 1 << 30, double* dst2 = nullptr);
 // Launches one kernel that adds every queued finish (in queue order, each in
 // block order) and clears the queue; no-op when the queue is empty.
 int red_flush(Ctx* c);

@@ -182,7 +182,7 @@ def test_learned_guess_errors():
     with pytest.raises(_lib.SmdError) as e:
         net.set_norm(bad)
     assert e.value.code == _lib.SMD_EINVAL
-    ind.params.guess = 4
+    ind.params.guess = 5
     with pytest.raises(_lib.SmdError) as e:
         ind.induce()
     assert e.value.code == _lib.SMD_EINVAL

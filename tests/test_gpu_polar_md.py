@@ -116,7 +116,7 @@ def test_preconditioned_solve_reaches_the_reference_dipoles(prec):
         it_t, conv, _ = ind2.induce()
         assert conv and rel_rms(ind2.results()[3], mu_ref) <= 1e-9
     # every start reaches the same dipoles
-    for guess in (1, 2):
+    for guess in (1, 2, 4):
         ind2.params.guess = guess
         ind2.params.tol = tol
         it_g, conv, _ = ind2.induce()
@@ -124,17 +124,20 @@ def test_preconditioned_solve_reaches_the_reference_dipoles(prec):
         assert conv and np.sqrt(np.mean(np.sum((mu_g - mu_ref) ** 2, axis=1))) <= 20 * tol
 
 
-def test_polar_md_preconditioned_steps_follow_the_reference_method():
-    """The step with the preconditioned FP64 solve at 1e-6 D, FP64, stays on
-    the reference-method trajectory over 10 steps to the accuracy the
-    tolerance allows, and needs fewer solve iterations per step."""
+@pytest.mark.parametrize("guess", [2, 4])
+def test_polar_md_preconditioned_steps_follow_the_reference_method(guess):
+    """The step with the preconditioned FP64 solve at 1e-6 D, FP64, from
+    alpha E0 (guess 2) or from the previous step's dipoles moved by alpha dE0
+    (guess 4), stays on the reference-method trajectory over 10 steps to the
+    accuracy the tolerance allows, and needs fewer solve iterations per step."""
     from swinmd_amd import _lib as L
     fx = Fixture("amoeba216_md10_polar")
     ctx = _ctx(fx.system)
     ctx.set_velocities(fx.vec("init_vel"))
-    _, md = _polar_md(fx, ctx, True, tol=1e-6 * L.DEBYE_E_NM, precond=True, guess=2)
+    _, md = _polar_md(fx, ctx, True, tol=1e-6 * L.DEBYE_E_NM, precond=True, guess=guess)
     md.init()
     rep = md.run(int(fx.extra))
+    print(f"guess {guess}: {rep['cg_iterations'] / rep['steps']:.2f} iterations per step", flush=True)
     assert rel_rms(ctx.positions(), fx.vec("end_pos")) <= 1e-7
     assert rel_rms(ctx.velocities(), fx.vec("end_vel")) <= 1e-4
     assert rel(rep["e_mp_real"], fx.meta["e_real"]) <= 1e-5
