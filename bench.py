@@ -180,16 +180,15 @@ def polar_cpu_baseline(system, reps=1):
     return base, forces
 
 
-def train_learned_guess(ctx, ind, nmol, seeds, epochs=60, batch=1024):
-    """Trains the learned guess on the GPU from converged solves of other
-    seeds' frames of the same workload; returns the network's report."""
-    from swinmd_amd import systems as S
+def train_learned_guess(ctx, ind, frames, epochs=60, batch=1024):
+    """Trains the learned guess on the GPU from converged solves of the given
+    frames (positions) of the workload; returns the network and its report."""
     from swinmd_amd.forces import LearnedGuess
     net = LearnedGuess(ctx, seed=1)
     guess = ind.params.guess
     t0 = time.perf_counter()
-    for sd in seeds:
-        ctx.set_positions(S.polar_water(nmol, seed=sd).pos)
+    for pos in frames:
+        ctx.set_positions(pos)
         ctx.set_pairlist(POLAR["cutoff"], POLAR["cut_buff"], True)
         ind.params.guess = 2
         _, conv, _ = ind.induce()
@@ -200,7 +199,7 @@ def train_learned_guess(ctx, ind, nmol, seeds, epochs=60, batch=1024):
     rep = net.train(epochs=epochs, batch=batch, lr=2e-3, lr_final=2e-5, seed=9)
     ind.params.guess = guess
     rep["frames_seconds"] = round(frames_s, 3)
-    rep["seeds"] = list(seeds)
+    rep["frames"] = len(frames)
     return net, rep
 
 
@@ -268,8 +267,10 @@ def bench_polar_md(args):
         ctx.set_velocities(vel)
         ctx.set_pairlist(POLAR["cutoff"], POLAR["cut_buff"], True)
         ctx.set_tuning("cg_batch", cg_batch)
+        if args.no_overlap:
+            ctx.set_tuning("recip_overlap", 0)
         ind = Induction(ctx, cutoff_ind=POLAR["cutoff_ind"], a_damp=POLAR["a_damp"], damp_tij=True, do_min=True,
-                        tol=tol, precond=True, guess=guess, prec=prec)
+                        tol=tol, precond=args.precond, guess=guess, prec=prec)
         if net_state is not None:
             net = LearnedGuess(ctx, seed=1)
             net.set_params(net_state[0])
@@ -280,33 +281,37 @@ def bench_polar_md(args):
         md.init()
         return ctx, ind, md
 
-    # the learned guess: trained once on the GPU on converged FP64 solves of
-    # other seeds' frames, then loaded into each context that uses it
-    log("training the learned guess on other seeds' frames")
-    tctx = Context(system)
-    tctx.set_pairlist(POLAR["cutoff"], POLAR["cut_buff"], True)
-    tind = Induction(tctx, cutoff_ind=POLAR["cutoff_ind"], a_damp=POLAR["a_damp"], damp_tij=True, do_min=True,
-                     tol=tol, precond=True, guess=2)
-    net, train_rep = train_learned_guess(tctx, tind, nmol, seeds=(1, 2, 3) if nmol == 8192 else (1, 2, 3, 4, 5, 6))
-    net_state = (net.params(), net.norm())
-    tctx.close()
-
     # the lattice start relaxes under the rigid point-charge model (same
     # coordinates, masses and constraints), then under the polarizable model
-    # on the mixed path; every timed path starts from that state
+    # on the mixed path from the extrapolated previous dipoles; three
+    # snapshots of that equilibration train the learned guess (on the GPU,
+    # converged solves), and every timed path starts from its end
     log(f"point-charge relaxation {args.polar_pre} steps")
     pctx, pmd = build_md(S.point_charge_twin(system), L.PREC_MIXED, system.pos, vel0)
     pmd.run(args.polar_pre)
     pos_pre, vel_pre = pctx.positions(), pctx.velocities()
     pctx.close()
     log(f"equilibrating {args.polar_equil} steps")
-    ctx, ind, md = make(pos_pre, vel_pre, L.PREC_MIXED, 3, net_state)
-    md.run(args.polar_equil)
+    ctx, ind, md = make(pos_pre, vel_pre, L.PREC_MIXED, 4)
+    snap_at = [args.polar_equil - 100, args.polar_equil - 50, args.polar_equil]
+    done, snaps = 0, []
+    for at in snap_at:
+        if at > done:
+            md.run(at - done)
+            done = at
+        snaps.append(ctx.positions())
     pos_eq, vel_eq = ctx.positions(), ctx.velocities()
     ctx.close()
+    log("training the learned guess on equilibration snapshots")
+    tctx = Context(system)
+    tctx.set_pairlist(POLAR["cutoff"], POLAR["cut_buff"], True)
+    tind = Induction(tctx, cutoff_ind=POLAR["cutoff_ind"], a_damp=POLAR["a_damp"], damp_tij=True, do_min=True,
+                     tol=tol, precond=args.precond, guess=2)
+    net, train_rep = train_learned_guess(tctx, tind, snaps)
+    train_rep["snapshot_steps"] = [max(a, 0) for a in snap_at]
+    net_state = (net.params(), net.norm())
+    tctx.close()
 
-    # guess 0: the reference's Pi = 1; 1: the previous step's dipoles; 2: alpha
-    # E0; 3: the learned guess; 4: the previous dipoles moved by alpha dE0
     paths = [("mixed_learned", L.PREC_MIXED, 3), ("fp64_reference_start", L.PREC_FP64, 0),
              ("fp64_learned", L.PREC_FP64, 3), ("mixed_reference_start", L.PREC_MIXED, 0),
              ("mixed_alpha_e0_start", L.PREC_MIXED, 2), ("mixed_previous_start", L.PREC_MIXED, 1),
@@ -357,12 +362,14 @@ def bench_polar_md(args):
         "scaling": "weak", "vs_baseline": None, "dtype": "f32 pair and A v math, f64 accumulation and solve",
         "data": f"synthetic: seeded lattice of {nmol} polarizable waters with random orientations (seed 2025), "
                 f"reference Maxwell velocities (seed 1), {args.polar_pre} untimed point-charge relaxation steps, "
-                f"then {args.polar_equil} untimed polarizable steps",
+                f"then {args.polar_equil} untimed polarizable steps; the learned guess trained on 3 snapshots "
+                f"of those steps",
         "config": {"workload": f"polarizable water MD step (buffered 14-7 + multipole Ewald + Thole induced dipoles "
                                f"+ SHAKE + Nose-Hoover), {nmol} molecules", "molecules": nmol, "atoms": n,
                    "box_nm": float(system.box[0]), "cutoff_nm": POLAR["cutoff"], "buffer_nm": POLAR["cut_buff"],
                    "alpha_per_nm": POLAR["alpha"], "n2max": N2MAX, "thole_a": POLAR["a_damp"],
-                   "scf": "CG preconditioned by diag(alpha) to an RMS dipole residual of 1e-6 D on every path",
+                   "scf": ("CG preconditioned by each molecule's 9x9 block of A" if args.precond == 2 else
+                           "CG preconditioned by diag(alpha)") + " to an RMS dipole residual of 1e-6 D on every path",
                    "path": "mixed precision + learned initial guess", "dt_ps": MD["dt"],
                    "thermostat": "Nose-Hoover 300 K", "parallelism": "single GPU",
                    "step_launch": "hipGraph replay of the force parts, captured batches of solve iterations"
@@ -402,6 +409,7 @@ def bench_polar_md(args):
     out["cg_apply"] = {"us": round(iso["cg_apply"], 2), "algorithmic_bytes": alg,
                        "achieved_GBs": round(alg / (iso["cg_apply"] * 1e-6) / 1e9, 1),
                        "hbm_frac": round(alg / (iso["cg_apply"] * 1e-6) / 1e9 / PEAK_HBM_GBS, 4),
+                       "traffic": measured_traffic(f"polar_water{nmol}").get("cg_apply"),
                        "note": "the mixed solve's A v: a 4 B partner index per directed pair, Thole factors "
                                "recomputed in FP32 from hi/lo site records"}
     if not args.no_cpu_baseline:
@@ -587,6 +595,9 @@ def main():
     ap.add_argument("--polar-equil", type=int, default=POLAR_EQUIL,
                     help="untimed equilibration steps of the polarizable workloads")
     ap.add_argument("--polar-profile-steps", type=int, default=10)
+    ap.add_argument("--precond", type=int, default=2, choices=[1, 2],
+                    help="preconditioner of every polarizable path's solve: 1 diag(alpha), 2 each molecule's 9x9 "
+                         "block of A")
     ap.add_argument("--polar-pre", type=int, default=1500,
                     help="point-charge relaxation steps before the polarizable equilibration")
     ap.add_argument("--gpus", type=int, default=1)
@@ -603,6 +614,8 @@ def main():
                     help="keep the default launch forms instead of timing every combination first")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="first copy MIB MiB device to device (counter calibration for profiling runs)")
+    ap.add_argument("--no-overlap", action="store_true",
+                    help="polarizable workloads: keep the k-space passes on the step's stream")
     ap.add_argument("--no-graph", action="store_true",
                     help="launch each step's kernels directly instead of replaying a captured hipGraph")
     args = ap.parse_args()

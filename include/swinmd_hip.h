@@ -256,12 +256,14 @@ typedef struct {
                               alpha (E0 - E0 of the previous guess-4 solve) */
     int32_t prec;          /* smd_prec: SMD_PREC_FP64, or SMD_PREC_MIXED (FP32 coupling table and
                               A v, FP64 vectors, scalars and residual; preconditioned solve only) */
-    int32_t precond;       /* 0: the reference's CG; 1: CG preconditioned by M^-1 = diag(alpha)
-                              (no counterpart in the reference, whose solve is unpreconditioned) */
+    int32_t precond;       /* 0: the reference's CG; 1: CG preconditioned by M^-1 = diag(alpha);
+                              2: CG preconditioned by the inverse of each 3-site molecule's 9x9
+                              diagonal block of A (block Jacobi).  1 and 2 have no counterpart in
+                              the reference, whose solve is unpreconditioned */
     int32_t tol_rms;       /* 0: stop when |r|_2 < tol (the reference, :2431-2457); 1: stop when
                               the RMS over polarizable sites of |alpha_i r_i| -- the dipole step
-                              of the preconditioner, in e nm -- is < tol (1e-6 D = 2.0819e-8
-                              e nm); 1 requires precond 1 and 0 requires precond 0 */
+                              of a Jacobi iteration, in e nm -- is < tol (1e-6 D = 2.0819e-8
+                              e nm); 1 requires precond 1 or 2 and 0 requires precond 0 */
 } smd_induce_params;
 
 /* 1 Debye in e nm: the unit of the tol_rms tolerance's configs. */

@@ -61,7 +61,7 @@ struct IndArgs {
     int32_t* tj;                // [n][tstride]
     double2* tab;               // [n][tstride] (a, b): one 16-byte load per entry (FP64 solve)
     // mixed solve: the factors are recomputed in FP32 from hi/lo site records
-    // rec32[2i] = (x_hi, y_hi, z_hi, 1/sqrt(alpha)), rec32[2i+1] = (x_lo, y_lo, z_lo, 0)
+    // rec32[2i] = (x_hi, y_hi, z_hi, 1/sqrt(alpha)), rec32[2i+1] = (x_lo, y_lo, z_lo, q / sqrt(C))
     int mixed;
     float4* rec32;
     float Lh[3], Ll[3], invL[3];
@@ -548,8 +548,9 @@ __global__ __launch_bounds__(256) void k_apply_table(IndArgs a, const double* __
 }
 
 // Hi/lo FP32 site records of the mixed solve: x_hi = float(x), x_lo =
-// float(x - x_hi), and 1/sqrt(alpha) (alpha clamped at POLMIN) in the first
-// record's w.
+// float(x - x_hi), 1/sqrt(alpha) (alpha clamped at POLMIN) in the first
+// record's w and q / sqrt(Coulomb constant) in the second's (the learned
+// guess's descriptors read it).
 __global__ __launch_bounds__(256) void k_pack_ind32(IndArgs a) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= a.n) return;
@@ -557,7 +558,8 @@ __global__ __launch_bounds__(256) void k_pack_ind32(IndArgs a) {
     const float xh = (float)x, yh = (float)y, zh = (float)z;
     const double p = a.polar[i] > POLMIN ? a.polar[i] : POLMIN;
     a.rec32[2 * i] = make_float4(xh, yh, zh, (float)(1.0 / sqrt(p)));
-    a.rec32[2 * i + 1] = make_float4((float)(x - (double)xh), (float)(y - (double)yh), (float)(z - (double)zh), 0.0f);
+    a.rec32[2 * i + 1] = make_float4((float)(x - (double)xh), (float)(y - (double)yh), (float)(z - (double)zh),
+                                     (float)(a.q[i] / SQRT_COULOMB));
 }
 
 // A v of the mixed solve over the coupling table's partners: each pair's
@@ -682,6 +684,8 @@ struct CgArgs {
     double* scal;     // [0] last residual norm
     float4* v32;      // mixed solve: FP32 copy of the direction (else null)
     double inv_npol;  // 1 / (polarizable sites), for the RMS dipole residual
+    int nmol;         // block preconditioner: 3-site molecules, sites 3m..3m+2
+    const double* blk;  // [nmol][45]: Cholesky factor of each molecule's 9x9 block of A (else null)
 };
 
 // x = guess (1 as the reference, or the caller's), then after z = A x:
@@ -832,6 +836,188 @@ __global__ __launch_bounds__(CG_THREADS) void k_pcg_dir(CgArgs g) {
     }
 }
 
+// ---- block-Jacobi preconditioner ---------------------------------------------------
+// M^-1 = the inverse of each molecule's 9x9 diagonal block of A (its three
+// sites' 1/alpha and the intramolecular Thole tensors, which A couples),
+// held as the block's Cholesky factor L (row-major lower triangle, 45
+// doubles) and applied as L^-T L^-1 r.  The stopping test stays the RMS of
+// alpha r.  One thread per molecule in the vector kernels.
+
+__device__ __forceinline__ int tri(int i, int j) { return i * (i + 1) / 2 + j; }
+
+// Assembles and factors each molecule's block: thread per molecule.
+__global__ __launch_bounds__(256) void k_block_pc(IndArgs a, double* __restrict__ blk, int nmol) {
+    const int m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= nmol) return;
+    double B[45];
+#pragma unroll
+    for (int k = 0; k < 45; ++k) B[k] = 0.0;
+    double pl[3];
+#pragma unroll
+    for (int s = 0; s < 3; ++s) {
+        const double p = a.polar[3 * m + s];
+        pl[s] = p > POLMIN ? p : POLMIN;
+#pragma unroll
+        for (int x = 0; x < 3; ++x) B[tri(3 * s + x, 3 * s + x)] = 1.0 / pl[s];
+    }
+    // A_st = -T_st for the pairs s > t of the molecule (inside cutoffInd, as A has them)
+#pragma unroll
+    for (int s = 1; s < 3; ++s)
+#pragma unroll
+        for (int t = 0; t < s; ++t) {
+            const int i = 3 * m + s, j = 3 * m + t;
+            double dx = a.pos[3 * i] - a.pos[3 * j], dy = a.pos[3 * i + 1] - a.pos[3 * j + 1];
+            double dz = a.pos[3 * i + 2] - a.pos[3 * j + 2];
+            pbc_near(dx, dy, dz, a.box);
+            const double r = sqrt(dx * dx + dy * dy + dz * dz);
+            Thole th;
+            if (!thole(a, r * r, r, pl[s], pl[t], th)) continue;
+            const double ca = a.damp ? th.l5 * 3.0 * th.r5 : 3.0 * th.r5;
+            const double cb = a.damp ? th.l3 * th.r3 : th.r3;
+            const double d[3] = {dx, dy, dz};
+#pragma unroll
+            for (int x = 0; x < 3; ++x)
+#pragma unroll
+                for (int y = 0; y < 3; ++y) B[tri(3 * s + x, 3 * t + y)] = -(ca * d[x] * d[y] - (x == y ? cb : 0.0));
+        }
+    // Cholesky B = L L^T in place (B is a principal block of the positive-definite A)
+#pragma unroll
+    for (int j = 0; j < 9; ++j) {
+        double dj = B[tri(j, j)];
+#pragma unroll
+        for (int k = 0; k < j; ++k) dj -= B[tri(j, k)] * B[tri(j, k)];
+        dj = sqrt(dj);
+        B[tri(j, j)] = dj;
+        const double inv = 1.0 / dj;
+#pragma unroll
+        for (int i = j + 1; i < 9; ++i) {
+            double v = B[tri(i, j)];
+#pragma unroll
+            for (int k = 0; k < j; ++k) v -= B[tri(i, k)] * B[tri(j, k)];
+            B[tri(i, j)] = v * inv;
+        }
+    }
+    double* out = blk + 45 * (size_t)m;
+#pragma unroll
+    for (int k = 0; k < 45; ++k) out[k] = B[k];
+}
+
+// z = L^-T L^-1 r for one molecule's 9 components.
+__device__ __forceinline__ void block_solve(const double* __restrict__ Lm, const double (&r)[9], double (&z)[9]) {
+    double L[45];
+#pragma unroll
+    for (int k = 0; k < 45; ++k) L[k] = Lm[k];
+    double y[9];
+#pragma unroll
+    for (int i = 0; i < 9; ++i) {
+        double v = r[i];
+#pragma unroll
+        for (int k = 0; k < i; ++k) v -= L[tri(i, k)] * y[k];
+        y[i] = v / L[tri(i, i)];
+    }
+#pragma unroll
+    for (int i = 8; i >= 0; --i) {
+        double v = y[i];
+#pragma unroll
+        for (int k = i + 1; k < 9; ++k) v -= L[tri(k, i)] * z[k];
+        z[i] = v / L[tri(i, i)];
+    }
+}
+
+__device__ __forceinline__ double mol_alpha(const CgArgs& g, int m, int c) { return site_alpha(g.polar, 9 * m + c); }
+
+// After z = A x: r = E0 - A x, p = M^-1 r; partials of r.M^-1 r and |alpha r|^2.
+__global__ __launch_bounds__(CG_THREADS) void k_pcg_start_blk(CgArgs g) {
+    const int m = blockIdx.x * blockDim.x + threadIdx.x;
+    double rz = 0.0, zz = 0.0;
+    if (m < g.nmol) {
+        double r[9], z[9];
+#pragma unroll
+        for (int c = 0; c < 9; ++c) {
+            const int k = 9 * m + c;
+            r[c] = g.e0[k] - g.z[k];
+            g.r[k] = r[c];
+            const double ar = mol_alpha(g, m, c) * r[c];
+            zz += ar * ar;
+        }
+        block_solve(g.blk + 45 * (size_t)m, r, z);
+#pragma unroll
+        for (int c = 0; c < 9; ++c) {
+            const int k = 9 * m + c;
+            g.v[k] = z[c];
+            store_dir32(g, k, z[c]);
+            rz += r[c] * z[c];
+        }
+    }
+    block_partial(rz, g.p_rr0);
+    __syncthreads();
+    block_partial(zz, g.p_dv);
+    if (blockIdx.x == 0 && threadIdx.x == 0) { g.state[0] = 0; g.state[1] = 0; g.state[2] = 0; }
+}
+
+// t = rz / p.Ap; x += t p; r -= t Ap; z = M^-1 r (into the A v buffer, whose
+// product is consumed); partials of the new r.z and |alpha r|^2.
+template <int PHASE>
+__global__ __launch_bounds__(CG_THREADS) void k_pcg_step_blk(CgArgs g) {
+    if (g.state[0]) return;
+    const double* rz_c = PHASE ? g.p_rr1 : g.p_rr0;
+    double* rz_d = PHASE ? g.p_rr0 : g.p_rr1;
+    double tot[2];   // p.Ap, r.z
+    block_totals<2>(g.p_vz, g.nparts_mv, rz_c, g.nparts_vec, nullptr, 0, tot);
+    const double t = tot[1] / tot[0];
+    const int m = blockIdx.x * blockDim.x + threadIdx.x;
+    double rzn = 0.0, zz = 0.0;
+    if (m < g.nmol) {
+        double r[9], z[9];
+#pragma unroll
+        for (int c = 0; c < 9; ++c) {
+            const int k = 9 * m + c;
+            g.x[k] += t * g.v[k];
+            r[c] = g.r[k] - t * g.z[k];
+            g.r[k] = r[c];
+            const double ar = mol_alpha(g, m, c) * r[c];
+            zz += ar * ar;
+        }
+        block_solve(g.blk + 45 * (size_t)m, r, z);
+#pragma unroll
+        for (int c = 0; c < 9; ++c) {
+            g.z[9 * m + c] = z[c];
+            rzn += r[c] * z[c];
+        }
+    }
+    block_partial(rzn, rz_d);
+    __syncthreads();
+    block_partial(zz, g.p_dv);
+    if (blockIdx.x == 0 && threadIdx.x == 0) g.state[1] += 1;
+}
+
+// Stop when the RMS of alpha r is below tol; else p = z + beta p.
+template <int PHASE>
+__global__ __launch_bounds__(CG_THREADS) void k_pcg_dir_blk(CgArgs g) {
+    if (g.state[0]) return;
+    const double* rz_c = PHASE ? g.p_rr1 : g.p_rr0;
+    const double* rz_d = PHASE ? g.p_rr0 : g.p_rr1;
+    double tot[3];   // |alpha r|^2, new r.z, old r.z
+    block_totals<3>(g.p_dv, g.nparts_vec, rz_d, g.nparts_vec, rz_c, g.nparts_vec, tot);
+    const double res = sqrt(tot[0] * g.inv_npol);
+    if (blockIdx.x == 0 && threadIdx.x == 0) g.scal[0] = res;
+    if (res < g.tol) {
+        if (blockIdx.x == 0 && threadIdx.x == 0) { g.state[0] = 1; g.state[2] = 2; }
+        return;
+    }
+    const double beta = tot[1] / tot[2];
+    const int m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m < g.nmol) {
+#pragma unroll
+        for (int c = 0; c < 9; ++c) {
+            const int k = 9 * m + c;
+            const double v = g.z[k] + beta * g.v[k];
+            g.v[k] = v;
+            store_dir32(g, k, v);
+        }
+    }
+}
+
 // FP32 copy of x (float4 per site) for the mixed solve's first A x.
 __global__ __launch_bounds__(256) void k_to_f32(int dim, const double* __restrict__ x, float* __restrict__ x32) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
@@ -950,13 +1136,13 @@ static int induce_check(Ctx* c, const smd_induce_params* p) {
         return fail(c, SMD_ENOTSUP, "cutoff_ind %g exceeds the pair-list cutoff %g", p->cutoff_ind, c->list_cutoff);
     if (p->prec != SMD_PREC_FP64 && p->prec != SMD_PREC_MIXED) return fail(c, SMD_EINVAL, "bad precision %d", p->prec);
     if (p->guess < 0 || p->guess > 4) return fail(c, SMD_EINVAL, "guess %d is not 0..4", p->guess);
-    if (p->precond != 0 && p->precond != 1) return fail(c, SMD_EINVAL, "precond %d is not 0 or 1", p->precond);
-    if (p->tol_rms != p->precond)
+    if (p->precond < 0 || p->precond > 2) return fail(c, SMD_EINVAL, "precond %d is not 0, 1 or 2", p->precond);
+    if (p->tol_rms != (p->precond ? 1 : 0))
         return fail(c, SMD_ENOTSUP, "the RMS dipole tolerance goes with the preconditioned solve, |r| with the reference's");
     if (p->prec == SMD_PREC_MIXED && (!p->precond || !p->do_min))
         return fail(c, SMD_ENOTSUP, "the mixed-precision solve is the preconditioned CG");
     for (int m = 0; m < c->nmol; ++m)
-        if (c->h_mol_start[m + 1] - c->h_mol_start[m] != 3)
+        if (c->h_mol_start[m + 1] - c->h_mol_start[m] != 3 || c->h_mol_start[m] != 3 * m)
             return fail(c, SMD_ENOTSUP, "induced dipoles are implemented for 3-site water");
     return SMD_OK;
 }
@@ -973,7 +1159,9 @@ static CgArgs cg_args(Ctx* c, const smd_induce_params* p) {
     g.e0 = c->d_e0;
     g.polar = c->d_polar;
     g.nparts_mv = (int)(((int64_t)n * c->cg_lanes + 255) / 256);   // blocks of the A v kernel
-    g.nparts_vec = (dim + CG_THREADS - 1) / CG_THREADS;
+    g.nmol = c->nmol;
+    g.blk = p->precond == 2 ? c->d_pc_blk : nullptr;
+    g.nparts_vec = p->precond == 2 ? (c->nmol + CG_THREADS - 1) / CG_THREADS : (dim + CG_THREADS - 1) / CG_THREADS;
     g.p_vz = c->d_cg_part;
     g.p_dv = c->d_cg_part + c->cg_parts;
     g.p_rr0 = c->d_cg_part + 2 * c->cg_parts;
@@ -1045,7 +1233,10 @@ int induce_begin(Ctx* c, const smd_induce_params* p, bool frames) {
     // guess 1: keep the previous solution in d_mu
     if (mixed) hipLaunchKernelGGL(k_to_f32, grid_vec, b256, 0, c->stream, dim, g.x, reinterpret_cast<float*>(g.v32));
     solve_apply(c, p, a, g.x, g.v32, g.z, nullptr, plain);
-    if (p->precond) hipLaunchKernelGGL(k_pcg_start, grid_vec, dim3(CG_THREADS), 0, c->stream, g);
+    if (p->precond == 2) {
+        hipLaunchKernelGGL(k_block_pc, dim3((c->nmol + 255) / 256), b256, 0, c->stream, a, c->d_pc_blk, c->nmol);
+        hipLaunchKernelGGL(k_pcg_start_blk, dim3(g.nparts_vec), dim3(CG_THREADS), 0, c->stream, g);
+    } else if (p->precond) hipLaunchKernelGGL(k_pcg_start, grid_vec, dim3(CG_THREADS), 0, c->stream, g);
     else hipLaunchKernelGGL(k_cg_start, grid_vec, dim3(CG_THREADS), 0, c->stream, g);
     SMD_HIP(c, hipGetLastError());
     return SMD_OK;
@@ -1055,7 +1246,16 @@ int induce_begin(Ctx* c, const smd_induce_params* p, bool frames) {
 static void cg_iterations(Ctx* c, const smd_induce_params* p, const IndArgs& a, const CgArgs& g, int count) {
     const dim3 grid_vec(g.nparts_vec), bt(CG_THREADS);
     for (int k = 0; k < count; ++k) {
-        if (p->precond) {
+        if (p->precond == 2) {
+            solve_apply(c, p, a, g.v, g.v32, g.z, g.p_vz, CgTop{g.state, nullptr, g.nparts_vec, g.tol});
+            if (k & 1) {
+                hipLaunchKernelGGL(k_pcg_step_blk<1>, grid_vec, bt, 0, c->stream, g);
+                hipLaunchKernelGGL(k_pcg_dir_blk<1>, grid_vec, bt, 0, c->stream, g);
+            } else {
+                hipLaunchKernelGGL(k_pcg_step_blk<0>, grid_vec, bt, 0, c->stream, g);
+                hipLaunchKernelGGL(k_pcg_dir_blk<0>, grid_vec, bt, 0, c->stream, g);
+            }
+        } else if (p->precond) {
             solve_apply(c, p, a, g.v, g.v32, g.z, g.p_vz, CgTop{g.state, nullptr, g.nparts_vec, g.tol});
             if (k & 1) {
                 hipLaunchKernelGGL(k_pcg_step<1>, grid_vec, bt, 0, c->stream, g);
@@ -1229,6 +1429,7 @@ extern "C" int smd_set_multipoles(smd_ctx* c, const double* dip_local, const dou
             (rc = A(c->d_mu, sizeof(double) * 3 * n)) || (rc = A(c->d_e0_prev, sizeof(double) * 3 * n)) ||
             (rc = A(c->d_cg, sizeof(double) * 9 * n)) ||
             (rc = A(c->d_cg32, sizeof(float4) * n)) || (rc = A(c->d_ind_rec32, sizeof(float4) * 2 * n)) ||
+            (rc = A(c->d_pc_blk, sizeof(double) * 45 * (size_t)(c->nmol > 0 ? c->nmol : 1))) ||
             (rc = A(c->d_site32, sizeof(float4) * 3 * n)) ||
             (rc = A(c->d_pind32, sizeof(float4) * n)) ||
             (rc = A(c->d_cg_part, sizeof(double) * 4 * parts)) || (rc = A(c->d_cg_state, sizeof(int) * 4)) ||

@@ -466,6 +466,10 @@ struct DescArgs {
     int tstride;               // the solve's coupling table: partners inside cutoffInd in list order
     const int32_t* tcnt;
     const int32_t* tj;
+    // mixed solves: the solve's hi/lo FP32 site records (x_hi, y_hi, z_hi, 1/sqrt(alpha)),
+    // (x_lo, y_lo, z_lo, q / sqrt(Coulomb constant)); null for FP64 descriptors
+    const float4* rec32;
+    float Lh[3], Ll[3], invL[3];
     double* frame;             // [n][9] rows x, y, z of each site's frame
     const double* x;           // alpha E0 [3n]
     const double* z;           // A (alpha E0) [3n]
@@ -524,6 +528,81 @@ __device__ __forceinline__ void rot3(const double* f, const double v[3], double 
     for (int k = 0; k < 3; ++k) o[k] = f[3 * k] * v[0] + f[3 * k + 1] * v[1] + f[3 * k + 2] * v[2];
 }
 
+// Writes site i's 32 descriptors from its shell sums acc (FP64, site i's
+// sums over its partners in the lab frame).
+__device__ __forceinline__ void descriptor_row(const DescArgs& a, int i, const double (&acc)[8][3]);
+
+// The shell sums of the mixed solves' descriptors: FP32 pair math from the
+// hi/lo records (no position, charge or molecule gathers beyond the two
+// records; partners of the same 3-site molecule are j / 3 == i / 3), four
+// entries per lane in flight, FP32 sums per lane, FP64 across the lanes.
+template <int G>
+__global__ __launch_bounds__(256) void k_descriptors32(DescArgs a) {
+    const int tid = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = tid / G, sub = tid % G;
+    float acc[8][3];
+#pragma unroll
+    for (int b = 0; b < 8; ++b) acc[b][0] = acc[b][1] = acc[b][2] = 0.0f;
+    if (i < a.n) {
+        const int cnt = a.tcnt[i];
+        const float4 hi = a.rec32[2 * i], lo = a.rec32[2 * i + 1];
+        const int32_t* row = a.tj + (int64_t)i * a.tstride;
+        const int mi = i / 3;
+        auto term = [&](int j, float4 h, float4 l) {
+            const float hx = hi.x - h.x, hy = hi.y - h.y, hz = hi.z - h.z;
+            const float nx = rintf(hx * a.invL[0]), ny = rintf(hy * a.invL[1]), nz = rintf(hz * a.invL[2]);
+            const float dx = (hx - nx * a.Lh[0]) + ((lo.x - l.x) - nx * a.Ll[0]);
+            const float dy = (hy - ny * a.Lh[1]) + ((lo.y - l.y) - ny * a.Ll[1]);
+            const float dz = (hz - nz * a.Lh[2]) + ((lo.z - l.z) - nz * a.Ll[2]);
+            const float r2 = dx * dx + dy * dy + dz * dz;
+            if (j / 3 == mi || !(r2 < 0.36f)) return;
+            const int shell = r2 < 0.09f ? 0 : r2 < 0.16f ? 1 : r2 < 0.25f ? 2 : 3;
+            const float qj = l.w;
+            const int b = 2 * shell + (qj > 0.0f ? 0 : 1);
+            const float r_1 = __builtin_amdgcn_rsqf(r2);
+            const float sc = qj * r_1 * r_1 * r_1;
+#pragma unroll
+            for (int bb = 0; bb < 8; ++bb)
+                if (bb == b) {
+                    acc[bb][0] += sc * dx;
+                    acc[bb][1] += sc * dy;
+                    acc[bb][2] += sc * dz;
+                }
+        };
+        int k = sub;
+        for (; k + 3 * G < cnt; k += 4 * G) {
+            int jj[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) jj[u] = row[k + u * G];
+            float4 h[4], l[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                h[u] = a.rec32[2 * jj[u]];
+                l[u] = a.rec32[2 * jj[u] + 1];
+            }
+#pragma unroll
+            for (int u = 0; u < 4; ++u) term(jj[u], h[u], l[u]);
+        }
+        for (; k < cnt; k += G) {
+            const int j = row[k];
+            term(j, a.rec32[2 * j], a.rec32[2 * j + 1]);
+        }
+    }
+    double acc64[8][3];
+#pragma unroll
+    for (int b = 0; b < 8; ++b)
+#pragma unroll
+        for (int x = 0; x < 3; ++x) acc64[b][x] = (double)acc[b][x];
+#pragma unroll
+    for (int o = G / 2; o > 0; o >>= 1)
+#pragma unroll
+        for (int b = 0; b < 8; ++b)
+#pragma unroll
+            for (int x = 0; x < 3; ++x) acc64[b][x] += __shfl_xor(acc64[b][x], o, G);
+    if (i >= a.n || sub != 0) return;
+    descriptor_row(a, i, acc64);
+}
+
 template <int G>
 __global__ __launch_bounds__(256) void k_descriptors(DescArgs a) {
     const int tid = blockIdx.x * blockDim.x + threadIdx.x;
@@ -565,6 +644,10 @@ __global__ __launch_bounds__(256) void k_descriptors(DescArgs a) {
 #pragma unroll
             for (int k = 0; k < 3; ++k) acc[b][k] += __shfl_xor(acc[b][k], o, G);
     if (i >= a.n || sub != 0) return;
+    descriptor_row(a, i, acc);
+}
+
+__device__ __forceinline__ void descriptor_row(const DescArgs& a, int i, const double (&acc)[8][3]) {
     const double* F = a.frame + 9 * (size_t)i;
     const double al = a.polar[i];
     const double alc = al > POLMIN_LG ? al : POLMIN_LG;
@@ -824,6 +907,13 @@ static DescArgs desc_args(Ctx* c) {
     d.tj = c->d_it_j;
     d.frame = c->mlp->frame;
     d.norm = c->mlp->norm;
+    d.rec32 = c->ind_last.prec == SMD_PREC_MIXED ? c->d_ind_rec32 : nullptr;
+    for (int k = 0; k < 3; ++k) {
+        const double L = c->box.L[k];
+        d.Lh[k] = (float)L;
+        d.Ll[k] = (float)(L - (double)d.Lh[k]);
+        d.invL[k] = (float)(1.0 / L);
+    }
     return d;
 }
 
@@ -848,7 +938,8 @@ static int descriptors_run(Ctx* c, const double* x, const double* z, float* raw,
     d.xn = xn;
     hipLaunchKernelGGL(k_site_frames, dim3((c->nmol + 255) / 256), dim3(256), 0, c->stream, d);
     const unsigned grid = (unsigned)(((int64_t)c->n * DESC_LANES + 255) / 256);
-    hipLaunchKernelGGL(k_descriptors<DESC_LANES>, dim3(grid), dim3(256), 0, c->stream, d);
+    if (d.rec32) hipLaunchKernelGGL(k_descriptors32<DESC_LANES>, dim3(grid), dim3(256), 0, c->stream, d);
+    else hipLaunchKernelGGL(k_descriptors<DESC_LANES>, dim3(grid), dim3(256), 0, c->stream, d);
     SMD_HIP(c, hipGetLastError());
     return SMD_OK;
 }

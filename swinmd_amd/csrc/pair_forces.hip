@@ -80,6 +80,11 @@ template <> __device__ __forceinline__ float erfc_<float>(float x) { return erfc
 template <typename R> __device__ __forceinline__ R exp_(R x);
 template <> __device__ __forceinline__ double exp_<double>(double x) { return exp(x); }
 template <> __device__ __forceinline__ float exp_<float>(float x) { return expf(x); }
+// a / b: the IEEE quotient in FP64, a times the hardware reciprocal (1 ulp)
+// in FP32
+template <typename R> __device__ __forceinline__ R div_(R a, R b);
+template <> __device__ __forceinline__ double div_<double>(double a, double b) { return a / b; }
+template <> __device__ __forceinline__ float div_<float>(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
 template <typename R> __device__ __forceinline__ R pow_(R x, R y);
 template <> __device__ __forceinline__ double pow_<double>(double x, double y) { return pow(x, y); }
 template <> __device__ __forceinline__ float pow_<float>(float x, float y) { return powf(x, y); }
@@ -277,11 +282,11 @@ __device__ __forceinline__ void pair_term(const PairArgs& a, const Owner& o, int
                 if (eps == R(0)) {
                     live = false;   // tmpE = tmpf = 0 (BuffvdWForce.cpp:190-192)
                 } else {
-                    const R invs = R(1) / sigma;
+                    const R invs = div_<R>(R(1), sigma);
                     const R m = (R)a.m_buff, nn = (R)a.n_buff, del = (R)a.delta, gam = (R)a.gamma;
                     const R rho = r * invs;
                     const R expnm = nn - m;
-                    const R par1 = (R(1) + del) / (rho + del);
+                    const R par1 = div_<R>(R(1) + del, rho + del);
                     R rho_m, rho_m1, p1n;
                     if (a.buff_147) {
                         // the 14-7 exponents by products: rho^6, rho^7, par1^7
@@ -295,10 +300,10 @@ __device__ __forceinline__ void pair_term(const PairArgs& a, const Owner& o, int
                         rho_m1 = pow_<R>(rho, m - R(1));
                         p1n = pow_<R>(par1, expnm);
                     }
-                    const R par2 = (R(1) + gam) / (rho_m + gam);
+                    const R par2 = div_<R>(R(1) + gam, rho_m + gam);
                     tmpE = eps * p1n * (par2 - R(2));
-                    tmpf = -eps * expnm * p1n * (par2 - R(2)) * invs / (rho + del) -
-                           eps * p1n * par2 * m * rho_m1 * invs / (rho_m + gam);
+                    tmpf = div_<R>(-eps * expnm * p1n * (par2 - R(2)) * invs, rho + del) -
+                           div_<R>(eps * p1n * par2 * m * rho_m1 * invs, rho_m + gam);
                     if (a.switch_on && rij2 > (R)a.sw_dist2) {
                         // The taper's coefficients scale as 1/(rc - rsw)^5:
                         // evaluated in FP64 from the FP64 distance.
@@ -317,7 +322,7 @@ __device__ __forceinline__ void pair_term(const PairArgs& a, const Owner& o, int
                 if constexpr (VDW == V_BUCK) {
                     fx = (double)(-tmpf * rx); fy = (double)(-tmpf * ry); fz = (double)(-tmpf * rz);
                 } else {
-                    const R r_1 = R(1) / r;
+                    const R r_1 = div_<R>(R(1), r);
                     fx = (double)(-tmpf * rx * r_1); fy = (double)(-tmpf * ry * r_1); fz = (double)(-tmpf * rz * r_1);
                 }
                 fix += fx; fiy += fy; fiz += fz;

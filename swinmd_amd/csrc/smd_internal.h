@@ -195,6 +195,7 @@ struct Ctx {
     float4* d_cg32 = nullptr;      // mixed solve: FP32 copy of the CG direction [n]
     float4* d_ind_rec32 = nullptr; // mixed solve: hi/lo FP32 site records [2n]
     double* d_e0_prev = nullptr;   // E0 of the previous solve with guess 4 [3n]
+    double* d_pc_blk = nullptr;    // block preconditioner: Cholesky factors [nmol][45]
     int mp_prec = 0;               // smd_prec of the multipole / induction real-space pair math
     float4* d_site32 = nullptr;    // mixed real space: FP32 site records [3n]
     float4* d_pind32 = nullptr;    // mixed real space: FP32 induced dipoles [n]

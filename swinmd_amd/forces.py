@@ -527,15 +527,17 @@ class Induction:
 
     def __init__(self, ctx: Context, *, cutoff_ind, a_damp=0.39, damp_tij=True, do_min=True, tol=0.0,
                  max_iter=0, guess=0, prec=L.PREC_FP64, precond=False):
-        """precond=False: the reference's CG, stopping at |r|_2 < tol (0: its
-        1e-14).  precond=True: CG preconditioned by diag(alpha), stopping when
-        the RMS dipole residual |alpha_i r_i| over the sites is below tol
-        (e nm; the configs' 1e-6 D is 1e-6 * L.DEBYE_E_NM); prec MIXED takes
-        this solve with an FP32 coupling table."""
+        """precond=False (0): the reference's CG, stopping at |r|_2 < tol (0:
+        its 1e-14).  precond=True (1): CG preconditioned by diag(alpha);
+        precond=2: by the inverse of each molecule's 9x9 block of A.  Both
+        stop when the RMS dipole residual |alpha_i r_i| over the sites is
+        below tol (e nm; the configs' 1e-6 D is 1e-6 * L.DEBYE_E_NM); prec
+        MIXED takes these solves with FP32 pair math in A v."""
         self.ctx = ctx
         set_multipoles(ctx)
+        pc = int(precond)
         self.params = L.InduceParams(cutoff_ind, a_damp, int(damp_tij), int(do_min), tol, max_iter, guess, prec,
-                                     int(precond), int(precond))
+                                     pc, 1 if pc else 0)
 
     def induce(self):
         """Runs the solve; returns (iterations, converged, residual)."""

@@ -86,12 +86,15 @@ def md_real(ctx):
     return r
 
 
+@pytest.mark.parametrize("precond", [1, 2])
 @pytest.mark.parametrize("prec", [0, 1])
-def test_preconditioned_solve_reaches_the_reference_dipoles(prec):
-    """CG preconditioned by diag(alpha), stopping at an RMS dipole residual of
-    1e-6 D, FP64 or with the FP32 coupling table: the dipoles are within a
-    few times the tolerance of the reference's (|r| < 1e-14) and the solve
-    needs fewer iterations; a tight tolerance reaches them to 1e-9."""
+def test_preconditioned_solve_reaches_the_reference_dipoles(prec, precond):
+    """CG preconditioned by diag(alpha) (1) or by each molecule's 9x9 block
+    of A (2), stopping at an RMS dipole residual of 1e-6 D, FP64 or with FP32
+    A v: the dipoles are within a few times the tolerance of the reference's
+    (|r| < 1e-14) and the solve needs fewer iterations; a tight tolerance
+    reaches them to 1e-9; the block preconditioner needs fewer iterations
+    than the diagonal one."""
     from swinmd_amd import _lib as L
     from swinmd_amd.forces import Induction
     fx = Fixture("amoeba216_induction_cg")
@@ -104,13 +107,19 @@ def test_preconditioned_solve_reaches_the_reference_dipoles(prec):
     assert conv and rel_rms(mu_ref, fx.vec("ind_pi")) <= 1e-10
     tol = 1e-6 * L.DEBYE_E_NM
     ind2 = Induction(ctx, cutoff_ind=c["cutoff_ind"], a_damp=c["a_damp"], damp_tij=c["ind_damp"], do_min=True,
-                     tol=tol, precond=True, prec=prec)
+                     tol=tol, precond=precond, prec=prec)
     it, conv, res = ind2.induce()
     mu = ind2.results()[3]
     assert conv and res < tol
     err = np.sqrt(np.mean(np.sum((mu - mu_ref) ** 2, axis=1)))
     assert err <= 20 * tol, (err, tol)
     assert it < it_ref
+    if precond == 2:
+        ind1 = Induction(ctx, cutoff_ind=c["cutoff_ind"], a_damp=c["a_damp"], damp_tij=c["ind_damp"], do_min=True,
+                         tol=tol, precond=1, prec=prec)
+        it1, conv1, _ = ind1.induce()
+        print(f"prec {prec}: diagonal {it1}, block {it} iterations", flush=True)
+        assert conv1 and it < it1
     if prec == 0:
         ind2.params.tol = 1e-9 * tol
         it_t, conv, _ = ind2.induce()
@@ -124,8 +133,8 @@ def test_preconditioned_solve_reaches_the_reference_dipoles(prec):
         assert conv and np.sqrt(np.mean(np.sum((mu_g - mu_ref) ** 2, axis=1))) <= 20 * tol
 
 
-@pytest.mark.parametrize("guess", [2, 4])
-def test_polar_md_preconditioned_steps_follow_the_reference_method(guess):
+@pytest.mark.parametrize("guess,precond", [(2, 1), (4, 1), (2, 2)])
+def test_polar_md_preconditioned_steps_follow_the_reference_method(guess, precond):
     """The step with the preconditioned FP64 solve at 1e-6 D, FP64, from
     alpha E0 (guess 2) or from the previous step's dipoles moved by alpha dE0
     (guess 4), stays on the reference-method trajectory over 10 steps to the
@@ -134,10 +143,11 @@ def test_polar_md_preconditioned_steps_follow_the_reference_method(guess):
     fx = Fixture("amoeba216_md10_polar")
     ctx = _ctx(fx.system)
     ctx.set_velocities(fx.vec("init_vel"))
-    _, md = _polar_md(fx, ctx, True, tol=1e-6 * L.DEBYE_E_NM, precond=True, guess=guess)
+    _, md = _polar_md(fx, ctx, True, tol=1e-6 * L.DEBYE_E_NM, precond=precond, guess=guess)
     md.init()
     rep = md.run(int(fx.extra))
-    print(f"guess {guess}: {rep['cg_iterations'] / rep['steps']:.2f} iterations per step", flush=True)
+    print(f"guess {guess} precond {precond}: {rep['cg_iterations'] / rep['steps']:.2f} iterations per step",
+          flush=True)
     assert rel_rms(ctx.positions(), fx.vec("end_pos")) <= 1e-7
     assert rel_rms(ctx.velocities(), fx.vec("end_vel")) <= 1e-4
     assert rel(rep["e_mp_real"], fx.meta["e_real"]) <= 1e-5
