@@ -32,6 +32,7 @@ namespace smd {
 
 constexpr int IND_LANES = 16;      // lanes per atom row in the field and T.mu kernels
 constexpr int CG_THREADS = 256;    // threads per block of the vector kernels
+constexpr int BLK_THREADS = 64;    // one wave per block: the block preconditioner's vector kernels (thread per molecule)
 constexpr double SQRT_COULOMB = 11.78708976;   // SQRTCOULOMBCONSTANT, src/NEMD_defs.h:103
 constexpr double POLMIN = 0.00000001;          // Induction.cpp:195
 
@@ -64,6 +65,7 @@ struct IndArgs {
     // rec32[2i] = (x_hi, y_hi, z_hi, 1/sqrt(alpha)), rec32[2i+1] = (x_lo, y_lo, z_lo, q / sqrt(C))
     int mixed;
     float4* rec32;
+    float4* site32;             // mixed solve: (q / sqrt(C), mu), (Q xx xy xz yy), (Q yz zz, alpha, 0) per site
     float Lh[3], Ll[3], invL[3];
 };
 
@@ -385,88 +387,131 @@ __global__ __launch_bounds__(256) void k_ind_table(IndArgs a) {
 }
 
 // The mixed solve's permanent field and coupling table in one pass over the
-// rows, with FP32 pair math.  Which pairs count is decided in FP64 (r^2 <=
-// cutoffInd^2, as in the FP64 kernels), and the field sums in FP64.  The
-// table holds the partners only: the mixed A v recomputes their Thole
-// factors.  The
-// field of a pair is field_from's expression gathered by powers of r:
-// f = (q c3 + c5 (3 r.d - 3 tr Q) + 15 c7 r.Q.r) r - c3 d - 6 c5 Q r with
-// cn = lambda_n / r^n.
+// rows, with FP32 pair math from the solve's hi/lo site records and FP32
+// multipole records (rec32, site32; the displacement's first difference is
+// exact, so which pairs count matches the FP64 test except within FP32
+// rounding of cutoffInd), four entries per lane in flight, FP64 field sums.
+// The table holds the partners only: the mixed A v recomputes their Thole
+// factors.  The field of a pair is field_from's expression gathered by
+// powers of r:
+//   f = (q c3 + c5 (3 r.d - 3 tr Q) + 15 c7 r.Q.r) r - c3 d - 6 c5 Q r,
+// cn = lambda_n / r^n.  With `shells`, the learned guess's charge shells
+// (alpha-less sums q_j r / r^3 of the intermolecular partners inside 0.3,
+// 0.4, 0.5, 0.6 nm split by the sign of q_j, 24 floats per site) come out of
+// the same pass.
 template <int G>
-__global__ __launch_bounds__(256) void k_field_table32(IndArgs a) {
+__global__ __launch_bounds__(256) void k_field_table32(IndArgs a, float* __restrict__ shells) {
     static_assert(WAVE % G == 0, "a lane group must not straddle waves");
     const int tid = blockIdx.x * blockDim.x + threadIdx.x;
     const int i = tid / G, sub = tid % G;
     const int gbase = (threadIdx.x % WAVE) - sub;
     const uint64_t gmask = G == 64 ? ~0ull : ((1ull << G) - 1ull);
     double f[3] = {0.0, 0.0, 0.0};
+    float sh[8][3];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) sh[q][0] = sh[q][1] = sh[q][2] = 0.0f;
     int count = 0;
     if (i < a.n) {
         const int own = min(a.nb_own[i], a.stride);
         const int cnt = min(a.nb_cnt[i], a.stride);
         const int e0 = a.excl_off[i], e1 = a.excl_off[i + 1];
-        const double xi = a.pos[3 * i], yi = a.pos[3 * i + 1], zi = a.pos[3 * i + 2];
-        const float pi = (float)(a.polar[i] > POLMIN ? a.polar[i] : POLMIN);
+        const float4 hi = a.rec32[2 * i], lo = a.rec32[2 * i + 1];
+        const float rs_i = hi.w;
         const float damp = (float)a.a;
-        const float inv_sqc = (float)(1.0 / SQRT_COULOMB);
+        const float cut2 = (float)a.cut2;
         const int total = cnt + (e1 - e0);
-        for (int kb = 0; kb < total; kb += G) {
-            const int k = kb + sub;
-            bool ok = false;
-            int j = 0;
-            if (k < total) {
-                j = k < cnt ? row_partner(a, i, own, k) : a.excl[e0 + (k - cnt)];
-                double dx = xi - a.pos[3 * j], dy = yi - a.pos[3 * j + 1], dz = zi - a.pos[3 * j + 2];
-                pbc_near(dx, dy, dz, a.box);
-                const double rd64 = sqrt(dx * dx + dy * dy + dz * dz);
-                if (rd64 * rd64 <= a.cut2) {   // the FP64 kernels' test, bit for bit
-                    ok = true;
-                    if (k < cnt) {
-                        // the permanent field (E0 leaves the excluded partners out)
-                        const float rx = (float)dx, ry = (float)dy, rz = (float)dz, r2 = (float)(rd64 * rd64);
-                        const float r_1 = __builtin_amdgcn_rsqf(r2);
-                        const float r = r2 * r_1;
-                        const float pj = (float)(a.polar[j] > POLMIN ? a.polar[j] : POLMIN);
-                        const float x = damp * (r2 * r) * __builtin_amdgcn_rsqf(pi * pj);
-                        const float e = __expf(-x);
-                        const float r3 = r_1 * r_1 * r_1, r5 = r3 * r_1 * r_1;
-                        const float l3 = 1.0f - e, l5 = 1.0f - (1.0f + x) * e;
-                        const float l7 = 1.0f - (1.0f + x + 0.6f * x * x) * e;
-                        const float c7 = l7 * r5 * r_1 * r_1;
-                        const float c3f = l3 * r3, c5f = l5 * r5;
-                        const float qj = (float)a.q[j] * inv_sqc;
-                        const double* dj = a.dip + 3 * j;
-                        const double* Q = a.quad + 6 * j;
-                        const float djx = (float)dj[0], djy = (float)dj[1], djz = (float)dj[2];
-                        const float Qxx = (float)Q[0], Qxy = (float)Q[1], Qxz = (float)Q[2];
-                        const float Qyy = (float)Q[3], Qyz = (float)Q[4], Qzz = (float)Q[5];
-                        const float QRx = Qxx * rx + Qxy * ry + Qxz * rz;
-                        const float QRy = Qxy * rx + Qyy * ry + Qyz * rz;
-                        const float QRz = Qxz * rx + Qyz * ry + Qzz * rz;
-                        const float QRR = rx * QRx + ry * QRy + rz * QRz;
-                        const float rd = rx * djx + ry * djy + rz * djz;
-                        const float s = qj * c3f + c5f * (3.0f * rd - 3.0f * (Qxx + Qyy + Qzz)) + 15.0f * c7 * QRR;
-                        f[0] += (double)(s * rx - c3f * djx - 6.0f * c5f * QRx);
-                        f[1] += (double)(s * ry - c3f * djy - 6.0f * c5f * QRy);
-                        f[2] += (double)(s * rz - c3f * djz - 6.0f * c5f * QRz);
+        for (int kb = 0; kb < total; kb += 4 * G) {
+            int jj[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const int k = kb + u * G + sub;
+                jj[u] = k < total ? (k < cnt ? row_partner(a, i, own, k) : a.excl[e0 + (k - cnt)]) : i;
+            }
+            float4 h[4], l[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                h[u] = a.rec32[2 * jj[u]];
+                l[u] = a.rec32[2 * jj[u] + 1];
+            }
+            bool ok[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const int k = kb + u * G + sub;
+                const int j = jj[u];
+                const float hx = hi.x - h[u].x, hy = hi.y - h[u].y, hz = hi.z - h[u].z;
+                const float nx = rintf(hx * a.invL[0]), ny = rintf(hy * a.invL[1]), nz = rintf(hz * a.invL[2]);
+                const float rx = (hx - nx * a.Lh[0]) + ((lo.x - l[u].x) - nx * a.Ll[0]);
+                const float ry = (hy - ny * a.Lh[1]) + ((lo.y - l[u].y) - ny * a.Ll[1]);
+                const float rz = (hz - nz * a.Lh[2]) + ((lo.z - l[u].z) - nz * a.Ll[2]);
+                const float r2 = rx * rx + ry * ry + rz * rz;
+                ok[u] = k < total && r2 <= cut2;
+                if (ok[u] && k < cnt) {
+                    // the permanent field (E0 leaves the excluded partners out)
+                    const float4 s0 = a.site32[3 * j], s1 = a.site32[3 * j + 1], s2 = a.site32[3 * j + 2];
+                    const float r_1 = __builtin_amdgcn_rsqf(r2);
+                    const float x = damp * (r2 * r2 * r_1) * (rs_i * h[u].w);
+                    const float e = __expf(-x);
+                    const float r3 = r_1 * r_1 * r_1, r5 = r3 * r_1 * r_1;
+                    const float l3 = 1.0f - e, l5 = 1.0f - (1.0f + x) * e;
+                    const float l7 = 1.0f - (1.0f + x + 0.6f * x * x) * e;
+                    const float c7 = l7 * r5 * r_1 * r_1;
+                    const float c3f = l3 * r3, c5f = l5 * r5;
+                    const float qj = s0.x, djx = s0.y, djy = s0.z, djz = s0.w;
+                    const float Qxx = s1.x, Qxy = s1.y, Qxz = s1.z, Qyy = s1.w, Qyz = s2.x, Qzz = s2.y;
+                    const float QRx = Qxx * rx + Qxy * ry + Qxz * rz;
+                    const float QRy = Qxy * rx + Qyy * ry + Qyz * rz;
+                    const float QRz = Qxz * rx + Qyz * ry + Qzz * rz;
+                    const float QRR = rx * QRx + ry * QRy + rz * QRz;
+                    const float rd = rx * djx + ry * djy + rz * djz;
+                    const float sc = qj * c3f + c5f * (3.0f * rd - 3.0f * (Qxx + Qyy + Qzz)) + 15.0f * c7 * QRR;
+                    f[0] += (double)(sc * rx - c3f * djx - 6.0f * c5f * QRx);
+                    f[1] += (double)(sc * ry - c3f * djy - 6.0f * c5f * QRy);
+                    f[2] += (double)(sc * rz - c3f * djz - 6.0f * c5f * QRz);
+                    if (shells && r2 < 0.36f) {
+                        const int b = 2 * (r2 < 0.09f ? 0 : r2 < 0.16f ? 1 : r2 < 0.25f ? 2 : 3) + (qj > 0.0f ? 0 : 1);
+                        const float w = qj * r3;
+#pragma unroll
+                        for (int q = 0; q < 8; ++q)
+                            if (q == b) {
+                                sh[q][0] += w * rx;
+                                sh[q][1] += w * ry;
+                                sh[q][2] += w * rz;
+                            }
                     }
                 }
             }
-            const uint64_t m = (__ballot(ok) >> gbase) & gmask;
-            if (ok) {
-                const int at = count + __popcll(m & ((1ull << sub) - 1ull));
-                a.tj[(int64_t)i * a.tstride + at] = j;
+            // ordered compaction: entry k = kb + u G + sub, u outermost
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const uint64_t m = (__ballot(ok[u]) >> gbase) & gmask;
+                if (ok[u]) {
+                    const int at = count + __popcll(m & ((1ull << sub) - 1ull));
+                    a.tj[(int64_t)i * a.tstride + at] = jj[u];
+                }
+                count += __popcll(m);
             }
-            count += __popcll(m);
         }
     }
 #pragma unroll
     for (int o = G / 2; o > 0; o >>= 1)
 #pragma unroll
         for (int x = 0; x < 3; ++x) f[x] += __shfl_xor(f[x], o, G);
+    if (shells) {
+#pragma unroll
+        for (int o = G / 2; o > 0; o >>= 1)
+#pragma unroll
+            for (int q = 0; q < 8; ++q)
+#pragma unroll
+                for (int x = 0; x < 3; ++x) sh[q][x] += __shfl_xor(sh[q][x], o, G);
+    }
     if (i < a.n && sub == 0) {
         a.tcnt[i] = count;
         for (int x = 0; x < 3; ++x) a.e0[3 * i + x] = f[x];
+        if (shells)
+#pragma unroll
+            for (int q = 0; q < 8; ++q)
+#pragma unroll
+                for (int x = 0; x < 3; ++x) shells[24 * (size_t)i + 3 * q + x] = sh[q][x];
     }
 }
 
@@ -560,6 +605,11 @@ __global__ __launch_bounds__(256) void k_pack_ind32(IndArgs a) {
     a.rec32[2 * i] = make_float4(xh, yh, zh, (float)(1.0 / sqrt(p)));
     a.rec32[2 * i + 1] = make_float4((float)(x - (double)xh), (float)(y - (double)yh), (float)(z - (double)zh),
                                      (float)(a.q[i] / SQRT_COULOMB));
+    const double* d = a.dip + 3 * i;
+    const double* Q = a.quad + 6 * i;
+    a.site32[3 * i] = make_float4((float)(a.q[i] / SQRT_COULOMB), (float)d[0], (float)d[1], (float)d[2]);
+    a.site32[3 * i + 1] = make_float4((float)Q[0], (float)Q[1], (float)Q[2], (float)Q[3]);
+    a.site32[3 * i + 2] = make_float4((float)Q[4], (float)Q[5], (float)a.polar[i], 0.0f);
 }
 
 // A v of the mixed solve over the coupling table's partners: each pair's
@@ -836,6 +886,27 @@ __global__ __launch_bounds__(CG_THREADS) void k_pcg_dir(CgArgs g) {
     }
 }
 
+// Two or three totals in block_total's order by one wave (the kernels of the
+// block preconditioner run one wave per block): the buffers' loads
+// interleave, each buffer's additions keep the order k, k + WAVE, ...
+template <int NB>
+__device__ __forceinline__ void wave_totals(const double* p0, int n0, const double* p1, int n1, const double* p2,
+                                            int n2, double (&out)[NB]) {
+    static_assert(NB >= 2 && NB <= 3, "two or three totals");
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+    const int nmax = max(n0, max(n1, NB == 3 ? n2 : 0));
+    for (int k = threadIdx.x % WAVE; k < nmax; k += WAVE) {
+        const double a0 = k < n0 ? p0[k] : 0.0, a1 = k < n1 ? p1[k] : 0.0;
+        const double a2 = (NB == 3 && k < n2) ? p2[k] : 0.0;
+        if (k < n0) s0 += a0;
+        if (k < n1) s1 += a1;
+        if (NB == 3 && k < n2) s2 += a2;
+    }
+    out[0] = wave_sum(s0);
+    out[1] = wave_sum(s1);
+    if constexpr (NB == 3) out[2] = wave_sum(s2);
+}
+
 // ---- block-Jacobi preconditioner ---------------------------------------------------
 // M^-1 = the inverse of each molecule's 9x9 diagonal block of A (its three
 // sites' 1/alpha and the intramolecular Thole tensors, which A couples),
@@ -927,7 +998,7 @@ __device__ __forceinline__ void block_solve(const double* __restrict__ Lm, const
 __device__ __forceinline__ double mol_alpha(const CgArgs& g, int m, int c) { return site_alpha(g.polar, 9 * m + c); }
 
 // After z = A x: r = E0 - A x, p = M^-1 r; partials of r.M^-1 r and |alpha r|^2.
-__global__ __launch_bounds__(CG_THREADS) void k_pcg_start_blk(CgArgs g) {
+__global__ __launch_bounds__(BLK_THREADS) void k_pcg_start_blk(CgArgs g) {
     const int m = blockIdx.x * blockDim.x + threadIdx.x;
     double rz = 0.0, zz = 0.0;
     if (m < g.nmol) {
@@ -958,12 +1029,12 @@ __global__ __launch_bounds__(CG_THREADS) void k_pcg_start_blk(CgArgs g) {
 // t = rz / p.Ap; x += t p; r -= t Ap; z = M^-1 r (into the A v buffer, whose
 // product is consumed); partials of the new r.z and |alpha r|^2.
 template <int PHASE>
-__global__ __launch_bounds__(CG_THREADS) void k_pcg_step_blk(CgArgs g) {
+__global__ __launch_bounds__(BLK_THREADS) void k_pcg_step_blk(CgArgs g) {
     if (g.state[0]) return;
     const double* rz_c = PHASE ? g.p_rr1 : g.p_rr0;
     double* rz_d = PHASE ? g.p_rr0 : g.p_rr1;
     double tot[2];   // p.Ap, r.z
-    block_totals<2>(g.p_vz, g.nparts_mv, rz_c, g.nparts_vec, nullptr, 0, tot);
+    wave_totals<2>(g.p_vz, g.nparts_mv, rz_c, g.nparts_vec, nullptr, 0, tot);
     const double t = tot[1] / tot[0];
     const int m = blockIdx.x * blockDim.x + threadIdx.x;
     double rzn = 0.0, zz = 0.0;
@@ -993,12 +1064,12 @@ __global__ __launch_bounds__(CG_THREADS) void k_pcg_step_blk(CgArgs g) {
 
 // Stop when the RMS of alpha r is below tol; else p = z + beta p.
 template <int PHASE>
-__global__ __launch_bounds__(CG_THREADS) void k_pcg_dir_blk(CgArgs g) {
+__global__ __launch_bounds__(BLK_THREADS) void k_pcg_dir_blk(CgArgs g) {
     if (g.state[0]) return;
     const double* rz_c = PHASE ? g.p_rr1 : g.p_rr0;
     const double* rz_d = PHASE ? g.p_rr0 : g.p_rr1;
     double tot[3];   // |alpha r|^2, new r.z, old r.z
-    block_totals<3>(g.p_dv, g.nparts_vec, rz_d, g.nparts_vec, rz_c, g.nparts_vec, tot);
+    wave_totals<3>(g.p_dv, g.nparts_vec, rz_d, g.nparts_vec, rz_c, g.nparts_vec, tot);
     const double res = sqrt(tot[0] * g.inv_npol);
     if (blockIdx.x == 0 && threadIdx.x == 0) g.scal[0] = res;
     if (res < g.tol) {
@@ -1076,6 +1147,7 @@ static IndArgs ind_args(Ctx* c, const smd_induce_params* p) {
     a.tab = reinterpret_cast<double2*>(c->d_it_a);
     a.mixed = p->prec == SMD_PREC_MIXED ? 1 : 0;
     a.rec32 = c->d_ind_rec32;
+    a.site32 = c->d_site32;
     for (int d = 0; d < 3; ++d) {
         const double L = c->box.L[d];
         a.Lh[d] = (float)L;
@@ -1161,7 +1233,7 @@ static CgArgs cg_args(Ctx* c, const smd_induce_params* p) {
     g.nparts_mv = (int)(((int64_t)n * c->cg_lanes + 255) / 256);   // blocks of the A v kernel
     g.nmol = c->nmol;
     g.blk = p->precond == 2 ? c->d_pc_blk : nullptr;
-    g.nparts_vec = p->precond == 2 ? (c->nmol + CG_THREADS - 1) / CG_THREADS : (dim + CG_THREADS - 1) / CG_THREADS;
+    g.nparts_vec = p->precond == 2 ? (c->nmol + BLK_THREADS - 1) / BLK_THREADS : (dim + CG_THREADS - 1) / CG_THREADS;
     g.p_vz = c->d_cg_part;
     g.p_dv = c->d_cg_part + c->cg_parts;
     g.p_rr0 = c->d_cg_part + 2 * c->cg_parts;
@@ -1188,6 +1260,7 @@ int induce_begin(Ctx* c, const smd_induce_params* p, bool frames) {
     int rc = induce_check(c, p);
     if (rc) return rc;
     c->ind_last_converged = false;
+    c->ind_shells_valid = false;
     if (p->do_min && (rc = table_alloc(c))) return rc;
     if (p->do_min) c->ind_table_built = true;
     c->ind_last = *p;   // the solve in progress (the learned guess reads its parameters)
@@ -1201,7 +1274,10 @@ int induce_begin(Ctx* c, const smd_induce_params* p, bool frames) {
         if (frames) hipLaunchKernelGGL(k_frames, dim3((c->nmol + 255) / 256), b256, 0, c->stream, a);
         if (p->do_min && a.mixed) {
             hipLaunchKernelGGL(k_pack_ind32, dim3((n + 255) / 256), b256, 0, c->stream, a);
-            hipLaunchKernelGGL((k_field_table32<IND_LANES>), grid_rows, b256, 0, c->stream, a);
+            // the learned guess's charge shells come out of the same pass
+            float* shells = p->guess == 3 ? c->d_shell32 : nullptr;
+            hipLaunchKernelGGL((k_field_table32<IND_LANES>), grid_rows, b256, 0, c->stream, a, shells);
+            c->ind_shells_valid = shells != nullptr;
         } else {
             hipLaunchKernelGGL((k_field<IND_LANES>), grid_rows, b256, 0, c->stream, a);
             if (p->do_min) hipLaunchKernelGGL((k_ind_table<IND_LANES>), grid_rows, b256, 0, c->stream, a);
@@ -1235,7 +1311,7 @@ int induce_begin(Ctx* c, const smd_induce_params* p, bool frames) {
     solve_apply(c, p, a, g.x, g.v32, g.z, nullptr, plain);
     if (p->precond == 2) {
         hipLaunchKernelGGL(k_block_pc, dim3((c->nmol + 255) / 256), b256, 0, c->stream, a, c->d_pc_blk, c->nmol);
-        hipLaunchKernelGGL(k_pcg_start_blk, dim3(g.nparts_vec), dim3(CG_THREADS), 0, c->stream, g);
+        hipLaunchKernelGGL(k_pcg_start_blk, dim3(g.nparts_vec), dim3(BLK_THREADS), 0, c->stream, g);
     } else if (p->precond) hipLaunchKernelGGL(k_pcg_start, grid_vec, dim3(CG_THREADS), 0, c->stream, g);
     else hipLaunchKernelGGL(k_cg_start, grid_vec, dim3(CG_THREADS), 0, c->stream, g);
     SMD_HIP(c, hipGetLastError());
@@ -1248,12 +1324,13 @@ static void cg_iterations(Ctx* c, const smd_induce_params* p, const IndArgs& a, 
     for (int k = 0; k < count; ++k) {
         if (p->precond == 2) {
             solve_apply(c, p, a, g.v, g.v32, g.z, g.p_vz, CgTop{g.state, nullptr, g.nparts_vec, g.tol});
+            const dim3 bw(BLK_THREADS);
             if (k & 1) {
-                hipLaunchKernelGGL(k_pcg_step_blk<1>, grid_vec, bt, 0, c->stream, g);
-                hipLaunchKernelGGL(k_pcg_dir_blk<1>, grid_vec, bt, 0, c->stream, g);
+                hipLaunchKernelGGL(k_pcg_step_blk<1>, grid_vec, bw, 0, c->stream, g);
+                hipLaunchKernelGGL(k_pcg_dir_blk<1>, grid_vec, bw, 0, c->stream, g);
             } else {
-                hipLaunchKernelGGL(k_pcg_step_blk<0>, grid_vec, bt, 0, c->stream, g);
-                hipLaunchKernelGGL(k_pcg_dir_blk<0>, grid_vec, bt, 0, c->stream, g);
+                hipLaunchKernelGGL(k_pcg_step_blk<0>, grid_vec, bw, 0, c->stream, g);
+                hipLaunchKernelGGL(k_pcg_dir_blk<0>, grid_vec, bw, 0, c->stream, g);
             }
         } else if (p->precond) {
             solve_apply(c, p, a, g.v, g.v32, g.z, g.p_vz, CgTop{g.state, nullptr, g.nparts_vec, g.tol});
@@ -1430,6 +1507,7 @@ extern "C" int smd_set_multipoles(smd_ctx* c, const double* dip_local, const dou
             (rc = A(c->d_cg, sizeof(double) * 9 * n)) ||
             (rc = A(c->d_cg32, sizeof(float4) * n)) || (rc = A(c->d_ind_rec32, sizeof(float4) * 2 * n)) ||
             (rc = A(c->d_pc_blk, sizeof(double) * 45 * (size_t)(c->nmol > 0 ? c->nmol : 1))) ||
+            (rc = A(c->d_shell32, sizeof(float) * 24 * n)) ||
             (rc = A(c->d_site32, sizeof(float4) * 3 * n)) ||
             (rc = A(c->d_pind32, sizeof(float4) * n)) ||
             (rc = A(c->d_cg_part, sizeof(double) * 4 * parts)) || (rc = A(c->d_cg_state, sizeof(int) * 4)) ||

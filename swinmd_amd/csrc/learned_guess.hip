@@ -470,6 +470,7 @@ struct DescArgs {
     // (x_lo, y_lo, z_lo, q / sqrt(Coulomb constant)); null for FP64 descriptors
     const float4* rec32;
     float Lh[3], Ll[3], invL[3];
+    const float* shells;       // the charge shells from the solve's mixed field pass [n][24], else null
     double* frame;             // [n][9] rows x, y, z of each site's frame
     const double* x;           // alpha E0 [3n]
     const double* z;           // A (alpha E0) [3n]
@@ -601,6 +602,18 @@ __global__ __launch_bounds__(256) void k_descriptors32(DescArgs a) {
             for (int x = 0; x < 3; ++x) acc64[b][x] += __shfl_xor(acc64[b][x], o, G);
     if (i >= a.n || sub != 0) return;
     descriptor_row(a, i, acc64);
+}
+
+// Descriptors from the shells the mixed field pass already summed: thread per site.
+__global__ __launch_bounds__(256) void k_descriptors_shells(DescArgs a) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= a.n) return;
+    double acc[8][3];
+#pragma unroll
+    for (int b = 0; b < 8; ++b)
+#pragma unroll
+        for (int x = 0; x < 3; ++x) acc[b][x] = (double)a.shells[24 * (size_t)i + 3 * b + x];
+    descriptor_row(a, i, acc);
 }
 
 template <int G>
@@ -908,6 +921,7 @@ static DescArgs desc_args(Ctx* c) {
     d.frame = c->mlp->frame;
     d.norm = c->mlp->norm;
     d.rec32 = c->ind_last.prec == SMD_PREC_MIXED ? c->d_ind_rec32 : nullptr;
+    d.shells = c->ind_shells_valid ? c->d_shell32 : nullptr;
     for (int k = 0; k < 3; ++k) {
         const double L = c->box.L[k];
         d.Lh[k] = (float)L;
@@ -938,7 +952,8 @@ static int descriptors_run(Ctx* c, const double* x, const double* z, float* raw,
     d.xn = xn;
     hipLaunchKernelGGL(k_site_frames, dim3((c->nmol + 255) / 256), dim3(256), 0, c->stream, d);
     const unsigned grid = (unsigned)(((int64_t)c->n * DESC_LANES + 255) / 256);
-    if (d.rec32) hipLaunchKernelGGL(k_descriptors32<DESC_LANES>, dim3(grid), dim3(256), 0, c->stream, d);
+    if (d.shells) hipLaunchKernelGGL(k_descriptors_shells, dim3((c->n + 255) / 256), dim3(256), 0, c->stream, d);
+    else if (d.rec32) hipLaunchKernelGGL(k_descriptors32<DESC_LANES>, dim3(grid), dim3(256), 0, c->stream, d);
     else hipLaunchKernelGGL(k_descriptors<DESC_LANES>, dim3(grid), dim3(256), 0, c->stream, d);
     SMD_HIP(c, hipGetLastError());
     return SMD_OK;

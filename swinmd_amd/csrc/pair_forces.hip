@@ -110,6 +110,116 @@ struct Owner {
 // One entry of row i.  FULL: a pair of the reference's half list in its
 // owner's row -- energy, volume derivatives, virial and force on i.
 // Otherwise: the same pair seen from its other atom -- force on i only.
+// Buckingham and buffered 14-7 on the virtual sites: one pair given the
+// partner's virtual-site position and type (loaded by the caller, so a row
+// walk can fetch them ahead).
+template <int VDW, bool LUSTIG, typename R, bool FULL>
+__device__ __forceinline__ void vsite_term(const PairArgs& a, const Owner& o, double ejx, double ejy, double ejz, int tj,
+                                           double qj, double (&acc)[A_N], double& fix, double& fiy, double& fiz) {
+        // rij = ri - rj on the virtual sites (BuckinghamForce.cpp:156,
+        // BuffvdWForce.cpp:172)
+        double dx = o.ex - ejx, dy = o.ey - ejy, dz = o.ez - ejz;
+        pbc_near(dx, dy, dz, a.box);
+        const R rx = (R)dx, ry = (R)dy, rz = (R)dz;
+        const R r = sqrt(rx * rx + ry * ry + rz * rz);
+        const R rij2 = r * r;
+        if (rij2 <= (R)a.cut2) {
+            const int t = o.ti * a.ntypes + tj;
+            R tmpE = 0, tmpf = 0, dulj = 0, d2ulj = 0;
+            bool live = true;
+            if constexpr (VDW == V_BUCK) {
+                const R Aa = (R)a.ta[t], Bb = (R)a.tb[t], Cc = (R)a.tc[t];
+                const R r_1 = R(1) / r;
+                const R rij_2 = r_1 * r_1;
+                const R rij6 = rij_2 * rij_2 * rij_2;
+                const R ex = exp_<R>(-Cc * r);
+                tmpE = Bb * ex - Aa * rij6;
+                tmpf = -Bb * Cc * r_1 * ex + R(6) * Aa * rij6 * rij_2;
+                if constexpr (FULL && LUSTIG) {
+                    dulj = -Bb * Cc * r * ex + R(6) * Aa * rij6;
+                    d2ulj = (R(2) * r + Cc * rij2) * Bb * Cc * ex - R(54) * Aa * rij6;
+                }
+                if (a.switch_on && rij2 > (R)a.sw_dist2) {
+                    double sv, dsv;
+                    charmm_switch(a, dx, dy, dz, sv, dsv);
+                    tmpf = (R)((double)tmpf * sv - (double)tmpE * dsv);
+                    tmpE = (R)((double)tmpE * sv);
+                }
+                if (a.coulomb) {
+                    const R qij = (R)(o.qi * qj);
+                    const R e = qij * r_1;
+                    const R f = e * rij_2;
+                    const R r_rc = r / (R)a.cutoff;
+                    const R r4 = rij2 * rij2 * (R)a.rcut_2 * (R)a.rcut_2;
+                    if constexpr (FULL)
+                        acc[A_VELEC] += (double)(e * (R(1) - R(1.666667) * r_rc + R(1.666667) * r4 - r_rc * r4));
+                    tmpf += f * (R(1) - R(5) * r4 + R(4) * r_rc * r4);
+                }
+            } else {
+                const R sigma = (R)a.ta[t], eps = (R)a.tb[t];
+                if (eps == R(0)) {
+                    live = false;   // tmpE = tmpf = 0 (BuffvdWForce.cpp:190-192)
+                } else {
+                    const R invs = div_<R>(R(1), sigma);
+                    const R m = (R)a.m_buff, nn = (R)a.n_buff, del = (R)a.delta, gam = (R)a.gamma;
+                    const R rho = r * invs;
+                    const R expnm = nn - m;
+                    const R par1 = div_<R>(R(1) + del, rho + del);
+                    R rho_m, rho_m1, p1n;
+                    if (a.buff_147) {
+                        // the 14-7 exponents by products: rho^6, rho^7, par1^7
+                        const R rho2 = rho * rho, rho4 = rho2 * rho2;
+                        rho_m1 = rho2 * rho4;
+                        rho_m = rho_m1 * rho;
+                        const R q2 = par1 * par1, q4 = q2 * q2;
+                        p1n = q4 * q2 * par1;
+                    } else {
+                        rho_m = pow_<R>(rho, m);
+                        rho_m1 = pow_<R>(rho, m - R(1));
+                        p1n = pow_<R>(par1, expnm);
+                    }
+                    const R par2 = div_<R>(R(1) + gam, rho_m + gam);
+                    tmpE = eps * p1n * (par2 - R(2));
+                    tmpf = div_<R>(-eps * expnm * p1n * (par2 - R(2)) * invs, rho + del) -
+                           div_<R>(eps * p1n * par2 * m * rho_m1 * invs, rho_m + gam);
+                    if (a.switch_on && rij2 > (R)a.sw_dist2) {
+                        // The taper's coefficients scale as 1/(rc - rsw)^5:
+                        // evaluated in FP64 from the FP64 distance.
+                        const double rd = sqrt(dx * dx + dy * dy + dz * dz);
+                        const double r2d = rd * rd, r3 = r2d * rd, r4 = r2d * r2d, r5 = r2d * r3;
+                        const double sv = a.s5 * r5 + a.s4 * r4 + a.s3 * r3 + a.s2 * r2d + a.s1 * rd + a.s0;
+                        const double dsv = 5.0 * a.s5 * r4 + 4.0 * a.s4 * r3 + 3.0 * a.s3 * r2d + 2.0 * a.s2 * rd + a.s1;
+                        tmpf = (R)((double)tmpf * sv + (double)tmpE * dsv);
+                        tmpE = (R)((double)tmpE * sv);
+                    }
+                    dulj = tmpf * r;   // always accumulated (BuffvdWForce.cpp:242-244)
+                }
+            }
+            if (live) {
+                double fx, fy, fz;
+                if constexpr (VDW == V_BUCK) {
+                    fx = (double)(-tmpf * rx); fy = (double)(-tmpf * ry); fz = (double)(-tmpf * rz);
+                } else {
+                    const R r_1 = div_<R>(R(1), r);
+                    fx = (double)(-tmpf * rx * r_1); fy = (double)(-tmpf * ry * r_1); fz = (double)(-tmpf * rz * r_1);
+                }
+                fix += fx; fiy += fy; fiz += fz;
+                if constexpr (FULL) {
+                    acc[A_VE] += (double)tmpE;
+                    acc[A_VDUDV] += (double)dulj;
+                    if constexpr (LUSTIG) acc[A_VD2UDV] += (double)d2ulj;
+                    // virial += -fij * rij for the buffered 14-7 (BuffvdWForce.cpp:276-284)
+                    const double sg = VDW == V_BUCK ? 1.0 : -1.0;
+                    const double vx = sg * fx, vy = sg * fy, vz = sg * fz;
+                    double* v = acc + A_VVIR;
+                    v[0] += vx * dx; v[1] += vy * dx; v[2] += vz * dx;
+                    v[3] += vx * dy; v[4] += vy * dy; v[5] += vz * dy;
+                    v[6] += vx * dz; v[7] += vy * dz; v[8] += vz * dz;
+                }
+            }
+        }
+}
+
 template <int VDW, bool EWALD, bool LUSTIG, typename R, bool FULL>
 __device__ __forceinline__ void pair_term(const PairArgs& a, const Owner& o, int j, double (&acc)[A_N], double& fix,
                                           double& fiy, double& fiz) {
@@ -238,108 +348,31 @@ __device__ __forceinline__ void pair_term(const PairArgs& a, const Owner& o, int
         }
         fix += (double)(s * rx); fiy += (double)(s * ry); fiz += (double)(s * rz);
     } else if constexpr (VDW == V_BUCK || VDW == V_BUFF) {
-        // rij = ri - rj on the virtual sites (BuckinghamForce.cpp:156,
-        // BuffvdWForce.cpp:172)
-        double dx = o.ex - a.epos[3 * j], dy = o.ey - a.epos[3 * j + 1], dz = o.ez - a.epos[3 * j + 2];
-        pbc_near(dx, dy, dz, a.box);
-        const R rx = (R)dx, ry = (R)dy, rz = (R)dz;
-        const R r = sqrt(rx * rx + ry * ry + rz * rz);
-        const R rij2 = r * r;
-        if (rij2 <= (R)a.cut2) {
-            const int t = o.ti * a.ntypes + a.type[j];
-            R tmpE = 0, tmpf = 0, dulj = 0, d2ulj = 0;
-            bool live = true;
-            if constexpr (VDW == V_BUCK) {
-                const R Aa = (R)a.ta[t], Bb = (R)a.tb[t], Cc = (R)a.tc[t];
-                const R r_1 = R(1) / r;
-                const R rij_2 = r_1 * r_1;
-                const R rij6 = rij_2 * rij_2 * rij_2;
-                const R ex = exp_<R>(-Cc * r);
-                tmpE = Bb * ex - Aa * rij6;
-                tmpf = -Bb * Cc * r_1 * ex + R(6) * Aa * rij6 * rij_2;
-                if constexpr (FULL && LUSTIG) {
-                    dulj = -Bb * Cc * r * ex + R(6) * Aa * rij6;
-                    d2ulj = (R(2) * r + Cc * rij2) * Bb * Cc * ex - R(54) * Aa * rij6;
-                }
-                if (a.switch_on && rij2 > (R)a.sw_dist2) {
-                    double sv, dsv;
-                    charmm_switch(a, dx, dy, dz, sv, dsv);
-                    tmpf = (R)((double)tmpf * sv - (double)tmpE * dsv);
-                    tmpE = (R)((double)tmpE * sv);
-                }
-                if (a.coulomb) {
-                    const R qij = (R)(o.qi * a.q[j]);
-                    const R e = qij * r_1;
-                    const R f = e * rij_2;
-                    const R r_rc = r / (R)a.cutoff;
-                    const R r4 = rij2 * rij2 * (R)a.rcut_2 * (R)a.rcut_2;
-                    if constexpr (FULL)
-                        acc[A_VELEC] += (double)(e * (R(1) - R(1.666667) * r_rc + R(1.666667) * r4 - r_rc * r4));
-                    tmpf += f * (R(1) - R(5) * r4 + R(4) * r_rc * r4);
-                }
-            } else {
-                const R sigma = (R)a.ta[t], eps = (R)a.tb[t];
-                if (eps == R(0)) {
-                    live = false;   // tmpE = tmpf = 0 (BuffvdWForce.cpp:190-192)
-                } else {
-                    const R invs = div_<R>(R(1), sigma);
-                    const R m = (R)a.m_buff, nn = (R)a.n_buff, del = (R)a.delta, gam = (R)a.gamma;
-                    const R rho = r * invs;
-                    const R expnm = nn - m;
-                    const R par1 = div_<R>(R(1) + del, rho + del);
-                    R rho_m, rho_m1, p1n;
-                    if (a.buff_147) {
-                        // the 14-7 exponents by products: rho^6, rho^7, par1^7
-                        const R rho2 = rho * rho, rho4 = rho2 * rho2;
-                        rho_m1 = rho2 * rho4;
-                        rho_m = rho_m1 * rho;
-                        const R q2 = par1 * par1, q4 = q2 * q2;
-                        p1n = q4 * q2 * par1;
-                    } else {
-                        rho_m = pow_<R>(rho, m);
-                        rho_m1 = pow_<R>(rho, m - R(1));
-                        p1n = pow_<R>(par1, expnm);
-                    }
-                    const R par2 = div_<R>(R(1) + gam, rho_m + gam);
-                    tmpE = eps * p1n * (par2 - R(2));
-                    tmpf = div_<R>(-eps * expnm * p1n * (par2 - R(2)) * invs, rho + del) -
-                           div_<R>(eps * p1n * par2 * m * rho_m1 * invs, rho_m + gam);
-                    if (a.switch_on && rij2 > (R)a.sw_dist2) {
-                        // The taper's coefficients scale as 1/(rc - rsw)^5:
-                        // evaluated in FP64 from the FP64 distance.
-                        const double rd = sqrt(dx * dx + dy * dy + dz * dz);
-                        const double r2d = rd * rd, r3 = r2d * rd, r4 = r2d * r2d, r5 = r2d * r3;
-                        const double sv = a.s5 * r5 + a.s4 * r4 + a.s3 * r3 + a.s2 * r2d + a.s1 * rd + a.s0;
-                        const double dsv = 5.0 * a.s5 * r4 + 4.0 * a.s4 * r3 + 3.0 * a.s3 * r2d + 2.0 * a.s2 * rd + a.s1;
-                        tmpf = (R)((double)tmpf * sv + (double)tmpE * dsv);
-                        tmpE = (R)((double)tmpE * sv);
-                    }
-                    dulj = tmpf * r;   // always accumulated (BuffvdWForce.cpp:242-244)
-                }
-            }
-            if (live) {
-                double fx, fy, fz;
-                if constexpr (VDW == V_BUCK) {
-                    fx = (double)(-tmpf * rx); fy = (double)(-tmpf * ry); fz = (double)(-tmpf * rz);
-                } else {
-                    const R r_1 = div_<R>(R(1), r);
-                    fx = (double)(-tmpf * rx * r_1); fy = (double)(-tmpf * ry * r_1); fz = (double)(-tmpf * rz * r_1);
-                }
-                fix += fx; fiy += fy; fiz += fz;
-                if constexpr (FULL) {
-                    acc[A_VE] += (double)tmpE;
-                    acc[A_VDUDV] += (double)dulj;
-                    if constexpr (LUSTIG) acc[A_VD2UDV] += (double)d2ulj;
-                    // virial += -fij * rij for the buffered 14-7 (BuffvdWForce.cpp:276-284)
-                    const double sg = VDW == V_BUCK ? 1.0 : -1.0;
-                    const double vx = sg * fx, vy = sg * fy, vz = sg * fz;
-                    double* v = acc + A_VVIR;
-                    v[0] += vx * dx; v[1] += vy * dx; v[2] += vz * dx;
-                    v[3] += vx * dy; v[4] += vy * dy; v[5] += vz * dy;
-                    v[6] += vx * dz; v[7] += vy * dz; v[8] += vz * dz;
-                }
-            }
-        }
+        vsite_term<VDW, LUSTIG, R, FULL>(a, o, a.epos[3 * j], a.epos[3 * j + 1], a.epos[3 * j + 2], a.type[j],
+                                         VDW == V_BUCK ? a.q[j] : 0.0, acc, fix, fiy, fiz);
+    }
+}
+
+// Entries k = first, first + G, ... < count of a row segment (partner at(k))
+// through vsite_term, with the next entry's virtual site and type loaded
+// before the current one is evaluated.
+template <int VDW, bool LUSTIG, typename R, bool FULL, int G, typename At>
+__device__ __forceinline__ void vsite_walk(const PairArgs& a, const Owner& o, int first, int count, At at,
+                                           double (&acc)[A_N], double& fix, double& fiy, double& fiz) {
+    if (first >= count) return;
+    int j = at(first);
+    int jn = first + G < count ? at(first + G) : j;
+    double ex = a.epos[3 * j], ey = a.epos[3 * j + 1], ez = a.epos[3 * j + 2];
+    int tj = a.type[j];
+    double qj = VDW == V_BUCK ? a.q[j] : 0.0;   // Buckingham's optional Coulomb term
+    for (int k = first; k < count; k += G) {
+        const int jnn = k + 2 * G < count ? at(k + 2 * G) : jn;
+        const double nx = a.epos[3 * jn], ny = a.epos[3 * jn + 1], nz = a.epos[3 * jn + 2];
+        const int nt = a.type[jn];
+        const double nq = VDW == V_BUCK ? a.q[jn] : 0.0;
+        vsite_term<VDW, LUSTIG, R, FULL>(a, o, ex, ey, ez, tj, qj, acc, fix, fiy, fiz);
+        jn = jnn;
+        ex = nx; ey = ny; ez = nz; tj = nt; qj = nq;
     }
 }
 
@@ -368,11 +401,19 @@ __global__ __launch_bounds__(256) void k_pairs(PairArgs a) {
         o.mi = a.mol[i];
         o.cx = o.cy = o.cz = 0.0;
         if constexpr (LUSTIG) { o.cx = a.com[3 * o.mi]; o.cy = a.com[3 * o.mi + 1]; o.cz = a.com[3 * o.mi + 2]; }
-        // the reference's row i: everything
-        for (int k = sub; k < own; k += G) pair_term<VDW, EWALD, LUSTIG, R, true>(a, o, row[k], acc, fix, fiy, fiz);
-        // partners whose reference row holds the pair: force on i only
         const int32_t* back = row + a.stride - 1;
-        for (int k = sub; k < non; k += G) pair_term<VDW, EWALD, LUSTIG, R, false>(a, o, back[-k], acc, fix, fiy, fiz);
+        if constexpr ((VDW == V_BUCK || VDW == V_BUFF) && !EWALD) {
+            // the partner's virtual site and type one entry ahead, its index two
+            vsite_walk<VDW, LUSTIG, R, true, G>(a, o, sub, own, [row](int k) { return row[k]; }, acc, fix, fiy, fiz);
+            vsite_walk<VDW, LUSTIG, R, false, G>(a, o, sub, non, [back](int k) { return back[-k]; }, acc, fix, fiy,
+                                                 fiz);
+        } else {
+            // the reference's row i: everything
+            for (int k = sub; k < own; k += G) pair_term<VDW, EWALD, LUSTIG, R, true>(a, o, row[k], acc, fix, fiy, fiz);
+            // partners whose reference row holds the pair: force on i only
+            for (int k = sub; k < non; k += G)
+                pair_term<VDW, EWALD, LUSTIG, R, false>(a, o, back[-k], acc, fix, fiy, fiz);
+        }
     }
 
     // Owner force: combine the G lanes.  Row i holds every partner of i, so

@@ -101,6 +101,39 @@ def test_descriptors_match_oracle():
     ctx.close()
 
 
+def test_mixed_descriptors_match_oracle():
+    """The mixed solve's descriptors: FP32 pair math over the partner table
+    (after an alpha E0 start) and the shells summed by the mixed field pass
+    (with the learned start), against the oracle on that solve's E0: 1e-4 of
+    each column's scale (FP32 displacements and sums; columns 3-5 go through
+    the mixed A v)."""
+    from oracle import learned_guess as G
+    from swinmd_amd import _lib as L
+    from swinmd_amd.forces import Context, Induction, LearnedGuess
+    s = _system(31)
+    ctx = Context(s)
+    ctx.set_pairlist(CUT, 0.05, True)
+    ind = Induction(ctx, cutoff_ind=CUT, a_damp=0.39, tol=1e-6 * L.DEBYE_E_NM, precond=2, prec=L.PREC_MIXED,
+                    guess=2)
+    net = LearnedGuess(ctx)
+    norm = np.concatenate([np.zeros(G.NF), np.ones(G.NF), np.zeros(G.NO), np.ones(G.NO)]).astype(np.float32)
+    net.set_norm(norm)
+    polar = np.array([s.types[t].polar for t in s.type])
+    for guess in (2, 3):
+        print(f"case mixed descriptors guess {guess}", flush=True)
+        ind.params.guess = guess
+        _, conv, _ = ind.induce()
+        assert conv
+        _, _, e0, _ = ind.results()
+        d = net.descriptors()
+        want = G.descriptors(s.pos, s.mol_start, s.mol, s.scaled_charge, polar, e0, np.asarray(s.box, dtype=float),
+                             CUT, 0.39)
+        scale = np.abs(want).max(axis=0) + 1e-300
+        err = (np.abs(d - want) / scale).max(axis=0)
+        assert err.max() <= 1e-4, (guess, err)
+    ctx.close()
+
+
 def _frames(net, ctx, ind, seeds):
     for sd in seeds:
         s = _system(sd)
