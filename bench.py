@@ -63,10 +63,10 @@ BYTES_PER_DIRECTED_PAIR_TMU = 20
 BYTES_PER_SITE_CG = 11 * 24
 #   A v alone (mixed solve): 4 B per table entry (the partner index; the
 #     Thole factors are recomputed), and per site the FP32 direction and the
-#     two hi/lo position records (16 + 32), v read and z written in FP64
+#     fixed-point position record (16 + 16), v read and z written in FP64
 #     (24 + 24), the polarizability and the row count (8 + 4).
 BYTES_PER_ENTRY_APPLY = 4
-BYTES_PER_SITE_APPLY = 108
+BYTES_PER_SITE_APPLY = 92
 
 
 def log(msg):
@@ -411,7 +411,7 @@ def bench_polar_md(args):
                        "hbm_frac": round(alg / (iso["cg_apply"] * 1e-6) / 1e9 / PEAK_HBM_GBS, 4),
                        "traffic": measured_traffic(f"polar_water{nmol}").get("cg_apply"),
                        "note": "the mixed solve's A v: a 4 B partner index per directed pair, Thole factors "
-                               "recomputed in FP32 from hi/lo site records"}
+                               "recomputed in FP32 from fixed-point site records"}
     if not args.no_cpu_baseline:
         log("polarizable cpu baseline (reference, 1 core, 1,024 molecules)")
         out["cpu_baseline"] = polar_cpu_baseline_md(S.polar_water(1024) if nmol != 1024 else system)

@@ -146,7 +146,7 @@ static void release(Ctx* c) {
                     c->mp_ks.fpart, c->ind_ks.kvec, c->ind_ks.kmod2, c->ind_ks.expk, c->ind_ks.part,
                     c->ind_ks.kf, c->ind_ks.fpart, c->d_pack, c->d_bd_bonds, c->d_bd_angles,
                     c->d_bd_bond_off, c->d_bd_angle_off, c->d_bd_par, c->d_bd_out, c->d_stress, c->d_real,
-                    c->d_it_cnt, c->d_it_j, c->d_it_a, c->d_part, c->d_cg32, c->d_ind_rec32, c->d_e0_prev, c->d_pc_blk, c->d_shell32, c->d_polar_terms, c->d_site32, c->d_pind32,
+                    c->d_it_cnt, c->d_it_j, c->d_it_a, c->d_part, c->d_cg32, c->d_ind_rec32, c->d_ind_fix, c->d_e0_prev, c->d_pc_blk, c->d_shell32, c->d_polar_terms, c->d_site32, c->d_pind32,
                     c->red.base};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);

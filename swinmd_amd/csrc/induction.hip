@@ -65,6 +65,10 @@ struct IndArgs {
     // rec32[2i] = (x_hi, y_hi, z_hi, 1/sqrt(alpha)), rec32[2i+1] = (x_lo, y_lo, z_lo, q / sqrt(C))
     int mixed;
     float4* rec32;
+    // mixed A v: positions as 32-bit fixed-point fractions of the box (the
+    // difference of two wraps to the minimum image) and 1/sqrt(alpha)
+    int4* fix;
+    float fscale[3];            // L / 2^32 per dimension
     float4* site32;             // mixed solve: (q / sqrt(C), mu), (Q xx xy xz yy), (Q yz zz, alpha, 0) per site
     float Lh[3], Ll[3], invL[3];
 };
@@ -387,10 +391,11 @@ __global__ __launch_bounds__(256) void k_ind_table(IndArgs a) {
 }
 
 // The mixed solve's permanent field and coupling table in one pass over the
-// rows, with FP32 pair math from the solve's hi/lo site records and FP32
-// multipole records (rec32, site32; the displacement's first difference is
-// exact, so which pairs count matches the FP64 test except within FP32
-// rounding of cutoffInd), four entries per lane in flight, FP64 field sums.
+// rows: which pairs count is decided in FP64 exactly as the FP64 kernels
+// decide it (the truncated field jumps by q/rc^2 at cutoffInd, so one pair
+// decided differently would move a dipole by a few percent), the pair math
+// is FP32 from FP32 multipole records (site32), four entries per lane in
+// flight, FP64 field sums.
 // The table holds the partners only: the mixed A v recomputes their Thole
 // factors.  The field of a pair is field_from's expression gathered by
 // powers of r:
@@ -415,10 +420,9 @@ __global__ __launch_bounds__(256) void k_field_table32(IndArgs a, float* __restr
         const int own = min(a.nb_own[i], a.stride);
         const int cnt = min(a.nb_cnt[i], a.stride);
         const int e0 = a.excl_off[i], e1 = a.excl_off[i + 1];
-        const float4 hi = a.rec32[2 * i], lo = a.rec32[2 * i + 1];
-        const float rs_i = hi.w;
+        const double xi = a.pos[3 * i], yi = a.pos[3 * i + 1], zi = a.pos[3 * i + 2];
+        const float rs_i = a.rec32[2 * i].w;
         const float damp = (float)a.a;
-        const float cut2 = (float)a.cut2;
         const int total = cnt + (e1 - e0);
         for (int kb = 0; kb < total; kb += 4 * G) {
             int jj[4];
@@ -427,29 +431,30 @@ __global__ __launch_bounds__(256) void k_field_table32(IndArgs a, float* __restr
                 const int k = kb + u * G + sub;
                 jj[u] = k < total ? (k < cnt ? row_partner(a, i, own, k) : a.excl[e0 + (k - cnt)]) : i;
             }
-            float4 h[4], l[4];
+            double px[4], py[4], pz[4];
+            float rsj[4];
 #pragma unroll
             for (int u = 0; u < 4; ++u) {
-                h[u] = a.rec32[2 * jj[u]];
-                l[u] = a.rec32[2 * jj[u] + 1];
+                px[u] = a.pos[3 * jj[u]];
+                py[u] = a.pos[3 * jj[u] + 1];
+                pz[u] = a.pos[3 * jj[u] + 2];
+                rsj[u] = a.rec32[2 * jj[u]].w;
             }
             bool ok[4];
 #pragma unroll
             for (int u = 0; u < 4; ++u) {
                 const int k = kb + u * G + sub;
                 const int j = jj[u];
-                const float hx = hi.x - h[u].x, hy = hi.y - h[u].y, hz = hi.z - h[u].z;
-                const float nx = rintf(hx * a.invL[0]), ny = rintf(hy * a.invL[1]), nz = rintf(hz * a.invL[2]);
-                const float rx = (hx - nx * a.Lh[0]) + ((lo.x - l[u].x) - nx * a.Ll[0]);
-                const float ry = (hy - ny * a.Lh[1]) + ((lo.y - l[u].y) - ny * a.Ll[1]);
-                const float rz = (hz - nz * a.Lh[2]) + ((lo.z - l[u].z) - nz * a.Ll[2]);
-                const float r2 = rx * rx + ry * ry + rz * rz;
-                ok[u] = k < total && r2 <= cut2;
+                double dx = xi - px[u], dy = yi - py[u], dz = zi - pz[u];
+                pbc_near(dx, dy, dz, a.box);
+                const double rd64 = sqrt(dx * dx + dy * dy + dz * dz);
+                ok[u] = k < total && rd64 * rd64 <= a.cut2;   // the FP64 kernels' test, bit for bit
                 if (ok[u] && k < cnt) {
                     // the permanent field (E0 leaves the excluded partners out)
+                    const float rx = (float)dx, ry = (float)dy, rz = (float)dz, r2 = (float)(rd64 * rd64);
                     const float4 s0 = a.site32[3 * j], s1 = a.site32[3 * j + 1], s2 = a.site32[3 * j + 2];
                     const float r_1 = __builtin_amdgcn_rsqf(r2);
-                    const float x = damp * (r2 * r2 * r_1) * (rs_i * h[u].w);
+                    const float x = damp * (r2 * r2 * r_1) * (rs_i * rsj[u]);
                     const float e = __expf(-x);
                     const float r3 = r_1 * r_1 * r_1, r5 = r3 * r_1 * r_1;
                     const float l3 = 1.0f - e, l5 = 1.0f - (1.0f + x) * e;
@@ -605,6 +610,18 @@ __global__ __launch_bounds__(256) void k_pack_ind32(IndArgs a) {
     a.rec32[2 * i] = make_float4(xh, yh, zh, (float)(1.0 / sqrt(p)));
     a.rec32[2 * i + 1] = make_float4((float)(x - (double)xh), (float)(y - (double)yh), (float)(z - (double)zh),
                                      (float)(a.q[i] / SQRT_COULOMB));
+    {
+        // fractions of the box in [0, 1) as unsigned 32-bit fixed point
+        unsigned u[3];
+        const double xs[3] = {x, y, z};
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            double f = xs[k] * a.box.inv[k];
+            f -= floor(f);
+            u[k] = (unsigned)((unsigned long long)(f * 4294967296.0) & 0xffffffffull);
+        }
+        a.fix[i] = make_int4((int)u[0], (int)u[1], (int)u[2], __float_as_int((float)(1.0 / sqrt(p))));
+    }
     const double* d = a.dip + 3 * i;
     const double* Q = a.quad + 6 * i;
     a.site32[3 * i] = make_float4((float)(a.q[i] / SQRT_COULOMB), (float)d[0], (float)d[1], (float)d[2]);
@@ -630,21 +647,21 @@ __global__ __launch_bounds__(256) void k_apply_pos32(IndArgs a, const float4* __
     if (i < a.n) {
         const int cnt = a.tcnt[i];
         pi = a.polar[i] > POLMIN ? a.polar[i] : POLMIN;
-        const float4 hi = a.rec32[2 * i], lo = a.rec32[2 * i + 1];
+        const int4 fi = a.fix[i];
+        const float rs_i = __int_as_float(fi.w);
         const float damp = (float)a.a;
         const int32_t* tj = a.tj + (int64_t)i * a.tstride;
-        auto term = [&](float4 h, float4 l, float4 vj) {
-            const float hx = h.x - hi.x, hy = h.y - hi.y, hz = h.z - hi.z;
-            const float nx = rintf(hx * a.invL[0]), ny = rintf(hy * a.invL[1]), nz = rintf(hz * a.invL[2]);
-            const float dx = (hx - nx * a.Lh[0]) + ((l.x - lo.x) - nx * a.Ll[0]);
-            const float dy = (hy - ny * a.Lh[1]) + ((l.y - lo.y) - ny * a.Ll[1]);
-            const float dz = (hz - nz * a.Lh[2]) + ((l.z - lo.z) - nz * a.Ll[2]);
+        auto term = [&](int4 fj, float4 vj) {
+            // 32-bit differences wrap to the minimum image; exact, then one rounding
+            const float dx = (float)(int)((unsigned)fj.x - (unsigned)fi.x) * a.fscale[0];
+            const float dy = (float)(int)((unsigned)fj.y - (unsigned)fi.y) * a.fscale[1];
+            const float dz = (float)(int)((unsigned)fj.z - (unsigned)fi.z) * a.fscale[2];
             const float r2 = dx * dx + dy * dy + dz * dz;
             const float r_1 = __builtin_amdgcn_rsqf(r2);
             const float r3 = r_1 * r_1 * r_1, r5 = r3 * r_1 * r_1;
             float ca = 3.0f * r5, cb = r3;
             if (a.damp) {
-                const float x = damp * (r2 * r2 * r_1) * (hi.w * h.w);
+                const float x = damp * (r2 * r2 * r_1) * (rs_i * __int_as_float(fj.w));
                 const float e = __expf(-x);
                 cb = (1.0f - e) * r3;
                 ca = 3.0f * (1.0f - (1.0f + x) * e) * r5;
@@ -659,19 +676,19 @@ __global__ __launch_bounds__(256) void k_apply_pos32(IndArgs a, const float4* __
             int jj[4];
 #pragma unroll
             for (int u = 0; u < 4; ++u) jj[u] = tj[k + u * G];
-            float4 h[4], l[4], vj[4];
+            int4 fj[4];
+            float4 vj[4];
 #pragma unroll
             for (int u = 0; u < 4; ++u) {
-                h[u] = a.rec32[2 * jj[u]];
-                l[u] = a.rec32[2 * jj[u] + 1];
+                fj[u] = a.fix[jj[u]];
                 vj[u] = v32[jj[u]];
             }
 #pragma unroll
-            for (int u = 0; u < 4; ++u) term(h[u], l[u], vj[u]);
+            for (int u = 0; u < 4; ++u) term(fj[u], vj[u]);
         }
         for (; k < cnt; k += G) {
             const int j = tj[k];
-            term(a.rec32[2 * j], a.rec32[2 * j + 1], v32[j]);
+            term(a.fix[j], v32[j]);
         }
     }
     double acc[3] = {ax, ay, az};
@@ -1147,6 +1164,8 @@ static IndArgs ind_args(Ctx* c, const smd_induce_params* p) {
     a.tab = reinterpret_cast<double2*>(c->d_it_a);
     a.mixed = p->prec == SMD_PREC_MIXED ? 1 : 0;
     a.rec32 = c->d_ind_rec32;
+    a.fix = c->d_ind_fix;
+    for (int d = 0; d < 3; ++d) a.fscale[d] = (float)(c->box.L[d] / 4294967296.0);
     a.site32 = c->d_site32;
     for (int d = 0; d < 3; ++d) {
         const double L = c->box.L[d];
@@ -1506,6 +1525,7 @@ extern "C" int smd_set_multipoles(smd_ctx* c, const double* dip_local, const dou
             (rc = A(c->d_mu, sizeof(double) * 3 * n)) || (rc = A(c->d_e0_prev, sizeof(double) * 3 * n)) ||
             (rc = A(c->d_cg, sizeof(double) * 9 * n)) ||
             (rc = A(c->d_cg32, sizeof(float4) * n)) || (rc = A(c->d_ind_rec32, sizeof(float4) * 2 * n)) ||
+            (rc = A(c->d_ind_fix, sizeof(int4) * n)) ||
             (rc = A(c->d_pc_blk, sizeof(double) * 45 * (size_t)(c->nmol > 0 ? c->nmol : 1))) ||
             (rc = A(c->d_shell32, sizeof(float) * 24 * n)) ||
             (rc = A(c->d_site32, sizeof(float4) * 3 * n)) ||

@@ -761,30 +761,35 @@ int ewald_real(Ctx* c, int prec, int lustig) {
 
 // ---------------------------------------------------------------------------
 // Mixed-precision LJ + Ewald pairs of the MD step from packed site records.
-// Every row reads one 32-byte record per partner instead of separate
-// position, charge, type and molecule gathers:
-//   rec[2j]   = (x_hi, y_hi, z_hi, q)            x_hi = float(x)
-//   rec[2j+1] = (x_lo, y_lo, z_lo, type<<24|mol)  x_lo = float(x - x_hi)
+// Every row reads one 16-byte record and one word per partner instead of
+// separate position, charge, type and molecule gathers:
+//   rec[j] = (x, y, z as 32-bit fixed-point fractions of the box, q)
+//   tm[j]  = type << 24 | mol
 // and, for the Lustig terms of owned pairs, off[j] = float(r_j - R_mol(j)).
-// The displacement is formed as (x_hi_j - x_hi_i - n L_hi) + (x_lo_j - x_lo_i
-// - n L_lo): the first difference is exact (the operands are within a factor
-// of two once the image is taken), so the FP32 displacement carries only its
-// own rounding, as the FP64 difference converted to FP32 does.  The molecule
-// vector of the Lustig terms is r_ij + off_i - off_j.
+// The difference of two fixed-point coordinates is exact and, read as a
+// signed integer, is already the minimum image; one rounding to FP32
+// follows (the quantum is L / 2^32).  The molecule vector of the Lustig
+// terms is r_ij + off_i - off_j.
 
 __global__ __launch_bounds__(256) void k_pack_sites(int n, const double* __restrict__ pos, const double* __restrict__ q,
                                                     const int32_t* __restrict__ type, const int32_t* __restrict__ mol,
-                                                    const double* __restrict__ com, float4* __restrict__ rec,
-                                                    float4* __restrict__ off) {
+                                                    const double* __restrict__ com, Box box, int4* __restrict__ rec,
+                                                    int32_t* __restrict__ tm, float4* __restrict__ off) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    const double x = pos[3 * i], y = pos[3 * i + 1], z = pos[3 * i + 2];
-    const float xh = (float)x, yh = (float)y, zh = (float)z;
+    const double xs[3] = {pos[3 * i], pos[3 * i + 1], pos[3 * i + 2]};
+    unsigned u[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        double f = xs[k] * box.inv[k];
+        f -= floor(f);
+        u[k] = (unsigned)((unsigned long long)(f * 4294967296.0) & 0xffffffffull);
+    }
     const int m = mol[i];
-    rec[2 * i] = make_float4(xh, yh, zh, (float)q[i]);
-    rec[2 * i + 1] = make_float4((float)(x - (double)xh), (float)(y - (double)yh), (float)(z - (double)zh),
-                                 __int_as_float((type[i] << 24) | (m & 0xffffff)));
-    off[i] = make_float4((float)(x - com[3 * m]), (float)(y - com[3 * m + 1]), (float)(z - com[3 * m + 2]), 0.0f);
+    rec[i] = make_int4((int)u[0], (int)u[1], (int)u[2], __float_as_int((float)q[i]));
+    tm[i] = (type[i] << 24) | (m & 0xffffff);
+    off[i] = make_float4((float)(xs[0] - com[3 * m]), (float)(xs[1] - com[3 * m + 1]), (float)(xs[2] - com[3 * m + 2]),
+                         0.0f);
 }
 
 struct PackedArgs {
@@ -792,14 +797,15 @@ struct PackedArgs {
     const int32_t* nb;
     const int32_t* nb_cnt;
     const int32_t* nb_own;
-    const float4* rec;
+    const int4* rec;
+    const int32_t* tm;
     const float4* off;
     const double* ta;   // sigma^2 [ntypes^2]
     const double* tb;   // 4 eps
     double* force;
     double* terms;
     Red red;
-    float Lh[3], Ll[3], invL[3];
+    float fscale[3];    // L / 2^32 per dimension
     float cut2, rcut_2, alpha, alpha2, alpha2PI, realcut2;
 };
 
@@ -807,7 +813,8 @@ struct PackedArgs {
 constexpr int PACKED_MAX_TYPES = 16;
 
 struct PackedOwner {
-    float xh, yh, zh, xl, yl, zl, qi, ox, oy, oz;
+    unsigned ux, uy, uz;
+    float qi, ox, oy, oz;
     int ti, mi;
 };
 
@@ -825,17 +832,12 @@ enum {
 // FP64 (the grid's in a fixed block order).
 template <bool LUSTIG, bool FULL>
 __device__ __forceinline__ void packed_term(const PackedArgs& a, const float2* __restrict__ ljt, const PackedOwner& o,
-                                            int j, float4 r0, float4 r1, double (&acc)[PK_N], double& fix,
+                                            int j, int4 r0, int tm, double (&acc)[PK_N], double& fix,
                                             double& fiy, double& fiz) {
-    float dx, dy, dz;
-    {
-        const float hx = r0.x - o.xh, hy = r0.y - o.yh, hz = r0.z - o.zh;
-        const float nx = rintf(hx * a.invL[0]), ny = rintf(hy * a.invL[1]), nz = rintf(hz * a.invL[2]);
-        dx = (hx - nx * a.Lh[0]) + ((r1.x - o.xl) - nx * a.Ll[0]);
-        dy = (hy - ny * a.Lh[1]) + ((r1.y - o.yl) - ny * a.Ll[1]);
-        dz = (hz - nz * a.Lh[2]) + ((r1.z - o.zl) - nz * a.Ll[2]);
-    }
-    const int tm = __float_as_int(r1.w);
+    const float dx = (float)(int)((unsigned)r0.x - o.ux) * a.fscale[0];
+    const float dy = (float)(int)((unsigned)r0.y - o.uy) * a.fscale[1];
+    const float dz = (float)(int)((unsigned)r0.z - o.uz) * a.fscale[2];
+    const float qj = __int_as_float(r0.w);
     const int mj = tm & 0xffffff, tj = (int)((unsigned)tm >> 24);
     const float r2 = dx * dx + dy * dy + dz * dz;
     const float r_1 = __builtin_amdgcn_rsqf(r2);
@@ -879,7 +881,7 @@ __device__ __forceinline__ void packed_term(const PackedArgs& a, const float2* _
         s -= tmpf;
     }
     if (r2 < a.realcut2) {
-        const float qij = o.qi * r0.w;
+        const float qij = o.qi * qj;
         float e, ex;
         ewald_erfc_f32(a.alpha, a.alpha2, r2, r, r_1, e, ex);
         const float f = (e + a.alpha2PI * ex) * rij_2;
@@ -914,12 +916,14 @@ __device__ __forceinline__ void packed_segment(const PackedArgs& a, const float2
     if (first >= count) return;
     int j = at(first);
     int jn = first + G < count ? at(first + G) : j;
-    float4 r0 = a.rec[2 * j], r1 = a.rec[2 * j + 1];
+    int4 r0 = a.rec[j];
+    int t0 = a.tm[j];
     for (int k = first; k < count; k += G) {
         const int jnn = k + 2 * G < count ? at(k + 2 * G) : jn;
-        const float4 n0 = a.rec[2 * jn], n1 = a.rec[2 * jn + 1];
-        packed_term<LUSTIG, FULL>(a, ljt, o, j, r0, r1, acc, fix, fiy, fiz);
-        j = jn; jn = jnn; r0 = n0; r1 = n1;
+        const int4 n0 = a.rec[jn];
+        const int nt = a.tm[jn];
+        packed_term<LUSTIG, FULL>(a, ljt, o, j, r0, t0, acc, fix, fiy, fiz);
+        j = jn; jn = jnn; r0 = n0; t0 = nt;
     }
 }
 
@@ -941,11 +945,10 @@ __global__ __launch_bounds__(256) void k_pairs_packed(PackedArgs a) {
         const int32_t* row = a.nb + (int64_t)i * a.stride;
         const int own = min(a.nb_own[i], a.stride);
         const int non = min(a.nb_cnt[i] - a.nb_own[i], a.stride - own);
-        const float4 r0 = a.rec[2 * i], r1 = a.rec[2 * i + 1];
+        const int4 r0 = a.rec[i];
         PackedOwner o;
-        o.xh = r0.x; o.yh = r0.y; o.zh = r0.z; o.qi = r0.w;
-        o.xl = r1.x; o.yl = r1.y; o.zl = r1.z;
-        const int tm = __float_as_int(r1.w);
+        o.ux = (unsigned)r0.x; o.uy = (unsigned)r0.y; o.uz = (unsigned)r0.z; o.qi = __int_as_float(r0.w);
+        const int tm = a.tm[i];
         o.mi = tm & 0xffffff;
         o.ti = (int)((unsigned)tm >> 24);
         o.ox = o.oy = o.oz = 0.f;
@@ -1009,10 +1012,11 @@ __global__ __launch_bounds__(256) void k_pairs_packed(PackedArgs a) {
 static int pairs_packed(Ctx* c, const PairArgs& pa, int lustig) {
     const int n = c->n;
     if (!c->d_pack) {
-        SMD_HIP(c, hipMalloc(&c->d_pack, sizeof(float4) * 3 * (size_t)n));   // rec [2n], off [n]
+        SMD_HIP(c, hipMalloc(&c->d_pack, sizeof(float4) * 3 * (size_t)n));   // rec [n], off [n], tm [n]
     }
-    float4* rec = reinterpret_cast<float4*>(c->d_pack);
-    float4* off = rec + 2 * (size_t)n;
+    int4* rec = reinterpret_cast<int4*>(c->d_pack);
+    float4* off = reinterpret_cast<float4*>(c->d_pack) + (size_t)n;
+    int32_t* tm = reinterpret_cast<int32_t*>(reinterpret_cast<float4*>(c->d_pack) + 2 * (size_t)n);
     PackedArgs a{};
     a.n = n;
     a.stride = pa.stride;
@@ -1024,18 +1028,14 @@ static int pairs_packed(Ctx* c, const PairArgs& pa, int lustig) {
     a.nb_cnt = pa.nb_cnt;
     a.nb_own = pa.nb_own;
     a.rec = rec;
+    a.tm = tm;
     a.off = off;
     a.ta = pa.ta;
     a.tb = pa.tb;
     a.force = pa.force;
     a.terms = pa.terms;
     a.red = pa.red;
-    for (int d = 0; d < 3; ++d) {
-        const double L = c->box.L[d];
-        a.Lh[d] = (float)L;
-        a.Ll[d] = (float)(L - (double)a.Lh[d]);
-        a.invL[d] = (float)(1.0 / L);
-    }
+    for (int d = 0; d < 3; ++d) a.fscale[d] = (float)(c->box.L[d] / 4294967296.0);
     a.cut2 = (float)pa.cut2;
     a.rcut_2 = (float)pa.rcut_2;
     a.alpha = (float)pa.alpha;
@@ -1045,7 +1045,7 @@ static int pairs_packed(Ctx* c, const PairArgs& pa, int lustig) {
     ProfScope ps(c, "pairs");
     if (!c->pack_reuse)
         hipLaunchKernelGGL(k_pack_sites, dim3((n + 255) / 256), dim3(256), 0, c->stream, n, c->d_pos, c->d_q,
-                           c->d_type, c->d_mol, c->d_com, rec, off);
+                           c->d_type, c->d_mol, c->d_com, c->box, rec, tm, off);
     const int pl = c->pair_lanes;
     const int lanes = pl == 4 || pl == 8 || pl == 32 || pl == 64 ? pl : 16;
     const dim3 grid((unsigned)(((int64_t)n * lanes + 255) / 256));

@@ -194,6 +194,7 @@ struct Ctx {
     int npol = 0;                  // sites with a polarizability above POLMIN
     float4* d_cg32 = nullptr;      // mixed solve: FP32 copy of the CG direction [n]
     float4* d_ind_rec32 = nullptr; // mixed solve: hi/lo FP32 site records [2n]
+    int4* d_ind_fix = nullptr;     // mixed A v: fixed-point positions and 1/sqrt(alpha) [n]
     double* d_e0_prev = nullptr;   // E0 of the previous solve with guess 4 [3n]
     double* d_pc_blk = nullptr;    // block preconditioner: Cholesky factors [nmol][45]
     float* d_shell32 = nullptr;    // the learned guess's charge shells from the mixed field pass [n][24]

@@ -81,5 +81,6 @@ def test_bench_reads_the_committed_traffic_summaries():
     md = bench.measured_traffic("water4096")
     assert md.get("pairs", 0) > 20_000_000 and md.get("recip_sfac", 0) > 5_000_000
     polar = bench.measured_traffic("polar_water8192")
-    assert polar.get("apply_table", 0) > 150_000_000
+    assert polar.get("cg_apply", 0) > 20_000_000
+    assert polar.get("field_table32", 0) > 50_000_000
     assert bench.measured_traffic("no_such_workload") == {}
