@@ -269,6 +269,8 @@ def bench_polar_md(args):
         ctx.set_tuning("cg_batch", cg_batch)
         if args.no_overlap:
             ctx.set_tuning("recip_overlap", 0)
+        if args.no_joint:
+            ctx.set_tuning("polar_joint_long", 0)
         ind = Induction(ctx, cutoff_ind=POLAR["cutoff_ind"], a_damp=POLAR["a_damp"], damp_tij=True, do_min=True,
                         tol=tol, precond=args.precond, guess=guess, prec=prec)
         if net_state is not None:
@@ -616,6 +618,8 @@ def main():
                     help="first copy MIB MiB device to device (counter calibration for profiling runs)")
     ap.add_argument("--no-overlap", action="store_true",
                     help="polarizable workloads: keep the k-space passes on the step's stream")
+    ap.add_argument("--no-joint", action="store_true",
+                    help="polarizable workloads: a k-space pass per class instead of one joint pass after the solve")
     ap.add_argument("--no-graph", action="store_true",
                     help="launch each step's kernels directly instead of replaying a captured hipGraph")
     args = ap.parse_args()

@@ -589,6 +589,9 @@ int smd_md_time_family(smd_ctx* ctx, const char* family, int reps, double* avg_u
  * them beyond FP rounding order).  "pair_packed" (0/1, default 1): the MD
  * step's mixed LJ + Ewald pairs read packed FP32 hi/lo site records.
  * "recip_overlap" (0/1, default 1): the MD step's k-space on a second stream.
+ * "polar_joint_long" (0/1, default 1): the polarizable step runs one k-space
+ * pass for Multipole and Induction after the dipole solve when both use the
+ * same alpha and kcut (0: a pass per class).
  * "md_fused_shake" (0/1, default 1): the MD step runs SHAKE and the kinetic
  * and pressure sums as one kernel (0: two kernels).
  * "cg_lanes" (8, 16 or 32, default 16): lanes sharing one row of the CG's A v.

@@ -419,6 +419,9 @@ extern "C" int smd_set_tuning(smd_ctx* c, const char* name, int value) {
         if (value != 0 && value != 1) return fail(c, SMD_EINVAL, "cg_graph must be 0 or 1 (got %d)", value);
         c->cg_graph = value;
         if (c->cg_exec) { (void)hipGraphExecDestroy(c->cg_exec); c->cg_exec = nullptr; }
+    } else if (std::string(name) == "polar_joint_long") {
+        if (value != 0 && value != 1) return fail(c, SMD_EINVAL, "polar_joint_long must be 0 or 1 (got %d)", value);
+        c->polar_joint = value;
     } else if (std::string(name) == "recip_overlap") {
         if (value != 0 && value != 1) return fail(c, SMD_EINVAL, "recip_overlap must be 0 or 1 (got %d)", value);
         c->recip_overlap = value;

@@ -1034,6 +1034,8 @@ static int kset_build(Ctx* c, Ctx::KSet& ks, double alpha, double kcut, int nsum
     ks.K = K;
     ks.chunks = chunks;
     ks.ksplit = ksplit;
+    ks.alpha = alpha;
+    ks.kcut = kcut;
     return SMD_OK;
 }
 
@@ -1591,6 +1593,99 @@ __global__ __launch_bounds__(WAVE) void k_ind_kforce(MpArgs a) {
     }
 }
 
+// ---- the polarizable step's joint k-space pass ----------------------------------
+//
+// Multipole and Induction on the same k set (same alpha, kcut and box): the
+// induction pass's 28 sums per k hold the 20 of the multipole pass (rows 0-3,
+// 2 kQk as 6-7, the constraint moments as 8-19 and 26-27), so one
+// structure-factor pass serves both classes' energies, virials and volume
+// derivatives, and one force pass applies both classes' coefficients:
+//   val   = c (P.k X - A U + Mu W) + s (P.k Y + A W + Mu U)
+//   tau   = (c U - s W) k x mu - 2 (c W + s U) k x Qk
+// with U = b Q + b Pc and W = b P - b Ps (b P, b Q the multipole pass's
+// coefficients; X, Y, b Pc, b Ps the induction pass's).
+
+// One wave per k: both classes' per-k terms from the one set of sums; the
+// joint force coefficients (X, Y, U, W) replace the induction pass's.
+__global__ __launch_bounds__(256) void k_joint_kfinish(MpArgs mp, MpArgs in) {
+    const int k = blockIdx.x * (blockDim.x / WAVE) + threadIdx.x / WAVE;
+    double vm[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0}, vi[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    if (k < in.K) {
+        double S[IN_NSUM];
+        wave_chunk_sum<IN_NSUM>(in, k, S);
+        if (threadIdx.x % WAVE == 0) {
+            double M[MP_NSUM];
+            M[0] = S[0]; M[1] = S[1]; M[2] = S[2]; M[3] = S[3]; M[4] = S[6]; M[5] = S[7];
+#pragma unroll
+            for (int x = 0; x < 12; ++x) M[6 + x] = S[8 + x];
+            M[18] = S[26]; M[19] = S[27];
+            mp_finish(mp, k, M, vm);
+            ind_finish(in, k, S, vi);
+            const double bP = mp.kf[4 * k], bQ = mp.kf[4 * k + 1];
+            const double bPc = in.kf[4 * k + 2], bPs = in.kf[4 * k + 3];
+            in.kf[4 * k + 2] = bQ + bPc;
+            in.kf[4 * k + 3] = bP - bPs;
+        }
+    }
+    block_part<9>(vm, mp.red, RS_MP_LONG);
+    __syncthreads();
+    block_part<9>(vi, in.red, RS_IND_LONG);
+}
+
+// The joint force pass: the induction pass's layout (site per lane, k-range
+// per block row) with both classes' force and torque per (site, k).
+template <typename R>
+__global__ __launch_bounds__(WAVE) void k_joint_kforce(MpArgs a) {
+    __shared__ double sk[MP_KTILE][7];
+    const int i = blockIdx.x * WAVE + threadIdx.x;
+    const int kb = blockIdx.y * a.kper, ke = min(a.K, kb + a.kper);
+    Site I64{};
+    V3 ri = v3(0, 0, 0), p64 = v3(0, 0, 0);
+    if (i < a.n) {
+        I64 = load_site(a, i);
+        ri = ldv(a.pos + 3 * i);
+        p64 = ldv(a.pind + 3 * i);
+    }
+    const SiteT<R> I = site_as<R>(I64);
+    const V3T<R> p = vec3<R>((R)p64.x, (R)p64.y, (R)p64.z);
+    V3 F = v3(0, 0, 0), T = v3(0, 0, 0);
+    for (int k0 = kb; k0 < ke; k0 += MP_KTILE) {
+        const int cnt = min(MP_KTILE, ke - k0);
+        __syncthreads();
+        for (int x = threadIdx.x; x < cnt; x += WAVE) {
+            sk[x][0] = a.kvec[3 * (k0 + x)];
+            sk[x][1] = a.kvec[3 * (k0 + x) + 1];
+            sk[x][2] = a.kvec[3 * (k0 + x) + 2];
+            sk[x][3] = a.kf[4 * (k0 + x)];
+            sk[x][4] = a.kf[4 * (k0 + x) + 1];
+            sk[x][5] = a.kf[4 * (k0 + x) + 2];
+            sk[x][6] = a.kf[4 * (k0 + x) + 3];
+        }
+        __syncthreads();
+        if (i < a.n) {
+            V3T<R> f = vec3<R>(R(0), R(0), R(0)), t = f;
+            for (int x = 0; x < cnt; ++x) {
+                R sn, cs;
+                phase_sincos<R>(ri.x * sk[x][0] + ri.y * sk[x][1] + ri.z * sk[x][2], sn, cs);
+                const V3T<R> kv = vec3<R>((R)sk[x][0], (R)sk[x][1], (R)sk[x][2]);
+                const R X = (R)sk[x][3], Y = (R)sk[x][4], U = (R)sk[x][5], W = (R)sk[x][6];
+                const V3T<R> Qk = smul(I.Q, kv);
+                const R A = I.q - dot(kv, Qk), Mu = dot(I.d, kv), Pk = dot(p, kv);
+                const R val = cs * (Pk * X - A * U + Mu * W) + sn * (Pk * Y + A * W + Mu * U);
+                f = f + val * kv;
+                if (a.torques) t = t + (cs * U - sn * W) * cross(kv, I.d) - (R(2) * (cs * W + sn * U)) * cross(kv, Qk);
+            }
+            F = F + to64(f);
+            T = T + to64(t);
+        }
+    }
+    if (i < a.n) {
+        double* q = a.fpart + ((size_t)blockIdx.y * a.n + i) * 6;
+        q[0] = F.x; q[1] = F.y; q[2] = F.z;
+        q[3] = T.x; q[4] = T.y; q[5] = T.z;
+    }
+}
+
 static MpArgs ind_args(Ctx* c) {
     const smd_indf_params& p = c->indf;
     MpArgs a = mp_args(c, c->ind_ks, p.alpha, p.real_cut, p.use_constraint, p.torques);
@@ -1741,7 +1836,7 @@ static int mp_long_sums(Ctx* c, const MpArgs& a, bool mp) {
 // context has one and reciprocal overlap is on): the second stream waits for
 // everything enqueued so far and becomes the context's stream until
 // polar_fork_end; polar_join makes the step's stream wait for it.
-static bool polar_fork_begin(Ctx* c) {
+bool polar_fork_begin(Ctx* c) {
     if (!c->recip_overlap || !c->stream2 || c->red_defer == false) return false;
     if (hipEventRecord(c->ev_fork, c->stream) != hipSuccess) return false;
     if (hipStreamWaitEvent(c->stream2, c->ev_fork, 0) != hipSuccess) return false;
@@ -1750,12 +1845,49 @@ static bool polar_fork_begin(Ctx* c) {
     return true;
 }
 
-static void polar_fork_end(Ctx* c) {
+void polar_fork_end(Ctx* c) {
     (void)hipEventRecord(c->ev_join, c->stream2);
     c->stream = c->polar_main;
 }
 
-static void polar_join(Ctx* c) { (void)hipStreamWaitEvent(c->stream, c->ev_join, 0); }
+void polar_join(Ctx* c) { (void)hipStreamWaitEvent(c->stream, c->ev_join, 0); }
+
+// The step's short-range kernels of each class (self, site sums, real
+// space, intramolecular, surface, their torques), into a.terms and d_force.
+static int mp_short_enqueue(Ctx* c, const MpArgs& a) {
+    int rc;
+    const dim3 b256(256), ga((c->n + 255) / 256), gm((c->nmol + 255) / 256);
+    ProfScope ps(c, "mp_short");
+    hipLaunchKernelGGL(k_mp_site_sums, ga, b256, 0, c->stream, a);
+    if ((rc = site_sums_finish(c, a, ga.x, false))) return rc;
+    if ((rc = launch_real(c, a, false))) return rc;
+    hipLaunchKernelGGL(k_mp_intra, gm, b256, 0, c->stream, a);
+    if ((rc = red_finish(c, RS_MP_INTRA, gm.x, 2, a.terms + M_INTRA_E))) return rc;
+    if (c->mp.surface) {
+        hipLaunchKernelGGL(k_mp_surface, ga, b256, 0, c->stream, a, 2.0 * PI * (1.0 / a.volume) / 3.0);
+        if ((rc = red_finish(c, RS_MP_SURF, ga.x, 9, a.terms + M_SU_VIR))) return rc;
+    }
+    if (c->mp.torques && (rc = launch_torques(c, a, gm, c->mp.surface ? 7u : 3u, RS_MP_DUDV))) return rc;
+    return SMD_OK;
+}
+
+static int ind_short_enqueue(Ctx* c, const MpArgs& a) {
+    int rc;
+    const dim3 b256(256), ga((c->n + 255) / 256), gm((c->nmol + 255) / 256);
+    ProfScope ps(c, "ind_short");
+    hipLaunchKernelGGL(k_ind_site_sums, ga, b256, 0, c->stream, a);
+    if ((rc = site_sums_finish(c, a, ga.x, true))) return rc;
+    if ((rc = launch_real(c, a, true))) return rc;
+    hipLaunchKernelGGL(k_ind_intra, gm, b256, 0, c->stream, a);
+    if ((rc = red_finish(c, RS_IND_INTRA, gm.x, 2, a.terms + M_INTRA_E))) return rc;
+    if (c->indf.surface) {
+        hipLaunchKernelGGL(k_ind_surface, ga, b256, 0, c->stream, a, 2.0 * PI * (1.0 / a.volume) / 3.0);
+        if ((rc = red_finish(c, RS_IND_SURF, ga.x, 9, a.terms + M_SU_VIR))) return rc;
+    }
+    if (c->indf.torques && (rc = launch_torques(c, a, gm, 16u | 3u | (c->indf.surface ? 4u : 0u), RS_IND_DUDV)))
+        return rc;
+    return SMD_OK;
+}
 
 int multipole_enqueue(Ctx* c, double* terms) {
     int rc;
@@ -1771,19 +1903,7 @@ int multipole_enqueue(Ctx* c, double* terms) {
     int rc_long = fork ? mp_long_sums(c, a, true) : SMD_OK;
     if (fork) polar_fork_end(c);
     if (rc_long) return rc_long;
-    {
-        ProfScope ps(c, "mp_short");
-        hipLaunchKernelGGL(k_mp_site_sums, ga, b256, 0, c->stream, a);
-        if ((rc = site_sums_finish(c, a, ga.x, false))) return rc;
-        if ((rc = launch_real(c, a, false))) return rc;
-        hipLaunchKernelGGL(k_mp_intra, gm, b256, 0, c->stream, a);
-        if ((rc = red_finish(c, RS_MP_INTRA, gm.x, 2, a.terms + M_INTRA_E))) return rc;
-        if (c->mp.surface) {
-            hipLaunchKernelGGL(k_mp_surface, ga, b256, 0, c->stream, a, 2.0 * PI * (1.0 / a.volume) / 3.0);
-            if ((rc = red_finish(c, RS_MP_SURF, ga.x, 9, a.terms + M_SU_VIR))) return rc;
-        }
-        if (c->mp.torques && (rc = launch_torques(c, a, gm, c->mp.surface ? 7u : 3u, RS_MP_DUDV))) return rc;
-    }
+    if ((rc = mp_short_enqueue(c, a))) return rc;
     if (fork) polar_join(c);
     else if ((rc = mp_long_sums(c, a, true))) return rc;
     {
@@ -1807,25 +1927,67 @@ int induction_enqueue(Ctx* c, double* terms) {
     int rc_long = fork ? mp_long_sums(c, a, false) : SMD_OK;
     if (fork) polar_fork_end(c);
     if (rc_long) return rc_long;
-    {
-        ProfScope ps(c, "ind_short");
-        hipLaunchKernelGGL(k_ind_site_sums, ga, b256, 0, c->stream, a);
-        if ((rc = site_sums_finish(c, a, ga.x, true))) return rc;
-        if ((rc = launch_real(c, a, true))) return rc;
-        hipLaunchKernelGGL(k_ind_intra, gm, b256, 0, c->stream, a);
-        if ((rc = red_finish(c, RS_IND_INTRA, gm.x, 2, a.terms + M_INTRA_E))) return rc;
-        if (c->indf.surface) {
-            hipLaunchKernelGGL(k_ind_surface, ga, b256, 0, c->stream, a, 2.0 * PI * (1.0 / a.volume) / 3.0);
-            if ((rc = red_finish(c, RS_IND_SURF, ga.x, 9, a.terms + M_SU_VIR))) return rc;
-        }
-        if (c->indf.torques && (rc = launch_torques(c, a, gm, 16u | 3u | (c->indf.surface ? 4u : 0u), RS_IND_DUDV))) return rc;
-    }
+    if ((rc = ind_short_enqueue(c, a))) return rc;
     if (fork) polar_join(c);
     else if ((rc = mp_long_sums(c, a, false))) return rc;
     {
         ProfScope ps(c, "ind_long");
         hipLaunchKernelGGL(k_kforce_sum, ga, b256, 0, c->stream, a);
         if (c->indf.torques && (rc = launch_torques(c, a, gm, 8u, RS_IND_DUDV_L))) return rc;
+    }
+    SMD_HIP(c, hipGetLastError());
+    return SMD_OK;
+}
+
+// The joint pass applies when both classes share the k set and the torque
+// switch (and the tuning switch allows it).
+bool polar_joint_long(Ctx* c) {
+    const Ctx::KSet &m = c->mp_ks, &i = c->ind_ks;
+    return c->polar_joint && c->mp_ready && c->indf_ready && m.K == i.K && m.alpha == i.alpha && m.kcut == i.kcut &&
+           m.chunks == i.chunks && m.ksplit == i.ksplit && (c->mp.torques != 0) == (c->indf.torques != 0);
+}
+
+// Both classes after the solve: the joint k-space pass (structure factors with
+// the converged dipoles, both classes' per-k terms, the joint force pass) on
+// the second stream beside both classes' short-range kernels, then the k-space
+// forces and torques once.  The k-space torque's volume-derivative share is
+// booked to Induction's terms (the step reports the sum).
+int polar_joint_enqueue(Ctx* c, double* mp_t, double* ind_t) {
+    int rc;
+    MpArgs am = mp_args(c), ai = ind_args(c);
+    am.terms = mp_t;
+    ai.terms = ind_t;
+    const dim3 b256(256), ga((c->n + 255) / 256), gm((c->nmol + 255) / 256);
+    hipLaunchKernelGGL(k_zero_terms, dim3(1), b256, 0, c->stream, mp_t);
+    hipLaunchKernelGGL(k_zero_terms, dim3(1), b256, 0, c->stream, ind_t);
+    auto long_sums = [&]() -> int {
+        ProfScope ps(c, "ind_long");
+        const dim3 g1((ai.K + MP_SF_THREADS - 1) / MP_SF_THREADS, ai.chunks);
+        const dim3 gk((c->n + WAVE - 1) / WAVE, ai.ksplit), gf((ai.K + 3) / 4);
+        const bool mixed = c->mp_prec == SMD_PREC_MIXED;
+        if (mixed) hipLaunchKernelGGL((k_ind_sfac<float>), g1, dim3(MP_SF_THREADS), 0, c->stream, ai);
+        else hipLaunchKernelGGL((k_ind_sfac<double>), g1, dim3(MP_SF_THREADS), 0, c->stream, ai);
+        hipLaunchKernelGGL(k_joint_kfinish, gf, b256, 0, c->stream, am, ai);
+        int r = red_finish(c, RS_MP_LONG, gf.x, 9, mp_t + M_LONG_E);
+        if (!r) r = red_finish(c, RS_IND_LONG, gf.x, 9, ind_t + M_LONG_E);
+        if (r) return r;
+        if (mixed) hipLaunchKernelGGL((k_joint_kforce<float>), gk, dim3(WAVE), 0, c->stream, ai);
+        else hipLaunchKernelGGL((k_joint_kforce<double>), gk, dim3(WAVE), 0, c->stream, ai);
+        SMD_HIP(c, hipGetLastError());
+        return SMD_OK;
+    };
+    const bool fork = polar_fork_begin(c);
+    const int rc_long = fork ? long_sums() : SMD_OK;
+    if (fork) polar_fork_end(c);
+    if (rc_long) return rc_long;
+    if ((rc = mp_short_enqueue(c, am))) return rc;
+    if ((rc = ind_short_enqueue(c, ai))) return rc;
+    if (fork) polar_join(c);
+    else if ((rc = long_sums())) return rc;
+    {
+        ProfScope ps(c, "ind_long");
+        hipLaunchKernelGGL(k_kforce_sum, ga, b256, 0, c->stream, ai);
+        if (c->indf.torques && (rc = launch_torques(c, ai, gm, 8u, RS_IND_DUDV_L))) return rc;
     }
     SMD_HIP(c, hipGetLastError());
     return SMD_OK;

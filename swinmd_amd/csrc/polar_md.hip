@@ -10,16 +10,20 @@
 //
 // A step is three parts on the context stream:
 //   A  the Verlet trigger and drift partials (cleared forces and terms), the
-//      gated list rebuild, vdW, the lab frames, Multipole's short- and
-//      long-range kernels, then the solve's field, coupling table, start,
-//      A x and residual;
+//      gated list rebuild, vdW, the lab frames, then the solve's field,
+//      coupling table, start, A x and residual;
 //   B  the solve's iterations in captured batches; the host reads the
 //      device's stop flag between batches (the first read comes after as
 //      many batches as the previous step needed);
-//   C  Induction's short- and long-range kernels with the converged dipoles,
-//      then SHAKE with the kinetic and pressure sums and the NH update.
-// In A and C the k-space sums and per-site partial forces run on a second
-// stream beside the real-space kernels (the recip_overlap tuning switch).
+//   C  Multipole's and Induction's short-range kernels and one k-space pass
+//      for both with the converged dipoles (the joint pass: both classes on
+//      the same k set), then SHAKE with the kinetic and pressure sums and the
+//      NH update.
+// A second stream carries vdW in A and the k-space pass in C, beside the
+// other kernels (the recip_overlap tuning switch).  Without the joint pass
+// (classes on different k sets, or the polar_joint_long switch off),
+// Multipole's kernels run in A with their own k-space pass on the second
+// stream, and C holds Induction's.
 // A and C hold no host synchronisation and no memset: each is captured once
 // as a hipGraph and replayed.
 #include <algorithm>
@@ -40,20 +44,33 @@ static int part_a(Ctx* c, bool on_device_rebuild) {
     RedDefer defer(c);   // one kernel adds the part's reductions at its end
     int rc = md_head_enqueue(c);
     if (!rc) rc = pairlist_build(c, p.cutoff, p.cut_buff, p.use_cells, on_device_rebuild);
-    if (!rc) rc = vdw_enqueue(c, &c->pmd_vdw, p.prec);
-    if (!rc) rc = frames_run(c);
-    if (!rc) rc = multipole_enqueue(c, mp_terms(c));
-    if (!rc) rc = induce_begin(c, &p.induce, false);
+    if (rc) return rc;
+    if (polar_joint_long(c)) {
+        // vdW on the second stream beside the frames and the solve's start;
+        // both classes' Multipole and Induction kernels run in part C
+        const bool fork = polar_fork_begin(c);
+        rc = vdw_enqueue(c, &c->pmd_vdw, p.prec);
+        if (fork) polar_fork_end(c);
+        if (!rc) rc = frames_run(c);
+        if (!rc) rc = induce_begin(c, &p.induce, false);
+        if (fork) polar_join(c);
+    } else {
+        rc = vdw_enqueue(c, &c->pmd_vdw, p.prec);
+        if (!rc) rc = frames_run(c);
+        if (!rc) rc = multipole_enqueue(c, mp_terms(c));
+        if (!rc) rc = induce_begin(c, &p.induce, false);
+    }
     if (!rc) rc = red_flush(c);
     return rc;
 }
 
-// Part C: induction terms and the integration tail.
+// Part C: induction terms (with the joint pass, both classes) and the
+// integration tail.
 static int part_c(Ctx* c) {
     int rc;
     {
         RedDefer defer(c);
-        rc = induction_enqueue(c, ind_terms(c));
+        rc = polar_joint_long(c) ? polar_joint_enqueue(c, mp_terms(c), ind_terms(c)) : induction_enqueue(c, ind_terms(c));
         if (!rc) rc = red_flush(c);
     }
     if (!rc) rc = md_tail_enqueue(c);
@@ -194,7 +211,9 @@ extern "C" int smd_polar_md_init(smd_ctx* c, const smd_polar_md_params* p) {
     int iters = 0;
     double res = 0.0;
     if ((rc = solve(c, 1, &iters, &res))) return rc;
-    if ((rc = induction_enqueue(c, ind_terms(c)))) return rc;
+    if (polar_joint_long(c)) rc = polar_joint_enqueue(c, mp_terms(c), ind_terms(c));
+    else rc = induction_enqueue(c, ind_terms(c));
+    if (rc) return rc;
     if ((rc = md_start_vtm(c))) return rc;
     SMD_HIP(c, hipStreamSynchronize(c->stream));
     SMD_HIP(c, hipMemsetAsync(c->d_disp, 0, sizeof(double) * 3 * c->n, c->stream));

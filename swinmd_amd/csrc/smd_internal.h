@@ -234,6 +234,7 @@ struct Ctx {
         double* kf = nullptr;       // [K][4] per-k force-pass coefficients
         int ksplit = 1;             // k-ranges of the force pass
         double* fpart = nullptr;    // [ksplit][n][6] force and torque per k-range
+        double alpha = 0, kcut = 0;   // the set is fixed by these and the box
     };
     bool mp_ready = false, indf_ready = false;
     smd_mp_params mp{};
@@ -261,6 +262,7 @@ struct Ctx {
     bool recip_store = false;         // k_recip_force stores into d_frecip instead of adding
     bool shake_frecip = false;        // k_shake adds d_frecip to the forces it reads
     int recip_overlap = 1;
+    int polar_joint = 1;            // one k-space pass for Multipole and Induction in the polarizable step
     int md_fused_shake = 1;           // the MD step's SHAKE and kinetics in one kernel
 
     Prof prof;
@@ -318,6 +320,11 @@ int frames_run(Ctx* c);
 constexpr int POLAR_TERMS = 128;   // doubles of one multipole/induction accumulator (>= M_COUNT)
 int multipole_enqueue(Ctx* c, double* terms);
 int induction_enqueue(Ctx* c, double* terms);
+bool polar_joint_long(Ctx* c);
+int polar_joint_enqueue(Ctx* c, double* mp_terms, double* ind_terms);
+bool polar_fork_begin(Ctx* c);
+void polar_fork_end(Ctx* c);
+void polar_join(Ctx* c);
 int polar_classes_check(Ctx* c);
 int polar_terms_energies(Ctx* c, const double* mp_t, const double* ind_t, double* e);
 int induce_direct_apply(Ctx* c, const smd_induce_params* p, double* x, double* z);

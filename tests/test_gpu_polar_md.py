@@ -33,12 +33,15 @@ def _polar_md(fx, ctx, use_graph, **induce_kw):
     return ind, md
 
 
-@pytest.mark.parametrize("use_graph", [False, True])
-def test_polar_md_matches_reference(use_graph):
+@pytest.mark.parametrize("use_graph,joint", [(False, 1), (True, 1), (True, 0)])
+def test_polar_md_matches_reference(use_graph, joint):
     """start() + 10 steps: buffered 14-7, Multipole, Induction (the
-    reference's CG from Pi = 1 to |r| < 1e-14), SHAKE, Nose-Hoover."""
+    reference's CG from Pi = 1 to |r| < 1e-14), SHAKE, Nose-Hoover; with
+    one k-space pass for both classes after the solve (joint) and with a
+    pass per class."""
     fx = Fixture("amoeba216_md10_polar")
     ctx = _ctx(fx.system)
+    ctx.set_tuning("polar_joint_long", joint)
     ctx.set_velocities(fx.vec("init_vel"))
     _, md = _polar_md(fx, ctx, use_graph)
     md.init()
