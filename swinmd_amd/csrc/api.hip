@@ -156,6 +156,7 @@ static void release(Ctx* c) {
     if (c->cg_exec) (void)hipGraphExecDestroy(c->cg_exec);
     if (c->pmd_a) (void)hipGraphExecDestroy(c->pmd_a);
     if (c->pmd_c) (void)hipGraphExecDestroy(c->pmd_c);
+    if (c->pmd_m) (void)hipGraphExecDestroy(c->pmd_m);
     if (c->graph) (void)hipGraphDestroy(c->graph);
     for (auto& r : c->prof.pending) { (void)hipEventDestroy(r.a); (void)hipEventDestroy(r.b); }
     if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);
@@ -432,6 +433,7 @@ extern "C" int smd_set_tuning(smd_ctx* c, const char* name, int value) {
     if (c->graph) { (void)hipGraphDestroy(c->graph); c->graph = nullptr; }
     if (c->pmd_a) { (void)hipGraphExecDestroy(c->pmd_a); c->pmd_a = nullptr; }
     if (c->pmd_c) { (void)hipGraphExecDestroy(c->pmd_c); c->pmd_c = nullptr; }
+    if (c->pmd_m) { (void)hipGraphExecDestroy(c->pmd_m); c->pmd_m = nullptr; }
     return SMD_OK;
 }
 

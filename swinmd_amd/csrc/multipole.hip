@@ -1947,18 +1947,30 @@ bool polar_joint_long(Ctx* c) {
            m.chunks == i.chunks && m.ksplit == i.ksplit && (c->mp.torques != 0) == (c->indf.torques != 0);
 }
 
+// Multipole's short-range kernels of the polarizable step with its terms
+// cleared first: they need neither the dipoles nor the k-space pass, so the
+// step can run them beside the solve.
+int polar_mp_short_enqueue(Ctx* c, double* mp_t) {
+    MpArgs am = mp_args(c);
+    am.terms = mp_t;
+    hipLaunchKernelGGL(k_zero_terms, dim3(1), dim3(256), 0, c->stream, mp_t);
+    return mp_short_enqueue(c, am);
+}
+
 // Both classes after the solve: the joint k-space pass (structure factors with
 // the converged dipoles, both classes' per-k terms, the joint force pass) on
-// the second stream beside both classes' short-range kernels, then the k-space
-// forces and torques once.  The k-space torque's volume-derivative share is
-// booked to Induction's terms (the step reports the sum).
-int polar_joint_enqueue(Ctx* c, double* mp_t, double* ind_t) {
+// the second stream beside the short-range kernels (Induction's, and
+// Multipole's when with_mp_short; otherwise polar_mp_short_enqueue ran them
+// and cleared Multipole's terms), then the k-space forces and torques once.
+// The k-space torque's volume-derivative share is booked to Induction's
+// terms (the step reports the sum).
+int polar_joint_enqueue(Ctx* c, double* mp_t, double* ind_t, bool with_mp_short) {
     int rc;
     MpArgs am = mp_args(c), ai = ind_args(c);
     am.terms = mp_t;
     ai.terms = ind_t;
     const dim3 b256(256), ga((c->n + 255) / 256), gm((c->nmol + 255) / 256);
-    hipLaunchKernelGGL(k_zero_terms, dim3(1), b256, 0, c->stream, mp_t);
+    if (with_mp_short) hipLaunchKernelGGL(k_zero_terms, dim3(1), b256, 0, c->stream, mp_t);
     hipLaunchKernelGGL(k_zero_terms, dim3(1), b256, 0, c->stream, ind_t);
     auto long_sums = [&]() -> int {
         ProfScope ps(c, "ind_long");
@@ -1980,7 +1992,7 @@ int polar_joint_enqueue(Ctx* c, double* mp_t, double* ind_t) {
     const int rc_long = fork ? long_sums() : SMD_OK;
     if (fork) polar_fork_end(c);
     if (rc_long) return rc_long;
-    if ((rc = mp_short_enqueue(c, am))) return rc;
+    if (with_mp_short && (rc = mp_short_enqueue(c, am))) return rc;
     if ((rc = ind_short_enqueue(c, ai))) return rc;
     if (fork) polar_join(c);
     else if ((rc = long_sums())) return rc;

@@ -15,17 +15,19 @@
 //   B  the solve's iterations in captured batches; the host reads the
 //      device's stop flag between batches (the first read comes after as
 //      many batches as the previous step needed);
-//   C  Multipole's and Induction's short-range kernels and one k-space pass
-//      for both with the converged dipoles (the joint pass: both classes on
-//      the same k set), then SHAKE with the kinetic and pressure sums and the
-//      NH update.
-// A second stream carries vdW in A and the k-space pass in C, beside the
-// other kernels (the recip_overlap tuning switch).  Without the joint pass
+//   M  Multipole's short-range kernels, on the second stream beside B;
+//   C  Induction's short-range kernels and one k-space pass for both classes
+//      with the converged dipoles (the joint pass: both classes on the same
+//      k set), then SHAKE with the kinetic and pressure sums and the NH
+//      update.
+// The second stream also carries vdW in A and the k-space pass in C, beside
+// the other kernels (the recip_overlap tuning switch; without it M's kernels
+// run in C on the step's stream).  Without the joint pass
 // (classes on different k sets, or the polar_joint_long switch off),
 // Multipole's kernels run in A with their own k-space pass on the second
 // stream, and C holds Induction's.
-// A and C hold no host synchronisation and no memset: each is captured once
-// as a hipGraph and replayed.
+// A, M and C hold no host synchronisation and no memset: each is captured
+// once as a hipGraph and replayed.
 #include <algorithm>
 #include <cstring>
 #include <vector>
@@ -64,13 +66,25 @@ static int part_a(Ctx* c, bool on_device_rebuild) {
     return rc;
 }
 
-// Part C: induction terms (with the joint pass, both classes) and the
-// integration tail.
+// Multipole's short range beside the solve: with the joint pass and a second
+// stream, part M runs on the second stream between A and C.
+static bool split_mp(Ctx* c) { return polar_joint_long(c) && c->recip_overlap && c->stream2; }
+
+static int part_m(Ctx* c) {
+    RedDefer defer(c);
+    int rc = polar_mp_short_enqueue(c, mp_terms(c));
+    if (!rc) rc = red_flush(c);
+    return rc;
+}
+
+// Part C: induction terms (with the joint pass, both classes; Multipole's
+// short range unless part M ran it) and the integration tail.
 static int part_c(Ctx* c) {
     int rc;
     {
         RedDefer defer(c);
-        rc = polar_joint_long(c) ? polar_joint_enqueue(c, mp_terms(c), ind_terms(c)) : induction_enqueue(c, ind_terms(c));
+        rc = polar_joint_long(c) ? polar_joint_enqueue(c, mp_terms(c), ind_terms(c), !split_mp(c))
+                                 : induction_enqueue(c, ind_terms(c));
         if (!rc) rc = red_flush(c);
     }
     if (!rc) rc = md_tail_enqueue(c);
@@ -126,7 +140,33 @@ static int solve(Ctx* c, int first, int* iters, double* res) {
 
 static void free_graphs(Ctx* c) {
     if (c->pmd_a) { (void)hipGraphExecDestroy(c->pmd_a); c->pmd_a = nullptr; }
+    if (c->pmd_m) { (void)hipGraphExecDestroy(c->pmd_m); c->pmd_m = nullptr; }
     if (c->pmd_c) { (void)hipGraphExecDestroy(c->pmd_c); c->pmd_c = nullptr; }
+}
+
+// Part M on the second stream (captured there, or launched directly): it
+// waits for part A and part C waits for it.
+static int launch_m(Ctx* c, bool graph) {
+    SMD_HIP(c, hipEventRecord(c->ev_fork, c->stream));
+    SMD_HIP(c, hipStreamWaitEvent(c->stream2, c->ev_fork, 0));
+    if (graph) {
+        if (!c->pmd_m) {
+            hipStream_t main = c->stream;
+            c->stream = c->stream2;
+            const int rc = capture(c, &c->pmd_m, part_m);
+            c->stream = main;
+            if (rc) return rc;
+        }
+        SMD_HIP(c, hipGraphLaunch(c->pmd_m, c->stream2));
+    } else {
+        hipStream_t main = c->stream;
+        c->stream = c->stream2;
+        const int rc = part_m(c);
+        c->stream = main;
+        if (rc) return rc;
+    }
+    SMD_HIP(c, hipEventRecord(c->ev_join, c->stream2));
+    return SMD_OK;
 }
 
 }  // namespace smd
@@ -211,7 +251,7 @@ extern "C" int smd_polar_md_init(smd_ctx* c, const smd_polar_md_params* p) {
     int iters = 0;
     double res = 0.0;
     if ((rc = solve(c, 1, &iters, &res))) return rc;
-    if (polar_joint_long(c)) rc = polar_joint_enqueue(c, mp_terms(c), ind_terms(c));
+    if (polar_joint_long(c)) rc = polar_joint_enqueue(c, mp_terms(c), ind_terms(c), true);
     else rc = induction_enqueue(c, ind_terms(c));
     if (rc) return rc;
     if ((rc = md_start_vtm(c))) return rc;
@@ -238,14 +278,17 @@ extern "C" int smd_polar_md_run(smd_ctx* c, int64_t nsteps, smd_polar_md_report*
     int64_t total_iters = 0;
     int iters = 0;
     double res = 0.0;
+    const bool split = split_mp(c);
     for (int64_t s = 0; s < nsteps; ++s) {
         if (graph) SMD_HIP(c, hipGraphLaunch(c->pmd_a, c->stream));
         else if ((rc = part_a(c, true))) return rc;
+        if (split && (rc = launch_m(c, graph))) return rc;
         // the first check after the batches the previous step needed
         const int first = (c->pmd_last_iters + 1 + c->cg_batch - 1) / c->cg_batch;
         if ((rc = solve(c, first, &iters, &res))) return rc;
         c->pmd_last_iters = iters;
         total_iters += iters;
+        if (split) SMD_HIP(c, hipStreamWaitEvent(c->stream, c->ev_join, 0));
         if (graph) SMD_HIP(c, hipGraphLaunch(c->pmd_c, c->stream));
         else if ((rc = part_c(c))) return rc;
         c->md_step++;

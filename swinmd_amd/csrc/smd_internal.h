@@ -295,6 +295,7 @@ struct Ctx {
     std::vector<double> pmd_tab[4];     // sigma2, eps4, type sigma, type eps
     double* d_polar_terms = nullptr;    // [2][POLAR_TERMS]: Multipole's, Induction's accumulators
     hipGraphExec_t pmd_a = nullptr;     // captured forces up to the solve
+    hipGraphExec_t pmd_m = nullptr;     // captured Multipole short range (second stream, beside the solve)
     hipGraphExec_t pmd_c = nullptr;     // captured induction terms and integration tail
     bool pmd_ready = false;
     int pmd_last_iters = 0;             // the previous step's solve iterations
@@ -321,7 +322,8 @@ constexpr int POLAR_TERMS = 128;   // doubles of one multipole/induction accumul
 int multipole_enqueue(Ctx* c, double* terms);
 int induction_enqueue(Ctx* c, double* terms);
 bool polar_joint_long(Ctx* c);
-int polar_joint_enqueue(Ctx* c, double* mp_terms, double* ind_terms);
+int polar_joint_enqueue(Ctx* c, double* mp_terms, double* ind_terms, bool with_mp_short);
+int polar_mp_short_enqueue(Ctx* c, double* mp_terms);
 bool polar_fork_begin(Ctx* c);
 void polar_fork_end(Ctx* c);
 void polar_join(Ctx* c);
