@@ -267,6 +267,8 @@ def bench_polar_md(args):
         ctx.set_velocities(vel)
         ctx.set_pairlist(POLAR["cutoff"], POLAR["cut_buff"], True)
         ctx.set_tuning("cg_batch", cg_batch)
+        if args.no_graph:
+            ctx.set_tuning("cg_graph", 0)   # direct launches throughout (profilers)
         if args.no_overlap:
             ctx.set_tuning("recip_overlap", 0)
         if args.no_joint:

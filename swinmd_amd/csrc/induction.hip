@@ -32,7 +32,8 @@ namespace smd {
 
 constexpr int IND_LANES = 16;      // lanes per atom row in the field and T.mu kernels
 constexpr int CG_THREADS = 256;    // threads per block of the vector kernels
-constexpr int BLK_THREADS = 64;    // one wave per block: the block preconditioner's vector kernels (thread per molecule)
+constexpr int BLK_MOLS = 64;       // molecules per block of the block preconditioner's vector kernels
+constexpr int BLK_THREADS = 9 * BLK_MOLS;   // a thread per dipole component (nine waves)
 constexpr double SQRT_COULOMB = 11.78708976;   // SQRTCOULOMBCONSTANT, src/NEMD_defs.h:103
 constexpr double POLMIN = 0.00000001;          // Induction.cpp:195
 
@@ -752,7 +753,7 @@ struct CgArgs {
     float4* v32;      // mixed solve: FP32 copy of the direction (else null)
     double inv_npol;  // 1 / (polarizable sites), for the RMS dipole residual
     int nmol;         // block preconditioner: 3-site molecules, sites 3m..3m+2
-    const double* blk;  // [nmol][45]: Cholesky factor of each molecule's 9x9 block of A (else null)
+    const double* blk;  // [nmol][45]: inverse of each molecule's 9x9 block of A, lower triangle (else null)
 };
 
 // x = guess (1 as the reference, or the caller's), then after z = A x:
@@ -903,37 +904,19 @@ __global__ __launch_bounds__(CG_THREADS) void k_pcg_dir(CgArgs g) {
     }
 }
 
-// Two or three totals in block_total's order by one wave (the kernels of the
-// block preconditioner run one wave per block): the buffers' loads
-// interleave, each buffer's additions keep the order k, k + WAVE, ...
-template <int NB>
-__device__ __forceinline__ void wave_totals(const double* p0, int n0, const double* p1, int n1, const double* p2,
-                                            int n2, double (&out)[NB]) {
-    static_assert(NB >= 2 && NB <= 3, "two or three totals");
-    double s0 = 0.0, s1 = 0.0, s2 = 0.0;
-    const int nmax = max(n0, max(n1, NB == 3 ? n2 : 0));
-    for (int k = threadIdx.x % WAVE; k < nmax; k += WAVE) {
-        const double a0 = k < n0 ? p0[k] : 0.0, a1 = k < n1 ? p1[k] : 0.0;
-        const double a2 = (NB == 3 && k < n2) ? p2[k] : 0.0;
-        if (k < n0) s0 += a0;
-        if (k < n1) s1 += a1;
-        if (NB == 3 && k < n2) s2 += a2;
-    }
-    out[0] = wave_sum(s0);
-    out[1] = wave_sum(s1);
-    if constexpr (NB == 3) out[2] = wave_sum(s2);
-}
-
 // ---- block-Jacobi preconditioner ---------------------------------------------------
 // M^-1 = the inverse of each molecule's 9x9 diagonal block of A (its three
 // sites' 1/alpha and the intramolecular Thole tensors, which A couples),
-// held as the block's Cholesky factor L (row-major lower triangle, 45
-// doubles) and applied as L^-T L^-1 r.  The stopping test stays the RMS of
-// alpha r.  One thread per molecule in the vector kernels.
+// formed once per solve from the block's Cholesky factor (M^-1 = L^-T L^-1)
+// and held as its lower triangle (45 doubles, row-major).  The vector
+// kernels run a thread per dipole component, BLK_MOLS molecules per block:
+// each component's z = M^-1 r is one row of the block times the molecule's
+// r, staged in LDS.  The stopping test stays the RMS of alpha r.
 
 __device__ __forceinline__ int tri(int i, int j) { return i * (i + 1) / 2 + j; }
+__device__ __forceinline__ int stri(int i, int j) { return i >= j ? tri(i, j) : tri(j, i); }
 
-// Assembles and factors each molecule's block: thread per molecule.
+// Assembles, factors and inverts each molecule's block: thread per molecule.
 __global__ __launch_bounds__(256) void k_block_pc(IndArgs a, double* __restrict__ blk, int nmol) {
     const int m = blockIdx.x * blockDim.x + threadIdx.x;
     if (m >= nmol) return;
@@ -985,62 +968,135 @@ __global__ __launch_bounds__(256) void k_block_pc(IndArgs a, double* __restrict_
             B[tri(i, j)] = v * inv;
         }
     }
+    // Li = L^-1 (lower triangular), then M^-1 = Li^T Li
+    double Li[45];
+#pragma unroll
+    for (int j = 0; j < 9; ++j) {
+        Li[tri(j, j)] = 1.0 / B[tri(j, j)];
+#pragma unroll
+        for (int i = j + 1; i < 9; ++i) {
+            double v = 0.0;
+#pragma unroll
+            for (int k = j; k < i; ++k) v -= B[tri(i, k)] * Li[tri(k, j)];
+            Li[tri(i, j)] = v / B[tri(i, i)];
+        }
+    }
     double* out = blk + 45 * (size_t)m;
 #pragma unroll
-    for (int k = 0; k < 45; ++k) out[k] = B[k];
+    for (int i = 0; i < 9; ++i)
+#pragma unroll
+        for (int j = 0; j <= i; ++j) {
+            double v = 0.0;
+#pragma unroll
+            for (int k = i; k < 9; ++k) v += Li[tri(k, i)] * Li[tri(k, j)];
+            out[tri(i, j)] = v;
+        }
 }
 
-// z = L^-T L^-1 r for one molecule's 9 components.
-__device__ __forceinline__ void block_solve(const double* __restrict__ Lm, const double (&r)[9], double (&z)[9]) {
-    double L[45];
+// Block totals of the vector kernels (BLK_THREADS threads): wave sums, then
+// the waves in order by one thread.
+__device__ __forceinline__ void blk_partial(double v, double* part) {
+    __shared__ double red[BLK_THREADS / WAVE];
+    v = wave_sum(v);
+    if (threadIdx.x % WAVE == 0) red[threadIdx.x / WAVE] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double s = 0.0;
 #pragma unroll
-    for (int k = 0; k < 45; ++k) L[k] = Lm[k];
-    double y[9];
-#pragma unroll
-    for (int i = 0; i < 9; ++i) {
-        double v = r[i];
-#pragma unroll
-        for (int k = 0; k < i; ++k) v -= L[tri(i, k)] * y[k];
-        y[i] = v / L[tri(i, i)];
-    }
-#pragma unroll
-    for (int i = 8; i >= 0; --i) {
-        double v = y[i];
-#pragma unroll
-        for (int k = i + 1; k < 9; ++k) v -= L[tri(k, i)] * z[k];
-        z[i] = v / L[tri(i, i)];
+        for (int w = 0; w < BLK_THREADS / WAVE; ++w) s += red[w];
+        part[blockIdx.x] = s;
     }
 }
 
-__device__ __forceinline__ double mol_alpha(const CgArgs& g, int m, int c) { return site_alpha(g.polar, 9 * m + c); }
+// This thread's component: molecule m (invalid past nmol), component c, the
+// molecule's slot in the block.
+struct BlkSlot { int m, c, ml, k; bool ok; };
+__device__ __forceinline__ BlkSlot blk_slot(const CgArgs& g) {
+    BlkSlot b;
+    b.ml = threadIdx.x / 9;
+    b.c = threadIdx.x % 9;
+    b.m = blockIdx.x * BLK_MOLS + b.ml;
+    b.k = 9 * b.m + b.c;
+    b.ok = b.m < g.nmol;
+    return b;
+}
+
+// z_c = (M^-1 r)_c for this thread's component: rs holds the block's r.
+__device__ __forceinline__ double blk_apply(const CgArgs& g, const BlkSlot& b, const double (*rs)[9]) {
+    const double* Mi = g.blk + 45 * (size_t)b.m;
+    double z = 0.0;
+#pragma unroll
+    for (int j = 0; j < 9; ++j) z += Mi[stri(b.c, j)] * rs[b.ml][j];
+    return z;
+}
 
 // After z = A x: r = E0 - A x, p = M^-1 r; partials of r.M^-1 r and |alpha r|^2.
 __global__ __launch_bounds__(BLK_THREADS) void k_pcg_start_blk(CgArgs g) {
-    const int m = blockIdx.x * blockDim.x + threadIdx.x;
-    double rz = 0.0, zz = 0.0;
-    if (m < g.nmol) {
-        double r[9], z[9];
-#pragma unroll
-        for (int c = 0; c < 9; ++c) {
-            const int k = 9 * m + c;
-            r[c] = g.e0[k] - g.z[k];
-            g.r[k] = r[c];
-            const double ar = mol_alpha(g, m, c) * r[c];
-            zz += ar * ar;
-        }
-        block_solve(g.blk + 45 * (size_t)m, r, z);
-#pragma unroll
-        for (int c = 0; c < 9; ++c) {
-            const int k = 9 * m + c;
-            g.v[k] = z[c];
-            store_dir32(g, k, z[c]);
-            rz += r[c] * z[c];
-        }
+    __shared__ double rs[BLK_MOLS][9];
+    const BlkSlot b = blk_slot(g);
+    double rz = 0.0, zz = 0.0, r = 0.0;
+    if (b.ok) {
+        r = g.e0[b.k] - g.z[b.k];
+        g.r[b.k] = r;
+        const double ar = site_alpha(g.polar, b.k) * r;
+        zz = ar * ar;
+        rs[b.ml][b.c] = r;
     }
-    block_partial(rz, g.p_rr0);
     __syncthreads();
-    block_partial(zz, g.p_dv);
+    if (b.ok) {
+        const double z = blk_apply(g, b, rs);
+        g.v[b.k] = z;
+        store_dir32(g, b.k, z);
+        rz = r * z;
+    }
+    blk_partial(rz, g.p_rr0);
+    __syncthreads();
+    blk_partial(zz, g.p_dv);
     if (blockIdx.x == 0 && threadIdx.x == 0) { g.state[0] = 0; g.state[1] = 0; g.state[2] = 0; }
+}
+
+// Two or three totals by the whole block, the same in every block: thread t
+// adds entries t, t + BLK_THREADS, ... (all loads issued before the adds),
+// then the wave sums, then the waves in order.
+template <int NB>
+__device__ __forceinline__ void blk_totals(const double* p0, int n0, const double* p1, int n1, const double* p2,
+                                           int n2, double (&out)[NB]) {
+    static_assert(NB >= 2 && NB <= 3, "two or three totals");
+    constexpr int PER = 4;   // entries per thread and buffer in one pass of loads
+    __shared__ double red[NB][BLK_THREADS / WAVE];
+    const double* ps[3] = {p0, p1, p2};
+    const int ns[3] = {n0, n1, NB == 3 ? n2 : 0};
+    double s[NB];
+#pragma unroll
+    for (int x = 0; x < NB; ++x) s[x] = 0.0;
+    const int nmax = max(n0, max(n1, NB == 3 ? n2 : 0));
+    for (int base = 0; base < nmax; base += PER * BLK_THREADS) {
+        double v[NB][PER];
+#pragma unroll
+        for (int x = 0; x < NB; ++x)
+#pragma unroll
+            for (int u = 0; u < PER; ++u) {
+                const int k = base + u * BLK_THREADS + (int)threadIdx.x;
+                v[x][u] = k < ns[x] ? ps[x][k] : 0.0;
+            }
+#pragma unroll
+        for (int x = 0; x < NB; ++x)
+#pragma unroll
+            for (int u = 0; u < PER; ++u) s[x] += v[x][u];
+    }
+#pragma unroll
+    for (int x = 0; x < NB; ++x) {
+        const double w = wave_sum(s[x]);
+        if (threadIdx.x % WAVE == 0) red[x][threadIdx.x / WAVE] = w;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int x = 0; x < NB; ++x) {
+        double t = 0.0;
+#pragma unroll
+        for (int w = 0; w < BLK_THREADS / WAVE; ++w) t += red[x][w];
+        out[x] = t;
+    }
 }
 
 // t = rz / p.Ap; x += t p; r -= t Ap; z = M^-1 r (into the A v buffer, whose
@@ -1048,34 +1104,31 @@ __global__ __launch_bounds__(BLK_THREADS) void k_pcg_start_blk(CgArgs g) {
 template <int PHASE>
 __global__ __launch_bounds__(BLK_THREADS) void k_pcg_step_blk(CgArgs g) {
     if (g.state[0]) return;
+    __shared__ double rs[BLK_MOLS][9];
     const double* rz_c = PHASE ? g.p_rr1 : g.p_rr0;
     double* rz_d = PHASE ? g.p_rr0 : g.p_rr1;
     double tot[2];   // p.Ap, r.z
-    wave_totals<2>(g.p_vz, g.nparts_mv, rz_c, g.nparts_vec, nullptr, 0, tot);
+    blk_totals<2>(g.p_vz, g.nparts_mv, rz_c, g.nparts_vec, nullptr, 0, tot);
     const double t = tot[1] / tot[0];
-    const int m = blockIdx.x * blockDim.x + threadIdx.x;
-    double rzn = 0.0, zz = 0.0;
-    if (m < g.nmol) {
-        double r[9], z[9];
-#pragma unroll
-        for (int c = 0; c < 9; ++c) {
-            const int k = 9 * m + c;
-            g.x[k] += t * g.v[k];
-            r[c] = g.r[k] - t * g.z[k];
-            g.r[k] = r[c];
-            const double ar = mol_alpha(g, m, c) * r[c];
-            zz += ar * ar;
-        }
-        block_solve(g.blk + 45 * (size_t)m, r, z);
-#pragma unroll
-        for (int c = 0; c < 9; ++c) {
-            g.z[9 * m + c] = z[c];
-            rzn += r[c] * z[c];
-        }
+    const BlkSlot b = blk_slot(g);
+    double rzn = 0.0, zz = 0.0, r = 0.0;
+    if (b.ok) {
+        g.x[b.k] += t * g.v[b.k];
+        r = g.r[b.k] - t * g.z[b.k];
+        g.r[b.k] = r;
+        const double ar = site_alpha(g.polar, b.k) * r;
+        zz = ar * ar;
+        rs[b.ml][b.c] = r;
     }
-    block_partial(rzn, rz_d);
     __syncthreads();
-    block_partial(zz, g.p_dv);
+    if (b.ok) {
+        const double z = blk_apply(g, b, rs);
+        g.z[b.k] = z;
+        rzn = r * z;
+    }
+    blk_partial(rzn, rz_d);
+    __syncthreads();
+    blk_partial(zz, g.p_dv);
     if (blockIdx.x == 0 && threadIdx.x == 0) g.state[1] += 1;
 }
 
@@ -1086,7 +1139,7 @@ __global__ __launch_bounds__(BLK_THREADS) void k_pcg_dir_blk(CgArgs g) {
     const double* rz_c = PHASE ? g.p_rr1 : g.p_rr0;
     const double* rz_d = PHASE ? g.p_rr0 : g.p_rr1;
     double tot[3];   // |alpha r|^2, new r.z, old r.z
-    wave_totals<3>(g.p_dv, g.nparts_vec, rz_d, g.nparts_vec, rz_c, g.nparts_vec, tot);
+    blk_totals<3>(g.p_dv, g.nparts_vec, rz_d, g.nparts_vec, rz_c, g.nparts_vec, tot);
     const double res = sqrt(tot[0] * g.inv_npol);
     if (blockIdx.x == 0 && threadIdx.x == 0) g.scal[0] = res;
     if (res < g.tol) {
@@ -1094,15 +1147,11 @@ __global__ __launch_bounds__(BLK_THREADS) void k_pcg_dir_blk(CgArgs g) {
         return;
     }
     const double beta = tot[1] / tot[2];
-    const int m = blockIdx.x * blockDim.x + threadIdx.x;
-    if (m < g.nmol) {
-#pragma unroll
-        for (int c = 0; c < 9; ++c) {
-            const int k = 9 * m + c;
-            const double v = g.z[k] + beta * g.v[k];
-            g.v[k] = v;
-            store_dir32(g, k, v);
-        }
+    const BlkSlot b = blk_slot(g);
+    if (b.ok) {
+        const double v = g.z[b.k] + beta * g.v[b.k];
+        g.v[b.k] = v;
+        store_dir32(g, b.k, v);
     }
 }
 
@@ -1252,7 +1301,7 @@ static CgArgs cg_args(Ctx* c, const smd_induce_params* p) {
     g.nparts_mv = (int)(((int64_t)n * c->cg_lanes + 255) / 256);   // blocks of the A v kernel
     g.nmol = c->nmol;
     g.blk = p->precond == 2 ? c->d_pc_blk : nullptr;
-    g.nparts_vec = p->precond == 2 ? (c->nmol + BLK_THREADS - 1) / BLK_THREADS : (dim + CG_THREADS - 1) / CG_THREADS;
+    g.nparts_vec = p->precond == 2 ? (c->nmol + BLK_MOLS - 1) / BLK_MOLS : (dim + CG_THREADS - 1) / CG_THREADS;
     g.p_vz = c->d_cg_part;
     g.p_dv = c->d_cg_part + c->cg_parts;
     g.p_rr0 = c->d_cg_part + 2 * c->cg_parts;

@@ -1605,20 +1605,98 @@ __global__ __launch_bounds__(WAVE) void k_ind_kforce(MpArgs a) {
 // with U = b Q + b Pc and W = b P - b Ps (b P, b Q the multipole pass's
 // coefficients; X, Y, b Pc, b Ps the induction pass's).
 
-// One wave per k: both classes' per-k terms from the one set of sums; the
+// The induced dipoles' own sums per (chunk, k), for the joint pass when the
+// permanent multipoles' 20 came from k_mp_sfac (before the solve):
+//   0 Pc  1 Ps  (p.k) cos/sin     2-4 (p.k) cos d     5-7 (p.k) sin d
+constexpr int INP_NSUM = 8;
+
+template <typename R>
+__global__ __launch_bounds__(MP_SF_THREADS) void k_indp_sfac(MpArgs a) {
+    __shared__ double shr[MP_SF_THREADS][3];
+    __shared__ R shm[MP_SF_THREADS][6];
+    const int k = blockIdx.x * MP_SF_THREADS + threadIdx.x;
+    const int chunk = blockIdx.y;
+    const int i0 = chunk * a.per, i1 = min(a.n, i0 + a.per);
+    V3 kv = v3(0, 0, 0);
+    if (k < a.K) kv = ldv(a.kvec + 3 * k);
+    const R kx = (R)kv.x, ky = (R)kv.y, kz = (R)kv.z;
+    double S[INP_NSUM];
+#pragma unroll
+    for (int x = 0; x < INP_NSUM; ++x) S[x] = 0.0;
+    for (int t0 = i0; t0 < i1; t0 += MP_SF_THREADS) {
+        const int i = t0 + threadIdx.x;
+        __syncthreads();
+        if (i < i1) {
+            shr[threadIdx.x][0] = a.pos[3 * i]; shr[threadIdx.x][1] = a.pos[3 * i + 1]; shr[threadIdx.x][2] = a.pos[3 * i + 2];
+            R* p = shm[threadIdx.x];
+            p[0] = (R)a.pind[3 * i]; p[1] = (R)a.pind[3 * i + 1]; p[2] = (R)a.pind[3 * i + 2];
+            const int m = a.mol[i];
+            p[3] = (R)(a.pos[3 * i] - a.com[3 * m]);
+            p[4] = (R)(a.pos[3 * i + 1] - a.com[3 * m + 1]);
+            p[5] = (R)(a.pos[3 * i + 2] - a.com[3 * m + 2]);
+        }
+        __syncthreads();
+        const int cnt = min(MP_SF_THREADS, i1 - t0);
+        if (k < a.K) {
+            auto tile = [&](auto& s) {
+            for (int x = 0; x < cnt; ++x) {
+                const R* p = shm[x];
+                R sn, cs;
+                phase_sincos<R>(shr[x][0] * kv.x + shr[x][1] * kv.y + shr[x][2] * kv.z, sn, cs);
+                const R Pk = p[0] * kx + p[1] * ky + p[2] * kz;
+                const R pc = Pk * cs, ps = Pk * sn;
+                s[0] += pc; s[1] += ps;
+                s[2] += pc * p[3]; s[3] += pc * p[4]; s[4] += pc * p[5];
+                s[5] += ps * p[3]; s[6] += ps * p[4]; s[7] += ps * p[5];
+            }
+            };
+            if constexpr (sizeof(R) == sizeof(double)) {
+                tile(S);
+            } else {
+                R s[INP_NSUM];
+#pragma unroll
+                for (int x = 0; x < INP_NSUM; ++x) s[x] = R(0);
+                tile(s);
+#pragma unroll
+                for (int x = 0; x < INP_NSUM; ++x) S[x] += (double)s[x];
+            }
+        }
+    }
+    if (k < a.K) {
+        double* out = a.part + ((size_t)chunk * a.K + k) * INP_NSUM;
+#pragma unroll
+        for (int x = 0; x < INP_NSUM; ++x) out[x] = S[x];
+    }
+}
+
+// One wave per k: both classes' per-k terms from one set of sums (SPLIT: the
+// permanent multipoles' 20 from the multipole pass's partials and the induced
+// dipoles' 8 from k_indp_sfac's; otherwise all 28 from k_ind_sfac's); the
 // joint force coefficients (X, Y, U, W) replace the induction pass's.
+template <bool SPLIT>
 __global__ __launch_bounds__(256) void k_joint_kfinish(MpArgs mp, MpArgs in) {
     const int k = blockIdx.x * (blockDim.x / WAVE) + threadIdx.x / WAVE;
     double vm[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0}, vi[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
     if (k < in.K) {
-        double S[IN_NSUM];
-        wave_chunk_sum<IN_NSUM>(in, k, S);
-        if (threadIdx.x % WAVE == 0) {
-            double M[MP_NSUM];
+        double S[IN_NSUM], M[MP_NSUM];
+        if constexpr (SPLIT) {
+            double P[INP_NSUM];
+            wave_chunk_sum<MP_NSUM>(mp, k, M);
+            wave_chunk_sum<INP_NSUM>(in, k, P);
+            S[0] = M[0]; S[1] = M[1]; S[2] = M[2]; S[3] = M[3]; S[4] = P[0]; S[5] = P[1]; S[6] = M[4]; S[7] = M[5];
+#pragma unroll
+            for (int x = 0; x < 12; ++x) S[8 + x] = M[6 + x];
+#pragma unroll
+            for (int x = 0; x < 6; ++x) S[20 + x] = P[2 + x];
+            S[26] = M[18]; S[27] = M[19];
+        } else {
+            wave_chunk_sum<IN_NSUM>(in, k, S);
             M[0] = S[0]; M[1] = S[1]; M[2] = S[2]; M[3] = S[3]; M[4] = S[6]; M[5] = S[7];
 #pragma unroll
             for (int x = 0; x < 12; ++x) M[6 + x] = S[8 + x];
             M[18] = S[26]; M[19] = S[27];
+        }
+        if (threadIdx.x % WAVE == 0) {
             mp_finish(mp, k, M, vm);
             ind_finish(in, k, S, vi);
             const double bP = mp.kf[4 * k], bQ = mp.kf[4 * k + 1];
@@ -1948,20 +2026,31 @@ bool polar_joint_long(Ctx* c) {
 }
 
 // Multipole's short-range kernels of the polarizable step with its terms
-// cleared first: they need neither the dipoles nor the k-space pass, so the
-// step can run them beside the solve.
+// cleared first, and the permanent multipoles' structure-factor sums: none
+// needs the dipoles, so the step can run them beside the solve.
 int polar_mp_short_enqueue(Ctx* c, double* mp_t) {
     MpArgs am = mp_args(c);
     am.terms = mp_t;
     hipLaunchKernelGGL(k_zero_terms, dim3(1), dim3(256), 0, c->stream, mp_t);
-    return mp_short_enqueue(c, am);
+    int rc = mp_short_enqueue(c, am);
+    if (rc) return rc;
+    // the permanent multipoles' structure-factor sums (the joint pass's other
+    // 8 per k follow the solve)
+    ProfScope ps(c, "mp_long");
+    const dim3 g1((am.K + MP_SF_THREADS - 1) / MP_SF_THREADS, am.chunks);
+    if (c->mp_prec == SMD_PREC_MIXED) hipLaunchKernelGGL((k_mp_sfac<float>), g1, dim3(MP_SF_THREADS), 0, c->stream, am);
+    else hipLaunchKernelGGL((k_mp_sfac<double>), g1, dim3(MP_SF_THREADS), 0, c->stream, am);
+    SMD_HIP(c, hipGetLastError());
+    return SMD_OK;
 }
 
 // Both classes after the solve: the joint k-space pass (structure factors with
 // the converged dipoles, both classes' per-k terms, the joint force pass) on
 // the second stream beside the short-range kernels (Induction's, and
-// Multipole's when with_mp_short; otherwise polar_mp_short_enqueue ran them
-// and cleared Multipole's terms), then the k-space forces and torques once.
+// Multipole's when with_mp_short; otherwise polar_mp_short_enqueue ran them,
+// cleared Multipole's terms and summed the permanent multipoles' structure
+// factors, so only the induced dipoles' 8 sums per k remain), then the
+// k-space forces and torques once.
 // The k-space torque's volume-derivative share is booked to Induction's
 // terms (the step reports the sum).
 int polar_joint_enqueue(Ctx* c, double* mp_t, double* ind_t, bool with_mp_short) {
@@ -1977,9 +2066,15 @@ int polar_joint_enqueue(Ctx* c, double* mp_t, double* ind_t, bool with_mp_short)
         const dim3 g1((ai.K + MP_SF_THREADS - 1) / MP_SF_THREADS, ai.chunks);
         const dim3 gk((c->n + WAVE - 1) / WAVE, ai.ksplit), gf((ai.K + 3) / 4);
         const bool mixed = c->mp_prec == SMD_PREC_MIXED;
-        if (mixed) hipLaunchKernelGGL((k_ind_sfac<float>), g1, dim3(MP_SF_THREADS), 0, c->stream, ai);
-        else hipLaunchKernelGGL((k_ind_sfac<double>), g1, dim3(MP_SF_THREADS), 0, c->stream, ai);
-        hipLaunchKernelGGL(k_joint_kfinish, gf, b256, 0, c->stream, am, ai);
+        if (with_mp_short) {
+            if (mixed) hipLaunchKernelGGL((k_ind_sfac<float>), g1, dim3(MP_SF_THREADS), 0, c->stream, ai);
+            else hipLaunchKernelGGL((k_ind_sfac<double>), g1, dim3(MP_SF_THREADS), 0, c->stream, ai);
+            hipLaunchKernelGGL(k_joint_kfinish<false>, gf, b256, 0, c->stream, am, ai);
+        } else {
+            if (mixed) hipLaunchKernelGGL((k_indp_sfac<float>), g1, dim3(MP_SF_THREADS), 0, c->stream, ai);
+            else hipLaunchKernelGGL((k_indp_sfac<double>), g1, dim3(MP_SF_THREADS), 0, c->stream, ai);
+            hipLaunchKernelGGL(k_joint_kfinish<true>, gf, b256, 0, c->stream, am, ai);
+        }
         int r = red_finish(c, RS_MP_LONG, gf.x, 9, mp_t + M_LONG_E);
         if (!r) r = red_finish(c, RS_IND_LONG, gf.x, 9, ind_t + M_LONG_E);
         if (r) return r;

@@ -10,19 +10,21 @@
 //
 // A step is three parts on the context stream:
 //   A  the Verlet trigger and drift partials (cleared forces and terms), the
-//      gated list rebuild, vdW, the lab frames, then the solve's field,
-//      coupling table, start, A x and residual;
+//      gated list rebuild, the lab frames, then the solve's field, coupling
+//      table, start, A x and residual;
 //   B  the solve's iterations in captured batches; the host reads the
 //      device's stop flag between batches (the first read comes after as
 //      many batches as the previous step needed);
-//   M  Multipole's short-range kernels, on the second stream beside B;
-//   C  Induction's short-range kernels and one k-space pass for both classes
-//      with the converged dipoles (the joint pass: both classes on the same
-//      k set), then SHAKE with the kinetic and pressure sums and the NH
-//      update.
-// The second stream also carries vdW in A and the k-space pass in C, beside
-// the other kernels (the recip_overlap tuning switch; without it M's kernels
-// run in C on the step's stream).  Without the joint pass
+//   M  vdW, Multipole's short-range kernels and the permanent multipoles'
+//      structure-factor sums, on the second stream beside B;
+//   C  Induction's short-range kernels and the rest of one k-space pass for
+//      both classes with the converged dipoles (the joint pass: both classes
+//      on the same k set), then SHAKE with the kinetic and pressure sums and
+//      the NH update.
+// The second stream also carries the k-space pass in C, beside the other
+// kernels (the recip_overlap tuning switch; without it M's kernels run in C
+// on the step's stream, and the structure factors there are summed in one
+// pass).  Without the joint pass
 // (classes on different k sets, or the polar_joint_long switch off),
 // Multipole's kernels run in A with their own k-space pass on the second
 // stream, and C holds Induction's.
@@ -48,14 +50,9 @@ static int part_a(Ctx* c, bool on_device_rebuild) {
     if (!rc) rc = pairlist_build(c, p.cutoff, p.cut_buff, p.use_cells, on_device_rebuild);
     if (rc) return rc;
     if (polar_joint_long(c)) {
-        // vdW on the second stream beside the frames and the solve's start;
-        // both classes' Multipole and Induction kernels run in part C
-        const bool fork = polar_fork_begin(c);
-        rc = vdw_enqueue(c, &c->pmd_vdw, p.prec);
-        if (fork) polar_fork_end(c);
-        if (!rc) rc = frames_run(c);
+        // vdW and Multipole's short range in part M beside the solve (or in C)
+        rc = frames_run(c);
         if (!rc) rc = induce_begin(c, &p.induce, false);
-        if (fork) polar_join(c);
     } else {
         rc = vdw_enqueue(c, &c->pmd_vdw, p.prec);
         if (!rc) rc = frames_run(c);
@@ -72,7 +69,8 @@ static bool split_mp(Ctx* c) { return polar_joint_long(c) && c->recip_overlap &&
 
 static int part_m(Ctx* c) {
     RedDefer defer(c);
-    int rc = polar_mp_short_enqueue(c, mp_terms(c));
+    int rc = vdw_enqueue(c, &c->pmd_vdw, c->pmd.prec);
+    if (!rc) rc = polar_mp_short_enqueue(c, mp_terms(c));
     if (!rc) rc = red_flush(c);
     return rc;
 }
@@ -83,8 +81,12 @@ static int part_c(Ctx* c) {
     int rc;
     {
         RedDefer defer(c);
-        rc = polar_joint_long(c) ? polar_joint_enqueue(c, mp_terms(c), ind_terms(c), !split_mp(c))
-                                 : induction_enqueue(c, ind_terms(c));
+        if (!polar_joint_long(c)) rc = induction_enqueue(c, ind_terms(c));
+        else if (split_mp(c)) rc = polar_joint_enqueue(c, mp_terms(c), ind_terms(c), false);
+        else {
+            rc = vdw_enqueue(c, &c->pmd_vdw, c->pmd.prec);
+            if (!rc) rc = polar_joint_enqueue(c, mp_terms(c), ind_terms(c), true);
+        }
         if (!rc) rc = red_flush(c);
     }
     if (!rc) rc = md_tail_enqueue(c);
@@ -251,8 +253,12 @@ extern "C" int smd_polar_md_init(smd_ctx* c, const smd_polar_md_params* p) {
     int iters = 0;
     double res = 0.0;
     if ((rc = solve(c, 1, &iters, &res))) return rc;
-    if (polar_joint_long(c)) rc = polar_joint_enqueue(c, mp_terms(c), ind_terms(c), true);
-    else rc = induction_enqueue(c, ind_terms(c));
+    if (polar_joint_long(c)) {
+        rc = vdw_enqueue(c, &c->pmd_vdw, p->prec);
+        if (!rc) rc = polar_joint_enqueue(c, mp_terms(c), ind_terms(c), true);
+    } else {
+        rc = induction_enqueue(c, ind_terms(c));
+    }
     if (rc) return rc;
     if ((rc = md_start_vtm(c))) return rc;
     SMD_HIP(c, hipStreamSynchronize(c->stream));

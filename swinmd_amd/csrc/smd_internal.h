@@ -196,7 +196,7 @@ struct Ctx {
     float4* d_ind_rec32 = nullptr; // mixed solve: hi/lo FP32 site records [2n]
     int4* d_ind_fix = nullptr;     // mixed A v: fixed-point positions and 1/sqrt(alpha) [n]
     double* d_e0_prev = nullptr;   // E0 of the previous solve with guess 4 [3n]
-    double* d_pc_blk = nullptr;    // block preconditioner: Cholesky factors [nmol][45]
+    double* d_pc_blk = nullptr;    // block preconditioner: inverse blocks [nmol][45]
     float* d_shell32 = nullptr;    // the learned guess's charge shells from the mixed field pass [n][24]
     bool ind_shells_valid = false; // d_shell32 holds the shells of the current solve's state
     int mp_prec = 0;               // smd_prec of the multipole / induction real-space pair math
