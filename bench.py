@@ -273,6 +273,8 @@ def bench_polar_md(args):
             ctx.set_tuning("recip_overlap", 0)
         if args.no_joint:
             ctx.set_tuning("polar_joint_long", 0)
+        if args.polar_m_parts is not None:
+            ctx.set_tuning("polar_m_parts", args.polar_m_parts)
         ind = Induction(ctx, cutoff_ind=POLAR["cutoff_ind"], a_damp=POLAR["a_damp"], damp_tij=True, do_min=True,
                         tol=tol, precond=args.precond, guess=guess, prec=prec)
         if net_state is not None:
@@ -622,6 +624,9 @@ def main():
                     help="polarizable workloads: keep the k-space passes on the step's stream")
     ap.add_argument("--no-joint", action="store_true",
                     help="polarizable workloads: a k-space pass per class instead of one joint pass after the solve")
+    ap.add_argument("--polar-m-parts", type=int, default=None, choices=[0, 1, 2, 3],
+                    help="polarizable workloads: what runs beside the dipole solve (1 vdW, 2 the permanent "
+                         "structure factors; default: the library's)")
     ap.add_argument("--no-graph", action="store_true",
                     help="launch each step's kernels directly instead of replaying a captured hipGraph")
     args = ap.parse_args()

@@ -420,6 +420,9 @@ extern "C" int smd_set_tuning(smd_ctx* c, const char* name, int value) {
         if (value != 0 && value != 1) return fail(c, SMD_EINVAL, "cg_graph must be 0 or 1 (got %d)", value);
         c->cg_graph = value;
         if (c->cg_exec) { (void)hipGraphExecDestroy(c->cg_exec); c->cg_exec = nullptr; }
+    } else if (std::string(name) == "polar_m_parts") {
+        if (value < 0 || value > 3) return fail(c, SMD_EINVAL, "polar_m_parts must be 0..3 (got %d)", value);
+        c->polar_m_parts = value;
     } else if (std::string(name) == "polar_joint_long") {
         if (value != 0 && value != 1) return fail(c, SMD_EINVAL, "polar_joint_long must be 0 or 1 (got %d)", value);
         c->polar_joint = value;

@@ -334,12 +334,17 @@ __device__ __forceinline__ void block_partial(double v, double* part) {
 }
 
 // Loop-top test of a CG iteration (:2431-2434), the same in every block:
-// stop when |v| < tol.
+// stop when sqrt(scale * sum of p_dv) < tol (the reference's CG: |v| with
+// scale 1; the single-reduction PCG: the RMS of alpha r, also stored in
+// res[0]).
 struct CgTop {
     int* state;          // [0] done, [2] stop reason; null: plain A v
-    const double* p_dv;  // partials of v.v
+    const double* p_dv;  // partials of the tested sum (null: the done flag only)
     int nparts;
     double tol;
+    double scale = 1.0;
+    double* res = nullptr;
+    int reason = 1;
 };
 
 // Coupling table of a solve.  A is fixed while CG runs, so the Thole factors
@@ -521,18 +526,25 @@ __global__ __launch_bounds__(256) void k_field_table32(IndArgs a, float* __restr
     }
 }
 
+__device__ __forceinline__ bool top_stop(const CgTop& top) {
+    if (!top.state) return false;
+    if (top.state[0]) return true;
+    if (!top.p_dv) return false;
+    const double res = sqrt(block_total(top.p_dv, top.nparts) * top.scale);
+    if (top.res && blockIdx.x == 0 && threadIdx.x == 0) top.res[0] = res;
+    if (res < top.tol) {
+        if (blockIdx.x == 0 && threadIdx.x == 0) { top.state[0] = 1; top.state[2] = top.reason; }
+        return true;
+    }
+    return false;
+}
+
 // A v from the coupling table: (A v)_i = v_i / alpha_i - sum_j (a_ij (r.v_j) r - b_ij v_j).
 template <int G>
 __global__ __launch_bounds__(256) void k_apply_table(IndArgs a, const double* __restrict__ v,
                                                      double* __restrict__ out, const double* __restrict__ dotv,
                                                      double* part, CgTop top) {
-    if (top.state) {
-        if (top.state[0]) return;
-        if (top.p_dv && sqrt(block_total(top.p_dv, top.nparts)) < top.tol) {
-            if (blockIdx.x == 0 && threadIdx.x == 0) { top.state[0] = 1; top.state[2] = 1; }
-            return;
-        }
-    }
+    if (top_stop(top)) return;
     const int tid = blockIdx.x * blockDim.x + threadIdx.x;
     const int i = tid / G, sub = tid % G;
     double ax = 0.0, ay = 0.0, az = 0.0;
@@ -640,7 +652,7 @@ template <int G>
 __global__ __launch_bounds__(256) void k_apply_pos32(IndArgs a, const float4* __restrict__ v32,
                                                      const double* __restrict__ v, double* __restrict__ out,
                                                      double* part, CgTop top) {
-    if (top.state && top.state[0]) return;
+    if (top_stop(top)) return;
     const int tid = blockIdx.x * blockDim.x + threadIdx.x;
     const int i = tid / G, sub = tid % G;
     double ax = 0.0, ay = 0.0, az = 0.0;
@@ -754,6 +766,8 @@ struct CgArgs {
     double inv_npol;  // 1 / (polarizable sites), for the RMS dipole residual
     int nmol;         // block preconditioner: 3-site molecules, sites 3m..3m+2
     const double* blk;  // [nmol][45]: inverse of each molecule's 9x9 block of A, lower triangle (else null)
+    double* u;        // block preconditioner: M^-1 r (A applies to it; z holds A u)
+    double* s;        // block preconditioner: the recurrence for A p
 };
 
 // x = guess (1 as the reference, or the caller's), then after z = A x:
@@ -1030,31 +1044,6 @@ __device__ __forceinline__ double blk_apply(const CgArgs& g, const BlkSlot& b, c
     return z;
 }
 
-// After z = A x: r = E0 - A x, p = M^-1 r; partials of r.M^-1 r and |alpha r|^2.
-__global__ __launch_bounds__(BLK_THREADS) void k_pcg_start_blk(CgArgs g) {
-    __shared__ double rs[BLK_MOLS][9];
-    const BlkSlot b = blk_slot(g);
-    double rz = 0.0, zz = 0.0, r = 0.0;
-    if (b.ok) {
-        r = g.e0[b.k] - g.z[b.k];
-        g.r[b.k] = r;
-        const double ar = site_alpha(g.polar, b.k) * r;
-        zz = ar * ar;
-        rs[b.ml][b.c] = r;
-    }
-    __syncthreads();
-    if (b.ok) {
-        const double z = blk_apply(g, b, rs);
-        g.v[b.k] = z;
-        store_dir32(g, b.k, z);
-        rz = r * z;
-    }
-    blk_partial(rz, g.p_rr0);
-    __syncthreads();
-    blk_partial(zz, g.p_dv);
-    if (blockIdx.x == 0 && threadIdx.x == 0) { g.state[0] = 0; g.state[1] = 0; g.state[2] = 0; }
-}
-
 // Two or three totals by the whole block, the same in every block: thread t
 // adds entries t, t + BLK_THREADS, ... (all loads issued before the adds),
 // then the wave sums, then the waves in order.
@@ -1099,22 +1088,76 @@ __device__ __forceinline__ void blk_totals(const double* p0, int n0, const doubl
     }
 }
 
-// t = rz / p.Ap; x += t p; r -= t Ap; z = M^-1 r (into the A v buffer, whose
-// product is consumed); partials of the new r.z and |alpha r|^2.
+// The block-preconditioned CG in the single-reduction form of Chronopoulos
+// and Gear (one A product and one vector kernel per iteration; in exact
+// arithmetic the iterates of the usual PCG):
+//   u = M^-1 r, w = A u, gamma = r.u, delta = w.u,
+//   beta = gamma / gamma_prev, alpha = gamma / (delta - beta gamma / alpha_prev),
+//   p = u + beta p, s = w + beta s, x += alpha p, r -= alpha s.
+// The A product of u runs the stopping test first (the RMS of alpha r from
+// the vector kernel's partials).  gamma_prev and alpha_prev pass between
+// iterations in scal[2..5] (written by one thread, read from the other half
+// of the ping-pong); the first iteration starts from gamma_prev = inf,
+// p = s = 0.
+
+// After z = A x: r = E0 - A x, u = M^-1 r, p = s = 0; partials of r.u and
+// |alpha r|^2.
+__global__ __launch_bounds__(BLK_THREADS) void k_pcg_start_blk(CgArgs g) {
+    __shared__ double rs[BLK_MOLS][9];
+    const BlkSlot b = blk_slot(g);
+    double ru = 0.0, zz = 0.0, r = 0.0;
+    if (b.ok) {
+        r = g.e0[b.k] - g.z[b.k];
+        g.r[b.k] = r;
+        g.v[b.k] = 0.0;
+        g.s[b.k] = 0.0;
+        const double ar = site_alpha(g.polar, b.k) * r;
+        zz = ar * ar;
+        rs[b.ml][b.c] = r;
+    }
+    __syncthreads();
+    if (b.ok) {
+        const double u = blk_apply(g, b, rs);
+        g.u[b.k] = u;
+        store_dir32(g, b.k, u);
+        ru = r * u;
+    }
+    blk_partial(ru, g.p_rr0);
+    __syncthreads();
+    blk_partial(zz, g.p_dv);
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        g.state[0] = 0; g.state[1] = 0; g.state[2] = 0;
+        g.scal[2] = __longlong_as_double(0x7ff0000000000000ll);   // gamma_prev = +inf: beta = 0
+        g.scal[3] = 1.0;
+    }
+}
+
+// One iteration after w = A u (in z) and its partials of w.u.
 template <int PHASE>
 __global__ __launch_bounds__(BLK_THREADS) void k_pcg_step_blk(CgArgs g) {
     if (g.state[0]) return;
     __shared__ double rs[BLK_MOLS][9];
-    const double* rz_c = PHASE ? g.p_rr1 : g.p_rr0;
-    double* rz_d = PHASE ? g.p_rr0 : g.p_rr1;
-    double tot[2];   // p.Ap, r.z
-    blk_totals<2>(g.p_vz, g.nparts_mv, rz_c, g.nparts_vec, nullptr, 0, tot);
-    const double t = tot[1] / tot[0];
+    const double* ru_c = PHASE ? g.p_rr1 : g.p_rr0;
+    double* ru_d = PHASE ? g.p_rr0 : g.p_rr1;
+    double tot[2];   // gamma = r.u, delta = w.u
+    blk_totals<2>(ru_c, g.nparts_vec, g.p_vz, g.nparts_mv, nullptr, 0, tot);
+    const double gam = tot[0], del = tot[1];
+    const double beta = gam / g.scal[2 + 2 * PHASE];
+    const double alpha = gam / (del - beta * gam / g.scal[3 + 2 * PHASE]);
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        g.scal[2 + 2 * (1 - PHASE)] = gam;
+        g.scal[3 + 2 * (1 - PHASE)] = alpha;
+        g.state[1] += 1;
+    }
     const BlkSlot b = blk_slot(g);
-    double rzn = 0.0, zz = 0.0, r = 0.0;
+    double run = 0.0, zz = 0.0, r = 0.0;
     if (b.ok) {
-        g.x[b.k] += t * g.v[b.k];
-        r = g.r[b.k] - t * g.z[b.k];
+        const double pk = g.u[b.k] + beta * g.v[b.k];
+        const double sk = g.z[b.k] + beta * g.s[b.k];
+        g.v[b.k] = pk;
+        g.s[b.k] = sk;
+        g.x[b.k] += alpha * pk;
+        r = g.r[b.k] - alpha * sk;
         g.r[b.k] = r;
         const double ar = site_alpha(g.polar, b.k) * r;
         zz = ar * ar;
@@ -1122,37 +1165,14 @@ __global__ __launch_bounds__(BLK_THREADS) void k_pcg_step_blk(CgArgs g) {
     }
     __syncthreads();
     if (b.ok) {
-        const double z = blk_apply(g, b, rs);
-        g.z[b.k] = z;
-        rzn = r * z;
+        const double u = blk_apply(g, b, rs);
+        g.u[b.k] = u;
+        store_dir32(g, b.k, u);
+        run = r * u;
     }
-    blk_partial(rzn, rz_d);
+    blk_partial(run, ru_d);
     __syncthreads();
     blk_partial(zz, g.p_dv);
-    if (blockIdx.x == 0 && threadIdx.x == 0) g.state[1] += 1;
-}
-
-// Stop when the RMS of alpha r is below tol; else p = z + beta p.
-template <int PHASE>
-__global__ __launch_bounds__(BLK_THREADS) void k_pcg_dir_blk(CgArgs g) {
-    if (g.state[0]) return;
-    const double* rz_c = PHASE ? g.p_rr1 : g.p_rr0;
-    const double* rz_d = PHASE ? g.p_rr0 : g.p_rr1;
-    double tot[3];   // |alpha r|^2, new r.z, old r.z
-    blk_totals<3>(g.p_dv, g.nparts_vec, rz_d, g.nparts_vec, rz_c, g.nparts_vec, tot);
-    const double res = sqrt(tot[0] * g.inv_npol);
-    if (blockIdx.x == 0 && threadIdx.x == 0) g.scal[0] = res;
-    if (res < g.tol) {
-        if (blockIdx.x == 0 && threadIdx.x == 0) { g.state[0] = 1; g.state[2] = 2; }
-        return;
-    }
-    const double beta = tot[1] / tot[2];
-    const BlkSlot b = blk_slot(g);
-    if (b.ok) {
-        const double v = g.z[b.k] + beta * g.v[b.k];
-        g.v[b.k] = v;
-        store_dir32(g, b.k, v);
-    }
 }
 
 // FP32 copy of x (float4 per site) for the mixed solve's first A x.
@@ -1296,6 +1316,8 @@ static CgArgs cg_args(Ctx* c, const smd_induce_params* p) {
     g.r = c->d_cg + 0 * (size_t)dim;
     g.v = c->d_cg + 1 * (size_t)dim;
     g.z = c->d_cg + 2 * (size_t)dim;
+    g.u = c->d_cg + 3 * (size_t)dim;
+    g.s = c->d_cg + 4 * (size_t)dim;
     g.e0 = c->d_e0;
     g.polar = c->d_polar;
     g.nparts_mv = (int)(((int64_t)n * c->cg_lanes + 255) / 256);   // blocks of the A v kernel
@@ -1311,6 +1333,15 @@ static CgArgs cg_args(Ctx* c, const smd_induce_params* p) {
     g.v32 = p->prec == SMD_PREC_MIXED ? c->d_cg32 : nullptr;
     g.inv_npol = 1.0 / (double)(c->npol > 0 ? c->npol : 1);
     return g;
+}
+
+// The block-preconditioned solve's stopping test at the top of A u.
+static CgTop blk_top(const CgArgs& g) {
+    CgTop t{g.state, g.p_dv, g.nparts_vec, g.tol};
+    t.scale = g.inv_npol;
+    t.res = g.scal;
+    t.reason = 2;
+    return t;
 }
 
 // A v of the solve in its precision; v32 is v's FP32 copy (mixed).
@@ -1380,26 +1411,23 @@ int induce_begin(Ctx* c, const smd_induce_params* p, bool frames) {
     if (p->precond == 2) {
         hipLaunchKernelGGL(k_block_pc, dim3((c->nmol + 255) / 256), b256, 0, c->stream, a, c->d_pc_blk, c->nmol);
         hipLaunchKernelGGL(k_pcg_start_blk, dim3(g.nparts_vec), dim3(BLK_THREADS), 0, c->stream, g);
+        solve_apply(c, p, a, g.u, g.v32, g.z, g.p_vz, blk_top(g));   // w = A u, with the stopping test
     } else if (p->precond) hipLaunchKernelGGL(k_pcg_start, grid_vec, dim3(CG_THREADS), 0, c->stream, g);
     else hipLaunchKernelGGL(k_cg_start, grid_vec, dim3(CG_THREADS), 0, c->stream, g);
     SMD_HIP(c, hipGetLastError());
     return SMD_OK;
 }
 
-// count CG iterations (from an even one): A v, the step, the direction.
+// count CG iterations (from an even one): A v, the step, the direction (the
+// block preconditioner's single-reduction form: the step, then A u).
 static void cg_iterations(Ctx* c, const smd_induce_params* p, const IndArgs& a, const CgArgs& g, int count) {
     const dim3 grid_vec(g.nparts_vec), bt(CG_THREADS);
     for (int k = 0; k < count; ++k) {
         if (p->precond == 2) {
-            solve_apply(c, p, a, g.v, g.v32, g.z, g.p_vz, CgTop{g.state, nullptr, g.nparts_vec, g.tol});
             const dim3 bw(BLK_THREADS);
-            if (k & 1) {
-                hipLaunchKernelGGL(k_pcg_step_blk<1>, grid_vec, bw, 0, c->stream, g);
-                hipLaunchKernelGGL(k_pcg_dir_blk<1>, grid_vec, bw, 0, c->stream, g);
-            } else {
-                hipLaunchKernelGGL(k_pcg_step_blk<0>, grid_vec, bw, 0, c->stream, g);
-                hipLaunchKernelGGL(k_pcg_dir_blk<0>, grid_vec, bw, 0, c->stream, g);
-            }
+            if (k & 1) hipLaunchKernelGGL(k_pcg_step_blk<1>, grid_vec, bw, 0, c->stream, g);
+            else hipLaunchKernelGGL(k_pcg_step_blk<0>, grid_vec, bw, 0, c->stream, g);
+            solve_apply(c, p, a, g.u, g.v32, g.z, g.p_vz, blk_top(g));
         } else if (p->precond) {
             solve_apply(c, p, a, g.v, g.v32, g.z, g.p_vz, CgTop{g.state, nullptr, g.nparts_vec, g.tol});
             if (k & 1) {
@@ -1572,7 +1600,7 @@ extern "C" int smd_set_multipoles(smd_ctx* c, const double* dip_local, const dou
             (rc = A(c->d_polar, sizeof(double) * n)) || (rc = A(c->d_dipg, sizeof(double) * 3 * n)) ||
             (rc = A(c->d_quadg, sizeof(double) * 6 * n)) || (rc = A(c->d_e0, sizeof(double) * 3 * n)) ||
             (rc = A(c->d_mu, sizeof(double) * 3 * n)) || (rc = A(c->d_e0_prev, sizeof(double) * 3 * n)) ||
-            (rc = A(c->d_cg, sizeof(double) * 9 * n)) ||
+            (rc = A(c->d_cg, sizeof(double) * 15 * n)) ||
             (rc = A(c->d_cg32, sizeof(float4) * n)) || (rc = A(c->d_ind_rec32, sizeof(float4) * 2 * n)) ||
             (rc = A(c->d_ind_fix, sizeof(int4) * n)) ||
             (rc = A(c->d_pc_blk, sizeof(double) * 45 * (size_t)(c->nmol > 0 ? c->nmol : 1))) ||
@@ -1580,7 +1608,7 @@ extern "C" int smd_set_multipoles(smd_ctx* c, const double* dip_local, const dou
             (rc = A(c->d_site32, sizeof(float4) * 3 * n)) ||
             (rc = A(c->d_pind32, sizeof(float4) * n)) ||
             (rc = A(c->d_cg_part, sizeof(double) * 4 * parts)) || (rc = A(c->d_cg_state, sizeof(int) * 4)) ||
-            (rc = A(c->d_cg_scal, sizeof(double) * 4)))
+            (rc = A(c->d_cg_scal, sizeof(double) * 8)))
             return rc;
         c->cg_parts = parts;
         SMD_HIP(c, hipMemset(c->d_mu, 0, sizeof(double) * 3 * n));

@@ -2026,14 +2026,14 @@ bool polar_joint_long(Ctx* c) {
 }
 
 // Multipole's short-range kernels of the polarizable step with its terms
-// cleared first, and the permanent multipoles' structure-factor sums: none
-// needs the dipoles, so the step can run them beside the solve.
-int polar_mp_short_enqueue(Ctx* c, double* mp_t) {
+// cleared first, and (with_sums) the permanent multipoles' structure-factor
+// sums: none needs the dipoles, so the step can run them beside the solve.
+int polar_mp_short_enqueue(Ctx* c, double* mp_t, bool with_sums) {
     MpArgs am = mp_args(c);
     am.terms = mp_t;
     hipLaunchKernelGGL(k_zero_terms, dim3(1), dim3(256), 0, c->stream, mp_t);
     int rc = mp_short_enqueue(c, am);
-    if (rc) return rc;
+    if (rc || !with_sums) return rc;
     // the permanent multipoles' structure-factor sums (the joint pass's other
     // 8 per k follow the solve)
     ProfScope ps(c, "mp_long");
@@ -2047,13 +2047,13 @@ int polar_mp_short_enqueue(Ctx* c, double* mp_t) {
 // Both classes after the solve: the joint k-space pass (structure factors with
 // the converged dipoles, both classes' per-k terms, the joint force pass) on
 // the second stream beside the short-range kernels (Induction's, and
-// Multipole's when with_mp_short; otherwise polar_mp_short_enqueue ran them,
-// cleared Multipole's terms and summed the permanent multipoles' structure
-// factors, so only the induced dipoles' 8 sums per k remain), then the
-// k-space forces and torques once.
+// Multipole's when with_mp_short; otherwise polar_mp_short_enqueue ran them
+// and cleared Multipole's terms), then the k-space forces and torques once.
+// mp_sums_done: polar_mp_short_enqueue also summed the permanent multipoles'
+// structure factors, so only the induced dipoles' 8 sums per k remain.
 // The k-space torque's volume-derivative share is booked to Induction's
 // terms (the step reports the sum).
-int polar_joint_enqueue(Ctx* c, double* mp_t, double* ind_t, bool with_mp_short) {
+int polar_joint_enqueue(Ctx* c, double* mp_t, double* ind_t, bool with_mp_short, bool mp_sums_done) {
     int rc;
     MpArgs am = mp_args(c), ai = ind_args(c);
     am.terms = mp_t;
@@ -2066,7 +2066,7 @@ int polar_joint_enqueue(Ctx* c, double* mp_t, double* ind_t, bool with_mp_short)
         const dim3 g1((ai.K + MP_SF_THREADS - 1) / MP_SF_THREADS, ai.chunks);
         const dim3 gk((c->n + WAVE - 1) / WAVE, ai.ksplit), gf((ai.K + 3) / 4);
         const bool mixed = c->mp_prec == SMD_PREC_MIXED;
-        if (with_mp_short) {
+        if (!mp_sums_done) {
             if (mixed) hipLaunchKernelGGL((k_ind_sfac<float>), g1, dim3(MP_SF_THREADS), 0, c->stream, ai);
             else hipLaunchKernelGGL((k_ind_sfac<double>), g1, dim3(MP_SF_THREADS), 0, c->stream, ai);
             hipLaunchKernelGGL(k_joint_kfinish<false>, gf, b256, 0, c->stream, am, ai);

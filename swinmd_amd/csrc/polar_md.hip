@@ -11,20 +11,22 @@
 // A step is three parts on the context stream:
 //   A  the Verlet trigger and drift partials (cleared forces and terms), the
 //      gated list rebuild, the lab frames, then the solve's field, coupling
-//      table, start, A x and residual;
+//      table, start, A x and residual (vdW beside them, see M);
 //   B  the solve's iterations in captured batches; the host reads the
 //      device's stop flag between batches (the first read comes after as
 //      many batches as the previous step needed);
-//   M  vdW, Multipole's short-range kernels and the permanent multipoles'
-//      structure-factor sums, on the second stream beside B;
+//   M  Multipole's short-range kernels, on the second stream beside B, with
+//      (the polar_m_parts tuning bits) vdW (else beside A's solve start) and
+//      the permanent multipoles' structure-factor sums (else in C's joint
+//      pass);
 //   C  Induction's short-range kernels and the rest of one k-space pass for
 //      both classes with the converged dipoles (the joint pass: both classes
 //      on the same k set), then SHAKE with the kinetic and pressure sums and
 //      the NH update.
 // The second stream also carries the k-space pass in C, beside the other
-// kernels (the recip_overlap tuning switch; without it M's kernels run in C
-// on the step's stream, and the structure factors there are summed in one
-// pass).  Without the joint pass
+// kernels (the recip_overlap tuning switch; without it vdW and M's kernels
+// run in C on the step's stream, and the structure factors there are summed
+// in one pass).  Without the joint pass
 // (classes on different k sets, or the polar_joint_long switch off),
 // Multipole's kernels run in A with their own k-space pass on the second
 // stream, and C holds Induction's.
@@ -42,6 +44,14 @@ namespace smd {
 static double* mp_terms(Ctx* c) { return c->d_polar_terms; }
 static double* ind_terms(Ctx* c) { return c->d_polar_terms + POLAR_TERMS; }
 
+// Multipole's short range beside the solve: with the joint pass and a second
+// stream, part M runs on the second stream between A and C.
+static bool split_mp(Ctx* c) { return polar_joint_long(c) && c->recip_overlap && c->stream2; }
+
+// vdW runs in part A (beside the solve's start) with the joint pass when part
+// M runs and does not take it.
+static bool vdw_in_a(Ctx* c) { return split_mp(c) && !(c->polar_m_parts & 1); }
+
 // Part A: everything up to the solve's first iteration.
 static int part_a(Ctx* c, bool on_device_rebuild) {
     const smd_polar_md_params& p = c->pmd;
@@ -50,9 +60,14 @@ static int part_a(Ctx* c, bool on_device_rebuild) {
     if (!rc) rc = pairlist_build(c, p.cutoff, p.cut_buff, p.use_cells, on_device_rebuild);
     if (rc) return rc;
     if (polar_joint_long(c)) {
-        // vdW and Multipole's short range in part M beside the solve (or in C)
-        rc = frames_run(c);
+        // Multipole's short range in part M beside the solve (or in C); vdW
+        // beside the solve's start on the second stream unless M has it
+        const bool fork = vdw_in_a(c) && polar_fork_begin(c);
+        if (vdw_in_a(c)) rc = vdw_enqueue(c, &c->pmd_vdw, p.prec);
+        if (fork) polar_fork_end(c);
+        if (!rc) rc = frames_run(c);
         if (!rc) rc = induce_begin(c, &p.induce, false);
+        if (fork) polar_join(c);
     } else {
         rc = vdw_enqueue(c, &c->pmd_vdw, p.prec);
         if (!rc) rc = frames_run(c);
@@ -63,14 +78,10 @@ static int part_a(Ctx* c, bool on_device_rebuild) {
     return rc;
 }
 
-// Multipole's short range beside the solve: with the joint pass and a second
-// stream, part M runs on the second stream between A and C.
-static bool split_mp(Ctx* c) { return polar_joint_long(c) && c->recip_overlap && c->stream2; }
-
 static int part_m(Ctx* c) {
     RedDefer defer(c);
-    int rc = vdw_enqueue(c, &c->pmd_vdw, c->pmd.prec);
-    if (!rc) rc = polar_mp_short_enqueue(c, mp_terms(c));
+    int rc = (c->polar_m_parts & 1) ? vdw_enqueue(c, &c->pmd_vdw, c->pmd.prec) : SMD_OK;
+    if (!rc) rc = polar_mp_short_enqueue(c, mp_terms(c), (c->polar_m_parts & 2) != 0);
     if (!rc) rc = red_flush(c);
     return rc;
 }
@@ -82,10 +93,10 @@ static int part_c(Ctx* c) {
     {
         RedDefer defer(c);
         if (!polar_joint_long(c)) rc = induction_enqueue(c, ind_terms(c));
-        else if (split_mp(c)) rc = polar_joint_enqueue(c, mp_terms(c), ind_terms(c), false);
+        else if (split_mp(c)) rc = polar_joint_enqueue(c, mp_terms(c), ind_terms(c), false, (c->polar_m_parts & 2) != 0);
         else {
             rc = vdw_enqueue(c, &c->pmd_vdw, c->pmd.prec);
-            if (!rc) rc = polar_joint_enqueue(c, mp_terms(c), ind_terms(c), true);
+            if (!rc) rc = polar_joint_enqueue(c, mp_terms(c), ind_terms(c), true, false);
         }
         if (!rc) rc = red_flush(c);
     }
@@ -254,8 +265,8 @@ extern "C" int smd_polar_md_init(smd_ctx* c, const smd_polar_md_params* p) {
     double res = 0.0;
     if ((rc = solve(c, 1, &iters, &res))) return rc;
     if (polar_joint_long(c)) {
-        rc = vdw_enqueue(c, &c->pmd_vdw, p->prec);
-        if (!rc) rc = polar_joint_enqueue(c, mp_terms(c), ind_terms(c), true);
+        rc = vdw_in_a(c) ? SMD_OK : vdw_enqueue(c, &c->pmd_vdw, p->prec);
+        if (!rc) rc = polar_joint_enqueue(c, mp_terms(c), ind_terms(c), true, false);
     } else {
         rc = induction_enqueue(c, ind_terms(c));
     }

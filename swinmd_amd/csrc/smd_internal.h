@@ -263,6 +263,7 @@ struct Ctx {
     bool shake_frecip = false;        // k_shake adds d_frecip to the forces it reads
     int recip_overlap = 1;
     int polar_joint = 1;            // one k-space pass for Multipole and Induction in the polarizable step
+    int polar_m_parts = 1;          // the polarizable step's part M: 1 vdW, 2 the permanent structure factors
     int md_fused_shake = 1;           // the MD step's SHAKE and kinetics in one kernel
 
     Prof prof;
@@ -322,8 +323,8 @@ constexpr int POLAR_TERMS = 128;   // doubles of one multipole/induction accumul
 int multipole_enqueue(Ctx* c, double* terms);
 int induction_enqueue(Ctx* c, double* terms);
 bool polar_joint_long(Ctx* c);
-int polar_joint_enqueue(Ctx* c, double* mp_terms, double* ind_terms, bool with_mp_short);
-int polar_mp_short_enqueue(Ctx* c, double* mp_terms);
+int polar_joint_enqueue(Ctx* c, double* mp_terms, double* ind_terms, bool with_mp_short, bool mp_sums_done);
+int polar_mp_short_enqueue(Ctx* c, double* mp_terms, bool with_sums);
 bool polar_fork_begin(Ctx* c);
 void polar_fork_end(Ctx* c);
 void polar_join(Ctx* c);
