@@ -273,8 +273,8 @@ def bench_polar_md(args):
             ctx.set_tuning("recip_overlap", 0)
         if args.no_joint:
             ctx.set_tuning("polar_joint_long", 0)
-        if args.polar_m_parts is not None:
-            ctx.set_tuning("polar_m_parts", args.polar_m_parts)
+        if args.polar_m_sums is not None:
+            ctx.set_tuning("polar_m_sums", args.polar_m_sums)
         ind = Induction(ctx, cutoff_ind=POLAR["cutoff_ind"], a_damp=POLAR["a_damp"], damp_tij=True, do_min=True,
                         tol=tol, precond=args.precond, guess=guess, prec=prec)
         if net_state is not None:
@@ -624,9 +624,9 @@ def main():
                     help="polarizable workloads: keep the k-space passes on the step's stream")
     ap.add_argument("--no-joint", action="store_true",
                     help="polarizable workloads: a k-space pass per class instead of one joint pass after the solve")
-    ap.add_argument("--polar-m-parts", type=int, default=None, choices=[0, 1, 2, 3],
-                    help="polarizable workloads: what runs beside the dipole solve (1 vdW, 2 the permanent "
-                         "structure factors; default: the library's)")
+    ap.add_argument("--polar-m-sums", type=int, default=None, choices=[0, 1],
+                    help="polarizable workloads: 1 sums the permanent structure factors beside the dipole solve "
+                         "(default: the library's)")
     ap.add_argument("--no-graph", action="store_true",
                     help="launch each step's kernels directly instead of replaying a captured hipGraph")
     args = ap.parse_args()

@@ -592,10 +592,10 @@ int smd_md_time_family(smd_ctx* ctx, const char* family, int reps, double* avg_u
  * "polar_joint_long" (0/1, default 1): the polarizable step runs one k-space
  * pass for Multipole and Induction after the dipole solve when both use the
  * same alpha and kcut (0: a pass per class).
- * "polar_m_parts" (0..3, default 1): with the joint pass and the second
- * stream, what runs beside the dipole solve with Multipole's short range:
- * bit 1 vdW (else beside the solve's start), bit 2 the permanent multipoles'
- * structure factors (else after the solve, in the joint pass).
+ * "polar_m_sums" (0/1, default 0): with the joint pass and the second
+ * stream, the permanent multipoles' structure factors are summed beside the
+ * dipole solve with vdW and Multipole's short range (0: after the solve, in
+ * the joint pass).
  * "md_fused_shake" (0/1, default 1): the MD step runs SHAKE and the kinetic
  * and pressure sums as one kernel (0: two kernels).
  * "cg_lanes" (8, 16 or 32, default 16): lanes sharing one row of the CG's A v.

@@ -146,7 +146,7 @@ static void release(Ctx* c) {
                     c->mp_ks.fpart, c->ind_ks.kvec, c->ind_ks.kmod2, c->ind_ks.expk, c->ind_ks.part,
                     c->ind_ks.kf, c->ind_ks.fpart, c->d_pack, c->d_bd_bonds, c->d_bd_angles,
                     c->d_bd_bond_off, c->d_bd_angle_off, c->d_bd_par, c->d_bd_out, c->d_stress, c->d_real,
-                    c->d_it_cnt, c->d_it_j, c->d_it_a, c->d_part, c->d_cg32, c->d_ind_rec32, c->d_ind_fix, c->d_e0_prev, c->d_pc_blk, c->d_shell32, c->d_polar_terms, c->d_site32, c->d_pind32,
+                    c->d_it_cnt, c->d_it_j, c->d_it_a, c->d_part, c->d_cg32, c->d_ind_rec32, c->d_ind_fix, c->d_e0_prev, c->d_vrec, c->d_pc_blk, c->d_shell32, c->d_polar_terms, c->d_site32, c->d_site32_ind, c->d_pind32,
                     c->red.base};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
@@ -157,6 +157,7 @@ static void release(Ctx* c) {
     if (c->pmd_a) (void)hipGraphExecDestroy(c->pmd_a);
     if (c->pmd_c) (void)hipGraphExecDestroy(c->pmd_c);
     if (c->pmd_m) (void)hipGraphExecDestroy(c->pmd_m);
+    if (c->pmd_a2) (void)hipGraphExecDestroy(c->pmd_a2);
     if (c->graph) (void)hipGraphDestroy(c->graph);
     for (auto& r : c->prof.pending) { (void)hipEventDestroy(r.a); (void)hipEventDestroy(r.b); }
     if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);
@@ -420,9 +421,9 @@ extern "C" int smd_set_tuning(smd_ctx* c, const char* name, int value) {
         if (value != 0 && value != 1) return fail(c, SMD_EINVAL, "cg_graph must be 0 or 1 (got %d)", value);
         c->cg_graph = value;
         if (c->cg_exec) { (void)hipGraphExecDestroy(c->cg_exec); c->cg_exec = nullptr; }
-    } else if (std::string(name) == "polar_m_parts") {
-        if (value < 0 || value > 3) return fail(c, SMD_EINVAL, "polar_m_parts must be 0..3 (got %d)", value);
-        c->polar_m_parts = value;
+    } else if (std::string(name) == "polar_m_sums") {
+        if (value != 0 && value != 1) return fail(c, SMD_EINVAL, "polar_m_sums must be 0 or 1 (got %d)", value);
+        c->polar_m_sums = value;
     } else if (std::string(name) == "polar_joint_long") {
         if (value != 0 && value != 1) return fail(c, SMD_EINVAL, "polar_joint_long must be 0 or 1 (got %d)", value);
         c->polar_joint = value;
@@ -437,6 +438,7 @@ extern "C" int smd_set_tuning(smd_ctx* c, const char* name, int value) {
     if (c->pmd_a) { (void)hipGraphExecDestroy(c->pmd_a); c->pmd_a = nullptr; }
     if (c->pmd_c) { (void)hipGraphExecDestroy(c->pmd_c); c->pmd_c = nullptr; }
     if (c->pmd_m) { (void)hipGraphExecDestroy(c->pmd_m); c->pmd_m = nullptr; }
+    if (c->pmd_a2) { (void)hipGraphExecDestroy(c->pmd_a2); c->pmd_a2 = nullptr; }
     return SMD_OK;
 }
 

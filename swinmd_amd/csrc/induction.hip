@@ -1235,7 +1235,7 @@ static IndArgs ind_args(Ctx* c, const smd_induce_params* p) {
     a.rec32 = c->d_ind_rec32;
     a.fix = c->d_ind_fix;
     for (int d = 0; d < 3; ++d) a.fscale[d] = (float)(c->box.L[d] / 4294967296.0);
-    a.site32 = c->d_site32;
+    a.site32 = c->d_site32_ind;   // its own copy: Multipole's kernels may pack d_site32 alongside
     for (int d = 0; d < 3; ++d) {
         const double L = c->box.L[d];
         a.Lh[d] = (float)L;
@@ -1605,7 +1605,7 @@ extern "C" int smd_set_multipoles(smd_ctx* c, const double* dip_local, const dou
             (rc = A(c->d_ind_fix, sizeof(int4) * n)) ||
             (rc = A(c->d_pc_blk, sizeof(double) * 45 * (size_t)(c->nmol > 0 ? c->nmol : 1))) ||
             (rc = A(c->d_shell32, sizeof(float) * 24 * n)) ||
-            (rc = A(c->d_site32, sizeof(float4) * 3 * n)) ||
+            (rc = A(c->d_site32, sizeof(float4) * 3 * n)) || (rc = A(c->d_site32_ind, sizeof(float4) * 3 * n)) ||
             (rc = A(c->d_pind32, sizeof(float4) * n)) ||
             (rc = A(c->d_cg_part, sizeof(double) * 4 * parts)) || (rc = A(c->d_cg_state, sizeof(int) * 4)) ||
             (rc = A(c->d_cg_scal, sizeof(double) * 8)))

@@ -58,6 +58,10 @@ struct PairArgs {
     int buff_147;                  // n_buff = 14 and m_buff = 7: integer powers
     // Ewald
     double alpha, alpha2, alpha2PI, realcut2;
+    // mixed buffered 14-7: fixed-point virtual-site records (x, y, z as 32-bit
+    // fractions of the box, type), or null
+    const int4* vrec;
+    float fscale[3];          // L / 2^32 per dimension
 };
 
 // CHARMM-type switch of MieForce/BuckinghamForce (src/forces/MieForce.cpp:145-154)
@@ -105,6 +109,7 @@ struct Owner {
     double cx, cy, cz;      // molecule centre (Lustig terms)
     double qi;
     int ti, mi;
+    unsigned ux, uy, uz;    // fixed-point virtual site (the packed mixed buffered 14-7)
 };
 
 // One entry of row i.  FULL: a pair of the reference's half list in its
@@ -113,13 +118,10 @@ struct Owner {
 // Buckingham and buffered 14-7 on the virtual sites: one pair given the
 // partner's virtual-site position and type (loaded by the caller, so a row
 // walk can fetch them ahead).
+// The pair at rij = ri - rj on the virtual sites (minimum image, FP64).
 template <int VDW, bool LUSTIG, typename R, bool FULL>
-__device__ __forceinline__ void vsite_term(const PairArgs& a, const Owner& o, double ejx, double ejy, double ejz, int tj,
+__device__ __forceinline__ void vsite_core(const PairArgs& a, const Owner& o, double dx, double dy, double dz, int tj,
                                            double qj, double (&acc)[A_N], double& fix, double& fiy, double& fiz) {
-        // rij = ri - rj on the virtual sites (BuckinghamForce.cpp:156,
-        // BuffvdWForce.cpp:172)
-        double dx = o.ex - ejx, dy = o.ey - ejy, dz = o.ez - ejz;
-        pbc_near(dx, dy, dz, a.box);
         const R rx = (R)dx, ry = (R)dy, rz = (R)dz;
         const R r = sqrt(rx * rx + ry * ry + rz * rz);
         const R rij2 = r * r;
@@ -218,6 +220,16 @@ __device__ __forceinline__ void vsite_term(const PairArgs& a, const Owner& o, do
                 }
             }
         }
+}
+
+template <int VDW, bool LUSTIG, typename R, bool FULL>
+__device__ __forceinline__ void vsite_term(const PairArgs& a, const Owner& o, double ejx, double ejy, double ejz, int tj,
+                                           double qj, double (&acc)[A_N], double& fix, double& fiy, double& fiz) {
+    // rij = ri - rj on the virtual sites (BuckinghamForce.cpp:156,
+    // BuffvdWForce.cpp:172)
+    double dx = o.ex - ejx, dy = o.ey - ejy, dz = o.ez - ejz;
+    pbc_near(dx, dy, dz, a.box);
+    vsite_core<VDW, LUSTIG, R, FULL>(a, o, dx, dy, dz, tj, qj, acc, fix, fiy, fiz);
 }
 
 template <int VDW, bool EWALD, bool LUSTIG, typename R, bool FULL>
@@ -376,6 +388,27 @@ __device__ __forceinline__ void vsite_walk(const PairArgs& a, const Owner& o, in
     }
 }
 
+// The mixed buffered 14-7's walk over fixed-point virtual-site records: one
+// 16-byte record per partner; the 32-bit coordinate differences are exact
+// and already the minimum image, then one rounding to FP32.
+template <int VDW, bool LUSTIG, bool FULL, int G, typename At>
+__device__ __forceinline__ void vsite_walk_packed(const PairArgs& a, const Owner& o, int first, int count, At at,
+                                                  double (&acc)[A_N], double& fix, double& fiy, double& fiz) {
+    if (first >= count) return;
+    int jn = first + G < count ? at(first + G) : at(first);
+    int4 r = a.vrec[at(first)];
+    for (int k = first; k < count; k += G) {
+        const int jnn = k + 2 * G < count ? at(k + 2 * G) : jn;
+        const int4 rn = a.vrec[jn];
+        const float dx = (float)(int)(o.ux - (unsigned)r.x) * a.fscale[0];
+        const float dy = (float)(int)(o.uy - (unsigned)r.y) * a.fscale[1];
+        const float dz = (float)(int)(o.uz - (unsigned)r.z) * a.fscale[2];
+        vsite_core<VDW, LUSTIG, float, FULL>(a, o, dx, dy, dz, r.w, 0.0, acc, fix, fiy, fiz);
+        jn = jnn;
+        r = rn;
+    }
+}
+
 template <int VDW, bool EWALD, bool LUSTIG, typename R, int G>
 __global__ __launch_bounds__(256) void k_pairs(PairArgs a) {
     const int tid = blockIdx.x * blockDim.x + threadIdx.x;
@@ -402,7 +435,21 @@ __global__ __launch_bounds__(256) void k_pairs(PairArgs a) {
         o.cx = o.cy = o.cz = 0.0;
         if constexpr (LUSTIG) { o.cx = a.com[3 * o.mi]; o.cy = a.com[3 * o.mi + 1]; o.cz = a.com[3 * o.mi + 2]; }
         const int32_t* back = row + a.stride - 1;
-        if constexpr ((VDW == V_BUCK || VDW == V_BUFF) && !EWALD) {
+        if constexpr (VDW == V_BUFF && !EWALD && sizeof(R) == sizeof(float)) {
+            if (a.vrec) {
+                const int4 ri = a.vrec[i];
+                o.ux = (unsigned)ri.x; o.uy = (unsigned)ri.y; o.uz = (unsigned)ri.z;
+                vsite_walk_packed<VDW, LUSTIG, true, G>(a, o, sub, own, [row](int k) { return row[k]; }, acc, fix,
+                                                        fiy, fiz);
+                vsite_walk_packed<VDW, LUSTIG, false, G>(a, o, sub, non, [back](int k) { return back[-k]; }, acc,
+                                                         fix, fiy, fiz);
+            } else {
+                vsite_walk<VDW, LUSTIG, R, true, G>(a, o, sub, own, [row](int k) { return row[k]; }, acc, fix, fiy,
+                                                    fiz);
+                vsite_walk<VDW, LUSTIG, R, false, G>(a, o, sub, non, [back](int k) { return back[-k]; }, acc, fix,
+                                                     fiy, fiz);
+            }
+        } else if constexpr ((VDW == V_BUCK || VDW == V_BUFF) && !EWALD) {
             // the partner's virtual site and type one entry ahead, its index two
             vsite_walk<VDW, LUSTIG, R, true, G>(a, o, sub, own, [row](int k) { return row[k]; }, acc, fix, fiy, fiz);
             vsite_walk<VDW, LUSTIG, R, false, G>(a, o, sub, non, [back](int k) { return back[-k]; }, acc, fix, fiy,
@@ -1088,6 +1135,30 @@ int pairs_lj_ewald(Ctx* c, const double* sigma2, const double* eps4, double cuto
     return SMD_OK;
 }
 
+// Fixed-point virtual-site records of the mixed buffered 14-7 (the
+// fractional coordinates as in k_pack_sites, and the type).
+__global__ __launch_bounds__(256) void k_pack_vsites(int n, const double* __restrict__ epos,
+                                                     const int32_t* __restrict__ type, Box box, int4* __restrict__ rec) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    unsigned u[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        double f = epos[3 * i + k] * box.inv[k];
+        f -= floor(f);
+        u[k] = (unsigned)((unsigned long long)(f * 4294967296.0) & 0xffffffffull);
+    }
+    rec[i] = make_int4((int)u[0], (int)u[1], (int)u[2], type[i]);
+}
+
+// The mixed buffered 14-7 reads packed virtual-site records once this buffer
+// exists (allocated outside any capture).
+int vdw_packed_alloc(Ctx* c) {
+    if (!c->d_vrec && hipMalloc(&c->d_vrec, sizeof(int4) * (size_t)(c->n > 0 ? c->n : 1)) != hipSuccess)
+        return fail(c, SMD_ENOMEM, "virtual-site records of %d sites", c->n);
+    return SMD_OK;
+}
+
 // vdW of the polarizable MD step: the kernels of vdw_run without the table
 // upload, the accumulator clear and the read-back (capturable).  Forces are
 // added to d_force; energies, Lustig sums and the virial to the step's terms.
@@ -1102,6 +1173,12 @@ int vdw_enqueue(Ctx* c, const smd_vdw_params* p, int prec) {
     case SMD_VDW_BUFF147:
         hipLaunchKernelGGL(k_virtual_sites, dim3((c->nmol + 255) / 256), dim3(256), 0, c->stream, c->nmol,
                            c->d_mol_start, c->d_type, c->d_pos, c->d_epos, p->kind, p->hfactor, p->co2_bond);
+        if (prec == SMD_PREC_MIXED && c->d_vrec) {
+            hipLaunchKernelGGL(k_pack_vsites, dim3((c->n + 255) / 256), dim3(256), 0, c->stream, c->n, c->d_epos,
+                               c->d_type, c->box, c->d_vrec);
+            a.vrec = c->d_vrec;
+            for (int d = 0; d < 3; ++d) a.fscale[d] = (float)(c->box.L[d] / 4294967296.0);
+        }
         e = launch_pairs<V_BUFF, false>(c, a, prec, lustig);
         if (e == hipSuccess) {
             hipLaunchKernelGGL(k_redistribute, dim3((c->nmol + 255) / 256), dim3(256), 0, c->stream, c->nmol,

@@ -10,15 +10,15 @@
 //
 // A step is three parts on the context stream:
 //   A  the Verlet trigger and drift partials (cleared forces and terms), the
-//      gated list rebuild, the lab frames, then the solve's field, coupling
-//      table, start, A x and residual (vdW beside them, see M);
+//      gated list rebuild, the lab frames (A1), then the solve's field,
+//      coupling table, start, A x and residual (A2);
 //   B  the solve's iterations in captured batches; the host reads the
 //      device's stop flag between batches (the first read comes after as
 //      many batches as the previous step needed);
-//   M  Multipole's short-range kernels, on the second stream beside B, with
-//      (the polar_m_parts tuning bits) vdW (else beside A's solve start) and
-//      the permanent multipoles' structure-factor sums (else in C's joint
-//      pass);
+//   M  vdW and Multipole's short-range kernels (they write forces, which
+//      nothing in A2 and B touches), on the second stream beside A2 and B,
+//      with (the polar_m_sums tuning switch) the permanent multipoles'
+//      structure-factor sums (else in C's joint pass);
 //   C  Induction's short-range kernels and the rest of one k-space pass for
 //      both classes with the converged dipoles (the joint pass: both classes
 //      on the same k set), then SHAKE with the kinetic and pressure sums and
@@ -30,8 +30,8 @@
 // (classes on different k sets, or the polar_joint_long switch off),
 // Multipole's kernels run in A with their own k-space pass on the second
 // stream, and C holds Induction's.
-// A, M and C hold no host synchronisation and no memset: each is captured
-// once as a hipGraph and replayed.
+// A (or A1 and A2), M and C hold no host synchronisation and no memset: each
+// is captured once as a hipGraph and replayed.
 #include <algorithm>
 #include <cstring>
 #include <vector>
@@ -48,10 +48,6 @@ static double* ind_terms(Ctx* c) { return c->d_polar_terms + POLAR_TERMS; }
 // stream, part M runs on the second stream between A and C.
 static bool split_mp(Ctx* c) { return polar_joint_long(c) && c->recip_overlap && c->stream2; }
 
-// vdW runs in part A (beside the solve's start) with the joint pass when part
-// M runs and does not take it.
-static bool vdw_in_a(Ctx* c) { return split_mp(c) && !(c->polar_m_parts & 1); }
-
 // Part A: everything up to the solve's first iteration.
 static int part_a(Ctx* c, bool on_device_rebuild) {
     const smd_polar_md_params& p = c->pmd;
@@ -60,14 +56,9 @@ static int part_a(Ctx* c, bool on_device_rebuild) {
     if (!rc) rc = pairlist_build(c, p.cutoff, p.cut_buff, p.use_cells, on_device_rebuild);
     if (rc) return rc;
     if (polar_joint_long(c)) {
-        // Multipole's short range in part M beside the solve (or in C); vdW
-        // beside the solve's start on the second stream unless M has it
-        const bool fork = vdw_in_a(c) && polar_fork_begin(c);
-        if (vdw_in_a(c)) rc = vdw_enqueue(c, &c->pmd_vdw, p.prec);
-        if (fork) polar_fork_end(c);
-        if (!rc) rc = frames_run(c);
+        // vdW and Multipole's short range in part M beside the solve (or in C)
+        rc = frames_run(c);
         if (!rc) rc = induce_begin(c, &p.induce, false);
-        if (fork) polar_join(c);
     } else {
         rc = vdw_enqueue(c, &c->pmd_vdw, p.prec);
         if (!rc) rc = frames_run(c);
@@ -78,10 +69,29 @@ static int part_a(Ctx* c, bool on_device_rebuild) {
     return rc;
 }
 
+// The split step's part A in two: A1 (head, gated list rebuild, lab frames),
+// after which part M starts on the second stream, and A2 (the solve's start).
+static int part_a1(Ctx* c, bool on_device_rebuild) {
+    const smd_polar_md_params& p = c->pmd;
+    RedDefer defer(c);
+    int rc = md_head_enqueue(c);
+    if (!rc) rc = pairlist_build(c, p.cutoff, p.cut_buff, p.use_cells, on_device_rebuild);
+    if (!rc) rc = frames_run(c);
+    if (!rc) rc = red_flush(c);
+    return rc;
+}
+
+static int part_a2(Ctx* c) {
+    RedDefer defer(c);
+    int rc = induce_begin(c, &c->pmd.induce, false);
+    if (!rc) rc = red_flush(c);
+    return rc;
+}
+
 static int part_m(Ctx* c) {
     RedDefer defer(c);
-    int rc = (c->polar_m_parts & 1) ? vdw_enqueue(c, &c->pmd_vdw, c->pmd.prec) : SMD_OK;
-    if (!rc) rc = polar_mp_short_enqueue(c, mp_terms(c), (c->polar_m_parts & 2) != 0);
+    int rc = vdw_enqueue(c, &c->pmd_vdw, c->pmd.prec);
+    if (!rc) rc = polar_mp_short_enqueue(c, mp_terms(c), c->polar_m_sums != 0);
     if (!rc) rc = red_flush(c);
     return rc;
 }
@@ -93,7 +103,7 @@ static int part_c(Ctx* c) {
     {
         RedDefer defer(c);
         if (!polar_joint_long(c)) rc = induction_enqueue(c, ind_terms(c));
-        else if (split_mp(c)) rc = polar_joint_enqueue(c, mp_terms(c), ind_terms(c), false, (c->polar_m_parts & 2) != 0);
+        else if (split_mp(c)) rc = polar_joint_enqueue(c, mp_terms(c), ind_terms(c), false, c->polar_m_sums != 0);
         else {
             rc = vdw_enqueue(c, &c->pmd_vdw, c->pmd.prec);
             if (!rc) rc = polar_joint_enqueue(c, mp_terms(c), ind_terms(c), true, false);
@@ -124,6 +134,7 @@ static int capture(Ctx* c, hipGraphExec_t* exec, int (*fn)(Ctx*)) {
 }
 
 static int part_a_dev(Ctx* c) { return part_a(c, true); }
+static int part_a1_dev(Ctx* c) { return part_a1(c, true); }
 
 // Runs the solve's iterations until the device says it stopped; the first
 // check comes after `first` batches.  Returns the iteration count.
@@ -153,6 +164,7 @@ static int solve(Ctx* c, int first, int* iters, double* res) {
 
 static void free_graphs(Ctx* c) {
     if (c->pmd_a) { (void)hipGraphExecDestroy(c->pmd_a); c->pmd_a = nullptr; }
+    if (c->pmd_a2) { (void)hipGraphExecDestroy(c->pmd_a2); c->pmd_a2 = nullptr; }
     if (c->pmd_m) { (void)hipGraphExecDestroy(c->pmd_m); c->pmd_m = nullptr; }
     if (c->pmd_c) { (void)hipGraphExecDestroy(c->pmd_c); c->pmd_c = nullptr; }
 }
@@ -227,6 +239,7 @@ extern "C" int smd_polar_md_init(smd_ctx* c, const smd_polar_md_params* p) {
     c->mp_prec = p->prec;
     int rc = upload_vdw_tables(c, &c->pmd_vdw);
     if (rc) return rc;
+    if (p->prec == SMD_PREC_MIXED && (rc = vdw_packed_alloc(c))) return rc;
     if ((rc = smd_multipole_setup(c, &p->mp, p->mp_kcut, nullptr))) return rc;
     if ((rc = smd_induction_setup(c, &p->indf, p->ind_kcut, nullptr))) return rc;
     if (!c->d_polar_terms) SMD_HIP(c, hipMalloc(&c->d_polar_terms, sizeof(double) * 2 * POLAR_TERMS));
@@ -265,7 +278,7 @@ extern "C" int smd_polar_md_init(smd_ctx* c, const smd_polar_md_params* p) {
     double res = 0.0;
     if ((rc = solve(c, 1, &iters, &res))) return rc;
     if (polar_joint_long(c)) {
-        rc = vdw_in_a(c) ? SMD_OK : vdw_enqueue(c, &c->pmd_vdw, p->prec);
+        rc = vdw_enqueue(c, &c->pmd_vdw, p->prec);
         if (!rc) rc = polar_joint_enqueue(c, mp_terms(c), ind_terms(c), true, false);
     } else {
         rc = induction_enqueue(c, ind_terms(c));
@@ -288,18 +301,31 @@ extern "C" int smd_polar_md_run(smd_ctx* c, int64_t nsteps, smd_polar_md_report*
     c->mp_prec = c->pmd.prec;
     const bool graph = c->pmd.use_graph && !c->prof.on;
     int rc;
+    const bool split = split_mp(c);
     if (graph && !c->pmd_a) {
-        if ((rc = capture(c, &c->pmd_a, part_a_dev))) return rc;
+        if (split) {
+            if ((rc = capture(c, &c->pmd_a, part_a1_dev))) return rc;
+            if ((rc = capture(c, &c->pmd_a2, part_a2))) return rc;
+        } else if ((rc = capture(c, &c->pmd_a, part_a_dev))) {
+            return rc;
+        }
         if ((rc = capture(c, &c->pmd_c, part_c))) return rc;
     }
     int64_t total_iters = 0;
     int iters = 0;
     double res = 0.0;
-    const bool split = split_mp(c);
     for (int64_t s = 0; s < nsteps; ++s) {
-        if (graph) SMD_HIP(c, hipGraphLaunch(c->pmd_a, c->stream));
-        else if ((rc = part_a(c, true))) return rc;
-        if (split && (rc = launch_m(c, graph))) return rc;
+        if (split) {
+            // M beside A2 and the solve
+            if (graph) SMD_HIP(c, hipGraphLaunch(c->pmd_a, c->stream));
+            else if ((rc = part_a1(c, true))) return rc;
+            if ((rc = launch_m(c, graph))) return rc;
+            if (graph) SMD_HIP(c, hipGraphLaunch(c->pmd_a2, c->stream));
+            else if ((rc = part_a2(c))) return rc;
+        } else {
+            if (graph) SMD_HIP(c, hipGraphLaunch(c->pmd_a, c->stream));
+            else if ((rc = part_a(c, true))) return rc;
+        }
         // the first check after the batches the previous step needed
         const int first = (c->pmd_last_iters + 1 + c->cg_batch - 1) / c->cg_batch;
         if ((rc = solve(c, first, &iters, &res))) return rc;

@@ -196,11 +196,13 @@ struct Ctx {
     float4* d_ind_rec32 = nullptr; // mixed solve: hi/lo FP32 site records [2n]
     int4* d_ind_fix = nullptr;     // mixed A v: fixed-point positions and 1/sqrt(alpha) [n]
     double* d_e0_prev = nullptr;   // E0 of the previous solve with guess 4 [3n]
+    int4* d_vrec = nullptr;        // fixed-point virtual-site records of the mixed buffered 14-7 [n]
     double* d_pc_blk = nullptr;    // block preconditioner: inverse blocks [nmol][45]
     float* d_shell32 = nullptr;    // the learned guess's charge shells from the mixed field pass [n][24]
     bool ind_shells_valid = false; // d_shell32 holds the shells of the current solve's state
     int mp_prec = 0;               // smd_prec of the multipole / induction real-space pair math
     float4* d_site32 = nullptr;    // mixed real space: FP32 site records [3n]
+    float4* d_site32_ind = nullptr;  // the mixed field pass's copy of them [3n]
     float4* d_pind32 = nullptr;    // mixed real space: FP32 induced dipoles [n]
     double* d_dipl = nullptr;     // [3n] local-frame dipoles
     double* d_quadl = nullptr;    // [6n] local-frame quadrupoles
@@ -263,7 +265,7 @@ struct Ctx {
     bool shake_frecip = false;        // k_shake adds d_frecip to the forces it reads
     int recip_overlap = 1;
     int polar_joint = 1;            // one k-space pass for Multipole and Induction in the polarizable step
-    int polar_m_parts = 1;          // the polarizable step's part M: 1 vdW, 2 the permanent structure factors
+    int polar_m_sums = 0;           // the polarizable step's part M also sums the permanent structure factors
     int md_fused_shake = 1;           // the MD step's SHAKE and kinetics in one kernel
 
     Prof prof;
@@ -296,6 +298,7 @@ struct Ctx {
     std::vector<double> pmd_tab[4];     // sigma2, eps4, type sigma, type eps
     double* d_polar_terms = nullptr;    // [2][POLAR_TERMS]: Multipole's, Induction's accumulators
     hipGraphExec_t pmd_a = nullptr;     // captured forces up to the solve
+    hipGraphExec_t pmd_a2 = nullptr;    // captured solve start (when part M runs beside it)
     hipGraphExec_t pmd_m = nullptr;     // captured Multipole short range (second stream, beside the solve)
     hipGraphExec_t pmd_c = nullptr;     // captured induction terms and integration tail
     bool pmd_ready = false;
@@ -521,6 +524,7 @@ __device__ __forceinline__ void atomic_add3(double* f, double x, double y, doubl
 int pairlist_build(Ctx* c, double cutoff, double cut_buff, int use_cells, bool on_device_flag);
 int pairlist_alloc(Ctx* c, int32_t stride);
 int vdw_run(Ctx* c, const smd_vdw_params* p, int prec, smd_vdw_out* out, bool read_back);
+int vdw_packed_alloc(Ctx* c);
 int pairs_lj_ewald(Ctx* c, const double* sigma2, const double* eps4, double cutoff, double alpha,
                    double real_cut, int use_constraint, int lustig, int prec, bool store_forces = false);
 int pairs_lj(Ctx* c, double cutoff, int switch_on, int use_constraint, int lustig, int prec, bool store_forces);

@@ -33,22 +33,21 @@ def _polar_md(fx, ctx, use_graph, **induce_kw):
     return ind, md
 
 
-@pytest.mark.parametrize("use_graph,joint,overlap,m_parts", [(False, 1, 1, 2), (True, 1, 1, 2), (True, 0, 1, 2),
-                                                             (True, 1, 0, 2), (True, 1, 1, 0), (True, 1, 1, 1),
-                                                             (True, 1, 1, 3)])
-def test_polar_md_matches_reference(use_graph, joint, overlap, m_parts):
+@pytest.mark.parametrize("use_graph,joint,overlap,m_sums", [(False, 1, 1, 0), (True, 1, 1, 0), (True, 0, 1, 0),
+                                                            (True, 1, 0, 0), (False, 1, 1, 1), (True, 1, 1, 1)])
+def test_polar_md_matches_reference(use_graph, joint, overlap, m_sums):
     """start() + 10 steps: buffered 14-7, Multipole, Induction (the
     reference's CG from Pi = 1 to |r| < 1e-14), SHAKE, Nose-Hoover; with
     one k-space pass for both classes after the solve (joint) and with a
-    pass per class, with and without the second stream, and with each
-    choice of what runs beside the solve (Multipole's short range, with or
-    without vdW and the permanent multipoles' structure factors)."""
+    pass per class, with and without the second stream (vdW and Multipole's
+    short range beside the solve), with and without the permanent
+    multipoles' structure factors beside it."""
     fx = Fixture("amoeba216_md10_polar")
-    print(f"case amoeba216 graph {use_graph} joint {joint} overlap {overlap} m_parts {m_parts}", flush=True)
+    print(f"case amoeba216 graph {use_graph} joint {joint} overlap {overlap} m_sums {m_sums}", flush=True)
     ctx = _ctx(fx.system)
     ctx.set_tuning("polar_joint_long", joint)
     ctx.set_tuning("recip_overlap", overlap)
-    ctx.set_tuning("polar_m_parts", m_parts)
+    ctx.set_tuning("polar_m_sums", m_sums)
     ctx.set_velocities(fx.vec("init_vel"))
     _, md = _polar_md(fx, ctx, use_graph)
     md.init()
