@@ -261,12 +261,14 @@ def bench_polar_md(args):
     use_graph = not args.no_graph
     vel0 = ensemble.init_velocity(system, MD["temperature"], seed=1, per_atom=True)
 
-    def make(pos, vel, prec, guess, net_state=None, cg_batch=4):
+    def make(pos, vel, prec, guess, net_state=None):
         ctx = Context(system)
         ctx.set_positions(pos)
         ctx.set_velocities(vel)
         ctx.set_pairlist(POLAR["cutoff"], POLAR["cut_buff"], True)
-        ctx.set_tuning("cg_batch", cg_batch)
+        ctx.set_tuning("cg_batch", args.cg_batch)
+        if args.cg_lanes:
+            ctx.set_tuning("cg_lanes", args.cg_lanes)
         if args.no_graph:
             ctx.set_tuning("cg_graph", 0)   # direct launches throughout (profilers)
         if args.no_overlap:
@@ -627,6 +629,11 @@ def main():
     ap.add_argument("--polar-m-sums", type=int, default=None, choices=[0, 1],
                     help="polarizable workloads: 1 sums the permanent structure factors beside the dipole solve "
                          "(default: the library's)")
+    ap.add_argument("--cg-batch", type=int, default=4,
+                    help="polarizable workloads: solve iterations per captured batch (the host checks the stop "
+                         "flag between batches)")
+    ap.add_argument("--cg-lanes", type=int, default=0, choices=[0, 8, 16, 32],
+                    help="polarizable workloads: lanes per row of the solve's A v (0: the library's)")
     ap.add_argument("--no-graph", action="store_true",
                     help="launch each step's kernels directly instead of replaying a captured hipGraph")
     args = ap.parse_args()

@@ -152,6 +152,7 @@ static void release(Ctx* c) {
         if (p) (void)hipFree(p);
     mlp_release(c);
     if (c->h_pinned) (void)hipHostFree(c->h_pinned);
+    if (c->h_cg_done) (void)hipHostFree(c->h_cg_done);
     if (c->graph_exec) (void)hipGraphExecDestroy(c->graph_exec);
     if (c->cg_exec) (void)hipGraphExecDestroy(c->cg_exec);
     if (c->pmd_a) (void)hipGraphExecDestroy(c->pmd_a);
@@ -162,6 +163,7 @@ static void release(Ctx* c) {
     for (auto& r : c->prof.pending) { (void)hipEventDestroy(r.a); (void)hipEventDestroy(r.b); }
     if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);
     if (c->ev_join) (void)hipEventDestroy(c->ev_join);
+    if (c->ev_cg) (void)hipEventDestroy(c->ev_cg);
     if (c->stream2) (void)hipStreamDestroy(c->stream2);
     if (c->stream) (void)hipStreamDestroy(c->stream);
 }

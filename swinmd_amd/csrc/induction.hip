@@ -345,6 +345,7 @@ struct CgTop {
     double scale = 1.0;
     double* res = nullptr;
     int reason = 1;
+    int* hdone = nullptr;   // mapped host memory told of the stop (see top_stop), or null
 };
 
 // Coupling table of a solve.  A is fixed while CG runs, so the Thole factors
@@ -533,7 +534,20 @@ __device__ __forceinline__ bool top_stop(const CgTop& top) {
     const double res = sqrt(block_total(top.p_dv, top.nparts) * top.scale);
     if (top.res && blockIdx.x == 0 && threadIdx.x == 0) top.res[0] = res;
     if (res < top.tol) {
-        if (blockIdx.x == 0 && threadIdx.x == 0) { top.state[0] = 1; top.state[2] = top.reason; }
+        if (blockIdx.x == 0 && threadIdx.x == 0) {
+            // block 0 alone writes state[0], so it is the block that sees the stop
+            top.state[0] = 1;
+            top.state[2] = top.reason;
+            if (top.hdone) {
+                // the host waits on hdone[0] instead of copying the state back
+                top.hdone[1] = top.state[1];
+                top.hdone[2] = top.reason;
+                reinterpret_cast<double*>(top.hdone + 4)[0] = res;
+                __threadfence_system();
+                top.hdone[0] = 1;
+                __threadfence_system();
+            }
+        }
         return true;
     }
     return false;
@@ -768,6 +782,7 @@ struct CgArgs {
     const double* blk;  // [nmol][45]: inverse of each molecule's 9x9 block of A, lower triangle (else null)
     double* u;        // block preconditioner: M^-1 r (A applies to it; z holds A u)
     double* s;        // block preconditioner: the recurrence for A p
+    int* hdone;       // block preconditioner: mapped host flag of the stop (or null)
 };
 
 // x = guess (1 as the reference, or the caller's), then after z = A x:
@@ -1318,6 +1333,7 @@ static CgArgs cg_args(Ctx* c, const smd_induce_params* p) {
     g.z = c->d_cg + 2 * (size_t)dim;
     g.u = c->d_cg + 3 * (size_t)dim;
     g.s = c->d_cg + 4 * (size_t)dim;
+    g.hdone = c->d_cg_done;
     g.e0 = c->d_e0;
     g.polar = c->d_polar;
     g.nparts_mv = (int)(((int64_t)n * c->cg_lanes + 255) / 256);   // blocks of the A v kernel
@@ -1341,6 +1357,7 @@ static CgTop blk_top(const CgArgs& g) {
     t.scale = g.inv_npol;
     t.res = g.scal;
     t.reason = 2;
+    t.hdone = g.hdone;
     return t;
 }
 

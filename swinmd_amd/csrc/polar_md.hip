@@ -33,6 +33,7 @@
 // A (or A1 and A2), M and C hold no host synchronisation and no memset: each
 // is captured once as a hipGraph and replayed.
 #include <algorithm>
+#include <atomic>
 #include <cstring>
 #include <vector>
 
@@ -138,6 +139,12 @@ static int part_a1_dev(Ctx* c) { return part_a1(c, true); }
 
 // Runs the solve's iterations until the device says it stopped; the first
 // check comes after `first` batches.  Returns the iteration count.
+// With the block preconditioner the stopping test also writes mapped host
+// memory (c->h_cg_done, cleared by the host before each step's part A): the
+// host spins on it, and on the batches' completion event, instead of
+// synchronising the stream and copying the state back, so part C is queued
+// as soon as the device stops (behind the batch's remaining, skipping
+// kernels).
 static int solve(Ctx* c, int first, int* iters, double* res) {
     const smd_induce_params& ip = c->pmd.induce;
     if (!ip.do_min) {
@@ -146,12 +153,30 @@ static int solve(Ctx* c, int first, int* iters, double* res) {
         return SMD_OK;
     }
     const int max_it = ip.max_iter > 0 ? ip.max_iter : 3 * c->n;
+    const bool mapped = c->h_cg_done && ip.precond == 2;
+    volatile int* h = c->h_cg_done;
     int launched = 0, done = 0, reason = 0, rc;
     int batch = std::max(1, first);
     while (true) {
         if ((rc = induce_batches(c, &ip, batch))) return rc;
         launched += batch * c->cg_batch;
-        if ((rc = induce_poll(c, &done, iters, &reason, res))) return rc;
+        if (mapped) {
+            SMD_HIP(c, hipEventRecord(c->ev_cg, c->stream));
+            bool drained = false;
+            while (!h[0] && !drained) {
+                const hipError_t q = hipEventQuery(c->ev_cg);
+                if (q == hipSuccess) drained = true;
+                else if (q != hipErrorNotReady) return hip_fail(c, q, "solve batches");
+            }
+            if (h[0]) {
+                std::atomic_thread_fence(std::memory_order_acquire);
+                *iters = h[1];
+                reason = h[2];
+                *res = reinterpret_cast<volatile double*>(h + 4)[0];
+                done = 1;
+            }
+        }
+        if (!done && (rc = induce_poll(c, &done, iters, &reason, res))) return rc;
         if (done) {
             c->ind_last_converged = reason != 0;
             break;
@@ -160,6 +185,13 @@ static int solve(Ctx* c, int first, int* iters, double* res) {
         batch = 1;
     }
     return SMD_OK;
+}
+
+static void clear_stop_flag(Ctx* c) {
+    if (c->h_cg_done) {
+        reinterpret_cast<volatile int*>(c->h_cg_done)[0] = 0;
+        std::atomic_thread_fence(std::memory_order_seq_cst);
+    }
 }
 
 static void free_graphs(Ctx* c) {
@@ -249,6 +281,13 @@ extern "C" int smd_polar_md_init(smd_ctx* c, const smd_polar_md_params* p) {
         SMD_HIP(c, hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming));
         SMD_HIP(c, hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming));
     }
+    if (!c->ev_cg) SMD_HIP(c, hipEventCreateWithFlags(&c->ev_cg, hipEventDisableTiming));
+    if (!c->h_cg_done) {
+        // the solve's stop flag in mapped (pinned, fine-grained) host memory
+        SMD_HIP(c, hipHostMalloc((void**)&c->h_cg_done, 64, hipHostMallocMapped | hipHostMallocCoherent));
+        std::memset(c->h_cg_done, 0, 64);
+        SMD_HIP(c, hipHostGetDevicePointer((void**)&c->d_cg_done, c->h_cg_done, 0));
+    }
     if (!c->d_stress) SMD_HIP(c, hipMalloc(&c->d_stress, sizeof(double) * 12));
     if ((rc = pairlist_build(c, p->cutoff, p->cut_buff, p->use_cells, false))) return rc;
     {
@@ -273,6 +312,7 @@ extern "C" int smd_polar_md_init(smd_ctx* c, const smd_polar_md_params* p) {
     SMD_HIP(c, hipMemsetAsync(c->d_flag, 0, sizeof(int) * 8, c->stream));
     // start(): the forces at t (a full evaluation with the list just built),
     // then vtMinus = v - dt/2 F/m
+    clear_stop_flag(c);
     if ((rc = part_a(c, false))) return rc;
     int iters = 0;
     double res = 0.0;
@@ -315,6 +355,7 @@ extern "C" int smd_polar_md_run(smd_ctx* c, int64_t nsteps, smd_polar_md_report*
     int iters = 0;
     double res = 0.0;
     for (int64_t s = 0; s < nsteps; ++s) {
+        clear_stop_flag(c);
         if (split) {
             // M beside A2 and the solve
             if (graph) SMD_HIP(c, hipGraphLaunch(c->pmd_a, c->stream));

@@ -138,6 +138,9 @@ struct Ctx {
     int nb_mols64 = 0;            // blocks of 64 over the molecules
     double* d_scalars = nullptr;  // [8]: fnh, fnh_minus, ...
     double* h_pinned = nullptr;   // pinned staging for small results
+    int* h_cg_done = nullptr;     // mapped host memory the solve's stopping test writes: [0] stopped,
+                                  // [1] iterations, [2] reason, [4..5] the residual (double)
+    int* d_cg_done = nullptr;     // its device address
 
     // pair list (ELL rows, reference order within a row)
     int32_t* d_cell_of = nullptr;
@@ -259,6 +262,7 @@ struct Ctx {
     hipStream_t stream2 = nullptr;
     hipStream_t polar_main = nullptr;   // the step's stream while a polarizable k-space fork is open
     hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+    hipEvent_t ev_cg = nullptr;       // after the polarizable step's solve batches
     hipEvent_t list_fork = nullptr;   // recorded by the list build after its preamble when set
     double* d_frecip = nullptr;       // [3n]
     bool recip_store = false;         // k_recip_force stores into d_frecip instead of adding
