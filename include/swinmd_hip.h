@@ -548,9 +548,10 @@ int smd_polar_md_init(smd_ctx* ctx, const smd_polar_md_params* p);
 int smd_polar_md_run(smd_ctx* ctx, int64_t nsteps, smd_polar_md_report* out);
 /* Average microseconds of one launch of the polarizable step's real-space
  * kernel "mp_real" (Multipole::real_term) or "ind_real" (Induction::
- * real_term), in the step's precision, reps launches back to back on the
- * current state; the step's forces and sums are left unchanged.  No
- * counterpart in the reference. */
+ * real_term), in the step's precision and form (the mixed "ind_real" over
+ * the solve's coupling table when the step uses it), reps launches back to
+ * back on the current state; the step's forces and sums are left
+ * unchanged.  No counterpart in the reference. */
 int smd_polar_time_kernel(smd_ctx* ctx, const char* name, int reps, double* avg_us);
 
 /* ---- checkpoint state ---------------------------------------------------
@@ -592,7 +593,7 @@ int smd_md_time_family(smd_ctx* ctx, const char* family, int reps, double* avg_u
  * "polar_joint_long" (0/1, default 1): the polarizable step runs one k-space
  * pass for Multipole and Induction after the dipole solve when both use the
  * same alpha and kcut (0: a pass per class).
- * "polar_m_sums" (0/1, default 0): with the joint pass and the second
+ * "polar_m_sums" (0/1, default 1): with the joint pass and the second
  * stream, the permanent multipoles' structure factors are summed beside the
  * dipole solve with vdW and Multipole's short range (0: after the solve, in
  * the joint pass).

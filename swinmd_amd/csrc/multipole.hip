@@ -61,6 +61,11 @@ enum MpSlot {
 struct MpArgs {
     int n, nmol, stride, K, chunks, per;
     Box box;
+    // the mixed solve's coupling table (the partners inside cutoffInd, the
+    // intramolecular ones included), when Induction's real space walks it
+    const int32_t* tcnt;
+    const int32_t* tj;
+    int tstride;
     const int32_t* mol_start;
     const int32_t* mol;
     const double* pos;
@@ -336,6 +341,7 @@ __global__ __launch_bounds__(256) void k_mp_real(MpArgs a) {
 }
 
 template <int G, typename R> __global__ __launch_bounds__(256) void k_ind_real(MpArgs a);
+template <int G> __global__ __launch_bounds__(256) void k_ind_real_tab(MpArgs a);
 __global__ __launch_bounds__(256) void k_pack_sites32(MpArgs a, float4* __restrict__ site32,
                                                       float4* __restrict__ pind32, int with_pind);
 
@@ -347,7 +353,8 @@ static int launch_real(Ctx* c, const MpArgs& a, bool induction) {
     if (c->mp_prec == SMD_PREC_MIXED) {
         hipLaunchKernelGGL(k_pack_sites32, dim3((c->n + 255) / 256), b256, 0, c->stream, a, c->d_site32, c->d_pind32,
                            induction ? 1 : 0);
-        if (induction) hipLaunchKernelGGL((k_ind_real<MP_LANES, float>), gr, b256, 0, c->stream, a);
+        if (induction && a.tj) hipLaunchKernelGGL((k_ind_real_tab<MP_LANES>), gr, b256, 0, c->stream, a);
+        else if (induction) hipLaunchKernelGGL((k_ind_real<MP_LANES, float>), gr, b256, 0, c->stream, a);
         else hipLaunchKernelGGL((k_mp_real<MP_LANES, float>), gr, b256, 0, c->stream, a);
     } else {
         if (induction) hipLaunchKernelGGL((k_ind_real<MP_LANES, double>), gr, b256, 0, c->stream, a);
@@ -1274,6 +1281,50 @@ __global__ __launch_bounds__(256) void k_ind_real(MpArgs a) {
     block_part<11>(acc, a.red, RS_IND_REAL);
 }
 
+// The mixed real space over the solve's coupling table instead of the list:
+// the table's rows hold exactly the partners inside cutoffInd (equal to the
+// real-space cutoff when this kernel is used), so no lane walks the list
+// buffer's partners beyond it; the intramolecular partners (the list's
+// excluded ones) are the same-molecule entries, at scale 0.  Every entry
+// keeps the list kernel's cutoff test and half-weight sums.
+template <int G>
+__global__ __launch_bounds__(256) void k_ind_real_tab(MpArgs a) {
+    const int tid = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = tid / G, sub = tid % G;
+    double acc[11];
+#pragma unroll
+    for (int k = 0; k < 11; ++k) acc[k] = 0.0;
+    V3 f = v3(0, 0, 0), t = v3(0, 0, 0);
+    if (i < a.n) {
+        const int cnt = a.tcnt[i];
+        const int32_t* row = a.tj + (int64_t)i * a.tstride;
+        const SiteT<float> I = site_r<float>(a, i);
+        const V3T<float> pi = pind_r<float>(a, i);
+        const float poli = polar_r<float>(a, i);
+        const V3 ri = ldv(a.pos + 3 * i);
+        const int mi = i / 3;
+        for (int k = sub; k < cnt; k += G) {
+            const int j = row[k];
+            ind_entry<float>(a, I, pi, poli, ri, j, j / 3 == mi ? 0.0f : 1.0f, f, t, acc);
+        }
+    }
+#pragma unroll
+    for (int o = G / 2; o > 0; o >>= 1) {
+        f.x += __shfl_xor(f.x, o, G); f.y += __shfl_xor(f.y, o, G); f.z += __shfl_xor(f.z, o, G);
+        t.x += __shfl_xor(t.x, o, G); t.y += __shfl_xor(t.y, o, G); t.z += __shfl_xor(t.z, o, G);
+    }
+    if (i < a.n && sub == 0) {
+        a.force[3 * i] += COULOMBS * f.x;
+        a.force[3 * i + 1] += COULOMBS * f.y;
+        a.force[3 * i + 2] += COULOMBS * f.z;
+        a.trq[3 * i] = COULOMBS * t.x;
+        a.trq[3 * i + 1] = COULOMBS * t.y;
+        a.trq[3 * i + 2] = COULOMBS * t.z;
+    }
+    acc[1] *= COULOMBS;
+    block_part<11>(acc, a.red, RS_IND_REAL);
+}
+
 // Per site: self energy and torque (:688-760) and the surface sums
 // (:2222-2250).
 __global__ __launch_bounds__(256) void k_ind_site_sums(MpArgs a) {
@@ -2058,6 +2109,11 @@ int polar_joint_enqueue(Ctx* c, double* mp_t, double* ind_t, bool with_mp_short,
     MpArgs am = mp_args(c), ai = ind_args(c);
     am.terms = mp_t;
     ai.terms = ind_t;
+    if (c->ind_real_table && c->mp_prec == SMD_PREC_MIXED) {
+        ai.tcnt = c->d_it_cnt;
+        ai.tj = c->d_it_j;
+        ai.tstride = c->it_stride;
+    }
     const dim3 b256(256), ga((c->n + 255) / 256), gm((c->nmol + 255) / 256);
     if (with_mp_short) hipLaunchKernelGGL(k_zero_terms, dim3(1), b256, 0, c->stream, mp_t);
     hipLaunchKernelGGL(k_zero_terms, dim3(1), b256, 0, c->stream, ind_t);
@@ -2211,6 +2267,15 @@ extern "C" int smd_polar_time_kernel(smd_ctx* c, const char* name, int reps, dou
     MpArgs a = ind ? ind_args(c) : mp_args(c);
     a.force = c->d_fvirt;
     a.terms = c->d_polar_terms + 2 * POLAR_TERMS;
+    // the mixed induction real space the step runs: over the solve's table
+    // when its cutoff is the real-space one
+    const bool tab = ind && c->mp_prec == SMD_PREC_MIXED && c->ind_table_built && c->pmd.induce.do_min &&
+                     c->pmd.induce.prec == SMD_PREC_MIXED && c->pmd.induce.cutoff_ind == c->indf.real_cut;
+    if (tab) {
+        a.tcnt = c->d_it_cnt;
+        a.tj = c->d_it_j;
+        a.tstride = c->it_stride;
+    }
     const dim3 gr((unsigned)(((int64_t)c->n * MP_LANES + 255) / 256)), b256(256);
     const bool mixed = c->mp_prec == SMD_PREC_MIXED;
     if (mixed)
@@ -2221,7 +2286,8 @@ extern "C" int smd_polar_time_kernel(smd_ctx* c, const char* name, int reps, dou
     SMD_HIP(c, hipEventRecord(e0, c->stream));
     for (int r = 0; r < reps; ++r) {
         if (ind) {
-            if (mixed) hipLaunchKernelGGL((k_ind_real<MP_LANES, float>), gr, b256, 0, c->stream, a);
+            if (tab) hipLaunchKernelGGL((k_ind_real_tab<MP_LANES>), gr, b256, 0, c->stream, a);
+            else if (mixed) hipLaunchKernelGGL((k_ind_real<MP_LANES, float>), gr, b256, 0, c->stream, a);
             else hipLaunchKernelGGL((k_ind_real<MP_LANES, double>), gr, b256, 0, c->stream, a);
         } else {
             if (mixed) hipLaunchKernelGGL((k_mp_real<MP_LANES, float>), gr, b256, 0, c->stream, a);

@@ -97,18 +97,28 @@ static int part_m(Ctx* c) {
     return rc;
 }
 
+// Induction's mixed real space walks the solve's coupling table (built this
+// step by the mixed field pass) when its cutoff is the table's.
+static bool real_on_table(Ctx* c) {
+    const smd_polar_md_params& p = c->pmd;
+    return p.prec == SMD_PREC_MIXED && p.induce.do_min && p.induce.prec == SMD_PREC_MIXED &&
+           p.induce.cutoff_ind == c->indf.real_cut;
+}
+
 // Part C: induction terms (with the joint pass, both classes; Multipole's
 // short range unless part M ran it) and the integration tail.
 static int part_c(Ctx* c) {
     int rc;
     {
         RedDefer defer(c);
+        c->ind_real_table = real_on_table(c);
         if (!polar_joint_long(c)) rc = induction_enqueue(c, ind_terms(c));
         else if (split_mp(c)) rc = polar_joint_enqueue(c, mp_terms(c), ind_terms(c), false, c->polar_m_sums != 0);
         else {
             rc = vdw_enqueue(c, &c->pmd_vdw, c->pmd.prec);
             if (!rc) rc = polar_joint_enqueue(c, mp_terms(c), ind_terms(c), true, false);
         }
+        c->ind_real_table = false;
         if (!rc) rc = red_flush(c);
     }
     if (!rc) rc = md_tail_enqueue(c);
@@ -319,7 +329,9 @@ extern "C" int smd_polar_md_init(smd_ctx* c, const smd_polar_md_params* p) {
     if ((rc = solve(c, 1, &iters, &res))) return rc;
     if (polar_joint_long(c)) {
         rc = vdw_enqueue(c, &c->pmd_vdw, p->prec);
+        c->ind_real_table = real_on_table(c);
         if (!rc) rc = polar_joint_enqueue(c, mp_terms(c), ind_terms(c), true, false);
+        c->ind_real_table = false;
     } else {
         rc = induction_enqueue(c, ind_terms(c));
     }

@@ -269,7 +269,7 @@ struct Ctx {
     bool shake_frecip = false;        // k_shake adds d_frecip to the forces it reads
     int recip_overlap = 1;
     int polar_joint = 1;            // one k-space pass for Multipole and Induction in the polarizable step
-    int polar_m_sums = 0;           // the polarizable step's part M also sums the permanent structure factors
+    int polar_m_sums = 1;           // the polarizable step's part M also sums the permanent structure factors
     int md_fused_shake = 1;           // the MD step's SHAKE and kinetics in one kernel
 
     Prof prof;
@@ -310,6 +310,7 @@ struct Ctx {
     smd_induce_params ind_last{};   // parameters of the last smd_induce
     bool ind_last_converged = false;
     bool ind_table_built = false;   // a minimised solve has filled the coupling table
+    bool ind_real_table = false;    // Induction's mixed real space may walk the current solve's coupling table
 };
 
 }  // namespace smd
