@@ -159,11 +159,14 @@ static void release(Ctx* c) {
     if (c->pmd_c) (void)hipGraphExecDestroy(c->pmd_c);
     if (c->pmd_m) (void)hipGraphExecDestroy(c->pmd_m);
     if (c->pmd_a2) (void)hipGraphExecDestroy(c->pmd_a2);
+    if (c->pmd_a3) (void)hipGraphExecDestroy(c->pmd_a3);
+    if (c->pmd_m2) (void)hipGraphExecDestroy(c->pmd_m2);
     if (c->graph) (void)hipGraphDestroy(c->graph);
     for (auto& r : c->prof.pending) { (void)hipEventDestroy(r.a); (void)hipEventDestroy(r.b); }
     if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);
     if (c->ev_join) (void)hipEventDestroy(c->ev_join);
     if (c->ev_cg) (void)hipEventDestroy(c->ev_cg);
+    if (c->ev_fork2) (void)hipEventDestroy(c->ev_fork2);
     if (c->stream2) (void)hipStreamDestroy(c->stream2);
     if (c->stream) (void)hipStreamDestroy(c->stream);
 }
@@ -441,6 +444,8 @@ extern "C" int smd_set_tuning(smd_ctx* c, const char* name, int value) {
     if (c->pmd_c) { (void)hipGraphExecDestroy(c->pmd_c); c->pmd_c = nullptr; }
     if (c->pmd_m) { (void)hipGraphExecDestroy(c->pmd_m); c->pmd_m = nullptr; }
     if (c->pmd_a2) { (void)hipGraphExecDestroy(c->pmd_a2); c->pmd_a2 = nullptr; }
+    if (c->pmd_a3) { (void)hipGraphExecDestroy(c->pmd_a3); c->pmd_a3 = nullptr; }
+    if (c->pmd_m2) { (void)hipGraphExecDestroy(c->pmd_m2); c->pmd_m2 = nullptr; }
     return SMD_OK;
 }
 

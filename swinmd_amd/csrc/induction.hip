@@ -1372,7 +1372,9 @@ static void solve_apply(Ctx* c, const smd_induce_params* p, const IndArgs& a, co
 // `frames`), the permanent field, the coupling table, the start, A x and the
 // residual.  Without minimisation it also sets Pi = alpha E0, and the solve
 // is complete.  Launches only (capturable once the table is sized).
-int induce_begin(Ctx* c, const smd_induce_params* p, bool frames) {
+// The solve's field pass: E0 and, when minimising, the coupling table (and
+// the learned guess's shells).
+int induce_field(Ctx* c, const smd_induce_params* p, bool frames) {
     int rc = induce_check(c, p);
     if (rc) return rc;
     c->ind_last_converged = false;
@@ -1399,6 +1401,19 @@ int induce_begin(Ctx* c, const smd_induce_params* p, bool frames) {
             if (p->do_min) hipLaunchKernelGGL((k_ind_table<IND_LANES>), grid_rows, b256, 0, c->stream, a);
         }
     }
+    SMD_HIP(c, hipGetLastError());
+    return SMD_OK;
+}
+
+// The solve's start after induce_field: the guess, A x, the residual and the
+// preconditioner (without minimisation, Pi = alpha E0).
+int induce_start(Ctx* c, const smd_induce_params* p) {
+    int rc;
+    IndArgs a = ind_args(c, p);
+    const int n = c->n, dim = 3 * n;
+    const dim3 b256(256);
+    const dim3 grid_vec((dim + CG_THREADS - 1) / CG_THREADS);
+    (void)n;
     if (!p->do_min) {
         hipLaunchKernelGGL(k_direct, grid_vec, b256, 0, c->stream, dim, c->d_polar, c->d_e0, c->d_mu);
         SMD_HIP(c, hipGetLastError());
@@ -1433,6 +1448,12 @@ int induce_begin(Ctx* c, const smd_induce_params* p, bool frames) {
     else hipLaunchKernelGGL(k_cg_start, grid_vec, dim3(CG_THREADS), 0, c->stream, g);
     SMD_HIP(c, hipGetLastError());
     return SMD_OK;
+}
+
+int induce_begin(Ctx* c, const smd_induce_params* p, bool frames) {
+    int rc = induce_field(c, p, frames);
+    if (!rc) rc = induce_start(c, p);
+    return rc;
 }
 
 // count CG iterations (from an even one): A v, the step, the direction (the

@@ -277,7 +277,7 @@ template <typename R> __device__ __forceinline__ SiteT<R> site_r(const MpArgs& a
 
 template <bool FULL, typename R>
 __device__ __forceinline__ void real_entry(const MpArgs& a, const SiteT<R>& I, V3 ri, int j, V3& f, V3& t,
-                                           double (&acc)[11]) {
+                                           double (&acc)[11], double w = 1.0) {
     double dx = ri.x - a.pos[3 * j], dy = ri.y - a.pos[3 * j + 1], dz = ri.z - a.pos[3 * j + 2];
     pbc(dx, dy, dz, a.box);
     const double r2d = dx * dx + dy * dy + dz * dz;
@@ -294,8 +294,8 @@ __device__ __forceinline__ void real_entry(const MpArgs& a, const SiteT<R>& I, V
     f = f + to64(fij);
     if (a.torques) t = t + to64(B[1] * o.Gt[1] + B[2] * o.Gt[2] + B[3] * o.Gt[3] + B[4] * o.Gt[4]);
     if constexpr (FULL) {
-        acc[0] += (double)(o.G[0] * B[0] + o.G[1] * B[1] + o.G[2] * B[2] + o.G[3] * B[3] + o.G[4] * B[4]);
-        const V3 fd = to64(fij), rd = to64(r);
+        acc[0] += w * (double)(o.G[0] * B[0] + o.G[1] * B[1] + o.G[2] * B[2] + o.G[3] * B[3] + o.G[4] * B[4]);
+        const V3 fd = w * to64(fij), rd = to64(r);
         acc[1] += -dot(fd, rd);
         acc[2] += fd.x * rd.x; acc[3] += fd.y * rd.x; acc[4] += fd.z * rd.x;
         acc[5] += fd.x * rd.y; acc[6] += fd.y * rd.y; acc[7] += fd.z * rd.y;
@@ -340,6 +340,49 @@ __global__ __launch_bounds__(256) void k_mp_real(MpArgs a) {
     block_part<11>(acc, a.red, RS_MP_REAL);
 }
 
+// The mixed real space over the solve's coupling table instead of the list
+// (the table's rows hold exactly the partners inside cutoffInd, equal to the
+// real-space cutoff when this kernel is used): the intramolecular entries
+// are skipped (k_mp_intra has them), and every pair, seen from both rows,
+// adds energy, virial and dU/dV at half weight.
+template <int G>
+__global__ __launch_bounds__(256) void k_mp_real_tab(MpArgs a) {
+    const int tid = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = tid / G, sub = tid % G;
+    double acc[11];
+#pragma unroll
+    for (int k = 0; k < 11; ++k) acc[k] = 0.0;
+    V3 f = v3(0, 0, 0), t = v3(0, 0, 0);
+    if (i < a.n) {
+        const int cnt = a.tcnt[i];
+        const int32_t* row = a.tj + (int64_t)i * a.tstride;
+        const SiteT<float> I = site_r<float>(a, i);
+        const V3 ri = ldv(a.pos + 3 * i);
+        const int mi = i / 3;
+        for (int k = sub; k < cnt; k += G) {
+            const int j = row[k];
+            if (j / 3 != mi) real_entry<true, float>(a, I, ri, j, f, t, acc, 0.5);
+        }
+    }
+#pragma unroll
+    for (int o = G / 2; o > 0; o >>= 1) {
+        f.x += __shfl_xor(f.x, o, G); f.y += __shfl_xor(f.y, o, G); f.z += __shfl_xor(f.z, o, G);
+        t.x += __shfl_xor(t.x, o, G); t.y += __shfl_xor(t.y, o, G); t.z += __shfl_xor(t.z, o, G);
+    }
+    if (i < a.n && sub == 0) {
+        a.force[3 * i] += COULOMBS * f.x;
+        a.force[3 * i + 1] += COULOMBS * f.y;
+        a.force[3 * i + 2] += COULOMBS * f.z;
+        double* tr = a.trq;   // real-space part
+        tr[3 * i] = COULOMBS * t.x;
+        tr[3 * i + 1] = COULOMBS * t.y;
+        tr[3 * i + 2] = COULOMBS * t.z;
+    }
+    acc[0] *= COULOMBS;
+    acc[1] *= COULOMBS;
+    block_part<11>(acc, a.red, RS_MP_REAL);
+}
+
 template <int G, typename R> __global__ __launch_bounds__(256) void k_ind_real(MpArgs a);
 template <int G> __global__ __launch_bounds__(256) void k_ind_real_tab(MpArgs a);
 __global__ __launch_bounds__(256) void k_pack_sites32(MpArgs a, float4* __restrict__ site32,
@@ -355,6 +398,7 @@ static int launch_real(Ctx* c, const MpArgs& a, bool induction) {
                            induction ? 1 : 0);
         if (induction && a.tj) hipLaunchKernelGGL((k_ind_real_tab<MP_LANES>), gr, b256, 0, c->stream, a);
         else if (induction) hipLaunchKernelGGL((k_ind_real<MP_LANES, float>), gr, b256, 0, c->stream, a);
+        else if (a.tj) hipLaunchKernelGGL((k_mp_real_tab<MP_LANES>), gr, b256, 0, c->stream, a);
         else hipLaunchKernelGGL((k_mp_real<MP_LANES, float>), gr, b256, 0, c->stream, a);
     } else {
         if (induction) hipLaunchKernelGGL((k_ind_real<MP_LANES, double>), gr, b256, 0, c->stream, a);
@@ -2079,9 +2123,18 @@ bool polar_joint_long(Ctx* c) {
 // Multipole's short-range kernels of the polarizable step with its terms
 // cleared first, and (with_sums) the permanent multipoles' structure-factor
 // sums: none needs the dipoles, so the step can run them beside the solve.
+// Points a's real space at the solve's coupling table when the caller allows it.
+static void use_table(Ctx* c, MpArgs& a, bool allowed) {
+    if (!allowed || c->mp_prec != SMD_PREC_MIXED) return;
+    a.tcnt = c->d_it_cnt;
+    a.tj = c->d_it_j;
+    a.tstride = c->it_stride;
+}
+
 int polar_mp_short_enqueue(Ctx* c, double* mp_t, bool with_sums) {
     MpArgs am = mp_args(c);
     am.terms = mp_t;
+    use_table(c, am, c->mp_real_table);
     hipLaunchKernelGGL(k_zero_terms, dim3(1), dim3(256), 0, c->stream, mp_t);
     int rc = mp_short_enqueue(c, am);
     if (rc || !with_sums) return rc;
@@ -2109,11 +2162,8 @@ int polar_joint_enqueue(Ctx* c, double* mp_t, double* ind_t, bool with_mp_short,
     MpArgs am = mp_args(c), ai = ind_args(c);
     am.terms = mp_t;
     ai.terms = ind_t;
-    if (c->ind_real_table && c->mp_prec == SMD_PREC_MIXED) {
-        ai.tcnt = c->d_it_cnt;
-        ai.tj = c->d_it_j;
-        ai.tstride = c->it_stride;
-    }
+    use_table(c, am, c->mp_real_table);
+    use_table(c, ai, c->ind_real_table);
     const dim3 b256(256), ga((c->n + 255) / 256), gm((c->nmol + 255) / 256);
     if (with_mp_short) hipLaunchKernelGGL(k_zero_terms, dim3(1), b256, 0, c->stream, mp_t);
     hipLaunchKernelGGL(k_zero_terms, dim3(1), b256, 0, c->stream, ind_t);
@@ -2269,8 +2319,9 @@ extern "C" int smd_polar_time_kernel(smd_ctx* c, const char* name, int reps, dou
     a.terms = c->d_polar_terms + 2 * POLAR_TERMS;
     // the mixed induction real space the step runs: over the solve's table
     // when its cutoff is the real-space one
-    const bool tab = ind && c->mp_prec == SMD_PREC_MIXED && c->ind_table_built && c->pmd.induce.do_min &&
-                     c->pmd.induce.prec == SMD_PREC_MIXED && c->pmd.induce.cutoff_ind == c->indf.real_cut;
+    const bool tab = c->mp_prec == SMD_PREC_MIXED && c->ind_table_built && c->pmd.induce.do_min &&
+                     c->pmd.induce.prec == SMD_PREC_MIXED &&
+                     c->pmd.induce.cutoff_ind == (ind ? c->indf.real_cut : c->mp.real_cut);
     if (tab) {
         a.tcnt = c->d_it_cnt;
         a.tj = c->d_it_j;
@@ -2290,7 +2341,8 @@ extern "C" int smd_polar_time_kernel(smd_ctx* c, const char* name, int reps, dou
             else if (mixed) hipLaunchKernelGGL((k_ind_real<MP_LANES, float>), gr, b256, 0, c->stream, a);
             else hipLaunchKernelGGL((k_ind_real<MP_LANES, double>), gr, b256, 0, c->stream, a);
         } else {
-            if (mixed) hipLaunchKernelGGL((k_mp_real<MP_LANES, float>), gr, b256, 0, c->stream, a);
+            if (tab) hipLaunchKernelGGL((k_mp_real_tab<MP_LANES>), gr, b256, 0, c->stream, a);
+            else if (mixed) hipLaunchKernelGGL((k_mp_real<MP_LANES, float>), gr, b256, 0, c->stream, a);
             else hipLaunchKernelGGL((k_mp_real<MP_LANES, double>), gr, b256, 0, c->stream, a);
         }
     }

@@ -70,8 +70,10 @@ static int part_a(Ctx* c, bool on_device_rebuild) {
     return rc;
 }
 
-// The split step's part A in two: A1 (head, gated list rebuild, lab frames),
-// after which part M starts on the second stream, and A2 (the solve's start).
+// The split step's part A in three: A1 (head, gated list rebuild, lab
+// frames), after which vdW (M1) starts on the second stream; A2 (the solve's
+// field pass and coupling table), after which Multipole's short range (M2)
+// follows on the second stream; A3 (the solve's start).
 static int part_a1(Ctx* c, bool on_device_rebuild) {
     const smd_polar_md_params& p = c->pmd;
     RedDefer defer(c);
@@ -84,25 +86,40 @@ static int part_a1(Ctx* c, bool on_device_rebuild) {
 
 static int part_a2(Ctx* c) {
     RedDefer defer(c);
-    int rc = induce_begin(c, &c->pmd.induce, false);
+    int rc = induce_field(c, &c->pmd.induce, false);
     if (!rc) rc = red_flush(c);
     return rc;
 }
 
-static int part_m(Ctx* c) {
+static int part_a3(Ctx* c) {
     RedDefer defer(c);
-    int rc = vdw_enqueue(c, &c->pmd_vdw, c->pmd.prec);
-    if (!rc) rc = polar_mp_short_enqueue(c, mp_terms(c), c->polar_m_sums != 0);
+    int rc = induce_start(c, &c->pmd.induce);
     if (!rc) rc = red_flush(c);
     return rc;
 }
 
-// Induction's mixed real space walks the solve's coupling table (built this
-// step by the mixed field pass) when its cutoff is the table's.
-static bool real_on_table(Ctx* c) {
+// The mixed real-space kernels walk the solve's coupling table (built this
+// step by the mixed field pass) when their cutoff is the table's.
+static bool table_real(Ctx* c, double real_cut) {
     const smd_polar_md_params& p = c->pmd;
     return p.prec == SMD_PREC_MIXED && p.induce.do_min && p.induce.prec == SMD_PREC_MIXED &&
-           p.induce.cutoff_ind == c->indf.real_cut;
+           p.induce.cutoff_ind == real_cut;
+}
+
+static int part_m1(Ctx* c) {
+    RedDefer defer(c);
+    int rc = vdw_enqueue(c, &c->pmd_vdw, c->pmd.prec);
+    if (!rc) rc = red_flush(c);
+    return rc;
+}
+
+static int part_m2(Ctx* c) {
+    RedDefer defer(c);
+    c->mp_real_table = table_real(c, c->mp.real_cut);
+    int rc = polar_mp_short_enqueue(c, mp_terms(c), c->polar_m_sums != 0);
+    c->mp_real_table = false;
+    if (!rc) rc = red_flush(c);
+    return rc;
 }
 
 // Part C: induction terms (with the joint pass, both classes; Multipole's
@@ -111,7 +128,8 @@ static int part_c(Ctx* c) {
     int rc;
     {
         RedDefer defer(c);
-        c->ind_real_table = real_on_table(c);
+        c->ind_real_table = table_real(c, c->indf.real_cut);
+        c->mp_real_table = table_real(c, c->mp.real_cut);
         if (!polar_joint_long(c)) rc = induction_enqueue(c, ind_terms(c));
         else if (split_mp(c)) rc = polar_joint_enqueue(c, mp_terms(c), ind_terms(c), false, c->polar_m_sums != 0);
         else {
@@ -119,6 +137,7 @@ static int part_c(Ctx* c) {
             if (!rc) rc = polar_joint_enqueue(c, mp_terms(c), ind_terms(c), true, false);
         }
         c->ind_real_table = false;
+        c->mp_real_table = false;
         if (!rc) rc = red_flush(c);
     }
     if (!rc) rc = md_tail_enqueue(c);
@@ -207,32 +226,32 @@ static void clear_stop_flag(Ctx* c) {
 static void free_graphs(Ctx* c) {
     if (c->pmd_a) { (void)hipGraphExecDestroy(c->pmd_a); c->pmd_a = nullptr; }
     if (c->pmd_a2) { (void)hipGraphExecDestroy(c->pmd_a2); c->pmd_a2 = nullptr; }
+    if (c->pmd_a3) { (void)hipGraphExecDestroy(c->pmd_a3); c->pmd_a3 = nullptr; }
+    if (c->pmd_m2) { (void)hipGraphExecDestroy(c->pmd_m2); c->pmd_m2 = nullptr; }
     if (c->pmd_m) { (void)hipGraphExecDestroy(c->pmd_m); c->pmd_m = nullptr; }
     if (c->pmd_c) { (void)hipGraphExecDestroy(c->pmd_c); c->pmd_c = nullptr; }
 }
 
-// Part M on the second stream (captured there, or launched directly): it
-// waits for part A and part C waits for it.
-static int launch_m(Ctx* c, bool graph) {
-    SMD_HIP(c, hipEventRecord(c->ev_fork, c->stream));
-    SMD_HIP(c, hipStreamWaitEvent(c->stream2, c->ev_fork, 0));
+// A part on the second stream (captured there, or launched directly), after
+// the work enqueued so far on the step's stream (event `ev`); with `join`,
+// ev_join marks its end for part C.
+static int launch_side(Ctx* c, hipGraphExec_t* exec, int (*fn)(Ctx*), hipEvent_t ev, bool graph, bool join) {
+    SMD_HIP(c, hipEventRecord(ev, c->stream));
+    SMD_HIP(c, hipStreamWaitEvent(c->stream2, ev, 0));
+    hipStream_t main = c->stream;
+    int rc = SMD_OK;
+    c->stream = c->stream2;
     if (graph) {
-        if (!c->pmd_m) {
-            hipStream_t main = c->stream;
-            c->stream = c->stream2;
-            const int rc = capture(c, &c->pmd_m, part_m);
-            c->stream = main;
-            if (rc) return rc;
-        }
-        SMD_HIP(c, hipGraphLaunch(c->pmd_m, c->stream2));
+        if (!*exec) rc = capture(c, exec, fn);
+        c->stream = main;
+        if (rc) return rc;
+        SMD_HIP(c, hipGraphLaunch(*exec, c->stream2));
     } else {
-        hipStream_t main = c->stream;
-        c->stream = c->stream2;
-        const int rc = part_m(c);
+        rc = fn(c);
         c->stream = main;
         if (rc) return rc;
     }
-    SMD_HIP(c, hipEventRecord(c->ev_join, c->stream2));
+    if (join) SMD_HIP(c, hipEventRecord(c->ev_join, c->stream2));
     return SMD_OK;
 }
 
@@ -292,6 +311,7 @@ extern "C" int smd_polar_md_init(smd_ctx* c, const smd_polar_md_params* p) {
         SMD_HIP(c, hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming));
     }
     if (!c->ev_cg) SMD_HIP(c, hipEventCreateWithFlags(&c->ev_cg, hipEventDisableTiming));
+    if (!c->ev_fork2) SMD_HIP(c, hipEventCreateWithFlags(&c->ev_fork2, hipEventDisableTiming));
     if (!c->h_cg_done) {
         // the solve's stop flag in mapped (pinned, fine-grained) host memory
         SMD_HIP(c, hipHostMalloc((void**)&c->h_cg_done, 64, hipHostMallocMapped | hipHostMallocCoherent));
@@ -329,9 +349,11 @@ extern "C" int smd_polar_md_init(smd_ctx* c, const smd_polar_md_params* p) {
     if ((rc = solve(c, 1, &iters, &res))) return rc;
     if (polar_joint_long(c)) {
         rc = vdw_enqueue(c, &c->pmd_vdw, p->prec);
-        c->ind_real_table = real_on_table(c);
+        c->ind_real_table = table_real(c, c->indf.real_cut);
+        c->mp_real_table = table_real(c, c->mp.real_cut);
         if (!rc) rc = polar_joint_enqueue(c, mp_terms(c), ind_terms(c), true, false);
         c->ind_real_table = false;
+        c->mp_real_table = false;
     } else {
         rc = induction_enqueue(c, ind_terms(c));
     }
@@ -358,6 +380,7 @@ extern "C" int smd_polar_md_run(smd_ctx* c, int64_t nsteps, smd_polar_md_report*
         if (split) {
             if ((rc = capture(c, &c->pmd_a, part_a1_dev))) return rc;
             if ((rc = capture(c, &c->pmd_a2, part_a2))) return rc;
+            if ((rc = capture(c, &c->pmd_a3, part_a3))) return rc;
         } else if ((rc = capture(c, &c->pmd_a, part_a_dev))) {
             return rc;
         }
@@ -369,12 +392,15 @@ extern "C" int smd_polar_md_run(smd_ctx* c, int64_t nsteps, smd_polar_md_report*
     for (int64_t s = 0; s < nsteps; ++s) {
         clear_stop_flag(c);
         if (split) {
-            // M beside A2 and the solve
+            // M1 beside A2, M2 beside A3 and the solve
             if (graph) SMD_HIP(c, hipGraphLaunch(c->pmd_a, c->stream));
             else if ((rc = part_a1(c, true))) return rc;
-            if ((rc = launch_m(c, graph))) return rc;
+            if ((rc = launch_side(c, &c->pmd_m, part_m1, c->ev_fork, graph, false))) return rc;
             if (graph) SMD_HIP(c, hipGraphLaunch(c->pmd_a2, c->stream));
             else if ((rc = part_a2(c))) return rc;
+            if ((rc = launch_side(c, &c->pmd_m2, part_m2, c->ev_fork2, graph, true))) return rc;
+            if (graph) SMD_HIP(c, hipGraphLaunch(c->pmd_a3, c->stream));
+            else if ((rc = part_a3(c))) return rc;
         } else {
             if (graph) SMD_HIP(c, hipGraphLaunch(c->pmd_a, c->stream));
             else if ((rc = part_a(c, true))) return rc;

@@ -263,6 +263,7 @@ struct Ctx {
     hipStream_t polar_main = nullptr;   // the step's stream while a polarizable k-space fork is open
     hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     hipEvent_t ev_cg = nullptr;       // after the polarizable step's solve batches
+    hipEvent_t ev_fork2 = nullptr;    // the polarizable step's second fork (after the field pass)
     hipEvent_t list_fork = nullptr;   // recorded by the list build after its preamble when set
     double* d_frecip = nullptr;       // [3n]
     bool recip_store = false;         // k_recip_force stores into d_frecip instead of adding
@@ -302,8 +303,10 @@ struct Ctx {
     std::vector<double> pmd_tab[4];     // sigma2, eps4, type sigma, type eps
     double* d_polar_terms = nullptr;    // [2][POLAR_TERMS]: Multipole's, Induction's accumulators
     hipGraphExec_t pmd_a = nullptr;     // captured forces up to the solve
-    hipGraphExec_t pmd_a2 = nullptr;    // captured solve start (when part M runs beside it)
-    hipGraphExec_t pmd_m = nullptr;     // captured Multipole short range (second stream, beside the solve)
+    hipGraphExec_t pmd_a2 = nullptr;    // captured field pass (when parts M1 and M2 run beside the solve)
+    hipGraphExec_t pmd_a3 = nullptr;    // captured solve start (idem)
+    hipGraphExec_t pmd_m = nullptr;     // captured vdW on the second stream (M1)
+    hipGraphExec_t pmd_m2 = nullptr;    // captured Multipole short range on the second stream (M2)
     hipGraphExec_t pmd_c = nullptr;     // captured induction terms and integration tail
     bool pmd_ready = false;
     int pmd_last_iters = 0;             // the previous step's solve iterations
@@ -311,6 +314,7 @@ struct Ctx {
     bool ind_last_converged = false;
     bool ind_table_built = false;   // a minimised solve has filled the coupling table
     bool ind_real_table = false;    // Induction's mixed real space may walk the current solve's coupling table
+    bool mp_real_table = false;     // Multipole's mixed real space may walk the current solve's coupling table
 };
 
 }  // namespace smd
@@ -324,6 +328,8 @@ int fail(Ctx* c, int code, const char* fmt, ...);
 void mlp_release(Ctx* c);
 int mlp_guess(Ctx* c, double* x, const double* z);
 int induce_begin(Ctx* c, const smd_induce_params* p, bool frames);
+int induce_field(Ctx* c, const smd_induce_params* p, bool frames);
+int induce_start(Ctx* c, const smd_induce_params* p);
 int induce_batches(Ctx* c, const smd_induce_params* p, int batches);
 int induce_poll(Ctx* c, int* done, int* iters, int* reason, double* res);
 int frames_run(Ctx* c);
