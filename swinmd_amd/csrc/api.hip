@@ -143,7 +143,7 @@ static void release(Ctx* c) {
                     c->d_host, c->d_dipl, c->d_quadl, c->d_polar, c->d_dipg, c->d_quadg, c->d_e0, c->d_mu,
                     c->d_cg, c->d_cg_part, c->d_cg_state, c->d_cg_scal, c->d_frecip, c->d_mp_trq,
                     c->d_mp_terms, c->mp_ks.kvec, c->mp_ks.kmod2, c->mp_ks.expk, c->mp_ks.part, c->mp_ks.kf,
-                    c->mp_ks.fpart, c->ind_ks.kvec, c->ind_ks.kmod2, c->ind_ks.expk, c->ind_ks.part,
+                    c->mp_ks.fpart, c->mp_ks.kstep, c->ind_ks.kstep, c->ind_ks.kvec, c->ind_ks.kmod2, c->ind_ks.expk, c->ind_ks.part,
                     c->ind_ks.kf, c->ind_ks.fpart, c->d_pack, c->d_bd_bonds, c->d_bd_angles,
                     c->d_bd_bond_off, c->d_bd_angle_off, c->d_bd_par, c->d_bd_out, c->d_stress, c->d_real,
                     c->d_it_cnt, c->d_it_j, c->d_it_a, c->d_part, c->d_cg32, c->d_ind_rec32, c->d_ind_fix, c->d_e0_prev, c->d_vrec, c->d_pc_blk, c->d_shell32, c->d_polar_terms, c->d_site32, c->d_site32_ind, c->d_pind32,

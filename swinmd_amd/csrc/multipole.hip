@@ -66,6 +66,8 @@ struct MpArgs {
     const int32_t* tcnt;
     const int32_t* tj;
     int tstride;
+    const int* kstep;           // the k set's nz-step flags (see KSet)
+    double kz_step;             // 2 pi / Lz
     const int32_t* mol_start;
     const int32_t* mol;
     const double* pos;
@@ -902,6 +904,8 @@ static MpArgs mp_args(Ctx* c, const Ctx::KSet& ks, double alpha, double real_cut
     a.part = ks.part;
     a.kf = ks.kf;
     a.fpart = ks.fpart;
+    a.kstep = ks.kstep;
+    a.kz_step = 2.0 * PI / c->box.L[2];
     a.ksplit = ks.ksplit;
     a.kper = ks.ksplit > 0 ? (ks.K + ks.ksplit - 1) / ks.ksplit : 0;
     a.alpha = alpha;
@@ -1040,6 +1044,8 @@ static int kset_build(Ctx* c, Ctx::KSet& ks, double alpha, double kcut, int nsum
         if (c->h_mol_start[m + 1] - c->h_mol_start[m] != 3)
             return fail(c, SMD_ENOTSUP, "multipole and induction Ewald are implemented for 3-site water (H1, O, H2)");
     std::vector<double> kv, k2v, ekv;
+    std::vector<int> kst;
+    int pnx = 0, pny = 0, pnz = 0;
     const double kCut2 = kcut * kcut, alphaR4 = 1.0 / (4.0 * (alpha * alpha));
     const int nxMax = (int)(kcut * c->box.L[0] / (2 * PI));
     const int nyMax = (int)(kcut * c->box.L[1] / (2 * PI));
@@ -1051,6 +1057,8 @@ static int kset_build(Ctx* c, Ctx::KSet& ks, double alpha, double kcut, int nsum
                 const double kx = nx * vx, ky = ny * vy, kz = nz * vz;
                 const double k2 = kx * kx + ky * ky + kz * kz;
                 if (k2 < kCut2) {
+                    kst.push_back(!kst.empty() && nx == pnx && ny == pny && nz == pnz + 1 ? 1 : 0);
+                    pnx = nx; pny = ny; pnz = nz;
                     kv.push_back(kx); kv.push_back(ky); kv.push_back(kz);
                     k2v.push_back(k2);
                     ekv.push_back(exp(-k2 * alphaR4) / k2);
@@ -1082,6 +1090,10 @@ static int kset_build(Ctx* c, Ctx::KSet& ks, double alpha, double kcut, int nsum
     SMD_HIP(c, hipMemcpy(ks.kvec, kv.data(), sizeof(double) * 3 * K, hipMemcpyHostToDevice));
     SMD_HIP(c, hipMemcpy(ks.kmod2, k2v.data(), sizeof(double) * K, hipMemcpyHostToDevice));
     SMD_HIP(c, hipMemcpy(ks.expk, ekv.data(), sizeof(double) * K, hipMemcpyHostToDevice));
+    if (ks.kstep) (void)hipFree(ks.kstep);
+    ks.kstep = nullptr;
+    if (hipMalloc(&ks.kstep, sizeof(int) * K) != hipSuccess) return fail(c, SMD_ENOMEM, "k-step flags of %d k", K);
+    SMD_HIP(c, hipMemcpy(ks.kstep, kst.data(), sizeof(int) * K, hipMemcpyHostToDevice));
     ks.K = K;
     ks.chunks = chunks;
     ks.ksplit = ksplit;
@@ -1807,9 +1819,14 @@ __global__ __launch_bounds__(256) void k_joint_kfinish(MpArgs mp, MpArgs in) {
 
 // The joint force pass: the induction pass's layout (site per lane, k-range
 // per block row) with both classes' force and torque per (site, k).
+// The phase e^{i k.r} of a k that continues the previous one's (nx, ny)
+// row with nz + 1 is the previous phase times e^{i 2 pi z / Lz} (a rotation,
+// uniform across the wave since every lane takes the same k); every eighth
+// k of a tile, and every k that starts a row, is evaluated directly.
 template <typename R>
 __global__ __launch_bounds__(WAVE) void k_joint_kforce(MpArgs a) {
     __shared__ double sk[MP_KTILE][7];
+    __shared__ int sst[MP_KTILE];
     const int i = blockIdx.x * WAVE + threadIdx.x;
     const int kb = blockIdx.y * a.kper, ke = min(a.K, kb + a.kper);
     Site I64{};
@@ -1821,6 +1838,8 @@ __global__ __launch_bounds__(WAVE) void k_joint_kforce(MpArgs a) {
     }
     const SiteT<R> I = site_as<R>(I64);
     const V3T<R> p = vec3<R>((R)p64.x, (R)p64.y, (R)p64.z);
+    R sz, cz;
+    phase_sincos<R>(ri.z * a.kz_step, sz, cz);
     V3 F = v3(0, 0, 0), T = v3(0, 0, 0);
     for (int k0 = kb; k0 < ke; k0 += MP_KTILE) {
         const int cnt = min(MP_KTILE, ke - k0);
@@ -1833,13 +1852,20 @@ __global__ __launch_bounds__(WAVE) void k_joint_kforce(MpArgs a) {
             sk[x][4] = a.kf[4 * (k0 + x) + 1];
             sk[x][5] = a.kf[4 * (k0 + x) + 2];
             sk[x][6] = a.kf[4 * (k0 + x) + 3];
+            sst[x] = a.kstep[k0 + x];
         }
         __syncthreads();
         if (i < a.n) {
             V3T<R> f = vec3<R>(R(0), R(0), R(0)), t = f;
+            R sn = R(0), cs = R(1);
             for (int x = 0; x < cnt; ++x) {
-                R sn, cs;
-                phase_sincos<R>(ri.x * sk[x][0] + ri.y * sk[x][1] + ri.z * sk[x][2], sn, cs);
+                if ((x & 7) != 0 && sst[x]) {
+                    const R c2 = cs * cz - sn * sz;
+                    sn = sn * cz + cs * sz;
+                    cs = c2;
+                } else {
+                    phase_sincos<R>(ri.x * sk[x][0] + ri.y * sk[x][1] + ri.z * sk[x][2], sn, cs);
+                }
                 const V3T<R> kv = vec3<R>((R)sk[x][0], (R)sk[x][1], (R)sk[x][2]);
                 const R X = (R)sk[x][3], Y = (R)sk[x][4], U = (R)sk[x][5], W = (R)sk[x][6];
                 const V3T<R> Qk = smul(I.Q, kv);

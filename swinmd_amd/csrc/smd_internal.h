@@ -237,6 +237,7 @@ struct Ctx {
         double* expk = nullptr;     // [K]
         double* part = nullptr;     // [chunks][K][sums]
         double* kf = nullptr;       // [K][4] per-k force-pass coefficients
+        int* kstep = nullptr;       // [K] 1 when k is the previous k with nz + 1 (same nx, ny)
         int ksplit = 1;             // k-ranges of the force pass
         double* fpart = nullptr;    // [ksplit][n][6] force and torque per k-range
         double alpha = 0, kcut = 0;   // the set is fixed by these and the box
