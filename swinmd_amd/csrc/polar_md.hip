@@ -8,29 +8,30 @@
 //   SHAKE, kinetics, NH friction    LFIntegrator.cpp:561-722
 //   per-step rewrap                 :897-910
 //
-// A step is three parts on the context stream:
-//   A  the Verlet trigger and drift partials (cleared forces and terms), the
-//      gated list rebuild, the lab frames (A1), then the solve's field,
-//      coupling table, start, A x and residual (A2);
-//   B  the solve's iterations in captured batches; the host reads the
-//      device's stop flag between batches (the first read comes after as
-//      many batches as the previous step needed);
-//   M  vdW and Multipole's short-range kernels (they write forces, which
-//      nothing in A2 and B touches), on the second stream beside A2 and B,
-//      with (the polar_m_sums tuning switch) the permanent multipoles'
-//      structure-factor sums (else in C's joint pass);
-//   C  Induction's short-range kernels and the rest of one k-space pass for
-//      both classes with the converged dipoles (the joint pass: both classes
-//      on the same k set), then SHAKE with the kinetic and pressure sums and
-//      the NH update.
-// The second stream also carries the k-space pass in C, beside the other
-// kernels (the recip_overlap tuning switch; without it vdW and M's kernels
-// run in C on the step's stream, and the structure factors there are summed
-// in one pass).  Without the joint pass
-// (classes on different k sets, or the polar_joint_long switch off),
-// Multipole's kernels run in A with their own k-space pass on the second
-// stream, and C holds Induction's.
-// A (or A1 and A2), M and C hold no host synchronisation and no memset: each
+// A step runs in parts on the context stream and a second stream:
+//   A1 the Verlet trigger and drift partials (cleared forces and terms), the
+//      gated list rebuild, the lab frames;
+//   M1 (second stream, beside A2) vdW;
+//   A2 the solve's field pass and coupling table;
+//   M2 (second stream, beside A3 and B) Multipole's short-range kernels,
+//      their real space over the coupling table in the mixed form, and (the
+//      polar_m_sums tuning switch) the permanent multipoles' structure-factor
+//      sums; M1 and M2 write forces and torques, which nothing in A2, A3 and
+//      B touches;
+//   A3 the solve's start, A x and residual;
+//   B  the solve's iterations in captured batches; the host waits for the
+//      device's stop flag between batches (the first wait comes after as many
+//      batches as the previous step needed);
+//   C  (after M2) Induction's short-range kernels and the rest of one k-space
+//      pass for both classes with the converged dipoles (the joint pass: both
+//      classes on the same k set; its k-space kernels on the second stream),
+//      then SHAKE with the kinetic and pressure sums and the NH update.
+// Without the second stream (the recip_overlap tuning switch) part A holds
+// A1, A2 and A3 and part C also runs vdW and M2's kernels, in order, with the
+// structure factors summed in one pass.  Without the joint pass (classes on
+// different k sets, or the polar_joint_long switch off), Multipole's kernels
+// run in A with their own k-space pass on the second stream, and C holds
+// Induction's.  Every part holds no host synchronisation and no memset: each
 // is captured once as a hipGraph and replayed.
 #include <algorithm>
 #include <atomic>
