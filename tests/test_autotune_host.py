@@ -25,15 +25,17 @@ class _Md:
 
     def __init__(self):
         self.ctx = _Ctx()
-        self.cost = {"pair_packed": 3e-6, "recip_overlap": 2e-6, "md_fused_shake": 1e-6}
+        # per-step costs large enough that sleep jitter on a busy host
+        # cannot reorder the combinations
+        self.cost = {"pair_packed": 3e-5, "recip_overlap": 2e-5, "md_fused_shake": 1e-5}
         self.seen = []
 
     def run(self, n):
         import time
         k = self.ctx.knobs
         self.seen.append(dict(k))
-        per = 1e-6 + sum(c for name, c in self.cost.items() if not k.get(name, 1))
-        per += {0: 0.0, 8: 0.5e-6, 32: 0.8e-6}[k.get("pair_lanes", 0)]
+        per = 1e-5 + sum(c for name, c in self.cost.items() if not k.get(name, 1))
+        per += {0: 0.0, 8: 5e-6, 32: 8e-6}[k.get("pair_lanes", 0)]
         time.sleep(n * per)
 
 
