@@ -1198,9 +1198,12 @@ __global__ __launch_bounds__(256) void k_to_f32(int dim, const double* __restric
 
 // Without minimisation: Pi = alpha E0 (:446-453).
 __global__ __launch_bounds__(256) void k_direct(int dim, const double* __restrict__ polar, const double* __restrict__ e0,
-                                                double* __restrict__ x) {
+                                                double* __restrict__ x, float* __restrict__ x32) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k < dim) x[k] = polar[k / 3] * e0[k];
+    if (k >= dim) return;
+    const double v = polar[k / 3] * e0[k];
+    x[k] = v;
+    if (x32) x32[4 * (k / 3) + k % 3] = (float)v;   // the mixed solve's FP32 copy (float4 per site)
 }
 
 // x += alpha (E0 - E0_prev); E0_prev = E0 (the solve's guess 4).
@@ -1297,7 +1300,8 @@ int induce_direct_apply(Ctx* c, const smd_induce_params* p, double* x, double* z
     IndArgs a = ind_args(c, p);
     const int dim = 3 * c->n;
     const dim3 grid_rows((unsigned)(((int64_t)c->n * IND_LANES + 255) / 256));
-    hipLaunchKernelGGL(k_direct, dim3((dim + 255) / 256), dim3(256), 0, c->stream, dim, c->d_polar, c->d_e0, x);
+    hipLaunchKernelGGL(k_direct, dim3((dim + 255) / 256), dim3(256), 0, c->stream, dim, c->d_polar, c->d_e0, x,
+                       (float*)nullptr);
     hipLaunchKernelGGL((k_apply_a<IND_LANES>), grid_rows, dim3(256), 0, c->stream, a, x, z);
     SMD_HIP(c, hipGetLastError());
     return SMD_OK;
@@ -1415,7 +1419,7 @@ int induce_start(Ctx* c, const smd_induce_params* p) {
     const dim3 grid_vec((dim + CG_THREADS - 1) / CG_THREADS);
     (void)n;
     if (!p->do_min) {
-        hipLaunchKernelGGL(k_direct, grid_vec, b256, 0, c->stream, dim, c->d_polar, c->d_e0, c->d_mu);
+        hipLaunchKernelGGL(k_direct, grid_vec, b256, 0, c->stream, dim, c->d_polar, c->d_e0, c->d_mu, (float*)nullptr);
         SMD_HIP(c, hipGetLastError());
         return SMD_OK;
     }
@@ -1423,14 +1427,19 @@ int induce_start(Ctx* c, const smd_induce_params* p) {
     const bool mixed = p->prec == SMD_PREC_MIXED;
     const CgTop plain{nullptr, nullptr, 0, 0.0};
     ProfScope ps(c, "ind_cg");
+    // the FP32 copy of x for the mixed A x, written by the start's last kernel
+    float* x32 = mixed ? reinterpret_cast<float*>(g.v32) : nullptr;
+    bool copied = false;
     if (p->guess == 0) hipLaunchKernelGGL(k_fill, grid_vec, b256, 0, c->stream, dim, 1.0, g.x);
-    else if (p->guess == 2) hipLaunchKernelGGL(k_direct, grid_vec, b256, 0, c->stream, dim, c->d_polar, c->d_e0, g.x);
-    else if (p->guess == 3) {
+    else if (p->guess == 2) {
+        hipLaunchKernelGGL(k_direct, grid_vec, b256, 0, c->stream, dim, c->d_polar, c->d_e0, g.x, x32);
+        copied = true;
+    } else if (p->guess == 3) {
         // learned: x = alpha E0, z = A x feed the descriptors, then x = the network's guess
-        hipLaunchKernelGGL(k_direct, grid_vec, b256, 0, c->stream, dim, c->d_polar, c->d_e0, g.x);
-        if (mixed) hipLaunchKernelGGL(k_to_f32, grid_vec, b256, 0, c->stream, dim, g.x, reinterpret_cast<float*>(g.v32));
+        hipLaunchKernelGGL(k_direct, grid_vec, b256, 0, c->stream, dim, c->d_polar, c->d_e0, g.x, x32);
         solve_apply(c, p, a, g.x, g.v32, g.z, nullptr, plain);
-        if ((rc = mlp_guess(c, g.x, g.z))) return rc;
+        if ((rc = mlp_guess(c, g.x, g.z, x32))) return rc;
+        copied = true;
     }
     else if (p->guess == 4) {
         // the previous solution moved by the change of the direct term:
@@ -1438,7 +1447,7 @@ int induce_start(Ctx* c, const smd_induce_params* p) {
         hipLaunchKernelGGL(k_extrapolate, grid_vec, b256, 0, c->stream, dim, c->d_polar, c->d_e0, c->d_e0_prev, g.x);
     }
     // guess 1: keep the previous solution in d_mu
-    if (mixed) hipLaunchKernelGGL(k_to_f32, grid_vec, b256, 0, c->stream, dim, g.x, reinterpret_cast<float*>(g.v32));
+    if (mixed && !copied) hipLaunchKernelGGL(k_to_f32, grid_vec, b256, 0, c->stream, dim, g.x, x32);
     solve_apply(c, p, a, g.x, g.v32, g.z, nullptr, plain);
     if (p->precond == 2) {
         hipLaunchKernelGGL(k_block_pc, dim3((c->nmol + 255) / 256), b256, 0, c->stream, a, c->d_pc_blk, c->nmol);

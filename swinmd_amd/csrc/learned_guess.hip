@@ -699,14 +699,17 @@ __global__ __launch_bounds__(256) void k_targets(int n, const double* frame, con
 
 // Pi_i = alpha_i E0_i + F_i^T (sd * y_i + mean)
 __global__ __launch_bounds__(256) void k_guess(int n, const double* frame, const double* polar, const double* e0,
-                                               const float* y, const float* norm, double* x) {
+                                               const float* y, const float* norm, double* x, float* x32) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const double* F = frame + 9 * (size_t)i;
     double loc[3];
     for (int k = 0; k < 3; ++k) loc[k] = (double)norm[2 * MF + MO + k] * y[(size_t)i * MO + k] + norm[2 * MF + k];
-    for (int k = 0; k < 3; ++k)
-        x[3 * i + k] = polar[i] * e0[3 * i + k] + F[k] * loc[0] + F[3 + k] * loc[1] + F[6 + k] * loc[2];
+    for (int k = 0; k < 3; ++k) {
+        const double v = polar[i] * e0[3 * i + k] + F[k] * loc[0] + F[3 + k] * loc[1] + F[6 + k] * loc[2];
+        x[3 * i + k] = v;
+        if (x32) x32[4 * i + k] = (float)v;   // the mixed solve's FP32 copy (float4 per site)
+    }
 }
 
 // Per-column mean and standard deviation of rows x cols FP32 data, one block
@@ -960,7 +963,7 @@ static int descriptors_run(Ctx* c, const double* x, const double* z, float* raw,
 }
 
 // The guess of smd_induce: x holds alpha E0 and z = A x on entry; x gets the guess.
-int mlp_guess(Ctx* c, double* x, const double* z) {
+int mlp_guess(Ctx* c, double* x, const double* z, float* x32) {
     Mlp* m = c->mlp;
     if (!m || !m->norm_set) return fail(c, SMD_ESTATE, "learned guess: no trained network (smd_mlp_train or smd_mlp_set_norm)");
     int rc;
@@ -971,7 +974,7 @@ int mlp_guess(Ctx* c, double* x, const double* z) {
     hipLaunchKernelGGL(k_mlp_infer, dim3((unsigned)((a.rows16 + 16 * INF_WAVES - 1) / (16 * INF_WAVES))),
                        dim3(INF_WAVES * WAVE), 0, c->stream, a);
     hipLaunchKernelGGL(k_guess, dim3((c->n + 255) / 256), dim3(256), 0, c->stream, c->n, m->frame, c->d_polar,
-                       c->d_e0, m->y, m->norm, x);
+                       c->d_e0, m->y, m->norm, x, x32);
     SMD_HIP(c, hipGetLastError());
     return SMD_OK;
 }

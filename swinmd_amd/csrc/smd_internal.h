@@ -327,7 +327,7 @@ namespace smd {
 int fail(Ctx* c, int code, const char* fmt, ...);
 // learned guess (learned_guess.hip) and the induction helpers it uses (induction.hip)
 void mlp_release(Ctx* c);
-int mlp_guess(Ctx* c, double* x, const double* z);
+int mlp_guess(Ctx* c, double* x, const double* z, float* x32);
 int induce_begin(Ctx* c, const smd_induce_params* p, bool frames);
 int induce_field(Ctx* c, const smd_induce_params* p, bool frames);
 int induce_start(Ctx* c, const smd_induce_params* p);
